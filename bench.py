@@ -1,0 +1,136 @@
+#!/usr/bin/env python3
+"""Closest-hit ray-query throughput of the HIP backend on one MI355X.
+
+Workload (one step = one launch over the whole batch): the 1,048,576-
+triangle heightfield (1024x512 quads over [0,1]^2, z = 0.1 * fBm with 6
+octaves, lacunarity 2, gain 0.5, seeded Perlin), binned-SAH BVH with 32
+bins and leaves of at most 4 triangles, and 4,194,304 f32 rays with
+origins uniform in the mesh box inflated by 10% and directions uniform on
+the sphere, tmax = inf. Rays, BVH and hits stay resident in HBM; the
+timed region is K back-to-back closest-hit launches.
+
+Prints one JSON line (value = whole-job Mrays/s at N=1).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "render.rs_amd"))
+
+N_RAYS = 4194304
+GRID = (1024, 512)
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trace_closest_heightfield_1m_traffic.json")
+
+
+def cpu_baseline(pos, idx, rays):
+    """The oracle (C++ f64 restatement of render.rs's CPU traversal) over
+    the same rays on the host's cores; BVH build excluded."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    threads = max(1, min(16, os.cpu_count() or 1))
+    m = orc.Mesh(pos, idx, bins=32, max_leaf=4, sah_leaf_mult=1)
+    t0 = time.perf_counter()
+    m.trace_closest(rays, threads=threads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(rays.shape[0] / dt / 1e6, 3),
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"all {rays.shape[0]} rays of the workload, closest hit, f64, "
+                  f"{threads} threads; BVH build excluded; {dt:.2f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: the ray-query workload is replicated-BVH / "
+              "split-rays data parallel with no exchange, and this build runs on a single "
+              "MI355X. Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import rrhip
+    ctx = rrhip.Context(0)
+    pos, idx = rrhip.gen_heightfield(GRID[0], GRID[1], args.seed)
+    rays = rrhip.gen_query_rays(pos.min(0), pos.max(0), 0.1, args.seed, N_RAYS)
+    scene = ctx.upload_mesh(pos, idx, bins=32, max_leaf=4)
+    info = scene.info()
+    d_rays = ctx.to_device(rays)
+    d_hits = ctx.alloc(N_RAYS * 16)
+    d_occ = ctx.alloc(N_RAYS)
+
+    for _ in range(args.warmup):
+        scene.trace_closest_device(d_rays, N_RAYS, d_hits)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ev_ms = scene.time_trace_device(d_rays, N_RAYS, d_hits, args.steps)
+    ctx.synchronize()
+    wall_s = time.perf_counter() - t0
+    ms_per_step = wall_s * 1e3 / args.steps
+    kernel_ms = ev_ms / args.steps
+    value = N_RAYS / (ms_per_step / 1e3) / 1e6
+
+    for _ in range(args.warmup):
+        scene.trace_any_device(d_rays, N_RAYS, d_occ)
+    any_ms = scene.time_trace_device(d_rays, N_RAYS, d_occ, args.steps, any_hit=True) / args.steps
+
+    nodes, tris = scene.trace_cost(rays)
+    alg_bytes = N_RAYS * (32 + 16) + 64 * nodes + 48 * tris
+    achieved = alg_bytes / (kernel_ms / 1e3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        t = json.load(open(TRAFFIC_FILE))
+        if t.get("rays") == N_RAYS and t.get("triangles") == idx.shape[0]:
+            traffic = t.get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": "closest-hit Mrays/s",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 re-test of candidate triangles)",
+        "data": "synthetic",
+        "config": {
+            "workload": "ray query: 1,048,576-triangle fBm heightfield, 4,194,304 rays, closest hit",
+            "rays": N_RAYS,
+            "triangles": int(idx.shape[0]),
+            "bvh": f"binned SAH 32 bins, leaf<=4, {info['num_nodes']} x 64B nodes, depth {info['depth']}",
+            "parallelism": "single GPU",
+        },
+        "kernel_ms": round(kernel_ms, 4),
+        "any_hit_mrays_s": round(N_RAYS / (any_ms / 1e3) / 1e6, 2),
+        "traversal": {"node_fetches_per_ray": round(nodes / N_RAYS, 3),
+                      "triangle_fetches_per_ray": round(tris / N_RAYS, 3),
+                      "algorithmic_bytes_per_ray": round(alg_bytes / N_RAYS, 1)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(pos, idx, rays)
+    for p in (d_rays, d_hits, d_occ):
+        ctx.free(p)
+    scene.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

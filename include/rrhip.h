@@ -1,0 +1,203 @@
+/*
+ * rrhip — C ABI of the MI355X (gfx950) path-tracing backend for render.rs.
+ *
+ * render.rs selects a backend with `enum Backend { Cpu, Metal }`
+ * (src/main.rs:11-17) and each dispatch arm calls the `raytracer::metal`
+ * function set directly (src/raytracer/metal/mod.rs:13). This header is the
+ * C-ABI counterpart of that set for a `Hip` arm: a Rust `extern "C"` block
+ * binds these symbols. All entry points are blocking
+ * unless they take a stream, return an `rrhip_status`, and leave a message
+ * for `rrhip_last_error()` on failure. Handles are opaque; everything the
+ * library allocates is freed by the matching *_destroy call.
+ *
+ * Threading: one context per host thread, like PtSession, which render.rs
+ * documents as "create and use it on one thread"
+ * (src/raytracer/metal/renderer.rs:256-259).
+ */
+#ifndef RRHIP_H
+#define RRHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RRHIP_ABI_VERSION 1u
+#define RRHIP_NO_HIT 0xFFFFFFFFu
+
+/* Status codes. render.rs returns anyhow::Result with a message
+ * (e.g. src/raytracer/metal/renderer.rs:72-91); here the class is the
+ * return value and the message is rrhip_last_error(). */
+typedef enum rrhip_status {
+  RRHIP_OK = 0,
+  RRHIP_ERR_INVALID_ARGUMENT = 1,
+  RRHIP_ERR_NO_DEVICE = 2,
+  RRHIP_ERR_DEVICE = 3, /* HIP runtime error or a kernel fault surfaced */
+  RRHIP_ERR_OUT_OF_MEMORY = 4,
+  RRHIP_ERR_UNSUPPORTED = 5,
+  RRHIP_ERR_INTERNAL = 6
+} rrhip_status;
+
+typedef struct rrhip_ctx rrhip_ctx;
+typedef struct rrhip_scene rrhip_scene;
+
+/* Query ray, 32 bytes. render.rs's GPU ray query takes [f32; 6]
+ * (origin, direction) and tests t > 1e-6 (src/raytracer/metal/renderer.rs:
+ * 813-877); its CPU mesh path accepts 1e-9 < t < t_max
+ * (src/geometry/mesh.rs:141). Here both bounds travel with the ray:
+ * a hit needs tmin < t < tmax, t measured along the direction as given
+ * (not renormalised, as for render.rs's instance-local rays,
+ * src/geometry/mesh.rs:316-323). */
+typedef struct rrhip_ray {
+  float ox, oy, oz, tmin;
+  float dx, dy, dz, tmax;
+} rrhip_ray;
+
+/* Closest hit, 16 bytes: t, the ORIGINAL triangle index (render.rs's CPU
+ * MeshHit.tri convention, src/accel/bvh.rs:168), barycentrics u, v of
+ * v1 and v2. prim == RRHIP_NO_HIT and t == +inf on a miss. */
+typedef struct rrhip_hit {
+  float t;
+  uint32_t prim;
+  float u, v;
+} rrhip_hit;
+
+/* BVH build parameters. render.rs builds with 16 bins and leaves of up to
+ * 4 (SAH may stop at 16) (src/accel/bvh.rs:91-92). Zero fields take the
+ * defaults: 32 bins, leaves of at most 4 triangles. */
+typedef struct rrhip_build_params {
+  uint32_t bins;
+  uint32_t max_leaf;
+  uint32_t reserved0;
+  uint32_t reserved1;
+} rrhip_build_params;
+
+/* One triangle mesh: f32 xyz positions and u32 vertex indices, 3 per
+ * triangle (render.rs Mesh storage, src/geometry/mesh.rs:20-34). */
+typedef struct rrhip_mesh_desc {
+  const float* positions;
+  uint64_t num_vertices;
+  const uint32_t* indices;
+  uint64_t num_triangles;
+} rrhip_mesh_desc;
+
+/* Scene description. `struct_size` must be sizeof(rrhip_scene_desc).
+ * The caller's arrays are copied during rrhip_scene_upload. */
+typedef struct rrhip_scene_desc {
+  uint32_t struct_size;
+  uint32_t num_meshes;          /* this ABI version: exactly 1 */
+  const rrhip_mesh_desc* meshes;
+  rrhip_build_params build;
+} rrhip_scene_desc;
+
+/* Scene statistics (what the upload built). */
+typedef struct rrhip_scene_info {
+  uint64_t num_triangles;
+  uint64_t num_nodes;     /* 64-byte child-pair nodes */
+  uint64_t num_leaves;
+  uint64_t device_bytes;  /* nodes + triangle records */
+  uint32_t depth;         /* node levels on the longest root-to-leaf path */
+  uint32_t node_bytes;    /* 64 */
+  uint32_t tri_bytes;     /* 48 */
+  uint32_t reserved;
+  double build_ms;        /* host build + flatten time */
+  float bounds_lo[3];
+  float bounds_hi[3];
+} rrhip_scene_info;
+
+/* ---- errors / version --------------------------------------------------- */
+uint32_t rrhip_abi_version(void);
+/* Message of the last failing call on this thread ("" if none). */
+const char* rrhip_last_error(void);
+
+/* ---- context ------------------------------------------------------------ */
+/* Opens HIP device `device` (gfx950 required) and creates the context's
+ * stream. Replaces render.rs's PtSession device/queue setup
+ * (src/raytracer/metal/renderer.rs:280-300). */
+int rrhip_init(int device, rrhip_ctx** out);
+int rrhip_destroy(rrhip_ctx* ctx);
+/* Device name (e.g. "AMD Instinct MI355X") and arch (e.g. "gfx950:..."). */
+int rrhip_device_name(rrhip_ctx* ctx, char* name, size_t name_len, char* arch, size_t arch_len);
+/* The context's HIP stream (hipStream_t), for callers that order their
+ * own work around the library's. */
+void* rrhip_stream(rrhip_ctx* ctx);
+int rrhip_synchronize(rrhip_ctx* ctx);
+
+/* ---- device buffers (for device-resident batches) ------------------------ */
+int rrhip_buffer_alloc(rrhip_ctx* ctx, uint64_t bytes, void** dptr);
+int rrhip_buffer_free(rrhip_ctx* ctx, void* dptr);
+int rrhip_copy_to_device(rrhip_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int rrhip_copy_to_host(rrhip_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+
+/* ---- scene -------------------------------------------------------------- */
+/* Builds the binned-SAH BVH on the host and uploads the 64-byte nodes and
+ * 48-byte leaf-ordered triangle records. Counterpart of render.rs's
+ * Mesh::new -> Bvh::build (src/geometry/mesh.rs:48-84, src/accel/bvh.rs:
+ * 96-108) followed by GpuPtScene::build's BLAS export
+ * (src/raytracer/metal/gpu_scene.rs:473-531). */
+int rrhip_scene_upload(rrhip_ctx* ctx, const rrhip_scene_desc* desc, rrhip_scene** out);
+int rrhip_scene_destroy(rrhip_scene* scene);
+int rrhip_scene_get_info(rrhip_scene* scene, rrhip_scene_info* info);
+/* Runs the host BVH build of rrhip_scene_upload without a device and
+ * reports what it built (device_bytes = what an upload would allocate). */
+int rrhip_build_info(const rrhip_scene_desc* desc, rrhip_scene_info* info);
+
+/* ---- ray queries -------------------------------------------------------- */
+/* Closest hit for n host rays into n host hits. Counterpart of render.rs's
+ * intersect_probe (src/raytracer/metal/renderer.rs:813-877) generalised to
+ * a BVH: Bvh::traverse + Mesh::intersect_triangle (src/accel/bvh.rs:141-204,
+ * src/geometry/mesh.rs:113-142). Hit ids and t equal the f64 CPU result. */
+int rrhip_trace_closest(rrhip_scene* scene, const rrhip_ray* rays, uint64_t n, rrhip_hit* hits);
+/* Any hit with tmin < t < tmax (occluded[i] = 0/1). Counterpart of
+ * Bvh::any_hit + Instance::occludes (src/accel/bvh.rs:208-245,
+ * src/geometry/mesh.rs:384-407). */
+int rrhip_trace_any(rrhip_scene* scene, const rrhip_ray* rays, uint64_t n, uint8_t* occluded);
+
+/* Device-resident variants: `rays`, `hits`, `occluded` are device
+ * pointers; work is enqueued on `stream` (hipStream_t; NULL = the
+ * context's stream) and the call returns without waiting. */
+int rrhip_trace_closest_device(rrhip_scene* scene, const rrhip_ray* rays, uint64_t n,
+                               rrhip_hit* hits, void* stream);
+int rrhip_trace_any_device(rrhip_scene* scene, const rrhip_ray* rays, uint64_t n,
+                           uint8_t* occluded, void* stream);
+
+/* Traversal cost of n host rays as the kernels execute it: totals of node
+ * fetches (64 B each) and triangle fetches (48 B each). any != 0 counts the
+ * any-hit traversal. out[0] = nodes, out[1] = triangles. No counterpart in
+ * render.rs; it feeds the HBM traffic model of the ray-query kernel. */
+int rrhip_trace_cost(rrhip_scene* scene, const rrhip_ray* rays, uint64_t n, int any,
+                     uint64_t out[2]);
+
+/* ---- timing ------------------------------------------------------------- */
+/* Enqueues `reps` back-to-back closest-hit (any == 0) or any-hit launches
+ * over device-resident rays on the context stream, bracketed by HIP
+ * events, waits, and returns the event-measured milliseconds of the
+ * whole sequence. No counterpart in render.rs (which times nothing). */
+int rrhip_time_trace_device(rrhip_scene* scene, const rrhip_ray* rays, uint64_t n, int any,
+                            void* out, uint32_t reps, float* elapsed_ms);
+
+/* ---- workload generators (deterministic, host) --------------------------- */
+/* Heightfield over [0,1]^2 with nx x ny quads (two triangles each),
+ * z = amplitude * fbm(frequency*x + o0, frequency*y + o1, o2) using
+ * render.rs's hash-gradient Perlin and fBm (src/geometry/displace.rs:36-97)
+ * with `octaves`, gain 0.5, lacunarity 2; the offset o = 1000*u, u from
+ * PCG32(seed, stream 0x5eed) (render.rs's Perlin has no seed; the offset is
+ * its only handle). positions: (nx+1)(ny+1)*3 floats, row-major in y then
+ * x; indices: nx*ny*6. */
+int rrhip_gen_heightfield(uint32_t nx, uint32_t ny, uint64_t seed, float amplitude,
+                          float frequency, uint32_t octaves, float* positions, uint32_t* indices);
+/* n query rays: origins uniform in the box centre +/- (inflate/2 + 1/2) *
+ * extent per axis, directions uniform on the sphere (z = 1-2u,
+ * phi = 2*pi*v), tmin = 1e-9, tmax = +inf. Ray i draws from render.rs's
+ * per-(pixel, sample) PCG32 seeding with pixel = i, sample = seed
+ * (src/raytracer/pt/sampler.rs:24-32), in the order ox, oy, oz, u, v. */
+int rrhip_gen_query_rays(const float lo[3], const float hi[3], float inflate, uint64_t seed,
+                         uint64_t n, rrhip_ray* rays);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RRHIP_H */

@@ -1,0 +1,403 @@
+// C-ABI layer: contexts, device buffers, scene upload and ray queries.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "api_internal.h"
+#include "bvh_build.h"
+#include "trace.h"
+
+namespace rrhip {
+
+static thread_local std::string g_last_error;
+
+int fail(int status, const std::string& msg) {
+  g_last_error = msg;
+  return status;
+}
+
+int fail_hip(hipError_t e, const char* where) {
+  std::string m = std::string(where) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
+  int status = RRHIP_ERR_DEVICE;
+  if (e == hipErrorOutOfMemory) status = RRHIP_ERR_OUT_OF_MEMORY;
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) status = RRHIP_ERR_NO_DEVICE;
+  return fail(status, m);
+}
+
+}  // namespace rrhip
+
+using namespace rrhip;
+
+struct rrhip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string name, arch;
+};
+
+struct rrhip_scene {
+  rrhip_ctx* ctx = nullptr;
+  Node2* d_nodes = nullptr;
+  TriRec* d_tris = nullptr;
+  DeviceBvh bvh;
+  rrhip_scene_info info{};
+  // Staging for the host-buffer entry points (grown on demand).
+  void* d_in = nullptr;
+  void* d_out = nullptr;
+  uint64_t in_cap = 0, out_cap = 0;
+};
+
+namespace {
+
+int set_device(rrhip_ctx* ctx) {
+  RRHIP_HIP_TRY(hipSetDevice(ctx->device));
+  return RRHIP_OK;
+}
+
+int ensure(void** p, uint64_t* cap, uint64_t bytes) {
+  if (*cap >= bytes) return RRHIP_OK;
+  if (*p) RRHIP_HIP_TRY(hipFree(*p));
+  *p = nullptr;
+  *cap = 0;
+  RRHIP_HIP_TRY(hipMalloc(p, bytes));
+  *cap = bytes;
+  return RRHIP_OK;
+}
+
+hipStream_t pick_stream(rrhip_scene* s, void* stream) {
+  return stream ? (hipStream_t)stream : s->ctx->stream;
+}
+
+int check_scene(rrhip_scene* s, const void* rays, uint64_t n, const void* out) {
+  if (!s) return fail(RRHIP_ERR_INVALID_ARGUMENT, "null scene");
+  if (n && (!rays || !out)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "null ray or output buffer");
+  return RRHIP_OK;
+}
+
+// Host-buffer query: copy in, trace, copy out, wait.
+int trace_host(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, TraceMode mode, void* out,
+               size_t out_elem) {
+  if (int st = check_scene(s, rays, n, out)) return st;
+  if (n == 0) return RRHIP_OK;
+  if (int st = set_device(s->ctx)) return st;
+  if (int st = ensure(&s->d_in, &s->in_cap, n * sizeof(rrhip_ray))) return st;
+  if (int st = ensure(&s->d_out, &s->out_cap, n * out_elem)) return st;
+  hipStream_t st = s->ctx->stream;
+  RRHIP_HIP_TRY(hipMemcpyAsync(s->d_in, rays, n * sizeof(rrhip_ray), hipMemcpyHostToDevice, st));
+  if (s->bvh.num_nodes == 0) {
+    // Empty scene: every ray misses.
+    if (mode == TraceMode::kAny) {
+      std::memset(out, 0, n);
+    } else {
+      auto* h = (rrhip_hit*)out;
+      for (uint64_t i = 0; i < n; i++) h[i] = rrhip_hit{INFINITY, RRHIP_NO_HIT, 0.f, 0.f};
+    }
+    RRHIP_HIP_TRY(hipStreamSynchronize(st));
+    return RRHIP_OK;
+  }
+  RRHIP_HIP_TRY(launch_trace(s->bvh, (const RayRec*)s->d_in, n, mode,
+                             mode == TraceMode::kClosest ? (HitRec*)s->d_out : nullptr,
+                             mode == TraceMode::kAny ? (uint8_t*)s->d_out : nullptr, nullptr, st));
+  RRHIP_HIP_TRY(hipMemcpyAsync(out, s->d_out, n * out_elem, hipMemcpyDeviceToHost, st));
+  RRHIP_HIP_TRY(hipStreamSynchronize(st));
+  return RRHIP_OK;
+}
+
+// Device-buffer query: enqueue only.
+int trace_dev(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, TraceMode mode, void* out,
+              void* stream) {
+  if (int st = check_scene(s, rays, n, out)) return st;
+  if (n == 0) return RRHIP_OK;
+  if (int st = set_device(s->ctx)) return st;
+  hipStream_t st = pick_stream(s, stream);
+  if (s->bvh.num_nodes == 0) {
+    if (mode == TraceMode::kAny) {
+      RRHIP_HIP_TRY(hipMemsetAsync(out, 0, n, st));
+      return RRHIP_OK;
+    }
+    std::vector<rrhip_hit> miss(n, rrhip_hit{INFINITY, RRHIP_NO_HIT, 0.f, 0.f});
+    RRHIP_HIP_TRY(hipMemcpyAsync(out, miss.data(), n * sizeof(rrhip_hit), hipMemcpyHostToDevice, st));
+    RRHIP_HIP_TRY(hipStreamSynchronize(st));
+    return RRHIP_OK;
+  }
+  RRHIP_HIP_TRY(launch_trace(s->bvh, (const RayRec*)rays, n, mode,
+                             mode == TraceMode::kClosest ? (HitRec*)out : nullptr,
+                             mode == TraceMode::kAny ? (uint8_t*)out : nullptr, nullptr, st));
+  return RRHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t rrhip_abi_version(void) { return RRHIP_ABI_VERSION; }
+
+const char* rrhip_last_error(void) { return g_last_error.c_str(); }
+
+int rrhip_init(int device, rrhip_ctx** out) {
+  if (!out) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_init: null output");
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0)
+    return fail(RRHIP_ERR_NO_DEVICE, "rrhip_init: no HIP device visible");
+  if (device < 0 || device >= count)
+    return fail(RRHIP_ERR_NO_DEVICE, "rrhip_init: device index out of range");
+  hipDeviceProp_t prop;
+  RRHIP_HIP_TRY(hipGetDeviceProperties(&prop, device));
+  std::string arch = prop.gcnArchName;
+  if (arch.rfind("gfx950", 0) != 0)
+    return fail(RRHIP_ERR_UNSUPPORTED, "rrhip_init: device is " + arch + ", gfx950 (MI355X) required");
+  auto* ctx = new (std::nothrow) rrhip_ctx();
+  if (!ctx) return fail(RRHIP_ERR_OUT_OF_MEMORY, "rrhip_init: host allocation failed");
+  ctx->device = device;
+  ctx->name = prop.name;
+  ctx->arch = arch;
+  e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+  if (e != hipSuccess) {
+    rrhip_destroy(ctx);
+    return fail_hip(e, "rrhip_init");
+  }
+  *out = ctx;
+  return RRHIP_OK;
+}
+
+int rrhip_destroy(rrhip_ctx* ctx) {
+  if (!ctx) return RRHIP_OK;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  if (ctx->ev0) hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) hipEventDestroy(ctx->ev1);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return RRHIP_OK;
+}
+
+int rrhip_device_name(rrhip_ctx* ctx, char* name, size_t name_len, char* arch, size_t arch_len) {
+  if (!ctx) return fail(RRHIP_ERR_INVALID_ARGUMENT, "null context");
+  if (name && name_len) {
+    std::strncpy(name, ctx->name.c_str(), name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (arch && arch_len) {
+    std::strncpy(arch, ctx->arch.c_str(), arch_len - 1);
+    arch[arch_len - 1] = 0;
+  }
+  return RRHIP_OK;
+}
+
+void* rrhip_stream(rrhip_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int rrhip_synchronize(rrhip_ctx* ctx) {
+  if (!ctx) return fail(RRHIP_ERR_INVALID_ARGUMENT, "null context");
+  if (int st = set_device(ctx)) return st;
+  RRHIP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return RRHIP_OK;
+}
+
+int rrhip_buffer_alloc(rrhip_ctx* ctx, uint64_t bytes, void** dptr) {
+  if (!ctx || !dptr) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_buffer_alloc: null argument");
+  *dptr = nullptr;
+  if (int st = set_device(ctx)) return st;
+  RRHIP_HIP_TRY(hipMalloc(dptr, bytes ? bytes : 1));
+  return RRHIP_OK;
+}
+
+int rrhip_buffer_free(rrhip_ctx* ctx, void* dptr) {
+  if (!ctx) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_buffer_free: null context");
+  if (!dptr) return RRHIP_OK;
+  if (int st = set_device(ctx)) return st;
+  RRHIP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  RRHIP_HIP_TRY(hipFree(dptr));
+  return RRHIP_OK;
+}
+
+int rrhip_copy_to_device(rrhip_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_copy_to_device: null argument");
+  if (int st = set_device(ctx)) return st;
+  RRHIP_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  RRHIP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return RRHIP_OK;
+}
+
+int rrhip_copy_to_host(rrhip_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_copy_to_host: null argument");
+  if (int st = set_device(ctx)) return st;
+  RRHIP_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  RRHIP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return RRHIP_OK;
+}
+
+namespace {
+
+int validate_desc(const rrhip_scene_desc* desc) {
+  if (desc->struct_size != sizeof(rrhip_scene_desc))
+    return fail(RRHIP_ERR_INVALID_ARGUMENT, "scene desc: struct_size mismatch (ABI version)");
+  if (desc->num_meshes != 1 || !desc->meshes)
+    return fail(RRHIP_ERR_UNSUPPORTED, "scene desc: exactly one mesh is supported");
+  const rrhip_mesh_desc& m = desc->meshes[0];
+  if (m.num_triangles && (!m.positions || !m.indices))
+    return fail(RRHIP_ERR_INVALID_ARGUMENT, "scene desc: null mesh arrays");
+  return RRHIP_OK;
+}
+
+int build_from_desc(const rrhip_scene_desc* desc, BuiltBvh* built, rrhip_scene_info* info) {
+  const rrhip_mesh_desc& m = desc->meshes[0];
+  BuildParams bp;
+  bp.bins = desc->build.bins ? (int)desc->build.bins : 32;
+  bp.max_leaf = desc->build.max_leaf ? (int)desc->build.max_leaf : 4;
+  std::string err;
+  auto t0 = std::chrono::steady_clock::now();
+  if (!build_triangle_bvh(m.positions, m.num_vertices, m.indices, m.num_triangles, bp, built, &err))
+    return fail(RRHIP_ERR_INVALID_ARGUMENT, "BVH build: " + err);
+  auto t1 = std::chrono::steady_clock::now();
+  *info = rrhip_scene_info{};
+  info->num_triangles = m.num_triangles;
+  info->num_nodes = built->nodes.size();
+  info->num_leaves = built->leaves;
+  info->device_bytes = built->nodes.size() * sizeof(Node2) + built->tris.size() * sizeof(TriRec);
+  info->depth = (uint32_t)(built->nodes.empty() ? 0 : built->depth + 1);
+  info->node_bytes = sizeof(Node2);
+  info->tri_bytes = sizeof(TriRec);
+  info->build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  for (int a = 0; a < 3; a++) {
+    info->bounds_lo[a] = built->root_lo[a];
+    info->bounds_hi[a] = built->root_hi[a];
+  }
+  return RRHIP_OK;
+}
+
+}  // namespace
+
+int rrhip_build_info(const rrhip_scene_desc* desc, rrhip_scene_info* info) {
+  if (!desc || !info) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_build_info: null argument");
+  if (int st = validate_desc(desc)) return st;
+  BuiltBvh built;
+  return build_from_desc(desc, &built, info);
+}
+
+int rrhip_scene_upload(rrhip_ctx* ctx, const rrhip_scene_desc* desc, rrhip_scene** out) {
+  if (!ctx || !desc || !out) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_scene_upload: null argument");
+  *out = nullptr;
+  if (int st = validate_desc(desc)) return st;
+  BuiltBvh built;
+  rrhip_scene_info info;
+  if (int st = build_from_desc(desc, &built, &info)) return st;
+  if (int st = set_device(ctx)) return st;
+  auto* s = new (std::nothrow) rrhip_scene();
+  if (!s) return fail(RRHIP_ERR_OUT_OF_MEMORY, "rrhip_scene_upload: host allocation failed");
+  s->ctx = ctx;
+  hipError_t e = hipSuccess;
+  if (!built.nodes.empty()) {
+    e = hipMalloc((void**)&s->d_nodes, built.nodes.size() * sizeof(Node2));
+    if (e == hipSuccess) e = hipMalloc((void**)&s->d_tris, built.tris.size() * sizeof(TriRec));
+    if (e == hipSuccess)
+      e = hipMemcpy(s->d_nodes, built.nodes.data(), built.nodes.size() * sizeof(Node2), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(s->d_tris, built.tris.data(), built.tris.size() * sizeof(TriRec), hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess) {
+    rrhip_scene_destroy(s);
+    return fail_hip(e, "rrhip_scene_upload");
+  }
+  s->bvh.nodes = s->d_nodes;
+  s->bvh.tris = s->d_tris;
+  s->bvh.num_nodes = built.nodes.size();
+  s->bvh.num_tris = built.tris.size();
+  s->info = info;
+  *out = s;
+  return RRHIP_OK;
+}
+
+int rrhip_scene_destroy(rrhip_scene* s) {
+  if (!s) return RRHIP_OK;
+  hipSetDevice(s->ctx->device);
+  hipStreamSynchronize(s->ctx->stream);
+  if (s->d_nodes) hipFree(s->d_nodes);
+  if (s->d_tris) hipFree(s->d_tris);
+  if (s->d_in) hipFree(s->d_in);
+  if (s->d_out) hipFree(s->d_out);
+  delete s;
+  return RRHIP_OK;
+}
+
+int rrhip_scene_get_info(rrhip_scene* s, rrhip_scene_info* info) {
+  if (!s || !info) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_scene_get_info: null argument");
+  *info = s->info;
+  return RRHIP_OK;
+}
+
+int rrhip_trace_closest(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, rrhip_hit* hits) {
+  return trace_host(s, rays, n, TraceMode::kClosest, hits, sizeof(rrhip_hit));
+}
+
+int rrhip_trace_any(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, uint8_t* occluded) {
+  return trace_host(s, rays, n, TraceMode::kAny, occluded, 1);
+}
+
+int rrhip_trace_closest_device(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, rrhip_hit* hits,
+                               void* stream) {
+  return trace_dev(s, rays, n, TraceMode::kClosest, hits, stream);
+}
+
+int rrhip_trace_any_device(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, uint8_t* occluded,
+                           void* stream) {
+  return trace_dev(s, rays, n, TraceMode::kAny, occluded, stream);
+}
+
+int rrhip_trace_cost(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, int any, uint64_t out[2]) {
+  if (int st = check_scene(s, rays, n, out)) return st;
+  out[0] = out[1] = 0;
+  if (n == 0 || s->bvh.num_nodes == 0) return RRHIP_OK;
+  if (int st = set_device(s->ctx)) return st;
+  hipStream_t st = s->ctx->stream;
+  const TraceMode mode = any ? TraceMode::kAny : TraceMode::kClosest;
+  const uint64_t out_elem = any ? 1 : sizeof(rrhip_hit);
+  if (int r = ensure(&s->d_in, &s->in_cap, n * sizeof(rrhip_ray))) return r;
+  if (int r = ensure(&s->d_out, &s->out_cap, n * out_elem)) return r;
+  uint32_t* d_counts = nullptr;
+  RRHIP_HIP_TRY(hipMalloc((void**)&d_counts, n * 2 * sizeof(uint32_t)));
+  std::vector<uint32_t> counts(2 * n);
+  hipError_t e = hipMemcpyAsync(s->d_in, rays, n * sizeof(rrhip_ray), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = launch_trace(s->bvh, (const RayRec*)s->d_in, n, mode,
+                     any ? nullptr : (HitRec*)s->d_out, any ? (uint8_t*)s->d_out : nullptr, d_counts, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(counts.data(), d_counts, n * 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  hipFree(d_counts);
+  if (e != hipSuccess) return fail_hip(e, "rrhip_trace_cost");
+  for (uint64_t i = 0; i < n; i++) {
+    out[0] += counts[2 * i];
+    out[1] += counts[2 * i + 1];
+  }
+  return RRHIP_OK;
+}
+
+int rrhip_time_trace_device(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, int any, void* out,
+                            uint32_t reps, float* elapsed_ms) {
+  if (int st = check_scene(s, rays, n, out)) return st;
+  if (!elapsed_ms || reps == 0) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_time_trace_device: bad reps/output");
+  if (int st = set_device(s->ctx)) return st;
+  hipStream_t st = s->ctx->stream;
+  const TraceMode mode = any ? TraceMode::kAny : TraceMode::kClosest;
+  RRHIP_HIP_TRY(hipEventRecord(s->ctx->ev0, st));
+  for (uint32_t r = 0; r < reps; r++) {
+    if (n && s->bvh.num_nodes)
+      RRHIP_HIP_TRY(launch_trace(s->bvh, (const RayRec*)rays, n, mode,
+                                 any ? nullptr : (HitRec*)out, any ? (uint8_t*)out : nullptr, nullptr, st));
+  }
+  RRHIP_HIP_TRY(hipEventRecord(s->ctx->ev1, st));
+  RRHIP_HIP_TRY(hipEventSynchronize(s->ctx->ev1));
+  RRHIP_HIP_TRY(hipEventElapsedTime(elapsed_ms, s->ctx->ev0, s->ctx->ev1));
+  return RRHIP_OK;
+}
+
+}  // extern "C"

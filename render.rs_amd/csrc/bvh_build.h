@@ -1,0 +1,38 @@
+// Binned-SAH BVH builder for triangle meshes (host, multithreaded).
+//
+// Plays the role of render.rs's `Bvh::build` (src/accel/bvh.rs:96-108,
+// 256-345) plus the GPU export (src/raytracer/metal/gpu_scene.rs:290-299,
+// 473-491), redesigned: f32 bounds taken exactly from the f32 vertices (no
+// f64->f32 rounding, so node boxes never shrink), SAH binned over all three
+// axes, subtrees built in parallel, output flattened to child-pair nodes
+// in depth-first order with triangles gathered into leaf order.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "layout.h"
+
+namespace rrhip {
+
+struct BuildParams {
+  int bins = 32;      // SAH bins per axis
+  int max_leaf = 4;   // leaves hold at most this many triangles
+  int threads = 0;    // 0 = hardware concurrency
+};
+
+struct BuiltBvh {
+  std::vector<Node2> nodes;  // nodes[0] is the root
+  std::vector<TriRec> tris;  // leaf order
+  float root_lo[3], root_hi[3];
+  int depth = 0;  // edges on the longest root-to-leaf path
+  uint64_t leaves = 0;
+};
+
+// Builds over `ntris` triangles given by `indices` into `positions`
+// (xyz f32 triples). Returns false with `err` set on invalid input.
+bool build_triangle_bvh(const float* positions, uint64_t nverts, const uint32_t* indices,
+                        uint64_t ntris, const BuildParams& params, BuiltBvh* out,
+                        std::string* err);
+
+}  // namespace rrhip

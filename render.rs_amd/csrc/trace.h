@@ -1,0 +1,41 @@
+// Ray-query kernels: launch interface used by the C ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "layout.h"
+
+namespace rrhip {
+
+// Query ray / hit records (byte-identical to rrhip_ray / rrhip_hit).
+struct RayRec {
+  float ox, oy, oz, tmin;
+  float dx, dy, dz, tmax;
+};
+struct HitRec {
+  float t;
+  uint32_t prim;
+  float u, v;
+};
+static_assert(sizeof(RayRec) == 32, "RayRec layout");
+static_assert(sizeof(HitRec) == 16, "HitRec layout");
+
+constexpr uint32_t kNoHit = 0xFFFFFFFFu;
+
+struct DeviceBvh {
+  const Node2* nodes = nullptr;
+  const TriRec* tris = nullptr;
+  uint64_t num_nodes = 0;
+  uint64_t num_tris = 0;
+};
+
+enum class TraceMode { kClosest, kAny };
+
+// Traces `n` rays. kClosest writes `hits`; kAny writes `occluded` (0/1).
+// `counts` (optional, 2 x u32 per ray) receives node fetches and triangle
+// fetches per ray, for the traffic model.
+hipError_t launch_trace(const DeviceBvh& bvh, const RayRec* rays, uint64_t n, TraceMode mode,
+                        HitRec* hits, uint8_t* occluded, uint32_t* counts, hipStream_t stream);
+
+}  // namespace rrhip

@@ -1,0 +1,269 @@
+"""ctypes binding of librrhip.so (the C ABI in include/rrhip.h).
+
+This is the harness the tests and bench.py drive the HIP backend through,
+the way a Rust `extern "C"` block would. It holds no compute: every call
+goes to the native library, and importing fails loudly when the library
+has not been built (there is no CPU fallback).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librrhip.so")
+
+NO_HIT = 0xFFFFFFFF
+
+RAY_DTYPE = np.dtype([("ox", "<f4"), ("oy", "<f4"), ("oz", "<f4"), ("tmin", "<f4"),
+                      ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("prim", "<u4"), ("u", "<f4"), ("v", "<f4")])
+
+STATUS_NAMES = {
+    0: "OK", 1: "INVALID_ARGUMENT", 2: "NO_DEVICE", 3: "DEVICE",
+    4: "OUT_OF_MEMORY", 5: "UNSUPPORTED", 6: "INTERNAL",
+}
+
+# Every symbol include/rrhip.h declares (checked by the CPU test suite).
+EXPORTED = [
+    "rrhip_abi_version", "rrhip_last_error", "rrhip_init", "rrhip_destroy",
+    "rrhip_device_name", "rrhip_stream", "rrhip_synchronize",
+    "rrhip_buffer_alloc", "rrhip_buffer_free", "rrhip_copy_to_device", "rrhip_copy_to_host",
+    "rrhip_scene_upload", "rrhip_scene_destroy", "rrhip_scene_get_info", "rrhip_build_info",
+    "rrhip_trace_closest", "rrhip_trace_any", "rrhip_trace_closest_device",
+    "rrhip_trace_any_device", "rrhip_trace_cost", "rrhip_time_trace_device",
+    "rrhip_gen_heightfield", "rrhip_gen_query_rays",
+]
+
+
+class RrhipError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"rrhip status {status} ({STATUS_NAMES.get(status, '?')}): {msg}")
+        self.status = status
+
+
+class BuildParams(ctypes.Structure):
+    _fields_ = [("bins", ctypes.c_uint32), ("max_leaf", ctypes.c_uint32),
+                ("reserved0", ctypes.c_uint32), ("reserved1", ctypes.c_uint32)]
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [("positions", ctypes.c_void_p), ("num_vertices", ctypes.c_uint64),
+                ("indices", ctypes.c_void_p), ("num_triangles", ctypes.c_uint64)]
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("num_meshes", ctypes.c_uint32),
+                ("meshes", ctypes.POINTER(MeshDesc)), ("build", BuildParams)]
+
+
+class SceneInfo(ctypes.Structure):
+    _fields_ = [("num_triangles", ctypes.c_uint64), ("num_nodes", ctypes.c_uint64),
+                ("num_leaves", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
+                ("depth", ctypes.c_uint32), ("node_bytes", ctypes.c_uint32),
+                ("tri_bytes", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("build_ms", ctypes.c_double),
+                ("bounds_lo", ctypes.c_float * 3), ("bounds_hi", ctypes.c_float * 3)]
+
+
+_lib = None
+
+
+def load():
+    """Loads librrhip.so (raises if it is missing: build it first)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                           "or python render.rs_amd/build_lib.py")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, u64, u32, i32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int
+    sig = {
+        "rrhip_abi_version": (u32, []),
+        "rrhip_last_error": (ctypes.c_char_p, []),
+        "rrhip_init": (i32, [i32, ctypes.POINTER(vp)]),
+        "rrhip_destroy": (i32, [vp]),
+        "rrhip_device_name": (i32, [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t]),
+        "rrhip_stream": (vp, [vp]),
+        "rrhip_synchronize": (i32, [vp]),
+        "rrhip_buffer_alloc": (i32, [vp, u64, ctypes.POINTER(vp)]),
+        "rrhip_buffer_free": (i32, [vp, vp]),
+        "rrhip_copy_to_device": (i32, [vp, vp, vp, u64]),
+        "rrhip_copy_to_host": (i32, [vp, vp, vp, u64]),
+        "rrhip_scene_upload": (i32, [vp, ctypes.POINTER(SceneDesc), ctypes.POINTER(vp)]),
+        "rrhip_scene_destroy": (i32, [vp]),
+        "rrhip_scene_get_info": (i32, [vp, ctypes.POINTER(SceneInfo)]),
+        "rrhip_build_info": (i32, [ctypes.POINTER(SceneDesc), ctypes.POINTER(SceneInfo)]),
+        "rrhip_trace_closest": (i32, [vp, vp, u64, vp]),
+        "rrhip_trace_any": (i32, [vp, vp, u64, vp]),
+        "rrhip_trace_closest_device": (i32, [vp, vp, u64, vp, vp]),
+        "rrhip_trace_any_device": (i32, [vp, vp, u64, vp, vp]),
+        "rrhip_trace_cost": (i32, [vp, vp, u64, i32, ctypes.POINTER(u64)]),
+        "rrhip_time_trace_device": (i32, [vp, vp, u64, i32, vp, u32, ctypes.POINTER(ctypes.c_float)]),
+        "rrhip_gen_heightfield": (i32, [u32, u32, u64, ctypes.c_float, ctypes.c_float, u32, vp, vp]),
+        "rrhip_gen_query_rays": (i32, [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                       ctypes.c_float, u64, u64, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(status):
+    if status != 0:
+        msg = load().rrhip_last_error().decode(errors="replace")
+        raise RrhipError(status, msg)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ---- workload generators -------------------------------------------------
+
+def gen_heightfield(nx, ny, seed, amplitude=0.1, frequency=4.0, octaves=6):
+    """Displaced heightfield grid (positions (V,3) f32, indices (T,3) u32)."""
+    lib = load()
+    pos = np.empty(((nx + 1) * (ny + 1), 3), np.float32)
+    idx = np.empty((nx * ny * 2, 3), np.uint32)
+    _check(lib.rrhip_gen_heightfield(nx, ny, seed, amplitude, frequency, octaves, _ptr(pos), _ptr(idx)))
+    return pos, idx
+
+
+def gen_query_rays(lo, hi, inflate, seed, n):
+    lib = load()
+    rays = np.empty(n, RAY_DTYPE)
+    lo_c = (ctypes.c_float * 3)(*[float(x) for x in lo])
+    hi_c = (ctypes.c_float * 3)(*[float(x) for x in hi])
+    _check(lib.rrhip_gen_query_rays(lo_c, hi_c, inflate, seed, n, _ptr(rays)))
+    return rays
+
+
+def mesh_bounds(positions):
+    return positions.min(axis=0), positions.max(axis=0)
+
+
+def _info_dict(i):
+    return {f: (list(getattr(i, f)) if f.startswith("bounds") else getattr(i, f))
+            for f, _ in SceneInfo._fields_ if not f.startswith("reserved")}
+
+
+def _mesh_desc(positions, indices, bins, max_leaf):
+    pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+    idx = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
+    mesh = MeshDesc(_ptr(pos).value, pos.shape[0], _ptr(idx).value, idx.shape[0])
+    desc = SceneDesc(ctypes.sizeof(SceneDesc), 1, ctypes.pointer(mesh), BuildParams(bins, max_leaf, 0, 0))
+    return pos, idx, mesh, desc
+
+
+def build_info(positions, indices, bins=32, max_leaf=4):
+    """Host-only BVH build statistics (no device needed)."""
+    keep = _mesh_desc(positions, indices, bins, max_leaf)
+    i = SceneInfo()
+    _check(load().rrhip_build_info(ctypes.byref(keep[3]), ctypes.byref(i)))
+    return _info_dict(i)
+
+
+# ---- context / scene -----------------------------------------------------
+
+class Context:
+    def __init__(self, device=0):
+        lib = load()
+        h = ctypes.c_void_p()
+        _check(lib.rrhip_init(device, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            load().rrhip_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def device_name(self):
+        name = ctypes.create_string_buffer(256)
+        arch = ctypes.create_string_buffer(256)
+        _check(load().rrhip_device_name(self.h, name, 256, arch, 256))
+        return name.value.decode(), arch.value.decode()
+
+    def synchronize(self):
+        _check(load().rrhip_synchronize(self.h))
+
+    def upload_mesh(self, positions, indices, bins=32, max_leaf=4):
+        return Scene(self, positions, indices, bins, max_leaf)
+
+    # device buffers
+    def alloc(self, nbytes):
+        p = ctypes.c_void_p()
+        _check(load().rrhip_buffer_alloc(self.h, nbytes, ctypes.byref(p)))
+        return p
+
+    def free(self, p):
+        _check(load().rrhip_buffer_free(self.h, p))
+
+    def to_device(self, arr):
+        arr = np.ascontiguousarray(arr)
+        p = self.alloc(arr.nbytes)
+        _check(load().rrhip_copy_to_device(self.h, p, _ptr(arr), arr.nbytes))
+        return p
+
+    def to_host(self, p, out):
+        _check(load().rrhip_copy_to_host(self.h, _ptr(out), p, out.nbytes))
+        return out
+
+
+class Scene:
+    def __init__(self, ctx, positions, indices, bins=32, max_leaf=4):
+        lib = load()
+        self.ctx = ctx
+        self._pos, self._idx, self._mesh, desc = _mesh_desc(positions, indices, bins, max_leaf)
+        h = ctypes.c_void_p()
+        _check(lib.rrhip_scene_upload(ctx.h, ctypes.byref(desc), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            load().rrhip_scene_destroy(self.h)
+            self.h = None
+
+    def info(self):
+        i = SceneInfo()
+        _check(load().rrhip_scene_get_info(self.h, ctypes.byref(i)))
+        return _info_dict(i)
+
+    def trace_closest(self, rays):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.empty(rays.shape[0], HIT_DTYPE)
+        _check(load().rrhip_trace_closest(self.h, _ptr(rays), rays.shape[0], _ptr(hits)))
+        return hits
+
+    def trace_any(self, rays):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        occ = np.empty(rays.shape[0], np.uint8)
+        _check(load().rrhip_trace_any(self.h, _ptr(rays), rays.shape[0], _ptr(occ)))
+        return occ
+
+    def trace_cost(self, rays, any_hit=False):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        out = (ctypes.c_uint64 * 2)()
+        _check(load().rrhip_trace_cost(self.h, _ptr(rays), rays.shape[0], int(any_hit), out))
+        return int(out[0]), int(out[1])
+
+    def trace_closest_device(self, d_rays, n, d_hits, stream=None):
+        _check(load().rrhip_trace_closest_device(self.h, d_rays, n, d_hits, stream))
+
+    def trace_any_device(self, d_rays, n, d_occ, stream=None):
+        _check(load().rrhip_trace_any_device(self.h, d_rays, n, d_occ, stream))
+
+    def time_trace_device(self, d_rays, n, d_out, reps, any_hit=False):
+        ms = ctypes.c_float()
+        _check(load().rrhip_time_trace_device(self.h, d_rays, n, int(any_hit), d_out, reps, ctypes.byref(ms)))
+        return ms.value
