@@ -123,8 +123,11 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const Node2* __restrict__ 
           double tt, uu, vv;
           if (!hit_f64(r, tr, tmin_d, best_t, tt, uu, vv)) continue;
           if (ANY) {
-            occ = true;
-            break;
+            if (tt < best_t) {  // the any-hit bound is exclusive (mesh.rs:141)
+              occ = true;
+              break;
+            }
+            continue;
           }
           // Strictly closer wins; an equal t only replaces a recorded hit
           // with a higher triangle id (the ray's own tmax stays exclusive).
