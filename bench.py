@@ -70,6 +70,9 @@ def main():
     d_rays = ctx.to_device(rays)
     d_hits = ctx.alloc(N_RAYS * 16)
     d_occ = ctx.alloc(N_RAYS)
+    # Launch configuration from the autotuner (setup, not timed); any-hit
+    # reuses the closest-hit choice.
+    tuning, _ = scene.autotune(d_rays, N_RAYS, d_hits)
 
     for _ in range(args.warmup):
         scene.trace_closest_device(d_rays, N_RAYS, d_hits)
@@ -114,6 +117,7 @@ def main():
             "triangles": int(idx.shape[0]),
             "bvh": f"binned SAH 32 bins, leaf<=4, {info['num_nodes']} x 64B nodes, depth {info['depth']}",
             "parallelism": "single GPU",
+            "tuning": tuning,
         },
         "kernel_ms": round(kernel_ms, 4),
         "any_hit_mrays_s": round(N_RAYS / (any_ms / 1e3) / 1e6, 2),

@@ -107,6 +107,18 @@ typedef struct rrhip_scene_info {
   float bounds_hi[3];
 } rrhip_scene_info;
 
+/* Launch configuration of the ray-query kernels (per scene). Each wave
+ * owns `rays_per_wave` consecutive rays and refills finished lanes from
+ * them; `stack` is the per-lane LDS traversal stack (>= the tree's depth).
+ * No counterpart in render.rs (its Metal host fixes threadgroups of 64 /
+ * 256, src/raytracer/metal/renderer.rs:390, 637). */
+typedef struct rrhip_trace_tuning {
+  uint32_t block;          /* 64, 128 or 256 threads */
+  uint32_t stack;          /* 16, 24, 32, 48 or 64 entries */
+  uint32_t rays_per_wave;  /* multiple of 64, <= 2^20 */
+  uint32_t reserved;
+} rrhip_trace_tuning;
+
 /* ---- errors / version --------------------------------------------------- */
 uint32_t rrhip_abi_version(void);
 /* Message of the last failing call on this thread ("" if none). */
@@ -143,6 +155,17 @@ int rrhip_scene_get_info(rrhip_scene* scene, rrhip_scene_info* info);
 /* Runs the host BVH build of rrhip_scene_upload without a device and
  * reports what it built (device_bytes = what an upload would allocate). */
 int rrhip_build_info(const rrhip_scene_desc* desc, rrhip_scene_info* info);
+
+/* Current / explicit launch configuration (set validates against the
+ * scene's depth and returns RRHIP_ERR_INVALID_ARGUMENT when unusable). */
+int rrhip_scene_get_tuning(rrhip_scene* scene, rrhip_trace_tuning* tuning);
+int rrhip_scene_set_tuning(rrhip_scene* scene, const rrhip_trace_tuning* tuning);
+/* Autotuner: times every supported configuration on the caller's
+ * device-resident rays (closest hit if any == 0, else any hit, writing
+ * into `out`), keeps the fastest for the scene and reports it with its
+ * per-launch milliseconds. */
+int rrhip_scene_autotune(rrhip_scene* scene, const rrhip_ray* rays, uint64_t n, int any, void* out,
+                         rrhip_trace_tuning* best, float* best_ms);
 
 /* ---- ray queries -------------------------------------------------------- */
 /* Closest hit for n host rays into n host hits. Counterpart of render.rs's

@@ -1,6 +1,7 @@
 // C-ABI layer: contexts, device buffers, scene upload and ray queries.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <new>
@@ -44,6 +45,7 @@ struct rrhip_scene {
   Node2* d_nodes = nullptr;
   TriRec* d_tris = nullptr;
   DeviceBvh bvh;
+  TraceTuning tuning;
   rrhip_scene_info info{};
   // Staging for the host-buffer entry points (grown on demand).
   void* d_in = nullptr;
@@ -101,7 +103,7 @@ int trace_host(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, TraceMode mode
   }
   RRHIP_HIP_TRY(launch_trace(s->bvh, (const RayRec*)s->d_in, n, mode,
                              mode == TraceMode::kClosest ? (HitRec*)s->d_out : nullptr,
-                             mode == TraceMode::kAny ? (uint8_t*)s->d_out : nullptr, nullptr, st));
+                             mode == TraceMode::kAny ? (uint8_t*)s->d_out : nullptr, nullptr, s->tuning, st));
   RRHIP_HIP_TRY(hipMemcpyAsync(out, s->d_out, n * out_elem, hipMemcpyDeviceToHost, st));
   RRHIP_HIP_TRY(hipStreamSynchronize(st));
   return RRHIP_OK;
@@ -126,7 +128,7 @@ int trace_dev(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, TraceMode mode,
   }
   RRHIP_HIP_TRY(launch_trace(s->bvh, (const RayRec*)rays, n, mode,
                              mode == TraceMode::kClosest ? (HitRec*)out : nullptr,
-                             mode == TraceMode::kAny ? (uint8_t*)out : nullptr, nullptr, st));
+                             mode == TraceMode::kAny ? (uint8_t*)out : nullptr, nullptr, s->tuning, st));
   return RRHIP_OK;
 }
 
@@ -311,6 +313,8 @@ int rrhip_scene_upload(rrhip_ctx* ctx, const rrhip_scene_desc* desc, rrhip_scene
   s->bvh.tris = s->d_tris;
   s->bvh.num_nodes = built.nodes.size();
   s->bvh.num_tris = built.tris.size();
+  s->bvh.depth = (int)info.depth;
+  s->tuning.stack = (uint32_t)std::max(16, stack_for_depth((int)info.depth));
   s->info = info;
   *out = s;
   return RRHIP_OK;
@@ -331,6 +335,76 @@ int rrhip_scene_destroy(rrhip_scene* s) {
 int rrhip_scene_get_info(rrhip_scene* s, rrhip_scene_info* info) {
   if (!s || !info) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_scene_get_info: null argument");
   *info = s->info;
+  return RRHIP_OK;
+}
+
+int rrhip_scene_get_tuning(rrhip_scene* s, rrhip_trace_tuning* t) {
+  if (!s || !t) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_scene_get_tuning: null argument");
+  *t = rrhip_trace_tuning{s->tuning.block, s->tuning.stack, s->tuning.chunk, 0};
+  return RRHIP_OK;
+}
+
+int rrhip_scene_set_tuning(rrhip_scene* s, const rrhip_trace_tuning* t) {
+  if (!s || !t) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_scene_set_tuning: null argument");
+  TraceTuning tu;
+  tu.block = t->block;
+  tu.stack = t->stack;
+  tu.chunk = t->rays_per_wave;
+  if (!tuning_valid(tu, s->bvh.depth))
+    return fail(RRHIP_ERR_INVALID_ARGUMENT,
+                "rrhip_scene_set_tuning: unsupported block/stack/rays_per_wave for this scene (depth " +
+                    std::to_string(s->bvh.depth) + ")");
+  s->tuning = tu;
+  return RRHIP_OK;
+}
+
+int rrhip_scene_autotune(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, int any, void* out,
+                         rrhip_trace_tuning* best, float* best_ms) {
+  if (int st = check_scene(s, rays, n, out)) return st;
+  if (n == 0 || s->bvh.num_nodes == 0) {
+    if (best) *best = rrhip_trace_tuning{s->tuning.block, s->tuning.stack, s->tuning.chunk, 0};
+    if (best_ms) *best_ms = 0.f;
+    return RRHIP_OK;
+  }
+  if (int st = set_device(s->ctx)) return st;
+  hipStream_t st = s->ctx->stream;
+  const TraceMode mode = any ? TraceMode::kAny : TraceMode::kClosest;
+  HitRec* h = any ? nullptr : (HitRec*)out;
+  uint8_t* o = any ? (uint8_t*)out : nullptr;
+  const int s0 = stack_for_depth(s->bvh.depth);
+  TraceTuning chosen = s->tuning;
+  float chosen_ms = 0.f;
+  bool have = false;
+  for (uint32_t block : {64u, 128u, 256u}) {
+    for (int stack : {16, 24, 32, 48, 64}) {
+      if (stack < s0 || stack > std::max(s0, 32)) continue;  // the smallest fitting sizes only
+      for (uint32_t chunk : {64u, 128u, 256u, 512u, 1024u, 4096u}) {
+        TraceTuning tu;
+        tu.block = block;
+        tu.stack = (uint32_t)stack;
+        tu.chunk = chunk;
+        if (!tuning_valid(tu, s->bvh.depth)) continue;
+        RRHIP_HIP_TRY(launch_trace(s->bvh, (const RayRec*)rays, n, mode, h, o, nullptr, tu, st));
+        RRHIP_HIP_TRY(hipEventRecord(s->ctx->ev0, st));
+        const int reps = 3;
+        for (int r = 0; r < reps; r++)
+          RRHIP_HIP_TRY(launch_trace(s->bvh, (const RayRec*)rays, n, mode, h, o, nullptr, tu, st));
+        RRHIP_HIP_TRY(hipEventRecord(s->ctx->ev1, st));
+        RRHIP_HIP_TRY(hipEventSynchronize(s->ctx->ev1));
+        float ms = 0.f;
+        RRHIP_HIP_TRY(hipEventElapsedTime(&ms, s->ctx->ev0, s->ctx->ev1));
+        ms /= reps;
+        if (!have || ms < chosen_ms) {
+          chosen = tu;
+          chosen_ms = ms;
+          have = true;
+        }
+      }
+    }
+  }
+  s->tuning = chosen;
+  if (best) *best = rrhip_trace_tuning{chosen.block, chosen.stack, chosen.chunk, 0};
+  if (best_ms) *best_ms = chosen_ms;
   return RRHIP_OK;
 }
 
@@ -368,7 +442,8 @@ int rrhip_trace_cost(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, int any,
   hipError_t e = hipMemcpyAsync(s->d_in, rays, n * sizeof(rrhip_ray), hipMemcpyHostToDevice, st);
   if (e == hipSuccess)
     e = launch_trace(s->bvh, (const RayRec*)s->d_in, n, mode,
-                     any ? nullptr : (HitRec*)s->d_out, any ? (uint8_t*)s->d_out : nullptr, d_counts, st);
+                     any ? nullptr : (HitRec*)s->d_out, any ? (uint8_t*)s->d_out : nullptr, d_counts,
+                     s->tuning, st);
   if (e == hipSuccess)
     e = hipMemcpyAsync(counts.data(), d_counts, n * 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -392,7 +467,8 @@ int rrhip_time_trace_device(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, i
   for (uint32_t r = 0; r < reps; r++) {
     if (n && s->bvh.num_nodes)
       RRHIP_HIP_TRY(launch_trace(s->bvh, (const RayRec*)rays, n, mode,
-                                 any ? nullptr : (HitRec*)out, any ? (uint8_t*)out : nullptr, nullptr, st));
+                                 any ? nullptr : (HitRec*)out, any ? (uint8_t*)out : nullptr, nullptr,
+                                 s->tuning, st));
   }
   RRHIP_HIP_TRY(hipEventRecord(s->ctx->ev1, st));
   RRHIP_HIP_TRY(hipEventSynchronize(s->ctx->ev1));

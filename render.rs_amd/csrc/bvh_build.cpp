@@ -236,12 +236,14 @@ bool build_triangle_bvh(const float* positions, uint64_t nverts, const uint32_t*
   out->tris.clear();
   out->depth = 0;
   out->leaves = 0;
-  if (params.bins < 2 || params.bins > 256 || params.max_leaf < 1 || params.max_leaf > 16) {
-    *err = "build params out of range (bins 2..256, max_leaf 1..16)";
+  // Leaf stack entries of the kernels pack (count - 1) in 3 bits and the
+  // first triangle slot in 28 bits.
+  if (params.bins < 2 || params.bins > 256 || params.max_leaf < 1 || params.max_leaf > 8) {
+    *err = "build params out of range (bins 2..256, max_leaf 1..8)";
     return false;
   }
-  if (ntris >= 0x7fffffffull || nverts >= 0xffffffffull) {
-    *err = "mesh too large for 32-bit indices";
+  if (ntris >= (1ull << 28) || nverts >= 0xffffffffull) {
+    *err = "mesh too large (at most 2^28 - 1 triangles)";
     return false;
   }
   for (uint64_t i = 0; i < 3 * nverts; i++) {
@@ -299,7 +301,7 @@ bool build_triangle_bvh(const float* positions, uint64_t nverts, const uint32_t*
   }
   out->depth = f.max_depth;
   out->leaves = f.leaves;
-  if (out->depth + 1 >= kMaxStack) {
+  if (out->depth + 1 > kMaxStack) {
     *err = "BVH too deep for the traversal stack";
     return false;
   }

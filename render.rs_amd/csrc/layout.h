@@ -33,8 +33,8 @@ struct alignas(16) TriRec {
 };
 static_assert(sizeof(TriRec) == 48, "TriRec must be 48 bytes");
 
-// Maximum traversal stack depth the kernels carry (in LDS). The builder
-// refuses trees whose depth would exceed it.
-constexpr int kMaxStack = 48;
+// Deepest traversal stack the kernels carry (in LDS). The builder refuses
+// trees with more node levels than this.
+constexpr int kMaxStack = 64;
 
 }  // namespace rrhip

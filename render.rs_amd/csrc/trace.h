@@ -28,7 +28,19 @@ struct DeviceBvh {
   const TriRec* tris = nullptr;
   uint64_t num_nodes = 0;
   uint64_t num_tris = 0;
+  int depth = 0;  // node levels: the traversal stack must hold this many entries
 };
+
+// Launch configuration of the traversal kernels.
+struct TraceTuning {
+  uint32_t block = 128;  // threads per block: 64, 128 or 256
+  uint32_t stack = 0;    // LDS stack entries per lane: 16, 24, 32, 48 or 64 (>= depth)
+  uint32_t chunk = 256;  // rays owned by each wave (multiple of 64)
+};
+
+// Smallest supported stack size holding `depth` entries (0 if none does).
+int stack_for_depth(int depth);
+bool tuning_valid(const TraceTuning& tu, int depth);
 
 enum class TraceMode { kClosest, kAny };
 
@@ -36,6 +48,7 @@ enum class TraceMode { kClosest, kAny };
 // `counts` (optional, 2 x u32 per ray) receives node fetches and triangle
 // fetches per ray, for the traffic model.
 hipError_t launch_trace(const DeviceBvh& bvh, const RayRec* rays, uint64_t n, TraceMode mode,
-                        HitRec* hits, uint8_t* occluded, uint32_t* counts, hipStream_t stream);
+                        HitRec* hits, uint8_t* occluded, uint32_t* counts,
+                        const TraceTuning& tuning, hipStream_t stream);
 
 }  // namespace rrhip

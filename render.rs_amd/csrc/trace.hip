@@ -15,8 +15,14 @@
 //    yields the same bits as the f64 CPU code;
 //  * equal-t candidates resolve to the lower triangle id.
 //
-// Layout: one ray per lane (wave64), traversal stack of node indices in
-// LDS laid out [depth][lane] so every lane hits its own bank.
+// Execution (wave64): every wave owns a contiguous chunk of rays and each
+// lane that finishes its ray takes the next one of the chunk (ballot +
+// mbcnt, no global atomics), so lanes do not idle behind the wave's
+// longest ray. Traversal is "while-while" (Aila & Laine 2009): lanes walk
+// interior nodes until each holds a leaf, then the wave tests leaf
+// triangles together. Each lane's stack of node indices lives in LDS,
+// laid out [depth][thread] so every lane hits its own bank; leaf entries
+// carry the top bit.
 #include <hip/hip_runtime.h>
 
 #include "trace.h"
@@ -25,181 +31,121 @@
 namespace rrhip {
 namespace {
 
-__device__ __forceinline__ Node2 load_node(const Node2* __restrict__ nodes, uint32_t i) {
-  const float4* p = reinterpret_cast<const float4*>(nodes + i);
-  float4 a = p[0], b = p[1], c = p[2], d = p[3];
-  Node2 n;
-  n.c0_lox = a.x;
-  n.c0_hix = a.y;
-  n.c0_loy = a.z;
-  n.c0_hiy = a.w;
-  n.c1_lox = b.x;
-  n.c1_hix = b.y;
-  n.c1_loy = b.z;
-  n.c1_hiy = b.w;
-  n.c0_loz = c.x;
-  n.c0_hiz = c.y;
-  n.c1_loz = c.z;
-  n.c1_hiz = c.w;
-  n.ref0 = __float_as_uint(d.x);
-  n.ref1 = __float_as_uint(d.y);
-  n.cnt0 = __float_as_uint(d.z);
-  n.cnt1 = __float_as_uint(d.w);
-  return n;
-}
-
-__device__ __forceinline__ TriRec load_tri(const TriRec* __restrict__ tris, uint32_t i) {
-  const float4* p = reinterpret_cast<const float4*>(tris + i);
-  float4 a = p[0], b = p[1], c = p[2];
-  TriRec t;
-  t.v0x = a.x;
-  t.v0y = a.y;
-  t.v0z = a.z;
-  t.v1x = a.w;
-  t.v1y = b.x;
-  t.v1z = b.y;
-  t.v2x = b.z;
-  t.v2y = b.w;
-  t.v2z = c.x;
-  t.prim = __float_as_uint(c.y);
-  return t;
-}
-
+// Wave-chunk while-while kernel. `chunk` (a multiple of 64) is the number
+// of rays a wave owns.
 template <int BLOCK, int STACK, bool ANY, bool COUNT>
 __global__ __launch_bounds__(BLOCK) void trace_kernel(const Node2* __restrict__ nodes,
                                                       const TriRec* __restrict__ tris,
                                                       const RayRec* __restrict__ rays,
                                                       HitRec* __restrict__ hits,
                                                       uint8_t* __restrict__ occluded,
-                                                      uint32_t* __restrict__ counts, uint32_t n) {
+                                                      uint32_t* __restrict__ counts, uint32_t n,
+                                                      uint32_t chunk) {
   __shared__ uint32_t s_stack[STACK * BLOCK];
-  const uint32_t lane = threadIdx.x;
-  const uint32_t i = blockIdx.x * BLOCK + lane;
-  if (i >= n) return;  // no block-level synchronisation below
-  uint32_t* stk = s_stack + lane;
+  const uint32_t tid = threadIdx.x;
+  uint32_t* stk = s_stack + tid;
+  const uint32_t wave = (blockIdx.x * BLOCK + tid) >> 6;
+  const uint64_t begin64 = (uint64_t)wave * chunk;
+  const uint32_t begin = begin64 < n ? (uint32_t)begin64 : n;
+  const uint32_t end = (uint64_t)begin + chunk < n ? begin + chunk : n;
+  uint32_t cursor = begin;  // wave-uniform
 
-  const float4* rp = reinterpret_cast<const float4*>(rays + i);
-  const float4 ra = rp[0], rb = rp[1];
-  Ray32 r;
-  r.ox = ra.x;
-  r.oy = ra.y;
-  r.oz = ra.z;
-  r.tmin = ra.w;
-  r.dx = rb.x;
-  r.dy = rb.y;
-  r.dz = rb.z;
-  r.tmax = rb.w;
-  r.ix = safe_rcp(r.dx);
-  r.iy = safe_rcp(r.dy);
-  r.iz = safe_rcp(r.dz);
-
-  const double tmin_d = (double)r.tmin;
-  double best_t = (double)r.tmax;
-  float tmax_f = r.tmax;  // f32 bound for culling, never below best_t
-  uint32_t best = kNoHit;
-  double best_u = 0.0, best_v = 0.0;
-  bool occ = false;
-  uint32_t n_nodes = 0, n_tris = 0;
-
-  int sp = 0;
-  uint32_t node = 0;
+  Lane L;
+  bool have = false;
+  uint32_t rid = 0;
   for (;;) {
-    const Node2 nd = load_node(nodes, node);
-    if (COUNT) n_nodes++;
-    bool h0, h1;
-    float t0, t1;
-    slab2(nd, r, tmax_f, h0, h1, t0, t1);
-    // Leaves are tested on the spot; interior children go on the stack.
-#pragma unroll
-    for (int c = 0; c < 2; c++) {
-      const bool hc = c == 0 ? h0 : h1;
-      const uint32_t cnt = c == 0 ? nd.cnt0 : nd.cnt1;
-      const uint32_t ref = c == 0 ? nd.ref0 : nd.ref1;
-      if (hc && cnt > 0) {
-        for (uint32_t k = ref; k < ref + cnt; k++) {
-          const TriRec tr = load_tri(tris, k);
-          if (COUNT) n_tris++;
-          if (!may_hit(r, tr, tmax_f)) continue;
-          double tt, uu, vv;
-          if (!hit_f64(r, tr, tmin_d, best_t, tt, uu, vv)) continue;
-          if (ANY) {
-            if (tt < best_t) {  // the any-hit bound is exclusive (mesh.rs:141)
-              occ = true;
-              break;
-            }
-            continue;
-          }
-          // Strictly closer wins; an equal t only replaces a recorded hit
-          // with a higher triangle id (the ray's own tmax stays exclusive).
-          if (tt < best_t || (tt == best_t && best != kNoHit && tr.prim < best)) {
-            best_t = tt;
-            best = tr.prim;
-            best_u = uu;
-            best_v = vv;
-            tmax_f = f32_round_up(tt);
-          }
+    // Refill idle lanes from the wave's chunk.
+    const uint64_t idle = __ballot(!have);
+    if (idle != 0 && cursor < end) {
+      const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+      if (!have && cursor + rank < end) {
+        rid = cursor + rank;
+        have = true;
+        lane_start(L, rays, rid);
+      }
+      cursor += (uint32_t)__popcll(idle);
+    }
+    if (__ballot(have) == 0) break;
+    if (have) {
+      // Interior phase: walk until this lane holds a leaf or is done.
+      while (L.lcnt == 0 && L.node != kDone) interior_step<BLOCK, COUNT>(L, nodes, stk);
+      // Leaf phase.
+      while (L.lcnt > 0) {
+        const TriRec tr = tris[L.lref];
+        if (COUNT) L.n_tris++;
+        test_triangle<ANY>(L, tr);
+        L.lref++;
+        L.lcnt--;
+        if (ANY && L.occ) {
+          L.lcnt = 0;
+          L.node = kDone;
         }
       }
-      if (ANY && occ) break;
+      if (L.node == kDone) {
+        lane_finish<ANY, COUNT>(L, rid, hits, occluded, counts);
+        have = false;
+      }
     }
-    if (ANY && occ) break;
-    const bool in0 = h0 && nd.cnt0 == 0;
-    const bool in1 = h1 && nd.cnt1 == 0;
-    if (in0 && in1) {
-      const bool first0 = t0 <= t1;
-      stk[sp * BLOCK] = first0 ? nd.ref1 : nd.ref0;
-      sp++;
-      node = first0 ? nd.ref0 : nd.ref1;
-    } else if (in0) {
-      node = nd.ref0;
-    } else if (in1) {
-      node = nd.ref1;
-    } else {
-      if (sp == 0) break;
-      sp--;
-      node = stk[sp * BLOCK];
-    }
-  }
-
-  if (ANY) {
-    occluded[i] = occ ? 1 : 0;
-  } else {
-    HitRec h;
-    if (best != kNoHit) {
-      h.t = (float)best_t;
-      h.prim = best;
-      h.u = (float)best_u;
-      h.v = (float)best_v;
-    } else {
-      h.t = __builtin_inff();
-      h.prim = kNoHit;
-      h.u = 0.0f;
-      h.v = 0.0f;
-    }
-    hits[i] = h;
-  }
-  if (COUNT) {
-    counts[2 * (size_t)i] = n_nodes;
-    counts[2 * (size_t)i + 1] = n_tris;
   }
 }
 
-constexpr int kBlock = 128;
-constexpr int kStack = kMaxStack;
+template <int BLOCK, int STACK, bool ANY, bool COUNT>
+hipError_t launch_cfg(const DeviceBvh& bvh, const RayRec* rays, uint32_t n, HitRec* hits,
+                      uint8_t* occ, uint32_t* counts, uint32_t chunk, hipStream_t stream) {
+  const uint64_t waves = ((uint64_t)n + chunk - 1) / chunk;
+  const uint64_t blocks = (waves * 64 + BLOCK - 1) / BLOCK;
+  hipLaunchKernelGGL((trace_kernel<BLOCK, STACK, ANY, COUNT>), dim3((uint32_t)blocks), dim3(BLOCK),
+                     0, stream, bvh.nodes, bvh.tris, rays, hits, occ, counts, n, chunk);
+  return hipGetLastError();
+}
+
+template <int BLOCK, bool ANY, bool COUNT>
+hipError_t launch_stack(const DeviceBvh& bvh, const RayRec* rays, uint32_t n, HitRec* hits,
+                        uint8_t* occ, uint32_t* counts, const TraceTuning& tu,
+                        hipStream_t stream) {
+  switch (tu.stack) {
+    case 16: return launch_cfg<BLOCK, 16, ANY, COUNT>(bvh, rays, n, hits, occ, counts, tu.chunk, stream);
+    case 24: return launch_cfg<BLOCK, 24, ANY, COUNT>(bvh, rays, n, hits, occ, counts, tu.chunk, stream);
+    case 32: return launch_cfg<BLOCK, 32, ANY, COUNT>(bvh, rays, n, hits, occ, counts, tu.chunk, stream);
+    case 48: return launch_cfg<BLOCK, 48, ANY, COUNT>(bvh, rays, n, hits, occ, counts, tu.chunk, stream);
+    case 64: return launch_cfg<BLOCK, 64, ANY, COUNT>(bvh, rays, n, hits, occ, counts, tu.chunk, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
 
 template <bool ANY, bool COUNT>
-hipError_t launch_one(const DeviceBvh& bvh, const RayRec* rays, uint32_t n, HitRec* hits,
-                      uint8_t* occ, uint32_t* counts, hipStream_t stream) {
-  const uint32_t blocks = (n + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL((trace_kernel<kBlock, kStack, ANY, COUNT>), dim3(blocks), dim3(kBlock), 0,
-                     stream, bvh.nodes, bvh.tris, rays, hits, occ, counts, n);
-  return hipGetLastError();
+hipError_t launch_block(const DeviceBvh& bvh, const RayRec* rays, uint32_t n, HitRec* hits,
+                        uint8_t* occ, uint32_t* counts, const TraceTuning& tu,
+                        hipStream_t stream) {
+  switch (tu.block) {
+    case 64: return launch_stack<64, ANY, COUNT>(bvh, rays, n, hits, occ, counts, tu, stream);
+    case 128: return launch_stack<128, ANY, COUNT>(bvh, rays, n, hits, occ, counts, tu, stream);
+    case 256: return launch_stack<256, ANY, COUNT>(bvh, rays, n, hits, occ, counts, tu, stream);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace
 
+int stack_for_depth(int depth) {
+  const int sizes[] = {16, 24, 32, 48, 64};
+  for (int s : sizes)
+    if (depth <= s) return s;
+  return 0;
+}
+
+bool tuning_valid(const TraceTuning& tu, int depth) {
+  const bool block_ok = tu.block == 64 || tu.block == 128 || tu.block == 256;
+  const bool stack_ok = stack_for_depth(tu.stack) == tu.stack && tu.stack >= depth;
+  const bool chunk_ok = tu.chunk >= 64 && tu.chunk % 64 == 0 && tu.chunk <= (1u << 20);
+  return block_ok && stack_ok && chunk_ok;
+}
+
 hipError_t launch_trace(const DeviceBvh& bvh, const RayRec* rays, uint64_t n, TraceMode mode,
-                        HitRec* hits, uint8_t* occluded, uint32_t* counts, hipStream_t stream) {
+                        HitRec* hits, uint8_t* occluded, uint32_t* counts,
+                        const TraceTuning& tu, hipStream_t stream) {
+  if (!tuning_valid(tu, bvh.depth)) return hipErrorInvalidValue;
   const uint64_t kChunk = 1ull << 30;
   for (uint64_t off = 0; off < n; off += kChunk) {
     const uint32_t m = (uint32_t)((n - off) < kChunk ? (n - off) : kChunk);
@@ -209,11 +155,11 @@ hipError_t launch_trace(const DeviceBvh& bvh, const RayRec* rays, uint64_t n, Tr
     uint32_t* c = counts ? counts + 2 * off : nullptr;
     hipError_t e;
     if (mode == TraceMode::kAny) {
-      e = c ? launch_one<true, true>(bvh, r, m, h, o, c, stream)
-            : launch_one<true, false>(bvh, r, m, h, o, c, stream);
+      e = c ? launch_block<true, true>(bvh, r, m, h, o, c, tu, stream)
+            : launch_block<true, false>(bvh, r, m, h, o, c, tu, stream);
     } else {
-      e = c ? launch_one<false, true>(bvh, r, m, h, o, c, stream)
-            : launch_one<false, false>(bvh, r, m, h, o, c, stream);
+      e = c ? launch_block<false, true>(bvh, r, m, h, o, c, tu, stream)
+            : launch_block<false, false>(bvh, r, m, h, o, c, tu, stream);
     }
     if (e != hipSuccess) return e;
   }

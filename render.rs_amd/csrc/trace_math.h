@@ -9,6 +9,7 @@
 #include <cstdint>
 
 #include "layout.h"
+#include "trace.h"
 
 #define RRHIP_HD __host__ __device__
 
@@ -116,6 +117,164 @@ RRHIP_HD inline bool hit_f64(const Ray32& r, const TriRec& tr, double tmin, doub
   u_out = u;
   v_out = v;
   return true;
+}
+
+// ---- per-lane traversal state machine (shared by the kernels and by host
+// checks of the control flow) ------------------------------------------------
+
+constexpr uint32_t kDone = 0xFFFFFFFFu;     // traversal finished
+constexpr uint32_t kNeedPop = 0xFFFFFFFEu;  // next action: pop the stack
+constexpr uint32_t kLeafBit = 0x80000000u;  // stack entry is a leaf range
+constexpr uint32_t kLeafFirstMask = 0x0FFFFFFFu;  // leaf entry: first triangle slot (28 bits)
+
+RRHIP_HD inline Ray32 load_ray(const RayRec* __restrict__ rays, uint32_t i) {
+  const RayRec q = rays[i];
+  Ray32 r;
+  r.ox = q.ox;
+  r.oy = q.oy;
+  r.oz = q.oz;
+  r.tmin = q.tmin;
+  r.dx = q.dx;
+  r.dy = q.dy;
+  r.dz = q.dz;
+  r.tmax = q.tmax;
+  r.ix = safe_rcp(r.dx);
+  r.iy = safe_rcp(r.dy);
+  r.iz = safe_rcp(r.dz);
+  return r;
+}
+
+// Per-lane traversal state.
+struct Lane {
+  Ray32 r;
+  double best_t;
+  float tmax_f;  // f32 culling bound, never below best_t
+  uint32_t best;
+  double best_u, best_v;
+  bool occ;
+  uint32_t node;        // interior node to visit, kNeedPop or kDone
+  uint32_t lref, lcnt;  // pending leaf range (lcnt == 0: none)
+  int sp;
+  uint32_t n_nodes, n_tris;
+};
+
+RRHIP_HD inline void lane_start(Lane& L, const RayRec* __restrict__ rays, uint32_t id) {
+  L.r = load_ray(rays, id);
+  L.best_t = (double)L.r.tmax;
+  L.tmax_f = L.r.tmax;
+  L.best = kNoHit;
+  L.best_u = 0.0;
+  L.best_v = 0.0;
+  L.occ = false;
+  L.node = 0;
+  L.lcnt = 0;
+  L.lref = 0;
+  L.sp = 0;
+  L.n_nodes = 0;
+  L.n_tris = 0;
+}
+
+template <bool ANY>
+RRHIP_HD inline void test_triangle(Lane& L, const TriRec& tr) {
+  if (!may_hit(L.r, tr, L.tmax_f)) return;
+  double tt, uu, vv;
+  if (!hit_f64(L.r, tr, (double)L.r.tmin, L.best_t, tt, uu, vv)) return;
+  if (ANY) {
+    if (tt < L.best_t) L.occ = true;  // the any-hit bound is exclusive (mesh.rs:141)
+    return;
+  }
+  // Strictly closer wins; an equal t only replaces a recorded hit with a
+  // higher triangle id (the ray's own tmax stays exclusive).
+  if (tt < L.best_t || (tt == L.best_t && L.best != kNoHit && tr.prim < L.best)) {
+    L.best_t = tt;
+    L.best = tr.prim;
+    L.best_u = uu;
+    L.best_v = vv;
+    L.tmax_f = f32_round_up(tt);
+  }
+}
+
+template <bool ANY, bool COUNT>
+RRHIP_HD inline void lane_finish(const Lane& L, uint32_t id, HitRec* __restrict__ hits,
+                                            uint8_t* __restrict__ occluded,
+                                            uint32_t* __restrict__ counts) {
+  if (ANY) {
+    occluded[id] = L.occ ? 1 : 0;
+  } else {
+    HitRec h;
+    if (L.best != kNoHit) {
+      h.t = (float)L.best_t;
+      h.prim = L.best;
+      h.u = (float)L.best_u;
+      h.v = (float)L.best_v;
+    } else {
+      h.t = INFINITY;
+      h.prim = kNoHit;
+      h.u = 0.0f;
+      h.v = 0.0f;
+    }
+    hits[id] = h;
+  }
+  if (COUNT) {
+    counts[2 * (size_t)id] = L.n_nodes;
+    counts[2 * (size_t)id + 1] = L.n_tris;
+  }
+}
+
+// One interior visit (or one pop). Leaf children become the pending leaf
+// (or a stacked leaf entry); interior children continue nearest-first.
+// At most one entry is pushed per visit, so the stack never holds more
+// entries than the tree has levels.
+template <int STRIDE, bool COUNT>
+RRHIP_HD inline void interior_step(Lane& L, const Node2* __restrict__ nodes,
+                                              uint32_t* stk) {
+  if (L.node == kNeedPop) {
+    if (L.sp == 0) {
+      L.node = kDone;
+      return;
+    }
+    L.sp--;
+    const uint32_t e = stk[L.sp * STRIDE];
+    if (e & kLeafBit) {
+      L.lref = e & kLeafFirstMask;
+      L.lcnt = ((e >> 28) & 7u) + 1u;
+    } else {
+      L.node = e;
+    }
+    return;
+  }
+  const Node2 nd = nodes[L.node];
+  if (COUNT) L.n_nodes++;
+  bool h0, h1;
+  float t0, t1;
+  slab2(nd, L.r, L.tmax_f, h0, h1, t0, t1);
+  const bool leaf0 = h0 && nd.cnt0 > 0, leaf1 = h1 && nd.cnt1 > 0;
+  const bool in0 = h0 && nd.cnt0 == 0, in1 = h1 && nd.cnt1 == 0;
+  if (leaf0) {
+    L.lref = nd.ref0;
+    L.lcnt = nd.cnt0;
+  }
+  if (leaf1) {
+    if (!leaf0) {
+      L.lref = nd.ref1;
+      L.lcnt = nd.cnt1;
+    } else {
+      stk[L.sp * STRIDE] = kLeafBit | ((nd.cnt1 - 1u) << 28) | nd.ref1;
+      L.sp++;
+    }
+  }
+  if (in0 && in1) {
+    const bool first0 = t0 <= t1;
+    stk[L.sp * STRIDE] = first0 ? nd.ref1 : nd.ref0;
+    L.sp++;
+    L.node = first0 ? nd.ref0 : nd.ref1;
+  } else if (in0) {
+    L.node = nd.ref0;
+  } else if (in1) {
+    L.node = nd.ref1;
+  } else {
+    L.node = kNeedPop;
+  }
 }
 
 }  // namespace rrhip

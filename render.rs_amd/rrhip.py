@@ -33,6 +33,7 @@ EXPORTED = [
     "rrhip_trace_closest", "rrhip_trace_any", "rrhip_trace_closest_device",
     "rrhip_trace_any_device", "rrhip_trace_cost", "rrhip_time_trace_device",
     "rrhip_gen_heightfield", "rrhip_gen_query_rays",
+    "rrhip_scene_get_tuning", "rrhip_scene_set_tuning", "rrhip_scene_autotune",
 ]
 
 
@@ -64,6 +65,14 @@ class SceneInfo(ctypes.Structure):
                 ("tri_bytes", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
                 ("build_ms", ctypes.c_double),
                 ("bounds_lo", ctypes.c_float * 3), ("bounds_hi", ctypes.c_float * 3)]
+
+
+class TraceTuning(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_uint32), ("stack", ctypes.c_uint32),
+                ("rays_per_wave", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {"block": self.block, "stack": self.stack, "rays_per_wave": self.rays_per_wave}
 
 
 _lib = None
@@ -101,6 +110,10 @@ def load():
         "rrhip_trace_any_device": (i32, [vp, vp, u64, vp, vp]),
         "rrhip_trace_cost": (i32, [vp, vp, u64, i32, ctypes.POINTER(u64)]),
         "rrhip_time_trace_device": (i32, [vp, vp, u64, i32, vp, u32, ctypes.POINTER(ctypes.c_float)]),
+        "rrhip_scene_get_tuning": (i32, [vp, ctypes.POINTER(TraceTuning)]),
+        "rrhip_scene_set_tuning": (i32, [vp, ctypes.POINTER(TraceTuning)]),
+        "rrhip_scene_autotune": (i32, [vp, vp, u64, i32, vp, ctypes.POINTER(TraceTuning),
+                                       ctypes.POINTER(ctypes.c_float)]),
         "rrhip_gen_heightfield": (i32, [u32, u32, u64, ctypes.c_float, ctypes.c_float, u32, vp, vp]),
         "rrhip_gen_query_rays": (i32, [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                        ctypes.c_float, u64, u64, vp]),
@@ -238,6 +251,22 @@ class Scene:
         i = SceneInfo()
         _check(load().rrhip_scene_get_info(self.h, ctypes.byref(i)))
         return _info_dict(i)
+
+    def get_tuning(self):
+        t = TraceTuning()
+        _check(load().rrhip_scene_get_tuning(self.h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, block, stack, rays_per_wave):
+        t = TraceTuning(block, stack, rays_per_wave, 0)
+        _check(load().rrhip_scene_set_tuning(self.h, ctypes.byref(t)))
+
+    def autotune(self, d_rays, n, d_out, any_hit=False):
+        t = TraceTuning()
+        ms = ctypes.c_float()
+        _check(load().rrhip_scene_autotune(self.h, d_rays, n, int(any_hit), d_out, ctypes.byref(t),
+                                           ctypes.byref(ms)))
+        return t.as_dict(), ms.value
 
     def trace_closest(self, rays):
         rays = np.ascontiguousarray(rays, RAY_DTYPE)
