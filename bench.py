@@ -46,6 +46,27 @@ def cpu_baseline(pos, idx, rays):
     }
 
 
+def copy_calibration(ctx, args):
+    """K launches of the plain copy kernel over 512 MiB (reads and writes
+    exactly that much per launch)."""
+    nbytes = 512 << 20
+    src = ctx.alloc(nbytes)
+    dst = ctx.alloc(nbytes)
+    for _ in range(args.warmup):
+        ctx.device_copy(dst, src, nbytes)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.device_copy(dst, src, nbytes)
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    print(json.dumps({"mode": "copy", "bytes_per_launch": nbytes, "ms_per_launch": round(dt * 1e3, 4),
+                      "GB/s (read+write)": round(2 * nbytes / dt / 1e9, 1)}), flush=True)
+    ctx.free(src)
+    ctx.free(dst)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,6 +74,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tuning", default=None,
+                    help="block,stack,rays_per_wave instead of the autotuner (profiling runs)")
+    ap.add_argument("--no-cost", action="store_true",
+                    help="skip the counting pass (profiling runs)")
+    ap.add_argument("--mode", choices=("trace", "copy"), default="trace",
+                    help="copy: time the plain copy kernel that calibrates HBM counters")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -63,6 +90,9 @@ def main():
 
     import rrhip
     ctx = rrhip.Context(0)
+    if args.mode == "copy":
+        copy_calibration(ctx, args)
+        return
     pos, idx = rrhip.gen_heightfield(GRID[0], GRID[1], args.seed)
     rays = rrhip.gen_query_rays(pos.min(0), pos.max(0), 0.1, args.seed, N_RAYS)
     scene = ctx.upload_mesh(pos, idx, bins=32, max_leaf=4)
@@ -72,7 +102,12 @@ def main():
     d_occ = ctx.alloc(N_RAYS)
     # Launch configuration from the autotuner (setup, not timed); any-hit
     # reuses the closest-hit choice.
-    tuning, _ = scene.autotune(d_rays, N_RAYS, d_hits)
+    if args.tuning:
+        b, st, rpw = (int(x) for x in args.tuning.split(","))
+        scene.set_tuning(b, st, rpw)
+        tuning = scene.get_tuning()
+    else:
+        tuning, _ = scene.autotune(d_rays, N_RAYS, d_hits)
 
     for _ in range(args.warmup):
         scene.trace_closest_device(d_rays, N_RAYS, d_hits)
@@ -89,7 +124,10 @@ def main():
         scene.trace_any_device(d_rays, N_RAYS, d_occ)
     any_ms = scene.time_trace_device(d_rays, N_RAYS, d_occ, args.steps, any_hit=True) / args.steps
 
-    nodes, tris = scene.trace_cost(rays)
+    if args.no_cost:
+        nodes, tris = 0, 0
+    else:
+        nodes, tris = scene.trace_cost(rays)
     alg_bytes = N_RAYS * (32 + 16) + 64 * nodes + 48 * tris
     achieved = alg_bytes / (kernel_ms / 1e3) / 1e9
     traffic = None

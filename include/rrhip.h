@@ -142,6 +142,10 @@ int rrhip_buffer_alloc(rrhip_ctx* ctx, uint64_t bytes, void** dptr);
 int rrhip_buffer_free(rrhip_ctx* ctx, void* dptr);
 int rrhip_copy_to_device(rrhip_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 int rrhip_copy_to_host(rrhip_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* Device-to-device copy by a plain kernel on `stream` (hipStream_t; no
+ * wait). 16-byte aligned buffers, bytes a multiple of 16. No counterpart in
+ * render.rs; its HBM counters calibrate traffic measurements. */
+int rrhip_device_copy(void* stream, void* dst, const void* src, uint64_t bytes);
 
 /* ---- scene -------------------------------------------------------------- */
 /* Builds the binned-SAH BVH on the host and uploads the 64-byte nodes and

@@ -34,6 +34,7 @@ EXPORTED = [
     "rrhip_trace_any_device", "rrhip_trace_cost", "rrhip_time_trace_device",
     "rrhip_gen_heightfield", "rrhip_gen_query_rays",
     "rrhip_scene_get_tuning", "rrhip_scene_set_tuning", "rrhip_scene_autotune",
+    "rrhip_device_copy",
 ]
 
 
@@ -110,6 +111,7 @@ def load():
         "rrhip_trace_any_device": (i32, [vp, vp, u64, vp, vp]),
         "rrhip_trace_cost": (i32, [vp, vp, u64, i32, ctypes.POINTER(u64)]),
         "rrhip_time_trace_device": (i32, [vp, vp, u64, i32, vp, u32, ctypes.POINTER(ctypes.c_float)]),
+        "rrhip_device_copy": (i32, [vp, vp, vp, u64]),
         "rrhip_scene_get_tuning": (i32, [vp, ctypes.POINTER(TraceTuning)]),
         "rrhip_scene_set_tuning": (i32, [vp, ctypes.POINTER(TraceTuning)]),
         "rrhip_scene_autotune": (i32, [vp, vp, u64, i32, vp, ctypes.POINTER(TraceTuning),
@@ -227,6 +229,12 @@ class Context:
         p = self.alloc(arr.nbytes)
         _check(load().rrhip_copy_to_device(self.h, p, _ptr(arr), arr.nbytes))
         return p
+
+    def stream(self):
+        return load().rrhip_stream(self.h)
+
+    def device_copy(self, dst, src, nbytes):
+        _check(load().rrhip_device_copy(self.stream(), dst, src, nbytes))
 
     def to_host(self, p, out):
         _check(load().rrhip_copy_to_host(self.h, _ptr(out), p, out.nbytes))
