@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define RRHIP_ABI_VERSION 1u
+#define RRHIP_ABI_VERSION 2u
 #define RRHIP_NO_HIT 0xFFFFFFFFu
 
 /* Status codes. render.rs returns anyhow::Result with a message
@@ -74,23 +74,144 @@ typedef struct rrhip_build_params {
   uint32_t reserved1;
 } rrhip_build_params;
 
-/* One triangle mesh: f32 xyz positions and u32 vertex indices, 3 per
- * triangle (render.rs Mesh storage, src/geometry/mesh.rs:20-34). */
+/* One triangle mesh: f32 xyz positions, u32 vertex indices (3 per
+ * triangle) and optional per-vertex f32 shading normals (NULL: geometric
+ * normals) — render.rs's Mesh storage (src/geometry/mesh.rs:20-34). */
 typedef struct rrhip_mesh_desc {
   const float* positions;
   uint64_t num_vertices;
   const uint32_t* indices;
   uint64_t num_triangles;
+  const float* normals;
 } rrhip_mesh_desc;
 
+/* A placed copy of a mesh: row-major 3x4 object-to-world affine transform
+ * and a material (render.rs Instance, src/geometry/mesh.rs:223-305). */
+typedef struct rrhip_instance_desc {
+  uint32_t mesh;
+  uint32_t material;
+  double object_to_world[12];
+} rrhip_instance_desc;
+
+/* Loose world-space triangle (render.rs builds these from Polygon,
+ * src/parser/scene_builder.rs:911-949; src/geometry/extra.rs:651-730). */
+typedef struct rrhip_triangle_desc {
+  double v0[3], v1[3], v2[3];
+  uint32_t material;
+  uint32_t reserved;
+} rrhip_triangle_desc;
+
+/* Loose sphere (render.rs Sphere with a translation, src/geometry/sphere.rs). */
+typedef struct rrhip_sphere_desc {
+  double center[3];
+  double radius;
+  uint32_t material;
+  uint32_t reserved;
+} rrhip_sphere_desc;
+
+/* Material: render.rs's PbrParams lobe stack (src/scene/pbr.rs:11-48) plus
+ * Material::emission / area_light (src/scene/material.rs:14-40). */
+typedef struct rrhip_material_desc {
+  double diffuse_gain;
+  double diffuse_color[3];
+  double diffuse_roughness;       /* Oren-Nayar sigma; 0 = Lambert */
+  double specular_face_color[3];  /* Schlick F0; black -> from specular_ior */
+  double specular_edge_color[3];  /* Schlick F90 */
+  double specular_roughness;
+  double specular_ior;            /* 1 disables the specular lobe */
+  double clearcoat_gain;
+  double clearcoat_roughness;
+  double fuzz_gain;
+  double fuzz_color[3];
+  double glass_gain;
+  double glass_ior;
+  double glass_roughness;
+  double refraction_color[3];
+  double glow[3];
+  double emission[3];
+  int32_t area_light;             /* light index for MIS on emitter hits, -1 = none */
+  uint32_t reserved;
+} rrhip_material_desc;
+
+/* Light kinds (render.rs LightType, src/scene/light.rs:7-33). */
+enum {
+  RRHIP_LIGHT_POINT = 0,
+  RRHIP_LIGHT_DISTANT = 1,
+  RRHIP_LIGHT_RECT = 2,
+  RRHIP_LIGHT_SPHERE = 3,
+  RRHIP_LIGHT_DISK = 4,
+  RRHIP_LIGHT_DOME = 5 /* constant radiance dome (no HDRI) */
+};
+
+/* Light: radiance = color * intensity. p = position (point), corner (rect),
+ * centre (sphere/disk) or travel direction (distant); e1/e2 = rect edges or
+ * disk axes; radius = sphere radius or distant angular radius. */
+typedef struct rrhip_light_desc {
+  uint32_t kind;
+  uint32_t reserved;
+  double intensity;
+  double color[3];
+  double p[3];
+  double e1[3];
+  double e2[3];
+  double radius;
+} rrhip_light_desc;
+
+/* Pinhole camera in RenderMan's left-handed frame: forward =
+ * normalize(look_at - eye), right = normalize(up x forward), up' =
+ * forward x right — render.rs's Camera::new frame (src/scene/camera.rs:
+ * 96-119: at the RIB default eye = 0, look +z, up +y it gives right = +x),
+ * placed at `eye`. Pixel filter per src/scene/camera.rs:14-66
+ * (0 box, 1 triangle, 2 gaussian). */
+typedef struct rrhip_camera_desc {
+  uint32_t width, height;
+  uint32_t filter;
+  uint32_t reserved;
+  double fov_deg; /* vertical */
+  double filter_width;
+  double eye[3], look_at[3], up[3];
+} rrhip_camera_desc;
+
 /* Scene description. `struct_size` must be sizeof(rrhip_scene_desc).
- * The caller's arrays are copied during rrhip_scene_upload. */
+ * The caller's arrays are copied during rrhip_scene_upload.
+ * Ray queries use exactly one mesh and no other geometry. Path-tracing
+ * scenes place meshes through instances and add loose triangles and
+ * spheres (render.rs Scene::objects, src/scene/mod.rs:27-56). */
 typedef struct rrhip_scene_desc {
   uint32_t struct_size;
-  uint32_t num_meshes;          /* this ABI version: exactly 1 */
+  uint32_t num_meshes;
   const rrhip_mesh_desc* meshes;
   rrhip_build_params build;
+  uint32_t num_instances;
+  uint32_t num_triangles;
+  const rrhip_instance_desc* instances;
+  const rrhip_triangle_desc* triangles;
+  uint32_t num_spheres;
+  uint32_t num_materials;
+  const rrhip_sphere_desc* spheres;
+  const rrhip_material_desc* materials;
+  uint32_t num_lights;
+  uint32_t reserved0;
+  const rrhip_light_desc* lights;
+  rrhip_camera_desc camera;
+  double background[3]; /* miss radiance when no dome light */
 } rrhip_scene_desc;
+
+/* Path-tracing parameters (render.rs constants, src/raytracer/pt/mod.rs:
+ * 26-32). */
+typedef struct rrhip_render_params {
+  uint32_t struct_size;
+  uint32_t max_depth;        /* bounces; render.rs CPU 10, Metal 8 */
+  uint32_t rr_start;         /* Russian roulette from this depth (3) */
+  uint32_t samples_per_pass; /* samples of every pixel in flight at once; 0 = auto */
+  double firefly_clamp;      /* per-sample luminance clamp (60); <= 0 disables */
+  uint64_t sample_base;      /* added to the sample index when seeding PCG32; 0 = render.rs */
+} rrhip_render_params;
+
+/* Per-pixel accumulator layout of rrhip_read_accum / orc_render:
+ * 8 doubles per pixel, row-major: sum r, g, b; sum of squares r, g, b;
+ * sample count; 0. */
+#define RRHIP_ACCUM_STRIDE 8
 
 /* Scene statistics (what the upload built). */
 typedef struct rrhip_scene_info {

@@ -147,3 +147,115 @@ def edge_distance(positions, indices, rays, prims):
     out = np.empty(r.shape[0], np.float64)
     load().orc_edge_distance(_p(pos), _p(idx), _p(r), _p(pr), r.shape[0], _p(out))
     return out
+
+
+# ---- path tracing ------------------------------------------------------------
+
+def _pt_sigs():
+    lib = load()
+    if getattr(lib, "_pt_ready", False):
+        return lib
+    vp, u64, u32, i32, f64 = (ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int,
+                              ctypes.c_double)
+    sig = {
+        "orc_render": (i32, [vp, vp, u32, u32, vp, i32, vp]),
+        "orc_bsdf_eval": (None, [vp, vp, vp, f64, vp, vp]),
+        "orc_bsdf_eval_n": (None, [vp, vp, vp, f64, u64, vp, vp]),
+        "orc_bsdf_sample_n": (None, [vp, vp, f64, u64, u64, u64, vp, vp, vp, vp]),
+        "orc_light_sampler_create": (vp, [vp, u32]),
+        "orc_light_sampler_destroy": (None, [vp]),
+        "orc_light_sampler_sample": (i32, [vp, vp, f64, ctypes.POINTER(u32), ctypes.POINTER(f64)]),
+        "orc_light_sampler_pmf": (f64, [vp, vp, u32]),
+        "orc_scene_intersect": (i32, [vp, vp, vp, ctypes.POINTER(f64), vp, vp, ctypes.POINTER(i32),
+                                      ctypes.POINTER(u32)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    lib._pt_ready = True
+    return lib
+
+
+def render(desc, params, start, count, accum=None, threads=None):
+    """Adds samples [start, start+count) of every pixel into accum
+    (npix x 8 doubles, the RRHIP_ACCUM layout). Returns (accum, rays)."""
+    lib = _pt_sigs()
+    npix = desc.camera.width * desc.camera.height
+    if accum is None:
+        accum = np.zeros((npix, 8), np.float64)
+    rays = ctypes.c_uint64()
+    lib.orc_render(ctypes.addressof(desc), ctypes.addressof(params), start, count, _p(accum),
+                   threads or default_threads(), ctypes.byref(rays))
+    return accum, rays.value
+
+
+def bsdf_eval(mat, wo, wi, eta):
+    lib = _pt_sigs()
+    wo = np.ascontiguousarray(wo, np.float64)
+    wi = np.ascontiguousarray(wi, np.float64)
+    f = np.zeros(3)
+    pdf = np.zeros(1)
+    lib.orc_bsdf_eval(ctypes.addressof(mat), _p(wo), _p(wi), eta, _p(f), _p(pdf))
+    return f, pdf[0]
+
+
+def bsdf_eval_n(mat, wo, wi, eta):
+    lib = _pt_sigs()
+    wo = np.ascontiguousarray(wo, np.float64)
+    wi = np.ascontiguousarray(wi, np.float64).reshape(-1, 3)
+    f = np.zeros_like(wi)
+    pdf = np.zeros(wi.shape[0])
+    lib.orc_bsdf_eval_n(ctypes.addressof(mat), _p(wo), _p(wi), eta, wi.shape[0], _p(f), _p(pdf))
+    return f, pdf
+
+
+def bsdf_sample_n(mat, wo, eta, seed, stream, n):
+    lib = _pt_sigs()
+    wo = np.ascontiguousarray(wo, np.float64)
+    wi = np.zeros((n, 3))
+    f = np.zeros((n, 3))
+    pdf = np.zeros(n)
+    ok = np.zeros(n, np.uint8)
+    lib.orc_bsdf_sample_n(ctypes.addressof(mat), _p(wo), eta, seed, stream, n, _p(wi), _p(f), _p(pdf), _p(ok))
+    return wi, f, pdf, ok.astype(bool)
+
+
+class LightSampler:
+    def __init__(self, lights):
+        lib = _pt_sigs()
+        self._arr = (type(lights[0]) * len(lights))(*lights)
+        self.h = lib.orc_light_sampler_create(ctypes.addressof(self._arr), len(lights))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _pt_sigs().orc_light_sampler_destroy(self.h)
+            self.h = None
+
+    def sample(self, p, u):
+        lib = _pt_sigs()
+        pp = np.ascontiguousarray(p, np.float64)
+        li = ctypes.c_uint32()
+        pmf = ctypes.c_double()
+        if not lib.orc_light_sampler_sample(self.h, _p(pp), u, ctypes.byref(li), ctypes.byref(pmf)):
+            return None
+        return li.value, pmf.value
+
+    def pmf(self, p, light):
+        pp = np.ascontiguousarray(p, np.float64)
+        return _pt_sigs().orc_light_sampler_pmf(self.h, _p(pp), light)
+
+
+def scene_intersect(desc, o, d):
+    lib = _pt_sigs()
+    o = np.ascontiguousarray(o, np.float64)
+    d = np.ascontiguousarray(d, np.float64)
+    t = ctypes.c_double()
+    point = np.zeros(3)
+    normal = np.zeros(3)
+    front = ctypes.c_int()
+    mat = ctypes.c_uint32()
+    if not lib.orc_scene_intersect(ctypes.addressof(desc), _p(o), _p(d), ctypes.byref(t), _p(point),
+                                   _p(normal), ctypes.byref(front), ctypes.byref(mat)):
+        return None
+    return t.value, point, normal, bool(front.value), mat.value

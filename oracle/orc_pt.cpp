@@ -1,0 +1,1143 @@
+// ORACLE — test infrastructure only (see orc_math.hpp header).
+#include "orc_pt.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace orc {
+
+namespace {
+
+double lum(const Vec3& c) { return 0.2126 * c.x + 0.7152 * c.y + 0.0722 * c.z; }
+
+Vec3 v3(const double a[3]) { return {a[0], a[1], a[2]}; }
+
+double clampd(double x, double lo, double hi) { return std::fmin(std::fmax(x, lo), hi); }
+
+double powi5(double x) { return x * x * x * x * x; }
+
+double powi4(double x) { return x * x * x * x; }
+
+// bxdf.rs:57-60
+Vec3 schlick(const Vec3& f0, const Vec3& f90, double cos) {
+  double m = powi5(clampd(1.0 - cos, 0.0, 1.0));
+  return f0 + (f90 - f0) * m;
+}
+
+// bxdf.rs:63-73
+double fresnel_dielectric(double cos_i, double eta) {
+  cos_i = clampd(cos_i, 0.0, 1.0);
+  double sin2_t = (1.0 - cos_i * cos_i) / (eta * eta);
+  if (sin2_t >= 1.0) return 1.0;
+  double cos_t = std::sqrt(1.0 - sin2_t);
+  double r_par = (eta * cos_i - cos_t) / (eta * cos_i + cos_t);
+  double r_perp = (cos_i - eta * cos_t) / (cos_i + eta * cos_t);
+  return 0.5 * (r_par * r_par + r_perp * r_perp);
+}
+
+// bxdf.rs:77-101
+double ggx_d(const Vec3& h, double alpha) {
+  if (h.z <= 0.0) return 0.0;
+  double a2 = alpha * alpha;
+  double d = h.z * h.z * (a2 - 1.0) + 1.0;
+  return a2 / (PI * d * d);
+}
+
+double ggx_lambda(const Vec3& w, double alpha) {
+  double cos2 = w.z * w.z;
+  if (cos2 <= 0.0) return 0.0;
+  double tan2 = std::fmax(1.0 - cos2, 0.0) / cos2;
+  return (std::sqrt(1.0 + alpha * alpha * tan2) - 1.0) * 0.5;
+}
+
+double ggx_g1(const Vec3& w, double alpha) { return 1.0 / (1.0 + ggx_lambda(w, alpha)); }
+
+double ggx_g2(const Vec3& wo, const Vec3& wi, double alpha) {
+  return 1.0 / (1.0 + ggx_lambda(wo, alpha) + ggx_lambda(wi, alpha));
+}
+
+// bxdf.rs:104-121 (Heitz 2018 VNDF)
+Vec3 ggx_sample_vndf(const Vec3& wo, double alpha, double u1, double u2) {
+  Vec3 v = Vec3{alpha * wo.x, alpha * wo.y, wo.z}.normalize();
+  double lensq = v.x * v.x + v.y * v.y;
+  Vec3 t1 = lensq > 1e-12 ? Vec3{-v.y, v.x, 0.0} / std::sqrt(lensq) : Vec3{1.0, 0.0, 0.0};
+  Vec3 t2 = v.cross(t1);
+  double r = std::sqrt(u1);
+  double phi = 2.0 * PI * u2;
+  double p1 = r * std::cos(phi);
+  double p2 = r * std::sin(phi);
+  double s = 0.5 * (1.0 + v.z);
+  p2 = (1.0 - s) * std::sqrt(std::fmax(1.0 - p1 * p1, 0.0)) + s * p2;
+  Vec3 nh = t1 * p1 + t2 * p2 + v * std::sqrt(std::fmax(1.0 - p1 * p1 - p2 * p2, 0.0));
+  return Vec3{alpha * nh.x, alpha * nh.y, std::fmax(nh.z, 1e-9)}.normalize();
+}
+
+// bxdf.rs:124-130
+double ggx_pdf_h(const Vec3& wo, const Vec3& h, double alpha) {
+  double odh = wo.dot(h);
+  if (odh <= 0.0 || wo.z <= 0.0) return 0.0;
+  return ggx_g1(wo, alpha) * odh * ggx_d(h, alpha) / wo.z;
+}
+
+// bxdf.rs:137-143
+double under_layer_scale(const PbrParams& p) {
+  Vec3 f0 = p.specular_f0();
+  double spec_take = 0.2126 * f0.x + 0.7152 * f0.y + 0.0722 * f0.z;
+  double coat_take = 0.04 * p.clearcoat_gain;
+  return clampd((1.0 - spec_take) * (1.0 - coat_take), 0.0, 1.0);
+}
+
+// bxdf.rs:145-165 (Oren-Nayar)
+Vec3 eval_diffuse(const PbrParams& p, const Vec3& wo, const Vec3& wi) {
+  double sigma2 = p.diffuse_roughness * p.diffuse_roughness;
+  double a = 1.0 - sigma2 / (2.0 * (sigma2 + 0.33));
+  double b = 0.45 * sigma2 / (sigma2 + 0.09);
+  double sin_o = std::sqrt(std::fmax(1.0 - wo.z * wo.z, 0.0));
+  double sin_i = std::sqrt(std::fmax(1.0 - wi.z * wi.z, 0.0));
+  double cos_dphi = (sin_o > 1e-6 && sin_i > 1e-6)
+                        ? clampd((wo.x * wi.x + wo.y * wi.y) / (sin_o * sin_i), -1.0, 1.0)
+                        : 0.0;
+  double sin_a, tan_b;
+  if (wo.z < wi.z) {
+    sin_a = sin_o;
+    tan_b = sin_i / std::fmax(wi.z, 1e-6);
+  } else {
+    sin_a = sin_i;
+    tan_b = sin_o / std::fmax(wo.z, 1e-6);
+  }
+  return p.diffuse_color * (p.diffuse_gain * under_layer_scale(p) / PI) *
+         (a + b * std::fmax(cos_dphi, 0.0) * sin_a * tan_b);
+}
+
+// bxdf.rs:167-171
+Vec3 eval_fuzz(const PbrParams& p, const Vec3& wo, const Vec3& wi) {
+  double rim = 0.5 * (powi4(1.0 - wi.z) + powi4(1.0 - wo.z));
+  return p.fuzz_color * (p.fuzz_gain / PI) * rim;
+}
+
+// bxdf.rs:173-179
+Vec3 eval_specular_lobe(const Vec3& f0, const Vec3& f90, double alpha, const Vec3& wo,
+                        const Vec3& wi) {
+  Vec3 h = (wo + wi).normalize();
+  double d = ggx_d(h, alpha);
+  double g = ggx_g2(wo, wi, alpha);
+  Vec3 f = schlick(f0, f90, std::fmax(wo.dot(h), 0.0));
+  return f * (d * g / std::fmax(4.0 * wo.z * wi.z, 1e-9));
+}
+
+// bxdf.rs:181-185
+double spec_pdf(const Vec3& wo, const Vec3& wi, double alpha) {
+  Vec3 h = (wo + wi).normalize();
+  double odh = std::fmax(wo.dot(h), 1e-9);
+  return ggx_pdf_h(wo, h, alpha) / (4.0 * odh);
+}
+
+// bxdf.rs:187-189
+double alpha_of(double roughness) { return clampd(roughness * roughness, 2.5e-5, 1.0); }
+
+// bxdf.rs:193-232
+void glass_transmit(const PbrParams& p, const Vec3& wo, const Vec3& wi, double eta, Vec3* f_out,
+                    double* pdf_out) {
+  *f_out = Vec3::zero();
+  *pdf_out = 0.0;
+  double alpha = alpha_of(p.glass_roughness);
+  Vec3 h = (wo + wi * eta).normalize();
+  if (h.z < 0.0) h = -h;
+  double odh = wo.dot(h), idh = wi.dot(h);
+  if (odh <= 0.0 || idh >= 0.0) return;
+  double fres = fresnel_dielectric(odh, eta);
+  double d = ggx_d(h, alpha);
+  double g = ggx_g2(wo, Vec3{wi.x, wi.y, -wi.z}, alpha);
+  double sqrt_denom = odh + eta * idh;
+  if (std::fabs(sqrt_denom) < 1e-9) return;
+  double factor = 1.0 / (eta * eta);
+  *f_out = p.refraction_color * (p.glass_gain * under_layer_scale(p) * (1.0 - fres) * d * g * factor *
+                                 std::fabs(idh * odh / (wi.z * wo.z)) * (eta * eta) /
+                                 (sqrt_denom * sqrt_denom));
+  double dwh_dwi = std::fabs(eta * eta * idh) / (sqrt_denom * sqrt_denom);
+  *pdf_out = ggx_pdf_h(wo, h, alpha) * dwh_dwi;
+}
+
+// bxdf.rs:236-238
+double glass_reflect_prob(const Vec3& wo, const Vec3& h, double eta) {
+  return fresnel_dielectric(std::fmax(wo.dot(h), 0.0), eta);
+}
+
+// bxdf.rs:301-306
+Vec3 cosine_sample_local(Pcg32& rng) {
+  double u = rng.next_f64(), v = rng.next_f64();
+  double r = std::sqrt(u);
+  double phi = 2.0 * PI * v;
+  return {r * std::cos(phi), r * std::sin(phi), std::sqrt(std::fmax(1.0 - u, 0.0))};
+}
+
+double power_heuristic(double a, double b) {  // pt/mod.rs:992-996
+  double a2 = a * a, b2 = b * b;
+  return (a2 + b2 <= 0.0) ? 0.0 : a2 / (a2 + b2);
+}
+
+}  // namespace
+
+// pbr.rs:84-93
+Vec3 PbrParams::specular_f0() const {
+  if (lum(specular_face_color) > 1e-6) return specular_face_color;
+  if (specular_ior > 1.0) {
+    double r = (specular_ior - 1.0) / (specular_ior + 1.0);
+    return Vec3::one() * (r * r);
+  }
+  return Vec3::zero();
+}
+
+bool PbrParams::has_specular() const { return lum(specular_f0()) > 1e-6; }
+
+// pbr.rs:191-202
+bool PbrParams::lobe_weights(double w[5]) const {
+  double wd = diffuse_gain * lum(diffuse_color);
+  double ws = std::sqrt(lum(specular_f0()));
+  double wc = clearcoat_gain * 0.2;
+  double wf = fuzz_gain * lum(fuzz_color) * 0.3;
+  double wg = glass_gain;
+  double total = wd + ws + wc + wf + wg;
+  if (total <= 1e-9) return false;
+  w[0] = wd / total;
+  w[1] = ws / total;
+  w[2] = wc / total;
+  w[3] = wf / total;
+  w[4] = wg / total;
+  return true;
+}
+
+// bxdf.rs:23-32
+Frame::Frame(const Vec3& n_) : n(n_) {
+  Vec3 t0 = std::fabs(n.x) > 0.9 ? Vec3{0.0, 1.0, 0.0} : Vec3{1.0, 0.0, 0.0};
+  t = n.cross(t0).normalize();
+  b = n.cross(t);
+}
+
+// bxdf.rs:241-299
+void bsdf_eval_pdf(const PbrParams& p, const Vec3& wo, const Vec3& wi, double eta_rel, Vec3* f_out,
+                   double* pdf_out) {
+  double w[5];
+  *f_out = Vec3::zero();
+  *pdf_out = 0.0;
+  if (!p.lobe_weights(w)) return;
+  const double wd = w[0], ws = w[1], wc = w[2], wf = w[3], wg = w[4];
+  if (wi.z > 0.0) {
+    Vec3 f = Vec3::zero();
+    double pdf = 0.0;
+    double cos_pdf = wi.z / PI;
+    if (wd > 0.0) {
+      f = f + eval_diffuse(p, wo, wi);
+      pdf += wd * cos_pdf;
+    }
+    if (wf > 0.0) {
+      f = f + eval_fuzz(p, wo, wi);
+      pdf += wf * cos_pdf;
+    }
+    if (ws > 0.0) {
+      double alpha = alpha_of(p.specular_roughness);
+      f = f + eval_specular_lobe(p.specular_f0(), p.specular_edge_color, alpha, wo, wi);
+      pdf += ws * spec_pdf(wo, wi, alpha);
+    }
+    if (wc > 0.0) {
+      double alpha = alpha_of(p.clearcoat_roughness);
+      Vec3 f0 = Vec3{0.04, 0.04, 0.04} * p.clearcoat_gain;
+      f = f + eval_specular_lobe(f0, Vec3::one() * p.clearcoat_gain, alpha, wo, wi);
+      pdf += wc * spec_pdf(wo, wi, alpha);
+    }
+    if (wg > 0.0) {
+      double alpha = alpha_of(p.glass_roughness);
+      Vec3 h = (wo + wi).normalize();
+      double fres = glass_reflect_prob(wo, h, eta_rel);
+      double d = ggx_d(h, alpha);
+      double g = ggx_g2(wo, wi, alpha);
+      f = f + Vec3::one() * (p.glass_gain * under_layer_scale(p) * fres * d * g /
+                             std::fmax(4.0 * wo.z * wi.z, 1e-9));
+      pdf += wg * fres * spec_pdf(wo, wi, alpha);
+    }
+    *f_out = f;
+    *pdf_out = pdf;
+  } else {
+    if (wg <= 0.0) return;
+    Vec3 f;
+    double pdf_lobe;
+    glass_transmit(p, wo, wi, eta_rel, &f, &pdf_lobe);
+    Vec3 h = (wo + wi * eta_rel).normalize();
+    if (h.z < 0.0) h = -h;
+    double fres = glass_reflect_prob(wo, h, eta_rel);
+    *f_out = f;
+    *pdf_out = wg * (1.0 - fres) * pdf_lobe;
+  }
+}
+
+// bxdf.rs:309-374
+bool bsdf_sample(const PbrParams& p, const Vec3& wo, double eta_rel, Pcg32& rng, BsdfSample* s) {
+  double w[5];
+  if (!p.lobe_weights(w)) return false;
+  const double wd = w[0], ws = w[1], wc = w[2], wf = w[3], wg = w[4];
+  double pick = rng.next_f64();
+  bool transmitted = false;
+  bool specular_lobe = pick >= wd + wf;
+  Vec3 wi;
+  if (pick < wd + wf) {
+    wi = cosine_sample_local(rng);
+  } else if (pick < wd + wf + ws) {
+    double alpha = alpha_of(p.specular_roughness);
+    double u1 = rng.next_f64(), u2 = rng.next_f64();
+    Vec3 h = ggx_sample_vndf(wo, alpha, u1, u2);
+    wi = (-wo).reflect(h);
+    if (wi.z <= 0.0) return false;
+  } else if (pick < wd + wf + ws + wc) {
+    double alpha = alpha_of(p.clearcoat_roughness);
+    double u1 = rng.next_f64(), u2 = rng.next_f64();
+    Vec3 h = ggx_sample_vndf(wo, alpha, u1, u2);
+    wi = (-wo).reflect(h);
+    if (wi.z <= 0.0) return false;
+  } else if (wg > 0.0) {
+    double alpha = alpha_of(p.glass_roughness);
+    double u1 = rng.next_f64(), u2 = rng.next_f64();
+    Vec3 h = ggx_sample_vndf(wo, alpha, u1, u2);
+    double fres = glass_reflect_prob(wo, h, eta_rel);
+    if (rng.next_f64() < fres) {
+      wi = (-wo).reflect(h);
+      if (wi.z <= 0.0) return false;
+    } else {
+      double cos_i = wo.dot(h);
+      double sin2_t = (1.0 - cos_i * cos_i) / (eta_rel * eta_rel);
+      if (sin2_t >= 1.0) return false;
+      double cos_t = std::sqrt(1.0 - sin2_t);
+      wi = ((-wo) / eta_rel + h * (cos_i / eta_rel - cos_t)).normalize();
+      if (wi.z >= 0.0) return false;
+      transmitted = true;
+    }
+  } else {
+    return false;
+  }
+  Vec3 f;
+  double pdf;
+  bsdf_eval_pdf(p, wo, wi, eta_rel, &f, &pdf);
+  if (pdf <= 0.0) return false;
+  s->wi = wi;
+  s->f = f;
+  s->pdf = pdf;
+  s->transmitted = transmitted;
+  s->specular_lobe = specular_lobe;
+  return true;
+}
+
+// ---- light BVH (light_sampler.rs) -------------------------------------------
+
+namespace {
+
+double lum_clamped(const Vec3& c) { return std::fmax(0.2126 * c.x + 0.7152 * c.y + 0.0722 * c.z, 1e-9); }
+
+// light_sampler.rs:51-70
+bool finite_light_info(const Light& l, Vec3* center, double* radius, double* power) {
+  double pw = lum_clamped(l.radiance());
+  switch (l.kind) {
+    case RRHIP_LIGHT_POINT:
+      *center = l.position;
+      *radius = 0.0;
+      *power = pw * 4.0 * PI;
+      return true;
+    case RRHIP_LIGHT_RECT:
+      *center = l.corner + l.edge1 * 0.5 + l.edge2 * 0.5;
+      *radius = (l.edge1.length() + l.edge2.length()) * 0.5;
+      *power = pw * l.area * PI;
+      return true;
+    case RRHIP_LIGHT_SPHERE:
+      *center = l.center;
+      *radius = l.radius;
+      *power = pw * 4.0 * PI * l.radius * l.radius * PI;
+      return true;
+    case RRHIP_LIGHT_DISK:
+      *center = l.center;
+      *radius = l.e1.length();
+      *power = pw * l.area * PI;
+      return true;
+    default:
+      return false;
+  }
+}
+
+// light_sampler.rs:74-87
+double node_importance(const LightBvhNode& n, const Vec3& p) {
+  double cx = ((double)n.min[0] + (double)n.max[0]) * 0.5;
+  double cy = ((double)n.min[1] + (double)n.max[1]) * 0.5;
+  double cz = ((double)n.min[2] + (double)n.max[2]) * 0.5;
+  double dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
+  double d2 = dx * dx + dy * dy + dz * dz;
+  double rx = (double)(n.max[0] - n.min[0]) * 0.5;
+  double ry = (double)(n.max[1] - n.min[1]) * 0.5;
+  double rz = (double)(n.max[2] - n.min[2]) * 0.5;
+  double r2 = rx * rx + ry * ry + rz * rz;
+  return (double)n.power / std::fmax(std::fmax(d2, r2), 1e-6);
+}
+
+struct BuildEntry {
+  uint32_t light;
+  Vec3 center;
+  double radius, power;
+};
+
+// light_sampler.rs:242-312 (median split on the widest axis)
+uint32_t build_recursive(const std::vector<BuildEntry>& e, std::vector<size_t>& order, size_t start,
+                         size_t end, uint32_t parent, std::vector<LightBvhNode>& nodes,
+                         std::vector<uint32_t>& light_leaf) {
+  uint32_t index = (uint32_t)nodes.size();
+  float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  double power = 0.0;
+  for (size_t k = start; k < end; k++) {
+    const BuildEntry& b = e[order[k]];
+    double c[3] = {b.center.x, b.center.y, b.center.z};
+    for (int a = 0; a < 3; a++) {
+      mn[a] = std::fmin(mn[a], (float)(c[a] - b.radius));
+      mx[a] = std::fmax(mx[a], (float)(c[a] + b.radius));
+    }
+    power += b.power;
+  }
+  LightBvhNode node{};
+  for (int a = 0; a < 3; a++) {
+    node.min[a] = mn[a];
+    node.max[a] = mx[a];
+  }
+  node.power = (float)power;
+  node.parent = parent;
+  node.a = 0;
+  node.b = 0xFFFFFFFFu;
+  nodes.push_back(node);
+  if (end - start == 1) {
+    uint32_t light = e[order[start]].light;
+    nodes[index].a = light;
+    light_leaf[light] = index;
+    return index;
+  }
+  int axis = 0;
+  float widest = -1.0f;
+  for (int a = 0; a < 3; a++) {
+    float w = mx[a] - mn[a];
+    if (w > widest) {
+      widest = w;
+      axis = a;
+    }
+  }
+  size_t mid = (start + end) / 2;
+  std::stable_sort(order.begin() + start, order.begin() + end, [&](size_t x, size_t y) {
+    return e[x].center[axis] < e[y].center[axis];
+  });
+  uint32_t a = build_recursive(e, order, start, mid, index, nodes, light_leaf);
+  uint32_t b = build_recursive(e, order, mid, end, index, nodes, light_leaf);
+  nodes[index].a = a;
+  nodes[index].b = b;
+  return index;
+}
+
+}  // namespace
+
+// light_sampler.rs:97-142
+void LightSampler::build(const std::vector<Light>& lights) {
+  nodes.clear();
+  infinite.clear();
+  infinite_power.clear();
+  std::vector<BuildEntry> finite;
+  for (size_t i = 0; i < lights.size(); i++) {
+    Vec3 c;
+    double r, pw;
+    if (finite_light_info(lights[i], &c, &r, &pw)) {
+      finite.push_back({(uint32_t)i, c, r, pw});
+    } else {
+      infinite.push_back((uint32_t)i);
+      infinite_power.push_back(lum_clamped(lights[i].radiance()) * 4.0 * PI);
+    }
+  }
+  infinite_total = 0;
+  for (double w : infinite_power) infinite_total += w;
+  light_leaf.assign(lights.size(), 0xFFFFFFFFu);
+  if (!finite.empty()) {
+    std::vector<size_t> order(finite.size());
+    for (size_t k = 0; k < order.size(); k++) order[k] = k;
+    build_recursive(finite, order, 0, finite.size(), 0xFFFFFFFFu, nodes, light_leaf);
+  }
+  double finite_total = 0;
+  for (auto& f : finite) finite_total += f.power;
+  if (infinite.empty())
+    p_infinite = 0.0;
+  else if (finite.empty())
+    p_infinite = 1.0;
+  else
+    p_infinite = clampd(infinite_total / (infinite_total + finite_total), 0.1, 0.9);
+}
+
+// light_sampler.rs:150-208
+bool LightSampler::sample(const Vec3& p, double u, size_t* light, double* pmf_out) const {
+  bool has_finite = !nodes.empty(), has_infinite = !infinite.empty();
+  if (!has_finite && !has_infinite) return false;
+  double pmf = 1.0;
+  bool inf;
+  if (has_finite && has_infinite) {
+    if (u < p_infinite) {
+      u /= p_infinite;
+      pmf *= p_infinite;
+      inf = true;
+    } else {
+      u = (u - p_infinite) / (1.0 - p_infinite);
+      pmf *= 1.0 - p_infinite;
+      inf = false;
+    }
+  } else {
+    inf = has_infinite;
+  }
+  if (inf) {
+    double acc = 0.0, target = u * infinite_total;
+    for (size_t k = 0; k < infinite_power.size(); k++) {
+      acc += infinite_power[k];
+      if (target <= acc || k == infinite.size() - 1) {
+        *light = infinite[k];
+        *pmf_out = pmf * infinite_power[k] / std::fmax(infinite_total, 1e-12);
+        return true;
+      }
+    }
+    return false;
+  }
+  size_t node = 0;
+  for (;;) {
+    const LightBvhNode& nd = nodes[node];
+    if (nd.b == 0xFFFFFFFFu) {
+      *light = nd.a;
+      *pmf_out = pmf;
+      return true;
+    }
+    double ia = node_importance(nodes[nd.a], p), ib = node_importance(nodes[nd.b], p);
+    double total = ia + ib;
+    double pa = total > 1e-30 ? ia / total : 0.5;
+    if (u < pa) {
+      u = std::fmin(u / pa, 1.0 - 1e-12);
+      pmf *= pa;
+      node = nd.a;
+    } else {
+      u = std::fmin((u - pa) / (1.0 - pa), 1.0 - 1e-12);
+      pmf *= 1.0 - pa;
+      node = nd.b;
+    }
+  }
+}
+
+// light_sampler.rs:212-238
+double LightSampler::pmf(const Vec3& p, size_t light) const {
+  bool has_finite = !nodes.empty(), has_infinite = !infinite.empty();
+  for (size_t k = 0; k < infinite.size(); k++) {
+    if (infinite[k] == light) {
+      double group = has_finite ? p_infinite : 1.0;
+      return group * infinite_power[k] / std::fmax(infinite_total, 1e-12);
+    }
+  }
+  uint32_t leaf = light_leaf[light];
+  if (leaf == 0xFFFFFFFFu) return 0.0;
+  double pmf = has_infinite ? 1.0 - p_infinite : 1.0;
+  uint32_t node = leaf;
+  while (nodes[node].parent != 0xFFFFFFFFu) {
+    uint32_t parent = nodes[node].parent;
+    const LightBvhNode& pn = nodes[parent];
+    double ia = node_importance(nodes[pn.a], p), ib = node_importance(nodes[pn.b], p);
+    double total = ia + ib;
+    double mine = node == pn.a ? ia : ib;
+    pmf *= total > 1e-30 ? mine / total : 0.5;
+    node = parent;
+  }
+  return pmf;
+}
+
+// ---- camera (camera.rs) -------------------------------------------------------
+
+// camera.rs:36-61
+double Camera::filter_1d(double u) const {
+  switch (filter) {
+    case 1: {
+      double half = filter_width * 0.5;
+      return u < 0.5 ? (std::sqrt(2.0 * u) - 1.0) * half : (1.0 - std::sqrt(2.0 * (1.0 - u))) * half;
+    }
+    case 2: {
+      double u1 = std::fmax(u, 1e-12);
+      double x = u * 2654435761.0;
+      double u2 = x - std::floor(x);
+      double sigma = filter_width / 4.0;
+      double g = std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * PI * u2);
+      return clampd(g * sigma, -filter_width * 0.5, filter_width * 0.5);
+    }
+    default:
+      return (u - 0.5) * filter_width;
+  }
+}
+
+// camera.rs:150-179 (pinhole)
+Ray Camera::generate_ray(double px, double py) const {
+  double u = (px / (double)width) * 2.0 - 1.0;
+  double v = 1.0 - (py / (double)height) * 2.0;
+  double aspect = (double)width / (double)height;
+  double half_height = std::tan(fov * PI / 180.0 / 2.0);
+  double half_width = aspect * half_height;
+  Vec3 dir = forward + right * (u * half_width) + up * (v * half_height);
+  return Ray::make(eye, dir);
+}
+
+// ---- geometry -----------------------------------------------------------------
+
+// extra.rs:673-720 with an identity transform (world-space vertices)
+bool TriangleObj::intersect(const Ray& ray, Intersection* out) const {
+  Vec3 lo = ray.origin, ld = ray.direction.normalize();
+  Vec3 e1 = v1 - v0, e2 = v2 - v0;
+  Vec3 pvec = ld.cross(e2);
+  double det = e1.dot(pvec);
+  if (std::fabs(det) < 1e-12) return false;
+  double inv_det = 1.0 / det;
+  Vec3 tvec = lo - v0;
+  double u = tvec.dot(pvec) * inv_det;
+  if (!(u >= 0.0 && u <= 1.0)) return false;
+  Vec3 qvec = tvec.cross(e1);
+  double v = ld.dot(qvec) * inv_det;
+  if (v < 0.0 || u + v > 1.0) return false;
+  double t = e2.dot(qvec) * inv_det;
+  if (t <= EPSILON) return false;
+  Vec3 p = lo + ld * t;
+  Vec3 n = e1.cross(e2).normalize();
+  bool front = n.dot(ld) < 0.0;
+  if (!front) n = -n;
+  out->t = (p - ray.origin).length();
+  out->point = p;
+  out->normal = n.normalize();
+  out->front_face = front;
+  out->material = material;
+  return true;
+}
+
+// sphere.rs:38-109 (full sphere, translation transform)
+bool SphereObj::intersect(const Ray& ray, Intersection* out) const {
+  Vec3 lo = ray.origin - center;
+  Vec3 ld = ray.direction.normalize();
+  double a = ld.dot(ld);
+  double b = 2.0 * lo.dot(ld);
+  double c = lo.dot(lo) - radius * radius;
+  double disc = b * b - 4.0 * a * c;
+  if (disc < 0.0) return false;
+  double sq = std::sqrt(disc);
+  double t1 = (-b - sq) / (2.0 * a), t2 = (-b + sq) / (2.0 * a);
+  double t;
+  if (t1 > EPSILON)
+    t = t1;
+  else if (t2 > EPSILON)
+    t = t2;
+  else
+    return false;
+  Vec3 lp = lo + ld * t;
+  Vec3 wp = lp + center;
+  out->t = (wp - ray.origin).length();
+  out->point = wp;
+  out->normal = lp.normalize().normalize();
+  out->front_face = ray.direction.dot(out->normal) < 0.0;
+  out->material = material;
+  return true;
+}
+
+// mesh.rs:308-381 -> the parametric hit and world normal of one instance
+bool Scene::instance_hit(const InstanceObj& inst, const Ray& ray, double t_max, double* t_param,
+                         Vec3* world_normal) const {
+  Ray local = Ray::raw(inst.inverse.transform_point(ray.origin), inst.inverse.transform_vec(ray.direction));
+  const MeshObj& mo = meshes[inst.mesh];
+  const Mesh& m = mo.mesh;
+  double hu = 0, hv = 0;
+  auto isect = [&](uint32_t tri, double tcur, double* tout) {
+    double tt, uu, vv;
+    if (!m.intersect_triangle(local, tri, 1e-9, tcur, &tt, &uu, &vv) || !(tt < tcur)) return false;
+    hu = uu;
+    hv = vv;
+    *tout = tt;
+    return true;
+  };
+  uint32_t tri;
+  double t;
+  if (!m.blas.traverse(local, t_max, isect, &tri, &t)) return false;
+  // The payload of the winning triangle (the traversal keeps the closest).
+  double tt;
+  m.intersect_triangle(local, tri, 1e-9, INF, &tt, &hu, &hv);
+  uint32_t i0 = m.indices[3 * tri], i1 = m.indices[3 * tri + 1], i2 = m.indices[3 * tri + 2];
+  Vec3 ln;
+  if (!mo.normals.empty()) {
+    auto nv = [&](uint32_t i) {
+      return Vec3{(double)mo.normals[3 * i], (double)mo.normals[3 * i + 1], (double)mo.normals[3 * i + 2]};
+    };
+    double w = 1.0 - hu - hv;
+    ln = (nv(i0) * w + nv(i1) * hu + nv(i2) * hv).normalize();
+  } else {
+    Vec3 v0 = m.vertex(i0);
+    ln = (m.vertex(i1) - v0).cross(m.vertex(i2) - v0).normalize();
+  }
+  *t_param = t;
+  *world_normal = inst.inverse.transform_normal(ln).normalize();
+  return true;
+}
+
+// scene/mod.rs:105-146
+bool Scene::intersect(const Ray& ray, Intersection* out) const {
+  bool found = false;
+  double closest_t = INF;
+  Intersection h;
+  for (const auto& tri : triangles) {
+    if (tri.intersect(ray, &h) && h.t < closest_t) {
+      closest_t = h.t;
+      *out = h;
+      found = true;
+    }
+  }
+  for (const auto& sp : spheres) {
+    if (sp.intersect(ray, &h) && h.t < closest_t) {
+      closest_t = h.t;
+      *out = h;
+      found = true;
+    }
+  }
+  if (!instances.empty()) {
+    double dir_len = std::fmax(ray.direction.length(), 1e-300);
+    double budget = closest_t / dir_len;
+    uint32_t best_inst = 0;
+    double best_t = 0;
+    Vec3 best_n;
+    auto isect = [&](uint32_t id, double tcur, double* tout) {
+      double tp;
+      Vec3 wn;
+      if (!instance_hit(instances[id], ray, tcur, &tp, &wn)) return false;
+      if (!(tp < tcur)) return false;
+      *tout = tp;
+      return true;
+    };
+    uint32_t win;
+    double wt;
+    if (tlas.traverse(ray, budget, isect, &win, &wt)) {
+      best_inst = win;
+      best_t = wt;
+      instance_hit(instances[best_inst], ray, INF, &best_t, &best_n);
+      // mesh.rs:427-439 (to_intersection)
+      Vec3 p = ray.at(best_t);
+      Vec3 n = best_n;
+      bool front = n.dot(ray.direction) < 0.0;
+      if (!front) n = -n;
+      double t = (p - ray.origin).length();
+      if (t < closest_t) {
+        out->t = t;
+        out->point = p;
+        out->normal = n.normalize();
+        out->front_face = front;
+        out->material = instances[best_inst].material;
+        found = true;
+      }
+    }
+  }
+  return found;
+}
+
+// ---- integrator (pt/mod.rs) -------------------------------------------------------
+
+// pt/mod.rs:340-356 (no lens, no motion)
+Ray Scene::camera_ray(uint32_t x, uint32_t y, Pcg32& rng) const {
+  double u1 = rng.next_f64(), u2 = rng.next_f64();
+  double dx = camera.filter_1d(u1), dy = camera.filter_1d(u2);
+  return camera.generate_ray((double)x + 0.5 + dx, (double)y + 0.5 + dy);
+}
+
+// pt/mod.rs:1243-1306 for opaque scenes: visible unless something is hit
+// before `remaining`.
+double Scene::visibility(const Vec3& origin, const Vec3& dir, double remaining, uint64_t* rays) const {
+  if (rays) ++*rays;
+  Intersection h;
+  Ray r = Ray::make(origin, dir);
+  if (!intersect(r, &h)) return 1.0;
+  return h.t >= remaining ? 1.0 : 0.0;
+}
+
+// pt/mod.rs:1177-1209
+Vec3 Scene::area_contribution(const Vec3& p, const Vec3& n, const Vec3& sample_point,
+                              const Vec3& light_normal, double area, const Light& light,
+                              const PbrParams& pbr, const Frame& frame, const Vec3& wo_l,
+                              double eta_rel, double pick_pmf, uint64_t* rays) const {
+  Vec3 to_light = sample_point - p;
+  double dist2 = std::fmax(to_light.length_squared(), 1e-12);
+  Vec3 wi = to_light.normalize();
+  double cos_light = std::fabs(wi.dot(light_normal));
+  if (cos_light < 1e-9 || area <= 0.0) return Vec3::zero();
+  Vec3 wi_l = frame.to_local(wi);
+  if (wi_l.z <= 0.0) return Vec3::zero();
+  Vec3 f;
+  double bsdf_pdf;
+  bsdf_eval_pdf(pbr, wo_l, wi_l, eta_rel, &f, &bsdf_pdf);
+  f = f * wi_l.z;
+  if (max_component(f) <= 0.0) return Vec3::zero();
+  // visibility_to (pt/mod.rs:1212-1224)
+  Vec3 dir = sample_point - p;
+  double dist = dir.length();
+  double vis = visibility(p + n * 1e-4, dir.normalize(), dist - 1e-3, rays);
+  if (vis <= 0.0) return Vec3::zero();
+  double pdf_sa = pick_pmf * dist2 / (cos_light * area);
+  double weight = power_heuristic(pdf_sa, bsdf_pdf);
+  return f * light.radiance() * vis * (weight / pdf_sa);
+}
+
+// pt/mod.rs:1032-1174
+Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const PbrParams& pbr,
+                         const Frame& frame, const Vec3& wo_l, double eta_rel, double pick_pmf,
+                         Pcg32& rng, uint64_t* rays) const {
+  auto eval = [&](const Vec3& wi_world, Vec3* f, double* pdf) {
+    Vec3 wi_l = frame.to_local(wi_world);
+    if (wi_l.z <= 0.0) {
+      *f = Vec3::zero();
+      *pdf = 0.0;
+      return;
+    }
+    bsdf_eval_pdf(pbr, wo_l, wi_l, eta_rel, f, pdf);
+    *f = *f * wi_l.z;
+  };
+  switch (light.kind) {
+    case RRHIP_LIGHT_POINT: {
+      Vec3 to_light = light.position - p;
+      double dist2 = std::fmax(to_light.length_squared(), 1e-12);
+      Vec3 wi = to_light.normalize();
+      Vec3 f;
+      double pdf;
+      eval(wi, &f, &pdf);
+      if (max_component(f) <= 0.0) return Vec3::zero();
+      Vec3 dir = light.position - p;
+      double vis = visibility(p + n * 1e-4, dir.normalize(), dir.length() - 1e-3, rays);
+      if (vis <= 0.0) return Vec3::zero();
+      return f * light.radiance() * vis / (dist2 * pick_pmf);
+    }
+    case RRHIP_LIGHT_DISTANT: {
+      Vec3 base = -light.direction;
+      Vec3 wi = base;
+      if (light.angular_radius > 1e-5) {
+        double cos_max = std::cos(light.angular_radius);
+        double u = rng.next_f64(), v = rng.next_f64();
+        double cos_t = 1.0 - u * (1.0 - cos_max);
+        double sin_t = std::sqrt(std::fmax(1.0 - cos_t * cos_t, 0.0));
+        double phi = 2.0 * PI * v;
+        Frame lf(base);
+        wi = lf.to_world(Vec3{sin_t * std::cos(phi), sin_t * std::sin(phi), cos_t});
+      }
+      Vec3 f;
+      double pdf;
+      eval(wi, &f, &pdf);
+      if (max_component(f) <= 0.0) return Vec3::zero();
+      double vis = visibility(p + n * 1e-4, wi, 1e8, rays);
+      if (vis <= 0.0) return Vec3::zero();
+      return f * light.radiance() * vis / pick_pmf;
+    }
+    case RRHIP_LIGHT_RECT: {
+      double u = rng.next_f64(), v = rng.next_f64();
+      Vec3 sp = light.corner + light.edge1 * u + light.edge2 * v;
+      return area_contribution(p, n, sp, light.normal, light.area, light, pbr, frame, wo_l, eta_rel,
+                               pick_pmf, rays);
+    }
+    case RRHIP_LIGHT_DISK: {
+      double u = rng.next_f64(), v = rng.next_f64();
+      double r = std::sqrt(u), phi = 2.0 * PI * v;
+      Vec3 sp = light.center + light.e1 * (r * std::cos(phi)) + light.e2 * (r * std::sin(phi));
+      return area_contribution(p, n, sp, light.normal, light.area, light, pbr, frame, wo_l, eta_rel,
+                               pick_pmf, rays);
+    }
+    case RRHIP_LIGHT_SPHERE: {
+      Vec3 to_center = light.center - p;
+      double dist2 = to_center.length_squared();
+      double r = light.radius;
+      if (dist2 <= r * r * 1.0001) return Vec3::zero();
+      double sin2 = std::fmin(r * r / dist2, 1.0);
+      double cos_max = std::sqrt(std::fmax(1.0 - sin2, 0.0));
+      double u = rng.next_f64(), v = rng.next_f64();
+      double cos_t = 1.0 - u * (1.0 - cos_max);
+      double sin_t = std::sqrt(std::fmax(1.0 - cos_t * cos_t, 0.0));
+      double phi = 2.0 * PI * v;
+      Frame lf(to_center.normalize());
+      Vec3 wi = lf.to_world(Vec3{sin_t * std::cos(phi), sin_t * std::sin(phi), cos_t});
+      Vec3 f;
+      double bsdf_pdf;
+      eval(wi, &f, &bsdf_pdf);
+      if (max_component(f) <= 0.0) return Vec3::zero();
+      double dist = std::sqrt(dist2) * cos_t - std::sqrt(std::fmax(r * r - dist2 * sin_t * sin_t, 0.0));
+      Vec3 target = p + wi * dist;
+      Vec3 dir = target - p;
+      double vis = visibility(p + n * 1e-4, dir.normalize(), dir.length() - 1e-3, rays);
+      if (vis <= 0.0) return Vec3::zero();
+      double pdf_sa = pick_pmf / std::fmax(2.0 * PI * (1.0 - cos_max), 1e-12);
+      double weight = power_heuristic(pdf_sa, bsdf_pdf);
+      return f * light.radiance() * vis * (weight / pdf_sa);
+    }
+    case RRHIP_LIGHT_DOME: {
+      double u = rng.next_f64(), v = rng.next_f64();
+      double z = 1.0 - 2.0 * u;
+      double rr = std::sqrt(std::fmax(1.0 - z * z, 0.0));
+      double phi = 2.0 * PI * v;
+      Vec3 wi{rr * std::cos(phi), rr * std::sin(phi), z};
+      double pdf_sa = (1.0 / (4.0 * PI)) * pick_pmf;
+      Vec3 f;
+      double bsdf_pdf;
+      eval(wi, &f, &bsdf_pdf);
+      if (max_component(f) <= 0.0) return Vec3::zero();
+      double vis = visibility(p + n * 1e-4, wi, 1e8, rays);
+      if (vis <= 0.0) return Vec3::zero();
+      double weight = power_heuristic(pdf_sa, bsdf_pdf);
+      return f * light.radiance() * vis * (weight / pdf_sa);
+    }
+    default:
+      return Vec3::zero();
+  }
+}
+
+// pt/mod.rs:1000-1023
+static double light_pdf_solid_angle(const Light& light, const Vec3& origin, const Intersection& hit) {
+  if (light.kind == RRHIP_LIGHT_RECT || light.kind == RRHIP_LIGHT_DISK) {
+    Vec3 d = hit.point - origin;
+    double dist2 = d.length_squared();
+    double cos_light = std::fabs(d.normalize().dot(light.normal));
+    if (cos_light < 1e-9 || light.area <= 0.0) return 0.0;
+    return dist2 / (cos_light * light.area);
+  }
+  if (light.kind == RRHIP_LIGHT_SPHERE) {
+    double dist2 = (light.center - origin).length_squared();
+    double sin2 = std::fmin(light.radius * light.radius / dist2, 1.0);
+    double cos_max = std::sqrt(std::fmax(1.0 - sin2, 0.0));
+    double sa = 2.0 * PI * (1.0 - cos_max);
+    if (sa < 1e-12) return 0.0;
+    return 1.0 / sa;
+  }
+  return 0.0;
+}
+
+// pt/mod.rs:449-990, surface subset
+Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) const {
+  Vec3 l = Vec3::zero(), beta = Vec3::one();
+  double prev_pdf = 0.0;
+  Vec3 prev_origin = ray.origin;
+  bool from_camera = true;
+  uint32_t depth = 0;
+  while (depth < rp.max_depth) {
+    if (rays) ++*rays;
+    Intersection hit;
+    if (!intersect(ray, &hit)) {
+      if (dome >= 0) {
+        const Light& d = lights[dome];
+        double weight = 1.0;
+        if (!from_camera) {
+          double pick = light_sampler.pmf(prev_origin, (size_t)dome);
+          weight = power_heuristic(prev_pdf, (1.0 / (4.0 * PI)) * pick);
+        }
+        l = l + beta * d.radiance() * weight;
+      } else {
+        l = l + beta * background;
+      }
+      break;
+    }
+    const Material& mat = materials[hit.material];
+    const PbrParams& pbr = mat.pbr;
+    Vec3 wo = -ray.direction.normalize();
+    bool entering = hit.front_face;
+    Vec3 n = hit.normal.dot(wo) >= 0.0 ? hit.normal : -hit.normal;
+    double eta_rel = entering ? pbr.glass_ior : 1.0 / pbr.glass_ior;
+    Vec3 emission = mat.emission + pbr.glow;
+    if (max_component(emission) > 0.0) {
+      double weight = 1.0;
+      if (!from_camera && mat.area_light >= 0) {
+        const Light& lt = lights[mat.area_light];
+        double pick = light_sampler.pmf(prev_origin, (size_t)mat.area_light);
+        double pdf_light = light_pdf_solid_angle(lt, prev_origin, hit) * pick;
+        weight = power_heuristic(prev_pdf, pdf_light);
+      }
+      l = l + beta * emission * weight;
+    }
+    Frame frame(n);
+    Vec3 wo_l = frame.to_local(wo);
+    double w5[5];
+    if (!lights.empty() && pbr.lobe_weights(w5)) {
+      size_t li;
+      double pmf;
+      if (light_sampler.sample(hit.point, rng.next_f64(), &li, &pmf)) {
+        Vec3 c = sample_light(lights[li], hit.point, n, pbr, frame, wo_l, eta_rel, pmf, rng, rays);
+        l = l + beta * c;
+      }
+    }
+    BsdfSample s;
+    if (!bsdf_sample(pbr, wo_l, eta_rel, rng, &s)) break;
+    Vec3 wi_world = frame.to_world(s.wi);
+    beta = beta * s.f * (std::fabs(s.wi.z) / s.pdf);
+    prev_pdf = s.pdf;
+    prev_origin = hit.point;
+    from_camera = false;
+    double offset = s.transmitted ? -1e-4 : 1e-4;
+    ray = Ray::make(hit.point + n * offset, wi_world);
+    depth++;
+    if (depth >= rp.rr_start) {
+      double q = std::fmin(max_component(beta), 0.95);
+      if (q <= 0.0 || rng.next_f64() > q) break;
+      beta = beta / q;
+    }
+  }
+  return l;
+}
+
+// ---- scene loading ------------------------------------------------------------------
+
+static bool mat_inverse(const Mat4& m, Mat4* out) {
+  // Gauss-Jordan with partial pivoting (f64).
+  double a[4][8];
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) {
+      a[i][j] = m.m[i][j];
+      a[i][j + 4] = i == j ? 1.0 : 0.0;
+    }
+  for (int c = 0; c < 4; c++) {
+    int piv = c;
+    for (int r = c + 1; r < 4; r++)
+      if (std::fabs(a[r][c]) > std::fabs(a[piv][c])) piv = r;
+    if (std::fabs(a[piv][c]) < 1e-300) return false;
+    if (piv != c)
+      for (int j = 0; j < 8; j++) std::swap(a[c][j], a[piv][j]);
+    double inv = 1.0 / a[c][c];
+    for (int j = 0; j < 8; j++) a[c][j] *= inv;
+    for (int r = 0; r < 4; r++) {
+      if (r == c) continue;
+      double f = a[r][c];
+      if (f != 0.0)
+        for (int j = 0; j < 8; j++) a[r][j] -= f * a[c][j];
+    }
+  }
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) out->m[i][j] = a[i][j + 4];
+  return true;
+}
+
+bool Mat4::inverse(Mat4& out) const { return mat_inverse(*this, &out); }
+
+void Scene::load(const rrhip_scene_desc& d) {
+  const rrhip_camera_desc& c = d.camera;
+  camera.width = c.width;
+  camera.height = c.height;
+  camera.fov = c.fov_deg;
+  camera.eye = v3(c.eye);
+  camera.filter = (int)c.filter;
+  camera.filter_width = c.filter_width > 0 ? c.filter_width : 1.0;
+  // RenderMan frame of Camera::new (camera.rs:96-119) aimed along look_at
+  camera.forward = (v3(c.look_at) - camera.eye).normalize();
+  camera.right = v3(c.up).cross(camera.forward).normalize();
+  camera.up = camera.forward.cross(camera.right).normalize();
+  background = v3(d.background);
+
+  materials.clear();
+  for (uint32_t i = 0; i < d.num_materials; i++) {
+    const rrhip_material_desc& m = d.materials[i];
+    Material mt;
+    PbrParams& p = mt.pbr;
+    p.diffuse_gain = m.diffuse_gain;
+    p.diffuse_color = v3(m.diffuse_color);
+    p.diffuse_roughness = m.diffuse_roughness;
+    p.specular_face_color = v3(m.specular_face_color);
+    p.specular_edge_color = v3(m.specular_edge_color);
+    p.specular_roughness = m.specular_roughness;
+    p.specular_ior = m.specular_ior;
+    p.clearcoat_gain = m.clearcoat_gain;
+    p.clearcoat_roughness = m.clearcoat_roughness;
+    p.fuzz_gain = m.fuzz_gain;
+    p.fuzz_color = v3(m.fuzz_color);
+    p.glass_gain = m.glass_gain;
+    p.glass_ior = m.glass_ior;
+    p.glass_roughness = m.glass_roughness;
+    p.refraction_color = v3(m.refraction_color);
+    p.glow = v3(m.glow);
+    mt.emission = v3(m.emission);
+    mt.area_light = m.area_light;
+    materials.push_back(mt);
+  }
+
+  lights.clear();
+  dome = -1;
+  for (uint32_t i = 0; i < d.num_lights; i++) {
+    const rrhip_light_desc& ld = d.lights[i];
+    Light l;
+    l.kind = (int)ld.kind;
+    l.intensity = ld.intensity;
+    l.color = v3(ld.color);
+    switch (l.kind) {
+      case RRHIP_LIGHT_POINT:
+        l.position = v3(ld.p);
+        break;
+      case RRHIP_LIGHT_DISTANT:
+        l.direction = v3(ld.p).normalize();
+        l.angular_radius = ld.radius;
+        break;
+      case RRHIP_LIGHT_RECT: {  // light.rs:99-116
+        l.corner = v3(ld.p);
+        l.edge1 = v3(ld.e1);
+        l.edge2 = v3(ld.e2);
+        Vec3 cr = l.edge1.cross(l.edge2);
+        l.area = cr.length();
+        l.normal = cr.normalize();
+        break;
+      }
+      case RRHIP_LIGHT_SPHERE:
+        l.center = v3(ld.p);
+        l.radius = ld.radius;
+        break;
+      case RRHIP_LIGHT_DISK:  // light.rs:129-140
+        l.center = v3(ld.p);
+        l.e1 = v3(ld.e1);
+        l.e2 = v3(ld.e2);
+        l.normal = l.e1.cross(l.e2).normalize();
+        l.area = PI * l.e1.length() * l.e2.length();
+        break;
+      case RRHIP_LIGHT_DOME:
+        if (dome < 0) dome = (int)i;
+        break;
+    }
+    lights.push_back(l);
+  }
+  light_sampler.build(lights);
+
+  triangles.clear();
+  for (uint32_t i = 0; i < d.num_triangles; i++) {
+    const rrhip_triangle_desc& t = d.triangles[i];
+    triangles.push_back({v3(t.v0), v3(t.v1), v3(t.v2), t.material});
+  }
+  spheres.clear();
+  for (uint32_t i = 0; i < d.num_spheres; i++) {
+    const rrhip_sphere_desc& s = d.spheres[i];
+    spheres.push_back({v3(s.center), s.radius, s.material});
+  }
+  meshes.clear();
+  meshes.resize(d.num_meshes);
+  BvhParams bp;
+  for (uint32_t i = 0; i < d.num_meshes; i++) {
+    const rrhip_mesh_desc& md = d.meshes[i];
+    meshes[i].mesh.build(md.positions, md.num_vertices, md.indices, md.num_triangles, bp);
+    if (md.normals) meshes[i].normals.assign(md.normals, md.normals + 3 * md.num_vertices);
+  }
+  instances.clear();
+  std::vector<Aabb> ib;
+  for (uint32_t i = 0; i < d.num_instances; i++) {
+    const rrhip_instance_desc& id = d.instances[i];
+    InstanceObj in;
+    in.mesh = id.mesh;
+    in.material = id.material;
+    in.transform = Mat4::identity();
+    for (int r = 0; r < 3; r++)
+      for (int cc = 0; cc < 4; cc++) in.transform.m[r][cc] = id.object_to_world[4 * r + cc];
+    if (!in.transform.inverse(in.inverse)) in.inverse = Mat4::identity();
+    // mesh.rs:266-292 world bounds from the 8 transformed corners
+    const Aabb& lb = meshes[in.mesh].mesh.local_bounds;
+    Aabb wb;
+    for (int k = 0; k < 8; k++) {
+      Vec3 corner{(k & 1) ? lb.max.x : lb.min.x, (k & 2) ? lb.max.y : lb.min.y,
+                  (k & 4) ? lb.max.z : lb.min.z};
+      wb.grow_point(in.transform.transform_point(corner));
+    }
+    in.world_bounds = wb;
+    ib.push_back(wb);
+    instances.push_back(in);
+  }
+  tlas.build(ib, bp);
+}
+
+}  // namespace orc

@@ -1,0 +1,185 @@
+// ORACLE — test infrastructure only (see orc_math.hpp header).
+//
+// f64 restatement of render.rs's CPU path tracer for the surface subset the
+// benchmark scenes use: src/raytracer/pt/mod.rs (trace_full without media,
+// hair, subsurface, presence or textures), src/raytracer/pt/bxdf.rs,
+// src/scene/{pbr,light,light_sampler,camera}.rs, the Sphere / Triangle /
+// Instance intersectors and Scene::intersect.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "../include/rrhip.h"
+#include "orc_accel.hpp"
+#include "orc_sampling.hpp"
+
+namespace orc {
+
+// pbr.rs:11-75 (fields used by the lobes; glow/presence/subsurface off)
+struct PbrParams {
+  double diffuse_gain = 1.0;
+  Vec3 diffuse_color{0.5, 0.5, 0.5};
+  double diffuse_roughness = 0.0;
+  Vec3 specular_face_color{0, 0, 0};
+  Vec3 specular_edge_color{1, 1, 1};
+  double specular_roughness = 0.2;
+  double specular_ior = 1.5;
+  double clearcoat_gain = 0.0;
+  double clearcoat_roughness = 0.1;
+  double fuzz_gain = 0.0;
+  Vec3 fuzz_color{1, 1, 1};
+  double glass_gain = 0.0;
+  double glass_ior = 1.5;
+  double glass_roughness = 0.05;
+  Vec3 refraction_color{1, 1, 1};
+  Vec3 glow{0, 0, 0};
+
+  Vec3 specular_f0() const;
+  bool has_specular() const;
+  bool lobe_weights(double w[5]) const;
+};
+
+struct Material {
+  PbrParams pbr;
+  Vec3 emission{0, 0, 0};
+  int area_light = -1;
+};
+
+// bxdf.rs:16-39
+struct Frame {
+  Vec3 t, b, n;
+  explicit Frame(const Vec3& n_);
+  Vec3 to_local(const Vec3& w) const { return {w.dot(t), w.dot(b), w.dot(n)}; }
+  Vec3 to_world(const Vec3& w) const { return t * w.x + b * w.y + n * w.z; }
+};
+
+struct BsdfSample {
+  Vec3 wi, f;
+  double pdf = 0;
+  bool transmitted = false;
+  bool specular_lobe = false;
+};
+
+void bsdf_eval_pdf(const PbrParams& p, const Vec3& wo, const Vec3& wi, double eta_rel, Vec3* f,
+                   double* pdf);
+bool bsdf_sample(const PbrParams& p, const Vec3& wo, double eta_rel, Pcg32& rng, BsdfSample* s);
+
+// light.rs:7-46 (no IES, no HDRI)
+struct Light {
+  int kind = RRHIP_LIGHT_POINT;
+  double intensity = 1.0;
+  Vec3 color{1, 1, 1};
+  Vec3 position;                    // point
+  Vec3 direction;                   // distant (normalised travel direction)
+  double angular_radius = 0;        // distant
+  Vec3 corner, edge1, edge2;        // rect
+  Vec3 center, e1, e2;              // sphere / disk
+  Vec3 normal;                      // rect / disk
+  double area = 0;                  // rect / disk
+  double radius = 0;                // sphere
+  Vec3 radiance() const { return color * intensity; }
+};
+
+struct LightBvhNode {  // light_sampler.rs:19-29 (f32 storage)
+  float min[3];
+  float power;
+  float max[3];
+  uint32_t parent;
+  uint32_t a, b;
+  uint32_t pad[2];
+};
+
+class LightSampler {  // light_sampler.rs:33-312
+ public:
+  std::vector<LightBvhNode> nodes;
+  std::vector<uint32_t> light_leaf, infinite;
+  std::vector<double> infinite_power;
+  double infinite_total = 0, p_infinite = 0;
+
+  void build(const std::vector<Light>& lights);
+  bool sample(const Vec3& p, double u, size_t* light, double* pmf) const;
+  double pmf(const Vec3& p, size_t light) const;
+};
+
+struct Camera {  // camera.rs:69-179 (perspective, pinhole)
+  uint32_t width = 0, height = 0;
+  double fov = 45;
+  Vec3 eye, forward, up, right;
+  int filter = 0;  // 0 box, 1 triangle, 2 gaussian
+  double filter_width = 1.0;
+  double filter_1d(double u) const;
+  Ray generate_ray(double px, double py) const;
+};
+
+struct Intersection {
+  double t = 0;
+  Vec3 point, normal;
+  bool front_face = true;
+  uint32_t material = 0;
+};
+
+struct TriangleObj {
+  Vec3 v0, v1, v2;
+  uint32_t material;
+  bool intersect(const Ray& ray, Intersection* out) const;
+};
+
+struct SphereObj {
+  Vec3 center;
+  double radius;
+  uint32_t material;
+  bool intersect(const Ray& ray, Intersection* out) const;
+};
+
+struct MeshObj {
+  Mesh mesh;
+  std::vector<float> normals;  // empty: geometric
+};
+
+struct InstanceObj {
+  uint32_t mesh, material;
+  Mat4 transform, inverse;
+  Aabb world_bounds;
+};
+
+struct RenderParams {
+  uint32_t max_depth = 10;
+  uint32_t rr_start = 3;
+  double firefly_clamp = 60.0;
+  uint64_t sample_base = 0;
+};
+
+class Scene {
+ public:
+  Camera camera;
+  std::vector<TriangleObj> triangles;
+  std::vector<SphereObj> spheres;
+  std::vector<MeshObj> meshes;
+  std::vector<InstanceObj> instances;
+  Bvh tlas;
+  std::vector<Light> lights;
+  std::vector<Material> materials;
+  LightSampler light_sampler;
+  Vec3 background{0, 0, 0};
+  int dome = -1;
+
+  void load(const rrhip_scene_desc& d);
+  bool intersect(const Ray& ray, Intersection* out) const;
+  // Closest-hit and shadow-ray counts accumulate into these when non-null.
+  Vec3 trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) const;
+  Ray camera_ray(uint32_t x, uint32_t y, Pcg32& rng) const;
+
+ private:
+  bool instance_hit(const InstanceObj& inst, const Ray& ray, double t_max, double* t_param,
+                    Vec3* world_normal) const;
+  Vec3 sample_light(const Light& light, const Vec3& p, const Vec3& n, const PbrParams& pbr,
+                    const Frame& frame, const Vec3& wo_l, double eta_rel, double pick_pmf,
+                    Pcg32& rng, uint64_t* rays) const;
+  Vec3 area_contribution(const Vec3& p, const Vec3& n, const Vec3& sample_point,
+                         const Vec3& light_normal, double area, const Light& light,
+                         const PbrParams& pbr, const Frame& frame, const Vec3& wo_l,
+                         double eta_rel, double pick_pmf, uint64_t* rays) const;
+  double visibility(const Vec3& origin, const Vec3& dir, double remaining, uint64_t* rays) const;
+};
+
+}  // namespace orc

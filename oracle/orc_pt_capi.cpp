@@ -1,0 +1,178 @@
+// ORACLE — test infrastructure only (see orc_math.hpp header).
+//
+// C entry points of the path-tracing oracle: renders into the accumulator
+// layout of include/rrhip.h (RRHIP_ACCUM_STRIDE doubles per pixel), and
+// exposes the BSDF and light sampler for unit tests.
+#include <atomic>
+#include <thread>
+#include <vector>
+
+#include "orc_pt.hpp"
+
+using namespace orc;
+
+namespace {
+
+PbrParams pbr_from(const rrhip_material_desc* m) {
+  rrhip_scene_desc d{};
+  d.struct_size = sizeof(d);
+  d.num_materials = 1;
+  d.materials = m;
+  Scene s;
+  s.load(d);
+  return s.materials[0].pbr;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Renders samples [start, start + count) of every pixel and ADDS them into
+// accum (RRHIP_ACCUM_STRIDE doubles per pixel). rays_out (optional):
+// total extension + shadow rays traced.
+int orc_render(const rrhip_scene_desc* desc, const rrhip_render_params* params, uint32_t start,
+               uint32_t count, double* accum, int nthreads, uint64_t* rays_out) {
+  Scene scene;
+  scene.load(*desc);
+  RenderParams rp;
+  rp.max_depth = params->max_depth;
+  rp.rr_start = params->rr_start;
+  rp.firefly_clamp = params->firefly_clamp;
+  rp.sample_base = params->sample_base;
+  const uint32_t W = scene.camera.width, H = scene.camera.height;
+  std::atomic<uint32_t> next_row{0};
+  std::atomic<uint64_t> total_rays{0};
+  auto work = [&]() {
+    uint64_t rays = 0;
+    for (;;) {
+      uint32_t y = next_row.fetch_add(1);
+      if (y >= H) break;
+      for (uint32_t x = 0; x < W; x++) {
+        uint64_t pixel = (uint64_t)y * W + x;
+        double* a = accum + RRHIP_ACCUM_STRIDE * pixel;
+        for (uint32_t s = start; s < start + count; s++) {
+          Pcg32 rng = Pcg32::for_pixel_sample(pixel, rp.sample_base + s);
+          Ray ray = scene.camera_ray(x, y, rng);
+          Vec3 l = scene.trace(ray, rng, rp, &rays);
+          double lm = luminance(l);
+          if (rp.firefly_clamp > 0.0 && lm > rp.firefly_clamp) l = l * (rp.firefly_clamp / lm);
+          a[0] += l.x;
+          a[1] += l.y;
+          a[2] += l.z;
+          a[3] += l.x * l.x;
+          a[4] += l.y * l.y;
+          a[5] += l.z * l.z;
+          a[6] += 1.0;
+        }
+      }
+    }
+    total_rays += rays;
+  };
+  int nt = nthreads > 0 ? nthreads : 1;
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; t++) pool.emplace_back(work);
+  for (auto& th : pool) th.join();
+  if (rays_out) *rays_out = total_rays.load();
+  return 0;
+}
+
+// bxdf.rs eval_pdf for one material (wo, wi in the local shading frame).
+void orc_bsdf_eval(const rrhip_material_desc* m, const double* wo, const double* wi, double eta,
+                   double* f, double* pdf) {
+  PbrParams p = pbr_from(m);
+  Vec3 ff;
+  bsdf_eval_pdf(p, Vec3{wo[0], wo[1], wo[2]}, Vec3{wi[0], wi[1], wi[2]}, eta, &ff, pdf);
+  f[0] = ff.x;
+  f[1] = ff.y;
+  f[2] = ff.z;
+}
+
+// eval_pdf for n directions wi (n x 3) at one wo.
+void orc_bsdf_eval_n(const rrhip_material_desc* m, const double* wo, const double* wi, double eta,
+                     uint64_t n, double* f, double* pdf) {
+  PbrParams p = pbr_from(m);
+  Vec3 w{wo[0], wo[1], wo[2]};
+  for (uint64_t i = 0; i < n; i++) {
+    Vec3 ff;
+    bsdf_eval_pdf(p, w, Vec3{wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]}, eta, &ff, pdf + i);
+    f[3 * i] = ff.x;
+    f[3 * i + 1] = ff.y;
+    f[3 * i + 2] = ff.z;
+  }
+}
+
+// n draws of bxdf.rs sample() from one PCG32(seed, stream) stream.
+// ok[i] = 0 when the sampler returned None.
+void orc_bsdf_sample_n(const rrhip_material_desc* m, const double* wo, double eta, uint64_t seed,
+                       uint64_t stream, uint64_t n, double* wi, double* f, double* pdf, uint8_t* ok) {
+  PbrParams p = pbr_from(m);
+  Pcg32 rng = Pcg32::make(seed, stream);
+  Vec3 w{wo[0], wo[1], wo[2]};
+  for (uint64_t i = 0; i < n; i++) {
+    BsdfSample s;
+    bool good = bsdf_sample(p, w, eta, rng, &s);
+    ok[i] = good ? 1 : 0;
+    wi[3 * i] = good ? s.wi.x : 0;
+    wi[3 * i + 1] = good ? s.wi.y : 0;
+    wi[3 * i + 2] = good ? s.wi.z : 0;
+    f[3 * i] = good ? s.f.x : 0;
+    f[3 * i + 1] = good ? s.f.y : 0;
+    f[3 * i + 2] = good ? s.f.z : 0;
+    pdf[i] = good ? s.pdf : 0;
+  }
+}
+
+void* orc_light_sampler_create(const rrhip_light_desc* lights, uint32_t n) {
+  rrhip_scene_desc d{};
+  d.struct_size = sizeof(d);
+  d.num_lights = n;
+  d.lights = lights;
+  auto* s = new Scene();
+  s->load(d);
+  return s;
+}
+
+void orc_light_sampler_destroy(void* h) { delete (Scene*)h; }
+
+int orc_light_sampler_sample(void* h, const double* p, double u, uint32_t* light, double* pmf) {
+  size_t li;
+  double pm;
+  if (!((Scene*)h)->light_sampler.sample(Vec3{p[0], p[1], p[2]}, u, &li, &pm)) return 0;
+  *light = (uint32_t)li;
+  *pmf = pm;
+  return 1;
+}
+
+double orc_light_sampler_pmf(void* h, const double* p, uint32_t light) {
+  return ((Scene*)h)->light_sampler.pmf(Vec3{p[0], p[1], p[2]}, light);
+}
+
+// Node array of the light BVH (48-byte nodes); returns the count.
+uint32_t orc_light_sampler_nodes(void* h, void* out, uint32_t cap) {
+  const auto& nodes = ((Scene*)h)->light_sampler.nodes;
+  uint32_t n = (uint32_t)nodes.size();
+  for (uint32_t i = 0; i < n && i < cap; i++) ((LightBvhNode*)out)[i] = nodes[i];
+  return n;
+}
+
+// Closest hit of one world ray against the scene (Scene::intersect):
+// returns 1 and t, point, normal, front, material on a hit.
+int orc_scene_intersect(const rrhip_scene_desc* desc, const double* o, const double* d, double* t,
+                        double* point, double* normal, int* front, uint32_t* material) {
+  Scene s;
+  s.load(*desc);
+  Intersection h;
+  if (!s.intersect(Ray::make(Vec3{o[0], o[1], o[2]}, Vec3{d[0], d[1], d[2]}), &h)) return 0;
+  *t = h.t;
+  point[0] = h.point.x;
+  point[1] = h.point.y;
+  point[2] = h.point.z;
+  normal[0] = h.normal.x;
+  normal[1] = h.normal.y;
+  normal[2] = h.normal.z;
+  *front = h.front_face ? 1 : 0;
+  *material = h.material;
+  return 1;
+}
+
+}  // extern "C"

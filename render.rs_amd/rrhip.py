@@ -51,12 +51,81 @@ class BuildParams(ctypes.Structure):
 
 class MeshDesc(ctypes.Structure):
     _fields_ = [("positions", ctypes.c_void_p), ("num_vertices", ctypes.c_uint64),
-                ("indices", ctypes.c_void_p), ("num_triangles", ctypes.c_uint64)]
+                ("indices", ctypes.c_void_p), ("num_triangles", ctypes.c_uint64),
+                ("normals", ctypes.c_void_p)]
+
+
+D3 = ctypes.c_double * 3
+
+
+class InstanceDesc(ctypes.Structure):
+    _fields_ = [("mesh", ctypes.c_uint32), ("material", ctypes.c_uint32),
+                ("object_to_world", ctypes.c_double * 12)]
+
+
+class TriangleDesc(ctypes.Structure):
+    _fields_ = [("v0", D3), ("v1", D3), ("v2", D3), ("material", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+class SphereDesc(ctypes.Structure):
+    _fields_ = [("center", D3), ("radius", ctypes.c_double), ("material", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+class MaterialDesc(ctypes.Structure):
+    _fields_ = [("diffuse_gain", ctypes.c_double), ("diffuse_color", D3),
+                ("diffuse_roughness", ctypes.c_double),
+                ("specular_face_color", D3), ("specular_edge_color", D3),
+                ("specular_roughness", ctypes.c_double), ("specular_ior", ctypes.c_double),
+                ("clearcoat_gain", ctypes.c_double), ("clearcoat_roughness", ctypes.c_double),
+                ("fuzz_gain", ctypes.c_double), ("fuzz_color", D3),
+                ("glass_gain", ctypes.c_double), ("glass_ior", ctypes.c_double),
+                ("glass_roughness", ctypes.c_double), ("refraction_color", D3),
+                ("glow", D3), ("emission", D3), ("area_light", ctypes.c_int32),
+                ("reserved", ctypes.c_uint32)]
+
+
+class LightDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("intensity", ctypes.c_double), ("color", D3), ("p", D3), ("e1", D3), ("e2", D3),
+                ("radius", ctypes.c_double)]
+
+
+class CameraDesc(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("filter", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("fov_deg", ctypes.c_double), ("filter_width", ctypes.c_double),
+                ("eye", D3), ("look_at", D3), ("up", D3)]
 
 
 class SceneDesc(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("num_meshes", ctypes.c_uint32),
-                ("meshes", ctypes.POINTER(MeshDesc)), ("build", BuildParams)]
+                ("meshes", ctypes.POINTER(MeshDesc)), ("build", BuildParams),
+                ("num_instances", ctypes.c_uint32), ("num_triangles", ctypes.c_uint32),
+                ("instances", ctypes.POINTER(InstanceDesc)),
+                ("triangles", ctypes.POINTER(TriangleDesc)),
+                ("num_spheres", ctypes.c_uint32), ("num_materials", ctypes.c_uint32),
+                ("spheres", ctypes.POINTER(SphereDesc)),
+                ("materials", ctypes.POINTER(MaterialDesc)),
+                ("num_lights", ctypes.c_uint32), ("reserved0", ctypes.c_uint32),
+                ("lights", ctypes.POINTER(LightDesc)),
+                ("camera", CameraDesc), ("background", D3)]
+
+
+class RenderParams(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
+                ("rr_start", ctypes.c_uint32), ("samples_per_pass", ctypes.c_uint32),
+                ("firefly_clamp", ctypes.c_double), ("sample_base", ctypes.c_uint64)]
+
+    @staticmethod
+    def make(max_depth=10, rr_start=3, samples_per_pass=0, firefly_clamp=60.0, sample_base=0):
+        return RenderParams(ctypes.sizeof(RenderParams), max_depth, rr_start, samples_per_pass,
+                            firefly_clamp, sample_base)
+
+
+LIGHT_POINT, LIGHT_DISTANT, LIGHT_RECT, LIGHT_SPHERE, LIGHT_DISK, LIGHT_DOME = range(6)
+ACCUM_STRIDE = 8
 
 
 class SceneInfo(ctypes.Structure):
@@ -170,8 +239,12 @@ def _info_dict(i):
 def _mesh_desc(positions, indices, bins, max_leaf):
     pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
     idx = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
-    mesh = MeshDesc(_ptr(pos).value, pos.shape[0], _ptr(idx).value, idx.shape[0])
-    desc = SceneDesc(ctypes.sizeof(SceneDesc), 1, ctypes.pointer(mesh), BuildParams(bins, max_leaf, 0, 0))
+    mesh = MeshDesc(_ptr(pos).value, pos.shape[0], _ptr(idx).value, idx.shape[0], None)
+    desc = SceneDesc()
+    desc.struct_size = ctypes.sizeof(SceneDesc)
+    desc.num_meshes = 1
+    desc.meshes = ctypes.pointer(mesh)
+    desc.build = BuildParams(bins, max_leaf, 0, 0)
     return pos, idx, mesh, desc
 
 

@@ -1,0 +1,243 @@
+"""Scene descriptions of the path-tracing workloads, built as the C-ABI
+structs of include/rrhip.h (rrhip_scene_desc and friends).
+
+`Scene` keeps every array it hands to C alive; `desc()` returns the
+rrhip_scene_desc. Materials follow render.rs's PbrParams mappings
+(src/scene/pbr.rs:103-130): `matte` = from_matte (diffuse only, specular_ior
+1), `metal` = from_metal, `glass` = a pure dielectric (glassRefractionGain 1,
+diffuseGain 0, specularIor 1 so no extra F0 = 0.04 coat).
+"""
+import ctypes
+
+import numpy as np
+
+import rrhip
+
+
+def _d3(v):
+    return rrhip.D3(*[float(x) for x in v])
+
+
+def material(diffuse_gain=1.0, diffuse_color=(0.5, 0.5, 0.5), diffuse_roughness=0.0,
+             specular_face_color=(0, 0, 0), specular_edge_color=(1, 1, 1), specular_roughness=0.2,
+             specular_ior=1.5, clearcoat_gain=0.0, clearcoat_roughness=0.1, fuzz_gain=0.0,
+             fuzz_color=(1, 1, 1), glass_gain=0.0, glass_ior=1.5, glass_roughness=0.05,
+             refraction_color=(1, 1, 1), glow=(0, 0, 0), emission=(0, 0, 0), area_light=-1):
+    """PbrParams defaults of pbr.rs:50-75."""
+    m = rrhip.MaterialDesc()
+    m.diffuse_gain = diffuse_gain
+    m.diffuse_color = _d3(diffuse_color)
+    m.diffuse_roughness = diffuse_roughness
+    m.specular_face_color = _d3(specular_face_color)
+    m.specular_edge_color = _d3(specular_edge_color)
+    m.specular_roughness = specular_roughness
+    m.specular_ior = specular_ior
+    m.clearcoat_gain = clearcoat_gain
+    m.clearcoat_roughness = clearcoat_roughness
+    m.fuzz_gain = fuzz_gain
+    m.fuzz_color = _d3(fuzz_color)
+    m.glass_gain = glass_gain
+    m.glass_ior = glass_ior
+    m.glass_roughness = glass_roughness
+    m.refraction_color = _d3(refraction_color)
+    m.glow = _d3(glow)
+    m.emission = _d3(emission)
+    m.area_light = area_light
+    return m
+
+
+def matte(color, **kw):
+    """pbr.rs:103-110 (PbrParams::from_matte)."""
+    return material(diffuse_gain=1.0, diffuse_color=color, specular_ior=1.0, **kw)
+
+
+def metal(color, roughness):
+    """pbr.rs:122-130 (PbrParams::from_metal)."""
+    return material(diffuse_gain=0.0, specular_face_color=color, specular_edge_color=(1, 1, 1),
+                    specular_roughness=max(roughness, 0.02))
+
+
+def glass(ior=1.5, roughness=0.05):
+    return material(diffuse_gain=0.0, specular_ior=1.0, glass_gain=1.0, glass_ior=ior,
+                    glass_roughness=roughness)
+
+
+def light(kind, intensity=1.0, color=(1, 1, 1), p=(0, 0, 0), e1=(0, 0, 0), e2=(0, 0, 0), radius=0.0):
+    l = rrhip.LightDesc()
+    l.kind = kind
+    l.intensity = intensity
+    l.color = _d3(color)
+    l.p = _d3(p)
+    l.e1 = _d3(e1)
+    l.e2 = _d3(e2)
+    l.radius = radius
+    return l
+
+
+class Scene:
+    def __init__(self, width, height, fov, eye, look_at, up=(0, 1, 0), filter=0, filter_width=1.0,
+                 background=(0, 0, 0)):
+        self.camera = rrhip.CameraDesc()
+        self.camera.width = width
+        self.camera.height = height
+        self.camera.filter = filter
+        self.camera.fov_deg = fov
+        self.camera.filter_width = filter_width
+        self.camera.eye = _d3(eye)
+        self.camera.look_at = _d3(look_at)
+        self.camera.up = _d3(up)
+        self.background = background
+        self.materials, self.lights, self.triangles, self.spheres = [], [], [], []
+        self.meshes, self.instances = [], []
+        self._keep = []
+
+    def add_material(self, m):
+        self.materials.append(m)
+        return len(self.materials) - 1
+
+    def add_light(self, l):
+        self.lights.append(l)
+        return len(self.lights) - 1
+
+    def add_triangle(self, v0, v1, v2, mat):
+        t = rrhip.TriangleDesc()
+        t.v0, t.v1, t.v2 = _d3(v0), _d3(v1), _d3(v2)
+        t.material = mat
+        self.triangles.append(t)
+
+    def add_polygon(self, verts, mat):
+        """Fan triangulation as render.rs's Polygon (scene_builder.rs:939-947)."""
+        for i in range(1, len(verts) - 1):
+            self.add_triangle(verts[0], verts[i], verts[i + 1], mat)
+
+    def add_sphere(self, center, radius, mat):
+        s = rrhip.SphereDesc()
+        s.center = _d3(center)
+        s.radius = radius
+        s.material = mat
+        self.spheres.append(s)
+
+    def add_mesh(self, positions, indices, normals=None):
+        pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        idx = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+        nrm = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        self._keep += [pos, idx, nrm]
+        self.meshes.append(rrhip.MeshDesc(pos.ctypes.data, pos.shape[0], idx.ctypes.data, idx.shape[0],
+                                          None if nrm is None else nrm.ctypes.data))
+        return len(self.meshes) - 1
+
+    def add_instance(self, mesh, mat, object_to_world_3x4):
+        i = rrhip.InstanceDesc()
+        i.mesh = mesh
+        i.material = mat
+        i.object_to_world = (ctypes.c_double * 12)(*[float(x) for x in np.asarray(object_to_world_3x4).ravel()])
+        self.instances.append(i)
+
+    def _arr(self, ctype, items):
+        if not items:
+            return None
+        a = (ctype * len(items))(*items)
+        self._keep.append(a)
+        return ctypes.cast(a, ctypes.POINTER(ctype))
+
+    def desc(self, bins=0, max_leaf=0):
+        d = rrhip.SceneDesc()
+        d.struct_size = ctypes.sizeof(rrhip.SceneDesc)
+        d.num_meshes = len(self.meshes)
+        d.meshes = self._arr(rrhip.MeshDesc, self.meshes)
+        d.build = rrhip.BuildParams(bins, max_leaf, 0, 0)
+        d.num_instances = len(self.instances)
+        d.instances = self._arr(rrhip.InstanceDesc, self.instances)
+        d.num_triangles = len(self.triangles)
+        d.triangles = self._arr(rrhip.TriangleDesc, self.triangles)
+        d.num_spheres = len(self.spheres)
+        d.spheres = self._arr(rrhip.SphereDesc, self.spheres)
+        d.num_materials = len(self.materials)
+        d.materials = self._arr(rrhip.MaterialDesc, self.materials)
+        d.num_lights = len(self.lights)
+        d.lights = self._arr(rrhip.LightDesc, self.lights)
+        d.camera = self.camera
+        d.background = _d3(self.background)
+        self._keep.append(d)
+        return d
+
+    @property
+    def npix(self):
+        return self.camera.width * self.camera.height
+
+
+def box_room(width=256, height=256, emission=15.0):
+    """The box-room workload: a closed room spanning [-1,1]^3 of 10
+    axis-aligned quads (floor, ceiling, back and front walls each split in
+    two halves at x = 0, left and right walls whole: 20 triangles), walls
+    albedo 0.75 except left (0.75, 0.1, 0.1) and right (0.1, 0.75, 0.1),
+    spheres r = 0.3 at (-0.4, -0.7, 0.2) and r = 0.35 at (0.4, -0.65, -0.3),
+    all Lambert (Oren-Nayar sigma = 0); a 0.5 x 0.5 rect light at y = 0.99
+    with emission 15, whose quad is also a white matte emitter surface (as
+    render.rs's AreaLightSource + Polygon, scene_builder.rs:924-936). Camera
+    inside the room at z = -0.9 looking at +z, 60 degree vertical fov, box
+    filter of width 1."""
+    s = Scene(width, height, 60.0, eye=(0, 0, -0.9), look_at=(0, 0, 1), up=(0, 1, 0))
+    white = s.add_material(matte((0.75, 0.75, 0.75)))
+    red = s.add_material(matte((0.75, 0.1, 0.1)))
+    green = s.add_material(matte((0.1, 0.75, 0.1)))
+    li = s.add_light(light(rrhip.LIGHT_RECT, intensity=emission, color=(1, 1, 1),
+                           p=(-0.25, 0.99, -0.25), e1=(0.5, 0, 0), e2=(0, 0, 0.5)))
+    emitter = s.add_material(matte((1, 1, 1), emission=(emission,) * 3, area_light=li))
+    for x0, x1 in ((-1, 0), (0, 1)):
+        s.add_polygon([(x0, -1, -1), (x1, -1, -1), (x1, -1, 1), (x0, -1, 1)], white)   # floor
+        s.add_polygon([(x0, 1, -1), (x0, 1, 1), (x1, 1, 1), (x1, 1, -1)], white)       # ceiling
+        s.add_polygon([(x0, -1, 1), (x1, -1, 1), (x1, 1, 1), (x0, 1, 1)], white)       # back
+        s.add_polygon([(x0, -1, -1), (x0, 1, -1), (x1, 1, -1), (x1, -1, -1)], white)   # front
+    s.add_polygon([(-1, -1, -1), (-1, -1, 1), (-1, 1, 1), (-1, 1, -1)], red)    # left
+    s.add_polygon([(1, -1, -1), (1, 1, -1), (1, 1, 1), (1, -1, 1)], green)      # right
+    # The light quad (corner, corner+e1, corner+e1+e2, corner+e2).
+    s.add_polygon([(-0.25, 0.99, -0.25), (0.25, 0.99, -0.25), (0.25, 0.99, 0.25), (-0.25, 0.99, 0.25)],
+                  emitter)
+    s.add_sphere((-0.4, -0.7, 0.2), 0.3, white)
+    s.add_sphere((0.4, -0.65, -0.3), 0.35, white)
+    return s
+
+
+def cornell_fixture(width=96, height=96):
+    """render.rs's tests/fixtures/cornell.rib as a scene description:
+    AreaLightSource intensity 17, lightcolor (1, 0.9, 0.75) on the ceiling
+    quad (white matte emitter surface), white (0.73) floor / ceiling / back,
+    red (0.63, 0.065, 0.05) left and green (0.14, 0.45, 0.091) right walls,
+    a white matte sphere and a "metal" roughness-0.08 sphere (0.95, 0.93,
+    0.88) of radius 0.35; RIB default camera at the origin looking +z with
+    fov 36 and a box filter of width 1."""
+    s = Scene(width, height, 36.0, eye=(0, 0, 0), look_at=(0, 0, 1), up=(0, 1, 0))
+    lc = (1.0, 0.9, 0.75)
+    quad = [(-0.25, 0.995, 3.95), (0.25, 0.995, 3.95), (0.25, 0.995, 4.45), (-0.25, 0.995, 4.45)]
+    c = np.array(quad[0])
+    li = s.add_light(light(rrhip.LIGHT_RECT, intensity=17.0, color=lc, p=c,
+                           e1=np.array(quad[1]) - c, e2=np.array(quad[3]) - c))
+    emitter = s.add_material(matte((1, 1, 1), emission=tuple(17.0 * x for x in lc), area_light=li))
+    s.add_polygon(quad, emitter)
+    white = s.add_material(matte((0.73, 0.73, 0.73)))
+    s.add_polygon([(-1, -1, 3), (1, -1, 3), (1, -1, 5), (-1, -1, 5)], white)
+    s.add_polygon([(-1, 1, 3), (-1, 1, 5), (1, 1, 5), (1, 1, 3)], white)
+    s.add_polygon([(-1, -1, 5), (1, -1, 5), (1, 1, 5), (-1, 1, 5)], white)
+    red = s.add_material(matte((0.63, 0.065, 0.05)))
+    s.add_polygon([(-1, -1, 3), (-1, -1, 5), (-1, 1, 5), (-1, 1, 3)], red)
+    green = s.add_material(matte((0.14, 0.45, 0.091)))
+    s.add_polygon([(1, -1, 3), (1, 1, 3), (1, 1, 5), (1, -1, 5)], green)
+    s.add_sphere((-0.42, -0.65, 4.4), 0.35, s.add_material(matte((0.73, 0.73, 0.73))))
+    s.add_sphere((0.4, -0.65, 3.9), 0.35, s.add_material(metal((0.95, 0.93, 0.88), 0.08)))
+    return s
+
+
+def furnace(width=64, height=64, albedo=1.0, mat=None, radius=1.5):
+    """render.rs's white furnace (tests/pbr_tests.rs:31-54): a sphere of
+    radius 1.5 at distance 4 inside a constant dome of radiance 1, 30 degree
+    fov. The default material is an albedo-1 diffuse (PxrSurface
+    diffuseGain 1, diffuseColor 1, default specularIor 1.5 -> F0 0.04 spec)
+    unless `mat` is given."""
+    s = Scene(width, height, 30.0, eye=(0, 0, 0), look_at=(0, 0, 1), up=(0, 1, 0))
+    if mat is None:
+        mat = material(diffuse_gain=1.0, diffuse_color=(albedo,) * 3, specular_ior=1.0)
+    m = s.add_material(mat)
+    s.add_light(light(rrhip.LIGHT_DOME, intensity=1.0))
+    s.add_sphere((0, 0, 4), radius, m)
+    return s
