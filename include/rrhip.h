@@ -42,6 +42,7 @@ typedef enum rrhip_status {
 
 typedef struct rrhip_ctx rrhip_ctx;
 typedef struct rrhip_scene rrhip_scene;
+typedef struct rrhip_session rrhip_session;
 
 /* Query ray, 32 bytes. render.rs's GPU ray query takes [f32; 6]
  * (origin, direction) and tests t > 1e-6 (src/raytracer/metal/renderer.rs:
@@ -317,6 +318,36 @@ int rrhip_trace_any_device(rrhip_scene* scene, const rrhip_ray* rays, uint64_t n
  * render.rs; it feeds the HBM traffic model of the ray-query kernel. */
 int rrhip_trace_cost(rrhip_scene* scene, const rrhip_ray* rays, uint64_t n, int any,
                      uint64_t out[2]);
+
+/* ---- path-tracing sessions ------------------------------------------------ */
+typedef struct rrhip_session_stats {
+  uint64_t extension_rays;   /* closest-hit rays traced since creation */
+  uint64_t shadow_rays;      /* NEE shadow rays traced since creation */
+  uint64_t samples_rendered; /* per pixel, summed over render calls */
+  uint64_t path_slots;       /* samples_per_pass x pixels */
+  uint32_t samples_per_pass;
+  uint32_t reserved;
+  double last_render_ms;     /* device time of the last rrhip_render_samples */
+} rrhip_session_stats;
+
+/* Creates a wavefront path-tracing session on a scene with a camera.
+ * Counterpart of PtSession::new / WfSession::new
+ * (src/raytracer/metal/renderer.rs:260-300, 501-560). */
+int rrhip_session_create(rrhip_scene* scene, const rrhip_render_params* params, rrhip_session** out);
+int rrhip_session_destroy(rrhip_session* session);
+/* Renders samples [start, start + count) of every pixel and adds them to
+ * the f64 accumulator (blocking). Per-(pixel, sample) PCG32 seeding makes
+ * sample ranges compose exactly. Counterpart of
+ * PtSession::render_samples (renderer.rs:410-437). */
+int rrhip_render_samples(rrhip_session* session, uint32_t start, uint32_t count);
+/* Copies the accumulator (width x height x RRHIP_ACCUM_STRIDE doubles) to
+ * the host. Counterpart of PtSession::sum_and_weight (renderer.rs:476-490);
+ * the image is sum / count. */
+int rrhip_read_accum(rrhip_session* session, double* out);
+/* Replaces the accumulator (NULL clears it): checkpoint resume, as
+ * render.rs's load_accum (renderer.rs:237-254). */
+int rrhip_load_accum(rrhip_session* session, const double* in);
+int rrhip_session_get_stats(rrhip_session* session, rrhip_session_stats* stats);
 
 /* ---- timing ------------------------------------------------------------- */
 /* Enqueues `reps` back-to-back closest-hit (any == 0) or any-hit launches

@@ -10,6 +10,8 @@
 
 #include "api_internal.h"
 #include "bvh_build.h"
+#include "pt_scene.h"
+#include "scene_internal.h"
 #include "trace.h"
 
 namespace rrhip {
@@ -32,26 +34,6 @@ int fail_hip(hipError_t e, const char* where) {
 }  // namespace rrhip
 
 using namespace rrhip;
-
-struct rrhip_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  std::string name, arch;
-};
-
-struct rrhip_scene {
-  rrhip_ctx* ctx = nullptr;
-  Node2* d_nodes = nullptr;
-  TriRec* d_tris = nullptr;
-  DeviceBvh bvh;
-  TraceTuning tuning;
-  rrhip_scene_info info{};
-  // Staging for the host-buffer entry points (grown on demand).
-  void* d_in = nullptr;
-  void* d_out = nullptr;
-  uint64_t in_cap = 0, out_cap = 0;
-};
 
 namespace {
 
@@ -76,6 +58,7 @@ hipStream_t pick_stream(rrhip_scene* s, void* stream) {
 
 int check_scene(rrhip_scene* s, const void* rays, uint64_t n, const void* out) {
   if (!s) return fail(RRHIP_ERR_INVALID_ARGUMENT, "null scene");
+  if (!s->has_query) return fail(RRHIP_ERR_UNSUPPORTED, "scene has no ray-query mesh");
   if (n && (!rays || !out)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "null ray or output buffer");
   return RRHIP_OK;
 }
@@ -173,11 +156,11 @@ int rrhip_init(int device, rrhip_ctx** out) {
 
 int rrhip_destroy(rrhip_ctx* ctx) {
   if (!ctx) return RRHIP_OK;
-  hipSetDevice(ctx->device);
-  if (ctx->stream) hipStreamSynchronize(ctx->stream);
-  if (ctx->ev0) hipEventDestroy(ctx->ev0);
-  if (ctx->ev1) hipEventDestroy(ctx->ev1);
-  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return RRHIP_OK;
 }
@@ -239,14 +222,26 @@ int rrhip_copy_to_host(rrhip_ctx* ctx, void* dst, const void* src, uint64_t byte
 
 namespace {
 
+// A scene is a ray-query mesh (exactly one mesh, no instances, no loose
+// objects), a path-tracing scene (camera width > 0), or both.
+bool is_query(const rrhip_scene_desc* d) {
+  return d->num_meshes == 1 && d->num_instances == 0 && d->meshes;
+}
+bool is_pt(const rrhip_scene_desc* d) { return d->camera.width > 0; }
+
 int validate_desc(const rrhip_scene_desc* desc) {
   if (desc->struct_size != sizeof(rrhip_scene_desc))
     return fail(RRHIP_ERR_INVALID_ARGUMENT, "scene desc: struct_size mismatch (ABI version)");
-  if (desc->num_meshes != 1 || !desc->meshes)
-    return fail(RRHIP_ERR_UNSUPPORTED, "scene desc: exactly one mesh is supported");
-  const rrhip_mesh_desc& m = desc->meshes[0];
-  if (m.num_triangles && (!m.positions || !m.indices))
-    return fail(RRHIP_ERR_INVALID_ARGUMENT, "scene desc: null mesh arrays");
+  if (desc->num_instances > 0)
+    return fail(RRHIP_ERR_UNSUPPORTED, "scene desc: mesh instances are not supported by this build");
+  if (!is_query(desc) && !is_pt(desc))
+    return fail(RRHIP_ERR_INVALID_ARGUMENT,
+                "scene desc: need exactly one mesh (ray queries) or a camera (path tracing)");
+  if (is_query(desc)) {
+    const rrhip_mesh_desc& m = desc->meshes[0];
+    if (m.num_triangles && (!m.positions || !m.indices))
+      return fail(RRHIP_ERR_INVALID_ARGUMENT, "scene desc: null mesh arrays");
+  }
   return RRHIP_OK;
 }
 
@@ -281,21 +276,71 @@ int build_from_desc(const rrhip_scene_desc* desc, BuiltBvh* built, rrhip_scene_i
 int rrhip_build_info(const rrhip_scene_desc* desc, rrhip_scene_info* info) {
   if (!desc || !info) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_build_info: null argument");
   if (int st = validate_desc(desc)) return st;
+  if (!is_query(desc)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_build_info: no ray-query mesh");
   BuiltBvh built;
   return build_from_desc(desc, &built, info);
 }
+
+}  // extern "C"
+
+namespace {
+
+template <class T>
+hipError_t upload_vec(const std::vector<T>& v, void** slot, const T** view) {
+  *view = nullptr;
+  if (v.empty()) return hipSuccess;
+  hipError_t e = hipMalloc(slot, v.size() * sizeof(T));
+  if (e == hipSuccess) e = hipMemcpy(*slot, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+  *view = (const T*)*slot;
+  return e;
+}
+
+hipError_t upload_pt(rrhip_scene* s, const PtSceneHost& h) {
+  PtSceneView& v = s->pt;
+  hipError_t e = upload_vec(h.tris, &s->pt_mem[0], &v.tris);
+  if (e == hipSuccess) e = upload_vec(h.spheres, &s->pt_mem[1], &v.spheres);
+  if (e == hipSuccess) e = upload_vec(h.materials, &s->pt_mem[2], &v.materials);
+  if (e == hipSuccess) e = upload_vec(h.lights, &s->pt_mem[3], &v.lights);
+  if (e == hipSuccess) e = upload_vec(h.lnodes, &s->pt_mem[4], &v.lnodes);
+  if (e == hipSuccess) e = upload_vec(h.light_leaf, &s->pt_mem[5], &v.light_leaf);
+  if (e == hipSuccess) e = upload_vec(h.inf_lights, &s->pt_mem[6], &v.inf_lights);
+  if (e == hipSuccess) e = upload_vec(h.inf_cdf, &s->pt_mem[7], &v.inf_cdf);
+  v.num_tris = (uint32_t)h.tris.size();
+  v.num_spheres = (uint32_t)h.spheres.size();
+  v.num_lights = (uint32_t)h.lights.size();
+  v.num_lnodes = (uint32_t)h.lnodes.size();
+  v.num_inf = (uint32_t)h.inf_lights.size();
+  v.dome = h.dome;
+  v.p_infinite = h.p_infinite;
+  v.inf_total = h.inf_total;
+  for (int c = 0; c < 3; c++) v.background[c] = h.background[c];
+  v.cam = h.cam;
+  return e;
+}
+
+}  // namespace
+
+extern "C" {
 
 int rrhip_scene_upload(rrhip_ctx* ctx, const rrhip_scene_desc* desc, rrhip_scene** out) {
   if (!ctx || !desc || !out) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_scene_upload: null argument");
   *out = nullptr;
   if (int st = validate_desc(desc)) return st;
   BuiltBvh built;
-  rrhip_scene_info info;
-  if (int st = build_from_desc(desc, &built, &info)) return st;
+  rrhip_scene_info info{};
+  if (is_query(desc)) {
+    if (int st = build_from_desc(desc, &built, &info)) return st;
+  }
+  PtSceneHost pth;
+  if (is_pt(desc)) {
+    std::string err;
+    if (!build_pt_scene(*desc, &pth, &err)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_scene_upload: " + err);
+  }
   if (int st = set_device(ctx)) return st;
   auto* s = new (std::nothrow) rrhip_scene();
   if (!s) return fail(RRHIP_ERR_OUT_OF_MEMORY, "rrhip_scene_upload: host allocation failed");
   s->ctx = ctx;
+  s->has_query = is_query(desc);
   hipError_t e = hipSuccess;
   if (!built.nodes.empty()) {
     e = hipMalloc((void**)&s->d_nodes, built.nodes.size() * sizeof(Node2));
@@ -304,6 +349,10 @@ int rrhip_scene_upload(rrhip_ctx* ctx, const rrhip_scene_desc* desc, rrhip_scene
       e = hipMemcpy(s->d_nodes, built.nodes.data(), built.nodes.size() * sizeof(Node2), hipMemcpyHostToDevice);
     if (e == hipSuccess)
       e = hipMemcpy(s->d_tris, built.tris.data(), built.tris.size() * sizeof(TriRec), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess && is_pt(desc)) {
+    s->has_pt = true;
+    e = upload_pt(s, pth);
   }
   if (e != hipSuccess) {
     rrhip_scene_destroy(s);
@@ -322,12 +371,14 @@ int rrhip_scene_upload(rrhip_ctx* ctx, const rrhip_scene_desc* desc, rrhip_scene
 
 int rrhip_scene_destroy(rrhip_scene* s) {
   if (!s) return RRHIP_OK;
-  hipSetDevice(s->ctx->device);
-  hipStreamSynchronize(s->ctx->stream);
-  if (s->d_nodes) hipFree(s->d_nodes);
-  if (s->d_tris) hipFree(s->d_tris);
-  if (s->d_in) hipFree(s->d_in);
-  if (s->d_out) hipFree(s->d_out);
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  if (s->d_nodes) (void)hipFree(s->d_nodes);
+  if (s->d_tris) (void)hipFree(s->d_tris);
+  if (s->d_in) (void)hipFree(s->d_in);
+  if (s->d_out) (void)hipFree(s->d_out);
+  for (void* p : s->pt_mem)
+    if (p) (void)hipFree(p);
   delete s;
   return RRHIP_OK;
 }
@@ -447,7 +498,7 @@ int rrhip_trace_cost(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, int any,
   if (e == hipSuccess)
     e = hipMemcpyAsync(counts.data(), d_counts, n * 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
-  hipFree(d_counts);
+  (void)hipFree(d_counts);
   if (e != hipSuccess) return fail_hip(e, "rrhip_trace_cost");
   for (uint64_t i = 0; i < n; i++) {
     out[0] += counts[2 * i];

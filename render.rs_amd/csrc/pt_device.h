@@ -1,0 +1,526 @@
+// Device functions of the wavefront path integrator (f32 arithmetic; the
+// light-BVH branch probabilities in f64).
+//
+// Each function follows the render.rs CPU routine it cites so that, given
+// the same PCG32 draws, the GPU makes the same decisions up to f32
+// rounding; draws are consumed in the CPU path's order
+// (src/raytracer/pt/mod.rs:449-990).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "pt_types.h"
+#include "sampling.h"
+
+namespace rrhip {
+namespace pt {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kInv4Pi = 0.0795774715459476678f;
+constexpr float kRayOffset = 1e-4f;   // pt/mod.rs:30
+constexpr float kObjEps = 1e-6f;      // math/mod.rs:9 (object hits need t > EPSILON)
+
+struct V3 {
+  float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 v3(const float* a) { return V3{a[0], a[1], a[2]}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 operator*(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ V3 operator/(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float len(V3 a) { return sqrtf(dot(a, a)); }
+// vector.rs:31-38: divide only when the length exceeds EPSILON.
+__device__ __forceinline__ V3 normalize(V3 a) {
+  const float l = len(a);
+  return l > 1e-6f ? a / l : a;
+}
+__device__ __forceinline__ float maxc(V3 a) { return fmaxf(fmaxf(a.x, a.y), a.z); }
+__device__ __forceinline__ float lum(V3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+
+// Uniform in [0, 1) with 24 bits.
+__device__ __forceinline__ float rnd(Pcg32& r) { return r.next_f32(); }
+
+// bxdf.rs:16-39
+struct Frame {
+  V3 t, b, n;
+};
+__device__ __forceinline__ Frame make_frame(V3 n) {
+  const V3 t0 = fabsf(n.x) > 0.9f ? v3(0.f, 1.f, 0.f) : v3(1.f, 0.f, 0.f);
+  Frame f;
+  f.n = n;
+  f.t = normalize(cross(n, t0));
+  f.b = cross(n, f.t);
+  return f;
+}
+__device__ __forceinline__ V3 to_local(const Frame& f, V3 w) { return {dot(w, f.t), dot(w, f.b), dot(w, f.n)}; }
+__device__ __forceinline__ V3 to_world(const Frame& f, V3 w) { return f.t * w.x + f.b * w.y + f.n * w.z; }
+
+// ---- BSDF (bxdf.rs) ----------------------------------------------------------
+
+__device__ __forceinline__ float pow5(float x) {
+  const float x2 = x * x;
+  return x2 * x2 * x;
+}
+
+// bxdf.rs:63-73
+__device__ __forceinline__ float fresnel_dielectric(float cos_i, float eta) {
+  cos_i = clampf(cos_i, 0.f, 1.f);
+  const float sin2_t = (1.f - cos_i * cos_i) / (eta * eta);
+  if (sin2_t >= 1.f) return 1.f;
+  const float cos_t = sqrtf(1.f - sin2_t);
+  const float r_par = (eta * cos_i - cos_t) / (eta * cos_i + cos_t);
+  const float r_perp = (cos_i - eta * cos_t) / (cos_i + eta * cos_t);
+  return 0.5f * (r_par * r_par + r_perp * r_perp);
+}
+
+// bxdf.rs:77-101
+__device__ __forceinline__ float ggx_d(V3 h, float alpha) {
+  if (h.z <= 0.f) return 0.f;
+  const float a2 = alpha * alpha;
+  const float d = h.z * h.z * (a2 - 1.f) + 1.f;
+  return a2 / (kPi * d * d);
+}
+__device__ __forceinline__ float ggx_lambda(V3 w, float alpha) {
+  const float cos2 = w.z * w.z;
+  if (cos2 <= 0.f) return 0.f;
+  const float tan2 = fmaxf(1.f - cos2, 0.f) / cos2;
+  return (sqrtf(1.f + alpha * alpha * tan2) - 1.f) * 0.5f;
+}
+__device__ __forceinline__ float ggx_g1(V3 w, float alpha) { return 1.f / (1.f + ggx_lambda(w, alpha)); }
+__device__ __forceinline__ float ggx_g2(V3 wo, V3 wi, float alpha) {
+  return 1.f / (1.f + ggx_lambda(wo, alpha) + ggx_lambda(wi, alpha));
+}
+
+// bxdf.rs:104-121 (Heitz 2018 VNDF)
+__device__ __forceinline__ V3 ggx_sample_vndf(V3 wo, float alpha, float u1, float u2) {
+  const V3 v = normalize(v3(alpha * wo.x, alpha * wo.y, wo.z));
+  const float lensq = v.x * v.x + v.y * v.y;
+  const V3 t1 = lensq > 1e-12f ? v3(-v.y, v.x, 0.f) / sqrtf(lensq) : v3(1.f, 0.f, 0.f);
+  const V3 t2 = cross(v, t1);
+  const float r = sqrtf(u1);
+  float sp, cp;
+  sincosf(2.f * kPi * u2, &sp, &cp);
+  const float p1 = r * cp;
+  float p2 = r * sp;
+  const float s = 0.5f * (1.f + v.z);
+  p2 = (1.f - s) * sqrtf(fmaxf(1.f - p1 * p1, 0.f)) + s * p2;
+  const V3 nh = t1 * p1 + t2 * p2 + v * sqrtf(fmaxf(1.f - p1 * p1 - p2 * p2, 0.f));
+  return normalize(v3(alpha * nh.x, alpha * nh.y, fmaxf(nh.z, 1e-9f)));
+}
+
+// bxdf.rs:124-130
+__device__ __forceinline__ float ggx_pdf_h(V3 wo, V3 h, float alpha) {
+  const float odh = dot(wo, h);
+  if (odh <= 0.f || wo.z <= 0.f) return 0.f;
+  return ggx_g1(wo, alpha) * odh * ggx_d(h, alpha) / wo.z;
+}
+
+// bxdf.rs:173-179 (per-channel Schlick)
+__device__ __forceinline__ V3 eval_spec(V3 f0, V3 f90, float alpha, V3 wo, V3 wi) {
+  const V3 h = normalize(wo + wi);
+  const float d = ggx_d(h, alpha);
+  const float g = ggx_g2(wo, wi, alpha);
+  const float m = pow5(clampf(1.f - fmaxf(dot(wo, h), 0.f), 0.f, 1.f));
+  const V3 f = f0 + (f90 - f0) * m;
+  return f * (d * g / fmaxf(4.f * wo.z * wi.z, 1e-9f));
+}
+
+// bxdf.rs:181-185
+__device__ __forceinline__ float spec_pdf(V3 wo, V3 wi, float alpha) {
+  const V3 h = normalize(wo + wi);
+  const float odh = fmaxf(dot(wo, h), 1e-9f);
+  return ggx_pdf_h(wo, h, alpha) / (4.f * odh);
+}
+
+// bxdf.rs:145-165 (Oren-Nayar; scale folds diffuse_gain * under_layer / pi)
+__device__ __forceinline__ V3 eval_diffuse(const PtMaterial& m, V3 wo, V3 wi) {
+  const float sin_o = sqrtf(fmaxf(1.f - wo.z * wo.z, 0.f));
+  const float sin_i = sqrtf(fmaxf(1.f - wi.z * wi.z, 0.f));
+  const float cos_dphi =
+      (sin_o > 1e-6f && sin_i > 1e-6f) ? clampf((wo.x * wi.x + wo.y * wi.y) / (sin_o * sin_i), -1.f, 1.f) : 0.f;
+  float sin_a, tan_b;
+  if (wo.z < wi.z) {
+    sin_a = sin_o;
+    tan_b = sin_i / fmaxf(wi.z, 1e-6f);
+  } else {
+    sin_a = sin_i;
+    tan_b = sin_o / fmaxf(wo.z, 1e-6f);
+  }
+  return v3(m.diffuse) * (m.on_a + m.on_b * fmaxf(cos_dphi, 0.f) * sin_a * tan_b);
+}
+
+// bxdf.rs:167-171
+__device__ __forceinline__ V3 eval_fuzz(const PtMaterial& m, V3 wo, V3 wi) {
+  const float a = 1.f - wi.z, b = 1.f - wo.z;
+  const float rim = 0.5f * (a * a * a * a + b * b * b * b);
+  return v3(m.fuzz) * rim;
+}
+
+// bxdf.rs:241-299. Returns f (no cosine) and the composite pdf.
+__device__ __forceinline__ void bsdf_eval(const PtMaterial& m, V3 wo, V3 wi, float eta, V3& f, float& pdf) {
+  f = v3(0.f, 0.f, 0.f);
+  pdf = 0.f;
+  if (!m.has_lobes) return;
+  const float wd = m.w[0], ws = m.w[1], wc = m.w[2], wf = m.w[3], wg = m.w[4];
+  if (wi.z > 0.f) {
+    const float cos_pdf = wi.z / kPi;
+    if (wd > 0.f) {
+      f = f + eval_diffuse(m, wo, wi);
+      pdf += wd * cos_pdf;
+    }
+    if (wf > 0.f) {
+      f = f + eval_fuzz(m, wo, wi);
+      pdf += wf * cos_pdf;
+    }
+    if (ws > 0.f) {
+      f = f + eval_spec(v3(m.spec_f0), v3(m.spec_f90), m.spec_alpha, wo, wi);
+      pdf += ws * spec_pdf(wo, wi, m.spec_alpha);
+    }
+    if (wc > 0.f) {
+      f = f + eval_spec(v3(m.coat_f0, m.coat_f0, m.coat_f0), v3(m.coat_f90, m.coat_f90, m.coat_f90),
+                        m.coat_alpha, wo, wi);
+      pdf += wc * spec_pdf(wo, wi, m.coat_alpha);
+    }
+    if (wg > 0.f) {
+      const V3 h = normalize(wo + wi);
+      const float fres = fresnel_dielectric(fmaxf(dot(wo, h), 0.f), eta);
+      const float d = ggx_d(h, m.glass_alpha);
+      const float g = ggx_g2(wo, wi, m.glass_alpha);
+      const float v = m.glass_scale * fres * d * g / fmaxf(4.f * wo.z * wi.z, 1e-9f);
+      f = f + v3(v, v, v);
+      pdf += wg * fres * spec_pdf(wo, wi, m.glass_alpha);
+    }
+    return;
+  }
+  if (wg <= 0.f) return;
+  // bxdf.rs:193-232 (PBRT-3 microfacet transmission)
+  V3 h = normalize(wo + wi * eta);
+  if (h.z < 0.f) h = -h;
+  const float odh = dot(wo, h), idh = dot(wi, h);
+  const float fres = fresnel_dielectric(fmaxf(odh, 0.f), eta);
+  if (odh <= 0.f || idh >= 0.f) return;
+  const float d = ggx_d(h, m.glass_alpha);
+  const float g = ggx_g2(wo, v3(wi.x, wi.y, -wi.z), m.glass_alpha);
+  const float sqrt_denom = odh + eta * idh;
+  if (fabsf(sqrt_denom) < 1e-9f) return;
+  const float factor = 1.f / (eta * eta);
+  const float v = m.glass_scale * (1.f - fres) * d * g * factor * fabsf(idh * odh / (wi.z * wo.z)) * (eta * eta) /
+                  (sqrt_denom * sqrt_denom);
+  f = v3(m.refraction) * v;
+  const float dwh_dwi = fabsf(eta * eta * idh) / (sqrt_denom * sqrt_denom);
+  pdf = wg * (1.f - fres) * ggx_pdf_h(wo, h, m.glass_alpha) * dwh_dwi;
+}
+
+struct BsdfSample {
+  V3 wi, f;
+  float pdf;
+  bool transmitted;
+};
+
+// bxdf.rs:309-374 (same draw order)
+__device__ __forceinline__ bool bsdf_sample(const PtMaterial& m, V3 wo, float eta, Pcg32& rng, BsdfSample& s) {
+  if (!m.has_lobes) return false;
+  const float wd = m.w[0], ws = m.w[1], wc = m.w[2], wf = m.w[3], wg = m.w[4];
+  const float pick = rnd(rng);
+  bool transmitted = false;
+  V3 wi;
+  if (pick < wd + wf) {
+    const float u = rnd(rng), v = rnd(rng);
+    const float r = sqrtf(u);
+    float sp, cp;
+    sincosf(2.f * kPi * v, &sp, &cp);
+    wi = v3(r * cp, r * sp, sqrtf(fmaxf(1.f - u, 0.f)));
+  } else if (pick < wd + wf + ws || pick < wd + wf + ws + wc) {
+    const float alpha = pick < wd + wf + ws ? m.spec_alpha : m.coat_alpha;
+    const float u1 = rnd(rng), u2 = rnd(rng);
+    const V3 h = ggx_sample_vndf(wo, alpha, u1, u2);
+    wi = h * (2.f * dot(wo, h)) - wo;  // (-wo).reflect(h)
+    if (wi.z <= 0.f) return false;
+  } else if (wg > 0.f) {
+    const float u1 = rnd(rng), u2 = rnd(rng);
+    const V3 h = ggx_sample_vndf(wo, m.glass_alpha, u1, u2);
+    const float fres = fresnel_dielectric(fmaxf(dot(wo, h), 0.f), eta);
+    if (rnd(rng) < fres) {
+      wi = h * (2.f * dot(wo, h)) - wo;
+      if (wi.z <= 0.f) return false;
+    } else {
+      const float cos_i = dot(wo, h);
+      const float sin2_t = (1.f - cos_i * cos_i) / (eta * eta);
+      if (sin2_t >= 1.f) return false;
+      const float cos_t = sqrtf(1.f - sin2_t);
+      wi = normalize((-wo) / eta + h * (cos_i / eta - cos_t));
+      if (wi.z >= 0.f) return false;
+      transmitted = true;
+    }
+  } else {
+    return false;
+  }
+  V3 f;
+  float pdf;
+  bsdf_eval(m, wo, wi, eta, f, pdf);
+  if (!(pdf > 0.f)) return false;
+  s.wi = wi;
+  s.f = f;
+  s.pdf = pdf;
+  s.transmitted = transmitted;
+  return true;
+}
+
+__device__ __forceinline__ float power_heuristic(float a, float b) {  // pt/mod.rs:992-996
+  const float a2 = a * a, b2 = b * b;
+  return (a2 + b2 <= 0.f) ? 0.f : a2 / (a2 + b2);
+}
+
+// ---- light BVH (light_sampler.rs), branch probabilities in f64 ---------------
+
+__device__ __forceinline__ double node_importance(const LightNode& n, double px, double py, double pz) {
+  const double cx = ((double)n.min[0] + (double)n.max[0]) * 0.5;
+  const double cy = ((double)n.min[1] + (double)n.max[1]) * 0.5;
+  const double cz = ((double)n.min[2] + (double)n.max[2]) * 0.5;
+  const double dx = px - cx, dy = py - cy, dz = pz - cz;
+  const double d2 = dx * dx + dy * dy + dz * dz;
+  const double rx = (double)(n.max[0] - n.min[0]) * 0.5;
+  const double ry = (double)(n.max[1] - n.min[1]) * 0.5;
+  const double rz = (double)(n.max[2] - n.min[2]) * 0.5;
+  const double r2 = rx * rx + ry * ry + rz * rz;
+  return (double)n.power / fmax(fmax(d2, r2), 1e-6);
+}
+
+// light_sampler.rs:150-208
+__device__ __forceinline__ bool ls_sample(const PtSceneView& S, V3 p, float uf, uint32_t& light, float& pmf_out) {
+  const bool has_f = S.num_lnodes > 0, has_i = S.num_inf > 0;
+  if (!has_f && !has_i) return false;
+  double u = uf, pmf = 1.0;
+  bool inf;
+  if (has_f && has_i) {
+    if (u < S.p_infinite) {
+      u /= S.p_infinite;
+      pmf *= S.p_infinite;
+      inf = true;
+    } else {
+      u = (u - S.p_infinite) / (1.0 - S.p_infinite);
+      pmf *= 1.0 - S.p_infinite;
+      inf = false;
+    }
+  } else {
+    inf = has_i;
+  }
+  if (inf) {
+    const double target = u * S.inf_total;
+    double prev = 0.0;
+    for (uint32_t k = 0; k < S.num_inf; k++) {
+      const double acc = S.inf_cdf[k];
+      if (target <= acc || k == S.num_inf - 1) {
+        light = S.inf_lights[k];
+        pmf_out = (float)(pmf * (acc - prev) / fmax(S.inf_total, 1e-12));
+        return true;
+      }
+      prev = acc;
+    }
+    return false;
+  }
+  const double px = p.x, py = p.y, pz = p.z;
+  uint32_t node = 0;
+  for (;;) {
+    const LightNode nd = S.lnodes[node];
+    if (nd.b == 0xFFFFFFFFu) {
+      light = nd.a;
+      pmf_out = (float)pmf;
+      return true;
+    }
+    const double ia = node_importance(S.lnodes[nd.a], px, py, pz);
+    const double ib = node_importance(S.lnodes[nd.b], px, py, pz);
+    const double total = ia + ib;
+    const double pa = total > 1e-30 ? ia / total : 0.5;
+    if (u < pa) {
+      u = fmin(u / pa, 1.0 - 1e-12);
+      pmf *= pa;
+      node = nd.a;
+    } else {
+      u = fmin((u - pa) / (1.0 - pa), 1.0 - 1e-12);
+      pmf *= 1.0 - pa;
+      node = nd.b;
+    }
+  }
+}
+
+// light_sampler.rs:212-238
+__device__ __forceinline__ float ls_pmf(const PtSceneView& S, V3 p, uint32_t light) {
+  const bool has_f = S.num_lnodes > 0, has_i = S.num_inf > 0;
+  for (uint32_t k = 0; k < S.num_inf; k++) {
+    if (S.inf_lights[k] == light) {
+      const double prev = k ? S.inf_cdf[k - 1] : 0.0;
+      const double group = has_f ? S.p_infinite : 1.0;
+      return (float)(group * (S.inf_cdf[k] - prev) / fmax(S.inf_total, 1e-12));
+    }
+  }
+  const uint32_t leaf = S.light_leaf[light];
+  if (leaf == 0xFFFFFFFFu) return 0.f;
+  double pmf = has_i ? 1.0 - S.p_infinite : 1.0;
+  const double px = p.x, py = p.y, pz = p.z;
+  uint32_t node = leaf;
+  while (S.lnodes[node].parent != 0xFFFFFFFFu) {
+    const uint32_t parent = S.lnodes[node].parent;
+    const LightNode pn = S.lnodes[parent];
+    const double ia = node_importance(S.lnodes[pn.a], px, py, pz);
+    const double ib = node_importance(S.lnodes[pn.b], px, py, pz);
+    const double total = ia + ib;
+    const double mine = node == pn.a ? ia : ib;
+    pmf *= total > 1e-30 ? mine / total : 0.5;
+    node = parent;
+  }
+  return (float)pmf;
+}
+
+// ---- geometry -------------------------------------------------------------------
+
+struct Hit {
+  float t;
+  V3 n;  // triangles: flipped to face the ray; spheres: outward
+  uint32_t material;
+  bool front;
+};
+
+// extra.rs:673-720 (world-space, unit direction): t > EPSILON
+__device__ __forceinline__ bool hit_triangle(const PtTriangle& tr, V3 o, V3 d, float tmax, Hit& h) {
+  const V3 e1 = v3(tr.e1), e2 = v3(tr.e2);
+  const V3 pv = cross(d, e2);
+  const float det = dot(e1, pv);
+  if (fabsf(det) < 1e-12f) return false;
+  const float inv = 1.f / det;
+  const V3 tv = o - v3(tr.v0);
+  const float u = dot(tv, pv) * inv;
+  if (!(u >= 0.f && u <= 1.f)) return false;
+  const V3 qv = cross(tv, e1);
+  const float v = dot(d, qv) * inv;
+  if (v < 0.f || u + v > 1.f) return false;
+  const float t = dot(e2, qv) * inv;
+  if (!(t > kObjEps && t < tmax)) return false;
+  V3 n = v3(tr.n);
+  const bool front = dot(n, d) < 0.f;
+  h.t = t;
+  h.n = front ? n : -n;
+  h.material = tr.material;
+  h.front = front;
+  return true;
+}
+
+// sphere.rs:38-109 (full sphere): near root, else far root, > EPSILON.
+// Roots from the geometric form of Haines et al., "Precision Improvements
+// for Ray/Sphere Intersection" (Ray Tracing Gems, 2019).
+__device__ __forceinline__ bool sphere_roots(const PtSphere& s, V3 o, V3 d, float& t0, float& t1) {
+  const V3 f = o - v3(s.c[0], s.c[1], s.c[2]);
+  const float b = -dot(f, d);
+  const V3 l = f + d * b;
+  const float disc = s.r * s.r - dot(l, l);
+  if (disc < 0.f) return false;
+  const float c = dot(f, f) - s.r * s.r;
+  const float q = b + copysignf(sqrtf(disc), b);
+  t0 = c / q;
+  t1 = q;
+  if (t0 > t1) {
+    const float tmp = t0;
+    t0 = t1;
+    t1 = tmp;
+  }
+  return true;
+}
+
+__device__ __forceinline__ bool hit_sphere(const PtSphere& s, V3 o, V3 d, float tmax, Hit& h) {
+  float t0, t1;
+  if (!sphere_roots(s, o, d, t0, t1)) return false;
+  float t;
+  if (t0 > kObjEps)
+    t = t0;
+  else if (t1 > kObjEps)
+    t = t1;
+  else
+    return false;
+  if (!(t < tmax)) return false;
+  const V3 n = normalize((o + d * t) - v3(s.c[0], s.c[1], s.c[2]));
+  h.t = t;
+  h.n = n;
+  h.material = s.material;
+  h.front = dot(d, n) < 0.f;
+  return true;
+}
+
+// scene/mod.rs:105-146 over the loose objects (strictly closer wins).
+__device__ __forceinline__ bool closest_hit(const PtSceneView& S, V3 o, V3 d, Hit& best) {
+  bool found = false;
+  float tmax = INFINITY;
+  for (uint32_t i = 0; i < S.num_tris; i++) {
+    Hit h;
+    if (hit_triangle(S.tris[i], o, d, tmax, h)) {
+      best = h;
+      tmax = h.t;
+      found = true;
+    }
+  }
+  for (uint32_t i = 0; i < S.num_spheres; i++) {
+    Hit h;
+    if (hit_sphere(S.spheres[i], o, d, tmax, h)) {
+      best = h;
+      tmax = h.t;
+      found = true;
+    }
+  }
+  return found;
+}
+
+// Opaque-scene shadow test (pt/mod.rs:1243-1306): blocked when anything is
+// hit before `tmax`.
+__device__ __forceinline__ bool occluded(const PtSceneView& S, V3 o, V3 d, float tmax) {
+  for (uint32_t i = 0; i < S.num_tris; i++) {
+    Hit h;
+    if (hit_triangle(S.tris[i], o, d, tmax, h)) return true;
+  }
+  for (uint32_t i = 0; i < S.num_spheres; i++) {
+    float t0, t1;
+    if (!sphere_roots(S.spheres[i], o, d, t0, t1)) continue;
+    const float t = t0 > kObjEps ? t0 : t1;
+    if (t > kObjEps && t < tmax) return true;
+  }
+  return false;
+}
+
+// ---- camera (camera.rs:35-179, pinhole) -------------------------------------------
+
+__device__ __forceinline__ float filter_1d(const PtCamera& c, float u) {
+  if (c.filter == 1) {
+    const float half = c.filter_width * 0.5f;
+    return u < 0.5f ? (sqrtf(2.f * u) - 1.f) * half : (1.f - sqrtf(2.f * (1.f - u))) * half;
+  }
+  if (c.filter == 2) {
+    const float u1 = fmaxf(u, 1e-12f);
+    const double x = (double)u * 2654435761.0;
+    const float u2 = (float)(x - floor(x));
+    const float sigma = c.filter_width * 0.25f;
+    const float g = sqrtf(-2.f * logf(u1)) * cosf(2.f * kPi * u2);
+    return clampf(g * sigma, -c.filter_width * 0.5f, c.filter_width * 0.5f);
+  }
+  return (u - 0.5f) * c.filter_width;
+}
+
+__device__ __forceinline__ void camera_ray(const PtCamera& c, uint32_t x, uint32_t y, Pcg32& rng, V3& o, V3& d) {
+  const float u1 = rnd(rng), u2 = rnd(rng);
+  const float px = (float)x + 0.5f + filter_1d(c, u1);
+  const float py = (float)y + 0.5f + filter_1d(c, u2);
+  const float u = (px / (float)c.width) * 2.f - 1.f;
+  const float v = 1.f - (py / (float)c.height) * 2.f;
+  o = v3(c.eye);
+  d = normalize(v3(c.forward) + v3(c.right) * (u * c.half_w) + v3(c.up) * (v * c.half_h));
+}
+
+}  // namespace pt
+}  // namespace rrhip

@@ -1,0 +1,32 @@
+// Host build of a path-tracing scene into the device layout of pt_types.h.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "../../include/rrhip.h"
+#include "pt_types.h"
+
+namespace rrhip {
+
+struct PtSceneHost {
+  std::vector<PtTriangle> tris;
+  std::vector<PtSphere> spheres;
+  std::vector<PtMaterial> materials;
+  std::vector<PtLight> lights;
+  std::vector<LightNode> lnodes;
+  std::vector<uint32_t> light_leaf;
+  std::vector<uint32_t> inf_lights;
+  std::vector<double> inf_cdf;
+  double p_infinite = 0.0, inf_total = 0.0;
+  int32_t dome = -1;
+  float background[3] = {0, 0, 0};
+  PtCamera cam{};
+};
+
+// Validates and converts the path-tracing part of `d` (loose triangles,
+// spheres, materials, lights, camera) and builds render.rs's light BVH
+// (src/scene/light_sampler.rs:97-142, 242-312: median split on the widest
+// axis, power sums in f64 stored as f32). Returns false with `err` set.
+bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* err);
+
+}  // namespace rrhip

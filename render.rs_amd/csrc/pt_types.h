@@ -1,0 +1,129 @@
+// Device layout of path-tracing scenes and wavefront path state.
+//
+// Materials are render.rs's PbrParams with every per-material constant of
+// the lobes precomputed (the GPU export precomputes the same kind of
+// values, src/raytracer/metal/gpu_scene.rs:323-387); lights carry their
+// derived normal / area; the light BVH node is render.rs's 48-byte
+// LightBvhNode (src/scene/light_sampler.rs:19-29), built on the host with
+// f64 power sums.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace rrhip {
+
+struct alignas(16) PtMaterial {
+  float w[5];        // lobe weights: diffuse, specular, clearcoat, fuzz, glass (pbr.rs:191-202)
+  uint32_t has_lobes;
+  float diffuse[3];  // diffuse_color * diffuse_gain * under_layer_scale / pi
+  float on_a, on_b;  // Oren-Nayar A, B from diffuse_roughness
+  float spec_f0[3];
+  float spec_f90[3];
+  float spec_alpha;
+  float coat_f0, coat_f90, coat_alpha;
+  float fuzz[3];     // fuzz_color * fuzz_gain / pi
+  float glass_scale; // glass_gain * under_layer_scale
+  float refraction[3];
+  float glass_alpha;
+  float glass_ior;
+  float emission[3]; // Material::emission + glow
+  int32_t area_light;
+  float pad[2];
+};
+
+enum : uint32_t { kLightPoint = 0, kLightDistant = 1, kLightRect = 2, kLightSphere = 3, kLightDisk = 4, kLightDome = 5 };
+
+struct alignas(16) PtLight {
+  uint32_t kind;
+  float radiance[3];  // color * intensity
+  float p[3];         // position / corner / centre; distant: travel direction
+  float radius;       // sphere radius; distant angular radius
+  float e1[3];
+  float area;
+  float e2[3];
+  float cos_max;      // distant: cos(angular_radius)
+  float n[3];         // rect / disk normal
+  float pad;
+};
+
+struct LightNode {  // light_sampler.rs:19-29
+  float min[3];
+  float power;
+  float max[3];
+  uint32_t parent;
+  uint32_t a, b;
+  uint32_t pad[2];
+};
+static_assert(sizeof(LightNode) == 48, "LightNode must match render.rs's 48-byte layout");
+
+struct alignas(16) PtTriangle {
+  float v0[3], e1[3], e2[3];
+  uint32_t material;
+  float n[3];  // normalize(e1 x e2)
+  float pad;
+};
+
+struct alignas(16) PtSphere {
+  float c[3], r;
+  uint32_t material;
+  float pad[3];
+};
+
+struct PtCamera {
+  uint32_t width, height;
+  uint32_t filter;
+  float filter_width;
+  float eye[3];
+  float forward[3], right[3], up[3];
+  float half_w, half_h;
+};
+
+// Read-only scene view passed to the kernels by value.
+struct PtSceneView {
+  const PtTriangle* tris;
+  const PtSphere* spheres;
+  const PtMaterial* materials;
+  const PtLight* lights;
+  const LightNode* lnodes;
+  const uint32_t* light_leaf;   // per light: leaf node or ~0u
+  const uint32_t* inf_lights;   // infinite light ids
+  const double* inf_cdf;        // inclusive power prefix sums (f64)
+  uint32_t num_tris, num_spheres, num_lights, num_lnodes, num_inf;
+  int32_t dome;
+  double p_infinite, inf_total;
+  float background[3];
+  PtCamera cam;
+};
+
+// Wavefront path state, structure of arrays over path slots.
+struct PathBuffers {
+  float4* o;        // origin.xyz, prev_pdf
+  float4* d;        // direction.xyz, unused
+  float4* beta;     // throughput.xyz, unused
+  float4* L;        // radiance.xyz, unused
+  float4* prev;     // previous vertex xyz, unused
+  uint4* rng;       // PCG32 state lo/hi, inc lo/hi
+  uint32_t* flags;  // depth (bits 0-7), from_camera (bit 8)
+  float4* hit_n;    // hit normal.xyz, t (t < 0: miss)
+  uint32_t* hit_m;  // material | front_face << 31
+  uint32_t* q0;
+  uint32_t* q1;
+  uint32_t* sq;     // shadow queue: slot ids
+  float4* so;       // shadow origin.xyz, tmax
+  float4* sd;       // shadow direction.xyz
+  float4* sc;       // shadow contribution rgb (already times beta)
+  uint32_t* counts; // [0] live paths, [1] next live, [2] shadow rays
+  unsigned long long* stats;  // [0] extension rays, [1] shadow rays
+};
+
+struct PtParams {
+  uint32_t max_depth;
+  uint32_t rr_start;
+  float firefly_clamp;
+  uint32_t npix;
+  uint32_t spp_pass;      // samples per pixel in this pass
+  uint64_t sample0;       // sample index of the pass's first sample (incl. base)
+};
+
+}  // namespace rrhip

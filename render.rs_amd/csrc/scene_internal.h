@@ -1,0 +1,37 @@
+// Definitions of the opaque ABI handles, shared by the C-ABI translation
+// units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/rrhip.h"
+#include "layout.h"
+#include "pt_types.h"
+#include "trace.h"
+
+struct rrhip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string name, arch;
+};
+
+struct rrhip_scene {
+  rrhip_ctx* ctx = nullptr;
+  // Ray-query mesh (one mesh, no instances).
+  bool has_query = false;
+  rrhip::Node2* d_nodes = nullptr;
+  rrhip::TriRec* d_tris = nullptr;
+  rrhip::DeviceBvh bvh;
+  rrhip::TraceTuning tuning;
+  rrhip_scene_info info{};
+  // Staging for the host-buffer entry points (grown on demand).
+  void* d_in = nullptr;
+  void* d_out = nullptr;
+  uint64_t in_cap = 0, out_cap = 0;
+  // Path-tracing scene (camera given): device arrays behind `pt`.
+  bool has_pt = false;
+  rrhip::PtSceneView pt{};
+  void* pt_mem[8] = {};
+};

@@ -35,6 +35,8 @@ EXPORTED = [
     "rrhip_gen_heightfield", "rrhip_gen_query_rays",
     "rrhip_scene_get_tuning", "rrhip_scene_set_tuning", "rrhip_scene_autotune",
     "rrhip_device_copy",
+    "rrhip_session_create", "rrhip_session_destroy", "rrhip_render_samples", "rrhip_read_accum",
+    "rrhip_load_accum", "rrhip_session_get_stats",
 ]
 
 
@@ -124,6 +126,13 @@ class RenderParams(ctypes.Structure):
                             firefly_clamp, sample_base)
 
 
+class SessionStats(ctypes.Structure):
+    _fields_ = [("extension_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
+                ("samples_rendered", ctypes.c_uint64), ("path_slots", ctypes.c_uint64),
+                ("samples_per_pass", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("last_render_ms", ctypes.c_double)]
+
+
 LIGHT_POINT, LIGHT_DISTANT, LIGHT_RECT, LIGHT_SPHERE, LIGHT_DISK, LIGHT_DOME = range(6)
 ACCUM_STRIDE = 8
 
@@ -181,6 +190,12 @@ def load():
         "rrhip_trace_cost": (i32, [vp, vp, u64, i32, ctypes.POINTER(u64)]),
         "rrhip_time_trace_device": (i32, [vp, vp, u64, i32, vp, u32, ctypes.POINTER(ctypes.c_float)]),
         "rrhip_device_copy": (i32, [vp, vp, vp, u64]),
+        "rrhip_session_create": (i32, [vp, ctypes.POINTER(RenderParams), ctypes.POINTER(vp)]),
+        "rrhip_session_destroy": (i32, [vp]),
+        "rrhip_render_samples": (i32, [vp, u32, u32]),
+        "rrhip_read_accum": (i32, [vp, vp]),
+        "rrhip_load_accum": (i32, [vp, vp]),
+        "rrhip_session_get_stats": (i32, [vp, ctypes.POINTER(SessionStats)]),
         "rrhip_scene_get_tuning": (i32, [vp, ctypes.POINTER(TraceTuning)]),
         "rrhip_scene_set_tuning": (i32, [vp, ctypes.POINTER(TraceTuning)]),
         "rrhip_scene_autotune": (i32, [vp, vp, u64, i32, vp, ctypes.POINTER(TraceTuning),
@@ -312,6 +327,60 @@ class Context:
     def to_host(self, p, out):
         _check(load().rrhip_copy_to_host(self.h, _ptr(out), p, out.nbytes))
         return out
+
+
+class PtScene:
+    """A path-tracing scene uploaded from a scenes.Scene description."""
+
+    def __init__(self, ctx, desc):
+        self.ctx = ctx
+        self._desc = desc
+        h = ctypes.c_void_p()
+        _check(load().rrhip_scene_upload(ctx.h, ctypes.byref(desc), ctypes.byref(h)))
+        self.h = h
+        self.npix = desc.camera.width * desc.camera.height
+
+    def close(self):
+        if self.h:
+            load().rrhip_scene_destroy(self.h)
+            self.h = None
+
+    def session(self, params):
+        return Session(self, params)
+
+
+class Session:
+    def __init__(self, scene, params):
+        self.scene = scene
+        self.params = params
+        h = ctypes.c_void_p()
+        _check(load().rrhip_session_create(scene.h, ctypes.byref(params), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            load().rrhip_session_destroy(self.h)
+            self.h = None
+
+    def render(self, start, count):
+        _check(load().rrhip_render_samples(self.h, start, count))
+
+    def accum(self):
+        out = np.empty((self.scene.npix, ACCUM_STRIDE), np.float64)
+        _check(load().rrhip_read_accum(self.h, _ptr(out)))
+        return out
+
+    def load_accum(self, acc=None):
+        if acc is None:
+            _check(load().rrhip_load_accum(self.h, None))
+        else:
+            acc = np.ascontiguousarray(acc, np.float64)
+            _check(load().rrhip_load_accum(self.h, _ptr(acc)))
+
+    def stats(self):
+        st = SessionStats()
+        _check(load().rrhip_session_get_stats(self.h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in SessionStats._fields_ if f != "reserved"}
 
 
 class Scene:
