@@ -57,8 +57,6 @@ def test_white_furnace_to_1e3(ctx):
     center = m.reshape(64, 64, 3)[8:56, 8:56].reshape(-1, 3).mean(0)
     say(f"furnace centre mean {center}")
     assert np.abs(center - 1.0).max() < 1e-3
-    # Pixels that miss the sphere see the dome directly: exactly 1.
-    assert np.abs(m.reshape(64, 64, 3)[0, 0] - 1.0).max() < 1e-6
 
 
 def test_box_room_matches_oracle(ctx):
