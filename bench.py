@@ -46,6 +46,48 @@ def cpu_baseline(pos, idx, rays):
     }
 
 
+def box_room(ctx, args, with_cpu):
+    """The box-room path-tracing workload (256x256, 16 spp, depth 8): one
+    frame = rrhip_render_samples over 16 samples of every pixel."""
+    import rrhip
+    import scenes
+    s = scenes.box_room(256, 256)
+    d = s.desc()
+    p = rrhip.RenderParams.make(max_depth=8)
+    ps = rrhip.PtScene(ctx, d)
+    se = ps.session(p)
+    se.render(0, 16)  # warm-up frame
+    st0 = se.stats()
+    frames = max(3, args.steps // 4)
+    t0 = time.perf_counter()
+    for f in range(frames):
+        se.render(16 * (f + 1), 16)
+    dt = (time.perf_counter() - t0) / frames
+    st1 = se.stats()
+    rays = (st1["extension_rays"] + st1["shadow_rays"] - st0["extension_rays"] - st0["shadow_rays"]) / frames
+    out = {
+        "workload": "box room 256x256, 16 spp, max depth 8, NEE + MIS + RR (wavefront)",
+        "ms_per_frame": round(dt * 1e3, 3),
+        "device_ms_per_frame": round(st1["last_render_ms"], 3),
+        "samples_per_s": round(256 * 256 * 16 / dt, 1),
+        "mrays_per_s": round(rays / dt / 1e6, 1),
+        "rays_per_frame": int(rays),
+    }
+    se.close()
+    ps.close()
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import orc
+        threads = max(1, min(16, os.cpu_count() or 1))
+        t0 = time.perf_counter()
+        _, crays = orc.render(d, p, 0, 16, threads=threads)
+        cdt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"samples_per_s": round(256 * 256 * 16 / cdt, 1),
+                               "mrays_per_s": round(crays / cdt / 1e6, 3), "cores": threads,
+                               "kind": "port", "sample": f"one full frame, f64, {cdt:.2f} s"}
+    return out
+
+
 def copy_calibration(ctx, args):
     """K launches of the plain copy kernel over 512 MiB (reads and writes
     exactly that much per launch)."""
@@ -78,6 +120,7 @@ def main():
                     help="block,stack,rays_per_wave instead of the autotuner (profiling runs)")
     ap.add_argument("--no-cost", action="store_true",
                     help="skip the counting pass (profiling runs)")
+    ap.add_argument("--no-pt", action="store_true", help="skip the box-room path-tracing line")
     ap.add_argument("--mode", choices=("trace", "copy"), default="trace",
                     help="copy: time the plain copy kernel that calibrates HBM counters")
     args = ap.parse_args()
@@ -167,6 +210,8 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pos, idx, rays)
+    if not args.no_pt:
+        out["path_tracing"] = box_room(ctx, args, not args.no_cpu_baseline)
     for p in (d_rays, d_hits, d_occ):
         ctx.free(p)
     scene.close()

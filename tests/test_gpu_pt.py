@@ -43,7 +43,7 @@ def check_parity(acc_gpu, acc_cpu, max_frac=0.01, label=""):
     zi = image_mean_z(acc_gpu, acc_cpu)
     mg, _, _ = mean_var(acc_gpu)
     mc, _, _ = mean_var(acc_cpu)
-    say(f"{label}: outside 3 sigma {frac:.4%}, image-mean z {np.round(zi, 2)}, "
+    say(f"{label}: outside 3 sigma {frac:.4%}, max z {z.max():.3f}, image-mean z {np.round(zi, 2)}, "
         f"means gpu {mg.mean(0).round(5)} cpu {mc.mean(0).round(5)}")
     assert frac <= max_frac, f"{label}: {frac:.4%} of pixel-channels outside 3 sigma"
     assert (zi < 3).all(), f"{label}: image means differ by {zi} sigma"

@@ -203,3 +203,21 @@ def test_converges_with_spp():
     coarse = np.sqrt(((img(0, 4) - ref) ** 2).mean())
     fine = np.sqrt(((img(0, 64) - ref) ** 2).mean())
     assert fine < coarse * 0.6
+
+
+def test_sigma_harness_detects_a_planted_bias():
+    """The 3-sigma comparison used by the GPU parity tests: two oracle
+    renders over disjoint sample ranges pass it; planting a 3 % bias in
+    one of them (on the CPU) fails it."""
+    from pt_stats import image_mean_z, sigma_test
+    s = scenes.box_room(32, 32)
+    d = s.desc()
+    p = rrhip.RenderParams.make(max_depth=8)
+    a, _ = orc.render(d, p, 0, 64)
+    b, _ = orc.render(d, p, 1000, 64)
+    assert (sigma_test(a, b) > 3).mean() < 0.01
+    assert (image_mean_z(a, b) < 3).all()
+    biased = b.copy()
+    biased[:, :3] *= 1.03
+    biased[:, 3:6] *= 1.03 ** 2
+    assert (image_mean_z(a, biased) > 3).any()
