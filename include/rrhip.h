@@ -375,6 +375,16 @@ int rrhip_gen_heightfield(uint32_t nx, uint32_t ny, uint64_t seed, float amplitu
 int rrhip_gen_query_rays(const float lo[3], const float hi[3], float inflate, uint64_t seed,
                          uint64_t n, rrhip_ray* rays);
 
+/* Unit icosphere subdivided `levels` times: 10*4^levels + 2 vertices and
+ * 20*4^levels triangles. Vertices move along their smooth normals by
+ * amplitude * fbm(p) (frequency, octaves, gain 0.5, lacunarity 2, offset
+ * from PCG32(seed, 0x5eed) as for the heightfield), then normals are
+ * rebuilt area-weighted from the displaced faces — render.rs's
+ * displace_mesh and smooth_normals (src/geometry/displace.rs:99-123,
+ * src/geometry/subdiv.rs:329-358). */
+int rrhip_gen_icosphere(uint32_t levels, uint64_t seed, float amplitude, float frequency,
+                        uint32_t octaves, float* positions, uint32_t* indices, float* normals);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,6 +10,7 @@
 
 #include "api_internal.h"
 #include "bvh_build.h"
+#include "pt_kernels.h"
 #include "pt_scene.h"
 #include "scene_internal.h"
 #include "trace.h"
@@ -232,8 +233,6 @@ bool is_pt(const rrhip_scene_desc* d) { return d->camera.width > 0; }
 int validate_desc(const rrhip_scene_desc* desc) {
   if (desc->struct_size != sizeof(rrhip_scene_desc))
     return fail(RRHIP_ERR_INVALID_ARGUMENT, "scene desc: struct_size mismatch (ABI version)");
-  if (desc->num_instances > 0)
-    return fail(RRHIP_ERR_UNSUPPORTED, "scene desc: mesh instances are not supported by this build");
   if (!is_query(desc) && !is_pt(desc))
     return fail(RRHIP_ERR_INVALID_ARGUMENT,
                 "scene desc: need exactly one mesh (ray queries) or a camera (path tracing)");
@@ -305,6 +304,13 @@ hipError_t upload_pt(rrhip_scene* s, const PtSceneHost& h) {
   if (e == hipSuccess) e = upload_vec(h.light_leaf, &s->pt_mem[5], &v.light_leaf);
   if (e == hipSuccess) e = upload_vec(h.inf_lights, &s->pt_mem[6], &v.inf_lights);
   if (e == hipSuccess) e = upload_vec(h.inf_cdf, &s->pt_mem[7], &v.inf_cdf);
+  if (e == hipSuccess) e = upload_vec(h.nodes, &s->pt_mem[8], &v.nodes);
+  if (e == hipSuccess) e = upload_vec(h.btris, &s->pt_mem[9], &v.btris);
+  if (e == hipSuccess) e = upload_vec(h.insts, &s->pt_mem[10], &v.insts);
+  if (e == hipSuccess) e = upload_vec(h.meshes, &s->pt_mem[11], &v.meshes);
+  if (e == hipSuccess) e = upload_vec(h.mesh_idx, &s->pt_mem[12], &v.mesh_idx);
+  if (e == hipSuccess) e = upload_vec(h.mesh_nrm, &s->pt_mem[13], &v.mesh_nrm);
+  v.num_insts = (uint32_t)h.insts.size();
   v.num_tris = (uint32_t)h.tris.size();
   v.num_spheres = (uint32_t)h.spheres.size();
   v.num_lights = (uint32_t)h.lights.size();
@@ -335,6 +341,9 @@ int rrhip_scene_upload(rrhip_ctx* ctx, const rrhip_scene_desc* desc, rrhip_scene
   if (is_pt(desc)) {
     std::string err;
     if (!build_pt_scene(*desc, &pth, &err)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_scene_upload: " + err);
+    if (pth.stack_need > kPtTraceStack)
+      return fail(RRHIP_ERR_UNSUPPORTED, "rrhip_scene_upload: instance hierarchy too deep for the " +
+                                             std::to_string(kPtTraceStack) + "-entry traversal stack");
   }
   if (int st = set_device(ctx)) return st;
   auto* s = new (std::nothrow) rrhip_scene();

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <functional>
 #include <cmath>
 #include <limits>
 #include <thread>
@@ -158,11 +159,14 @@ class Builder {
 struct Flattener {
   const std::vector<BNode>& bn;
   const std::vector<uint32_t>& ids;
-  const float* pos;
+  const float* pos;       // triangle build: vertices and indices
   const uint32_t* idx;
   BuiltBvh* out;
+  std::vector<uint32_t>* order = nullptr;  // box build: leaf-ordered primitive ids
   int max_depth = 0;
   uint64_t leaves = 0;
+
+  uint32_t leaf_slot() const { return order ? (uint32_t)order->size() : (uint32_t)out->tris.size(); }
 
   void put_child(Node2& n, int k, uint32_t child, int depth) {
     const BNode& c = bn[child];
@@ -180,10 +184,15 @@ struct Flattener {
     }
     uint32_t ref, cnt;
     if (c.count > 0) {
-      ref = (uint32_t)out->tris.size();
+      ref = leaf_slot();
       cnt = c.count;
       leaves++;
-      for (uint32_t s = c.first; s < c.first + c.count; s++) emit_tri(ids[s]);
+      for (uint32_t s = c.first; s < c.first + c.count; s++) {
+        if (order)
+          order->push_back(ids[s]);
+        else
+          emit_tri(ids[s]);
+      }
     } else {
       ref = flatten(child, depth + 1);
       cnt = 0;
@@ -296,6 +305,75 @@ bool build_triangle_bvh(const float* positions, uint64_t nverts, const uint32_t*
     n.ref1 = n.ref0;
     n.cnt1 = n.cnt0;
     out->nodes[0] = n;
+  } else {
+    f.flatten(0, 0);
+  }
+  out->depth = f.max_depth;
+  out->leaves = f.leaves;
+  if (out->depth + 1 > kMaxStack) {
+    *err = "BVH too deep for the traversal stack";
+    return false;
+  }
+  return true;
+}
+
+bool build_box_bvh(const float* boxes, uint64_t n, const BuildParams& params, BuiltBvh* out,
+                   std::vector<uint32_t>* order, std::string* err) {
+  out->nodes.clear();
+  out->tris.clear();
+  out->depth = 0;
+  out->leaves = 0;
+  order->clear();
+  if (params.bins < 2 || params.bins > 256 || params.max_leaf < 1 || params.max_leaf > 8) {
+    *err = "build params out of range (bins 2..256, max_leaf 1..8)";
+    return false;
+  }
+  if (n >= (1ull << 28)) {
+    *err = "too many primitives (at most 2^28 - 1)";
+    return false;
+  }
+  std::vector<Box> pb(n);
+  std::vector<float> cen(3 * n);
+  Box root;
+  for (uint64_t i = 0; i < n; i++) {
+    Box b;
+    for (int a = 0; a < 3; a++) {
+      b.lo[a] = boxes[6 * i + a];
+      b.hi[a] = boxes[6 * i + 3 + a];
+      if (!std::isfinite(b.lo[a]) || !std::isfinite(b.hi[a]) || b.lo[a] > b.hi[a]) {
+        *err = "primitive boxes must be finite with lo <= hi";
+        return false;
+      }
+      cen[3 * i + a] = 0.5f * (b.lo[a] + b.hi[a]);
+    }
+    pb[i] = b;
+    root.grow(b);
+  }
+  for (int a = 0; a < 3; a++) {
+    out->root_lo[a] = root.lo[a];
+    out->root_hi[a] = root.hi[a];
+  }
+  if (n == 0) return true;
+  Builder builder(pb, cen, params);
+  builder.run();
+  const std::vector<BNode>& bn = builder.nodes();
+  out->nodes.reserve(builder.node_count() / 2 + 1);
+  order->reserve(n);
+  Flattener f{bn, builder.ids(), nullptr, nullptr, out};
+  f.order = order;
+  if (bn[0].count > 0) {
+    Node2 nd{};
+    out->nodes.push_back(nd);
+    f.put_child(nd, 0, 0, 0);
+    nd.c1_lox = nd.c0_lox;
+    nd.c1_hix = nd.c0_hix;
+    nd.c1_loy = nd.c0_loy;
+    nd.c1_hiy = nd.c0_hiy;
+    nd.c1_loz = nd.c0_loz;
+    nd.c1_hiz = nd.c0_hiz;
+    nd.ref1 = nd.ref0;
+    nd.cnt1 = nd.cnt0;
+    out->nodes[0] = nd;
   } else {
     f.flatten(0, 0);
   }

@@ -35,4 +35,10 @@ bool build_triangle_bvh(const float* positions, uint64_t nverts, const uint32_t*
                         uint64_t ntris, const BuildParams& params, BuiltBvh* out,
                         std::string* err);
 
+// Builds over `n` axis-aligned boxes (6 floats each: lo xyz, hi xyz), e.g.
+// instance world bounds for a TLAS. Leaves reference slots of `order`,
+// which receives the primitive ids in leaf order; out->tris stays empty.
+bool build_box_bvh(const float* boxes, uint64_t n, const BuildParams& params, BuiltBvh* out,
+                   std::vector<uint32_t>* order, std::string* err);
+
 }  // namespace rrhip

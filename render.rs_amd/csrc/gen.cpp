@@ -1,7 +1,9 @@
 // Deterministic workload generators (host): the displaced heightfield of
 // the ray-query benchmark and its query rays.
+#include <algorithm>
 #include <cmath>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "api_internal.h"
@@ -89,5 +91,92 @@ extern "C" int rrhip_gen_query_rays(const float lo[3], const float hi[3], float 
   std::vector<std::thread> pool;
   for (unsigned t = 0; t < nt; t++) pool.emplace_back(work, n * t / nt, n * (t + 1) / nt);
   for (auto& th : pool) th.join();
+  return RRHIP_OK;
+}
+
+namespace {
+
+// Area-weighted vertex normals (render.rs smooth_normals, src/geometry/subdiv.rs:329-358).
+void smooth_normals(const std::vector<float>& pos, const std::vector<uint32_t>& idx, float* out) {
+  const size_t nv = pos.size() / 3;
+  std::vector<double> acc(3 * nv, 0.0);
+  for (size_t t = 0; t + 2 < idx.size(); t += 3) {
+    const float* a = &pos[3 * (size_t)idx[t]];
+    const float* b = &pos[3 * (size_t)idx[t + 1]];
+    const float* c = &pos[3 * (size_t)idx[t + 2]];
+    const double e1[3] = {(double)b[0] - a[0], (double)b[1] - a[1], (double)b[2] - a[2]};
+    const double e2[3] = {(double)c[0] - a[0], (double)c[1] - a[1], (double)c[2] - a[2]};
+    const double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    for (int k = 0; k < 3; k++)
+      for (int a2 = 0; a2 < 3; a2++) acc[3 * (size_t)idx[t + k] + a2] += n[a2];
+  }
+  for (size_t v = 0; v < nv; v++) {
+    const double* n = &acc[3 * v];
+    const double l = std::fmax(std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]), 1e-12);
+    for (int a = 0; a < 3; a++) out[3 * v + a] = (float)(n[a] / l);
+  }
+}
+
+}  // namespace
+
+// Unit icosphere subdivided `levels` times (20 * 4^levels triangles), its
+// vertices displaced along their smooth normals by amplitude * fBm
+// (render.rs displace_mesh, src/geometry/displace.rs:99-123) and normals
+// rebuilt from the displaced faces.
+extern "C" int rrhip_gen_icosphere(uint32_t levels, uint64_t seed, float amplitude, float frequency,
+                                   uint32_t octaves, float* positions, uint32_t* indices, float* normals) {
+  if (levels > 8 || !positions || !indices || !normals)
+    return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_gen_icosphere: levels must be <= 8 and outputs non-null");
+  const double t = (1.0 + std::sqrt(5.0)) / 2.0;
+  std::vector<double> v = {-1, t, 0, 1, t, 0, -1, -t, 0, 1, -t, 0, 0, -1, t, 0, 1, t,
+                           0, -1, -t, 0, 1, -t, t, 0, -1, t, 0, 1, -t, 0, -1, -t, 0, 1};
+  std::vector<uint32_t> f = {0, 11, 5, 0, 5,  1, 0, 1, 7, 0, 7,  10, 0, 10, 11, 1, 5, 9, 5, 11,
+                             4, 11, 10, 2, 10, 7, 6, 7, 1, 8, 3, 9, 4, 3, 4,  2, 3, 2, 6, 3,
+                             6, 8, 3, 8,  9, 4, 9, 5, 2, 4, 11, 6, 2, 10, 8, 6, 7, 9, 8, 1};
+  auto unit = [&](size_t i) {
+    double* p = &v[3 * i];
+    const double l = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    p[0] /= l;
+    p[1] /= l;
+    p[2] /= l;
+  };
+  for (size_t i = 0; i < 12; i++) unit(i);
+  for (uint32_t lv = 0; lv < levels; lv++) {
+    std::vector<uint32_t> nf;
+    nf.reserve(f.size() * 4);
+    std::unordered_map<uint64_t, uint32_t> mid;
+    auto midpoint = [&](uint32_t a, uint32_t b) {
+      const uint64_t key = a < b ? ((uint64_t)a << 32) | b : ((uint64_t)b << 32) | a;
+      auto it = mid.find(key);
+      if (it != mid.end()) return it->second;
+      const uint32_t id = (uint32_t)(v.size() / 3);
+      for (int k = 0; k < 3; k++) v.push_back(0.5 * (v[3 * (size_t)a + k] + v[3 * (size_t)b + k]));
+      unit(id);
+      mid.emplace(key, id);
+      return id;
+    };
+    for (size_t k = 0; k < f.size(); k += 3) {
+      const uint32_t a = f[k], b = f[k + 1], c = f[k + 2];
+      const uint32_t ab = midpoint(a, b), bc = midpoint(b, c), ca = midpoint(c, a);
+      const uint32_t tri[12] = {a, ab, ca, b, bc, ab, c, ca, bc, ab, bc, ca};
+      nf.insert(nf.end(), tri, tri + 12);
+    }
+    f.swap(nf);
+  }
+  const size_t nv = v.size() / 3;
+  std::vector<float> pos(3 * nv);
+  for (size_t i = 0; i < 3 * nv; i++) pos[i] = (float)v[i];
+  std::vector<float> base(3 * nv);
+  smooth_normals(pos, f, base.data());
+  Pcg32 r = Pcg32::make(seed, 0x5eed);
+  const double off[3] = {1000.0 * r.next_f64(), 1000.0 * r.next_f64(), 1000.0 * r.next_f64()};
+  for (size_t i = 0; i < nv; i++) {
+    const double p[3] = {(double)pos[3 * i], (double)pos[3 * i + 1], (double)pos[3 * i + 2]};
+    const double d = fbm3(p[0], p[1], p[2], (double)frequency, octaves, 0.5, 2.0, off) * (double)amplitude;
+    for (int k = 0; k < 3; k++) positions[3 * i + k] = (float)(p[k] + (double)base[3 * i + k] * d);
+  }
+  std::vector<float> disp(positions, positions + 3 * nv);
+  smooth_normals(disp, f, normals);
+  std::copy(f.begin(), f.end(), indices);
   return RRHIP_OK;
 }

@@ -6,6 +6,9 @@
 
 namespace rrhip {
 
+// Traversal stack entries available per lane to the instance path.
+constexpr uint32_t kPtTraceStack = 48;
+
 // Path slots [0, total) with slot = s * npix + pixel: camera rays of
 // samples prm.sample0 + s; fills the live queue (q0) and resets counters.
 hipError_t pt_launch_raygen(const PtSceneView& S, const PathBuffers& P, const PtParams& prm,

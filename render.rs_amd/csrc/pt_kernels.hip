@@ -17,6 +17,7 @@
 
 #include "pt_device.h"
 #include "pt_kernels.h"
+#include "pt_trace.h"
 
 namespace rrhip {
 namespace {
@@ -25,6 +26,9 @@ using namespace pt;
 
 constexpr int kBlock = 256;
 constexpr uint32_t kMaxGrid = 4096;
+// Traversal kernels: 128 lanes per block, each with a 48-entry LDS stack.
+constexpr int kTraceBlock = 128;
+constexpr int kTraceStack = 48;
 
 constexpr uint32_t kFlagFromCamera = 1u << 8;
 
@@ -81,14 +85,16 @@ __global__ __launch_bounds__(kBlock) void raygen_kernel(PtSceneView S, PathBuffe
   }
 }
 
-__global__ __launch_bounds__(kBlock) void extend_kernel(PtSceneView S, PathBuffers P, const uint32_t* __restrict__ q) {
+__global__ __launch_bounds__(kTraceBlock) void extend_kernel(PtSceneView S, PathBuffers P, const uint32_t* __restrict__ q) {
+  __shared__ uint32_t s_stack[kTraceStack * kTraceBlock];
+  uint32_t* stk = s_stack + threadIdx.x;
   const uint32_t n = P.counts[0];
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const uint32_t slot = q[i];
     const float4 o4 = P.o[slot], d4 = P.d[slot];
     Hit h;
-    if (closest_hit(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), h)) {
+    if (scene_closest<kTraceBlock>(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), stk, h)) {
       P.hit_n[slot] = make_float4(h.n.x, h.n.y, h.n.z, h.t);
       P.hit_m[slot] = h.material | (h.front ? 0x80000000u : 0u);
     } else {
@@ -358,12 +364,14 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtSceneView S, PathBuffer
   }
 }
 
-__global__ __launch_bounds__(kBlock) void shadow_kernel(PtSceneView S, PathBuffers P) {
+__global__ __launch_bounds__(kTraceBlock) void shadow_kernel(PtSceneView S, PathBuffers P) {
+  __shared__ uint32_t s_stack[kTraceStack * kTraceBlock];
+  uint32_t* stk = s_stack + threadIdx.x;
   const uint32_t n = P.counts[2];
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const float4 o = P.so[i], d = P.sd[i];
-    if (occluded(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w)) continue;
+    if (scene_occluded<kTraceBlock>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, stk)) continue;
     const uint32_t slot = P.sq[i];
     const float4 c = P.sc[i];
     float4 l = P.L[slot];
@@ -411,8 +419,8 @@ __global__ __launch_bounds__(kBlock) void resolve_kernel(PathBuffers P, PtParams
   }
 }
 
-uint32_t grid_for(uint64_t n) {
-  const uint64_t b = (n + kBlock - 1) / kBlock;
+uint32_t grid_for(uint64_t n, int block = kBlock) {
+  const uint64_t b = (n + block - 1) / block;
   return (uint32_t)(b < 1 ? 1 : (b > kMaxGrid ? kMaxGrid : b));
 }
 
@@ -427,9 +435,10 @@ hipError_t pt_launch_raygen(const PtSceneView& S, const PathBuffers& P, const Pt
 hipError_t pt_launch_bounce(const PtSceneView& S, const PathBuffers& P, const PtParams& prm, const uint32_t* q,
                             uint32_t* qnext, uint32_t capacity, hipStream_t st) {
   const uint32_t g = grid_for(capacity);
-  hipLaunchKernelGGL(extend_kernel, dim3(g), dim3(kBlock), 0, st, S, P, q);
+  const uint32_t gt = grid_for(capacity, kTraceBlock);
+  hipLaunchKernelGGL(extend_kernel, dim3(gt), dim3(kTraceBlock), 0, st, S, P, q);
   hipLaunchKernelGGL(shade_kernel, dim3(g), dim3(kBlock), 0, st, S, P, prm, q, qnext);
-  hipLaunchKernelGGL(shadow_kernel, dim3(g), dim3(kBlock), 0, st, S, P);
+  hipLaunchKernelGGL(shadow_kernel, dim3(gt), dim3(kTraceBlock), 0, st, S, P);
   hipLaunchKernelGGL(rotate_kernel, dim3(1), dim3(1), 0, st, P);
   return hipGetLastError();
 }

@@ -1,7 +1,10 @@
 // Host build of path-tracing scenes (see pt_scene.h).
 #include "pt_scene.h"
 
+#include "bvh_build.h"
+
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <numeric>
 
@@ -168,6 +171,115 @@ uint32_t light_build(const std::vector<Entry>& e, std::vector<size_t>& order, si
   return index;
 }
 
+// Inverse of a 3x4 affine map (row-major), in f64.
+bool affine_inverse(const double m[12], double inv[12]) {
+  const double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], i = m[10];
+  const double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
+  const double det = a * A + b * B + c * C;
+  if (!(std::fabs(det) > 1e-300) || !std::isfinite(det)) return false;
+  const double r[9] = {A / det, -(b * i - c * h) / det, (b * f - c * e) / det,
+                       B / det, (a * i - c * g) / det, -(a * f - c * d) / det,
+                       C / det, -(a * h - b * g) / det, (a * e - b * d) / det};
+  for (int row = 0; row < 3; row++) {
+    for (int col = 0; col < 3; col++) inv[4 * row + col] = r[3 * row + col];
+    inv[4 * row + 3] = -(r[3 * row] * m[3] + r[3 * row + 1] * m[7] + r[3 * row + 2] * m[11]);
+  }
+  return true;
+}
+
+// Per-mesh BLAS, instance records, TLAS over instance world bounds
+// (render.rs: Mesh::new -> Bvh::build, Instance::build's 8-corner world
+// bounds, Scene::build_tlas; src/geometry/mesh.rs:48-84, 266-292,
+// src/scene/mod.rs:82-86). Node references are rebased into one array.
+bool build_instances(const rrhip_scene_desc& d, PtSceneHost* out, std::string* err) {
+  if (d.num_instances == 0) return true;
+  if (!d.instances || !d.meshes) return *err = "null instances or meshes", false;
+  BuildParams bp;
+  bp.bins = d.build.bins ? (int)d.build.bins : 32;
+  bp.max_leaf = d.build.max_leaf ? (int)d.build.max_leaf : 4;
+  // BLAS per mesh.
+  std::vector<BuiltBvh> blas(d.num_meshes);
+  std::vector<std::array<float, 6>> local(d.num_meshes);
+  int blas_levels = 0;
+  for (uint32_t k = 0; k < d.num_meshes; k++) {
+    const rrhip_mesh_desc& m = d.meshes[k];
+    if (m.num_triangles && (!m.positions || !m.indices)) return *err = "null mesh arrays", false;
+    if (!build_triangle_bvh(m.positions, m.num_vertices, m.indices, m.num_triangles, bp, &blas[k], err))
+      return false;
+    for (int a = 0; a < 3; a++) {
+      local[k][a] = blas[k].root_lo[a];
+      local[k][3 + a] = blas[k].root_hi[a];
+    }
+    blas_levels = std::max(blas_levels, blas[k].nodes.empty() ? 0 : blas[k].depth + 1);
+  }
+  // Instances and their world bounds.
+  std::vector<float> boxes(6 * (size_t)d.num_instances);
+  std::vector<PtInstance> raw(d.num_instances);
+  for (uint32_t i = 0; i < d.num_instances; i++) {
+    const rrhip_instance_desc& in = d.instances[i];
+    if (in.mesh >= d.num_meshes) return *err = "instance mesh out of range", false;
+    if (in.material >= d.num_materials) return *err = "instance material out of range", false;
+    double inv[12];
+    if (!affine_inverse(in.object_to_world, inv)) return *err = "instance transform is singular", false;
+    PtInstance& g = raw[i];
+    for (int k = 0; k < 12; k++) g.inv[k] = (float)inv[k];
+    g.mesh = in.mesh;
+    g.material = in.material;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    const auto& lb = local[in.mesh];
+    const bool empty = d.meshes[in.mesh].num_triangles == 0;
+    for (int c = 0; c < 8 && !empty; c++) {
+      const double p[3] = {(c & 1) ? lb[3] : lb[0], (c & 2) ? lb[4] : lb[1], (c & 4) ? lb[5] : lb[2]};
+      for (int r = 0; r < 3; r++) {
+        const double* M = in.object_to_world + 4 * r;
+        const double w = M[0] * p[0] + M[1] * p[1] + M[2] * p[2] + M[3];
+        lo[r] = std::min(lo[r], w);
+        hi[r] = std::max(hi[r], w);
+      }
+    }
+    for (int r = 0; r < 3; r++) {
+      if (empty) lo[r] = hi[r] = 0.0;
+      // Round outward so f32 boxes still contain the f64 corners.
+      boxes[6 * (size_t)i + r] = std::nextafter((float)lo[r], -INFINITY);
+      boxes[6 * (size_t)i + 3 + r] = std::nextafter((float)hi[r], INFINITY);
+    }
+  }
+  BuiltBvh tlas;
+  std::vector<uint32_t> order;
+  if (!build_box_bvh(boxes.data(), d.num_instances, bp, &tlas, &order, err)) return false;
+  out->nodes = tlas.nodes;
+  for (uint32_t k : order) out->insts.push_back(raw[k]);
+  const int tlas_levels = tlas.nodes.empty() ? 0 : tlas.depth + 1;
+  // Concatenate the BLASes, rebasing child references.
+  out->meshes.resize(d.num_meshes);
+  for (uint32_t k = 0; k < d.num_meshes; k++) {
+    const uint32_t node_base = (uint32_t)out->nodes.size();
+    const uint32_t tri_base = (uint32_t)out->btris.size();
+    for (Node2 n : blas[k].nodes) {
+      n.ref0 += n.cnt0 ? tri_base : node_base;
+      n.ref1 += n.cnt1 ? tri_base : node_base;
+      out->nodes.push_back(n);
+    }
+    out->btris.insert(out->btris.end(), blas[k].tris.begin(), blas[k].tris.end());
+    const rrhip_mesh_desc& m = d.meshes[k];
+    PtMeshInfo& mi = out->meshes[k];
+    mi.root = blas[k].nodes.empty() ? 0xFFFFFFFFu : node_base;
+    mi.idx_base = (uint32_t)out->mesh_idx.size();
+    out->mesh_idx.insert(out->mesh_idx.end(), m.indices, m.indices + 3 * m.num_triangles);
+    if (m.normals) {
+      mi.nrm_base = (uint32_t)(out->mesh_nrm.size() / 3);
+      out->mesh_nrm.insert(out->mesh_nrm.end(), m.normals, m.normals + 3 * m.num_vertices);
+    } else {
+      mi.nrm_base = 0xFFFFFFFFu;
+    }
+  }
+  if (out->nodes.size() >= (1ull << 31) || out->btris.size() >= (1ull << 28))
+    return *err = "instanced scene too large", false;
+  // A TLAS leaf pair can stack 2 * max_leaf instance entries.
+  out->stack_need = (uint32_t)(tlas_levels + 2 * bp.max_leaf + blas_levels);
+  return true;
+}
+
 }  // namespace
 
 bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* err) {
@@ -285,6 +397,7 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
     out->spheres.push_back(g);
   }
   put(out->background, d3(d.background));
+  if (!build_instances(d, out, err)) return false;
 
   // Camera: RenderMan frame of Camera::new (camera.rs:96-119), aimed.
   const rrhip_camera_desc& c = d.camera;

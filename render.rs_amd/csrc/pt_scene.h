@@ -21,10 +21,19 @@ struct PtSceneHost {
   int32_t dome = -1;
   float background[3] = {0, 0, 0};
   PtCamera cam{};
+  // Instanced meshes (see PtSceneView).
+  std::vector<Node2> nodes;
+  std::vector<TriRec> btris;
+  std::vector<PtInstance> insts;
+  std::vector<PtMeshInfo> meshes;
+  std::vector<uint32_t> mesh_idx;
+  std::vector<float> mesh_nrm;
+  uint32_t stack_need = 0;  // traversal stack entries the instance path needs
 };
 
 // Validates and converts the path-tracing part of `d` (loose triangles,
-// spheres, materials, lights, camera) and builds render.rs's light BVH
+// spheres, meshes and their instances with per-mesh BLAS and a TLAS,
+// materials, lights, camera) and builds render.rs's light BVH
 // (src/scene/light_sampler.rs:97-142, 242-312: median split on the widest
 // axis, power sums in f64 stored as f32). Returns false with `err` set.
 bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* err);

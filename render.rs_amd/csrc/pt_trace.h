@@ -1,0 +1,202 @@
+// Scene traversal of the path integrator: loose objects (linear, as
+// render.rs's Scene::objects) then the two-level instance hierarchy
+// (Scene::intersect, src/scene/mod.rs:105-146; Instance::intersect,
+// src/geometry/mesh.rs:308-381). The world ray is mapped into each
+// instance's object space WITHOUT renormalising the direction, so t is the
+// same parameter at both levels and one f32 bound culls both.
+//
+// One traversal stack per lane in LDS ([entry][thread]): TLAS node
+// indices, instance entries (top bit) for TLAS leaves still to enter, and
+// the current instance's BLAS nodes above `base`.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "pt_device.h"
+#include "trace_math.h"
+
+namespace rrhip {
+namespace pt {
+
+constexpr uint32_t kInstEntry = 0x80000000u;
+
+struct MeshHit {
+  float t;
+  uint32_t inst;  // index into insts
+  uint32_t slot;  // BLAS triangle record
+  float u, v;
+};
+
+__device__ __forceinline__ V3 xform_point(const float* m, V3 p) {
+  return {m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
+          m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]};
+}
+__device__ __forceinline__ V3 xform_vec(const float* m, V3 v) {
+  return {m[0] * v.x + m[1] * v.y + m[2] * v.z, m[4] * v.x + m[5] * v.y + m[6] * v.z,
+          m[8] * v.x + m[9] * v.y + m[10] * v.z};
+}
+// Transpose of the upper 3x3 (matrix.rs:222-227, with the inverse).
+__device__ __forceinline__ V3 xform_normal(const float* m, V3 n) {
+  return {m[0] * n.x + m[4] * n.y + m[8] * n.z, m[1] * n.x + m[5] * n.y + m[9] * n.z,
+          m[2] * n.x + m[6] * n.y + m[10] * n.z};
+}
+
+__device__ __forceinline__ Ray32 make_ray32(V3 o, V3 d) {
+  Ray32 r;
+  r.ox = o.x;
+  r.oy = o.y;
+  r.oz = o.z;
+  r.tmin = 0.f;
+  r.dx = d.x;
+  r.dy = d.y;
+  r.dz = d.z;
+  r.tmax = INFINITY;
+  r.ix = safe_rcp(d.x);
+  r.iy = safe_rcp(d.y);
+  r.iz = safe_rcp(d.z);
+  return r;
+}
+
+// mesh.rs:117-142 in f32 (1e-9 < t < tmax). Returns t, u, v.
+__device__ __forceinline__ bool mesh_tri(const TriRec& tr, const Ray32& r, float tmax, float& t, float& u,
+                                         float& v) {
+  const V3 v0 = v3(tr.v0x, tr.v0y, tr.v0z);
+  const V3 e1 = v3(tr.v1x, tr.v1y, tr.v1z) - v0, e2 = v3(tr.v2x, tr.v2y, tr.v2z) - v0;
+  const V3 d = v3(r.dx, r.dy, r.dz);
+  const V3 p = cross(d, e2);
+  const float det = dot(e1, p);
+  if (fabsf(det) < 1e-14f) return false;
+  const float inv = 1.f / det;
+  const V3 tv = v3(r.ox, r.oy, r.oz) - v0;
+  u = dot(tv, p) * inv;
+  if (!(u >= 0.f && u <= 1.f)) return false;
+  const V3 q = cross(tv, e1);
+  v = dot(d, q) * inv;
+  if (v < 0.f || u + v > 1.f) return false;
+  t = dot(e2, q) * inv;
+  return t > 1e-9f && t < tmax;
+}
+
+// Closest (ANY = false) or any (ANY = true) instance hit with t < tmax.
+template <int STRIDE, bool ANY>
+__device__ __forceinline__ bool trace_instances(const PtSceneView& S, V3 o, V3 d, float tmax, uint32_t* stk,
+                                                MeshHit& hit) {
+  if (S.num_insts == 0) return false;
+  const Ray32 world = make_ray32(o, d);
+  Ray32 r = world;
+  bool found = false;
+  int sp = 0, base = -1;
+  uint32_t node = 0, inst = 0;
+  bool have = true;
+  for (;;) {
+    if (have) {
+      const Node2 nd = S.nodes[node];
+      bool h0, h1;
+      float t0, t1;
+      slab2(nd, r, tmax, h0, h1, t0, t1);
+      const bool in0 = h0 && nd.cnt0 == 0, in1 = h1 && nd.cnt1 == 0;
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        const bool hc = c == 0 ? h0 : h1;
+        const uint32_t cnt = c == 0 ? nd.cnt0 : nd.cnt1;
+        const uint32_t ref = c == 0 ? nd.ref0 : nd.ref1;
+        if (!hc || cnt == 0) continue;
+        if (base < 0) {
+          for (uint32_t k = 0; k < cnt; k++) stk[(sp++) * STRIDE] = kInstEntry | (ref + k);
+        } else {
+          for (uint32_t k = ref; k < ref + cnt; k++) {
+            float t, u, v;
+            if (mesh_tri(S.btris[k], r, tmax, t, u, v)) {
+              hit.t = t;
+              hit.inst = inst;
+              hit.slot = k;
+              hit.u = u;
+              hit.v = v;
+              found = true;
+              if (ANY) return true;
+              tmax = t;
+            }
+          }
+        }
+      }
+      if (in0 && in1) {
+        const bool first0 = t0 <= t1;
+        stk[(sp++) * STRIDE] = first0 ? nd.ref1 : nd.ref0;
+        node = first0 ? nd.ref0 : nd.ref1;
+      } else if (in0) {
+        node = nd.ref0;
+      } else if (in1) {
+        node = nd.ref1;
+      } else {
+        have = false;
+      }
+      continue;
+    }
+    if (base >= 0 && sp == base) {  // this instance's BLAS is done
+      base = -1;
+      r = world;
+    }
+    if (sp == 0) break;
+    const uint32_t e = stk[(--sp) * STRIDE];
+    if (e & kInstEntry) {
+      inst = e & ~kInstEntry;
+      const PtInstance I = S.insts[inst];
+      const uint32_t root = S.meshes[I.mesh].root;
+      if (root == 0xFFFFFFFFu) continue;
+      r = make_ray32(xform_point(I.inv, o), xform_vec(I.inv, d));
+      base = sp;
+      node = root;
+    } else {
+      node = e;
+    }
+    have = true;
+  }
+  return found;
+}
+
+// Shading normal and material of an instance hit (mesh.rs:199-220, 355-356,
+// 427-439): interpolated or geometric local normal to world space through
+// the inverse transpose, flipped to face the ray; front_face from the
+// unflipped normal.
+__device__ __forceinline__ void instance_normal(const PtSceneView& S, const MeshHit& h, V3 d, V3& n,
+                                                bool& front, uint32_t& material) {
+  const PtInstance I = S.insts[h.inst];
+  const TriRec tr = S.btris[h.slot];
+  const PtMeshInfo mi = S.meshes[I.mesh];
+  V3 ln;
+  if (mi.nrm_base != 0xFFFFFFFFu) {
+    const uint32_t* ix = S.mesh_idx + mi.idx_base + 3 * (size_t)tr.prim;
+    const float* nb = S.mesh_nrm + 3 * (size_t)mi.nrm_base;
+    const float w = 1.f - h.u - h.v;
+    ln = normalize(v3(nb + 3 * (size_t)ix[0]) * w + v3(nb + 3 * (size_t)ix[1]) * h.u + v3(nb + 3 * (size_t)ix[2]) * h.v);
+  } else {
+    const V3 v0 = v3(tr.v0x, tr.v0y, tr.v0z);
+    ln = normalize(cross(v3(tr.v1x, tr.v1y, tr.v1z) - v0, v3(tr.v2x, tr.v2y, tr.v2z) - v0));
+  }
+  n = normalize(xform_normal(I.inv, ln));
+  front = dot(n, d) < 0.f;
+  if (!front) n = -n;
+  material = I.material;
+}
+
+// Closest hit over loose objects then instances.
+template <int STRIDE>
+__device__ __forceinline__ bool scene_closest(const PtSceneView& S, V3 o, V3 d, uint32_t* stk, Hit& best) {
+  bool found = closest_hit(S, o, d, best);
+  MeshHit mh;
+  if (trace_instances<STRIDE, false>(S, o, d, found ? best.t : INFINITY, stk, mh)) {
+    best.t = mh.t;
+    instance_normal(S, mh, d, best.n, best.front, best.material);
+    found = true;
+  }
+  return found;
+}
+
+template <int STRIDE>
+__device__ __forceinline__ bool scene_occluded(const PtSceneView& S, V3 o, V3 d, float tmax, uint32_t* stk) {
+  if (occluded(S, o, d, tmax)) return true;
+  MeshHit mh;
+  return trace_instances<STRIDE, true>(S, o, d, tmax, stk, mh);
+}
+
+}  // namespace pt
+}  // namespace rrhip

@@ -11,6 +11,8 @@
 
 #include <cstdint>
 
+#include "layout.h"
+
 namespace rrhip {
 
 struct alignas(16) PtMaterial {
@@ -70,6 +72,22 @@ struct alignas(16) PtSphere {
   float pad[3];
 };
 
+// A placed mesh (render.rs Instance, src/geometry/mesh.rs:223-305): the
+// world-to-object affine map (row-major 3x4) and what it references.
+struct alignas(16) PtInstance {
+  float inv[12];
+  uint32_t mesh;
+  uint32_t material;
+  uint32_t pad[2];
+};
+
+struct PtMeshInfo {
+  uint32_t root;      // BLAS root in the shared node array
+  uint32_t idx_base;  // first index of the mesh in mesh_idx
+  uint32_t nrm_base;  // first normal (vertex) of the mesh in mesh_nrm; ~0u = geometric normals
+  uint32_t pad;
+};
+
 struct PtCamera {
   uint32_t width, height;
   uint32_t filter;
@@ -89,6 +107,15 @@ struct PtSceneView {
   const uint32_t* light_leaf;   // per light: leaf node or ~0u
   const uint32_t* inf_lights;   // infinite light ids
   const double* inf_cdf;        // inclusive power prefix sums (f64)
+  // Instanced meshes: TLAS nodes [0, ...) then every mesh's BLAS nodes;
+  // BLAS triangle records; instances in TLAS leaf order.
+  const Node2* nodes;
+  const TriRec* btris;
+  const PtInstance* insts;
+  const PtMeshInfo* meshes;
+  const uint32_t* mesh_idx;
+  const float* mesh_nrm;
+  uint32_t num_insts;
   uint32_t num_tris, num_spheres, num_lights, num_lnodes, num_inf;
   int32_t dome;
   double p_infinite, inf_total;

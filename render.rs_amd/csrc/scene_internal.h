@@ -33,5 +33,5 @@ struct rrhip_scene {
   // Path-tracing scene (camera given): device arrays behind `pt`.
   bool has_pt = false;
   rrhip::PtSceneView pt{};
-  void* pt_mem[8] = {};
+  void* pt_mem[14] = {};
 };

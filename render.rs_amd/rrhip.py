@@ -36,7 +36,7 @@ EXPORTED = [
     "rrhip_scene_get_tuning", "rrhip_scene_set_tuning", "rrhip_scene_autotune",
     "rrhip_device_copy",
     "rrhip_session_create", "rrhip_session_destroy", "rrhip_render_samples", "rrhip_read_accum",
-    "rrhip_load_accum", "rrhip_session_get_stats",
+    "rrhip_load_accum", "rrhip_session_get_stats", "rrhip_gen_icosphere",
 ]
 
 
@@ -200,6 +200,7 @@ def load():
         "rrhip_scene_set_tuning": (i32, [vp, ctypes.POINTER(TraceTuning)]),
         "rrhip_scene_autotune": (i32, [vp, vp, u64, i32, vp, ctypes.POINTER(TraceTuning),
                                        ctypes.POINTER(ctypes.c_float)]),
+        "rrhip_gen_icosphere": (i32, [u32, u64, ctypes.c_float, ctypes.c_float, u32, vp, vp, vp]),
         "rrhip_gen_heightfield": (i32, [u32, u32, u64, ctypes.c_float, ctypes.c_float, u32, vp, vp]),
         "rrhip_gen_query_rays": (i32, [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                        ctypes.c_float, u64, u64, vp]),
@@ -231,6 +232,18 @@ def gen_heightfield(nx, ny, seed, amplitude=0.1, frequency=4.0, octaves=6):
     idx = np.empty((nx * ny * 2, 3), np.uint32)
     _check(lib.rrhip_gen_heightfield(nx, ny, seed, amplitude, frequency, octaves, _ptr(pos), _ptr(idx)))
     return pos, idx
+
+
+def gen_icosphere(levels, seed, amplitude=0.15, frequency=3.0, octaves=4):
+    """Displaced icosphere: positions (V,3), indices (T,3), normals (V,3)."""
+    lib = load()
+    nv = 10 * 4 ** levels + 2
+    nt = 20 * 4 ** levels
+    pos = np.empty((nv, 3), np.float32)
+    idx = np.empty((nt, 3), np.uint32)
+    nrm = np.empty((nv, 3), np.float32)
+    _check(lib.rrhip_gen_icosphere(levels, seed, amplitude, frequency, octaves, _ptr(pos), _ptr(idx), _ptr(nrm)))
+    return pos, idx, nrm
 
 
 def gen_query_rays(lo, hi, inflate, seed, n):

@@ -241,3 +241,55 @@ def furnace(width=64, height=64, albedo=1.0, mat=None, radius=1.5):
     s.add_light(light(rrhip.LIGHT_DOME, intensity=1.0))
     s.add_sphere((0, 0, 4), radius, m)
     return s
+
+
+def random_rotation(u1, u2, u3):
+    """Uniform random rotation from three uniforms (Shoemake's quaternion)."""
+    a, b = np.sqrt(1.0 - u1), np.sqrt(u1)
+    t1, t2 = 2 * np.pi * u2, 2 * np.pi * u3
+    x, y, z, w = a * np.sin(t1), a * np.cos(t1), b * np.sin(t2), b * np.cos(t2)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def instanced_field(width=1920, height=1080, n_instances=10000, levels=5, n_point_lights=64, seed=1,
+                    box=(200.0, 20.0, 200.0)):
+    """The instanced path-tracing workload: `n_instances` TLAS instances of
+    one displaced icosphere (levels = 5: 20,480 triangles; amplitude 0.15,
+    frequency 3, 4 octaves), each with a uniformly random rotation, uniform
+    scale in [0.5, 2] and position uniform in a 200 x 20 x 200 box (y up);
+    materials chosen 50/50 between GGX metal (pbr.rs from_metal, colour
+    uniform in [0.5, 1]^3) and a rough dielectric of IOR 1.5, roughness
+    U(0.05, 1); `n_point_lights` point lights with intensity log-uniform in
+    [1, 100] placed 25-40 above the field, plus one 60 x 60 rect area light
+    at y = 60 facing down. A matte 0.5 ground quad at y = -2 (a recorded
+    addition: the configuration names no floor). Camera at (0, 45, -140)
+    looking at the origin, 45 degree vertical fov."""
+    rng = np.random.default_rng(seed)
+    s = Scene(width, height, 45.0, eye=(0, 45, -140), look_at=(0, 0, 0))
+    pos, idx, nrm = rrhip.gen_icosphere(levels, seed)
+    mesh = s.add_mesh(pos, idx, nrm)
+    ground = s.add_material(matte((0.5, 0.5, 0.5)))
+    half = 0.75 * max(box[0], box[2])
+    s.add_polygon([(-half, -2, -half), (half, -2, -half), (half, -2, half), (-half, -2, half)], ground)
+    for _ in range(n_point_lights):
+        p = (rng.uniform(-box[0] / 2, box[0] / 2), rng.uniform(25, 40), rng.uniform(-box[2] / 2, box[2] / 2))
+        s.add_light(light(rrhip.LIGHT_POINT, intensity=float(np.exp(rng.uniform(0, np.log(100)))),
+                          color=(1, 1, 1), p=p))
+    corner = np.array([-30.0, 60.0, -30.0])
+    li = s.add_light(light(rrhip.LIGHT_RECT, intensity=3.0, color=(1, 0.95, 0.9), p=corner,
+                           e1=(0, 0, 60), e2=(60, 0, 0)))
+    emitter = s.add_material(matte((1, 1, 1), emission=(3.0, 2.85, 2.7), area_light=li))
+    s.add_polygon([corner, corner + (0, 0, 60), corner + (60, 0, 60), corner + (60, 0, 0)], emitter)
+    for _ in range(n_instances):
+        R = random_rotation(*rng.uniform(0, 1, 3)) * rng.uniform(0.5, 2.0)
+        t = (rng.uniform(-box[0] / 2, box[0] / 2), rng.uniform(0, box[1]), rng.uniform(-box[2] / 2, box[2] / 2))
+        rough = rng.uniform(0.05, 1.0)
+        if rng.uniform() < 0.5:
+            m = s.add_material(metal(tuple(rng.uniform(0.5, 1.0, 3)), rough))
+        else:
+            m = s.add_material(glass(1.5, rough))
+        xf = np.concatenate([R, np.array(t).reshape(3, 1)], axis=1)
+        s.add_instance(mesh, m, xf)
+    return s
