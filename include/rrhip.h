@@ -348,6 +348,12 @@ int rrhip_read_accum(rrhip_session* session, double* out);
  * render.rs's load_accum (renderer.rs:237-254). */
 int rrhip_load_accum(rrhip_session* session, const double* in);
 int rrhip_session_get_stats(rrhip_session* session, rrhip_session_stats* stats);
+/* relMSE of `session`'s mean image against `reference`'s (same size and
+ * context): mean over pixels and channels of (x - r)^2 / (r^2 + eps),
+ * reduced in FP64 on the GPU. No counterpart in render.rs (its tests use
+ * RMSE, tests/pt_tests.rs:77-88); it is the error measure of the
+ * time-to-quality benchmark (eps = 0.01). */
+int rrhip_relmse(rrhip_session* session, rrhip_session* reference, double eps, double* out);
 
 /* ---- timing ------------------------------------------------------------- */
 /* Enqueues `reps` back-to-back closest-hit (any == 0) or any-hit launches

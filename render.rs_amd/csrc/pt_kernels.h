@@ -22,4 +22,9 @@ hipError_t pt_launch_bounce(const PtSceneView& S, const PathBuffers& P, const Pt
 // accumulator in sample order (deterministic, no atomics).
 hipError_t pt_launch_resolve(const PathBuffers& P, const PtParams& prm, double* accum, hipStream_t st);
 
+// Per-block partial sums of relMSE between two accumulators (`blocks`
+// doubles into `partial`, <= 1024 blocks); the caller adds them.
+hipError_t pt_launch_relmse(const double* accum, const double* ref_accum, uint32_t npix, double eps,
+                            double* partial, uint32_t blocks, hipStream_t st);
+
 }  // namespace rrhip

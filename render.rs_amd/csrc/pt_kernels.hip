@@ -449,3 +449,42 @@ hipError_t pt_launch_resolve(const PathBuffers& P, const PtParams& prm, double* 
 }
 
 }  // namespace rrhip
+
+namespace rrhip {
+namespace {
+
+// relMSE partial sums (f64): per pixel and channel (x - r)^2 / (r^2 + eps)
+// with x, r the per-pixel means of two accumulators.
+__global__ __launch_bounds__(256) void relmse_kernel(const double* __restrict__ a, const double* __restrict__ r,
+                                                    uint32_t npix, double eps, double* __restrict__ partial) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += stride) {
+    const double* pa = a + 8 * (size_t)p;
+    const double* pr = r + 8 * (size_t)p;
+    const double na = pa[6] > 0.0 ? pa[6] : 1.0, nr = pr[6] > 0.0 ? pr[6] : 1.0;
+    for (int c = 0; c < 3; c++) {
+      const double x = pa[c] / na, ref = pr[c] / nr;
+      const double d = x - ref;
+      acc += d * d / (ref * ref + eps);
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+}  // namespace
+
+hipError_t pt_launch_relmse(const double* accum, const double* ref_accum, uint32_t npix, double eps,
+                            double* partial, uint32_t blocks, hipStream_t st) {
+  hipLaunchKernelGGL(relmse_kernel, dim3(blocks), dim3(256), 0, st, accum, ref_accum, npix, eps, partial);
+  return hipGetLastError();
+}
+
+}  // namespace rrhip

@@ -29,6 +29,7 @@ struct rrhip_session {
   PathBuffers P{};
   std::vector<void*> mem;
   double* d_accum = nullptr;
+  double* relmse_partial = nullptr;
   uint64_t samples_done = 0;
   double last_ms = 0.0;
 };
@@ -171,6 +172,30 @@ int rrhip_load_accum(rrhip_session* s, const double* in) {
     RRHIP_HIP_TRY(hipMemsetAsync(s->d_accum, 0, bytes, s->scene->ctx->stream));
   }
   RRHIP_HIP_TRY(hipStreamSynchronize(s->scene->ctx->stream));
+  return RRHIP_OK;
+}
+
+int rrhip_relmse(rrhip_session* s, rrhip_session* ref, double eps, double* out) {
+  if (int st = check_session(s)) return st;
+  if (int st = check_session(ref)) return st;
+  if (!out) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_relmse: null output");
+  if (s->prm.npix != ref->prm.npix || s->scene->ctx != ref->scene->ctx)
+    return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_relmse: sessions differ in size or context");
+  RRHIP_HIP_TRY(hipSetDevice(s->scene->ctx->device));
+  hipStream_t st = s->scene->ctx->stream;
+  const uint32_t blocks = 512;
+  double* partial = s->relmse_partial;
+  if (!partial) {
+    RRHIP_HIP_TRY(alloc_n(s, &s->relmse_partial, blocks));
+    partial = s->relmse_partial;
+  }
+  RRHIP_HIP_TRY(pt_launch_relmse(s->d_accum, ref->d_accum, s->prm.npix, eps, partial, blocks, st));
+  std::vector<double> h(blocks);
+  RRHIP_HIP_TRY(hipMemcpyAsync(h.data(), partial, blocks * sizeof(double), hipMemcpyDeviceToHost, st));
+  RRHIP_HIP_TRY(hipStreamSynchronize(st));
+  double sum = 0.0;
+  for (double v : h) sum += v;
+  *out = sum / (3.0 * (double)s->prm.npix);
   return RRHIP_OK;
 }
 

@@ -36,7 +36,7 @@ EXPORTED = [
     "rrhip_scene_get_tuning", "rrhip_scene_set_tuning", "rrhip_scene_autotune",
     "rrhip_device_copy",
     "rrhip_session_create", "rrhip_session_destroy", "rrhip_render_samples", "rrhip_read_accum",
-    "rrhip_load_accum", "rrhip_session_get_stats", "rrhip_gen_icosphere",
+    "rrhip_load_accum", "rrhip_session_get_stats", "rrhip_gen_icosphere", "rrhip_relmse",
 ]
 
 
@@ -196,6 +196,7 @@ def load():
         "rrhip_read_accum": (i32, [vp, vp]),
         "rrhip_load_accum": (i32, [vp, vp]),
         "rrhip_session_get_stats": (i32, [vp, ctypes.POINTER(SessionStats)]),
+        "rrhip_relmse": (i32, [vp, vp, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]),
         "rrhip_scene_get_tuning": (i32, [vp, ctypes.POINTER(TraceTuning)]),
         "rrhip_scene_set_tuning": (i32, [vp, ctypes.POINTER(TraceTuning)]),
         "rrhip_scene_autotune": (i32, [vp, vp, u64, i32, vp, ctypes.POINTER(TraceTuning),
@@ -389,6 +390,11 @@ class Session:
         else:
             acc = np.ascontiguousarray(acc, np.float64)
             _check(load().rrhip_load_accum(self.h, _ptr(acc)))
+
+    def relmse(self, reference, eps=0.01):
+        out = ctypes.c_double()
+        _check(load().rrhip_relmse(self.h, reference.h, eps, ctypes.byref(out)))
+        return out.value
 
     def stats(self):
         st = SessionStats()
