@@ -197,3 +197,37 @@ def test_instanced_field_full_size_properties(ctx):
     a.close()
     b.close()
     ps.close()
+
+
+def test_relmse_on_device_matches_numpy(ctx):
+    """relMSE = mean over pixels and channels of (x - r)^2 / (r^2 + eps),
+    x and r the per-pixel means of two sessions, reduced in f64 on the
+    device; checked against the same sum over the read-back accumulators,
+    against zero for a session and itself, and against a session that
+    differs by a known constant factor (closed form)."""
+    say("relMSE on the device: box room 64x64, 16 spp vs 256 spp")
+    s = scenes.box_room(64, 64)
+    ps = rrhip.PtScene(ctx, s.desc())
+    a = ps.session(rrhip.RenderParams.make(max_depth=8))
+    a.render(0, 16)
+    r = ps.session(rrhip.RenderParams.make(max_depth=8, sample_base=1 << 32))
+    r.render(0, 256)
+    acc_a, acc_r = a.accum(), r.accum()
+    x = acc_a[:, :3] / acc_a[:, 6:7]
+    ref = acc_r[:, :3] / acc_r[:, 6:7]
+    for eps in (0.01, 1.0):
+        want = float(np.mean((x - ref) ** 2 / (ref * ref + eps)))
+        got = a.relmse(r, eps)
+        say(f"eps {eps}: device {got!r} numpy {want!r}")
+        assert abs(got - want) <= 1e-12 * want
+    assert a.relmse(a) == 0.0
+    # Scaling every sum of the reference by 2 makes x = r / 2 wherever the
+    # two accumulators agree: load a's own sums, doubled, into r.
+    doubled = acc_a.copy()
+    doubled[:, :6] *= 2.0
+    r.load_accum(doubled)
+    want = float(np.mean((x - 2 * x) ** 2 / ((2 * x) ** 2 + 0.01)))
+    assert abs(a.relmse(r) - want) <= 1e-12 * want
+    a.close()
+    r.close()
+    ps.close()
