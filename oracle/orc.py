@@ -259,3 +259,126 @@ def scene_intersect(desc, o, d):
                                    _p(normal), ctypes.byref(front), ctypes.byref(mat)):
         return None
     return t.value, point, normal, bool(front.value), mat.value
+
+
+# ---- neural radiance cache ---------------------------------------------------
+
+MLP_PARAMS = 19648
+N_IN = 48
+
+
+def _nrc_sigs():
+    lib = load()
+    if getattr(lib, "_nrc_ready", False):
+        return lib
+    vp, u64, u32, i32, f64 = (ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int,
+                              ctypes.c_double)
+    sig = {
+        "orc_nrc_level_res": (None, [vp, vp]),
+        "orc_nrc_corner_index": (u32, [vp, i32, u32, u32, u32]),
+        "orc_nrc_inputs": (None, [vp, vp, vp, u64, vp]),
+        "orc_nrc_forward": (None, [vp, vp, u64, i32, vp]),
+        "orc_nrc_infer": (None, [vp, vp, vp, vp, u64, i32, vp]),
+        "orc_nrc_gradients": (f64, [vp, vp, vp, vp, vp, u64, i32, vp, vp]),
+        "orc_nrc_mlp_gradients": (f64, [vp, vp, vp, u64, i32, vp, vp]),
+        "orc_nrc_adam": (None, [vp, vp, vp, vp, u64, u64, f64, f64, f64, f64]),
+        "orc_bf16_round": (f64, [f64]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    lib._nrc_ready = True
+    return lib
+
+
+class NrcConfig:
+    """Hash-grid geometry: levels, log2 entries per level, coarsest and
+    finest resolution (cells per axis), position bounds."""
+
+    def __init__(self, levels=16, log2_table=19, base_res=16.0, max_res=2048.0, lo=(0, 0, 0), hi=(1, 1, 1)):
+        self.levels, self.log2_table = levels, log2_table
+        self.base_res, self.max_res = base_res, max_res
+        self.lo, self.hi = tuple(lo), tuple(hi)
+
+    def arr(self):
+        return np.array([self.levels, self.log2_table, self.base_res, self.max_res, *self.lo, *self.hi],
+                        np.float64)
+
+    @property
+    def table_floats(self):
+        return self.levels * (1 << self.log2_table) * 2
+
+
+def nrc_level_res(cfg):
+    out = np.zeros(cfg.levels, np.int32)
+    _nrc_sigs().orc_nrc_level_res(_p(cfg.arr()), _p(out))
+    return out
+
+
+def nrc_corner_index(cfg, level, x, y, z):
+    return _nrc_sigs().orc_nrc_corner_index(_p(cfg.arr()), level, x, y, z)
+
+
+def _q(queries):
+    q = np.ascontiguousarray(queries, np.float32).reshape(-1, 16)
+    return q
+
+
+def nrc_inputs(cfg, table, queries):
+    q = _q(queries)
+    t = np.ascontiguousarray(table, np.float32)
+    out = np.zeros((q.shape[0], N_IN))
+    _nrc_sigs().orc_nrc_inputs(_p(cfg.arr()), _p(t), _p(q), q.shape[0], _p(out))
+    return out
+
+
+def nrc_forward(w, x, mimic=False):
+    w = np.ascontiguousarray(w, np.float64)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, N_IN)
+    out = np.zeros((x.shape[0], 3))
+    _nrc_sigs().orc_nrc_forward(_p(w), _p(x), x.shape[0], int(mimic), _p(out))
+    return out
+
+
+def nrc_infer(cfg, table, w, queries, mimic=False):
+    q = _q(queries)
+    t = np.ascontiguousarray(table, np.float32)
+    w = np.ascontiguousarray(w, np.float64)
+    out = np.zeros((q.shape[0], 3))
+    _nrc_sigs().orc_nrc_infer(_p(cfg.arr()), _p(t), _p(w), _p(q), q.shape[0], int(mimic), _p(out))
+    return out
+
+
+def nrc_gradients(cfg, table, w, queries, targets, mimic=False, want_table=True):
+    """Returns (loss, grad_w, grad_table or None)."""
+    q = _q(queries)
+    t = np.ascontiguousarray(table, np.float32)
+    w = np.ascontiguousarray(w, np.float64)
+    tg = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+    gw = np.zeros(MLP_PARAMS)
+    gt = np.zeros(cfg.table_floats) if want_table else None
+    loss = _nrc_sigs().orc_nrc_gradients(_p(cfg.arr()), _p(t), _p(w), _p(q), _p(tg), q.shape[0], int(mimic),
+                                         _p(gw), _p(gt) if want_table else None)
+    return loss, gw, gt
+
+
+def nrc_mlp_gradients(w, x, targets, mimic=False):
+    """MLP alone on raw inputs: returns (loss, grad_w, grad_x)."""
+    w = np.ascontiguousarray(w, np.float64)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, N_IN)
+    tg = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+    gw = np.zeros(MLP_PARAMS)
+    gx = np.zeros_like(x)
+    loss = _nrc_sigs().orc_nrc_mlp_gradients(_p(w), _p(x), _p(tg), x.shape[0], int(mimic), _p(gw), _p(gx))
+    return loss, gw, gx
+
+
+def nrc_adam(p, g, m, v, t, lr=1e-2, b1=0.9, b2=0.99, eps=1e-15):
+    """In place on f64 arrays p, m, v."""
+    g = np.ascontiguousarray(g, np.float64)
+    _nrc_sigs().orc_nrc_adam(_p(p), _p(g), _p(m), _p(v), p.size, t, lr, b1, b2, eps)
+
+
+def bf16_round(x):
+    return _nrc_sigs().orc_bf16_round(float(x))
