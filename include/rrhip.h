@@ -43,6 +43,7 @@ typedef enum rrhip_status {
 typedef struct rrhip_ctx rrhip_ctx;
 typedef struct rrhip_scene rrhip_scene;
 typedef struct rrhip_session rrhip_session;
+typedef struct rrhip_nrc rrhip_nrc;
 
 /* Query ray, 32 bytes. render.rs's GPU ray query takes [f32; 6]
  * (origin, direction) and tests t > 1e-6 (src/raytracer/metal/renderer.rs:
@@ -354,6 +355,75 @@ int rrhip_session_get_stats(rrhip_session* session, rrhip_session_stats* stats);
  * RMSE, tests/pt_tests.rs:77-88); it is the error measure of the
  * time-to-quality benchmark (eps = 0.01). */
 int rrhip_relmse(rrhip_session* session, rrhip_session* reference, double eps, double* out);
+
+/* ---- neural radiance cache ------------------------------------------------ */
+/* render.rs has no radiance cache: this section is the learned part of the
+ * HIP backend (a multiresolution hash grid + 48-64x5-3 ReLU MLP with bf16
+ * weights on the matrix cores, trained online with Adam on the relative
+ * L2 loss). */
+typedef struct rrhip_nrc_config {
+  uint32_t struct_size;     /* sizeof(rrhip_nrc_config) */
+  uint32_t levels;          /* hash-grid levels, 1..16 */
+  uint32_t log2_table;      /* entries per level = 2^log2_table, 8..24 */
+  uint32_t seed;            /* parameter initialisation */
+  float base_res, max_res;  /* cells per axis of the coarsest / finest level */
+  float lo[3], hi[3];       /* position bounds; hi <= lo on an axis: the scene's bounds */
+  float learning_rate, beta1, beta2, epsilon; /* Adam */
+  uint32_t batch;           /* samples per optimiser step, 64..65536 */
+  uint32_t steps_per_pass;  /* optimiser steps per rendering pass */
+  uint32_t train_paths;     /* training paths traced per rendering pass */
+  float spread;             /* area-spread termination constant c */
+} rrhip_nrc_config;
+
+/* A cache query / training sample (64 bytes): the network input at a
+ * path vertex. dir points from the vertex towards the viewer; normal is
+ * the shading normal on dir's side. */
+typedef struct rrhip_nrc_query {
+  float pos[3], roughness;
+  float dir[3], pad0;
+  float normal[3], pad1;
+  float albedo[3], pad2;
+} rrhip_nrc_query;
+
+typedef struct rrhip_nrc_info {
+  uint32_t levels, log2_table;
+  uint32_t res[16];         /* cells per axis of each level */
+  uint64_t mlp_params;      /* f32 count of rrhip_nrc_get_params' mlp array */
+  uint64_t grid_params;     /* levels * 2^log2_table * 2 */
+  uint64_t steps;           /* optimiser steps taken */
+  float last_loss;          /* loss of the last training batch */
+  float lo[3], hi[3];       /* bounds in use */
+  uint32_t reserved;
+} rrhip_nrc_info;
+
+/* Creates a cache with freshly initialised parameters (grid U(-1e-4,
+ * 1e-4), He-uniform weights, both from PCG32(seed)). config = NULL takes
+ * the defaults: 16 levels, 2^19 entries, resolutions 16..2048, Adam
+ * (1e-2, 0.9, 0.99, 1e-15), batch 16384, 4 steps and 16384 training
+ * paths per pass, c = 0.01. */
+int rrhip_nrc_create(rrhip_ctx* ctx, const rrhip_nrc_config* config, rrhip_nrc** out);
+int rrhip_nrc_destroy(rrhip_nrc* nrc);
+int rrhip_nrc_get_info(rrhip_nrc* nrc, rrhip_nrc_info* info);
+/* Parameters in f32: mlp = W0 [64][48], W1..W4 [64][64], W5 [3][64]
+ * (row = output unit), grid = [level][entry][feature]. Either may be NULL.
+ * set_params also clears the optimiser state. */
+int rrhip_nrc_get_params(rrhip_nrc* nrc, float* mlp, float* grid);
+int rrhip_nrc_set_params(rrhip_nrc* nrc, const float* mlp, const float* grid);
+/* Cache radiance (rgb, n x 3 floats) of n host queries. */
+int rrhip_nrc_infer(rrhip_nrc* nrc, const rrhip_nrc_query* queries, uint64_t n, float* rgb);
+/* One optimiser step on n <= 65536 samples with targets (n x 3); the
+ * batch's loss (before the step) into *loss when non-NULL. */
+int rrhip_nrc_train_step(rrhip_nrc* nrc, const rrhip_nrc_query* queries, const float* targets, uint64_t n,
+                         float* loss);
+/* Loss and gradients of a batch without a step (mlp_grad / grid_grad in
+ * the get_params layouts; either may be NULL). */
+int rrhip_nrc_gradients(rrhip_nrc* nrc, const rrhip_nrc_query* queries, const float* targets, uint64_t n,
+                        float* loss, float* mlp_grad, float* grid_grad);
+/* Attaches a cache to a session (NULL detaches): render passes then
+ * terminate paths into the cache by the area-spread criterion and train
+ * it online on the suffixes of extra training paths. Cache and session
+ * must share a context. */
+int rrhip_session_set_nrc(rrhip_session* session, rrhip_nrc* nrc);
 
 /* ---- timing ------------------------------------------------------------- */
 /* Enqueues `reps` back-to-back closest-hit (any == 0) or any-hit launches

@@ -321,6 +321,10 @@ hipError_t upload_pt(rrhip_scene* s, const PtSceneHost& h) {
   v.inf_total = h.inf_total;
   for (int c = 0; c < 3; c++) v.background[c] = h.background[c];
   v.cam = h.cam;
+  for (int c = 0; c < 3; c++) {
+    s->pt_lo[c] = h.lo[c];
+    s->pt_hi[c] = h.hi[c];
+  }
   return e;
 }
 

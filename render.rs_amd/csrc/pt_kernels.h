@@ -11,13 +11,16 @@ constexpr uint32_t kPtTraceStack = 48;
 
 // Path slots [0, total) with slot = s * npix + pixel: camera rays of
 // samples prm.sample0 + s; fills the live queue (q0) and resets counters.
+// With N.ntrain > 0 also training paths in slots [total, total + ntrain).
 hipError_t pt_launch_raygen(const PtSceneView& S, const PathBuffers& P, const PtParams& prm,
-                            uint32_t total, hipStream_t st);
+                            uint32_t total, const NrcPathBuffers& N, hipStream_t st);
 // One bounce over the live queue `q`: closest hit, shading (emission + MIS,
 // NEE shadow rays, BSDF continuation, Russian roulette) into `qnext`,
-// shadow rays, counter rotation. `capacity` bounds every queue.
+// shadow rays, counter rotation. `capacity` bounds every queue. With
+// N.q set, paths terminate into radiance-cache queries (see NrcPathBuffers).
 hipError_t pt_launch_bounce(const PtSceneView& S, const PathBuffers& P, const PtParams& prm,
-                            const uint32_t* q, uint32_t* qnext, uint32_t capacity, hipStream_t st);
+                            const uint32_t* q, uint32_t* qnext, uint32_t capacity, const NrcPathBuffers& N,
+                            hipStream_t st);
 // Adds each pixel's pass samples (firefly-clamped) into the f64
 // accumulator in sample order (deterministic, no atomics).
 hipError_t pt_launch_resolve(const PathBuffers& P, const PtParams& prm, double* accum, hipStream_t st);
@@ -26,5 +29,14 @@ hipError_t pt_launch_resolve(const PathBuffers& P, const PtParams& prm, double* 
 // doubles into `partial`, <= 1024 blocks); the caller adds them.
 hipError_t pt_launch_relmse(const double* accum, const double* ref_accum, uint32_t npix, double eps,
                             double* partial, uint32_t blocks, hipStream_t st);
+
+// Training samples of the pass's training paths, in (path, vertex) order
+// (deterministic): inputs into bq, targets into bt (3 floats each), at
+// most `cap`; the count into N.counts[1]. `scratch` holds ntrain u32.
+hipError_t pt_launch_nrc_samples(const PathBuffers& P, const NrcPathBuffers& N, float clamp, uint32_t* scratch,
+                                 uint32_t cap, NrcQuery* bq, float* bt, hipStream_t st);
+// *n_out = the size of optimiser batch k (of `batch` samples) of the pass.
+hipError_t pt_launch_nrc_set_batch(const NrcPathBuffers& N, uint32_t k, uint32_t batch, uint32_t* n_out,
+                                   hipStream_t st);
 
 }  // namespace rrhip

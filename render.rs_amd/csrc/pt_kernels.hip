@@ -31,6 +31,8 @@ constexpr int kTraceBlock = 128;
 constexpr int kTraceStack = 48;
 
 constexpr uint32_t kFlagFromCamera = 1u << 8;
+constexpr uint32_t kFlagTrain = 1u << 9;   // radiance-cache training path
+constexpr uint32_t kFlagSuffix = 1u << 10; // training path past its termination vertex
 
 __device__ __forceinline__ uint32_t lane_id() {
   return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
@@ -59,19 +61,38 @@ __device__ __forceinline__ uint4 store_rng(const Pcg32& r) {
   return make_uint4((uint32_t)r.state, (uint32_t)(r.state >> 32), (uint32_t)r.inc, (uint32_t)(r.inc >> 32));
 }
 
-__global__ __launch_bounds__(kBlock) void raygen_kernel(PtSceneView S, PathBuffers P, PtParams prm, uint32_t total) {
+__global__ __launch_bounds__(kBlock) void raygen_kernel(PtSceneView S, PathBuffers P, PtParams prm, uint32_t total,
+                                                       NrcPathBuffers N) {
+  const uint32_t all = total + N.ntrain;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    P.counts[0] = total;
+    P.counts[0] = all;
     P.counts[1] = 0;
     P.counts[2] = 0;
+    if (N.counts) {
+      N.counts[0] = 0;
+      N.counts[1] = 0;
+    }
   }
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < total; slot += stride) {
-    const uint32_t pixel = slot % prm.npix;
-    const uint32_t s = slot / prm.npix;
+  for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < all; slot += stride) {
+    uint32_t pixel, flags = kFlagFromCamera;
+    uint64_t sample;
+    if (slot < total) {
+      pixel = slot % prm.npix;
+      sample = prm.sample0 + slot / prm.npix;
+    } else {
+      // Training path i of this pass: a hashed pixel and a PCG32 stream of
+      // its own (top bit set: never a rendering sample index).
+      const uint32_t i = slot - total;
+      const uint64_t tag = ((prm.sample0 << 22) | i) & 0x7FFFFFFFFFFFFFFFull;
+      pixel = (uint32_t)(Pcg32::mix(tag ^ 0x6e72637472616e69ull) % prm.npix);
+      sample = 0x8000000000000000ull | tag;
+      flags |= kFlagTrain;
+      N.tn[i] = 0;
+    }
     const uint32_t x = pixel % S.cam.width, y = pixel / S.cam.width;
     // sampler.rs:24-32 per-(pixel, sample) stream; pt/mod.rs:340-356
-    Pcg32 rng = Pcg32::for_pixel_sample(pixel, prm.sample0 + s);
+    Pcg32 rng = Pcg32::for_pixel_sample(pixel, sample);
     V3 o, d;
     camera_ray(S.cam, x, y, rng, o, d);
     P.o[slot] = make_float4(o.x, o.y, o.z, 0.f);
@@ -80,7 +101,7 @@ __global__ __launch_bounds__(kBlock) void raygen_kernel(PtSceneView S, PathBuffe
     P.L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
     P.prev[slot] = make_float4(o.x, o.y, o.z, 0.f);
     P.rng[slot] = store_rng(rng);
-    P.flags[slot] = kFlagFromCamera;
+    P.flags[slot] = flags;
     P.q0[slot] = slot;
   }
 }
@@ -258,16 +279,24 @@ __device__ __forceinline__ bool sample_light(const PtSceneView& S, const PtLight
   }
 }
 
+// With NRC, the area-spread heuristic of Mueller et al. 2021 ("Real-time
+// neural radiance caching for path tracing", sec. 4): a(x1..xn) =
+// (sum_{i=2..n} sqrt(|x_{i-1} - x_i|^2 / (p(w_i) |cos theta_i|)))^2 against
+// c * a0, a0 = |x0 - x1|^2 / (4 pi |cos theta_1|).
+template <bool NRC>
 __global__ __launch_bounds__(kBlock) void shade_kernel(PtSceneView S, PathBuffers P, PtParams prm,
-                                                      const uint32_t* __restrict__ q, uint32_t* __restrict__ qnext) {
+                                                      const uint32_t* __restrict__ q, uint32_t* __restrict__ qnext,
+                                                      NrcPathBuffers N) {
   const uint32_t n = P.counts[0];
   const uint32_t stride = gridDim.x * blockDim.x;
   // Block-uniform loop bound so every lane reaches the wave-wide appends.
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
     const uint32_t i = base + threadIdx.x;
     const bool act = i < n;
-    bool cont = false, shadow = false;
+    bool cont = false, shadow = false, query = false;
     ShadowOut so;
+    NrcQuery nq;
+    V3 qbeta;
     uint32_t slot = 0;
     if (act) {
       slot = q[i];
@@ -281,6 +310,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtSceneView S, PathBuffer
       const V3 prev_origin = v3(pv4.x, pv4.y, pv4.z);
       const bool from_camera = (flags & kFlagFromCamera) != 0;
       uint32_t depth = flags & 0xFFu;
+      float a0 = d4.w, asum = pv4.w;
       if (hn.w < 0.f) {
         // Miss: dome (with MIS) or background (pt/mod.rs:643-659).
         if (S.dome >= 0) {
@@ -310,10 +340,64 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtSceneView S, PathBuffer
           }
           Lr = Lr + beta * em * w;
         }
+        bool scatter = true;
+        if constexpr (NRC) {
+          const float cosv = fmaxf(fabsf(dot(ng, rd)), 1e-4f);
+          const float t2 = hn.w * hn.w;
+          if (from_camera) {
+            a0 = t2 / (4.f * kPi * cosv);
+            asum = 0.f;
+          } else {
+            asum += sqrtf(t2 / fmaxf(prev_pdf * cosv, 1e-30f));
+          }
+          const bool wide = asum * asum > N.spread * a0;
+          const bool train = (flags & kFlagTrain) != 0;
+          bool record = false;
+          if (!train) {
+            query = wide && m.has_lobes;
+          } else if (!(flags & kFlagSuffix)) {
+            if (wide) {
+              flags |= kFlagSuffix;
+              asum = 0.f;
+            }
+            record = m.has_lobes != 0;
+          } else {
+            query = wide && m.has_lobes;
+            record = !query && m.has_lobes;
+          }
+          if (query || record) {
+            nq.pos[0] = p.x;
+            nq.pos[1] = p.y;
+            nq.pos[2] = p.z;
+            nq.roughness = m.nrc_rough;
+            nq.dir[0] = wo.x;
+            nq.dir[1] = wo.y;
+            nq.dir[2] = wo.z;
+            nq.pad0 = 0.f;
+            nq.normal[0] = nrm.x;
+            nq.normal[1] = nrm.y;
+            nq.normal[2] = nrm.z;
+            nq.pad1 = 0.f;
+            nq.albedo[0] = m.nrc_albedo[0];
+            nq.albedo[1] = m.nrc_albedo[1];
+            nq.albedo[2] = m.nrc_albedo[2];
+            nq.pad2 = 0.f;
+          }
+          if (record && depth < N.max_vertices) {
+            const uint32_t tp = slot - N.train_base;
+            const size_t r = (size_t)tp * N.max_vertices + depth;
+            N.tq[r] = nq;
+            N.tb[r] = make_float4(beta.x, beta.y, beta.z, 0.f);
+            N.tl[r] = make_float4(Lr.x, Lr.y, Lr.z, 0.f);
+            N.tn[tp] = depth + 1;
+          }
+          qbeta = beta;
+          scatter = !query;
+        }
         const Frame fr = make_frame(nrm);
         const V3 wo_l = to_local(fr, wo);
         // NEE through the light BVH (pt/mod.rs:924-956)
-        if (S.num_lights > 0 && m.has_lobes) {
+        if (scatter && S.num_lights > 0 && m.has_lobes) {
           uint32_t li;
           float pick;
           if (ls_sample(S, p, rnd(rng), li, pick)) {
@@ -325,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtSceneView S, PathBuffer
         }
         // BSDF continuation (pt/mod.rs:958-987)
         BsdfSample bs;
-        if (bsdf_sample(m, wo_l, eta, rng, bs)) {
+        if (scatter && bsdf_sample(m, wo_l, eta, rng, bs)) {
           const V3 wi = to_world(fr, bs.wi);
           beta = beta * bs.f * (fabsf(bs.wi.z) / bs.pdf);
           const V3 org = p + nrm * (bs.transmitted ? -kRayOffset : kRayOffset);
@@ -342,9 +426,9 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtSceneView S, PathBuffer
           }
           cont = alive && depth < prm.max_depth;
           P.o[slot] = make_float4(org.x, org.y, org.z, bs.pdf);
-          P.d[slot] = make_float4(nd.x, nd.y, nd.z, 0.f);
-          P.prev[slot] = make_float4(p.x, p.y, p.z, 0.f);
-          flags = depth;  // from_camera cleared
+          P.d[slot] = make_float4(nd.x, nd.y, nd.z, a0);
+          P.prev[slot] = make_float4(p.x, p.y, p.z, asum);
+          flags = depth | (flags & (kFlagTrain | kFlagSuffix));  // from_camera cleared
         }
       }
       P.beta[slot] = make_float4(beta.x, beta.y, beta.z, 0.f);
@@ -361,6 +445,14 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtSceneView S, PathBuffer
     }
     const uint32_t nidx = wave_append(&P.counts[1], act && cont);
     if (act && cont) qnext[nidx] = slot;
+    if constexpr (NRC) {
+      const uint32_t qidx = wave_append(&N.counts[0], act && query);
+      if (act && query) {
+        N.q[qidx] = nq;
+        N.q_slot[qidx] = slot;
+        N.q_beta[qidx] = make_float4(qbeta.x, qbeta.y, qbeta.z, 0.f);
+      }
+    }
   }
 }
 
@@ -427,17 +519,21 @@ uint32_t grid_for(uint64_t n, int block = kBlock) {
 }  // namespace
 
 hipError_t pt_launch_raygen(const PtSceneView& S, const PathBuffers& P, const PtParams& prm, uint32_t total,
-                            hipStream_t st) {
-  hipLaunchKernelGGL(raygen_kernel, dim3(grid_for(total)), dim3(kBlock), 0, st, S, P, prm, total);
+                            const NrcPathBuffers& N, hipStream_t st) {
+  hipLaunchKernelGGL(raygen_kernel, dim3(grid_for((uint64_t)total + N.ntrain)), dim3(kBlock), 0, st, S, P, prm, total,
+                     N);
   return hipGetLastError();
 }
 
 hipError_t pt_launch_bounce(const PtSceneView& S, const PathBuffers& P, const PtParams& prm, const uint32_t* q,
-                            uint32_t* qnext, uint32_t capacity, hipStream_t st) {
+                            uint32_t* qnext, uint32_t capacity, const NrcPathBuffers& N, hipStream_t st) {
   const uint32_t g = grid_for(capacity);
   const uint32_t gt = grid_for(capacity, kTraceBlock);
   hipLaunchKernelGGL(extend_kernel, dim3(gt), dim3(kTraceBlock), 0, st, S, P, q);
-  hipLaunchKernelGGL(shade_kernel, dim3(g), dim3(kBlock), 0, st, S, P, prm, q, qnext);
+  if (N.q)
+    hipLaunchKernelGGL(shade_kernel<true>, dim3(g), dim3(kBlock), 0, st, S, P, prm, q, qnext, N);
+  else
+    hipLaunchKernelGGL(shade_kernel<false>, dim3(g), dim3(kBlock), 0, st, S, P, prm, q, qnext, N);
   hipLaunchKernelGGL(shadow_kernel, dim3(gt), dim3(kTraceBlock), 0, st, S, P);
   hipLaunchKernelGGL(rotate_kernel, dim3(1), dim3(1), 0, st, P);
   return hipGetLastError();
@@ -484,6 +580,119 @@ __global__ __launch_bounds__(256) void relmse_kernel(const double* __restrict__ 
 hipError_t pt_launch_relmse(const double* accum, const double* ref_accum, uint32_t npix, double eps,
                             double* partial, uint32_t blocks, hipStream_t st) {
   hipLaunchKernelGGL(relmse_kernel, dim3(blocks), dim3(256), 0, st, accum, ref_accum, npix, eps, partial);
+  return hipGetLastError();
+}
+
+}  // namespace rrhip
+
+namespace rrhip {
+namespace {
+
+// Target of training record r (path radiance `lf` at the end): the
+// radiance scattered at the vertex, (L_end - L_at_vertex) / throughput,
+// with the pixel estimator's firefly clamp on its luminance. Records whose
+// throughput has a channel below 1e-6 carry no usable target.
+__device__ __forceinline__ bool nrc_target(const NrcPathBuffers& N, size_t r, float4 lf, float clamp, float t[3]) {
+  const float4 b = N.tb[r], l = N.tl[r];
+  const float bb[3] = {b.x, b.y, b.z}, ll[3] = {l.x, l.y, l.z}, ff[3] = {lf.x, lf.y, lf.z};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    if (!(bb[c] > 1e-6f)) return false;
+    t[c] = fmaxf((ff[c] - ll[c]) / bb[c], 0.f);
+    if (!isfinite(t[c])) return false;
+  }
+  const float lm = 0.2126f * t[0] + 0.7152f * t[1] + 0.0722f * t[2];
+  if (clamp > 0.f && lm > clamp) {
+    const float s = clamp / lm;
+    t[0] *= s;
+    t[1] *= s;
+    t[2] *= s;
+  }
+  return true;
+}
+
+__global__ __launch_bounds__(256) void nrc_count_kernel(PathBuffers P, NrcPathBuffers N, float clamp,
+                                                       uint32_t* __restrict__ cnt) {
+  const uint32_t tp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tp >= N.ntrain) return;
+  const uint32_t nv = N.tn[tp];
+  const float4 lf = P.L[N.train_base + tp];
+  uint32_t c = 0;
+  for (uint32_t v = 0; v < nv; v++) {
+    float t[3];
+    c += nrc_target(N, (size_t)tp * N.max_vertices + v, lf, clamp, t) ? 1u : 0u;
+  }
+  cnt[tp] = c;
+}
+
+// Exclusive prefix sum of cnt[0, n) in place (one block of 1024 threads,
+// each over a contiguous chunk); the clamped total into N.counts[1].
+__global__ __launch_bounds__(1024) void nrc_scan_kernel(uint32_t* __restrict__ cnt, uint32_t n, uint32_t cap,
+                                                       NrcPathBuffers N) {
+  __shared__ uint32_t part[1024];
+  const uint32_t t = threadIdx.x;
+  const uint32_t chunk = (n + 1023) / 1024;
+  const uint32_t b = t * chunk, e = min(n, b + chunk);
+  uint32_t s = 0;
+  for (uint32_t i = b; i < e; i++) s += cnt[i];
+  part[t] = s;
+  __syncthreads();
+  for (uint32_t off = 1; off < 1024; off <<= 1) {
+    const uint32_t v = t >= off ? part[t - off] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[t] - s;  // exclusive
+  for (uint32_t i = b; i < e; i++) {
+    const uint32_t c = cnt[i];
+    cnt[i] = run;
+    run += c;
+  }
+  if (t == 1023) N.counts[1] = min(part[1023], cap);
+}
+
+__global__ __launch_bounds__(256) void nrc_scatter_kernel(PathBuffers P, NrcPathBuffers N, float clamp,
+                                                         const uint32_t* __restrict__ off, uint32_t cap,
+                                                         NrcQuery* __restrict__ bq, float* __restrict__ bt) {
+  const uint32_t tp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tp >= N.ntrain) return;
+  const uint32_t nv = N.tn[tp];
+  const float4 lf = P.L[N.train_base + tp];
+  uint32_t o = off[tp];
+  for (uint32_t v = 0; v < nv && o < cap; v++) {
+    const size_t r = (size_t)tp * N.max_vertices + v;
+    float t[3];
+    if (!nrc_target(N, r, lf, clamp, t)) continue;
+    bq[o] = N.tq[r];
+    bt[3 * (size_t)o + 0] = t[0];
+    bt[3 * (size_t)o + 1] = t[1];
+    bt[3 * (size_t)o + 2] = t[2];
+    o++;
+  }
+}
+
+__global__ void nrc_set_batch_kernel(const NrcPathBuffers N, uint32_t k, uint32_t batch, uint32_t* __restrict__ n_out) {
+  const uint32_t total = N.counts[1];
+  const uint64_t first = (uint64_t)k * batch;
+  *n_out = total > first ? (uint32_t)min<uint64_t>(total - first, batch) : 0u;
+}
+
+}  // namespace
+
+hipError_t pt_launch_nrc_samples(const PathBuffers& P, const NrcPathBuffers& N, float clamp, uint32_t* scratch,
+                                 uint32_t cap, NrcQuery* bq, float* bt, hipStream_t st) {
+  if (N.ntrain == 0) return hipSuccess;
+  const uint32_t g = (N.ntrain + 255) / 256;
+  hipLaunchKernelGGL(nrc_count_kernel, dim3(g), dim3(256), 0, st, P, N, clamp, scratch);
+  hipLaunchKernelGGL(nrc_scan_kernel, dim3(1), dim3(1024), 0, st, scratch, N.ntrain, cap, N);
+  hipLaunchKernelGGL(nrc_scatter_kernel, dim3(g), dim3(256), 0, st, P, N, clamp, scratch, cap, bq, bt);
+  return hipGetLastError();
+}
+
+hipError_t pt_launch_nrc_set_batch(const NrcPathBuffers& N, uint32_t k, uint32_t batch, uint32_t* n_out,
+                                   hipStream_t st) {
+  hipLaunchKernelGGL(nrc_set_batch_kernel, dim3(1), dim3(1), 0, st, N, k, batch, n_out);
   return hipGetLastError();
 }
 

@@ -82,6 +82,14 @@ PtMaterial make_material(const rrhip_material_desc& m) {
   g.glass_ior = (float)m.glass_ior;
   put(g.emission, add(d3(m.emission), d3(m.glow)));
   g.area_light = m.area_light;
+  // Radiance-cache inputs: the lobes' summed albedo (clamped to [0, 1])
+  // and the roughness of the strongest non-diffuse lobe (1 if none).
+  D3 alb = add(add(mul(d3(m.diffuse_color), m.diffuse_gain * under), ws > 0 ? f0 : D3{0, 0, 0}),
+               mul(d3(m.refraction_color), m.glass_gain * under));
+  const double av[3] = {alb.x, alb.y, alb.z};
+  for (int c = 0; c < 3; c++) g.nrc_albedo[c] = (float)std::min(std::max(av[c], 0.0), 1.0);
+  g.nrc_rough = 1.f;
+  if (ws > 0 || wg > 0) g.nrc_rough = (float)(wg > ws ? m.glass_roughness : m.specular_roughness);
   return g;
 }
 
@@ -248,6 +256,12 @@ bool build_instances(const rrhip_scene_desc& d, PtSceneHost* out, std::string* e
   std::vector<uint32_t> order;
   if (!build_box_bvh(boxes.data(), d.num_instances, bp, &tlas, &order, err)) return false;
   out->nodes = tlas.nodes;
+  if (!tlas.nodes.empty()) {
+    for (int a = 0; a < 3; a++) {
+      out->lo[a] = std::min(out->lo[a], tlas.root_lo[a]);
+      out->hi[a] = std::max(out->hi[a], tlas.root_hi[a]);
+    }
+  }
   for (uint32_t k : order) out->insts.push_back(raw[k]);
   const int tlas_levels = tlas.nodes.empty() ? 0 : tlas.depth + 1;
   // Concatenate the BLASes, rebasing child references.
@@ -397,7 +411,31 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
     out->spheres.push_back(g);
   }
   put(out->background, d3(d.background));
+  for (int a = 0; a < 3; a++) {
+    out->lo[a] = INFINITY;
+    out->hi[a] = -INFINITY;
+  }
+  auto grow = [&](D3 p, double r) {
+    const double v[3] = {p.x, p.y, p.z};
+    for (int a = 0; a < 3; a++) {
+      out->lo[a] = std::min(out->lo[a], (float)(v[a] - r));
+      out->hi[a] = std::max(out->hi[a], (float)(v[a] + r));
+    }
+  };
+  for (uint32_t i = 0; i < d.num_triangles; i++) {
+    grow(d3(d.triangles[i].v0), 0.0);
+    grow(d3(d.triangles[i].v1), 0.0);
+    grow(d3(d.triangles[i].v2), 0.0);
+  }
+  for (uint32_t i = 0; i < d.num_spheres; i++) grow(d3(d.spheres[i].center), d.spheres[i].radius);
   if (!build_instances(d, out, err)) return false;
+  for (int a = 0; a < 3; a++) {
+    if (!(out->hi[a] >= out->lo[a])) out->lo[a] = out->hi[a] = 0.f;
+    if (out->hi[a] - out->lo[a] < 1e-6f) {  // flat scenes: give the axis a unit extent
+      out->lo[a] -= 0.5f;
+      out->hi[a] += 0.5f;
+    }
+  }
 
   // Camera: RenderMan frame of Camera::new (camera.rs:96-119), aimed.
   const rrhip_camera_desc& c = d.camera;

@@ -29,6 +29,7 @@ struct PtSceneHost {
   std::vector<uint32_t> mesh_idx;
   std::vector<float> mesh_nrm;
   uint32_t stack_need = 0;  // traversal stack entries the instance path needs
+  float lo[3], hi[3];       // bounds of all geometry (radiance-cache input range)
 };
 
 // Validates and converts the path-tracing part of `d` (loose triangles,

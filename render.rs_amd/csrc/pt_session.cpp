@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "api_internal.h"
+#include "nrc_internal.h"
 #include "pt_kernels.h"
 #include "scene_internal.h"
 
@@ -25,30 +26,80 @@ struct rrhip_session {
   PtParams prm{};
   uint64_t sample_base = 0;
   uint32_t spp_pass = 1;
-  uint64_t slots = 0;
+  uint64_t slots = 0;       // rendering path slots (spp_pass x pixels)
+  uint64_t path_cap = 0;    // slots of the path buffers (rendering + training)
   PathBuffers P{};
-  std::vector<void*> mem;
+  std::vector<void*> mem;       // accumulator and small buffers
+  std::vector<void*> path_mem;  // path buffers and radiance-cache pass state
   double* d_accum = nullptr;
   double* relmse_partial = nullptr;
   uint64_t samples_done = 0;
   double last_ms = 0.0;
+  // Radiance cache (null: plain path tracing).
+  rrhip_nrc* nrc = nullptr;
+  NrcPathBuffers N{};
+  uint32_t* nrc_scan = nullptr;
+  NrcQuery* nrc_bq = nullptr;
+  float* nrc_bt = nullptr;
+  uint32_t nrc_bcap = 0;
 };
 
 namespace {
 
 template <class T>
-hipError_t alloc_n(rrhip_session* s, T** p, uint64_t n) {
+hipError_t alloc_in(std::vector<void*>& mem, T** p, uint64_t n) {
   void* v = nullptr;
   hipError_t e = hipMalloc(&v, std::max<uint64_t>(n, 1) * sizeof(T));
-  if (e == hipSuccess) s->mem.push_back(v);
+  if (e == hipSuccess) mem.push_back(v);
   *p = (T*)v;
   return e;
 }
 
+template <class T>
+hipError_t alloc_n(rrhip_session* s, T** p, uint64_t n) {
+  return alloc_in(s->mem, p, n);
+}
+
+void free_paths(rrhip_session* s) {
+  for (void* p : s->path_mem) (void)hipFree(p);
+  s->path_mem.clear();
+  unsigned long long* stats = s->P.stats;  // lives in s->mem
+  s->P = PathBuffers{};
+  s->P.stats = stats;
+  s->N = NrcPathBuffers{};
+  s->path_cap = 0;
+}
+
 void free_session(rrhip_session* s) {
+  free_paths(s);
   for (void* p : s->mem) (void)hipFree(p);
   s->mem.clear();
   delete s;
+}
+
+// (Re)allocates the path buffers for n slots.
+hipError_t alloc_paths(rrhip_session* s, uint64_t n) {
+  free_paths(s);
+  std::vector<void*>& m = s->path_mem;
+  PathBuffers& P = s->P;
+  hipError_t e = alloc_in(m, &P.o, n);
+  if (e == hipSuccess) e = alloc_in(m, &P.d, n);
+  if (e == hipSuccess) e = alloc_in(m, &P.beta, n);
+  if (e == hipSuccess) e = alloc_in(m, &P.L, n);
+  if (e == hipSuccess) e = alloc_in(m, &P.prev, n);
+  if (e == hipSuccess) e = alloc_in(m, &P.rng, n);
+  if (e == hipSuccess) e = alloc_in(m, &P.flags, n);
+  if (e == hipSuccess) e = alloc_in(m, &P.hit_n, n);
+  if (e == hipSuccess) e = alloc_in(m, &P.hit_m, n);
+  if (e == hipSuccess) e = alloc_in(m, &P.q0, n);
+  if (e == hipSuccess) e = alloc_in(m, &P.q1, n);
+  if (e == hipSuccess) e = alloc_in(m, &P.sq, n);
+  if (e == hipSuccess) e = alloc_in(m, &P.so, n);
+  if (e == hipSuccess) e = alloc_in(m, &P.sd, n);
+  if (e == hipSuccess) e = alloc_in(m, &P.sc, n);
+  if (e == hipSuccess) e = alloc_in(m, &P.counts, 4);
+  if (e == hipSuccess) s->path_cap = n;
+  return e;
 }
 
 int check_session(rrhip_session* s) {
@@ -84,28 +135,13 @@ int rrhip_session_create(rrhip_scene* scene, const rrhip_render_params* params, 
   s->prm.rr_start = params->rr_start;
   s->prm.firefly_clamp = (float)params->firefly_clamp;
   s->prm.npix = (uint32_t)npix;
-  PathBuffers& P = s->P;
-  const uint64_t n = s->slots;
-  hipError_t e = alloc_n(s, &P.o, n);
-  if (e == hipSuccess) e = alloc_n(s, &P.d, n);
-  if (e == hipSuccess) e = alloc_n(s, &P.beta, n);
-  if (e == hipSuccess) e = alloc_n(s, &P.L, n);
-  if (e == hipSuccess) e = alloc_n(s, &P.prev, n);
-  if (e == hipSuccess) e = alloc_n(s, &P.rng, n);
-  if (e == hipSuccess) e = alloc_n(s, &P.flags, n);
-  if (e == hipSuccess) e = alloc_n(s, &P.hit_n, n);
-  if (e == hipSuccess) e = alloc_n(s, &P.hit_m, n);
-  if (e == hipSuccess) e = alloc_n(s, &P.q0, n);
-  if (e == hipSuccess) e = alloc_n(s, &P.q1, n);
-  if (e == hipSuccess) e = alloc_n(s, &P.sq, n);
-  if (e == hipSuccess) e = alloc_n(s, &P.so, n);
-  if (e == hipSuccess) e = alloc_n(s, &P.sd, n);
-  if (e == hipSuccess) e = alloc_n(s, &P.sc, n);
-  if (e == hipSuccess) e = alloc_n(s, &P.counts, 4);
-  if (e == hipSuccess) e = alloc_n(s, &P.stats, 2);
+  hipError_t e = alloc_paths(s, s->slots);
+  unsigned long long* stats = nullptr;
+  if (e == hipSuccess) e = alloc_n(s, &stats, 2);
+  s->P.stats = stats;
   if (e == hipSuccess) e = alloc_n(s, &s->d_accum, npix * RRHIP_ACCUM_STRIDE);
   if (e == hipSuccess) e = hipMemset(s->d_accum, 0, npix * RRHIP_ACCUM_STRIDE * sizeof(double));
-  if (e == hipSuccess) e = hipMemset(P.stats, 0, 2 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMemset(stats, 0, 2 * sizeof(unsigned long long));
   if (e != hipSuccess) {
     free_session(s);
     return fail_hip(e, "rrhip_session_create");
@@ -128,17 +164,35 @@ int rrhip_render_samples(rrhip_session* s, uint32_t start, uint32_t count) {
   RRHIP_HIP_TRY(hipSetDevice(sc->ctx->device));
   hipStream_t st = sc->ctx->stream;
   RRHIP_HIP_TRY(hipEventRecord(sc->ctx->ev0, st));
+  rrhip_nrc* nrc = s->nrc;
+  NrcPathBuffers N = nrc ? s->N : NrcPathBuffers{};
   for (uint32_t done = 0; done < count;) {
     const uint32_t n = std::min(s->spp_pass, count - done);
     PtParams prm = s->prm;
     prm.spp_pass = n;
     prm.sample0 = s->sample_base + (uint64_t)start + done;
     const uint32_t total = n * prm.npix;
-    RRHIP_HIP_TRY(pt_launch_raygen(sc->pt, s->P, prm, total, st));
+    N.train_base = total;  // training paths follow this pass's rendering paths
+    const uint32_t paths = total + N.ntrain;
+    RRHIP_HIP_TRY(pt_launch_raygen(sc->pt, s->P, prm, total, N, st));
     for (uint32_t b = 0; b < prm.max_depth; b++) {
       const uint32_t* q = (b & 1) ? s->P.q1 : s->P.q0;
       uint32_t* qn = (b & 1) ? s->P.q0 : s->P.q1;
-      RRHIP_HIP_TRY(pt_launch_bounce(sc->pt, s->P, prm, q, qn, total, st));
+      RRHIP_HIP_TRY(pt_launch_bounce(sc->pt, s->P, prm, q, qn, paths, N, st));
+    }
+    if (nrc) {
+      // cache radiance at the terminal vertices (and training tails), then
+      // this pass's training samples and optimiser steps
+      RRHIP_HIP_TRY(nrc_launch_infer_paths(nrc->G, N.q, &N.counts[0], paths, N.q_slot, N.q_beta, s->P.L, st));
+      RRHIP_HIP_TRY(pt_launch_nrc_samples(s->P, N, prm.firefly_clamp, s->nrc_scan, s->nrc_bcap, s->nrc_bq,
+                                          s->nrc_bt, st));
+      const uint32_t batch = nrc->cfg.batch;
+      for (uint32_t k = 0; k < nrc->cfg.steps_per_pass && N.ntrain > 0; k++) {
+        if ((uint64_t)k * batch >= s->nrc_bcap) break;
+        RRHIP_HIP_TRY(pt_launch_nrc_set_batch(N, k, batch, &nrc->d_ctl->n, st));
+        RRHIP_HIP_TRY(nrc_step_device(nrc, s->nrc_bq + (size_t)k * batch, s->nrc_bt + 3 * (size_t)k * batch, batch,
+                                      st));
+      }
     }
     RRHIP_HIP_TRY(pt_launch_resolve(s->P, prm, s->d_accum, st));
     done += n;
@@ -217,3 +271,50 @@ int rrhip_session_get_stats(rrhip_session* s, rrhip_session_stats* out) {
 }
 
 }  // extern "C"
+
+extern "C" int rrhip_session_set_nrc(rrhip_session* s, rrhip_nrc* nrc) {
+  if (int st = check_session(s)) return st;
+  if (!nrc) {
+    s->nrc = nullptr;
+    return RRHIP_OK;
+  }
+  rrhip_scene* sc = s->scene;
+  if (nrc->ctx != sc->ctx) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_set_nrc: cache and session differ in context");
+  const uint64_t ntrain = nrc->cfg.train_paths;
+  const uint64_t cap = s->slots + ntrain;
+  if (cap >= (1ull << 31)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_set_nrc: path slots exceed 2^31");
+  RRHIP_HIP_TRY(hipSetDevice(sc->ctx->device));
+  RRHIP_HIP_TRY(hipStreamSynchronize(sc->ctx->stream));
+  if (nrc->bounds_from_scene) nrc_set_bounds(nrc, sc->pt_lo, sc->pt_hi);
+  s->nrc = nullptr;
+  hipError_t e = alloc_paths(s, cap);
+  std::vector<void*>& m = s->path_mem;
+  NrcPathBuffers& N = s->N;
+  const uint32_t D = s->prm.max_depth;
+  if (e == hipSuccess) e = alloc_in(m, &N.q, cap);
+  if (e == hipSuccess) e = alloc_in(m, &N.q_slot, cap);
+  if (e == hipSuccess) e = alloc_in(m, &N.q_beta, cap);
+  if (e == hipSuccess) e = alloc_in(m, &N.counts, 2);
+  if (e == hipSuccess) e = alloc_in(m, &N.tq, ntrain * D);
+  if (e == hipSuccess) e = alloc_in(m, &N.tb, ntrain * D);
+  if (e == hipSuccess) e = alloc_in(m, &N.tl, ntrain * D);
+  if (e == hipSuccess) e = alloc_in(m, &N.tn, ntrain);
+  if (e == hipSuccess) e = alloc_in(m, &s->nrc_scan, ntrain);
+  const uint64_t bcap = std::min<uint64_t>(ntrain * D, (uint64_t)nrc->cfg.steps_per_pass * nrc->cfg.batch);
+  s->nrc_bcap = (uint32_t)bcap;
+  if (e == hipSuccess) e = alloc_in(m, &s->nrc_bq, bcap);
+  if (e == hipSuccess) e = alloc_in(m, &s->nrc_bt, 3 * bcap);
+  if (e != hipSuccess) {
+    // leave a usable plain session behind
+    (void)hipGetLastError();
+    free_paths(s);
+    if (alloc_paths(s, s->slots) != hipSuccess) return fail_hip(e, "rrhip_session_set_nrc (session unusable)");
+    return fail_hip(e, "rrhip_session_set_nrc");
+  }
+  N.train_base = (uint32_t)s->slots;
+  N.ntrain = (uint32_t)ntrain;
+  N.max_vertices = D;
+  N.spread = nrc->cfg.spread;
+  s->nrc = nrc;
+  return RRHIP_OK;
+}

@@ -31,6 +31,8 @@ struct alignas(16) PtMaterial {
   float glass_ior;
   float emission[3]; // Material::emission + glow
   int32_t area_light;
+  float nrc_albedo[3];  // radiance-cache input: summed lobe albedo
+  float nrc_rough;      // radiance-cache input: roughness of the strongest glossy lobe
   float pad[2];
 };
 
@@ -126,12 +128,12 @@ struct PtSceneView {
 // Wavefront path state, structure of arrays over path slots.
 struct PathBuffers {
   float4* o;        // origin.xyz, prev_pdf
-  float4* d;        // direction.xyz, unused
+  float4* d;        // direction.xyz, primary footprint a0 (radiance cache)
   float4* beta;     // throughput.xyz, unused
   float4* L;        // radiance.xyz, unused
-  float4* prev;     // previous vertex xyz, unused
+  float4* prev;     // previous vertex xyz, area-spread sum (radiance cache)
   uint4* rng;       // PCG32 state lo/hi, inc lo/hi
-  uint32_t* flags;  // depth (bits 0-7), from_camera (bit 8)
+  uint32_t* flags;  // depth (bits 0-7), from_camera (bit 8), training path (9), suffix (10)
   float4* hit_n;    // hit normal.xyz, t (t < 0: miss)
   uint32_t* hit_m;  // material | front_face << 31
   uint32_t* q0;
@@ -140,8 +142,38 @@ struct PathBuffers {
   float4* so;       // shadow origin.xyz, tmax
   float4* sd;       // shadow direction.xyz
   float4* sc;       // shadow contribution rgb (already times beta)
-  uint32_t* counts; // [0] live paths, [1] next live, [2] shadow rays
+  uint32_t* counts; // [0] live paths, [1] next live, [2] shadow rays, [3] unused
   unsigned long long* stats;  // [0] extension rays, [1] shadow rays
+};
+
+// One radiance-cache query / training sample (the 64-byte rrhip_nrc_query
+// record): position, roughness, direction towards the viewer, shading
+// normal on that side, albedo.
+struct alignas(16) NrcQuery {
+  float pos[3], roughness;
+  float dir[3], pad0;
+  float normal[3], pad1;
+  float albedo[3], pad2;
+};
+static_assert(sizeof(NrcQuery) == 64, "query record is 64 bytes");
+
+// Radiance-cache state of a rendering pass. Render paths stop at the
+// vertex where their area spread exceeds `spread` times the primary
+// footprint and queue a cache query there; training paths (slots
+// [train_base, train_base + ntrain)) record every vertex, continue past
+// that point for a suffix of the same spread, and queue the cache at the
+// suffix's end as its tail.
+struct NrcPathBuffers {
+  NrcQuery* q;          // queries
+  uint32_t* q_slot;     // path slot of each query
+  float4* q_beta;       // throughput at the query vertex
+  uint32_t* counts;     // [0] queries, [1] training samples built
+  NrcQuery* tq;         // training records [path][vertex]
+  float4* tb;           // throughput arriving at the vertex
+  float4* tl;           // path radiance after the vertex's emission
+  uint32_t* tn;         // recorded vertices per training path
+  uint32_t train_base, ntrain, max_vertices;
+  float spread;
 };
 
 struct PtParams {

@@ -34,4 +34,5 @@ struct rrhip_scene {
   bool has_pt = false;
   rrhip::PtSceneView pt{};
   void* pt_mem[14] = {};
+  float pt_lo[3] = {0, 0, 0}, pt_hi[3] = {0, 0, 0};  // bounds of all geometry
 };

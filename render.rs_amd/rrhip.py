@@ -37,6 +37,9 @@ EXPORTED = [
     "rrhip_device_copy",
     "rrhip_session_create", "rrhip_session_destroy", "rrhip_render_samples", "rrhip_read_accum",
     "rrhip_load_accum", "rrhip_session_get_stats", "rrhip_gen_icosphere", "rrhip_relmse",
+    "rrhip_nrc_create", "rrhip_nrc_destroy", "rrhip_nrc_get_info", "rrhip_nrc_get_params",
+    "rrhip_nrc_set_params", "rrhip_nrc_infer", "rrhip_nrc_train_step", "rrhip_nrc_gradients",
+    "rrhip_session_set_nrc",
 ]
 
 
@@ -133,6 +136,33 @@ class SessionStats(ctypes.Structure):
                 ("last_render_ms", ctypes.c_double)]
 
 
+class NrcConfig(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("levels", ctypes.c_uint32),
+                ("log2_table", ctypes.c_uint32), ("seed", ctypes.c_uint32),
+                ("base_res", ctypes.c_float), ("max_res", ctypes.c_float),
+                ("lo", ctypes.c_float * 3), ("hi", ctypes.c_float * 3),
+                ("learning_rate", ctypes.c_float), ("beta1", ctypes.c_float),
+                ("beta2", ctypes.c_float), ("epsilon", ctypes.c_float),
+                ("batch", ctypes.c_uint32), ("steps_per_pass", ctypes.c_uint32),
+                ("train_paths", ctypes.c_uint32), ("spread", ctypes.c_float)]
+
+    @staticmethod
+    def make(levels=16, log2_table=19, seed=1, base_res=16.0, max_res=2048.0, lo=(0, 0, 0), hi=(0, 0, 0),
+             learning_rate=1e-2, beta1=0.9, beta2=0.99, epsilon=1e-15, batch=16384, steps_per_pass=4,
+             train_paths=16384, spread=0.01):
+        return NrcConfig(ctypes.sizeof(NrcConfig), levels, log2_table, seed, base_res, max_res,
+                         (ctypes.c_float * 3)(*lo), (ctypes.c_float * 3)(*hi), learning_rate, beta1, beta2,
+                         epsilon, batch, steps_per_pass, train_paths, spread)
+
+
+class NrcInfo(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_uint32), ("log2_table", ctypes.c_uint32),
+                ("res", ctypes.c_uint32 * 16), ("mlp_params", ctypes.c_uint64),
+                ("grid_params", ctypes.c_uint64), ("steps", ctypes.c_uint64),
+                ("last_loss", ctypes.c_float), ("lo", ctypes.c_float * 3), ("hi", ctypes.c_float * 3),
+                ("reserved", ctypes.c_uint32)]
+
+
 LIGHT_POINT, LIGHT_DISTANT, LIGHT_RECT, LIGHT_SPHERE, LIGHT_DISK, LIGHT_DOME = range(6)
 ACCUM_STRIDE = 8
 
@@ -197,6 +227,15 @@ def load():
         "rrhip_load_accum": (i32, [vp, vp]),
         "rrhip_session_get_stats": (i32, [vp, ctypes.POINTER(SessionStats)]),
         "rrhip_relmse": (i32, [vp, vp, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]),
+        "rrhip_nrc_create": (i32, [vp, ctypes.POINTER(NrcConfig), ctypes.POINTER(vp)]),
+        "rrhip_nrc_destroy": (i32, [vp]),
+        "rrhip_nrc_get_info": (i32, [vp, ctypes.POINTER(NrcInfo)]),
+        "rrhip_nrc_get_params": (i32, [vp, vp, vp]),
+        "rrhip_nrc_set_params": (i32, [vp, vp, vp]),
+        "rrhip_nrc_infer": (i32, [vp, vp, u64, vp]),
+        "rrhip_nrc_train_step": (i32, [vp, vp, vp, u64, ctypes.POINTER(ctypes.c_float)]),
+        "rrhip_nrc_gradients": (i32, [vp, vp, vp, u64, ctypes.POINTER(ctypes.c_float), vp, vp]),
+        "rrhip_session_set_nrc": (i32, [vp, vp]),
         "rrhip_scene_get_tuning": (i32, [vp, ctypes.POINTER(TraceTuning)]),
         "rrhip_scene_set_tuning": (i32, [vp, ctypes.POINTER(TraceTuning)]),
         "rrhip_scene_autotune": (i32, [vp, vp, u64, i32, vp, ctypes.POINTER(TraceTuning),
@@ -391,6 +430,11 @@ class Session:
             acc = np.ascontiguousarray(acc, np.float64)
             _check(load().rrhip_load_accum(self.h, _ptr(acc)))
 
+    def set_nrc(self, nrc):
+        """Attaches a radiance cache (Nrc) to the session, or detaches it (None)."""
+        _check(load().rrhip_session_set_nrc(self.h, None if nrc is None else nrc.h))
+        self._nrc = nrc  # the cache must outlive its use by the session
+
     def relmse(self, reference, eps=0.01):
         out = ctypes.c_double()
         _check(load().rrhip_relmse(self.h, reference.h, eps, ctypes.byref(out)))
@@ -465,3 +509,91 @@ class Scene:
         ms = ctypes.c_float()
         _check(load().rrhip_time_trace_device(self.h, d_rays, n, int(any_hit), d_out, reps, ctypes.byref(ms)))
         return ms.value
+
+
+# ---- neural radiance cache -------------------------------------------------
+
+NRC_MLP_PARAMS = 19648
+
+
+def nrc_queries(pos, dir, normal, roughness, albedo):
+    """Packs per-query arrays into the (n, 16) f32 rrhip_nrc_query layout."""
+    pos = np.asarray(pos, np.float32).reshape(-1, 3)
+    q = np.zeros((pos.shape[0], 16), np.float32)
+    q[:, 0:3] = pos
+    q[:, 3] = np.asarray(roughness, np.float32).reshape(-1)
+    q[:, 4:7] = np.asarray(dir, np.float32).reshape(-1, 3)
+    q[:, 8:11] = np.asarray(normal, np.float32).reshape(-1, 3)
+    q[:, 12:15] = np.asarray(albedo, np.float32).reshape(-1, 3)
+    return q
+
+
+class Nrc:
+    """Neural radiance cache (rrhip_nrc_*): hash grid + MLP on the device."""
+
+    def __init__(self, ctx, config=None):
+        h = ctypes.c_void_p()
+        cfg = config if config is not None else NrcConfig.make()
+        _check(load().rrhip_nrc_create(ctx.h, ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self.ctx = ctx
+        self.config = cfg
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().rrhip_nrc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def info(self):
+        i = NrcInfo()
+        _check(load().rrhip_nrc_get_info(self.h, ctypes.byref(i)))
+        d = {f: getattr(i, f) for f, _ in NrcInfo._fields_ if f != "reserved"}
+        d["res"] = list(i.res)[: i.levels]
+        d["lo"], d["hi"] = list(i.lo), list(i.hi)
+        return d
+
+    def get_params(self):
+        info = self.info()
+        mlp = np.empty(NRC_MLP_PARAMS, np.float32)
+        grid = np.empty(info["grid_params"], np.float32)
+        _check(load().rrhip_nrc_get_params(self.h, _ptr(mlp), _ptr(grid)))
+        return mlp, grid
+
+    def set_params(self, mlp=None, grid=None):
+        m = None if mlp is None else np.ascontiguousarray(mlp, np.float32)
+        g = None if grid is None else np.ascontiguousarray(grid, np.float32)
+        if m is not None and m.size != NRC_MLP_PARAMS:
+            raise ValueError("mlp must hold 19648 floats")
+        if g is not None and g.size != self.info()["grid_params"]:
+            raise ValueError("grid size mismatch")
+        _check(load().rrhip_nrc_set_params(self.h, None if m is None else _ptr(m), None if g is None else _ptr(g)))
+
+    def infer(self, queries):
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 16)
+        out = np.empty((q.shape[0], 3), np.float32)
+        _check(load().rrhip_nrc_infer(self.h, _ptr(q), q.shape[0], _ptr(out)))
+        return out
+
+    def train_step(self, queries, targets):
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 16)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        if t.shape[0] != q.shape[0]:
+            raise ValueError("one target per query")
+        loss = ctypes.c_float()
+        _check(load().rrhip_nrc_train_step(self.h, _ptr(q), _ptr(t), q.shape[0], ctypes.byref(loss)))
+        return loss.value
+
+    def gradients(self, queries, targets):
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 16)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        if t.shape[0] != q.shape[0]:
+            raise ValueError("one target per query")
+        loss = ctypes.c_float()
+        gm = np.empty(NRC_MLP_PARAMS, np.float32)
+        gg = np.empty(self.info()["grid_params"], np.float32)
+        _check(load().rrhip_nrc_gradients(self.h, _ptr(q), _ptr(t), q.shape[0], ctypes.byref(loss), _ptr(gm),
+                                          _ptr(gg)))
+        return loss.value, gm, gg

@@ -1,0 +1,247 @@
+"""GPU radiance cache (hash grid + bf16 MLP on the matrix cores) against
+the f64 oracle.
+
+Bars: inference and gradients equal the oracle's bf16-mimic evaluation
+(bf16 rounding at the same points as the kernels, f64 elsewhere) up to
+f32 accumulation order; Adam steps equal the textbook update applied to
+the device's own gradients; training is bit-reproducible and fits a smooth
+target."""
+import sys
+
+import numpy as np
+import pytest
+
+import orc
+import rrhip
+
+pytestmark = pytest.mark.gpu
+
+LO, HI = (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def small_config(**kw):
+    args = dict(levels=8, log2_table=14, base_res=4.0, max_res=160.0, lo=LO, hi=HI, batch=4096)
+    args.update(kw)
+    return rrhip.NrcConfig.make(**args)
+
+
+def oracle_config(cfg):
+    return orc.NrcConfig(cfg.levels, cfg.log2_table, cfg.base_res, cfg.max_res, tuple(cfg.lo), tuple(cfg.hi))
+
+
+def he_weights(rng):
+    shapes = [(64, 48), (64, 64), (64, 64), (64, 64), (64, 64), (3, 64)]
+    fan = [43, 64, 64, 64, 64, 64]
+    ws = []
+    for (r, c), f in zip(shapes, fan):
+        w = rng.uniform(-1, 1, (r, c)) * np.sqrt(6.0 / f)
+        if c == 48:
+            w[:, 43:] = 0.0
+        ws.append(w.reshape(-1))
+    return np.concatenate(ws).astype(np.float32)
+
+
+def random_queries(rng, n, spread=1.2):
+    pos = rng.uniform(-spread, spread, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    nn = rng.normal(size=(n, 3))
+    nn /= np.linalg.norm(nn, axis=1, keepdims=True)
+    return rrhip.nrc_queries(pos, d, nn, rng.uniform(0, 1, n), rng.uniform(0, 1, (n, 3)))
+
+
+def make_cache(ctx, rng, **kw):
+    cfg = small_config(**kw)
+    c = rrhip.Nrc(ctx, cfg)
+    info = c.info()
+    oc = oracle_config(cfg)
+    assert list(orc.nrc_level_res(oc)) == info["res"]
+    w = he_weights(rng)
+    grid = rng.uniform(-0.5, 0.5, info["grid_params"]).astype(np.float32)
+    c.set_params(w, grid)
+    return c, oc, w, grid
+
+
+def test_inference_matches_oracle(ctx):
+    say("cache inference: 8 levels (dense + hashed), 5000 queries vs oracle bf16 mimic")
+    rng = np.random.default_rng(1)
+    c, oc, w, grid = make_cache(ctx, rng)
+    q = random_queries(rng, 5000)
+    y = c.infer(q).astype(np.float64)
+    ym = orc.nrc_infer(oc, grid, w.astype(np.float64), q, mimic=True)
+    ye = orc.nrc_infer(oc, grid, w.astype(np.float64), q, mimic=False)
+    scale = np.sqrt(np.mean(ym ** 2))
+    err = np.abs(y - ym)
+    say(f"rms output {scale:.4f}; |gpu - mimic| max {err.max():.3e} p99 {np.quantile(err, 0.99):.3e}; "
+        f"|gpu - exact| max {np.abs(y - ye).max():.3e}")
+    # Same bf16 rounding points; f32 vs f64 accumulation can flip a bf16
+    # rounding (2^-8 relative) of a few activations, which moves an output
+    # by a fraction of a percent of the output scale.
+    assert err.max() <= 2e-2 * scale
+    assert np.quantile(err, 0.99) <= 5e-3 * scale
+    assert np.abs(y - ye).max() <= 0.05 * scale
+    # parameters read back are those set
+    w2, g2 = c.get_params()
+    assert np.array_equal(w2, w) and np.array_equal(g2, grid)
+    c.close()
+
+
+@pytest.mark.parametrize("n", [1, 65, 1000])
+def test_gradients_match_oracle(ctx, n):
+    say(f"cache gradients: batch of {n} vs oracle bf16 mimic")
+    rng = np.random.default_rng(2 + n)
+    c, oc, w, grid = make_cache(ctx, rng)
+    q = random_queries(rng, n)
+    t = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    loss, gm, gg = c.gradients(q, t)
+    lref, rm, rg = orc.nrc_gradients(oc, grid, w.astype(np.float64), q, t, mimic=True)
+    say(f"loss gpu {loss:.6f} oracle {lref:.6f}")
+    assert abs(loss - lref) <= 2e-3 * lref
+    offs = [0, 3072, 7168, 11264, 15360, 19456, 19648]
+    for l in range(6):
+        a, b = gm[offs[l]:offs[l + 1]].astype(np.float64), rm[offs[l]:offs[l + 1]]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        say(f"  W{l}: |g| {np.linalg.norm(b):.3e} rel err {rel:.2e}")
+        assert rel <= 2e-2, (l, rel)
+    # W0 columns 43..47 see zero inputs
+    assert not gm[:3072].reshape(64, 48)[:, 43:].any()
+    rel = np.linalg.norm(gg - rg) / np.linalg.norm(rg)
+    say(f"  grid: touched {np.count_nonzero(rg)} entries, rel err {rel:.2e}")
+    assert rel <= 2e-2
+    assert np.array_equal(gg != 0, rg != 0) or np.count_nonzero((gg != 0) != (rg != 0)) <= 0.01 * np.count_nonzero(rg)
+    c.close()
+
+
+def test_adam_steps_follow_the_textbook_update(ctx):
+    say("Adam: three steps vs Kingma-Ba on the device's own gradients")
+    rng = np.random.default_rng(7)
+    c, oc, w, grid = make_cache(ctx, rng)
+    p = np.concatenate([w, grid]).astype(np.float64)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    for step in range(1, 4):
+        q = random_queries(rng, 2000)
+        t = rng.uniform(0, 1, (2000, 3)).astype(np.float32)
+        _, gm, gg = c.gradients(q, t)
+        g = np.concatenate([gm, gg]).astype(np.float64)
+        orc.nrc_adam(p, g, m, v, step, lr=1e-2, b1=0.9, b2=0.99, eps=1e-15)
+        c.train_step(q, t)
+        wd, gd = c.get_params()
+        got = np.concatenate([wd, gd]).astype(np.float64)
+        err = np.abs(got - p).max()
+        say(f"step {step}: max |device - textbook| {err:.3e}")
+        assert err <= 2e-6
+    assert c.info()["steps"] == 3
+    c.close()
+
+
+def test_training_is_deterministic(ctx):
+    say("training determinism: two caches, same seed and batches")
+    rng = np.random.default_rng(9)
+    batches = [(random_queries(rng, 3000), rng.uniform(0, 1, (3000, 3)).astype(np.float32)) for _ in range(3)]
+    out = []
+    for _ in range(2):
+        c = rrhip.Nrc(ctx, small_config(seed=5))
+        for q, t in batches:
+            c.train_step(q, t)
+        out.append(c.get_params())
+        c.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_cache_fits_a_smooth_radiance_field(ctx):
+    say("online fit: 300 steps of 4096 samples of a smooth field")
+    rng = np.random.default_rng(3)
+    c = rrhip.Nrc(ctx, small_config(seed=2))
+
+    def field(q):
+        x, y, z = q[:, 0], q[:, 1], q[:, 2]
+        base = 0.3 + 0.2 * np.sin(3 * x) * np.cos(2 * y) + 0.1 * z
+        return np.stack([base, base * 0.8 + 0.05 * q[:, 6], base * 0.6 + 0.1 * q[:, 12]], 1).astype(np.float32)
+
+    hold = random_queries(rng, 4096, spread=1.0)
+    first = None
+    for i in range(300):
+        q = random_queries(rng, 4096, spread=1.0)
+        loss = c.train_step(q, field(q))
+        first = loss if first is None else first
+    y = c.infer(hold)
+    err = np.abs(y - field(hold))
+    say(f"loss first {first:.4f} last {loss:.5f}; held-out mean |err| {err.mean():.4f} max {err.max():.4f}")
+    assert loss < 0.05 * first
+    assert err.mean() < 0.03
+    c.close()
+
+
+def test_full_size_batch_runs(ctx):
+    say("default cache (16 levels, 2^19 entries): one step on 65536 samples")
+    rng = np.random.default_rng(4)
+    c = rrhip.Nrc(ctx, rrhip.NrcConfig.make(lo=LO, hi=HI))
+    info = c.info()
+    assert info["res"][0] == 16 and info["res"][-1] == 2048 and info["grid_params"] == 16 * 2 ** 19 * 2
+    q = random_queries(rng, 65536)
+    loss = c.train_step(q, rng.uniform(0, 1, (65536, 3)).astype(np.float32))
+    assert np.isfinite(loss)
+    y = c.infer(q[:1000])
+    assert np.isfinite(y).all()
+    with pytest.raises(rrhip.RrhipError):
+        c.train_step(random_queries(rng, 65537), np.zeros((65537, 3), np.float32))
+    c.close()
+
+
+def nrc_field_scene():
+    import scenes
+    return scenes.instanced_field(96, 54, n_instances=300, levels=3, n_point_lights=16)
+
+
+def test_cached_render_gains_over_truncation_and_is_reproducible(ctx):
+    """The cache's gain: paths stop by the area-spread criterion in both
+    runs; with the trained cache their tails come from the network, with a
+    zeroed output layer (and no training) they are dropped. Against a
+    1024-spp plain reference the trained cache must at least halve the
+    relMSE of truncation at the same 256 spp; renders with a cache are
+    bit-reproducible."""
+    say("cache gain: instanced field 96x54, 64 frames x 4 spp, trained cache vs truncation vs plain")
+    s = nrc_field_scene()
+    ps = rrhip.PtScene(ctx, s.desc())
+    ref = ps.session(rrhip.RenderParams.make(max_depth=12, samples_per_pass=32, sample_base=1 << 32))
+    for k in range(0, 1024, 256):
+        ref.render(k, 256)
+    cfg = dict(log2_table=16, train_paths=4096, batch=4096, steps_per_pass=4)
+
+    def run(trained):
+        c = rrhip.Nrc(ctx, rrhip.NrcConfig.make(**cfg) if trained else rrhip.NrcConfig.make(train_paths=0, **{
+            k: v for k, v in cfg.items() if k != "train_paths"}))
+        if not trained:
+            w, _ = c.get_params()
+            w[19456:] = 0.0  # output layer: the cache returns 0
+            c.set_params(w, None)
+        se = ps.session(rrhip.RenderParams.make(max_depth=12, samples_per_pass=4))
+        se.set_nrc(c)
+        for f in range(64):
+            se.render(4 * f, 4)
+        out = (se.relmse(ref), se.accum(), c.info())
+        se.close()
+        c.close()
+        return out
+
+    r_nrc, acc1, info = run(True)
+    r_trunc, _, _ = run(False)
+    plain = ps.session(rrhip.RenderParams.make(max_depth=12, samples_per_pass=4))
+    plain.render(0, 256)
+    r_plain = plain.relmse(ref)
+    say(f"relMSE at 256 spp: trained cache {r_nrc:.4f}, truncation {r_trunc:.4f}, plain {r_plain:.4f}; "
+        f"cache steps {info['steps']} last loss {info['last_loss']:.4f}")
+    assert info["steps"] == 64 * 4
+    assert np.isfinite(acc1).all() and (acc1[:, :6] >= 0).all() and (acc1[:, 6] == 256).all()
+    assert r_nrc < 0.5 * r_trunc
+    _, acc2, _ = run(True)
+    assert acc2.tobytes() == acc1.tobytes()
+    plain.close()
+    ref.close()
+    ps.close()
