@@ -399,7 +399,7 @@ typedef struct rrhip_nrc_info {
 /* Creates a cache with freshly initialised parameters (grid U(-1e-4,
  * 1e-4), He-uniform weights, both from PCG32(seed)). config = NULL takes
  * the defaults: 16 levels, 2^19 entries, resolutions 16..2048, Adam
- * (1e-2, 0.9, 0.99, 1e-15), batch 16384, 4 steps and 16384 training
+ * (1e-2, 0.9, 0.99, 1e-15), batch 16384, 4 steps and 65536 training
  * paths per pass, c = 0.01. */
 int rrhip_nrc_create(rrhip_ctx* ctx, const rrhip_nrc_config* config, rrhip_nrc** out);
 int rrhip_nrc_destroy(rrhip_nrc* nrc);

@@ -44,7 +44,7 @@ rrhip_nrc_config default_config() {
   c.epsilon = 1e-15f;
   c.batch = 16384;
   c.steps_per_pass = 4;
-  c.train_paths = 16384;
+  c.train_paths = 65536;
   c.spread = 0.01f;
   return c;
 }
