@@ -9,7 +9,9 @@ origins uniform in the mesh box inflated by 10% and directions uniform on
 the sphere, tmax = inf. Rays, BVH and hits stay resident in HBM; the
 timed region is K back-to-back closest-hit launches.
 
-Prints one JSON line (value = whole-job Mrays/s at N=1).
+Prints one JSON line (value = whole-job Mrays/s at N=1). The line also
+carries the box-room path-tracing throughput and the time to relMSE 0.01
+at 1080p of the radiance cache vs plain wavefront path tracing.
 """
 import argparse
 import json
@@ -88,6 +90,98 @@ def box_room(ctx, args, with_cpu):
     return out
 
 
+def time_to_relmse(ctx, args, with_cpu):
+    """Time to relMSE <= 0.01 at 1080p, radiance cache vs plain wavefront
+    path tracing, on the instanced workload (10,000 instances of a 20,480-
+    triangle displaced icosphere, 64 point lights + 1 rect light, depth 12).
+    Both render 4 spp per frame from frame 0 into a running mean; after
+    each frame the relMSE of the mean against a 4096-spp plain render
+    (independent sample streams) is taken on the device (not timed). The
+    clock sums the wall time of the render calls (the cache's inference
+    and training included)."""
+    import rrhip
+    import scenes
+    W, H = 1920, 1080
+    s = scenes.instanced_field(W, H)
+    d = s.desc()
+    ps = rrhip.PtScene(ctx, d)
+    params = rrhip.RenderParams.make(max_depth=12, samples_per_pass=4)
+    t0 = time.perf_counter()
+    ref = ps.session(rrhip.RenderParams.make(max_depth=12, samples_per_pass=4, sample_base=1 << 32))
+    for k in range(0, args.ref_spp, 64):
+        ref.render(k, min(64, args.ref_spp - k))
+    ref_s = time.perf_counter() - t0
+
+    def run(nrc_config):
+        se = ps.session(params)
+        cache = None
+        if nrc_config is not None:
+            cache = rrhip.Nrc(ctx, nrc_config)
+            se.set_nrc(cache)
+        spent, frames, r = 0.0, 0, float("inf")
+        curve = []
+        while frames < args.max_frames:
+            t1 = time.perf_counter()
+            se.render(4 * frames, 4)
+            spent += time.perf_counter() - t1
+            frames += 1
+            r = se.relmse(ref, 0.01)
+            if frames in (1, 4, 16, 64, 128, 256):
+                curve.append([frames, round(spent, 3), round(r, 5)])
+            if r <= 0.01:
+                break
+        st = se.stats()
+        out = {"seconds": round(spent, 3) if r <= 0.01 else None, "frames": frames, "spp": 4 * frames,
+               "relmse": round(r, 5), "ms_per_frame": round(spent / frames * 1e3, 2),
+               "mrays_per_frame": round((st["extension_rays"] + st["shadow_rays"]) / frames / 1e6, 2),
+               "curve [frames, s, relMSE]": curve}
+        if cache is not None:
+            info = cache.info()
+            out["optimizer_steps"] = info["steps"]
+            cache.close()
+        se.close()
+        return out
+
+    plain = run(None)
+    nrc = run(rrhip.NrcConfig.make())
+    ref.close()
+    out = {
+        "workload": "instanced field 1920x1080 (10,000 x 20,480-triangle instances, 64 point + 1 rect light), "
+                    "4 spp/frame, depth 12; relMSE eps 0.01 vs a 4096-spp plain render",
+        "reference_spp": args.ref_spp, "reference_render_s": round(ref_s, 2),
+        "plain": plain, "nrc": nrc,
+        "nrc_config": "16 levels x 2 features x 2^19, 48-64x5-3 MLP bf16 on MFMA, Adam 1e-2, 4 x 16384 samples "
+                      "and 65536 training paths per frame, area-spread c = 0.01",
+        "speedup": (round(plain["seconds"] / nrc["seconds"], 3)
+                    if plain["seconds"] and nrc["seconds"] else None),
+    }
+    if with_cpu and plain["seconds"]:
+        # The oracle's plain path tracer on the same scene at 480x270 and
+        # 4 spp, scaled by pixels to one 1080p frame; setup (scene load and
+        # BVH builds) measured on an 8x8 render and subtracted.
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import orc
+        threads = max(1, min(16, os.cpu_count() or 1))
+        small_scene = scenes.instanced_field(480, 270)  # keeps the arrays the desc points at alive
+        tiny_scene = scenes.instanced_field(8, 8)
+        t1 = time.perf_counter()
+        orc.render(tiny_scene.desc(), params, 0, 1, threads=threads)
+        setup = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        _, crays = orc.render(small_scene.desc(), params, 0, 4, threads=threads)
+        cdt = max(time.perf_counter() - t1 - setup, 1e-6)
+        frame_s = cdt * (W * H) / (480 * 270)
+        out["cpu_baseline"] = {
+            "value": round(frame_s * plain["frames"], 1), "unit": "s (plain PT to relMSE 0.01, extrapolated)",
+            "cores": threads, "kind": "port",
+            "sample": f"oracle plain PT, same scene at 480x270 x 4 spp ({cdt:.2f} s after {setup:.2f} s setup, "
+                      f"{crays / cdt / 1e6:.2f} Mrays/s), scaled by pixel count to {frame_s:.2f} s per 1080p "
+                      f"frame x the {plain['frames']} frames the GPU plain run needed",
+        }
+    ps.close()
+    return out
+
+
 def copy_calibration(ctx, args):
     """K launches of the plain copy kernel over 512 MiB (reads and writes
     exactly that much per launch)."""
@@ -121,6 +215,9 @@ def main():
     ap.add_argument("--no-cost", action="store_true",
                     help="skip the counting pass (profiling runs)")
     ap.add_argument("--no-pt", action="store_true", help="skip the box-room path-tracing line")
+    ap.add_argument("--no-relmse", action="store_true", help="skip the time-to-relMSE comparison")
+    ap.add_argument("--ref-spp", type=int, default=4096, help="spp of the relMSE reference render")
+    ap.add_argument("--max-frames", type=int, default=400, help="frame cap of the time-to-relMSE runs")
     ap.add_argument("--mode", choices=("trace", "copy"), default="trace",
                     help="copy: time the plain copy kernel that calibrates HBM counters")
     args = ap.parse_args()
@@ -212,6 +309,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(pos, idx, rays)
     if not args.no_pt:
         out["path_tracing"] = box_room(ctx, args, not args.no_cpu_baseline)
+    if not args.no_relmse:
+        out["time_to_relmse"] = time_to_relmse(ctx, args, not args.no_cpu_baseline)
     for p in (d_rays, d_hits, d_occ):
         ctx.free(p)
     scene.close()
