@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define RRHIP_ABI_VERSION 2u
+#define RRHIP_ABI_VERSION 3u
 #define RRHIP_NO_HIT 0xFFFFFFFFu
 
 /* Status codes. render.rs returns anyhow::Result with a message
@@ -179,6 +179,25 @@ typedef struct rrhip_camera_desc {
  * Ray queries use exactly one mesh and no other geometry. Path-tracing
  * scenes place meshes through instances and add loose triangles and
  * spheres (render.rs Scene::objects, src/scene/mod.rs:27-56). */
+/* A participating medium (render.rs Medium, src/scene/medium.rs:28-65):
+ * absorption and scattering coefficients, Henyey-Greenstein anisotropy,
+ * emission, extent along any ray, and an optional density field. render.rs
+ * evaluates a procedural fBm per lookup; here the density is a voxel grid
+ * (nx x ny x nz samples, x fastest, values in [0, 1], trilinear between
+ * samples placed on the corners of [grid_lo, grid_hi]; zero outside).
+ * nx = 0: homogeneous. */
+typedef struct rrhip_medium_desc {
+  double sigma_a[3];
+  double sigma_s[3];
+  double g;
+  double emission[3];
+  double max_distance;  /* <= 0: unbounded */
+  uint32_t nx, ny, nz, reserved;
+  const float* density;
+  double grid_lo[3];
+  double grid_hi[3];
+} rrhip_medium_desc;
+
 typedef struct rrhip_scene_desc {
   uint32_t struct_size;
   uint32_t num_meshes;
@@ -197,6 +216,14 @@ typedef struct rrhip_scene_desc {
   const rrhip_light_desc* lights;
   rrhip_camera_desc camera;
   double background[3]; /* miss radiance when no dome light */
+  /* Participating media; `atmosphere` (-1: none) is the medium every ray
+   * travels in (render.rs Scene::atmosphere, src/scene/mod.rs). Point
+   * lights inside an atmosphere (render.rs's equiangular strategy,
+   * pt/mod.rs:507-560) are not supported: upload fails with
+   * RRHIP_ERR_UNSUPPORTED. */
+  uint32_t num_media;
+  int32_t atmosphere;
+  const rrhip_medium_desc* media;
 } rrhip_scene_desc;
 
 /* Path-tracing parameters (render.rs constants, src/raytracer/pt/mod.rs:
@@ -450,6 +477,14 @@ int rrhip_gen_heightfield(uint32_t nx, uint32_t ny, uint64_t seed, float amplitu
  * (src/raytracer/pt/sampler.rs:24-32), in the order ox, oy, oz, u, v. */
 int rrhip_gen_query_rays(const float lo[3], const float hi[3], float inflate, uint64_t seed,
                          uint64_t n, rrhip_ray* rays);
+
+/* Density grid of render.rs's fBm cloud field (Medium::density_at,
+ * src/scene/medium.rs:55-63): rho = clamp((fbm(p) * 0.5 + 0.5 - (1 -
+ * coverage)) * sharpness, 0, 1) with fbm(frequency, octaves, gain 0.5,
+ * lacunarity 2, offset from PCG32(seed, 0x5eed) as for the heightfield)
+ * sampled on n x n x n points spanning [lo, hi] (x fastest). */
+int rrhip_gen_density_grid(uint32_t n, uint64_t seed, float frequency, uint32_t octaves, float coverage,
+                           float sharpness, const float lo[3], const float hi[3], float* out);
 
 /* Unit icosphere subdivided `levels` times: 10*4^levels + 2 vertices and
  * 20*4^levels triangles. Vertices move along their smooth normals by

@@ -382,3 +382,65 @@ def nrc_adam(p, g, m, v, t, lr=1e-2, b1=0.9, b2=0.99, eps=1e-15):
 
 def bf16_round(x):
     return _nrc_sigs().orc_bf16_round(float(x))
+
+
+# ---- participating media -------------------------------------------------------
+
+def _med_sigs():
+    lib = load()
+    if getattr(lib, "_med_ready", False):
+        return lib
+    vp, u64, f64 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double
+    sig = {
+        "orc_medium_density": (None, [vp, vp, u64, vp]),
+        "orc_medium_sample_n": (None, [vp, vp, vp, f64, vp, u64, u64, u64, vp, vp, vp]),
+        "orc_medium_transmittance_n": (None, [vp, vp, vp, f64, u64, u64, u64, vp]),
+        "orc_hg_phase": (f64, [f64, f64]),
+        "orc_hg_sample_n": (None, [f64, vp, u64, u64, u64, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    lib._med_ready = True
+    return lib
+
+
+def medium_density(medium, pts):
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    out = np.zeros(pts.shape[0])
+    _med_sigs().orc_medium_density(ctypes.addressof(medium), _p(pts), pts.shape[0], _p(out))
+    return out
+
+
+def medium_sample(medium, o, d, t_max, beta, seed, stream, n):
+    """Returns (scatter bool[n], t[n], weight[n, 3])."""
+    o = np.ascontiguousarray(o, np.float64)
+    d = np.ascontiguousarray(d, np.float64)
+    b = np.ascontiguousarray(beta, np.float64)
+    sc = np.zeros(n, np.uint8)
+    t = np.zeros(n)
+    w = np.zeros((n, 3))
+    _med_sigs().orc_medium_sample_n(ctypes.addressof(medium), _p(o), _p(d), t_max, _p(b), seed, stream, n,
+                                    _p(sc), _p(t), _p(w))
+    return sc.astype(bool), t, w
+
+
+def medium_transmittance(medium, o, d, dist, seed, stream, n):
+    o = np.ascontiguousarray(o, np.float64)
+    d = np.ascontiguousarray(d, np.float64)
+    out = np.zeros((n, 3))
+    _med_sigs().orc_medium_transmittance_n(ctypes.addressof(medium), _p(o), _p(d), dist, seed, stream, n, _p(out))
+    return out
+
+
+def hg_phase(g, cos_theta):
+    return _med_sigs().orc_hg_phase(g, cos_theta)
+
+
+def hg_sample(g, wo, seed, stream, n):
+    wo = np.ascontiguousarray(wo, np.float64)
+    wi = np.zeros((n, 3))
+    pdf = np.zeros(n)
+    _med_sigs().orc_hg_sample_n(g, _p(wo), seed, stream, n, _p(wi), _p(pdf))
+    return wi, pdf

@@ -743,57 +743,189 @@ Ray Scene::camera_ray(uint32_t x, uint32_t y, Pcg32& rng) const {
   return camera.generate_ray((double)x + 0.5 + dx, (double)y + 0.5 + dy);
 }
 
-// pt/mod.rs:1243-1306 for opaque scenes: visible unless something is hit
-// before `remaining`.
-double Scene::visibility(const Vec3& origin, const Vec3& dir, double remaining, uint64_t* rays) const {
+// ---- participating media (medium.rs:28-101, pt/volume.rs:29-189) -------------------
+
+double Medium::majorant() const {
+  const Vec3 st = sigma_t();
+  return std::fmax(std::fmax(st.x, st.y), st.z);
+}
+
+// Trilinear over the vertex grid, zero outside; 1 when homogeneous.
+double Medium::density_at(const Vec3& p) const {
+  if (nx == 0) return 1.0;
+  const double pc[3] = {p.x, p.y, p.z};
+  const uint32_t n[3] = {nx, ny, nz};
+  int i[3];
+  double f[3];
+  for (int a = 0; a < 3; a++) {
+    const double u = (pc[a] - lo[a]) * ((double)(n[a] - 1) / (hi[a] - lo[a]));
+    if (!(u >= 0.0 && u <= (double)(n[a] - 1))) return 0.0;
+    i[a] = std::min((int)u, (int)n[a] - 2);
+    f[a] = u - (double)i[a];
+  }
+  auto at = [&](int dx, int dy, int dz) {
+    return (double)grid[((size_t)(i[2] + dz) * ny + (i[1] + dy)) * nx + (i[0] + dx)];
+  };
+  const double c00 = at(0, 0, 0) + (at(1, 0, 0) - at(0, 0, 0)) * f[0];
+  const double c10 = at(0, 1, 0) + (at(1, 1, 0) - at(0, 1, 0)) * f[0];
+  const double c01 = at(0, 0, 1) + (at(1, 0, 1) - at(0, 0, 1)) * f[0];
+  const double c11 = at(0, 1, 1) + (at(1, 1, 1) - at(0, 1, 1)) * f[0];
+  const double c0 = c00 + (c10 - c00) * f[1], c1 = c01 + (c11 - c01) * f[1];
+  return c0 + (c1 - c0) * f[2];
+}
+
+namespace {
+
+// Part of the ray inside the grid box (density is zero outside, so delta
+// and ratio tracking start at the entry and stop at the exit).
+bool medium_range(const Medium& m, const Vec3& o, const Vec3& d, double t_max, double* t0, double* t1) {
+  *t0 = 0.0;
+  *t1 = std::fmin(t_max, m.max_distance);
+  if (m.nx == 0) return *t1 > *t0;
+  const double oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+  for (int a = 0; a < 3; a++) {
+    const double inv = 1.0 / dd[a];
+    double ta = (m.lo[a] - oo[a]) * inv, tb = (m.hi[a] - oo[a]) * inv;
+    if (ta > tb) std::swap(ta, tb);
+    if (std::isnan(ta)) ta = -INFINITY;  // 0 * inf when the origin lies on a slab plane
+    if (std::isnan(tb)) tb = INFINITY;
+    *t0 = std::fmax(*t0, ta);
+    *t1 = std::fmin(*t1, tb);
+  }
+  return *t1 > *t0;
+}
+
+double avg3(const Vec3& v) { return (v.x + v.y + v.z) / 3.0; }
+Vec3 exp3(const Vec3& v, double t) { return {std::exp(-v.x * t), std::exp(-v.y * t), std::exp(-v.z * t)}; }
+double exp_step(Pcg32& rng, double sigma) { return -std::log(std::fmax(1.0 - rng.next_f64(), 1e-12)) / sigma; }
+
+}  // namespace
+
+MediumEvent sample_distance(const Medium& m, const Vec3& o, const Vec3& d, double t_max, const Vec3& beta,
+                            Pcg32& rng) {
+  const Vec3 st = m.sigma_t();
+  MediumEvent ev;
+  if (m.nx == 0) {  // volume.rs:60-102
+    const double tm = std::fmin(t_max, m.max_distance);
+    if (avg3(st) <= 1e-9) return ev;
+    const Vec3 b{std::fmax(beta.x, 0.0), std::fmax(beta.y, 0.0), std::fmax(beta.z, 0.0)};
+    const double tot = b.x + b.y + b.z;
+    const double w[3] = {tot > 1e-12 ? b.x / tot : 1.0 / 3.0, tot > 1e-12 ? b.y / tot : 1.0 / 3.0,
+                         tot > 1e-12 ? b.z / tot : 1.0 / 3.0};
+    const double u = rng.next_f64();
+    const double sc[3] = {st.x, st.y, st.z};
+    const int c = u < w[0] ? 0 : (u < w[0] + w[1] ? 1 : 2);
+    const double t = exp_step(rng, std::fmax(sc[c], 1e-9));
+    if (t < tm) {
+      const Vec3 tr = exp3(st, t);
+      const double pdf = w[0] * st.x * tr.x + w[1] * st.y * tr.y + w[2] * st.z * tr.z;
+      if (pdf <= 1e-300) return MediumEvent{false, 0.0, Vec3::zero()};
+      return MediumEvent{true, t, m.sigma_s * tr / pdf};
+    }
+    const Vec3 tr = exp3(st, tm);
+    const double pdf = w[0] * tr.x + w[1] * tr.y + w[2] * tr.z;
+    if (pdf <= 1e-300) return MediumEvent{false, 0.0, Vec3::zero()};
+    return MediumEvent{false, 0.0, tr / pdf};
+  }
+  // volume.rs:104-149 weighted delta tracking
+  const double maj = m.majorant();
+  double t, t_end;
+  if (maj <= 1e-9 || !medium_range(m, o, d, t_max, &t, &t_end)) return ev;
+  Vec3 weight{1, 1, 1};
+  for (int i = 0; i < 10000; i++) {
+    t += exp_step(rng, maj);
+    if (t >= t_end) return MediumEvent{false, 0.0, weight};
+    const double rho = m.density_at(o + d * t);
+    const Vec3 sh = st * rho;
+    const double p_real = std::fmin(std::fmax(avg3(sh) / maj, 0.0), 1.0);
+    if (rng.next_f64() < p_real) return MediumEvent{true, t, weight * m.sigma_s * (rho / (maj * p_real))};
+    const double p_null = std::fmax(1.0 - p_real, 1e-9);
+    weight = weight * Vec3{(1.0 - sh.x / maj) / p_null, (1.0 - sh.y / maj) / p_null, (1.0 - sh.z / maj) / p_null};
+    if (max_component(weight) < 1e-6) return MediumEvent{false, 0.0, Vec3::zero()};
+  }
+  return MediumEvent{false, 0.0, weight};
+}
+
+// volume.rs:153-189
+Vec3 medium_transmittance(const Medium& m, const Vec3& o, const Vec3& d, double dist, Pcg32& rng) {
+  const Vec3 st = m.sigma_t();
+  if (m.nx == 0) return exp3(st, std::fmin(dist, m.max_distance));
+  const double maj = m.majorant();
+  double t, t_end;
+  Vec3 tr{1, 1, 1};
+  if (maj <= 1e-9 || !medium_range(m, o, d, dist, &t, &t_end)) return tr;
+  for (int i = 0; i < 10000; i++) {
+    t += exp_step(rng, maj);
+    if (t >= t_end) break;
+    const double rho = m.density_at(o + d * t);
+    tr = tr * Vec3{1.0 - st.x * rho / maj, 1.0 - st.y * rho / maj, 1.0 - st.z * rho / maj};
+    if (max_component(tr) < 1e-5) return Vec3::zero();
+  }
+  return tr;
+}
+
+// medium.rs:73-101
+double hg_phase(double g, double cos_theta) {
+  const double denom = 1.0 + g * g + 2.0 * g * cos_theta;
+  return (1.0 - g * g) / (4.0 * PI * denom * std::sqrt(std::fmax(denom, 1e-9)));
+}
+
+Vec3 hg_sample(double g, const Vec3& wo, double u1, double u2, double* pdf) {
+  double cos_t;
+  if (std::fabs(g) < 1e-3) {
+    cos_t = 1.0 - 2.0 * u1;
+  } else {
+    const double sq = (1.0 - g * g) / (1.0 + g - 2.0 * g * u1);
+    cos_t = -(1.0 + g * g - sq * sq) / (2.0 * g);
+  }
+  const double sin_t = std::sqrt(std::fmax(1.0 - cos_t * cos_t, 0.0));
+  const double phi = 2.0 * PI * u2;
+  const Vec3 w = -wo;
+  const Vec3 a = std::fabs(w.x) > 0.9 ? Vec3{0, 1, 0} : Vec3{1, 0, 0};
+  const Vec3 t = w.cross(a).normalize();
+  const Vec3 b = w.cross(t);
+  const Vec3 wi = (t * (sin_t * std::cos(phi)) + b * (sin_t * std::sin(phi)) + w * cos_t).normalize();
+  *pdf = hg_phase(g, -wo.dot(wi));
+  return wi;
+}
+
+// pt/mod.rs:1243-1306 for opaque scenes in an optional atmosphere.
+Vec3 Scene::visibility(const Vec3& origin, const Vec3& dir, double remaining, Pcg32& rng, uint64_t* rays) const {
+  Vec3 tr{1, 1, 1};
+  if (atmosphere >= 0) tr = medium_transmittance(media[(size_t)atmosphere], origin, dir, remaining, rng);
   if (rays) ++*rays;
   Intersection h;
   Ray r = Ray::make(origin, dir);
-  if (!intersect(r, &h)) return 1.0;
-  return h.t >= remaining ? 1.0 : 0.0;
+  if (!intersect(r, &h)) return tr;
+  return h.t >= remaining ? tr : Vec3::zero();
 }
 
 // pt/mod.rs:1177-1209
 Vec3 Scene::area_contribution(const Vec3& p, const Vec3& n, const Vec3& sample_point,
-                              const Vec3& light_normal, double area, const Light& light,
-                              const PbrParams& pbr, const Frame& frame, const Vec3& wo_l,
-                              double eta_rel, double pick_pmf, uint64_t* rays) const {
+                              const Vec3& light_normal, double area, const Light& light, const EvalFn& eval,
+                              double pick_pmf, Pcg32& rng, uint64_t* rays) const {
   Vec3 to_light = sample_point - p;
   double dist2 = std::fmax(to_light.length_squared(), 1e-12);
   Vec3 wi = to_light.normalize();
   double cos_light = std::fabs(wi.dot(light_normal));
   if (cos_light < 1e-9 || area <= 0.0) return Vec3::zero();
-  Vec3 wi_l = frame.to_local(wi);
-  if (wi_l.z <= 0.0) return Vec3::zero();
   Vec3 f;
   double bsdf_pdf;
-  bsdf_eval_pdf(pbr, wo_l, wi_l, eta_rel, &f, &bsdf_pdf);
-  f = f * wi_l.z;
+  eval(wi, &f, &bsdf_pdf);
   if (max_component(f) <= 0.0) return Vec3::zero();
   // visibility_to (pt/mod.rs:1212-1224)
   Vec3 dir = sample_point - p;
   double dist = dir.length();
-  double vis = visibility(p + n * 1e-4, dir.normalize(), dist - 1e-3, rays);
-  if (vis <= 0.0) return Vec3::zero();
+  Vec3 vis = visibility(p + n * 1e-4, dir.normalize(), dist - 1e-3, rng, rays);
+  if (max_component(vis) <= 0.0) return Vec3::zero();
   double pdf_sa = pick_pmf * dist2 / (cos_light * area);
   double weight = power_heuristic(pdf_sa, bsdf_pdf);
   return f * light.radiance() * vis * (weight / pdf_sa);
 }
 
 // pt/mod.rs:1032-1174
-Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const PbrParams& pbr,
-                         const Frame& frame, const Vec3& wo_l, double eta_rel, double pick_pmf,
+Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const EvalFn& eval, double pick_pmf,
                          Pcg32& rng, uint64_t* rays) const {
-  auto eval = [&](const Vec3& wi_world, Vec3* f, double* pdf) {
-    Vec3 wi_l = frame.to_local(wi_world);
-    if (wi_l.z <= 0.0) {
-      *f = Vec3::zero();
-      *pdf = 0.0;
-      return;
-    }
-    bsdf_eval_pdf(pbr, wo_l, wi_l, eta_rel, f, pdf);
-    *f = *f * wi_l.z;
-  };
   switch (light.kind) {
     case RRHIP_LIGHT_POINT: {
       Vec3 to_light = light.position - p;
@@ -804,8 +936,8 @@ Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const
       eval(wi, &f, &pdf);
       if (max_component(f) <= 0.0) return Vec3::zero();
       Vec3 dir = light.position - p;
-      double vis = visibility(p + n * 1e-4, dir.normalize(), dir.length() - 1e-3, rays);
-      if (vis <= 0.0) return Vec3::zero();
+      Vec3 vis = visibility(p + n * 1e-4, dir.normalize(), dir.length() - 1e-3, rng, rays);
+      if (max_component(vis) <= 0.0) return Vec3::zero();
       return f * light.radiance() * vis / (dist2 * pick_pmf);
     }
     case RRHIP_LIGHT_DISTANT: {
@@ -824,22 +956,20 @@ Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const
       double pdf;
       eval(wi, &f, &pdf);
       if (max_component(f) <= 0.0) return Vec3::zero();
-      double vis = visibility(p + n * 1e-4, wi, 1e8, rays);
-      if (vis <= 0.0) return Vec3::zero();
+      Vec3 vis = visibility(p + n * 1e-4, wi, 1e8, rng, rays);
+      if (max_component(vis) <= 0.0) return Vec3::zero();
       return f * light.radiance() * vis / pick_pmf;
     }
     case RRHIP_LIGHT_RECT: {
       double u = rng.next_f64(), v = rng.next_f64();
       Vec3 sp = light.corner + light.edge1 * u + light.edge2 * v;
-      return area_contribution(p, n, sp, light.normal, light.area, light, pbr, frame, wo_l, eta_rel,
-                               pick_pmf, rays);
+      return area_contribution(p, n, sp, light.normal, light.area, light, eval, pick_pmf, rng, rays);
     }
     case RRHIP_LIGHT_DISK: {
       double u = rng.next_f64(), v = rng.next_f64();
       double r = std::sqrt(u), phi = 2.0 * PI * v;
       Vec3 sp = light.center + light.e1 * (r * std::cos(phi)) + light.e2 * (r * std::sin(phi));
-      return area_contribution(p, n, sp, light.normal, light.area, light, pbr, frame, wo_l, eta_rel,
-                               pick_pmf, rays);
+      return area_contribution(p, n, sp, light.normal, light.area, light, eval, pick_pmf, rng, rays);
     }
     case RRHIP_LIGHT_SPHERE: {
       Vec3 to_center = light.center - p;
@@ -861,8 +991,8 @@ Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const
       double dist = std::sqrt(dist2) * cos_t - std::sqrt(std::fmax(r * r - dist2 * sin_t * sin_t, 0.0));
       Vec3 target = p + wi * dist;
       Vec3 dir = target - p;
-      double vis = visibility(p + n * 1e-4, dir.normalize(), dir.length() - 1e-3, rays);
-      if (vis <= 0.0) return Vec3::zero();
+      Vec3 vis = visibility(p + n * 1e-4, dir.normalize(), dir.length() - 1e-3, rng, rays);
+      if (max_component(vis) <= 0.0) return Vec3::zero();
       double pdf_sa = pick_pmf / std::fmax(2.0 * PI * (1.0 - cos_max), 1e-12);
       double weight = power_heuristic(pdf_sa, bsdf_pdf);
       return f * light.radiance() * vis * (weight / pdf_sa);
@@ -878,8 +1008,8 @@ Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const
       double bsdf_pdf;
       eval(wi, &f, &bsdf_pdf);
       if (max_component(f) <= 0.0) return Vec3::zero();
-      double vis = visibility(p + n * 1e-4, wi, 1e8, rays);
-      if (vis <= 0.0) return Vec3::zero();
+      Vec3 vis = visibility(p + n * 1e-4, wi, 1e8, rng, rays);
+      if (max_component(vis) <= 0.0) return Vec3::zero();
       double weight = power_heuristic(pdf_sa, bsdf_pdf);
       return f * light.radiance() * vis * (weight / pdf_sa);
     }
@@ -918,7 +1048,52 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
   while (depth < rp.max_depth) {
     if (rays) ++*rays;
     Intersection hit;
-    if (!intersect(ray, &hit)) {
+    const bool hit_found = intersect(ray, &hit);
+    // Atmosphere along this segment (pt/mod.rs:486-633; point lights, and
+    // so the equiangular strategy, are excluded by the scene loader).
+    if (atmosphere >= 0) {
+      const Medium& md = media[(size_t)atmosphere];
+      const double t_limit = hit_found ? hit.t : 1e8;
+      size_t seg_li = 0;
+      double seg_pick = 0.0;
+      bool has_pick = false;
+      if (!lights.empty()) {
+        const Vec3 mid = ray.at(std::fmin(std::fmin(t_limit, md.max_distance) * 0.5, 50.0));
+        has_pick = light_sampler.sample(mid, rng.next_f64(), &seg_li, &seg_pick);
+      }
+      const MediumEvent ev = sample_distance(md, ray.origin, ray.direction, t_limit, beta, rng);
+      beta = beta * ev.weight;
+      if (ev.scatter) {
+        const Vec3 p = ray.at(ev.t);
+        if (max_component(md.emission) > 0.0) l = l + beta * md.emission;
+        const Vec3 wo = -ray.direction.normalize();
+        if (has_pick) {
+          const double g = md.g;
+          const EvalFn eval = [&](const Vec3& wi, Vec3* f, double* pdf) {
+            const double ph = hg_phase(g, -wo.dot(wi));
+            *f = Vec3{ph, ph, ph};
+            *pdf = ph;
+          };
+          l = l + beta * sample_light(lights[seg_li], p, Vec3::zero(), eval, seg_pick, rng, rays);
+        }
+        const double u1 = rng.next_f64(), u2 = rng.next_f64();
+        double pdf;
+        const Vec3 wi = hg_sample(md.g, wo, u1, u2, &pdf);
+        prev_pdf = pdf;
+        prev_origin = p;
+        from_camera = false;
+        ray = Ray::make(p, wi);
+        depth++;
+        if (depth >= rp.rr_start) {
+          double q = std::fmin(max_component(beta), 0.95);
+          if (q <= 0.0 || rng.next_f64() > q) break;
+          beta = beta / q;
+        }
+        continue;
+      }
+      if (!(max_component(beta) > 0.0)) break;
+    }
+    if (!hit_found) {
       if (dome >= 0) {
         const Light& d = lights[dome];
         double weight = 1.0;
@@ -956,7 +1131,17 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
       size_t li;
       double pmf;
       if (light_sampler.sample(hit.point, rng.next_f64(), &li, &pmf)) {
-        Vec3 c = sample_light(lights[li], hit.point, n, pbr, frame, wo_l, eta_rel, pmf, rng, rays);
+        const EvalFn eval = [&](const Vec3& wi_world, Vec3* f, double* pdf) {
+          Vec3 wi_l = frame.to_local(wi_world);
+          if (wi_l.z <= 0.0) {
+            *f = Vec3::zero();
+            *pdf = 0.0;
+            return;
+          }
+          bsdf_eval_pdf(pbr, wo_l, wi_l, eta_rel, f, pdf);
+          *f = *f * wi_l.z;
+        };
+        Vec3 c = sample_light(lights[li], hit.point, n, eval, pmf, rng, rays);
         l = l + beta * c;
       }
     }
@@ -1025,6 +1210,29 @@ void Scene::load(const rrhip_scene_desc& d) {
   camera.right = v3(c.up).cross(camera.forward).normalize();
   camera.up = camera.forward.cross(camera.right).normalize();
   background = v3(d.background);
+  media.clear();
+  for (uint32_t i = 0; i < d.num_media; i++) {
+    const rrhip_medium_desc& md = d.media[i];
+    Medium m;
+    m.sigma_a = v3(md.sigma_a);
+    m.sigma_s = v3(md.sigma_s);
+    m.emission = v3(md.emission);
+    m.g = md.g;
+    m.max_distance = md.max_distance > 0.0 ? md.max_distance : 1e300;
+    if (md.nx > 0) {
+      m.nx = md.nx;
+      m.ny = md.ny;
+      m.nz = md.nz;
+      m.grid.assign(md.density, md.density + (size_t)md.nx * md.ny * md.nz);
+      for (float& v : m.grid) v = std::min(std::max(v, 0.f), 1.f);
+      for (int a = 0; a < 3; a++) {
+        m.lo[a] = md.grid_lo[a];
+        m.hi[a] = md.grid_hi[a];
+      }
+    }
+    media.push_back(m);
+  }
+  atmosphere = d.num_media > 0 ? d.atmosphere : -1;
 
   materials.clear();
   for (uint32_t i = 0; i < d.num_materials; i++) {

@@ -1,12 +1,14 @@
 // ORACLE — test infrastructure only (see orc_math.hpp header).
 //
-// f64 restatement of render.rs's CPU path tracer for the surface subset the
-// benchmark scenes use: src/raytracer/pt/mod.rs (trace_full without media,
-// hair, subsurface, presence or textures), src/raytracer/pt/bxdf.rs,
+// f64 restatement of render.rs's CPU path tracer for the subset the
+// benchmark scenes use: src/raytracer/pt/mod.rs (trace_full with a global
+// atmosphere medium; no hulls, hair, subsurface, presence or textures),
+// src/raytracer/pt/{bxdf,volume}.rs, src/scene/medium.rs,
 // src/scene/{pbr,light,light_sampler,camera}.rs, the Sphere / Triangle /
 // Instance intersectors and Scene::intersect.
 #pragma once
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 #include "../include/rrhip.h"
@@ -142,6 +144,31 @@ struct InstanceObj {
   Aabb world_bounds;
 };
 
+// medium.rs:28-65 with the density as a vertex grid (empty: homogeneous).
+struct Medium {
+  Vec3 sigma_a, sigma_s, emission;
+  double g = 0;
+  double max_distance = 1e300;
+  uint32_t nx = 0, ny = 0, nz = 0;
+  std::vector<float> grid;
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  Vec3 sigma_t() const { return sigma_a + sigma_s; }
+  double majorant() const;
+  double density_at(const Vec3& p) const;
+};
+
+// volume.rs:13-20
+struct MediumEvent {
+  bool scatter = false;
+  double t = 0;
+  Vec3 weight{1, 1, 1};
+};
+MediumEvent sample_distance(const Medium& m, const Vec3& o, const Vec3& d, double t_max, const Vec3& beta,
+                            Pcg32& rng);
+Vec3 medium_transmittance(const Medium& m, const Vec3& o, const Vec3& d, double dist, Pcg32& rng);
+double hg_phase(double g, double cos_theta);
+Vec3 hg_sample(double g, const Vec3& wo, double u1, double u2, double* pdf);
+
 struct RenderParams {
   uint32_t max_depth = 10;
   uint32_t rr_start = 3;
@@ -162,6 +189,8 @@ class Scene {
   LightSampler light_sampler;
   Vec3 background{0, 0, 0};
   int dome = -1;
+  std::vector<Medium> media;
+  int atmosphere = -1;
 
   void load(const rrhip_scene_desc& d);
   bool intersect(const Ray& ray, Intersection* out) const;
@@ -172,14 +201,17 @@ class Scene {
  private:
   bool instance_hit(const InstanceObj& inst, const Ray& ray, double t_max, double* t_param,
                     Vec3* world_normal) const;
-  Vec3 sample_light(const Light& light, const Vec3& p, const Vec3& n, const PbrParams& pbr,
-                    const Frame& frame, const Vec3& wo_l, double eta_rel, double pick_pmf,
+  // eval(wi, f, pdf): f * cos (or the phase value) towards world wi and the
+  // pdf of the scattering strategy, for MIS.
+  using EvalFn = std::function<void(const Vec3&, Vec3*, double*)>;
+  Vec3 sample_light(const Light& light, const Vec3& p, const Vec3& n, const EvalFn& eval, double pick_pmf,
                     Pcg32& rng, uint64_t* rays) const;
   Vec3 area_contribution(const Vec3& p, const Vec3& n, const Vec3& sample_point,
-                         const Vec3& light_normal, double area, const Light& light,
-                         const PbrParams& pbr, const Frame& frame, const Vec3& wo_l,
-                         double eta_rel, double pick_pmf, uint64_t* rays) const;
-  double visibility(const Vec3& origin, const Vec3& dir, double remaining, uint64_t* rays) const;
+                         const Vec3& light_normal, double area, const Light& light, const EvalFn& eval,
+                         double pick_pmf, Pcg32& rng, uint64_t* rays) const;
+  // Fractional visibility: the atmosphere's transmittance along the whole
+  // segment (drawn first, as the kernels do), then 0 if anything blocks it.
+  Vec3 visibility(const Vec3& origin, const Vec3& dir, double remaining, Pcg32& rng, uint64_t* rays) const;
 };
 
 }  // namespace orc

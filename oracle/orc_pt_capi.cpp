@@ -3,6 +3,7 @@
 // C entry points of the path-tracing oracle: renders into the accumulator
 // layout of include/rrhip.h (RRHIP_ACCUM_STRIDE doubles per pixel), and
 // exposes the BSDF and light sampler for unit tests.
+#include <algorithm>
 #include <atomic>
 #include <thread>
 #include <vector>
@@ -173,6 +174,84 @@ int orc_scene_intersect(const rrhip_scene_desc* desc, const double* o, const dou
   *front = h.front_face ? 1 : 0;
   *material = h.material;
   return 1;
+}
+
+}  // extern "C"
+
+// ---- participating media (tests) -----------------------------------------------------
+namespace {
+orc::Medium medium_from(const rrhip_medium_desc* md) {
+  orc::Medium m;
+  m.sigma_a = orc::Vec3{md->sigma_a[0], md->sigma_a[1], md->sigma_a[2]};
+  m.sigma_s = orc::Vec3{md->sigma_s[0], md->sigma_s[1], md->sigma_s[2]};
+  m.emission = orc::Vec3{md->emission[0], md->emission[1], md->emission[2]};
+  m.g = md->g;
+  m.max_distance = md->max_distance > 0.0 ? md->max_distance : 1e300;
+  if (md->nx > 0) {
+    m.nx = md->nx;
+    m.ny = md->ny;
+    m.nz = md->nz;
+    m.grid.assign(md->density, md->density + (size_t)md->nx * md->ny * md->nz);
+    for (float& v : m.grid) v = std::min(std::max(v, 0.f), 1.f);
+    for (int a = 0; a < 3; a++) {
+      m.lo[a] = md->grid_lo[a];
+      m.hi[a] = md->grid_hi[a];
+    }
+  }
+  return m;
+}
+}  // namespace
+
+extern "C" {
+
+void orc_medium_density(const rrhip_medium_desc* md, const double* pts, uint64_t n, double* out) {
+  const orc::Medium m = medium_from(md);
+  for (uint64_t i = 0; i < n; i++) out[i] = m.density_at(orc::Vec3{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]});
+}
+
+// n independent sample_distance calls from one PCG32(seed, stream).
+void orc_medium_sample_n(const rrhip_medium_desc* md, const double* o, const double* d, double t_max,
+                         const double* beta, uint64_t seed, uint64_t stream, uint64_t n, uint8_t* scatter,
+                         double* t, double* w) {
+  const orc::Medium m = medium_from(md);
+  orc::Pcg32 rng = orc::Pcg32::make(seed, stream);
+  const orc::Vec3 ov{o[0], o[1], o[2]}, dv{d[0], d[1], d[2]}, bv{beta[0], beta[1], beta[2]};
+  for (uint64_t i = 0; i < n; i++) {
+    const orc::MediumEvent ev = orc::sample_distance(m, ov, dv, t_max, bv, rng);
+    scatter[i] = ev.scatter ? 1 : 0;
+    t[i] = ev.t;
+    w[3 * i] = ev.weight.x;
+    w[3 * i + 1] = ev.weight.y;
+    w[3 * i + 2] = ev.weight.z;
+  }
+}
+
+void orc_medium_transmittance_n(const rrhip_medium_desc* md, const double* o, const double* d, double dist,
+                                uint64_t seed, uint64_t stream, uint64_t n, double* out) {
+  const orc::Medium m = medium_from(md);
+  orc::Pcg32 rng = orc::Pcg32::make(seed, stream);
+  const orc::Vec3 ov{o[0], o[1], o[2]}, dv{d[0], d[1], d[2]};
+  for (uint64_t i = 0; i < n; i++) {
+    const orc::Vec3 tr = orc::medium_transmittance(m, ov, dv, dist, rng);
+    out[3 * i] = tr.x;
+    out[3 * i + 1] = tr.y;
+    out[3 * i + 2] = tr.z;
+  }
+}
+
+double orc_hg_phase(double g, double cos_theta) { return orc::hg_phase(g, cos_theta); }
+
+void orc_hg_sample_n(double g, const double* wo, uint64_t seed, uint64_t stream, uint64_t n, double* wi,
+                     double* pdf) {
+  orc::Pcg32 rng = orc::Pcg32::make(seed, stream);
+  const orc::Vec3 w{wo[0], wo[1], wo[2]};
+  for (uint64_t i = 0; i < n; i++) {
+    const double u1 = rng.next_f64(), u2 = rng.next_f64();
+    const orc::Vec3 v = orc::hg_sample(g, w, u1, u2, &pdf[i]);
+    wi[3 * i] = v.x;
+    wi[3 * i + 1] = v.y;
+    wi[3 * i + 2] = v.z;
+  }
 }
 
 }  // extern "C"

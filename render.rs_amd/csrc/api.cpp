@@ -310,6 +310,18 @@ hipError_t upload_pt(rrhip_scene* s, const PtSceneHost& h) {
   if (e == hipSuccess) e = upload_vec(h.meshes, &s->pt_mem[11], &v.meshes);
   if (e == hipSuccess) e = upload_vec(h.mesh_idx, &s->pt_mem[12], &v.mesh_idx);
   if (e == hipSuccess) e = upload_vec(h.mesh_nrm, &s->pt_mem[13], &v.mesh_nrm);
+  // media: each density grid in its own allocation, then the medium table
+  std::vector<PtMedium> media = h.media;
+  for (size_t i = 0; i < media.size() && e == hipSuccess; i++) {
+    if (h.media_grids[i].empty()) continue;
+    void* gp = nullptr;
+    const float* view = nullptr;
+    e = upload_vec(h.media_grids[i], &gp, &view);
+    if (gp) s->media_mem.push_back(gp);
+    media[i].grid = view;
+  }
+  if (e == hipSuccess) e = upload_vec(media, &s->pt_mem[14], &v.media);
+  v.atmosphere = h.atmosphere;
   v.num_insts = (uint32_t)h.insts.size();
   v.num_tris = (uint32_t)h.tris.size();
   v.num_spheres = (uint32_t)h.spheres.size();
@@ -344,7 +356,10 @@ int rrhip_scene_upload(rrhip_ctx* ctx, const rrhip_scene_desc* desc, rrhip_scene
   PtSceneHost pth;
   if (is_pt(desc)) {
     std::string err;
-    if (!build_pt_scene(*desc, &pth, &err)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_scene_upload: " + err);
+    if (!build_pt_scene(*desc, &pth, &err)) {
+      const bool unsupported = err.rfind("UNSUPPORTED", 0) == 0;
+      return fail(unsupported ? RRHIP_ERR_UNSUPPORTED : RRHIP_ERR_INVALID_ARGUMENT, "rrhip_scene_upload: " + err);
+    }
     if (pth.stack_need > kPtTraceStack)
       return fail(RRHIP_ERR_UNSUPPORTED, "rrhip_scene_upload: instance hierarchy too deep for the " +
                                              std::to_string(kPtTraceStack) + "-entry traversal stack");
@@ -390,6 +405,7 @@ int rrhip_scene_destroy(rrhip_scene* s) {
   if (s->d_tris) (void)hipFree(s->d_tris);
   if (s->d_in) (void)hipFree(s->d_in);
   if (s->d_out) (void)hipFree(s->d_out);
+  for (void* p : s->media_mem) (void)hipFree(p);
   for (void* p : s->pt_mem)
     if (p) (void)hipFree(p);
   delete s;

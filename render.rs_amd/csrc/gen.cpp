@@ -180,3 +180,34 @@ extern "C" int rrhip_gen_icosphere(uint32_t levels, uint64_t seed, float amplitu
   std::copy(f.begin(), f.end(), indices);
   return RRHIP_OK;
 }
+
+// medium.rs:55-63 (DensityField::Fbm) sampled on an n^3 vertex grid.
+extern "C" int rrhip_gen_density_grid(uint32_t n, uint64_t seed, float frequency, uint32_t octaves,
+                                      float coverage, float sharpness, const float lo[3], const float hi[3],
+                                      float* out) {
+  if (n < 2 || n > 1024 || !lo || !hi || !out)
+    return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_gen_density_grid: need 2 <= n <= 1024 and non-null arrays");
+  for (int a = 0; a < 3; a++)
+    if (!(hi[a] > lo[a])) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_gen_density_grid: empty bounds");
+  Pcg32 r = Pcg32::make(seed, 0x5eed);
+  const double off[3] = {1000.0 * r.next_f64(), 1000.0 * r.next_f64(), 1000.0 * r.next_f64()};
+  auto slabs = [&](uint32_t k0, uint32_t k1) {
+    for (uint32_t k = k0; k < k1; k++)
+      for (uint32_t j = 0; j < n; j++)
+        for (uint32_t i = 0; i < n; i++) {
+          const uint32_t idx[3] = {i, j, k};
+          double p[3];
+          for (int a = 0; a < 3; a++)
+            p[a] = (double)lo[a] + ((double)hi[a] - (double)lo[a]) * (double)idx[a] / (double)(n - 1);
+          const double v = fbm3(p[0], p[1], p[2], (double)frequency, octaves, 0.5, 2.0, off) * 0.5 + 0.5;
+          const double rho = (v - (1.0 - (double)coverage)) * (double)sharpness;
+          out[((size_t)k * n + j) * n + i] = (float)std::min(std::max(rho, 0.0), 1.0);
+        }
+  };
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < nt; t++)
+    pool.emplace_back(slabs, (uint32_t)((uint64_t)n * t / nt), (uint32_t)((uint64_t)n * (t + 1) / nt));
+  for (auto& th : pool) th.join();
+  return RRHIP_OK;
+}

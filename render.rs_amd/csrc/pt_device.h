@@ -12,6 +12,7 @@
 #include <cstdint>
 
 #include "pt_types.h"
+#include "trace_math.h"
 #include "sampling.h"
 
 namespace rrhip {
@@ -492,6 +493,159 @@ __device__ __forceinline__ bool occluded(const PtSceneView& S, V3 o, V3 d, float
     if (t > kObjEps && t < tmax) return true;
   }
   return false;
+}
+
+// ---- participating media (medium.rs:28-101, pt/volume.rs:1-189) ---------------------
+
+// Density multiplier at p: trilinear over the vertex grid, 0 outside it;
+// 1 for homogeneous media.
+__device__ __forceinline__ float density_at(const PtMedium& m, V3 p) {
+  if (!m.grid) return 1.f;
+  const float u[3] = {(p.x - m.lo[0]) * m.scale[0], (p.y - m.lo[1]) * m.scale[1], (p.z - m.lo[2]) * m.scale[2]};
+  const uint32_t n[3] = {m.nx, m.ny, m.nz};
+  int i[3];
+  float f[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (!(u[a] >= 0.f && u[a] <= (float)(n[a] - 1))) return 0.f;
+    i[a] = min((int)u[a], (int)n[a] - 2);
+    f[a] = u[a] - (float)i[a];
+  }
+  const float* g = m.grid + ((size_t)i[2] * m.ny + i[1]) * m.nx + i[0];
+  const size_t sy = m.nx, sz = (size_t)m.nx * m.ny;
+  const float c00 = g[0] + (g[1] - g[0]) * f[0];
+  const float c10 = g[sy] + (g[sy + 1] - g[sy]) * f[0];
+  const float c01 = g[sz] + (g[sz + 1] - g[sz]) * f[0];
+  const float c11 = g[sz + sy] + (g[sz + sy + 1] - g[sz + sy]) * f[0];
+  const float c0 = c00 + (c10 - c00) * f[1], c1 = c01 + (c11 - c01) * f[1];
+  return c0 + (c1 - c0) * f[2];
+}
+
+// Parametric range of the ray inside the medium's grid box (the whole
+// line for homogeneous media). Outside the grid the density is zero, so
+// tracking can start at the entry and stop at the exit: null collisions
+// there change nothing and the exponential steps are memoryless.
+__device__ __forceinline__ bool medium_range(const PtMedium& m, V3 o, V3 d, float t_max, float& t0, float& t1) {
+  t0 = 0.f;
+  t1 = fminf(t_max, m.max_distance);
+  if (!m.grid) return t1 > t0;
+  const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const float inv = safe_rcp(dd[a]);
+    float ta = (m.lo[a] - oo[a]) * inv, tb = (m.hi[a] - oo[a]) * inv;
+    if (ta > tb) {
+      const float x = ta;
+      ta = tb;
+      tb = x;
+    }
+    t0 = fmaxf(t0, ta);
+    t1 = fminf(t1, tb);
+  }
+  return t1 > t0;
+}
+
+struct MediumEvent {
+  bool scatter;
+  float t;
+  V3 weight;
+};
+
+__device__ __forceinline__ float avg3(V3 v) { return (v.x + v.y + v.z) * (1.f / 3.f); }
+__device__ __forceinline__ V3 exp3(V3 v, float t) { return v3(expf(-v.x * t), expf(-v.y * t), expf(-v.z * t)); }
+__device__ __forceinline__ float exp_step(Pcg32& rng, float sigma) {
+  return -logf(fmaxf(1.f - rnd(rng), 1e-12f)) / sigma;
+}
+
+// volume.rs:29-47 sample_distance: spectral one-sample MIS in homogeneous
+// media (volume.rs:60-102), weighted delta tracking against the
+// channel-max majorant in gridded ones (volume.rs:104-149).
+__device__ __forceinline__ MediumEvent sample_distance(const PtMedium& m, V3 o, V3 d, float t_max, V3 beta,
+                                                      Pcg32& rng) {
+  const V3 st = v3(m.sigma_t);
+  if (!m.grid) {
+    const float tm = fminf(t_max, m.max_distance);
+    const float sa = avg3(st);
+    if (sa <= 1e-9f) return {false, 0.f, v3(1.f, 1.f, 1.f)};
+    const V3 b = v3(fmaxf(beta.x, 0.f), fmaxf(beta.y, 0.f), fmaxf(beta.z, 0.f));
+    const float tot = b.x + b.y + b.z;
+    const float w[3] = {tot > 1e-12f ? b.x / tot : 1.f / 3.f, tot > 1e-12f ? b.y / tot : 1.f / 3.f,
+                        tot > 1e-12f ? b.z / tot : 1.f / 3.f};
+    const float u = rnd(rng);
+    const float sc[3] = {st.x, st.y, st.z};
+    const int c = u < w[0] ? 0 : (u < w[0] + w[1] ? 1 : 2);
+    const float t = exp_step(rng, fmaxf(sc[c], 1e-9f));
+    if (t < tm) {
+      const V3 tr = exp3(st, t);
+      const float pdf = w[0] * st.x * tr.x + w[1] * st.y * tr.y + w[2] * st.z * tr.z;
+      if (!(pdf > 0.f)) return {false, 0.f, v3(0.f, 0.f, 0.f)};
+      return {true, t, v3(m.sigma_s) * tr / pdf};
+    }
+    const V3 tr = exp3(st, tm);
+    const float pdf = w[0] * tr.x + w[1] * tr.y + w[2] * tr.z;
+    if (!(pdf > 0.f)) return {false, 0.f, v3(0.f, 0.f, 0.f)};
+    return {false, 0.f, tr / pdf};
+  }
+  V3 weight = v3(1.f, 1.f, 1.f);
+  const float maj = m.majorant;
+  float t, t_end;
+  if (maj <= 1e-9f || !medium_range(m, o, d, t_max, t, t_end)) return {false, 0.f, weight};
+  for (int i = 0; i < 10000; i++) {
+    t += exp_step(rng, maj);
+    if (t >= t_end) return {false, 0.f, weight};
+    const float rho = density_at(m, o + d * t);
+    const V3 sh = st * rho;
+    const float p_real = fminf(fmaxf(avg3(sh) / maj, 0.f), 1.f);
+    if (rnd(rng) < p_real) return {true, t, weight * v3(m.sigma_s) * (rho / (maj * p_real))};
+    const float p_null = fmaxf(1.f - p_real, 1e-9f);
+    weight = weight * v3((1.f - sh.x / maj) / p_null, (1.f - sh.y / maj) / p_null, (1.f - sh.z / maj) / p_null);
+    if (maxc(weight) < 1e-6f) return {false, 0.f, v3(0.f, 0.f, 0.f)};
+  }
+  return {false, 0.f, weight};
+}
+
+// volume.rs:153-189: closed form (homogeneous) or ratio tracking.
+__device__ __forceinline__ V3 medium_transmittance(const PtMedium& m, V3 o, V3 d, float dist, Pcg32& rng) {
+  const V3 st = v3(m.sigma_t);
+  if (!m.grid) return exp3(st, fminf(dist, m.max_distance));
+  const float maj = m.majorant;
+  float t, t_end;
+  V3 tr = v3(1.f, 1.f, 1.f);
+  if (maj <= 1e-9f || !medium_range(m, o, d, dist, t, t_end)) return tr;
+  for (int i = 0; i < 10000; i++) {
+    t += exp_step(rng, maj);
+    if (t >= t_end) break;
+    const float rho = density_at(m, o + d * t);
+    tr = tr * v3(1.f - st.x * rho / maj, 1.f - st.y * rho / maj, 1.f - st.z * rho / maj);
+    if (maxc(tr) < 1e-5f) return v3(0.f, 0.f, 0.f);
+  }
+  return tr;
+}
+
+// medium.rs:73-101
+__device__ __forceinline__ float hg_phase(float g, float cos_theta) {
+  const float denom = 1.f + g * g + 2.f * g * cos_theta;
+  return (1.f - g * g) / (4.f * kPi * denom * sqrtf(fmaxf(denom, 1e-9f)));
+}
+
+__device__ __forceinline__ V3 hg_sample(float g, V3 wo, float u1, float u2, float& pdf) {
+  float cos_t;
+  if (fabsf(g) < 1e-3f) {
+    cos_t = 1.f - 2.f * u1;
+  } else {
+    const float sq = (1.f - g * g) / (1.f + g - 2.f * g * u1);
+    cos_t = -(1.f + g * g - sq * sq) / (2.f * g);
+  }
+  const float sin_t = sqrtf(fmaxf(1.f - cos_t * cos_t, 0.f));
+  float sp, cp;
+  sincosf(2.f * kPi * u2, &sp, &cp);
+  const V3 w = -wo;
+  const V3 a = fabsf(w.x) > 0.9f ? v3(0.f, 1.f, 0.f) : v3(1.f, 0.f, 0.f);
+  const V3 t = normalize(cross(w, a));
+  const V3 b = cross(w, t);
+  const V3 wi = normalize(t * (sin_t * cp) + b * (sin_t * sp) + w * cos_t);
+  pdf = hg_phase(g, -dot(wo, wi));
+  return wi;
 }
 
 // ---- camera (camera.rs:35-179, pinhole) -------------------------------------------

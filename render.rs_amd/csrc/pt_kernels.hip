@@ -167,8 +167,10 @@ __device__ __forceinline__ void eval_world(const PtMaterial& m, const Frame& fr,
 
 // sample_light + area_light_contribution (pt/mod.rs:1032-1209), without the
 // visibility test, which becomes the returned shadow ray.
-__device__ __forceinline__ bool sample_light(const PtSceneView& S, const PtLight& L, V3 p, V3 n,
-                                             const PtMaterial& m, const Frame& fr, V3 wo_l, float eta,
+// `eval(wi, f, pdf)` gives f * cos (or the phase value) and the pdf of
+// the BSDF (or phase) sampling strategy for MIS; n = 0 for medium points.
+template <class Eval>
+__device__ __forceinline__ bool sample_light(const PtSceneView& S, const PtLight& L, V3 p, V3 n, const Eval& eval,
                                              float pick, Pcg32& rng, ShadowOut& out) {
   V3 f;
   float bsdf_pdf;
@@ -179,7 +181,7 @@ __device__ __forceinline__ bool sample_light(const PtSceneView& S, const PtLight
       const V3 to = v3(L.p) - p;
       const float dist2 = fmaxf(dot(to, to), 1e-12f);
       const V3 wi = normalize(to);
-      eval_world(m, fr, wo_l, eta, wi, f, bsdf_pdf);
+      eval(wi, f, bsdf_pdf);
       if (maxc(f) <= 0.f) return false;
       out.o = org;
       out.d = wi;
@@ -198,7 +200,7 @@ __device__ __forceinline__ bool sample_light(const PtSceneView& S, const PtLight
         sincosf(2.f * kPi * v, &sp, &cp);
         wi = to_world(make_frame(base), v3(sin_t * cp, sin_t * sp, cos_t));
       }
-      eval_world(m, fr, wo_l, eta, wi, f, bsdf_pdf);
+      eval(wi, f, bsdf_pdf);
       if (maxc(f) <= 0.f) return false;
       out.o = org;
       out.d = wi;
@@ -223,7 +225,7 @@ __device__ __forceinline__ bool sample_light(const PtSceneView& S, const PtLight
       const V3 wi = normalize(to);
       const float cos_light = fabsf(dot(wi, v3(L.n)));  // double-sided emitter
       if (cos_light < 1e-9f || L.area <= 0.f) return false;
-      eval_world(m, fr, wo_l, eta, wi, f, bsdf_pdf);
+      eval(wi, f, bsdf_pdf);
       if (maxc(f) <= 0.f) return false;
       const float pdf_sa = pick * dist2 / (cos_light * L.area);
       const float w = power_heuristic(pdf_sa, bsdf_pdf);
@@ -246,7 +248,7 @@ __device__ __forceinline__ bool sample_light(const PtSceneView& S, const PtLight
       float sn, cs;
       sincosf(2.f * kPi * v, &sn, &cs);
       const V3 wi = to_world(make_frame(normalize(tc)), v3(sin_t * cs, sin_t * sn, cos_t));
-      eval_world(m, fr, wo_l, eta, wi, f, bsdf_pdf);
+      eval(wi, f, bsdf_pdf);
       if (maxc(f) <= 0.f) return false;
       const float dist = sqrtf(dist2) * cos_t - sqrtf(fmaxf(r * r - dist2 * sin_t * sin_t, 0.f));
       const float pdf_sa = pick / fmaxf(2.f * kPi * (1.f - cos_max), 1e-12f);
@@ -265,7 +267,7 @@ __device__ __forceinline__ bool sample_light(const PtSceneView& S, const PtLight
       sincosf(2.f * kPi * v, &sn, &cs);
       const V3 wi = v3(rr * cs, rr * sn, z);
       const float pdf_sa = kInv4Pi * pick;
-      eval_world(m, fr, wo_l, eta, wi, f, bsdf_pdf);
+      eval(wi, f, bsdf_pdf);
       if (maxc(f) <= 0.f) return false;
       const float w = power_heuristic(pdf_sa, bsdf_pdf);
       out.o = org;
@@ -311,7 +313,62 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtSceneView S, PathBuffer
       const bool from_camera = (flags & kFlagFromCamera) != 0;
       uint32_t depth = flags & 0xFFu;
       float a0 = d4.w, asum = pv4.w;
-      if (hn.w < 0.f) {
+      // Atmosphere along this segment (pt/mod.rs:486-633): one light pick
+      // per segment, distance sampling, and at a real collision NEE with
+      // the phase function and an HG continuation instead of the surface.
+      bool in_medium = false, absorbed = false;
+      if (S.atmosphere >= 0) {
+        const PtMedium& md = S.media[S.atmosphere];
+        const float t_limit = hn.w < 0.f ? 1e8f : hn.w;
+        uint32_t seg_li = 0;
+        float seg_pick = 0.f;
+        bool has_pick = false;
+        if (S.num_lights > 0)
+          has_pick = ls_sample(S, ro + rd * fminf(fminf(t_limit, md.max_distance) * 0.5f, 50.f), rnd(rng), seg_li,
+                               seg_pick);
+        const MediumEvent ev = sample_distance(md, ro, rd, t_limit, beta, rng);
+        beta = beta * ev.weight;
+        if (ev.scatter) {
+          in_medium = true;
+          const V3 p = ro + rd * ev.t;
+          if (maxc(v3(md.emission)) > 0.f) Lr = Lr + beta * v3(md.emission);
+          const V3 wo = -rd;
+          if (has_pick) {
+            const auto eval = [&](V3 wi, V3& f, float& pdf) {
+              const float ph = hg_phase(md.g, -dot(wo, wi));
+              f = v3(ph, ph, ph);
+              pdf = ph;
+            };
+            if (sample_light(S, S.lights[seg_li], p, v3(0.f, 0.f, 0.f), eval, seg_pick, rng, so)) {
+              so.c = beta * so.c * medium_transmittance(md, so.o, so.d, so.tmax, rng);
+              shadow = maxc(so.c) > 0.f;
+            }
+          }
+          float pdf;
+          const float u1 = rnd(rng), u2 = rnd(rng);
+          const V3 wi = hg_sample(md.g, wo, u1, u2, pdf);
+          depth++;
+          bool alive = true;
+          if (depth >= prm.rr_start) {
+            const float qq = fminf(maxc(beta), 0.95f);
+            if (qq <= 0.f || rnd(rng) > qq) {
+              alive = false;
+            } else {
+              beta = beta / qq;
+            }
+          }
+          cont = alive && depth < prm.max_depth;
+          P.o[slot] = make_float4(p.x, p.y, p.z, pdf);
+          P.d[slot] = make_float4(wi.x, wi.y, wi.z, a0);
+          P.prev[slot] = make_float4(p.x, p.y, p.z, asum);
+          flags = depth | (flags & (kFlagTrain | kFlagSuffix));
+        } else {
+          absorbed = !(maxc(beta) > 0.f);
+        }
+      }
+      if (in_medium || absorbed) {
+        // handled above
+      } else if (hn.w < 0.f) {
         // Miss: dome (with MIS) or background (pt/mod.rs:643-659).
         if (S.dome >= 0) {
           float w = 1.f;
@@ -401,8 +458,11 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtSceneView S, PathBuffer
           uint32_t li;
           float pick;
           if (ls_sample(S, p, rnd(rng), li, pick)) {
-            if (sample_light(S, S.lights[li], p, nrm, m, fr, wo_l, eta, pick, rng, so)) {
+            const auto eval = [&](V3 wi, V3& f, float& pdf) { eval_world(m, fr, wo_l, eta, wi, f, pdf); };
+            if (sample_light(S, S.lights[li], p, nrm, eval, pick, rng, so)) {
               so.c = beta * so.c;
+              // shadow rays through the atmosphere carry its transmittance
+              if (S.atmosphere >= 0) so.c = so.c * medium_transmittance(S.media[S.atmosphere], so.o, so.d, so.tmax, rng);
               shadow = maxc(so.c) > 0.f;
             }
           }

@@ -411,6 +411,56 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
     out->spheres.push_back(g);
   }
   put(out->background, d3(d.background));
+  // Media (medium.rs:28-65)
+  if (d.num_media > 0 && !d.media) return *err = "num_media > 0 with null media", false;
+  if (d.atmosphere >= (int32_t)d.num_media || d.atmosphere < -1) return *err = "atmosphere index out of range", false;
+  for (uint32_t i = 0; i < d.num_media; i++) {
+    const rrhip_medium_desc& md = d.media[i];
+    if (!finite3(md.sigma_a) || !finite3(md.sigma_s) || !finite3(md.emission) || !std::isfinite(md.g))
+      return *err = "medium parameters must be finite", false;
+    if (!(md.g > -1.0 && md.g < 1.0)) return *err = "medium g must be in (-1, 1)", false;
+    for (int c = 0; c < 3; c++)
+      if (md.sigma_a[c] < 0.0 || md.sigma_s[c] < 0.0) return *err = "medium coefficients must be >= 0", false;
+    PtMedium g{};
+    put(g.sigma_a, d3(md.sigma_a));
+    put(g.sigma_s, d3(md.sigma_s));
+    const D3 st = add(d3(md.sigma_a), d3(md.sigma_s));
+    put(g.sigma_t, st);
+    g.majorant = (float)std::max(std::max(st.x, st.y), st.z);
+    g.g = (float)md.g;
+    put(g.emission, d3(md.emission));
+    g.max_distance = md.max_distance > 0.0 ? (float)md.max_distance : INFINITY;
+    std::vector<float> grid;
+    if (md.nx > 0) {
+      if (md.nx < 2 || md.ny < 2 || md.nz < 2 || !md.density)
+        return *err = "medium density grid needs >= 2 samples per axis and data", false;
+      if ((uint64_t)md.nx * md.ny * md.nz > (1ull << 31)) return *err = "medium density grid too large", false;
+      for (int a = 0; a < 3; a++)
+        if (!(md.grid_hi[a] > md.grid_lo[a]) || !std::isfinite(md.grid_lo[a]) || !std::isfinite(md.grid_hi[a]))
+          return *err = "medium grid bounds must be finite and non-empty", false;
+      grid.assign(md.density, md.density + (size_t)md.nx * md.ny * md.nz);
+      for (float& v : grid) {
+        if (!std::isfinite(v)) return *err = "medium density must be finite", false;
+        v = std::min(std::max(v, 0.f), 1.f);  // density fields are clamped to [0, 1] (medium.rs:48-52)
+      }
+      g.nx = md.nx;
+      g.ny = md.ny;
+      g.nz = md.nz;
+      const uint32_t n[3] = {md.nx, md.ny, md.nz};
+      for (int a = 0; a < 3; a++) {
+        g.lo[a] = (float)md.grid_lo[a];
+        g.hi[a] = (float)md.grid_hi[a];
+        g.scale[a] = (float)((double)(n[a] - 1) / (md.grid_hi[a] - md.grid_lo[a]));
+      }
+    }
+    out->media.push_back(g);
+    out->media_grids.push_back(std::move(grid));
+  }
+  out->atmosphere = d.num_media > 0 ? d.atmosphere : -1;
+  if (out->atmosphere >= 0)
+    for (uint32_t i = 0; i < d.num_lights; i++)
+      if (d.lights[i].kind == RRHIP_LIGHT_POINT)
+        return *err = "UNSUPPORTED: point lights inside an atmosphere (equiangular sampling) are not implemented", false;
   for (int a = 0; a < 3; a++) {
     out->lo[a] = INFINITY;
     out->hi[a] = -INFINITY;

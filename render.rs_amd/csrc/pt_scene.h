@@ -29,6 +29,9 @@ struct PtSceneHost {
   std::vector<uint32_t> mesh_idx;
   std::vector<float> mesh_nrm;
   uint32_t stack_need = 0;  // traversal stack entries the instance path needs
+  std::vector<PtMedium> media;                // grid pointers filled at upload
+  std::vector<std::vector<float>> media_grids;
+  int32_t atmosphere = -1;
   float lo[3], hi[3];       // bounds of all geometry (radiance-cache input range)
 };
 

@@ -280,6 +280,8 @@ extern "C" int rrhip_session_set_nrc(rrhip_session* s, rrhip_nrc* nrc) {
   }
   rrhip_scene* sc = s->scene;
   if (nrc->ctx != sc->ctx) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_set_nrc: cache and session differ in context");
+  if (sc->pt.atmosphere >= 0)
+    return fail(RRHIP_ERR_UNSUPPORTED, "rrhip_session_set_nrc: the radiance cache does not model participating media");
   const uint64_t ntrain = nrc->cfg.train_paths;
   const uint64_t cap = s->slots + ntrain;
   if (cap >= (1ull << 31)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_set_nrc: path slots exceed 2^31");

@@ -83,6 +83,24 @@ struct alignas(16) PtInstance {
   uint32_t pad[2];
 };
 
+// A participating medium (medium.rs:28-65) with its density as a vertex
+// grid over [lo, hi] (grid == nullptr: homogeneous).
+struct alignas(16) PtMedium {
+  float sigma_a[3], g;
+  float sigma_s[3], majorant;  // channel-max extinction
+  float sigma_t[3], max_distance;
+  float emission[3];
+  uint32_t nx;
+  float lo[3];
+  uint32_t ny;
+  float hi[3];
+  uint32_t nz;
+  float scale[3];              // (n - 1) / (hi - lo) per axis
+  uint32_t pad;
+  const float* grid;
+  const void* pad2;
+};
+
 struct PtMeshInfo {
   uint32_t root;      // BLAS root in the shared node array
   uint32_t idx_base;  // first index of the mesh in mesh_idx
@@ -117,6 +135,8 @@ struct PtSceneView {
   const PtMeshInfo* meshes;
   const uint32_t* mesh_idx;
   const float* mesh_nrm;
+  const PtMedium* media;
+  int32_t atmosphere;           // medium every ray travels in (-1: none)
   uint32_t num_insts;
   uint32_t num_tris, num_spheres, num_lights, num_lnodes, num_inf;
   int32_t dome;

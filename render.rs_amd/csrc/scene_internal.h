@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/rrhip.h"
 #include "layout.h"
@@ -33,6 +34,7 @@ struct rrhip_scene {
   // Path-tracing scene (camera given): device arrays behind `pt`.
   bool has_pt = false;
   rrhip::PtSceneView pt{};
-  void* pt_mem[14] = {};
+  void* pt_mem[15] = {};
+  std::vector<void*> media_mem;  // density grids
   float pt_lo[3] = {0, 0, 0}, pt_hi[3] = {0, 0, 0};  // bounds of all geometry
 };

@@ -39,7 +39,7 @@ EXPORTED = [
     "rrhip_load_accum", "rrhip_session_get_stats", "rrhip_gen_icosphere", "rrhip_relmse",
     "rrhip_nrc_create", "rrhip_nrc_destroy", "rrhip_nrc_get_info", "rrhip_nrc_get_params",
     "rrhip_nrc_set_params", "rrhip_nrc_infer", "rrhip_nrc_train_step", "rrhip_nrc_gradients",
-    "rrhip_session_set_nrc",
+    "rrhip_session_set_nrc", "rrhip_gen_density_grid",
 ]
 
 
@@ -104,6 +104,13 @@ class CameraDesc(ctypes.Structure):
                 ("eye", D3), ("look_at", D3), ("up", D3)]
 
 
+class MediumDesc(ctypes.Structure):
+    _fields_ = [("sigma_a", D3), ("sigma_s", D3), ("g", ctypes.c_double), ("emission", D3),
+                ("max_distance", ctypes.c_double), ("nx", ctypes.c_uint32), ("ny", ctypes.c_uint32),
+                ("nz", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("density", ctypes.c_void_p),
+                ("grid_lo", D3), ("grid_hi", D3)]
+
+
 class SceneDesc(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("num_meshes", ctypes.c_uint32),
                 ("meshes", ctypes.POINTER(MeshDesc)), ("build", BuildParams),
@@ -115,7 +122,9 @@ class SceneDesc(ctypes.Structure):
                 ("materials", ctypes.POINTER(MaterialDesc)),
                 ("num_lights", ctypes.c_uint32), ("reserved0", ctypes.c_uint32),
                 ("lights", ctypes.POINTER(LightDesc)),
-                ("camera", CameraDesc), ("background", D3)]
+                ("camera", CameraDesc), ("background", D3),
+                ("num_media", ctypes.c_uint32), ("atmosphere", ctypes.c_int32),
+                ("media", ctypes.POINTER(MediumDesc))]
 
 
 class RenderParams(ctypes.Structure):
@@ -240,6 +249,8 @@ def load():
         "rrhip_scene_set_tuning": (i32, [vp, ctypes.POINTER(TraceTuning)]),
         "rrhip_scene_autotune": (i32, [vp, vp, u64, i32, vp, ctypes.POINTER(TraceTuning),
                                        ctypes.POINTER(ctypes.c_float)]),
+        "rrhip_gen_density_grid": (i32, [u32, u64, ctypes.c_float, u32, ctypes.c_float, ctypes.c_float,
+                                         ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), vp]),
         "rrhip_gen_icosphere": (i32, [u32, u64, ctypes.c_float, ctypes.c_float, u32, vp, vp, vp]),
         "rrhip_gen_heightfield": (i32, [u32, u32, u64, ctypes.c_float, ctypes.c_float, u32, vp, vp]),
         "rrhip_gen_query_rays": (i32, [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
@@ -284,6 +295,16 @@ def gen_icosphere(levels, seed, amplitude=0.15, frequency=3.0, octaves=4):
     nrm = np.empty((nv, 3), np.float32)
     _check(lib.rrhip_gen_icosphere(levels, seed, amplitude, frequency, octaves, _ptr(pos), _ptr(idx), _ptr(nrm)))
     return pos, idx, nrm
+
+
+def gen_density_grid(n, seed, frequency=2.0, octaves=4, coverage=0.5, sharpness=3.0, lo=(-1, -1, -1),
+                     hi=(1, 1, 1)):
+    """fBm cloud density (medium.rs DensityField::Fbm) on an n^3 vertex grid (z, y, x order)."""
+    out = np.empty((n, n, n), np.float32)
+    lo_c = (ctypes.c_float * 3)(*[float(x) for x in lo])
+    hi_c = (ctypes.c_float * 3)(*[float(x) for x in hi])
+    _check(load().rrhip_gen_density_grid(n, seed, frequency, octaves, coverage, sharpness, lo_c, hi_c, _ptr(out)))
+    return out
 
 
 def gen_query_rays(lo, hi, inflate, seed, n):

@@ -89,6 +89,8 @@ class Scene:
         self.background = background
         self.materials, self.lights, self.triangles, self.spheres = [], [], [], []
         self.meshes, self.instances = [], []
+        self.media = []
+        self.atmosphere = -1
         self._keep = []
 
     def add_material(self, m):
@@ -133,6 +135,23 @@ class Scene:
         i.object_to_world = (ctypes.c_double * 12)(*[float(x) for x in np.asarray(object_to_world_3x4).ravel()])
         self.instances.append(i)
 
+    def add_medium(self, sigma_a, sigma_s, g=0.0, emission=(0, 0, 0), max_distance=0.0, density=None,
+                   grid_lo=(-1, -1, -1), grid_hi=(1, 1, 1)):
+        """A medium (medium.rs Medium); density: None (homogeneous) or a
+        (nz, ny, nx) array of [0, 1] values on the vertices of the grid box."""
+        m = rrhip.MediumDesc()
+        m.sigma_a, m.sigma_s, m.emission = _d3(sigma_a), _d3(sigma_s), _d3(emission)
+        m.g = g
+        m.max_distance = max_distance
+        if density is not None:
+            dens = np.ascontiguousarray(density, np.float32)
+            self._keep.append(dens)
+            m.nz, m.ny, m.nx = dens.shape
+            m.density = dens.ctypes.data
+            m.grid_lo, m.grid_hi = _d3(grid_lo), _d3(grid_hi)
+        self.media.append(m)
+        return len(self.media) - 1
+
     def _arr(self, ctype, items):
         if not items:
             return None
@@ -158,6 +177,9 @@ class Scene:
         d.lights = self._arr(rrhip.LightDesc, self.lights)
         d.camera = self.camera
         d.background = _d3(self.background)
+        d.num_media = len(self.media)
+        d.atmosphere = self.atmosphere
+        d.media = self._arr(rrhip.MediumDesc, self.media)
         self._keep.append(d)
         return d
 
@@ -196,6 +218,24 @@ def box_room(width=256, height=256, emission=15.0):
                   emitter)
     s.add_sphere((-0.4, -0.7, 0.2), 0.3, white)
     s.add_sphere((0.4, -0.65, -0.3), 0.35, white)
+    return s
+
+
+def cloud_room(width=128, height=128, n=128, seed=1, sigma_s=(2.0, 2.0, 2.0), sigma_a=(0.2, 0.2, 0.2), g=0.3,
+               emission=15.0, spheres=None):
+    """The box room filled with a heterogeneous atmosphere: an n^3 fBm cloud
+    density grid over the room's [-1, 1]^3 (medium.rs's DensityField::Fbm:
+    frequency 2, 4 octaves, coverage 0.5, sharpness 3, offset from `seed`),
+    sigma_s 2, sigma_a 0.2, Henyey-Greenstein g = 0.3. `spheres` replaces
+    the two room spheres with (centre, radius) pairs when given."""
+    s = box_room(width, height, emission)
+    if spheres is not None:
+        s.spheres = []
+        white = 0
+        for c, r in spheres:
+            s.add_sphere(c, r, white)
+    dens = rrhip.gen_density_grid(n, seed, frequency=2.0, octaves=4, coverage=0.5, sharpness=3.0)
+    s.atmosphere = s.add_medium(sigma_a, sigma_s, g, density=dens)
     return s
 
 

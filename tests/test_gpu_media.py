@@ -1,0 +1,129 @@
+"""Participating media on the GPU (atmosphere with a voxel fBm cloud:
+delta tracking, ratio-tracked shadow transmittance, Henyey-Greenstein
+scattering) against the f64 oracle and closed forms."""
+import sys
+
+import numpy as np
+import pytest
+
+import orc
+import rrhip
+import scenes
+from pt_stats import image_mean_z, mean_var, sigma_test
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def render(ctx, s, params, start, count):
+    ps = rrhip.PtScene(ctx, s.desc())
+    se = ps.session(params)
+    se.render(start, count)
+    acc = se.accum()
+    se.close()
+    ps.close()
+    return acc
+
+
+def test_cloud_room_matches_oracle(ctx):
+    say("cloud room 64x64 (32^3 fBm grid, g 0.3), depth 64, 256 spp vs oracle")
+    s = scenes.cloud_room(64, 64, n=32)
+    p = rrhip.RenderParams.make(max_depth=64)
+    acc_g = render(ctx, s, p, 0, 256)
+    acc_c, _ = orc.render(s.desc(), p, 0, 256)
+    z = sigma_test(acc_g, acc_c)
+    frac = float((z > 3).mean())
+    zi = image_mean_z(acc_g, acc_c)
+    mg, _, _ = mean_var(acc_g)
+    mc, _, _ = mean_var(acc_c)
+    say(f"outside 3 sigma {frac:.4%}, image-mean z {np.round(zi, 2)}, means gpu {mg.mean(0).round(5)} "
+        f"cpu {mc.mean(0).round(5)}")
+    assert frac <= 0.01 and (zi < 3).all()
+
+
+def test_absorbing_slab_matches_beer_lambert(ctx):
+    """A purely absorbing medium of constant density over [-1,1]^3 seen
+    against a unit dome: every pixel is exp(-sigma_a L), L the chord of its
+    camera ray through the box (delta tracking turns each real collision
+    into a zero-weight scatter)."""
+    say("absorbing box 48x48 vs exp(-sigma L), 512 spp")
+    W = H = 48
+    s = scenes.Scene(W, H, 40.0, eye=(0.3, 0.2, -4.0), look_at=(0, 0, 0), background=(0, 0, 0))
+    s.add_light(scenes.light(rrhip.LIGHT_DOME, intensity=1.0, color=(1, 1, 1)))
+    ones = np.ones((4, 4, 4), np.float32)
+    s.atmosphere = s.add_medium((0.8, 0.8, 0.8), (0.0, 0.0, 0.0), 0.0, density=ones)
+    acc = render(ctx, s, rrhip.RenderParams.make(max_depth=8, firefly_clamp=0.0), 0, 512)
+    m, var, n = mean_var(acc)
+    # chord lengths of the (jittered) pixel rays: average exp(-sigma L) over
+    # each pixel's footprint by supersampling the same camera on the host
+    d = s.desc()
+    cam = d.camera
+    fwd = np.array(cam.look_at) - np.array(cam.eye)
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(np.array(cam.up), fwd)
+    right /= np.linalg.norm(right)
+    up = np.cross(fwd, right)
+    hh = np.tan(np.radians(cam.fov_deg) / 2)
+    hw = hh * W / H
+    k = 8
+    sub = (np.arange(k) + 0.5) / k
+    want = np.zeros((H, W))
+    for y in range(H):
+        for x in range(W):
+            px = x + sub[None, :]
+            py = y + sub[:, None]
+            u = (2 * px / W - 1) * hw
+            v = (1 - 2 * py / H) * hh
+            dirs = fwd[None, None, :] + u[..., None] * right + v[..., None] * up
+            dirs /= np.linalg.norm(dirs, axis=-1, keepdims=True)
+            o = np.array(cam.eye)
+            inv = 1.0 / dirs
+            t0 = (-1 - o) * inv
+            t1 = (1 - o) * inv
+            tn = np.minimum(t0, t1).max(-1)
+            tf = np.maximum(t0, t1).min(-1)
+            L = np.clip(tf - np.maximum(tn, 0), 0, None)
+            want[y, x] = np.exp(-0.8 * L).mean()
+    got = m[:, 0].reshape(H, W)
+    err = np.abs(got - want)
+    se = np.sqrt(var[:, 0] / n).reshape(H, W)
+    say(f"max |mean - exp(-sigma L)| {err.max():.4f}, median {np.median(err):.4f}")
+    # the per-pixel statistical bound plus the footprint-quadrature error
+    assert (err <= 4 * se + 0.01).mean() > 0.99
+    assert abs(got.mean() - want.mean()) < 3e-3
+
+
+def test_scattering_cloud_in_a_white_furnace(ctx):
+    """A conservative (sigma_a = 0) cloud lit by a unit dome with nothing
+    else in the scene scatters energy without loss: every pixel's
+    expectation is 1."""
+    say("white furnace with a pure-scattering fBm cloud, 64x64, 512 spp")
+    s = scenes.Scene(64, 64, 30.0, eye=(0, 0, -5), look_at=(0, 0, 0))
+    s.add_light(scenes.light(rrhip.LIGHT_DOME, intensity=1.0, color=(1, 1, 1)))
+    dens = rrhip.gen_density_grid(32, 2, frequency=2.0, octaves=4, coverage=0.6, sharpness=3.0)
+    s.atmosphere = s.add_medium((0, 0, 0), (3.0, 3.0, 3.0), 0.3, density=dens)
+    acc = render(ctx, s, rrhip.RenderParams.make(max_depth=64, firefly_clamp=0.0), 0, 512)
+    m, var, n = mean_var(acc)
+    mean = m.mean()
+    se = np.sqrt(var.mean() / n.mean() / m.size)
+    say(f"furnace mean {mean:.5f} (standard error {se:.5f})")
+    assert abs(mean - 1.0) < max(4 * se, 2e-3)
+
+
+def test_media_rejections(ctx):
+    s = scenes.cloud_room(16, 16, n=8)
+    s.add_light(scenes.light(rrhip.LIGHT_POINT, intensity=1.0, color=(1, 1, 1), p=(0, 0.5, 0)))
+    with pytest.raises(rrhip.RrhipError) as e:
+        rrhip.PtScene(ctx, s.desc())
+    assert e.value.status == 5  # UNSUPPORTED: point lights in an atmosphere
+    s2 = scenes.cloud_room(16, 16, n=8)
+    ps = rrhip.PtScene(ctx, s2.desc())
+    se = ps.session(rrhip.RenderParams.make(max_depth=16))
+    with pytest.raises(rrhip.RrhipError):
+        se.set_nrc(rrhip.Nrc(ctx, rrhip.NrcConfig.make(log2_table=12, train_paths=64)))
+    se.close()
+    ps.close()

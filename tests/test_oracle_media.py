@@ -1,0 +1,135 @@
+"""Oracle participating media (medium.rs, pt/volume.rs restated) pinned by
+render.rs's own volume KATs (volume.rs:296-364, medium.rs:107-150), the
+Henyey-Greenstein closed forms, and the fBm cloud density formula."""
+import ctypes
+
+import numpy as np
+
+import orc
+import rrhip
+
+
+def medium(sigma_a, sigma_s, g=0.0, density=None, lo=(-1, -1, -1), hi=(1, 1, 1)):
+    m = rrhip.MediumDesc()
+    m.sigma_a = (ctypes.c_double * 3)(*[float(x) for x in np.broadcast_to(sigma_a, 3)])
+    m.sigma_s = (ctypes.c_double * 3)(*[float(x) for x in np.broadcast_to(sigma_s, 3)])
+    m.g = g
+    if density is not None:
+        dens = np.ascontiguousarray(density, np.float32)
+        m._keep = dens
+        m.nz, m.ny, m.nx = dens.shape
+        m.density = dens.ctypes.data
+        m.grid_lo = (ctypes.c_double * 3)(*lo)
+        m.grid_hi = (ctypes.c_double * 3)(*hi)
+    return m
+
+
+def test_hg_normalises_and_has_mean_cosine_g():
+    """medium.rs:73-101 evaluates hg_phase(g, -wo.wi), i.e. the textbook
+    Henyey-Greenstein function of cos(wo, wi) with wo pointing back along
+    the incoming ray: its integral is 1, hg_sample returns the phase value
+    as pdf, and the sampled wi has mean cosine g with wo (so positive g
+    scatters back towards the previous vertex, render.rs's convention,
+    which the kernels keep)."""
+    mu = np.linspace(-1, 1, 200001)
+    for g in (-0.6, 0.0, 0.3, 0.8):
+        vals = np.array([orc.hg_phase(g, x) for x in mu[::50]])
+        integral = np.trapezoid(vals, mu[::50]) * 2 * np.pi
+        assert abs(integral - 1.0) < 2e-4, (g, integral)
+        wo = np.array([0.0, 0.0, 1.0])
+        wi, pdf = orc.hg_sample(g, wo, 5, 7, 200000)
+        cos_wo = wi @ wo
+        want = [(1 - g * g) / (4 * np.pi * (1 + g * g - 2 * g * c) ** 1.5) for c in cos_wo[:100]]
+        assert np.allclose(pdf[:100], want, rtol=1e-9)
+        assert abs(cos_wo.mean() - g) < 0.01, (g, cos_wo.mean())
+        assert np.allclose(np.linalg.norm(wi, axis=1), 1.0)
+
+
+def test_homogeneous_distance_sampling_is_unbiased():
+    """volume.rs:296-317: sigma_t = 1, pass weight -> e^-1.3, scatter weight
+    -> 0.6 (1 - e^-1.3)."""
+    m = medium(0.4, 0.6)
+    sc, t, w = orc.medium_sample(m, (0, 0, 0), (0, 0, 1), 1.3, (1, 1, 1), 3, 7, 200000)
+    n = sc.size
+    pass_w = w[~sc].sum(0) / n
+    scat_w = w[sc].sum(0) / n
+    assert np.all(np.abs(pass_w - np.exp(-1.3)) < 0.01)
+    assert np.all(np.abs(scat_w - 0.6 * (1 - np.exp(-1.3))) < 0.01)
+    assert np.all(t[sc] < 1.3)
+
+
+def test_chromatic_homogeneous_sampling_is_unbiased_per_channel():
+    """Spectral one-sample MIS (volume.rs:60-102): each channel's expected
+    pass weight is its own transmittance."""
+    st = np.array([0.5, 1.0, 2.0])
+    m = medium(0.3 * st, 0.7 * st)
+    sc, t, w = orc.medium_sample(m, (0, 0, 0), (0, 0, 1), 0.9, (1.0, 0.5, 0.25), 11, 3, 300000)
+    pass_w = w[~sc].sum(0) / sc.size
+    assert np.all(np.abs(pass_w - np.exp(-st * 0.9)) < 0.01), pass_w
+
+
+def test_delta_tracking_matches_closed_form_in_constant_density():
+    """A density grid of ones over [-1,1]^3 is a homogeneous medium inside
+    the box: delta tracking from inside to the far wall must pass with
+    probability e^{-sigma_t L} and scatter with weight sigma_s/sigma_t."""
+    ones = np.ones((8, 8, 8), np.float32)
+    m = medium(0.5, 1.5, density=ones)
+    L = 1.0 + 0.3  # from z = -0.3 to the box face at z = 1 (t_max beyond it)
+    sc, t, w = orc.medium_sample(m, (0.1, 0.2, -0.3), (0, 0, 1), 5.0, (1, 1, 1), 9, 1, 200000)
+    assert abs((~sc).mean() - np.exp(-2.0 * L)) < 0.01
+    assert np.allclose(w[sc], 0.75)
+    assert np.all(w[~sc] == 1.0) and np.all(t[sc] < L)
+
+
+def test_ratio_tracking_matches_quadrature():
+    """volume.rs:324-364 on the voxel fBm cloud: ratio-tracked
+    transmittance vs exp(-Riemann optical depth)."""
+    dens = rrhip.gen_density_grid(32, 5, frequency=0.8 * 2, octaves=3, coverage=0.6, sharpness=3.0,
+                                  lo=(-2, -2, -2), hi=(2, 2, 2))
+    m = medium(0.3, 0.7, density=dens, lo=(-2, -2, -2), hi=(2, 2, 2))
+    o = np.array([0.3, 0.2, -1.0])
+    d = np.array([0.2, 0.1, 1.0])
+    d /= np.linalg.norm(d)
+    dist = 2.5
+    steps = 20000
+    ts = (np.arange(steps) + 0.5) / steps * dist
+    rho = orc.medium_density(m, o[None, :] + ts[:, None] * d[None, :])
+    expected = np.exp(-(rho.sum() * dist / steps))
+    tr = orc.medium_transmittance(m, o, d, dist, 11, 4, 30000)
+    assert 0.05 < expected < 0.95
+    assert abs(tr[:, 0].mean() - expected) < 0.02, (tr[:, 0].mean(), expected)
+
+
+def test_density_is_trilinear_and_zero_outside():
+    rng = np.random.default_rng(1)
+    grid = rng.uniform(0, 1, (5, 4, 3)).astype(np.float32)  # nz=5, ny=4, nx=3
+    m = medium(0.1, 0.1, density=grid, lo=(0, 0, 0), hi=(2, 3, 4))
+    # vertices reproduce the samples
+    for (k, j, i) in [(0, 0, 0), (4, 3, 2), (2, 1, 1), (3, 2, 0)]:
+        p = (2 * i / 2, 3 * j / 3, 4 * k / 4)
+        assert abs(orc.medium_density(m, p)[0] - grid[k, j, i]) < 1e-7
+    # a linear field is reproduced exactly between vertices
+    zz, yy, xx = np.meshgrid(np.arange(5), np.arange(4), np.arange(3), indexing="ij")
+    lin = (0.1 * xx + 0.05 * yy + 0.02 * zz).astype(np.float32)
+    ml = medium(0.1, 0.1, density=lin, lo=(0, 0, 0), hi=(2, 3, 4))
+    pts = rng.uniform((0, 0, 0), (2, 3, 4), (200, 3))
+    want = 0.1 * pts[:, 0] / 1.0 + 0.05 * pts[:, 1] / 1.0 + 0.02 * pts[:, 2] / 1.0
+    assert np.abs(orc.medium_density(ml, pts) - want).max() < 1e-6
+    assert orc.medium_density(ml, (2.01, 1, 1))[0] == 0.0
+    assert orc.medium_density(ml, (-0.01, 1, 1))[0] == 0.0
+
+
+def test_cloud_density_grid_follows_the_fbm_formula():
+    """rrhip_gen_density_grid = medium.rs:55-63 on the grid vertices, with the
+    oracle's fBm (displace.rs) and the seeded offset."""
+    n, seed = 9, 3
+    grid = rrhip.gen_density_grid(n, seed, frequency=2.0, octaves=4, coverage=0.5, sharpness=3.0)
+    u = orc.pcg32_seq(seed, 0x5eed, 3).astype(np.float64) / 2 ** 32
+    off = 1000.0 * u
+    idx = np.array([(0, 0, 0), (8, 8, 8), (3, 5, 1), (7, 2, 4)])
+    pts = -1.0 + 2.0 * idx / (n - 1)
+    f = orc.fbm(pts, frequency=2.0, octaves=4, gain=0.5, lacunarity=2.0, offset=off)
+    want = np.clip((f * 0.5 + 0.5 - 0.5) * 3.0, 0, 1)
+    got = grid[idx[:, 2], idx[:, 1], idx[:, 0]]
+    assert np.abs(got - want).max() < 1e-6
+    assert grid.min() >= 0 and grid.max() <= 1 and 0.05 < (grid > 0).mean() < 0.95
