@@ -10,8 +10,10 @@ the sphere, tmax = inf. Rays, BVH and hits stay resident in HBM; the
 timed region is K back-to-back closest-hit launches.
 
 Prints one JSON line (value = whole-job Mrays/s at N=1). The line also
-carries the box-room path-tracing throughput and the time to relMSE 0.01
-at 1080p of the radiance cache vs plain wavefront path tracing.
+carries the box-room path-tracing throughput, the batched-jobs comparison
+(32 rooms, half with clouds: one launch sequence vs 32 sequential
+sessions) and the time to relMSE 0.01 at 1080p of the radiance cache vs
+plain wavefront path tracing.
 """
 import argparse
 import json
@@ -182,6 +184,76 @@ def time_to_relmse(ctx, args, with_cpu):
     return out
 
 
+def batched_jobs(ctx, args, with_cpu):
+    """32 heterogeneous room jobs (1-40 spheres, 64^2..512^2, 4-64 spp,
+    every other one filled with a 128^3 fBm cloud) rendered (a) one after
+    another, each by its own session, and (b) as one batch whose passes
+    advance every unfinished job through a single launch sequence.
+    Completion times are per job from the start of the whole workload."""
+    import numpy as np
+    import rrhip
+    import scenes
+    jobs = scenes.batch_rooms(32, seed=1)
+    pss = [rrhip.PtScene(ctx, s.desc()) for s, _, _ in jobs]
+    prm = [rrhip.RenderParams.make(max_depth=d) for _, _, d in jobs]
+    # warm-up (first launches of each kernel)
+    w = pss[0].session(prm[0])
+    w.render(0, 1)
+    w.close()
+    seq_done, elapsed, seq_rays = [], 0.0, 0
+    for ps, p, (_, spp, _) in zip(pss, prm, jobs):
+        se = ps.session(p)
+        t0 = time.perf_counter()
+        se.render(0, spp)
+        elapsed += time.perf_counter() - t0
+        seq_done.append(elapsed * 1e3)
+        st = se.stats()
+        seq_rays += st["extension_rays"] + st["shadow_rays"]
+        se.close()
+    b = rrhip.Batch(ctx, [(ps, spp, p) for ps, p, (_, spp, _) in zip(pss, prm, jobs)])
+    b.render()  # warm-up
+    t0 = time.perf_counter()
+    job_ms, total_ms = b.render()
+    wall_ms = (time.perf_counter() - t0) * 1e3
+    bst = b.stats()
+    brays = bst["extension_rays"] + bst["shadow_rays"]
+
+    def pct(v, q):
+        return round(float(np.percentile(v, q)), 2)
+
+    out = {
+        "workload": "32 box rooms: 1-40 spheres r~U(0.05,0.3), 64^2-512^2, 4-64 spp, half with a 128^3 fBm cloud "
+                    "(delta tracking, HG g 0.3, depth 64), others depth 8",
+        "samples": int(sum(s.npix * spp for s, spp, _ in jobs)),
+        "sequential": {"total_ms": round(elapsed * 1e3, 2), "mrays_per_s": round(seq_rays / elapsed / 1e6, 1),
+                       "p50_ms": pct(seq_done, 50), "p99_ms": pct(seq_done, 99)},
+        "batched": {"total_ms": round(wall_ms, 2), "device_ms": round(total_ms, 2),
+                    "mrays_per_s": round(brays / (wall_ms / 1e3) / 1e6, 1), "p50_ms": pct(job_ms, 50),
+                    "p99_ms": pct(job_ms, 99), "passes": bst["passes"], "samples_per_pass": bst["samples_per_pass"]},
+        "rays": int(brays),
+    }
+    b.close()
+    if with_cpu:
+        # The oracle renders every job at 1 spp; the sequential CPU time is
+        # that sum scaled by each job's spp (cost is linear in samples).
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import orc
+        threads = max(1, min(16, os.cpu_count() or 1))
+        est, sample_s = 0.0, 0.0
+        for (s, spp, d) in jobs:
+            t1 = time.perf_counter()
+            orc.render(s.desc(), rrhip.RenderParams.make(max_depth=d), 0, 1, threads=threads)
+            dt = time.perf_counter() - t1
+            sample_s += dt
+            est += dt * spp
+        out["cpu_baseline"] = {"value": round(est, 1), "unit": "s (32 sequential oracle renders, extrapolated)",
+                               "cores": threads, "kind": "port",
+                               "sample": f"every job at 1 spp ({sample_s:.1f} s), scaled by its spp"}
+    for ps in pss:
+        ps.close()
+    return out
+
+
 def copy_calibration(ctx, args):
     """K launches of the plain copy kernel over 512 MiB (reads and writes
     exactly that much per launch)."""
@@ -216,6 +288,7 @@ def main():
                     help="skip the counting pass (profiling runs)")
     ap.add_argument("--no-pt", action="store_true", help="skip the box-room path-tracing line")
     ap.add_argument("--no-relmse", action="store_true", help="skip the time-to-relMSE comparison")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs comparison")
     ap.add_argument("--ref-spp", type=int, default=4096, help="spp of the relMSE reference render")
     ap.add_argument("--max-frames", type=int, default=400, help="frame cap of the time-to-relMSE runs")
     ap.add_argument("--mode", choices=("trace", "copy"), default="trace",
@@ -309,6 +382,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(pos, idx, rays)
     if not args.no_pt:
         out["path_tracing"] = box_room(ctx, args, not args.no_cpu_baseline)
+    if not args.no_batch:
+        out["batched_jobs"] = batched_jobs(ctx, args, not args.no_cpu_baseline)
     if not args.no_relmse:
         out["time_to_relmse"] = time_to_relmse(ctx, args, not args.no_cpu_baseline)
     for p in (d_rays, d_hits, d_occ):

@@ -44,6 +44,7 @@ typedef struct rrhip_ctx rrhip_ctx;
 typedef struct rrhip_scene rrhip_scene;
 typedef struct rrhip_session rrhip_session;
 typedef struct rrhip_nrc rrhip_nrc;
+typedef struct rrhip_batch rrhip_batch;
 
 /* Query ray, 32 bytes. render.rs's GPU ray query takes [f32; 6]
  * (origin, direction) and tests t > 1e-6 (src/raytracer/metal/renderer.rs:
@@ -451,6 +452,34 @@ int rrhip_nrc_gradients(rrhip_nrc* nrc, const rrhip_nrc_query* queries, const fl
  * it online on the suffixes of extra training paths. Cache and session
  * must share a context. */
 int rrhip_session_set_nrc(rrhip_session* session, rrhip_nrc* nrc);
+
+/* ---- batched jobs ---------------------------------------------------------- */
+/* Many independent renders (different scenes, resolutions, sample counts,
+ * media) advanced together: every pass runs one raygen, one extend /
+ * shade / shadow per bounce and one resolve over the path slots of all
+ * unfinished jobs, instead of one launch sequence per job. No counterpart
+ * in render.rs, which renders one scene per process; each job's
+ * accumulator equals that of a session with the batch's
+ * samples_per_pass. */
+typedef struct rrhip_batch_job {
+  rrhip_scene* scene;          /* a path-tracing scene (camera set) */
+  uint32_t spp;                /* samples of every pixel */
+  uint32_t reserved;
+  rrhip_render_params params;  /* max_depth, rr_start, firefly_clamp, sample_base (samples_per_pass ignored) */
+} rrhip_batch_job;
+
+/* samples_per_pass = 0 picks max(1, min(64, 2^22 / total pixels)). */
+int rrhip_batch_create(rrhip_ctx* ctx, uint32_t num_jobs, const rrhip_batch_job* jobs, uint32_t samples_per_pass,
+                       rrhip_batch** out);
+int rrhip_batch_destroy(rrhip_batch* batch);
+/* Renders every job to completion (blocking, accumulators cleared
+ * first). job_ms (num_jobs entries, may be NULL) receives each job's
+ * completion time from the start, measured with device events at pass
+ * granularity; total_ms the whole batch. */
+int rrhip_batch_render(rrhip_batch* batch, double* job_ms, double* total_ms);
+int rrhip_batch_read_accum(rrhip_batch* batch, uint32_t job, double* out);
+/* Rays traced by the last render (extension, shadow), passes, samples_per_pass. */
+int rrhip_batch_get_stats(rrhip_batch* batch, rrhip_session_stats* stats);
 
 /* ---- timing ------------------------------------------------------------- */
 /* Enqueues `reps` back-to-back closest-hit (any == 0) or any-hit launches

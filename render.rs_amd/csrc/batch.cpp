@@ -1,0 +1,232 @@
+// Batched path-tracing jobs (C ABI): many scenes advanced through one
+// launch sequence per pass (pt_launch_batch_pass).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "api_internal.h"
+#include "pt_kernels.h"
+#include "scene_internal.h"
+
+using namespace rrhip;
+
+struct rrhip_batch {
+  rrhip_ctx* ctx = nullptr;
+  std::vector<rrhip_batch_job> jobs;
+  std::vector<uint32_t> npix;
+  uint32_t spp_pass = 1;
+  uint64_t cap = 0;  // path slots
+  uint64_t total_pix = 0;
+  PathBuffers P{};
+  std::vector<void*> mem;
+  std::vector<double*> accum;
+  PtJob* d_jobs = nullptr;  // one table per pass, each `jobs.size()` entries
+  uint32_t passes = 0;
+  unsigned long long last_rays[2] = {0, 0};
+};
+
+namespace {
+
+template <class T>
+hipError_t alloc_n(rrhip_batch* b, T** p, uint64_t n) {
+  void* v = nullptr;
+  hipError_t e = hipMalloc(&v, std::max<uint64_t>(n, 1) * sizeof(T));
+  if (e == hipSuccess) b->mem.push_back(v);
+  *p = (T*)v;
+  return e;
+}
+
+void free_batch(rrhip_batch* b) {
+  for (void* p : b->mem) (void)hipFree(p);
+  delete b;
+}
+
+// The pass schedule: for pass k, the jobs still rendering, with their
+// sample count, first sample, slot and pixel offsets.
+std::vector<std::vector<PtJob>> schedule(const rrhip_batch* b) {
+  std::vector<std::vector<PtJob>> out;
+  for (uint32_t k = 0;; k++) {
+    std::vector<PtJob> pass;
+    uint32_t slot = 0, pix = 0;
+    for (size_t j = 0; j < b->jobs.size(); j++) {
+      const rrhip_batch_job& J = b->jobs[j];
+      const uint64_t done = (uint64_t)k * b->spp_pass;
+      if (done >= J.spp) continue;
+      PtJob t{};
+      t.S = J.scene->pt;
+      t.prm.max_depth = J.params.max_depth;
+      t.prm.rr_start = J.params.rr_start;
+      t.prm.firefly_clamp = (float)J.params.firefly_clamp;
+      t.prm.npix = b->npix[j];
+      t.prm.spp_pass = (uint32_t)std::min<uint64_t>(b->spp_pass, J.spp - done);
+      t.prm.sample0 = J.params.sample_base + done;
+      t.slot_begin = slot;
+      t.pix_begin = pix;
+      t.accum = b->accum[j];
+      slot += t.prm.spp_pass * t.prm.npix;
+      pix += t.prm.npix;
+      pass.push_back(t);
+    }
+    if (pass.empty()) break;
+    out.push_back(std::move(pass));
+  }
+  return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rrhip_batch_create(rrhip_ctx* ctx, uint32_t num_jobs, const rrhip_batch_job* jobs, uint32_t samples_per_pass,
+                       rrhip_batch** out) {
+  if (!ctx || !jobs || !out || num_jobs == 0) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_batch_create: null argument or no jobs");
+  *out = nullptr;
+  uint64_t total_pix = 0;
+  for (uint32_t j = 0; j < num_jobs; j++) {
+    const rrhip_batch_job& J = jobs[j];
+    if (!J.scene || !J.scene->has_pt) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_batch_create: job without a path-tracing scene");
+    if (J.scene->ctx != ctx) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_batch_create: job scene from another context");
+    if (J.params.struct_size != sizeof(rrhip_render_params))
+      return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_batch_create: render params struct_size mismatch");
+    if (J.params.max_depth == 0 || J.params.max_depth > 255)
+      return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_batch_create: max_depth must be in 1..255");
+    if (J.spp == 0) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_batch_create: spp must be > 0");
+    total_pix += (uint64_t)J.scene->pt.cam.width * J.scene->pt.cam.height;
+  }
+  uint32_t spp = samples_per_pass;
+  if (spp == 0) spp = (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(1, (4ull << 20) / total_pix));
+  if ((uint64_t)spp * total_pix >= (1ull << 31))
+    return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_batch_create: samples_per_pass x pixels exceeds 2^31");
+  RRHIP_HIP_TRY(hipSetDevice(ctx->device));
+  auto* b = new (std::nothrow) rrhip_batch();
+  if (!b) return fail(RRHIP_ERR_OUT_OF_MEMORY, "rrhip_batch_create: host allocation failed");
+  b->ctx = ctx;
+  b->jobs.assign(jobs, jobs + num_jobs);
+  b->spp_pass = spp;
+  b->total_pix = total_pix;
+  b->cap = (uint64_t)spp * total_pix;
+  hipError_t e = hipSuccess;
+  for (uint32_t j = 0; j < num_jobs && e == hipSuccess; j++) {
+    const uint32_t np = jobs[j].scene->pt.cam.width * jobs[j].scene->pt.cam.height;
+    b->npix.push_back(np);
+    double* a = nullptr;
+    e = alloc_n(b, &a, (uint64_t)np * RRHIP_ACCUM_STRIDE);
+    b->accum.push_back(a);
+  }
+  PathBuffers& P = b->P;
+  const uint64_t n = b->cap;
+  if (e == hipSuccess) e = alloc_n(b, &P.o, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.d, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.beta, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.L, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.prev, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.rng, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.flags, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.hit_n, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.hit_m, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.q0, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.q1, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.sq, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.so, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.sd, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.sc, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.job, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.counts, 8);
+  if (e == hipSuccess) e = alloc_n(b, &P.stats, 2);
+  // The whole schedule's job tables, uploaded once.
+  std::vector<std::vector<PtJob>> sched;
+  if (e == hipSuccess) {
+    sched = schedule(b);
+    b->passes = (uint32_t)sched.size();
+    e = alloc_n(b, &b->d_jobs, (uint64_t)b->passes * num_jobs);
+  }
+  if (e == hipSuccess) {
+    std::vector<PtJob> flat((size_t)b->passes * num_jobs);
+    for (uint32_t k = 0; k < b->passes; k++) std::copy(sched[k].begin(), sched[k].end(), flat.begin() + (size_t)k * num_jobs);
+    e = hipMemcpy(b->d_jobs, flat.data(), flat.size() * sizeof(PtJob), hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess) {
+    free_batch(b);
+    return fail_hip(e, "rrhip_batch_create");
+  }
+  *out = b;
+  return RRHIP_OK;
+}
+
+int rrhip_batch_destroy(rrhip_batch* b) {
+  if (!b) return RRHIP_OK;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  free_batch(b);
+  return RRHIP_OK;
+}
+
+int rrhip_batch_render(rrhip_batch* b, double* job_ms, double* total_ms) {
+  if (!b) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_batch_render: null batch");
+  RRHIP_HIP_TRY(hipSetDevice(b->ctx->device));
+  hipStream_t st = b->ctx->stream;
+  const std::vector<std::vector<PtJob>> sched = schedule(b);
+  const uint32_t nj = (uint32_t)b->jobs.size();
+  for (uint32_t j = 0; j < nj; j++)
+    RRHIP_HIP_TRY(hipMemsetAsync(b->accum[j], 0, (size_t)b->npix[j] * RRHIP_ACCUM_STRIDE * sizeof(double), st));
+  RRHIP_HIP_TRY(hipMemsetAsync(b->P.stats, 0, 2 * sizeof(unsigned long long), st));
+  std::vector<hipEvent_t> ev(sched.size() + 1, nullptr);
+  for (auto& x : ev) RRHIP_HIP_TRY(hipEventCreate(&x));
+  std::vector<uint32_t> last_pass(nj, 0);
+  int status = RRHIP_OK;
+  hipError_t e = hipEventRecord(ev[0], st);
+  for (size_t k = 0; k < sched.size() && e == hipSuccess; k++) {
+    const std::vector<PtJob>& pass = sched[k];
+    uint32_t total = 0, pix = 0, depth = 0;
+    for (const PtJob& t : pass) {
+      total += t.prm.spp_pass * t.prm.npix;
+      pix += t.prm.npix;
+      depth = std::max(depth, t.prm.max_depth);
+    }
+    e = pt_launch_batch_pass(b->d_jobs + k * nj, (uint32_t)pass.size(), b->P, total, pix, depth, st);
+    if (e == hipSuccess) e = hipEventRecord(ev[k + 1], st);
+    for (const PtJob& t : pass) {
+      const size_t j = (size_t)(std::find(b->accum.begin(), b->accum.end(), t.accum) - b->accum.begin());
+      last_pass[j] = (uint32_t)k + 1;
+    }
+  }
+  if (e == hipSuccess) e = hipEventSynchronize(ev.back());
+  if (e == hipSuccess) {
+    float ms = 0.f;
+    for (uint32_t j = 0; j < nj && e == hipSuccess; j++) {
+      e = hipEventElapsedTime(&ms, ev[0], ev[last_pass[j]]);
+      if (job_ms) job_ms[j] = ms;
+    }
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev.back());
+    if (total_ms) *total_ms = ms;
+  }
+  if (e == hipSuccess) e = hipMemcpy(b->last_rays, b->P.stats, sizeof(b->last_rays), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) status = fail_hip(e, "rrhip_batch_render");
+  for (auto& x : ev) (void)hipEventDestroy(x);
+  return status;
+}
+
+int rrhip_batch_read_accum(rrhip_batch* b, uint32_t job, double* out) {
+  if (!b || !out) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_batch_read_accum: null argument");
+  if (job >= b->jobs.size()) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_batch_read_accum: job out of range");
+  RRHIP_HIP_TRY(hipSetDevice(b->ctx->device));
+  RRHIP_HIP_TRY(hipMemcpyAsync(out, b->accum[job], (size_t)b->npix[job] * RRHIP_ACCUM_STRIDE * sizeof(double),
+                               hipMemcpyDeviceToHost, b->ctx->stream));
+  RRHIP_HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  return RRHIP_OK;
+}
+
+int rrhip_batch_get_stats(rrhip_batch* b, rrhip_session_stats* out) {
+  if (!b || !out) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_batch_get_stats: null argument");
+  *out = rrhip_session_stats{};
+  out->extension_rays = b->last_rays[0];
+  out->shadow_rays = b->last_rays[1];
+  out->path_slots = b->cap;
+  out->samples_per_pass = b->spp_pass;
+  out->reserved = b->passes;
+  return RRHIP_OK;
+}
+
+}  // extern "C"

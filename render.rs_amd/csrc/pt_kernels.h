@@ -21,6 +21,12 @@ hipError_t pt_launch_raygen(const PtSceneView& S, const PathBuffers& P, const Pt
 hipError_t pt_launch_bounce(const PtSceneView& S, const PathBuffers& P, const PtParams& prm,
                             const uint32_t* q, uint32_t* qnext, uint32_t capacity, const NrcPathBuffers& N,
                             hipStream_t st);
+// One pass of a batch: raygen over every job's slots (jobs sorted by
+// slot_begin and pix_begin), `bounces` rounds of extend / shade / shadow
+// with each path reading its own job's scene and parameters, and the
+// resolve of every job's pixels into its accumulator.
+hipError_t pt_launch_batch_pass(const PtJob* jobs, uint32_t nj, const PathBuffers& P, uint32_t total,
+                                 uint32_t total_pix, uint32_t bounces, hipStream_t st);
 // Adds each pixel's pass samples (firefly-clamped) into the f64
 // accumulator in sample order (deterministic, no atomics).
 hipError_t pt_launch_resolve(const PathBuffers& P, const PtParams& prm, double* accum, hipStream_t st);

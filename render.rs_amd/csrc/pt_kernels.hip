@@ -61,6 +61,24 @@ __device__ __forceinline__ uint4 store_rng(const Pcg32& r) {
   return make_uint4((uint32_t)r.state, (uint32_t)(r.state >> 32), (uint32_t)r.inc, (uint32_t)(r.inc >> 32));
 }
 
+// Camera ray of (pixel, sample) into path slot `slot`.
+__device__ __forceinline__ void init_path(const PtSceneView& S, const PathBuffers& P, uint32_t slot, uint32_t pixel,
+                                          uint64_t sample, uint32_t flags) {
+  const uint32_t x = pixel % S.cam.width, y = pixel / S.cam.width;
+  // sampler.rs:24-32 per-(pixel, sample) stream; pt/mod.rs:340-356
+  Pcg32 rng = Pcg32::for_pixel_sample(pixel, sample);
+  V3 o, d;
+  camera_ray(S.cam, x, y, rng, o, d);
+  P.o[slot] = make_float4(o.x, o.y, o.z, 0.f);
+  P.d[slot] = make_float4(d.x, d.y, d.z, 0.f);
+  P.beta[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+  P.L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+  P.prev[slot] = make_float4(o.x, o.y, o.z, 0.f);
+  P.rng[slot] = store_rng(rng);
+  P.flags[slot] = flags;
+  P.q0[slot] = slot;
+}
+
 __global__ __launch_bounds__(kBlock) void raygen_kernel(PtSceneView S, PathBuffers P, PtParams prm, uint32_t total,
                                                        NrcPathBuffers N) {
   const uint32_t all = total + N.ntrain;
@@ -90,19 +108,20 @@ __global__ __launch_bounds__(kBlock) void raygen_kernel(PtSceneView S, PathBuffe
       flags |= kFlagTrain;
       N.tn[i] = 0;
     }
-    const uint32_t x = pixel % S.cam.width, y = pixel / S.cam.width;
-    // sampler.rs:24-32 per-(pixel, sample) stream; pt/mod.rs:340-356
-    Pcg32 rng = Pcg32::for_pixel_sample(pixel, sample);
-    V3 o, d;
-    camera_ray(S.cam, x, y, rng, o, d);
-    P.o[slot] = make_float4(o.x, o.y, o.z, 0.f);
-    P.d[slot] = make_float4(d.x, d.y, d.z, 0.f);
-    P.beta[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
-    P.L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
-    P.prev[slot] = make_float4(o.x, o.y, o.z, 0.f);
-    P.rng[slot] = store_rng(rng);
-    P.flags[slot] = flags;
-    P.q0[slot] = slot;
+    init_path(S, P, slot, pixel, sample, flags);
+  }
+}
+
+// Closest hit of the path in `slot` into hit_n / hit_m.
+__device__ __forceinline__ void extend_one(const PtSceneView& S, const PathBuffers& P, uint32_t slot, uint32_t* stk) {
+  const float4 o4 = P.o[slot], d4 = P.d[slot];
+  Hit h;
+  if (scene_closest<kTraceBlock>(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), stk, h)) {
+    P.hit_n[slot] = make_float4(h.n.x, h.n.y, h.n.z, h.t);
+    P.hit_m[slot] = h.material | (h.front ? 0x80000000u : 0u);
+  } else {
+    P.hit_n[slot] = make_float4(0.f, 0.f, 0.f, -1.f);
+    P.hit_m[slot] = 0;
   }
 }
 
@@ -111,18 +130,7 @@ __global__ __launch_bounds__(kTraceBlock) void extend_kernel(PtSceneView S, Path
   uint32_t* stk = s_stack + threadIdx.x;
   const uint32_t n = P.counts[0];
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const uint32_t slot = q[i];
-    const float4 o4 = P.o[slot], d4 = P.d[slot];
-    Hit h;
-    if (scene_closest<kTraceBlock>(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), stk, h)) {
-      P.hit_n[slot] = make_float4(h.n.x, h.n.y, h.n.z, h.t);
-      P.hit_m[slot] = h.material | (h.front ? 0x80000000u : 0u);
-    } else {
-      P.hit_n[slot] = make_float4(0.f, 0.f, 0.f, -1.f);
-      P.hit_m[slot] = 0;
-    }
-  }
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) extend_one(S, P, q[i], stk);
 }
 
 // pt/mod.rs:1000-1023
@@ -285,6 +293,207 @@ __device__ __forceinline__ bool sample_light(const PtSceneView& S, const PtLight
 // neural radiance caching for path tracing", sec. 4): a(x1..xn) =
 // (sum_{i=2..n} sqrt(|x_{i-1} - x_i|^2 / (p(w_i) |cos theta_i|)))^2 against
 // c * a0, a0 = |x0 - x1|^2 / (4 pi |cos theta_1|).
+// One path's shading step (the per-lane part of shade_kernel): sets
+// cont (continue in the next queue), shadow (+ so), query (+ nq, qbeta).
+template <bool NRC>
+__device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffers& P, const PtParams& prm,
+                                          const NrcPathBuffers& N, uint32_t slot, bool& cont, bool& shadow,
+                                          bool& query, ShadowOut& so, NrcQuery& nq, V3& qbeta) {
+    const float4 o4 = P.o[slot], d4 = P.d[slot], b4 = P.beta[slot], l4 = P.L[slot], pv4 = P.prev[slot];
+    const float4 hn = P.hit_n[slot];
+    uint32_t flags = P.flags[slot];
+    Pcg32 rng = load_rng(P.rng[slot]);
+    V3 beta = v3(b4.x, b4.y, b4.z), Lr = v3(l4.x, l4.y, l4.z);
+    const V3 ro = v3(o4.x, o4.y, o4.z), rd = v3(d4.x, d4.y, d4.z);
+    const float prev_pdf = o4.w;
+    const V3 prev_origin = v3(pv4.x, pv4.y, pv4.z);
+    const bool from_camera = (flags & kFlagFromCamera) != 0;
+    uint32_t depth = flags & 0xFFu;
+    float a0 = d4.w, asum = pv4.w;
+    // Atmosphere along this segment (pt/mod.rs:486-633): one light pick
+    // per segment, distance sampling, and at a real collision NEE with
+    // the phase function and an HG continuation instead of the surface.
+    bool in_medium = false, absorbed = false;
+    if (S.atmosphere >= 0) {
+      const PtMedium& md = S.media[S.atmosphere];
+      const float t_limit = hn.w < 0.f ? 1e8f : hn.w;
+      uint32_t seg_li = 0;
+      float seg_pick = 0.f;
+      bool has_pick = false;
+      if (S.num_lights > 0)
+        has_pick = ls_sample(S, ro + rd * fminf(fminf(t_limit, md.max_distance) * 0.5f, 50.f), rnd(rng), seg_li,
+                             seg_pick);
+      const MediumEvent ev = sample_distance(md, ro, rd, t_limit, beta, rng);
+      beta = beta * ev.weight;
+      if (ev.scatter) {
+        in_medium = true;
+        const V3 p = ro + rd * ev.t;
+        if (maxc(v3(md.emission)) > 0.f) Lr = Lr + beta * v3(md.emission);
+        const V3 wo = -rd;
+        if (has_pick) {
+          const auto eval = [&](V3 wi, V3& f, float& pdf) {
+            const float ph = hg_phase(md.g, -dot(wo, wi));
+            f = v3(ph, ph, ph);
+            pdf = ph;
+          };
+          if (sample_light(S, S.lights[seg_li], p, v3(0.f, 0.f, 0.f), eval, seg_pick, rng, so)) {
+            so.c = beta * so.c * medium_transmittance(md, so.o, so.d, so.tmax, rng);
+            shadow = maxc(so.c) > 0.f;
+          }
+        }
+        float pdf;
+        const float u1 = rnd(rng), u2 = rnd(rng);
+        const V3 wi = hg_sample(md.g, wo, u1, u2, pdf);
+        depth++;
+        bool alive = true;
+        if (depth >= prm.rr_start) {
+          const float qq = fminf(maxc(beta), 0.95f);
+          if (qq <= 0.f || rnd(rng) > qq) {
+            alive = false;
+          } else {
+            beta = beta / qq;
+          }
+        }
+        cont = alive && depth < prm.max_depth;
+        P.o[slot] = make_float4(p.x, p.y, p.z, pdf);
+        P.d[slot] = make_float4(wi.x, wi.y, wi.z, a0);
+        P.prev[slot] = make_float4(p.x, p.y, p.z, asum);
+        flags = depth | (flags & (kFlagTrain | kFlagSuffix));
+      } else {
+        absorbed = !(maxc(beta) > 0.f);
+      }
+    }
+    if (in_medium || absorbed) {
+      // handled above
+    } else if (hn.w < 0.f) {
+      // Miss: dome (with MIS) or background (pt/mod.rs:643-659).
+      if (S.dome >= 0) {
+        float w = 1.f;
+        if (!from_camera) w = power_heuristic(prev_pdf, kInv4Pi * ls_pmf(S, prev_origin, (uint32_t)S.dome));
+        Lr = Lr + beta * v3(S.lights[S.dome].radiance) * w;
+      } else {
+        Lr = Lr + beta * v3(S.background);
+      }
+    } else {
+      const uint32_t hm = P.hit_m[slot];
+      const PtMaterial m = S.materials[hm & 0x7FFFFFFFu];
+      const bool entering = (hm >> 31) != 0;
+      const V3 p = ro + rd * hn.w;
+      const V3 ng = v3(hn.x, hn.y, hn.z);
+      const V3 wo = -rd;
+      const V3 nrm = dot(ng, wo) >= 0.f ? ng : -ng;
+      const float eta = entering ? m.glass_ior : 1.f / m.glass_ior;
+      // Emitter hit (pt/mod.rs:723-736)
+      const V3 em = v3(m.emission);
+      if (maxc(em) > 0.f) {
+        float w = 1.f;
+        if (!from_camera && m.area_light >= 0) {
+          const PtLight& L = S.lights[m.area_light];
+          const float pick = ls_pmf(S, prev_origin, (uint32_t)m.area_light);
+          w = power_heuristic(prev_pdf, light_pdf_solid_angle(L, prev_origin, p) * pick);
+        }
+        Lr = Lr + beta * em * w;
+      }
+      bool scatter = true;
+      if constexpr (NRC) {
+        const float cosv = fmaxf(fabsf(dot(ng, rd)), 1e-4f);
+        const float t2 = hn.w * hn.w;
+        if (from_camera) {
+          a0 = t2 / (4.f * kPi * cosv);
+          asum = 0.f;
+        } else {
+          asum += sqrtf(t2 / fmaxf(prev_pdf * cosv, 1e-30f));
+        }
+        const bool wide = asum * asum > N.spread * a0;
+        const bool train = (flags & kFlagTrain) != 0;
+        bool record = false;
+        if (!train) {
+          query = wide && m.has_lobes;
+        } else if (!(flags & kFlagSuffix)) {
+          if (wide) {
+            flags |= kFlagSuffix;
+            asum = 0.f;
+          }
+          record = m.has_lobes != 0;
+        } else {
+          query = wide && m.has_lobes;
+          record = !query && m.has_lobes;
+        }
+        if (query || record) {
+          nq.pos[0] = p.x;
+          nq.pos[1] = p.y;
+          nq.pos[2] = p.z;
+          nq.roughness = m.nrc_rough;
+          nq.dir[0] = wo.x;
+          nq.dir[1] = wo.y;
+          nq.dir[2] = wo.z;
+          nq.pad0 = 0.f;
+          nq.normal[0] = nrm.x;
+          nq.normal[1] = nrm.y;
+          nq.normal[2] = nrm.z;
+          nq.pad1 = 0.f;
+          nq.albedo[0] = m.nrc_albedo[0];
+          nq.albedo[1] = m.nrc_albedo[1];
+          nq.albedo[2] = m.nrc_albedo[2];
+          nq.pad2 = 0.f;
+        }
+        if (record && depth < N.max_vertices) {
+          const uint32_t tp = slot - N.train_base;
+          const size_t r = (size_t)tp * N.max_vertices + depth;
+          N.tq[r] = nq;
+          N.tb[r] = make_float4(beta.x, beta.y, beta.z, 0.f);
+          N.tl[r] = make_float4(Lr.x, Lr.y, Lr.z, 0.f);
+          N.tn[tp] = depth + 1;
+        }
+        qbeta = beta;
+        scatter = !query;
+      }
+      const Frame fr = make_frame(nrm);
+      const V3 wo_l = to_local(fr, wo);
+      // NEE through the light BVH (pt/mod.rs:924-956)
+      if (scatter && S.num_lights > 0 && m.has_lobes) {
+        uint32_t li;
+        float pick;
+        if (ls_sample(S, p, rnd(rng), li, pick)) {
+          const auto eval = [&](V3 wi, V3& f, float& pdf) { eval_world(m, fr, wo_l, eta, wi, f, pdf); };
+          if (sample_light(S, S.lights[li], p, nrm, eval, pick, rng, so)) {
+            so.c = beta * so.c;
+            // shadow rays through the atmosphere carry its transmittance
+            if (S.atmosphere >= 0) so.c = so.c * medium_transmittance(S.media[S.atmosphere], so.o, so.d, so.tmax, rng);
+            shadow = maxc(so.c) > 0.f;
+          }
+        }
+      }
+      // BSDF continuation (pt/mod.rs:958-987)
+      BsdfSample bs;
+      if (scatter && bsdf_sample(m, wo_l, eta, rng, bs)) {
+        const V3 wi = to_world(fr, bs.wi);
+        beta = beta * bs.f * (fabsf(bs.wi.z) / bs.pdf);
+        const V3 org = p + nrm * (bs.transmitted ? -kRayOffset : kRayOffset);
+        const V3 nd = normalize(wi);
+        depth++;
+        bool alive = true;
+        if (depth >= prm.rr_start) {
+          const float qq = fminf(maxc(beta), 0.95f);
+          if (qq <= 0.f || rnd(rng) > qq) {
+            alive = false;
+          } else {
+            beta = beta / qq;
+          }
+        }
+        cont = alive && depth < prm.max_depth;
+        P.o[slot] = make_float4(org.x, org.y, org.z, bs.pdf);
+        P.d[slot] = make_float4(nd.x, nd.y, nd.z, a0);
+        P.prev[slot] = make_float4(p.x, p.y, p.z, asum);
+        flags = depth | (flags & (kFlagTrain | kFlagSuffix));  // from_camera cleared
+      }
+    }
+    P.beta[slot] = make_float4(beta.x, beta.y, beta.z, 0.f);
+    P.L[slot] = make_float4(Lr.x, Lr.y, Lr.z, 0.f);
+    P.rng[slot] = store_rng(rng);
+    P.flags[slot] = flags;
+}
+
 template <bool NRC>
 __global__ __launch_bounds__(kBlock) void shade_kernel(PtSceneView S, PathBuffers P, PtParams prm,
                                                       const uint32_t* __restrict__ q, uint32_t* __restrict__ qnext,
@@ -302,199 +511,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtSceneView S, PathBuffer
     uint32_t slot = 0;
     if (act) {
       slot = q[i];
-      const float4 o4 = P.o[slot], d4 = P.d[slot], b4 = P.beta[slot], l4 = P.L[slot], pv4 = P.prev[slot];
-      const float4 hn = P.hit_n[slot];
-      uint32_t flags = P.flags[slot];
-      Pcg32 rng = load_rng(P.rng[slot]);
-      V3 beta = v3(b4.x, b4.y, b4.z), Lr = v3(l4.x, l4.y, l4.z);
-      const V3 ro = v3(o4.x, o4.y, o4.z), rd = v3(d4.x, d4.y, d4.z);
-      const float prev_pdf = o4.w;
-      const V3 prev_origin = v3(pv4.x, pv4.y, pv4.z);
-      const bool from_camera = (flags & kFlagFromCamera) != 0;
-      uint32_t depth = flags & 0xFFu;
-      float a0 = d4.w, asum = pv4.w;
-      // Atmosphere along this segment (pt/mod.rs:486-633): one light pick
-      // per segment, distance sampling, and at a real collision NEE with
-      // the phase function and an HG continuation instead of the surface.
-      bool in_medium = false, absorbed = false;
-      if (S.atmosphere >= 0) {
-        const PtMedium& md = S.media[S.atmosphere];
-        const float t_limit = hn.w < 0.f ? 1e8f : hn.w;
-        uint32_t seg_li = 0;
-        float seg_pick = 0.f;
-        bool has_pick = false;
-        if (S.num_lights > 0)
-          has_pick = ls_sample(S, ro + rd * fminf(fminf(t_limit, md.max_distance) * 0.5f, 50.f), rnd(rng), seg_li,
-                               seg_pick);
-        const MediumEvent ev = sample_distance(md, ro, rd, t_limit, beta, rng);
-        beta = beta * ev.weight;
-        if (ev.scatter) {
-          in_medium = true;
-          const V3 p = ro + rd * ev.t;
-          if (maxc(v3(md.emission)) > 0.f) Lr = Lr + beta * v3(md.emission);
-          const V3 wo = -rd;
-          if (has_pick) {
-            const auto eval = [&](V3 wi, V3& f, float& pdf) {
-              const float ph = hg_phase(md.g, -dot(wo, wi));
-              f = v3(ph, ph, ph);
-              pdf = ph;
-            };
-            if (sample_light(S, S.lights[seg_li], p, v3(0.f, 0.f, 0.f), eval, seg_pick, rng, so)) {
-              so.c = beta * so.c * medium_transmittance(md, so.o, so.d, so.tmax, rng);
-              shadow = maxc(so.c) > 0.f;
-            }
-          }
-          float pdf;
-          const float u1 = rnd(rng), u2 = rnd(rng);
-          const V3 wi = hg_sample(md.g, wo, u1, u2, pdf);
-          depth++;
-          bool alive = true;
-          if (depth >= prm.rr_start) {
-            const float qq = fminf(maxc(beta), 0.95f);
-            if (qq <= 0.f || rnd(rng) > qq) {
-              alive = false;
-            } else {
-              beta = beta / qq;
-            }
-          }
-          cont = alive && depth < prm.max_depth;
-          P.o[slot] = make_float4(p.x, p.y, p.z, pdf);
-          P.d[slot] = make_float4(wi.x, wi.y, wi.z, a0);
-          P.prev[slot] = make_float4(p.x, p.y, p.z, asum);
-          flags = depth | (flags & (kFlagTrain | kFlagSuffix));
-        } else {
-          absorbed = !(maxc(beta) > 0.f);
-        }
-      }
-      if (in_medium || absorbed) {
-        // handled above
-      } else if (hn.w < 0.f) {
-        // Miss: dome (with MIS) or background (pt/mod.rs:643-659).
-        if (S.dome >= 0) {
-          float w = 1.f;
-          if (!from_camera) w = power_heuristic(prev_pdf, kInv4Pi * ls_pmf(S, prev_origin, (uint32_t)S.dome));
-          Lr = Lr + beta * v3(S.lights[S.dome].radiance) * w;
-        } else {
-          Lr = Lr + beta * v3(S.background);
-        }
-      } else {
-        const uint32_t hm = P.hit_m[slot];
-        const PtMaterial m = S.materials[hm & 0x7FFFFFFFu];
-        const bool entering = (hm >> 31) != 0;
-        const V3 p = ro + rd * hn.w;
-        const V3 ng = v3(hn.x, hn.y, hn.z);
-        const V3 wo = -rd;
-        const V3 nrm = dot(ng, wo) >= 0.f ? ng : -ng;
-        const float eta = entering ? m.glass_ior : 1.f / m.glass_ior;
-        // Emitter hit (pt/mod.rs:723-736)
-        const V3 em = v3(m.emission);
-        if (maxc(em) > 0.f) {
-          float w = 1.f;
-          if (!from_camera && m.area_light >= 0) {
-            const PtLight& L = S.lights[m.area_light];
-            const float pick = ls_pmf(S, prev_origin, (uint32_t)m.area_light);
-            w = power_heuristic(prev_pdf, light_pdf_solid_angle(L, prev_origin, p) * pick);
-          }
-          Lr = Lr + beta * em * w;
-        }
-        bool scatter = true;
-        if constexpr (NRC) {
-          const float cosv = fmaxf(fabsf(dot(ng, rd)), 1e-4f);
-          const float t2 = hn.w * hn.w;
-          if (from_camera) {
-            a0 = t2 / (4.f * kPi * cosv);
-            asum = 0.f;
-          } else {
-            asum += sqrtf(t2 / fmaxf(prev_pdf * cosv, 1e-30f));
-          }
-          const bool wide = asum * asum > N.spread * a0;
-          const bool train = (flags & kFlagTrain) != 0;
-          bool record = false;
-          if (!train) {
-            query = wide && m.has_lobes;
-          } else if (!(flags & kFlagSuffix)) {
-            if (wide) {
-              flags |= kFlagSuffix;
-              asum = 0.f;
-            }
-            record = m.has_lobes != 0;
-          } else {
-            query = wide && m.has_lobes;
-            record = !query && m.has_lobes;
-          }
-          if (query || record) {
-            nq.pos[0] = p.x;
-            nq.pos[1] = p.y;
-            nq.pos[2] = p.z;
-            nq.roughness = m.nrc_rough;
-            nq.dir[0] = wo.x;
-            nq.dir[1] = wo.y;
-            nq.dir[2] = wo.z;
-            nq.pad0 = 0.f;
-            nq.normal[0] = nrm.x;
-            nq.normal[1] = nrm.y;
-            nq.normal[2] = nrm.z;
-            nq.pad1 = 0.f;
-            nq.albedo[0] = m.nrc_albedo[0];
-            nq.albedo[1] = m.nrc_albedo[1];
-            nq.albedo[2] = m.nrc_albedo[2];
-            nq.pad2 = 0.f;
-          }
-          if (record && depth < N.max_vertices) {
-            const uint32_t tp = slot - N.train_base;
-            const size_t r = (size_t)tp * N.max_vertices + depth;
-            N.tq[r] = nq;
-            N.tb[r] = make_float4(beta.x, beta.y, beta.z, 0.f);
-            N.tl[r] = make_float4(Lr.x, Lr.y, Lr.z, 0.f);
-            N.tn[tp] = depth + 1;
-          }
-          qbeta = beta;
-          scatter = !query;
-        }
-        const Frame fr = make_frame(nrm);
-        const V3 wo_l = to_local(fr, wo);
-        // NEE through the light BVH (pt/mod.rs:924-956)
-        if (scatter && S.num_lights > 0 && m.has_lobes) {
-          uint32_t li;
-          float pick;
-          if (ls_sample(S, p, rnd(rng), li, pick)) {
-            const auto eval = [&](V3 wi, V3& f, float& pdf) { eval_world(m, fr, wo_l, eta, wi, f, pdf); };
-            if (sample_light(S, S.lights[li], p, nrm, eval, pick, rng, so)) {
-              so.c = beta * so.c;
-              // shadow rays through the atmosphere carry its transmittance
-              if (S.atmosphere >= 0) so.c = so.c * medium_transmittance(S.media[S.atmosphere], so.o, so.d, so.tmax, rng);
-              shadow = maxc(so.c) > 0.f;
-            }
-          }
-        }
-        // BSDF continuation (pt/mod.rs:958-987)
-        BsdfSample bs;
-        if (scatter && bsdf_sample(m, wo_l, eta, rng, bs)) {
-          const V3 wi = to_world(fr, bs.wi);
-          beta = beta * bs.f * (fabsf(bs.wi.z) / bs.pdf);
-          const V3 org = p + nrm * (bs.transmitted ? -kRayOffset : kRayOffset);
-          const V3 nd = normalize(wi);
-          depth++;
-          bool alive = true;
-          if (depth >= prm.rr_start) {
-            const float qq = fminf(maxc(beta), 0.95f);
-            if (qq <= 0.f || rnd(rng) > qq) {
-              alive = false;
-            } else {
-              beta = beta / qq;
-            }
-          }
-          cont = alive && depth < prm.max_depth;
-          P.o[slot] = make_float4(org.x, org.y, org.z, bs.pdf);
-          P.d[slot] = make_float4(nd.x, nd.y, nd.z, a0);
-          P.prev[slot] = make_float4(p.x, p.y, p.z, asum);
-          flags = depth | (flags & (kFlagTrain | kFlagSuffix));  // from_camera cleared
-        }
-      }
-      P.beta[slot] = make_float4(beta.x, beta.y, beta.z, 0.f);
-      P.L[slot] = make_float4(Lr.x, Lr.y, Lr.z, 0.f);
-      P.rng[slot] = store_rng(rng);
-      P.flags[slot] = flags;
+      shade_one<NRC>(S, P, prm, N, slot, cont, shadow, query, so, nq, qbeta);
     }
     const uint32_t sidx = wave_append(&P.counts[2], act && shadow);
     if (act && shadow) {
@@ -516,22 +533,25 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtSceneView S, PathBuffer
   }
 }
 
+// Shadow ray i: adds its contribution to its path when unoccluded.
+__device__ __forceinline__ void shadow_one(const PtSceneView& S, const PathBuffers& P, uint32_t i, uint32_t* stk) {
+  const float4 o = P.so[i], d = P.sd[i];
+  if (scene_occluded<kTraceBlock>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, stk)) return;
+  const uint32_t slot = P.sq[i];
+  const float4 c = P.sc[i];
+  float4 l = P.L[slot];
+  l.x += c.x;
+  l.y += c.y;
+  l.z += c.z;
+  P.L[slot] = l;
+}
+
 __global__ __launch_bounds__(kTraceBlock) void shadow_kernel(PtSceneView S, PathBuffers P) {
   __shared__ uint32_t s_stack[kTraceStack * kTraceBlock];
   uint32_t* stk = s_stack + threadIdx.x;
   const uint32_t n = P.counts[2];
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const float4 o = P.so[i], d = P.sd[i];
-    if (scene_occluded<kTraceBlock>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, stk)) continue;
-    const uint32_t slot = P.sq[i];
-    const float4 c = P.sc[i];
-    float4 l = P.L[slot];
-    l.x += c.x;
-    l.y += c.y;
-    l.z += c.z;
-    P.L[slot] = l;
-  }
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) shadow_one(S, P, i, stk);
 }
 
 __global__ void rotate_kernel(PathBuffers P) {
@@ -542,32 +562,129 @@ __global__ void rotate_kernel(PathBuffers P) {
   P.counts[2] = 0;
 }
 
+// Adds pixel px's pass samples (slots slot_base + k * npix + px) into its
+// accumulator in sample order, with the firefly clamp (pt/mod.rs:88-95).
+__device__ __forceinline__ void resolve_one(const PathBuffers& P, const PtParams& prm, double* accum, uint32_t px,
+                                            uint32_t slot_base) {
+  double s[3] = {0, 0, 0}, q[3] = {0, 0, 0};
+  for (uint32_t k = 0; k < prm.spp_pass; k++) {
+    const float4 l4 = P.L[slot_base + (size_t)k * prm.npix + px];
+    double l[3] = {l4.x, l4.y, l4.z};
+    const double lm = 0.2126 * l[0] + 0.7152 * l[1] + 0.0722 * l[2];
+    if (prm.firefly_clamp > 0.f && lm > (double)prm.firefly_clamp) {
+      const double sc = (double)prm.firefly_clamp / lm;
+      l[0] *= sc;
+      l[1] *= sc;
+      l[2] *= sc;
+    }
+    for (int c = 0; c < 3; c++) {
+      s[c] += l[c];
+      q[c] += l[c] * l[c];
+    }
+  }
+  double* a = accum + 8 * (size_t)px;
+  for (int c = 0; c < 3; c++) {
+    a[c] += s[c];
+    a[3 + c] += q[c];
+  }
+  a[6] += (double)prm.spp_pass;
+}
+
 __global__ __launch_bounds__(kBlock) void resolve_kernel(PathBuffers P, PtParams prm, double* __restrict__ accum) {
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < prm.npix; px += stride) {
-    double s[3] = {0, 0, 0}, q[3] = {0, 0, 0};
-    for (uint32_t k = 0; k < prm.spp_pass; k++) {
-      const float4 l4 = P.L[(size_t)k * prm.npix + px];
-      double l[3] = {l4.x, l4.y, l4.z};
-      // pt/mod.rs:88-95 firefly clamp on the sample's luminance
-      const double lm = 0.2126 * l[0] + 0.7152 * l[1] + 0.0722 * l[2];
-      if (prm.firefly_clamp > 0.f && lm > (double)prm.firefly_clamp) {
-        const double sc = (double)prm.firefly_clamp / lm;
-        l[0] *= sc;
-        l[1] *= sc;
-        l[2] *= sc;
-      }
-      for (int c = 0; c < 3; c++) {
-        s[c] += l[c];
-        q[c] += l[c] * l[c];
-      }
+  for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < prm.npix; px += stride) resolve_one(P, prm, accum, px, 0);
+}
+
+// ---- batched jobs: one set of launches over the path slots of many scenes ----
+
+// Last job whose begin (slot_begin or pix_begin) is <= x.
+template <bool PIX>
+__device__ __forceinline__ uint32_t find_job(const PtJob* jobs, uint32_t nj, uint32_t x) {
+  uint32_t lo = 0, hi = nj;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if ((PIX ? jobs[mid].pix_begin : jobs[mid].slot_begin) <= x)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void raygen_batch_kernel(const PtJob* __restrict__ jobs, uint32_t nj, PathBuffers P,
+                                                             uint32_t total) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    P.counts[0] = total;
+    P.counts[1] = 0;
+    P.counts[2] = 0;
+  }
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < total; slot += stride) {
+    const uint32_t j = find_job<false>(jobs, nj, slot);
+    const PtJob& J = jobs[j];
+    const uint32_t local = slot - J.slot_begin;
+    init_path(J.S, P, slot, local % J.prm.npix, J.prm.sample0 + local / J.prm.npix, kFlagFromCamera);
+    P.job[slot] = j;
+  }
+}
+
+__global__ __launch_bounds__(kTraceBlock) void extend_batch_kernel(const PtJob* __restrict__ jobs, PathBuffers P,
+                                                                  const uint32_t* __restrict__ q) {
+  __shared__ uint32_t s_stack[kTraceStack * kTraceBlock];
+  uint32_t* stk = s_stack + threadIdx.x;
+  const uint32_t n = P.counts[0];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t slot = q[i];
+    extend_one(jobs[P.job[slot]].S, P, slot, stk);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void shade_batch_kernel(const PtJob* __restrict__ jobs, PathBuffers P,
+                                                            const uint32_t* __restrict__ q, uint32_t* __restrict__ qnext) {
+  const uint32_t n = P.counts[0];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const NrcPathBuffers N{};
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+    const uint32_t i = base + threadIdx.x;
+    const bool act = i < n;
+    bool cont = false, shadow = false, query = false;
+    ShadowOut so;
+    NrcQuery nq;
+    V3 qbeta;
+    uint32_t slot = 0;
+    if (act) {
+      slot = q[i];
+      const PtJob& J = jobs[P.job[slot]];
+      shade_one<false>(J.S, P, J.prm, N, slot, cont, shadow, query, so, nq, qbeta);
     }
-    double* a = accum + 8 * (size_t)px;
-    for (int c = 0; c < 3; c++) {
-      a[c] += s[c];
-      a[3 + c] += q[c];
+    const uint32_t sidx = wave_append(&P.counts[2], act && shadow);
+    if (act && shadow) {
+      P.sq[sidx] = slot;
+      P.so[sidx] = make_float4(so.o.x, so.o.y, so.o.z, so.tmax);
+      P.sd[sidx] = make_float4(so.d.x, so.d.y, so.d.z, 0.f);
+      P.sc[sidx] = make_float4(so.c.x, so.c.y, so.c.z, 0.f);
     }
-    a[6] += (double)prm.spp_pass;
+    const uint32_t nidx = wave_append(&P.counts[1], act && cont);
+    if (act && cont) qnext[nidx] = slot;
+  }
+}
+
+__global__ __launch_bounds__(kTraceBlock) void shadow_batch_kernel(const PtJob* __restrict__ jobs, PathBuffers P) {
+  __shared__ uint32_t s_stack[kTraceStack * kTraceBlock];
+  uint32_t* stk = s_stack + threadIdx.x;
+  const uint32_t n = P.counts[2];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    shadow_one(jobs[P.job[P.sq[i]]].S, P, i, stk);
+}
+
+__global__ __launch_bounds__(kBlock) void resolve_batch_kernel(const PtJob* __restrict__ jobs, uint32_t nj, PathBuffers P,
+                                                              uint32_t total_pix) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total_pix; g += stride) {
+    const PtJob& J = jobs[find_job<true>(jobs, nj, g)];
+    resolve_one(P, J.prm, J.accum, g - J.pix_begin, J.slot_begin);
   }
 }
 
@@ -596,6 +713,23 @@ hipError_t pt_launch_bounce(const PtSceneView& S, const PathBuffers& P, const Pt
     hipLaunchKernelGGL(shade_kernel<false>, dim3(g), dim3(kBlock), 0, st, S, P, prm, q, qnext, N);
   hipLaunchKernelGGL(shadow_kernel, dim3(gt), dim3(kTraceBlock), 0, st, S, P);
   hipLaunchKernelGGL(rotate_kernel, dim3(1), dim3(1), 0, st, P);
+  return hipGetLastError();
+}
+
+hipError_t pt_launch_batch_pass(const PtJob* jobs, uint32_t nj, const PathBuffers& P, uint32_t total,
+                                 uint32_t total_pix, uint32_t bounces, hipStream_t st) {
+  const uint32_t g = grid_for(total);
+  const uint32_t gt = grid_for(total, kTraceBlock);
+  hipLaunchKernelGGL(raygen_batch_kernel, dim3(g), dim3(kBlock), 0, st, jobs, nj, P, total);
+  for (uint32_t b = 0; b < bounces; b++) {
+    const uint32_t* q = (b & 1) ? P.q1 : P.q0;
+    uint32_t* qn = (b & 1) ? P.q0 : P.q1;
+    hipLaunchKernelGGL(extend_batch_kernel, dim3(gt), dim3(kTraceBlock), 0, st, jobs, P, q);
+    hipLaunchKernelGGL(shade_batch_kernel, dim3(g), dim3(kBlock), 0, st, jobs, P, q, qn);
+    hipLaunchKernelGGL(shadow_batch_kernel, dim3(gt), dim3(kTraceBlock), 0, st, jobs, P);
+    hipLaunchKernelGGL(rotate_kernel, dim3(1), dim3(1), 0, st, P);
+  }
+  hipLaunchKernelGGL(resolve_batch_kernel, dim3(grid_for(total_pix)), dim3(kBlock), 0, st, jobs, nj, P, total_pix);
   return hipGetLastError();
 }
 

@@ -164,6 +164,7 @@ struct PathBuffers {
   float4* sc;       // shadow contribution rgb (already times beta)
   uint32_t* counts; // [0] live paths, [1] next live, [2] shadow rays, [3] unused
   unsigned long long* stats;  // [0] extension rays, [1] shadow rays
+  uint32_t* job;    // batches: job of each slot (null in sessions)
 };
 
 // One radiance-cache query / training sample (the 64-byte rrhip_nrc_query
@@ -203,6 +204,15 @@ struct PtParams {
   uint32_t npix;
   uint32_t spp_pass;      // samples per pixel in this pass
   uint64_t sample0;       // sample index of the pass's first sample (incl. base)
+};
+
+// One job of a batch pass: its scene, its pass parameters, where its
+// slots and pixels start in the batch, and its accumulator.
+struct PtJob {
+  PtSceneView S;
+  PtParams prm;
+  uint32_t slot_begin, pix_begin;
+  double* accum;
 };
 
 }  // namespace rrhip

@@ -40,6 +40,8 @@ EXPORTED = [
     "rrhip_nrc_create", "rrhip_nrc_destroy", "rrhip_nrc_get_info", "rrhip_nrc_get_params",
     "rrhip_nrc_set_params", "rrhip_nrc_infer", "rrhip_nrc_train_step", "rrhip_nrc_gradients",
     "rrhip_session_set_nrc", "rrhip_gen_density_grid",
+    "rrhip_batch_create", "rrhip_batch_destroy", "rrhip_batch_render", "rrhip_batch_read_accum",
+    "rrhip_batch_get_stats",
 ]
 
 
@@ -136,6 +138,11 @@ class RenderParams(ctypes.Structure):
     def make(max_depth=10, rr_start=3, samples_per_pass=0, firefly_clamp=60.0, sample_base=0):
         return RenderParams(ctypes.sizeof(RenderParams), max_depth, rr_start, samples_per_pass,
                             firefly_clamp, sample_base)
+
+
+class BatchJob(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_void_p), ("spp", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("params", RenderParams)]
 
 
 class SessionStats(ctypes.Structure):
@@ -249,6 +256,11 @@ def load():
         "rrhip_scene_set_tuning": (i32, [vp, ctypes.POINTER(TraceTuning)]),
         "rrhip_scene_autotune": (i32, [vp, vp, u64, i32, vp, ctypes.POINTER(TraceTuning),
                                        ctypes.POINTER(ctypes.c_float)]),
+        "rrhip_batch_create": (i32, [vp, u32, vp, u32, ctypes.POINTER(vp)]),
+        "rrhip_batch_destroy": (i32, [vp]),
+        "rrhip_batch_render": (i32, [vp, vp, ctypes.POINTER(ctypes.c_double)]),
+        "rrhip_batch_read_accum": (i32, [vp, u32, vp]),
+        "rrhip_batch_get_stats": (i32, [vp, ctypes.POINTER(SessionStats)]),
         "rrhip_gen_density_grid": (i32, [u32, u64, ctypes.c_float, u32, ctypes.c_float, ctypes.c_float,
                                          ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), vp]),
         "rrhip_gen_icosphere": (i32, [u32, u64, ctypes.c_float, ctypes.c_float, u32, vp, vp, vp]),
@@ -618,3 +630,47 @@ class Nrc:
         _check(load().rrhip_nrc_gradients(self.h, _ptr(q), _ptr(t), q.shape[0], ctypes.byref(loss), _ptr(gm),
                                           _ptr(gg)))
         return loss.value, gm, gg
+
+
+# ---- batched jobs -------------------------------------------------------------
+
+class Batch:
+    """Many path-tracing jobs advanced through one launch sequence per
+    pass (rrhip_batch_*). jobs: list of (PtScene, spp, RenderParams)."""
+
+    def __init__(self, ctx, jobs, samples_per_pass=0):
+        self.ctx = ctx
+        self._scenes = [j[0] for j in jobs]  # the scenes must outlive the batch
+        arr = (BatchJob * len(jobs))(*[BatchJob(sc.h, spp, 0, prm) for sc, spp, prm in jobs])
+        self.npix = [sc.npix for sc, _, _ in jobs]
+        h = ctypes.c_void_p()
+        _check(load().rrhip_batch_create(ctx.h, len(jobs), ctypes.addressof(arr), samples_per_pass,
+                                         ctypes.byref(h)))
+        self.h = h
+        self.n = len(jobs)
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().rrhip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def render(self):
+        """Returns (per-job completion ms from the start, total ms)."""
+        job_ms = np.zeros(self.n)
+        total = ctypes.c_double()
+        _check(load().rrhip_batch_render(self.h, _ptr(job_ms), ctypes.byref(total)))
+        return job_ms, total.value
+
+    def accum(self, job):
+        out = np.empty((self.npix[job], ACCUM_STRIDE), np.float64)
+        _check(load().rrhip_batch_read_accum(self.h, job, _ptr(out)))
+        return out
+
+    def stats(self):
+        st = SessionStats()
+        _check(load().rrhip_batch_get_stats(self.h, ctypes.byref(st)))
+        return {"extension_rays": st.extension_rays, "shadow_rays": st.shadow_rays,
+                "path_slots": st.path_slots, "samples_per_pass": st.samples_per_pass, "passes": st.reserved}

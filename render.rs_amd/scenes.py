@@ -333,3 +333,30 @@ def instanced_field(width=1920, height=1080, n_instances=10000, levels=5, n_poin
         xf = np.concatenate([R, np.array(t).reshape(3, 1)], axis=1)
         s.add_instance(mesh, m, xf)
     return s
+
+
+def batch_rooms(n_jobs=32, seed=1, grid=128):
+    """The batched-jobs workload: `n_jobs` box rooms, each with 1-40 white
+    matte spheres of radius U(0.05, 0.3) centred uniformly in [-0.7, 0.7]^3,
+    a square resolution drawn from {64, 128, 256, 512}, spp uniform in
+    4..64, and every other job filled with the fBm cloud atmosphere
+    (grid^3 density, g 0.3). Returns [(Scene, spp, max_depth)]: depth 8 for
+    surface-only rooms, 64 for rooms with media (render.rs's budget,
+    pt/mod.rs:484)."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for j in range(n_jobs):
+        res = int(rng.choice([64, 128, 256, 512]))
+        k = int(rng.integers(1, 41))
+        spheres = [(tuple(rng.uniform(-0.7, 0.7, 3)), float(rng.uniform(0.05, 0.3))) for _ in range(k)]
+        spp = int(rng.integers(4, 65))
+        media = j % 2 == 1
+        if media:
+            s = cloud_room(res, res, n=grid, seed=seed * 1000 + j, spheres=spheres)
+        else:
+            s = box_room(res, res)
+            s.spheres = []
+            for c, r in spheres:
+                s.add_sphere(c, r, 0)
+        jobs.append((s, spp, 64 if media else 8))
+    return jobs

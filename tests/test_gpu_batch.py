@@ -1,0 +1,61 @@
+"""Batched jobs: every job of a batch equals, bit for bit, the same job
+rendered by its own session with the batch's samples_per_pass (same
+per-path code, streams and accumulation order), whatever the other jobs."""
+import sys
+
+import numpy as np
+import pytest
+
+import rrhip
+import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def test_batch_jobs_equal_their_own_sessions(ctx):
+    say("batch of 6 mixed rooms (3 with clouds) vs 6 sessions")
+    rng = np.random.default_rng(3)
+    jobs = []
+    for j, (res, spp) in enumerate([(32, 5), (48, 12), (16, 3), (40, 7), (24, 9), (32, 1)]):
+        spheres = [(tuple(rng.uniform(-0.7, 0.7, 3)), float(rng.uniform(0.05, 0.3))) for _ in range(j * 3 + 1)]
+        if j % 2:
+            s = scenes.cloud_room(res, res, n=16, seed=j, spheres=spheres)
+            depth = 64
+        else:
+            s = scenes.box_room(res, res)
+            s.spheres = []
+            for c, r in spheres:
+                s.add_sphere(c, r, 0)
+            depth = 8
+        jobs.append((s, spp, depth))
+    pss = [rrhip.PtScene(ctx, s.desc()) for s, _, _ in jobs]
+    spp_pass = 4
+    b = rrhip.Batch(ctx, [(ps, spp, rrhip.RenderParams.make(max_depth=d)) for ps, (_, spp, d) in zip(pss, jobs)],
+                    samples_per_pass=spp_pass)
+    job_ms, total = b.render()
+    st = b.stats()
+    say(f"batch: {total:.2f} ms, per job {np.round(job_ms, 2)}, stats {st}")
+    assert st["passes"] == 3 and (job_ms <= total + 1e-6).all()
+    rays = 0
+    for j, (ps, (s, spp, d)) in enumerate(zip(pss, jobs)):
+        se = ps.session(rrhip.RenderParams.make(max_depth=d, samples_per_pass=spp_pass))
+        se.render(0, spp)
+        want = se.accum()
+        got = b.accum(j)
+        assert got.tobytes() == want.tobytes(), f"job {j} differs"
+        sst = se.stats()
+        rays += sst["extension_rays"] + sst["shadow_rays"]
+        se.close()
+    assert rays == st["extension_rays"] + st["shadow_rays"]
+    # a second render of the same batch reproduces it
+    first = b.accum(1)
+    b.render()
+    assert b.accum(1).tobytes() == first.tobytes()
+    b.close()
+    for ps in pss:
+        ps.close()
