@@ -307,8 +307,11 @@ int rrhip_device_copy(void* stream, void* dst, const void* src, uint64_t bytes);
 int rrhip_scene_upload(rrhip_ctx* ctx, const rrhip_scene_desc* desc, rrhip_scene** out);
 int rrhip_scene_destroy(rrhip_scene* scene);
 int rrhip_scene_get_info(rrhip_scene* scene, rrhip_scene_info* info);
-/* Runs the host BVH build of rrhip_scene_upload without a device and
- * reports what it built (device_bytes = what an upload would allocate). */
+/* Runs the host build of rrhip_scene_upload without a device and reports
+ * what it built: for a ray-query mesh the BVH (device_bytes = what an
+ * upload would allocate); for a path-tracing scene the instance hierarchy
+ * (num_nodes, BLAS + loose triangles, num_leaves = instances, depth = the
+ * traversal stack entries a lane can need, bounds of all geometry). */
 int rrhip_build_info(const rrhip_scene_desc* desc, rrhip_scene_info* info);
 
 /* Current / explicit launch configuration (set validates against the

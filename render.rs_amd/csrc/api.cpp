@@ -275,9 +275,28 @@ int build_from_desc(const rrhip_scene_desc* desc, BuiltBvh* built, rrhip_scene_i
 int rrhip_build_info(const rrhip_scene_desc* desc, rrhip_scene_info* info) {
   if (!desc || !info) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_build_info: null argument");
   if (int st = validate_desc(desc)) return st;
-  if (!is_query(desc)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_build_info: no ray-query mesh");
-  BuiltBvh built;
-  return build_from_desc(desc, &built, info);
+  if (is_query(desc)) {
+    BuiltBvh built;
+    return build_from_desc(desc, &built, info);
+  }
+  if (!is_pt(desc)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_build_info: no ray-query mesh or camera");
+  // Path-tracing scene: instance hierarchy statistics; depth = the
+  // traversal stack entries a lane can need.
+  PtSceneHost h;
+  std::string err;
+  if (!build_pt_scene(*desc, &h, &err)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_build_info: " + err);
+  *info = rrhip_scene_info{};
+  info->num_triangles = h.btris.size() + h.tris.size();
+  info->num_nodes = h.nodes.size();
+  info->num_leaves = h.insts.size();
+  info->depth = h.stack_need;
+  info->node_bytes = sizeof(Node2);
+  info->tri_bytes = sizeof(TriRec);
+  for (int a = 0; a < 3; a++) {
+    info->bounds_lo[a] = h.lo[a];
+    info->bounds_hi[a] = h.hi[a];
+  }
+  return RRHIP_OK;
 }
 
 }  // extern "C"
@@ -322,6 +341,7 @@ hipError_t upload_pt(rrhip_scene* s, const PtSceneHost& h) {
   }
   if (e == hipSuccess) e = upload_vec(media, &s->pt_mem[14], &v.media);
   v.atmosphere = h.atmosphere;
+  v.stack_need = h.stack_need;
   v.num_insts = (uint32_t)h.insts.size();
   v.num_tris = (uint32_t)h.tris.size();
   v.num_spheres = (uint32_t)h.spheres.size();

@@ -125,8 +125,11 @@ __device__ __forceinline__ void extend_one(const PtSceneView& S, const PathBuffe
   }
 }
 
+// STACK: LDS entries per lane, the smallest size >= the scene's bound
+// (fewer bytes per block, more resident waves).
+template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) void extend_kernel(PtSceneView S, PathBuffers P, const uint32_t* __restrict__ q) {
-  __shared__ uint32_t s_stack[kTraceStack * kTraceBlock];
+  __shared__ uint32_t s_stack[STACK * kTraceBlock];
   uint32_t* stk = s_stack + threadIdx.x;
   const uint32_t n = P.counts[0];
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -546,8 +549,9 @@ __device__ __forceinline__ void shadow_one(const PtSceneView& S, const PathBuffe
   P.L[slot] = l;
 }
 
+template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) void shadow_kernel(PtSceneView S, PathBuffers P) {
-  __shared__ uint32_t s_stack[kTraceStack * kTraceBlock];
+  __shared__ uint32_t s_stack[STACK * kTraceBlock];
   uint32_t* stk = s_stack + threadIdx.x;
   const uint32_t n = P.counts[2];
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -706,12 +710,19 @@ hipError_t pt_launch_bounce(const PtSceneView& S, const PathBuffers& P, const Pt
                             uint32_t* qnext, uint32_t capacity, const NrcPathBuffers& N, hipStream_t st) {
   const uint32_t g = grid_for(capacity);
   const uint32_t gt = grid_for(capacity, kTraceBlock);
-  hipLaunchKernelGGL(extend_kernel, dim3(gt), dim3(kTraceBlock), 0, st, S, P, q);
+  const bool small = S.stack_need <= 32;
+  if (small)
+    hipLaunchKernelGGL(extend_kernel<32>, dim3(gt), dim3(kTraceBlock), 0, st, S, P, q);
+  else
+    hipLaunchKernelGGL(extend_kernel<kTraceStack>, dim3(gt), dim3(kTraceBlock), 0, st, S, P, q);
   if (N.q)
     hipLaunchKernelGGL(shade_kernel<true>, dim3(g), dim3(kBlock), 0, st, S, P, prm, q, qnext, N);
   else
     hipLaunchKernelGGL(shade_kernel<false>, dim3(g), dim3(kBlock), 0, st, S, P, prm, q, qnext, N);
-  hipLaunchKernelGGL(shadow_kernel, dim3(gt), dim3(kTraceBlock), 0, st, S, P);
+  if (small)
+    hipLaunchKernelGGL(shadow_kernel<32>, dim3(gt), dim3(kTraceBlock), 0, st, S, P);
+  else
+    hipLaunchKernelGGL(shadow_kernel<kTraceStack>, dim3(gt), dim3(kTraceBlock), 0, st, S, P);
   hipLaunchKernelGGL(rotate_kernel, dim3(1), dim3(1), 0, st, P);
   return hipGetLastError();
 }

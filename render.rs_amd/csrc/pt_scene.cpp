@@ -202,6 +202,7 @@ bool affine_inverse(const double m[12], double inv[12]) {
 bool build_instances(const rrhip_scene_desc& d, PtSceneHost* out, std::string* err) {
   if (d.num_instances == 0) return true;
   if (!d.instances || !d.meshes) return *err = "null instances or meshes", false;
+  if (d.num_instances >= (1u << 28)) return *err = "too many instances (limit 2^28)", false;
   BuildParams bp;
   bp.bins = d.build.bins ? (int)d.build.bins : 32;
   bp.max_leaf = d.build.max_leaf ? (int)d.build.max_leaf : 4;
@@ -289,8 +290,9 @@ bool build_instances(const rrhip_scene_desc& d, PtSceneHost* out, std::string* e
   }
   if (out->nodes.size() >= (1ull << 31) || out->btris.size() >= (1ull << 28))
     return *err = "instanced scene too large", false;
-  // A TLAS leaf pair can stack 2 * max_leaf instance entries.
-  out->stack_need = (uint32_t)(tlas_levels + 2 * bp.max_leaf + blas_levels);
+  // One entry per TLAS level (two at the last), one per BLAS level, and
+  // the remainder of a TLAS leaf re-pushed while its first instance runs.
+  out->stack_need = (uint32_t)(tlas_levels + blas_levels + 2);
   return true;
 }
 

@@ -17,7 +17,12 @@
 namespace rrhip {
 namespace pt {
 
+// Stack entry of a TLAS leaf still to enter: flag, count - 1 (bits 28-30),
+// first instance (bits 0-27). One entry per leaf keeps the TLAS phase at
+// one entry per level (two at the last), so a lane needs at most
+// tlas_levels + blas_levels + 2 entries.
 constexpr uint32_t kInstEntry = 0x80000000u;
+constexpr uint32_t kInstFirst = 0x0FFFFFFFu;
 
 struct MeshHit {
   float t;
@@ -101,7 +106,7 @@ __device__ __forceinline__ bool trace_instances(const PtSceneView& S, V3 o, V3 d
         const uint32_t ref = c == 0 ? nd.ref0 : nd.ref1;
         if (!hc || cnt == 0) continue;
         if (base < 0) {
-          for (uint32_t k = 0; k < cnt; k++) stk[(sp++) * STRIDE] = kInstEntry | (ref + k);
+          stk[(sp++) * STRIDE] = kInstEntry | ((cnt - 1) << 28) | ref;
         } else {
           for (uint32_t k = ref; k < ref + cnt; k++) {
             float t, u, v;
@@ -138,7 +143,9 @@ __device__ __forceinline__ bool trace_instances(const PtSceneView& S, V3 o, V3 d
     if (sp == 0) break;
     const uint32_t e = stk[(--sp) * STRIDE];
     if (e & kInstEntry) {
-      inst = e & ~kInstEntry;
+      inst = e & kInstFirst;
+      const uint32_t more = (e >> 28) & 7u;  // instances of the leaf after this one
+      if (more) stk[(sp++) * STRIDE] = kInstEntry | ((more - 1) << 28) | (inst + 1);
       const PtInstance I = S.insts[inst];
       const uint32_t root = S.meshes[I.mesh].root;
       if (root == 0xFFFFFFFFu) continue;

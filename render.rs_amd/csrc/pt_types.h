@@ -137,6 +137,7 @@ struct PtSceneView {
   const float* mesh_nrm;
   const PtMedium* media;
   int32_t atmosphere;           // medium every ray travels in (-1: none)
+  uint32_t stack_need;          // traversal stack entries a lane can need
   uint32_t num_insts;
   uint32_t num_tris, num_spheres, num_lights, num_lnodes, num_inf;
   int32_t dome;
