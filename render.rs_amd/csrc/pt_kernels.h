@@ -12,8 +12,11 @@ constexpr uint32_t kPtTraceStack = 48;
 // Path slots [0, total) with slot = s * npix + pixel: camera rays of
 // samples prm.sample0 + s; fills the live queue (q0) and resets counters.
 // With N.ntrain > 0 also training paths in slots [total, total + ntrain).
+// sample0_dev (may be null) overrides prm.sample0 from device memory.
 hipError_t pt_launch_raygen(const PtSceneView& S, const PathBuffers& P, const PtParams& prm,
-                            uint32_t total, const NrcPathBuffers& N, hipStream_t st);
+                            uint32_t total, const NrcPathBuffers& N, const uint64_t* sample0_dev, hipStream_t st);
+// *dst = v, in stream order (a kernel, so it never becomes a memset node).
+hipError_t pt_launch_set_u64(uint64_t* dst, uint64_t v, hipStream_t st);
 // One bounce over the live queue `q`: closest hit, shading (emission + MIS,
 // NEE shadow rays, BSDF continuation, Russian roulette) into `qnext`,
 // shadow rays, counter rotation. `capacity` bounds every queue. With

@@ -79,8 +79,11 @@ __device__ __forceinline__ void init_path(const PtSceneView& S, const PathBuffer
   P.q0[slot] = slot;
 }
 
+// sample0_dev (when set) supplies the pass's first sample index from device
+// memory, so a captured pass graph can be replayed for any sample range.
 __global__ __launch_bounds__(kBlock) void raygen_kernel(PtSceneView S, PathBuffers P, PtParams prm, uint32_t total,
-                                                       NrcPathBuffers N) {
+                                                       NrcPathBuffers N, const uint64_t* __restrict__ sample0_dev) {
+  if (sample0_dev) prm.sample0 = *sample0_dev;
   const uint32_t all = total + N.ntrain;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     P.counts[0] = all;
@@ -699,10 +702,17 @@ uint32_t grid_for(uint64_t n, int block = kBlock) {
 
 }  // namespace
 
+__global__ void set_u64_kernel(uint64_t* dst, uint64_t v) { *dst = v; }
+
 hipError_t pt_launch_raygen(const PtSceneView& S, const PathBuffers& P, const PtParams& prm, uint32_t total,
-                            const NrcPathBuffers& N, hipStream_t st) {
+                            const NrcPathBuffers& N, const uint64_t* sample0_dev, hipStream_t st) {
   hipLaunchKernelGGL(raygen_kernel, dim3(grid_for((uint64_t)total + N.ntrain)), dim3(kBlock), 0, st, S, P, prm, total,
-                     N);
+                     N, sample0_dev);
+  return hipGetLastError();
+}
+
+hipError_t pt_launch_set_u64(uint64_t* dst, uint64_t v, hipStream_t st) {
+  hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(1), 0, st, dst, v);
   return hipGetLastError();
 }
 

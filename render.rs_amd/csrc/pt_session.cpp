@@ -42,6 +42,11 @@ struct rrhip_session {
   NrcQuery* nrc_bq = nullptr;
   float* nrc_bt = nullptr;
   uint32_t nrc_bcap = 0;
+  // Launch mode 1: one hipGraph per pass size, captured on first use and
+  // replayed with the first sample index written to d_sample0.
+  uint32_t launch_mode = 0;
+  uint64_t* d_sample0 = nullptr;
+  std::vector<std::pair<uint32_t, hipGraphExec_t>> graphs;
 };
 
 namespace {
@@ -70,7 +75,13 @@ void free_paths(rrhip_session* s) {
   s->path_cap = 0;
 }
 
+void drop_graphs(rrhip_session* s) {
+  for (auto& g : s->graphs) (void)hipGraphExecDestroy(g.second);
+  s->graphs.clear();
+}
+
 void free_session(rrhip_session* s) {
+  drop_graphs(s);
   free_paths(s);
   for (void* p : s->mem) (void)hipFree(p);
   s->mem.clear();
@@ -158,43 +169,89 @@ int rrhip_session_destroy(rrhip_session* s) {
   return RRHIP_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Enqueues one pass of n samples per pixel (raygen, bounces, cache work,
+// resolve). sample0_dev: null = prm.sample0 is used as given.
+hipError_t enqueue_pass(rrhip_session* s, uint32_t n, uint64_t sample0, const uint64_t* sample0_dev, hipStream_t st) {
+  rrhip_scene* sc = s->scene;
+  rrhip_nrc* nrc = s->nrc;
+  NrcPathBuffers N = nrc ? s->N : NrcPathBuffers{};
+  PtParams prm = s->prm;
+  prm.spp_pass = n;
+  prm.sample0 = sample0;
+  const uint32_t total = n * prm.npix;
+  N.train_base = total;  // training paths follow this pass's rendering paths
+  const uint32_t paths = total + N.ntrain;
+  hipError_t e = pt_launch_raygen(sc->pt, s->P, prm, total, N, sample0_dev, st);
+  for (uint32_t b = 0; b < prm.max_depth && e == hipSuccess; b++) {
+    const uint32_t* q = (b & 1) ? s->P.q1 : s->P.q0;
+    uint32_t* qn = (b & 1) ? s->P.q0 : s->P.q1;
+    e = pt_launch_bounce(sc->pt, s->P, prm, q, qn, paths, N, st);
+  }
+  if (nrc && e == hipSuccess) {
+    // cache radiance at the terminal vertices (and training tails), then
+    // this pass's training samples and optimiser steps
+    e = nrc_launch_infer_paths(nrc->G, N.q, &N.counts[0], paths, N.q_slot, N.q_beta, s->P.L, st);
+    if (e == hipSuccess)
+      e = pt_launch_nrc_samples(s->P, N, prm.firefly_clamp, s->nrc_scan, s->nrc_bcap, s->nrc_bq, s->nrc_bt, st);
+    const uint32_t batch = nrc->cfg.batch;
+    for (uint32_t k = 0; k < nrc->cfg.steps_per_pass && N.ntrain > 0 && e == hipSuccess; k++) {
+      if ((uint64_t)k * batch >= s->nrc_bcap) break;
+      e = pt_launch_nrc_set_batch(N, k, batch, &nrc->d_ctl->n, st);
+      if (e == hipSuccess)
+        e = nrc_step_device(nrc, s->nrc_bq + (size_t)k * batch, s->nrc_bt + 3 * (size_t)k * batch, batch, st);
+    }
+  }
+  if (e == hipSuccess) e = pt_launch_resolve(s->P, prm, s->d_accum, st);
+  return e;
+}
+
+// The pass graph of size n (captured on first use).
+hipError_t pass_graph(rrhip_session* s, uint32_t n, hipStream_t st, hipGraphExec_t* out) {
+  for (auto& g : s->graphs)
+    if (g.first == n) {
+      *out = g.second;
+      return hipSuccess;
+    }
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  const hipError_t pe = enqueue_pass(s, n, 0, s->d_sample0, st);
+  e = hipStreamEndCapture(st, &graph);
+  if (pe != hipSuccess) e = pe;
+  hipGraphExec_t exec = nullptr;
+  if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return e;
+  s->graphs.push_back({n, exec});
+  *out = exec;
+  return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
 int rrhip_render_samples(rrhip_session* s, uint32_t start, uint32_t count) {
   if (int st = check_session(s)) return st;
   rrhip_scene* sc = s->scene;
   RRHIP_HIP_TRY(hipSetDevice(sc->ctx->device));
   hipStream_t st = sc->ctx->stream;
   RRHIP_HIP_TRY(hipEventRecord(sc->ctx->ev0, st));
-  rrhip_nrc* nrc = s->nrc;
-  NrcPathBuffers N = nrc ? s->N : NrcPathBuffers{};
   for (uint32_t done = 0; done < count;) {
     const uint32_t n = std::min(s->spp_pass, count - done);
-    PtParams prm = s->prm;
-    prm.spp_pass = n;
-    prm.sample0 = s->sample_base + (uint64_t)start + done;
-    const uint32_t total = n * prm.npix;
-    N.train_base = total;  // training paths follow this pass's rendering paths
-    const uint32_t paths = total + N.ntrain;
-    RRHIP_HIP_TRY(pt_launch_raygen(sc->pt, s->P, prm, total, N, st));
-    for (uint32_t b = 0; b < prm.max_depth; b++) {
-      const uint32_t* q = (b & 1) ? s->P.q1 : s->P.q0;
-      uint32_t* qn = (b & 1) ? s->P.q0 : s->P.q1;
-      RRHIP_HIP_TRY(pt_launch_bounce(sc->pt, s->P, prm, q, qn, paths, N, st));
+    const uint64_t sample0 = s->sample_base + (uint64_t)start + done;
+    if (s->launch_mode == 1) {
+      hipGraphExec_t exec = nullptr;
+      RRHIP_HIP_TRY(pass_graph(s, n, st, &exec));
+      RRHIP_HIP_TRY(pt_launch_set_u64(s->d_sample0, sample0, st));
+      RRHIP_HIP_TRY(hipGraphLaunch(exec, st));
+    } else {
+      RRHIP_HIP_TRY(enqueue_pass(s, n, sample0, nullptr, st));
     }
-    if (nrc) {
-      // cache radiance at the terminal vertices (and training tails), then
-      // this pass's training samples and optimiser steps
-      RRHIP_HIP_TRY(nrc_launch_infer_paths(nrc->G, N.q, &N.counts[0], paths, N.q_slot, N.q_beta, s->P.L, st));
-      RRHIP_HIP_TRY(pt_launch_nrc_samples(s->P, N, prm.firefly_clamp, s->nrc_scan, s->nrc_bcap, s->nrc_bq,
-                                          s->nrc_bt, st));
-      const uint32_t batch = nrc->cfg.batch;
-      for (uint32_t k = 0; k < nrc->cfg.steps_per_pass && N.ntrain > 0; k++) {
-        if ((uint64_t)k * batch >= s->nrc_bcap) break;
-        RRHIP_HIP_TRY(pt_launch_nrc_set_batch(N, k, batch, &nrc->d_ctl->n, st));
-        RRHIP_HIP_TRY(nrc_step_device(nrc, s->nrc_bq + (size_t)k * batch, s->nrc_bt + 3 * (size_t)k * batch, batch,
-                                      st));
-      }
-    }
-    RRHIP_HIP_TRY(pt_launch_resolve(s->P, prm, s->d_accum, st));
     done += n;
   }
   RRHIP_HIP_TRY(hipEventRecord(sc->ctx->ev1, st));
@@ -203,6 +260,17 @@ int rrhip_render_samples(rrhip_session* s, uint32_t start, uint32_t count) {
   RRHIP_HIP_TRY(hipEventElapsedTime(&ms, sc->ctx->ev0, sc->ctx->ev1));
   s->last_ms = ms;
   s->samples_done += count;
+  return RRHIP_OK;
+}
+
+int rrhip_session_set_launch_mode(rrhip_session* s, uint32_t mode) {
+  if (int st = check_session(s)) return st;
+  if (mode > 1) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_set_launch_mode: mode must be 0 or 1");
+  RRHIP_HIP_TRY(hipSetDevice(s->scene->ctx->device));
+  RRHIP_HIP_TRY(hipStreamSynchronize(s->scene->ctx->stream));
+  if (mode == 1 && !s->d_sample0) RRHIP_HIP_TRY(alloc_n(s, &s->d_sample0, 1));
+  drop_graphs(s);
+  s->launch_mode = mode;
   return RRHIP_OK;
 }
 
@@ -275,6 +343,7 @@ int rrhip_session_get_stats(rrhip_session* s, rrhip_session_stats* out) {
 extern "C" int rrhip_session_set_nrc(rrhip_session* s, rrhip_nrc* nrc) {
   if (int st = check_session(s)) return st;
   if (!nrc) {
+    drop_graphs(s);
     s->nrc = nullptr;
     return RRHIP_OK;
   }
@@ -287,6 +356,7 @@ extern "C" int rrhip_session_set_nrc(rrhip_session* s, rrhip_nrc* nrc) {
   if (cap >= (1ull << 31)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_set_nrc: path slots exceed 2^31");
   RRHIP_HIP_TRY(hipSetDevice(sc->ctx->device));
   RRHIP_HIP_TRY(hipStreamSynchronize(sc->ctx->stream));
+  drop_graphs(s);  // captured passes hold the old buffers
   if (nrc->bounds_from_scene) nrc_set_bounds(nrc, sc->pt_lo, sc->pt_hi);
   s->nrc = nullptr;
   hipError_t e = alloc_paths(s, cap);

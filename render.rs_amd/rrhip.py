@@ -41,7 +41,7 @@ EXPORTED = [
     "rrhip_nrc_set_params", "rrhip_nrc_infer", "rrhip_nrc_train_step", "rrhip_nrc_gradients",
     "rrhip_session_set_nrc", "rrhip_gen_density_grid",
     "rrhip_batch_create", "rrhip_batch_destroy", "rrhip_batch_render", "rrhip_batch_read_accum",
-    "rrhip_batch_get_stats",
+    "rrhip_batch_get_stats", "rrhip_session_set_launch_mode",
 ]
 
 
@@ -256,6 +256,7 @@ def load():
         "rrhip_scene_set_tuning": (i32, [vp, ctypes.POINTER(TraceTuning)]),
         "rrhip_scene_autotune": (i32, [vp, vp, u64, i32, vp, ctypes.POINTER(TraceTuning),
                                        ctypes.POINTER(ctypes.c_float)]),
+        "rrhip_session_set_launch_mode": (i32, [vp, u32]),
         "rrhip_batch_create": (i32, [vp, u32, vp, u32, ctypes.POINTER(vp)]),
         "rrhip_batch_destroy": (i32, [vp]),
         "rrhip_batch_render": (i32, [vp, vp, ctypes.POINTER(ctypes.c_double)]),
@@ -462,6 +463,10 @@ class Session:
         else:
             acc = np.ascontiguousarray(acc, np.float64)
             _check(load().rrhip_load_accum(self.h, _ptr(acc)))
+
+    def set_launch_mode(self, mode):
+        """0: kernels launched one by one; 1: one hipGraph per pass size."""
+        _check(load().rrhip_session_set_launch_mode(self.h, mode))
 
     def set_nrc(self, nrc):
         """Attaches a radiance cache (Nrc) to the session, or detaches it (None)."""
