@@ -380,12 +380,18 @@ int rrhip_read_accum(rrhip_session* session, double* out);
  * render.rs's load_accum (renderer.rs:237-254). */
 int rrhip_load_accum(rrhip_session* session, const double* in);
 int rrhip_session_get_stats(rrhip_session* session, rrhip_session_stats* stats);
-/* How passes are launched: 0 = kernel by kernel on the stream (default),
+/* How passes are launched (bit flags, results identical in every mode):
  * 1 = each pass size captured once as a hipGraph and replayed (the first
- * sample index is fed from device memory). Results are identical. No
- * counterpart in render.rs (the Metal host encodes command buffers per
+ * sample index is fed from device memory); 2 = the live queue sorted by
+ * shading class (miss / active-lobe set of the hit material) before shade,
+ * so a wave runs one BSDF kind. 0 = plain kernel launches (default). No
+ * counterpart in render.rs (its Metal host encodes one command buffer per
  * dispatch, renderer.rs:380-437). */
 int rrhip_session_set_launch_mode(rrhip_session* session, uint32_t mode);
+/* Autotuner: times every launch mode on `samples` samples per pixel
+ * (discarded: the accumulator, statistics and any attached cache are left
+ * as they were), keeps the fastest and reports it with its milliseconds. */
+int rrhip_session_autotune(rrhip_session* session, uint32_t samples, uint32_t* best_mode, float* best_ms);
 /* relMSE of `session`'s mean image against `reference`'s (same size and
  * context): mean over pixels and channels of (x - r)^2 / (r^2 + eps),
  * reduced in FP64 on the GPU. No counterpart in render.rs (its tests use

@@ -695,6 +695,72 @@ __global__ __launch_bounds__(kBlock) void resolve_batch_kernel(const PtJob* __re
   }
 }
 
+// ---- shading-class sort of the live queue (before shade) ------------------------
+// Key: 0 = miss, else the hit material's sort_key (1 + active-lobe mask).
+// Counting sort over fixed contiguous block ranges: per-block histograms,
+// one scan over (class, block), then each block scatters its range.
+constexpr uint32_t kSortKeys = 33;
+constexpr uint32_t kSortBlocks = 512;
+
+__device__ __forceinline__ uint32_t sort_key(const PtSceneView& S, const PathBuffers& P, uint32_t slot) {
+  if (P.hit_n[slot].w < 0.f) return 0;
+  return S.materials[P.hit_m[slot] & 0x7FFFFFFFu].sort_key;
+}
+
+__device__ __forceinline__ void block_range(uint32_t n, uint32_t& b0, uint32_t& b1) {
+  const uint32_t chunk = (n + kSortBlocks - 1) / kSortBlocks;
+  b0 = min(n, blockIdx.x * chunk);
+  b1 = min(n, b0 + chunk);
+}
+
+__global__ __launch_bounds__(kBlock) void sort_count_kernel(PtSceneView S, PathBuffers P, const uint32_t* __restrict__ q) {
+  __shared__ uint32_t h[kSortKeys];
+  for (uint32_t i = threadIdx.x; i < kSortKeys; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  uint32_t b0, b1;
+  block_range(P.counts[0], b0, b1);
+  for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) atomicAdd(&h[sort_key(S, P, q[i])], 1u);
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < kSortKeys; k += blockDim.x) P.sort_hist[k * kSortBlocks + blockIdx.x] = h[k];
+}
+
+// Exclusive scan of the (class, block) counts in place.
+__global__ __launch_bounds__(1024) void sort_scan_kernel(PathBuffers P) {
+  __shared__ uint32_t part[1024];
+  constexpr uint32_t n = kSortKeys * kSortBlocks;
+  const uint32_t t = threadIdx.x;
+  const uint32_t chunk = (n + 1023) / 1024;
+  const uint32_t b = t * chunk, e = min(n, b + chunk);
+  uint32_t s = 0;
+  for (uint32_t i = b; i < e; i++) s += P.sort_hist[i];
+  part[t] = s;
+  __syncthreads();
+  for (uint32_t off = 1; off < 1024; off <<= 1) {
+    const uint32_t v = t >= off ? part[t - off] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[t] - s;
+  for (uint32_t i = b; i < e; i++) {
+    const uint32_t c = P.sort_hist[i];
+    P.sort_hist[i] = run;
+    run += c;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void sort_scatter_kernel(PtSceneView S, PathBuffers P, const uint32_t* __restrict__ q) {
+  __shared__ uint32_t cur[kSortKeys];
+  for (uint32_t k = threadIdx.x; k < kSortKeys; k += blockDim.x) cur[k] = P.sort_hist[k * kSortBlocks + blockIdx.x];
+  __syncthreads();
+  uint32_t b0, b1;
+  block_range(P.counts[0], b0, b1);
+  for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+    const uint32_t slot = q[i];
+    P.qs[atomicAdd(&cur[sort_key(S, P, slot)], 1u)] = slot;
+  }
+}
+
 uint32_t grid_for(uint64_t n, int block = kBlock) {
   const uint64_t b = (n + block - 1) / block;
   return (uint32_t)(b < 1 ? 1 : (b > kMaxGrid ? kMaxGrid : b));
@@ -725,10 +791,17 @@ hipError_t pt_launch_bounce(const PtSceneView& S, const PathBuffers& P, const Pt
     hipLaunchKernelGGL(extend_kernel<32>, dim3(gt), dim3(kTraceBlock), 0, st, S, P, q);
   else
     hipLaunchKernelGGL(extend_kernel<kTraceStack>, dim3(gt), dim3(kTraceBlock), 0, st, S, P, q);
+  const uint32_t* qshade = q;
+  if (P.qs) {  // group the queue by shading class so shade waves run one BSDF kind
+    hipLaunchKernelGGL(sort_count_kernel, dim3(kSortBlocks), dim3(kBlock), 0, st, S, P, q);
+    hipLaunchKernelGGL(sort_scan_kernel, dim3(1), dim3(1024), 0, st, P);
+    hipLaunchKernelGGL(sort_scatter_kernel, dim3(kSortBlocks), dim3(kBlock), 0, st, S, P, q);
+    qshade = P.qs;
+  }
   if (N.q)
-    hipLaunchKernelGGL(shade_kernel<true>, dim3(g), dim3(kBlock), 0, st, S, P, prm, q, qnext, N);
+    hipLaunchKernelGGL(shade_kernel<true>, dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
   else
-    hipLaunchKernelGGL(shade_kernel<false>, dim3(g), dim3(kBlock), 0, st, S, P, prm, q, qnext, N);
+    hipLaunchKernelGGL(shade_kernel<false>, dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
   if (small)
     hipLaunchKernelGGL(shadow_kernel<32>, dim3(gt), dim3(kTraceBlock), 0, st, S, P);
   else

@@ -88,6 +88,9 @@ PtMaterial make_material(const rrhip_material_desc& m) {
                mul(d3(m.refraction_color), m.glass_gain * under));
   const double av[3] = {alb.x, alb.y, alb.z};
   for (int c = 0; c < 3; c++) g.nrc_albedo[c] = (float)std::min(std::max(av[c], 0.0), 1.0);
+  g.sort_key = 1;
+  for (int i = 0; i < 5; i++)
+    if (g.w[i] > 0.f) g.sort_key += 1u << i;
   g.nrc_rough = 1.f;
   if (ws > 0 || wg > 0) g.nrc_rough = (float)(wg > ws ? m.glass_roughness : m.specular_roughness);
   return g;

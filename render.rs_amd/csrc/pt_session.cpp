@@ -42,8 +42,9 @@ struct rrhip_session {
   NrcQuery* nrc_bq = nullptr;
   float* nrc_bt = nullptr;
   uint32_t nrc_bcap = 0;
-  // Launch mode 1: one hipGraph per pass size, captured on first use and
-  // replayed with the first sample index written to d_sample0.
+  // Launch mode bit 0: one hipGraph per pass size, captured on first use
+  // and replayed with the first sample index written to d_sample0; bit 1:
+  // the live queue sorted by shading class before shade.
   uint32_t launch_mode = 0;
   uint64_t* d_sample0 = nullptr;
   std::vector<std::pair<uint32_t, hipGraphExec_t>> graphs;
@@ -185,11 +186,13 @@ hipError_t enqueue_pass(rrhip_session* s, uint32_t n, uint64_t sample0, const ui
   const uint32_t total = n * prm.npix;
   N.train_base = total;  // training paths follow this pass's rendering paths
   const uint32_t paths = total + N.ntrain;
-  hipError_t e = pt_launch_raygen(sc->pt, s->P, prm, total, N, sample0_dev, st);
+  PathBuffers P = s->P;
+  if (!(s->launch_mode & 2)) P.qs = nullptr;
+  hipError_t e = pt_launch_raygen(sc->pt, P, prm, total, N, sample0_dev, st);
   for (uint32_t b = 0; b < prm.max_depth && e == hipSuccess; b++) {
-    const uint32_t* q = (b & 1) ? s->P.q1 : s->P.q0;
-    uint32_t* qn = (b & 1) ? s->P.q0 : s->P.q1;
-    e = pt_launch_bounce(sc->pt, s->P, prm, q, qn, paths, N, st);
+    const uint32_t* q = (b & 1) ? P.q1 : P.q0;
+    uint32_t* qn = (b & 1) ? P.q0 : P.q1;
+    e = pt_launch_bounce(sc->pt, P, prm, q, qn, paths, N, st);
   }
   if (nrc && e == hipSuccess) {
     // cache radiance at the terminal vertices (and training tails), then
@@ -244,7 +247,7 @@ int rrhip_render_samples(rrhip_session* s, uint32_t start, uint32_t count) {
   for (uint32_t done = 0; done < count;) {
     const uint32_t n = std::min(s->spp_pass, count - done);
     const uint64_t sample0 = s->sample_base + (uint64_t)start + done;
-    if (s->launch_mode == 1) {
+    if (s->launch_mode & 1) {
       hipGraphExec_t exec = nullptr;
       RRHIP_HIP_TRY(pass_graph(s, n, st, &exec));
       RRHIP_HIP_TRY(pt_launch_set_u64(s->d_sample0, sample0, st));
@@ -265,12 +268,59 @@ int rrhip_render_samples(rrhip_session* s, uint32_t start, uint32_t count) {
 
 int rrhip_session_set_launch_mode(rrhip_session* s, uint32_t mode) {
   if (int st = check_session(s)) return st;
-  if (mode > 1) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_set_launch_mode: mode must be 0 or 1");
+  if (mode > 3) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_set_launch_mode: mode must be 0..3");
   RRHIP_HIP_TRY(hipSetDevice(s->scene->ctx->device));
   RRHIP_HIP_TRY(hipStreamSynchronize(s->scene->ctx->stream));
-  if (mode == 1 && !s->d_sample0) RRHIP_HIP_TRY(alloc_n(s, &s->d_sample0, 1));
+  if ((mode & 1) && !s->d_sample0) RRHIP_HIP_TRY(alloc_n(s, &s->d_sample0, 1));
   drop_graphs(s);
   s->launch_mode = mode;
+  return RRHIP_OK;
+}
+
+int rrhip_session_autotune(rrhip_session* s, uint32_t samples, uint32_t* best_mode, float* best_ms) {
+  if (int st = check_session(s)) return st;
+  if (samples == 0) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_autotune: samples must be > 0");
+  rrhip_scene* sc = s->scene;
+  RRHIP_HIP_TRY(hipSetDevice(sc->ctx->device));
+  hipStream_t st = sc->ctx->stream;
+  // Keep the accumulator, statistics and (with a cache) nothing else: the
+  // trial renders use sample indices past 2^40 and are discarded.
+  const uint64_t abytes = (uint64_t)s->prm.npix * RRHIP_ACCUM_STRIDE * sizeof(double);
+  std::vector<double> keep(abytes / sizeof(double));
+  unsigned long long stats[2];
+  RRHIP_HIP_TRY(hipMemcpyAsync(keep.data(), s->d_accum, abytes, hipMemcpyDeviceToHost, st));
+  RRHIP_HIP_TRY(hipMemcpyAsync(stats, s->P.stats, sizeof(stats), hipMemcpyDeviceToHost, st));
+  RRHIP_HIP_TRY(hipStreamSynchronize(st));
+  const uint64_t base = s->sample_base;
+  const uint64_t done = s->samples_done;
+  rrhip_nrc* nrc = s->nrc;
+  s->nrc = nullptr;  // trials never train the cache
+  s->sample_base = base + (1ull << 40);
+  uint32_t best = 0;
+  float best_t = 0.f;
+  int status = RRHIP_OK;
+  for (uint32_t mode = 0; mode < 4 && status == RRHIP_OK; mode++) {
+    status = rrhip_session_set_launch_mode(s, mode);
+    if (status == RRHIP_OK) status = rrhip_render_samples(s, 0, std::min(samples, s->spp_pass));  // warm-up / capture
+    if (status == RRHIP_OK) status = rrhip_render_samples(s, 0, samples);
+    if (status == RRHIP_OK && (mode == 0 || s->last_ms < best_t)) {
+      best = mode;
+      best_t = (float)s->last_ms;
+    }
+  }
+  s->nrc = nrc;
+  s->sample_base = base;
+  s->samples_done = done;
+  if (status == RRHIP_OK) status = rrhip_session_set_launch_mode(s, best);
+  if (status == RRHIP_OK) {
+    hipError_t e = hipMemcpyAsync(s->d_accum, keep.data(), abytes, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->P.stats, stats, sizeof(stats), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) status = fail_hip(e, "rrhip_session_autotune");
+  }
+  if (status != RRHIP_OK) return status;
+  if (best_mode) *best_mode = best;
+  if (best_ms) *best_ms = best_t;
   return RRHIP_OK;
 }
 

@@ -33,7 +33,8 @@ struct alignas(16) PtMaterial {
   int32_t area_light;
   float nrc_albedo[3];  // radiance-cache input: summed lobe albedo
   float nrc_rough;      // radiance-cache input: roughness of the strongest glossy lobe
-  float pad[2];
+  uint32_t sort_key;    // shading class for sorted queues: 1 + bitmask of the active lobes
+  float pad;
 };
 
 enum : uint32_t { kLightPoint = 0, kLightDistant = 1, kLightRect = 2, kLightSphere = 3, kLightDisk = 4, kLightDome = 5 };
@@ -166,6 +167,8 @@ struct PathBuffers {
   uint32_t* counts; // [0] live paths, [1] next live, [2] shadow rays, [3] unused
   unsigned long long* stats;  // [0] extension rays, [1] shadow rays
   uint32_t* job;    // batches: job of each slot (null in sessions)
+  uint32_t* qs;     // sorted-queue mode: the live queue grouped by shading class
+  uint32_t* sort_hist;  // sorted-queue mode: per-block class counts
 };
 
 // One radiance-cache query / training sample (the 64-byte rrhip_nrc_query

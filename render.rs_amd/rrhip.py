@@ -41,7 +41,7 @@ EXPORTED = [
     "rrhip_nrc_set_params", "rrhip_nrc_infer", "rrhip_nrc_train_step", "rrhip_nrc_gradients",
     "rrhip_session_set_nrc", "rrhip_gen_density_grid",
     "rrhip_batch_create", "rrhip_batch_destroy", "rrhip_batch_render", "rrhip_batch_read_accum",
-    "rrhip_batch_get_stats", "rrhip_session_set_launch_mode",
+    "rrhip_batch_get_stats", "rrhip_session_set_launch_mode", "rrhip_session_autotune",
 ]
 
 
@@ -257,6 +257,7 @@ def load():
         "rrhip_scene_autotune": (i32, [vp, vp, u64, i32, vp, ctypes.POINTER(TraceTuning),
                                        ctypes.POINTER(ctypes.c_float)]),
         "rrhip_session_set_launch_mode": (i32, [vp, u32]),
+        "rrhip_session_autotune": (i32, [vp, u32, ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_float)]),
         "rrhip_batch_create": (i32, [vp, u32, vp, u32, ctypes.POINTER(vp)]),
         "rrhip_batch_destroy": (i32, [vp]),
         "rrhip_batch_render": (i32, [vp, vp, ctypes.POINTER(ctypes.c_double)]),
@@ -465,8 +466,15 @@ class Session:
             _check(load().rrhip_load_accum(self.h, _ptr(acc)))
 
     def set_launch_mode(self, mode):
-        """0: kernels launched one by one; 1: one hipGraph per pass size."""
+        """Bit 0: one hipGraph per pass size; bit 1: queue sorted by shading class."""
         _check(load().rrhip_session_set_launch_mode(self.h, mode))
+
+    def autotune(self, samples):
+        """Times every launch mode, keeps the fastest: returns (mode, ms)."""
+        mode = ctypes.c_uint32()
+        ms = ctypes.c_float()
+        _check(load().rrhip_session_autotune(self.h, samples, ctypes.byref(mode), ctypes.byref(ms)))
+        return mode.value, ms.value
 
     def set_nrc(self, nrc):
         """Attaches a radiance cache (Nrc) to the session, or detaches it (None)."""

@@ -233,35 +233,45 @@ def test_relmse_on_device_matches_numpy(ctx):
     ps.close()
 
 
-def test_graph_launches_equal_stream_launches(ctx):
-    """Launch mode 1 (one captured hipGraph per pass size, replayed) gives
-    the accumulators of launch mode 0 bit for bit, replay after replay,
-    with other sessions' eager work between the replays; also with a
-    radiance cache attached (its optimiser kernels are in the graph)."""
-    say("hipGraph pass replays vs stream launches (box room 48x48, instanced field with cache)")
+def test_launch_modes_equal_stream_launches(ctx):
+    """Launch modes (bit 0: one captured hipGraph per pass size, replayed;
+    bit 1: queue sorted by shading class) give the accumulators of plain
+    launches bit for bit, replay after replay, with other sessions' eager
+    work between the replays; also with a radiance cache attached (its
+    optimiser kernels are in the graph). The autotuner leaves the
+    accumulator as it was."""
+    say("launch modes vs stream launches (box room 48x48, instanced field with cache)")
     s = scenes.box_room(48, 48)
     ps = rrhip.PtScene(ctx, s.desc())
     p = rrhip.RenderParams.make(max_depth=8, samples_per_pass=4)
     ref = ps.session(p)
     ref.render(0, 14)  # passes of 4, 4, 4, 2
-    g = ps.session(p)
-    g.set_launch_mode(1)
     other = ps.session(rrhip.RenderParams.make(max_depth=8, samples_per_pass=3))
-    for start, count in ((0, 6), (6, 5), (11, 3)):
-        g.render(start, count)
-        other.render(start, count)  # eager work between replays
-    assert g.accum().tobytes() != b"" and np.allclose(g.accum(), ref.accum(), rtol=1e-12, atol=1e-12)
-    g2 = ps.session(p)
-    g2.set_launch_mode(1)
-    g2.render(0, 14)
-    assert g2.accum().tobytes() == ref.accum().tobytes()
-    for x in (ref, g, g2, other):
+    for mode in (1, 2, 3):
+        g = ps.session(p)
+        g.set_launch_mode(mode)
+        for start, count in ((0, 6), (6, 5), (11, 3)):
+            g.render(start, count)
+            other.render(start, count)  # eager work between replays
+        assert np.allclose(g.accum(), ref.accum(), rtol=1e-12, atol=1e-12), mode
+        g2 = ps.session(p)
+        g2.set_launch_mode(mode)
+        g2.render(0, 14)
+        assert g2.accum().tobytes() == ref.accum().tobytes(), mode
+        g.close()
+        g2.close()
+    before = ref.accum()
+    mode, ms = ref.autotune(4)
+    say(f"autotune picked mode {mode} ({ms:.3f} ms for 4 spp)")
+    assert 0 <= mode <= 3 and ms > 0
+    assert ref.accum().tobytes() == before.tobytes()
+    for x in (ref, other):
         x.close()
     ps.close()
     f = scenes.instanced_field(96, 54, n_instances=300, levels=3, n_point_lights=16)
     pf = rrhip.PtScene(ctx, f.desc())
     out = []
-    for mode in (0, 1):
+    for mode in (0, 3):
         c = rrhip.Nrc(ctx, rrhip.NrcConfig.make(log2_table=14, train_paths=2048, batch=2048))
         se = pf.session(rrhip.RenderParams.make(max_depth=12, samples_per_pass=2))
         se.set_nrc(c)
