@@ -140,6 +140,8 @@ def time_to_relmse(ctx, args, with_cpu):
         if cache is not None:
             info = cache.info()
             out["optimizer_steps"] = info["steps"]
+            out["cache_queries_per_frame"] = int(st["cache_queries"] / frames)
+            out["training_samples_per_frame"] = int(st["cache_samples"] / frames)
             cache.close()
         se.close()
         return out

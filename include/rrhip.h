@@ -360,6 +360,8 @@ typedef struct rrhip_session_stats {
   uint32_t samples_per_pass;
   uint32_t reserved;
   double last_render_ms;     /* device time of the last rrhip_render_samples */
+  uint64_t cache_queries;    /* radiance-cache queries (terminal vertices and training tails) */
+  uint64_t cache_samples;    /* radiance-cache training samples */
 } rrhip_session_stats;
 
 /* Creates a wavefront path-tracing session on a scene with a camera.

@@ -134,7 +134,7 @@ int rrhip_batch_create(rrhip_ctx* ctx, uint32_t num_jobs, const rrhip_batch_job*
   if (e == hipSuccess) e = alloc_n(b, &P.sc, n);
   if (e == hipSuccess) e = alloc_n(b, &P.job, n);
   if (e == hipSuccess) e = alloc_n(b, &P.counts, 8);
-  if (e == hipSuccess) e = alloc_n(b, &P.stats, 2);
+  if (e == hipSuccess) e = alloc_n(b, &P.stats, 4);
   // The whole schedule's job tables, uploaded once.
   std::vector<std::vector<PtJob>> sched;
   if (e == hipSuccess) {
@@ -171,7 +171,7 @@ int rrhip_batch_render(rrhip_batch* b, double* job_ms, double* total_ms) {
   const uint32_t nj = (uint32_t)b->jobs.size();
   for (uint32_t j = 0; j < nj; j++)
     RRHIP_HIP_TRY(hipMemsetAsync(b->accum[j], 0, (size_t)b->npix[j] * RRHIP_ACCUM_STRIDE * sizeof(double), st));
-  RRHIP_HIP_TRY(hipMemsetAsync(b->P.stats, 0, 2 * sizeof(unsigned long long), st));
+  RRHIP_HIP_TRY(hipMemsetAsync(b->P.stats, 0, 4 * sizeof(unsigned long long), st));
   std::vector<hipEvent_t> ev(sched.size() + 1, nullptr);
   for (auto& x : ev) RRHIP_HIP_TRY(hipEventCreate(&x));
   std::vector<uint32_t> last_pass(nj, 0);

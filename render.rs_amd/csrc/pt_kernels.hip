@@ -916,7 +916,7 @@ __global__ __launch_bounds__(256) void nrc_count_kernel(PathBuffers P, NrcPathBu
 // Exclusive prefix sum of cnt[0, n) in place (one block of 1024 threads,
 // each over a contiguous chunk); the clamped total into N.counts[1].
 __global__ __launch_bounds__(1024) void nrc_scan_kernel(uint32_t* __restrict__ cnt, uint32_t n, uint32_t cap,
-                                                       NrcPathBuffers N) {
+                                                       NrcPathBuffers N, unsigned long long* __restrict__ stats) {
   __shared__ uint32_t part[1024];
   const uint32_t t = threadIdx.x;
   const uint32_t chunk = (n + 1023) / 1024;
@@ -937,7 +937,11 @@ __global__ __launch_bounds__(1024) void nrc_scan_kernel(uint32_t* __restrict__ c
     cnt[i] = run;
     run += c;
   }
-  if (t == 1023) N.counts[1] = min(part[1023], cap);
+  if (t == 1023) {
+    N.counts[1] = min(part[1023], cap);
+    stats[2] += N.counts[0];           // cache queries of the pass
+    stats[3] += min(part[1023], cap);  // training samples of the pass
+  }
 }
 
 __global__ __launch_bounds__(256) void nrc_scatter_kernel(PathBuffers P, NrcPathBuffers N, float clamp,
@@ -973,7 +977,7 @@ hipError_t pt_launch_nrc_samples(const PathBuffers& P, const NrcPathBuffers& N, 
   if (N.ntrain == 0) return hipSuccess;
   const uint32_t g = (N.ntrain + 255) / 256;
   hipLaunchKernelGGL(nrc_count_kernel, dim3(g), dim3(256), 0, st, P, N, clamp, scratch);
-  hipLaunchKernelGGL(nrc_scan_kernel, dim3(1), dim3(1024), 0, st, scratch, N.ntrain, cap, N);
+  hipLaunchKernelGGL(nrc_scan_kernel, dim3(1), dim3(1024), 0, st, scratch, N.ntrain, cap, N, P.stats);
   hipLaunchKernelGGL(nrc_scatter_kernel, dim3(g), dim3(256), 0, st, P, N, clamp, scratch, cap, bq, bt);
   return hipGetLastError();
 }

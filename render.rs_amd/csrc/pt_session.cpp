@@ -149,11 +149,11 @@ int rrhip_session_create(rrhip_scene* scene, const rrhip_render_params* params, 
   s->prm.npix = (uint32_t)npix;
   hipError_t e = alloc_paths(s, s->slots);
   unsigned long long* stats = nullptr;
-  if (e == hipSuccess) e = alloc_n(s, &stats, 2);
+  if (e == hipSuccess) e = alloc_n(s, &stats, 4);
   s->P.stats = stats;
   if (e == hipSuccess) e = alloc_n(s, &s->d_accum, npix * RRHIP_ACCUM_STRIDE);
   if (e == hipSuccess) e = hipMemset(s->d_accum, 0, npix * RRHIP_ACCUM_STRIDE * sizeof(double));
-  if (e == hipSuccess) e = hipMemset(stats, 0, 2 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMemset(stats, 0, 4 * sizeof(unsigned long long));
   if (e != hipSuccess) {
     free_session(s);
     return fail_hip(e, "rrhip_session_create");
@@ -287,7 +287,7 @@ int rrhip_session_autotune(rrhip_session* s, uint32_t samples, uint32_t* best_mo
   // trial renders use sample indices past 2^40 and are discarded.
   const uint64_t abytes = (uint64_t)s->prm.npix * RRHIP_ACCUM_STRIDE * sizeof(double);
   std::vector<double> keep(abytes / sizeof(double));
-  unsigned long long stats[2];
+  unsigned long long stats[4];
   RRHIP_HIP_TRY(hipMemcpyAsync(keep.data(), s->d_accum, abytes, hipMemcpyDeviceToHost, st));
   RRHIP_HIP_TRY(hipMemcpyAsync(stats, s->P.stats, sizeof(stats), hipMemcpyDeviceToHost, st));
   RRHIP_HIP_TRY(hipStreamSynchronize(st));
@@ -374,13 +374,15 @@ int rrhip_relmse(rrhip_session* s, rrhip_session* ref, double eps, double* out) 
 int rrhip_session_get_stats(rrhip_session* s, rrhip_session_stats* out) {
   if (int st = check_session(s)) return st;
   if (!out) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_get_stats: null output");
-  unsigned long long st[2] = {0, 0};
+  unsigned long long st[4] = {0, 0, 0, 0};
   RRHIP_HIP_TRY(hipSetDevice(s->scene->ctx->device));
   RRHIP_HIP_TRY(hipMemcpyAsync(st, s->P.stats, sizeof(st), hipMemcpyDeviceToHost, s->scene->ctx->stream));
   RRHIP_HIP_TRY(hipStreamSynchronize(s->scene->ctx->stream));
   *out = rrhip_session_stats{};
   out->extension_rays = st[0];
   out->shadow_rays = st[1];
+  out->cache_queries = st[2];
+  out->cache_samples = st[3];
   out->samples_rendered = s->samples_done;
   out->path_slots = s->slots;
   out->samples_per_pass = s->spp_pass;

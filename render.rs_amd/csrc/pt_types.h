@@ -165,7 +165,7 @@ struct PathBuffers {
   float4* sd;       // shadow direction.xyz
   float4* sc;       // shadow contribution rgb (already times beta)
   uint32_t* counts; // [0] live paths, [1] next live, [2] shadow rays, [3] unused
-  unsigned long long* stats;  // [0] extension rays, [1] shadow rays
+  unsigned long long* stats;  // [0] extension rays, [1] shadow rays, [2] cache queries, [3] training samples
   uint32_t* job;    // batches: job of each slot (null in sessions)
   uint32_t* qs;     // sorted-queue mode: the live queue grouped by shading class
   uint32_t* sort_hist;  // sorted-queue mode: per-block class counts

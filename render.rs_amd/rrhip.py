@@ -149,7 +149,8 @@ class SessionStats(ctypes.Structure):
     _fields_ = [("extension_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
                 ("samples_rendered", ctypes.c_uint64), ("path_slots", ctypes.c_uint64),
                 ("samples_per_pass", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
-                ("last_render_ms", ctypes.c_double)]
+                ("last_render_ms", ctypes.c_double), ("cache_queries", ctypes.c_uint64),
+                ("cache_samples", ctypes.c_uint64)]
 
 
 class NrcConfig(ctypes.Structure):
