@@ -339,6 +339,14 @@ def main():
         scene.trace_any_device(d_rays, N_RAYS, d_occ)
     any_ms = scene.time_trace_device(d_rays, N_RAYS, d_occ, args.steps, any_hit=True) / args.steps
 
+    # PCIe-inclusive rate of the host-buffer entry point (rays in, hits out
+    # through pageable host memory); reported beside, never as, the value.
+    scene.trace_closest(rays)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        scene.trace_closest(rays)
+    host_mrays = N_RAYS * 3 / (time.perf_counter() - t0) / 1e6
+
     if args.no_cost:
         nodes, tris = 0, 0
     else:
@@ -374,6 +382,7 @@ def main():
         },
         "kernel_ms": round(kernel_ms, 4),
         "any_hit_mrays_s": round(N_RAYS / (any_ms / 1e3) / 1e6, 2),
+        "host_buffers_mrays_s": round(host_mrays, 1),
         "traversal": {"node_fetches_per_ray": round(nodes / N_RAYS, 3),
                       "triangle_fetches_per_ray": round(tris / N_RAYS, 3),
                       "algorithmic_bytes_per_ray": round(alg_bytes / N_RAYS, 1)},
