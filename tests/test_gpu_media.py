@@ -90,7 +90,7 @@ def test_absorbing_slab_matches_beer_lambert(ctx):
             want[y, x] = np.exp(-0.8 * L).mean()
     got = m[:, 0].reshape(H, W)
     err = np.abs(got - want)
-    se = np.sqrt(var[:, 0] / n).reshape(H, W)
+    se = np.sqrt(var[:, 0] / n[:, 0]).reshape(H, W)
     say(f"max |mean - exp(-sigma L)| {err.max():.4f}, median {np.median(err):.4f}")
     # the per-pixel statistical bound plus the footprint-quadrature error
     assert (err <= 4 * se + 0.01).mean() > 0.99
@@ -109,7 +109,7 @@ def test_scattering_cloud_in_a_white_furnace(ctx):
     acc = render(ctx, s, rrhip.RenderParams.make(max_depth=64, firefly_clamp=0.0), 0, 512)
     m, var, n = mean_var(acc)
     mean = m.mean()
-    se = np.sqrt(var.mean() / n.mean() / m.size)
+    se = np.sqrt(var.mean() / n.mean() / m.size)  # standard error of the image mean
     say(f"furnace mean {mean:.5f} (standard error {se:.5f})")
     assert abs(mean - 1.0) < max(4 * se, 2e-3)
 
