@@ -155,7 +155,7 @@ def time_to_relmse(ctx, args, with_cpu):
         "reference_spp": args.ref_spp, "reference_render_s": round(ref_s, 2),
         "plain": plain, "nrc": nrc,
         "nrc_config": "16 levels x 2 features x 2^19, 48-64x5-3 MLP bf16 on MFMA, Adam 1e-2, 4 x 16384 samples "
-                      "and 65536 training paths per frame, area-spread c = 0.01",
+                      "and 65536 training paths per frame, area-spread c = 1e-4",
         "speedup": (round(plain["seconds"] / nrc["seconds"], 3)
                     if plain["seconds"] and nrc["seconds"] else None),
     }

@@ -445,7 +445,7 @@ typedef struct rrhip_nrc_info {
  * 1e-4), He-uniform weights, both from PCG32(seed)). config = NULL takes
  * the defaults: 16 levels, 2^19 entries, resolutions 16..2048, Adam
  * (1e-2, 0.9, 0.99, 1e-15), batch 16384, 4 steps and 65536 training
- * paths per pass, c = 0.01. */
+ * paths per pass, c = 1e-4. */
 int rrhip_nrc_create(rrhip_ctx* ctx, const rrhip_nrc_config* config, rrhip_nrc** out);
 int rrhip_nrc_destroy(rrhip_nrc* nrc);
 int rrhip_nrc_get_info(rrhip_nrc* nrc, rrhip_nrc_info* info);

@@ -45,7 +45,7 @@ rrhip_nrc_config default_config() {
   c.batch = 16384;
   c.steps_per_pass = 4;
   c.train_paths = 65536;
-  c.spread = 0.01f;
+  c.spread = 1e-4f;
   return c;
 }
 
