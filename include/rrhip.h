@@ -485,7 +485,10 @@ typedef struct rrhip_batch_job {
   rrhip_render_params params;  /* max_depth, rr_start, firefly_clamp, sample_base (samples_per_pass ignored) */
 } rrhip_batch_job;
 
-/* samples_per_pass = 0 picks max(1, min(64, 2^22 / total pixels)). */
+/* samples_per_pass > 0: every unfinished job renders that many samples per
+ * pass (its accumulator then equals a session's with the same
+ * samples_per_pass). 0: passes of up to 2^24 path slots filled shortest
+ * job first (fewest samples x pixels), so small jobs complete early. */
 int rrhip_batch_create(rrhip_ctx* ctx, uint32_t num_jobs, const rrhip_batch_job* jobs, uint32_t samples_per_pass,
                        rrhip_batch** out);
 int rrhip_batch_destroy(rrhip_batch* batch);

@@ -17,7 +17,9 @@ struct rrhip_batch {
   std::vector<rrhip_batch_job> jobs;
   std::vector<uint32_t> npix;
   uint32_t spp_pass = 1;
-  uint64_t cap = 0;  // path slots
+  bool auto_schedule = false;
+  uint64_t slot_budget = 0;  // auto schedule: path slots per pass
+  uint64_t cap = 0;          // path slots
   uint64_t total_pix = 0;
   PathBuffers P{};
   std::vector<void*> mem;
@@ -43,33 +45,62 @@ void free_batch(rrhip_batch* b) {
   delete b;
 }
 
-// The pass schedule: for pass k, the jobs still rendering, with their
-// sample count, first sample, slot and pixel offsets.
+// The pass schedule: for each pass, the jobs rendering in it with their
+// sample count, first sample, slot and pixel offsets. With a fixed
+// samples_per_pass every unfinished job renders that many samples per
+// pass; with samples_per_pass = 0 (auto) passes are filled up to a slot
+// budget shortest job first (fewest samples x pixels), so small jobs
+// finish in the first passes while large ones keep the device full.
 std::vector<std::vector<PtJob>> schedule(const rrhip_batch* b) {
+  const size_t nj = b->jobs.size();
+  std::vector<uint64_t> done(nj, 0);
+  std::vector<size_t> order(nj);
+  for (size_t j = 0; j < nj; j++) order[j] = j;
+  if (b->auto_schedule)
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+      return (uint64_t)b->jobs[x].spp * b->npix[x] < (uint64_t)b->jobs[y].spp * b->npix[y];
+    });
   std::vector<std::vector<PtJob>> out;
-  for (uint32_t k = 0;; k++) {
+  for (;;) {
+    std::vector<std::pair<size_t, uint32_t>> take;  // (job, samples this pass)
+    uint64_t left = b->slot_budget;
+    for (size_t j : order) {
+      const uint64_t rem = b->jobs[j].spp - done[j];
+      if (rem == 0) continue;
+      uint64_t n;
+      if (b->auto_schedule) {
+        n = std::min<uint64_t>(rem, left / b->npix[j]);
+        if (n == 0 && take.empty()) n = 1;  // a job larger than the budget still progresses
+      } else {
+        n = std::min<uint64_t>(rem, b->spp_pass);
+      }
+      if (n == 0) continue;
+      take.push_back({j, (uint32_t)n});
+      left -= std::min<uint64_t>(left, n * b->npix[j]);
+    }
+    if (take.empty()) break;
+    std::sort(take.begin(), take.end());  // slots in job order
     std::vector<PtJob> pass;
     uint32_t slot = 0, pix = 0;
-    for (size_t j = 0; j < b->jobs.size(); j++) {
+    for (const auto& tj : take) {
+      const size_t j = tj.first;
       const rrhip_batch_job& J = b->jobs[j];
-      const uint64_t done = (uint64_t)k * b->spp_pass;
-      if (done >= J.spp) continue;
       PtJob t{};
       t.S = J.scene->pt;
       t.prm.max_depth = J.params.max_depth;
       t.prm.rr_start = J.params.rr_start;
       t.prm.firefly_clamp = (float)J.params.firefly_clamp;
       t.prm.npix = b->npix[j];
-      t.prm.spp_pass = (uint32_t)std::min<uint64_t>(b->spp_pass, J.spp - done);
-      t.prm.sample0 = J.params.sample_base + done;
+      t.prm.spp_pass = tj.second;
+      t.prm.sample0 = J.params.sample_base + done[j];
       t.slot_begin = slot;
       t.pix_begin = pix;
       t.accum = b->accum[j];
       slot += t.prm.spp_pass * t.prm.npix;
       pix += t.prm.npix;
+      done[j] += tj.second;
       pass.push_back(t);
     }
-    if (pass.empty()) break;
     out.push_back(std::move(pass));
   }
   return out;
@@ -96,8 +127,7 @@ int rrhip_batch_create(rrhip_ctx* ctx, uint32_t num_jobs, const rrhip_batch_job*
     total_pix += (uint64_t)J.scene->pt.cam.width * J.scene->pt.cam.height;
   }
   uint32_t spp = samples_per_pass;
-  if (spp == 0) spp = (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(1, (4ull << 20) / total_pix));
-  if ((uint64_t)spp * total_pix >= (1ull << 31))
+  if (spp > 0 && (uint64_t)spp * total_pix >= (1ull << 31))
     return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_batch_create: samples_per_pass x pixels exceeds 2^31");
   RRHIP_HIP_TRY(hipSetDevice(ctx->device));
   auto* b = new (std::nothrow) rrhip_batch();
@@ -105,8 +135,9 @@ int rrhip_batch_create(rrhip_ctx* ctx, uint32_t num_jobs, const rrhip_batch_job*
   b->ctx = ctx;
   b->jobs.assign(jobs, jobs + num_jobs);
   b->spp_pass = spp;
+  b->auto_schedule = spp == 0;
+  b->slot_budget = 1ull << 24;
   b->total_pix = total_pix;
-  b->cap = (uint64_t)spp * total_pix;
   hipError_t e = hipSuccess;
   for (uint32_t j = 0; j < num_jobs && e == hipSuccess; j++) {
     const uint32_t np = jobs[j].scene->pt.cam.width * jobs[j].scene->pt.cam.height;
@@ -114,6 +145,20 @@ int rrhip_batch_create(rrhip_ctx* ctx, uint32_t num_jobs, const rrhip_batch_job*
     double* a = nullptr;
     e = alloc_n(b, &a, (uint64_t)np * RRHIP_ACCUM_STRIDE);
     b->accum.push_back(a);
+  }
+  // capacity: the largest pass of the schedule
+  std::vector<std::vector<PtJob>> sched;
+  if (e == hipSuccess) {
+    sched = schedule(b);
+    for (const auto& pass : sched) {
+      uint64_t n = 0;
+      for (const PtJob& t : pass) n += (uint64_t)t.prm.spp_pass * t.prm.npix;
+      b->cap = std::max(b->cap, n);
+    }
+    if (b->cap >= (1ull << 31)) {
+      free_batch(b);
+      return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_batch_create: a pass exceeds 2^31 path slots");
+    }
   }
   PathBuffers& P = b->P;
   const uint64_t n = b->cap;
@@ -136,9 +181,7 @@ int rrhip_batch_create(rrhip_ctx* ctx, uint32_t num_jobs, const rrhip_batch_job*
   if (e == hipSuccess) e = alloc_n(b, &P.counts, 8);
   if (e == hipSuccess) e = alloc_n(b, &P.stats, 4);
   // The whole schedule's job tables, uploaded once.
-  std::vector<std::vector<PtJob>> sched;
   if (e == hipSuccess) {
-    sched = schedule(b);
     b->passes = (uint32_t)sched.size();
     e = alloc_n(b, &b->d_jobs, (uint64_t)b->passes * num_jobs);
   }
@@ -224,7 +267,7 @@ int rrhip_batch_get_stats(rrhip_batch* b, rrhip_session_stats* out) {
   out->extension_rays = b->last_rays[0];
   out->shadow_rays = b->last_rays[1];
   out->path_slots = b->cap;
-  out->samples_per_pass = b->spp_pass;
+  out->samples_per_pass = b->auto_schedule ? 0 : b->spp_pass;
   out->reserved = b->passes;
   return RRHIP_OK;
 }
