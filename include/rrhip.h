@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define RRHIP_ABI_VERSION 3u
+#define RRHIP_ABI_VERSION 4u
 #define RRHIP_NO_HIT 0xFFFFFFFFu
 
 /* Status codes. render.rs returns anyhow::Result with a message
@@ -240,7 +240,8 @@ typedef struct rrhip_render_params {
 
 /* Per-pixel accumulator layout of rrhip_read_accum / orc_render:
  * 8 doubles per pixel, row-major: sum r, g, b; sum of squares r, g, b;
- * sample count; 0. */
+ * sample count; sum of squared luminance (0.2126 r + 0.7152 g + 0.0722 b).
+ * All sums are of firefly-clamped samples. */
 #define RRHIP_ACCUM_STRIDE 8
 
 /* Scene statistics (what the upload built). */
@@ -394,6 +395,19 @@ int rrhip_session_set_launch_mode(rrhip_session* session, uint32_t mode);
  * (discarded: the accumulator, statistics and any attached cache are left
  * as they were), keeps the fastest and reports it with its milliseconds. */
 int rrhip_session_autotune(rrhip_session* session, uint32_t samples, uint32_t* best_mode, float* best_ms);
+/* Adaptive sampling: with tolerance > 0, rrhip_render_samples checks every
+ * pixel before samples 32, 48, 64, ... (index relative to sample_base) and
+ * stops sampling those whose 95 % confidence half-width of the mean
+ * luminance, 1.96 sqrt(var / n), is below tolerance * max(mean, 0.05);
+ * stopped pixels stay stopped until this call or rrhip_load_accum. 0 turns
+ * it off. Counterpart of WfSession::set_adaptive (renderer.rs:580-584) with
+ * the schedule of pt::render_adaptive (src/raytracer/pt/mod.rs:285-336):
+ * rendering samples [0, max_spp) from a cleared accumulator is that
+ * function. Passes then run as plain launches (no graph replay). */
+int rrhip_session_set_adaptive(rrhip_session* session, double tolerance);
+/* Mean sample count over the pixels (the accumulator's count column).
+ * Counterpart of WfSession::average_spp (renderer.rs:587-591). */
+int rrhip_session_average_spp(rrhip_session* session, double* out);
 /* relMSE of `session`'s mean image against `reference`'s (same size and
  * context): mean over pixels and channels of (x - r)^2 / (r^2 + eps),
  * reduced in FP64 on the GPU. No counterpart in render.rs (its tests use

@@ -159,6 +159,7 @@ def _pt_sigs():
                               ctypes.c_double)
     sig = {
         "orc_render": (i32, [vp, vp, u32, u32, vp, i32, vp]),
+        "orc_render_adaptive": (i32, [vp, vp, u32, f64, vp, i32]),
         "orc_bsdf_eval": (None, [vp, vp, vp, f64, vp, vp]),
         "orc_bsdf_eval_n": (None, [vp, vp, vp, f64, u64, vp, vp]),
         "orc_bsdf_sample_n": (None, [vp, vp, f64, u64, u64, u64, vp, vp, vp, vp]),
@@ -188,6 +189,17 @@ def render(desc, params, start, count, accum=None, threads=None):
     lib.orc_render(ctypes.addressof(desc), ctypes.addressof(params), start, count, _p(accum),
                    threads or default_threads(), ctypes.byref(rays))
     return accum, rays.value
+
+
+def render_adaptive(desc, params, max_spp, tol, threads=None):
+    """pt::render_adaptive (pt/mod.rs:285-336) into a fresh accumulator
+    (npix x 8 doubles; column 6 is each pixel's sample count)."""
+    lib = _pt_sigs()
+    npix = desc.camera.width * desc.camera.height
+    accum = np.zeros((npix, 8), np.float64)
+    lib.orc_render_adaptive(ctypes.addressof(desc), ctypes.addressof(params), max_spp, float(tol), _p(accum),
+                            threads or default_threads())
+    return accum
 
 
 def bsdf_eval(mat, wo, wi, eta):

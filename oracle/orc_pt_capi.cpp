@@ -5,6 +5,7 @@
 // exposes the BSDF and light sampler for unit tests.
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <thread>
 #include <vector>
 
@@ -13,6 +14,19 @@
 using namespace orc;
 
 namespace {
+
+// One firefly-clamped sample into an RRHIP_ACCUM record.
+void add_sample(double* a, const Vec3& l) {
+  const double lm = luminance(l);
+  a[0] += l.x;
+  a[1] += l.y;
+  a[2] += l.z;
+  a[3] += l.x * l.x;
+  a[4] += l.y * l.y;
+  a[5] += l.z * l.z;
+  a[6] += 1.0;
+  a[7] += lm * lm;
+}
 
 PbrParams pbr_from(const rrhip_material_desc* m) {
   rrhip_scene_desc d{};
@@ -57,13 +71,7 @@ int orc_render(const rrhip_scene_desc* desc, const rrhip_render_params* params, 
           Vec3 l = scene.trace(ray, rng, rp, &rays);
           double lm = luminance(l);
           if (rp.firefly_clamp > 0.0 && lm > rp.firefly_clamp) l = l * (rp.firefly_clamp / lm);
-          a[0] += l.x;
-          a[1] += l.y;
-          a[2] += l.z;
-          a[3] += l.x * l.x;
-          a[4] += l.y * l.y;
-          a[5] += l.z * l.z;
-          a[6] += 1.0;
+          add_sample(a, l);
         }
       }
     }
@@ -74,6 +82,61 @@ int orc_render(const rrhip_scene_desc* desc, const rrhip_render_params* params, 
   for (int t = 0; t < nt; t++) pool.emplace_back(work);
   for (auto& th : pool) th.join();
   if (rays_out) *rays_out = total_rays.load();
+  return 0;
+}
+
+// pt::render_adaptive (src/raytracer/pt/mod.rs:285-336): every pixel takes
+// samples 0, 1, ... (plus sample_base) until, from the 32nd sample on and
+// at every 16th, the 95 % confidence half-width of its mean luminance
+// (Welford running mean and M2, unbiased variance) is below
+// tol * max(mean, 0.05), or max_spp samples are taken. The samples are
+// ADDED into accum in the RRHIP_ACCUM layout (so accum[6] is the count).
+int orc_render_adaptive(const rrhip_scene_desc* desc, const rrhip_render_params* params, uint32_t max_spp,
+                        double tol, double* accum, int nthreads) {
+  Scene scene;
+  scene.load(*desc);
+  RenderParams rp;
+  rp.max_depth = params->max_depth;
+  rp.rr_start = params->rr_start;
+  rp.firefly_clamp = params->firefly_clamp;
+  rp.sample_base = params->sample_base;
+  const uint32_t W = scene.camera.width, H = scene.camera.height;
+  const uint32_t cap = std::max(max_spp, 1u);
+  std::atomic<uint32_t> next_row{0};
+  auto work = [&]() {
+    uint64_t rays = 0;
+    for (;;) {
+      uint32_t y = next_row.fetch_add(1);
+      if (y >= H) break;
+      for (uint32_t x = 0; x < W; x++) {
+        uint64_t pixel = (uint64_t)y * W + x;
+        double* a = accum + RRHIP_ACCUM_STRIDE * pixel;
+        double lum_mean = 0.0, m2 = 0.0;
+        for (uint32_t n = 0; n < cap;) {
+          Pcg32 rng = Pcg32::for_pixel_sample(pixel, rp.sample_base + n);
+          Ray ray = scene.camera_ray(x, y, rng);
+          Vec3 l = scene.trace(ray, rng, rp, &rays);
+          const double lm = luminance(l);
+          if (rp.firefly_clamp > 0.0 && lm > rp.firefly_clamp) l = l * (rp.firefly_clamp / lm);
+          add_sample(a, l);
+          n++;
+          const double lc = luminance(l);
+          const double delta = lc - lum_mean;
+          lum_mean += delta / (double)n;
+          m2 += delta * (lc - lum_mean);
+          if (n >= 32 && n % 16 == 0) {
+            const double var = m2 / ((double)n - 1.0);
+            const double ci = 1.96 * std::sqrt(var / (double)n);
+            if (ci < tol * std::max(lum_mean, 0.05)) break;
+          }
+        }
+      }
+    }
+  };
+  int nt = nthreads > 0 ? nthreads : 1;
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; t++) pool.emplace_back(work);
+  for (auto& th : pool) th.join();
   return 0;
 }
 

@@ -9,8 +9,9 @@ namespace rrhip {
 // Traversal stack entries available per lane to the instance path.
 constexpr uint32_t kPtTraceStack = 48;
 
-// Path slots [0, total) with slot = s * npix + pixel: camera rays of
-// samples prm.sample0 + s; fills the live queue (q0) and resets counters.
+// Path slots [0, total) with slot = s * npix + column: camera rays of
+// samples prm.sample0 + s of the column's pixel (prm.live[column], or the
+// column itself); fills the live queue (q0) and resets counters.
 // With N.ntrain > 0 also training paths in slots [total, total + ntrain).
 // sample0_dev (may be null) overrides prm.sample0 from device memory.
 hipError_t pt_launch_raygen(const PtSceneView& S, const PathBuffers& P, const PtParams& prm,
@@ -38,6 +39,17 @@ hipError_t pt_launch_resolve(const PathBuffers& P, const PtParams& prm, double* 
 // doubles into `partial`, <= 1024 blocks); the caller adds them.
 hipError_t pt_launch_relmse(const double* accum, const double* ref_accum, uint32_t npix, double eps,
                             double* partial, uint32_t blocks, hipStream_t st);
+
+// Per-block partial sums of the accumulators' sample counts (`blocks`
+// doubles into `partial`); the caller adds them.
+hipError_t pt_launch_count_sum(const double* accum, uint32_t npix, double* partial, uint32_t blocks,
+                               hipStream_t st);
+// Adaptive sampling: the pixels of live list `in` (n columns; null: pixel
+// = column) whose accumulators have not met the stopping rule of
+// pt/mod.rs:319-325 at tolerance `tol`, in column order, into `out`; their
+// number into *nlive. bcount holds ceil(n / 256) u32. n >= 1.
+hipError_t pt_launch_adaptive_prune(const double* accum, const uint32_t* in, uint32_t n, double tol,
+                                    uint32_t* bcount, uint32_t* out, uint32_t* nlive, hipStream_t st);
 
 // Training samples of the pass's training paths, in (path, vertex) order
 // (deterministic): inputs into bq, targets into bt (3 floats each), at

@@ -101,12 +101,14 @@ __global__ __launch_bounds__(kBlock) void raygen_kernel(PtSceneView S, PathBuffe
     if (slot < total) {
       pixel = slot % prm.npix;
       sample = prm.sample0 + slot / prm.npix;
+      if (prm.live) pixel = prm.live[pixel];
     } else {
       // Training path i of this pass: a hashed pixel and a PCG32 stream of
       // its own (top bit set: never a rendering sample index).
       const uint32_t i = slot - total;
       const uint64_t tag = ((prm.sample0 << 22) | i) & 0x7FFFFFFFFFFFFFFFull;
       pixel = (uint32_t)(Pcg32::mix(tag ^ 0x6e72637472616e69ull) % prm.npix);
+      if (prm.live) pixel = prm.live[pixel];
       sample = 0x8000000000000000ull | tag;
       flags |= kFlagTrain;
       N.tn[i] = 0;
@@ -569,37 +571,43 @@ __global__ void rotate_kernel(PathBuffers P) {
   P.counts[2] = 0;
 }
 
-// Adds pixel px's pass samples (slots slot_base + k * npix + px) into its
-// accumulator in sample order, with the firefly clamp (pt/mod.rs:88-95).
-__device__ __forceinline__ void resolve_one(const PathBuffers& P, const PtParams& prm, double* accum, uint32_t px,
+// Adds column col's pass samples (slots slot_base + k * npix + col) into
+// its pixel's accumulator in sample order, with the firefly clamp
+// (pt/mod.rs:88-95): sum of rgb, sum of rgb squared, count, and the sum of
+// squared luminance that adaptive sampling reads (pt/mod.rs:314-318).
+__device__ __forceinline__ void resolve_one(const PathBuffers& P, const PtParams& prm, double* accum, uint32_t col,
                                             uint32_t slot_base) {
-  double s[3] = {0, 0, 0}, q[3] = {0, 0, 0};
+  double s[3] = {0, 0, 0}, q[3] = {0, 0, 0}, ql = 0.0;
   for (uint32_t k = 0; k < prm.spp_pass; k++) {
-    const float4 l4 = P.L[slot_base + (size_t)k * prm.npix + px];
+    const float4 l4 = P.L[slot_base + (size_t)k * prm.npix + col];
     double l[3] = {l4.x, l4.y, l4.z};
-    const double lm = 0.2126 * l[0] + 0.7152 * l[1] + 0.0722 * l[2];
+    double lm = 0.2126 * l[0] + 0.7152 * l[1] + 0.0722 * l[2];
     if (prm.firefly_clamp > 0.f && lm > (double)prm.firefly_clamp) {
       const double sc = (double)prm.firefly_clamp / lm;
       l[0] *= sc;
       l[1] *= sc;
       l[2] *= sc;
+      lm = 0.2126 * l[0] + 0.7152 * l[1] + 0.0722 * l[2];
     }
     for (int c = 0; c < 3; c++) {
       s[c] += l[c];
       q[c] += l[c] * l[c];
     }
+    ql += lm * lm;
   }
+  const uint32_t px = prm.live ? prm.live[col] : col;
   double* a = accum + 8 * (size_t)px;
   for (int c = 0; c < 3; c++) {
     a[c] += s[c];
     a[3 + c] += q[c];
   }
   a[6] += (double)prm.spp_pass;
+  a[7] += ql;
 }
 
 __global__ __launch_bounds__(kBlock) void resolve_kernel(PathBuffers P, PtParams prm, double* __restrict__ accum) {
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < prm.npix; px += stride) resolve_one(P, prm, accum, px, 0);
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < prm.npix; c += stride) resolve_one(P, prm, accum, c, 0);
 }
 
 // ---- batched jobs: one set of launches over the path slots of many scenes ----
@@ -868,6 +876,120 @@ __global__ __launch_bounds__(256) void relmse_kernel(const double* __restrict__ 
 hipError_t pt_launch_relmse(const double* accum, const double* ref_accum, uint32_t npix, double eps,
                             double* partial, uint32_t blocks, hipStream_t st) {
   hipLaunchKernelGGL(relmse_kernel, dim3(blocks), dim3(256), 0, st, accum, ref_accum, npix, eps, partial);
+  return hipGetLastError();
+}
+
+namespace {
+
+// Per-block partial sums of the accumulators' sample counts.
+__global__ __launch_bounds__(256) void count_sum_kernel(const double* __restrict__ a, uint32_t npix,
+                                                        double* __restrict__ partial) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += stride) acc += a[8 * (size_t)p + 6];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Adaptive stopping rule of pt/mod.rs:319-325: after the 32-sample warm-up
+// a pixel stops once the 95 % confidence half-width of its mean luminance,
+// 1.96 sqrt(var / n) with var the unbiased sample variance, is below
+// tol * max(mean, 0.05). The moments come from the accumulator's sums
+// (luminance of the rgb sum, sum of squared luminance, count), evaluated
+// without fused operations so that the host can repeat the decision.
+__device__ bool pixel_converged(const double* a, double tol) {
+#pragma clang fp contract(off)
+  const double n = a[6];
+  if (n < 32.0) return false;
+  const double sum = 0.2126 * a[0] + 0.7152 * a[1] + 0.0722 * a[2];
+  const double mean = sum / n;
+  const double var = fmax((a[7] - sum * mean) / (n - 1.0), 0.0);
+  return 1.96 * sqrt(var / n) < tol * fmax(mean, 0.05);
+}
+
+constexpr uint32_t kPruneBlock = 256;
+
+// Column c of the live list `in` (null: column = pixel) keeps sampling.
+__device__ __forceinline__ bool keep_column(const double* accum, const uint32_t* in, uint32_t c, double tol,
+                                            uint32_t* px) {
+  *px = in ? in[c] : c;
+  return !pixel_converged(accum + 8 * (size_t)*px, tol);
+}
+
+__global__ __launch_bounds__(kPruneBlock) void prune_count_kernel(const double* __restrict__ accum,
+                                                                 const uint32_t* __restrict__ in, uint32_t n,
+                                                                 double tol, uint32_t* __restrict__ bcount) {
+  const uint32_t c = blockIdx.x * kPruneBlock + threadIdx.x;
+  uint32_t px = 0;
+  const bool k = c < n && keep_column(accum, in, c, tol, &px);
+  const int cnt = __syncthreads_count(k);
+  if (threadIdx.x == 0) bcount[blockIdx.x] = (uint32_t)cnt;
+}
+
+// Exclusive prefix of the nb block counts in place; the total into *total.
+__global__ __launch_bounds__(1024) void prune_scan_kernel(uint32_t* __restrict__ bcount, uint32_t nb,
+                                                         uint32_t* __restrict__ total) {
+  __shared__ uint32_t part[1024];
+  const uint32_t per = (nb + 1023) / 1024;
+  const uint32_t b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
+  uint32_t sum = 0;
+  for (uint32_t b = b0; b < b1; b++) sum += bcount[b];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (uint32_t off = 1; off < 1024; off <<= 1) {
+    const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[threadIdx.x] - sum;
+  for (uint32_t b = b0; b < b1; b++) {
+    const uint32_t c = bcount[b];
+    bcount[b] = run;
+    run += c;
+  }
+  if (threadIdx.x == 1023) *total = part[1023];
+}
+
+// Writes the pixels that keep sampling, in column order.
+__global__ __launch_bounds__(kPruneBlock) void prune_scatter_kernel(const double* __restrict__ accum,
+                                                                   const uint32_t* __restrict__ in, uint32_t n,
+                                                                   double tol, const uint32_t* __restrict__ boff,
+                                                                   uint32_t* __restrict__ out) {
+  __shared__ uint32_t wsum[kPruneBlock / 64];
+  const uint32_t c = blockIdx.x * kPruneBlock + threadIdx.x;
+  uint32_t px = 0;
+  const bool k = c < n && keep_column(accum, in, c, tol, &px);
+  const uint64_t m = __ballot(k);
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) wsum[w] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint32_t base = boff[blockIdx.x];
+  for (uint32_t i = 0; i < w; i++) base += wsum[i];
+  if (k) out[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = px;
+}
+
+}  // namespace
+
+hipError_t pt_launch_count_sum(const double* accum, uint32_t npix, double* partial, uint32_t blocks,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(count_sum_kernel, dim3(blocks), dim3(256), 0, st, accum, npix, partial);
+  return hipGetLastError();
+}
+
+hipError_t pt_launch_adaptive_prune(const double* accum, const uint32_t* in, uint32_t n, double tol,
+                                    uint32_t* bcount, uint32_t* out, uint32_t* nlive, hipStream_t st) {
+  const uint32_t nb = (n + kPruneBlock - 1) / kPruneBlock;
+  if (nb == 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(prune_count_kernel, dim3(nb), dim3(kPruneBlock), 0, st, accum, in, n, tol, bcount);
+  hipLaunchKernelGGL(prune_scan_kernel, dim3(1), dim3(1024), 0, st, bcount, nb, nlive);
+  hipLaunchKernelGGL(prune_scatter_kernel, dim3(nb), dim3(kPruneBlock), 0, st, accum, in, n, tol, bcount, out);
   return hipGetLastError();
 }
 

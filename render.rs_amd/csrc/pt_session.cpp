@@ -48,6 +48,16 @@ struct rrhip_session {
   uint32_t launch_mode = 0;
   uint64_t* d_sample0 = nullptr;
   std::vector<std::pair<uint32_t, hipGraphExec_t>> graphs;
+  // Adaptive sampling (tolerance 0: off): ping-pong lists of the pixels
+  // still sampling, per-block offsets for their compaction and the device
+  // count. Until the first check (`pruned` false) every pixel samples.
+  double ad_tol = 0.0;
+  uint32_t* ad_live[2] = {nullptr, nullptr};
+  uint32_t* ad_bcount = nullptr;
+  uint32_t* ad_nlive = nullptr;
+  uint32_t ad_cur = 0;
+  uint32_t nlive = 0;
+  bool pruned = false;
 };
 
 namespace {
@@ -174,16 +184,20 @@ int rrhip_session_destroy(rrhip_session* s) {
 
 namespace {
 
-// Enqueues one pass of n samples per pixel (raygen, bounces, cache work,
-// resolve). sample0_dev: null = prm.sample0 is used as given.
-hipError_t enqueue_pass(rrhip_session* s, uint32_t n, uint64_t sample0, const uint64_t* sample0_dev, hipStream_t st) {
+// Enqueues one pass of n samples of each of ncol pixels (raygen, bounces,
+// cache work, resolve): the pixels of list `live`, or all when it is null.
+// sample0_dev: null = sample0 is used as given.
+hipError_t enqueue_pass(rrhip_session* s, uint32_t n, uint32_t ncol, const uint32_t* live, uint64_t sample0,
+                        const uint64_t* sample0_dev, hipStream_t st) {
   rrhip_scene* sc = s->scene;
   rrhip_nrc* nrc = s->nrc;
   NrcPathBuffers N = nrc ? s->N : NrcPathBuffers{};
   PtParams prm = s->prm;
   prm.spp_pass = n;
   prm.sample0 = sample0;
-  const uint32_t total = n * prm.npix;
+  prm.npix = ncol;
+  prm.live = live;
+  const uint32_t total = n * ncol;
   N.train_base = total;  // training paths follow this pass's rendering paths
   const uint32_t paths = total + N.ntrain;
   PathBuffers P = s->P;
@@ -222,7 +236,7 @@ hipError_t pass_graph(rrhip_session* s, uint32_t n, hipStream_t st, hipGraphExec
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return e;
-  const hipError_t pe = enqueue_pass(s, n, 0, s->d_sample0, st);
+  const hipError_t pe = enqueue_pass(s, n, s->prm.npix, nullptr, 0, s->d_sample0, st);
   e = hipStreamEndCapture(st, &graph);
   if (pe != hipSuccess) e = pe;
   hipGraphExec_t exec = nullptr;
@@ -231,6 +245,24 @@ hipError_t pass_graph(rrhip_session* s, uint32_t n, hipStream_t st, hipGraphExec
   if (e != hipSuccess) return e;
   s->graphs.push_back({n, exec});
   *out = exec;
+  return hipSuccess;
+}
+
+// Adaptive sampling check: drops the pixels that met the stopping rule
+// from the live list (in order) and reads the new count.
+hipError_t adaptive_prune(rrhip_session* s, hipStream_t st) {
+  const uint32_t* in = s->pruned ? s->ad_live[s->ad_cur] : nullptr;
+  const uint32_t n = s->pruned ? s->nlive : s->prm.npix;
+  if (n == 0) return hipSuccess;
+  uint32_t* out = s->ad_live[s->ad_cur ^ 1];
+  hipError_t e = pt_launch_adaptive_prune(s->d_accum, in, n, s->ad_tol, s->ad_bcount, out, s->ad_nlive, st);
+  uint32_t live = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&live, s->ad_nlive, sizeof(live), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return e;
+  s->ad_cur ^= 1;
+  s->nlive = live;
+  s->pruned = true;
   return hipSuccess;
 }
 
@@ -245,15 +277,27 @@ int rrhip_render_samples(rrhip_session* s, uint32_t start, uint32_t count) {
   hipStream_t st = sc->ctx->stream;
   RRHIP_HIP_TRY(hipEventRecord(sc->ctx->ev0, st));
   for (uint32_t done = 0; done < count;) {
-    const uint32_t n = std::min(s->spp_pass, count - done);
+    uint32_t n = std::min(s->spp_pass, count - done);
     const uint64_t sample0 = s->sample_base + (uint64_t)start + done;
-    if (s->launch_mode & 1) {
+    if (s->ad_tol > 0.0) {
+      // pt/mod.rs:306-327: pixels are checked before samples 32, 48, 64, ...
+      // (sample index relative to the session's base); passes end there.
+      const uint64_t k = (uint64_t)start + done;
+      if (k >= 32 && k % 16 == 0) RRHIP_HIP_TRY(adaptive_prune(s, st));
+      if (s->pruned && s->nlive == 0) break;
+      const uint64_t next = k < 32 ? 32 : (k / 16 + 1) * 16;
+      n = (uint32_t)std::min<uint64_t>(n, next - k);
+      if (s->pruned)
+        RRHIP_HIP_TRY(enqueue_pass(s, n, s->nlive, s->ad_live[s->ad_cur], sample0, nullptr, st));
+      else
+        RRHIP_HIP_TRY(enqueue_pass(s, n, s->prm.npix, nullptr, sample0, nullptr, st));
+    } else if (s->launch_mode & 1) {
       hipGraphExec_t exec = nullptr;
       RRHIP_HIP_TRY(pass_graph(s, n, st, &exec));
       RRHIP_HIP_TRY(pt_launch_set_u64(s->d_sample0, sample0, st));
       RRHIP_HIP_TRY(hipGraphLaunch(exec, st));
     } else {
-      RRHIP_HIP_TRY(enqueue_pass(s, n, sample0, nullptr, st));
+      RRHIP_HIP_TRY(enqueue_pass(s, n, s->prm.npix, nullptr, sample0, nullptr, st));
     }
     done += n;
   }
@@ -294,7 +338,9 @@ int rrhip_session_autotune(rrhip_session* s, uint32_t samples, uint32_t* best_mo
   const uint64_t base = s->sample_base;
   const uint64_t done = s->samples_done;
   rrhip_nrc* nrc = s->nrc;
-  s->nrc = nullptr;  // trials never train the cache
+  const double tol = s->ad_tol;
+  s->nrc = nullptr;  // trials never train the cache nor prune pixels
+  s->ad_tol = 0.0;
   s->sample_base = base + (1ull << 40);
   uint32_t best = 0;
   float best_t = 0.f;
@@ -309,6 +355,7 @@ int rrhip_session_autotune(rrhip_session* s, uint32_t samples, uint32_t* best_mo
     }
   }
   s->nrc = nrc;
+  s->ad_tol = tol;
   s->sample_base = base;
   s->samples_done = done;
   if (status == RRHIP_OK) status = rrhip_session_set_launch_mode(s, best);
@@ -344,6 +391,42 @@ int rrhip_load_accum(rrhip_session* s, const double* in) {
     RRHIP_HIP_TRY(hipMemsetAsync(s->d_accum, 0, bytes, s->scene->ctx->stream));
   }
   RRHIP_HIP_TRY(hipStreamSynchronize(s->scene->ctx->stream));
+  s->pruned = false;  // every pixel samples again until the next check
+  return RRHIP_OK;
+}
+
+int rrhip_session_set_adaptive(rrhip_session* s, double tolerance) {
+  if (int st = check_session(s)) return st;
+  if (!(tolerance >= 0.0) || tolerance > 1e30)
+    return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_set_adaptive: tolerance must be finite and >= 0");
+  RRHIP_HIP_TRY(hipSetDevice(s->scene->ctx->device));
+  RRHIP_HIP_TRY(hipStreamSynchronize(s->scene->ctx->stream));
+  if (tolerance > 0.0 && !s->ad_nlive) {
+    const uint64_t npix = s->prm.npix;
+    RRHIP_HIP_TRY(alloc_n(s, &s->ad_live[0], npix));
+    RRHIP_HIP_TRY(alloc_n(s, &s->ad_live[1], npix));
+    RRHIP_HIP_TRY(alloc_n(s, &s->ad_bcount, (npix + 255) / 256));
+    RRHIP_HIP_TRY(alloc_n(s, &s->ad_nlive, 1));
+  }
+  s->ad_tol = tolerance;
+  s->pruned = false;
+  return RRHIP_OK;
+}
+
+int rrhip_session_average_spp(rrhip_session* s, double* out) {
+  if (int st = check_session(s)) return st;
+  if (!out) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_average_spp: null output");
+  RRHIP_HIP_TRY(hipSetDevice(s->scene->ctx->device));
+  hipStream_t st = s->scene->ctx->stream;
+  const uint32_t blocks = 512;
+  if (!s->relmse_partial) RRHIP_HIP_TRY(alloc_n(s, &s->relmse_partial, blocks));
+  RRHIP_HIP_TRY(pt_launch_count_sum(s->d_accum, s->prm.npix, s->relmse_partial, blocks, st));
+  std::vector<double> h(blocks);
+  RRHIP_HIP_TRY(hipMemcpyAsync(h.data(), s->relmse_partial, blocks * sizeof(double), hipMemcpyDeviceToHost, st));
+  RRHIP_HIP_TRY(hipStreamSynchronize(st));
+  double sum = 0.0;
+  for (double v : h) sum += v;
+  *out = sum / (double)s->prm.npix;
   return RRHIP_OK;
 }
 

@@ -205,9 +205,10 @@ struct PtParams {
   uint32_t max_depth;
   uint32_t rr_start;
   float firefly_clamp;
-  uint32_t npix;
+  uint32_t npix;          // pixel columns of this pass (slot = sample * npix + column)
   uint32_t spp_pass;      // samples per pixel in this pass
   uint64_t sample0;       // sample index of the pass's first sample (incl. base)
+  const uint32_t* live;   // pixel of each column (adaptive sampling); null: column = pixel
 };
 
 // One job of a batch pass: its scene, its pass parameters, where its

@@ -42,6 +42,7 @@ EXPORTED = [
     "rrhip_session_set_nrc", "rrhip_gen_density_grid",
     "rrhip_batch_create", "rrhip_batch_destroy", "rrhip_batch_render", "rrhip_batch_read_accum",
     "rrhip_batch_get_stats", "rrhip_session_set_launch_mode", "rrhip_session_autotune",
+    "rrhip_session_set_adaptive", "rrhip_session_average_spp",
 ]
 
 
@@ -259,6 +260,8 @@ def load():
                                        ctypes.POINTER(ctypes.c_float)]),
         "rrhip_session_set_launch_mode": (i32, [vp, u32]),
         "rrhip_session_autotune": (i32, [vp, u32, ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_float)]),
+        "rrhip_session_set_adaptive": (i32, [vp, ctypes.c_double]),
+        "rrhip_session_average_spp": (i32, [vp, ctypes.POINTER(ctypes.c_double)]),
         "rrhip_batch_create": (i32, [vp, u32, vp, u32, ctypes.POINTER(vp)]),
         "rrhip_batch_destroy": (i32, [vp]),
         "rrhip_batch_render": (i32, [vp, vp, ctypes.POINTER(ctypes.c_double)]),
@@ -476,6 +479,29 @@ class Session:
         ms = ctypes.c_float()
         _check(load().rrhip_session_autotune(self.h, samples, ctypes.byref(mode), ctypes.byref(ms)))
         return mode.value, ms.value
+
+    def set_adaptive(self, tolerance):
+        """Adaptive sampling (WfSession::set_adaptive, renderer.rs:580-584):
+        pixels stop once their 95 % confidence half-width of the mean
+        luminance is below tolerance * max(mean, 0.05), checked before
+        samples 32, 48, 64, ...; 0 turns it off."""
+        _check(load().rrhip_session_set_adaptive(self.h, float(tolerance)))
+
+    def average_spp(self):
+        """Mean samples per pixel (WfSession::average_spp, renderer.rs:587-591)."""
+        out = ctypes.c_double()
+        _check(load().rrhip_session_average_spp(self.h, ctypes.byref(out)))
+        return out.value
+
+    def render_adaptive(self, max_spp, tolerance):
+        """pt::render_adaptive (src/raytracer/pt/mod.rs:285-336) on this
+        session: clears the accumulator, renders samples [0, max_spp) with
+        adaptive stopping, returns the average samples per pixel; the image
+        is accum()[:, :3] / accum()[:, 6:7]."""
+        self.load_accum(None)
+        self.set_adaptive(tolerance)
+        self.render(0, max(int(max_spp), 1))
+        return self.average_spp()
 
     def set_nrc(self, nrc):
         """Attaches a radiance cache (Nrc) to the session, or detaches it (None)."""

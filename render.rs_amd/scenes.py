@@ -221,6 +221,40 @@ def box_room(width=256, height=256, emission=15.0):
     return s
 
 
+def plastic(color, roughness=0.1):
+    """RenderMan "plastic" as PbrParams::from_plastic (pbr.rs:112-120)."""
+    return material(diffuse_gain=0.9, diffuse_color=color, specular_ior=1.5,
+                    specular_roughness=max(roughness, 0.02))
+
+
+def sphere_zoo(width=96, height=96):
+    """A background- and direct-light-dominated scene for adaptive
+    sampling, laid out as render.rs's quadric_zoo fixture
+    (tests/fixtures/quadric_zoo.rib) with every quadric replaced by a
+    sphere of similar size at its position and with its colour and surface:
+    two rows of four over a grey ground quad at y = -2.2, lit by a distant
+    key light (from (4, 8, -6) towards (0, 0, 7), intensity 0.85) and a
+    point light at (-5, 4, 2) (intensity 0.35); black background, camera
+    at the origin looking down +z with a 40 degree fov."""
+    s = Scene(width, height, 40.0, eye=(0, 0, 0), look_at=(0, 0, 1), up=(0, 1, 0))
+    rows = [((-3.4, 1.2, 7), 0.9, plastic((0.9, 0.25, 0.2), 0.15)),
+            ((-1.1, 1.2, 7), 0.75, metal((0.95, 0.7, 0.15), 0.08)),
+            ((1.1, 1.2, 7), 0.7, plastic((0.2, 0.55, 0.9))),
+            ((3.4, 0.8, 7), 0.8, plastic((0.25, 0.8, 0.35))),
+            ((-3.4, -1.0, 7), 0.7, plastic((0.7, 0.3, 0.85))),
+            ((-1.1, -1.1, 7), 0.6, metal((0.9, 0.45, 0.6), 0.15)),
+            ((1.1, -1.1, 7), 0.75, metal((0.85, 0.85, 0.9), 0.05)),
+            ((3.4, -1.1, 7), 0.8, plastic((0.3, 0.85, 0.8)))]
+    for c, r, m in rows:
+        s.add_sphere(c, r, s.add_material(m))
+    ground = s.add_material(matte((0.75, 0.75, 0.75)))
+    s.add_polygon([(-12, -2.2, 0.5), (12, -2.2, 0.5), (12, -2.2, 18), (-12, -2.2, 18)], ground)
+    d = np.array([0.0, 0.0, 7.0]) - np.array([4.0, 8.0, -6.0])
+    s.add_light(light(rrhip.LIGHT_DISTANT, intensity=0.85, p=tuple(d / np.linalg.norm(d))))
+    s.add_light(light(rrhip.LIGHT_POINT, intensity=0.35, p=(-5, 4, 2)))
+    return s
+
+
 def cloud_room(width=128, height=128, n=128, seed=1, sigma_s=(2.0, 2.0, 2.0), sigma_a=(0.2, 0.2, 0.2), g=0.3,
                emission=15.0, spheres=None):
     """The box room filled with a heterogeneous atmosphere: an n^3 fBm cloud
