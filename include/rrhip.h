@@ -371,8 +371,10 @@ typedef struct rrhip_session_stats {
 int rrhip_session_create(rrhip_scene* scene, const rrhip_render_params* params, rrhip_session** out);
 int rrhip_session_destroy(rrhip_session* session);
 /* Renders samples [start, start + count) of every pixel and adds them to
- * the f64 accumulator (blocking). Per-(pixel, sample) PCG32 seeding makes
- * sample ranges compose exactly. Counterpart of
+ * the f64 accumulator (blocking; with a radiance cache attached, the
+ * optimiser steps of the last pass may still run on the cache's stream,
+ * and every later use of the cache waits for them). Per-(pixel, sample)
+ * PCG32 seeding makes sample ranges compose exactly. Counterpart of
  * PtSession::render_samples (renderer.rs:410-437). */
 int rrhip_render_samples(rrhip_session* session, uint32_t start, uint32_t count);
 /* Copies the accumulator (width x height x RRHIP_ACCUM_STRIDE doubles) to
@@ -385,9 +387,14 @@ int rrhip_load_accum(rrhip_session* session, const double* in);
 int rrhip_session_get_stats(rrhip_session* session, rrhip_session_stats* stats);
 /* How passes are launched (bit flags, results identical in every mode):
  * 1 = each pass size captured once as a hipGraph and replayed (the first
- * sample index is fed from device memory); 2 = the live queue sorted by
- * shading class (miss / active-lobe set of the hit material) before shade,
- * so a wave runs one BSDF kind. 0 = plain kernel launches (default). No
+ * sample index is fed from device memory; not with a radiance cache or
+ * adaptive sampling, whose passes are launched directly); 2 = the live
+ * queue sorted by shading class (miss / active-lobe set of the hit
+ * material) before shade, so a wave runs one BSDF kind; 4 = a radiance
+ * cache's optimiser steps stay on the rendering stream (by default they
+ * run on the cache's own stream, overlapped with the next pass's tracing
+ * also across rrhip_render_samples calls, and every later use of the
+ * cache waits for them). 0 = plain kernel launches (default). No
  * counterpart in render.rs (its Metal host encodes one command buffer per
  * dispatch, renderer.rs:380-437). */
 int rrhip_session_set_launch_mode(rrhip_session* session, uint32_t mode);
@@ -432,6 +439,9 @@ typedef struct rrhip_nrc_config {
   uint32_t steps_per_pass;  /* optimiser steps per rendering pass */
   uint32_t train_paths;     /* training paths traced per rendering pass */
   float spread;             /* area-spread termination constant c */
+  float ema;                /* inference uses the debiased moving average of the
+                               parameters with this decay, [0, 1); 0 = the
+                               trained parameters themselves */
 } rrhip_nrc_config;
 
 /* A cache query / training sample (64 bytes): the network input at a
@@ -458,8 +468,8 @@ typedef struct rrhip_nrc_info {
 /* Creates a cache with freshly initialised parameters (grid U(-1e-4,
  * 1e-4), He-uniform weights, both from PCG32(seed)). config = NULL takes
  * the defaults: 16 levels, 2^19 entries, resolutions 16..2048, Adam
- * (1e-2, 0.9, 0.99, 1e-15), batch 16384, 4 steps and 65536 training
- * paths per pass, c = 1e-4. */
+ * (1e-2, 0.9, 0.99, 1e-15), batch 8192, 16 steps and 65536 training
+ * paths per pass, c = 1e-4, no moving average. */
 int rrhip_nrc_create(rrhip_ctx* ctx, const rrhip_nrc_config* config, rrhip_nrc** out);
 int rrhip_nrc_destroy(rrhip_nrc* nrc);
 int rrhip_nrc_get_info(rrhip_nrc* nrc, rrhip_nrc_info* info);

@@ -81,11 +81,17 @@ hipError_t nrc_launch_train(const NrcGridView& G, const NrcQuery* q, const float
 hipError_t nrc_launch_reduce_mlp(const float* part, NrcControl* ctl, uint32_t capacity, float* w, float* m,
                                  float* v, float* grad_out, NrcHyper hp, bool apply, hipStream_t st);
 // Adam on the grid (or, without `apply`, grid gradient as f32 into
-// grad_out); clears the fixed-point gradients.
+// grad_out); clears the fixed-point gradients. With `ema` set, also the
+// moving average ema = alpha ema + (1 - alpha) table and its debiased
+// value (step ctl->step + 1) into `inf`.
 hipError_t nrc_launch_adam_grid(float* table, float* m, float* v, unsigned long long* grid_grad,
                                 uint64_t n, const NrcControl* ctl, float* grad_out, NrcHyper hp, bool apply,
-                                hipStream_t st);
-// bf16 copies from the master weights; counts the step when ctl->n > 0.
+                                hipStream_t st, float* ema = nullptr, float* inf = nullptr, float alpha = 0.f);
+// The same moving average over the MLP master weights (when ctl->n > 0).
+hipError_t nrc_launch_ema_mlp(const float* w, float* ema, float* inf, const NrcControl* ctl, float alpha,
+                              hipStream_t st);
+// bf16 copies from the master weights (wb null: the forward copy only);
+// counts the step when count_step and ctl->n > 0.
 hipError_t nrc_launch_pack(const float* w, uint16_t* wf, uint16_t* wb, NrcControl* ctl, bool count_step,
                            hipStream_t st);
 

@@ -506,13 +506,23 @@ __global__ __launch_bounds__(256) void nrc_reduce_mlp_kernel(const float* __rest
   v[i] = vv;
 }
 
+// Moving average of the parameters for inference: e_t = a e_{t-1} +
+// (1 - a) p_t, used as e_t / (1 - a^t) (t counts steps since the
+// parameters were set).
+__device__ __forceinline__ float ema_debias(const NrcControl* ctl, float a) {
+  return (float)(1.0 / (1.0 - pow((double)a, (double)ctl->step + 1.0)));
+}
+
 __global__ __launch_bounds__(256) void nrc_adam_grid_kernel(float* __restrict__ table, float* __restrict__ m,
                                                            float* __restrict__ v, unsigned long long* __restrict__ gg,
                                                            uint64_t n, const NrcControl* __restrict__ ctl,
-                                                           float* __restrict__ grad_out, NrcHyper hp, int apply) {
+                                                           float* __restrict__ grad_out, NrcHyper hp, int apply,
+                                                           float* __restrict__ ema, float* __restrict__ inf,
+                                                           float alpha) {
   if (ctl->n == 0) return;
   float c1 = 1.f, c2 = 1.f;
   if (apply) bias_corrections(ctl, hp, c1, c2);
+  const float corr = ema ? ema_debias(ctl, alpha) : 1.f;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const long long fx = (long long)gg[i];
@@ -527,7 +537,23 @@ __global__ __launch_bounds__(256) void nrc_adam_grid_kernel(float* __restrict__ 
     table[i] = p;
     m[i] = mm;
     v[i] = vv;
+    if (ema) {
+      const float e = alpha * ema[i] + (1.f - alpha) * p;
+      ema[i] = e;
+      inf[i] = e * corr;
+    }
   }
+}
+
+__global__ __launch_bounds__(256) void nrc_ema_mlp_kernel(const float* __restrict__ w, float* __restrict__ ema,
+                                                         float* __restrict__ inf, const NrcControl* __restrict__ ctl,
+                                                         float alpha) {
+  if (ctl->n == 0) return;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kNrcMlpParams) return;
+  const float e = alpha * ema[i] + (1.f - alpha) * w[i];
+  ema[i] = e;
+  inf[i] = e * ema_debias(ctl, alpha);
 }
 
 __global__ __launch_bounds__(256) void nrc_pack_kernel(const float* __restrict__ w, uint16_t* __restrict__ wf,
@@ -550,7 +576,7 @@ __global__ __launch_bounds__(256) void nrc_pack_kernel(const float* __restrict__
     return;
   }
   const uint32_t j = i - kNrcFwdSize;
-  if (j >= kNrcBwdSize) return;
+  if (!wb || j >= kNrcBwdSize) return;
   float val;
   if (j < kNrcBwdOff[1]) {
     const uint32_t mm = j / 64, ks = j % 64;  // input dim mm < 32
@@ -605,18 +631,25 @@ hipError_t nrc_launch_reduce_mlp(const float* part, NrcControl* ctl, uint32_t ca
 }
 
 hipError_t nrc_launch_adam_grid(float* table, float* m, float* v, unsigned long long* grid_grad, uint64_t n,
-                                const NrcControl* ctl, float* grad_out, NrcHyper hp, bool apply, hipStream_t st) {
+                                const NrcControl* ctl, float* grad_out, NrcHyper hp, bool apply, hipStream_t st,
+                                float* ema, float* inf, float alpha) {
   const uint64_t b = (n + 255) / 256;
   const uint32_t blocks = (uint32_t)(b > 8192 ? 8192 : (b < 1 ? 1 : b));
   hipLaunchKernelGGL(nrc_adam_grid_kernel, dim3(blocks), dim3(256), 0, st, table, m, v, grid_grad, n, ctl, grad_out,
-                     hp, apply ? 1 : 0);
+                     hp, apply ? 1 : 0, ema, inf, alpha);
   return hipGetLastError();
 }
 
 hipError_t nrc_launch_pack(const float* w, uint16_t* wf, uint16_t* wb, NrcControl* ctl, bool count_step,
                            hipStream_t st) {
-  hipLaunchKernelGGL(nrc_pack_kernel, dim3((kNrcFwdSize + kNrcBwdSize + 255) / 256), dim3(256), 0, st, w, wf, wb,
-                     ctl, count_step ? 1 : 0);
+  const uint32_t n = wb ? kNrcFwdSize + kNrcBwdSize : kNrcFwdSize;
+  hipLaunchKernelGGL(nrc_pack_kernel, dim3((n + 255) / 256), dim3(256), 0, st, w, wf, wb, ctl, count_step ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t nrc_launch_ema_mlp(const float* w, float* ema, float* inf, const NrcControl* ctl, float alpha,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(nrc_ema_mlp_kernel, dim3((kNrcMlpParams + 255) / 256), dim3(256), 0, st, w, ema, inf, ctl, alpha);
   return hipGetLastError();
 }
 

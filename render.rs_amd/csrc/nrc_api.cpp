@@ -26,6 +26,9 @@ hipError_t alloc_n(rrhip_nrc* c, T** p, uint64_t n) {
 }
 
 void free_nrc(rrhip_nrc* c) {
+  if (c->train_st) (void)hipStreamDestroy(c->train_st);
+  if (c->ev_samples) (void)hipEventDestroy(c->ev_samples);
+  if (c->ev_trained) (void)hipEventDestroy(c->ev_trained);
   for (void* p : c->mem) (void)hipFree(p);
   delete c;
 }
@@ -42,8 +45,8 @@ rrhip_nrc_config default_config() {
   c.beta1 = 0.9f;
   c.beta2 = 0.99f;
   c.epsilon = 1e-15f;
-  c.batch = 16384;
-  c.steps_per_pass = 4;
+  c.batch = 8192;
+  c.steps_per_pass = 16;
   c.train_paths = 65536;
   c.spread = 1e-4f;
   return c;
@@ -66,6 +69,7 @@ int check_config(const rrhip_nrc_config& c) {
   if (c.train_paths > (1u << 22))
     return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_nrc_create: train_paths must be <= 2^22");
   if (!(c.spread > 0.f)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_nrc_create: spread must be > 0");
+  if (!(c.ema >= 0.f && c.ema < 1.f)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_nrc_create: ema must be in [0, 1)");
   return RRHIP_OK;
 }
 
@@ -96,14 +100,32 @@ hipError_t ensure_staging(rrhip_nrc* c, uint64_t n) {
   return e;
 }
 
+// Null check, then the cache's own stream waits for optimiser steps a
+// session may still be running on it.
 int check_nrc(rrhip_nrc* c) {
   if (!c) return fail(RRHIP_ERR_INVALID_ARGUMENT, "null cache");
+  if (c->pending) {
+    RRHIP_HIP_TRY(hipSetDevice(c->ctx->device));
+    RRHIP_HIP_TRY(nrc_join(c, c->ctx->stream));
+  }
   return RRHIP_OK;
 }
 
 }  // namespace
 
 namespace rrhip {
+
+hipError_t nrc_join(rrhip_nrc* c, hipStream_t st) {
+  if (!c->pending) return hipSuccess;
+  const hipError_t e = hipStreamWaitEvent(st, c->ev_trained, 0);
+  if (e == hipSuccess) c->pending = false;
+  return e;
+}
+
+hipError_t nrc_drain(rrhip_nrc* c) {
+  c->pending = false;
+  return c->train_st ? hipStreamSynchronize(c->train_st) : hipSuccess;
+}
 
 void nrc_set_bounds(rrhip_nrc* c, const float lo[3], const float hi[3]) {
   for (int a = 0; a < 3; a++) {
@@ -119,7 +141,12 @@ hipError_t nrc_step_device(rrhip_nrc* c, const NrcQuery* q, const float* targets
   if (e == hipSuccess)
     e = nrc_launch_reduce_mlp(c->d_part, c->d_ctl, capacity, c->d_w, c->d_wm, c->d_wv, nullptr, c->hp, true, st);
   if (e == hipSuccess)
-    e = nrc_launch_adam_grid(c->d_table, c->d_tm, c->d_tv, c->d_tg, c->grid_n, c->d_ctl, nullptr, c->hp, true, st);
+    e = nrc_launch_adam_grid(c->d_table, c->d_tm, c->d_tv, c->d_tg, c->grid_n, c->d_ctl, nullptr, c->hp, true, st,
+                             c->d_ema_table, c->d_inf_table, c->cfg.ema);
+  if (e == hipSuccess && c->d_inf_table) {
+    e = nrc_launch_ema_mlp(c->d_w, c->d_ema_w, c->d_inf_w, c->d_ctl, c->cfg.ema, st);
+    if (e == hipSuccess) e = nrc_launch_pack(c->d_inf_w, c->d_inf_wf, nullptr, c->d_ctl, false, st);
+  }
   if (e == hipSuccess) e = nrc_launch_pack(c->d_w, c->d_wf, c->d_wb, c->d_ctl, true, st);
   return e;
 }
@@ -156,7 +183,10 @@ int rrhip_nrc_create(rrhip_ctx* ctx, const rrhip_nrc_config* config, rrhip_nrc**
   const float unit_lo[3] = {0.f, 0.f, 0.f}, unit_hi[3] = {1.f, 1.f, 1.f};
   nrc_set_bounds(c, scene_bounds ? unit_lo : cfg.lo, scene_bounds ? unit_hi : cfg.hi);
 
-  hipError_t e = alloc_n(c, &c->d_table, c->grid_n);
+  hipError_t e = hipStreamCreateWithFlags(&c->train_st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_samples, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_trained, hipEventDisableTiming);
+  if (e == hipSuccess) e = alloc_n(c, &c->d_table, c->grid_n);
   if (e == hipSuccess) e = alloc_n(c, &c->d_tm, c->grid_n);
   if (e == hipSuccess) e = alloc_n(c, &c->d_tv, c->grid_n);
   if (e == hipSuccess) e = alloc_n(c, &c->d_tg, c->grid_n);
@@ -168,6 +198,13 @@ int rrhip_nrc_create(rrhip_ctx* ctx, const rrhip_nrc_config* config, rrhip_nrc**
   if (e == hipSuccess) e = alloc_n(c, &c->d_wb, (uint64_t)kNrcBwdSize);
   if (e == hipSuccess) e = alloc_n(c, &c->d_part, (uint64_t)(kNrcMaxBatch / kNrcTrainBlock) * kNrcPartStride);
   if (e == hipSuccess) e = alloc_n(c, &c->d_ctl, 1);
+  if (cfg.ema > 0.f) {
+    if (e == hipSuccess) e = alloc_n(c, &c->d_ema_table, c->grid_n);
+    if (e == hipSuccess) e = alloc_n(c, &c->d_inf_table, c->grid_n);
+    if (e == hipSuccess) e = alloc_n(c, &c->d_ema_w, (uint64_t)kNrcMlpParams);
+    if (e == hipSuccess) e = alloc_n(c, &c->d_inf_w, (uint64_t)kNrcMlpParams);
+    if (e == hipSuccess) e = alloc_n(c, &c->d_inf_wf, (uint64_t)kNrcFwdSize);
+  }
   if (e != hipSuccess) {
     free_nrc(c);
     return fail_hip(e, "rrhip_nrc_create");
@@ -204,6 +241,7 @@ int rrhip_nrc_destroy(rrhip_nrc* c) {
   if (!c) return RRHIP_OK;
   (void)hipSetDevice(c->ctx->device);
   (void)hipStreamSynchronize(c->ctx->stream);
+  (void)nrc_drain(c);
   free_nrc(c);
   return RRHIP_OK;
 }
@@ -253,6 +291,13 @@ int rrhip_nrc_set_params(rrhip_nrc* c, const float* mlp, const float* grid) {
   RRHIP_HIP_TRY(hipMemsetAsync(c->d_tg, 0, c->grid_n * sizeof(unsigned long long), st));
   RRHIP_HIP_TRY(hipMemsetAsync(c->d_ctl, 0, sizeof(NrcControl), st));
   RRHIP_HIP_TRY(nrc_launch_pack(c->d_w, c->d_wf, c->d_wb, c->d_ctl, false, st));
+  if (c->d_inf_table) {  // averages restart from the parameters set
+    RRHIP_HIP_TRY(hipMemsetAsync(c->d_ema_table, 0, c->grid_n * sizeof(float), st));
+    RRHIP_HIP_TRY(hipMemsetAsync(c->d_ema_w, 0, kNrcMlpParams * sizeof(float), st));
+    RRHIP_HIP_TRY(hipMemcpyAsync(c->d_inf_table, c->d_table, c->grid_n * sizeof(float), hipMemcpyDeviceToDevice, st));
+    RRHIP_HIP_TRY(hipMemcpyAsync(c->d_inf_w, c->d_w, kNrcMlpParams * sizeof(float), hipMemcpyDeviceToDevice, st));
+    RRHIP_HIP_TRY(nrc_launch_pack(c->d_inf_w, c->d_inf_wf, nullptr, c->d_ctl, false, st));
+  }
   RRHIP_HIP_TRY(hipStreamSynchronize(st));
   return RRHIP_OK;
 }
@@ -266,7 +311,7 @@ int rrhip_nrc_infer(rrhip_nrc* c, const rrhip_nrc_query* queries, uint64_t n, fl
   hipStream_t st = c->ctx->stream;
   RRHIP_HIP_TRY(ensure_staging(c, n));
   RRHIP_HIP_TRY(hipMemcpyAsync(c->d_q, queries, n * sizeof(NrcQuery), hipMemcpyHostToDevice, st));
-  RRHIP_HIP_TRY(nrc_launch_infer(c->G, c->d_q, nullptr, (uint32_t)n, c->d_out, st));
+  RRHIP_HIP_TRY(nrc_launch_infer(nrc_infer_view(c), c->d_q, nullptr, (uint32_t)n, c->d_out, st));
   RRHIP_HIP_TRY(hipMemcpyAsync(rgb, c->d_out, 3 * n * sizeof(float), hipMemcpyDeviceToHost, st));
   RRHIP_HIP_TRY(hipStreamSynchronize(st));
   return RRHIP_OK;

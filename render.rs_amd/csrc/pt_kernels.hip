@@ -89,10 +89,10 @@ __global__ __launch_bounds__(kBlock) void raygen_kernel(PtSceneView S, PathBuffe
     P.counts[0] = all;
     P.counts[1] = 0;
     P.counts[2] = 0;
-    if (N.counts) {
-      N.counts[0] = 0;
-      N.counts[1] = 0;
-    }
+    // N.counts[1] (training samples of the previous pass) is left alone: the
+    // previous pass's optimiser steps may still be reading it, and this
+    // pass's scan overwrites it before its own steps.
+    if (N.counts) N.counts[0] = 0;
   }
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < all; slot += stride) {

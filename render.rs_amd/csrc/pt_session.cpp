@@ -44,7 +44,8 @@ struct rrhip_session {
   uint32_t nrc_bcap = 0;
   // Launch mode bit 0: one hipGraph per pass size, captured on first use
   // and replayed with the first sample index written to d_sample0; bit 1:
-  // the live queue sorted by shading class before shade.
+  // the live queue sorted by shading class before shade; bit 2: the cache's
+  // optimiser steps stay on the rendering stream (no overlap).
   uint32_t launch_mode = 0;
   uint64_t* d_sample0 = nullptr;
   std::vector<std::pair<uint32_t, hipGraphExec_t>> graphs;
@@ -176,6 +177,7 @@ int rrhip_session_destroy(rrhip_session* s) {
   if (!s) return RRHIP_OK;
   (void)hipSetDevice(s->scene->ctx->device);
   (void)hipStreamSynchronize(s->scene->ctx->stream);
+  if (s->nrc) (void)nrc_drain(s->nrc);  // its steps may still read this session's buffers
   free_session(s);
   return RRHIP_OK;
 }
@@ -209,17 +211,33 @@ hipError_t enqueue_pass(rrhip_session* s, uint32_t n, uint32_t ncol, const uint3
     e = pt_launch_bounce(sc->pt, P, prm, q, qn, paths, N, st);
   }
   if (nrc && e == hipSuccess) {
-    // cache radiance at the terminal vertices (and training tails), then
-    // this pass's training samples and optimiser steps
-    e = nrc_launch_infer_paths(nrc->G, N.q, &N.counts[0], paths, N.q_slot, N.q_beta, s->P.L, st);
+    // cache radiance at the terminal vertices (and training tails) once the
+    // previous pass's optimiser steps are done, then this pass's training
+    // samples and optimiser steps: on the cache's training stream, where
+    // they overlap the next pass's tracing (also across calls), unless
+    // launch mode bit 2 or graph capture keeps them on this stream
+    const bool overlap = !sample0_dev && !(s->launch_mode & 4);
+    e = nrc_join(nrc, st);
+    if (e == hipSuccess) e = nrc_launch_infer_paths(nrc_infer_view(nrc), N.q, &N.counts[0], paths, N.q_slot, N.q_beta, s->P.L, st);
     if (e == hipSuccess)
       e = pt_launch_nrc_samples(s->P, N, prm.firefly_clamp, s->nrc_scan, s->nrc_bcap, s->nrc_bq, s->nrc_bt, st);
     const uint32_t batch = nrc->cfg.batch;
-    for (uint32_t k = 0; k < nrc->cfg.steps_per_pass && N.ntrain > 0 && e == hipSuccess; k++) {
+    const bool steps = N.ntrain > 0 && nrc->cfg.steps_per_pass > 0;
+    hipStream_t ts = st;
+    if (steps && overlap && e == hipSuccess) {
+      e = hipEventRecord(nrc->ev_samples, st);
+      if (e == hipSuccess) e = hipStreamWaitEvent(nrc->train_st, nrc->ev_samples, 0);
+      ts = nrc->train_st;
+    }
+    for (uint32_t k = 0; k < nrc->cfg.steps_per_pass && steps && e == hipSuccess; k++) {
       if ((uint64_t)k * batch >= s->nrc_bcap) break;
-      e = pt_launch_nrc_set_batch(N, k, batch, &nrc->d_ctl->n, st);
+      e = pt_launch_nrc_set_batch(N, k, batch, &nrc->d_ctl->n, ts);
       if (e == hipSuccess)
-        e = nrc_step_device(nrc, s->nrc_bq + (size_t)k * batch, s->nrc_bt + 3 * (size_t)k * batch, batch, st);
+        e = nrc_step_device(nrc, s->nrc_bq + (size_t)k * batch, s->nrc_bt + 3 * (size_t)k * batch, batch, ts);
+    }
+    if (steps && overlap && e == hipSuccess) {
+      e = hipEventRecord(nrc->ev_trained, ts);
+      nrc->pending = e == hipSuccess;
     }
   }
   if (e == hipSuccess) e = pt_launch_resolve(s->P, prm, s->d_accum, st);
@@ -291,7 +309,7 @@ int rrhip_render_samples(rrhip_session* s, uint32_t start, uint32_t count) {
         RRHIP_HIP_TRY(enqueue_pass(s, n, s->nlive, s->ad_live[s->ad_cur], sample0, nullptr, st));
       else
         RRHIP_HIP_TRY(enqueue_pass(s, n, s->prm.npix, nullptr, sample0, nullptr, st));
-    } else if (s->launch_mode & 1) {
+    } else if ((s->launch_mode & 1) && !s->nrc) {
       hipGraphExec_t exec = nullptr;
       RRHIP_HIP_TRY(pass_graph(s, n, st, &exec));
       RRHIP_HIP_TRY(pt_launch_set_u64(s->d_sample0, sample0, st));
@@ -312,7 +330,7 @@ int rrhip_render_samples(rrhip_session* s, uint32_t start, uint32_t count) {
 
 int rrhip_session_set_launch_mode(rrhip_session* s, uint32_t mode) {
   if (int st = check_session(s)) return st;
-  if (mode > 3) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_set_launch_mode: mode must be 0..3");
+  if (mode > 7) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_set_launch_mode: mode must be 0..7");
   RRHIP_HIP_TRY(hipSetDevice(s->scene->ctx->device));
   RRHIP_HIP_TRY(hipStreamSynchronize(s->scene->ctx->stream));
   if ((mode & 1) && !s->d_sample0) RRHIP_HIP_TRY(alloc_n(s, &s->d_sample0, 1));
@@ -478,6 +496,8 @@ int rrhip_session_get_stats(rrhip_session* s, rrhip_session_stats* out) {
 extern "C" int rrhip_session_set_nrc(rrhip_session* s, rrhip_nrc* nrc) {
   if (int st = check_session(s)) return st;
   if (!nrc) {
+    RRHIP_HIP_TRY(hipSetDevice(s->scene->ctx->device));
+    if (s->nrc) RRHIP_HIP_TRY(nrc_drain(s->nrc));
     drop_graphs(s);
     s->nrc = nullptr;
     return RRHIP_OK;
@@ -491,6 +511,8 @@ extern "C" int rrhip_session_set_nrc(rrhip_session* s, rrhip_nrc* nrc) {
   if (cap >= (1ull << 31)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_set_nrc: path slots exceed 2^31");
   RRHIP_HIP_TRY(hipSetDevice(sc->ctx->device));
   RRHIP_HIP_TRY(hipStreamSynchronize(sc->ctx->stream));
+  if (s->nrc) RRHIP_HIP_TRY(nrc_drain(s->nrc));
+  RRHIP_HIP_TRY(nrc_drain(nrc));  // its steps may still read another session's buffers
   drop_graphs(s);  // captured passes hold the old buffers
   if (nrc->bounds_from_scene) nrc_set_bounds(nrc, sc->pt_lo, sc->pt_hi);
   s->nrc = nullptr;

@@ -162,15 +162,15 @@ class NrcConfig(ctypes.Structure):
                 ("learning_rate", ctypes.c_float), ("beta1", ctypes.c_float),
                 ("beta2", ctypes.c_float), ("epsilon", ctypes.c_float),
                 ("batch", ctypes.c_uint32), ("steps_per_pass", ctypes.c_uint32),
-                ("train_paths", ctypes.c_uint32), ("spread", ctypes.c_float)]
+                ("train_paths", ctypes.c_uint32), ("spread", ctypes.c_float), ("ema", ctypes.c_float)]
 
     @staticmethod
     def make(levels=16, log2_table=19, seed=1, base_res=16.0, max_res=2048.0, lo=(0, 0, 0), hi=(0, 0, 0),
-             learning_rate=1e-2, beta1=0.9, beta2=0.99, epsilon=1e-15, batch=16384, steps_per_pass=4,
-             train_paths=65536, spread=1e-4):
+             learning_rate=1e-2, beta1=0.9, beta2=0.99, epsilon=1e-15, batch=8192, steps_per_pass=16,
+             train_paths=65536, spread=1e-4, ema=0.0):
         return NrcConfig(ctypes.sizeof(NrcConfig), levels, log2_table, seed, base_res, max_res,
                          (ctypes.c_float * 3)(*lo), (ctypes.c_float * 3)(*hi), learning_rate, beta1, beta2,
-                         epsilon, batch, steps_per_pass, train_paths, spread)
+                         epsilon, batch, steps_per_pass, train_paths, spread, ema)
 
 
 class NrcInfo(ctypes.Structure):
@@ -470,7 +470,8 @@ class Session:
             _check(load().rrhip_load_accum(self.h, _ptr(acc)))
 
     def set_launch_mode(self, mode):
-        """Bit 0: one hipGraph per pass size; bit 1: queue sorted by shading class."""
+        """Bit 0: one hipGraph per pass size; bit 1: queue sorted by shading
+        class; bit 2: cache optimiser steps on the rendering stream."""
         _check(load().rrhip_session_set_launch_mode(self.h, mode))
 
     def autotune(self, samples):

@@ -241,3 +241,65 @@ def test_cached_render_gains_over_truncation_and_is_reproducible(ctx):
     plain.close()
     ref.close()
     ps.close()
+
+
+def test_overlapped_training_equals_in_stream_training(ctx):
+    """The cache's optimiser steps of pass f overlap pass f + 1's tracing on
+    a second stream (joined before f + 1's cache queries); the result is
+    bit-identical to running them on the rendering stream (launch mode 4),
+    and graph mode (1) falls back to the same direct launches with a cache."""
+    import scenes
+    say("cache training overlap: box room 64x64, 8 passes x 2 spp, modes 0 / 4 / 1 / 3")
+    s = scenes.box_room(64, 64)
+    ps = rrhip.PtScene(ctx, s.desc())
+    out = {}
+    for mode in (0, 4, 1, 3):
+        c = rrhip.Nrc(ctx, rrhip.NrcConfig.make(log2_table=16, train_paths=4096, batch=2048))
+        se = ps.session(rrhip.RenderParams.make(max_depth=8, samples_per_pass=2))
+        se.set_launch_mode(mode)
+        se.set_nrc(c)
+        for f in range(8):
+            se.render(2 * f, 2)
+        w, g = c.get_params()
+        out[mode] = (se.accum(), w, g, c.info()["steps"])
+        se.close()
+        c.close()
+    ps.close()
+    a0, w0, g0, n0 = out[0]
+    assert n0 >= 8  # at least one optimiser step per pass
+    for mode in (4, 1, 3):
+        a, w, g, n = out[mode]
+        assert n == n0 and a.tobytes() == a0.tobytes() and w.tobytes() == w0.tobytes() and g.tobytes() == g0.tobytes(), mode
+
+
+def test_inference_uses_the_debiased_moving_average(ctx):
+    """With ema = a, inference reads e_t / (1 - a^t), e_t = a e_{t-1} +
+    (1 - a) p_t over the trained parameters p_t since they were set (and
+    the parameters themselves before the first step); training and
+    get_params keep using p_t."""
+    say("cache ema 0.9: inference vs oracle on the debiased moving average of 4 steps")
+    rng = np.random.default_rng(11)
+    a = 0.9
+    c, oc, w, grid = make_cache(ctx, rng, ema=a)
+    q = random_queries(rng, 3000)
+
+    def check(pw, pg, label):
+        y = c.infer(q).astype(np.float64)
+        ym = orc.nrc_infer(oc, pg.astype(np.float32), pw, q, mimic=True)
+        scale = np.sqrt(np.mean(ym ** 2))
+        err = np.abs(y - ym)
+        say(f"{label}: |gpu - mimic| max {err.max():.3e} p99 {np.quantile(err, 0.99):.3e} (scale {scale:.3f})")
+        assert err.max() <= 2e-2 * scale and np.quantile(err, 0.99) <= 5e-3 * scale
+
+    check(w.astype(np.float64), grid.astype(np.float64), "before any step")
+    ew = np.zeros(w.size)
+    eg = np.zeros(grid.size)
+    for t in range(1, 5):
+        c.train_step(random_queries(rng, 2000), rng.uniform(0, 1, (2000, 3)).astype(np.float32))
+        pw, pg = c.get_params()
+        ew = a * ew + (1 - a) * pw
+        eg = a * eg + (1 - a) * pg
+    d = 1.0 - a ** 4
+    assert not np.array_equal(pw, w)
+    check(ew / d, eg / d, "after 4 steps")
+    c.close()
