@@ -181,7 +181,7 @@ def time_to_relmse(ctx, args, with_cpu):
                     "4 spp/frame, depth 12; relMSE eps 0.01 vs a 4096-spp plain render",
         "reference_spp": args.ref_spp, "reference_render_s": round(ref_s, 2),
         "plain": plain, "nrc": nrc,
-        "nrc_config": "16 levels x 2 features x 2^19, 48-64x5-3 MLP bf16 on MFMA, Adam 1e-2, 16 x 8192 samples "
+        "nrc_config": "16 levels x 2 features x 2^19, 48-64x5-3 MLP bf16 on MFMA, Adam 1e-2, 16 x 4096 samples "
                       "and 65536 training paths per frame (steps overlap the next frame), area-spread c = 1e-4",
         "speedup": (round(plain["seconds"] / nrc["seconds"], 3)
                     if plain["seconds"] and nrc["seconds"] else None),

@@ -468,8 +468,8 @@ typedef struct rrhip_nrc_info {
 /* Creates a cache with freshly initialised parameters (grid U(-1e-4,
  * 1e-4), He-uniform weights, both from PCG32(seed)). config = NULL takes
  * the defaults: 16 levels, 2^19 entries, resolutions 16..2048, Adam
- * (1e-2, 0.9, 0.99, 1e-15), batch 8192, 16 steps and 65536 training
- * paths per pass, c = 1e-4, no moving average. */
+ * (1e-2, 0.9, 0.99, 1e-15), 16 steps of 4096 samples (65536 samples)
+ * and 65536 training paths per pass, c = 1e-4, no moving average. */
 int rrhip_nrc_create(rrhip_ctx* ctx, const rrhip_nrc_config* config, rrhip_nrc** out);
 int rrhip_nrc_destroy(rrhip_nrc* nrc);
 int rrhip_nrc_get_info(rrhip_nrc* nrc, rrhip_nrc_info* info);

@@ -408,6 +408,8 @@ def _med_sigs():
         "orc_medium_sample_n": (None, [vp, vp, vp, f64, vp, u64, u64, u64, vp, vp, vp]),
         "orc_medium_transmittance_n": (None, [vp, vp, vp, f64, u64, u64, u64, vp]),
         "orc_hg_phase": (f64, [f64, f64]),
+        "orc_equiangular_n": (None, [vp, vp, f64, vp, vp, u64, vp, vp, vp]),
+        "orc_distance_pdf_approx": (f64, [vp, f64, f64]),
         "orc_hg_sample_n": (None, [f64, vp, u64, u64, u64, vp, vp]),
     }
     for name, (res, args) in sig.items():
@@ -448,6 +450,20 @@ def medium_transmittance(medium, o, d, dist, seed, stream, n):
 
 def hg_phase(g, cos_theta):
     return _med_sigs().orc_hg_phase(g, cos_theta)
+
+
+def equiangular(o, d, t_max, light, u):
+    """volume.rs:195-232 for each u: (t, returned pdf, equiangular_pdf(t))."""
+    lib = _med_sigs()
+    o, d, light = (np.ascontiguousarray(x, np.float64) for x in (o, d, light))
+    u = np.ascontiguousarray(u, np.float64)
+    t, pdf, pdf_at = np.zeros_like(u), np.zeros_like(u), np.zeros_like(u)
+    lib.orc_equiangular_n(_p(o), _p(d), float(t_max), _p(light), _p(u), u.size, _p(t), _p(pdf), _p(pdf_at))
+    return t, pdf, pdf_at
+
+
+def distance_pdf_approx(medium, t, t_max):
+    return _med_sigs().orc_distance_pdf_approx(ctypes.addressof(medium), float(t), float(t_max))
 
 
 def hg_sample(g, wo, seed, stream, n):

@@ -889,6 +889,38 @@ Vec3 hg_sample(double g, const Vec3& wo, double u1, double u2, double* pdf) {
   return wi;
 }
 
+// volume.rs:195-216: t with density proportional to 1/r^2 towards the
+// light over [0, t_max], h the light's distance from the ray's line.
+double equiangular_sample(const Vec3& o, const Vec3& d, double t_max, const Vec3& light, double u, double* pdf) {
+  const double delta = (light - o).dot(d);
+  const double h = std::fmax((light - (o + d * delta)).length(), 1e-6);
+  const double th_a = std::atan(-delta / h);
+  const double th_b = std::atan((t_max - delta) / h);
+  const double span = std::fmax(th_b - th_a, 1e-9);
+  double t = delta + h * std::tan(th_a + u * span);
+  t = std::fmin(std::fmax(t, 0.0), t_max);
+  *pdf = h / (span * (h * h + (t - delta) * (t - delta)));
+  return t;
+}
+
+// volume.rs:219-232
+double equiangular_pdf(const Vec3& o, const Vec3& d, double t_max, const Vec3& light, double t) {
+  const double delta = (light - o).dot(d);
+  const double h = std::fmax((light - (o + d * delta)).length(), 1e-6);
+  const double th_a = std::atan(-delta / h);
+  const double th_b = std::atan((t_max - delta) / h);
+  const double span = std::fmax(th_b - th_a, 1e-9);
+  return h / (span * (h * h + (t - delta) * (t - delta)));
+}
+
+// volume.rs:234-240: an exponential in the mean extinction (a collision
+// before t_max) or its tail mass (no collision).
+double distance_pdf_approx(const Medium& m, double t, double t_max) {
+  const Vec3 st = m.sigma_t();
+  const double s = std::fmax((st.x + st.y + st.z) / 3.0, 1e-9);
+  return t < t_max ? s * std::exp(-s * t) : std::exp(-s * t_max);
+}
+
 // pt/mod.rs:1243-1306 for opaque scenes in an optional atmosphere.
 Vec3 Scene::visibility(const Vec3& origin, const Vec3& dir, double remaining, Pcg32& rng, uint64_t* rays) const {
   Vec3 tr{1, 1, 1};
@@ -1049,17 +1081,45 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
     if (rays) ++*rays;
     Intersection hit;
     const bool hit_found = intersect(ray, &hit);
-    // Atmosphere along this segment (pt/mod.rs:486-633; point lights, and
-    // so the equiangular strategy, are excluded by the scene loader).
+    // Atmosphere along this segment (pt/mod.rs:486-633).
     if (atmosphere >= 0) {
       const Medium& md = media[(size_t)atmosphere];
       const double t_limit = hit_found ? hit.t : 1e8;
+      const double t_seg = std::fmin(t_limit, md.max_distance);
       size_t seg_li = 0;
       double seg_pick = 0.0;
       bool has_pick = false;
       if (!lights.empty()) {
-        const Vec3 mid = ray.at(std::fmin(std::fmin(t_limit, md.max_distance) * 0.5, 50.0));
+        const Vec3 mid = ray.at(std::fmin(t_seg * 0.5, 50.0));
         has_pick = light_sampler.sample(mid, rng.next_f64(), &seg_li, &seg_pick);
+      }
+      const bool point_pick = has_pick && lights[seg_li].kind == RRHIP_LIGHT_POINT;
+      // Equiangular single-scatter connection to a picked point light
+      // (pt/mod.rs:501-551), MIS-weighted against distance sampling.
+      if (point_pick && t_seg > 1e-6) {
+        const Light& lt = lights[seg_li];
+        double pdf_eq;
+        const double t_eq = equiangular_sample(ray.origin, ray.direction, t_seg, lt.position, rng.next_f64(), &pdf_eq);
+        if (pdf_eq > 1e-12) {
+          const Vec3 p = ray.at(t_eq);
+          const double rho = md.density_at(p);
+          if (rho > 0.0) {
+            const Vec3 tr = medium_transmittance(md, ray.origin, ray.direction, t_eq, rng);
+            const Vec3 sigma_s = md.sigma_s * rho;
+            const Vec3 to_l = lt.position - p;
+            const double dist2 = std::fmax(to_l.length_squared(), 1e-12);
+            const Vec3 wi = to_l.normalize();
+            const Vec3 wo = -ray.direction.normalize();
+            const double ph = hg_phase(md.g, -wo.dot(wi));
+            const Vec3 vis = visibility(p, wi, std::sqrt(dist2) - 1e-3, rng, rays);
+            if (max_component(vis) > 0.0) {
+              const double pdf_d = distance_pdf_approx(md, t_eq, t_seg);
+              const double w_eq = pdf_eq / (pdf_eq + pdf_d);
+              const Vec3 c = sigma_s * tr * vis * lt.radiance() * (ph * w_eq / (dist2 * pdf_eq * seg_pick));
+              l = l + beta * c;
+            }
+          }
+        }
       }
       const MediumEvent ev = sample_distance(md, ray.origin, ray.direction, t_limit, beta, rng);
       beta = beta * ev.weight;
@@ -1074,7 +1134,15 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
             *f = Vec3{ph, ph, ph};
             *pdf = ph;
           };
-          l = l + beta * sample_light(lights[seg_li], p, Vec3::zero(), eval, seg_pick, rng, rays);
+          // collision NEE; for a point light paired with equiangular
+          // sampling by the MIS weight of pt/mod.rs:583-598
+          double w_dist = 1.0;
+          if (point_pick) {
+            const double pdf_d = distance_pdf_approx(md, ev.t, t_seg);
+            const double pdf_e = equiangular_pdf(ray.origin, ray.direction, t_seg, lights[seg_li].position, ev.t);
+            w_dist = pdf_d / (pdf_d + pdf_e);
+          }
+          l = l + beta * sample_light(lights[seg_li], p, Vec3::zero(), eval, seg_pick, rng, rays) * w_dist;
         }
         const double u1 = rng.next_f64(), u2 = rng.next_f64();
         double pdf;

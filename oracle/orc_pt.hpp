@@ -168,6 +168,11 @@ MediumEvent sample_distance(const Medium& m, const Vec3& o, const Vec3& d, doubl
 Vec3 medium_transmittance(const Medium& m, const Vec3& o, const Vec3& d, double dist, Pcg32& rng);
 double hg_phase(double g, double cos_theta);
 Vec3 hg_sample(double g, const Vec3& wo, double u1, double u2, double* pdf);
+// volume.rs:195-240 (Kulla-Fajardo equiangular sampling towards a point,
+// its pdf, and the approximate pdf of distance sampling used against it).
+double equiangular_sample(const Vec3& o, const Vec3& d, double t_max, const Vec3& light, double u, double* pdf);
+double equiangular_pdf(const Vec3& o, const Vec3& d, double t_max, const Vec3& light, double t);
+double distance_pdf_approx(const Medium& m, double t, double t_max);
 
 struct RenderParams {
   uint32_t max_depth = 10;

@@ -304,6 +304,21 @@ void orc_medium_transmittance_n(const rrhip_medium_desc* md, const double* o, co
 
 double orc_hg_phase(double g, double cos_theta) { return orc::hg_phase(g, cos_theta); }
 
+// Equiangular sampling (volume.rs:195-232) for n uniforms: t, the pdf the
+// sampler returns, and equiangular_pdf re-evaluated at t.
+void orc_equiangular_n(const double* o, const double* d, double t_max, const double* light, const double* u,
+                       uint64_t n, double* t, double* pdf, double* pdf_at) {
+  const Vec3 O{o[0], o[1], o[2]}, D{d[0], d[1], d[2]}, L{light[0], light[1], light[2]};
+  for (uint64_t i = 0; i < n; i++) {
+    t[i] = orc::equiangular_sample(O, D, t_max, L, u[i], &pdf[i]);
+    pdf_at[i] = orc::equiangular_pdf(O, D, t_max, L, t[i]);
+  }
+}
+
+double orc_distance_pdf_approx(const rrhip_medium_desc* md, double t, double t_max) {
+  return orc::distance_pdf_approx(medium_from(md), t, t_max);
+}
+
 void orc_hg_sample_n(double g, const double* wo, uint64_t seed, uint64_t stream, uint64_t n, double* wi,
                      double* pdf) {
   orc::Pcg32 rng = orc::Pcg32::make(seed, stream);

@@ -357,6 +357,9 @@ hipError_t upload_pt(rrhip_scene* s, const PtSceneHost& h) {
     s->pt_lo[c] = h.lo[c];
     s->pt_hi[c] = h.hi[c];
   }
+  s->pt_eq = false;
+  if (h.atmosphere >= 0)
+    for (const PtLight& l : h.lights) s->pt_eq |= l.kind == kLightPoint;
   return e;
 }
 

@@ -177,6 +177,13 @@ int rrhip_batch_create(rrhip_ctx* ctx, uint32_t num_jobs, const rrhip_batch_job*
   if (e == hipSuccess) e = alloc_n(b, &P.so, n);
   if (e == hipSuccess) e = alloc_n(b, &P.sd, n);
   if (e == hipSuccess) e = alloc_n(b, &P.sc, n);
+  bool eq = false;
+  for (uint32_t j = 0; j < num_jobs; j++) eq |= jobs[j].scene->pt_eq;
+  if (e == hipSuccess && eq) {
+    e = alloc_n(b, &P.so2, n);
+    if (e == hipSuccess) e = alloc_n(b, &P.sd2, n);
+    if (e == hipSuccess) e = alloc_n(b, &P.sc2, n);
+  }
   if (e == hipSuccess) e = alloc_n(b, &P.job, n);
   if (e == hipSuccess) e = alloc_n(b, &P.counts, 8);
   if (e == hipSuccess) e = alloc_n(b, &P.stats, 4);

@@ -45,7 +45,7 @@ rrhip_nrc_config default_config() {
   c.beta1 = 0.9f;
   c.beta2 = 0.99f;
   c.epsilon = 1e-15f;
-  c.batch = 8192;
+  c.batch = 4096;
   c.steps_per_pass = 16;
   c.train_paths = 65536;
   c.spread = 1e-4f;

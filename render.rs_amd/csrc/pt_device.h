@@ -622,6 +622,38 @@ __device__ __forceinline__ V3 medium_transmittance(const PtMedium& m, V3 o, V3 d
   return tr;
 }
 
+// volume.rs:195-232: equiangular distance sampling towards a point light
+// (Kulla and Fajardo 2012) over [0, t_max] of a unit-direction ray, and
+// its pdf at t.
+__device__ __forceinline__ void equiangular_frame(V3 o, V3 d, float t_max, V3 light, float& delta, float& h,
+                                                  float& th_a, float& span) {
+  delta = dot(light - o, d);
+  h = fmaxf(len(light - (o + d * delta)), 1e-6f);
+  th_a = atanf(-delta / h);
+  span = fmaxf(atanf((t_max - delta) / h) - th_a, 1e-9f);
+}
+
+__device__ __forceinline__ float equiangular_sample(V3 o, V3 d, float t_max, V3 light, float u, float& pdf) {
+  float delta, h, th_a, span;
+  equiangular_frame(o, d, t_max, light, delta, h, th_a, span);
+  const float t = fminf(fmaxf(delta + h * tanf(th_a + u * span), 0.f), t_max);
+  pdf = h / (span * (h * h + (t - delta) * (t - delta)));
+  return t;
+}
+
+__device__ __forceinline__ float equiangular_pdf(V3 o, V3 d, float t_max, V3 light, float t) {
+  float delta, h, th_a, span;
+  equiangular_frame(o, d, t_max, light, delta, h, th_a, span);
+  return h / (span * (h * h + (t - delta) * (t - delta)));
+}
+
+// volume.rs:234-240: the approximate distance-sampling pdf that weighs
+// equiangular sampling (mean extinction; tail mass past t_max).
+__device__ __forceinline__ float distance_pdf_approx(const PtMedium& m, float t, float t_max) {
+  const float s = fmaxf((m.sigma_t[0] + m.sigma_t[1] + m.sigma_t[2]) * (1.f / 3.f), 1e-9f);
+  return t < t_max ? s * expf(-s * t) : expf(-s * t_max);
+}
+
 // medium.rs:73-101
 __device__ __forceinline__ float hg_phase(float g, float cos_theta) {
   const float denom = 1.f + g * g + 2.f * g * cos_theta;

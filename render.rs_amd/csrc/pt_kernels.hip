@@ -302,11 +302,13 @@ __device__ __forceinline__ bool sample_light(const PtSceneView& S, const PtLight
 // (sum_{i=2..n} sqrt(|x_{i-1} - x_i|^2 / (p(w_i) |cos theta_i|)))^2 against
 // c * a0, a0 = |x0 - x1|^2 / (4 pi |cos theta_1|).
 // One path's shading step (the per-lane part of shade_kernel): sets
-// cont (continue in the next queue), shadow (+ so), query (+ nq, qbeta).
+// cont (continue in the next queue), shadow (+ so), shadow2 (+ se, an
+// equiangular connection in a medium), query (+ nq, qbeta).
 template <bool NRC>
 __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffers& P, const PtParams& prm,
                                           const NrcPathBuffers& N, uint32_t slot, bool& cont, bool& shadow,
-                                          bool& query, ShadowOut& so, NrcQuery& nq, V3& qbeta) {
+                                          bool& shadow2, ShadowOut& se, bool& query, ShadowOut& so, NrcQuery& nq,
+                                          V3& qbeta) {
     const float4 o4 = P.o[slot], d4 = P.d[slot], b4 = P.beta[slot], l4 = P.L[slot], pv4 = P.prev[slot];
     const float4 hn = P.hit_n[slot];
     uint32_t flags = P.flags[slot];
@@ -319,18 +321,50 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
     uint32_t depth = flags & 0xFFu;
     float a0 = d4.w, asum = pv4.w;
     // Atmosphere along this segment (pt/mod.rs:486-633): one light pick
-    // per segment, distance sampling, and at a real collision NEE with
-    // the phase function and an HG continuation instead of the surface.
+    // per segment; for a point light an equiangular single-scatter
+    // connection (pt/mod.rs:501-551); distance sampling, and at a real
+    // collision NEE with the phase function (MIS-paired with the
+    // equiangular strategy for point lights) and an HG continuation
+    // instead of the surface.
     bool in_medium = false, absorbed = false;
-    if (S.atmosphere >= 0) {
+    if (!NRC && S.atmosphere >= 0) {  // caches are never attached to scenes with media
       const PtMedium& md = S.media[S.atmosphere];
       const float t_limit = hn.w < 0.f ? 1e8f : hn.w;
+      const float t_seg = fminf(t_limit, md.max_distance);
       uint32_t seg_li = 0;
       float seg_pick = 0.f;
       bool has_pick = false;
-      if (S.num_lights > 0)
-        has_pick = ls_sample(S, ro + rd * fminf(fminf(t_limit, md.max_distance) * 0.5f, 50.f), rnd(rng), seg_li,
-                             seg_pick);
+      if (S.num_lights > 0) has_pick = ls_sample(S, ro + rd * fminf(t_seg * 0.5f, 50.f), rnd(rng), seg_li, seg_pick);
+      const bool point_pick = has_pick && P.so2 && S.lights[seg_li].kind == kLightPoint;
+      if (point_pick && t_seg > 1e-6f) {
+        const PtLight& L = S.lights[seg_li];
+        const V3 lp = v3(L.p);
+        float pdf_eq;
+        const float t_eq = equiangular_sample(ro, rd, t_seg, lp, rnd(rng), pdf_eq);
+        if (pdf_eq > 1e-12f) {
+          const V3 p = ro + rd * t_eq;
+          const float rho = density_at(md, p);
+          if (rho > 0.f) {
+            const V3 tr = medium_transmittance(md, ro, rd, t_eq, rng);
+            const V3 to_l = lp - p;
+            const float dist2 = fmaxf(dot(to_l, to_l), 1e-12f);
+            const V3 wi = normalize(to_l);
+            const float ph = hg_phase(md.g, dot(rd, wi));  // hg_phase(g, -wo.wi), wo = -rd
+            const float dist = sqrtf(dist2) - 1e-3f;
+            const V3 vis = medium_transmittance(md, p, wi, dist, rng);
+            if (maxc(vis) > 0.f) {
+              const float pdf_d = distance_pdf_approx(md, t_eq, t_seg);
+              const float w_eq = pdf_eq / (pdf_eq + pdf_d);
+              se.o = p;
+              se.d = wi;
+              se.tmax = dist;
+              se.c = beta * v3(md.sigma_s) * tr * vis * v3(L.radiance) *
+                     (rho * ph * w_eq / (dist2 * pdf_eq * seg_pick));
+              shadow2 = maxc(se.c) > 0.f;
+            }
+          }
+        }
+      }
       const MediumEvent ev = sample_distance(md, ro, rd, t_limit, beta, rng);
       beta = beta * ev.weight;
       if (ev.scatter) {
@@ -346,6 +380,11 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
           };
           if (sample_light(S, S.lights[seg_li], p, v3(0.f, 0.f, 0.f), eval, seg_pick, rng, so)) {
             so.c = beta * so.c * medium_transmittance(md, so.o, so.d, so.tmax, rng);
+            if (point_pick) {  // pt/mod.rs:583-598
+              const float pdf_d = distance_pdf_approx(md, ev.t, t_seg);
+              const float pdf_e = equiangular_pdf(ro, rd, t_seg, v3(S.lights[seg_li].p), ev.t);
+              so.c = so.c * (pdf_d / (pdf_d + pdf_e));
+            }
             shadow = maxc(so.c) > 0.f;
           }
         }
@@ -467,7 +506,8 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
           if (sample_light(S, S.lights[li], p, nrm, eval, pick, rng, so)) {
             so.c = beta * so.c;
             // shadow rays through the atmosphere carry its transmittance
-            if (S.atmosphere >= 0) so.c = so.c * medium_transmittance(S.media[S.atmosphere], so.o, so.d, so.tmax, rng);
+            if (!NRC && S.atmosphere >= 0)
+              so.c = so.c * medium_transmittance(S.media[S.atmosphere], so.o, so.d, so.tmax, rng);
             shadow = maxc(so.c) > 0.f;
           }
         }
@@ -502,6 +542,27 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
     P.flags[slot] = flags;
 }
 
+// Appends a lane's shadow record (all lanes of the wave call it): the
+// first ray is `a` (or `b` alone); with both, `b` goes to the second
+// arrays and sc.w = 1 marks the record.
+__device__ __forceinline__ void push_shadow(const PathBuffers& P, bool act, uint32_t slot, bool s1, const ShadowOut& a,
+                                            bool s2, const ShadowOut& b) {
+  const bool any = act && (s1 || s2);
+  const uint32_t idx = wave_append(&P.counts[2], any);
+  if (!any) return;
+  const ShadowOut& f = s1 ? a : b;
+  const bool both = s1 && s2;
+  P.sq[idx] = slot;
+  P.so[idx] = make_float4(f.o.x, f.o.y, f.o.z, f.tmax);
+  P.sd[idx] = make_float4(f.d.x, f.d.y, f.d.z, 0.f);
+  P.sc[idx] = make_float4(f.c.x, f.c.y, f.c.z, both ? 1.f : 0.f);
+  if (both) {
+    P.so2[idx] = make_float4(b.o.x, b.o.y, b.o.z, b.tmax);
+    P.sd2[idx] = make_float4(b.d.x, b.d.y, b.d.z, 0.f);
+    P.sc2[idx] = make_float4(b.c.x, b.c.y, b.c.z, 0.f);
+  }
+}
+
 template <bool NRC>
 __global__ __launch_bounds__(kBlock) void shade_kernel(PtSceneView S, PathBuffers P, PtParams prm,
                                                       const uint32_t* __restrict__ q, uint32_t* __restrict__ qnext,
@@ -512,22 +573,16 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtSceneView S, PathBuffer
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
     const uint32_t i = base + threadIdx.x;
     const bool act = i < n;
-    bool cont = false, shadow = false, query = false;
-    ShadowOut so;
+    bool cont = false, shadow = false, shadow2 = false, query = false;
+    ShadowOut so, se;
     NrcQuery nq;
     V3 qbeta;
     uint32_t slot = 0;
     if (act) {
       slot = q[i];
-      shade_one<NRC>(S, P, prm, N, slot, cont, shadow, query, so, nq, qbeta);
+      shade_one<NRC>(S, P, prm, N, slot, cont, shadow, shadow2, se, query, so, nq, qbeta);
     }
-    const uint32_t sidx = wave_append(&P.counts[2], act && shadow);
-    if (act && shadow) {
-      P.sq[sidx] = slot;
-      P.so[sidx] = make_float4(so.o.x, so.o.y, so.o.z, so.tmax);
-      P.sd[sidx] = make_float4(so.d.x, so.d.y, so.d.z, 0.f);
-      P.sc[sidx] = make_float4(so.c.x, so.c.y, so.c.z, 0.f);
-    }
+    push_shadow(P, act, slot, shadow, so, shadow2, se);
     const uint32_t nidx = wave_append(&P.counts[1], act && cont);
     if (act && cont) qnext[nidx] = slot;
     if constexpr (NRC) {
@@ -541,16 +596,31 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtSceneView S, PathBuffer
   }
 }
 
-// Shadow ray i: adds its contribution to its path when unoccluded.
+// Shadow record i: adds each unoccluded ray's contribution to its path
+// (the first ray, then the second when the record has one).
 __device__ __forceinline__ void shadow_one(const PtSceneView& S, const PathBuffers& P, uint32_t i, uint32_t* stk) {
-  const float4 o = P.so[i], d = P.sd[i];
-  if (scene_occluded<kTraceBlock>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, stk)) return;
+  const float4 o = P.so[i], d = P.sd[i], c = P.sc[i];
+  const bool v1 = !scene_occluded<kTraceBlock>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, stk);
+  bool v2 = false;
+  float4 c2 = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (c.w != 0.f) {
+    const float4 o2 = P.so2[i], d2 = P.sd2[i];
+    v2 = !scene_occluded<kTraceBlock>(S, v3(o2.x, o2.y, o2.z), v3(d2.x, d2.y, d2.z), o2.w, stk);
+    c2 = P.sc2[i];
+  }
+  if (!v1 && !v2) return;
   const uint32_t slot = P.sq[i];
-  const float4 c = P.sc[i];
   float4 l = P.L[slot];
-  l.x += c.x;
-  l.y += c.y;
-  l.z += c.z;
+  if (v1) {
+    l.x += c.x;
+    l.y += c.y;
+    l.z += c.z;
+  }
+  if (v2) {
+    l.x += c2.x;
+    l.y += c2.y;
+    l.z += c2.z;
+  }
   P.L[slot] = l;
 }
 
@@ -663,23 +733,17 @@ __global__ __launch_bounds__(kBlock) void shade_batch_kernel(const PtJob* __rest
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
     const uint32_t i = base + threadIdx.x;
     const bool act = i < n;
-    bool cont = false, shadow = false, query = false;
-    ShadowOut so;
+    bool cont = false, shadow = false, shadow2 = false, query = false;
+    ShadowOut so, se;
     NrcQuery nq;
     V3 qbeta;
     uint32_t slot = 0;
     if (act) {
       slot = q[i];
       const PtJob& J = jobs[P.job[slot]];
-      shade_one<false>(J.S, P, J.prm, N, slot, cont, shadow, query, so, nq, qbeta);
+      shade_one<false>(J.S, P, J.prm, N, slot, cont, shadow, shadow2, se, query, so, nq, qbeta);
     }
-    const uint32_t sidx = wave_append(&P.counts[2], act && shadow);
-    if (act && shadow) {
-      P.sq[sidx] = slot;
-      P.so[sidx] = make_float4(so.o.x, so.o.y, so.o.z, so.tmax);
-      P.sd[sidx] = make_float4(so.d.x, so.d.y, so.d.z, 0.f);
-      P.sc[sidx] = make_float4(so.c.x, so.c.y, so.c.z, 0.f);
-    }
+    push_shadow(P, act, slot, shadow, so, shadow2, se);
     const uint32_t nidx = wave_append(&P.counts[1], act && cont);
     if (act && cont) qnext[nidx] = slot;
   }

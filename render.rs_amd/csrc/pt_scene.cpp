@@ -462,10 +462,6 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
     out->media_grids.push_back(std::move(grid));
   }
   out->atmosphere = d.num_media > 0 ? d.atmosphere : -1;
-  if (out->atmosphere >= 0)
-    for (uint32_t i = 0; i < d.num_lights; i++)
-      if (d.lights[i].kind == RRHIP_LIGHT_POINT)
-        return *err = "UNSUPPORTED: point lights inside an atmosphere (equiangular sampling) are not implemented", false;
   for (int a = 0; a < 3; a++) {
     out->lo[a] = INFINITY;
     out->hi[a] = -INFINITY;

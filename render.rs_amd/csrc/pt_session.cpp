@@ -120,6 +120,11 @@ hipError_t alloc_paths(rrhip_session* s, uint64_t n) {
   if (e == hipSuccess) e = alloc_in(m, &P.so, n);
   if (e == hipSuccess) e = alloc_in(m, &P.sd, n);
   if (e == hipSuccess) e = alloc_in(m, &P.sc, n);
+  if (e == hipSuccess && s->scene->pt_eq) {
+    e = alloc_in(m, &P.so2, n);
+    if (e == hipSuccess) e = alloc_in(m, &P.sd2, n);
+    if (e == hipSuccess) e = alloc_in(m, &P.sc2, n);
+  }
   if (e == hipSuccess) e = alloc_in(m, &P.counts, 4);
   if (e == hipSuccess) s->path_cap = n;
   return e;

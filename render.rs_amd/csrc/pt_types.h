@@ -163,7 +163,10 @@ struct PathBuffers {
   uint32_t* sq;     // shadow queue: slot ids
   float4* so;       // shadow origin.xyz, tmax
   float4* sd;       // shadow direction.xyz
-  float4* sc;       // shadow contribution rgb (already times beta)
+  float4* sc;       // shadow contribution rgb (already times beta), w = 1: the record has a second ray
+  float4* so2;      // second ray of a record (equiangular connections in media): origin, tmax;
+  float4* sd2;      //   direction; contribution (all three null when no scene needs them)
+  float4* sc2;
   uint32_t* counts; // [0] live paths, [1] next live, [2] shadow rays, [3] unused
   unsigned long long* stats;  // [0] extension rays, [1] shadow rays, [2] cache queries, [3] training samples
   uint32_t* job;    // batches: job of each slot (null in sessions)

@@ -37,4 +37,5 @@ struct rrhip_scene {
   void* pt_mem[15] = {};
   std::vector<void*> media_mem;  // density grids
   float pt_lo[3] = {0, 0, 0}, pt_hi[3] = {0, 0, 0};  // bounds of all geometry
+  bool pt_eq = false;  // point lights in an atmosphere: shadow records carry a second ray
 };

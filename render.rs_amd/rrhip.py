@@ -166,7 +166,7 @@ class NrcConfig(ctypes.Structure):
 
     @staticmethod
     def make(levels=16, log2_table=19, seed=1, base_res=16.0, max_res=2048.0, lo=(0, 0, 0), hi=(0, 0, 0),
-             learning_rate=1e-2, beta1=0.9, beta2=0.99, epsilon=1e-15, batch=8192, steps_per_pass=16,
+             learning_rate=1e-2, beta1=0.9, beta2=0.99, epsilon=1e-15, batch=4096, steps_per_pass=16,
              train_paths=65536, spread=1e-4, ema=0.0):
         return NrcConfig(ctypes.sizeof(NrcConfig), levels, log2_table, seed, base_res, max_res,
                          (ctypes.c_float * 3)(*lo), (ctypes.c_float * 3)(*hi), learning_rate, beta1, beta2,

@@ -18,13 +18,15 @@ def say(msg):
 
 
 def test_batch_jobs_equal_their_own_sessions(ctx):
-    say("batch of 6 mixed rooms (3 with clouds) vs 6 sessions")
+    say("batch of 6 mixed rooms (3 with clouds, one with a point light in it) vs 6 sessions")
     rng = np.random.default_rng(3)
     jobs = []
     for j, (res, spp) in enumerate([(32, 5), (48, 12), (16, 3), (40, 7), (24, 9), (32, 1)]):
         spheres = [(tuple(rng.uniform(-0.7, 0.7, 3)), float(rng.uniform(0.05, 0.3))) for _ in range(j * 3 + 1)]
         if j % 2:
             s = scenes.cloud_room(res, res, n=16, seed=j, spheres=spheres)
+            if j == 3:  # a point light in the cloud: two-ray shadow records (equiangular)
+                s.add_light(scenes.light(rrhip.LIGHT_POINT, intensity=2.0, p=(0.2, 0.1, 0.0)))
             depth = 64
         else:
             s = scenes.box_room(res, res)

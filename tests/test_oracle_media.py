@@ -133,3 +133,83 @@ def test_cloud_density_grid_follows_the_fbm_formula():
     got = grid[idx[:, 2], idx[:, 1], idx[:, 0]]
     assert np.abs(got - want).max() < 1e-6
     assert grid.min() >= 0 and grid.max() <= 1 and 0.05 < (grid > 0).mean() < 0.95
+
+
+def test_equiangular_sampling_inverts_its_cdf_and_normalises():
+    """volume.rs:195-232: t = delta + h tan(theta_a + u span) inverts the
+    CDF (atan((t - delta) / h) - theta_a) / span; the returned pdf equals
+    equiangular_pdf(t) and integrates to 1 over [0, t_max]; the samples
+    follow it (binned counts)."""
+    rng = np.random.default_rng(5)
+    for _ in range(4):
+        o = rng.uniform(-2, 2, 3)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        light = rng.uniform(-3, 3, 3)
+        t_max = rng.uniform(0.5, 8.0)
+        u = rng.uniform(0, 1, 200000)
+        t, pdf, pdf_at = orc.equiangular(o, d, t_max, light, u)
+        delta = (light - o) @ d
+        h = max(np.linalg.norm(light - (o + d * delta)), 1e-6)
+        th_a, th_b = np.arctan(-delta / h), np.arctan((t_max - delta) / h)
+        cdf = (np.arctan((t - delta) / h) - th_a) / (th_b - th_a)
+        assert np.allclose(cdf, u, atol=1e-9)
+        assert np.allclose(pdf, pdf_at, rtol=1e-12)
+        grid = np.linspace(0, t_max, 200001)
+        dens = h / ((th_b - th_a) * (h * h + (grid - delta) ** 2))
+        assert abs(np.trapezoid(dens, grid) - 1.0) < 1e-6
+        hist, edges = np.histogram(t, bins=20, range=(0, t_max))
+        want = np.array([np.trapezoid(dens[(grid >= a) & (grid <= b)], grid[(grid >= a) & (grid <= b)])
+                         for a, b in zip(edges[:-1], edges[1:])]) * t.size
+        assert np.all(np.abs(hist - want) < 5 * np.sqrt(want) + 5)
+
+
+def test_distance_pdf_approx_is_a_density():
+    """volume.rs:234-240: mean extinction s, s e^{-st} before t_max plus
+    the tail mass e^{-s t_max} sums to 1."""
+    m = medium((0.2, 0.4, 0.6), (0.3, 0.1, 0.5))
+    s = (0.5 + 0.5 + 1.1) / 3
+    t_max = 3.0
+    ts = np.linspace(0, t_max, 100001)[:-1]
+    vals = np.array([orc.distance_pdf_approx(m, t, t_max) for t in ts[::1000]])
+    assert np.allclose(vals, s * np.exp(-s * ts[::1000]), rtol=1e-12)
+    tail = orc.distance_pdf_approx(m, t_max, t_max)
+    assert abs(tail - np.exp(-s * t_max)) < 1e-12
+    assert abs((1 - np.exp(-s * t_max)) + tail - 1.0) < 1e-12
+
+
+def single_scatter_scene(w=1, h=1):
+    """A camera looking down +z through a homogeneous atmosphere at nothing,
+    a point light off the axis: with one bounce the pixel holds the single-
+    scattered radiance (equiangular + collision NEE)."""
+    import scenes
+    s = scenes.Scene(w, h, 0.01, eye=(0, 0, 0), look_at=(0, 0, 1))
+    s.add_light(scenes.light(rrhip.LIGHT_POINT, intensity=20.0, color=(1.0, 0.8, 0.6), p=(0.8, 0.3, 4.0)))
+    s.atmosphere = s.add_medium((0.05, 0.1, 0.15), (0.3, 0.25, 0.2), 0.3)
+    return s
+
+
+def single_scatter_quadrature():
+    """L = int_0^inf sigma_s e^{-sigma_t t} hg(g, d.wi) I / r^2
+    e^{-sigma_t (r - 1e-3)} dt along the axis (hg_phase(g, -wo.wi) with wo =
+    -d, as pt/mod.rs:524-530 evaluates it)."""
+    sa, ss, g = np.array([0.05, 0.1, 0.15]), np.array([0.3, 0.25, 0.2]), 0.3
+    st = sa + ss
+    light, I = np.array([0.8, 0.3, 4.0]), 20.0 * np.array([1.0, 0.8, 0.6])
+    t = np.linspace(0, 150, 3_000_001)
+    to_l = light[None, :] - np.stack([np.zeros_like(t), np.zeros_like(t), t], 1)
+    r = np.linalg.norm(to_l, axis=1)
+    cos = to_l[:, 2] / r
+    ph = (1 - g * g) / (4 * np.pi * (1 + g * g + 2 * g * cos) ** 1.5)
+    f = ss[None, :] * np.exp(-np.outer(t, st)) * (ph / r ** 2)[:, None] * I[None, :] * np.exp(-np.outer(r - 1e-3, st))
+    return np.trapezoid(f, t, axis=0)
+
+
+def test_single_scatter_towards_a_point_light_matches_quadrature():
+    s = single_scatter_scene()
+    acc, _ = orc.render(s.desc(), rrhip.RenderParams.make(max_depth=1, firefly_clamp=0.0), 0, 200000)
+    n = acc[0, 6]
+    mean = acc[0, :3] / n
+    se = np.sqrt(np.maximum(acc[0, 3:6] / n - mean ** 2, 0) / n)
+    want = single_scatter_quadrature()
+    assert np.all(np.abs(mean - want) < 4 * se + 1e-4 * want), (mean, want, se)
