@@ -133,7 +133,11 @@ typedef struct rrhip_material_desc {
   double glow[3];
   double emission[3];
   int32_t area_light;             /* light index for MIS on emitter hits, -1 = none */
-  uint32_t reserved;
+  uint32_t interior;              /* 1 + index of the medium inside closed surfaces of this
+                                     material (Material::interior); 0 = none. With no lobes and
+                                     no emission the surface is an invisible volume hull */
+  double cutout;                  /* 1 - presence (PbrParams::presence): the probability that
+                                     a ray passes through the surface; 0 = opaque */
 } rrhip_material_desc;
 
 /* Light kinds (render.rs LightType, src/scene/light.rs:7-33). */

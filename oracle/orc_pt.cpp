@@ -921,21 +921,55 @@ double distance_pdf_approx(const Medium& m, double t, double t_max) {
   return t < t_max ? s * std::exp(-s * t) : std::exp(-s * t_max);
 }
 
-// pt/mod.rs:1243-1306 for opaque scenes in an optional atmosphere.
-Vec3 Scene::visibility(const Vec3& origin, const Vec3& dir, double remaining, Pcg32& rng, uint64_t* rays) const {
-  Vec3 tr{1, 1, 1};
-  if (atmosphere >= 0) tr = medium_transmittance(media[(size_t)atmosphere], origin, dir, remaining, rng);
-  if (rays) ++*rays;
-  Intersection h;
-  Ray r = Ray::make(origin, dir);
-  if (!intersect(r, &h)) return tr;
-  return h.t >= remaining ? tr : Vec3::zero();
+bool Material::hull() const {
+  double w[5];
+  return interior >= 0 && !pbr.lobe_weights(w) && max_component(emission + pbr.glow) <= 0.0;
+}
+
+// pt/mod.rs:1238-1306. The stack starts from the atmosphere with the
+// shading point's medium on top (when it is another one).
+Vec3 Scene::visibility(const Vec3& origin0, const Vec3& dir, double remaining, Pcg32& rng, uint64_t* rays,
+                       int medium) const {
+  MediumStack ms;
+  ms.atmosphere = atmosphere;
+  if (medium >= 0 && medium != atmosphere) ms.push(medium);
+  Vec3 vis{1, 1, 1};
+  Vec3 origin = origin0;
+  for (int i = 0; i < 16; i++) {
+    if (rays) ++*rays;
+    Intersection h;
+    const bool hit = intersect(Ray::make(origin, dir), &h);
+    const double seg = std::fmin(hit ? h.t : remaining, remaining);
+    const int cur = ms.current();
+    if (cur >= 0) {
+      vis = vis * medium_transmittance(media[(size_t)cur], origin, dir, seg, rng);
+      if (max_component(vis) < 1e-4) return Vec3::zero();
+    }
+    if (!hit || h.t >= remaining) return vis;
+    const Material& m = materials[h.material];
+    if (m.hull()) {
+      if (h.front_face)
+        ms.push(m.interior);
+      else
+        ms.pop();
+      origin = h.point + dir * 1e-4;
+      remaining -= h.t + 1e-4;
+      continue;
+    }
+    const double presence = std::fmin(std::fmax(m.presence, 0.0), 1.0);
+    if (presence >= 1.0) return Vec3::zero();
+    vis = vis * (1.0 - presence);
+    if (max_component(vis) < 1e-4) return Vec3::zero();
+    origin = h.point + dir * 1e-4;
+    remaining -= h.t + 1e-4;
+  }
+  return vis;
 }
 
 // pt/mod.rs:1177-1209
 Vec3 Scene::area_contribution(const Vec3& p, const Vec3& n, const Vec3& sample_point,
                               const Vec3& light_normal, double area, const Light& light, const EvalFn& eval,
-                              double pick_pmf, Pcg32& rng, uint64_t* rays) const {
+                              double pick_pmf, Pcg32& rng, uint64_t* rays, int medium) const {
   Vec3 to_light = sample_point - p;
   double dist2 = std::fmax(to_light.length_squared(), 1e-12);
   Vec3 wi = to_light.normalize();
@@ -948,7 +982,7 @@ Vec3 Scene::area_contribution(const Vec3& p, const Vec3& n, const Vec3& sample_p
   // visibility_to (pt/mod.rs:1212-1224)
   Vec3 dir = sample_point - p;
   double dist = dir.length();
-  Vec3 vis = visibility(p + n * 1e-4, dir.normalize(), dist - 1e-3, rng, rays);
+  Vec3 vis = visibility(p + n * 1e-4, dir.normalize(), dist - 1e-3, rng, rays, medium);
   if (max_component(vis) <= 0.0) return Vec3::zero();
   double pdf_sa = pick_pmf * dist2 / (cos_light * area);
   double weight = power_heuristic(pdf_sa, bsdf_pdf);
@@ -957,7 +991,7 @@ Vec3 Scene::area_contribution(const Vec3& p, const Vec3& n, const Vec3& sample_p
 
 // pt/mod.rs:1032-1174
 Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const EvalFn& eval, double pick_pmf,
-                         Pcg32& rng, uint64_t* rays) const {
+                         Pcg32& rng, uint64_t* rays, int medium) const {
   switch (light.kind) {
     case RRHIP_LIGHT_POINT: {
       Vec3 to_light = light.position - p;
@@ -968,7 +1002,7 @@ Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const
       eval(wi, &f, &pdf);
       if (max_component(f) <= 0.0) return Vec3::zero();
       Vec3 dir = light.position - p;
-      Vec3 vis = visibility(p + n * 1e-4, dir.normalize(), dir.length() - 1e-3, rng, rays);
+      Vec3 vis = visibility(p + n * 1e-4, dir.normalize(), dir.length() - 1e-3, rng, rays, medium);
       if (max_component(vis) <= 0.0) return Vec3::zero();
       return f * light.radiance() * vis / (dist2 * pick_pmf);
     }
@@ -988,20 +1022,20 @@ Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const
       double pdf;
       eval(wi, &f, &pdf);
       if (max_component(f) <= 0.0) return Vec3::zero();
-      Vec3 vis = visibility(p + n * 1e-4, wi, 1e8, rng, rays);
+      Vec3 vis = visibility(p + n * 1e-4, wi, 1e8, rng, rays, medium);
       if (max_component(vis) <= 0.0) return Vec3::zero();
       return f * light.radiance() * vis / pick_pmf;
     }
     case RRHIP_LIGHT_RECT: {
       double u = rng.next_f64(), v = rng.next_f64();
       Vec3 sp = light.corner + light.edge1 * u + light.edge2 * v;
-      return area_contribution(p, n, sp, light.normal, light.area, light, eval, pick_pmf, rng, rays);
+      return area_contribution(p, n, sp, light.normal, light.area, light, eval, pick_pmf, rng, rays, medium);
     }
     case RRHIP_LIGHT_DISK: {
       double u = rng.next_f64(), v = rng.next_f64();
       double r = std::sqrt(u), phi = 2.0 * PI * v;
       Vec3 sp = light.center + light.e1 * (r * std::cos(phi)) + light.e2 * (r * std::sin(phi));
-      return area_contribution(p, n, sp, light.normal, light.area, light, eval, pick_pmf, rng, rays);
+      return area_contribution(p, n, sp, light.normal, light.area, light, eval, pick_pmf, rng, rays, medium);
     }
     case RRHIP_LIGHT_SPHERE: {
       Vec3 to_center = light.center - p;
@@ -1023,7 +1057,7 @@ Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const
       double dist = std::sqrt(dist2) * cos_t - std::sqrt(std::fmax(r * r - dist2 * sin_t * sin_t, 0.0));
       Vec3 target = p + wi * dist;
       Vec3 dir = target - p;
-      Vec3 vis = visibility(p + n * 1e-4, dir.normalize(), dir.length() - 1e-3, rng, rays);
+      Vec3 vis = visibility(p + n * 1e-4, dir.normalize(), dir.length() - 1e-3, rng, rays, medium);
       if (max_component(vis) <= 0.0) return Vec3::zero();
       double pdf_sa = pick_pmf / std::fmax(2.0 * PI * (1.0 - cos_max), 1e-12);
       double weight = power_heuristic(pdf_sa, bsdf_pdf);
@@ -1040,7 +1074,7 @@ Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const
       double bsdf_pdf;
       eval(wi, &f, &bsdf_pdf);
       if (max_component(f) <= 0.0) return Vec3::zero();
-      Vec3 vis = visibility(p + n * 1e-4, wi, 1e8, rng, rays);
+      Vec3 vis = visibility(p + n * 1e-4, wi, 1e8, rng, rays, medium);
       if (max_component(vis) <= 0.0) return Vec3::zero();
       double weight = power_heuristic(pdf_sa, bsdf_pdf);
       return f * light.radiance() * vis * (weight / pdf_sa);
@@ -1077,13 +1111,17 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
   Vec3 prev_origin = ray.origin;
   bool from_camera = true;
   uint32_t depth = 0;
+  MediumStack ms;  // pt/mod.rs:482
+  ms.atmosphere = atmosphere;
+  uint32_t skips = 0;  // hull crossings and cutout pass-throughs (pt/mod.rs:476)
   while (depth < rp.max_depth) {
     if (rays) ++*rays;
     Intersection hit;
     const bool hit_found = intersect(ray, &hit);
-    // Atmosphere along this segment (pt/mod.rs:486-633).
-    if (atmosphere >= 0) {
-      const Medium& md = media[(size_t)atmosphere];
+    // The medium this segment runs through (pt/mod.rs:486-633).
+    const int medium = ms.current();
+    if (medium >= 0) {
+      const Medium& md = media[(size_t)medium];
       const double t_limit = hit_found ? hit.t : 1e8;
       const double t_seg = std::fmin(t_limit, md.max_distance);
       size_t seg_li = 0;
@@ -1111,7 +1149,7 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
             const Vec3 wi = to_l.normalize();
             const Vec3 wo = -ray.direction.normalize();
             const double ph = hg_phase(md.g, -wo.dot(wi));
-            const Vec3 vis = visibility(p, wi, std::sqrt(dist2) - 1e-3, rng, rays);
+            const Vec3 vis = visibility(p, wi, std::sqrt(dist2) - 1e-3, rng, rays, medium);
             if (max_component(vis) > 0.0) {
               const double pdf_d = distance_pdf_approx(md, t_eq, t_seg);
               const double w_eq = pdf_eq / (pdf_eq + pdf_d);
@@ -1142,7 +1180,7 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
             const double pdf_e = equiangular_pdf(ray.origin, ray.direction, t_seg, lights[seg_li].position, ev.t);
             w_dist = pdf_d / (pdf_d + pdf_e);
           }
-          l = l + beta * sample_light(lights[seg_li], p, Vec3::zero(), eval, seg_pick, rng, rays) * w_dist;
+          l = l + beta * sample_light(lights[seg_li], p, Vec3::zero(), eval, seg_pick, rng, rays, medium) * w_dist;
         }
         const double u1 = rng.next_f64(), u2 = rng.next_f64();
         double pdf;
@@ -1176,6 +1214,25 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
       break;
     }
     const Material& mat = materials[hit.material];
+    // Invisible volume hull: the crossing toggles the medium
+    // (pt/mod.rs:662-683).
+    if (mat.hull() && skips < 16) {
+      skips++;
+      if (hit.front_face)
+        ms.push(mat.interior);
+      else
+        ms.pop();
+      ray = Ray::make(hit.point + ray.direction.normalize() * 1e-4, ray.direction);
+      continue;
+    }
+    // Presence cutout: pass through with probability 1 - presence
+    // (pt/mod.rs:702-710).
+    const double presence = std::fmin(std::fmax(mat.presence, 0.0), 1.0);
+    if (presence < 1.0 && rng.next_f64() >= presence && skips < 16) {
+      skips++;
+      ray = Ray::make(hit.point + ray.direction.normalize() * 1e-4, ray.direction);
+      continue;
+    }
     const PbrParams& pbr = mat.pbr;
     Vec3 wo = -ray.direction.normalize();
     bool entering = hit.front_face;
@@ -1209,7 +1266,7 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
           bsdf_eval_pdf(pbr, wo_l, wi_l, eta_rel, f, pdf);
           *f = *f * wi_l.z;
         };
-        Vec3 c = sample_light(lights[li], hit.point, n, eval, pmf, rng, rays);
+        Vec3 c = sample_light(lights[li], hit.point, n, eval, pmf, rng, rays, medium);
         l = l + beta * c;
       }
     }
@@ -1220,6 +1277,12 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
     prev_pdf = s.pdf;
     prev_origin = hit.point;
     from_camera = false;
+    if (s.transmitted && mat.interior >= 0) {  // pt/mod.rs:969-975
+      if (entering)
+        ms.push(mat.interior);
+      else
+        ms.pop();
+    }
     double offset = s.transmitted ? -1e-4 : 1e-4;
     ray = Ray::make(hit.point + n * offset, wi_world);
     depth++;
@@ -1325,6 +1388,8 @@ void Scene::load(const rrhip_scene_desc& d) {
     p.glow = v3(m.glow);
     mt.emission = v3(m.emission);
     mt.area_light = m.area_light;
+    mt.presence = 1.0 - m.cutout;
+    mt.interior = (int)m.interior - 1;
     materials.push_back(mt);
   }
 

@@ -45,6 +45,25 @@ struct Material {
   PbrParams pbr;
   Vec3 emission{0, 0, 0};
   int area_light = -1;
+  double presence = 1.0;  // PbrParams::presence (pbr.rs:39)
+  int interior = -1;      // Material::interior (material.rs:37)
+  // pt/mod.rs:662-667: an interior medium and no lobes or emission.
+  bool hull() const;
+};
+
+// pt/mod.rs:33-69: the media a path is inside of, innermost on top; the
+// atmosphere is the implicit bottom.
+struct MediumStack {
+  int stack[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int depth = 0;
+  int atmosphere = -1;
+  int current() const { return depth > 0 ? stack[depth - 1] : atmosphere; }
+  void push(int m) {
+    if (depth < 8) stack[depth++] = m;
+  }
+  void pop() {
+    if (depth > 0) depth--;
+  }
 };
 
 // bxdf.rs:16-39
@@ -209,14 +228,17 @@ class Scene {
   // eval(wi, f, pdf): f * cos (or the phase value) towards world wi and the
   // pdf of the scattering strategy, for MIS.
   using EvalFn = std::function<void(const Vec3&, Vec3*, double*)>;
+  // `medium`: the medium the shading point is in (-1: none).
   Vec3 sample_light(const Light& light, const Vec3& p, const Vec3& n, const EvalFn& eval, double pick_pmf,
-                    Pcg32& rng, uint64_t* rays) const;
+                    Pcg32& rng, uint64_t* rays, int medium) const;
   Vec3 area_contribution(const Vec3& p, const Vec3& n, const Vec3& sample_point,
                          const Vec3& light_normal, double area, const Light& light, const EvalFn& eval,
-                         double pick_pmf, Pcg32& rng, uint64_t* rays) const;
-  // Fractional visibility: the atmosphere's transmittance along the whole
-  // segment (drawn first, as the kernels do), then 0 if anything blocks it.
-  Vec3 visibility(const Vec3& origin, const Vec3& dir, double remaining, Pcg32& rng, uint64_t* rays) const;
+                         double pick_pmf, Pcg32& rng, uint64_t* rays, int medium) const;
+  // Fractional visibility (pt/mod.rs:1238-1306): walks the blockers,
+  // multiplying each segment's medium transmittance; hulls push / pop
+  // media, cutouts attenuate by 1 - presence, anything else blocks.
+  Vec3 visibility(const Vec3& origin, const Vec3& dir, double remaining, Pcg32& rng, uint64_t* rays,
+                  int medium) const;
 };
 
 }  // namespace orc

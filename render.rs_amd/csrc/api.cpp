@@ -342,6 +342,7 @@ hipError_t upload_pt(rrhip_scene* s, const PtSceneHost& h) {
   if (e == hipSuccess) e = upload_vec(media, &s->pt_mem[14], &v.media);
   v.atmosphere = h.atmosphere;
   v.stack_need = h.stack_need;
+  v.transparent = h.transparent;
   v.num_insts = (uint32_t)h.insts.size();
   v.num_tris = (uint32_t)h.tris.size();
   v.num_spheres = (uint32_t)h.spheres.size();

@@ -177,12 +177,19 @@ int rrhip_batch_create(rrhip_ctx* ctx, uint32_t num_jobs, const rrhip_batch_job*
   if (e == hipSuccess) e = alloc_n(b, &P.so, n);
   if (e == hipSuccess) e = alloc_n(b, &P.sd, n);
   if (e == hipSuccess) e = alloc_n(b, &P.sc, n);
-  bool eq = false;
-  for (uint32_t j = 0; j < num_jobs; j++) eq |= jobs[j].scene->pt_eq;
+  bool eq = false, transparent = false;
+  for (uint32_t j = 0; j < num_jobs; j++) {
+    eq |= jobs[j].scene->pt_eq;
+    transparent |= jobs[j].scene->pt.transparent != 0;
+  }
   if (e == hipSuccess && eq) {
     e = alloc_n(b, &P.so2, n);
     if (e == hipSuccess) e = alloc_n(b, &P.sd2, n);
     if (e == hipSuccess) e = alloc_n(b, &P.sc2, n);
+  }
+  if (e == hipSuccess && transparent) {
+    e = alloc_n(b, &P.mstk, n);
+    if (e == hipSuccess) e = alloc_n(b, &P.srng, n);
   }
   if (e == hipSuccess) e = alloc_n(b, &P.job, n);
   if (e == hipSuccess) e = alloc_n(b, &P.counts, 8);
@@ -233,7 +240,7 @@ int rrhip_batch_render(rrhip_batch* b, double* job_ms, double* total_ms) {
     for (const PtJob& t : pass) {
       total += t.prm.spp_pass * t.prm.npix;
       pix += t.prm.npix;
-      depth = std::max(depth, t.prm.max_depth);
+      depth = std::max(depth, t.prm.max_depth + (t.S.transparent ? kPtMaxSkips : 0));
     }
     e = pt_launch_batch_pass(b->d_jobs + k * nj, (uint32_t)pass.size(), b->P, total, pix, depth, st);
     if (e == hipSuccess) e = hipEventRecord(ev[k + 1], st);

@@ -8,6 +8,9 @@ namespace rrhip {
 
 // Traversal stack entries available per lane to the instance path.
 constexpr uint32_t kPtTraceStack = 48;
+// Hull crossings and cutout pass-throughs per path (pt/mod.rs:32): they do
+// not count as bounces, so transparent scenes run this many extra rounds.
+constexpr uint32_t kPtMaxSkips = 16;
 
 // Path slots [0, total) with slot = s * npix + column: camera rays of
 // samples prm.sample0 + s of the column's pixel (prm.live[column], or the

@@ -33,6 +33,32 @@ constexpr int kTraceStack = 48;
 constexpr uint32_t kFlagFromCamera = 1u << 8;
 constexpr uint32_t kFlagTrain = 1u << 9;   // radiance-cache training path
 constexpr uint32_t kFlagSuffix = 1u << 10; // training path past its termination vertex
+// Transparent scenes: medium-stack depth (bits 11-14, 0..8) and the count of
+// hull crossings / cutout pass-throughs (bits 15-19, 0..16).
+constexpr uint32_t kStackShift = 11, kSkipShift = 15, kStackSkipMask = 0x1FFu << 11;
+constexpr uint32_t kMaxSkips = 16;  // pt/mod.rs:32 MAX_PRESENCE_SKIPS
+
+// pt/mod.rs:33-69 MediumStack: up to 8 medium indices, 8 bits each,
+// innermost at depth - 1; the atmosphere is the implicit bottom.
+__device__ __forceinline__ int stack_top(uint2 st, uint32_t depth, int atmosphere) {
+  if (depth == 0) return atmosphere;
+  const uint32_t i = depth - 1;
+  return (int)(((i < 4 ? st.x : st.y) >> (8 * (i & 3))) & 0xFFu);
+}
+
+__device__ __forceinline__ void stack_push(uint2& st, uint32_t& depth, uint32_t m) {
+  if (depth >= 8) return;
+  const uint32_t sh = 8 * (depth & 3);
+  if (depth < 4)
+    st.x = (st.x & ~(0xFFu << sh)) | (m << sh);
+  else
+    st.y = (st.y & ~(0xFFu << sh)) | (m << sh);
+  depth++;
+}
+
+__device__ __forceinline__ void stack_pop(uint32_t& depth) {
+  if (depth > 0) depth--;
+}
 
 __device__ __forceinline__ uint32_t lane_id() {
   return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
@@ -77,6 +103,7 @@ __device__ __forceinline__ void init_path(const PtSceneView& S, const PathBuffer
   P.rng[slot] = store_rng(rng);
   P.flags[slot] = flags;
   P.q0[slot] = slot;
+  if (P.mstk) P.mstk[slot] = make_uint2(0u, 0u);
 }
 
 // sample0_dev (when set) supplies the pass's first sample index from device
@@ -164,7 +191,9 @@ __device__ __forceinline__ float light_pdf_solid_angle(const PtLight& L, V3 orig
 struct ShadowOut {
   V3 o, d;
   float tmax;
-  V3 c;  // contribution before visibility (f cos L weight / pdf)
+  V3 c;        // contribution before visibility (f cos L weight / pdf)
+  int medium;  // transparent scenes: the medium at the ray's origin (the walk applies its transmittance)
+  uint2 seed;  // transparent scenes: random stream of the blocker walk
 };
 
 // Evaluates the BSDF towards world direction wi: f * cos(wi) and the pdf
@@ -304,7 +333,8 @@ __device__ __forceinline__ bool sample_light(const PtSceneView& S, const PtLight
 // One path's shading step (the per-lane part of shade_kernel): sets
 // cont (continue in the next queue), shadow (+ so), shadow2 (+ se, an
 // equiangular connection in a medium), query (+ nq, qbeta).
-template <bool NRC>
+// TR: the scene is transparent (medium stack, hulls, cutouts, shadow walks).
+template <bool NRC, bool TR>
 __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffers& P, const PtParams& prm,
                                           const NrcPathBuffers& N, uint32_t slot, bool& cont, bool& shadow,
                                           bool& shadow2, ShadowOut& se, bool& query, ShadowOut& so, NrcQuery& nq,
@@ -320,15 +350,26 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
     const bool from_camera = (flags & kFlagFromCamera) != 0;
     uint32_t depth = flags & 0xFFu;
     float a0 = d4.w, asum = pv4.w;
-    // Atmosphere along this segment (pt/mod.rs:486-633): one light pick
+    // Transparent scenes: the medium stack and the skip count
+    uint2 mst = make_uint2(0u, 0u);
+    uint32_t sdepth = 0, skips = 0;
+    if (TR) {
+      mst = P.mstk[slot];
+      sdepth = (flags >> kStackShift) & 0xFu;
+      skips = (flags >> kSkipShift) & 0x1Fu;
+    }
+    const int cur = NRC ? -1 : stack_top(mst, sdepth, S.atmosphere);  // the segment's medium
+    constexpr bool walk = TR;  // shadow rays pick up media transmittance on their walk
+    bool passed = false;                   // hull crossing or cutout pass-through
+    // The medium along this segment (pt/mod.rs:486-633): one light pick
     // per segment; for a point light an equiangular single-scatter
     // connection (pt/mod.rs:501-551); distance sampling, and at a real
     // collision NEE with the phase function (MIS-paired with the
     // equiangular strategy for point lights) and an HG continuation
     // instead of the surface.
     bool in_medium = false, absorbed = false;
-    if (!NRC && S.atmosphere >= 0) {  // caches are never attached to scenes with media
-      const PtMedium& md = S.media[S.atmosphere];
+    if (!NRC && cur >= 0) {  // caches are never attached to scenes with media
+      const PtMedium& md = S.media[cur];
       const float t_limit = hn.w < 0.f ? 1e8f : hn.w;
       const float t_seg = fminf(t_limit, md.max_distance);
       uint32_t seg_li = 0;
@@ -351,8 +392,9 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
             const V3 wi = normalize(to_l);
             const float ph = hg_phase(md.g, dot(rd, wi));  // hg_phase(g, -wo.wi), wo = -rd
             const float dist = sqrtf(dist2) - 1e-3f;
-            const V3 vis = medium_transmittance(md, p, wi, dist, rng);
+            const V3 vis = walk ? v3(1.f, 1.f, 1.f) : medium_transmittance(md, p, wi, dist, rng);
             if (maxc(vis) > 0.f) {
+              se.medium = cur;
               const float pdf_d = distance_pdf_approx(md, t_eq, t_seg);
               const float w_eq = pdf_eq / (pdf_eq + pdf_d);
               se.o = p;
@@ -379,7 +421,9 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
             pdf = ph;
           };
           if (sample_light(S, S.lights[seg_li], p, v3(0.f, 0.f, 0.f), eval, seg_pick, rng, so)) {
-            so.c = beta * so.c * medium_transmittance(md, so.o, so.d, so.tmax, rng);
+            so.c = beta * so.c;
+            so.medium = cur;
+            if (!walk) so.c = so.c * medium_transmittance(md, so.o, so.d, so.tmax, rng);
             if (point_pick) {  // pt/mod.rs:583-598
               const float pdf_d = distance_pdf_approx(md, ev.t, t_seg);
               const float pdf_e = equiangular_pdf(ro, rd, t_seg, v3(S.lights[seg_li].p), ev.t);
@@ -422,6 +466,34 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
         Lr = Lr + beta * v3(S.background);
       }
     } else {
+      const uint32_t hm = P.hit_m[slot];
+      const PtMaterial m = S.materials[hm & 0x7FFFFFFFu];
+      const bool entering = (hm >> 31) != 0;
+      const V3 p = ro + rd * hn.w;
+      // Volume hull crossings toggle the medium (pt/mod.rs:662-683); a
+      // cutout passes the ray with probability 1 - presence
+      // (pt/mod.rs:702-710). Neither counts as a bounce.
+      if (TR) {
+        if (m.hull && skips < kMaxSkips) {
+          passed = true;
+          if (entering)
+            stack_push(mst, sdepth, (uint32_t)m.interior);
+          else
+            stack_pop(sdepth);
+        } else if (m.presence < 1.f && rnd(rng) >= m.presence && skips < kMaxSkips) {
+          passed = true;
+        }
+        if (passed) {
+          skips++;
+          const V3 org = p + rd * kRayOffset;
+          P.o[slot] = make_float4(org.x, org.y, org.z, prev_pdf);
+          cont = true;
+        }
+      }
+    }
+    if (passed) {
+      // the ray continues unchanged past the surface
+    } else if (hn.w >= 0.f && !in_medium && !absorbed) {
       const uint32_t hm = P.hit_m[slot];
       const PtMaterial m = S.materials[hm & 0x7FFFFFFFu];
       const bool entering = (hm >> 31) != 0;
@@ -505,9 +577,11 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
           const auto eval = [&](V3 wi, V3& f, float& pdf) { eval_world(m, fr, wo_l, eta, wi, f, pdf); };
           if (sample_light(S, S.lights[li], p, nrm, eval, pick, rng, so)) {
             so.c = beta * so.c;
-            // shadow rays through the atmosphere carry its transmittance
-            if (!NRC && S.atmosphere >= 0)
-              so.c = so.c * medium_transmittance(S.media[S.atmosphere], so.o, so.d, so.tmax, rng);
+            so.medium = cur;
+            // shadow rays through a medium carry its transmittance (or
+            // pick it up on their blocker walk)
+            if (!NRC && cur >= 0 && !walk)
+              so.c = so.c * medium_transmittance(S.media[cur], so.o, so.d, so.tmax, rng);
             shadow = maxc(so.c) > 0.f;
           }
         }
@@ -519,6 +593,12 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
         beta = beta * bs.f * (fabsf(bs.wi.z) / bs.pdf);
         const V3 org = p + nrm * (bs.transmitted ? -kRayOffset : kRayOffset);
         const V3 nd = normalize(wi);
+        if (TR && bs.transmitted && m.interior >= 0) {  // pt/mod.rs:969-975
+          if (entering)
+            stack_push(mst, sdepth, (uint32_t)m.interior);
+          else
+            stack_pop(sdepth);
+        }
         depth++;
         bool alive = true;
         if (depth >= prm.rr_start) {
@@ -536,6 +616,14 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
         flags = depth | (flags & (kFlagTrain | kFlagSuffix));  // from_camera cleared
       }
     }
+    if (TR && (shadow || shadow2)) {
+      so.seed = make_uint2(rng.next_u32(), rng.next_u32());
+      se.seed = so.seed;
+    }
+    if (TR) {
+      P.mstk[slot] = mst;
+      flags = (flags & ~kStackSkipMask) | (sdepth << kStackShift) | (skips << kSkipShift);
+    }
     P.beta[slot] = make_float4(beta.x, beta.y, beta.z, 0.f);
     P.L[slot] = make_float4(Lr.x, Lr.y, Lr.z, 0.f);
     P.rng[slot] = store_rng(rng);
@@ -545,6 +633,7 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
 // Appends a lane's shadow record (all lanes of the wave call it): the
 // first ray is `a` (or `b` alone); with both, `b` goes to the second
 // arrays and sc.w = 1 marks the record.
+template <bool TR>
 __device__ __forceinline__ void push_shadow(const PathBuffers& P, bool act, uint32_t slot, bool s1, const ShadowOut& a,
                                             bool s2, const ShadowOut& b) {
   const bool any = act && (s1 || s2);
@@ -552,18 +641,21 @@ __device__ __forceinline__ void push_shadow(const PathBuffers& P, bool act, uint
   if (!any) return;
   const ShadowOut& f = s1 ? a : b;
   const bool both = s1 && s2;
+  // transparent scenes: d.w holds the medium at the origin, srng the walk's seed
+  constexpr bool walk = TR;
   P.sq[idx] = slot;
   P.so[idx] = make_float4(f.o.x, f.o.y, f.o.z, f.tmax);
-  P.sd[idx] = make_float4(f.d.x, f.d.y, f.d.z, 0.f);
+  P.sd[idx] = make_float4(f.d.x, f.d.y, f.d.z, walk ? (float)f.medium : 0.f);
   P.sc[idx] = make_float4(f.c.x, f.c.y, f.c.z, both ? 1.f : 0.f);
+  if (walk) P.srng[idx] = f.seed;
   if (both) {
     P.so2[idx] = make_float4(b.o.x, b.o.y, b.o.z, b.tmax);
-    P.sd2[idx] = make_float4(b.d.x, b.d.y, b.d.z, 0.f);
+    P.sd2[idx] = make_float4(b.d.x, b.d.y, b.d.z, walk ? (float)b.medium : 0.f);
     P.sc2[idx] = make_float4(b.c.x, b.c.y, b.c.z, 0.f);
   }
 }
 
-template <bool NRC>
+template <bool NRC, bool TR>
 __global__ __launch_bounds__(kBlock) void shade_kernel(PtSceneView S, PathBuffers P, PtParams prm,
                                                       const uint32_t* __restrict__ q, uint32_t* __restrict__ qnext,
                                                       NrcPathBuffers N) {
@@ -580,9 +672,9 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtSceneView S, PathBuffer
     uint32_t slot = 0;
     if (act) {
       slot = q[i];
-      shade_one<NRC>(S, P, prm, N, slot, cont, shadow, shadow2, se, query, so, nq, qbeta);
+      shade_one<NRC, TR>(S, P, prm, N, slot, cont, shadow, shadow2, se, query, so, nq, qbeta);
     }
-    push_shadow(P, act, slot, shadow, so, shadow2, se);
+    push_shadow<TR>(P, act, slot, shadow, so, shadow2, se);
     const uint32_t nidx = wave_append(&P.counts[1], act && cont);
     if (act && cont) qnext[nidx] = slot;
     if constexpr (NRC) {
@@ -596,17 +688,76 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtSceneView S, PathBuffer
   }
 }
 
+// Fractional visibility along a shadow ray in a transparent scene
+// (pt/mod.rs:1238-1306): closest hits in turn; each segment multiplies the
+// transmittance of the medium on top of the stack (the origin's medium
+// over the atmosphere); hulls push / pop their media, cutouts let 1 -
+// presence through, anything else blocks; at most 16 blockers.
+__device__ __forceinline__ V3 walk_visibility(const PtSceneView& S, V3 o, V3 d, float remaining, int medium,
+                                              Pcg32& rng, uint32_t* stk) {
+  uint2 st = make_uint2(0u, 0u);
+  uint32_t depth = 0;
+  if (medium >= 0 && medium != S.atmosphere) stack_push(st, depth, (uint32_t)medium);
+  V3 vis = v3(1.f, 1.f, 1.f);
+  for (uint32_t i = 0; i < kMaxSkips; i++) {
+    Hit h;
+    const bool hit = scene_closest<kTraceBlock>(S, o, d, stk, h);
+    const float seg = fminf(hit ? h.t : remaining, remaining);
+    const int cur = stack_top(st, depth, S.atmosphere);
+    if (cur >= 0) {
+      vis = vis * medium_transmittance(S.media[cur], o, d, seg, rng);
+      if (maxc(vis) < 1e-4f) return v3(0.f, 0.f, 0.f);
+    }
+    if (!hit || h.t >= remaining) return vis;
+    const PtMaterial& m = S.materials[h.material];
+    if (m.hull) {
+      if (h.front)
+        stack_push(st, depth, (uint32_t)m.interior);
+      else
+        stack_pop(depth);
+    } else {
+      if (m.presence >= 1.f) return v3(0.f, 0.f, 0.f);
+      vis = vis * (1.f - m.presence);
+      if (maxc(vis) < 1e-4f) return v3(0.f, 0.f, 0.f);
+    }
+    o = o + d * (h.t + kRayOffset);
+    remaining -= h.t + kRayOffset;
+  }
+  return vis;
+}
+
 // Shadow record i: adds each unoccluded ray's contribution to its path
-// (the first ray, then the second when the record has one).
-__device__ __forceinline__ void shadow_one(const PtSceneView& S, const PathBuffers& P, uint32_t i, uint32_t* stk) {
-  const float4 o = P.so[i], d = P.sd[i], c = P.sc[i];
-  const bool v1 = !scene_occluded<kTraceBlock>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, stk);
-  bool v2 = false;
+// (the first ray, then the second when the record has one); with WALK the
+// rays walk their blockers (transparent scenes), else one any-hit test.
+template <bool WALK>
+__device__ __forceinline__ void shadow_one_t(const PtSceneView& S, const PathBuffers& P, uint32_t i, uint32_t* stk) {
+  const float4 o = P.so[i], d = P.sd[i];
+  float4 c = P.sc[i];
+  const bool two = c.w != 0.f;
   float4 c2 = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (c.w != 0.f) {
-    const float4 o2 = P.so2[i], d2 = P.sd2[i];
-    v2 = !scene_occluded<kTraceBlock>(S, v3(o2.x, o2.y, o2.z), v3(d2.x, d2.y, d2.z), o2.w, stk);
-    c2 = P.sc2[i];
+  bool v1, v2 = false;
+  if constexpr (WALK) {
+    const uint2 sd = P.srng[i];
+    const uint64_t seed = ((uint64_t)sd.y << 32) | sd.x;
+    Pcg32 r1 = Pcg32::make(seed, 1);
+    const V3 t1 = walk_visibility(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, (int)d.w, r1, stk);
+    c = make_float4(c.x * t1.x, c.y * t1.y, c.z * t1.z, c.w);
+    v1 = maxc(t1) > 0.f;
+    if (two) {
+      const float4 o2 = P.so2[i], d2 = P.sd2[i];
+      Pcg32 r2 = Pcg32::make(seed, 2);
+      const V3 t2 = walk_visibility(S, v3(o2.x, o2.y, o2.z), v3(d2.x, d2.y, d2.z), o2.w, (int)d2.w, r2, stk);
+      c2 = P.sc2[i];
+      c2 = make_float4(c2.x * t2.x, c2.y * t2.y, c2.z * t2.z, 0.f);
+      v2 = maxc(t2) > 0.f;
+    }
+  } else {
+    v1 = !scene_occluded<kTraceBlock>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, stk);
+    if (two) {
+      const float4 o2 = P.so2[i], d2 = P.sd2[i];
+      v2 = !scene_occluded<kTraceBlock>(S, v3(o2.x, o2.y, o2.z), v3(d2.x, d2.y, d2.z), o2.w, stk);
+      c2 = P.sc2[i];
+    }
   }
   if (!v1 && !v2) return;
   const uint32_t slot = P.sq[i];
@@ -624,13 +775,20 @@ __device__ __forceinline__ void shadow_one(const PtSceneView& S, const PathBuffe
   P.L[slot] = l;
 }
 
-template <int STACK>
+__device__ __forceinline__ void shadow_one(const PtSceneView& S, const PathBuffers& P, uint32_t i, uint32_t* stk) {
+  if (S.transparent)
+    shadow_one_t<true>(S, P, i, stk);
+  else
+    shadow_one_t<false>(S, P, i, stk);
+}
+
+template <int STACK, bool WALK>
 __global__ __launch_bounds__(kTraceBlock) void shadow_kernel(PtSceneView S, PathBuffers P) {
   __shared__ uint32_t s_stack[STACK * kTraceBlock];
   uint32_t* stk = s_stack + threadIdx.x;
   const uint32_t n = P.counts[2];
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) shadow_one(S, P, i, stk);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) shadow_one_t<WALK>(S, P, i, stk);
 }
 
 __global__ void rotate_kernel(PathBuffers P) {
@@ -741,9 +899,15 @@ __global__ __launch_bounds__(kBlock) void shade_batch_kernel(const PtJob* __rest
     if (act) {
       slot = q[i];
       const PtJob& J = jobs[P.job[slot]];
-      shade_one<false>(J.S, P, J.prm, N, slot, cont, shadow, shadow2, se, query, so, nq, qbeta);
+      if (J.S.transparent)
+        shade_one<false, true>(J.S, P, J.prm, N, slot, cont, shadow, shadow2, se, query, so, nq, qbeta);
+      else
+        shade_one<false, false>(J.S, P, J.prm, N, slot, cont, shadow, shadow2, se, query, so, nq, qbeta);
     }
-    push_shadow(P, act, slot, shadow, so, shadow2, se);
+    if (P.srng)
+      push_shadow<true>(P, act, slot, shadow, so, shadow2, se);
+    else
+      push_shadow<false>(P, act, slot, shadow, so, shadow2, se);
     const uint32_t nidx = wave_append(&P.counts[1], act && cont);
     if (act && cont) qnext[nidx] = slot;
   }
@@ -871,13 +1035,21 @@ hipError_t pt_launch_bounce(const PtSceneView& S, const PathBuffers& P, const Pt
     qshade = P.qs;
   }
   if (N.q)
-    hipLaunchKernelGGL(shade_kernel<true>, dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
+    hipLaunchKernelGGL((shade_kernel<true, false>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
+  else if (S.transparent)
+    hipLaunchKernelGGL((shade_kernel<false, true>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
   else
-    hipLaunchKernelGGL(shade_kernel<false>, dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
-  if (small)
-    hipLaunchKernelGGL(shadow_kernel<32>, dim3(gt), dim3(kTraceBlock), 0, st, S, P);
-  else
-    hipLaunchKernelGGL(shadow_kernel<kTraceStack>, dim3(gt), dim3(kTraceBlock), 0, st, S, P);
+    hipLaunchKernelGGL((shade_kernel<false, false>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
+  if (S.transparent) {
+    if (small)
+      hipLaunchKernelGGL((shadow_kernel<32, true>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
+    else
+      hipLaunchKernelGGL((shadow_kernel<kTraceStack, true>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
+  } else if (small) {
+    hipLaunchKernelGGL((shadow_kernel<32, false>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
+  } else {
+    hipLaunchKernelGGL((shadow_kernel<kTraceStack, false>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
+  }
   hipLaunchKernelGGL(rotate_kernel, dim3(1), dim3(1), 0, st, P);
   return hipGetLastError();
 }

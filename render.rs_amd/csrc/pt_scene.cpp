@@ -93,6 +93,9 @@ PtMaterial make_material(const rrhip_material_desc& m) {
     if (g.w[i] > 0.f) g.sort_key += 1u << i;
   g.nrc_rough = 1.f;
   if (ws > 0 || wg > 0) g.nrc_rough = (float)(wg > ws ? m.glass_roughness : m.specular_roughness);
+  g.interior = (int32_t)m.interior - 1;
+  g.presence = (float)std::min(std::max(1.0 - m.cutout, 0.0), 1.0);
+  g.hull = g.interior >= 0 && !g.has_lobes && std::max(g.emission[0], std::max(g.emission[1], g.emission[2])) <= 0.f;
   return g;
 }
 
@@ -310,6 +313,8 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
   for (uint32_t i = 0; i < d.num_materials; i++) {
     const rrhip_material_desc& m = d.materials[i];
     if (m.area_light >= (int32_t)d.num_lights) return *err = "material area_light out of range", false;
+    if (m.interior > d.num_media) return *err = "material interior medium out of range", false;
+    if (!(m.cutout >= 0.0 && m.cutout <= 1.0)) return *err = "material cutout must be in [0, 1]", false;
     out->materials.push_back(make_material(m));
   }
   // Lights (light.rs:59-151)
@@ -462,6 +467,9 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
     out->media_grids.push_back(std::move(grid));
   }
   out->atmosphere = d.num_media > 0 ? d.atmosphere : -1;
+  for (const PtMaterial& m : out->materials) out->transparent |= (m.interior >= 0 || m.presence < 1.f) ? 1u : 0u;
+  if (out->transparent && d.num_media > 255)
+    return *err = "at most 255 media with interior media or cutouts (8-bit medium stack)", false;
   for (int a = 0; a < 3; a++) {
     out->lo[a] = INFINITY;
     out->hi[a] = -INFINITY;

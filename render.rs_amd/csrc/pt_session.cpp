@@ -125,6 +125,10 @@ hipError_t alloc_paths(rrhip_session* s, uint64_t n) {
     if (e == hipSuccess) e = alloc_in(m, &P.sd2, n);
     if (e == hipSuccess) e = alloc_in(m, &P.sc2, n);
   }
+  if (e == hipSuccess && s->scene->pt.transparent) {
+    e = alloc_in(m, &P.mstk, n);
+    if (e == hipSuccess) e = alloc_in(m, &P.srng, n);
+  }
   if (e == hipSuccess) e = alloc_in(m, &P.counts, 4);
   if (e == hipSuccess) s->path_cap = n;
   return e;
@@ -210,7 +214,8 @@ hipError_t enqueue_pass(rrhip_session* s, uint32_t n, uint32_t ncol, const uint3
   PathBuffers P = s->P;
   if (!(s->launch_mode & 2)) P.qs = nullptr;
   hipError_t e = pt_launch_raygen(sc->pt, P, prm, total, N, sample0_dev, st);
-  for (uint32_t b = 0; b < prm.max_depth && e == hipSuccess; b++) {
+  const uint32_t rounds = prm.max_depth + (sc->pt.transparent ? kPtMaxSkips : 0);
+  for (uint32_t b = 0; b < rounds && e == hipSuccess; b++) {
     const uint32_t* q = (b & 1) ? P.q1 : P.q0;
     uint32_t* qn = (b & 1) ? P.q0 : P.q1;
     e = pt_launch_bounce(sc->pt, P, prm, q, qn, paths, N, st);
@@ -509,8 +514,9 @@ extern "C" int rrhip_session_set_nrc(rrhip_session* s, rrhip_nrc* nrc) {
   }
   rrhip_scene* sc = s->scene;
   if (nrc->ctx != sc->ctx) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_set_nrc: cache and session differ in context");
-  if (sc->pt.atmosphere >= 0)
-    return fail(RRHIP_ERR_UNSUPPORTED, "rrhip_session_set_nrc: the radiance cache does not model participating media");
+  if (sc->pt.atmosphere >= 0 || sc->pt.transparent)
+    return fail(RRHIP_ERR_UNSUPPORTED,
+                "rrhip_session_set_nrc: the radiance cache does not model participating media or cutouts");
   const uint64_t ntrain = nrc->cfg.train_paths;
   const uint64_t cap = s->slots + ntrain;
   if (cap >= (1ull << 31)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_set_nrc: path slots exceed 2^31");

@@ -34,7 +34,10 @@ struct alignas(16) PtMaterial {
   float nrc_albedo[3];  // radiance-cache input: summed lobe albedo
   float nrc_rough;      // radiance-cache input: roughness of the strongest glossy lobe
   uint32_t sort_key;    // shading class for sorted queues: 1 + bitmask of the active lobes
-  float pad;
+  int32_t interior;     // medium inside closed surfaces of this material (-1: none)
+  float presence;       // probability that a ray stops at the surface (pbr.rs:39)
+  uint32_t hull;        // interior medium, no lobes, no emission: an invisible volume boundary
+  uint32_t pad[2];
 };
 
 enum : uint32_t { kLightPoint = 0, kLightDistant = 1, kLightRect = 2, kLightSphere = 3, kLightDisk = 4, kLightDome = 5 };
@@ -139,6 +142,8 @@ struct PtSceneView {
   const PtMedium* media;
   int32_t atmosphere;           // medium every ray travels in (-1: none)
   uint32_t stack_need;          // traversal stack entries a lane can need
+  uint32_t transparent;         // some material has an interior medium or presence < 1: paths keep
+                                // a medium stack, shadow rays walk their blockers
   uint32_t num_insts;
   uint32_t num_tris, num_spheres, num_lights, num_lnodes, num_inf;
   int32_t dome;
@@ -167,6 +172,8 @@ struct PathBuffers {
   float4* so2;      // second ray of a record (equiangular connections in media): origin, tmax;
   float4* sd2;      //   direction; contribution (all three null when no scene needs them)
   float4* sc2;
+  uint2* mstk;      // transparent scenes: each path's medium stack (8 x 8-bit medium indices)
+  uint2* srng;      // transparent scenes: 64-bit seed of each shadow record's blocker walk
   uint32_t* counts; // [0] live paths, [1] next live, [2] shadow rays, [3] unused
   unsigned long long* stats;  // [0] extension rays, [1] shadow rays, [2] cache queries, [3] training samples
   uint32_t* job;    // batches: job of each slot (null in sessions)

@@ -91,7 +91,7 @@ class MaterialDesc(ctypes.Structure):
                 ("glass_gain", ctypes.c_double), ("glass_ior", ctypes.c_double),
                 ("glass_roughness", ctypes.c_double), ("refraction_color", D3),
                 ("glow", D3), ("emission", D3), ("area_light", ctypes.c_int32),
-                ("reserved", ctypes.c_uint32)]
+                ("interior", ctypes.c_uint32), ("cutout", ctypes.c_double)]
 
 
 class LightDesc(ctypes.Structure):

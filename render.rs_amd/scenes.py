@@ -22,8 +22,10 @@ def material(diffuse_gain=1.0, diffuse_color=(0.5, 0.5, 0.5), diffuse_roughness=
              specular_face_color=(0, 0, 0), specular_edge_color=(1, 1, 1), specular_roughness=0.2,
              specular_ior=1.5, clearcoat_gain=0.0, clearcoat_roughness=0.1, fuzz_gain=0.0,
              fuzz_color=(1, 1, 1), glass_gain=0.0, glass_ior=1.5, glass_roughness=0.05,
-             refraction_color=(1, 1, 1), glow=(0, 0, 0), emission=(0, 0, 0), area_light=-1):
-    """PbrParams defaults of pbr.rs:50-75."""
+             refraction_color=(1, 1, 1), glow=(0, 0, 0), emission=(0, 0, 0), area_light=-1, presence=1.0,
+             interior=None):
+    """PbrParams defaults of pbr.rs:50-75; presence (pbr.rs:39) and the
+    interior medium index (material.rs:37, None: no medium)."""
     m = rrhip.MaterialDesc()
     m.diffuse_gain = diffuse_gain
     m.diffuse_color = _d3(diffuse_color)
@@ -43,7 +45,15 @@ def material(diffuse_gain=1.0, diffuse_color=(0.5, 0.5, 0.5), diffuse_roughness=
     m.glow = _d3(glow)
     m.emission = _d3(emission)
     m.area_light = area_light
+    m.cutout = 1.0 - presence
+    m.interior = 0 if interior is None else interior + 1
     return m
+
+
+def hull(interior):
+    """An invisible volume boundary (no lobes, no emission) around the
+    medium `interior`."""
+    return material(diffuse_gain=0.0, specular_ior=1.0, interior=interior)
 
 
 def matte(color, **kw):

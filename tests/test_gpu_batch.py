@@ -33,6 +33,10 @@ def test_batch_jobs_equal_their_own_sessions(ctx):
             s.spheres = []
             for c, r in spheres:
                 s.add_sphere(c, r, 0)
+            if j == 4:  # a hull of ink and a cutout sphere: medium stacks and blocker walks
+                ink = s.add_medium((1.5, 0.6, 0.2), (0.2, 0.2, 0.2), 0.0)
+                s.add_sphere((0.4, -0.5, 0.0), 0.35, s.add_material(scenes.hull(ink)))
+                s.add_sphere((-0.3, 0.2, 0.3), 0.25, s.add_material(scenes.matte((0.8, 0.5, 0.3), presence=0.5)))
             depth = 8
         jobs.append((s, spp, depth))
     pss = [rrhip.PtScene(ctx, s.desc()) for s, _, _ in jobs]

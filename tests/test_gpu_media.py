@@ -160,3 +160,107 @@ def test_media_rejections(ctx):
         se.set_nrc(rrhip.Nrc(ctx, rrhip.NrcConfig.make(log2_table=12, train_paths=64)))
     se.close()
     ps.close()
+    # cutouts without media: no cache either
+    s3 = scenes.box_room(16, 16)
+    s3.add_sphere((0, 0, 0), 0.2, s3.add_material(scenes.matte((0.5, 0.5, 0.5), presence=0.5)))
+    ps3 = rrhip.PtScene(ctx, s3.desc())
+    se3 = ps3.session(rrhip.RenderParams.make(max_depth=8))
+    cache = rrhip.Nrc(ctx, rrhip.NrcConfig.make(log2_table=12, train_paths=64))
+    with pytest.raises(rrhip.RrhipError) as e:
+        se3.set_nrc(cache)
+    assert e.value.status == 5
+    cache.close()
+    se3.close()
+    ps3.close()
+    # an interior medium index past the media, a cutout outside [0, 1]
+    s4 = scenes.box_room(16, 16)
+    s4.add_sphere((0, 0, 0), 0.2, s4.add_material(scenes.hull(3)))
+    with pytest.raises(rrhip.RrhipError):
+        rrhip.PtScene(ctx, s4.desc())
+    s5 = scenes.box_room(16, 16)
+    s5.add_sphere((0, 0, 0), 0.2, s5.add_material(scenes.matte((0.5, 0.5, 0.5), presence=1.5)))
+    with pytest.raises(rrhip.RrhipError):
+        rrhip.PtScene(ctx, s5.desc())
+
+
+def test_hull_sphere_absorbs_by_its_chord(ctx):
+    """Hull crossings push and pop the medium (pt/mod.rs:662-683): an
+    absorbing interior transmits exp(-sigma_a chord) of the dome."""
+    from test_oracle_media import center_chords, hull_sphere_scene
+    say("hull sphere with an absorbing interior, 24x24, 1024 spp vs exp(-sigma chord)")
+    W = H = 24
+    s = hull_sphere_scene(W, H, (0.8, 0.5, 0.2), (0, 0, 0))
+    acc = render(ctx, s, rrhip.RenderParams.make(max_depth=8, firefly_clamp=0.0), 0, 1024)
+    m, var, n = mean_var(acc)
+    se = np.sqrt(var / n)
+    chord, clean = center_chords(W, H)
+    want = np.exp(-np.outer(chord, [0.8, 0.5, 0.2]))
+    inside = clean & (chord > 0)
+    z = np.abs(m - want)[inside] / np.maximum(se[inside], 1e-12)
+    out = m[clean & (chord == 0)]
+    say(f"inside: max z {z.max():.2f}, mean z {z.mean():.2f}; outside max |m - 1| {np.abs(out - 1).max():.2e}")
+    assert np.abs(out - 1).max() < 1e-6
+    assert z.max() < 5.5 and z.mean() < 1.2
+
+
+def test_hull_white_furnace(ctx):
+    from test_oracle_media import hull_sphere_scene
+    say("conservative cloud inside a hull in a white furnace, 32x32, 512 spp")
+    s = hull_sphere_scene(32, 32, (0, 0, 0), (2.0, 2.0, 2.0), 0.3)
+    acc = render(ctx, s, rrhip.RenderParams.make(max_depth=64, firefly_clamp=0.0), 0, 512)
+    m, var, n = mean_var(acc)
+    mean = m.mean()
+    se = np.sqrt(var.mean() / n.mean() / m.size)
+    say(f"furnace mean {mean:.5f} (standard error {se:.5f})")
+    assert abs(mean - 1.0) < max(4 * se, 2e-3)
+
+
+def test_cutouts_attenuate_shadows_and_pass_camera_rays(ctx):
+    from test_oracle_media import cutout_scene
+    say("cutout quad: shadow attenuation 1 - presence (exact), camera pass-through 1 - presence")
+    W = H = 16
+    acc = render(ctx, cutout_scene(W, H, 0.6), rrhip.RenderParams.make(max_depth=4, firefly_clamp=0.0), 0, 16)
+    m = (acc[:, :3] / acc[:, 6:7])[:, 0].reshape(H, W)
+    lit, shaded = m[:, W // 2 + 2:], m[:, :W // 2 - 2]
+    say(f"lit {lit.mean():.6f} (want {0.5 / np.pi:.6f}), shaded {shaded.mean():.6f} (want {0.2 / np.pi:.6f})")
+    assert np.allclose(lit, 0.5 / np.pi, rtol=2e-5)
+    assert np.allclose(shaded, 0.5 / np.pi * 0.4, rtol=2e-5)
+    s2 = scenes.Scene(64, 64, 20.0, eye=(0, 0, -2), look_at=(0, 0, 0))
+    s2.add_light(scenes.light(rrhip.LIGHT_DOME, intensity=1.0))
+    q = s2.add_material(scenes.material(diffuse_gain=0.0, specular_ior=1.0, presence=0.3))
+    s2.add_polygon([(-3, -3, 0), (3, -3, 0), (3, 3, 0), (-3, 3, 0)], q)
+    acc2 = render(ctx, s2, rrhip.RenderParams.make(max_depth=4, firefly_clamp=0.0), 0, 256)
+    mean = acc2[:, 0].sum() / acc2[:, 6].sum()
+    assert abs(mean - 0.7) < 4 * np.sqrt(0.21 / acc2[:, 6].sum()), mean
+
+
+def transparent_room(w, h):
+    """The box room with a glass sphere holding a scattering cloud (pushed
+    and popped on refraction), a hull sphere around an absorbing medium,
+    and a cutout sphere of presence 0.5."""
+    s = scenes.box_room(w, h)
+    s.spheres = []
+    cloud = s.add_medium((0.05, 0.05, 0.05), (1.5, 1.2, 0.9), 0.3)
+    ink = s.add_medium((1.5, 0.6, 0.2), (0, 0, 0), 0.0)
+    glass = s.add_material(scenes.material(diffuse_gain=0.0, specular_ior=1.0, glass_gain=1.0, glass_ior=1.3,
+                                           glass_roughness=0.1, interior=cloud))
+    s.add_sphere((-0.4, -0.6, 0.2), 0.35, glass)
+    s.add_sphere((0.45, -0.55, -0.1), 0.4, s.add_material(scenes.hull(ink)))
+    s.add_sphere((0.0, 0.3, 0.4), 0.25, s.add_material(scenes.matte((0.8, 0.6, 0.2), presence=0.5)))
+    return s
+
+
+def test_transparent_room_matches_oracle(ctx):
+    say("room with a cloud-filled glass sphere, a hull sphere of ink and a cutout sphere, 48x48, 256 spp vs oracle")
+    s = transparent_room(48, 48)
+    p = rrhip.RenderParams.make(max_depth=16)
+    acc_g = render(ctx, s, p, 0, 256)
+    acc_c, _ = orc.render(s.desc(), p, 0, 256)
+    z = sigma_test(acc_g, acc_c)
+    frac = float((z > 3).mean())
+    zi = image_mean_z(acc_g, acc_c)
+    mg, _, _ = mean_var(acc_g)
+    mc, _, _ = mean_var(acc_c)
+    say(f"outside 3 sigma {frac:.4%}, image-mean z {np.round(zi, 2)}, means gpu {mg.mean(0).round(5)} "
+        f"cpu {mc.mean(0).round(5)}")
+    assert frac <= 0.01 and (zi < 3).all()

@@ -213,3 +213,106 @@ def test_single_scatter_towards_a_point_light_matches_quadrature():
     se = np.sqrt(np.maximum(acc[0, 3:6] / n - mean ** 2, 0) / n)
     want = single_scatter_quadrature()
     assert np.all(np.abs(mean - want) < 4 * se + 1e-4 * want), (mean, want, se)
+
+
+def hull_sphere_scene(w, h, sigma_a, sigma_s, g=0.0):
+    """A unit sphere hull (no lobes, medium 0 inside, no atmosphere) seen
+    from z = -4 against a unit dome."""
+    import scenes
+    s = scenes.Scene(w, h, 30.0, eye=(0, 0, -4), look_at=(0, 0, 0))
+    s.add_light(scenes.light(rrhip.LIGHT_DOME, intensity=1.0))
+    med = s.add_medium(sigma_a, sigma_s, g)
+    s.add_sphere((0, 0, 0), 1.0, s.add_material(scenes.hull(med)))
+    return s
+
+
+def center_chords(w, h, fov=30.0, eye=(0, 0, -4)):
+    """Chord through the unit sphere of each pixel-centre ray, and the
+    pixels whose whole footprint lies inside or outside the silhouette."""
+    th = np.tan(np.radians(fov) / 2)
+    ys, xs = np.mgrid[0:h, 0:w]
+    u = ((xs + 0.5) / w * 2 - 1) * th * w / h
+    v = (1 - (ys + 0.5) / h * 2) * th
+    d = np.stack([u, v, np.ones_like(u)], -1)
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    o = np.array(eye, float)
+    b = d @ o
+    c = o @ o - 1.0
+    disc = b * b - c
+    chord = np.where(disc > 0, 2 * np.sqrt(np.maximum(disc, 0)), 0.0)
+    # distance of the ray from the centre against a margin of a few pixel
+    # footprints at the sphere
+    dist_axis = np.sqrt(np.maximum(o @ o - b * b, 0))
+    margin = 4.0 * (2 * th / h) * 2.0
+    clean = np.abs(dist_axis - 1.0) > margin
+    return chord.reshape(-1), clean.reshape(-1)
+
+
+def test_hull_sphere_absorbs_by_its_chord():
+    """Entering the hull pushes its medium and leaving pops it
+    (pt/mod.rs:662-683): a purely absorbing interior transmits
+    exp(-sigma_a chord) of the dome."""
+    W = H = 24
+    s = hull_sphere_scene(W, H, (0.8, 0.5, 0.2), (0, 0, 0))
+    acc, _ = orc.render(s.desc(), rrhip.RenderParams.make(max_depth=8, firefly_clamp=0.0), 0, 1024)
+    n = acc[:, 6:7]
+    m = acc[:, :3] / n
+    se = np.sqrt(np.maximum(acc[:, 3:6] / n - m * m, 0) / n)
+    chord, clean = center_chords(W, H)
+    want = np.exp(-np.outer(chord, [0.8, 0.5, 0.2]))
+    inside = clean & (chord > 0)
+    assert inside.sum() > 150 and (clean & (chord == 0)).sum() >= 30
+    # outside the silhouette: the dome exactly; inside: exp(-sigma_a chord)
+    assert np.array_equal(m[clean & (chord == 0)], np.ones_like(m[clean & (chord == 0)]))
+    z = np.abs(m - want)[inside] / np.maximum(se[inside], 1e-12)
+    assert z.max() < 5.5 and z.mean() < 1.2, (z.max(), z.mean())
+
+
+def test_hull_white_furnace():
+    """A conservative cloud inside a hull, lit by a unit dome: every pixel
+    is 1 (scattering inside, NEE walking out through the hull)."""
+    s = hull_sphere_scene(16, 16, (0, 0, 0), (2.0, 2.0, 2.0), 0.3)
+    acc, _ = orc.render(s.desc(), rrhip.RenderParams.make(max_depth=64, firefly_clamp=0.0), 0, 256)
+    n = acc[:, 6]
+    mean = acc[:, :3].sum() / (3 * n.sum())
+    var = np.maximum(acc[:, 3:6] / acc[:, 6:7] - (acc[:, :3] / acc[:, 6:7]) ** 2, 0)
+    se = np.sqrt(var.mean() / n.mean() / acc.shape[0])
+    assert abs(mean - 1.0) < 4 * se + 1e-3, (mean, se)
+
+
+def cutout_scene(w, h, presence):
+    """A camera at y = 0.5 looking straight down at a matte 0.5 ground; a
+    black quad of the given presence at y = 1 over x < 0 (above the camera,
+    never seen) between the ground and a distant light shining straight
+    down."""
+    import scenes
+    s = scenes.Scene(w, h, 60.0, eye=(0, 0.5, 0), look_at=(0, 0, 0), up=(0, 0, 1))
+    s.add_light(scenes.light(rrhip.LIGHT_DISTANT, intensity=1.0, p=(0, -1, 0)))
+    ground = s.add_material(scenes.matte((0.5, 0.5, 0.5)))
+    s.add_polygon([(-5, 0, -5), (-5, 0, 5), (5, 0, 5), (5, 0, -5)], ground)
+    cut = s.add_material(scenes.material(diffuse_gain=0.0, specular_ior=1.0, presence=presence))
+    s.add_polygon([(-5, 1, -5), (0, 1, -5), (0, 1, 5), (-5, 1, 5)], cut)
+    return s
+
+
+def test_cutout_attenuates_shadow_rays_and_passes_camera_rays():
+    """pt/mod.rs:1292-1303: a blocker of presence p lets 1 - p of the light
+    through; pt/mod.rs:702-710: a camera ray passes a cutout with
+    probability 1 - p."""
+    W = H = 16
+    s = cutout_scene(W, H, 0.6)
+    acc, _ = orc.render(s.desc(), rrhip.RenderParams.make(max_depth=4, firefly_clamp=0.0), 0, 16)
+    m = (acc[:, :3] / acc[:, 6:7])[:, 0].reshape(H, W)
+    # image x runs along world x: left half (x < 0) is under the cutout
+    lit, shaded = m[:, W // 2 + 2:], m[:, :W // 2 - 2]
+    assert np.allclose(lit, 0.5 / np.pi, rtol=1e-9)
+    assert np.allclose(shaded, 0.5 / np.pi * 0.4, rtol=1e-9)
+    # camera rays through a cutout: a black quad of presence 0.3 in front of a dome
+    import scenes
+    s2 = scenes.Scene(8, 8, 20.0, eye=(0, 0, -2), look_at=(0, 0, 0))
+    s2.add_light(scenes.light(rrhip.LIGHT_DOME, intensity=1.0))
+    q = s2.add_material(scenes.material(diffuse_gain=0.0, specular_ior=1.0, presence=0.3))
+    s2.add_polygon([(-3, -3, 0), (3, -3, 0), (3, 3, 0), (-3, 3, 0)], q)
+    acc2, _ = orc.render(s2.desc(), rrhip.RenderParams.make(max_depth=4, firefly_clamp=0.0), 0, 4096)
+    mean = acc2[:, 0].sum() / acc2[:, 6].sum()
+    assert abs(mean - 0.7) < 4 * np.sqrt(0.21 / acc2[:, 6].sum()), mean
