@@ -164,19 +164,23 @@ typedef struct rrhip_light_desc {
   double radius;
 } rrhip_light_desc;
 
-/* Pinhole camera in RenderMan's left-handed frame: forward =
- * normalize(look_at - eye), right = normalize(up x forward), up' =
- * forward x right — render.rs's Camera::new frame (src/scene/camera.rs:
- * 96-119: at the RIB default eye = 0, look +z, up +y it gives right = +x),
- * placed at `eye`. Pixel filter per src/scene/camera.rs:14-66
- * (0 box, 1 triangle, 2 gaussian). */
+/* Camera in RenderMan's left-handed frame: forward = normalize(look_at -
+ * eye), right = normalize(up x forward), up' = forward x right —
+ * render.rs's Camera::new frame (src/scene/camera.rs: 96-119: at the RIB
+ * default eye = 0, look +z, up +y it gives right = +x), placed at `eye`.
+ * Pixel filter per src/scene/camera.rs:14-66 (0 box, 1 triangle, 2
+ * gaussian); rays per Camera::generate_ray_lens (camera.rs:150-179):
+ * perspective with an optional thin lens, or orthographic. */
 typedef struct rrhip_camera_desc {
   uint32_t width, height;
   uint32_t filter;
-  uint32_t reserved;
-  double fov_deg; /* vertical */
+  uint32_t projection;    /* 0 perspective, 1 orthographic */
+  double fov_deg;         /* vertical (perspective) */
   double filter_width;
   double eye[3], look_at[3], up[3];
+  double lens_radius;     /* aperture radius of the thin lens; 0 = pinhole */
+  double focal_distance;  /* focal plane distance along forward (lens_radius > 0) */
+  double ortho_half[2];   /* orthographic screen-window half extents along right, up */
 } rrhip_camera_desc;
 
 /* Scene description. `struct_size` must be sizeof(rrhip_scene_desc).

@@ -160,6 +160,7 @@ def _pt_sigs():
     sig = {
         "orc_render": (i32, [vp, vp, u32, u32, vp, i32, vp]),
         "orc_render_adaptive": (i32, [vp, vp, u32, f64, vp, i32]),
+        "orc_camera_rays": (None, [vp, u64, vp, vp, vp, vp]),
         "orc_bsdf_eval": (None, [vp, vp, vp, f64, vp, vp]),
         "orc_bsdf_eval_n": (None, [vp, vp, vp, f64, u64, vp, vp]),
         "orc_bsdf_sample_n": (None, [vp, vp, f64, u64, u64, u64, vp, vp, vp, vp]),
@@ -189,6 +190,19 @@ def render(desc, params, start, count, accum=None, threads=None):
     lib.orc_render(ctypes.addressof(desc), ctypes.addressof(params), start, count, _p(accum),
                    threads or default_threads(), ctypes.byref(rays))
     return accum, rays.value
+
+
+def camera_rays(desc, xy, samples):
+    """Camera rays (origins, unit directions) of pixels xy (n x 2) at the
+    given sample indices."""
+    lib = _pt_sigs()
+    xy = np.ascontiguousarray(xy, np.uint32).reshape(-1, 2)
+    samples = np.ascontiguousarray(samples, np.uint64)
+    n = xy.shape[0]
+    o = np.zeros((n, 3))
+    d = np.zeros((n, 3))
+    lib.orc_camera_rays(ctypes.addressof(desc), n, _p(xy), _p(samples), _p(o), _p(d))
+    return o, d
 
 
 def render_adaptive(desc, params, max_spp, tol, threads=None):

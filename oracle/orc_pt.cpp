@@ -570,14 +570,20 @@ double Camera::filter_1d(double u) const {
 }
 
 // camera.rs:150-179 (pinhole)
-Ray Camera::generate_ray(double px, double py) const {
+// camera.rs:150-179
+Ray Camera::generate_ray_lens(double px, double py, double lu, double lv) const {
   double u = (px / (double)width) * 2.0 - 1.0;
   double v = 1.0 - (py / (double)height) * 2.0;
+  if (projection == 1) return Ray::make(eye + right * (u * ortho_half[0]) + up * (v * ortho_half[1]), forward);
   double aspect = (double)width / (double)height;
   double half_height = std::tan(fov * PI / 180.0 / 2.0);
   double half_width = aspect * half_height;
   Vec3 dir = forward + right * (u * half_width) + up * (v * half_height);
-  return Ray::make(eye, dir);
+  if (lens_radius <= 0.0) return Ray::make(eye, dir);
+  const double r = lens_radius * std::sqrt(lu), phi = 2.0 * PI * lv;
+  const Vec3 origin = eye + right * (r * std::cos(phi)) + up * (r * std::sin(phi));
+  const Vec3 focus = eye + dir * focal_distance;
+  return Ray::make(origin, focus - origin);
 }
 
 // ---- geometry -----------------------------------------------------------------
@@ -738,9 +744,15 @@ bool Scene::intersect(const Ray& ray, Intersection* out) const {
 
 // pt/mod.rs:340-356 (no lens, no motion)
 Ray Scene::camera_ray(uint32_t x, uint32_t y, Pcg32& rng) const {
+  // pt/mod.rs:340-356
   double u1 = rng.next_f64(), u2 = rng.next_f64();
   double dx = camera.filter_1d(u1), dy = camera.filter_1d(u2);
-  return camera.generate_ray((double)x + 0.5 + dx, (double)y + 0.5 + dy);
+  double lu = 0.5, lv = 0.5;
+  if (camera.lens_radius > 0.0) {
+    lu = rng.next_f64();
+    lv = rng.next_f64();
+  }
+  return camera.generate_ray_lens((double)x + 0.5 + dx, (double)y + 0.5 + dy, lu, lv);
 }
 
 // ---- participating media (medium.rs:28-101, pt/volume.rs:29-189) -------------------
@@ -1336,6 +1348,11 @@ void Scene::load(const rrhip_scene_desc& d) {
   camera.eye = v3(c.eye);
   camera.filter = (int)c.filter;
   camera.filter_width = c.filter_width > 0 ? c.filter_width : 1.0;
+  camera.projection = (int)c.projection;
+  camera.lens_radius = c.lens_radius;
+  camera.focal_distance = c.focal_distance;
+  camera.ortho_half[0] = c.ortho_half[0];
+  camera.ortho_half[1] = c.ortho_half[1];
   // RenderMan frame of Camera::new (camera.rs:96-119) aimed along look_at
   camera.forward = (v3(c.look_at) - camera.eye).normalize();
   camera.right = v3(c.up).cross(camera.forward).normalize();

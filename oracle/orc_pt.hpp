@@ -128,8 +128,11 @@ struct Camera {  // camera.rs:69-179 (perspective, pinhole)
   Vec3 eye, forward, up, right;
   int filter = 0;  // 0 box, 1 triangle, 2 gaussian
   double filter_width = 1.0;
+  int projection = 0;  // 0 perspective, 1 orthographic
+  double lens_radius = 0, focal_distance = 1;
+  double ortho_half[2] = {1, 1};
   double filter_1d(double u) const;
-  Ray generate_ray(double px, double py) const;
+  Ray generate_ray_lens(double px, double py, double lu, double lv) const;
 };
 
 struct Intersection {

@@ -140,6 +140,25 @@ int orc_render_adaptive(const rrhip_scene_desc* desc, const rrhip_render_params*
   return 0;
 }
 
+// Camera rays (pt/mod.rs:340-356) of n (pixel x, y, sample index) triples
+// from the scene's camera: origins and unit directions.
+void orc_camera_rays(const rrhip_scene_desc* desc, uint64_t n, const uint32_t* xy, const uint64_t* samples,
+                     double* o, double* d) {
+  Scene scene;
+  scene.load(*desc);
+  for (uint64_t i = 0; i < n; i++) {
+    const uint32_t x = xy[2 * i], y = xy[2 * i + 1];
+    Pcg32 rng = Pcg32::for_pixel_sample((uint64_t)y * scene.camera.width + x, samples[i]);
+    const Ray r = scene.camera_ray(x, y, rng);
+    o[3 * i] = r.origin.x;
+    o[3 * i + 1] = r.origin.y;
+    o[3 * i + 2] = r.origin.z;
+    d[3 * i] = r.direction.x;
+    d[3 * i + 1] = r.direction.y;
+    d[3 * i + 2] = r.direction.z;
+  }
+}
+
 // bxdf.rs eval_pdf for one material (wo, wi in the local shading frame).
 void orc_bsdf_eval(const rrhip_material_desc* m, const double* wo, const double* wi, double eta,
                    double* f, double* pdf) {

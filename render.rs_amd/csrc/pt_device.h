@@ -698,14 +698,34 @@ __device__ __forceinline__ float filter_1d(const PtCamera& c, float u) {
   return (u - 0.5f) * c.filter_width;
 }
 
+// pt/mod.rs:340-356 + camera.rs:150-179: filter offsets (u1, u2), then the
+// aperture sample (lu, lv) when the lens is open; thin lens through the
+// focal plane at forward distance focal_distance, or orthographic.
 __device__ __forceinline__ void camera_ray(const PtCamera& c, uint32_t x, uint32_t y, Pcg32& rng, V3& o, V3& d) {
   const float u1 = rnd(rng), u2 = rnd(rng);
   const float px = (float)x + 0.5f + filter_1d(c, u1);
   const float py = (float)y + 0.5f + filter_1d(c, u2);
+  float lu = 0.5f, lv = 0.5f;
+  if (c.lens_radius > 0.f) {
+    lu = rnd(rng);
+    lv = rnd(rng);
+  }
   const float u = (px / (float)c.width) * 2.f - 1.f;
   const float v = 1.f - (py / (float)c.height) * 2.f;
-  o = v3(c.eye);
-  d = normalize(v3(c.forward) + v3(c.right) * (u * c.half_w) + v3(c.up) * (v * c.half_h));
+  if (c.projection == 1) {
+    o = v3(c.eye) + v3(c.right) * (u * c.ortho_w) + v3(c.up) * (v * c.ortho_h);
+    d = v3(c.forward);
+    return;
+  }
+  const V3 dir = v3(c.forward) + v3(c.right) * (u * c.half_w) + v3(c.up) * (v * c.half_h);
+  if (c.lens_radius > 0.f) {
+    const float r = c.lens_radius * sqrtf(lu), phi = 2.f * kPi * lv;
+    o = v3(c.eye) + v3(c.right) * (r * cosf(phi)) + v3(c.up) * (r * sinf(phi));
+    d = normalize(v3(c.eye) + dir * c.focal_distance - o);
+  } else {
+    o = v3(c.eye);
+    d = normalize(dir);
+  }
 }
 
 }  // namespace pt

@@ -499,7 +499,13 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
   // Camera: RenderMan frame of Camera::new (camera.rs:96-119), aimed.
   const rrhip_camera_desc& c = d.camera;
   if (c.width == 0 || c.height == 0) return *err = "camera needs width and height", false;
-  if (!(c.fov_deg > 0.0 && c.fov_deg < 180.0)) return *err = "camera fov must be in (0, 180)", false;
+  if (c.projection > 1) return *err = "camera projection must be 0 (perspective) or 1 (orthographic)", false;
+  if (c.projection == 0 && !(c.fov_deg > 0.0 && c.fov_deg < 180.0)) return *err = "camera fov must be in (0, 180)", false;
+  if (!(c.lens_radius >= 0.0 && c.lens_radius < 1e30)) return *err = "camera lens_radius must be finite and >= 0", false;
+  if (c.lens_radius > 0.0 && !(c.focal_distance > 0.0 && c.focal_distance < 1e30))
+    return *err = "camera focal_distance must be > 0 with a lens", false;
+  if (c.projection == 1 && !(c.ortho_half[0] > 0.0 && c.ortho_half[1] > 0.0))
+    return *err = "orthographic camera needs positive ortho_half", false;
   PtCamera& cam = out->cam;
   cam.width = c.width;
   cam.height = c.height;
@@ -516,6 +522,11 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
   double hh = std::tan(c.fov_deg * kPi / 180.0 / 2.0);
   cam.half_h = (float)hh;
   cam.half_w = (float)(hh * (double)c.width / (double)c.height);
+  cam.projection = c.projection;
+  cam.lens_radius = (float)c.lens_radius;
+  cam.focal_distance = (float)c.focal_distance;
+  cam.ortho_w = (float)c.ortho_half[0];
+  cam.ortho_h = (float)c.ortho_half[1];
   return true;
 }
 

@@ -119,6 +119,9 @@ struct PtCamera {
   float eye[3];
   float forward[3], right[3], up[3];
   float half_w, half_h;
+  uint32_t projection;          // 0 perspective, 1 orthographic
+  float lens_radius, focal_distance;
+  float ortho_w, ortho_h;       // orthographic screen-window half extents
 };
 
 // Read-only scene view passed to the kernels by value.

@@ -102,9 +102,11 @@ class LightDesc(ctypes.Structure):
 
 class CameraDesc(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
-                ("filter", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("filter", ctypes.c_uint32), ("projection", ctypes.c_uint32),
                 ("fov_deg", ctypes.c_double), ("filter_width", ctypes.c_double),
-                ("eye", D3), ("look_at", D3), ("up", D3)]
+                ("eye", D3), ("look_at", D3), ("up", D3),
+                ("lens_radius", ctypes.c_double), ("focal_distance", ctypes.c_double),
+                ("ortho_half", ctypes.c_double * 2)]
 
 
 class MediumDesc(ctypes.Structure):

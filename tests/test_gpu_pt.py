@@ -284,3 +284,33 @@ def test_launch_modes_equal_stream_launches(ctx):
     assert out[0][0].tobytes() == out[1][0].tobytes()
     assert out[0][1][0].tobytes() == out[1][1][0].tobytes()
     pf.close()
+
+
+def test_thin_lens_and_orthographic_cameras_match_oracle(ctx):
+    """Camera::generate_ray_lens (camera.rs:150-179): a thin lens focused
+    on the back spheres' depth and an orthographic screen window, each
+    rendered on the device and by the oracle."""
+    import ctypes
+    say("box room 96x96, thin lens (r 0.05, focus 1.0) and orthographic, 128 spp vs oracle")
+    p = rrhip.RenderParams.make(max_depth=8)
+    s = scenes.box_room(96, 96)
+    s.camera.lens_radius = 0.05
+    s.camera.focal_distance = 1.0
+    acc_g, _ = gpu_render(ctx, s, p, 0, 128)
+    acc_c, _ = orc.render(s.desc(), p, 0, 128)
+    check_parity(acc_g, acc_c, label="thin lens")
+    s2 = scenes.box_room(96, 96)
+    s2.camera.projection = 1
+    s2.camera.ortho_half = (ctypes.c_double * 2)(0.9, 0.9)
+    acc_g, _ = gpu_render(ctx, s2, p, 0, 128)
+    acc_c, _ = orc.render(s2.desc(), p, 0, 128)
+    check_parity(acc_g, acc_c, label="orthographic")
+    # invalid cameras are refused
+    s3 = scenes.box_room(8, 8)
+    s3.camera.lens_radius = 0.1  # focal_distance 0
+    with pytest.raises(rrhip.RrhipError):
+        rrhip.PtScene(ctx, s3.desc())
+    s4 = scenes.box_room(8, 8)
+    s4.camera.projection = 1  # ortho_half 0
+    with pytest.raises(rrhip.RrhipError):
+        rrhip.PtScene(ctx, s4.desc())

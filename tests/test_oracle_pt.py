@@ -4,6 +4,8 @@ consistency), light_sampler.rs:322-371 (pmf consistency, near-light
 importance), camera.rs:213-236 (filter support), pbr_tests.rs:31-89 (white
 furnace, metal/glass energy) and pt_tests.rs:17-95 (Cornell GI sanity,
 determinism, convergence), plus closed forms (Lambert f = rho/pi)."""
+import ctypes
+
 import numpy as np
 
 import orc
@@ -221,3 +223,50 @@ def test_sigma_harness_detects_a_planted_bias():
     biased[:, :3] *= 1.03
     biased[:, 3:6] *= 1.03 ** 2
     assert (image_mean_z(a, biased) > 3).any()
+
+
+def test_thin_lens_focuses_on_the_focal_plane():
+    """camera.rs:150-179: every ray of a pixel (fixed sub-pixel position)
+    passes through the same point of the focal plane, eye + dir *
+    focal_distance; origins fill the aperture disk uniformly (E r^2 =
+    R^2 / 2) in the right/up plane."""
+    s = scenes.box_room(32, 24)
+    s.camera.lens_radius = 0.1
+    s.camera.focal_distance = 1.5
+    s.camera.filter_width = 1e-9  # pixel centres
+    d = s.desc()
+    n = 4000
+    xy = np.tile([[20, 7]], (n, 1))
+    o, dirs = orc.camera_rays(d, xy, np.arange(n))
+    eye = np.array([0, 0, -0.9])
+    fwd = np.array([0, 0, 1.0])
+    t = (1.5 - (o - eye) @ fwd) / (dirs @ fwd)
+    hit = o + dirs * t[:, None]
+    assert np.abs(hit - hit[0]).max() < 1e-9
+    off = o - eye
+    assert np.abs(off @ fwd).max() < 1e-15
+    r2 = (off ** 2).sum(1)
+    assert r2.max() <= 0.1 ** 2 * (1 + 1e-12)
+    assert abs(r2.mean() / 0.01 - 0.5) < 0.02
+    # pinhole: the same pixel centre without a lens passes through that point too
+    s.camera.lens_radius = 0.0
+    o0, d0 = orc.camera_rays(s.desc(), xy[:1], np.arange(1))
+    t0 = 1.5 / (d0[0] @ fwd)
+    assert np.allclose(o0[0] + d0[0] * t0, hit[0], atol=1e-12)
+
+
+def test_orthographic_rays_are_parallel_on_the_screen_window():
+    """camera.rs:154-159: origin eye + right u hw + up v hh, direction
+    forward."""
+    s = scenes.box_room(16, 8)
+    s.camera.projection = 1
+    s.camera.ortho_half = (ctypes.c_double * 2)(0.8, 0.4)
+    s.camera.filter_width = 1e-9
+    xs, ys = np.meshgrid(np.arange(16), np.arange(8))
+    xy = np.stack([xs.ravel(), ys.ravel()], 1)
+    o, dirs = orc.camera_rays(s.desc(), xy, np.zeros(len(xy), np.uint64))
+    assert np.allclose(dirs, [0, 0, 1], atol=1e-15)
+    u = (xy[:, 0] + 0.5) / 16 * 2 - 1
+    v = 1 - (xy[:, 1] + 0.5) / 8 * 2
+    want = np.stack([u * 0.8, v * 0.4, np.full(len(u), -0.9)], 1)
+    assert np.allclose(o, want, atol=1e-9)
