@@ -94,6 +94,10 @@ typedef struct rrhip_instance_desc {
   uint32_t mesh;
   uint32_t material;
   double object_to_world[12];
+  double object_to_world1[12];  /* at shutter close (Instance::transform1) when `moving`; rays at
+                                   time t see the element-wise lerp of the two (mesh.rs:294-305) */
+  uint32_t moving;
+  uint32_t reserved;
 } rrhip_instance_desc;
 
 /* Loose world-space triangle (render.rs builds these from Polygon,
@@ -181,6 +185,10 @@ typedef struct rrhip_camera_desc {
   double lens_radius;     /* aperture radius of the thin lens; 0 = pinhole */
   double focal_distance;  /* focal plane distance along forward (lens_radius > 0) */
   double ortho_half[2];   /* orthographic screen-window half extents along right, up */
+  double motion_inv[12];  /* Camera::motion_inv (3x4 row-major): rays at shutter time t are mapped by
+                             lerp(I, motion_inv, t) (camera.rs:134-146) when has_motion_inv */
+  uint32_t has_motion_inv;
+  uint32_t reserved;
 } rrhip_camera_desc;
 
 /* Scene description. `struct_size` must be sizeof(rrhip_scene_desc).

@@ -570,6 +570,18 @@ double Camera::filter_1d(double u) const {
 }
 
 // camera.rs:150-179 (pinhole)
+// camera.rs:134-146
+Ray Camera::apply_motion(const Ray& r, double t) const {
+  if (!has_motion_inv || t <= 0.0) return r;
+  Mat4 m;
+  const Mat4 id = Mat4::identity();
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) m.m[i][j] = id.m[i][j] + (motion_inv.m[i][j] - id.m[i][j]) * t;
+  Ray out = Ray::make(m.transform_point(r.origin), m.transform_vec(r.direction));
+  out.time = r.time;
+  return out;
+}
+
 // camera.rs:150-179
 Ray Camera::generate_ray_lens(double px, double py, double lu, double lv) const {
   double u = (px / (double)width) * 2.0 - 1.0;
@@ -644,10 +656,20 @@ bool SphereObj::intersect(const Ray& ray, Intersection* out) const {
   return true;
 }
 
+Mat4 InstanceObj::inverse_at(double time) const {
+  if (!moving || !(time > 0.0)) return inverse;
+  Mat4 fwd, inv;
+  for (int r = 0; r < 4; r++)
+    for (int c = 0; c < 4; c++) fwd.m[r][c] = transform.m[r][c] + (transform1.m[r][c] - transform.m[r][c]) * time;
+  if (!fwd.inverse(inv)) return Mat4::identity();
+  return inv;
+}
+
 // mesh.rs:308-381 -> the parametric hit and world normal of one instance
 bool Scene::instance_hit(const InstanceObj& inst, const Ray& ray, double t_max, double* t_param,
                          Vec3* world_normal) const {
-  Ray local = Ray::raw(inst.inverse.transform_point(ray.origin), inst.inverse.transform_vec(ray.direction));
+  const Mat4 inverse = inst.inverse_at(ray.time);
+  Ray local = Ray::raw(inverse.transform_point(ray.origin), inverse.transform_vec(ray.direction));
   const MeshObj& mo = meshes[inst.mesh];
   const Mesh& m = mo.mesh;
   double hu = 0, hv = 0;
@@ -678,7 +700,7 @@ bool Scene::instance_hit(const InstanceObj& inst, const Ray& ray, double t_max, 
     ln = (m.vertex(i1) - v0).cross(m.vertex(i2) - v0).normalize();
   }
   *t_param = t;
-  *world_normal = inst.inverse.transform_normal(ln).normalize();
+  *world_normal = inverse.transform_normal(ln).normalize();
   return true;
 }
 
@@ -752,7 +774,13 @@ Ray Scene::camera_ray(uint32_t x, uint32_t y, Pcg32& rng) const {
     lu = rng.next_f64();
     lv = rng.next_f64();
   }
-  return camera.generate_ray_lens((double)x + 0.5 + dx, (double)y + 0.5 + dy, lu, lv);
+  Ray ray = camera.generate_ray_lens((double)x + 0.5 + dx, (double)y + 0.5 + dy, lu, lv);
+  if (has_motion) {
+    const double t = rng.next_f64();
+    ray.time = t;
+    ray = camera.apply_motion(ray, t);
+  }
+  return ray;
 }
 
 // ---- participating media (medium.rs:28-101, pt/volume.rs:29-189) -------------------
@@ -941,7 +969,7 @@ bool Material::hull() const {
 // pt/mod.rs:1238-1306. The stack starts from the atmosphere with the
 // shading point's medium on top (when it is another one).
 Vec3 Scene::visibility(const Vec3& origin0, const Vec3& dir, double remaining, Pcg32& rng, uint64_t* rays,
-                       int medium) const {
+                       int medium, double time) const {
   MediumStack ms;
   ms.atmosphere = atmosphere;
   if (medium >= 0 && medium != atmosphere) ms.push(medium);
@@ -950,7 +978,9 @@ Vec3 Scene::visibility(const Vec3& origin0, const Vec3& dir, double remaining, P
   for (int i = 0; i < 16; i++) {
     if (rays) ++*rays;
     Intersection h;
-    const bool hit = intersect(Ray::make(origin, dir), &h);
+    Ray r = Ray::make(origin, dir);
+    r.time = time;
+    const bool hit = intersect(r, &h);
     const double seg = std::fmin(hit ? h.t : remaining, remaining);
     const int cur = ms.current();
     if (cur >= 0) {
@@ -981,7 +1011,7 @@ Vec3 Scene::visibility(const Vec3& origin0, const Vec3& dir, double remaining, P
 // pt/mod.rs:1177-1209
 Vec3 Scene::area_contribution(const Vec3& p, const Vec3& n, const Vec3& sample_point,
                               const Vec3& light_normal, double area, const Light& light, const EvalFn& eval,
-                              double pick_pmf, Pcg32& rng, uint64_t* rays, int medium) const {
+                              double pick_pmf, Pcg32& rng, uint64_t* rays, int medium, double time) const {
   Vec3 to_light = sample_point - p;
   double dist2 = std::fmax(to_light.length_squared(), 1e-12);
   Vec3 wi = to_light.normalize();
@@ -994,7 +1024,7 @@ Vec3 Scene::area_contribution(const Vec3& p, const Vec3& n, const Vec3& sample_p
   // visibility_to (pt/mod.rs:1212-1224)
   Vec3 dir = sample_point - p;
   double dist = dir.length();
-  Vec3 vis = visibility(p + n * 1e-4, dir.normalize(), dist - 1e-3, rng, rays, medium);
+  Vec3 vis = visibility(p + n * 1e-4, dir.normalize(), dist - 1e-3, rng, rays, medium, time);
   if (max_component(vis) <= 0.0) return Vec3::zero();
   double pdf_sa = pick_pmf * dist2 / (cos_light * area);
   double weight = power_heuristic(pdf_sa, bsdf_pdf);
@@ -1003,7 +1033,7 @@ Vec3 Scene::area_contribution(const Vec3& p, const Vec3& n, const Vec3& sample_p
 
 // pt/mod.rs:1032-1174
 Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const EvalFn& eval, double pick_pmf,
-                         Pcg32& rng, uint64_t* rays, int medium) const {
+                         Pcg32& rng, uint64_t* rays, int medium, double time) const {
   switch (light.kind) {
     case RRHIP_LIGHT_POINT: {
       Vec3 to_light = light.position - p;
@@ -1014,7 +1044,7 @@ Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const
       eval(wi, &f, &pdf);
       if (max_component(f) <= 0.0) return Vec3::zero();
       Vec3 dir = light.position - p;
-      Vec3 vis = visibility(p + n * 1e-4, dir.normalize(), dir.length() - 1e-3, rng, rays, medium);
+      Vec3 vis = visibility(p + n * 1e-4, dir.normalize(), dir.length() - 1e-3, rng, rays, medium, time);
       if (max_component(vis) <= 0.0) return Vec3::zero();
       return f * light.radiance() * vis / (dist2 * pick_pmf);
     }
@@ -1034,20 +1064,20 @@ Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const
       double pdf;
       eval(wi, &f, &pdf);
       if (max_component(f) <= 0.0) return Vec3::zero();
-      Vec3 vis = visibility(p + n * 1e-4, wi, 1e8, rng, rays, medium);
+      Vec3 vis = visibility(p + n * 1e-4, wi, 1e8, rng, rays, medium, time);
       if (max_component(vis) <= 0.0) return Vec3::zero();
       return f * light.radiance() * vis / pick_pmf;
     }
     case RRHIP_LIGHT_RECT: {
       double u = rng.next_f64(), v = rng.next_f64();
       Vec3 sp = light.corner + light.edge1 * u + light.edge2 * v;
-      return area_contribution(p, n, sp, light.normal, light.area, light, eval, pick_pmf, rng, rays, medium);
+      return area_contribution(p, n, sp, light.normal, light.area, light, eval, pick_pmf, rng, rays, medium, time);
     }
     case RRHIP_LIGHT_DISK: {
       double u = rng.next_f64(), v = rng.next_f64();
       double r = std::sqrt(u), phi = 2.0 * PI * v;
       Vec3 sp = light.center + light.e1 * (r * std::cos(phi)) + light.e2 * (r * std::sin(phi));
-      return area_contribution(p, n, sp, light.normal, light.area, light, eval, pick_pmf, rng, rays, medium);
+      return area_contribution(p, n, sp, light.normal, light.area, light, eval, pick_pmf, rng, rays, medium, time);
     }
     case RRHIP_LIGHT_SPHERE: {
       Vec3 to_center = light.center - p;
@@ -1069,7 +1099,7 @@ Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const
       double dist = std::sqrt(dist2) * cos_t - std::sqrt(std::fmax(r * r - dist2 * sin_t * sin_t, 0.0));
       Vec3 target = p + wi * dist;
       Vec3 dir = target - p;
-      Vec3 vis = visibility(p + n * 1e-4, dir.normalize(), dir.length() - 1e-3, rng, rays, medium);
+      Vec3 vis = visibility(p + n * 1e-4, dir.normalize(), dir.length() - 1e-3, rng, rays, medium, time);
       if (max_component(vis) <= 0.0) return Vec3::zero();
       double pdf_sa = pick_pmf / std::fmax(2.0 * PI * (1.0 - cos_max), 1e-12);
       double weight = power_heuristic(pdf_sa, bsdf_pdf);
@@ -1086,7 +1116,7 @@ Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const
       double bsdf_pdf;
       eval(wi, &f, &bsdf_pdf);
       if (max_component(f) <= 0.0) return Vec3::zero();
-      Vec3 vis = visibility(p + n * 1e-4, wi, 1e8, rng, rays, medium);
+      Vec3 vis = visibility(p + n * 1e-4, wi, 1e8, rng, rays, medium, time);
       if (max_component(vis) <= 0.0) return Vec3::zero();
       double weight = power_heuristic(pdf_sa, bsdf_pdf);
       return f * light.radiance() * vis * (weight / pdf_sa);
@@ -1118,6 +1148,7 @@ static double light_pdf_solid_angle(const Light& light, const Vec3& origin, cons
 
 // pt/mod.rs:449-990, surface subset
 Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) const {
+  const double time = ray.time;  // the path's shutter time
   Vec3 l = Vec3::zero(), beta = Vec3::one();
   double prev_pdf = 0.0;
   Vec3 prev_origin = ray.origin;
@@ -1161,7 +1192,7 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
             const Vec3 wi = to_l.normalize();
             const Vec3 wo = -ray.direction.normalize();
             const double ph = hg_phase(md.g, -wo.dot(wi));
-            const Vec3 vis = visibility(p, wi, std::sqrt(dist2) - 1e-3, rng, rays, medium);
+            const Vec3 vis = visibility(p, wi, std::sqrt(dist2) - 1e-3, rng, rays, medium, time);
             if (max_component(vis) > 0.0) {
               const double pdf_d = distance_pdf_approx(md, t_eq, t_seg);
               const double w_eq = pdf_eq / (pdf_eq + pdf_d);
@@ -1192,7 +1223,7 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
             const double pdf_e = equiangular_pdf(ray.origin, ray.direction, t_seg, lights[seg_li].position, ev.t);
             w_dist = pdf_d / (pdf_d + pdf_e);
           }
-          l = l + beta * sample_light(lights[seg_li], p, Vec3::zero(), eval, seg_pick, rng, rays, medium) * w_dist;
+          l = l + beta * sample_light(lights[seg_li], p, Vec3::zero(), eval, seg_pick, rng, rays, medium, time) * w_dist;
         }
         const double u1 = rng.next_f64(), u2 = rng.next_f64();
         double pdf;
@@ -1201,6 +1232,7 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
         prev_origin = p;
         from_camera = false;
         ray = Ray::make(p, wi);
+        ray.time = time;
         depth++;
         if (depth >= rp.rr_start) {
           double q = std::fmin(max_component(beta), 0.95);
@@ -1235,6 +1267,7 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
       else
         ms.pop();
       ray = Ray::make(hit.point + ray.direction.normalize() * 1e-4, ray.direction);
+      ray.time = time;
       continue;
     }
     // Presence cutout: pass through with probability 1 - presence
@@ -1243,6 +1276,7 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
     if (presence < 1.0 && rng.next_f64() >= presence && skips < 16) {
       skips++;
       ray = Ray::make(hit.point + ray.direction.normalize() * 1e-4, ray.direction);
+      ray.time = time;
       continue;
     }
     const PbrParams& pbr = mat.pbr;
@@ -1278,7 +1312,7 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
           bsdf_eval_pdf(pbr, wo_l, wi_l, eta_rel, f, pdf);
           *f = *f * wi_l.z;
         };
-        Vec3 c = sample_light(lights[li], hit.point, n, eval, pmf, rng, rays, medium);
+        Vec3 c = sample_light(lights[li], hit.point, n, eval, pmf, rng, rays, medium, time);
         l = l + beta * c;
       }
     }
@@ -1297,6 +1331,7 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
     }
     double offset = s.transmitted ? -1e-4 : 1e-4;
     ray = Ray::make(hit.point + n * offset, wi_world);
+    ray.time = time;
     depth++;
     if (depth >= rp.rr_start) {
       double q = std::fmin(max_component(beta), 0.95);
@@ -1353,6 +1388,11 @@ void Scene::load(const rrhip_scene_desc& d) {
   camera.focal_distance = c.focal_distance;
   camera.ortho_half[0] = c.ortho_half[0];
   camera.ortho_half[1] = c.ortho_half[1];
+  camera.has_motion_inv = c.has_motion_inv != 0;
+  camera.motion_inv = Mat4::identity();
+  if (camera.has_motion_inv)
+    for (int r = 0; r < 3; r++)
+      for (int cc = 0; cc < 4; cc++) camera.motion_inv.m[r][cc] = c.motion_inv[4 * r + cc];
   // RenderMan frame of Camera::new (camera.rs:96-119) aimed along look_at
   camera.forward = (v3(c.look_at) - camera.eye).normalize();
   camera.right = v3(c.up).cross(camera.forward).normalize();
@@ -1483,19 +1523,28 @@ void Scene::load(const rrhip_scene_desc& d) {
     for (int r = 0; r < 3; r++)
       for (int cc = 0; cc < 4; cc++) in.transform.m[r][cc] = id.object_to_world[4 * r + cc];
     if (!in.transform.inverse(in.inverse)) in.inverse = Mat4::identity();
-    // mesh.rs:266-292 world bounds from the 8 transformed corners
+    in.moving = id.moving != 0;
+    in.transform1 = Mat4::identity();
+    if (in.moving)
+      for (int r = 0; r < 3; r++)
+        for (int cc = 0; cc < 4; cc++) in.transform1.m[r][cc] = id.object_to_world1[4 * r + cc];
+    // mesh.rs:266-292 world bounds from the 8 transformed corners, at both
+    // shutter ends when moving
     const Aabb& lb = meshes[in.mesh].mesh.local_bounds;
     Aabb wb;
-    for (int k = 0; k < 8; k++) {
-      Vec3 corner{(k & 1) ? lb.max.x : lb.min.x, (k & 2) ? lb.max.y : lb.min.y,
-                  (k & 4) ? lb.max.z : lb.min.z};
-      wb.grow_point(in.transform.transform_point(corner));
-    }
+    for (int e = 0; e < (in.moving ? 2 : 1); e++)
+      for (int k = 0; k < 8; k++) {
+        Vec3 corner{(k & 1) ? lb.max.x : lb.min.x, (k & 2) ? lb.max.y : lb.min.y,
+                    (k & 4) ? lb.max.z : lb.min.z};
+        wb.grow_point((e ? in.transform1 : in.transform).transform_point(corner));
+      }
     in.world_bounds = wb;
     ib.push_back(wb);
     instances.push_back(in);
   }
   tlas.build(ib, bp);
+  has_motion = camera.has_motion_inv;
+  for (const InstanceObj& in : instances) has_motion = has_motion || in.moving;
 }
 
 }  // namespace orc

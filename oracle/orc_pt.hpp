@@ -131,8 +131,11 @@ struct Camera {  // camera.rs:69-179 (perspective, pinhole)
   int projection = 0;  // 0 perspective, 1 orthographic
   double lens_radius = 0, focal_distance = 1;
   double ortho_half[2] = {1, 1};
+  bool has_motion_inv = false;
+  Mat4 motion_inv;
   double filter_1d(double u) const;
   Ray generate_ray_lens(double px, double py, double lu, double lv) const;
+  Ray apply_motion(const Ray& r, double t) const;  // camera.rs:134-146
 };
 
 struct Intersection {
@@ -163,7 +166,11 @@ struct MeshObj {
 struct InstanceObj {
   uint32_t mesh, material;
   Mat4 transform, inverse;
+  bool moving = false;
+  Mat4 transform1;  // at shutter close
   Aabb world_bounds;
+  // mesh.rs:294-305: the inverse transform at shutter time `time`.
+  Mat4 inverse_at(double time) const;
 };
 
 // medium.rs:28-65 with the density as a vertex grid (empty: homogeneous).
@@ -218,6 +225,7 @@ class Scene {
   int dome = -1;
   std::vector<Medium> media;
   int atmosphere = -1;
+  bool has_motion = false;  // scene_builder.rs:239-241 (moving instances or camera)
 
   void load(const rrhip_scene_desc& d);
   bool intersect(const Ray& ray, Intersection* out) const;
@@ -231,17 +239,18 @@ class Scene {
   // eval(wi, f, pdf): f * cos (or the phase value) towards world wi and the
   // pdf of the scattering strategy, for MIS.
   using EvalFn = std::function<void(const Vec3&, Vec3*, double*)>;
-  // `medium`: the medium the shading point is in (-1: none).
+  // `medium`: the medium the shading point is in (-1: none); `time`: the
+  // path's shutter time.
   Vec3 sample_light(const Light& light, const Vec3& p, const Vec3& n, const EvalFn& eval, double pick_pmf,
-                    Pcg32& rng, uint64_t* rays, int medium) const;
+                    Pcg32& rng, uint64_t* rays, int medium, double time) const;
   Vec3 area_contribution(const Vec3& p, const Vec3& n, const Vec3& sample_point,
                          const Vec3& light_normal, double area, const Light& light, const EvalFn& eval,
-                         double pick_pmf, Pcg32& rng, uint64_t* rays, int medium) const;
+                         double pick_pmf, Pcg32& rng, uint64_t* rays, int medium, double time) const;
   // Fractional visibility (pt/mod.rs:1238-1306): walks the blockers,
   // multiplying each segment's medium transmittance; hulls push / pop
   // media, cutouts attenuate by 1 - presence, anything else blocks.
   Vec3 visibility(const Vec3& origin, const Vec3& dir, double remaining, Pcg32& rng, uint64_t* rays,
-                  int medium) const;
+                  int medium, double time) const;
 };
 
 }  // namespace orc

@@ -343,6 +343,8 @@ hipError_t upload_pt(rrhip_scene* s, const PtSceneHost& h) {
   v.atmosphere = h.atmosphere;
   v.stack_need = h.stack_need;
   v.transparent = h.transparent;
+  v.motion = h.motion;
+  if (e == hipSuccess) e = upload_vec(h.inst_motion, &s->pt_mem[15], &v.inst_motion);
   v.num_insts = (uint32_t)h.insts.size();
   v.num_tris = (uint32_t)h.tris.size();
   v.num_spheres = (uint32_t)h.spheres.size();

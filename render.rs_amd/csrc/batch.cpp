@@ -177,10 +177,11 @@ int rrhip_batch_create(rrhip_ctx* ctx, uint32_t num_jobs, const rrhip_batch_job*
   if (e == hipSuccess) e = alloc_n(b, &P.so, n);
   if (e == hipSuccess) e = alloc_n(b, &P.sd, n);
   if (e == hipSuccess) e = alloc_n(b, &P.sc, n);
-  bool eq = false, transparent = false;
+  bool eq = false, transparent = false, motion = false;
   for (uint32_t j = 0; j < num_jobs; j++) {
     eq |= jobs[j].scene->pt_eq;
     transparent |= jobs[j].scene->pt.transparent != 0;
+    motion |= jobs[j].scene->pt.motion != 0;
   }
   if (e == hipSuccess && eq) {
     e = alloc_n(b, &P.so2, n);
@@ -191,6 +192,7 @@ int rrhip_batch_create(rrhip_ctx* ctx, uint32_t num_jobs, const rrhip_batch_job*
     e = alloc_n(b, &P.mstk, n);
     if (e == hipSuccess) e = alloc_n(b, &P.srng, n);
   }
+  if (e == hipSuccess && motion) e = alloc_n(b, &P.stime, n);
   if (e == hipSuccess) e = alloc_n(b, &P.job, n);
   if (e == hipSuccess) e = alloc_n(b, &P.counts, 8);
   if (e == hipSuccess) e = alloc_n(b, &P.stats, 4);

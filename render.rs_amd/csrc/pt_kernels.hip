@@ -95,9 +95,23 @@ __device__ __forceinline__ void init_path(const PtSceneView& S, const PathBuffer
   Pcg32 rng = Pcg32::for_pixel_sample(pixel, sample);
   V3 o, d;
   camera_ray(S.cam, x, y, rng, o, d);
+  // shutter time (after the filter and lens draws) and camera motion,
+  // pt/mod.rs:351-355, camera.rs:134-146
+  float time = 0.f;
+  if (S.motion) {
+    time = rnd(rng);
+    if (S.cam.has_motion_inv && time > 0.f) {
+      const float* M = S.cam.motion_inv;
+      float m[12];
+#pragma unroll
+      for (int k = 0; k < 12; k++) m[k] = ((k % 5 == 0) ? 1.f : 0.f) + (M[k] - ((k % 5 == 0) ? 1.f : 0.f)) * time;
+      o = xform_point(m, o);
+      d = normalize(xform_vec(m, d));
+    }
+  }
   P.o[slot] = make_float4(o.x, o.y, o.z, 0.f);
   P.d[slot] = make_float4(d.x, d.y, d.z, 0.f);
-  P.beta[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+  P.beta[slot] = make_float4(1.f, 1.f, 1.f, time);
   P.L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
   P.prev[slot] = make_float4(o.x, o.y, o.z, 0.f);
   P.rng[slot] = store_rng(rng);
@@ -144,11 +158,14 @@ __global__ __launch_bounds__(kBlock) void raygen_kernel(PtSceneView S, PathBuffe
   }
 }
 
-// Closest hit of the path in `slot` into hit_n / hit_m.
+// Closest hit of the path in `slot` into hit_n / hit_m (MOTION: at the
+// path's shutter time).
+template <bool MOTION>
 __device__ __forceinline__ void extend_one(const PtSceneView& S, const PathBuffers& P, uint32_t slot, uint32_t* stk) {
   const float4 o4 = P.o[slot], d4 = P.d[slot];
+  const float time = MOTION ? P.beta[slot].w : 0.f;
   Hit h;
-  if (scene_closest<kTraceBlock>(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), stk, h)) {
+  if (scene_closest<kTraceBlock, MOTION>(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), time, stk, h)) {
     P.hit_n[slot] = make_float4(h.n.x, h.n.y, h.n.z, h.t);
     P.hit_m[slot] = h.material | (h.front ? 0x80000000u : 0u);
   } else {
@@ -159,13 +176,13 @@ __device__ __forceinline__ void extend_one(const PtSceneView& S, const PathBuffe
 
 // STACK: LDS entries per lane, the smallest size >= the scene's bound
 // (fewer bytes per block, more resident waves).
-template <int STACK>
+template <int STACK, bool MOTION>
 __global__ __launch_bounds__(kTraceBlock) void extend_kernel(PtSceneView S, PathBuffers P, const uint32_t* __restrict__ q) {
   __shared__ uint32_t s_stack[STACK * kTraceBlock];
   uint32_t* stk = s_stack + threadIdx.x;
   const uint32_t n = P.counts[0];
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) extend_one(S, P, q[i], stk);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) extend_one<MOTION>(S, P, q[i], stk);
 }
 
 // pt/mod.rs:1000-1023
@@ -191,6 +208,7 @@ __device__ __forceinline__ float light_pdf_solid_angle(const PtLight& L, V3 orig
 struct ShadowOut {
   V3 o, d;
   float tmax;
+  float time;  // the path's shutter time
   V3 c;        // contribution before visibility (f cos L weight / pdf)
   int medium;  // transparent scenes: the medium at the ray's origin (the walk applies its transmittance)
   uint2 seed;  // transparent scenes: random stream of the blocker walk
@@ -620,11 +638,13 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
       so.seed = make_uint2(rng.next_u32(), rng.next_u32());
       se.seed = so.seed;
     }
+    so.time = b4.w;
+    se.time = b4.w;
     if (TR) {
       P.mstk[slot] = mst;
       flags = (flags & ~kStackSkipMask) | (sdepth << kStackShift) | (skips << kSkipShift);
     }
-    P.beta[slot] = make_float4(beta.x, beta.y, beta.z, 0.f);
+    P.beta[slot] = make_float4(beta.x, beta.y, beta.z, b4.w);  // w: shutter time
     P.L[slot] = make_float4(Lr.x, Lr.y, Lr.z, 0.f);
     P.rng[slot] = store_rng(rng);
     P.flags[slot] = flags;
@@ -648,6 +668,7 @@ __device__ __forceinline__ void push_shadow(const PathBuffers& P, bool act, uint
   P.sd[idx] = make_float4(f.d.x, f.d.y, f.d.z, walk ? (float)f.medium : 0.f);
   P.sc[idx] = make_float4(f.c.x, f.c.y, f.c.z, both ? 1.f : 0.f);
   if (walk) P.srng[idx] = f.seed;
+  if (P.stime) P.stime[idx] = f.time;
   if (both) {
     P.so2[idx] = make_float4(b.o.x, b.o.y, b.o.z, b.tmax);
     P.sd2[idx] = make_float4(b.d.x, b.d.y, b.d.z, walk ? (float)b.medium : 0.f);
@@ -655,8 +676,10 @@ __device__ __forceinline__ void push_shadow(const PathBuffers& P, bool act, uint
   }
 }
 
+// Three waves per SIMD (<= 168 VGPRs): shading is latency-bound on
+// scattered path-state loads, so occupancy matters more than registers.
 template <bool NRC, bool TR>
-__global__ __launch_bounds__(kBlock) void shade_kernel(PtSceneView S, PathBuffers P, PtParams prm,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void shade_kernel(PtSceneView S, PathBuffers P, PtParams prm,
                                                       const uint32_t* __restrict__ q, uint32_t* __restrict__ qnext,
                                                       NrcPathBuffers N) {
   const uint32_t n = P.counts[0];
@@ -693,15 +716,16 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtSceneView S, PathBuffer
 // transmittance of the medium on top of the stack (the origin's medium
 // over the atmosphere); hulls push / pop their media, cutouts let 1 -
 // presence through, anything else blocks; at most 16 blockers.
-__device__ __forceinline__ V3 walk_visibility(const PtSceneView& S, V3 o, V3 d, float remaining, int medium,
-                                              Pcg32& rng, uint32_t* stk) {
+template <bool MOTION>
+__device__ __forceinline__ V3 walk_visibility(const PtSceneView& S, V3 o, V3 d, float time, float remaining,
+                                              int medium, Pcg32& rng, uint32_t* stk) {
   uint2 st = make_uint2(0u, 0u);
   uint32_t depth = 0;
   if (medium >= 0 && medium != S.atmosphere) stack_push(st, depth, (uint32_t)medium);
   V3 vis = v3(1.f, 1.f, 1.f);
   for (uint32_t i = 0; i < kMaxSkips; i++) {
     Hit h;
-    const bool hit = scene_closest<kTraceBlock>(S, o, d, stk, h);
+    const bool hit = scene_closest<kTraceBlock, MOTION>(S, o, d, time, stk, h);
     const float seg = fminf(hit ? h.t : remaining, remaining);
     const int cur = stack_top(st, depth, S.atmosphere);
     if (cur >= 0) {
@@ -729,33 +753,35 @@ __device__ __forceinline__ V3 walk_visibility(const PtSceneView& S, V3 o, V3 d, 
 // Shadow record i: adds each unoccluded ray's contribution to its path
 // (the first ray, then the second when the record has one); with WALK the
 // rays walk their blockers (transparent scenes), else one any-hit test.
-template <bool WALK>
+template <bool WALK, bool MOTION>
 __device__ __forceinline__ void shadow_one_t(const PtSceneView& S, const PathBuffers& P, uint32_t i, uint32_t* stk) {
   const float4 o = P.so[i], d = P.sd[i];
   float4 c = P.sc[i];
   const bool two = c.w != 0.f;
+  const float time = MOTION ? P.stime[i] : 0.f;
   float4 c2 = make_float4(0.f, 0.f, 0.f, 0.f);
   bool v1, v2 = false;
   if constexpr (WALK) {
     const uint2 sd = P.srng[i];
     const uint64_t seed = ((uint64_t)sd.y << 32) | sd.x;
     Pcg32 r1 = Pcg32::make(seed, 1);
-    const V3 t1 = walk_visibility(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, (int)d.w, r1, stk);
+    const V3 t1 = walk_visibility<MOTION>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), time, o.w, (int)d.w, r1, stk);
     c = make_float4(c.x * t1.x, c.y * t1.y, c.z * t1.z, c.w);
     v1 = maxc(t1) > 0.f;
     if (two) {
       const float4 o2 = P.so2[i], d2 = P.sd2[i];
       Pcg32 r2 = Pcg32::make(seed, 2);
-      const V3 t2 = walk_visibility(S, v3(o2.x, o2.y, o2.z), v3(d2.x, d2.y, d2.z), o2.w, (int)d2.w, r2, stk);
+      const V3 t2 = walk_visibility<MOTION>(S, v3(o2.x, o2.y, o2.z), v3(d2.x, d2.y, d2.z), time, o2.w, (int)d2.w, r2,
+                                    stk);
       c2 = P.sc2[i];
       c2 = make_float4(c2.x * t2.x, c2.y * t2.y, c2.z * t2.z, 0.f);
       v2 = maxc(t2) > 0.f;
     }
   } else {
-    v1 = !scene_occluded<kTraceBlock>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, stk);
+    v1 = !scene_occluded<kTraceBlock, MOTION>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), time, o.w, stk);
     if (two) {
       const float4 o2 = P.so2[i], d2 = P.sd2[i];
-      v2 = !scene_occluded<kTraceBlock>(S, v3(o2.x, o2.y, o2.z), v3(d2.x, d2.y, d2.z), o2.w, stk);
+      v2 = !scene_occluded<kTraceBlock, MOTION>(S, v3(o2.x, o2.y, o2.z), v3(d2.x, d2.y, d2.z), time, o2.w, stk);
       c2 = P.sc2[i];
     }
   }
@@ -777,18 +803,44 @@ __device__ __forceinline__ void shadow_one_t(const PtSceneView& S, const PathBuf
 
 __device__ __forceinline__ void shadow_one(const PtSceneView& S, const PathBuffers& P, uint32_t i, uint32_t* stk) {
   if (S.transparent)
-    shadow_one_t<true>(S, P, i, stk);
+    S.motion ? shadow_one_t<true, true>(S, P, i, stk) : shadow_one_t<true, false>(S, P, i, stk);
   else
-    shadow_one_t<false>(S, P, i, stk);
+    S.motion ? shadow_one_t<false, true>(S, P, i, stk) : shadow_one_t<false, false>(S, P, i, stk);
 }
 
-template <int STACK, bool WALK>
+template <int STACK, bool WALK, bool MOTION>
 __global__ __launch_bounds__(kTraceBlock) void shadow_kernel(PtSceneView S, PathBuffers P) {
   __shared__ uint32_t s_stack[STACK * kTraceBlock];
   uint32_t* stk = s_stack + threadIdx.x;
   const uint32_t n = P.counts[2];
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) shadow_one_t<WALK>(S, P, i, stk);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    shadow_one_t<WALK, MOTION>(S, P, i, stk);
+}
+
+// Launch of extend / shadow for the scene's stack size, transparency and motion.
+template <int STACK, bool MOTION>
+void launch_trace_kernels(const PtSceneView& S, const PathBuffers& P, const uint32_t* q, uint32_t gt, bool extend,
+                          hipStream_t st) {
+  if (extend)
+    hipLaunchKernelGGL((extend_kernel<STACK, MOTION>), dim3(gt), dim3(kTraceBlock), 0, st, S, P, q);
+  else if (S.transparent)
+    hipLaunchKernelGGL((shadow_kernel<STACK, true, MOTION>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
+  else
+    hipLaunchKernelGGL((shadow_kernel<STACK, false, MOTION>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
+}
+
+void launch_trace(const PtSceneView& S, const PathBuffers& P, const uint32_t* q, uint32_t gt, bool extend,
+                  hipStream_t st) {
+  const bool small = S.stack_need <= 32;
+  if (small && S.motion)
+    launch_trace_kernels<32, true>(S, P, q, gt, extend, st);
+  else if (small)
+    launch_trace_kernels<32, false>(S, P, q, gt, extend, st);
+  else if (S.motion)
+    launch_trace_kernels<kTraceStack, true>(S, P, q, gt, extend, st);
+  else
+    launch_trace_kernels<kTraceStack, false>(S, P, q, gt, extend, st);
 }
 
 __global__ void rotate_kernel(PathBuffers P) {
@@ -879,7 +931,11 @@ __global__ __launch_bounds__(kTraceBlock) void extend_batch_kernel(const PtJob* 
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const uint32_t slot = q[i];
-    extend_one(jobs[P.job[slot]].S, P, slot, stk);
+    const PtSceneView& S = jobs[P.job[slot]].S;
+    if (S.motion)
+      extend_one<true>(S, P, slot, stk);
+    else
+      extend_one<false>(S, P, slot, stk);
   }
 }
 
@@ -1022,11 +1078,7 @@ hipError_t pt_launch_bounce(const PtSceneView& S, const PathBuffers& P, const Pt
                             uint32_t* qnext, uint32_t capacity, const NrcPathBuffers& N, hipStream_t st) {
   const uint32_t g = grid_for(capacity);
   const uint32_t gt = grid_for(capacity, kTraceBlock);
-  const bool small = S.stack_need <= 32;
-  if (small)
-    hipLaunchKernelGGL(extend_kernel<32>, dim3(gt), dim3(kTraceBlock), 0, st, S, P, q);
-  else
-    hipLaunchKernelGGL(extend_kernel<kTraceStack>, dim3(gt), dim3(kTraceBlock), 0, st, S, P, q);
+  launch_trace(S, P, q, gt, true, st);
   const uint32_t* qshade = q;
   if (P.qs) {  // group the queue by shading class so shade waves run one BSDF kind
     hipLaunchKernelGGL(sort_count_kernel, dim3(kSortBlocks), dim3(kBlock), 0, st, S, P, q);
@@ -1040,16 +1092,7 @@ hipError_t pt_launch_bounce(const PtSceneView& S, const PathBuffers& P, const Pt
     hipLaunchKernelGGL((shade_kernel<false, true>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
   else
     hipLaunchKernelGGL((shade_kernel<false, false>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
-  if (S.transparent) {
-    if (small)
-      hipLaunchKernelGGL((shadow_kernel<32, true>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
-    else
-      hipLaunchKernelGGL((shadow_kernel<kTraceStack, true>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
-  } else if (small) {
-    hipLaunchKernelGGL((shadow_kernel<32, false>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
-  } else {
-    hipLaunchKernelGGL((shadow_kernel<kTraceStack, false>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
-  }
+  launch_trace(S, P, nullptr, gt, false, st);
   hipLaunchKernelGGL(rotate_kernel, dim3(1), dim3(1), 0, st, P);
   return hipGetLastError();
 }

@@ -240,18 +240,34 @@ bool build_instances(const rrhip_scene_desc& d, PtSceneHost* out, std::string* e
     for (int k = 0; k < 12; k++) g.inv[k] = (float)inv[k];
     g.mesh = in.mesh;
     g.material = in.material;
+    g.motion = 0xFFFFFFFFu;
+    if (in.moving) {
+      for (int k = 0; k < 12; k++)
+        if (!std::isfinite(in.object_to_world1[k])) return *err = "instance transform1 must be finite", false;
+      PtInstMotion mo;
+      for (int k = 0; k < 12; k++) {
+        mo.fwd0[k] = (float)in.object_to_world[k];
+        mo.fwd1[k] = (float)in.object_to_world1[k];
+      }
+      g.motion = (uint32_t)out->inst_motion.size();
+      out->inst_motion.push_back(mo);
+      out->motion = 1;
+    }
+    // mesh.rs:274-288: world bounds from the 8 transformed corners, at both
+    // shutter ends when moving (corners move linearly in between)
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     const auto& lb = local[in.mesh];
     const bool empty = d.meshes[in.mesh].num_triangles == 0;
-    for (int c = 0; c < 8 && !empty; c++) {
-      const double p[3] = {(c & 1) ? lb[3] : lb[0], (c & 2) ? lb[4] : lb[1], (c & 4) ? lb[5] : lb[2]};
-      for (int r = 0; r < 3; r++) {
-        const double* M = in.object_to_world + 4 * r;
-        const double w = M[0] * p[0] + M[1] * p[1] + M[2] * p[2] + M[3];
-        lo[r] = std::min(lo[r], w);
-        hi[r] = std::max(hi[r], w);
+    for (int e = 0; e < (in.moving ? 2 : 1); e++)
+      for (int c = 0; c < 8 && !empty; c++) {
+        const double p[3] = {(c & 1) ? lb[3] : lb[0], (c & 2) ? lb[4] : lb[1], (c & 4) ? lb[5] : lb[2]};
+        for (int r = 0; r < 3; r++) {
+          const double* M = (e ? in.object_to_world1 : in.object_to_world) + 4 * r;
+          const double w = M[0] * p[0] + M[1] * p[1] + M[2] * p[2] + M[3];
+          lo[r] = std::min(lo[r], w);
+          hi[r] = std::max(hi[r], w);
+        }
       }
-    }
     for (int r = 0; r < 3; r++) {
       if (empty) lo[r] = hi[r] = 0.0;
       // Round outward so f32 boxes still contain the f64 corners.
@@ -527,6 +543,12 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
   cam.focal_distance = (float)c.focal_distance;
   cam.ortho_w = (float)c.ortho_half[0];
   cam.ortho_h = (float)c.ortho_half[1];
+  cam.has_motion_inv = c.has_motion_inv ? 1u : 0u;
+  for (int k = 0; k < 12; k++) {
+    if (c.has_motion_inv && !std::isfinite(c.motion_inv[k])) return *err = "camera motion_inv must be finite", false;
+    cam.motion_inv[k] = (float)c.motion_inv[k];
+  }
+  if (cam.has_motion_inv) out->motion = 1;
   return true;
 }
 

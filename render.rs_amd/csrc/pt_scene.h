@@ -30,6 +30,8 @@ struct PtSceneHost {
   std::vector<float> mesh_nrm;
   uint32_t stack_need = 0;  // traversal stack entries the instance path needs
   uint32_t transparent = 0; // a material has an interior medium or presence < 1
+  uint32_t motion = 0;      // moving instances or camera motion
+  std::vector<PtInstMotion> inst_motion;
   std::vector<PtMedium> media;                // grid pointers filled at upload
   std::vector<std::vector<float>> media_grids;
   int32_t atmosphere = -1;

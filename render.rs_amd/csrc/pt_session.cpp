@@ -129,6 +129,7 @@ hipError_t alloc_paths(rrhip_session* s, uint64_t n) {
     e = alloc_in(m, &P.mstk, n);
     if (e == hipSuccess) e = alloc_in(m, &P.srng, n);
   }
+  if (e == hipSuccess && s->scene->pt.motion) e = alloc_in(m, &P.stime, n);
   if (e == hipSuccess) e = alloc_in(m, &P.counts, 4);
   if (e == hipSuccess) s->path_cap = n;
   return e;

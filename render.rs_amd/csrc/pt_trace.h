@@ -81,10 +81,49 @@ __device__ __forceinline__ bool mesh_tri(const TriRec& tr, const Ray32& r, float
   return t > 1e-9f && t < tmax;
 }
 
-// Closest (ANY = false) or any (ANY = true) instance hit with t < tmax.
-template <int STRIDE, bool ANY>
-__device__ __forceinline__ bool trace_instances(const PtSceneView& S, V3 o, V3 d, float tmax, uint32_t* stk,
-                                                MeshHit& hit) {
+// World-to-object transform of instance I at shutter time `time`
+// (mesh.rs:294-305): its stored inverse, or for a moving instance the
+// inverse of the element-wise lerp of its two transforms (identity when
+// singular, as Matrix4::inverse().unwrap_or).
+__device__ __forceinline__ void instance_inverse(const PtSceneView& S, const PtInstance& I, float time, float inv[12]) {
+  if (I.motion == 0xFFFFFFFFu || !(time > 0.f)) {
+#pragma unroll
+    for (int k = 0; k < 12; k++) inv[k] = I.inv[k];
+    return;
+  }
+  const PtInstMotion& M = S.inst_motion[I.motion];
+  float m[12];
+#pragma unroll
+  for (int k = 0; k < 12; k++) m[k] = M.fwd0[k] + (M.fwd1[k] - M.fwd0[k]) * time;
+  const float a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], i = m[10];
+  const float A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
+  const float det = a * A + b * B + c * C;
+  if (!(fabsf(det) > 1e-30f)) {
+#pragma unroll
+    for (int k = 0; k < 12; k++) inv[k] = (k % 5 == 0) ? 1.f : 0.f;
+    return;
+  }
+  const float r = 1.f / det;
+  inv[0] = A * r;
+  inv[1] = -(b * i - c * h) * r;
+  inv[2] = (b * f - c * e) * r;
+  inv[4] = B * r;
+  inv[5] = (a * i - c * g) * r;
+  inv[6] = -(a * f - c * d) * r;
+  inv[8] = C * r;
+  inv[9] = -(a * h - b * g) * r;
+  inv[10] = (a * e - b * d) * r;
+#pragma unroll
+  for (int row = 0; row < 3; row++)
+    inv[4 * row + 3] = -(inv[4 * row] * m[3] + inv[4 * row + 1] * m[7] + inv[4 * row + 2] * m[11]);
+}
+
+// Closest (ANY = false) or any (ANY = true) instance hit with t < tmax, for
+// a ray at shutter time `time` (MOTION: the scene has moving instances;
+// without it the time is ignored and the code is the static traversal).
+template <int STRIDE, bool ANY, bool MOTION>
+__device__ __forceinline__ bool trace_instances(const PtSceneView& S, V3 o, V3 d, float time, float tmax,
+                                                uint32_t* stk, MeshHit& hit) {
   if (S.num_insts == 0) return false;
   const Ray32 world = make_ray32(o, d);
   Ray32 r = world;
@@ -149,7 +188,13 @@ __device__ __forceinline__ bool trace_instances(const PtSceneView& S, V3 o, V3 d
       const PtInstance I = S.insts[inst];
       const uint32_t root = S.meshes[I.mesh].root;
       if (root == 0xFFFFFFFFu) continue;
-      r = make_ray32(xform_point(I.inv, o), xform_vec(I.inv, d));
+      if (!MOTION || I.motion == 0xFFFFFFFFu) {
+        r = make_ray32(xform_point(I.inv, o), xform_vec(I.inv, d));
+      } else {
+        float inv[12];
+        instance_inverse(S, I, time, inv);
+        r = make_ray32(xform_point(inv, o), xform_vec(inv, d));
+      }
       base = sp;
       node = root;
     } else {
@@ -164,7 +209,8 @@ __device__ __forceinline__ bool trace_instances(const PtSceneView& S, V3 o, V3 d
 // 427-439): interpolated or geometric local normal to world space through
 // the inverse transpose, flipped to face the ray; front_face from the
 // unflipped normal.
-__device__ __forceinline__ void instance_normal(const PtSceneView& S, const MeshHit& h, V3 d, V3& n,
+template <bool MOTION>
+__device__ __forceinline__ void instance_normal(const PtSceneView& S, const MeshHit& h, V3 d, float time, V3& n,
                                                 bool& front, uint32_t& material) {
   const PtInstance I = S.insts[h.inst];
   const TriRec tr = S.btris[h.slot];
@@ -179,30 +225,38 @@ __device__ __forceinline__ void instance_normal(const PtSceneView& S, const Mesh
     const V3 v0 = v3(tr.v0x, tr.v0y, tr.v0z);
     ln = normalize(cross(v3(tr.v1x, tr.v1y, tr.v1z) - v0, v3(tr.v2x, tr.v2y, tr.v2z) - v0));
   }
-  n = normalize(xform_normal(I.inv, ln));
+  if (!MOTION || I.motion == 0xFFFFFFFFu) {
+    n = normalize(xform_normal(I.inv, ln));
+  } else {
+    float inv[12];
+    instance_inverse(S, I, time, inv);
+    n = normalize(xform_normal(inv, ln));
+  }
   front = dot(n, d) < 0.f;
   if (!front) n = -n;
   material = I.material;
 }
 
 // Closest hit over loose objects then instances.
-template <int STRIDE>
-__device__ __forceinline__ bool scene_closest(const PtSceneView& S, V3 o, V3 d, uint32_t* stk, Hit& best) {
+template <int STRIDE, bool MOTION>
+__device__ __forceinline__ bool scene_closest(const PtSceneView& S, V3 o, V3 d, float time, uint32_t* stk,
+                                              Hit& best) {
   bool found = closest_hit(S, o, d, best);
   MeshHit mh;
-  if (trace_instances<STRIDE, false>(S, o, d, found ? best.t : INFINITY, stk, mh)) {
+  if (trace_instances<STRIDE, false, MOTION>(S, o, d, time, found ? best.t : INFINITY, stk, mh)) {
     best.t = mh.t;
-    instance_normal(S, mh, d, best.n, best.front, best.material);
+    instance_normal<MOTION>(S, mh, d, time, best.n, best.front, best.material);
     found = true;
   }
   return found;
 }
 
-template <int STRIDE>
-__device__ __forceinline__ bool scene_occluded(const PtSceneView& S, V3 o, V3 d, float tmax, uint32_t* stk) {
+template <int STRIDE, bool MOTION>
+__device__ __forceinline__ bool scene_occluded(const PtSceneView& S, V3 o, V3 d, float time, float tmax,
+                                               uint32_t* stk) {
   if (occluded(S, o, d, tmax)) return true;
   MeshHit mh;
-  return trace_instances<STRIDE, true>(S, o, d, tmax, stk, mh);
+  return trace_instances<STRIDE, true, MOTION>(S, o, d, time, tmax, stk, mh);
 }
 
 }  // namespace pt

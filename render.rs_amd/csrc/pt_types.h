@@ -84,7 +84,15 @@ struct alignas(16) PtInstance {
   float inv[12];
   uint32_t mesh;
   uint32_t material;
-  uint32_t pad[2];
+  uint32_t motion;  // index of its PtInstMotion, ~0u: static
+  uint32_t pad;
+};
+
+// A moving instance (Instance::transform, transform1): object-to-world at
+// shutter open and close; rays at time t use the inverse of their lerp.
+struct alignas(16) PtInstMotion {
+  float fwd0[12];
+  float fwd1[12];
 };
 
 // A participating medium (medium.rs:28-65) with its density as a vertex
@@ -122,6 +130,8 @@ struct PtCamera {
   uint32_t projection;          // 0 perspective, 1 orthographic
   float lens_radius, focal_distance;
   float ortho_w, ortho_h;       // orthographic screen-window half extents
+  uint32_t has_motion_inv;      // camera motion: rays at time t mapped by lerp(I, motion_inv, t)
+  float motion_inv[12];
 };
 
 // Read-only scene view passed to the kernels by value.
@@ -147,6 +157,8 @@ struct PtSceneView {
   uint32_t stack_need;          // traversal stack entries a lane can need
   uint32_t transparent;         // some material has an interior medium or presence < 1: paths keep
                                 // a medium stack, shadow rays walk their blockers
+  uint32_t motion;              // moving instances or camera: rays carry a shutter time
+  const PtInstMotion* inst_motion;
   uint32_t num_insts;
   uint32_t num_tris, num_spheres, num_lights, num_lnodes, num_inf;
   int32_t dome;
@@ -177,6 +189,7 @@ struct PathBuffers {
   float4* sc2;
   uint2* mstk;      // transparent scenes: each path's medium stack (8 x 8-bit medium indices)
   uint2* srng;      // transparent scenes: 64-bit seed of each shadow record's blocker walk
+  float* stime;     // scenes with motion: shutter time of each shadow record (paths keep theirs in beta.w)
   uint32_t* counts; // [0] live paths, [1] next live, [2] shadow rays, [3] unused
   unsigned long long* stats;  // [0] extension rays, [1] shadow rays, [2] cache queries, [3] training samples
   uint32_t* job;    // batches: job of each slot (null in sessions)

@@ -68,7 +68,8 @@ D3 = ctypes.c_double * 3
 
 class InstanceDesc(ctypes.Structure):
     _fields_ = [("mesh", ctypes.c_uint32), ("material", ctypes.c_uint32),
-                ("object_to_world", ctypes.c_double * 12)]
+                ("object_to_world", ctypes.c_double * 12), ("object_to_world1", ctypes.c_double * 12),
+                ("moving", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class TriangleDesc(ctypes.Structure):
@@ -106,7 +107,8 @@ class CameraDesc(ctypes.Structure):
                 ("fov_deg", ctypes.c_double), ("filter_width", ctypes.c_double),
                 ("eye", D3), ("look_at", D3), ("up", D3),
                 ("lens_radius", ctypes.c_double), ("focal_distance", ctypes.c_double),
-                ("ortho_half", ctypes.c_double * 2)]
+                ("ortho_half", ctypes.c_double * 2), ("motion_inv", ctypes.c_double * 12),
+                ("has_motion_inv", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class MediumDesc(ctypes.Structure):

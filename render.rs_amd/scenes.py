@@ -138,11 +138,17 @@ class Scene:
                                           None if nrm is None else nrm.ctypes.data))
         return len(self.meshes) - 1
 
-    def add_instance(self, mesh, mat, object_to_world_3x4):
+    def add_instance(self, mesh, mat, object_to_world_3x4, object_to_world1_3x4=None):
+        """An instance; with object_to_world1_3x4 it moves from the first
+        transform at shutter open to the second at close."""
         i = rrhip.InstanceDesc()
         i.mesh = mesh
         i.material = mat
         i.object_to_world = (ctypes.c_double * 12)(*[float(x) for x in np.asarray(object_to_world_3x4).ravel()])
+        if object_to_world1_3x4 is not None:
+            i.object_to_world1 = (ctypes.c_double * 12)(*[float(x) for x in
+                                                          np.asarray(object_to_world1_3x4).ravel()])
+            i.moving = 1
         self.instances.append(i)
 
     def add_medium(self, sigma_a, sigma_s, g=0.0, emission=(0, 0, 0), max_distance=0.0, density=None,

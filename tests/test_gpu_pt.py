@@ -7,6 +7,7 @@ fraction bounded by the heavy tails of 3 sigma at the sample counts used
 (albedo-1 diffuse sphere in a unit dome) equal to 1 within 1e-3; renders
 bit-reproducible.
 """
+import ctypes
 import sys
 
 import numpy as np
@@ -314,3 +315,46 @@ def test_thin_lens_and_orthographic_cameras_match_oracle(ctx):
     s4.camera.projection = 1  # ortho_half 0
     with pytest.raises(rrhip.RrhipError):
         rrhip.PtScene(ctx, s4.desc())
+
+
+@pytest.mark.parametrize("camera_motion", [False, True])
+def test_motion_blur_matches_shutter_fractions(ctx, camera_motion):
+    """Moving instance (mesh.rs:294-305) or moving camera (camera.rs:
+    134-146): each pixel sees the dome for 1 - the fraction of the shutter
+    the sliding black quad covers it."""
+    from test_oracle_pt import covered_fraction, moving_quad_scene
+    say(f"motion blur ({'camera' if camera_motion else 'instance'}): 32x4, 4096 spp vs shutter fractions")
+    W = 32
+    s = moving_quad_scene(W, 4, camera_motion)
+    acc, _ = gpu_render(ctx, s, rrhip.RenderParams.make(max_depth=2, firefly_clamp=0.0), 0, 4096)
+    m = (acc[:, 0] / acc[:, 6]).reshape(4, W)
+    x = ((np.arange(W) + 0.5) / W * 2 - 1) * 2.0
+    want = 1 - covered_fraction(-x if camera_motion else x)
+    se = np.sqrt(np.maximum(want * (1 - want), 1e-12) / (2 * 4096))
+    got = m[1:3].mean(0)
+    say(f"max |got - want| {np.abs(got - want).max():.4f}, max z {(np.abs(got - want) / np.maximum(se, 1e-9)).max():.2f}")
+    assert np.array_equal(m[0], np.ones(W)) and np.array_equal(m[3], np.ones(W))
+    assert np.all(np.abs(got - want) < 4.5 * se + 1e-6)
+
+
+def test_moving_instances_match_oracle(ctx):
+    """A small instanced field where every third instance moves (a
+    translation plus a different rotation at shutter close), against the
+    oracle."""
+    say("instanced field 96x54 with moving instances, 128 spp vs oracle")
+    s = scenes.instanced_field(96, 54, n_instances=300, levels=3, n_point_lights=16)
+    rng = np.random.default_rng(5)
+    for k, inst in enumerate(s.instances):
+        if k % 3:
+            continue
+        m0 = np.array(list(inst.object_to_world), float).reshape(3, 4)
+        m1 = m0.copy()
+        m1[:, 3] += rng.uniform(-3, 3, 3)
+        c, sn = np.cos(0.3), np.sin(0.3)
+        m1[:, :3] = m1[:, :3] @ np.array([[c, 0, sn], [0, 1, 0], [-sn, 0, c]])
+        inst.object_to_world1 = (ctypes.c_double * 12)(*m1.ravel())
+        inst.moving = 1
+    p = rrhip.RenderParams.make(max_depth=12)
+    acc_g, _ = gpu_render(ctx, s, p, 0, 128)
+    acc_c, _ = orc.render(s.desc(), p, 0, 128)
+    check_parity(acc_g, acc_c, label="moving instances")

@@ -7,6 +7,7 @@ determinism, convergence), plus closed forms (Lambert f = rho/pi)."""
 import ctypes
 
 import numpy as np
+import pytest
 
 import orc
 import rrhip
@@ -270,3 +271,58 @@ def test_orthographic_rays_are_parallel_on_the_screen_window():
     v = 1 - (xy[:, 1] + 0.5) / 8 * 2
     want = np.stack([u * 0.8, v * 0.4, np.full(len(u), -0.9)], 1)
     assert np.allclose(o, want, atol=1e-9)
+
+
+def moving_quad_scene(w, h, camera_motion=False):
+    """An orthographic camera (window 2 x 1) looking down +z at a black
+    unit quad instance at z = 2 whose centre slides from x = -1 (shutter
+    open) to x = +1 (close) — or, with camera_motion, a static quad at the
+    origin seen by rays that camera motion shifts by the same amount —
+    against a unit dome: pixel x = X is covered during a known fraction of
+    the shutter."""
+    s = scenes.Scene(w, h, 40.0, eye=(0, 0, 0), look_at=(0, 0, 1))
+    s.camera.projection = 1
+    s.camera.ortho_half = (ctypes.c_double * 2)(2.0, 1.0)
+    s.camera.filter_width = 1e-9
+    s.add_light(scenes.light(rrhip.LIGHT_DOME, intensity=1.0))
+    black = s.add_material(scenes.material(diffuse_gain=0.0, specular_ior=1.0))
+    quad = s.add_mesh([(-0.5, -0.5, 0), (0.5, -0.5, 0), (0.5, 0.5, 0), (-0.5, 0.5, 0)], [(0, 1, 2), (0, 2, 3)])
+    if camera_motion:
+        s.add_instance(quad, black, [[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 2]])
+        # camera at x = -1: the quad sits at image x = +1 at t = 0; rays at
+        # time t move by lerp(I, M, t), M a shift of +2 in x, so the quad
+        # slides to image x = -1 at t = 1 (covered_fraction mirrored)
+        m = np.array([[1, 0, 0, 2.0], [0, 1, 0, 0], [0, 0, 1, 0]])
+        s.camera.motion_inv = (ctypes.c_double * 12)(*m.ravel())
+        s.camera.has_motion_inv = 1
+        s.camera.eye = (ctypes.c_double * 3)(-1.0, 0, 0)
+        s.camera.look_at = (ctypes.c_double * 3)(-1.0, 0, 1)
+    else:
+        s.add_instance(quad, black, [[1, 0, 0, -1], [0, 1, 0, 0], [0, 0, 1, 2]],
+                       [[1, 0, 0, 1], [0, 1, 0, 0], [0, 0, 1, 2]])
+    return s
+
+
+def covered_fraction(x):
+    """The quad's centre at time t is -1 + 2t: it covers x for t in
+    [(x + 0.5) / 2, (x + 1.5) / 2] within [0, 1]."""
+    lo = np.clip((x + 0.5) / 2, 0, 1)
+    hi = np.clip((x + 1.5) / 2, 0, 1)
+    return hi - lo
+
+
+@pytest.mark.parametrize("camera_motion", [False, True])
+def test_motion_blur_covers_pixels_for_their_shutter_fraction(camera_motion):
+    """mesh.rs:294-305 (instance transform lerped at the ray's shutter
+    time) and camera.rs:134-146 (camera motion): each pixel of a row sees
+    the dome for 1 - (fraction of the shutter the quad covers it)."""
+    W = 32
+    s = moving_quad_scene(W, 4, camera_motion)
+    acc, _ = orc.render(s.desc(), rrhip.RenderParams.make(max_depth=2, firefly_clamp=0.0), 0, 4096)
+    m = (acc[:, 0] / acc[:, 6]).reshape(4, W)
+    x = ((np.arange(W) + 0.5) / W * 2 - 1) * 2.0
+    want = 1 - covered_fraction(-x if camera_motion else x)
+    se = np.sqrt(np.maximum(want * (1 - want), 1e-12) / (2 * 4096))
+    got = m[1:3].mean(0)  # rows at y = -0.25, 0.25 (the quad spans |y| <= 0.5)
+    assert np.array_equal(m[0], np.ones(W)) and np.array_equal(m[3], np.ones(W))
+    assert np.all(np.abs(got - want) < 4.5 * se + 1e-9), np.c_[x, got, want]
