@@ -312,7 +312,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tuning", default=None,
-                    help="block,stack,rays_per_wave instead of the autotuner (profiling runs)")
+                    help="block,stack,rays_per_wave[,width] instead of the autotuner (profiling runs)")
     ap.add_argument("--no-cost", action="store_true",
                     help="skip the counting pass (profiling runs)")
     ap.add_argument("--no-pt", action="store_true", help="skip the box-room path-tracing line")
@@ -345,8 +345,8 @@ def main():
     # Launch configuration from the autotuner (setup, not timed); any-hit
     # reuses the closest-hit choice.
     if args.tuning:
-        b, st, rpw = (int(x) for x in args.tuning.split(","))
-        scene.set_tuning(b, st, rpw)
+        vals = [int(x) for x in args.tuning.split(",")]
+        scene.set_tuning(*vals)  # block, stack, rays_per_wave[, width]
         tuning = scene.get_tuning()
     else:
         tuning, _ = scene.autotune(d_rays, N_RAYS, d_hits)
@@ -378,7 +378,8 @@ def main():
         nodes, tris = 0, 0
     else:
         nodes, tris = scene.trace_cost(rays)
-    alg_bytes = N_RAYS * (32 + 16) + 64 * nodes + 48 * tris
+    node_bytes = 128 if tuning.get("width", 2) == 4 else 64  # Node4 or Node2 per visit
+    alg_bytes = N_RAYS * (32 + 16) + node_bytes * nodes + 48 * tris
     achieved = alg_bytes / (kernel_ms / 1e3) / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
@@ -403,14 +404,15 @@ def main():
             "workload": "ray query: 1,048,576-triangle fBm heightfield, 4,194,304 rays, closest hit",
             "rays": N_RAYS,
             "triangles": int(idx.shape[0]),
-            "bvh": f"binned SAH 32 bins, leaf<=4, {info['num_nodes']} x 64B nodes, depth {info['depth']}",
+            "bvh": f"binned SAH 32 bins, leaf<=4, {info['num_nodes']} x 64B nodes, depth {info['depth']} "
+                   f"(collapsed four-wide: {info['depth4']} levels of 128B nodes)",
             "parallelism": "single GPU",
             "tuning": tuning,
         },
         "kernel_ms": round(kernel_ms, 4),
         "any_hit_mrays_s": round(N_RAYS / (any_ms / 1e3) / 1e6, 2),
         "host_buffers_mrays_s": round(host_mrays, 1),
-        "traversal": {"node_fetches_per_ray": round(nodes / N_RAYS, 3),
+        "traversal": {"node_bytes": node_bytes, "node_fetches_per_ray": round(nodes / N_RAYS, 3),
                       "triangle_fetches_per_ray": round(tris / N_RAYS, 3),
                       "algorithmic_bytes_per_ray": round(alg_bytes / N_RAYS, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -269,7 +269,7 @@ typedef struct rrhip_scene_info {
   uint32_t depth;         /* node levels on the longest root-to-leaf path */
   uint32_t node_bytes;    /* 64 */
   uint32_t tri_bytes;     /* 48 */
-  uint32_t reserved;
+  uint32_t depth4;        /* node levels of the four-wide tree (128-byte nodes); 0: not built */
   double build_ms;        /* host build + flatten time */
   float bounds_lo[3];
   float bounds_hi[3];
@@ -284,7 +284,7 @@ typedef struct rrhip_trace_tuning {
   uint32_t block;          /* 64, 128 or 256 threads */
   uint32_t stack;          /* 16, 24, 32, 48 or 64 entries */
   uint32_t rays_per_wave;  /* multiple of 64, <= 2^20 */
-  uint32_t reserved;
+  uint32_t width;          /* tree width: 2 (64-byte child-pair nodes) or 4 (128-byte nodes); 0 = 2 */
 } rrhip_trace_tuning;
 
 /* ---- errors / version --------------------------------------------------- */

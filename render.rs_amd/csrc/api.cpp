@@ -396,7 +396,16 @@ int rrhip_scene_upload(rrhip_ctx* ctx, const rrhip_scene_desc* desc, rrhip_scene
   s->ctx = ctx;
   s->has_query = is_query(desc);
   hipError_t e = hipSuccess;
-  if (!built.nodes.empty()) {
+  std::vector<Node4> nodes4;
+  int depth4 = 0;
+  collapse_bvh4(built.nodes, &nodes4, &depth4);
+  if (depth4 + 1 > kMaxStack || nodes4.size() > kNode4Mask) nodes4.clear();  // beyond the four-wide stack entries
+  if (!nodes4.empty()) {
+    e = hipMalloc((void**)&s->d_nodes4, nodes4.size() * sizeof(Node4));
+    if (e == hipSuccess)
+      e = hipMemcpy(s->d_nodes4, nodes4.data(), nodes4.size() * sizeof(Node4), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess && !built.nodes.empty()) {
     e = hipMalloc((void**)&s->d_nodes, built.nodes.size() * sizeof(Node2));
     if (e == hipSuccess) e = hipMalloc((void**)&s->d_tris, built.tris.size() * sizeof(TriRec));
     if (e == hipSuccess)
@@ -417,7 +426,12 @@ int rrhip_scene_upload(rrhip_ctx* ctx, const rrhip_scene_desc* desc, rrhip_scene
   s->bvh.num_nodes = built.nodes.size();
   s->bvh.num_tris = built.tris.size();
   s->bvh.depth = (int)info.depth;
+  s->bvh.nodes4 = s->d_nodes4;
+  s->bvh.num_nodes4 = nodes4.size();
+  s->bvh.depth4 = depth4;
   s->tuning.stack = (uint32_t)std::max(16, stack_for_depth((int)info.depth));
+  info.depth4 = nodes4.empty() ? 0u : (uint32_t)(depth4 + 1);
+  info.device_bytes += nodes4.size() * sizeof(Node4);
   s->info = info;
   *out = s;
   return RRHIP_OK;
@@ -429,6 +443,7 @@ int rrhip_scene_destroy(rrhip_scene* s) {
   (void)hipStreamSynchronize(s->ctx->stream);
   if (s->d_nodes) (void)hipFree(s->d_nodes);
   if (s->d_tris) (void)hipFree(s->d_tris);
+  if (s->d_nodes4) (void)hipFree(s->d_nodes4);
   if (s->d_in) (void)hipFree(s->d_in);
   if (s->d_out) (void)hipFree(s->d_out);
   for (void* p : s->media_mem) (void)hipFree(p);
@@ -446,7 +461,7 @@ int rrhip_scene_get_info(rrhip_scene* s, rrhip_scene_info* info) {
 
 int rrhip_scene_get_tuning(rrhip_scene* s, rrhip_trace_tuning* t) {
   if (!s || !t) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_scene_get_tuning: null argument");
-  *t = rrhip_trace_tuning{s->tuning.block, s->tuning.stack, s->tuning.chunk, 0};
+  *t = rrhip_trace_tuning{s->tuning.block, s->tuning.stack, s->tuning.chunk, s->tuning.width};
   return RRHIP_OK;
 }
 
@@ -456,10 +471,11 @@ int rrhip_scene_set_tuning(rrhip_scene* s, const rrhip_trace_tuning* t) {
   tu.block = t->block;
   tu.stack = t->stack;
   tu.chunk = t->rays_per_wave;
-  if (!tuning_valid(tu, s->bvh.depth))
+  tu.width = t->width ? t->width : 2;
+  if (!tuning_valid(tu, s->bvh))
     return fail(RRHIP_ERR_INVALID_ARGUMENT,
-                "rrhip_scene_set_tuning: unsupported block/stack/rays_per_wave for this scene (depth " +
-                    std::to_string(s->bvh.depth) + ")");
+                "rrhip_scene_set_tuning: unsupported block/stack/rays_per_wave/width for this scene (stack need " +
+                    std::to_string(stack_need(s->bvh, tu.width)) + ")");
   s->tuning = tu;
   return RRHIP_OK;
 }
@@ -468,7 +484,7 @@ int rrhip_scene_autotune(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, int 
                          rrhip_trace_tuning* best, float* best_ms) {
   if (int st = check_scene(s, rays, n, out)) return st;
   if (n == 0 || s->bvh.num_nodes == 0) {
-    if (best) *best = rrhip_trace_tuning{s->tuning.block, s->tuning.stack, s->tuning.chunk, 0};
+    if (best) *best = rrhip_trace_tuning{s->tuning.block, s->tuning.stack, s->tuning.chunk, s->tuning.width};
     if (best_ms) *best_ms = 0.f;
     return RRHIP_OK;
   }
@@ -477,11 +493,13 @@ int rrhip_scene_autotune(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, int 
   const TraceMode mode = any ? TraceMode::kAny : TraceMode::kClosest;
   HitRec* h = any ? nullptr : (HitRec*)out;
   uint8_t* o = any ? (uint8_t*)out : nullptr;
-  const int s0 = stack_for_depth(s->bvh.depth);
   TraceTuning chosen = s->tuning;
   float chosen_ms = 0.f;
   bool have = false;
-  for (uint32_t block : {64u, 128u, 256u}) {
+  for (uint32_t width : {2u, 4u}) {
+   const int s0 = stack_for_depth(stack_need(s->bvh, width));
+   if (s0 == 0) continue;
+   for (uint32_t block : {64u, 128u, 256u}) {
     for (int stack : {16, 24, 32, 48, 64}) {
       if (stack < s0 || stack > std::max(s0, 32)) continue;  // the smallest fitting sizes only
       for (uint32_t chunk : {64u, 128u, 256u, 512u, 1024u, 4096u}) {
@@ -489,7 +507,8 @@ int rrhip_scene_autotune(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, int 
         tu.block = block;
         tu.stack = (uint32_t)stack;
         tu.chunk = chunk;
-        if (!tuning_valid(tu, s->bvh.depth)) continue;
+        tu.width = width;
+        if (!tuning_valid(tu, s->bvh)) continue;
         RRHIP_HIP_TRY(launch_trace(s->bvh, (const RayRec*)rays, n, mode, h, o, nullptr, tu, st));
         RRHIP_HIP_TRY(hipEventRecord(s->ctx->ev0, st));
         const int reps = 3;
@@ -507,9 +526,10 @@ int rrhip_scene_autotune(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, int 
         }
       }
     }
+   }
   }
   s->tuning = chosen;
-  if (best) *best = rrhip_trace_tuning{chosen.block, chosen.stack, chosen.chunk, 0};
+  if (best) *best = rrhip_trace_tuning{chosen.block, chosen.stack, chosen.chunk, chosen.width};
   if (best_ms) *best_ms = chosen_ms;
   return RRHIP_OK;
 }

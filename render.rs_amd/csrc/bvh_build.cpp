@@ -317,6 +317,87 @@ bool build_triangle_bvh(const float* positions, uint64_t nverts, const uint32_t*
   return true;
 }
 
+namespace {
+
+struct Child4 {
+  float lo[3], hi[3];
+  uint32_t ref, cnt;
+  float area() const {
+    const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return dx * dy + dy * dz + dz * dx;
+  }
+};
+
+Child4 child_of(const Node2& n, int k) {
+  Child4 c;
+  if (k == 0) {
+    c.lo[0] = n.c0_lox; c.hi[0] = n.c0_hix; c.lo[1] = n.c0_loy; c.hi[1] = n.c0_hiy;
+    c.lo[2] = n.c0_loz; c.hi[2] = n.c0_hiz; c.ref = n.ref0; c.cnt = n.cnt0;
+  } else {
+    c.lo[0] = n.c1_lox; c.hi[0] = n.c1_hix; c.lo[1] = n.c1_loy; c.hi[1] = n.c1_hiy;
+    c.lo[2] = n.c1_loz; c.hi[2] = n.c1_hiz; c.ref = n.ref1; c.cnt = n.cnt1;
+  }
+  return c;
+}
+
+struct Collapser {
+  const std::vector<Node2>& n2;
+  std::vector<Node4>* out;
+  int depth = 0;
+
+  uint32_t build(uint32_t i, int level) {
+    depth = std::max(depth, level);
+    const uint32_t idx = (uint32_t)out->size();
+    out->push_back(Node4{});
+    Child4 c[4];
+    int m = 2;
+    c[0] = child_of(n2[i], 0);
+    c[1] = child_of(n2[i], 1);
+    while (m < 4) {
+      int best = -1;
+      float best_area = -1.f;
+      for (int k = 0; k < m; k++)
+        if (c[k].cnt == 0 && c[k].area() > best_area) {
+          best = k;
+          best_area = c[k].area();
+        }
+      if (best < 0) break;
+      const Node2& open = n2[c[best].ref];
+      c[best] = child_of(open, 0);
+      c[m++] = child_of(open, 1);
+    }
+    Node4 nd;
+    for (int k = 0; k < 4; k++) {
+      if (k < m) {
+        nd.lox[k] = c[k].lo[0]; nd.hix[k] = c[k].hi[0];
+        nd.loy[k] = c[k].lo[1]; nd.hiy[k] = c[k].hi[1];
+        nd.loz[k] = c[k].lo[2]; nd.hiz[k] = c[k].hi[2];
+        nd.cnt[k] = c[k].cnt;
+        nd.ref[k] = c[k].cnt ? c[k].ref : build(c[k].ref, level + 1);
+      } else {
+        nd.lox[k] = nd.loy[k] = nd.loz[k] = 0.f;
+        nd.hix[k] = nd.hiy[k] = nd.hiz[k] = 0.f;
+        nd.cnt[k] = kEmpty4;
+        nd.ref[k] = 0;
+      }
+    }
+    (*out)[idx] = nd;
+    return idx;
+  }
+};
+
+}  // namespace
+
+void collapse_bvh4(const std::vector<Node2>& nodes, std::vector<Node4>* out, int* depth4) {
+  out->clear();
+  *depth4 = 0;
+  if (nodes.empty()) return;
+  out->reserve(nodes.size() / 2 + 1);
+  Collapser c{nodes, out};
+  c.build(0, 0);
+  *depth4 = c.depth;
+}
+
 bool build_box_bvh(const float* boxes, uint64_t n, const BuildParams& params, BuiltBvh* out,
                    std::vector<uint32_t>* order, std::string* err) {
   out->nodes.clear();

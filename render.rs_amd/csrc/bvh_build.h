@@ -29,6 +29,12 @@ struct BuiltBvh {
   uint64_t leaves = 0;
 };
 
+// Collapses a binary tree into four-wide nodes with the same leaves: each
+// node takes its two children and repeatedly opens the interior child of
+// largest surface area until it has four (Wald et al. 2008's collapse).
+// depth4: edges on the longest root-to-leaf path of the result.
+void collapse_bvh4(const std::vector<Node2>& nodes, std::vector<Node4>* out, int* depth4);
+
 // Builds over `ntris` triangles given by `indices` into `positions`
 // (xyz f32 triples). Returns false with `err` set on invalid input.
 bool build_triangle_bvh(const float* positions, uint64_t nverts, const uint32_t* indices,

@@ -24,6 +24,24 @@ struct alignas(64) Node2 {
 };
 static_assert(sizeof(Node2) == 64, "Node2 must be one 64-byte line");
 
+// Four-wide node (the binary tree collapsed, same leaves): the four
+// children's boxes as per-axis float4s (lane k = child k), their
+// references (node index, or first triangle slot of a leaf) and leaf
+// triangle counts (0: interior child, kEmpty4: no child). One 128-byte
+// line, so a visit is one cache line and decides four slab tests.
+struct alignas(128) Node4 {
+  float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+  uint32_t ref[4];
+  uint32_t cnt[4];
+};
+static_assert(sizeof(Node4) == 128, "Node4 must be one 128-byte line");
+constexpr uint32_t kEmpty4 = 0xFFFFFFFFu;
+// Four-wide stack entries: a node (bits 0-26) and the mask of its hit
+// children not yet taken (bits 27-30), so a visit pushes at most one entry
+// (the node itself) and the stack never holds more entries than the tree
+// has levels.
+constexpr uint32_t kNode4Mask = (1u << 27) - 1u;
+
 struct alignas(16) TriRec {
   float v0x, v0y, v0z;
   float v1x, v1y, v1z;

@@ -24,6 +24,7 @@ struct rrhip_scene {
   bool has_query = false;
   rrhip::Node2* d_nodes = nullptr;
   rrhip::TriRec* d_tris = nullptr;
+  rrhip::Node4* d_nodes4 = nullptr;  // the query tree collapsed four-wide
   rrhip::DeviceBvh bvh;
   rrhip::TraceTuning tuning;
   rrhip_scene_info info{};

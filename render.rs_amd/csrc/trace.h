@@ -26,21 +26,29 @@ constexpr uint32_t kNoHit = 0xFFFFFFFFu;
 struct DeviceBvh {
   const Node2* nodes = nullptr;
   const TriRec* tris = nullptr;
+  const Node4* nodes4 = nullptr;  // the same tree collapsed four-wide
   uint64_t num_nodes = 0;
+  uint64_t num_nodes4 = 0;
   uint64_t num_tris = 0;
-  int depth = 0;  // node levels: the traversal stack must hold this many entries
+  int depth = 0;   // binary tree: edges on the longest path (the stack holds that many entries)
+  int depth4 = 0;  // four-wide tree: edges on the longest path
 };
 
 // Launch configuration of the traversal kernels.
 struct TraceTuning {
   uint32_t block = 128;  // threads per block: 64, 128 or 256
-  uint32_t stack = 0;    // LDS stack entries per lane: 16, 24, 32, 48 or 64 (>= depth)
+  uint32_t stack = 0;    // LDS stack entries per lane: 16, 24, 32, 48 or 64 (>= stack_need)
   uint32_t chunk = 256;  // rays owned by each wave (multiple of 64)
+  uint32_t width = 2;    // tree width: 2 (Node2) or 4 (Node4)
 };
 
 // Smallest supported stack size holding `depth` entries (0 if none does).
 int stack_for_depth(int depth);
-bool tuning_valid(const TraceTuning& tu, int depth);
+// Stack entries a lane can need: one per level in both trees (one push
+// per visit: a binary sibling, or a four-wide node with its remaining
+// children).
+int stack_need(const DeviceBvh& bvh, uint32_t width);
+bool tuning_valid(const TraceTuning& tu, const DeviceBvh& bvh);
 
 enum class TraceMode { kClosest, kAny };
 

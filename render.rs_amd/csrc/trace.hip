@@ -32,9 +32,10 @@ namespace rrhip {
 namespace {
 
 // Wave-chunk while-while kernel. `chunk` (a multiple of 64) is the number
-// of rays a wave owns.
-template <int BLOCK, int STACK, bool ANY, bool COUNT>
+// of rays a wave owns; WIDTH selects the binary or the four-wide tree.
+template <int BLOCK, int STACK, bool ANY, bool COUNT, int WIDTH>
 __global__ __launch_bounds__(BLOCK) void trace_kernel(const Node2* __restrict__ nodes,
+                                                      const Node4* __restrict__ nodes4,
                                                       const TriRec* __restrict__ tris,
                                                       const RayRec* __restrict__ rays,
                                                       HitRec* __restrict__ hits,
@@ -69,7 +70,12 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const Node2* __restrict__ 
     if (__ballot(have) == 0) break;
     if (have) {
       // Interior phase: walk until this lane holds a leaf or is done.
-      while (L.lcnt == 0 && L.node != kDone) interior_step<BLOCK, COUNT>(L, nodes, stk);
+      while (L.lcnt == 0 && L.node != kDone) {
+        if constexpr (WIDTH == 4)
+          interior_step4<BLOCK, COUNT>(L, nodes4, stk);
+        else
+          interior_step<BLOCK, COUNT>(L, nodes, stk);
+      }
       // Leaf phase.
       while (L.lcnt > 0) {
         const TriRec tr = tris[L.lref];
@@ -90,40 +96,47 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const Node2* __restrict__ 
   }
 }
 
-template <int BLOCK, int STACK, bool ANY, bool COUNT>
+template <int BLOCK, int STACK, bool ANY, bool COUNT, int WIDTH>
 hipError_t launch_cfg(const DeviceBvh& bvh, const RayRec* rays, uint32_t n, HitRec* hits,
                       uint8_t* occ, uint32_t* counts, uint32_t chunk, hipStream_t stream) {
   const uint64_t waves = ((uint64_t)n + chunk - 1) / chunk;
   const uint64_t blocks = (waves * 64 + BLOCK - 1) / BLOCK;
-  hipLaunchKernelGGL((trace_kernel<BLOCK, STACK, ANY, COUNT>), dim3((uint32_t)blocks), dim3(BLOCK),
-                     0, stream, bvh.nodes, bvh.tris, rays, hits, occ, counts, n, chunk);
+  hipLaunchKernelGGL((trace_kernel<BLOCK, STACK, ANY, COUNT, WIDTH>), dim3((uint32_t)blocks), dim3(BLOCK),
+                     0, stream, bvh.nodes, bvh.nodes4, bvh.tris, rays, hits, occ, counts, n, chunk);
   return hipGetLastError();
 }
 
-template <int BLOCK, bool ANY, bool COUNT>
+template <int BLOCK, bool ANY, bool COUNT, int WIDTH>
 hipError_t launch_stack(const DeviceBvh& bvh, const RayRec* rays, uint32_t n, HitRec* hits,
                         uint8_t* occ, uint32_t* counts, const TraceTuning& tu,
                         hipStream_t stream) {
   switch (tu.stack) {
-    case 16: return launch_cfg<BLOCK, 16, ANY, COUNT>(bvh, rays, n, hits, occ, counts, tu.chunk, stream);
-    case 24: return launch_cfg<BLOCK, 24, ANY, COUNT>(bvh, rays, n, hits, occ, counts, tu.chunk, stream);
-    case 32: return launch_cfg<BLOCK, 32, ANY, COUNT>(bvh, rays, n, hits, occ, counts, tu.chunk, stream);
-    case 48: return launch_cfg<BLOCK, 48, ANY, COUNT>(bvh, rays, n, hits, occ, counts, tu.chunk, stream);
-    case 64: return launch_cfg<BLOCK, 64, ANY, COUNT>(bvh, rays, n, hits, occ, counts, tu.chunk, stream);
+    case 16: return launch_cfg<BLOCK, 16, ANY, COUNT, WIDTH>(bvh, rays, n, hits, occ, counts, tu.chunk, stream);
+    case 24: return launch_cfg<BLOCK, 24, ANY, COUNT, WIDTH>(bvh, rays, n, hits, occ, counts, tu.chunk, stream);
+    case 32: return launch_cfg<BLOCK, 32, ANY, COUNT, WIDTH>(bvh, rays, n, hits, occ, counts, tu.chunk, stream);
+    case 48: return launch_cfg<BLOCK, 48, ANY, COUNT, WIDTH>(bvh, rays, n, hits, occ, counts, tu.chunk, stream);
+    case 64: return launch_cfg<BLOCK, 64, ANY, COUNT, WIDTH>(bvh, rays, n, hits, occ, counts, tu.chunk, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <bool ANY, bool COUNT, int WIDTH>
+hipError_t launch_block(const DeviceBvh& bvh, const RayRec* rays, uint32_t n, HitRec* hits,
+                        uint8_t* occ, uint32_t* counts, const TraceTuning& tu,
+                        hipStream_t stream) {
+  switch (tu.block) {
+    case 64: return launch_stack<64, ANY, COUNT, WIDTH>(bvh, rays, n, hits, occ, counts, tu, stream);
+    case 128: return launch_stack<128, ANY, COUNT, WIDTH>(bvh, rays, n, hits, occ, counts, tu, stream);
+    case 256: return launch_stack<256, ANY, COUNT, WIDTH>(bvh, rays, n, hits, occ, counts, tu, stream);
     default: return hipErrorInvalidValue;
   }
 }
 
 template <bool ANY, bool COUNT>
-hipError_t launch_block(const DeviceBvh& bvh, const RayRec* rays, uint32_t n, HitRec* hits,
-                        uint8_t* occ, uint32_t* counts, const TraceTuning& tu,
-                        hipStream_t stream) {
-  switch (tu.block) {
-    case 64: return launch_stack<64, ANY, COUNT>(bvh, rays, n, hits, occ, counts, tu, stream);
-    case 128: return launch_stack<128, ANY, COUNT>(bvh, rays, n, hits, occ, counts, tu, stream);
-    case 256: return launch_stack<256, ANY, COUNT>(bvh, rays, n, hits, occ, counts, tu, stream);
-    default: return hipErrorInvalidValue;
-  }
+hipError_t launch_width(const DeviceBvh& bvh, const RayRec* rays, uint32_t n, HitRec* hits,
+                        uint8_t* occ, uint32_t* counts, const TraceTuning& tu, hipStream_t stream) {
+  if (tu.width == 4) return launch_block<ANY, COUNT, 4>(bvh, rays, n, hits, occ, counts, tu, stream);
+  return launch_block<ANY, COUNT, 2>(bvh, rays, n, hits, occ, counts, tu, stream);
 }
 
 }  // namespace
@@ -135,17 +148,22 @@ int stack_for_depth(int depth) {
   return 0;
 }
 
-bool tuning_valid(const TraceTuning& tu, int depth) {
+int stack_need(const DeviceBvh& bvh, uint32_t width) {
+  return width == 4 ? bvh.depth4 + 1 : bvh.depth;
+}
+
+bool tuning_valid(const TraceTuning& tu, const DeviceBvh& bvh) {
   const bool block_ok = tu.block == 64 || tu.block == 128 || tu.block == 256;
-  const bool stack_ok = stack_for_depth(tu.stack) == tu.stack && tu.stack >= depth;
+  const bool width_ok = tu.width == 2 || (tu.width == 4 && bvh.nodes4 != nullptr);
+  const bool stack_ok = stack_for_depth(tu.stack) == (int)tu.stack && (int)tu.stack >= stack_need(bvh, tu.width);
   const bool chunk_ok = tu.chunk >= 64 && tu.chunk % 64 == 0 && tu.chunk <= (1u << 20);
-  return block_ok && stack_ok && chunk_ok;
+  return block_ok && width_ok && stack_ok && chunk_ok;
 }
 
 hipError_t launch_trace(const DeviceBvh& bvh, const RayRec* rays, uint64_t n, TraceMode mode,
                         HitRec* hits, uint8_t* occluded, uint32_t* counts,
                         const TraceTuning& tu, hipStream_t stream) {
-  if (!tuning_valid(tu, bvh.depth)) return hipErrorInvalidValue;
+  if (!tuning_valid(tu, bvh)) return hipErrorInvalidValue;
   const uint64_t kChunk = 1ull << 30;
   for (uint64_t off = 0; off < n; off += kChunk) {
     const uint32_t m = (uint32_t)((n - off) < kChunk ? (n - off) : kChunk);
@@ -155,11 +173,11 @@ hipError_t launch_trace(const DeviceBvh& bvh, const RayRec* rays, uint64_t n, Tr
     uint32_t* c = counts ? counts + 2 * off : nullptr;
     hipError_t e;
     if (mode == TraceMode::kAny) {
-      e = c ? launch_block<true, true>(bvh, r, m, h, o, c, tu, stream)
-            : launch_block<true, false>(bvh, r, m, h, o, c, tu, stream);
+      e = c ? launch_width<true, true>(bvh, r, m, h, o, c, tu, stream)
+            : launch_width<true, false>(bvh, r, m, h, o, c, tu, stream);
     } else {
-      e = c ? launch_block<false, true>(bvh, r, m, h, o, c, tu, stream)
-            : launch_block<false, false>(bvh, r, m, h, o, c, tu, stream);
+      e = c ? launch_width<false, true>(bvh, r, m, h, o, c, tu, stream)
+            : launch_width<false, false>(bvh, r, m, h, o, c, tu, stream);
     }
     if (e != hipSuccess) return e;
   }

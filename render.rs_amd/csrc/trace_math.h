@@ -154,6 +154,7 @@ struct Lane {
   bool occ;
   uint32_t node;        // interior node to visit, kNeedPop or kDone
   uint32_t lref, lcnt;  // pending leaf range (lcnt == 0: none)
+  uint32_t mask4;       // four-wide: children of `node` still to visit
   int sp;
   uint32_t n_nodes, n_tris;
 };
@@ -167,6 +168,7 @@ RRHIP_HD inline void lane_start(Lane& L, const RayRec* __restrict__ rays, uint32
   L.best_v = 0.0;
   L.occ = false;
   L.node = 0;
+  L.mask4 = 0xFu;
   L.lcnt = 0;
   L.lref = 0;
   L.sp = 0;
@@ -275,6 +277,75 @@ RRHIP_HD inline void interior_step(Lane& L, const Node2* __restrict__ nodes,
   } else {
     L.node = kNeedPop;
   }
+}
+
+// One four-wide interior visit (or one pop). Children still in L.mask4
+// are slab-tested (the arithmetic of slab2); the nearest hit leaf becomes
+// the pending leaf and the nearest hit interior child the next node
+// (min reductions over register values: no sort, no dynamic indexing);
+// when hit children remain the node is pushed back with their mask, to be
+// re-tested against the then tighter bound when popped, so each pop takes
+// the nearest remaining children again.
+template <int STRIDE, bool COUNT>
+RRHIP_HD inline void interior_step4(Lane& L, const Node4* __restrict__ nodes, uint32_t* stk) {
+  if (L.node == kNeedPop) {
+    if (L.sp == 0) {
+      L.node = kDone;
+      return;
+    }
+    L.sp--;
+    const uint32_t e = stk[L.sp * STRIDE];
+    L.node = e & kNode4Mask;
+    L.mask4 = e >> 27;
+    return;
+  }
+  const uint32_t self = L.node;
+  const Node4 nd = nodes[self];
+  if (COUNT) L.n_nodes++;
+  const Ray32& r = L.r;
+  uint32_t hit = 0;
+  float best_leaf_t = INFINITY, best_node_t = INFINITY;
+  int best_leaf = -1, best_node = -1;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const float a0 = (nd.lox[k] - r.ox) * r.ix, b0 = (nd.hix[k] - r.ox) * r.ix;
+    const float a1 = (nd.loy[k] - r.oy) * r.iy, b1 = (nd.hiy[k] - r.oy) * r.iy;
+    const float a2 = (nd.loz[k] - r.oz) * r.iz, b2 = (nd.hiz[k] - r.oz) * r.iz;
+    const float tn = fmaxf(fmaxf(fminf(a0, b0), fminf(a1, b1)), fmaxf(fminf(a2, b2), r.tmin));
+    const float tf = fminf(fminf(fmaxf(a0, b0), fmaxf(a1, b1)), fmaxf(a2, b2)) * kSlabUp;
+    const bool h = ((L.mask4 >> k) & 1u) && nd.cnt[k] != kEmpty4 && tn <= fminf(tf, L.tmax_f);
+    hit |= (h ? 1u : 0u) << k;
+    const bool leaf = nd.cnt[k] > 0;
+    if (h && leaf && (best_leaf < 0 || tn < best_leaf_t)) {
+      best_leaf = k;
+      best_leaf_t = tn;
+    }
+    if (h && !leaf && (best_node < 0 || tn < best_node_t)) {
+      best_node = k;
+      best_node_t = tn;
+    }
+  }
+  uint32_t next = kNeedPop, lref = 0, lcnt = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (k == best_leaf) {
+      lref = nd.ref[k];
+      lcnt = nd.cnt[k];
+    }
+    if (k == best_node) next = nd.ref[k];
+  }
+  if (best_leaf >= 0) {
+    L.lref = lref;
+    L.lcnt = lcnt;
+    hit &= ~(1u << best_leaf);
+  }
+  if (best_node >= 0) hit &= ~(1u << best_node);
+  if (hit) {
+    stk[L.sp * STRIDE] = (hit << 27) | self;
+    L.sp++;
+  }
+  L.node = next;
+  L.mask4 = 0xFu;
 }
 
 }  // namespace rrhip

@@ -193,17 +193,18 @@ class SceneInfo(ctypes.Structure):
     _fields_ = [("num_triangles", ctypes.c_uint64), ("num_nodes", ctypes.c_uint64),
                 ("num_leaves", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
                 ("depth", ctypes.c_uint32), ("node_bytes", ctypes.c_uint32),
-                ("tri_bytes", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("tri_bytes", ctypes.c_uint32), ("depth4", ctypes.c_uint32),
                 ("build_ms", ctypes.c_double),
                 ("bounds_lo", ctypes.c_float * 3), ("bounds_hi", ctypes.c_float * 3)]
 
 
 class TraceTuning(ctypes.Structure):
     _fields_ = [("block", ctypes.c_uint32), ("stack", ctypes.c_uint32),
-                ("rays_per_wave", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("rays_per_wave", ctypes.c_uint32), ("width", ctypes.c_uint32)]
 
     def as_dict(self):
-        return {"block": self.block, "stack": self.stack, "rays_per_wave": self.rays_per_wave}
+        return {"block": self.block, "stack": self.stack, "rays_per_wave": self.rays_per_wave,
+                "width": self.width or 2}
 
 
 _lib = None
@@ -548,8 +549,8 @@ class Scene:
         _check(load().rrhip_scene_get_tuning(self.h, ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, block, stack, rays_per_wave):
-        t = TraceTuning(block, stack, rays_per_wave, 0)
+    def set_tuning(self, block, stack, rays_per_wave, width=2):
+        t = TraceTuning(block, stack, rays_per_wave, width)
         _check(load().rrhip_scene_set_tuning(self.h, ctypes.byref(t)))
 
     def autotune(self, d_rays, n, d_out, any_hit=False):

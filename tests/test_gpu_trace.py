@@ -175,24 +175,29 @@ def test_query_benchmark_full_size(ctx):
 
 
 def test_every_launch_configuration_agrees(ctx):
-    say("launch configurations: block x stack x rays_per_wave")
+    say("launch configurations: width x block x stack x rays_per_wave")
     pos, idx = rrhip.gen_heightfield(256, 128, 5)
     rays = rrhip.gen_query_rays(pos.min(0), pos.max(0), 0.1, 5, 40000)
     m = orc.Mesh(pos, idx, bins=32, max_leaf=4, sah_leaf_mult=1)
     op, ot, ou, ov = m.trace_closest(rays)
     oany = m.trace_any(rays)
     sc = ctx.upload_mesh(pos, idx)
-    depth = sc.info()["depth"]
-    stacks = [s for s in (16, 24, 32, 48, 64) if s >= depth]
-    for block in (64, 128, 256):
-        for stack in stacks[:2]:
-            for rpw in (64, 192, 1024):
-                sc.set_tuning(block, stack, rpw)
-                assert sc.get_tuning() == {"block": block, "stack": stack, "rays_per_wave": rpw}
-                assert_closest_equal(sc.trace_closest(rays), op, ot, ou, ov)
-                assert np.array_equal(sc.trace_any(rays), oany)
+    info = sc.info()
+    depth, depth4 = info["depth"], info["depth4"]
+    assert 0 < depth4 < depth
+    for width, need in ((2, depth), (4, depth4)):
+        stacks = [s for s in (16, 24, 32, 48, 64) if s >= need]
+        for block in (64, 128, 256):
+            for stack in stacks[:2]:
+                for rpw in (64, 192, 1024):
+                    sc.set_tuning(block, stack, rpw, width)
+                    assert sc.get_tuning() == {"block": block, "stack": stack, "rays_per_wave": rpw, "width": width}
+                    assert_closest_equal(sc.trace_closest(rays), op, ot, ou, ov)
+                    assert np.array_equal(sc.trace_any(rays), oany)
     with pytest.raises(rrhip.RrhipError):
         sc.set_tuning(128, 16 if depth > 16 else 8, 256)
+    with pytest.raises(rrhip.RrhipError):
+        sc.set_tuning(128, 64, 256, 3)
     with pytest.raises(rrhip.RrhipError):
         sc.set_tuning(96, stacks[0], 256)
     d_rays = ctx.to_device(rays)
