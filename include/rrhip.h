@@ -283,7 +283,8 @@ typedef struct rrhip_scene_info {
 typedef struct rrhip_trace_tuning {
   uint32_t block;          /* 64, 128 or 256 threads */
   uint32_t stack;          /* 16, 24, 32, 48 or 64 entries */
-  uint32_t rays_per_wave;  /* multiple of 64, <= 2^20 */
+  uint32_t rays_per_wave;  /* multiple of 64, <= 2^20; 0 = fill: the rays split evenly over the
+                              wave slots the device holds at once (one round of resident waves) */
   uint32_t width;          /* tree width: 2 (64-byte child-pair nodes) or 4 (128-byte nodes); 0 = 2 */
 } rrhip_trace_tuning;
 
@@ -332,13 +333,15 @@ int rrhip_scene_get_info(rrhip_scene* scene, rrhip_scene_info* info);
 int rrhip_build_info(const rrhip_scene_desc* desc, rrhip_scene_info* info);
 
 /* Current / explicit launch configuration (set validates against the
- * scene's depth and returns RRHIP_ERR_INVALID_ARGUMENT when unusable). */
+ * scene's depth and returns RRHIP_ERR_INVALID_ARGUMENT when unusable).
+ * A scene keeps one configuration per query mode: set applies to both,
+ * get reports the closest-hit one. */
 int rrhip_scene_get_tuning(rrhip_scene* scene, rrhip_trace_tuning* tuning);
 int rrhip_scene_set_tuning(rrhip_scene* scene, const rrhip_trace_tuning* tuning);
 /* Autotuner: times every supported configuration on the caller's
  * device-resident rays (closest hit if any == 0, else any hit, writing
- * into `out`), keeps the fastest for the scene and reports it with its
- * per-launch milliseconds. */
+ * into `out`), keeps the fastest as the scene's configuration for that
+ * mode and reports it with its per-launch milliseconds. */
 int rrhip_scene_autotune(rrhip_scene* scene, const rrhip_ray* rays, uint64_t n, int any, void* out,
                          rrhip_trace_tuning* best, float* best_ms);
 

@@ -64,6 +64,12 @@ int check_scene(rrhip_scene* s, const void* rays, uint64_t n, const void* out) {
   return RRHIP_OK;
 }
 
+// Launch configuration of a query mode (closest and any hit are tuned
+// apart: their kernels differ in registers and so in resident waves).
+const TraceTuning& tuning_for(const rrhip_scene* s, TraceMode mode) {
+  return mode == TraceMode::kAny ? s->tuning_any : s->tuning;
+}
+
 // Host-buffer query: copy in, trace, copy out, wait.
 int trace_host(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, TraceMode mode, void* out,
                size_t out_elem) {
@@ -87,7 +93,7 @@ int trace_host(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, TraceMode mode
   }
   RRHIP_HIP_TRY(launch_trace(s->bvh, (const RayRec*)s->d_in, n, mode,
                              mode == TraceMode::kClosest ? (HitRec*)s->d_out : nullptr,
-                             mode == TraceMode::kAny ? (uint8_t*)s->d_out : nullptr, nullptr, s->tuning, st));
+                             mode == TraceMode::kAny ? (uint8_t*)s->d_out : nullptr, nullptr, tuning_for(s, mode), st));
   RRHIP_HIP_TRY(hipMemcpyAsync(out, s->d_out, n * out_elem, hipMemcpyDeviceToHost, st));
   RRHIP_HIP_TRY(hipStreamSynchronize(st));
   return RRHIP_OK;
@@ -112,7 +118,7 @@ int trace_dev(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, TraceMode mode,
   }
   RRHIP_HIP_TRY(launch_trace(s->bvh, (const RayRec*)rays, n, mode,
                              mode == TraceMode::kClosest ? (HitRec*)out : nullptr,
-                             mode == TraceMode::kAny ? (uint8_t*)out : nullptr, nullptr, s->tuning, st));
+                             mode == TraceMode::kAny ? (uint8_t*)out : nullptr, nullptr, tuning_for(s, mode), st));
   return RRHIP_OK;
 }
 
@@ -430,6 +436,7 @@ int rrhip_scene_upload(rrhip_ctx* ctx, const rrhip_scene_desc* desc, rrhip_scene
   s->bvh.num_nodes4 = nodes4.size();
   s->bvh.depth4 = depth4;
   s->tuning.stack = (uint32_t)std::max(16, stack_for_depth((int)info.depth));
+  s->tuning_any = s->tuning;
   info.depth4 = nodes4.empty() ? 0u : (uint32_t)(depth4 + 1);
   info.device_bytes += nodes4.size() * sizeof(Node4);
   s->info = info;
@@ -477,6 +484,7 @@ int rrhip_scene_set_tuning(rrhip_scene* s, const rrhip_trace_tuning* t) {
                 "rrhip_scene_set_tuning: unsupported block/stack/rays_per_wave/width for this scene (stack need " +
                     std::to_string(stack_need(s->bvh, tu.width)) + ")");
   s->tuning = tu;
+  s->tuning_any = tu;
   return RRHIP_OK;
 }
 
@@ -484,7 +492,8 @@ int rrhip_scene_autotune(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, int 
                          rrhip_trace_tuning* best, float* best_ms) {
   if (int st = check_scene(s, rays, n, out)) return st;
   if (n == 0 || s->bvh.num_nodes == 0) {
-    if (best) *best = rrhip_trace_tuning{s->tuning.block, s->tuning.stack, s->tuning.chunk, s->tuning.width};
+    const TraceTuning& cur = any ? s->tuning_any : s->tuning;
+    if (best) *best = rrhip_trace_tuning{cur.block, cur.stack, cur.chunk, cur.width};
     if (best_ms) *best_ms = 0.f;
     return RRHIP_OK;
   }
@@ -493,7 +502,7 @@ int rrhip_scene_autotune(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, int 
   const TraceMode mode = any ? TraceMode::kAny : TraceMode::kClosest;
   HitRec* h = any ? nullptr : (HitRec*)out;
   uint8_t* o = any ? (uint8_t*)out : nullptr;
-  TraceTuning chosen = s->tuning;
+  TraceTuning chosen = any ? s->tuning_any : s->tuning;
   float chosen_ms = 0.f;
   bool have = false;
   for (uint32_t width : {2u, 4u}) {
@@ -502,12 +511,19 @@ int rrhip_scene_autotune(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, int 
    for (uint32_t block : {64u, 128u, 256u}) {
     for (int stack : {16, 24, 32, 48, 64}) {
       if (stack < s0 || stack > std::max(s0, 32)) continue;  // the smallest fitting sizes only
-      for (uint32_t chunk : {64u, 128u, 256u, 512u, 1024u, 4096u}) {
-        TraceTuning tu;
-        tu.block = block;
-        tu.stack = (uint32_t)stack;
+      TraceTuning base;
+      base.block = block;
+      base.stack = (uint32_t)stack;
+      base.width = width;
+      // The fill chunk (one round of resident waves) and a few larger
+      // chunks (fewer waves, in case fewer fit than the occupancy query
+      // reports), plus fixed sizes.
+      const uint32_t fill = fill_chunk(n, trace_resident_waves(base, mode));
+      const uint32_t cands[] = {0u, (fill * 9 / 8 + 63) / 64 * 64, (fill * 5 / 4 + 63) / 64 * 64,
+                                (fill * 3 / 2 + 63) / 64 * 64, 256u, 1024u};
+      for (uint32_t chunk : cands) {
+        TraceTuning tu = base;
         tu.chunk = chunk;
-        tu.width = width;
         if (!tuning_valid(tu, s->bvh)) continue;
         RRHIP_HIP_TRY(launch_trace(s->bvh, (const RayRec*)rays, n, mode, h, o, nullptr, tu, st));
         RRHIP_HIP_TRY(hipEventRecord(s->ctx->ev0, st));
@@ -528,7 +544,7 @@ int rrhip_scene_autotune(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, int 
     }
    }
   }
-  s->tuning = chosen;
+  (any ? s->tuning_any : s->tuning) = chosen;
   if (best) *best = rrhip_trace_tuning{chosen.block, chosen.stack, chosen.chunk, chosen.width};
   if (best_ms) *best_ms = chosen_ms;
   return RRHIP_OK;
@@ -569,7 +585,7 @@ int rrhip_trace_cost(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, int any,
   if (e == hipSuccess)
     e = launch_trace(s->bvh, (const RayRec*)s->d_in, n, mode,
                      any ? nullptr : (HitRec*)s->d_out, any ? (uint8_t*)s->d_out : nullptr, d_counts,
-                     s->tuning, st);
+                     tuning_for(s, mode), st);
   if (e == hipSuccess)
     e = hipMemcpyAsync(counts.data(), d_counts, n * 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -594,7 +610,7 @@ int rrhip_time_trace_device(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, i
     if (n && s->bvh.num_nodes)
       RRHIP_HIP_TRY(launch_trace(s->bvh, (const RayRec*)rays, n, mode,
                                  any ? nullptr : (HitRec*)out, any ? (uint8_t*)out : nullptr, nullptr,
-                                 s->tuning, st));
+                                 tuning_for(s, mode), st));
   }
   RRHIP_HIP_TRY(hipEventRecord(s->ctx->ev1, st));
   RRHIP_HIP_TRY(hipEventSynchronize(s->ctx->ev1));

@@ -26,7 +26,8 @@ struct rrhip_scene {
   rrhip::TriRec* d_tris = nullptr;
   rrhip::Node4* d_nodes4 = nullptr;  // the query tree collapsed four-wide
   rrhip::DeviceBvh bvh;
-  rrhip::TraceTuning tuning;
+  rrhip::TraceTuning tuning;      // closest-hit launches
+  rrhip::TraceTuning tuning_any;  // any-hit launches
   rrhip_scene_info info{};
   // Staging for the host-buffer entry points (grown on demand).
   void* d_in = nullptr;

@@ -38,9 +38,17 @@ struct DeviceBvh {
 struct TraceTuning {
   uint32_t block = 128;  // threads per block: 64, 128 or 256
   uint32_t stack = 0;    // LDS stack entries per lane: 16, 24, 32, 48 or 64 (>= stack_need)
-  uint32_t chunk = 256;  // rays owned by each wave (multiple of 64)
+  uint32_t chunk = 0;    // rays owned by each wave (multiple of 64); 0: fill the device's resident waves
   uint32_t width = 2;    // tree width: 2 (Node2) or 4 (Node4)
 };
+
+enum class TraceMode : int;
+// Wave slots of the current device that hold the query kernel of this
+// tuning and mode at once (0 if unknown).
+uint64_t trace_resident_waves(const TraceTuning& tu, TraceMode mode);
+// Rays per wave that spread `n` rays over `slots` resident waves (whole
+// wavefronts of rays, at least 64, at most 2^20; 256 when slots is 0).
+uint32_t fill_chunk(uint64_t n, uint64_t slots);
 
 // Smallest supported stack size holding `depth` entries (0 if none does).
 int stack_for_depth(int depth);
@@ -50,7 +58,7 @@ int stack_for_depth(int depth);
 int stack_need(const DeviceBvh& bvh, uint32_t width);
 bool tuning_valid(const TraceTuning& tu, const DeviceBvh& bvh);
 
-enum class TraceMode { kClosest, kAny };
+enum class TraceMode : int { kClosest, kAny };
 
 // Traces `n` rays. kClosest writes `hits`; kAny writes `occluded` (0/1).
 // `counts` (optional, 2 x u32 per ray) receives node fetches and triangle

@@ -223,10 +223,12 @@ RRHIP_HD inline void lane_finish(const Lane& L, uint32_t id, HitRec* __restrict_
   }
 }
 
-// One interior visit (or one pop). Leaf children become the pending leaf
-// (or a stacked leaf entry); interior children continue nearest-first.
-// At most one entry is pushed per visit, so the stack never holds more
-// entries than the tree has levels.
+// One interior step: a pop when the lane needs one, then the visit of the
+// interior node it yields (a popped leaf entry becomes the pending leaf
+// instead). Leaf children become the pending leaf (or a stacked leaf
+// entry); interior children continue nearest-first. At most one entry is
+// pushed per visit, so the stack never holds more entries than the tree
+// has levels.
 template <int STRIDE, bool COUNT>
 RRHIP_HD inline void interior_step(Lane& L, const Node2* __restrict__ nodes,
                                               uint32_t* stk) {
@@ -240,10 +242,9 @@ RRHIP_HD inline void interior_step(Lane& L, const Node2* __restrict__ nodes,
     if (e & kLeafBit) {
       L.lref = e & kLeafFirstMask;
       L.lcnt = ((e >> 28) & 7u) + 1u;
-    } else {
-      L.node = e;
+      return;
     }
-    return;
+    L.node = e;
   }
   const Node2 nd = nodes[L.node];
   if (COUNT) L.n_nodes++;
@@ -279,7 +280,8 @@ RRHIP_HD inline void interior_step(Lane& L, const Node2* __restrict__ nodes,
   }
 }
 
-// One four-wide interior visit (or one pop). Children still in L.mask4
+// One four-wide interior step (a pop when needed, then the visit of the
+// popped or current node). Children still in L.mask4
 // are slab-tested (the arithmetic of slab2); the nearest hit leaf becomes
 // the pending leaf and the nearest hit interior child the next node
 // (min reductions over register values: no sort, no dynamic indexing);
@@ -297,7 +299,6 @@ RRHIP_HD inline void interior_step4(Lane& L, const Node4* __restrict__ nodes, ui
     const uint32_t e = stk[L.sp * STRIDE];
     L.node = e & kNode4Mask;
     L.mask4 = e >> 27;
-    return;
   }
   const uint32_t self = L.node;
   const Node4 nd = nodes[self];

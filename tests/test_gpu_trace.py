@@ -189,7 +189,7 @@ def test_every_launch_configuration_agrees(ctx):
         stacks = [s for s in (16, 24, 32, 48, 64) if s >= need]
         for block in (64, 128, 256):
             for stack in stacks[:2]:
-                for rpw in (64, 192, 1024):
+                for rpw in (0, 64, 192, 1024):  # 0: one round of resident waves
                     sc.set_tuning(block, stack, rpw, width)
                     assert sc.get_tuning() == {"block": block, "stack": stack, "rays_per_wave": rpw, "width": width}
                     assert_closest_equal(sc.trace_closest(rays), op, ot, ou, ov)
@@ -200,12 +200,22 @@ def test_every_launch_configuration_agrees(ctx):
         sc.set_tuning(128, 64, 256, 3)
     with pytest.raises(rrhip.RrhipError):
         sc.set_tuning(96, stacks[0], 256)
+    with pytest.raises(rrhip.RrhipError):
+        sc.set_tuning(128, stacks[0], 100)
     d_rays = ctx.to_device(rays)
     d_hits = ctx.alloc(rays.shape[0] * 16)
     best, ms = sc.autotune(d_rays, rays.shape[0], d_hits)
     say(f"autotune chose {best} at {ms:.3f} ms")
     assert sc.get_tuning() == best and ms > 0
     assert_closest_equal(sc.trace_closest(rays), op, ot, ou, ov)
+    # any hit is tuned apart; the closest-hit configuration stays
+    d_occ = ctx.alloc(rays.shape[0])
+    best_any, ms_any = sc.autotune(d_rays, rays.shape[0], d_occ, any_hit=True)
+    say(f"any-hit autotune chose {best_any} at {ms_any:.3f} ms")
+    assert sc.get_tuning() == best and ms_any > 0
+    assert np.array_equal(sc.trace_any(rays), oany)
+    assert_closest_equal(sc.trace_closest(rays), op, ot, ou, ov)
+    ctx.free(d_occ)
     ctx.free(d_rays)
     ctx.free(d_hits)
     sc.close()
