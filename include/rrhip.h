@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define RRHIP_ABI_VERSION 4u
+#define RRHIP_ABI_VERSION 5u
 #define RRHIP_NO_HIT 0xFFFFFFFFu
 
 /* Status codes. render.rs returns anyhow::Result with a message
@@ -140,6 +140,7 @@ typedef struct rrhip_material_desc {
   uint32_t interior;              /* 1 + index of the medium inside closed surfaces of this
                                      material (Material::interior); 0 = none. With no lobes and
                                      no emission the surface is an invisible volume hull */
+  uint32_t id;                    /* object id for the id AOV (Material::id); 0 = none */
   double cutout;                  /* 1 - presence (PbrParams::presence): the probability that
                                      a ray passes through the surface; 0 = opaque */
 } rrhip_material_desc;
@@ -259,6 +260,17 @@ typedef struct rrhip_render_params {
  * sample count; sum of squared luminance (0.2126 r + 0.7152 g + 0.0722 b).
  * All sums are of firefly-clamped samples. */
 #define RRHIP_ACCUM_STRIDE 8
+
+/* Per-pixel layout of rrhip_read_film / orc_render_film (render.rs's Film,
+ * src/output/film.rs:41-55, as render_film fills it, pt/mod.rs:163-276):
+ * 18 doubles per pixel, row-major: beauty r, g, b; diffuse r, g, b
+ * (radiance whose first continuation was not specular-like); specular
+ * r, g, b (max(beauty - diffuse, 0)); albedo r, g, b and shading normal
+ * x, y, z at the first real hit (sample means, 0 where a sample hit
+ * nothing); depth (mean distance travelled to the first real hit over the
+ * samples that had one, 0 if none); dominant object id; that id's
+ * fraction of the samples. */
+#define RRHIP_FILM_CHANNELS 18
 
 /* Scene statistics (what the upload built). */
 typedef struct rrhip_scene_info {
@@ -434,6 +446,16 @@ int rrhip_session_set_adaptive(rrhip_session* session, double tolerance);
 /* Mean sample count over the pixels (the accumulator's count column).
  * Counterpart of WfSession::average_spp (renderer.rs:587-591). */
 int rrhip_session_average_spp(rrhip_session* session, double* out);
+/* Film mode (render.rs's render_pt_film / pt::render_film): with enable != 0
+ * every later sample also accumulates the AOVs of RRHIP_FILM_CHANNELS
+ * (cleared now, and with the accumulator by rrhip_load_accum); 0 turns it
+ * off and frees its buffers. The id vote keeps the first four distinct ids
+ * a pixel sees (exact when a pixel sees at most four; ties go to the id
+ * seen first). RRHIP_ERR_UNSUPPORTED with a radiance cache attached. */
+int rrhip_session_set_film(rrhip_session* session, int enable);
+/* Copies the film (width x height x RRHIP_FILM_CHANNELS doubles) to `out`;
+ * RRHIP_ERR_INVALID_ARGUMENT when film mode is off. */
+int rrhip_read_film(rrhip_session* session, double* out);
 /* relMSE of `session`'s mean image against `reference`'s (same size and
  * context): mean over pixels and channels of (x - r)^2 / (r^2 + eps),
  * reduced in FP64 on the GPU. No counterpart in render.rs (its tests use
@@ -510,7 +532,7 @@ int rrhip_nrc_gradients(rrhip_nrc* nrc, const rrhip_nrc_query* queries, const fl
 /* Attaches a cache to a session (NULL detaches): render passes then
  * terminate paths into the cache by the area-spread criterion and train
  * it online on the suffixes of extra training paths. Cache and session
- * must share a context. */
+ * must share a context. RRHIP_ERR_UNSUPPORTED in film mode. */
 int rrhip_session_set_nrc(rrhip_session* session, rrhip_nrc* nrc);
 
 /* ---- batched jobs ---------------------------------------------------------- */

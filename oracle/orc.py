@@ -160,6 +160,7 @@ def _pt_sigs():
     sig = {
         "orc_render": (i32, [vp, vp, u32, u32, vp, i32, vp]),
         "orc_render_adaptive": (i32, [vp, vp, u32, f64, vp, i32]),
+        "orc_render_film": (i32, [vp, vp, u32, u32, vp, i32]),
         "orc_camera_rays": (None, [vp, u64, vp, vp, vp, vp]),
         "orc_bsdf_eval": (None, [vp, vp, vp, f64, vp, vp]),
         "orc_bsdf_eval_n": (None, [vp, vp, vp, f64, u64, vp, vp]),
@@ -190,6 +191,18 @@ def render(desc, params, start, count, accum=None, threads=None):
     lib.orc_render(ctypes.addressof(desc), ctypes.addressof(params), start, count, _p(accum),
                    threads or default_threads(), ctypes.byref(rays))
     return accum, rays.value
+
+
+def render_film(desc, params, start, count, threads=None):
+    """pt::render_film over samples [start, start+count): (npix, 18) f64 in
+    the RRHIP_FILM layout (beauty, diffuse, specular, albedo, normal rgb /
+    xyz, depth, dominant id, its coverage)."""
+    lib = _pt_sigs()
+    npix = desc.camera.width * desc.camera.height
+    film = np.zeros((npix, 18), np.float64)
+    lib.orc_render_film(ctypes.addressof(desc), ctypes.addressof(params), start, count, _p(film),
+                        threads or default_threads())
+    return film
 
 
 def camera_rays(desc, xy, samples):

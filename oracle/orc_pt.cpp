@@ -1147,9 +1147,19 @@ static double light_pdf_solid_angle(const Light& light, const Vec3& origin, cons
 }
 
 // pt/mod.rs:449-990, surface subset
-Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) const {
+Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays, AuxSample* aux) const {
   const double time = ray.time;  // the path's shutter time
   Vec3 l = Vec3::zero(), beta = Vec3::one();
+  // AOVs (pt/mod.rs:457-470): which lobe the first continuation used
+  // (-1 none yet, 0 diffuse-like, 1 specular-like); contributions go to
+  // the diffuse AOV unless it was specular-like.
+  int first_specular = -1;
+  bool aux_set = aux == nullptr;
+  double travel = 0.0;
+  const auto emit = [&](const Vec3& c) {
+    l = l + c;
+    if (aux && first_specular != 1) aux->diffuse = aux->diffuse + c;
+  };
   double prev_pdf = 0.0;
   Vec3 prev_origin = ray.origin;
   bool from_camera = true;
@@ -1197,7 +1207,7 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
               const double pdf_d = distance_pdf_approx(md, t_eq, t_seg);
               const double w_eq = pdf_eq / (pdf_eq + pdf_d);
               const Vec3 c = sigma_s * tr * vis * lt.radiance() * (ph * w_eq / (dist2 * pdf_eq * seg_pick));
-              l = l + beta * c;
+              emit(beta * c);
             }
           }
         }
@@ -1206,7 +1216,17 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
       beta = beta * ev.weight;
       if (ev.scatter) {
         const Vec3 p = ray.at(ev.t);
-        if (max_component(md.emission) > 0.0) l = l + beta * md.emission;
+        if (!aux_set) {  // pt/mod.rs:559-570
+          aux_set = true;
+          const Vec3 st = md.sigma_t();
+          aux->albedo = Vec3{md.sigma_s.x / std::fmax(st.x, 1e-9), md.sigma_s.y / std::fmax(st.y, 1e-9),
+                             md.sigma_s.z / std::fmax(st.z, 1e-9)};
+          aux->normal = -ray.direction.normalize();
+          aux->depth = travel + ev.t;
+          aux->id = 0;
+        }
+        if (first_specular < 0) first_specular = 0;
+        if (max_component(md.emission) > 0.0) emit(beta * md.emission);
         const Vec3 wo = -ray.direction.normalize();
         if (has_pick) {
           const double g = md.g;
@@ -1223,7 +1243,7 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
             const double pdf_e = equiangular_pdf(ray.origin, ray.direction, t_seg, lights[seg_li].position, ev.t);
             w_dist = pdf_d / (pdf_d + pdf_e);
           }
-          l = l + beta * sample_light(lights[seg_li], p, Vec3::zero(), eval, seg_pick, rng, rays, medium, time) * w_dist;
+          emit(beta * sample_light(lights[seg_li], p, Vec3::zero(), eval, seg_pick, rng, rays, medium, time) * w_dist);
         }
         const double u1 = rng.next_f64(), u2 = rng.next_f64();
         double pdf;
@@ -1251,9 +1271,9 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
           double pick = light_sampler.pmf(prev_origin, (size_t)dome);
           weight = power_heuristic(prev_pdf, (1.0 / (4.0 * PI)) * pick);
         }
-        l = l + beta * d.radiance() * weight;
+        emit(beta * d.radiance() * weight);
       } else {
-        l = l + beta * background;
+        emit(beta * background);
       }
       break;
     }
@@ -1262,6 +1282,7 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
     // (pt/mod.rs:662-683).
     if (mat.hull() && skips < 16) {
       skips++;
+      travel += hit.t;
       if (hit.front_face)
         ms.push(mat.interior);
       else
@@ -1269,6 +1290,19 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
       ray = Ray::make(hit.point + ray.direction.normalize() * 1e-4, ray.direction);
       ray.time = time;
       continue;
+    }
+    // The first real hit's AOVs, cutouts included (pt/mod.rs:688-700).
+    travel += hit.t;
+    if (!aux_set) {
+      aux_set = true;
+      const PbrParams& pb = mat.pbr;
+      const Vec3 a = pb.diffuse_color * pb.diffuse_gain + pb.specular_f0() * (pb.has_specular() ? 1.0 : 0.0) +
+                     pb.refraction_color * pb.glass_gain;
+      aux->albedo = Vec3{std::fmin(a.x, 1.0), std::fmin(a.y, 1.0), std::fmin(a.z, 1.0)};
+      const Vec3 wo_a = -ray.direction.normalize();
+      aux->normal = hit.normal.dot(wo_a) >= 0.0 ? hit.normal : -hit.normal;
+      aux->depth = travel;
+      aux->id = mat.id;
     }
     // Presence cutout: pass through with probability 1 - presence
     // (pt/mod.rs:702-710).
@@ -1293,7 +1327,7 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
         double pdf_light = light_pdf_solid_angle(lt, prev_origin, hit) * pick;
         weight = power_heuristic(prev_pdf, pdf_light);
       }
-      l = l + beta * emission * weight;
+      emit(beta * emission * weight);
     }
     Frame frame(n);
     Vec3 wo_l = frame.to_local(wo);
@@ -1313,13 +1347,14 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) c
           *f = *f * wi_l.z;
         };
         Vec3 c = sample_light(lights[li], hit.point, n, eval, pmf, rng, rays, medium, time);
-        l = l + beta * c;
+        emit(beta * c);
       }
     }
     BsdfSample s;
     if (!bsdf_sample(pbr, wo_l, eta_rel, rng, &s)) break;
     Vec3 wi_world = frame.to_world(s.wi);
     beta = beta * s.f * (std::fabs(s.wi.z) / s.pdf);
+    if (first_specular < 0) first_specular = s.specular_lobe ? 1 : 0;
     prev_pdf = s.pdf;
     prev_origin = hit.point;
     from_camera = false;
@@ -1447,6 +1482,7 @@ void Scene::load(const rrhip_scene_desc& d) {
     mt.area_light = m.area_light;
     mt.presence = 1.0 - m.cutout;
     mt.interior = (int)m.interior - 1;
+    mt.id = m.id;
     materials.push_back(mt);
   }
 

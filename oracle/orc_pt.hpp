@@ -47,6 +47,7 @@ struct Material {
   int area_light = -1;
   double presence = 1.0;  // PbrParams::presence (pbr.rs:39)
   int interior = -1;      // Material::interior (material.rs:37)
+  uint32_t id = 0;        // Material::id, the id AOV's object id (material.rs:39)
   // pt/mod.rs:662-667: an interior medium and no lobes or emission.
   bool hull() const;
 };
@@ -72,6 +73,16 @@ struct Frame {
   explicit Frame(const Vec3& n_);
   Vec3 to_local(const Vec3& w) const { return {w.dot(t), w.dot(b), w.dot(n)}; }
   Vec3 to_world(const Vec3& w) const { return t * w.x + b * w.y + n * w.z; }
+};
+
+// film.rs:11-24: the first real hit's albedo, normal, distance travelled
+// and object id, and the radiance whose first continuation was not
+// specular-like.
+struct AuxSample {
+  Vec3 albedo{0, 0, 0}, normal{0, 0, 0};
+  double depth = 0.0;
+  uint32_t id = 0;
+  Vec3 diffuse{0, 0, 0};
 };
 
 struct BsdfSample {
@@ -229,8 +240,9 @@ class Scene {
 
   void load(const rrhip_scene_desc& d);
   bool intersect(const Ray& ray, Intersection* out) const;
-  // Closest-hit and shadow-ray counts accumulate into these when non-null.
-  Vec3 trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays) const;
+  // Closest-hit and shadow-ray counts accumulate into these when non-null;
+  // aux (optional) receives the sample's AOVs (film.rs:11-24).
+  Vec3 trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays, AuxSample* aux = nullptr) const;
   Ray camera_ray(uint32_t x, uint32_t y, Pcg32& rng) const;
 
  private:

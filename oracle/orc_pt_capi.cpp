@@ -85,6 +85,90 @@ int orc_render(const rrhip_scene_desc* desc, const rrhip_render_params* params, 
   return 0;
 }
 
+// pt::render_film (src/raytracer/pt/mod.rs:163-276) over samples
+// [start, start + count): per pixel the RRHIP_FILM_CHANNELS layout of
+// include/rrhip.h. The id vote counts every sample's id; of equal counts
+// the id seen first wins (the reference's HashMap leaves that order open).
+int orc_render_film(const rrhip_scene_desc* desc, const rrhip_render_params* params, uint32_t start,
+                    uint32_t count, double* film, int nthreads) {
+  Scene scene;
+  scene.load(*desc);
+  RenderParams rp;
+  rp.max_depth = params->max_depth;
+  rp.rr_start = params->rr_start;
+  rp.firefly_clamp = params->firefly_clamp;
+  rp.sample_base = params->sample_base;
+  const uint32_t W = scene.camera.width, H = scene.camera.height;
+  std::atomic<uint32_t> next_row{0};
+  auto work = [&]() {
+    for (;;) {
+      const uint32_t y = next_row.fetch_add(1);
+      if (y >= H) break;
+      for (uint32_t x = 0; x < W; x++) {
+        const uint64_t pixel = (uint64_t)y * W + x;
+        Vec3 sum = Vec3::zero(), dsum = Vec3::zero(), alb = Vec3::zero(), nrm = Vec3::zero();
+        double depth_sum = 0.0;
+        uint32_t hits = 0;
+        std::vector<std::pair<uint32_t, uint32_t>> votes;  // (id, count) in first-seen order
+        for (uint32_t s = start; s < start + count; s++) {
+          Pcg32 rng = Pcg32::for_pixel_sample(pixel, rp.sample_base + s);
+          Ray ray = scene.camera_ray(x, y, rng);
+          AuxSample aux;
+          Vec3 l = scene.trace(ray, rng, rp, nullptr, &aux);
+          const double lm = luminance(l);
+          if (rp.firefly_clamp > 0.0 && lm > rp.firefly_clamp) {
+            const double k = rp.firefly_clamp / lm;
+            l = l * k;
+            aux.diffuse = aux.diffuse * k;
+          }
+          sum = sum + l;
+          dsum = dsum + aux.diffuse;
+          alb = alb + aux.albedo;
+          nrm = nrm + aux.normal;
+          if (aux.depth > 0.0) {
+            depth_sum += aux.depth;
+            hits++;
+          }
+          bool found = false;
+          for (auto& v : votes)
+            if (v.first == aux.id) {
+              v.second++;
+              found = true;
+              break;
+            }
+          if (!found) votes.push_back({aux.id, 1});
+        }
+        const double inv = count > 0 ? 1.0 / count : 0.0;
+        double* o = film + RRHIP_FILM_CHANNELS * pixel;
+        const Vec3 b = sum * inv, d = dsum * inv, a = alb * inv, nn = nrm * inv;
+        const double bv[3] = {b.x, b.y, b.z}, dv[3] = {d.x, d.y, d.z}, av[3] = {a.x, a.y, a.z},
+                     nv[3] = {nn.x, nn.y, nn.z};
+        for (int c = 0; c < 3; c++) {
+          o[c] = bv[c];
+          o[3 + c] = dv[c];
+          o[6 + c] = std::fmax(bv[c] - dv[c], 0.0);
+          o[9 + c] = av[c];
+          o[12 + c] = nv[c];
+        }
+        o[15] = hits > 0 ? depth_sum / hits : 0.0;
+        uint32_t best_id = 0, best_n = 0;
+        for (const auto& v : votes)
+          if (v.second > best_n) {
+            best_n = v.second;
+            best_id = v.first;
+          }
+        o[16] = (double)best_id;
+        o[17] = best_n * inv;
+      }
+    }
+  };
+  const int nt = nthreads > 0 ? nthreads : 1;
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; t++) pool.emplace_back(work);
+  for (auto& th : pool) th.join();
+  return 0;
+}
+
 // pt::render_adaptive (src/raytracer/pt/mod.rs:285-336): every pixel takes
 // samples 0, 1, ... (plus sample_base) until, from the 32nd sample on and
 // at every 16th, the 95 % confidence half-width of its mean luminance

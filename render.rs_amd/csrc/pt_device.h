@@ -226,6 +226,7 @@ struct BsdfSample {
   V3 wi, f;
   float pdf;
   bool transmitted;
+  bool specular;  // a lobe other than diffuse / fuzz was picked (bxdf.rs:315)
 };
 
 // bxdf.rs:309-374 (same draw order)
@@ -274,6 +275,7 @@ __device__ __forceinline__ bool bsdf_sample(const PtMaterial& m, V3 wo, float et
   s.f = f;
   s.pdf = pdf;
   s.transmitted = transmitted;
+  s.specular = pick >= wd + wf;
   return true;
 }
 

@@ -8,6 +8,12 @@ namespace rrhip {
 
 // Traversal stack entries available per lane to the instance path.
 constexpr uint32_t kPtTraceStack = 48;
+// Film accumulator per pixel (doubles): diffuse sums rgb [0, 3), albedo
+// sums [3, 6), normal sums [6, 9), depth sum [9], samples with a first
+// real hit [10], kFilmIds vote ids [11, 15), their counts [15, 19),
+// samples whose id found no slot [19].
+constexpr uint32_t kFilmIds = 4;
+constexpr uint32_t kFilmStride = 12 + 2 * kFilmIds;
 // Hull crossings and cutout pass-throughs per path (pt/mod.rs:32): they do
 // not count as bounces, so transparent scenes run this many extra rounds.
 constexpr uint32_t kPtMaxSkips = 16;
@@ -36,7 +42,8 @@ hipError_t pt_launch_batch_pass(const PtJob* jobs, uint32_t nj, const PathBuffer
                                  uint32_t total_pix, uint32_t bounces, hipStream_t st);
 // Adds each pixel's pass samples (firefly-clamped) into the f64
 // accumulator in sample order (deterministic, no atomics).
-hipError_t pt_launch_resolve(const PathBuffers& P, const PtParams& prm, double* accum, hipStream_t st);
+hipError_t pt_launch_resolve(const PathBuffers& P, const PtParams& prm, double* accum, double* film,
+                             hipStream_t st);
 
 // Per-block partial sums of relMSE between two accumulators (`blocks`
 // doubles into `partial`, <= 1024 blocks); the caller adds them.

@@ -37,6 +37,11 @@ constexpr uint32_t kFlagSuffix = 1u << 10; // training path past its termination
 // hull crossings / cutout pass-throughs (bits 15-19, 0..16).
 constexpr uint32_t kStackShift = 11, kSkipShift = 15, kStackSkipMask = 0x1FFu << 11;
 constexpr uint32_t kMaxSkips = 16;  // pt/mod.rs:32 MAX_PRESENCE_SKIPS
+// Film mode: the first real hit's AOVs are recorded (bit 20); the first
+// continuation has been sampled (bit 21) and was specular-like (bit 22),
+// pt/mod.rs:457-470, 959-961.
+constexpr uint32_t kFlagAux = 1u << 20, kFlagFirstSet = 1u << 21, kFlagFirstSpec = 1u << 22;
+constexpr uint32_t kFlagKeep = kFlagTrain | kFlagSuffix | kFlagAux | kFlagFirstSet | kFlagFirstSpec;
 
 // pt/mod.rs:33-69 MediumStack: up to 8 medium indices, 8 bits each,
 // innermost at depth - 1; the atmosphere is the implicit bottom.
@@ -118,6 +123,12 @@ __device__ __forceinline__ void init_path(const PtSceneView& S, const PathBuffer
   P.flags[slot] = flags;
   P.q0[slot] = slot;
   if (P.mstk) P.mstk[slot] = make_uint2(0u, 0u);
+  if (P.ld) {
+    P.ld[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+    P.aux0[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+    P.aux1[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+    P.aux_id[slot] = 0u;
+  }
 }
 
 // sample0_dev (when set) supplies the pass's first sample index from device
@@ -212,6 +223,7 @@ struct ShadowOut {
   V3 c;        // contribution before visibility (f cos L weight / pdf)
   int medium;  // transparent scenes: the medium at the ray's origin (the walk applies its transmittance)
   uint2 seed;  // transparent scenes: random stream of the blocker walk
+  bool diff;   // film mode: the contribution belongs to the diffuse AOV
 };
 
 // Evaluates the BSDF towards world direction wi: f * cos(wi) and the pdf
@@ -352,7 +364,9 @@ __device__ __forceinline__ bool sample_light(const PtSceneView& S, const PtLight
 // cont (continue in the next queue), shadow (+ so), shadow2 (+ se, an
 // equiangular connection in a medium), query (+ nq, qbeta).
 // TR: the scene is transparent (medium stack, hulls, cutouts, shadow walks).
-template <bool NRC, bool TR>
+// FILM: also the diffuse split of the radiance and the first real hit's
+// AOVs (pt/mod.rs:457-470, 559-573, 690-700, 959-961).
+template <bool NRC, bool TR, bool FILM = false>
 __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffers& P, const PtParams& prm,
                                           const NrcPathBuffers& N, uint32_t slot, bool& cont, bool& shadow,
                                           bool& shadow2, ShadowOut& se, bool& query, ShadowOut& so, NrcQuery& nq,
@@ -377,6 +391,23 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
       skips = (flags >> kSkipShift) & 0x1Fu;
     }
     const int cur = NRC ? -1 : stack_top(mst, sdepth, S.atmosphere);  // the segment's medium
+    // Film: radiance of this path that goes to the diffuse AOV (all of it
+    // until a first continuation is sampled specular-like); aux_set: the
+    // first real hit is recorded; travel: distance to it so far.
+    V3 Ld = v3(0.f, 0.f, 0.f);
+    const bool diff = FILM && !(flags & kFlagFirstSpec);
+    bool aux_set = true;
+    float travel = 0.f;
+    if (FILM) {
+      const float4 ld4 = P.ld[slot];
+      Ld = v3(ld4.x, ld4.y, ld4.z);
+      aux_set = (flags & kFlagAux) != 0;
+      if (!aux_set) travel = P.aux1[slot].w;
+    }
+    const auto emit = [&](V3 c) {
+      Lr = Lr + c;
+      if (FILM && diff) Ld = Ld + c;
+    };
     constexpr bool walk = TR;  // shadow rays pick up media transmittance on their walk
     bool passed = false;                   // hull crossing or cutout pass-through
     // The medium along this segment (pt/mod.rs:486-633): one light pick
@@ -430,7 +461,17 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
       if (ev.scatter) {
         in_medium = true;
         const V3 p = ro + rd * ev.t;
-        if (maxc(v3(md.emission)) > 0.f) Lr = Lr + beta * v3(md.emission);
+        if (FILM && !aux_set) {  // pt/mod.rs:559-570
+          aux_set = true;
+          flags |= kFlagAux;
+          const float st[3] = {md.sigma_t[0], md.sigma_t[1], md.sigma_t[2]};
+          P.aux0[slot] = make_float4(md.sigma_s[0] / fmaxf(st[0], 1e-9f), md.sigma_s[1] / fmaxf(st[1], 1e-9f),
+                                     md.sigma_s[2] / fmaxf(st[2], 1e-9f), travel + ev.t);
+          P.aux1[slot] = make_float4(-rd.x, -rd.y, -rd.z, 0.f);
+          P.aux_id[slot] = 0u;
+        }
+        if (FILM) flags |= kFlagFirstSet;  // a medium scatter is a diffuse-like first continuation
+        if (maxc(v3(md.emission)) > 0.f) emit(beta * v3(md.emission));
         const V3 wo = -rd;
         if (has_pick) {
           const auto eval = [&](V3 wi, V3& f, float& pdf) {
@@ -467,7 +508,7 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
         P.o[slot] = make_float4(p.x, p.y, p.z, pdf);
         P.d[slot] = make_float4(wi.x, wi.y, wi.z, a0);
         P.prev[slot] = make_float4(p.x, p.y, p.z, asum);
-        flags = depth | (flags & (kFlagTrain | kFlagSuffix));
+        flags = depth | (flags & kFlagKeep);
       } else {
         absorbed = !(maxc(beta) > 0.f);
       }
@@ -479,15 +520,29 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
       if (S.dome >= 0) {
         float w = 1.f;
         if (!from_camera) w = power_heuristic(prev_pdf, kInv4Pi * ls_pmf(S, prev_origin, (uint32_t)S.dome));
-        Lr = Lr + beta * v3(S.lights[S.dome].radiance) * w;
+        emit(beta * v3(S.lights[S.dome].radiance) * w);
       } else {
-        Lr = Lr + beta * v3(S.background);
+        emit(beta * v3(S.background));
       }
     } else {
       const uint32_t hm = P.hit_m[slot];
       const PtMaterial m = S.materials[hm & 0x7FFFFFFFu];
       const bool entering = (hm >> 31) != 0;
       const V3 p = ro + rd * hn.w;
+      if (FILM && !aux_set) {
+        if (TR && m.hull && skips < kMaxSkips) {
+          travel += hn.w;  // a hull crossing is looked through (pt/mod.rs:671)
+          P.aux1[slot].w = travel;
+        } else {  // the first real hit, cutouts included (pt/mod.rs:688-700)
+          aux_set = true;
+          flags |= kFlagAux;
+          const V3 ng = v3(hn.x, hn.y, hn.z);
+          const V3 nf = dot(ng, rd) <= 0.f ? ng : -ng;
+          P.aux0[slot] = make_float4(m.film_albedo[0], m.film_albedo[1], m.film_albedo[2], travel + hn.w);
+          P.aux1[slot] = make_float4(nf.x, nf.y, nf.z, 0.f);
+          P.aux_id[slot] = m.id;
+        }
+      }
       // Volume hull crossings toggle the medium (pt/mod.rs:662-683); a
       // cutout passes the ray with probability 1 - presence
       // (pt/mod.rs:702-710). Neither counts as a bounce.
@@ -529,7 +584,7 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
           const float pick = ls_pmf(S, prev_origin, (uint32_t)m.area_light);
           w = power_heuristic(prev_pdf, light_pdf_solid_angle(L, prev_origin, p) * pick);
         }
-        Lr = Lr + beta * em * w;
+        emit(beta * em * w);
       }
       bool scatter = true;
       if constexpr (NRC) {
@@ -609,6 +664,7 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
       if (scatter && bsdf_sample(m, wo_l, eta, rng, bs)) {
         const V3 wi = to_world(fr, bs.wi);
         beta = beta * bs.f * (fabsf(bs.wi.z) / bs.pdf);
+        if (FILM && !(flags & kFlagFirstSet)) flags |= kFlagFirstSet | (bs.specular ? kFlagFirstSpec : 0u);
         const V3 org = p + nrm * (bs.transmitted ? -kRayOffset : kRayOffset);
         const V3 nd = normalize(wi);
         if (TR && bs.transmitted && m.interior >= 0) {  // pt/mod.rs:969-975
@@ -631,7 +687,7 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
         P.o[slot] = make_float4(org.x, org.y, org.z, bs.pdf);
         P.d[slot] = make_float4(nd.x, nd.y, nd.z, a0);
         P.prev[slot] = make_float4(p.x, p.y, p.z, asum);
-        flags = depth | (flags & (kFlagTrain | kFlagSuffix));  // from_camera cleared
+        flags = depth | (flags & kFlagKeep);  // from_camera cleared
       }
     }
     if (TR && (shadow || shadow2)) {
@@ -640,6 +696,9 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
     }
     so.time = b4.w;
     se.time = b4.w;
+    so.diff = diff;
+    se.diff = diff;
+    if (FILM) P.ld[slot] = make_float4(Ld.x, Ld.y, Ld.z, 0.f);
     if (TR) {
       P.mstk[slot] = mst;
       flags = (flags & ~kStackSkipMask) | (sdepth << kStackShift) | (skips << kSkipShift);
@@ -653,7 +712,9 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
 // Appends a lane's shadow record (all lanes of the wave call it): the
 // first ray is `a` (or `b` alone); with both, `b` goes to the second
 // arrays and sc.w = 1 marks the record.
-template <bool TR>
+// sc.w: bit 0 = the record has a second ray, bit 1 (FILM) = its
+// contributions go to the diffuse AOV as well.
+template <bool TR, bool FILM = false>
 __device__ __forceinline__ void push_shadow(const PathBuffers& P, bool act, uint32_t slot, bool s1, const ShadowOut& a,
                                             bool s2, const ShadowOut& b) {
   const bool any = act && (s1 || s2);
@@ -666,7 +727,7 @@ __device__ __forceinline__ void push_shadow(const PathBuffers& P, bool act, uint
   P.sq[idx] = slot;
   P.so[idx] = make_float4(f.o.x, f.o.y, f.o.z, f.tmax);
   P.sd[idx] = make_float4(f.d.x, f.d.y, f.d.z, walk ? (float)f.medium : 0.f);
-  P.sc[idx] = make_float4(f.c.x, f.c.y, f.c.z, both ? 1.f : 0.f);
+  P.sc[idx] = make_float4(f.c.x, f.c.y, f.c.z, (both ? 1.f : 0.f) + (FILM && f.diff ? 2.f : 0.f));
   if (walk) P.srng[idx] = f.seed;
   if (P.stime) P.stime[idx] = f.time;
   if (both) {
@@ -678,7 +739,7 @@ __device__ __forceinline__ void push_shadow(const PathBuffers& P, bool act, uint
 
 // Three waves per SIMD (<= 168 VGPRs): shading is latency-bound on
 // scattered path-state loads, so occupancy matters more than registers.
-template <bool NRC, bool TR>
+template <bool NRC, bool TR, bool FILM = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void shade_kernel(PtSceneView S, PathBuffers P, PtParams prm,
                                                       const uint32_t* __restrict__ q, uint32_t* __restrict__ qnext,
                                                       NrcPathBuffers N) {
@@ -695,9 +756,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     uint32_t slot = 0;
     if (act) {
       slot = q[i];
-      shade_one<NRC, TR>(S, P, prm, N, slot, cont, shadow, shadow2, se, query, so, nq, qbeta);
+      shade_one<NRC, TR, FILM>(S, P, prm, N, slot, cont, shadow, shadow2, se, query, so, nq, qbeta);
     }
-    push_shadow<TR>(P, act, slot, shadow, so, shadow2, se);
+    push_shadow<TR, FILM>(P, act, slot, shadow, so, shadow2, se);
     const uint32_t nidx = wave_append(&P.counts[1], act && cont);
     if (act && cont) qnext[nidx] = slot;
     if constexpr (NRC) {
@@ -757,7 +818,8 @@ template <bool WALK, bool MOTION>
 __device__ __forceinline__ void shadow_one_t(const PtSceneView& S, const PathBuffers& P, uint32_t i, uint32_t* stk) {
   const float4 o = P.so[i], d = P.sd[i];
   float4 c = P.sc[i];
-  const bool two = c.w != 0.f;
+  const uint32_t cw = (uint32_t)c.w;  // bit 0: second ray, bit 1: diffuse AOV
+  const bool two = (cw & 1u) != 0;
   const float time = MOTION ? P.stime[i] : 0.f;
   float4 c2 = make_float4(0.f, 0.f, 0.f, 0.f);
   bool v1, v2 = false;
@@ -787,18 +849,22 @@ __device__ __forceinline__ void shadow_one_t(const PtSceneView& S, const PathBuf
   }
   if (!v1 && !v2) return;
   const uint32_t slot = P.sq[i];
-  float4 l = P.L[slot];
-  if (v1) {
-    l.x += c.x;
-    l.y += c.y;
-    l.z += c.z;
-  }
-  if (v2) {
-    l.x += c2.x;
-    l.y += c2.y;
-    l.z += c2.z;
-  }
-  P.L[slot] = l;
+  const auto add = [&](float4* buf) {
+    float4 l = buf[slot];
+    if (v1) {
+      l.x += c.x;
+      l.y += c.y;
+      l.z += c.z;
+    }
+    if (v2) {
+      l.x += c2.x;
+      l.y += c2.y;
+      l.z += c2.z;
+    }
+    buf[slot] = l;
+  };
+  add(P.L);
+  if (P.ld && (cw & 2u)) add(P.ld);
 }
 
 __device__ __forceinline__ void shadow_one(const PtSceneView& S, const PathBuffers& P, uint32_t i, uint32_t* stk) {
@@ -888,6 +954,80 @@ __device__ __forceinline__ void resolve_one(const PathBuffers& P, const PtParams
 __global__ __launch_bounds__(kBlock) void resolve_kernel(PathBuffers P, PtParams prm, double* __restrict__ accum) {
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < prm.npix; c += stride) resolve_one(P, prm, accum, c, 0);
+}
+
+// Film accumulators of column col's pass samples, in sample order
+// (pt/mod.rs:200-221): the diffuse split with the sample's firefly scale,
+// albedo and normal sums, the depth sum and count over samples with a
+// first real hit, and the id vote (kFilmIds slots of id and count, then
+// the count of samples whose id found no slot).
+__device__ __forceinline__ void resolve_film_one(const PathBuffers& P, const PtParams& prm, double* film,
+                                                 uint32_t col) {
+  const uint32_t px = prm.live ? prm.live[col] : col;
+  double* a = film + (size_t)kFilmStride * px;
+  double ds[3] = {0, 0, 0}, al[3] = {0, 0, 0}, nm[3] = {0, 0, 0}, dep = 0.0, hits = 0.0, over = 0.0;
+  uint32_t ids[kFilmIds], cnt[kFilmIds];
+#pragma unroll
+  for (int j = 0; j < (int)kFilmIds; j++) {
+    ids[j] = (uint32_t)a[11 + j];
+    cnt[j] = (uint32_t)a[11 + kFilmIds + j];
+  }
+  for (uint32_t k = 0; k < prm.spp_pass; k++) {
+    const size_t slot = (size_t)k * prm.npix + col;
+    const float4 l4 = P.L[slot], d4 = P.ld[slot], x0 = P.aux0[slot], x1 = P.aux1[slot];
+    const uint32_t id = P.aux_id[slot];
+    double sc = 1.0;
+    const double lm = 0.2126 * (double)l4.x + 0.7152 * (double)l4.y + 0.0722 * (double)l4.z;
+    if (prm.firefly_clamp > 0.f && lm > (double)prm.firefly_clamp) sc = (double)prm.firefly_clamp / lm;
+    ds[0] += (double)d4.x * sc;
+    ds[1] += (double)d4.y * sc;
+    ds[2] += (double)d4.z * sc;
+    al[0] += x0.x;
+    al[1] += x0.y;
+    al[2] += x0.z;
+    nm[0] += x1.x;
+    nm[1] += x1.y;
+    nm[2] += x1.z;
+    if (x0.w > 0.f) {
+      dep += x0.w;
+      hits += 1.0;
+    }
+    bool placed = false;
+#pragma unroll
+    for (int j = 0; j < (int)kFilmIds; j++) {
+      if (!placed && cnt[j] != 0 && ids[j] == id) {
+        cnt[j]++;
+        placed = true;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < (int)kFilmIds; j++) {
+      if (!placed && cnt[j] == 0) {
+        ids[j] = id;
+        cnt[j] = 1;
+        placed = true;
+      }
+    }
+    if (!placed) over += 1.0;
+  }
+  for (int c = 0; c < 3; c++) {
+    a[c] += ds[c];
+    a[3 + c] += al[c];
+    a[6 + c] += nm[c];
+  }
+  a[9] += dep;
+  a[10] += hits;
+#pragma unroll
+  for (int j = 0; j < (int)kFilmIds; j++) {
+    a[11 + j] = (double)ids[j];
+    a[11 + kFilmIds + j] = (double)cnt[j];
+  }
+  a[11 + 2 * kFilmIds] += over;
+}
+
+__global__ __launch_bounds__(kBlock) void resolve_film_kernel(PathBuffers P, PtParams prm, double* __restrict__ film) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < prm.npix; c += stride) resolve_film_one(P, prm, film, c);
 }
 
 // ---- batched jobs: one set of launches over the path slots of many scenes ----
@@ -1088,6 +1228,10 @@ hipError_t pt_launch_bounce(const PtSceneView& S, const PathBuffers& P, const Pt
   }
   if (N.q)
     hipLaunchKernelGGL((shade_kernel<true, false>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
+  else if (P.ld && S.transparent)
+    hipLaunchKernelGGL((shade_kernel<false, true, true>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
+  else if (P.ld)
+    hipLaunchKernelGGL((shade_kernel<false, false, true>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
   else if (S.transparent)
     hipLaunchKernelGGL((shade_kernel<false, true>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
   else
@@ -1114,8 +1258,10 @@ hipError_t pt_launch_batch_pass(const PtJob* jobs, uint32_t nj, const PathBuffer
   return hipGetLastError();
 }
 
-hipError_t pt_launch_resolve(const PathBuffers& P, const PtParams& prm, double* accum, hipStream_t st) {
+hipError_t pt_launch_resolve(const PathBuffers& P, const PtParams& prm, double* accum, double* film,
+                             hipStream_t st) {
   hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(prm.npix)), dim3(kBlock), 0, st, P, prm, accum);
+  if (film) hipLaunchKernelGGL(resolve_film_kernel, dim3(grid_for(prm.npix)), dim3(kBlock), 0, st, P, prm, film);
   return hipGetLastError();
 }
 

@@ -93,6 +93,12 @@ PtMaterial make_material(const rrhip_material_desc& m) {
     if (g.w[i] > 0.f) g.sort_key += 1u << i;
   g.nrc_rough = 1.f;
   if (ws > 0 || wg > 0) g.nrc_rough = (float)(wg > ws ? m.glass_roughness : m.specular_roughness);
+  // Albedo AOV (pt/mod.rs:690-696): diffuse, the primary specular F0 when
+  // that lobe is active, refraction; each channel capped at 1.
+  const D3 fa = add(add(mul(d3(m.diffuse_color), m.diffuse_gain), lum(f0) > 1e-6 ? f0 : D3{0, 0, 0}),
+                    mul(d3(m.refraction_color), m.glass_gain));
+  put(g.film_albedo, D3{std::min(fa.x, 1.0), std::min(fa.y, 1.0), std::min(fa.z, 1.0)});
+  g.id = m.id;
   g.interior = (int32_t)m.interior - 1;
   g.presence = (float)std::min(std::max(1.0 - m.cutout, 0.0), 1.0);
   g.hull = g.interior >= 0 && !g.has_lobes && std::max(g.emission[0], std::max(g.emission[1], g.emission[2])) <= 0.f;

@@ -59,6 +59,10 @@ struct rrhip_session {
   uint32_t ad_cur = 0;
   uint32_t nlive = 0;
   bool pruned = false;
+  // Film mode: per-slot AOV buffers live with the path buffers; the film
+  // accumulator (kFilmStride doubles per pixel) here.
+  bool film = false;
+  double* d_film = nullptr;
 };
 
 namespace {
@@ -95,6 +99,7 @@ void drop_graphs(rrhip_session* s) {
 void free_session(rrhip_session* s) {
   drop_graphs(s);
   free_paths(s);
+  if (s->d_film) (void)hipFree(s->d_film);
   for (void* p : s->mem) (void)hipFree(p);
   s->mem.clear();
   delete s;
@@ -130,6 +135,12 @@ hipError_t alloc_paths(rrhip_session* s, uint64_t n) {
     if (e == hipSuccess) e = alloc_in(m, &P.srng, n);
   }
   if (e == hipSuccess && s->scene->pt.motion) e = alloc_in(m, &P.stime, n);
+  if (e == hipSuccess && s->film) {
+    e = alloc_in(m, &P.ld, n);
+    if (e == hipSuccess) e = alloc_in(m, &P.aux0, n);
+    if (e == hipSuccess) e = alloc_in(m, &P.aux1, n);
+    if (e == hipSuccess) e = alloc_in(m, &P.aux_id, n);
+  }
   if (e == hipSuccess) e = alloc_in(m, &P.counts, 4);
   if (e == hipSuccess) s->path_cap = n;
   return e;
@@ -251,7 +262,7 @@ hipError_t enqueue_pass(rrhip_session* s, uint32_t n, uint32_t ncol, const uint3
       nrc->pending = e == hipSuccess;
     }
   }
-  if (e == hipSuccess) e = pt_launch_resolve(s->P, prm, s->d_accum, st);
+  if (e == hipSuccess) e = pt_launch_resolve(s->P, prm, s->d_accum, s->film ? s->d_film : nullptr, st);
   return e;
 }
 
@@ -419,8 +430,80 @@ int rrhip_load_accum(rrhip_session* s, const double* in) {
   } else {
     RRHIP_HIP_TRY(hipMemsetAsync(s->d_accum, 0, bytes, s->scene->ctx->stream));
   }
+  if (s->film) {
+    const uint64_t fbytes = (uint64_t)s->prm.npix * kFilmStride * sizeof(double);
+    RRHIP_HIP_TRY(hipMemsetAsync(s->d_film, 0, fbytes, s->scene->ctx->stream));
+  }
   RRHIP_HIP_TRY(hipStreamSynchronize(s->scene->ctx->stream));
   s->pruned = false;  // every pixel samples again until the next check
+  return RRHIP_OK;
+}
+
+int rrhip_session_set_film(rrhip_session* s, int enable) {
+  if (int st = check_session(s)) return st;
+  if (enable && s->nrc)
+    return fail(RRHIP_ERR_UNSUPPORTED, "rrhip_session_set_film: not with a radiance cache attached");
+  RRHIP_HIP_TRY(hipSetDevice(s->scene->ctx->device));
+  RRHIP_HIP_TRY(hipStreamSynchronize(s->scene->ctx->stream));
+  drop_graphs(s);  // captured passes hold the old buffers
+  if (s->d_film) {
+    (void)hipFree(s->d_film);
+    s->d_film = nullptr;
+  }
+  s->film = enable != 0;
+  hipError_t e = alloc_paths(s, s->path_cap);
+  const uint64_t fbytes = (uint64_t)s->prm.npix * kFilmStride * sizeof(double);
+  if (e == hipSuccess && s->film) e = hipMalloc((void**)&s->d_film, fbytes);
+  if (e == hipSuccess && s->film) e = hipMemset(s->d_film, 0, fbytes);
+  if (e != hipSuccess) {
+    // leave a usable plain session behind
+    (void)hipGetLastError();
+    s->film = false;
+    if (s->d_film) (void)hipFree(s->d_film);
+    s->d_film = nullptr;
+    if (alloc_paths(s, s->path_cap) != hipSuccess) return fail_hip(e, "rrhip_session_set_film (session unusable)");
+    return fail_hip(e, "rrhip_session_set_film");
+  }
+  return RRHIP_OK;
+}
+
+int rrhip_read_film(rrhip_session* s, double* out) {
+  if (int st = check_session(s)) return st;
+  if (!out) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_read_film: null output");
+  if (!s->film) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_read_film: film mode is off");
+  RRHIP_HIP_TRY(hipSetDevice(s->scene->ctx->device));
+  const uint64_t npix = s->prm.npix;
+  std::vector<double> acc(npix * RRHIP_ACCUM_STRIDE), film(npix * kFilmStride);
+  hipStream_t st = s->scene->ctx->stream;
+  RRHIP_HIP_TRY(hipMemcpyAsync(acc.data(), s->d_accum, acc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  RRHIP_HIP_TRY(hipMemcpyAsync(film.data(), s->d_film, film.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  RRHIP_HIP_TRY(hipStreamSynchronize(st));
+  // pt/mod.rs:223-274: sample means; specular = max(beauty - diffuse, 0);
+  // depth over the samples with a first real hit; the most voted id (the
+  // first of equal counts) and its share of the samples.
+  for (uint64_t p = 0; p < npix; p++) {
+    const double* a = &acc[p * RRHIP_ACCUM_STRIDE];
+    const double* f = &film[p * kFilmStride];
+    double* o = out + p * RRHIP_FILM_CHANNELS;
+    const double n = a[6] > 0.0 ? a[6] : 1.0;
+    for (int c = 0; c < 3; c++) {
+      o[c] = a[c] / n;
+      o[3 + c] = f[c] / n;
+      o[6 + c] = std::max(o[c] - o[3 + c], 0.0);
+      o[9 + c] = f[3 + c] / n;
+      o[12 + c] = f[6 + c] / n;
+    }
+    o[15] = f[10] > 0.0 ? f[9] / f[10] : 0.0;
+    double best_id = 0.0, best_n = 0.0;
+    for (uint32_t j = 0; j < kFilmIds; j++) {
+      if (f[11 + kFilmIds + j] > best_n) {
+        best_n = f[11 + kFilmIds + j];
+        best_id = f[11 + j];
+      }
+    }
+    o[16] = best_id;
+    o[17] = best_n / n;
+  }
   return RRHIP_OK;
 }
 
@@ -518,6 +601,7 @@ extern "C" int rrhip_session_set_nrc(rrhip_session* s, rrhip_nrc* nrc) {
   if (sc->pt.atmosphere >= 0 || sc->pt.transparent)
     return fail(RRHIP_ERR_UNSUPPORTED,
                 "rrhip_session_set_nrc: the radiance cache does not model participating media or cutouts");
+  if (s->film) return fail(RRHIP_ERR_UNSUPPORTED, "rrhip_session_set_nrc: not in film mode");
   const uint64_t ntrain = nrc->cfg.train_paths;
   const uint64_t cap = s->slots + ntrain;
   if (cap >= (1ull << 31)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_set_nrc: path slots exceed 2^31");

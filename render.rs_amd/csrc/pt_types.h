@@ -37,7 +37,10 @@ struct alignas(16) PtMaterial {
   int32_t interior;     // medium inside closed surfaces of this material (-1: none)
   float presence;       // probability that a ray stops at the surface (pbr.rs:39)
   uint32_t hull;        // interior medium, no lobes, no emission: an invisible volume boundary
-  uint32_t pad[2];
+  uint32_t id;          // object id of the id AOV (Material::id), 0 = none
+  uint32_t pad;
+  float film_albedo[3]; // albedo AOV: diffuse + primary specular F0 + refraction, each <= 1
+  float pad2;
 };
 
 enum : uint32_t { kLightPoint = 0, kLightDistant = 1, kLightRect = 2, kLightSphere = 3, kLightDisk = 4, kLightDome = 5 };
@@ -195,6 +198,12 @@ struct PathBuffers {
   uint32_t* job;    // batches: job of each slot (null in sessions)
   uint32_t* qs;     // sorted-queue mode: the live queue grouped by shading class
   uint32_t* sort_hist;  // sorted-queue mode: per-block class counts
+  // Film (AOV) mode, null otherwise: radiance whose first continuation was
+  // not specular-like, and the first real hit's data
+  float4* ld;       // diffuse-split radiance rgb
+  float4* aux0;     // albedo rgb, depth (0 until set)
+  float4* aux1;     // shading normal facing the viewer xyz, distance travelled before the first real hit
+  uint32_t* aux_id; // object id of the first real hit
 };
 
 // One radiance-cache query / training sample (the 64-byte rrhip_nrc_query

@@ -43,6 +43,7 @@ EXPORTED = [
     "rrhip_batch_create", "rrhip_batch_destroy", "rrhip_batch_render", "rrhip_batch_read_accum",
     "rrhip_batch_get_stats", "rrhip_session_set_launch_mode", "rrhip_session_autotune",
     "rrhip_session_set_adaptive", "rrhip_session_average_spp",
+    "rrhip_session_set_film", "rrhip_read_film",
 ]
 
 
@@ -92,7 +93,7 @@ class MaterialDesc(ctypes.Structure):
                 ("glass_gain", ctypes.c_double), ("glass_ior", ctypes.c_double),
                 ("glass_roughness", ctypes.c_double), ("refraction_color", D3),
                 ("glow", D3), ("emission", D3), ("area_light", ctypes.c_int32),
-                ("interior", ctypes.c_uint32), ("cutout", ctypes.c_double)]
+                ("interior", ctypes.c_uint32), ("id", ctypes.c_uint32), ("cutout", ctypes.c_double)]
 
 
 class LightDesc(ctypes.Structure):
@@ -187,6 +188,10 @@ class NrcInfo(ctypes.Structure):
 
 LIGHT_POINT, LIGHT_DISTANT, LIGHT_RECT, LIGHT_SPHERE, LIGHT_DISK, LIGHT_DOME = range(6)
 ACCUM_STRIDE = 8
+# rrhip_read_film columns (RRHIP_FILM_CHANNELS)
+FILM_CHANNELS = 18
+FILM_COLUMNS = {"beauty": slice(0, 3), "diffuse": slice(3, 6), "specular": slice(6, 9),
+                "albedo": slice(9, 12), "normal": slice(12, 15), "depth": 15, "id": 16, "coverage": 17}
 
 
 class SceneInfo(ctypes.Structure):
@@ -266,6 +271,8 @@ def load():
         "rrhip_session_set_launch_mode": (i32, [vp, u32]),
         "rrhip_session_autotune": (i32, [vp, u32, ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_float)]),
         "rrhip_session_set_adaptive": (i32, [vp, ctypes.c_double]),
+        "rrhip_session_set_film": (i32, [vp, i32]),
+        "rrhip_read_film": (i32, [vp, vp]),
         "rrhip_session_average_spp": (i32, [vp, ctypes.POINTER(ctypes.c_double)]),
         "rrhip_batch_create": (i32, [vp, u32, vp, u32, ctypes.POINTER(vp)]),
         "rrhip_batch_destroy": (i32, [vp]),
@@ -492,6 +499,18 @@ class Session:
         luminance is below tolerance * max(mean, 0.05), checked before
         samples 32, 48, 64, ...; 0 turns it off."""
         _check(load().rrhip_session_set_adaptive(self.h, float(tolerance)))
+
+    def set_film(self, enable=True):
+        """Film mode (pt::render_film): later samples also accumulate the
+        AOVs; the film is cleared now and with the accumulator."""
+        _check(load().rrhip_session_set_film(self.h, 1 if enable else 0))
+
+    def film(self):
+        """(npix, FILM_CHANNELS) f64: beauty, diffuse, specular, albedo,
+        normal, depth, dominant id, its coverage (FILM_COLUMNS)."""
+        out = np.empty((self.scene.npix, FILM_CHANNELS), np.float64)
+        _check(load().rrhip_read_film(self.h, _ptr(out)))
+        return out
 
     def average_spp(self):
         """Mean samples per pixel (WfSession::average_spp, renderer.rs:587-591)."""

@@ -23,9 +23,10 @@ def material(diffuse_gain=1.0, diffuse_color=(0.5, 0.5, 0.5), diffuse_roughness=
              specular_ior=1.5, clearcoat_gain=0.0, clearcoat_roughness=0.1, fuzz_gain=0.0,
              fuzz_color=(1, 1, 1), glass_gain=0.0, glass_ior=1.5, glass_roughness=0.05,
              refraction_color=(1, 1, 1), glow=(0, 0, 0), emission=(0, 0, 0), area_light=-1, presence=1.0,
-             interior=None):
-    """PbrParams defaults of pbr.rs:50-75; presence (pbr.rs:39) and the
-    interior medium index (material.rs:37, None: no medium)."""
+             interior=None, object_id=0):
+    """PbrParams defaults of pbr.rs:50-75; presence (pbr.rs:39), the
+    interior medium index (material.rs:37, None: no medium) and the object
+    id of the id AOV (material.rs:39, 0: none)."""
     m = rrhip.MaterialDesc()
     m.diffuse_gain = diffuse_gain
     m.diffuse_color = _d3(diffuse_color)
@@ -47,6 +48,7 @@ def material(diffuse_gain=1.0, diffuse_color=(0.5, 0.5, 0.5), diffuse_roughness=
     m.area_light = area_light
     m.cutout = 1.0 - presence
     m.interior = 0 if interior is None else interior + 1
+    m.id = object_id
     return m
 
 
