@@ -235,13 +235,18 @@ typedef struct rrhip_scene_desc {
   rrhip_camera_desc camera;
   double background[3]; /* miss radiance when no dome light */
   /* Participating media; `atmosphere` (-1: none) is the medium every ray
-   * travels in (render.rs Scene::atmosphere, src/scene/mod.rs). Point
-   * lights inside an atmosphere (render.rs's equiangular strategy,
-   * pt/mod.rs:507-560) are not supported: upload fails with
-   * RRHIP_ERR_UNSUPPORTED. */
+   * travels in (render.rs Scene::atmosphere, src/scene/mod.rs). */
   uint32_t num_media;
   int32_t atmosphere;
   const rrhip_medium_desc* media;
+  /* Lat-long environment map of the dome light (render.rs EnvMap,
+   * src/scene/envmap.rs): env_width x env_height texels of linear rgb
+   * radiance (3 floats each), row-major from the +y pole; a direction d
+   * maps to u = atan2(d.x, -d.z) / 2 pi (wrapped to [0, 1)) and
+   * v = acos(d.y) / pi; scaled by the dome's color * intensity. NULL: a
+   * constant dome. Requires a dome light. */
+  uint32_t env_width, env_height;
+  const float* env_rgb;
 } rrhip_scene_desc;
 
 /* Path-tracing parameters (render.rs constants, src/raytracer/pt/mod.rs:

@@ -161,6 +161,8 @@ def _pt_sigs():
         "orc_render": (i32, [vp, vp, u32, u32, vp, i32, vp]),
         "orc_render_adaptive": (i32, [vp, vp, u32, f64, vp, i32]),
         "orc_render_film": (i32, [vp, vp, u32, u32, vp, i32]),
+        "orc_env_pdf_eval_n": (None, [u32, u32, vp, vp, u64, vp, vp]),
+        "orc_env_sample_n": (None, [u32, u32, vp, vp, u64, vp, vp, vp, ctypes.POINTER(f64)]),
         "orc_camera_rays": (None, [vp, u64, vp, vp, vp, vp]),
         "orc_bsdf_eval": (None, [vp, vp, vp, f64, vp, vp]),
         "orc_bsdf_eval_n": (None, [vp, vp, vp, f64, u64, vp, vp]),
@@ -203,6 +205,32 @@ def render_film(desc, params, start, count, threads=None):
     lib.orc_render_film(ctypes.addressof(desc), ctypes.addressof(params), start, count, _p(film),
                         threads or default_threads())
     return film
+
+
+def env_pdf_eval(rgb, dirs):
+    """EnvMap::pdf and ::eval (envmap.rs:84-148) of the (h, w, 3) map at
+    directions dirs (n x 3): (pdf (n,), radiance (n, 3))."""
+    lib = _pt_sigs()
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    dirs = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    n = dirs.shape[0]
+    pdf, rad = np.zeros(n), np.zeros((n, 3))
+    lib.orc_env_pdf_eval_n(rgb.shape[1], rgb.shape[0], _p(rgb), _p(dirs), n, _p(pdf), _p(rad))
+    return pdf, rad
+
+
+def env_sample(rgb, u):
+    """EnvMap::sample (envmap.rs:99-131) for uniform pairs u (n x 2):
+    (dirs (n, 3), radiance (n, 3), pdf (n,), mean luminance)."""
+    lib = _pt_sigs()
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    u = np.ascontiguousarray(u, np.float64).reshape(-1, 2)
+    n = u.shape[0]
+    dirs, rad, pdf = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+    ml = ctypes.c_double()
+    lib.orc_env_sample_n(rgb.shape[1], rgb.shape[0], _p(rgb), _p(u), n, _p(dirs), _p(rad), _p(pdf),
+                         ctypes.byref(ml))
+    return dirs, rad, pdf, ml.value
 
 
 def camera_rays(desc, xy, samples):

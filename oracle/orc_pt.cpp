@@ -434,6 +434,82 @@ uint32_t build_recursive(const std::vector<BuildEntry>& e, std::vector<size_t>& 
 
 }  // namespace
 
+// ---- EnvMap (envmap.rs) -----------------------------------------------------
+
+static double texel_lum(const float* c) { return 0.2126 * c[0] + 0.7152 * c[1] + 0.0722 * c[2]; }
+
+void EnvMap::from_data(uint32_t w, uint32_t h, const float* rgb) {  // envmap.rs:36-54
+  width = w;
+  height = h;
+  data.assign(rgb, rgb + 3ull * w * h);
+  marginal.assign(h + 1, 0.0);
+  conditional.assign((size_t)(w + 1) * h, 0.0);
+  for (uint32_t y = 0; y < h; y++) {
+    const double sin_theta = std::sin(PI * (y + 0.5) / h);
+    const size_t base = (size_t)y * (w + 1);
+    double acc = 0.0;
+    for (uint32_t x = 0; x < w; x++) {
+      acc += texel_lum(&data[3 * ((size_t)y * w + x)]) * sin_theta;
+      conditional[base + x + 1] = acc;
+    }
+    marginal[y + 1] = marginal[y] + acc;
+  }
+  total = std::fmax(marginal[h], 1e-12);
+}
+
+double EnvMap::mean_luminance() const {  // envmap.rs:61-66
+  const double weight_sum = (double)width * height * (2.0 / PI);
+  return total / std::fmax(weight_sum, 1e-12);
+}
+
+void EnvMap::texel_of(const Vec3& dir, uint32_t* x, uint32_t* y) const {  // envmap.rs:84-91
+  const Vec3 d = dir.normalize();
+  double u = std::atan2(d.x, -d.z) / (2.0 * PI);
+  u -= std::floor(u);
+  const double v = std::fmin(std::fmax(std::acos(std::fmin(std::fmax(d.y, -1.0), 1.0)) / PI, 0.0), 1.0);
+  *x = std::min((uint32_t)(u * width), width - 1);
+  *y = std::min((uint32_t)(v * height), height - 1);
+}
+
+Vec3 EnvMap::texel(uint32_t x, uint32_t y) const {
+  const float* c = &data[3 * ((size_t)std::min(y, height - 1) * width + std::min(x, width - 1))];
+  return Vec3{c[0], c[1], c[2]};
+}
+
+Vec3 EnvMap::eval(const Vec3& dir) const {
+  uint32_t x, y;
+  texel_of(dir, &x, &y);
+  return texel(x, y);
+}
+
+double EnvMap::pdf(const Vec3& dir) const {  // envmap.rs:134-148
+  uint32_t x, y;
+  texel_of(dir, &x, &y);
+  const double sin_theta = std::sin(PI * (y + 0.5) / height);
+  if (sin_theta < 1e-9) return 0.0;
+  const double texel_weight = texel_lum(&data[3 * ((size_t)y * width + x)]) * sin_theta;
+  return texel_weight * width * height / (total * 2.0 * PI * PI * sin_theta);
+}
+
+// The index the reference's binary search lands on: the last entry <= t.
+static uint32_t last_not_above(const double* c, size_t n, double t) {
+  const size_t i = (size_t)(std::upper_bound(c, c + n, t) - c);
+  return (uint32_t)(i == 0 ? 0 : i - 1);
+}
+
+void EnvMap::sample(double u1, double u2, Vec3* dir, Vec3* radiance, double* pdf_out) const {  // envmap.rs:99-131
+  const uint32_t y = std::min(last_not_above(marginal.data(), height + 1, u1 * total), height - 1);
+  const double row_total = std::fmax(marginal[y + 1] - marginal[y], 1e-12);
+  const double* cond = &conditional[(size_t)y * (width + 1)];
+  const uint32_t x = std::min(last_not_above(cond, width + 1, u2 * row_total), width - 1);
+  const double u = (x + 0.5) / width, v = (y + 0.5) / height;
+  const double phi = u * 2.0 * PI, theta = v * PI;
+  const double sin_t = std::sin(theta);
+  *dir = Vec3{sin_t * std::sin(phi), std::cos(theta), -sin_t * std::cos(phi)};
+  *radiance = texel(x, y);
+  *pdf_out = pdf(*dir);
+}
+
 // light_sampler.rs:97-142
 void LightSampler::build(const std::vector<Light>& lights) {
   nodes.clear();
@@ -447,7 +523,11 @@ void LightSampler::build(const std::vector<Light>& lights) {
       finite.push_back({(uint32_t)i, c, r, pw});
     } else {
       infinite.push_back((uint32_t)i);
-      infinite_power.push_back(lum_clamped(lights[i].radiance()) * 4.0 * PI);
+      // a dome with a map: its mean luminance times the intensity
+      // (light_sampler.rs:111-118)
+      const double base = lights[i].env ? std::fmax(lights[i].env->mean_luminance() * lights[i].intensity, 1e-9)
+                                        : lum_clamped(lights[i].radiance());
+      infinite_power.push_back(base * 4.0 * PI);
     }
   }
   infinite_total = 0;
@@ -1105,13 +1185,22 @@ Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const
       double weight = power_heuristic(pdf_sa, bsdf_pdf);
       return f * light.radiance() * vis * (weight / pdf_sa);
     }
-    case RRHIP_LIGHT_DOME: {
+    case RRHIP_LIGHT_DOME: {  // pt/mod.rs:1138-1172
       double u = rng.next_f64(), v = rng.next_f64();
-      double z = 1.0 - 2.0 * u;
-      double rr = std::sqrt(std::fmax(1.0 - z * z, 0.0));
-      double phi = 2.0 * PI * v;
-      Vec3 wi{rr * std::cos(phi), rr * std::sin(phi), z};
-      double pdf_sa = (1.0 / (4.0 * PI)) * pick_pmf;
+      Vec3 wi, rad = light.radiance();
+      double pdf_d = 1.0 / (4.0 * PI);
+      if (light.env) {
+        Vec3 texel;
+        light.env->sample(u, v, &wi, &texel, &pdf_d);
+        if (pdf_d <= 0.0) return Vec3::zero();
+        rad = texel * light.radiance();
+      } else {
+        double z = 1.0 - 2.0 * u;
+        double rr = std::sqrt(std::fmax(1.0 - z * z, 0.0));
+        double phi = 2.0 * PI * v;
+        wi = Vec3{rr * std::cos(phi), rr * std::sin(phi), z};
+      }
+      double pdf_sa = pdf_d * pick_pmf;
       Vec3 f;
       double bsdf_pdf;
       eval(wi, &f, &bsdf_pdf);
@@ -1119,7 +1208,7 @@ Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const
       Vec3 vis = visibility(p + n * 1e-4, wi, 1e8, rng, rays, medium, time);
       if (max_component(vis) <= 0.0) return Vec3::zero();
       double weight = power_heuristic(pdf_sa, bsdf_pdf);
-      return f * light.radiance() * vis * (weight / pdf_sa);
+      return f * rad * vis * (weight / pdf_sa);
     }
     default:
       return Vec3::zero();
@@ -1264,14 +1353,16 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays, A
       if (!(max_component(beta) > 0.0)) break;
     }
     if (!hit_found) {
-      if (dome >= 0) {
+      if (dome >= 0) {  // pt/mod.rs:645-654 with dome_radiance / dome_pdf (375-387)
         const Light& d = lights[dome];
+        const Vec3 dir = ray.direction.normalize();
         double weight = 1.0;
         if (!from_camera) {
           double pick = light_sampler.pmf(prev_origin, (size_t)dome);
-          weight = power_heuristic(prev_pdf, (1.0 / (4.0 * PI)) * pick);
+          const double pdf_dome = d.env ? d.env->pdf(dir) : 1.0 / (4.0 * PI);
+          weight = power_heuristic(prev_pdf, pdf_dome * pick);
         }
-        emit(beta * d.radiance() * weight);
+        emit(beta * (d.env ? d.env->eval(dir) * d.radiance() : d.radiance()) * weight);
       } else {
         emit(beta * background);
       }
@@ -1527,6 +1618,11 @@ void Scene::load(const rrhip_scene_desc& d) {
         break;
     }
     lights.push_back(l);
+  }
+  if (d.env_rgb && dome >= 0) {
+    auto env = std::make_shared<EnvMap>();
+    env->from_data(d.env_width, d.env_height, d.env_rgb);
+    lights[(size_t)dome].env = env;
   }
   light_sampler.build(lights);
 

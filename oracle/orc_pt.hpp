@@ -9,6 +9,7 @@
 #pragma once
 #include <cstdint>
 #include <functional>
+#include <memory>
 #include <vector>
 
 #include "../include/rrhip.h"
@@ -97,6 +98,25 @@ void bsdf_eval_pdf(const PbrParams& p, const Vec3& wo, const Vec3& wi, double et
 bool bsdf_sample(const PbrParams& p, const Vec3& wo, double eta_rel, Pcg32& rng, BsdfSample* s);
 
 // light.rs:7-46 (no IES, no HDRI)
+// envmap.rs:10-148: a lat-long environment map with 2D-CDF importance
+// sampling of sin-weighted luminance (nearest texel).
+struct EnvMap {
+  uint32_t width = 0, height = 0;
+  std::vector<float> data;  // rgb, row-major from the +y pole
+  std::vector<double> marginal, conditional;
+  double total = 0.0;
+  void from_data(uint32_t w, uint32_t h, const float* rgb);
+  double mean_luminance() const;
+  Vec3 eval(const Vec3& dir) const;
+  // (dir, radiance, pdf_sa)
+  void sample(double u1, double u2, Vec3* dir, Vec3* radiance, double* pdf) const;
+  double pdf(const Vec3& dir) const;
+
+ private:
+  void texel_of(const Vec3& dir, uint32_t* x, uint32_t* y) const;
+  Vec3 texel(uint32_t x, uint32_t y) const;
+};
+
 struct Light {
   int kind = RRHIP_LIGHT_POINT;
   double intensity = 1.0;
@@ -109,6 +129,7 @@ struct Light {
   Vec3 normal;                      // rect / disk
   double area = 0;                  // rect / disk
   double radius = 0;                // sphere
+  std::shared_ptr<const EnvMap> env;  // dome: lat-long map (light.rs:41)
   Vec3 radiance() const { return color * intensity; }
 };
 

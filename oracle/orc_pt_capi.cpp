@@ -407,6 +407,39 @@ void orc_medium_transmittance_n(const rrhip_medium_desc* md, const double* o, co
 
 double orc_hg_phase(double g, double cos_theta) { return orc::hg_phase(g, cos_theta); }
 
+// EnvMap (envmap.rs) on an explicit map: pdf and eval at n directions, and
+// n samples from uniforms (u1, u2) pairs.
+void orc_env_pdf_eval_n(uint32_t w, uint32_t h, const float* rgb, const double* dirs, uint64_t n, double* pdf,
+                        double* rad) {
+  EnvMap env;
+  env.from_data(w, h, rgb);
+  for (uint64_t i = 0; i < n; i++) {
+    const Vec3 d{dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]};
+    pdf[i] = env.pdf(d);
+    const Vec3 c = env.eval(d);
+    rad[3 * i] = c.x;
+    rad[3 * i + 1] = c.y;
+    rad[3 * i + 2] = c.z;
+  }
+}
+
+void orc_env_sample_n(uint32_t w, uint32_t h, const float* rgb, const double* u, uint64_t n, double* dirs,
+                      double* rad, double* pdf, double* mean_lum) {
+  EnvMap env;
+  env.from_data(w, h, rgb);
+  for (uint64_t i = 0; i < n; i++) {
+    Vec3 d, c;
+    env.sample(u[2 * i], u[2 * i + 1], &d, &c, &pdf[i]);
+    dirs[3 * i] = d.x;
+    dirs[3 * i + 1] = d.y;
+    dirs[3 * i + 2] = d.z;
+    rad[3 * i] = c.x;
+    rad[3 * i + 1] = c.y;
+    rad[3 * i + 2] = c.z;
+  }
+  if (mean_lum) *mean_lum = env.mean_luminance();
+}
+
 // Equiangular sampling (volume.rs:195-232) for n uniforms: t, the pdf the
 // sampler returns, and equiangular_pdf re-evaluated at t.
 void orc_equiangular_n(const double* o, const double* d, double t_max, const double* light, const double* u,

@@ -351,6 +351,12 @@ hipError_t upload_pt(rrhip_scene* s, const PtSceneHost& h) {
   v.transparent = h.transparent;
   v.motion = h.motion;
   if (e == hipSuccess) e = upload_vec(h.inst_motion, &s->pt_mem[15], &v.inst_motion);
+  if (e == hipSuccess) e = upload_vec(h.env_rgb, &s->pt_mem[16], &v.env_rgb);
+  if (e == hipSuccess) e = upload_vec(h.env_marg, &s->pt_mem[17], &v.env_marg);
+  if (e == hipSuccess) e = upload_vec(h.env_cond, &s->pt_mem[18], &v.env_cond);
+  v.env_w = h.env_w;
+  v.env_h = h.env_h;
+  v.env_total = h.env_total;
   v.num_insts = (uint32_t)h.insts.size();
   v.num_tris = (uint32_t)h.tris.size();
   v.num_spheres = (uint32_t)h.spheres.size();

@@ -284,6 +284,79 @@ __device__ __forceinline__ float power_heuristic(float a, float b) {  // pt/mod.
   return (a2 + b2 <= 0.f) ? 0.f : a2 / (a2 + b2);
 }
 
+// ---- dome environment map (envmap.rs), nearest texel, CDFs in f64 ---------------
+
+// Texel of direction d (unit): u = atan2(x, -z) / 2pi wrapped, v = acos(y) / pi
+// (envmap.rs:84-91).
+__device__ __forceinline__ void env_texel(const PtSceneView& S, V3 d, uint32_t& x, uint32_t& y) {
+  float u = atan2f(d.x, -d.z) * (0.5f / kPi);
+  u -= floorf(u);
+  const float v = fminf(fmaxf(acosf(fminf(fmaxf(d.y, -1.f), 1.f)) / kPi, 0.f), 1.f);
+  x = min((uint32_t)(u * (float)S.env_w), S.env_w - 1);
+  y = min((uint32_t)(v * (float)S.env_h), S.env_h - 1);
+}
+
+__device__ __forceinline__ V3 env_rgb(const PtSceneView& S, uint32_t x, uint32_t y) {
+  const float* c = S.env_rgb + 3 * ((size_t)y * S.env_w + x);
+  return v3(c[0], c[1], c[2]);
+}
+
+// Solid-angle pdf of sampling d (envmap.rs:134-148): the texel's share of
+// the sin-weighted luminance over the 2 pi^2 sin(theta) Jacobian.
+__device__ __forceinline__ float env_pdf(const PtSceneView& S, V3 d) {
+  uint32_t x, y;
+  env_texel(S, d, x, y);
+  const double sin_t = sin(3.14159265358979323846 * ((double)y + 0.5) / (double)S.env_h);
+  if (sin_t < 1e-9) return 0.f;
+  const double w = (double)lum(env_rgb(S, x, y)) * sin_t;
+  return (float)(w * (double)S.env_w * (double)S.env_h /
+                 (S.env_total * 2.0 * 3.14159265358979323846 * 3.14159265358979323846 * sin_t));
+}
+
+// Largest i in [0, n) with c[i] <= t (the row / column the reference's
+// binary search lands on), at most n - 1.
+__device__ __forceinline__ uint32_t cdf_find(const double* c, uint32_t n, double t) {
+  uint32_t lo = 0, hi = n;  // c[lo] <= t holds for lo = 0 (c[0] = 0 <= t)
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (c[mid] <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Importance-sampled direction, its texel radiance and pdf (envmap.rs:
+// 99-131): the row from the marginal CDF, the column from the row's
+// conditional CDF, the texel centre's direction.
+__device__ __forceinline__ void env_sample(const PtSceneView& S, float u1, float u2, V3& dir, V3& rad, float& pdf) {
+  const uint32_t y = min(cdf_find(S.env_marg, S.env_h + 1, (double)u1 * S.env_total), S.env_h - 1);
+  const double row_total = fmax(S.env_marg[y + 1] - S.env_marg[y], 1e-12);
+  const double* cond = S.env_cond + (size_t)y * (S.env_w + 1);
+  const uint32_t x = min(cdf_find(cond, S.env_w + 1, (double)u2 * row_total), S.env_w - 1);
+  const float phi = ((float)x + 0.5f) / (float)S.env_w * 2.f * kPi;
+  const float theta = ((float)y + 0.5f) / (float)S.env_h * kPi;
+  float sp, cp, st, ct;
+  sincosf(phi, &sp, &cp);
+  sincosf(theta, &st, &ct);
+  dir = v3(st * sp, ct, -st * cp);
+  rad = env_rgb(S, x, y);
+  pdf = env_pdf(S, dir);
+}
+
+// Dome radiance along d and the pdf the dome's light sampling has for it
+// (pt/mod.rs:375-387).
+__device__ __forceinline__ V3 dome_radiance(const PtSceneView& S, const PtLight& L, V3 d) {
+  if (!S.env_rgb) return v3(L.radiance);
+  uint32_t x, y;
+  env_texel(S, d, x, y);
+  return env_rgb(S, x, y) * v3(L.radiance);
+}
+__device__ __forceinline__ float dome_pdf(const PtSceneView& S, V3 d) {
+  return S.env_rgb ? env_pdf(S, d) : kInv4Pi;
+}
+
 // ---- light BVH (light_sampler.rs), branch probabilities in f64 ---------------
 
 __device__ __forceinline__ double node_importance(const LightNode& n, double px, double py, double pz) {

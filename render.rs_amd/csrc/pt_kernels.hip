@@ -334,21 +334,30 @@ __device__ __forceinline__ bool sample_light(const PtSceneView& S, const PtLight
       out.c = f * rad * (w / pdf_sa);
       return true;
     }
-    case kLightDome: {
+    case kLightDome: {  // pt/mod.rs:1138-1172
       const float u = rnd(rng), v = rnd(rng);
-      const float z = 1.f - 2.f * u;
-      const float rr = sqrtf(fmaxf(1.f - z * z, 0.f));
-      float sn, cs;
-      sincosf(2.f * kPi * v, &sn, &cs);
-      const V3 wi = v3(rr * cs, rr * sn, z);
-      const float pdf_sa = kInv4Pi * pick;
+      V3 wi, lrad = rad;
+      float pdf_d = kInv4Pi;
+      if (S.env_rgb) {  // importance-sampled map texel
+        V3 texel;
+        env_sample(S, u, v, wi, texel, pdf_d);
+        if (!(pdf_d > 0.f)) return false;
+        lrad = texel * rad;
+      } else {  // constant dome: uniform sphere, matching dome_pdf
+        const float z = 1.f - 2.f * u;
+        const float rr = sqrtf(fmaxf(1.f - z * z, 0.f));
+        float sn, cs;
+        sincosf(2.f * kPi * v, &sn, &cs);
+        wi = v3(rr * cs, rr * sn, z);
+      }
+      const float pdf_sa = pdf_d * pick;
       eval(wi, f, bsdf_pdf);
       if (maxc(f) <= 0.f) return false;
       const float w = power_heuristic(pdf_sa, bsdf_pdf);
       out.o = org;
       out.d = wi;
       out.tmax = 1e8f;
-      out.c = f * rad * (w / pdf_sa);
+      out.c = f * lrad * (w / pdf_sa);
       return true;
     }
     default:
@@ -519,8 +528,8 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
       // Miss: dome (with MIS) or background (pt/mod.rs:643-659).
       if (S.dome >= 0) {
         float w = 1.f;
-        if (!from_camera) w = power_heuristic(prev_pdf, kInv4Pi * ls_pmf(S, prev_origin, (uint32_t)S.dome));
-        emit(beta * v3(S.lights[S.dome].radiance) * w);
+        if (!from_camera) w = power_heuristic(prev_pdf, dome_pdf(S, rd) * ls_pmf(S, prev_origin, (uint32_t)S.dome));
+        emit(beta * dome_radiance(S, S.lights[S.dome], rd) * w);
       } else {
         emit(beta * v3(S.background));
       }

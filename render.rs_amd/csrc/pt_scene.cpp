@@ -384,6 +384,32 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
     }
     out->lights.push_back(g);
   }
+  // Dome environment map: CDFs as EnvMap::from_data (envmap.rs:36-54).
+  if (d.env_rgb) {
+    if (out->dome < 0) return *err = "an environment map needs a dome light", false;
+    if (d.env_width == 0 || d.env_height == 0 || (uint64_t)d.env_width * d.env_height > (1ull << 28))
+      return *err = "environment map size must be 1..2^28 texels", false;
+    const uint32_t w = d.env_width, h = d.env_height;
+    out->env_w = w;
+    out->env_h = h;
+    out->env_rgb.assign(d.env_rgb, d.env_rgb + 3ull * w * h);
+    for (float v : out->env_rgb)
+      if (!(v >= 0.f) || !std::isfinite(v)) return *err = "environment map texels must be finite and >= 0", false;
+    out->env_marg.assign(h + 1, 0.0);
+    out->env_cond.assign((size_t)(w + 1) * h, 0.0);
+    for (uint32_t y = 0; y < h; y++) {
+      const double sin_t = std::sin(kPi * (y + 0.5) / h);
+      const float* row = &out->env_rgb[3ull * y * w];
+      const size_t base = (size_t)y * (w + 1);
+      double acc = 0.0;
+      for (uint32_t x = 0; x < w; x++) {
+        acc += (0.2126 * row[3 * x] + 0.7152 * row[3 * x + 1] + 0.0722 * row[3 * x + 2]) * sin_t;
+        out->env_cond[base + x + 1] = acc;
+      }
+      out->env_marg[y + 1] = out->env_marg[y] + acc;
+    }
+    out->env_total = std::max(out->env_marg[h], 1e-12);
+  }
   // Light sampler (light_sampler.rs:97-142)
   std::vector<Entry> finite;
   std::vector<double> inf_power;
@@ -394,7 +420,13 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
       finite.push_back({i, {c.x, c.y, c.z}, r, pw});
     } else {
       out->inf_lights.push_back(i);
-      inf_power.push_back(std::max(lum(mul(d3(d.lights[i].color), d.lights[i].intensity)), 1e-9) * 4.0 * kPi);
+      // a dome with a map: its sin-weighted mean luminance times the
+      // intensity (light_sampler.rs:111-118, envmap.rs:61-66)
+      const bool mapped = (int32_t)i == out->dome && !out->env_rgb.empty();
+      const double base = mapped ? out->env_total / ((double)out->env_w * out->env_h * (2.0 / kPi)) *
+                                       d.lights[i].intensity
+                                 : lum(mul(d3(d.lights[i].color), d.lights[i].intensity));
+      inf_power.push_back(std::max(base, 1e-9) * 4.0 * kPi);
     }
   }
   double acc = 0.0;

@@ -36,6 +36,13 @@ struct PtSceneHost {
   std::vector<std::vector<float>> media_grids;
   int32_t atmosphere = -1;
   float lo[3], hi[3];       // bounds of all geometry (radiance-cache input range)
+  // Dome environment map (envmap.rs): rgb texels, the row-marginal CDF
+  // (height + 1) and per-row conditional CDFs ((width + 1) x height) of
+  // sin-weighted luminance, in f64, and their total.
+  uint32_t env_w = 0, env_h = 0;
+  std::vector<float> env_rgb;
+  std::vector<double> env_marg, env_cond;
+  double env_total = 0.0;
 };
 
 // Validates and converts the path-tracing part of `d` (loose triangles,

@@ -162,6 +162,13 @@ struct PtSceneView {
                                 // a medium stack, shadow rays walk their blockers
   uint32_t motion;              // moving instances or camera: rays carry a shutter time
   const PtInstMotion* inst_motion;
+  // Dome environment map (null: constant dome): rgb texels, row-marginal
+  // and per-row conditional CDFs of sin-weighted luminance, their total.
+  const float* env_rgb;
+  const double* env_marg;
+  const double* env_cond;
+  uint32_t env_w, env_h;
+  double env_total;
   uint32_t num_insts;
   uint32_t num_tris, num_spheres, num_lights, num_lnodes, num_inf;
   int32_t dome;

@@ -132,7 +132,9 @@ class SceneDesc(ctypes.Structure):
                 ("lights", ctypes.POINTER(LightDesc)),
                 ("camera", CameraDesc), ("background", D3),
                 ("num_media", ctypes.c_uint32), ("atmosphere", ctypes.c_int32),
-                ("media", ctypes.POINTER(MediumDesc))]
+                ("media", ctypes.POINTER(MediumDesc)),
+                ("env_width", ctypes.c_uint32), ("env_height", ctypes.c_uint32),
+                ("env_rgb", ctypes.POINTER(ctypes.c_float))]
 
 
 class RenderParams(ctypes.Structure):

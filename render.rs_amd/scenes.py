@@ -198,8 +198,19 @@ class Scene:
         d.num_media = len(self.media)
         d.atmosphere = self.atmosphere
         d.media = self._arr(rrhip.MediumDesc, self.media)
+        env = getattr(self, "env", None)
+        if env is not None:
+            env = np.ascontiguousarray(env, np.float32)
+            self._keep.append(env)
+            d.env_height, d.env_width = env.shape[0], env.shape[1]
+            d.env_rgb = env.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
         self._keep.append(d)
         return d
+
+    def set_env(self, rgb):
+        """Lat-long environment map of the dome light: (height, width, 3)
+        linear radiance, rows from the +y pole (envmap.rs)."""
+        self.env = np.ascontiguousarray(rgb, np.float32)
 
     @property
     def npix(self):
