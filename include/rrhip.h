@@ -152,7 +152,7 @@ enum {
   RRHIP_LIGHT_RECT = 2,
   RRHIP_LIGHT_SPHERE = 3,
   RRHIP_LIGHT_DISK = 4,
-  RRHIP_LIGHT_DOME = 5 /* constant radiance dome (no HDRI) */
+  RRHIP_LIGHT_DOME = 5 /* dome: constant radiance, or the scene desc's environment map */
 };
 
 /* Light: radiance = color * intensity. p = position (point), corner (rect),
