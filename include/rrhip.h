@@ -116,6 +116,26 @@ typedef struct rrhip_sphere_desc {
   uint32_t reserved;
 } rrhip_sphere_desc;
 
+/* RenderMan quadrics as render.rs intersects them (src/geometry/sphere.rs,
+ * cylinder.rs, cone.rs, extra.rs Disk / Paraboloid): object space is the
+ * quadric's own (z the axis), placed by object_to_world (row-major 3x4);
+ * thetamax in degrees. sphere: radius, zmin, zmax; cylinder: radius, zmin,
+ * zmax; cone: height, radius (apex at the origin, radius at z = height);
+ * disk: height (its z), radius; paraboloid: radius (= rmax), zmin, zmax. */
+enum {
+  RRHIP_QUADRIC_SPHERE = 0,
+  RRHIP_QUADRIC_CYLINDER = 1,
+  RRHIP_QUADRIC_CONE = 2,
+  RRHIP_QUADRIC_DISK = 3,
+  RRHIP_QUADRIC_PARABOLOID = 4
+};
+typedef struct rrhip_quadric_desc {
+  uint32_t kind;
+  uint32_t material;
+  double radius, zmin, zmax, height, thetamax;
+  double object_to_world[12];
+} rrhip_quadric_desc;
+
 /* Material: render.rs's PbrParams lobe stack (src/scene/pbr.rs:11-48) plus
  * Material::emission / area_light (src/scene/material.rs:14-40). */
 typedef struct rrhip_material_desc {
@@ -247,6 +267,10 @@ typedef struct rrhip_scene_desc {
    * constant dome. Requires a dome light. */
   uint32_t env_width, env_height;
   const float* env_rgb;
+  /* Loose quadrics (besides the spheres above), tested like them. */
+  uint32_t num_quadrics;
+  uint32_t reserved;
+  const rrhip_quadric_desc* quadrics;
 } rrhip_scene_desc;
 
 /* Path-tracing parameters (render.rs constants, src/raytracer/pt/mod.rs:

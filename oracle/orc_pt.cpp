@@ -785,6 +785,149 @@ bool Scene::instance_hit(const InstanceObj& inst, const Ray& ray, double t_max, 
 }
 
 // scene/mod.rs:105-146
+// extra.rs:44-55
+static bool theta_ok(double x, double y, double thetamax) {
+  if (thetamax >= 360.0) return true;
+  double phi = std::atan2(y, x) * 180.0 / PI;
+  if (phi < 0.0) phi += 360.0;
+  return phi <= thetamax;
+}
+
+bool QuadricObj::intersect(const Ray& ray, Intersection* out) const {
+  const Vec3 lo = inverse.transform_point(ray.origin);
+  const Vec3 ld = inverse.transform_vec(ray.direction).normalize();
+  double t = 0.0;
+  Vec3 p, ln;
+  switch (kind) {
+    case RRHIP_QUADRIC_SPHERE: {  // sphere.rs:38-75
+      const double a = ld.dot(ld), b = 2.0 * lo.dot(ld), c = lo.dot(lo) - radius * radius;
+      const double disc = b * b - 4.0 * a * c;
+      if (disc < 0.0) return false;
+      const double s = std::sqrt(disc);
+      const double t1 = (-b - s) / (2.0 * a), t2 = (-b + s) / (2.0 * a);
+      if (t1 > EPSILON)
+        t = t1;
+      else if (t2 > EPSILON)
+        t = t2;
+      else
+        return false;
+      p = lo + ld * t;
+      if (p.z < zmin || p.z > zmax || !theta_ok(p.x, p.y, thetamax)) return false;
+      ln = p.normalize();
+      break;
+    }
+    case RRHIP_QUADRIC_CYLINDER: {  // cylinder.rs:36-93
+      const double a = ld.x * ld.x + ld.y * ld.y;
+      const double b = 2.0 * (lo.x * ld.x + lo.y * ld.y);
+      const double c = lo.x * lo.x + lo.y * lo.y - radius * radius;
+      if (std::fabs(a) < EPSILON) return false;
+      const double disc = b * b - 4.0 * a * c;
+      if (disc < 0.0) return false;
+      const double s = std::sqrt(disc);
+      const double t1 = (-b - s) / (2.0 * a), t2 = (-b + s) / (2.0 * a);
+      t = t1 > EPSILON ? t1 : t2;
+      if (t <= EPSILON) return false;
+      p = lo + ld * t;
+      if (p.z < zmin || p.z > zmax) {
+        if (t == t1 && t2 > EPSILON) {
+          t = t2;
+          p = lo + ld * t;
+          if (p.z < zmin || p.z > zmax) return false;
+        } else {
+          return false;
+        }
+      }
+      if (!theta_ok(p.x, p.y, thetamax)) return false;
+      ln = Vec3{p.x, p.y, 0.0}.normalize();
+      break;
+    }
+    case RRHIP_QUADRIC_CONE: {  // cone.rs:48-101, 110-125
+      const double k = (radius / height) * (radius / height);
+      const double a = ld.x * ld.x + ld.y * ld.y - k * ld.z * ld.z;
+      const double b = 2.0 * (lo.x * ld.x + lo.y * ld.y - k * lo.z * ld.z);
+      const double c = lo.x * lo.x + lo.y * lo.y - k * lo.z * lo.z;
+      bool hit = false;
+      if (std::fabs(a) < EPSILON) {
+        if (std::fabs(b) < EPSILON) return false;
+        t = -c / b;
+        if (t > EPSILON) {
+          p = lo + ld * t;
+          hit = p.z >= 0.0 && p.z <= height;
+        }
+      } else {
+        const double disc = b * b - 4.0 * a * c;
+        if (disc < 0.0) return false;
+        const double s = std::sqrt(disc);
+        const double ts[2] = {(-b - s) / (2.0 * a), (-b + s) / (2.0 * a)};
+        for (double tc : ts) {
+          if (!(tc > EPSILON)) continue;
+          const Vec3 pc = lo + ld * tc;
+          if (pc.z >= 0.0 && pc.z <= height) {
+            if (!theta_ok(pc.x, pc.y, thetamax)) continue;
+            t = tc;
+            p = pc;
+            hit = true;
+            break;
+          }
+        }
+      }
+      if (!hit) return false;
+      const double r = std::sqrt(p.x * p.x + p.y * p.y);
+      ln = Vec3{p.x / r, p.y / r, -(radius / height)}.normalize();
+      break;
+    }
+    case RRHIP_QUADRIC_DISK: {  // extra.rs Disk::intersect
+      if (std::fabs(ld.z) < EPSILON) return false;
+      t = (height - lo.z) / ld.z;
+      if (t <= EPSILON) return false;
+      p = lo + ld * t;
+      if (p.x * p.x + p.y * p.y > radius * radius || !theta_ok(p.x, p.y, thetamax)) return false;
+      ln = Vec3{0.0, 0.0, 1.0};
+      break;
+    }
+    default: {  // extra.rs Paraboloid::intersect
+      if (std::fabs(zmax) < EPSILON) return false;
+      const double k = radius * radius / zmax;
+      const double a = ld.x * ld.x + ld.y * ld.y;
+      const double b = 2.0 * (lo.x * ld.x + lo.y * ld.y) - k * ld.z;
+      const double c = lo.x * lo.x + lo.y * lo.y - k * lo.z;
+      double cands[2];
+      int nc = 2;
+      if (std::fabs(a) < EPSILON) {
+        if (std::fabs(b) < EPSILON) return false;
+        cands[0] = -c / b;
+        nc = 1;
+      } else {
+        const double disc = b * b - 4.0 * a * c;
+        if (disc < 0.0) return false;
+        const double s = std::sqrt(disc);
+        cands[0] = (-b - s) / (2.0 * a);
+        cands[1] = (-b + s) / (2.0 * a);
+      }
+      bool hit = false;
+      for (int i = 0; i < nc && !hit; i++) {
+        const double tc = cands[i];
+        if (tc <= EPSILON) continue;
+        const Vec3 pc = lo + ld * tc;
+        if (pc.z < zmin || pc.z > zmax || !theta_ok(pc.x, pc.y, thetamax)) continue;
+        t = tc;
+        p = pc;
+        hit = true;
+      }
+      if (!hit) return false;
+      ln = Vec3{2.0 * p.x, 2.0 * p.y, -k}.normalize();
+      break;
+    }
+  }
+  const Vec3 wp = transform.transform_point(p);
+  out->t = (wp - ray.origin).length();
+  out->point = wp;
+  out->normal = inverse.transform_normal(ln).normalize();
+  out->front_face = kind == RRHIP_QUADRIC_SPHERE ? ray.direction.dot(out->normal) < 0.0 : true;
+  out->material = material;
+  return true;
+}
+
 bool Scene::intersect(const Ray& ray, Intersection* out) const {
   bool found = false;
   double closest_t = INF;
@@ -798,6 +941,13 @@ bool Scene::intersect(const Ray& ray, Intersection* out) const {
   }
   for (const auto& sp : spheres) {
     if (sp.intersect(ray, &h) && h.t < closest_t) {
+      closest_t = h.t;
+      *out = h;
+      found = true;
+    }
+  }
+  for (const auto& q : quadrics) {
+    if (q.intersect(ray, &h) && h.t < closest_t) {
       closest_t = h.t;
       *out = h;
       found = true;
@@ -1635,6 +1785,23 @@ void Scene::load(const rrhip_scene_desc& d) {
   for (uint32_t i = 0; i < d.num_spheres; i++) {
     const rrhip_sphere_desc& s = d.spheres[i];
     spheres.push_back({v3(s.center), s.radius, s.material});
+  }
+  quadrics.clear();
+  for (uint32_t i = 0; i < d.num_quadrics; i++) {
+    const rrhip_quadric_desc& q = d.quadrics[i];
+    QuadricObj o;
+    o.kind = q.kind;
+    o.material = q.material;
+    o.radius = q.radius;
+    o.zmin = q.zmin;
+    o.zmax = q.zmax;
+    o.height = q.height;
+    o.thetamax = q.thetamax;
+    o.transform = Mat4::identity();
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 4; c++) o.transform.m[r][c] = q.object_to_world[4 * r + c];
+    if (!o.transform.inverse(o.inverse)) o.inverse = Mat4::identity();
+    quadrics.push_back(o);
   }
   meshes.clear();
   meshes.resize(d.num_meshes);

@@ -190,6 +190,15 @@ struct SphereObj {
   bool intersect(const Ray& ray, Intersection* out) const;
 };
 
+// RenderMan quadrics (sphere.rs, cylinder.rs, cone.rs, extra.rs Disk /
+// Paraboloid) with their object-to-world transform.
+struct QuadricObj {
+  uint32_t kind = 0, material = 0;
+  double radius = 0, zmin = 0, zmax = 0, height = 0, thetamax = 360;
+  Mat4 transform, inverse;
+  bool intersect(const Ray& ray, Intersection* out) const;
+};
+
 struct MeshObj {
   Mesh mesh;
   std::vector<float> normals;  // empty: geometric
@@ -247,6 +256,7 @@ class Scene {
   Camera camera;
   std::vector<TriangleObj> triangles;
   std::vector<SphereObj> spheres;
+  std::vector<QuadricObj> quadrics;
   std::vector<MeshObj> meshes;
   std::vector<InstanceObj> instances;
   Bvh tlas;

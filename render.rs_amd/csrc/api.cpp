@@ -354,6 +354,8 @@ hipError_t upload_pt(rrhip_scene* s, const PtSceneHost& h) {
   if (e == hipSuccess) e = upload_vec(h.env_rgb, &s->pt_mem[16], &v.env_rgb);
   if (e == hipSuccess) e = upload_vec(h.env_marg, &s->pt_mem[17], &v.env_marg);
   if (e == hipSuccess) e = upload_vec(h.env_cond, &s->pt_mem[18], &v.env_cond);
+  if (e == hipSuccess) e = upload_vec(h.quadrics, &s->pt_mem[19], &v.quadrics);
+  v.num_quadrics = (uint32_t)h.quadrics.size();
   v.env_w = h.env_w;
   v.env_h = h.env_h;
   v.env_total = h.env_total;

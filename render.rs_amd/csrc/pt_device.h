@@ -531,6 +531,179 @@ __device__ __forceinline__ bool hit_sphere(const PtSphere& s, V3 o, V3 d, float 
   return true;
 }
 
+// ---- quadrics (sphere.rs, cylinder.rs, cone.rs, extra.rs Disk / Paraboloid) ----
+
+__device__ __forceinline__ V3 q_point(const float* m, V3 p) {
+  return v3(m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
+            m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]);
+}
+__device__ __forceinline__ V3 q_vec(const float* m, V3 v) {
+  return v3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[4] * v.x + m[5] * v.y + m[6] * v.z,
+            m[8] * v.x + m[9] * v.y + m[10] * v.z);
+}
+__device__ __forceinline__ V3 q_normal(const float* m, V3 n) {  // transpose of the inverse's 3x3
+  return v3(m[0] * n.x + m[4] * n.y + m[8] * n.z, m[1] * n.x + m[5] * n.y + m[9] * n.z,
+            m[2] * n.x + m[6] * n.y + m[10] * n.z);
+}
+
+// phi in [0, 360) from object x / y accepted when <= thetamax (extra.rs:44-55).
+__device__ __forceinline__ bool theta_ok(float x, float y, float thetamax) {
+  if (thetamax >= 360.f) return true;
+  float phi = atan2f(y, x) * (180.f / kPi);
+  if (phi < 0.f) phi += 360.f;
+  return phi <= thetamax;
+}
+
+// Roots of a t^2 + b t + c as the reference names them: rm = (-b - s) / 2a,
+// rp = (-b + s) / 2a, computed without cancellation. False if disc < 0.
+__device__ __forceinline__ bool quad_roots(float a, float b, float c, float& rm, float& rp) {
+  const float disc = b * b - 4.f * a * c;
+  if (disc < 0.f) return false;
+  const float s = sqrtf(disc);
+  const float q = -0.5f * (b + copysignf(s, b));
+  if (q == 0.f) {
+    rm = rp = 0.f;
+    return true;
+  }
+  const float r1 = q / a, r2 = c / q;  // r1 = rm when b >= 0, else rp
+  if (b >= 0.f) {
+    rm = r1;
+    rp = r2;
+  } else {
+    rm = r2;
+    rp = r1;
+  }
+  return true;
+}
+
+// The quadric's hit as render.rs computes it: the object-space ray with its
+// direction normalised, the kind's root order and windows, t = the world
+// distance to the hit, the normal through the inverse transpose (not
+// flipped); front face only the sphere decides (sphere.rs:105), the others
+// keep Intersection::new's `true`. Accepted when t < tmax.
+__device__ __forceinline__ bool hit_quadric(const PtQuadric& q, V3 o, V3 d, float tmax, Hit& h) {
+  const V3 lo = q_point(q.inv, o);
+  const V3 ld = normalize(q_vec(q.inv, d));
+  float t = 0.f;
+  V3 p, ln;
+  switch (q.kind) {
+    case 0: {  // sphere.rs:38-75: nearest root > EPSILON, then the z / theta windows
+      float rm, rp;
+      if (!quad_roots(dot(ld, ld), 2.f * dot(lo, ld), dot(lo, lo) - q.radius * q.radius, rm, rp)) return false;
+      if (rm > kObjEps)
+        t = rm;
+      else if (rp > kObjEps)
+        t = rp;
+      else
+        return false;
+      p = lo + ld * t;
+      if (p.z < q.zmin || p.z > q.zmax || !theta_ok(p.x, p.y, q.thetamax)) return false;
+      ln = normalize(p);
+      break;
+    }
+    case 1: {  // cylinder.rs:36-93: the far root when the near one leaves [zmin, zmax]
+      const float a = ld.x * ld.x + ld.y * ld.y;
+      if (fabsf(a) < kObjEps) return false;
+      float t1, t2;
+      if (!quad_roots(a, 2.f * (lo.x * ld.x + lo.y * ld.y), lo.x * lo.x + lo.y * lo.y - q.radius * q.radius, t1, t2))
+        return false;
+      t = t1 > kObjEps ? t1 : t2;
+      if (t <= kObjEps) return false;
+      p = lo + ld * t;
+      if (p.z < q.zmin || p.z > q.zmax) {
+        if (t == t1 && t2 > kObjEps) {
+          t = t2;
+          p = lo + ld * t;
+          if (p.z < q.zmin || p.z > q.zmax) return false;
+        } else {
+          return false;
+        }
+      }
+      if (!theta_ok(p.x, p.y, q.thetamax)) return false;
+      ln = normalize(v3(p.x, p.y, 0.f));
+      break;
+    }
+    case 2: {  // cone.rs:48-101: x^2 + y^2 = k z^2, 0 <= z <= height
+      const float a = ld.x * ld.x + ld.y * ld.y - q.k * ld.z * ld.z;
+      const float b = 2.f * (lo.x * ld.x + lo.y * ld.y - q.k * lo.z * ld.z);
+      const float c = lo.x * lo.x + lo.y * lo.y - q.k * lo.z * lo.z;
+      bool hit = false;
+      if (fabsf(a) < kObjEps) {  // one root, no theta window (cone.rs:63-75)
+        if (fabsf(b) < kObjEps) return false;
+        t = -c / b;
+        if (t > kObjEps) {
+          p = lo + ld * t;
+          hit = p.z >= 0.f && p.z <= q.height;
+        }
+      } else {
+        float t1, t2;
+        if (!quad_roots(a, b, c, t1, t2)) return false;
+        for (int i = 0; i < 2 && !hit; i++) {
+          const float tc = i == 0 ? t1 : t2;
+          if (!(tc > kObjEps)) continue;
+          const V3 pc = lo + ld * tc;
+          if (pc.z >= 0.f && pc.z <= q.height && theta_ok(pc.x, pc.y, q.thetamax)) {
+            t = tc;
+            p = pc;
+            hit = true;
+          }
+        }
+      }
+      if (!hit) return false;
+      const float r = sqrtf(p.x * p.x + p.y * p.y);
+      ln = normalize(v3(p.x / r, p.y / r, -q.radius / q.height));  // cone.rs:118-125
+      break;
+    }
+    case 3: {  // extra.rs Disk: the plane z = height within radius
+      if (fabsf(ld.z) < kObjEps) return false;
+      t = (q.height - lo.z) / ld.z;
+      if (t <= kObjEps) return false;
+      p = lo + ld * t;
+      if (p.x * p.x + p.y * p.y > q.radius * q.radius || !theta_ok(p.x, p.y, q.thetamax)) return false;
+      ln = v3(0.f, 0.f, 1.f);
+      break;
+    }
+    default: {  // extra.rs Paraboloid: x^2 + y^2 = k z, first root in the windows
+      if (fabsf(q.zmax) < kObjEps) return false;
+      const float a = ld.x * ld.x + ld.y * ld.y;
+      const float b = 2.f * (lo.x * ld.x + lo.y * ld.y) - q.k * ld.z;
+      const float c = lo.x * lo.x + lo.y * lo.y - q.k * lo.z;
+      float t1, t2;
+      int nc = 2;
+      if (fabsf(a) < kObjEps) {
+        if (fabsf(b) < kObjEps) return false;
+        t1 = -c / b;
+        t2 = 0.f;
+        nc = 1;
+      } else if (!quad_roots(a, b, c, t1, t2)) {
+        return false;
+      }
+      bool hit = false;
+      for (int i = 0; i < nc && !hit; i++) {
+        const float tc = i == 0 ? t1 : t2;
+        if (tc <= kObjEps) continue;
+        const V3 pc = lo + ld * tc;
+        if (pc.z < q.zmin || pc.z > q.zmax || !theta_ok(pc.x, pc.y, q.thetamax)) continue;
+        t = tc;
+        p = pc;
+        hit = true;
+      }
+      if (!hit) return false;
+      ln = normalize(v3(2.f * p.x, 2.f * p.y, -q.k));
+      break;
+    }
+  }
+  const V3 wp = q_point(q.fwd, p);
+  const V3 dw = wp - o;
+  const float tw = sqrtf(dot(dw, dw));
+  if (!(tw < tmax)) return false;
+  h.t = tw;
+  h.n = normalize(q_normal(q.inv, ln));
+  h.material = q.material;
+  h.front = q.kind == 0 ? dot(d, h.n) < 0.f : true;
+  return true;
+}
+
 // scene/mod.rs:105-146 over the loose objects (strictly closer wins).
 __device__ __forceinline__ bool closest_hit(const PtSceneView& S, V3 o, V3 d, Hit& best) {
   bool found = false;
@@ -551,6 +724,14 @@ __device__ __forceinline__ bool closest_hit(const PtSceneView& S, V3 o, V3 d, Hi
       found = true;
     }
   }
+  for (uint32_t i = 0; i < S.num_quadrics; i++) {
+    Hit h;
+    if (hit_quadric(S.quadrics[i], o, d, tmax, h)) {
+      best = h;
+      tmax = h.t;
+      found = true;
+    }
+  }
   return found;
 }
 
@@ -566,6 +747,10 @@ __device__ __forceinline__ bool occluded(const PtSceneView& S, V3 o, V3 d, float
     if (!sphere_roots(S.spheres[i], o, d, t0, t1)) continue;
     const float t = t0 > kObjEps ? t0 : t1;
     if (t > kObjEps && t < tmax) return true;
+  }
+  for (uint32_t i = 0; i < S.num_quadrics; i++) {
+    Hit h;
+    if (hit_quadric(S.quadrics[i], o, d, tmax, h)) return true;
   }
   return false;
 }

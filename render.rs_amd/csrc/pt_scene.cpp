@@ -474,6 +474,54 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
     g.material = s.material;
     out->spheres.push_back(g);
   }
+  if (d.num_quadrics && !d.quadrics) return *err = "null quadrics", false;
+  for (uint32_t i = 0; i < d.num_quadrics; i++) {
+    const rrhip_quadric_desc& q = d.quadrics[i];
+    if (q.kind > RRHIP_QUADRIC_PARABOLOID) return *err = "unknown quadric kind", false;
+    if (q.material >= d.num_materials) return *err = "quadric material out of range", false;
+    const double par[5] = {q.radius, q.zmin, q.zmax, q.height, q.thetamax};
+    for (double v : par)
+      if (!std::isfinite(v)) return *err = "quadric parameters must be finite", false;
+    for (double v : q.object_to_world)
+      if (!std::isfinite(v)) return *err = "quadric transform must be finite", false;
+    if (!(q.thetamax > 0.0)) return *err = "quadric thetamax must be > 0", false;
+    if (q.kind == RRHIP_QUADRIC_CONE && !(q.height != 0.0)) return *err = "cone height must be nonzero", false;
+    // Matrix4::inverse().unwrap_or(identity) (sphere.rs / cylinder.rs constructors)
+    const double* m = q.object_to_world;
+    const double a = m[0], b = m[1], c = m[2], dd = m[4], e = m[5], f = m[6], g = m[8], h = m[9], ii = m[10];
+    const double A = e * ii - f * h, B = -(dd * ii - f * g), C = dd * h - e * g;
+    const double det = a * A + b * B + c * C;
+    double inv[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    if (std::fabs(det) > 1e-300) {
+      const double r = 1.0 / det;
+      inv[0] = A * r;
+      inv[1] = -(b * ii - c * h) * r;
+      inv[2] = (b * f - c * e) * r;
+      inv[4] = B * r;
+      inv[5] = (a * ii - c * g) * r;
+      inv[6] = -(a * f - c * dd) * r;
+      inv[8] = C * r;
+      inv[9] = -(a * h - b * g) * r;
+      inv[10] = (a * e - b * dd) * r;
+      for (int row = 0; row < 3; row++)
+        inv[4 * row + 3] = -(inv[4 * row] * m[3] + inv[4 * row + 1] * m[7] + inv[4 * row + 2] * m[11]);
+    }
+    PtQuadric g2{};
+    g2.kind = q.kind;
+    g2.material = q.material;
+    g2.radius = (float)q.radius;
+    g2.zmin = (float)q.zmin;
+    g2.zmax = (float)q.zmax;
+    g2.height = (float)q.height;
+    g2.thetamax = (float)q.thetamax;
+    if (q.kind == RRHIP_QUADRIC_CONE) g2.k = (float)((q.radius / q.height) * (q.radius / q.height));
+    if (q.kind == RRHIP_QUADRIC_PARABOLOID && q.zmax != 0.0) g2.k = (float)(q.radius * q.radius / q.zmax);
+    for (int k = 0; k < 12; k++) {
+      g2.inv[k] = (float)inv[k];
+      g2.fwd[k] = (float)m[k];
+    }
+    out->quadrics.push_back(g2);
+  }
   put(out->background, d3(d.background));
   // Media (medium.rs:28-65)
   if (d.num_media > 0 && !d.media) return *err = "num_media > 0 with null media", false;
@@ -541,6 +589,28 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
     grow(d3(d.triangles[i].v2), 0.0);
   }
   for (uint32_t i = 0; i < d.num_spheres; i++) grow(d3(d.spheres[i].center), d.spheres[i].radius);
+  for (uint32_t i = 0; i < d.num_quadrics; i++) {  // the object-space extent's corners, placed
+    const rrhip_quadric_desc& q = d.quadrics[i];
+    const double r = std::fabs(q.radius);
+    double zlo = q.zmin, zhi = q.zmax;
+    if (q.kind == RRHIP_QUADRIC_CONE || q.kind == RRHIP_QUADRIC_DISK) {
+      zlo = std::min(0.0, q.height);
+      zhi = std::max(0.0, q.height);
+    }
+    if (q.kind == RRHIP_QUADRIC_SPHERE) {
+      zlo = std::max(zlo, -r);
+      zhi = std::min(zhi, r);
+    }
+    const double* m = q.object_to_world;
+    for (int cx = 0; cx < 2; cx++)
+      for (int cy = 0; cy < 2; cy++)
+        for (int cz = 0; cz < 2; cz++) {
+          const double x = cx ? r : -r, y = cy ? r : -r, z = cz ? zhi : zlo;
+          grow(D3{m[0] * x + m[1] * y + m[2] * z + m[3], m[4] * x + m[5] * y + m[6] * z + m[7],
+                  m[8] * x + m[9] * y + m[10] * z + m[11]},
+               0.0);
+        }
+  }
   if (!build_instances(d, out, err)) return false;
   for (int a = 0; a < 3; a++) {
     if (!(out->hi[a] >= out->lo[a])) out->lo[a] = out->hi[a] = 0.f;

@@ -39,6 +39,7 @@ struct PtSceneHost {
   // Dome environment map (envmap.rs): rgb texels, the row-marginal CDF
   // (height + 1) and per-row conditional CDFs ((width + 1) x height) of
   // sin-weighted luminance, in f64, and their total.
+  std::vector<PtQuadric> quadrics;
   uint32_t env_w = 0, env_h = 0;
   std::vector<float> env_rgb;
   std::vector<double> env_marg, env_cond;

@@ -81,6 +81,16 @@ struct alignas(16) PtSphere {
   float pad[3];
 };
 
+// A loose quadric: kind (RRHIP_QUADRIC_*), its parameters, and the
+// world-to-object / object-to-world affine maps (row-major 3x4).
+struct alignas(16) PtQuadric {
+  uint32_t kind, material;
+  float radius, zmin;
+  float zmax, height, thetamax, k;  // k: cone (radius / height)^2, paraboloid rmax^2 / zmax
+  float inv[12];
+  float fwd[12];
+};
+
 // A placed mesh (render.rs Instance, src/geometry/mesh.rs:223-305): the
 // world-to-object affine map (row-major 3x4) and what it references.
 struct alignas(16) PtInstance {
@@ -169,6 +179,8 @@ struct PtSceneView {
   const double* env_cond;
   uint32_t env_w, env_h;
   double env_total;
+  const PtQuadric* quadrics;
+  uint32_t num_quadrics;
   uint32_t num_insts;
   uint32_t num_tris, num_spheres, num_lights, num_lnodes, num_inf;
   int32_t dome;

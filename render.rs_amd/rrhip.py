@@ -119,6 +119,16 @@ class MediumDesc(ctypes.Structure):
                 ("grid_lo", D3), ("grid_hi", D3)]
 
 
+# rrhip_quadric_desc kinds
+QUADRIC_SPHERE, QUADRIC_CYLINDER, QUADRIC_CONE, QUADRIC_DISK, QUADRIC_PARABOLOID = 0, 1, 2, 3, 4
+
+
+class QuadricDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("material", ctypes.c_uint32), ("radius", ctypes.c_double),
+                ("zmin", ctypes.c_double), ("zmax", ctypes.c_double), ("height", ctypes.c_double),
+                ("thetamax", ctypes.c_double), ("object_to_world", ctypes.c_double * 12)]
+
+
 class SceneDesc(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("num_meshes", ctypes.c_uint32),
                 ("meshes", ctypes.POINTER(MeshDesc)), ("build", BuildParams),
@@ -134,7 +144,9 @@ class SceneDesc(ctypes.Structure):
                 ("num_media", ctypes.c_uint32), ("atmosphere", ctypes.c_int32),
                 ("media", ctypes.POINTER(MediumDesc)),
                 ("env_width", ctypes.c_uint32), ("env_height", ctypes.c_uint32),
-                ("env_rgb", ctypes.POINTER(ctypes.c_float))]
+                ("env_rgb", ctypes.POINTER(ctypes.c_float)),
+                ("num_quadrics", ctypes.c_uint32), ("reserved_q", ctypes.c_uint32),
+                ("quadrics", ctypes.POINTER(QuadricDesc))]
 
 
 class RenderParams(ctypes.Structure):

@@ -198,6 +198,9 @@ class Scene:
         d.num_media = len(self.media)
         d.atmosphere = self.atmosphere
         d.media = self._arr(rrhip.MediumDesc, self.media)
+        qs = getattr(self, "quadrics", [])
+        d.num_quadrics = len(qs)
+        d.quadrics = self._arr(rrhip.QuadricDesc, qs)
         env = getattr(self, "env", None)
         if env is not None:
             env = np.ascontiguousarray(env, np.float32)
@@ -206,6 +209,20 @@ class Scene:
             d.env_rgb = env.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
         self._keep.append(d)
         return d
+
+    def add_quadric(self, kind, mat, object_to_world, radius=1.0, zmin=-1.0, zmax=1.0, height=1.0,
+                    thetamax=360.0):
+        """A RenderMan quadric (rrhip_quadric_desc): object_to_world a 3x4
+        or 4x4 row-major matrix."""
+        q = rrhip.QuadricDesc()
+        q.kind, q.material = kind, mat
+        q.radius, q.zmin, q.zmax, q.height, q.thetamax = radius, zmin, zmax, height, thetamax
+        m = np.asarray(object_to_world, np.float64)[:3].reshape(-1)
+        for k in range(12):
+            q.object_to_world[k] = m[k]
+        if not hasattr(self, "quadrics"):
+            self.quadrics = []
+        self.quadrics.append(q)
 
     def set_env(self, rgb):
         """Lat-long environment map of the dome light: (height, width, 3)
@@ -423,3 +440,45 @@ def batch_rooms(n_jobs=32, seed=1, grid=128):
                 s.add_sphere(c, r, 0)
         jobs.append((s, spp, 64 if media else 8))
     return jobs
+
+
+def _place(t, rx_deg=0.0, scale=1.0):
+    """Row-major 3x4 object-to-world: scale, then a rotation about x, then a
+    translation (RIB's Translate; Rotate; order)."""
+    a = np.radians(rx_deg)
+    r = np.array([[1, 0, 0], [0, np.cos(a), -np.sin(a)], [0, np.sin(a), np.cos(a)]]) * scale
+    return np.hstack([r, np.asarray(t, float).reshape(3, 1)])
+
+
+def quadric_zoo(width=96, height=54):
+    """render.rs's quadric_zoo fixture (tests/fixtures/quadric_zoo.rib) with
+    the quadrics this backend has: a sphere, a cylinder and a cone on the
+    top row, a paraboloid, a disk and a 270-degree partial sphere below,
+    plus a z-clipped sphere where the fixture has a torus and a second cone
+    where it has a hyperboloid; the fixture's colours, "plastic" / "metal"
+    surfaces, distant key light, point fill light and ground quad."""
+    s = Scene(width, height, 40.0, eye=(0, 0, 0), look_at=(0, 0, 1))
+    Q = rrhip
+    d = np.array([0.0, 0.0, 7.0]) - np.array([4.0, 8.0, -6.0])
+    s.add_light(light(rrhip.LIGHT_DISTANT, intensity=0.85, p=tuple(d / np.linalg.norm(d))))
+    s.add_light(light(rrhip.LIGHT_POINT, intensity=0.35, p=(-5, 4, 2)))
+    add = s.add_quadric
+    add(Q.QUADRIC_SPHERE, s.add_material(plastic((0.9, 0.25, 0.2), 0.15)), _place((-3.4, 1.2, 7)),
+        radius=0.9, zmin=-0.9, zmax=0.9)
+    add(Q.QUADRIC_SPHERE, s.add_material(metal((0.95, 0.7, 0.15), 0.08)), _place((-1.1, 1.2, 7), -55),
+        radius=0.65, zmin=-0.3, zmax=0.3)
+    add(Q.QUADRIC_CYLINDER, s.add_material(plastic((0.2, 0.55, 0.9))), _place((1.1, 1.2, 7), -90),
+        radius=0.55, zmin=-0.85, zmax=0.85)
+    add(Q.QUADRIC_CONE, s.add_material(plastic((0.25, 0.8, 0.35))), _place((3.4, 0.35, 7), -90),
+        radius=0.75, height=1.7)
+    add(Q.QUADRIC_PARABOLOID, s.add_material(plastic((0.7, 0.3, 0.85))), _place((-3.4, -1.6, 7), -90),
+        radius=0.8, zmin=0.0, zmax=1.5)
+    add(Q.QUADRIC_CONE, s.add_material(metal((0.9, 0.45, 0.6), 0.15)), _place((-1.1, -1.1, 7), -90),
+        radius=0.6, height=1.2, thetamax=300.0)
+    add(Q.QUADRIC_DISK, s.add_material(metal((0.85, 0.85, 0.9), 0.05)), _place((1.1, -1.1, 7), -65),
+        radius=0.8, height=0.0)
+    add(Q.QUADRIC_SPHERE, s.add_material(plastic((0.3, 0.85, 0.8))), _place((3.4, -1.1, 7)),
+        radius=0.8, zmin=-0.8, zmax=0.8, thetamax=270.0)
+    ground = s.add_material(matte((0.75, 0.75, 0.75)))
+    s.add_polygon([(-12, -2.2, 0.5), (12, -2.2, 0.5), (12, -2.2, 18), (-12, -2.2, 18)], ground)
+    return s

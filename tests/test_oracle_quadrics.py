@@ -1,0 +1,109 @@
+"""The oracle's RenderMan quadrics (src/geometry/sphere.rs, cylinder.rs,
+cone.rs, extra.rs Disk / Paraboloid) pinned by closed-form hits: the hit
+distance and normal of axis-aligned rays against each kind, the z and
+theta windows (including the cylinder's fall-back to its far root and the
+sphere's absence of one), placement by a transform (translation, rotation
+and scale move the hit and scale t), and the front-face rule (only the
+sphere decides it)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "render.rs_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import orc  # noqa: E402
+import rrhip  # noqa: E402
+import scenes  # noqa: E402
+
+I34 = np.hstack([np.eye(3), np.zeros((3, 1))])
+
+
+def one(kind, m=I34, **kw):
+    s = scenes.Scene(8, 8, 40.0, eye=(0, 0, -5), look_at=(0, 0, 0))
+    mat = s.add_material(scenes.matte((0.5, 0.5, 0.5)))
+    s.add_quadric(kind, mat, m, **kw)
+    return s.desc()
+
+
+def hit(desc, o, d):
+    return orc.scene_intersect(desc, np.array(o, float), np.array(d, float))
+
+
+def test_cylinder_closed_form_and_windows():
+    d = one(rrhip.QUADRIC_CYLINDER, radius=1.0, zmin=-1.0, zmax=1.0)
+    t, p, n, front, _ = hit(d, (5, 0, 0.5), (-1, 0, 0))
+    assert t == pytest.approx(4.0, abs=1e-12)
+    np.testing.assert_allclose(n, [1, 0, 0], atol=1e-12)
+    assert front  # the cylinder keeps Intersection::new's front_face
+    # outside the z window: a miss; from inside the near root is behind
+    assert hit(d, (5, 0, 1.5), (-1, 0, 0)) is None
+    t, _, n, _, _ = hit(d, (0, 0, 0.5), (1, 0, 0))
+    assert t == pytest.approx(1.0, abs=1e-12)
+    # near root below zmin, far root inside: the far one (cylinder.rs:74-84)
+    a = np.array([-3.0, 0.0, -2.0])
+    b = np.array([3.0, 0.0, 0.0])
+    dd = (b - a) / np.linalg.norm(b - a)
+    t, p, _, _, _ = hit(d, a, dd)
+    np.testing.assert_allclose(p[0], 1.0, atol=1e-12)  # leaves through x = +1 inside the window
+    # theta window: phi of (-1, 0) is 180 > 90
+    d90 = one(rrhip.QUADRIC_CYLINDER, radius=1.0, zmin=-1.0, zmax=1.0, thetamax=90.0)
+    assert hit(d90, (-5, 0, 0), (1, 0, 0)) is None
+    assert hit(d90, (0.5, 5, 0), (0, -1, 0)) is not None
+
+
+def test_sphere_windows_have_no_far_root():
+    d = one(rrhip.QUADRIC_SPHERE, radius=1.0, zmin=-0.5, zmax=1.0)
+    t, p, n, front, _ = hit(d, (0, 0, 5), (0, 0, -1))
+    assert t == pytest.approx(4.0, abs=1e-12) and front
+    np.testing.assert_allclose(n, [0, 0, 1], atol=1e-12)
+    # from below, the near root z = -1 is cut away and the sphere does not fall back
+    assert hit(d, (0, 0, -5), (0, 0, 1)) is None
+    # from inside the normal points out and the hit is not a front face
+    t, _, n, front, _ = hit(d, (0, 0, 0), (0, 0, 1))
+    assert t == pytest.approx(1.0) and not front
+    # theta window: 270 degrees keeps phi = 180, drops phi = 315
+    d2 = one(rrhip.QUADRIC_SPHERE, radius=1.0, zmin=-1.0, zmax=1.0, thetamax=270.0)
+    assert hit(d2, (-5, 0, 0), (1, 0, 0)) is not None
+    c = np.sqrt(0.5)
+    assert hit(d2, (5 * c, -5 * c, 0), (-c, c, 0)) is None
+
+
+def test_cone_disk_paraboloid_closed_forms():
+    # cone x^2 + y^2 = (r / h)^2 z^2, h = 2, r = 1: at z = 1 the radius is 0.5
+    d = one(rrhip.QUADRIC_CONE, radius=1.0, height=2.0)
+    t, p, n, _, _ = hit(d, (5, 0, 1), (-1, 0, 0))
+    assert t == pytest.approx(4.5, abs=1e-12)
+    np.testing.assert_allclose(n, np.array([1, 0, -0.5]) / np.sqrt(1.25), atol=1e-12)
+    assert hit(d, (5, 0, 2.5), (-1, 0, 0)) is None  # above the height
+    # disk at z = 1, radius 2: normal +z for rays from either side
+    d = one(rrhip.QUADRIC_DISK, radius=2.0, height=1.0)
+    t, p, n, front, _ = hit(d, (0.5, 0, 5), (0, 0, -1))
+    assert t == pytest.approx(4.0) and front
+    np.testing.assert_allclose(n, [0, 0, 1], atol=1e-12)
+    t, _, n, _, _ = hit(d, (0.5, 0, -5), (0, 0, 1))
+    assert t == pytest.approx(6.0)
+    np.testing.assert_allclose(n, [0, 0, 1], atol=1e-12)
+    assert hit(d, (2.5, 0, 5), (0, 0, -1)) is None
+    # paraboloid x^2 + y^2 = k z, k = rmax^2 / zmax = 1
+    d = one(rrhip.QUADRIC_PARABOLOID, radius=1.0, zmin=0.0, zmax=1.0)
+    t, p, n, _, _ = hit(d, (5, 0, 0.25), (-1, 0, 0))
+    assert t == pytest.approx(4.5, abs=1e-12)
+    np.testing.assert_allclose(n, np.array([1.0, 0, -1.0]) / np.sqrt(2.0), atol=1e-12)
+    assert hit(d, (5, 0, 1.5), (-1, 0, 0)) is None
+
+
+def test_transformed_quadric():
+    """Scale 2, rotate 90 degrees about x (object z -> world y), translate
+    (1, 2, 3): a unit cylinder becomes radius 2 along y through (1, *, 3)."""
+    R = np.array([[1, 0, 0], [0, 0, 1], [0, -1, 0]], float) * 2.0
+    m = np.hstack([R, np.array([[1.0], [2.0], [3.0]])])
+    d = one(rrhip.QUADRIC_CYLINDER, m, radius=1.0, zmin=-1.0, zmax=1.0)
+    t, p, n, _, _ = hit(d, (10, 2.5, 3), (-1, 0, 0))
+    assert t == pytest.approx(10 - 3.0, abs=1e-9)  # surface at x = 1 + 2
+    np.testing.assert_allclose(p, [3, 2.5, 3], atol=1e-9)
+    np.testing.assert_allclose(n, [1, 0, 0], atol=1e-9)
+    assert hit(d, (10, 4.5, 3), (-1, 0, 0)) is None  # beyond the scaled z window (|y - 2| > 2)
