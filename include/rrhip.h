@@ -121,18 +121,23 @@ typedef struct rrhip_sphere_desc {
  * quadric's own (z the axis), placed by object_to_world (row-major 3x4);
  * thetamax in degrees. sphere: radius, zmin, zmax; cylinder: radius, zmin,
  * zmax; cone: height, radius (apex at the origin, radius at z = height);
- * disk: height (its z), radius; paraboloid: radius (= rmax), zmin, zmax. */
+ * disk: height (its z), radius; paraboloid: radius (= rmax), zmin, zmax;
+ * hyperboloid: the line p1 - p2 swept about z; torus: radius (major),
+ * height (minor), zmin / zmax (phimin / phimax, degrees). */
 enum {
   RRHIP_QUADRIC_SPHERE = 0,
   RRHIP_QUADRIC_CYLINDER = 1,
   RRHIP_QUADRIC_CONE = 2,
   RRHIP_QUADRIC_DISK = 3,
-  RRHIP_QUADRIC_PARABOLOID = 4
+  RRHIP_QUADRIC_PARABOLOID = 4,
+  RRHIP_QUADRIC_HYPERBOLOID = 5,
+  RRHIP_QUADRIC_TORUS = 6
 };
 typedef struct rrhip_quadric_desc {
   uint32_t kind;
   uint32_t material;
   double radius, zmin, zmax, height, thetamax;
+  double p1[3], p2[3];
   double object_to_world[12];
 } rrhip_quadric_desc;
 

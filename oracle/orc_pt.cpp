@@ -793,6 +793,102 @@ static bool theta_ok(double x, double y, double thetamax) {
   return phi <= thetamax;
 }
 
+// Real roots of a cubic / quartic, ascending (extra.rs solve_cubic /
+// solve_quartic: Cardano or the trigonometric form; Ferrari through the
+// resolvent cubic, Newton-polished, residual-filtered, near-duplicates
+// merged).
+static std::vector<double> cubic_real_roots(double a3, double a2, double a1, double a0) {
+  std::vector<double> v;
+  if (std::fabs(a3) < 1e-12) {
+    if (std::fabs(a2) < 1e-12) {
+      if (std::fabs(a1) >= 1e-12) v.push_back(-a0 / a1);
+      return v;
+    }
+    const double disc = a1 * a1 - 4.0 * a2 * a0;
+    if (disc < 0.0) return v;
+    const double s = std::sqrt(disc);
+    v = {(-a1 - s) / (2.0 * a2), (-a1 + s) / (2.0 * a2)};
+    std::sort(v.begin(), v.end());
+    return v;
+  }
+  const double a = a2 / a3, b = a1 / a3, c = a0 / a3;
+  const double p = b - a * a / 3.0, q = 2.0 * a * a * a / 27.0 - a * b / 3.0 + c, shift = -a / 3.0;
+  const double disc = q * q / 4.0 + p * p * p / 27.0;
+  if (disc > 0.0) {
+    const double s = std::sqrt(disc);
+    v.push_back(std::cbrt(-q / 2.0 + s) + std::cbrt(-q / 2.0 - s) + shift);
+  } else {
+    const double m = 2.0 * std::sqrt(std::fmax(-p / 3.0, 0.0));
+    if (m < 1e-30) {
+      v.push_back(shift);
+    } else {
+      const double th = std::acos(std::fmin(std::fmax(3.0 * q / (p * m), -1.0), 1.0)) / 3.0;
+      for (int k = 0; k < 3; k++) v.push_back(m * std::cos(th - 2.0 * PI * k / 3.0) + shift);
+    }
+  }
+  std::sort(v.begin(), v.end());
+  return v;
+}
+
+static std::vector<double> quartic_real_roots(double a4, double a3, double a2, double a1, double a0) {
+  if (std::fabs(a4) < 1e-12) return cubic_real_roots(a3, a2, a1, a0);
+  const double b = a3 / a4, c = a2 / a4, d = a1 / a4, e = a0 / a4;
+  const double p = c - 3.0 * b * b / 8.0;
+  const double q = d - b * c / 2.0 + b * b * b / 8.0;
+  const double r = e - b * d / 4.0 + b * b * c / 16.0 - 3.0 * b * b * b * b / 256.0;
+  const double shift = -b / 4.0;
+  const auto poly = [&](double t) { return (((a4 * t + a3) * t + a2) * t + a1) * t + a0; };
+  const auto dpoly = [&](double t) { return ((4.0 * a4 * t + 3.0 * a3) * t + 2.0 * a2) * t + a1; };
+  const auto polish = [&](double t) {
+    for (int it = 0; it < 8; it++) {
+      const double df = dpoly(t);
+      if (std::fabs(df) < 1e-30) break;
+      const double step = poly(t) / df;
+      t -= step;
+      if (std::fabs(step) < 1e-12 * std::fmax(std::fabs(t), 1.0)) break;
+    }
+    return t;
+  };
+  std::vector<double> roots;
+  if (std::fabs(q) < 1e-12) {
+    const double disc = p * p - 4.0 * r;
+    if (disc >= 0.0) {
+      const double s = std::sqrt(disc);
+      for (double y2 : {(-p - s) / 2.0, (-p + s) / 2.0})
+        if (y2 >= 0.0) {
+          const double y = std::sqrt(y2);
+          roots.push_back(polish(y + shift));
+          roots.push_back(polish(-y + shift));
+        }
+    }
+  } else {
+    double z = -1.0;
+    for (double zz : cubic_real_roots(1.0, 2.0 * p, p * p - 4.0 * r, -q * q))
+      if (zz > 1e-12 && zz > z) z = zz;
+    if (z > 0.0) {
+      const double m = std::sqrt(z), n = q / (2.0 * m);
+      for (int sg = 0; sg < 2; sg++) {
+        const double bq = sg == 0 ? m : -m, cq = (p + z) / 2.0 + (sg == 0 ? -n : n);
+        const double disc = bq * bq - 4.0 * cq;
+        if (disc >= 0.0) {
+          const double s = std::sqrt(disc);
+          roots.push_back(polish((-bq - s) / 2.0 + shift));
+          roots.push_back(polish((-bq + s) / 2.0 + shift));
+        }
+      }
+    }
+  }
+  std::vector<double> kept;
+  for (double t : roots)
+    if (std::isfinite(t) && std::fabs(poly(t)) < 1e-6 * (1.0 + std::pow(std::fabs(t), 4) * std::fabs(a4)))
+      kept.push_back(t);
+  std::sort(kept.begin(), kept.end());
+  std::vector<double> out;
+  for (double t : kept)
+    if (out.empty() || std::fabs(t - out.back()) >= 1e-9) out.push_back(t);
+  return out;
+}
+
 bool QuadricObj::intersect(const Ray& ray, Intersection* out) const {
   const Vec3 lo = inverse.transform_point(ray.origin);
   const Vec3 ld = inverse.transform_vec(ray.direction).normalize();
@@ -885,7 +981,82 @@ bool QuadricObj::intersect(const Ray& ray, Intersection* out) const {
       ln = Vec3{0.0, 0.0, 1.0};
       break;
     }
-    default: {  // extra.rs Paraboloid::intersect
+    case RRHIP_QUADRIC_HYPERBOLOID: {  // extra.rs Hyperboloid::intersect
+      const double zlo = std::fmin(p1[2], p2[2]), zhi = std::fmax(p1[2], p2[2]);
+      if (std::fabs(p2[2] - p1[2]) < EPSILON) {
+        if (std::fabs(ld.z) < EPSILON) return false;
+        t = (p1[2] - lo.z) / ld.z;
+        if (t <= EPSILON) return false;
+        p = lo + ld * t;
+        const double rr = p.x * p.x + p.y * p.y;
+        const double ra = p1[0] * p1[0] + p1[1] * p1[1], rb = p2[0] * p2[0] + p2[1] * p2[1];
+        if (rr < std::fmin(ra, rb) || rr > std::fmax(ra, rb) || !theta_ok(p.x, p.y, thetamax)) return false;
+        ln = Vec3{0.0, 0.0, 1.0};
+        break;
+      }
+      const double dz = p2[2] - p1[2];
+      const double sx = (p2[0] - p1[0]) / dz, sy = (p2[1] - p1[1]) / dz;
+      const double cx = p1[0] - p1[2] * sx, cy = p1[1] - p1[2] * sy;
+      const double pa = sx * sx + sy * sy, pb = 2.0 * (cx * sx + cy * sy), pc = cx * cx + cy * cy;
+      const double a = ld.x * ld.x + ld.y * ld.y - pa * ld.z * ld.z;
+      const double b = 2.0 * (lo.x * ld.x + lo.y * ld.y - pa * lo.z * ld.z) - pb * ld.z;
+      const double c = lo.x * lo.x + lo.y * lo.y - pa * lo.z * lo.z - pb * lo.z - pc;
+      double cands[2];
+      int nc = 2;
+      if (std::fabs(a) < EPSILON) {
+        if (std::fabs(b) < EPSILON) return false;
+        cands[0] = -c / b;
+        nc = 1;
+      } else {
+        const double disc = b * b - 4.0 * a * c;
+        if (disc < 0.0) return false;
+        const double s = std::sqrt(disc);
+        cands[0] = (-b - s) / (2.0 * a);
+        cands[1] = (-b + s) / (2.0 * a);
+      }
+      bool hit = false;
+      for (int i = 0; i < nc && !hit; i++) {
+        const double tc = cands[i];
+        if (tc <= EPSILON) continue;
+        const Vec3 pcand = lo + ld * tc;
+        if (pcand.z < zlo || pcand.z > zhi || !theta_ok(pcand.x, pcand.y, thetamax)) continue;
+        t = tc;
+        p = pcand;
+        hit = true;
+      }
+      if (!hit) return false;
+      ln = Vec3{2.0 * p.x, 2.0 * p.y, -(2.0 * pa * p.z + pb)}.normalize();
+      break;
+    }
+    case RRHIP_QUADRIC_TORUS: {  // extra.rs Torus::intersect / phi_ok
+      const double R = radius, r = height;
+      const double g = ld.dot(ld), h = 2.0 * lo.dot(ld), i = lo.dot(lo) + R * R - r * r;
+      const double j = ld.x * ld.x + ld.y * ld.y, k = 2.0 * (lo.x * ld.x + lo.y * ld.y), l = lo.x * lo.x + lo.y * lo.y;
+      const double f4 = 4.0 * R * R;
+      bool hit = false;
+      for (double tc : quartic_real_roots(g * g, 2.0 * g * h, h * h + 2.0 * g * i - f4 * j, 2.0 * h * i - f4 * k,
+                                          i * i - f4 * l)) {
+        if (tc <= EPSILON) continue;
+        const Vec3 pcand = lo + ld * tc;
+        if (zmax - zmin < 360.0) {
+          const double rxy = std::sqrt(pcand.x * pcand.x + pcand.y * pcand.y);
+          double phi = std::atan2(pcand.z, rxy - R) * 180.0 / PI;
+          while (phi < zmin) phi += 360.0;
+          while (phi >= zmin + 360.0) phi -= 360.0;
+          if (phi > zmax) continue;
+        }
+        if (!theta_ok(pcand.x, pcand.y, thetamax)) continue;
+        const double rxy = std::fmax(std::sqrt(pcand.x * pcand.x + pcand.y * pcand.y), 1e-12), sc = 1.0 - R / rxy;
+        t = tc;
+        p = pcand;
+        ln = Vec3{p.x * sc, p.y * sc, p.z}.normalize();
+        hit = true;
+        break;
+      }
+      if (!hit) return false;
+      break;
+    }
+    case RRHIP_QUADRIC_PARABOLOID: {  // extra.rs Paraboloid::intersect
       if (std::fabs(zmax) < EPSILON) return false;
       const double k = radius * radius / zmax;
       const double a = ld.x * ld.x + ld.y * ld.y;
@@ -918,6 +1089,8 @@ bool QuadricObj::intersect(const Ray& ray, Intersection* out) const {
       ln = Vec3{2.0 * p.x, 2.0 * p.y, -k}.normalize();
       break;
     }
+    default:
+      return false;
   }
   const Vec3 wp = transform.transform_point(p);
   out->t = (wp - ray.origin).length();
@@ -1797,6 +1970,10 @@ void Scene::load(const rrhip_scene_desc& d) {
     o.zmax = q.zmax;
     o.height = q.height;
     o.thetamax = q.thetamax;
+    for (int c = 0; c < 3; c++) {
+      o.p1[c] = q.p1[c];
+      o.p2[c] = q.p2[c];
+    }
     o.transform = Mat4::identity();
     for (int r = 0; r < 3; r++)
       for (int c = 0; c < 4; c++) o.transform.m[r][c] = q.object_to_world[4 * r + c];

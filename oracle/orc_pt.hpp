@@ -195,6 +195,7 @@ struct SphereObj {
 struct QuadricObj {
   uint32_t kind = 0, material = 0;
   double radius = 0, zmin = 0, zmax = 0, height = 0, thetamax = 360;
+  double p1[3] = {0, 0, 0}, p2[3] = {0, 0, 0};  // hyperboloid
   Mat4 transform, inverse;
   bool intersect(const Ray& ray, Intersection* out) const;
 };

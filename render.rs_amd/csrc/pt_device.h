@@ -576,6 +576,122 @@ __device__ __forceinline__ bool quad_roots(float a, float b, float c, float& rm,
   return true;
 }
 
+// Real roots of a3 t^3 + a2 t^2 + a1 t + a0 (ascending; a3 ~ 0 falls back to
+// the quadratic / linear case): Cardano with one real root, the
+// trigonometric form with three.
+__device__ inline int cubic_roots(double a3, double a2, double a1, double a0, double* r) {
+  if (fabs(a3) < 1e-12) {
+    if (fabs(a2) < 1e-12) {
+      if (fabs(a1) < 1e-12) return 0;
+      r[0] = -a0 / a1;
+      return 1;
+    }
+    const double disc = a1 * a1 - 4.0 * a2 * a0;
+    if (disc < 0.0) return 0;
+    const double s = sqrt(disc);
+    const double x = (-a1 - s) / (2.0 * a2), y = (-a1 + s) / (2.0 * a2);
+    r[0] = fmin(x, y);
+    r[1] = fmax(x, y);
+    return 2;
+  }
+  const double a = a2 / a3, b = a1 / a3, c = a0 / a3;
+  const double p = b - a * a / 3.0, q = 2.0 * a * a * a / 27.0 - a * b / 3.0 + c, shift = -a / 3.0;
+  const double disc = q * q / 4.0 + p * p * p / 27.0;
+  if (disc > 0.0) {
+    const double s = sqrt(disc);
+    r[0] = cbrt(-q / 2.0 + s) + cbrt(-q / 2.0 - s) + shift;
+    return 1;
+  }
+  const double m = 2.0 * sqrt(fmax(-p / 3.0, 0.0));
+  if (m < 1e-30) {
+    r[0] = shift;
+    return 1;
+  }
+  const double th = acos(fmin(fmax(3.0 * q / (p * m), -1.0), 1.0)) / 3.0;
+  for (int k = 0; k < 3; k++) r[k] = m * cos(th - 2.0 * 3.14159265358979323846 * k / 3.0) + shift;
+  for (int i = 0; i < 3; i++)  // ascending
+    for (int j = i + 1; j < 3; j++)
+      if (r[j] < r[i]) {
+        const double tmp = r[i];
+        r[i] = r[j];
+        r[j] = tmp;
+      }
+  return 3;
+}
+
+// Real roots of a4 t^4 + ... + a0 in f64 (ascending, near-duplicates
+// merged): Ferrari's reduction to a depressed quartic, split into two
+// quadratics through the largest positive root of the resolvent cubic (or
+// a biquadratic when the odd term vanishes), each root polished by Newton
+// steps and kept when its residual is small (extra.rs solve_quartic).
+__device__ inline int quartic_roots(double a4, double a3, double a2, double a1, double a0, double* out) {
+  if (fabs(a4) < 1e-12) return cubic_roots(a3, a2, a1, a0, out);
+  const double b = a3 / a4, c = a2 / a4, d = a1 / a4, e = a0 / a4;
+  const double p = c - 3.0 * b * b / 8.0;
+  const double q = d - b * c / 2.0 + b * b * b / 8.0;
+  const double r = e - b * d / 4.0 + b * b * c / 16.0 - 3.0 * b * b * b * b / 256.0;
+  const double shift = -b / 4.0;
+  double cand[4];
+  int n = 0;
+  if (fabs(q) < 1e-12) {
+    const double disc = p * p - 4.0 * r;
+    if (disc >= 0.0) {
+      const double s = sqrt(disc);
+      const double y2s[2] = {(-p - s) / 2.0, (-p + s) / 2.0};
+      for (int i = 0; i < 2; i++)
+        if (y2s[i] >= 0.0) {
+          const double y = sqrt(y2s[i]);
+          cand[n++] = y + shift;
+          cand[n++] = -y + shift;
+        }
+    }
+  } else {
+    double z3[3];
+    const int nz = cubic_roots(1.0, 2.0 * p, p * p - 4.0 * r, -q * q, z3);
+    double z = -1.0;
+    for (int i = 0; i < nz; i++)
+      if (z3[i] > 1e-12 && z3[i] > z) z = z3[i];
+    if (z > 0.0) {
+      const double m = sqrt(z), nn = q / (2.0 * m);
+      for (int sgn = 0; sgn < 2; sgn++) {
+        const double bq = sgn == 0 ? m : -m;
+        const double cq = (p + z) / 2.0 + (sgn == 0 ? -nn : nn);
+        const double disc = bq * bq - 4.0 * cq;
+        if (disc >= 0.0) {
+          const double s = sqrt(disc);
+          cand[n++] = (-bq - s) / 2.0 + shift;
+          cand[n++] = (-bq + s) / 2.0 + shift;
+        }
+      }
+    }
+  }
+  int m = 0;
+  for (int i = 0; i < n; i++) {
+    double t = cand[i];
+    for (int it = 0; it < 8; it++) {
+      const double f = (((a4 * t + a3) * t + a2) * t + a1) * t + a0;
+      const double df = ((4.0 * a4 * t + 3.0 * a3) * t + 2.0 * a2) * t + a1;
+      if (fabs(df) < 1e-30) break;
+      const double step = f / df;
+      t -= step;
+      if (fabs(step) < 1e-12 * fmax(fabs(t), 1.0)) break;
+    }
+    const double res = (((a4 * t + a3) * t + a2) * t + a1) * t + a0;
+    if (isfinite(t) && fabs(res) < 1e-6 * (1.0 + t * t * t * t * fabs(a4))) out[m++] = t;
+  }
+  for (int i = 0; i < m; i++)  // ascending, then merge near-duplicates
+    for (int j = i + 1; j < m; j++)
+      if (out[j] < out[i]) {
+        const double tmp = out[i];
+        out[i] = out[j];
+        out[j] = tmp;
+      }
+  int k = 0;
+  for (int i = 0; i < m; i++)
+    if (k == 0 || fabs(out[i] - out[k - 1]) >= 1e-9) out[k++] = out[i];
+  return k;
+}
+
 // The quadric's hit as render.rs computes it: the object-space ray with its
 // direction normalised, the kind's root order and windows, t = the world
 // distance to the hit, the normal through the inverse transpose (not
@@ -663,7 +779,78 @@ __device__ __forceinline__ bool hit_quadric(const PtQuadric& q, V3 o, V3 d, floa
       ln = v3(0.f, 0.f, 1.f);
       break;
     }
-    default: {  // extra.rs Paraboloid: x^2 + y^2 = k z, first root in the windows
+    case 5: {  // extra.rs Hyperboloid: the swept line x^2 + y^2 = pa z^2 + pb z + pc
+      const float zlo = fminf(q.z1, q.z2), zhi = fmaxf(q.z1, q.z2);
+      if (fabsf(q.z2 - q.z1) < kObjEps) {  // an annulus in the plane z = z1
+        if (fabsf(ld.z) < kObjEps) return false;
+        t = (q.z1 - lo.z) / ld.z;
+        if (t <= kObjEps) return false;
+        p = lo + ld * t;
+        const float rr = p.x * p.x + p.y * p.y;
+        if (rr < fminf(q.r1, q.r2) || rr > fmaxf(q.r1, q.r2) || !theta_ok(p.x, p.y, q.thetamax)) return false;
+        ln = v3(0.f, 0.f, 1.f);
+        break;
+      }
+      const float a = ld.x * ld.x + ld.y * ld.y - q.pa * ld.z * ld.z;
+      const float b = 2.f * (lo.x * ld.x + lo.y * ld.y - q.pa * lo.z * ld.z) - q.pb * ld.z;
+      const float c = lo.x * lo.x + lo.y * lo.y - q.pa * lo.z * lo.z - q.pb * lo.z - q.pc;
+      float t1, t2;
+      int nc = 2;
+      if (fabsf(a) < kObjEps) {
+        if (fabsf(b) < kObjEps) return false;
+        t1 = -c / b;
+        t2 = 0.f;
+        nc = 1;
+      } else if (!quad_roots(a, b, c, t1, t2)) {
+        return false;
+      }
+      bool hit = false;
+      for (int i = 0; i < nc && !hit; i++) {
+        const float tc = i == 0 ? t1 : t2;
+        if (tc <= kObjEps) continue;
+        const V3 pc = lo + ld * tc;
+        if (pc.z < zlo || pc.z > zhi || !theta_ok(pc.x, pc.y, q.thetamax)) continue;
+        t = tc;
+        p = pc;
+        hit = true;
+      }
+      if (!hit) return false;
+      ln = normalize(v3(2.f * p.x, 2.f * p.y, -(2.f * q.pa * p.z + q.pb)));
+      break;
+    }
+    case 6: {  // extra.rs Torus (f64 quartic): the first root in the phi and theta windows
+      const double R = q.radius, r = q.height;
+      const double ox = lo.x, oy = lo.y, oz = lo.z, dx = ld.x, dy = ld.y, dz = ld.z;
+      const double g = dx * dx + dy * dy + dz * dz;
+      const double hh = 2.0 * (ox * dx + oy * dy + oz * dz);
+      const double ii = ox * ox + oy * oy + oz * oz + R * R - r * r;
+      const double j = dx * dx + dy * dy, k = 2.0 * (ox * dx + oy * dy), l = ox * ox + oy * oy;
+      const double f4 = 4.0 * R * R;
+      double roots[4];
+      const int nr = quartic_roots(g * g, 2.0 * g * hh, hh * hh + 2.0 * g * ii - f4 * j, 2.0 * hh * ii - f4 * k,
+                                   ii * ii - f4 * l, roots);
+      bool hit = false;
+      for (int i = 0; i < nr && !hit; i++) {
+        if (roots[i] <= 1e-6) continue;
+        const double px = ox + dx * roots[i], py = oy + dy * roots[i], pz = oz + dz * roots[i];
+        if (q.zmax - q.zmin < 360.f) {  // phi window (extra.rs Torus::phi_ok)
+          const double rxy = sqrt(px * px + py * py);
+          double phi = atan2(pz, rxy - R) * (180.0 / 3.14159265358979323846);
+          while (phi < q.zmin) phi += 360.0;
+          while (phi >= q.zmin + 360.0) phi -= 360.0;
+          if (phi > q.zmax) continue;
+        }
+        if (!theta_ok((float)px, (float)py, q.thetamax)) continue;
+        const double rxy = fmax(sqrt(px * px + py * py), 1e-12), sc = 1.0 - R / rxy;
+        t = (float)roots[i];
+        p = v3((float)px, (float)py, (float)pz);
+        ln = normalize(v3((float)(px * sc), (float)(py * sc), (float)pz));
+        hit = true;
+      }
+      if (!hit) return false;
+      break;
+    }
+    case 4: {  // extra.rs Paraboloid: x^2 + y^2 = k z, first root in the windows
       if (fabsf(q.zmax) < kObjEps) return false;
       const float a = ld.x * ld.x + ld.y * ld.y;
       const float b = 2.f * (lo.x * ld.x + lo.y * ld.y) - q.k * ld.z;
@@ -692,6 +879,8 @@ __device__ __forceinline__ bool hit_quadric(const PtQuadric& q, V3 o, V3 d, floa
       ln = normalize(v3(2.f * p.x, 2.f * p.y, -q.k));
       break;
     }
+    default:
+      return false;
   }
   const V3 wp = q_point(q.fwd, p);
   const V3 dw = wp - o;

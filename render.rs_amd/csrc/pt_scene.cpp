@@ -477,7 +477,8 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
   if (d.num_quadrics && !d.quadrics) return *err = "null quadrics", false;
   for (uint32_t i = 0; i < d.num_quadrics; i++) {
     const rrhip_quadric_desc& q = d.quadrics[i];
-    if (q.kind > RRHIP_QUADRIC_PARABOLOID) return *err = "unknown quadric kind", false;
+    if (q.kind > RRHIP_QUADRIC_TORUS) return *err = "unknown quadric kind", false;
+    if (!finite3(q.p1) || !finite3(q.p2)) return *err = "quadric parameters must be finite", false;
     if (q.material >= d.num_materials) return *err = "quadric material out of range", false;
     const double par[5] = {q.radius, q.zmin, q.zmax, q.height, q.thetamax};
     for (double v : par)
@@ -516,6 +517,20 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
     g2.thetamax = (float)q.thetamax;
     if (q.kind == RRHIP_QUADRIC_CONE) g2.k = (float)((q.radius / q.height) * (q.radius / q.height));
     if (q.kind == RRHIP_QUADRIC_PARABOLOID && q.zmax != 0.0) g2.k = (float)(q.radius * q.radius / q.zmax);
+    if (q.kind == RRHIP_QUADRIC_HYPERBOLOID) {  // extra.rs Hyperboloid::profile
+      const double dz = q.p2[2] - q.p1[2];
+      if (std::fabs(dz) >= 1e-6) {
+        const double sx = (q.p2[0] - q.p1[0]) / dz, sy = (q.p2[1] - q.p1[1]) / dz;
+        const double cx = q.p1[0] - q.p1[2] * sx, cy = q.p1[1] - q.p1[2] * sy;
+        g2.pa = (float)(sx * sx + sy * sy);
+        g2.pb = (float)(2.0 * (cx * sx + cy * sy));
+        g2.pc = (float)(cx * cx + cy * cy);
+      }
+      g2.z1 = (float)q.p1[2];
+      g2.z2 = (float)q.p2[2];
+      g2.r1 = (float)(q.p1[0] * q.p1[0] + q.p1[1] * q.p1[1]);
+      g2.r2 = (float)(q.p2[0] * q.p2[0] + q.p2[1] * q.p2[1]);
+    }
     for (int k = 0; k < 12; k++) {
       g2.inv[k] = (float)inv[k];
       g2.fwd[k] = (float)m[k];
@@ -591,11 +606,21 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
   for (uint32_t i = 0; i < d.num_spheres; i++) grow(d3(d.spheres[i].center), d.spheres[i].radius);
   for (uint32_t i = 0; i < d.num_quadrics; i++) {  // the object-space extent's corners, placed
     const rrhip_quadric_desc& q = d.quadrics[i];
-    const double r = std::fabs(q.radius);
+    double r = std::fabs(q.radius);
     double zlo = q.zmin, zhi = q.zmax;
     if (q.kind == RRHIP_QUADRIC_CONE || q.kind == RRHIP_QUADRIC_DISK) {
       zlo = std::min(0.0, q.height);
       zhi = std::max(0.0, q.height);
+    }
+    if (q.kind == RRHIP_QUADRIC_HYPERBOLOID) {
+      r = std::sqrt(std::max(q.p1[0] * q.p1[0] + q.p1[1] * q.p1[1], q.p2[0] * q.p2[0] + q.p2[1] * q.p2[1]));
+      zlo = std::min(q.p1[2], q.p2[2]);
+      zhi = std::max(q.p1[2], q.p2[2]);
+    }
+    if (q.kind == RRHIP_QUADRIC_TORUS) {
+      r = std::fabs(q.radius) + std::fabs(q.height);
+      zlo = -std::fabs(q.height);
+      zhi = std::fabs(q.height);
     }
     if (q.kind == RRHIP_QUADRIC_SPHERE) {
       zlo = std::max(zlo, -r);

@@ -89,6 +89,10 @@ struct alignas(16) PtQuadric {
   float zmax, height, thetamax, k;  // k: cone (radius / height)^2, paraboloid rmax^2 / zmax
   float inv[12];
   float fwd[12];
+  // hyperboloid: the profile x^2 + y^2 = pa z^2 + pb z + pc, the end points'
+  // z and squared radii; torus: radius = major, height = minor, zmin / zmax =
+  // phimin / phimax
+  float pa, pb, pc, z1, z2, r1, r2, pad;
 };
 
 // A placed mesh (render.rs Instance, src/geometry/mesh.rs:223-305): the

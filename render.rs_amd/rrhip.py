@@ -121,12 +121,14 @@ class MediumDesc(ctypes.Structure):
 
 # rrhip_quadric_desc kinds
 QUADRIC_SPHERE, QUADRIC_CYLINDER, QUADRIC_CONE, QUADRIC_DISK, QUADRIC_PARABOLOID = 0, 1, 2, 3, 4
+QUADRIC_HYPERBOLOID, QUADRIC_TORUS = 5, 6
 
 
 class QuadricDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_uint32), ("material", ctypes.c_uint32), ("radius", ctypes.c_double),
                 ("zmin", ctypes.c_double), ("zmax", ctypes.c_double), ("height", ctypes.c_double),
-                ("thetamax", ctypes.c_double), ("object_to_world", ctypes.c_double * 12)]
+                ("thetamax", ctypes.c_double), ("p1", ctypes.c_double * 3), ("p2", ctypes.c_double * 3),
+                ("object_to_world", ctypes.c_double * 12)]
 
 
 class SceneDesc(ctypes.Structure):

@@ -211,12 +211,14 @@ class Scene:
         return d
 
     def add_quadric(self, kind, mat, object_to_world, radius=1.0, zmin=-1.0, zmax=1.0, height=1.0,
-                    thetamax=360.0):
+                    thetamax=360.0, p1=(0, 0, 0), p2=(0, 0, 0)):
         """A RenderMan quadric (rrhip_quadric_desc): object_to_world a 3x4
         or 4x4 row-major matrix."""
         q = rrhip.QuadricDesc()
         q.kind, q.material = kind, mat
         q.radius, q.zmin, q.zmax, q.height, q.thetamax = radius, zmin, zmax, height, thetamax
+        q.p1 = (ctypes.c_double * 3)(*p1)
+        q.p2 = (ctypes.c_double * 3)(*p2)
         m = np.asarray(object_to_world, np.float64)[:3].reshape(-1)
         for k in range(12):
             q.object_to_world[k] = m[k]
@@ -451,12 +453,12 @@ def _place(t, rx_deg=0.0, scale=1.0):
 
 
 def quadric_zoo(width=96, height=54):
-    """render.rs's quadric_zoo fixture (tests/fixtures/quadric_zoo.rib) with
-    the quadrics this backend has: a sphere, a cylinder and a cone on the
-    top row, a paraboloid, a disk and a 270-degree partial sphere below,
-    plus a z-clipped sphere where the fixture has a torus and a second cone
-    where it has a hyperboloid; the fixture's colours, "plastic" / "metal"
-    surfaces, distant key light, point fill light and ground quad."""
+    """render.rs's quadric_zoo fixture (tests/fixtures/quadric_zoo.rib):
+    sphere, torus, cylinder and cone on the top row, paraboloid,
+    hyperboloid, disk and a 270-degree partial sphere below, with the
+    fixture's sizes, placements (Translate then Rotate about x), colours,
+    "plastic" / "metal" surfaces, distant key light, point fill light and
+    ground quad."""
     s = Scene(width, height, 40.0, eye=(0, 0, 0), look_at=(0, 0, 1))
     Q = rrhip
     d = np.array([0.0, 0.0, 7.0]) - np.array([4.0, 8.0, -6.0])
@@ -465,16 +467,16 @@ def quadric_zoo(width=96, height=54):
     add = s.add_quadric
     add(Q.QUADRIC_SPHERE, s.add_material(plastic((0.9, 0.25, 0.2), 0.15)), _place((-3.4, 1.2, 7)),
         radius=0.9, zmin=-0.9, zmax=0.9)
-    add(Q.QUADRIC_SPHERE, s.add_material(metal((0.95, 0.7, 0.15), 0.08)), _place((-1.1, 1.2, 7), -55),
-        radius=0.65, zmin=-0.3, zmax=0.3)
+    add(Q.QUADRIC_TORUS, s.add_material(metal((0.95, 0.7, 0.15), 0.08)), _place((-1.1, 1.2, 7), -55),
+        radius=0.65, height=0.28, zmin=0.0, zmax=360.0)
     add(Q.QUADRIC_CYLINDER, s.add_material(plastic((0.2, 0.55, 0.9))), _place((1.1, 1.2, 7), -90),
         radius=0.55, zmin=-0.85, zmax=0.85)
     add(Q.QUADRIC_CONE, s.add_material(plastic((0.25, 0.8, 0.35))), _place((3.4, 0.35, 7), -90),
         radius=0.75, height=1.7)
     add(Q.QUADRIC_PARABOLOID, s.add_material(plastic((0.7, 0.3, 0.85))), _place((-3.4, -1.6, 7), -90),
         radius=0.8, zmin=0.0, zmax=1.5)
-    add(Q.QUADRIC_CONE, s.add_material(metal((0.9, 0.45, 0.6), 0.15)), _place((-1.1, -1.1, 7), -90),
-        radius=0.6, height=1.2, thetamax=300.0)
+    add(Q.QUADRIC_HYPERBOLOID, s.add_material(metal((0.9, 0.45, 0.6), 0.15)), _place((-1.1, -1.1, 7), -90),
+        p1=(0.7, 0.0, -0.6), p2=(0.35, 0.35, 0.6))
     add(Q.QUADRIC_DISK, s.add_material(metal((0.85, 0.85, 0.9), 0.05)), _place((1.1, -1.1, 7), -65),
         radius=0.8, height=0.0)
     add(Q.QUADRIC_SPHERE, s.add_material(plastic((0.3, 0.85, 0.8))), _place((3.4, -1.1, 7)),

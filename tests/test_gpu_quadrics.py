@@ -61,3 +61,17 @@ def test_glass_quadrics_match_oracle(ctx):
     s.add_quadric(rrhip.QUADRIC_SPHERE, g, scenes._place((0.8, 0, 0)), radius=0.6, zmin=-0.6, zmax=0.3)
     p = rrhip.RenderParams.make(max_depth=8)
     parity(gpu_render(ctx, s, p, 0, 64), orc.render(s.desc(), p, 0, 64)[0], "glass")
+
+
+def test_partial_torus_and_hyperboloid_match_oracle(ctx):
+    say("half-swept torus and hyperboloid under a rect light, 64x64, 64 spp")
+    s = scenes.Scene(64, 64, 40.0, eye=(0, 2.5, -5), look_at=(0, 0, 0))
+    s.add_light(scenes.light(rrhip.LIGHT_RECT, intensity=8.0, p=(-1, 3, -1), e1=(2, 0, 0), e2=(0, 0, 2)))
+    floor = s.add_material(scenes.matte((0.6, 0.6, 0.6)))
+    s.add_polygon([(-6, -1, -6), (6, -1, -6), (6, -1, 6), (-6, -1, 6)], floor)
+    s.add_quadric(rrhip.QUADRIC_TORUS, s.add_material(scenes.plastic((0.8, 0.4, 0.1))),
+                  scenes._place((-1.2, 0, 0), -90), radius=0.8, height=0.3, zmin=-90.0, zmax=120.0, thetamax=300.0)
+    s.add_quadric(rrhip.QUADRIC_HYPERBOLOID, s.add_material(scenes.plastic((0.2, 0.5, 0.8))),
+                  scenes._place((1.2, 0, 0), -90), p1=(0.6, 0.0, -0.8), p2=(0.2, 0.5, 0.8), thetamax=250.0)
+    p = rrhip.RenderParams.make(max_depth=6)
+    parity(gpu_render(ctx, s, p, 0, 64), orc.render(s.desc(), p, 0, 64)[0], "torus+hyperboloid")

@@ -107,3 +107,41 @@ def test_transformed_quadric():
     np.testing.assert_allclose(p, [3, 2.5, 3], atol=1e-9)
     np.testing.assert_allclose(n, [1, 0, 0], atol=1e-9)
     assert hit(d, (10, 4.5, 3), (-1, 0, 0)) is None  # beyond the scaled z window (|y - 2| > 2)
+
+
+def test_hyperboloid_closed_forms():
+    # a vertical line at x = 1 sweeps a unit cylinder
+    d = one(rrhip.QUADRIC_HYPERBOLOID, p1=(1, 0, -1), p2=(1, 0, 1))
+    t, _, n, front, _ = hit(d, (5, 0, 0.5), (-1, 0, 0))
+    assert t == pytest.approx(4.0, abs=1e-12) and front
+    np.testing.assert_allclose(n, [1, 0, 0], atol=1e-12)
+    # the line (1, -1, -1) - (1, 1, 1) sweeps x^2 + y^2 = z^2 + 1
+    d = one(rrhip.QUADRIC_HYPERBOLOID, p1=(1, -1, -1), p2=(1, 1, 1))
+    t, p, n, _, _ = hit(d, (5, 0, 1), (-1, 0, 0))
+    assert t == pytest.approx(5 - np.sqrt(2.0), abs=1e-12)
+    np.testing.assert_allclose(n, np.array([np.sqrt(2.0), 0, -1]) / np.sqrt(3.0), atol=1e-12)
+    assert hit(d, (5, 0, 1.5), (-1, 0, 0)) is None  # beyond the end points' z
+    # equal z: an annulus between the end points' radii
+    d = one(rrhip.QUADRIC_HYPERBOLOID, p1=(0.5, 0, 0), p2=(1, 0, 0))
+    t, _, n, _, _ = hit(d, (0.75, 0, 3), (0, 0, -1))
+    assert t == pytest.approx(3.0) and np.allclose(n, [0, 0, 1])
+    assert hit(d, (0.25, 0, 3), (0, 0, -1)) is None
+
+
+def test_torus_closed_forms():
+    d = one(rrhip.QUADRIC_TORUS, radius=2.0, height=0.5, zmin=0.0, zmax=360.0)
+    t, _, n, front, _ = hit(d, (5, 0, 0), (-1, 0, 0))
+    assert t == pytest.approx(2.5, abs=1e-9) and front
+    np.testing.assert_allclose(n, [1, 0, 0], atol=1e-9)
+    assert hit(d, (0, 0, 5), (0, 0, -1)) is None  # through the hole
+    t, _, n, _, _ = hit(d, (2, 0, 5), (0, 0, -1))
+    assert t == pytest.approx(4.5, abs=1e-9)
+    np.testing.assert_allclose(n, [0, 0, 1], atol=1e-9)
+    # a grazing chord through the tube: the nearer of two roots
+    t, p, _, _, _ = hit(d, (5, 0.3, 0), (-1, 0, 0))
+    want = 5 - np.sqrt((2 + np.sqrt(0.25)) ** 2 - 0.09)
+    assert t == pytest.approx(want, abs=1e-9)
+    # phi window [0, 180]: only the upper half of the tube's cross-section
+    dh = one(rrhip.QUADRIC_TORUS, radius=2.0, height=0.5, zmin=0.0, zmax=180.0)
+    t, _, _, _, _ = hit(dh, (2, 0, -5), (0, 0, 1))
+    assert t == pytest.approx(5.5, abs=1e-9)  # the bottom (phi 270) is cut, the top from inside remains
