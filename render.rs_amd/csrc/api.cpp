@@ -359,6 +359,9 @@ hipError_t upload_pt(rrhip_scene* s, const PtSceneHost& h) {
   v.env_w = h.env_w;
   v.env_h = h.env_h;
   v.env_total = h.env_total;
+  v.env_pdf_k = h.env_rgb.empty() ? 0.0
+                                  : (double)h.env_w * (double)h.env_h /
+                                        (h.env_total * 2.0 * 3.14159265358979323846 * 3.14159265358979323846);
   v.num_insts = (uint32_t)h.insts.size();
   v.num_tris = (uint32_t)h.tris.size();
   v.num_spheres = (uint32_t)h.spheres.size();

@@ -239,12 +239,14 @@ int rrhip_batch_render(rrhip_batch* b, double* job_ms, double* total_ms) {
   for (size_t k = 0; k < sched.size() && e == hipSuccess; k++) {
     const std::vector<PtJob>& pass = sched[k];
     uint32_t total = 0, pix = 0, depth = 0;
+    bool quadrics = false;
     for (const PtJob& t : pass) {
+      quadrics |= t.S.num_quadrics > 0;
       total += t.prm.spp_pass * t.prm.npix;
       pix += t.prm.npix;
       depth = std::max(depth, t.prm.max_depth + (t.S.transparent ? kPtMaxSkips : 0));
     }
-    e = pt_launch_batch_pass(b->d_jobs + k * nj, (uint32_t)pass.size(), b->P, total, pix, depth, st);
+    e = pt_launch_batch_pass(b->d_jobs + k * nj, (uint32_t)pass.size(), b->P, total, pix, depth, quadrics, st);
     if (e == hipSuccess) e = hipEventRecord(ev[k + 1], st);
     for (const PtJob& t : pass) {
       const size_t j = (size_t)(std::find(b->accum.begin(), b->accum.end(), t.accum) - b->accum.begin());

@@ -302,15 +302,14 @@ __device__ __forceinline__ V3 env_rgb(const PtSceneView& S, uint32_t x, uint32_t
 }
 
 // Solid-angle pdf of sampling d (envmap.rs:134-148): the texel's share of
-// the sin-weighted luminance over the 2 pi^2 sin(theta) Jacobian.
+// the sin-weighted luminance over the 2 pi^2 sin(theta) Jacobian. The row's
+// sin(theta) cancels (it is >= sin(pi / 2^29) > 1e-9 for every row the
+// scene builder accepts, so the reference's zero test never fires), which
+// leaves lum * W * H / (2 pi^2 total) with the factor precomputed in f64.
 __device__ __forceinline__ float env_pdf(const PtSceneView& S, V3 d) {
   uint32_t x, y;
   env_texel(S, d, x, y);
-  const double sin_t = sin(3.14159265358979323846 * ((double)y + 0.5) / (double)S.env_h);
-  if (sin_t < 1e-9) return 0.f;
-  const double w = (double)lum(env_rgb(S, x, y)) * sin_t;
-  return (float)(w * (double)S.env_w * (double)S.env_h /
-                 (S.env_total * 2.0 * 3.14159265358979323846 * 3.14159265358979323846 * sin_t));
+  return (float)((double)lum(env_rgb(S, x, y)) * S.env_pdf_k);
 }
 
 // Largest i in [0, n) with c[i] <= t (the row / column the reference's
@@ -894,6 +893,9 @@ __device__ __forceinline__ bool hit_quadric(const PtQuadric& q, V3 o, V3 d, floa
 }
 
 // scene/mod.rs:105-146 over the loose objects (strictly closer wins).
+// QUAD: the scene holds quadrics (kernels without them keep the quadric
+// solvers, and their registers, out of their code).
+template <bool QUAD>
 __device__ __forceinline__ bool closest_hit(const PtSceneView& S, V3 o, V3 d, Hit& best) {
   bool found = false;
   float tmax = INFINITY;
@@ -913,7 +915,7 @@ __device__ __forceinline__ bool closest_hit(const PtSceneView& S, V3 o, V3 d, Hi
       found = true;
     }
   }
-  for (uint32_t i = 0; i < S.num_quadrics; i++) {
+  for (uint32_t i = 0; QUAD && i < S.num_quadrics; i++) {
     Hit h;
     if (hit_quadric(S.quadrics[i], o, d, tmax, h)) {
       best = h;
@@ -926,6 +928,7 @@ __device__ __forceinline__ bool closest_hit(const PtSceneView& S, V3 o, V3 d, Hi
 
 // Opaque-scene shadow test (pt/mod.rs:1243-1306): blocked when anything is
 // hit before `tmax`.
+template <bool QUAD>
 __device__ __forceinline__ bool occluded(const PtSceneView& S, V3 o, V3 d, float tmax) {
   for (uint32_t i = 0; i < S.num_tris; i++) {
     Hit h;
@@ -937,7 +940,7 @@ __device__ __forceinline__ bool occluded(const PtSceneView& S, V3 o, V3 d, float
     const float t = t0 > kObjEps ? t0 : t1;
     if (t > kObjEps && t < tmax) return true;
   }
-  for (uint32_t i = 0; i < S.num_quadrics; i++) {
+  for (uint32_t i = 0; QUAD && i < S.num_quadrics; i++) {
     Hit h;
     if (hit_quadric(S.quadrics[i], o, d, tmax, h)) return true;
   }

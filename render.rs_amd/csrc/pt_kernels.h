@@ -37,9 +37,10 @@ hipError_t pt_launch_bounce(const PtSceneView& S, const PathBuffers& P, const Pt
 // One pass of a batch: raygen over every job's slots (jobs sorted by
 // slot_begin and pix_begin), `bounces` rounds of extend / shade / shadow
 // with each path reading its own job's scene and parameters, and the
-// resolve of every job's pixels into its accumulator.
+// resolve of every job's pixels into its accumulator. `quadrics`: some
+// job's scene holds quadrics (the trace kernels then carry their solvers).
 hipError_t pt_launch_batch_pass(const PtJob* jobs, uint32_t nj, const PathBuffers& P, uint32_t total,
-                                 uint32_t total_pix, uint32_t bounces, hipStream_t st);
+                                 uint32_t total_pix, uint32_t bounces, bool quadrics, hipStream_t st);
 // Adds each pixel's pass samples (firefly-clamped) into the f64
 // accumulator in sample order (deterministic, no atomics).
 hipError_t pt_launch_resolve(const PathBuffers& P, const PtParams& prm, double* accum, double* film,

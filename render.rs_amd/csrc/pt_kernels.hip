@@ -170,13 +170,13 @@ __global__ __launch_bounds__(kBlock) void raygen_kernel(PtSceneView S, PathBuffe
 }
 
 // Closest hit of the path in `slot` into hit_n / hit_m (MOTION: at the
-// path's shutter time).
-template <bool MOTION>
+// path's shutter time; QUAD: the scene holds quadrics).
+template <bool MOTION, bool QUAD>
 __device__ __forceinline__ void extend_one(const PtSceneView& S, const PathBuffers& P, uint32_t slot, uint32_t* stk) {
   const float4 o4 = P.o[slot], d4 = P.d[slot];
   const float time = MOTION ? P.beta[slot].w : 0.f;
   Hit h;
-  if (scene_closest<kTraceBlock, MOTION>(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), time, stk, h)) {
+  if (scene_closest<kTraceBlock, MOTION, QUAD>(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), time, stk, h)) {
     P.hit_n[slot] = make_float4(h.n.x, h.n.y, h.n.z, h.t);
     P.hit_m[slot] = h.material | (h.front ? 0x80000000u : 0u);
   } else {
@@ -187,13 +187,13 @@ __device__ __forceinline__ void extend_one(const PtSceneView& S, const PathBuffe
 
 // STACK: LDS entries per lane, the smallest size >= the scene's bound
 // (fewer bytes per block, more resident waves).
-template <int STACK, bool MOTION>
+template <int STACK, bool MOTION, bool QUAD>
 __global__ __launch_bounds__(kTraceBlock) void extend_kernel(PtSceneView S, PathBuffers P, const uint32_t* __restrict__ q) {
   __shared__ uint32_t s_stack[STACK * kTraceBlock];
   uint32_t* stk = s_stack + threadIdx.x;
   const uint32_t n = P.counts[0];
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) extend_one<MOTION>(S, P, q[i], stk);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) extend_one<MOTION, QUAD>(S, P, q[i], stk);
 }
 
 // pt/mod.rs:1000-1023
@@ -786,7 +786,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 // transmittance of the medium on top of the stack (the origin's medium
 // over the atmosphere); hulls push / pop their media, cutouts let 1 -
 // presence through, anything else blocks; at most 16 blockers.
-template <bool MOTION>
+template <bool MOTION, bool QUAD>
 __device__ __forceinline__ V3 walk_visibility(const PtSceneView& S, V3 o, V3 d, float time, float remaining,
                                               int medium, Pcg32& rng, uint32_t* stk) {
   uint2 st = make_uint2(0u, 0u);
@@ -795,7 +795,7 @@ __device__ __forceinline__ V3 walk_visibility(const PtSceneView& S, V3 o, V3 d, 
   V3 vis = v3(1.f, 1.f, 1.f);
   for (uint32_t i = 0; i < kMaxSkips; i++) {
     Hit h;
-    const bool hit = scene_closest<kTraceBlock, MOTION>(S, o, d, time, stk, h);
+    const bool hit = scene_closest<kTraceBlock, MOTION, QUAD>(S, o, d, time, stk, h);
     const float seg = fminf(hit ? h.t : remaining, remaining);
     const int cur = stack_top(st, depth, S.atmosphere);
     if (cur >= 0) {
@@ -823,7 +823,7 @@ __device__ __forceinline__ V3 walk_visibility(const PtSceneView& S, V3 o, V3 d, 
 // Shadow record i: adds each unoccluded ray's contribution to its path
 // (the first ray, then the second when the record has one); with WALK the
 // rays walk their blockers (transparent scenes), else one any-hit test.
-template <bool WALK, bool MOTION>
+template <bool WALK, bool MOTION, bool QUAD>
 __device__ __forceinline__ void shadow_one_t(const PtSceneView& S, const PathBuffers& P, uint32_t i, uint32_t* stk) {
   const float4 o = P.so[i], d = P.sd[i];
   float4 c = P.sc[i];
@@ -836,23 +836,23 @@ __device__ __forceinline__ void shadow_one_t(const PtSceneView& S, const PathBuf
     const uint2 sd = P.srng[i];
     const uint64_t seed = ((uint64_t)sd.y << 32) | sd.x;
     Pcg32 r1 = Pcg32::make(seed, 1);
-    const V3 t1 = walk_visibility<MOTION>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), time, o.w, (int)d.w, r1, stk);
+    const V3 t1 = walk_visibility<MOTION, QUAD>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), time, o.w, (int)d.w, r1, stk);
     c = make_float4(c.x * t1.x, c.y * t1.y, c.z * t1.z, c.w);
     v1 = maxc(t1) > 0.f;
     if (two) {
       const float4 o2 = P.so2[i], d2 = P.sd2[i];
       Pcg32 r2 = Pcg32::make(seed, 2);
-      const V3 t2 = walk_visibility<MOTION>(S, v3(o2.x, o2.y, o2.z), v3(d2.x, d2.y, d2.z), time, o2.w, (int)d2.w, r2,
+      const V3 t2 = walk_visibility<MOTION, QUAD>(S, v3(o2.x, o2.y, o2.z), v3(d2.x, d2.y, d2.z), time, o2.w, (int)d2.w, r2,
                                     stk);
       c2 = P.sc2[i];
       c2 = make_float4(c2.x * t2.x, c2.y * t2.y, c2.z * t2.z, 0.f);
       v2 = maxc(t2) > 0.f;
     }
   } else {
-    v1 = !scene_occluded<kTraceBlock, MOTION>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), time, o.w, stk);
+    v1 = !scene_occluded<kTraceBlock, MOTION, QUAD>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), time, o.w, stk);
     if (two) {
       const float4 o2 = P.so2[i], d2 = P.sd2[i];
-      v2 = !scene_occluded<kTraceBlock, MOTION>(S, v3(o2.x, o2.y, o2.z), v3(d2.x, d2.y, d2.z), time, o2.w, stk);
+      v2 = !scene_occluded<kTraceBlock, MOTION, QUAD>(S, v3(o2.x, o2.y, o2.z), v3(d2.x, d2.y, d2.z), time, o2.w, stk);
       c2 = P.sc2[i];
     }
   }
@@ -876,46 +876,55 @@ __device__ __forceinline__ void shadow_one_t(const PtSceneView& S, const PathBuf
   if (P.ld && (cw & 2u)) add(P.ld);
 }
 
+template <bool QUAD>
 __device__ __forceinline__ void shadow_one(const PtSceneView& S, const PathBuffers& P, uint32_t i, uint32_t* stk) {
   if (S.transparent)
-    S.motion ? shadow_one_t<true, true>(S, P, i, stk) : shadow_one_t<true, false>(S, P, i, stk);
+    S.motion ? shadow_one_t<true, true, QUAD>(S, P, i, stk) : shadow_one_t<true, false, QUAD>(S, P, i, stk);
   else
-    S.motion ? shadow_one_t<false, true>(S, P, i, stk) : shadow_one_t<false, false>(S, P, i, stk);
+    S.motion ? shadow_one_t<false, true, QUAD>(S, P, i, stk) : shadow_one_t<false, false, QUAD>(S, P, i, stk);
 }
 
-template <int STACK, bool WALK, bool MOTION>
+template <int STACK, bool WALK, bool MOTION, bool QUAD>
 __global__ __launch_bounds__(kTraceBlock) void shadow_kernel(PtSceneView S, PathBuffers P) {
   __shared__ uint32_t s_stack[STACK * kTraceBlock];
   uint32_t* stk = s_stack + threadIdx.x;
   const uint32_t n = P.counts[2];
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    shadow_one_t<WALK, MOTION>(S, P, i, stk);
+    shadow_one_t<WALK, MOTION, QUAD>(S, P, i, stk);
 }
 
-// Launch of extend / shadow for the scene's stack size, transparency and motion.
-template <int STACK, bool MOTION>
+// Launch of extend / shadow for the scene's stack size, transparency,
+// motion and quadrics.
+template <int STACK, bool MOTION, bool QUAD>
 void launch_trace_kernels(const PtSceneView& S, const PathBuffers& P, const uint32_t* q, uint32_t gt, bool extend,
                           hipStream_t st) {
   if (extend)
-    hipLaunchKernelGGL((extend_kernel<STACK, MOTION>), dim3(gt), dim3(kTraceBlock), 0, st, S, P, q);
+    hipLaunchKernelGGL((extend_kernel<STACK, MOTION, QUAD>), dim3(gt), dim3(kTraceBlock), 0, st, S, P, q);
   else if (S.transparent)
-    hipLaunchKernelGGL((shadow_kernel<STACK, true, MOTION>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
+    hipLaunchKernelGGL((shadow_kernel<STACK, true, MOTION, QUAD>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
   else
-    hipLaunchKernelGGL((shadow_kernel<STACK, false, MOTION>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
+    hipLaunchKernelGGL((shadow_kernel<STACK, false, MOTION, QUAD>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
 }
 
+// Scenes with quadrics take the full-stack kernels with the quadric solvers
+// (a rare, small case: one instantiation per motion flag); every other
+// scene keeps kernels without them.
 void launch_trace(const PtSceneView& S, const PathBuffers& P, const uint32_t* q, uint32_t gt, bool extend,
                   hipStream_t st) {
   const bool small = S.stack_need <= 32;
-  if (small && S.motion)
-    launch_trace_kernels<32, true>(S, P, q, gt, extend, st);
+  if (S.num_quadrics && S.motion)
+    launch_trace_kernels<kTraceStack, true, true>(S, P, q, gt, extend, st);
+  else if (S.num_quadrics)
+    launch_trace_kernels<kTraceStack, false, true>(S, P, q, gt, extend, st);
+  else if (small && S.motion)
+    launch_trace_kernels<32, true, false>(S, P, q, gt, extend, st);
   else if (small)
-    launch_trace_kernels<32, false>(S, P, q, gt, extend, st);
+    launch_trace_kernels<32, false, false>(S, P, q, gt, extend, st);
   else if (S.motion)
-    launch_trace_kernels<kTraceStack, true>(S, P, q, gt, extend, st);
+    launch_trace_kernels<kTraceStack, true, false>(S, P, q, gt, extend, st);
   else
-    launch_trace_kernels<kTraceStack, false>(S, P, q, gt, extend, st);
+    launch_trace_kernels<kTraceStack, false, false>(S, P, q, gt, extend, st);
 }
 
 __global__ void rotate_kernel(PathBuffers P) {
@@ -1072,6 +1081,7 @@ __global__ __launch_bounds__(kBlock) void raygen_batch_kernel(const PtJob* __res
   }
 }
 
+template <bool QUAD>
 __global__ __launch_bounds__(kTraceBlock) void extend_batch_kernel(const PtJob* __restrict__ jobs, PathBuffers P,
                                                                   const uint32_t* __restrict__ q) {
   __shared__ uint32_t s_stack[kTraceStack * kTraceBlock];
@@ -1082,9 +1092,9 @@ __global__ __launch_bounds__(kTraceBlock) void extend_batch_kernel(const PtJob* 
     const uint32_t slot = q[i];
     const PtSceneView& S = jobs[P.job[slot]].S;
     if (S.motion)
-      extend_one<true>(S, P, slot, stk);
+      extend_one<true, QUAD>(S, P, slot, stk);
     else
-      extend_one<false>(S, P, slot, stk);
+      extend_one<false, QUAD>(S, P, slot, stk);
   }
 }
 
@@ -1118,13 +1128,14 @@ __global__ __launch_bounds__(kBlock) void shade_batch_kernel(const PtJob* __rest
   }
 }
 
+template <bool QUAD>
 __global__ __launch_bounds__(kTraceBlock) void shadow_batch_kernel(const PtJob* __restrict__ jobs, PathBuffers P) {
   __shared__ uint32_t s_stack[kTraceStack * kTraceBlock];
   uint32_t* stk = s_stack + threadIdx.x;
   const uint32_t n = P.counts[2];
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    shadow_one(jobs[P.job[P.sq[i]]].S, P, i, stk);
+    shadow_one<QUAD>(jobs[P.job[P.sq[i]]].S, P, i, stk);
 }
 
 __global__ __launch_bounds__(kBlock) void resolve_batch_kernel(const PtJob* __restrict__ jobs, uint32_t nj, PathBuffers P,
@@ -1251,16 +1262,22 @@ hipError_t pt_launch_bounce(const PtSceneView& S, const PathBuffers& P, const Pt
 }
 
 hipError_t pt_launch_batch_pass(const PtJob* jobs, uint32_t nj, const PathBuffers& P, uint32_t total,
-                                 uint32_t total_pix, uint32_t bounces, hipStream_t st) {
+                                 uint32_t total_pix, uint32_t bounces, bool quadrics, hipStream_t st) {
   const uint32_t g = grid_for(total);
   const uint32_t gt = grid_for(total, kTraceBlock);
   hipLaunchKernelGGL(raygen_batch_kernel, dim3(g), dim3(kBlock), 0, st, jobs, nj, P, total);
   for (uint32_t b = 0; b < bounces; b++) {
     const uint32_t* q = (b & 1) ? P.q1 : P.q0;
     uint32_t* qn = (b & 1) ? P.q0 : P.q1;
-    hipLaunchKernelGGL(extend_batch_kernel, dim3(gt), dim3(kTraceBlock), 0, st, jobs, P, q);
+    if (quadrics)
+      hipLaunchKernelGGL(extend_batch_kernel<true>, dim3(gt), dim3(kTraceBlock), 0, st, jobs, P, q);
+    else
+      hipLaunchKernelGGL(extend_batch_kernel<false>, dim3(gt), dim3(kTraceBlock), 0, st, jobs, P, q);
     hipLaunchKernelGGL(shade_batch_kernel, dim3(g), dim3(kBlock), 0, st, jobs, P, q, qn);
-    hipLaunchKernelGGL(shadow_batch_kernel, dim3(gt), dim3(kTraceBlock), 0, st, jobs, P);
+    if (quadrics)
+      hipLaunchKernelGGL(shadow_batch_kernel<true>, dim3(gt), dim3(kTraceBlock), 0, st, jobs, P);
+    else
+      hipLaunchKernelGGL(shadow_batch_kernel<false>, dim3(gt), dim3(kTraceBlock), 0, st, jobs, P);
     hipLaunchKernelGGL(rotate_kernel, dim3(1), dim3(1), 0, st, P);
   }
   hipLaunchKernelGGL(resolve_batch_kernel, dim3(grid_for(total_pix)), dim3(kBlock), 0, st, jobs, nj, P, total_pix);

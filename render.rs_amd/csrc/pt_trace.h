@@ -237,11 +237,11 @@ __device__ __forceinline__ void instance_normal(const PtSceneView& S, const Mesh
   material = I.material;
 }
 
-// Closest hit over loose objects then instances.
-template <int STRIDE, bool MOTION>
+// Closest hit over loose objects then instances (QUAD: with quadrics).
+template <int STRIDE, bool MOTION, bool QUAD>
 __device__ __forceinline__ bool scene_closest(const PtSceneView& S, V3 o, V3 d, float time, uint32_t* stk,
                                               Hit& best) {
-  bool found = closest_hit(S, o, d, best);
+  bool found = closest_hit<QUAD>(S, o, d, best);
   MeshHit mh;
   if (trace_instances<STRIDE, false, MOTION>(S, o, d, time, found ? best.t : INFINITY, stk, mh)) {
     best.t = mh.t;
@@ -251,10 +251,10 @@ __device__ __forceinline__ bool scene_closest(const PtSceneView& S, V3 o, V3 d, 
   return found;
 }
 
-template <int STRIDE, bool MOTION>
+template <int STRIDE, bool MOTION, bool QUAD>
 __device__ __forceinline__ bool scene_occluded(const PtSceneView& S, V3 o, V3 d, float time, float tmax,
                                                uint32_t* stk) {
-  if (occluded(S, o, d, tmax)) return true;
+  if (occluded<QUAD>(S, o, d, tmax)) return true;
   MeshHit mh;
   return trace_instances<STRIDE, true, MOTION>(S, o, d, time, tmax, stk, mh);
 }

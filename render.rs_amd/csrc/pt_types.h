@@ -177,12 +177,14 @@ struct PtSceneView {
   uint32_t motion;              // moving instances or camera: rays carry a shutter time
   const PtInstMotion* inst_motion;
   // Dome environment map (null: constant dome): rgb texels, row-marginal
-  // and per-row conditional CDFs of sin-weighted luminance, their total.
+  // and per-row conditional CDFs of sin-weighted luminance, their total,
+  // and W * H / (2 pi^2 total), the factor of a texel's solid-angle pdf.
   const float* env_rgb;
   const double* env_marg;
   const double* env_cond;
   uint32_t env_w, env_h;
   double env_total;
+  double env_pdf_k;
   const PtQuadric* quadrics;
   uint32_t num_quadrics;
   uint32_t num_insts;
