@@ -1,5 +1,7 @@
-"""Radiance RGBE (.hdr) reader for test fixtures: flat or new-style
-run-length encoded scanlines, "-Y h +X w" orientation (top row first)."""
+"""Radiance RGBE (.hdr) reader for environment maps (the lat-long maps of
+PxrDomeLight "lightColorMap", which render.rs opens with the image crate,
+src/scene/envmap.rs:28-35): flat or new-style run-length encoded
+scanlines, "-Y h +X w" orientation (top row first)."""
 import numpy as np
 
 

@@ -1,0 +1,841 @@
+"""RIB front end: RenderMan scene files to a scene description for the GPU
+path tracer.
+
+Two stages, as in render.rs:
+
+* `parse` (src/parser/parser.rs): the ASCII RIB token stream -> requests,
+  each a name with a list of values (a number, a string, a numeric array
+  or a string array; src/parser/ast.rs:1-45). `#` comments run to the end
+  of the line.
+* `Builder` (src/parser/scene_builder.rs): requests -> `scenes.Scene`
+  with the builder's semantics: attribute and transform stacks, the
+  legacy "matte" / "plastic" / "metal" surfaces and `Bxdf "PxrSurface"`
+  parameters (src/scene/pbr.rs:100-185), polygons fan-triangulated into
+  loose triangles (an area-light quad also becomes a rect light), the
+  RenderMan quadrics as transformed quadric objects, PointsPolygons meshes
+  as instances, ObjectBegin / ObjectInstance sharing meshes, classic and
+  Pxr lights, Interior / Atmosphere media, the camera options and the id
+  AOV's object ids.
+
+Requests the GPU path has no counterpart for (subdivision surfaces,
+curves, patches, displacement, pattern graphs, hair, procedural density,
+motion blur of RIB motion blocks) raise `RibUnsupported` by default; with
+`strict=False` they are skipped with one warning each, as render.rs skips
+requests it does not implement (a motion block then keeps its first
+sample, as render.rs does for requests other than transforms). Inline
+archives (ArchiveBegin / ReadArchive) and file archives are replayed.
+Requests that carry no render effect (Display, Declare, ShadingRate,
+Sides, ...) are accepted silently.
+"""
+import math
+import os
+import warnings
+
+import numpy as np
+
+import rgbe
+import rrhip
+import scenes
+
+
+class RibError(ValueError):
+    """Malformed RIB text."""
+
+
+class RibUnsupported(RibError):
+    """A request whose semantics this backend does not implement."""
+
+
+# ---- tokenizer ---------------------------------------------------------------------
+
+_NUM_START = set("0123456789+-.")
+
+
+def _number(text, i):
+    j = i
+    n = len(text)
+    if j < n and text[j] in "+-":
+        j += 1
+    while j < n and (text[j].isdigit() or text[j] == "."):
+        j += 1
+    if j < n and text[j] in "eE":
+        k = j + 1
+        if k < n and text[k] in "+-":
+            k += 1
+        if k < n and text[k].isdigit():
+            j = k
+            while j < n and text[j].isdigit():
+                j += 1
+    try:
+        return float(text[i:j]), j
+    except ValueError:
+        raise RibError(f"bad number {text[i:j]!r} at offset {i}") from None
+
+
+def _string(text, i):
+    """A double-quoted string starting at text[i] == '"' (backslash escapes)."""
+    out = []
+    j = i + 1
+    n = len(text)
+    while j < n and text[j] != '"':
+        if text[j] == "\\" and j + 1 < n:
+            j += 1
+            out.append({"n": "\n", "t": "\t"}.get(text[j], text[j]))
+        else:
+            out.append(text[j])
+        j += 1
+    if j >= n:
+        raise RibError(f"unterminated string at offset {i}")
+    return "".join(out), j + 1
+
+
+def parse(text):
+    """RIB text -> [(name, [values])]: a value is a float, a str, a tuple of
+    floats (numeric array) or a list of str (string array)."""
+    reqs = []
+    i, n = 0, len(text)
+    cur = None
+    while i < n:
+        c = text[i]
+        if c.isspace():
+            i += 1
+        elif c == "#":
+            e = text.find("\n", i)
+            i = n if e < 0 else e + 1
+        elif c == '"':
+            s, i = _string(text, i)
+            if cur is None:
+                raise RibError(f"value before any request at offset {i}")
+            cur[1].append(s)
+        elif c == "[":
+            i += 1
+            nums, strs = [], []
+            while True:
+                while i < n and (text[i].isspace() or text[i] == "#"):
+                    if text[i] == "#":
+                        e = text.find("\n", i)
+                        i = n if e < 0 else e
+                    i += 1
+                if i >= n:
+                    raise RibError("unterminated array")
+                if text[i] == "]":
+                    i += 1
+                    break
+                if text[i] == '"':
+                    s, i = _string(text, i)
+                    strs.append(s)
+                elif text[i] in _NUM_START:
+                    v, i = _number(text, i)
+                    nums.append(v)
+                else:
+                    raise RibError(f"unexpected {text[i]!r} in array at offset {i}")
+            if nums and strs:
+                raise RibError("array mixes numbers and strings")
+            if cur is None:
+                raise RibError("array before any request")
+            cur[1].append(list(strs) if strs else tuple(nums))
+        elif c in _NUM_START:
+            v, i = _number(text, i)
+            if cur is None:
+                raise RibError("number before any request")
+            cur[1].append(v)
+        elif c.isalpha() or c == "_":
+            j = i
+            while j < n and (text[j].isalnum() or text[j] == "_"):
+                j += 1
+            cur = (text[i:j], [])
+            reqs.append(cur)
+            i = j
+        else:
+            raise RibError(f"unexpected character {c!r} at offset {i}")
+    return reqs
+
+
+# ---- value access (ast.rs RibValue / RibRequest / ParamList) ----------------------------
+
+def _as_number(v):
+    if isinstance(v, float):
+        return v
+    if isinstance(v, tuple) and len(v) == 1:
+        return v[0]
+    return None
+
+
+def _as_str(v):
+    if isinstance(v, str):
+        return v
+    if isinstance(v, list) and len(v) == 1:
+        return v[0]
+    return None
+
+
+def _as_numbers(v):
+    if isinstance(v, tuple):
+        return v
+    if isinstance(v, float):
+        return (v,)
+    return None
+
+
+def _num(vals, i):
+    return _as_number(vals[i]) if i < len(vals) else None
+
+
+def _str(vals, i):
+    return _as_str(vals[i]) if i < len(vals) else None
+
+
+class _Params:
+    """Token / value pairs from `start`; a token matches a name by its last
+    word, so inline declarations ("float intensity") resolve."""
+
+    def __init__(self, vals, start):
+        self.items = []
+        i = start
+        while i + 1 < len(vals):
+            tok = _as_str(vals[i])
+            if tok is None:
+                break
+            self.items.append((tok, vals[i + 1]))
+            i += 2
+
+    def get(self, name):
+        for tok, v in self.items:
+            w = tok.split()
+            if w and w[-1] == name:
+                return v
+        return None
+
+    def number(self, name, default=None):
+        v = self.get(name)
+        r = None if v is None else _as_number(v)
+        return default if r is None else r
+
+    def numbers(self, name):
+        v = self.get(name)
+        return None if v is None else _as_numbers(v)
+
+    def string(self, name):
+        v = self.get(name)
+        return None if v is None else _as_str(v)
+
+    def color(self, name, default):
+        v = self.numbers(name)
+        return tuple(v[:3]) if v is not None and len(v) >= 3 else default
+
+
+def _pstart(vals):
+    """Light / Bxdf / AreaLightSource parameters start after the type and
+    an optional handle: the handle is there when the value count is even."""
+    return 2 if len(vals) % 2 == 0 else 1
+
+
+# ---- transforms (math/matrix.rs: column-vector 4x4) ---------------------------------
+
+def _translate(x, y, z):
+    m = np.eye(4)
+    m[:3, 3] = (x, y, z)
+    return m
+
+
+def _scale(x, y, z):
+    return np.diag([x, y, z, 1.0])
+
+
+def _rotate(deg, x, y, z):
+    """Rotation by `deg` degrees about axis (x, y, z) (matrix.rs:81-113)."""
+    a = np.array([x, y, z], float)
+    a /= np.linalg.norm(a)
+    x, y, z = a
+    r = math.radians(deg)
+    c, s = math.cos(r), math.sin(r)
+    t = 1.0 - c
+    m = np.eye(4)
+    m[:3, :3] = [[c + x * x * t, x * y * t - z * s, x * z * t + y * s],
+                 [y * x * t + z * s, c + y * y * t, y * z * t - x * s],
+                 [z * x * t - y * s, z * y * t + x * s, c + z * z * t]]
+    return m
+
+
+def _from_rib(m):
+    """RIB's 16 row-major row-vector numbers -> the column-vector matrix
+    (scene_builder.rs:1925-1938: a transpose)."""
+    if len(m) != 16:
+        return None
+    return np.array(m, float).reshape(4, 4).T
+
+
+def _pt(m, p):
+    return tuple(float(x) for x in m[:3, :3] @ np.asarray(p, float) + m[:3, 3])
+
+
+def _vec(m, v):
+    return m[:3, :3] @ np.asarray(v, float)
+
+
+# ---- scene builder -------------------------------------------------------------------
+
+_FILTERS = {"box": 0, "triangle": 1, "gaussian": 2}
+
+# Requests with no effect on the rendered image (recorded or ignored by
+# render.rs too: scene_builder.rs:271, 310, 541-575, 662-678).
+_NO_EFFECT = {"version", "WorldEnd", "FrameBegin", "FrameEnd", "Display", "Declare", "Option",
+              "Clipping", "ClippingPlane", "CropWindow", "FrameAspectRatio", "PixelVariance",
+              "Exposure", "Quantize", "Hider", "Integrator", "TextureCoordinates",
+              "ShadingInterpolation", "RelativeDetail", "Opacity", "Orientation",
+              "ReverseOrientation", "Sides", "ShadingRate", "Shutter", "Exterior",
+              "CoordinateSystem", "ScopedCoordinateSystem", "IfBegin", "ElseIf", "Else", "IfEnd",
+              "Basis", "Attribute"}
+
+# Requests render.rs implements that this backend does not.
+_UNSUPPORTED = {"SubdivisionMesh", "HierarchicalSubdivisionMesh", "Curves", "Points", "PatchMesh",
+                "Patch", "NuPatch", "GeneralPolygon", "PointsGeneralPolygons", "Displace", "Pattern",
+                "Procedural"}
+
+_QUADRIC_ARGS = {"Sphere": 4, "Cylinder": 4, "Cone": 3, "Torus": 5, "Disk": 3, "Paraboloid": 4,
+                 "Hyperboloid": 7}
+
+
+class _State:
+    """GraphicsState defaults (scene_builder.rs:55-80)."""
+
+    def __init__(self):
+        self.color = (1.0, 1.0, 1.0)
+        self.surface = "matte"
+        self.roughness = 0.1
+        self.area_light = None  # (intensity, color)
+        self.bxdf = None        # PxrSurface parameters as keyword arguments of scenes.material
+        self.interior = None
+        self.identifier = None
+
+    def copy(self):
+        s = _State()
+        s.__dict__.update(self.__dict__)
+        return s
+
+
+def _pxr_surface(p):
+    """PbrParams::from_bxdf_params (pbr.rs:133-185) as scenes.material
+    keyword arguments."""
+    kw = dict(diffuse_gain=p.number("diffuseGain", 1.0),
+              diffuse_color=p.color("diffuseColor", (0.18, 0.18, 0.18)),
+              diffuse_roughness=p.number("diffuseRoughness", 0.0),
+              specular_face_color=p.color("specularFaceColor", (0.0, 0.0, 0.0)),
+              specular_edge_color=p.color("specularEdgeColor", (1.0, 1.0, 1.0)),
+              specular_roughness=p.number("specularRoughness", 0.2),
+              specular_ior=p.number("specularIor", 1.5),
+              clearcoat_gain=p.number("clearcoatGain", 0.0),
+              clearcoat_roughness=p.number("clearcoatRoughness", 0.1),
+              fuzz_gain=p.number("fuzzGain", 0.0),
+              fuzz_color=p.color("fuzzColor", (1.0, 1.0, 1.0)),
+              glass_gain=p.number("glassRefractionGain", 0.0),
+              glass_ior=p.number("glassIor", 1.5),
+              glass_roughness=p.number("glassRoughness", 0.05),
+              refraction_color=p.color("refractionColor", (1.0, 1.0, 1.0)),
+              presence=p.number("presence", 1.0))
+    gg = p.number("glowGain", 0.0)
+    gc = p.color("glowColor", (1.0, 1.0, 1.0))
+    kw["glow"] = tuple(gg * c for c in gc)
+    for k in ("specular_roughness", "glass_roughness", "clearcoat_roughness"):
+        kw[k] = min(max(kw[k], 0.005), 1.0)
+    return kw
+
+
+class Builder:
+    """SceneBuilder (scene_builder.rs) over scenes.Scene. After `run`,
+    `scene()` returns the scene and `spp` the PixelSamples product."""
+
+    def __init__(self, base_dir=None, strict=True):
+        self.base_dir = base_dir
+        self.strict = strict
+        self.width, self.height, self.fov = 640, 480, 45.0
+        self.projection = 0
+        self.dof = None
+        self.filter = (0, 1.0)
+        self.screen_window = None
+        self.pixel_samples = (1, 1)
+        self.background = (0.0, 0.0, 0.0)
+        self.state = _State()
+        self.attr_stack = []
+        self.xf_stack = [np.eye(4)]
+        self.object_defs = {}
+        self.defining = None  # (handle, [(mesh, local transform, material)])
+        self.object_ids = {}
+        self.next_object_id = 1
+        self.warned = set()
+        # scene data, collected until the camera is known
+        self.materials, self.lights, self.tris, self.quadrics = [], [], [], []
+        self.meshes, self.instances, self.media = [], [], []
+        self.atmosphere = -1
+        self.env = None
+        self.archives = {}
+        self.recording = None  # (name, [requests]) between ArchiveBegin and ArchiveEnd
+        self.motion = None     # requests of an open MotionBegin block
+        self.archive_depth = 0
+
+    # -- helpers
+    def _skip(self, what):
+        if self.strict:
+            raise RibUnsupported(what)
+        if what not in self.warned:
+            self.warned.add(what)
+            warnings.warn(f"RIB: {what}; skipping")
+
+    @property
+    def xf(self):
+        return self.xf_stack[-1]
+
+    def _apply(self, m):
+        self.xf_stack[-1] = self.xf_stack[-1] @ m
+
+    def _object_id(self):
+        """assign_object_id (scene_builder.rs:1888-1903)."""
+        name = self.state.identifier or f"object_{self.next_object_id}"
+        if name in self.object_ids:
+            return self.object_ids[name]
+        oid = self.next_object_id
+        self.next_object_id += 1
+        self.object_ids[name] = oid
+        return oid
+
+    def _material(self):
+        """make_material (scene_builder.rs:1850-1879) -> a new material index."""
+        st = self.state
+        if st.bxdf is not None:
+            m = scenes.material(**st.bxdf)
+        elif st.surface == "plastic":
+            m = scenes.plastic(st.color, st.roughness)
+        elif st.surface == "metal":
+            m = scenes.metal(st.color, st.roughness)
+        else:
+            m = scenes.matte(st.color)
+        m.interior = 0 if st.interior is None else st.interior + 1
+        m.id = self._object_id()
+        self.materials.append(m)
+        return len(self.materials) - 1
+
+    def _emitter(self, color, scale, light_index):
+        """The black matte stand-in of a Pxr area light (scene_builder.rs:1501-1507)."""
+        m = scenes.matte((0.0, 0.0, 0.0), emission=tuple(c * scale for c in color), area_light=light_index)
+        self.materials.append(m)
+        return len(self.materials) - 1
+
+    def _quadric(self, kind, mat, m, **kw):
+        self.quadrics.append((kind, mat, m[:3, :4].copy(), kw))
+
+    # -- requests
+    def run(self, reqs):
+        """scene_builder.rs:246-266: archive recording and motion blocks
+        capture requests verbatim before they are processed."""
+        for name, vals in reqs:
+            if self.recording is not None:
+                if name == "ArchiveEnd":
+                    self.archives[self.recording[0]] = self.recording[1]
+                    self.recording = None
+                else:
+                    self.recording[1].append((name, vals))
+                continue
+            if self.motion is not None and name != "MotionEnd":
+                self.motion.append((name, vals))
+                continue
+            self.process(name, vals)
+        return self
+
+    def _read_archive(self, name):
+        if self.archive_depth >= 16:
+            raise RibError(f'ReadArchive "{name}": nesting deeper than 16')
+        reqs = self.archives.get(name)
+        if reqs is None:
+            path = name if self.base_dir is None else os.path.join(self.base_dir, name)
+            if not os.path.exists(path):
+                return self._skip(f'ReadArchive "{name}" not found')
+            with open(path, "r", encoding="utf-8", errors="replace") as f:
+                reqs = parse(f.read())
+        self.archive_depth += 1
+        try:
+            self.run(reqs)
+        finally:
+            self.archive_depth -= 1
+
+    def process(self, name, vals):
+        st = self.state
+        if name == "WorldBegin" or name in _NO_EFFECT and name not in ("Option", "Attribute"):
+            return
+        if name in _UNSUPPORTED:
+            return self._skip(f"request {name} is not supported by this backend")
+        if name == "ArchiveBegin":
+            self.recording = (_str(vals, 0) or "", [])
+        elif name == "ArchiveEnd":
+            pass  # a stray end
+        elif name == "ReadArchive":
+            if _str(vals, 0) is not None:
+                self._read_archive(_str(vals, 0))
+        elif name == "MotionBegin":
+            self.motion = []
+        elif name == "MotionEnd":
+            block, self.motion = self.motion or [], None
+            if block:
+                self._skip("motion blur of RIB motion blocks is not supported by this backend")
+                self.process(*block[0])
+        elif name == "AttributeBegin":
+            self.attr_stack.append(st.copy())
+            self.xf_stack.append(self.xf.copy())
+        elif name == "AttributeEnd":
+            if self.attr_stack:
+                self.state = self.attr_stack.pop()
+            if len(self.xf_stack) > 1:
+                self.xf_stack.pop()
+        elif name == "TransformBegin":
+            self.xf_stack.append(self.xf.copy())
+        elif name == "TransformEnd":
+            if len(self.xf_stack) > 1:
+                self.xf_stack.pop()
+        elif name == "ObjectBegin":
+            handle = _str(vals, 0) or (str(_num(vals, 0)) if _num(vals, 0) is not None else "0")
+            self.attr_stack.append(st.copy())
+            self.xf_stack.append(np.eye(4))
+            self.defining = (handle, [])
+        elif name == "ObjectEnd":
+            if self.defining is not None:
+                self.object_defs[self.defining[0]] = self.defining[1]
+                self.defining = None
+            if self.attr_stack:
+                self.state = self.attr_stack.pop()
+            if len(self.xf_stack) > 1:
+                self.xf_stack.pop()
+        elif name == "ObjectInstance":
+            handle = _str(vals, 0) or (str(_num(vals, 0)) if _num(vals, 0) is not None else "0")
+            entries = self.object_defs.get(handle)
+            if entries is None:
+                self._skip(f'ObjectInstance "{handle}" has no definition')
+                return
+            for mesh, local, mat in entries:
+                self.instances.append((mesh, mat, (self.xf @ local)[:3, :4].copy()))
+        elif name == "Format":
+            if _num(vals, 0) is not None and _num(vals, 1) is not None:
+                self.width, self.height = int(_num(vals, 0)), int(_num(vals, 1))
+        elif name == "Projection":
+            kind = _str(vals, 0)
+            if kind == "perspective":
+                self.projection = 0
+                self.fov = _Params(vals, 1).number("fov", self.fov)
+            elif kind == "orthographic":
+                self.projection = 1
+            else:
+                self._skip(f'Projection "{kind}" is not supported')
+        elif name == "DepthOfField":
+            f = [_num(vals, i) for i in range(3)]
+            if all(x is not None for x in f):
+                self.dof = tuple(f)
+        elif name == "PixelFilter":
+            fname = _str(vals, 0)
+            if fname is not None:
+                xw = _num(vals, 1) if _num(vals, 1) is not None else 2.0
+                yw = _num(vals, 2) if _num(vals, 2) is not None else xw
+                w = (xw + yw) * 0.5
+                self.filter = (_FILTERS[fname], max(w, 1e-3)) if fname in _FILTERS else (0, 1.0)
+        elif name == "ScreenWindow":
+            f = [_num(vals, i) for i in range(4)]
+            if all(x is not None for x in f):
+                self.screen_window = tuple(f)
+        elif name == "PixelSamples":
+            if _num(vals, 0) is not None and _num(vals, 1) is not None:
+                self.pixel_samples = (max(int(_num(vals, 0)), 1), max(int(_num(vals, 1)), 1))
+        elif name == "Option":
+            if _str(vals, 0) == "background":
+                c = _Params(vals, 1).numbers("color")
+                if c is not None and len(c) >= 3:
+                    self.background = tuple(c[:3])
+        elif name == "Attribute":
+            if _str(vals, 0) == "identifier":
+                n = _Params(vals, 1).string("name")
+                if n is not None:
+                    st.identifier = n
+        elif name == "Identity":
+            self.xf_stack[-1] = np.eye(4)
+        elif name in ("Transform", "ConcatTransform"):
+            m = _as_numbers(vals[0]) if vals else None
+            m = None if m is None else _from_rib(m)
+            if m is not None:
+                if name == "Transform":
+                    self.xf_stack[-1] = m
+                else:
+                    self._apply(m)
+        elif name == "Translate":
+            f = [_num(vals, i) for i in range(3)]
+            if all(x is not None for x in f):
+                self._apply(_translate(*f))
+        elif name == "Rotate":
+            f = [_num(vals, i) for i in range(4)]
+            if all(x is not None for x in f):
+                self._apply(_rotate(*f))
+        elif name == "Scale":
+            f = [_num(vals, i) for i in range(3)]
+            if all(x is not None for x in f):
+                self._apply(_scale(*f))
+        elif name == "Color":
+            f = [_num(vals, i) for i in range(3)]
+            if all(x is not None for x in f):
+                st.color = tuple(f)
+            elif vals and _as_numbers(vals[0]) is not None and len(_as_numbers(vals[0])) >= 3:
+                st.color = tuple(_as_numbers(vals[0])[:3])
+        elif name == "Surface":
+            s = _str(vals, 0)
+            if s is not None:
+                st.surface = s
+                st.roughness = _Params(vals, 1).number("roughness", 0.1)
+        elif name == "Bxdf":
+            kind = _str(vals, 0) or ""
+            if kind == "PxrMarschnerHair":
+                return self._skip("Bxdf PxrMarschnerHair is not supported by this backend")
+            p = _Params(vals, _pstart(vals))
+            if kind != "PxrSurface":
+                self._skip(f'Bxdf "{kind}" treated as PxrSurface')
+            if any("reference" in tok.split() for tok, _ in p.items):
+                self._skip("Bxdf pattern references are not supported by this backend")
+            if p.number("subsurfaceGain", 0.0) > 0.0:
+                self._skip("PxrSurface subsurface is not supported by this backend")
+            st.bxdf = _pxr_surface(p)
+        elif name == "AreaLightSource":
+            p = _Params(vals, _pstart(vals))
+            st.area_light = (p.number("intensity", 1.0), p.color("lightcolor", (1.0, 1.0, 1.0)))
+        elif name == "LightSource":
+            self._light_source(vals)
+        elif name == "Light":
+            self._pxr_light(vals)
+        elif name in ("Interior", "Atmosphere"):
+            self._medium(name, vals)
+        elif name in _QUADRIC_ARGS:
+            self._quadric_request(name, vals)
+        elif name == "Polygon":
+            self._polygon(vals)
+        elif name == "PointsPolygons":
+            self._points_polygons(vals)
+        else:
+            self._skip(f"request {name} is not implemented")
+
+    def _light_source(self, vals):
+        """parse_light (scene_builder.rs:1582-1635): positions are taken as
+        given (not through the current transform), colour white."""
+        kind = _str(vals, 0)
+        p = _Params(vals, _pstart(vals))
+        inten = p.number("intensity", 1.0)
+        if kind == "pointlight":
+            if p.string("iesProfile") is not None:
+                return self._skip("IES profiles are not supported by this backend")
+            f = p.numbers("from")
+            pos = tuple(f[:3]) if f is not None and len(f) >= 3 else (0.0, 0.0, 0.0)
+            self.lights.append(scenes.light(rrhip.LIGHT_POINT, intensity=inten, p=pos))
+        elif kind == "distantlight":
+            f = p.numbers("from")
+            t = p.numbers("to")
+            a = np.array(f[:3] if f is not None and len(f) >= 3 else (0.0, 0.0, -10.0))
+            b = np.array(t[:3] if t is not None and len(t) >= 3 else (0.0, 0.0, 0.0))
+            d = b - a
+            self.lights.append(scenes.light(rrhip.LIGHT_DISTANT, intensity=inten,
+                                            p=tuple(d / np.linalg.norm(d))))
+        # other classic shaders: no light (render.rs returns None)
+
+    def _pxr_light(self, vals):
+        """build_light (scene_builder.rs:1488-1580)."""
+        kind = _str(vals, 0)
+        if kind is None:
+            return
+        p = _Params(vals, _pstart(vals))
+        scale = p.number("intensity", 1.0) * 2.0 ** p.number("exposure", 0.0)
+        color = p.color("lightColor", (1.0, 1.0, 1.0))
+        xf = self.xf
+        origin = _pt(xf, (0, 0, 0))
+        li = len(self.lights)
+        if kind == "PxrRectLight":
+            corner = _pt(xf, (-0.5, -0.5, 0.0))
+            self.lights.append(scenes.light(rrhip.LIGHT_RECT, intensity=scale, color=color, p=corner,
+                                            e1=tuple(_vec(xf, (1, 0, 0))), e2=tuple(_vec(xf, (0, 1, 0)))))
+            mat = self._emitter(color, scale, li)
+            v = [_pt(xf, (x, y, 0.0)) for x, y in ((-0.5, -0.5), (0.5, -0.5), (0.5, 0.5), (-0.5, 0.5))]
+            self.tris.append((v[0], v[1], v[2], mat))
+            self.tris.append((v[0], v[2], v[3], mat))
+        elif kind == "PxrSphereLight":
+            radius = float(np.linalg.norm(_vec(xf, (0.5, 0, 0))))
+            self.lights.append(scenes.light(rrhip.LIGHT_SPHERE, intensity=scale, color=color, p=origin,
+                                            radius=radius))
+            self._quadric(rrhip.QUADRIC_SPHERE, self._emitter(color, scale, li), xf, radius=0.5, zmin=-0.5,
+                          zmax=0.5, thetamax=360.0)
+        elif kind == "PxrDiskLight":
+            self.lights.append(scenes.light(rrhip.LIGHT_DISK, intensity=scale, color=color, p=origin,
+                                            e1=tuple(_vec(xf, (0.5, 0, 0))), e2=tuple(_vec(xf, (0, 0.5, 0)))))
+            self._quadric(rrhip.QUADRIC_DISK, self._emitter(color, scale, li), xf, radius=0.5, height=0.0,
+                          thetamax=360.0)
+        elif kind == "PxrDistantLight":
+            d = _vec(xf, (0, 0, -1))
+            ang = p.number("angleExtent", 0.53)
+            self.lights.append(scenes.light(rrhip.LIGHT_DISTANT, intensity=scale, color=color,
+                                            p=tuple(d / np.linalg.norm(d)), radius=math.radians(ang * 0.5)))
+        elif kind == "PxrDomeLight":
+            mp = p.string("lightColorMap")
+            if mp is not None:
+                if self.env is not None:
+                    return self._skip("a second environment map")
+                path = mp if self.base_dir is None else os.path.join(self.base_dir, mp)
+                self.env = rgbe.read_hdr(path)
+            self.lights.append(scenes.light(rrhip.LIGHT_DOME, intensity=scale, color=color))
+        else:
+            self._skip(f'Light "{kind}" is not implemented')
+
+    def _medium(self, name, vals):
+        """scene_builder.rs:328-373: homogeneous media (the procedural fBm
+        density field is not supported by this backend)."""
+        p = _Params(vals, 1)
+        if p.string("density") is not None:
+            return self._skip(f'{name} density "{p.string("density")}" is not supported by this backend')
+        far = 1200.0 if name == "Atmosphere" else 0.0  # 0: unbounded
+        idx = len(self.media)
+        self.media.append(dict(sigma_a=p.color("sigma_a", (0.1, 0.1, 0.1)),
+                               sigma_s=p.color("sigma_s", (0.5, 0.5, 0.5)), g=p.number("g", 0.0),
+                               emission=p.color("emission", (0.0, 0.0, 0.0)),
+                               max_distance=p.number("maxdistance", far)))
+        if name == "Atmosphere":
+            self.atmosphere = idx
+        else:
+            self.state.interior = idx
+
+    def _quadric_request(self, name, vals):
+        v = [_num(vals, i) for i in range(_QUADRIC_ARGS[name])]
+        if any(x is None for x in v):
+            return
+        if self.defining is not None:
+            return self._skip(f"{name} inside ObjectBegin is not instanced")
+        mat = self._material()
+        Q = rrhip
+        if name == "Sphere":
+            self._quadric(Q.QUADRIC_SPHERE, mat, self.xf, radius=v[0], zmin=v[1], zmax=v[2], thetamax=v[3])
+        elif name == "Cylinder":
+            self._quadric(Q.QUADRIC_CYLINDER, mat, self.xf, radius=v[0], zmin=v[1], zmax=v[2], thetamax=v[3])
+        elif name == "Cone":
+            self._quadric(Q.QUADRIC_CONE, mat, self.xf, height=v[0], radius=v[1], thetamax=v[2])
+        elif name == "Torus":
+            self._quadric(Q.QUADRIC_TORUS, mat, self.xf, radius=v[0], height=v[1], zmin=v[2], zmax=v[3],
+                          thetamax=v[4])
+        elif name == "Disk":
+            self._quadric(Q.QUADRIC_DISK, mat, self.xf, height=v[0], radius=v[1], thetamax=v[2])
+        elif name == "Paraboloid":
+            self._quadric(Q.QUADRIC_PARABOLOID, mat, self.xf, radius=v[0], zmin=v[1], zmax=v[2], thetamax=v[3])
+        else:
+            self._quadric(Q.QUADRIC_HYPERBOLOID, mat, self.xf, p1=tuple(v[0:3]), p2=tuple(v[3:6]), thetamax=v[6])
+
+    def _polygon(self, vals):
+        """scene_builder.rs:911-949: fan triangles through the current
+        transform; under an AreaLightSource the material emits and a quad
+        (4 vertices) also becomes a rect light."""
+        pts = _Params(vals, 0).numbers("P")
+        if pts is None or len(pts) < 9:
+            return
+        if self.defining is not None:
+            return self._skip("Polygon inside ObjectBegin is not instanced")
+        verts = [_pt(self.xf, pts[3 * i:3 * i + 3]) for i in range(len(pts) // 3)]
+        mat = self._material()
+        al = self.state.area_light
+        if al is not None:
+            inten, col = al
+            m = self.materials[mat]
+            m.emission = scenes._d3(tuple(c * inten for c in col))
+            if len(pts) == 12:
+                c = np.array(verts[0])
+                m.area_light = len(self.lights)
+                self.lights.append(scenes.light(rrhip.LIGHT_RECT, intensity=inten, color=col, p=verts[0],
+                                                e1=tuple(np.array(verts[1]) - c), e2=tuple(np.array(verts[3]) - c)))
+        for i in range(1, len(verts) - 1):
+            self.tris.append((verts[0], verts[i], verts[i + 1], mat))
+
+    def _points_polygons(self, vals):
+        """points_polygons + add_tessellated_mesh (scene_builder.rs:982-1051,
+        1195-1233): fan-triangulated faces, per-vertex normals when "N" has
+        one per point; an instance of the mesh at the current transform, or
+        an entry of the object being defined."""
+        nverts = _as_numbers(vals[0]) if len(vals) > 0 else None
+        idx = _as_numbers(vals[1]) if len(vals) > 1 else None
+        if nverts is None or idx is None:
+            return
+        p = _Params(vals, 2)
+        P = p.numbers("P")
+        if P is None:
+            return self._skip('PointsPolygons without "P"')
+        N = p.numbers("N")
+        if N is not None and len(N) != len(P):
+            N = None
+        tri = []
+        cur = 0
+        for nv in nverts:
+            nv = int(nv)
+            if cur + nv > len(idx):
+                break
+            poly = idx[cur:cur + nv]
+            for i in range(1, nv - 1):
+                tri += [poly[0], poly[i], poly[i + 1]]
+            cur += nv
+        if not tri:
+            return
+        mesh = len(self.meshes)
+        self.meshes.append((np.array(P, np.float32).reshape(-1, 3), np.array(tri, np.uint32).reshape(-1, 3),
+                            None if N is None else np.array(N, np.float32).reshape(-1, 3)))
+        mat = self._material()
+        if self.state.area_light is not None:
+            inten, col = self.state.area_light
+            self.materials[mat].emission = scenes._d3(tuple(c * inten for c in col))
+        if self.defining is not None:
+            self.defining[1].append((mesh, self.xf.copy(), mat))
+        else:
+            self.instances.append((mesh, mat, self.xf[:3, :4].copy()))
+
+    # -- result
+    @property
+    def spp(self):
+        return self.pixel_samples[0] * self.pixel_samples[1]
+
+    def scene(self):
+        """The scene (SceneBuilder::build, scene_builder.rs:208-244): the
+        camera at the origin of the baked frame looking down +z."""
+        s = scenes.Scene(self.width, self.height, self.fov, eye=(0, 0, 0), look_at=(0, 0, 1), up=(0, 1, 0),
+                         filter=self.filter[0], filter_width=self.filter[1], background=self.background)
+        s.camera.projection = self.projection
+        if self.dof is not None:
+            fstop, fl, fd = self.dof
+            if math.isfinite(fstop) and fstop > 0.0:
+                s.camera.lens_radius = fl / (2.0 * fstop)
+                s.camera.focal_distance = fd
+        if self.screen_window is not None:
+            l, r, b, t = self.screen_window
+            s.camera.ortho_half[0] = abs(r - l) * 0.5
+            s.camera.ortho_half[1] = abs(t - b) * 0.5
+        s.materials = list(self.materials)
+        s.lights = list(self.lights)
+        for v0, v1, v2, mat in self.tris:
+            s.add_triangle(v0, v1, v2, mat)
+        for kind, mat, m, kw in self.quadrics:
+            s.add_quadric(kind, mat, m, **kw)
+        for pos, idx, nrm in self.meshes:
+            s.add_mesh(pos, idx, nrm)
+        for mesh, mat, m in self.instances:
+            s.add_instance(mesh, mat, m)
+        for md in self.media:
+            s.add_medium(**md)
+        s.atmosphere = self.atmosphere
+        if self.env is not None:
+            s.set_env(self.env)
+        return s
+
+
+def load(path, strict=True):
+    """Parses and builds a RIB file; returns (scene, spp) with spp the
+    PixelSamples product. Relative resource paths (environment maps)
+    resolve against the file's directory."""
+    with open(path, "r", encoding="utf-8", errors="replace") as f:
+        text = f.read()
+    b = Builder(base_dir=os.path.dirname(os.path.abspath(path)), strict=strict).run(parse(text))
+    return b.scene(), b.spp
+
+
+def loads(text, base_dir=None, strict=True):
+    """`load` for RIB text."""
+    b = Builder(base_dir=base_dir, strict=strict).run(parse(text))
+    return b.scene(), b.spp

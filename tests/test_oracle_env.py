@@ -14,7 +14,6 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
-sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 
 import orc  # noqa: E402
 import rgbe  # noqa: E402

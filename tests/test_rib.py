@@ -1,0 +1,315 @@
+"""The RIB front end (render.rs_amd/rib.py) against render.rs's parser and
+scene-builder semantics:
+
+* the scene builder's own unit tests (src/parser/scene_builder.rs:
+  2026-2200) restated on the built scene description;
+* the reference's RIB fixtures (tests/golden/rib, copied unchanged) built
+  into the same scenes as the hand-written restatements the GPU parity
+  tests already use (cornell.rib, quadric_zoo.rib): equal lights, equal
+  quadrics and triangles, and identical closest hits for a fan of camera
+  rays through the oracle's scene intersection;
+* closed forms of the Pxr lights' placement (unit square, sphere and disk
+  under the current transform), the PxrSurface parameter map and clamps
+  (pbr.rs:133-185), media defaults and interior binding, and the object
+  ids of the id AOV (identifier-named geometry shares an id).
+"""
+import os
+import warnings
+
+import numpy as np
+import pytest
+
+import orc
+import rib
+import rrhip
+import scenes
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+RIBS = os.path.join(GOLD, "rib")
+
+
+def build(text, strict=True):
+    return rib.Builder(strict=strict).run(rib.parse(text))
+
+
+def d3(v):
+    return np.array(list(v), float)
+
+
+def n_objects(b):
+    return len(b.tris) + len(b.quadrics)
+
+
+def mat_fields(m):
+    """Every material field but the object id."""
+    scal = [getattr(m, f) for f in ("diffuse_gain", "diffuse_roughness", "specular_roughness", "specular_ior",
+                                    "clearcoat_gain", "clearcoat_roughness", "fuzz_gain", "glass_gain",
+                                    "glass_ior", "glass_roughness", "area_light", "cutout", "interior")]
+    vec = [d3(getattr(m, f)) for f in ("diffuse_color", "specular_face_color", "specular_edge_color",
+                                       "fuzz_color", "refraction_color", "glow", "emission")]
+    return tuple(np.round(np.concatenate([np.array(scal, float)] + vec), 12))
+
+
+def light_fields(l):
+    return np.r_[l.kind, l.intensity, d3(l.color), d3(l.p), d3(l.e1), d3(l.e2), l.radius]
+
+
+# ---- tokenizer ---------------------------------------------------------------------
+
+def test_tokenizer_values_comments_and_declarations():
+    reqs = rib.parse('##RenderMan RIB\nversion 3.04 # trailing\n'
+                     'Light "PxrRectLight" "key" "float intensity" [2.5e1] "color lightColor" [1 .5 -2E-1]\n'
+                     'Attribute "identifier" "name" ["a \\"b\\""]\nPolygon "P" [\n# inside\n 0 0 0 1 0 0 0 1 0 ]\n')
+    assert [r[0] for r in reqs] == ["version", "Light", "Attribute", "Polygon"]
+    assert reqs[0][1] == [3.04]
+    assert reqs[1][1][:3] == ["PxrRectLight", "key", "float intensity"]
+    assert reqs[1][1][3] == (25.0,) and reqs[1][1][5] == (1.0, 0.5, -0.2)
+    assert reqs[2][1][2] == ['a "b"']
+    assert reqs[3][1][1] == (0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0)
+    p = rib._Params(reqs[1][1], rib._pstart(reqs[1][1]))
+    assert p.number("intensity") == 25.0 and p.color("lightColor", None) == (1.0, 0.5, -0.2)
+    with pytest.raises(rib.RibError):
+        rib.parse('Polygon "P" [0 0 "x"]')
+    with pytest.raises(rib.RibError):
+        rib.parse('Surface "matte')
+
+
+# ---- scene_builder.rs unit tests ---------------------------------------------------------
+
+def test_single_pass_build_with_attribute_blocks():
+    b = build('''Format 320 240 1.0
+        PixelSamples 2 2
+        Projection "perspective" "fov" [45]
+        WorldBegin
+            LightSource "distantlight" 1 "from" [5 10 -10] "to" [0 0 0] "intensity" [0.8]
+            AttributeBegin Color 0 0 1 Surface "plastic" Translate -2 0 5 Sphere 1 -1 1 360 AttributeEnd
+            AttributeBegin Color 1 0 0 Surface "matte" Translate 2 0 5 Cylinder 0.5 -1 1 360 AttributeEnd
+        WorldEnd''')
+    assert n_objects(b) == 2 and len(b.lights) == 1 and len(b.materials) == 2
+    assert b.pixel_samples == (2, 2) and b.spp == 4 and b.width == 320
+    np.testing.assert_array_equal(d3(b.materials[0].diffuse_color), [0, 0, 1])
+    np.testing.assert_array_equal(d3(b.materials[1].diffuse_color), [1, 0, 0])
+    assert b.materials[0].diffuse_gain == 0.9 and b.materials[0].specular_ior == 1.5  # from_plastic
+    d = np.array([-5.0, -10.0, 10.0])
+    np.testing.assert_allclose(d3(b.lights[0].p), d / np.linalg.norm(d), rtol=0, atol=1e-15)
+    np.testing.assert_array_equal(b.quadrics[0][2][:, 3], [-2, 0, 5])
+
+
+def test_attribute_end_restores_state():
+    b = build('WorldBegin Color 1 1 1 AttributeBegin Color 0 1 0 Surface "metal" AttributeEnd '
+              'Translate 0 0 5 Sphere 1 -1 1 360 WorldEnd')
+    assert n_objects(b) == 1
+    m = b.materials[0]
+    np.testing.assert_array_equal(d3(m.diffuse_color), [1, 1, 1])
+    assert m.diffuse_gain == 1.0 and m.specular_ior == 1.0  # matte, not metal
+
+
+def test_new_quadrics_and_polygon():
+    b = build('''WorldBegin Translate 0 0 5
+        Torus 1 0.3 0 360 360  Disk 0 1 360  Paraboloid 1 0 1 360  Hyperboloid 1 0 -1 1 0 1 360
+        Polygon "P" [-1 -1 0  1 -1 0  1 1 0  -1 1 0] WorldEnd''')
+    assert n_objects(b) == 6
+    assert [q[0] for q in b.quadrics] == [rrhip.QUADRIC_TORUS, rrhip.QUADRIC_DISK, rrhip.QUADRIC_PARABOLOID,
+                                          rrhip.QUADRIC_HYPERBOLOID]
+    assert b.tris[0][:3] == ((-1.0, -1.0, 5.0), (1.0, -1.0, 5.0), (1.0, 1.0, 5.0))
+
+
+def test_concat_transform_matches_translate():
+    a = build("WorldBegin ConcatTransform [1 0 0 0 0 1 0 0 0 0 1 0 3 4 5 1] Sphere 1 -1 1 360 WorldEnd")
+    c = build("WorldBegin Translate 3 4 5 Sphere 1 -1 1 360 WorldEnd")
+    np.testing.assert_array_equal(a.quadrics[0][2], c.quadrics[0][2])
+
+
+def test_unknown_requests_strict_and_lenient():
+    text = '''Option "searchpath" "string shader" ["/tmp"]
+        Attribute "identifier" "name" ["ball"]
+        GeometricApproximation "flatness" 0.5
+        WorldBegin SubdivisionMesh "catmull-clark" [3] [0 1 2] [] [] [] [] Sphere 1 -1 1 360 WorldEnd'''
+    with pytest.raises(rib.RibUnsupported):
+        build(text)
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        b = build(text, strict=False)
+    assert n_objects(b) == 1 and len(w) == 2
+
+
+def test_motion_block_takes_first_sample():
+    text = "WorldBegin MotionBegin [0 1] Translate 0 0 5 Translate 0 0 9 MotionEnd Sphere 1 -1 1 360 WorldEnd"
+    with pytest.raises(rib.RibUnsupported):
+        build(text)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        b = build(text, strict=False)
+    assert b.quadrics[0][2][2, 3] == 5.0
+
+
+def test_points_polygons_builds_mesh():
+    b = build('''WorldBegin Translate 0 0 5
+        PointsPolygons [4 4] [0 1 2 3  4 5 6 7]
+          "P" [-1 -1 0  1 -1 0  1 1 0  -1 1 0  -1 -1 1  1 -1 1  1 1 1  -1 1 1] WorldEnd''')
+    assert len(b.meshes) == 1 and len(b.instances) == 1
+    assert b.meshes[0][1].shape == (4, 3)
+    np.testing.assert_array_equal(b.meshes[0][1][:2], [[0, 1, 2], [0, 2, 3]])
+    np.testing.assert_array_equal(b.instances[0][2][:, 3], [0, 0, 5])
+
+
+def test_object_instance_shares_mesh():
+    b = build('''WorldBegin
+        ObjectBegin "quad" PointsPolygons [4] [0 1 2 3] "P" [-1 -1 0  1 -1 0  1 1 0  -1 1 0] ObjectEnd
+        Translate -2 0 5 ObjectInstance "quad" Translate 4 0 0 ObjectInstance "quad" WorldEnd''')
+    assert len(b.meshes) == 1 and len(b.instances) == 2
+    assert b.instances[0][2][0, 3] == -2.0 and b.instances[1][2][0, 3] == 2.0
+    assert b.instances[0][0] == b.instances[1][0] == 0
+
+
+def test_inline_archive():
+    b = build('''ArchiveBegin "ball" Sphere 1 -1 1 360 ArchiveEnd
+        WorldBegin Translate 0 0 5 ReadArchive "ball" Translate 3 0 0 ReadArchive "ball" WorldEnd''')
+    assert len(b.quadrics) == 2
+    assert b.quadrics[0][2][0, 3] == 0.0 and b.quadrics[1][2][0, 3] == 3.0
+
+
+# ---- the reference's fixtures ---------------------------------------------------------
+
+def camera_fan(n=41):
+    u = np.linspace(-0.32, 0.32, n)
+    d = np.stack(np.meshgrid(u, u, indexing="ij"), -1).reshape(-1, 2)
+    d = np.c_[d, np.ones(len(d))]
+    return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+
+def same_hits(a, b, label, tol=1e-9):
+    """Closest hits of the two scenes' descriptions agree for every ray of
+    a fan from the origin: hit / miss, distance, normal and the hit
+    material's parameters."""
+    da, db = a.desc(), b.desc()
+    nhit = 0
+    for d in camera_fan():
+        ha = orc.scene_intersect(da, np.zeros(3), d)
+        hb = orc.scene_intersect(db, np.zeros(3), d)
+        assert (ha is None) == (hb is None), (label, d)
+        if ha is None:
+            continue
+        nhit += 1
+        assert ha[0] == pytest.approx(hb[0], abs=tol), (label, d)
+        np.testing.assert_allclose(ha[2], hb[2], atol=1e-7)
+        assert mat_fields(a.materials[ha[4]]) == mat_fields(b.materials[hb[4]]), (label, d)
+    assert nhit > 100
+
+
+def test_cornell_fixture_builds_the_restated_scene():
+    s, spp = rib.load(os.path.join(RIBS, "cornell.rib"))
+    ref = scenes.cornell_fixture(600, 600)
+    assert spp == 64 and (s.camera.width, s.camera.height, s.camera.fov_deg) == (600, 600, 36.0)
+    assert len(s.lights) == 1
+    np.testing.assert_allclose(light_fields(s.lights[0]), light_fields(ref.lights[0]), rtol=0, atol=1e-15)
+    # the ceiling emitter carries the light's index for MIS
+    emit = [m for m in s.materials if d3(m.emission).max() > 0]
+    assert len(emit) == 1 and emit[0].area_light == 0
+    np.testing.assert_allclose(d3(emit[0].emission), 17.0 * np.array([1.0, 0.9, 0.75]))
+    # every object has its own id (no identifiers in the file)
+    assert sorted(m.id for m in s.materials) == list(range(1, len(s.materials) + 1))
+    same_hits(s, ref, "cornell")
+
+
+def test_quadric_zoo_fixture_builds_the_restated_scene():
+    s, spp = rib.load(os.path.join(RIBS, "quadric_zoo.rib"))
+    ref = scenes.quadric_zoo(960, 540)
+    assert spp == 4
+    for a, b in zip(s.lights, ref.lights):
+        np.testing.assert_allclose(light_fields(a), light_fields(b), rtol=0, atol=1e-15)
+    assert len(s.quadrics) == len(ref.quadrics) == 8
+    for qa, qb in zip(s.quadrics, ref.quadrics):
+        assert qa.kind == qb.kind
+        for f in ("radius", "zmin", "zmax", "height", "thetamax"):
+            assert getattr(qa, f) == getattr(qb, f), f
+        np.testing.assert_array_equal(list(qa.p1), list(qb.p1))
+        np.testing.assert_array_equal(list(qa.p2), list(qb.p2))
+        np.testing.assert_allclose(list(qa.object_to_world), list(qb.object_to_world), rtol=0, atol=1e-15)
+        assert mat_fields(s.materials[qa.material]) == mat_fields(ref.materials[qb.material])
+    same_hits(s, ref, "zoo")
+
+
+@pytest.mark.parametrize("name", ["minimal", "three_primitives", "transforms", "shadows", "materials", "bright",
+                                  "simple_sphere"])
+def test_small_fixtures_build_and_render(name):
+    s, spp = rib.load(os.path.join(RIBS, name + ".rib"))
+    s.camera.width, s.camera.height = 16, 12
+    acc, _ = orc.render(s.desc(), rrhip.RenderParams.make(max_depth=4), 0, 1)
+    assert np.isfinite(acc).all() and (acc[:, 6] == 1).all()
+
+
+# ---- lights, materials, media, ids ---------------------------------------------------
+
+def test_pxr_area_lights_placement():
+    b = build('''WorldBegin
+        AttributeBegin Translate 0 2 5 Rotate 90 1 0 0 Scale 2 3 1
+          Light "PxrRectLight" "rect" "intensity" [4] "exposure" [1] "lightColor" [1 0.5 0.25] AttributeEnd
+        AttributeBegin Translate 1 1 1 Scale 3 3 3 Light "PxrSphereLight" "s" "intensity" [2] AttributeEnd
+        AttributeBegin Translate 0 0 4 Scale 2 2 2 Light "PxrDiskLight" "d" AttributeEnd
+        AttributeBegin Rotate 90 0 1 0 Light "PxrDistantLight" "sun" "angleExtent" [2] AttributeEnd
+        WorldEnd''')
+    rect, sph, disk, sun = b.lights
+    # Rotate 90 about x maps y to z: the unit square spans x in [-1, 1], z in [3.5, 6.5] at y = 2
+    assert rect.kind == rrhip.LIGHT_RECT and rect.intensity == 8.0
+    np.testing.assert_allclose(d3(rect.p), [-1, 2, 3.5], atol=1e-12)
+    np.testing.assert_allclose(d3(rect.e1), [2, 0, 0], atol=1e-12)
+    np.testing.assert_allclose(d3(rect.e2), [0, 0, 3], atol=1e-12)
+    em = b.materials[b.tris[0][3]]
+    assert em.area_light == 0 and em.diffuse_gain == 1.0 and d3(em.diffuse_color).max() == 0.0
+    np.testing.assert_allclose(d3(em.emission), [8, 4, 2])
+    corners = {tuple(np.round(v, 9)) for t in b.tris[:2] for v in t[:3]}
+    assert corners == {(-1.0, 2.0, 3.5), (1.0, 2.0, 3.5), (1.0, 2.0, 6.5), (-1.0, 2.0, 6.5)}
+    assert sph.kind == rrhip.LIGHT_SPHERE and sph.radius == pytest.approx(1.5)
+    np.testing.assert_allclose(d3(sph.p), [1, 1, 1])
+    assert disk.kind == rrhip.LIGHT_DISK
+    np.testing.assert_allclose(d3(disk.e1), [1, 0, 0], atol=1e-12)
+    kinds = [(q[0], b.materials[q[1]].area_light) for q in b.quadrics]
+    assert kinds == [(rrhip.QUADRIC_SPHERE, 1), (rrhip.QUADRIC_DISK, 2)]
+    # Rotate 90 about y maps -z to -x
+    assert sun.kind == rrhip.LIGHT_DISTANT and sun.radius == pytest.approx(np.radians(1.0))
+    np.testing.assert_allclose(d3(sun.p), [-1, 0, 0], atol=1e-12)
+
+
+def test_pxr_surface_parameters_and_clamps():
+    b = build('''WorldBegin Bxdf "PxrSurface" "m" "diffuseGain" [0.5] "diffuseColor" [0.1 0.2 0.3]
+        "specularFaceColor" [0.9 0.8 0.7] "specularRoughness" [0.001] "specularIor" [1.8]
+        "clearcoatGain" [0.4] "clearcoatRoughness" [3] "fuzzGain" [0.2] "glassRefractionGain" [0.6]
+        "glassIor" [1.33] "glassRoughness" [0.3] "glowGain" [2] "glowColor" [0.5 1 0] "presence" [0.75]
+        Sphere 1 -1 1 360
+        Bxdf "PxrSurface" "plain" Sphere 1 -1 1 360 WorldEnd''')
+    m, d = b.materials
+    assert (m.diffuse_gain, m.specular_ior, m.clearcoat_gain, m.fuzz_gain) == (0.5, 1.8, 0.4, 0.2)
+    assert (m.glass_gain, m.glass_ior, m.glass_roughness) == (0.6, 1.33, 0.3)
+    assert m.specular_roughness == 0.005 and m.clearcoat_roughness == 1.0  # clamped to [0.005, 1]
+    np.testing.assert_allclose(d3(m.glow), [1, 2, 0])
+    assert m.cutout == pytest.approx(0.25)
+    # PxrSurface defaults (pbr.rs:134-140)
+    np.testing.assert_array_equal(d3(d.diffuse_color), [0.18, 0.18, 0.18])
+    assert (d.diffuse_gain, d.specular_ior, d.specular_roughness) == (1.0, 1.5, 0.2)
+    with pytest.raises(rib.RibUnsupported):
+        build('WorldBegin Bxdf "PxrSurface" "s" "subsurfaceGain" [1] Sphere 1 -1 1 360 WorldEnd')
+
+
+def test_media_and_object_ids():
+    b = build('''WorldBegin Atmosphere "fog" "sigma_s" [0.2 0.2 0.2]
+        AttributeBegin Attribute "identifier" "name" ["glass"] Interior "water" "sigma_a" [1 2 3] "g" [0.5]
+          Sphere 1 -1 1 360 Translate 0 0 3 Sphere 1 -1 1 360 AttributeEnd
+        Sphere 2 -2 2 360 WorldEnd''')
+    fog, water = b.media
+    assert b.atmosphere == 0 and fog["max_distance"] == 1200.0
+    assert fog["sigma_a"] == (0.1, 0.1, 0.1) and fog["sigma_s"] == (0.2, 0.2, 0.2)
+    assert water["max_distance"] == 0.0 and water["sigma_a"] == (1.0, 2.0, 3.0) and water["g"] == 0.5
+    m0, m1, m2 = b.materials
+    assert m0.interior == m1.interior == 2 and m2.interior == 0  # medium index + 1; 0 = none
+    assert m0.id == m1.id == 1 and m2.id == 2  # identifier-named geometry shares an id
+
+
+def test_dome_light_map_from_file():
+    b = rib.Builder(base_dir=GOLD).run(rib.parse(
+        'WorldBegin Light "PxrDomeLight" "sky" "intensity" [2] "lightColorMap" ["sky.hdr"] WorldEnd'))
+    assert b.env.shape == (256, 512, 3) and b.lights[0].kind == rrhip.LIGHT_DOME
+    assert b.lights[0].intensity == 2.0
+    s = b.scene()
+    d = s.desc()
+    assert (d.env_width, d.env_height) == (512, 256)
