@@ -487,8 +487,35 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
       if (!std::isfinite(v)) return *err = "quadric transform must be finite", false;
     if (!(q.thetamax > 0.0)) return *err = "quadric thetamax must be > 0", false;
     if (q.kind == RRHIP_QUADRIC_CONE && !(q.height != 0.0)) return *err = "cone height must be nonzero", false;
-    // Matrix4::inverse().unwrap_or(identity) (sphere.rs / cylinder.rs constructors)
     const double* m = q.object_to_world;
+    // A whole sphere (no z or theta cut) under a similarity transform
+    // (rotation, uniform scale, translation) is the loose sphere with the
+    // translation as centre and radius * scale: the same hit, outward normal
+    // and front face (dot(d, n) < 0), tested without the quadric solvers,
+    // so scenes whose quadrics are all such spheres (RIB's Sphere request)
+    // keep the trace kernels without them. Only the order of exact ties
+    // with other loose objects can differ.
+    if (q.kind == RRHIP_QUADRIC_SPHERE && q.radius > 0.0 && q.zmin <= -q.radius && q.zmax >= q.radius &&
+        q.thetamax >= 360.0) {
+      double gram[3][3];
+      for (int x = 0; x < 3; x++)
+        for (int y = 0; y < 3; y++) gram[x][y] = m[x] * m[y] + m[4 + x] * m[4 + y] + m[8 + x] * m[8 + y];
+      const double s2 = gram[0][0];
+      bool similar = s2 > 0.0;
+      for (int x = 0; x < 3; x++)
+        for (int y = 0; y < 3; y++) similar &= std::fabs(gram[x][y] - (x == y ? s2 : 0.0)) <= 1e-12 * s2;
+      if (similar) {
+        PtSphere g{};
+        g.c[0] = (float)m[3];
+        g.c[1] = (float)m[7];
+        g.c[2] = (float)m[11];
+        g.r = (float)(q.radius * std::sqrt(s2));
+        g.material = q.material;
+        out->spheres.push_back(g);
+        continue;
+      }
+    }
+    // Matrix4::inverse().unwrap_or(identity) (sphere.rs / cylinder.rs constructors)
     const double a = m[0], b = m[1], c = m[2], dd = m[4], e = m[5], f = m[6], g = m[8], h = m[9], ii = m[10];
     const double A = e * ii - f * h, B = -(dd * ii - f * g), C = dd * h - e * g;
     const double det = a * A + b * B + c * C;
