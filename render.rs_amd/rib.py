@@ -19,11 +19,13 @@ Two stages, as in render.rs:
 
 Requests the GPU path has no counterpart for (subdivision surfaces,
 curves, patches, displacement, pattern graphs, hair, procedural density,
-motion blur of RIB motion blocks) raise `RibUnsupported` by default; with
+deformation motion) raise `RibUnsupported` by default; with
 `strict=False` they are skipped with one warning each, as render.rs skips
-requests it does not implement (a motion block then keeps its first
-sample, as render.rs does for requests other than transforms). Inline
-archives (ArchiveBegin / ReadArchive) and file archives are replayed.
+requests it does not implement (a motion block over other requests then
+keeps its first sample, as render.rs does). Transform motion blocks move
+the mesh instances created under them, and before WorldBegin the camera
+(scene_builder.rs:272-287, 1637-1692). Inline archives (ArchiveBegin /
+ReadArchive) and file archives are replayed.
 Requests that carry no render effect (Display, Declare, ShadingRate,
 Sides, ...) are accepted silently.
 """
@@ -307,6 +309,7 @@ class _State:
         self.bxdf = None        # PxrSurface parameters as keyword arguments of scenes.material
         self.interior = None
         self.identifier = None
+        self.motion = None      # (t0, t1) of the last transform motion block
 
     def copy(self):
         s = _State()
@@ -372,6 +375,7 @@ class Builder:
         self.recording = None  # (name, [requests]) between ArchiveBegin and ArchiveEnd
         self.motion = None     # requests of an open MotionBegin block
         self.archive_depth = 0
+        self.camera_motion_inv = None
 
     # -- helpers
     def _skip(self, what):
@@ -459,7 +463,13 @@ class Builder:
 
     def process(self, name, vals):
         st = self.state
-        if name == "WorldBegin" or name in _NO_EFFECT and name not in ("Option", "Attribute"):
+        if name == "WorldBegin":
+            if st.motion is not None:  # a motion block before the world moves the camera
+                t0, t1 = st.motion
+                self.camera_motion_inv = np.linalg.inv(t1 @ np.linalg.inv(t0))
+                st.motion = None
+            return
+        if name in _NO_EFFECT and name not in ("Option", "Attribute"):
             return
         if name in _UNSUPPORTED:
             return self._skip(f"request {name} is not supported by this backend")
@@ -474,9 +484,7 @@ class Builder:
             self.motion = []
         elif name == "MotionEnd":
             block, self.motion = self.motion or [], None
-            if block:
-                self._skip("motion blur of RIB motion blocks is not supported by this backend")
-                self.process(*block[0])
+            self._motion_block(block)
         elif name == "AttributeBegin":
             self.attr_stack.append(st.copy())
             self.xf_stack.append(self.xf.copy())
@@ -509,8 +517,10 @@ class Builder:
             if entries is None:
                 self._skip(f'ObjectInstance "{handle}" has no definition')
                 return
+            m1 = self._endpoint(self.xf)
             for mesh, local, mat in entries:
-                self.instances.append((mesh, mat, (self.xf @ local)[:3, :4].copy()))
+                self.instances.append((mesh, mat, (self.xf @ local)[:3, :4].copy(),
+                                       None if m1 is None else (m1 @ local)[:3, :4].copy()))
         elif name == "Format":
             if _num(vals, 0) is not None and _num(vals, 1) is not None:
                 self.width, self.height = int(_num(vals, 0)), int(_num(vals, 1))
@@ -613,6 +623,36 @@ class Builder:
             self._points_polygons(vals)
         else:
             self._skip(f"request {name} is not implemented")
+
+    _XF_REQS = ("Translate", "Rotate", "Scale", "ConcatTransform", "Transform", "Identity")
+
+    def _motion_block(self, block):
+        """finish_motion_block (scene_builder.rs:1637-1684): a block of
+        transforms records the shutter-open (first sample) and -close (last
+        sample) transforms; anything else keeps its first sample."""
+        if not block:
+            return
+        if all(n in self._XF_REQS for n, _ in block):
+            base = self.xf.copy()
+            self.process(*block[0])
+            t0 = self.xf.copy()
+            if len(block) > 1:
+                self.xf_stack[-1] = base
+                self.process(*block[-1])
+                t1 = self.xf.copy()
+                self.xf_stack[-1] = t0
+                self.state.motion = (t0, t1)
+            return
+        self._skip(f'MotionBegin over "{block[0][0]}" is not supported by this backend')
+        self.process(*block[0])
+
+    def _endpoint(self, current):
+        """motion_endpoint (scene_builder.rs:1686-1692): the shutter-close
+        transform of geometry created now, t1 * t0^-1 * current."""
+        if self.state.motion is None:
+            return None
+        t0, t1 = self.state.motion
+        return t1 @ np.linalg.inv(t0) @ current
 
     def _light_source(self, vals):
         """parse_light (scene_builder.rs:1582-1635): positions are taken as
@@ -785,7 +825,8 @@ class Builder:
         if self.defining is not None:
             self.defining[1].append((mesh, self.xf.copy(), mat))
         else:
-            self.instances.append((mesh, mat, self.xf[:3, :4].copy()))
+            m1 = self._endpoint(self.xf)
+            self.instances.append((mesh, mat, self.xf[:3, :4].copy(), None if m1 is None else m1[:3, :4].copy()))
 
     # -- result
     @property
@@ -807,6 +848,10 @@ class Builder:
             l, r, b, t = self.screen_window
             s.camera.ortho_half[0] = abs(r - l) * 0.5
             s.camera.ortho_half[1] = abs(t - b) * 0.5
+        if self.camera_motion_inv is not None:
+            for k, v in enumerate(self.camera_motion_inv[:3, :4].reshape(-1)):
+                s.camera.motion_inv[k] = float(v)
+            s.camera.has_motion_inv = 1
         s.materials = list(self.materials)
         s.lights = list(self.lights)
         for v0, v1, v2, mat in self.tris:
@@ -815,8 +860,8 @@ class Builder:
             s.add_quadric(kind, mat, m, **kw)
         for pos, idx, nrm in self.meshes:
             s.add_mesh(pos, idx, nrm)
-        for mesh, mat, m in self.instances:
-            s.add_instance(mesh, mat, m)
+        for mesh, mat, m, m1 in self.instances:
+            s.add_instance(mesh, mat, m, m1)
         for md in self.media:
             s.add_medium(**md)
         s.atmosphere = self.atmosphere

@@ -97,3 +97,38 @@ def test_pxr_lights_surfaces_instances_media_match_oracle(ctx):
     s, _ = rib.loads(PXR_RIB)
     p = rrhip.RenderParams.make(max_depth=8)
     parity(gpu_render(ctx, s, p, 0, 64), orc.render(s.desc(), p, 0, 64)[0], "pxr")
+
+
+MOTION_RIB = '''
+Format 48 36 1
+Projection "perspective" "fov" [45]
+MotionBegin [0 1] Translate 0 0 0 Translate 0.15 0 0 MotionEnd
+WorldBegin
+  Light "PxrDomeLight" "sky" "intensity" [0.8]
+  AttributeBegin
+    Translate 0 3 4 Rotate 90 1 0 0 Scale 2 2 1
+    Light "PxrRectLight" "key" "intensity" [5]
+  AttributeEnd
+  Bxdf "PxrSurface" "floor" "diffuseColor" [0.6 0.6 0.6] "specularIor" [1]
+  Polygon "P" [-4 -1 1  4 -1 1  4 -1 9  -4 -1 9]
+  ObjectBegin "box"
+    Bxdf "PxrSurface" "red" "diffuseColor" [0.8 0.2 0.1]
+    PointsPolygons [4 4 4 4] [0 1 2 3  4 5 6 7  0 4 7 3  1 2 6 5]
+      "P" [-0.5 -0.5 -0.5  0.5 -0.5 -0.5  0.5 0.5 -0.5  -0.5 0.5 -0.5
+           -0.5 -0.5 0.5  -0.5 0.5 0.5  0.5 0.5 0.5  0.5 -0.5 0.5]
+  ObjectEnd
+  AttributeBegin
+    MotionBegin [0 1] Translate -0.6 -0.4 4 Translate 0.4 -0.2 4 MotionEnd
+    Rotate 30 0 1 0
+    ObjectInstance "box"
+  AttributeEnd
+WorldEnd
+'''
+
+
+def test_motion_blocks_match_oracle(ctx):
+    say("RIB transform motion blocks: moving camera and a moving instance, 48x36, 64 spp")
+    s, _ = rib.loads(MOTION_RIB)
+    assert s.camera.has_motion_inv == 1 and s.instances[0].moving == 1
+    p = rrhip.RenderParams.make(max_depth=6)
+    parity(gpu_render(ctx, s, p, 0, 64), orc.render(s.desc(), p, 0, 64)[0], "motion")

@@ -134,13 +134,34 @@ def test_unknown_requests_strict_and_lenient():
 
 
 def test_motion_block_takes_first_sample():
-    text = "WorldBegin MotionBegin [0 1] Translate 0 0 5 Translate 0 0 9 MotionEnd Sphere 1 -1 1 360 WorldEnd"
-    with pytest.raises(rib.RibUnsupported):
-        build(text)
-    with warnings.catch_warnings():
-        warnings.simplefilter("ignore")
-        b = build(text, strict=False)
+    """scene_builder.rs:2128-2143: the quadric keeps the block's first sample."""
+    b = build("WorldBegin MotionBegin [0 1] Translate 0 0 5 Translate 0 0 9 MotionEnd Sphere 1 -1 1 360 WorldEnd")
     assert b.quadrics[0][2][2, 3] == 5.0
+    with pytest.raises(rib.RibUnsupported):
+        build('WorldBegin MotionBegin [0 1] Color 1 0 0 Color 0 1 0 MotionEnd WorldEnd')
+
+
+def test_motion_block_moves_instances_and_camera():
+    """Instances created after a transform block move from t0 to
+    t1 * t0^-1 * placement (scene_builder.rs:1686-1692); a block before
+    WorldBegin becomes the camera's motion, inverse(t1 * t0^-1)."""
+    b = build('''MotionBegin [0 1] Translate 0 0 0 Translate 0.5 0 0 MotionEnd
+        WorldBegin
+        ObjectBegin "q" PointsPolygons [3] [0 1 2] "P" [0 0 0 1 0 0 0 1 0] ObjectEnd
+        AttributeBegin MotionBegin [0 1] Translate 0 0 5 Translate 0 1 5 MotionEnd Scale 2 2 2
+          PointsPolygons [3] [0 1 2] "P" [0 0 0 1 0 0 0 1 0]
+          ObjectInstance "q" AttributeEnd
+        PointsPolygons [3] [0 1 2] "P" [0 0 0 1 0 0 0 1 0] WorldEnd''')
+    s = b.scene()
+    assert s.camera.has_motion_inv == 1
+    np.testing.assert_allclose(list(s.camera.motion_inv), [1, 0, 0, -0.5, 0, 1, 0, 0, 0, 0, 1, 0], atol=1e-15)
+    (_, _, m0, m1), (_, _, i0, i1), (_, _, k0, k1) = b.instances
+    np.testing.assert_allclose(m0[:, 3], [0, 0, 5]) and np.testing.assert_allclose(m0[0, 0], 2.0)
+    np.testing.assert_allclose(m1[:, 3], [0, 1, 5])
+    np.testing.assert_allclose(m1[:, :3], m0[:, :3])
+    np.testing.assert_allclose(i1, m1)
+    assert k1 is None  # the attribute block ended: static again
+    assert [bool(x.moving) for x in s.instances] == [True, True, False]
 
 
 def test_points_polygons_builds_mesh():
