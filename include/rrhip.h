@@ -463,9 +463,12 @@ int rrhip_session_get_stats(rrhip_session* session, rrhip_session_stats* stats);
  * counterpart in render.rs (its Metal host encodes one command buffer per
  * dispatch, renderer.rs:380-437). */
 int rrhip_session_set_launch_mode(rrhip_session* session, uint32_t mode);
-/* Autotuner: times every launch mode on `samples` samples per pixel
- * (discarded: the accumulator, statistics and any attached cache are left
- * as they were), keeps the fastest and reports it with its milliseconds. */
+/* Autotuner: times the graph and sorted-queue launch modes (bits 0 and 1;
+ * bit 2 stays as the caller set it) on `samples` samples per pixel,
+ * keeps the fastest and reports it with its milliseconds. The trials are
+ * discarded: the accumulator, the film accumulator, the statistics and
+ * any attached cache are left as they were, also when a trial fails (the
+ * launch mode then stays the caller's). */
 int rrhip_session_autotune(rrhip_session* session, uint32_t samples, uint32_t* best_mode, float* best_ms);
 /* Adaptive sampling: with tolerance > 0, rrhip_render_samples checks every
  * pixel before samples 32, 48, 64, ... (index relative to sample_base) and
@@ -496,6 +499,14 @@ int rrhip_read_film(rrhip_session* session, double* out);
  * RMSE, tests/pt_tests.rs:77-88); it is the error measure of the
  * time-to-quality benchmark (eps = 0.01). */
 int rrhip_relmse(rrhip_session* session, rrhip_session* reference, double eps, double* out);
+
+/* Diagnostic: the first `count` (1..4096) u32 draws of the PCG32 stream of
+ * each (pixels[i], samples[i]), generated on the device by the seeding and
+ * step the camera-path kernel uses, into out[i * count + k]. Counterpart of
+ * Pcg32::for_pixel_sample + next_u32 (src/raytracer/pt/sampler.rs:24-42);
+ * lets a caller check the device stream against the CPU path's. */
+int rrhip_pcg32_draws(rrhip_ctx* ctx, const uint64_t* pixels, const uint64_t* samples, uint32_t n,
+                      uint32_t count, uint32_t* out);
 
 /* ---- neural radiance cache ------------------------------------------------ */
 /* render.rs has no radiance cache: this section is the learned part of the

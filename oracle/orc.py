@@ -159,6 +159,7 @@ def _pt_sigs():
                               ctypes.c_double)
     sig = {
         "orc_render": (i32, [vp, vp, u32, u32, vp, i32, vp]),
+        "orc_render_pixels": (i32, [vp, vp, vp, u64, u32, u32, vp, i32, vp]),
         "orc_render_adaptive": (i32, [vp, vp, u32, f64, vp, i32]),
         "orc_render_film": (i32, [vp, vp, u32, u32, vp, i32]),
         "orc_env_pdf_eval_n": (None, [u32, u32, vp, vp, u64, vp, vp]),
@@ -192,6 +193,19 @@ def render(desc, params, start, count, accum=None, threads=None):
     rays = ctypes.c_uint64()
     lib.orc_render(ctypes.addressof(desc), ctypes.addressof(params), start, count, _p(accum),
                    threads or default_threads(), ctypes.byref(rays))
+    return accum, rays.value
+
+
+def render_pixels(desc, params, pixels, start, count, threads=None):
+    """render over the listed pixels only (full-image indices y * width +
+    x, so each keeps its whole-image seeds): (len(pixels) x 8 doubles,
+    rays)."""
+    lib = _pt_sigs()
+    px = np.ascontiguousarray(pixels, np.uint32).ravel()
+    accum = np.zeros((px.size, 8), np.float64)
+    rays = ctypes.c_uint64()
+    lib.orc_render_pixels(ctypes.addressof(desc), ctypes.addressof(params), _p(px), px.size, start, count,
+                          _p(accum), threads or default_threads(), ctypes.byref(rays))
     return accum, rays.value
 
 

@@ -42,11 +42,15 @@ PbrParams pbr_from(const rrhip_material_desc* m) {
 
 extern "C" {
 
-// Renders samples [start, start + count) of every pixel and ADDS them into
-// accum (RRHIP_ACCUM_STRIDE doubles per pixel). rays_out (optional):
-// total extension + shadow rays traced.
-int orc_render(const rrhip_scene_desc* desc, const rrhip_render_params* params, uint32_t start,
-               uint32_t count, double* accum, int nthreads, uint64_t* rays_out) {
+// Renders samples [start, start + count) of the pixels pixels[0..npix) (all
+// pixels in row order when `pixels` is null) and ADDS them into accum
+// (RRHIP_ACCUM_STRIDE doubles per listed pixel, in list order). Each pixel
+// keeps its full-image index for seeding, so a listed pixel gets exactly
+// the samples a whole-image render gives it. rays_out (optional): total
+// extension + shadow rays traced.
+static int render_list(const rrhip_scene_desc* desc, const rrhip_render_params* params, const uint32_t* pixels,
+                       uint64_t npix, uint32_t start, uint32_t count, double* accum, int nthreads,
+                       uint64_t* rays_out) {
   Scene scene;
   scene.load(*desc);
   RenderParams rp;
@@ -54,17 +58,20 @@ int orc_render(const rrhip_scene_desc* desc, const rrhip_render_params* params, 
   rp.rr_start = params->rr_start;
   rp.firefly_clamp = params->firefly_clamp;
   rp.sample_base = params->sample_base;
-  const uint32_t W = scene.camera.width, H = scene.camera.height;
-  std::atomic<uint32_t> next_row{0};
+  const uint32_t W = scene.camera.width;
+  if (!pixels) npix = (uint64_t)W * scene.camera.height;
+  const uint64_t chunk = pixels ? 16 : W;  // whole rows for whole images
+  std::atomic<uint64_t> next{0};
   std::atomic<uint64_t> total_rays{0};
   auto work = [&]() {
     uint64_t rays = 0;
     for (;;) {
-      uint32_t y = next_row.fetch_add(1);
-      if (y >= H) break;
-      for (uint32_t x = 0; x < W; x++) {
-        uint64_t pixel = (uint64_t)y * W + x;
-        double* a = accum + RRHIP_ACCUM_STRIDE * pixel;
+      const uint64_t i0 = next.fetch_add(chunk);
+      if (i0 >= npix) break;
+      for (uint64_t i = i0; i < std::min(npix, i0 + chunk); i++) {
+        const uint64_t pixel = pixels ? pixels[i] : i;
+        const uint32_t x = (uint32_t)(pixel % W), y = (uint32_t)(pixel / W);
+        double* a = accum + RRHIP_ACCUM_STRIDE * i;
         for (uint32_t s = start; s < start + count; s++) {
           Pcg32 rng = Pcg32::for_pixel_sample(pixel, rp.sample_base + s);
           Ray ray = scene.camera_ray(x, y, rng);
@@ -83,6 +90,23 @@ int orc_render(const rrhip_scene_desc* desc, const rrhip_render_params* params, 
   for (auto& th : pool) th.join();
   if (rays_out) *rays_out = total_rays.load();
   return 0;
+}
+
+// Renders samples [start, start + count) of every pixel and ADDS them into
+// accum (RRHIP_ACCUM_STRIDE doubles per pixel). rays_out (optional):
+// total extension + shadow rays traced.
+int orc_render(const rrhip_scene_desc* desc, const rrhip_render_params* params, uint32_t start,
+               uint32_t count, double* accum, int nthreads, uint64_t* rays_out) {
+  return render_list(desc, params, nullptr, 0, start, count, accum, nthreads, rays_out);
+}
+
+// The same for the listed pixels only (indices y * width + x of the full
+// image): a tile of a large image at the full image's seeds.
+int orc_render_pixels(const rrhip_scene_desc* desc, const rrhip_render_params* params, const uint32_t* pixels,
+                      uint64_t npix, uint32_t start, uint32_t count, double* accum, int nthreads,
+                      uint64_t* rays_out) {
+  if (!pixels) return -1;
+  return render_list(desc, params, pixels, npix, start, count, accum, nthreads, rays_out);
 }
 
 // pt::render_film (src/raytracer/pt/mod.rs:163-276) over samples

@@ -115,9 +115,17 @@ def read_rckd(path, width, height):
 def render_checkpointed(session, width, height, spp, path, every=4):
     """render_pt_checkpointed (renderer.rs:765-808) over a session: resume
     from a matching checkpoint at `path` (RCKD, else render.rs's RCKP),
-    render the remaining samples `every` at a time and write an RCKD
-    checkpoint after each step. Returns the samples rendered by this call."""
+    render the remaining samples `every` at a time and write a checkpoint
+    after each step, in the format the resumed one had (RCKP stays RCKP, so
+    render.rs can still resume the file; RCKD otherwise). Without a
+    checkpoint the session's accumulator is cleared first, as a fresh
+    render_pt_checkpointed starts from zero. Returns the samples rendered
+    by this call."""
+    if width * height != session.scene.npix:
+        raise ValueError(f"render_checkpointed: {width} x {height} does not match the session's "
+                         f"{session.scene.npix} pixels")
     start = 0
+    write = write_rckd
     got = read_rckd(path, width, height)
     if got is not None:
         start, acc = got
@@ -126,15 +134,18 @@ def render_checkpointed(session, width, height, spp, path, every=4):
         if got is not None:
             start, rgbn = got
             acc = from_rgbn(rgbn)
+            write = write_rckp
     if got is not None:
         start = min(start, spp)
         session.load_accum(acc)
+    else:
+        session.load_accum(None)
     s = start
     while s < spp:
         step = min(every, spp - s)
         session.render(s, step)
         s += step
-        write_rckd(path, width, height, s, session.accum())
+        write(path, width, height, s, session.accum())
     return spp - start
 
 

@@ -46,6 +46,11 @@ hipError_t pt_launch_batch_pass(const PtJob* jobs, uint32_t nj, const PathBuffer
 hipError_t pt_launch_resolve(const PathBuffers& P, const PtParams& prm, double* accum, double* film,
                              hipStream_t st);
 
+// out[i * count + k] = draw k of Pcg32::for_pixel_sample(pixel[i],
+// sample[i]), the stream a camera path of that pixel and sample runs on.
+hipError_t pt_launch_pcg32_draws(const uint64_t* pixel, const uint64_t* sample, uint32_t n, uint32_t count,
+                                 uint32_t* out, hipStream_t st);
+
 // Per-block partial sums of relMSE between two accumulators (`blocks`
 // doubles into `partial`, <= 1024 blocks); the caller adds them.
 hipError_t pt_launch_relmse(const double* accum, const double* ref_accum, uint32_t npix, double eps,

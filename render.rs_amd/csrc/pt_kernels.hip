@@ -131,6 +131,17 @@ __device__ __forceinline__ void init_path(const PtSceneView& S, const PathBuffer
   }
 }
 
+// The first `count` u32 draws of the per-(pixel, sample) streams, seeded
+// exactly as init_path seeds a camera path: out[i * count + k].
+__global__ __launch_bounds__(kBlock) void pcg32_draws_kernel(const uint64_t* __restrict__ pixel,
+                                                            const uint64_t* __restrict__ sample, uint32_t n,
+                                                            uint32_t count, uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Pcg32 rng = Pcg32::for_pixel_sample(pixel[i], sample[i]);
+  for (uint32_t k = 0; k < count; k++) out[(size_t)i * count + k] = rng.next_u32();
+}
+
 // sample0_dev (when set) supplies the pass's first sample index from device
 // memory, so a captured pass graph can be replayed for any sample range.
 __global__ __launch_bounds__(kBlock) void raygen_kernel(PtSceneView S, PathBuffers P, PtParams prm, uint32_t total,
@@ -1281,6 +1292,14 @@ hipError_t pt_launch_batch_pass(const PtJob* jobs, uint32_t nj, const PathBuffer
     hipLaunchKernelGGL(rotate_kernel, dim3(1), dim3(1), 0, st, P);
   }
   hipLaunchKernelGGL(resolve_batch_kernel, dim3(grid_for(total_pix)), dim3(kBlock), 0, st, jobs, nj, P, total_pix);
+  return hipGetLastError();
+}
+
+hipError_t pt_launch_pcg32_draws(const uint64_t* pixel, const uint64_t* sample, uint32_t n, uint32_t count,
+                                 uint32_t* out, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(pcg32_draws_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, pixel, sample, n,
+                     count, out);
   return hipGetLastError();
 }
 

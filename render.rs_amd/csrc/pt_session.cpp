@@ -367,16 +367,20 @@ int rrhip_session_autotune(rrhip_session* s, uint32_t samples, uint32_t* best_mo
   rrhip_scene* sc = s->scene;
   RRHIP_HIP_TRY(hipSetDevice(sc->ctx->device));
   hipStream_t st = sc->ctx->stream;
-  // Keep the accumulator, statistics and (with a cache) nothing else: the
-  // trial renders use sample indices past 2^40 and are discarded.
+  // Keep the accumulator, the film accumulator and the statistics: the
+  // trial renders use sample indices past 2^40 and are discarded, on every
+  // exit path (a cache is detached, so the trials never train it).
   const uint64_t abytes = (uint64_t)s->prm.npix * RRHIP_ACCUM_STRIDE * sizeof(double);
-  std::vector<double> keep(abytes / sizeof(double));
+  const uint64_t fbytes = s->film ? (uint64_t)s->prm.npix * kFilmStride * sizeof(double) : 0;
+  std::vector<double> keep(abytes / sizeof(double)), keep_film(fbytes / sizeof(double));
   unsigned long long stats[4];
   RRHIP_HIP_TRY(hipMemcpyAsync(keep.data(), s->d_accum, abytes, hipMemcpyDeviceToHost, st));
+  if (fbytes) RRHIP_HIP_TRY(hipMemcpyAsync(keep_film.data(), s->d_film, fbytes, hipMemcpyDeviceToHost, st));
   RRHIP_HIP_TRY(hipMemcpyAsync(stats, s->P.stats, sizeof(stats), hipMemcpyDeviceToHost, st));
   RRHIP_HIP_TRY(hipStreamSynchronize(st));
   const uint64_t base = s->sample_base;
   const uint64_t done = s->samples_done;
+  const uint32_t mode0 = s->launch_mode;
   rrhip_nrc* nrc = s->nrc;
   const double tol = s->ad_tol;
   s->nrc = nullptr;  // trials never train the cache nor prune pixels
@@ -386,7 +390,7 @@ int rrhip_session_autotune(rrhip_session* s, uint32_t samples, uint32_t* best_mo
   float best_t = 0.f;
   int status = RRHIP_OK;
   for (uint32_t mode = 0; mode < 4 && status == RRHIP_OK; mode++) {
-    status = rrhip_session_set_launch_mode(s, mode);
+    status = rrhip_session_set_launch_mode(s, mode | (mode0 & 4));
     if (status == RRHIP_OK) status = rrhip_render_samples(s, 0, std::min(samples, s->spp_pass));  // warm-up / capture
     if (status == RRHIP_OK) status = rrhip_render_samples(s, 0, samples);
     if (status == RRHIP_OK && (mode == 0 || s->last_ms < best_t)) {
@@ -398,15 +402,17 @@ int rrhip_session_autotune(rrhip_session* s, uint32_t samples, uint32_t* best_mo
   s->ad_tol = tol;
   s->sample_base = base;
   s->samples_done = done;
-  if (status == RRHIP_OK) status = rrhip_session_set_launch_mode(s, best);
-  if (status == RRHIP_OK) {
-    hipError_t e = hipMemcpyAsync(s->d_accum, keep.data(), abytes, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(s->P.stats, stats, sizeof(stats), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) status = fail_hip(e, "rrhip_session_autotune");
-  }
+  // Restore whatever the trials touched, also after a failed trial (then
+  // the launch mode goes back to the caller's; the first error is kept).
+  int rs = rrhip_session_set_launch_mode(s, status == RRHIP_OK ? best | (mode0 & 4) : mode0);
+  hipError_t e = hipMemcpyAsync(s->d_accum, keep.data(), abytes, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && fbytes) e = hipMemcpyAsync(s->d_film, keep_film.data(), fbytes, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(s->P.stats, stats, sizeof(stats), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (status == RRHIP_OK) status = rs;
+  if (status == RRHIP_OK && e != hipSuccess) status = fail_hip(e, "rrhip_session_autotune");
   if (status != RRHIP_OK) return status;
-  if (best_mode) *best_mode = best;
+  if (best_mode) *best_mode = best | (mode0 & 4);
   if (best_ms) *best_ms = best_t;
   return RRHIP_OK;
 }
@@ -563,6 +569,30 @@ int rrhip_relmse(rrhip_session* s, rrhip_session* ref, double eps, double* out) 
   double sum = 0.0;
   for (double v : h) sum += v;
   *out = sum / (3.0 * (double)s->prm.npix);
+  return RRHIP_OK;
+}
+
+int rrhip_pcg32_draws(rrhip_ctx* ctx, const uint64_t* pixels, const uint64_t* samples, uint32_t n, uint32_t count,
+                      uint32_t* out) {
+  if (!ctx) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_pcg32_draws: null context");
+  if (n && (!pixels || !samples || !out)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_pcg32_draws: null buffer");
+  if (count == 0 || count > 4096) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_pcg32_draws: count must be 1..4096");
+  if (n == 0) return RRHIP_OK;
+  RRHIP_HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const size_t ib = (size_t)n * sizeof(uint64_t), ob = (size_t)n * count * sizeof(uint32_t);
+  char* d = nullptr;
+  RRHIP_HIP_TRY(hipMalloc((void**)&d, 2 * ib + ob));
+  uint64_t* dp = (uint64_t*)d;
+  uint64_t* ds = (uint64_t*)(d + ib);
+  uint32_t* dout = (uint32_t*)(d + 2 * ib);
+  hipError_t e = hipMemcpyAsync(dp, pixels, ib, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(ds, samples, ib, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = pt_launch_pcg32_draws(dp, ds, n, count, dout, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, ob, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail_hip(e, "rrhip_pcg32_draws");
   return RRHIP_OK;
 }
 

@@ -43,7 +43,7 @@ EXPORTED = [
     "rrhip_batch_create", "rrhip_batch_destroy", "rrhip_batch_render", "rrhip_batch_read_accum",
     "rrhip_batch_get_stats", "rrhip_session_set_launch_mode", "rrhip_session_autotune",
     "rrhip_session_set_adaptive", "rrhip_session_average_spp",
-    "rrhip_session_set_film", "rrhip_read_film",
+    "rrhip_session_set_film", "rrhip_read_film", "rrhip_pcg32_draws",
 ]
 
 
@@ -271,6 +271,7 @@ def load():
         "rrhip_load_accum": (i32, [vp, vp]),
         "rrhip_session_get_stats": (i32, [vp, ctypes.POINTER(SessionStats)]),
         "rrhip_relmse": (i32, [vp, vp, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]),
+        "rrhip_pcg32_draws": (i32, [vp, vp, vp, u32, u32, vp]),
         "rrhip_nrc_create": (i32, [vp, ctypes.POINTER(NrcConfig), ctypes.POINTER(vp)]),
         "rrhip_nrc_destroy": (i32, [vp]),
         "rrhip_nrc_get_info": (i32, [vp, ctypes.POINTER(NrcInfo)]),
@@ -448,6 +449,17 @@ class Context:
         _check(load().rrhip_copy_to_host(self.h, _ptr(out), p, out.nbytes))
         return out
 
+    def pcg32_draws(self, pixels, samples, count):
+        """(n, count) u32: the first draws of each (pixel, sample) stream,
+        generated on the device."""
+        px = np.ascontiguousarray(pixels, np.uint64).ravel()
+        sm = np.ascontiguousarray(samples, np.uint64).ravel()
+        if px.shape != sm.shape:
+            raise ValueError("pcg32_draws: pixels and samples differ in length")
+        out = np.empty((px.size, count), np.uint32)
+        _check(load().rrhip_pcg32_draws(self.h, _ptr(px), _ptr(sm), px.size, count, _ptr(out)))
+        return out
+
 
 class PtScene:
     """A path-tracing scene uploaded from a scenes.Scene description."""
@@ -491,11 +503,17 @@ class Session:
         return out
 
     def load_accum(self, acc=None):
+        """Replaces the accumulator with `acc`, an (npix, ACCUM_STRIDE)
+        array (None clears it). Any other shape is refused: the C call
+        reads exactly npix * ACCUM_STRIDE doubles."""
         if acc is None:
             _check(load().rrhip_load_accum(self.h, None))
-        else:
-            acc = np.ascontiguousarray(acc, np.float64)
-            _check(load().rrhip_load_accum(self.h, _ptr(acc)))
+            return
+        acc = np.ascontiguousarray(acc, np.float64)
+        if acc.shape != (self.scene.npix, ACCUM_STRIDE):
+            raise ValueError(f"load_accum: expected an accumulator of shape ({self.scene.npix}, {ACCUM_STRIDE}), "
+                             f"got {acc.shape}")
+        _check(load().rrhip_load_accum(self.h, _ptr(acc)))
 
     def set_launch_mode(self, mode):
         """Bit 0: one hipGraph per pass size; bit 1: queue sorted by shading

@@ -211,3 +211,30 @@ def test_merge_of_sample_ranges_equals_one_render(tmp_path):
     accum_io.write_accum_exr(pb, 12, 48, b)
     with pytest.raises(ValueError):
         accum_io.merge([pa, pb])
+
+
+class _FakeScene:
+    npix = 6
+
+
+def test_load_accum_refuses_a_wrong_shape():
+    """Session.load_accum hands the C call a pointer to npix * 8 doubles:
+    any other shape is refused before the library is reached."""
+    se = rrhip.Session.__new__(rrhip.Session)
+    se.scene, se.h = _FakeScene(), None
+    for bad in (np.zeros((5, rrhip.ACCUM_STRIDE)), np.zeros((6, 4)), np.zeros(6 * rrhip.ACCUM_STRIDE)):
+        with pytest.raises(ValueError):
+            se.load_accum(bad)
+
+
+def test_render_checkpointed_refuses_another_resolution(tmp_path):
+    class FakeSession:
+        scene = _FakeScene()
+
+        def load_accum(self, acc=None):
+            raise AssertionError("must not be reached")
+
+    p = str(tmp_path / "run.rckd")
+    accum_io.write_rckd(p, 2, 3, 4, rand_acc(6))
+    with pytest.raises(ValueError):
+        accum_io.render_checkpointed(FakeSession(), 3, 3, 8, p)

@@ -62,6 +62,23 @@ def test_checkpoint_resume_is_exact(ctx, tmp_path):
     ps.close()
     np.testing.assert_allclose(res[:, 0:3], full[:, 0:3], rtol=1e-6, atol=1e-6)
     assert np.array_equal(res[:, 6], full[:, 6])
+    # a resumed RCKP stays an RCKP, so render.rs can still resume it
+    done, _ = accum_io.read_rckp(rk, W, H)
+    assert done == 16 and accum_io.read_rckd(rk, W, H) is None
+
+
+def test_checkpointed_render_without_a_checkpoint_starts_from_zero(ctx, tmp_path):
+    say("cornell 32x32: a checkpointed render in a used session starts from a clear accumulator")
+    s = scenes.cornell_fixture(W, H)
+    ps, se = session(ctx, s)
+    se.render(0, 8)
+    ref = se.accum()
+    se.render(8, 4)  # leftovers a fresh checkpointed render must not keep
+    ck = str(tmp_path / "fresh.rckd")
+    assert accum_io.render_checkpointed(se, W, H, 8, ck) == 8
+    assert np.array_equal(se.accum().view(np.uint64), ref.view(np.uint64))
+    se.close()
+    ps.close()
 
 
 def test_sample_ranges_merge(ctx, tmp_path):

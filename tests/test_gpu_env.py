@@ -11,10 +11,9 @@ import sys
 import numpy as np
 import pytest
 
-import orc
 import rrhip
 import scenes
-from pt_stats import image_mean_z, mean_var, sigma_test
+from pt_stats import image_mean_z, mean_var, oracle_render, sigma_test
 import rgbe
 
 pytestmark = pytest.mark.gpu
@@ -56,7 +55,7 @@ def test_sky_dome_matches_oracle(ctx):
     s = sky_scene(64, 48)
     p = rrhip.RenderParams.make(max_depth=8)
     g = gpu_render(ctx, s, p, 0, 64)
-    c, _ = orc.render(s.desc(), p, 0, 64)
+    c, _ = oracle_render(s.desc(), p, 0, 64)
     z = sigma_test(g, c)
     frac = float((z > 3).mean())
     zi = image_mean_z(g, c)

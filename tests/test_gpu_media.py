@@ -6,10 +6,9 @@ import sys
 import numpy as np
 import pytest
 
-import orc
 import rrhip
 import scenes
-from pt_stats import image_mean_z, mean_var, sigma_test
+from pt_stats import image_mean_z, mean_var, oracle_render, sigma_test
 
 pytestmark = pytest.mark.gpu
 
@@ -34,7 +33,7 @@ def test_cloud_room_matches_oracle(ctx):
     s = scenes.cloud_room(64, 64, n=32)
     p = rrhip.RenderParams.make(max_depth=64)
     acc_g = render(ctx, s, p, 0, 256)
-    acc_c, _ = orc.render(s.desc(), p, 0, 256)
+    acc_c, _ = oracle_render(s.desc(), p, 0, 256)
     z = sigma_test(acc_g, acc_c)
     frac = float((z > 3).mean())
     zi = image_mean_z(acc_g, acc_c)
@@ -128,7 +127,7 @@ def test_single_scatter_towards_a_point_light_matches_quadrature(ctx):
     mean = m.mean(0)
     se = np.sqrt(var.mean(0) / n.mean() / m.shape[0])
     want = single_scatter_quadrature()
-    acc_c, _ = orc.render(single_scatter_scene(8, 8).desc(), p, 0, 4096)
+    acc_c, _ = oracle_render(single_scatter_scene(8, 8).desc(), p, 0, 4096)
     mc = (acc_c[:, :3] / acc_c[:, 6:7]).mean(0)
     say(f"gpu {mean.round(6)} (se {se.round(6)}), quadrature {want.round(6)}, oracle {mc.round(6)}")
     assert np.all(np.abs(mean - want) < 4 * se + 3e-4 * want)
@@ -141,7 +140,7 @@ def test_point_light_in_a_cloud_matches_oracle(ctx):
     s.add_light(scenes.light(rrhip.LIGHT_POINT, intensity=2.0, color=(1.0, 0.9, 0.7), p=(0.3, 0.2, 0.1)))
     p = rrhip.RenderParams.make(max_depth=16)
     acc_g = render(ctx, s, p, 0, 256)
-    acc_c, _ = orc.render(s.desc(), p, 0, 256)
+    acc_c, _ = oracle_render(s.desc(), p, 0, 256)
     z = sigma_test(acc_g, acc_c)
     frac = float((z > 3).mean())
     zi = image_mean_z(acc_g, acc_c)
@@ -255,7 +254,7 @@ def test_transparent_room_matches_oracle(ctx):
     s = transparent_room(48, 48)
     p = rrhip.RenderParams.make(max_depth=16)
     acc_g = render(ctx, s, p, 0, 256)
-    acc_c, _ = orc.render(s.desc(), p, 0, 256)
+    acc_c, _ = oracle_render(s.desc(), p, 0, 256)
     z = sigma_test(acc_g, acc_c)
     frac = float((z > 3).mean())
     zi = image_mean_z(acc_g, acc_c)

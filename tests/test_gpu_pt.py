@@ -13,10 +13,9 @@ import sys
 import numpy as np
 import pytest
 
-import orc
 import rrhip
 import scenes
-from pt_stats import image_mean_z, mean_var, sigma_test
+from pt_stats import image_mean_z, mean_var, oracle_render, sigma_test
 
 pytestmark = pytest.mark.gpu
 
@@ -65,7 +64,7 @@ def test_box_room_matches_oracle(ctx):
     s = scenes.box_room(256, 256)
     p = rrhip.RenderParams.make(max_depth=8)
     acc_g, st = gpu_render(ctx, s, p, 0, 16)
-    acc_c, rays = orc.render(s.desc(), p, 0, 16)
+    acc_c, rays = oracle_render(s.desc(), p, 0, 16)
     say(f"gpu stats {st}; oracle rays {rays}")
     check_parity(acc_g, acc_c, label="box room")
     total = st["extension_rays"] + st["shadow_rays"]
@@ -77,7 +76,7 @@ def test_box_room_high_spp_matches_oracle(ctx):
     s = scenes.box_room(96, 96)
     p = rrhip.RenderParams.make(max_depth=8)
     acc_g, _ = gpu_render(ctx, s, p, 0, 256)
-    acc_c, _ = orc.render(s.desc(), p, 0, 256)
+    acc_c, _ = oracle_render(s.desc(), p, 0, 256)
     check_parity(acc_g, acc_c, label="box room 256 spp")
 
 
@@ -93,7 +92,7 @@ def test_cornell_fixture_gi_and_parity(ctx):
     right = img[24:72, 64:].reshape(-1, 3).mean(0)
     assert left[0] > left[1] and right[1] > right[0]
     assert img[4, 4].sum() > 0.03
-    acc_c, _ = orc.render(s.desc(), p, 0, 48)
+    acc_c, _ = oracle_render(s.desc(), p, 0, 48)
     check_parity(acc_g, acc_c, label="cornell")
 
 
@@ -109,7 +108,7 @@ def test_furnace_metal_and_glass(ctx, which):
     m, _, _ = mean_var(acc_g)
     avg = m.reshape(48, 48, 3)[6:42, 6:42].mean()
     assert 0.55 <= avg <= 1.05
-    acc_c, _ = orc.render(s.desc(), p, 0, 256)
+    acc_c, _ = oracle_render(s.desc(), p, 0, 256)
     check_parity(acc_g, acc_c, label=which)
 
 
@@ -135,7 +134,7 @@ def test_every_light_kind_and_lobe_matches_oracle(ctx):
     s = lights_scene()
     p = rrhip.RenderParams.make(max_depth=10)
     acc_g, _ = gpu_render(ctx, s, p, 0, 256)
-    acc_c, _ = orc.render(s.desc(), p, 0, 256)
+    acc_c, _ = oracle_render(s.desc(), p, 0, 256)
     check_parity(acc_g, acc_c, label="lights")
 
 
@@ -169,7 +168,7 @@ def test_instanced_field_matches_oracle(ctx):
     s = scenes.instanced_field(96, 54, n_instances=300, levels=3, n_point_lights=16)
     p = rrhip.RenderParams.make(max_depth=12)
     acc_g, st = gpu_render(ctx, s, p, 0, 128)
-    acc_c, rays = orc.render(s.desc(), p, 0, 128)
+    acc_c, rays = oracle_render(s.desc(), p, 0, 128)
     say(f"gpu rays {st['extension_rays'] + st['shadow_rays']} oracle rays {rays}")
     check_parity(acc_g, acc_c, label="instanced field")
 
@@ -298,13 +297,13 @@ def test_thin_lens_and_orthographic_cameras_match_oracle(ctx):
     s.camera.lens_radius = 0.05
     s.camera.focal_distance = 1.0
     acc_g, _ = gpu_render(ctx, s, p, 0, 128)
-    acc_c, _ = orc.render(s.desc(), p, 0, 128)
+    acc_c, _ = oracle_render(s.desc(), p, 0, 128)
     check_parity(acc_g, acc_c, label="thin lens")
     s2 = scenes.box_room(96, 96)
     s2.camera.projection = 1
     s2.camera.ortho_half = (ctypes.c_double * 2)(0.9, 0.9)
     acc_g, _ = gpu_render(ctx, s2, p, 0, 128)
-    acc_c, _ = orc.render(s2.desc(), p, 0, 128)
+    acc_c, _ = oracle_render(s2.desc(), p, 0, 128)
     check_parity(acc_g, acc_c, label="orthographic")
     # invalid cameras are refused
     s3 = scenes.box_room(8, 8)
@@ -356,5 +355,5 @@ def test_moving_instances_match_oracle(ctx):
         inst.moving = 1
     p = rrhip.RenderParams.make(max_depth=12)
     acc_g, _ = gpu_render(ctx, s, p, 0, 128)
-    acc_c, _ = orc.render(s.desc(), p, 0, 128)
+    acc_c, _ = oracle_render(s.desc(), p, 0, 128)
     check_parity(acc_g, acc_c, label="moving instances")

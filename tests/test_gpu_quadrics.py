@@ -8,10 +8,9 @@ import sys
 import numpy as np
 import pytest
 
-import orc
 import rrhip
 import scenes
-from pt_stats import image_mean_z, mean_var, sigma_test
+from pt_stats import image_mean_z, mean_var, oracle_render, sigma_test
 
 pytestmark = pytest.mark.gpu
 
@@ -46,7 +45,7 @@ def test_quadric_zoo_matches_oracle(ctx):
     say("quadric zoo 96x54, 64 spp")
     s = scenes.quadric_zoo(96, 54)
     p = rrhip.RenderParams.make(max_depth=6)
-    parity(gpu_render(ctx, s, p, 0, 64), orc.render(s.desc(), p, 0, 64)[0], "zoo")
+    parity(gpu_render(ctx, s, p, 0, 64), oracle_render(s.desc(), p, 0, 64)[0], "zoo")
 
 
 def test_glass_quadrics_match_oracle(ctx):
@@ -60,7 +59,7 @@ def test_glass_quadrics_match_oracle(ctx):
     s.add_quadric(rrhip.QUADRIC_CYLINDER, g, scenes._place((-0.7, 0, 0), -90), radius=0.5, zmin=-1.0, zmax=0.8)
     s.add_quadric(rrhip.QUADRIC_SPHERE, g, scenes._place((0.8, 0, 0)), radius=0.6, zmin=-0.6, zmax=0.3)
     p = rrhip.RenderParams.make(max_depth=8)
-    parity(gpu_render(ctx, s, p, 0, 64), orc.render(s.desc(), p, 0, 64)[0], "glass")
+    parity(gpu_render(ctx, s, p, 0, 64), oracle_render(s.desc(), p, 0, 64)[0], "glass")
 
 
 def test_partial_torus_and_hyperboloid_match_oracle(ctx):
@@ -74,4 +73,4 @@ def test_partial_torus_and_hyperboloid_match_oracle(ctx):
     s.add_quadric(rrhip.QUADRIC_HYPERBOLOID, s.add_material(scenes.plastic((0.2, 0.5, 0.8))),
                   scenes._place((1.2, 0, 0), -90), p1=(0.6, 0.0, -0.8), p2=(0.2, 0.5, 0.8), thetamax=250.0)
     p = rrhip.RenderParams.make(max_depth=6)
-    parity(gpu_render(ctx, s, p, 0, 64), orc.render(s.desc(), p, 0, 64)[0], "torus+hyperboloid")
+    parity(gpu_render(ctx, s, p, 0, 64), oracle_render(s.desc(), p, 0, 64)[0], "torus+hyperboloid")

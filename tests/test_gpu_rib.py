@@ -10,10 +10,9 @@ import sys
 import numpy as np
 import pytest
 
-import orc
 import rib
 import rrhip
-from pt_stats import image_mean_z, mean_var, sigma_test
+from pt_stats import image_mean_z, mean_var, oracle_render, sigma_test
 
 pytestmark = pytest.mark.gpu
 
@@ -89,14 +88,14 @@ def test_rib_fixture_matches_oracle(ctx, name, w, h, spp):
     s, _ = rib.load(os.path.join(RIBS, name + ".rib"))
     s.camera.width, s.camera.height = w, h
     p = rrhip.RenderParams.make(max_depth=6)
-    parity(gpu_render(ctx, s, p, 0, spp), orc.render(s.desc(), p, 0, spp)[0], name)
+    parity(gpu_render(ctx, s, p, 0, spp), oracle_render(s.desc(), p, 0, spp)[0], name)
 
 
 def test_pxr_lights_surfaces_instances_media_match_oracle(ctx):
     say("Pxr lights, PxrSurface, instanced PointsPolygons and an interior medium, 64x48, 64 spp")
     s, _ = rib.loads(PXR_RIB)
     p = rrhip.RenderParams.make(max_depth=8)
-    parity(gpu_render(ctx, s, p, 0, 64), orc.render(s.desc(), p, 0, 64)[0], "pxr")
+    parity(gpu_render(ctx, s, p, 0, 64), oracle_render(s.desc(), p, 0, 64)[0], "pxr")
 
 
 MOTION_RIB = '''
@@ -131,4 +130,4 @@ def test_motion_blocks_match_oracle(ctx):
     s, _ = rib.loads(MOTION_RIB)
     assert s.camera.has_motion_inv == 1 and s.instances[0].moving == 1
     p = rrhip.RenderParams.make(max_depth=6)
-    parity(gpu_render(ctx, s, p, 0, 64), orc.render(s.desc(), p, 0, 64)[0], "motion")
+    parity(gpu_render(ctx, s, p, 0, 64), oracle_render(s.desc(), p, 0, 64)[0], "motion")

@@ -193,6 +193,18 @@ def test_sample_ranges_compose():
     assert np.allclose(whole, part, rtol=1e-12, atol=0)
 
 
+def test_pixel_list_render_equals_the_whole_image():
+    """orc.render_pixels gives the listed pixels exactly the sums a whole-
+    image render gives them (same full-image seeds)."""
+    s = scenes.box_room(24, 16)
+    d = s.desc()
+    p = rrhip.RenderParams.make(max_depth=8)
+    whole, _ = orc.render(d, p, 0, 6)
+    px = np.array([0, 5, 23, 24, 200, 383, 17], np.uint32)
+    part, _ = orc.render_pixels(d, p, px, 0, 6, threads=3)
+    assert part.tobytes() == whole[px].tobytes()
+
+
 def test_converges_with_spp():
     # pt_tests.rs:72-95
     s = scenes.cornell_fixture(32, 32)
@@ -208,22 +220,44 @@ def test_converges_with_spp():
     assert fine < coarse * 0.6
 
 
-def test_sigma_harness_detects_a_planted_bias():
-    """The 3-sigma comparison used by the GPU parity tests: two oracle
-    renders over disjoint sample ranges pass it; planting a 3 % bias in
-    one of them (on the CPU) fails it."""
-    from pt_stats import image_mean_z, sigma_test
-    s = scenes.box_room(32, 32)
+def _plant(acc, factor):
+    """Scales each pixel's samples by `factor` (per pixel, broadcast over
+    channels): sums by f, sums of squares by f^2."""
+    out = acc.copy()
+    out[:, :3] *= factor
+    out[:, 3:6] *= factor * factor
+    return out
+
+
+def _checker(w, block=8):
+    yy, xx = np.meshgrid(np.arange(w), np.arange(w), indexing="ij")
+    return ((((yy // block) + (xx // block)) % 2) * 2 - 1).ravel()[:, None].astype(np.float64)
+
+
+@pytest.mark.parametrize("w,spp,uniform_bias,local_bias", [(64, 16, 0.05, 0.20), (96, 256, 0.02, 0.10)])
+def test_sigma_harness_detects_a_planted_bias(w, spp, uniform_bias, local_bias):
+    """The 3-sigma comparison of the GPU parity tests (at the sample counts
+    they use: 16 spp for the box room, 256 for most others), fed two
+    independently seeded oracle renders of the box room as the GPU tests
+    feed it (pt_stats.oracle_render): they pass both bars. Planted on the
+    CPU into one of them: a uniform bias fails the image-mean bar (5 % at
+    64 x 64 x 16 spp, 2 % at 96 x 96 x 256 spp), and a bias of +-local_bias
+    in alternating 8 x 8 blocks, which cancels in the image mean, fails
+    the per-pixel bar (more than 1 % of pixel-channels beyond 3 sigma):
+    +-20 % at 16 spp, +-10 % at 256 spp."""
+    from pt_stats import image_mean_z, oracle_render, sigma_test
+    s = scenes.box_room(w, w)
     d = s.desc()
     p = rrhip.RenderParams.make(max_depth=8)
-    a, _ = orc.render(d, p, 0, 64)
-    b, _ = orc.render(d, p, 1000, 64)
+    a, _ = orc.render(d, p, 0, spp)
+    b, _ = oracle_render(d, p, 0, spp)
+    assert not np.array_equal(a[:, :3], b[:, :3])
     assert (sigma_test(a, b) > 3).mean() < 0.01
     assert (image_mean_z(a, b) < 3).all()
-    biased = b.copy()
-    biased[:, :3] *= 1.03
-    biased[:, 3:6] *= 1.03 ** 2
-    assert (image_mean_z(a, biased) > 3).any()
+    assert (image_mean_z(a, _plant(b, 1 + uniform_bias)) > 3).any()
+    local = _plant(b, 1 + local_bias * _checker(w))
+    assert (image_mean_z(a, local) < 3).all()
+    assert (sigma_test(a, local) > 3).mean() > 0.01
 
 
 def test_thin_lens_focuses_on_the_focal_plane():

@@ -156,7 +156,8 @@ def test_motion_block_moves_instances_and_camera():
     assert s.camera.has_motion_inv == 1
     np.testing.assert_allclose(list(s.camera.motion_inv), [1, 0, 0, -0.5, 0, 1, 0, 0, 0, 0, 1, 0], atol=1e-15)
     (_, _, m0, m1), (_, _, i0, i1), (_, _, k0, k1) = b.instances
-    np.testing.assert_allclose(m0[:, 3], [0, 0, 5]) and np.testing.assert_allclose(m0[0, 0], 2.0)
+    np.testing.assert_allclose(m0[:, 3], [0, 0, 5])
+    np.testing.assert_allclose(m0[0, 0], 2.0)
     np.testing.assert_allclose(m1[:, 3], [0, 1, 5])
     np.testing.assert_allclose(m1[:, :3], m0[:, :3])
     np.testing.assert_allclose(i1, m1)
