@@ -410,6 +410,18 @@ int rrhip_trace_closest_device(rrhip_scene* scene, const rrhip_ray* rays, uint64
 int rrhip_trace_any_device(rrhip_scene* scene, const rrhip_ray* rays, uint64_t n,
                            uint8_t* occluded, void* stream);
 
+/* rrhip_trace_closest without any device allocation: no counterpart in
+ * render.rs. For callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc). Rays and hits are staged through
+ * a fixed per-device array of RRHIP_NOALLOC_MAX_RAYS x 48 bytes (48 MiB),
+ * resolved by rrhip_init and held until the process ends; n above
+ * RRHIP_NOALLOC_MAX_RAYS is rejected with RRHIP_ERR_INVALID_ARGUMENT.
+ * Results equal rrhip_trace_closest's. Every call on a device shares that
+ * device's array: calls that overlap on one device (from several threads or
+ * contexts) race. */
+#define RRHIP_NOALLOC_MAX_RAYS (1u << 20)
+int rrhip_trace_closest_noalloc(rrhip_scene* scene, const rrhip_ray* rays, uint64_t n, rrhip_hit* hits);
+
 /* Traversal cost of n host rays as the kernels execute it: totals of node
  * fetches (64 B each) and triangle fetches (48 B each). any != 0 counts the
  * any-hit traversal. out[0] = nodes, out[1] = triangles. No counterpart in

@@ -157,6 +157,10 @@ int rrhip_init(int device, rrhip_ctx** out) {
     rrhip_destroy(ctx);
     return fail_hip(e, "rrhip_init");
   }
+  if (int st = noalloc_setup(device)) {
+    rrhip_destroy(ctx);
+    return st;
+  }
   *out = ctx;
   return RRHIP_OK;
 }

@@ -12,6 +12,9 @@ namespace rrhip {
 int fail(int status, const std::string& msg);
 // Maps a HIP error to a status (and records where it happened).
 int fail_hip(hipError_t e, const char* where);
+// Resolves the current device's copy of the no-allocation staging array
+// (once per device; the device must be current).
+int noalloc_setup(int device);
 
 }  // namespace rrhip
 

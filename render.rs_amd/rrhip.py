@@ -19,6 +19,9 @@ RAY_DTYPE = np.dtype([("ox", "<f4"), ("oy", "<f4"), ("oz", "<f4"), ("tmin", "<f4
                       ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("prim", "<u4"), ("u", "<f4"), ("v", "<f4")])
 
+# Rays one rrhip_trace_closest_noalloc call may stage (RRHIP_NOALLOC_MAX_RAYS).
+NOALLOC_MAX_RAYS = 1 << 20
+
 STATUS_NAMES = {
     0: "OK", 1: "INVALID_ARGUMENT", 2: "NO_DEVICE", 3: "DEVICE",
     4: "OUT_OF_MEMORY", 5: "UNSUPPORTED", 6: "INTERNAL",
@@ -44,6 +47,7 @@ EXPORTED = [
     "rrhip_batch_get_stats", "rrhip_session_set_launch_mode", "rrhip_session_autotune",
     "rrhip_session_set_adaptive", "rrhip_session_average_spp",
     "rrhip_session_set_film", "rrhip_read_film", "rrhip_pcg32_draws",
+    "rrhip_trace_closest_noalloc",
 ]
 
 
@@ -260,6 +264,7 @@ def load():
         "rrhip_trace_closest": (i32, [vp, vp, u64, vp]),
         "rrhip_trace_any": (i32, [vp, vp, u64, vp]),
         "rrhip_trace_closest_device": (i32, [vp, vp, u64, vp, vp]),
+        "rrhip_trace_closest_noalloc": (i32, [vp, vp, u64, vp]),
         "rrhip_trace_any_device": (i32, [vp, vp, u64, vp, vp]),
         "rrhip_trace_cost": (i32, [vp, vp, u64, i32, ctypes.POINTER(u64)]),
         "rrhip_time_trace_device": (i32, [vp, vp, u64, i32, vp, u32, ctypes.POINTER(ctypes.c_float)]),
@@ -617,6 +622,14 @@ class Scene:
         rays = np.ascontiguousarray(rays, RAY_DTYPE)
         hits = np.empty(rays.shape[0], HIT_DTYPE)
         _check(load().rrhip_trace_closest(self.h, _ptr(rays), rays.shape[0], _ptr(hits)))
+        return hits
+
+    def trace_closest_noalloc(self, rays):
+        """Closest hit staged through the library's fixed per-device array
+        (no device allocation; at most NOALLOC_MAX_RAYS rays)."""
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.empty(rays.shape[0], HIT_DTYPE)
+        _check(load().rrhip_trace_closest_noalloc(self.h, _ptr(rays), rays.shape[0], _ptr(hits)))
         return hits
 
     def trace_any(self, rays):
