@@ -30,12 +30,44 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trace_closest_heightfield_1m_traffic.json")
 
 
+# A one-GPU share of the host runs at most this many worker threads.
+HOST_THREAD_CAP = 16
+
+
+def host_threads():
+    """Worker threads of the CPU baselines: every CPU this process may run
+    on (its affinity mask), at most the one-GPU host share."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(HOST_THREAD_CAP, n))
+
+
+def host_description():
+    """CPU model and the CPUs this process may use."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    return {"cpu_model": model, "affinity_cpus": affinity, "threads": host_threads(),
+            "thread_cap": HOST_THREAD_CAP}
+
+
 def cpu_baseline(pos, idx, rays):
     """The oracle (C++ f64 restatement of render.rs's CPU traversal) over
     the same rays on the host's cores; BVH build excluded."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_threads()
     m = orc.Mesh(pos, idx, bins=32, max_leaf=4, sah_leaf_mult=1)
     t0 = time.perf_counter()
     m.trace_closest(rays, threads=threads)
@@ -47,7 +79,68 @@ def cpu_baseline(pos, idx, rays):
         "kind": "port",
         "sample": f"all {rays.shape[0]} rays of the workload, closest hit, f64, "
                   f"{threads} threads; BVH build excluded; {dt:.2f} s",
+        "host": host_description(),
     }
+
+
+# Algorithmic bytes of the path tracer's trace kernels, per traced ray
+# (the B model): the ray's path state and the record it writes, plus what
+# its traversal fetches as the profiling kernels count it — 64-byte nodes,
+# 48-byte triangle records, 80 bytes per instance entered (instance and
+# mesh records) — and every loose object (64-byte triangles, 32-byte
+# spheres, 128-byte quadrics), which each ray tests.
+EXTEND_STATE_BYTES = 4 + 32 + 52   # queue entry; origin, direction; hit normal/t, material, two origins
+SHADOW_STATE_BYTES = 4 + 48 + 32   # queue entry; origin, direction, contribution; radiance read + write
+HBM_PEAK = 8000.0  # GB/s
+MFMA_BF16_PEAK = 2500.0  # dense TFLOP/s
+NRC_FWD_FLOP = 2 * (48 * 64 + 4 * 64 * 64 + 64 * 3)  # 39,296 per query
+NRC_TRAIN_FLOP = 3 * NRC_FWD_FLOP                   # forward, input gradients, weight gradients
+NRC_QUERY_BYTES = 64 + 16 * 8 * 8 + 16 + 32          # record; 16 levels x 8 corners x 2 f32; throughput; radiance
+
+
+def trace_roofline(prof, scene):
+    """B-model roofline of the extend and shadow kernels from a profile
+    (rrhip_session_get_profile) of the scene description."""
+    loose = 64 * scene.num_triangles + 32 * scene.num_spheres + 128 * scene.num_quadrics
+
+    def model(kind, state):
+        rays = prof[f"{kind}_rays"]
+        b = (state + loose) * rays + 64 * prof[f"{kind}_nodes"] + 48 * prof[f"{kind}_triangles"] + \
+            80 * prof[f"{kind}_instances"]
+        ms = prof[f"{kind}_ms"]
+        return {"rays": int(rays), "bytes_per_ray": round(b / max(rays, 1), 1),
+                "node_fetches_per_ray": round(prof[f"{kind}_nodes"] / max(rays, 1), 2),
+                "triangle_tests_per_ray": round(prof[f"{kind}_triangles"] / max(rays, 1), 2),
+                "instances_per_ray": round(prof[f"{kind}_instances"] / max(rays, 1), 2),
+                "ms": round(ms, 3), "gbs": round(b / (ms * 1e6), 1) if ms > 0 else None,
+                "frac": round(b / (ms * 1e6) / HBM_PEAK, 4) if ms > 0 else None, "bytes": b}
+    ext, sh = model("extend", EXTEND_STATE_BYTES), model("shadow", SHADOW_STATE_BYTES)
+    total_b, total_ms = ext.pop("bytes") + sh.pop("bytes"), prof["extend_ms"] + prof["shadow_ms"]
+    achieved = total_b / (total_ms * 1e6)
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+            "kernels": {"extend": ext, "shadow": sh, "shade_ms": round(prof["shade_ms"], 3)},
+            "note": "extend + shadow kernels over one profiled frame: B-model bytes (path state, node / "
+                    "triangle / instance fetches counted by the profiling kernels, loose objects) over "
+                    "their event-timed device time"}
+
+
+def profiled_frame(ps, params, start, count, nrc_config=None, ctx=None):
+    """One profiled (untimed) render of samples [start, start + count)."""
+    import rrhip
+    se = ps.session(params)
+    cache = None
+    if nrc_config is not None:
+        cache = rrhip.Nrc(ctx, nrc_config)
+        se.set_nrc(cache)
+        se.render(0, start)  # the cache at the state the timed runs reach there
+    se.set_profiling(True)
+    se.render(start, count)
+    prof = se.profile()
+    se.close()
+    if cache is not None:
+        cache.close()
+    return prof
 
 
 def box_room(ctx, args, with_cpu):
@@ -78,17 +171,105 @@ def box_room(ctx, args, with_cpu):
         "rays_per_frame": int(rays),
     }
     se.close()
+    out["roofline"] = trace_roofline(profiled_frame(ps, p, 0, 16), d)
     ps.close()
     if with_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import orc
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads = host_threads()
         t0 = time.perf_counter()
         _, crays = orc.render(d, p, 0, 16, threads=threads)
         cdt = time.perf_counter() - t0
         out["cpu_baseline"] = {"samples_per_s": round(256 * 256 * 16 / cdt, 1),
                                "mrays_per_s": round(crays / cdt / 1e6, 3), "cores": threads,
                                "kind": "port", "sample": f"one full frame, f64, {cdt:.2f} s"}
+    return out
+
+
+def instanced_pt(ctx, args, with_cpu):
+    """The instanced path-tracing workload at its configured size: 10,000
+    instances of a 20,480-triangle displaced icosphere (64 point lights
+    through the light BVH + one rect light), 1920x1080, 64 spp, depth 12,
+    f32 traversal, f64 accumulators; one frame = rrhip_render_samples over
+    64 samples of every pixel in passes of 8 (the launch mode picked by the
+    session autotuner, trials discarded). The roofline comes from a
+    separate profiled 8-spp render of the same frame's first samples."""
+    import rrhip
+    import scenes
+    W, H, SPP = 1920, 1080, 64
+    s = scenes.instanced_field(W, H)
+    d = s.desc()
+    ps = rrhip.PtScene(ctx, d)
+    p = rrhip.RenderParams.make(max_depth=12, samples_per_pass=8)
+    se = ps.session(p)
+    mode, tune_ms = se.autotune(8)
+    se.render(0, 8)  # warm-up pass
+    st0 = se.stats()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    se.render(8, SPP)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    st1 = se.stats()
+    se.close()
+    rays = st1["extension_rays"] + st1["shadow_rays"] - st0["extension_rays"] - st0["shadow_rays"]
+    out = {
+        "workload": "instanced field 1920x1080 (10,000 x 20,480-triangle instances, 64 point + 1 rect light), "
+                    "64 spp, depth 12, NEE + MIS + RR (wavefront, passes of 8 spp)",
+        "ms_per_frame": round(dt * 1e3, 1),
+        "device_ms_per_frame": round(st1["last_render_ms"], 1),
+        "mrays_per_s": round(rays / dt / 1e6, 1),
+        "rays_per_frame": int(rays),
+        "samples_per_s": round(W * H * SPP / dt, 1),
+        "launch_mode": mode,
+        "roofline": trace_roofline(profiled_frame(ps, p, 0, 8), d),
+    }
+    if with_cpu:
+        # the oracle's plain path tracer on the same scene at 480x270 and
+        # 2 spp, scaled by pixels and samples to the 64-spp 1080p frame;
+        # setup (scene load, BVH builds) measured on an 8x8 render and
+        # subtracted
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import orc
+        threads = host_threads()
+        small, tiny = scenes.instanced_field(480, 270), scenes.instanced_field(8, 8)
+        t1 = time.perf_counter()
+        orc.render(tiny.desc(), p, 0, 1, threads=threads)
+        setup = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        _, crays = orc.render(small.desc(), p, 0, 2, threads=threads)
+        cdt = max(time.perf_counter() - t1 - setup, 1e-6)
+        frame_s = cdt * (W * H) / (480 * 270) * SPP / 2
+        out["cpu_baseline"] = {
+            "value": round(crays / cdt / 1e6, 3), "unit": "Mrays/s", "frame_s": round(frame_s, 1),
+            "cores": threads, "kind": "port",
+            "sample": f"oracle plain PT, same scene at 480x270 x 2 spp ({cdt:.2f} s after {setup:.2f} s setup), "
+                      f"scaled by pixels and samples to {frame_s:.0f} s per 64-spp 1080p frame",
+        }
+    ps.close()
+    return out
+
+
+def cache_kernels(prof, frames_profiled=1):
+    """Per-frame radiance-cache kernel fractions from a profiled frame:
+    inference against the dense bf16 MFMA peak (39,296 FLOP per query) and
+    its hash-grid gathers against HBM (1,168 B per query: record, 16 levels
+    x 8 corners x 2 f32, throughput, radiance update); the optimiser steps
+    (117,888 FLOP per sample: forward, input and weight gradients)."""
+    q, n = prof["cache_queries"], prof["cache_samples"]
+    infer_s, steps_s = prof["cache_infer_ms"] / 1e3, prof["cache_steps_ms"] / 1e3
+    out = {"queries": int(q), "training_samples": int(n)}
+    if infer_s > 0:
+        out["inference"] = {"ms": round(prof["cache_infer_ms"], 3),
+                            "mfma_tflops": round(q * NRC_FWD_FLOP / infer_s / 1e12, 2),
+                            "mfma_frac": round(q * NRC_FWD_FLOP / infer_s / 1e12 / MFMA_BF16_PEAK, 5),
+                            "gather_gbs": round(q * NRC_QUERY_BYTES / infer_s / 1e9, 1),
+                            "gather_frac": round(q * NRC_QUERY_BYTES / infer_s / 1e9 / HBM_PEAK, 4)}
+    if steps_s > 0:
+        out["training"] = {"ms": round(prof["cache_steps_ms"], 3),
+                           "mfma_tflops": round(n * NRC_TRAIN_FLOP / steps_s / 1e12, 3),
+                           "mfma_frac": round(n * NRC_TRAIN_FLOP / steps_s / 1e12 / MFMA_BF16_PEAK, 6),
+                           "note": "the steps run on the cache's stream, overlapped with the next frame's tracing"}
     return out
 
 
@@ -176,6 +357,12 @@ def time_to_relmse(ctx, args, with_cpu):
     plain = run(None)
     nrc = run(rrhip.NrcConfig.make())
     ref.close()
+    # rooflines from profiled (untimed) frames: plain frame 0; the cached
+    # frame after 16 frames of training
+    plain["roofline"] = trace_roofline(profiled_frame(ps, params, 0, 4), d)
+    pn = profiled_frame(ps, params, 64, 4, nrc_config=rrhip.NrcConfig.make(), ctx=ctx)
+    nrc["roofline"] = trace_roofline(pn, d)
+    nrc["cache_kernels"] = cache_kernels(pn)
     out = {
         "workload": "instanced field 1920x1080 (10,000 x 20,480-triangle instances, 64 point + 1 rect light), "
                     "4 spp/frame, depth 12; relMSE eps 0.01 vs a 4096-spp plain render",
@@ -192,7 +379,7 @@ def time_to_relmse(ctx, args, with_cpu):
         # BVH builds) measured on an 8x8 render and subtracted.
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import orc
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads = host_threads()
         small_scene = scenes.instanced_field(480, 270)  # keeps the arrays the desc points at alive
         tiny_scene = scenes.instanced_field(8, 8)
         t1 = time.perf_counter()
@@ -267,7 +454,7 @@ def batched_jobs(ctx, args, with_cpu):
         # that sum scaled by each job's spp (cost is linear in samples).
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import orc
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads = host_threads()
         est, sample_s = 0.0, 0.0
         for (s, spp, d) in jobs:
             t1 = time.perf_counter()
@@ -318,6 +505,7 @@ def main():
     ap.add_argument("--no-pt", action="store_true", help="skip the box-room path-tracing line")
     ap.add_argument("--no-relmse", action="store_true", help="skip the time-to-relMSE comparison")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs comparison")
+    ap.add_argument("--no-instanced", action="store_true", help="skip the 64-spp instanced-field frame")
     ap.add_argument("--ref-spp", type=int, default=4096, help="spp of the relMSE reference render")
     ap.add_argument("--max-frames", type=int, default=400, help="frame cap of the time-to-relMSE runs")
     ap.add_argument("--mode", choices=("trace", "copy"), default="trace",
@@ -384,11 +572,19 @@ def main():
     node_bytes = 128 if tuning.get("width", 2) == 4 else 64  # Node4 or Node2 per visit
     alg_bytes = N_RAYS * (32 + 16) + node_bytes * nodes + 48 * tris
     achieved = alg_bytes / (kernel_ms / 1e3) / 1e9
-    traffic = None
+    # Counter figures of the committed profile, used only when it was taken
+    # on this workload at the launch configuration timed here (the file
+    # holds one entry per profiled configuration).
+    traffic, limiter = None, None
     if os.path.exists(TRAFFIC_FILE):
-        t = json.load(open(TRAFFIC_FILE))
-        if t.get("rays") == N_RAYS and t.get("triangles") == idx.shape[0]:
-            traffic = t.get("hbm_bytes_per_launch")
+        want = {k: tuning.get(k) for k in ("block", "stack", "rays_per_wave", "width")}
+        for t in json.load(open(TRAFFIC_FILE)).get("closest_hit", []):
+            if t.get("rays") == N_RAYS and t.get("triangles") == idx.shape[0] and t.get("tuning") == want:
+                traffic = t.get("hbm_bytes_per_launch")
+                limiter = {"resource": "VALU issue", "frac": t.get("valu_issue_frac"),
+                           "how": "SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x kernel cycles)",
+                           "fabric_gbs": round(traffic / (t["avg_duration_us"] * 1e3), 1),
+                           "l2_hit_rate": t.get("l2_hit_rate"), "source": t.get("source")}
 
     out = {
         "metric": "closest-hit Mrays/s",
@@ -420,12 +616,17 @@ def main():
                       "triangle_fetches_per_ray": round(tris / N_RAYS, 3),
                       "algorithmic_bytes_per_ray": round(alg_bytes / N_RAYS, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "note": "achieved = algorithmic node/triangle/ray bytes per launch / kernel time; "
+                             "the binding resource measured by counters is in 'limiter'"},
+        "limiter": limiter,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pos, idx, rays)
     if not args.no_pt:
         out["path_tracing"] = box_room(ctx, args, not args.no_cpu_baseline)
+    if not args.no_instanced:
+        out["instanced_path_tracing"] = instanced_pt(ctx, args, not args.no_cpu_baseline)
     if not args.no_batch:
         out["batched_jobs"] = batched_jobs(ctx, args, not args.no_cpu_baseline)
     if not args.no_relmse:

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define RRHIP_ABI_VERSION 5u
+#define RRHIP_ABI_VERSION 6u
 #define RRHIP_NO_HIT 0xFFFFFFFFu
 
 /* Status codes. render.rs returns anyhow::Result with a message
@@ -287,6 +287,15 @@ typedef struct rrhip_render_params {
   uint32_t samples_per_pass; /* samples of every pixel in flight at once; 0 = auto */
   double firefly_clamp;      /* per-sample luminance clamp (60); <= 0 disables */
   uint64_t sample_base;      /* added to the sample index when seeding PCG32; 0 = render.rs */
+  double ray_offset;         /* ray origins leaving a surface. 0 (default): error-bounded — the hit
+                                point is rebuilt from the surface (barycentrics, sphere reprojection),
+                                carries a bound on its f32 rounding error, and rays start outside
+                                that bound along the geometric normal, one ulp further (Pharr, Jakob,
+                                Humphreys, PBRT 3rd ed. sec. 3.9); shadow rays also stop short of the
+                                light by the sampled light point's bound. > 0: render.rs's fixed
+                                push of this length along the shading normal (1e-4,
+                                src/raytracer/pt/mod.rs:30, 976-977, 1212-1224), which falls below
+                                one ulp once coordinates pass ~1e3 */
 } rrhip_render_params;
 
 /* Per-pixel accumulator layout of rrhip_read_accum / orc_render:
@@ -505,6 +514,29 @@ int rrhip_session_set_film(rrhip_session* session, int enable);
 /* Copies the film (width x height x RRHIP_FILM_CHANNELS doubles) to `out`;
  * RRHIP_ERR_INVALID_ARGUMENT when film mode is off. */
 int rrhip_read_film(rrhip_session* session, double* out);
+/* Profiling: with enable != 0 every later pass launches counting variants
+ * of the trace kernels (same results) and times each bounce's kernels and
+ * the radiance cache's work with device events, waiting for each pass
+ * (graph replay is off meanwhile); the profile starts empty now. 0 turns
+ * it off. No counterpart in render.rs; it feeds the traversal bytes model
+ * of the path-tracing benchmarks. */
+typedef struct rrhip_pt_profile {
+  uint64_t extend_rays, extend_nodes, extend_triangles, extend_instances;
+  uint64_t shadow_rays, shadow_nodes, shadow_triangles, shadow_instances;
+  /* nodes: 64-byte TLAS / BLAS node fetches; triangles: 48-byte triangle
+   * records tested; instances: instances entered (64-byte record + 16-byte
+   * mesh record); loose objects are tested by every ray and not counted */
+  uint64_t cache_queries, cache_samples;
+  uint64_t passes, bounce_rounds;
+  double extend_ms, shade_ms, shadow_ms; /* summed device time (shade includes a queue sort) */
+  double cache_infer_ms;                 /* radiance-cache inference */
+  double cache_steps_ms;                 /* optimiser steps, first begin to last end (cache stream) */
+} rrhip_pt_profile;
+int rrhip_session_set_profiling(rrhip_session* session, int enable);
+/* The profile since profiling was enabled (RRHIP_ERR_INVALID_ARGUMENT
+ * when it is off). */
+int rrhip_session_get_profile(rrhip_session* session, rrhip_pt_profile* out);
+
 /* relMSE of `session`'s mean image against `reference`'s (same size and
  * context): mean over pixels and channels of (x - r)^2 / (r^2 + eps),
  * reduced in FP64 on the GPU. No counterpart in render.rs (its tests use

@@ -1397,16 +1397,16 @@ Vec3 Scene::visibility(const Vec3& origin0, const Vec3& dir, double remaining, P
         ms.push(m.interior);
       else
         ms.pop();
-      origin = h.point + dir * 1e-4;
-      remaining -= h.t + 1e-4;
+      origin = h.point + dir * ray_offset;
+      remaining -= h.t + ray_offset;
       continue;
     }
     const double presence = std::fmin(std::fmax(m.presence, 0.0), 1.0);
     if (presence >= 1.0) return Vec3::zero();
     vis = vis * (1.0 - presence);
     if (max_component(vis) < 1e-4) return Vec3::zero();
-    origin = h.point + dir * 1e-4;
-    remaining -= h.t + 1e-4;
+    origin = h.point + dir * ray_offset;
+    remaining -= h.t + ray_offset;
   }
   return vis;
 }
@@ -1427,7 +1427,7 @@ Vec3 Scene::area_contribution(const Vec3& p, const Vec3& n, const Vec3& sample_p
   // visibility_to (pt/mod.rs:1212-1224)
   Vec3 dir = sample_point - p;
   double dist = dir.length();
-  Vec3 vis = visibility(p + n * 1e-4, dir.normalize(), dist - 1e-3, rng, rays, medium, time);
+  Vec3 vis = visibility(p + n * ray_offset, dir.normalize(), dist - 1e-3, rng, rays, medium, time);
   if (max_component(vis) <= 0.0) return Vec3::zero();
   double pdf_sa = pick_pmf * dist2 / (cos_light * area);
   double weight = power_heuristic(pdf_sa, bsdf_pdf);
@@ -1447,7 +1447,7 @@ Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const
       eval(wi, &f, &pdf);
       if (max_component(f) <= 0.0) return Vec3::zero();
       Vec3 dir = light.position - p;
-      Vec3 vis = visibility(p + n * 1e-4, dir.normalize(), dir.length() - 1e-3, rng, rays, medium, time);
+      Vec3 vis = visibility(p + n * ray_offset, dir.normalize(), dir.length() - 1e-3, rng, rays, medium, time);
       if (max_component(vis) <= 0.0) return Vec3::zero();
       return f * light.radiance() * vis / (dist2 * pick_pmf);
     }
@@ -1467,7 +1467,7 @@ Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const
       double pdf;
       eval(wi, &f, &pdf);
       if (max_component(f) <= 0.0) return Vec3::zero();
-      Vec3 vis = visibility(p + n * 1e-4, wi, 1e8, rng, rays, medium, time);
+      Vec3 vis = visibility(p + n * ray_offset, wi, 1e8, rng, rays, medium, time);
       if (max_component(vis) <= 0.0) return Vec3::zero();
       return f * light.radiance() * vis / pick_pmf;
     }
@@ -1502,7 +1502,7 @@ Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const
       double dist = std::sqrt(dist2) * cos_t - std::sqrt(std::fmax(r * r - dist2 * sin_t * sin_t, 0.0));
       Vec3 target = p + wi * dist;
       Vec3 dir = target - p;
-      Vec3 vis = visibility(p + n * 1e-4, dir.normalize(), dir.length() - 1e-3, rng, rays, medium, time);
+      Vec3 vis = visibility(p + n * ray_offset, dir.normalize(), dir.length() - 1e-3, rng, rays, medium, time);
       if (max_component(vis) <= 0.0) return Vec3::zero();
       double pdf_sa = pick_pmf / std::fmax(2.0 * PI * (1.0 - cos_max), 1e-12);
       double weight = power_heuristic(pdf_sa, bsdf_pdf);
@@ -1528,7 +1528,7 @@ Vec3 Scene::sample_light(const Light& light, const Vec3& p, const Vec3& n, const
       double bsdf_pdf;
       eval(wi, &f, &bsdf_pdf);
       if (max_component(f) <= 0.0) return Vec3::zero();
-      Vec3 vis = visibility(p + n * 1e-4, wi, 1e8, rng, rays, medium, time);
+      Vec3 vis = visibility(p + n * ray_offset, wi, 1e8, rng, rays, medium, time);
       if (max_component(vis) <= 0.0) return Vec3::zero();
       double weight = power_heuristic(pdf_sa, bsdf_pdf);
       return f * rad * vis * (weight / pdf_sa);
@@ -1701,7 +1701,7 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays, A
         ms.push(mat.interior);
       else
         ms.pop();
-      ray = Ray::make(hit.point + ray.direction.normalize() * 1e-4, ray.direction);
+      ray = Ray::make(hit.point + ray.direction.normalize() * ray_offset, ray.direction);
       ray.time = time;
       continue;
     }
@@ -1723,7 +1723,7 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays, A
     const double presence = std::fmin(std::fmax(mat.presence, 0.0), 1.0);
     if (presence < 1.0 && rng.next_f64() >= presence && skips < 16) {
       skips++;
-      ray = Ray::make(hit.point + ray.direction.normalize() * 1e-4, ray.direction);
+      ray = Ray::make(hit.point + ray.direction.normalize() * ray_offset, ray.direction);
       ray.time = time;
       continue;
     }
@@ -1778,7 +1778,7 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays, A
       else
         ms.pop();
     }
-    double offset = s.transmitted ? -1e-4 : 1e-4;
+    double offset = s.transmitted ? -ray_offset : ray_offset;
     ray = Ray::make(hit.point + n * offset, wi_world);
     ray.time = time;
     depth++;

@@ -250,6 +250,7 @@ struct RenderParams {
   uint32_t rr_start = 3;
   double firefly_clamp = 60.0;
   uint64_t sample_base = 0;
+  double ray_offset = 1e-4;  // pt/mod.rs:30
 };
 
 class Scene {
@@ -269,6 +270,10 @@ class Scene {
   std::vector<Medium> media;
   int atmosphere = -1;
   bool has_motion = false;  // scene_builder.rs:239-241 (moving instances or camera)
+  // Push of ray origins off surfaces (pt/mod.rs:30: 1e-4). Tests set a
+  // negligible push to model error-bounded origins (f64 rounding of hit
+  // points near unit scale is ~1e-16).
+  double ray_offset = 1e-4;
 
   void load(const rrhip_scene_desc& d);
   bool intersect(const Ray& ray, Intersection* out) const;

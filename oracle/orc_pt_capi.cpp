@@ -58,6 +58,10 @@ static int render_list(const rrhip_scene_desc* desc, const rrhip_render_params* 
   rp.rr_start = params->rr_start;
   rp.firefly_clamp = params->firefly_clamp;
   rp.sample_base = params->sample_base;
+  // render.rs's 1e-4 push unless a test asks for another (the GPU's 0 means
+  // its error-bounded origins; the oracle keeps the reference's push then)
+  rp.ray_offset = params->ray_offset > 0.0 ? params->ray_offset : 1e-4;
+  scene.ray_offset = rp.ray_offset;
   const uint32_t W = scene.camera.width;
   if (!pixels) npix = (uint64_t)W * scene.camera.height;
   const uint64_t chunk = pixels ? 16 : W;  // whole rows for whole images
@@ -122,6 +126,10 @@ int orc_render_film(const rrhip_scene_desc* desc, const rrhip_render_params* par
   rp.rr_start = params->rr_start;
   rp.firefly_clamp = params->firefly_clamp;
   rp.sample_base = params->sample_base;
+  // render.rs's 1e-4 push unless a test asks for another (the GPU's 0 means
+  // its error-bounded origins; the oracle keeps the reference's push then)
+  rp.ray_offset = params->ray_offset > 0.0 ? params->ray_offset : 1e-4;
+  scene.ray_offset = rp.ray_offset;
   const uint32_t W = scene.camera.width, H = scene.camera.height;
   std::atomic<uint32_t> next_row{0};
   auto work = [&]() {
@@ -208,6 +216,10 @@ int orc_render_adaptive(const rrhip_scene_desc* desc, const rrhip_render_params*
   rp.rr_start = params->rr_start;
   rp.firefly_clamp = params->firefly_clamp;
   rp.sample_base = params->sample_base;
+  // render.rs's 1e-4 push unless a test asks for another (the GPU's 0 means
+  // its error-bounded origins; the oracle keeps the reference's push then)
+  rp.ray_offset = params->ray_offset > 0.0 ? params->ray_offset : 1e-4;
+  scene.ray_offset = rp.ray_offset;
   const uint32_t W = scene.camera.width, H = scene.camera.height;
   const uint32_t cap = std::max(max_spp, 1u);
   std::atomic<uint32_t> next_row{0};

@@ -359,6 +359,7 @@ hipError_t upload_pt(rrhip_scene* s, const PtSceneHost& h) {
   if (e == hipSuccess) e = upload_vec(h.env_marg, &s->pt_mem[17], &v.env_marg);
   if (e == hipSuccess) e = upload_vec(h.env_cond, &s->pt_mem[18], &v.env_cond);
   if (e == hipSuccess) e = upload_vec(h.quadrics, &s->pt_mem[19], &v.quadrics);
+  if (e == hipSuccess) e = upload_vec(h.inst_fwd, &s->pt_mem[20], &v.inst_fwd);
   v.num_quadrics = (uint32_t)h.quadrics.size();
   v.env_w = h.env_w;
   v.env_h = h.env_h;
@@ -550,7 +551,9 @@ int rrhip_scene_autotune(rrhip_scene* s, const rrhip_ray* rays, uint64_t n, int 
         float ms = 0.f;
         RRHIP_HIP_TRY(hipEventElapsedTime(&ms, s->ctx->ev0, s->ctx->ev1));
         ms /= reps;
-        if (!have || ms < chosen_ms) {
+        // a later candidate must win by more than 1 % (timing noise), so
+        // near-ties keep the earlier, smaller configuration
+        if (!have || ms < chosen_ms * 0.99f) {
           chosen = tu;
           chosen_ms = ms;
           have = true;

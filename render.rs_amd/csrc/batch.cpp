@@ -87,6 +87,7 @@ std::vector<std::vector<PtJob>> schedule(const rrhip_batch* b) {
       const rrhip_batch_job& J = b->jobs[j];
       PtJob t{};
       t.S = J.scene->pt;
+      t.S.ray_offset = (float)J.params.ray_offset;
       t.prm.max_depth = J.params.max_depth;
       t.prm.rr_start = J.params.rr_start;
       t.prm.firefly_clamp = (float)J.params.firefly_clamp;
@@ -124,6 +125,8 @@ int rrhip_batch_create(rrhip_ctx* ctx, uint32_t num_jobs, const rrhip_batch_job*
     if (J.params.max_depth == 0 || J.params.max_depth > 255)
       return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_batch_create: max_depth must be in 1..255");
     if (J.spp == 0) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_batch_create: spp must be > 0");
+    if (!(J.params.ray_offset >= 0.0 && J.params.ray_offset < 1e30))
+      return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_batch_create: ray_offset must be finite and >= 0");
     total_pix += (uint64_t)J.scene->pt.cam.width * J.scene->pt.cam.height;
   }
   uint32_t spp = samples_per_pass;
@@ -171,6 +174,8 @@ int rrhip_batch_create(rrhip_ctx* ctx, uint32_t num_jobs, const rrhip_batch_job*
   if (e == hipSuccess) e = alloc_n(b, &P.flags, n);
   if (e == hipSuccess) e = alloc_n(b, &P.hit_n, n);
   if (e == hipSuccess) e = alloc_n(b, &P.hit_m, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.hit_front, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.hit_back, n);
   if (e == hipSuccess) e = alloc_n(b, &P.q0, n);
   if (e == hipSuccess) e = alloc_n(b, &P.q1, n);
   if (e == hipSuccess) e = alloc_n(b, &P.sq, n);

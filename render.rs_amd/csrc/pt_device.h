@@ -20,7 +20,6 @@ namespace pt {
 
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kInv4Pi = 0.0795774715459476678f;
-constexpr float kRayOffset = 1e-4f;   // pt/mod.rs:30
 constexpr float kObjEps = 1e-6f;      // math/mod.rs:9 (object hits need t > EPSILON)
 
 struct V3 {
@@ -50,6 +49,52 @@ __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fm
 
 // Uniform in [0, 1) with 24 bits.
 __device__ __forceinline__ float rnd(Pcg32& r) { return r.next_f32(); }
+
+// ---- error-bounded ray origins -------------------------------------------------
+// The method of Pharr, Jakob and Humphreys, "Physically Based Rendering"
+// (3rd ed., 2016), sec. 3.9 "Managing Rounding Error": a hit point
+// computed in f32 carries a componentwise bound on its distance from the
+// exact surface; a ray leaving the surface starts outside that box, pushed
+// along the geometric normal by dot(|n|, bound) and then one ulp further
+// away from the surface in every component the push moved. render.rs moves
+// origins a fixed 1e-4 instead (pt/mod.rs:30), which falls below one ulp
+// once coordinates exceed ~1e3.
+
+// gamma(n) = n u / (1 - n u), u = 2^-24: the relative error bound of n
+// chained f32 operations.
+__device__ __forceinline__ constexpr float err_gamma(int n) {
+  return (float)n * 5.9604645e-8f / (1.f - (float)n * 5.9604645e-8f);
+}
+__device__ __forceinline__ V3 vabs(V3 a) { return {fabsf(a.x), fabsf(a.y), fabsf(a.z)}; }
+
+// The next representable float above / below x (x finite).
+__device__ __forceinline__ float float_up(float x) {
+  if (x == 0.f) return 1.4e-45f;
+  const int b = __float_as_int(x);
+  return __int_as_float(x > 0.f ? b + 1 : b - 1);
+}
+__device__ __forceinline__ float float_down(float x) {
+  if (x == 0.f) return -1.4e-45f;
+  const int b = __float_as_int(x);
+  return __int_as_float(x > 0.f ? b - 1 : b + 1);
+}
+
+// Origin of a ray leaving hit point p on the side of offset vector `off`
+// (the bound's push along the geometric normal, or its negation).
+__device__ __forceinline__ V3 offset_origin(V3 p, V3 off) {
+  V3 o = p + off;
+  o.x = off.x > 0.f ? float_up(o.x) : (off.x < 0.f ? float_down(o.x) : o.x);
+  o.y = off.y > 0.f ? float_up(o.y) : (off.y < 0.f ? float_down(o.y) : o.y);
+  o.z = off.z > 0.f ? float_up(o.z) : (off.z < 0.f ? float_down(o.z) : o.z);
+  return o;
+}
+
+// Push of a hit point with error bound `err` along unit geometric normal
+// ng, oriented to the side direction d came from.
+__device__ __forceinline__ V3 bound_push(V3 ng, V3 err, V3 d) {
+  const float k = dot(vabs(ng), err);
+  return ng * (dot(ng, d) < 0.f ? k : -k);
+}
 
 // bxdf.rs:16-39
 struct Frame {
@@ -464,7 +509,13 @@ struct Hit {
   V3 n;  // triangles: flipped to face the ray; spheres: outward
   uint32_t material;
   bool front;
+  uint32_t obj;  // loose object: kind (kObjTri / kObjSphere / kObjQuadric) << 30 | index
+  // Filled by scene_closest for the winning hit: the hit point rebuilt from
+  // the surface (barycentrics, sphere reprojection) and the push that puts
+  // an origin outside its rounding-error box, towards the ray's side.
+  V3 p, e;
 };
+constexpr uint32_t kObjTri = 0, kObjSphere = 1, kObjQuadric = 2, kObjIndex = 0x3FFFFFFFu;
 
 // extra.rs:673-720 (world-space, unit direction): t > EPSILON
 __device__ __forceinline__ bool hit_triangle(const PtTriangle& tr, V3 o, V3 d, float tmax, Hit& h) {
@@ -903,6 +954,7 @@ __device__ __forceinline__ bool closest_hit(const PtSceneView& S, V3 o, V3 d, Hi
     Hit h;
     if (hit_triangle(S.tris[i], o, d, tmax, h)) {
       best = h;
+      best.obj = (kObjTri << 30) | i;
       tmax = h.t;
       found = true;
     }
@@ -911,6 +963,7 @@ __device__ __forceinline__ bool closest_hit(const PtSceneView& S, V3 o, V3 d, Hi
     Hit h;
     if (hit_sphere(S.spheres[i], o, d, tmax, h)) {
       best = h;
+      best.obj = (kObjSphere << 30) | i;
       tmax = h.t;
       found = true;
     }
@@ -919,11 +972,46 @@ __device__ __forceinline__ bool closest_hit(const PtSceneView& S, V3 o, V3 d, Hi
     Hit h;
     if (hit_quadric(S.quadrics[i], o, d, tmax, h)) {
       best = h;
+      best.obj = (kObjQuadric << 30) | i;
       tmax = h.t;
       found = true;
     }
   }
   return found;
+}
+
+// Hit point and push of a loose-object hit `h` of ray (o, d) (see Hit):
+// triangles as v0 + e1 u + e2 v from the stored vertex and edges the
+// intersection test uses (the exact point lies on that plane for any u, v;
+// two products and two sums bound the rounding by gamma(3) of the summed
+// magnitudes), spheres reprojected onto the radius (gamma(5) of the offset
+// from the centre, plus the centre's addition), quadrics along the ray with
+// a wide bound covering their f32 root solve.
+__device__ __forceinline__ void loose_spawn(const PtSceneView& S, V3 o, V3 d, Hit& h) {
+  const uint32_t kind = h.obj >> 30, i = h.obj & kObjIndex;
+  if (kind == kObjTri) {
+    const PtTriangle& tr = S.tris[i];
+    const V3 v0 = v3(tr.v0), e1 = v3(tr.e1), e2 = v3(tr.e2);
+    const V3 pv = cross(d, e2);
+    const float inv = 1.f / dot(e1, pv);
+    const V3 tv = o - v0;
+    const float u = dot(tv, pv) * inv;
+    const float v = dot(d, cross(tv, e1)) * inv;
+    const V3 a = e1 * u, b = e2 * v;
+    h.p = v0 + a + b;
+    h.e = bound_push(v3(tr.n), (vabs(v0) + vabs(a) + vabs(b)) * err_gamma(3), d);
+  } else if (kind == kObjSphere) {
+    const PtSphere& sp = S.spheres[i];
+    const V3 c = v3(sp.c[0], sp.c[1], sp.c[2]);
+    V3 pl = (o - c) + d * h.t;
+    pl = pl * (sp.r / fmaxf(len(pl), 1e-30f));
+    h.p = c + pl;
+    h.e = bound_push(pl / sp.r, vabs(pl) * err_gamma(6) + vabs(c) * err_gamma(1), d);
+  } else {
+    const V3 td = d * h.t;
+    h.p = o + td;
+    h.e = bound_push(h.n, (vabs(o) + vabs(td)) * err_gamma(64), d);
+  }
 }
 
 // Opaque-scene shadow test (pt/mod.rs:1243-1306): blocked when anything is

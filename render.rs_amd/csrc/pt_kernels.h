@@ -31,9 +31,11 @@ hipError_t pt_launch_set_u64(uint64_t* dst, uint64_t v, hipStream_t st);
 // NEE shadow rays, BSDF continuation, Russian roulette) into `qnext`,
 // shadow rays, counter rotation. `capacity` bounds every queue. With
 // N.q set, paths terminate into radiance-cache queries (see NrcPathBuffers).
+// ev (profiling, may be null): four events recorded before extend, before
+// shade (the queue sort included), before shadow, and after it.
 hipError_t pt_launch_bounce(const PtSceneView& S, const PathBuffers& P, const PtParams& prm,
                             const uint32_t* q, uint32_t* qnext, uint32_t capacity, const NrcPathBuffers& N,
-                            hipStream_t st);
+                            hipStream_t st, const hipEvent_t* ev = nullptr);
 // One pass of a batch: raygen over every job's slots (jobs sorted by
 // slot_begin and pix_begin), `bounces` rounds of extend / shade / shadow
 // with each path reading its own job's scene and parameters, and the

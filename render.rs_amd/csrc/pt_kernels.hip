@@ -182,14 +182,19 @@ __global__ __launch_bounds__(kBlock) void raygen_kernel(PtSceneView S, PathBuffe
 
 // Closest hit of the path in `slot` into hit_n / hit_m (MOTION: at the
 // path's shutter time; QUAD: the scene holds quadrics).
-template <bool MOTION, bool QUAD>
-__device__ __forceinline__ void extend_one(const PtSceneView& S, const PathBuffers& P, uint32_t slot, uint32_t* stk) {
+template <bool MOTION, bool QUAD, bool COUNT = false>
+__device__ __forceinline__ void extend_one(const PtSceneView& S, const PathBuffers& P, uint32_t slot, uint32_t* stk,
+                                           TraceCount* tc = nullptr) {
   const float4 o4 = P.o[slot], d4 = P.d[slot];
   const float time = MOTION ? P.beta[slot].w : 0.f;
   Hit h;
-  if (scene_closest<kTraceBlock, MOTION, QUAD>(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), time, stk, h)) {
+  if (scene_closest<kTraceBlock, MOTION, QUAD, true, COUNT>(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), time,
+                                                             stk, h, tc)) {
     P.hit_n[slot] = make_float4(h.n.x, h.n.y, h.n.z, h.t);
     P.hit_m[slot] = h.material | (h.front ? 0x80000000u : 0u);
+    const V3 front = offset_origin(h.p, h.e), back = offset_origin(h.p, -h.e);
+    P.hit_front[slot] = make_float4(front.x, front.y, front.z, 0.f);
+    P.hit_back[slot] = make_float4(back.x, back.y, back.z, 0.f);
   } else {
     P.hit_n[slot] = make_float4(0.f, 0.f, 0.f, -1.f);
     P.hit_m[slot] = 0;
@@ -198,13 +203,29 @@ __device__ __forceinline__ void extend_one(const PtSceneView& S, const PathBuffe
 
 // STACK: LDS entries per lane, the smallest size >= the scene's bound
 // (fewer bytes per block, more resident waves).
-template <int STACK, bool MOTION, bool QUAD>
+// Profiling kernels: a wave's traversal fetches summed into prof[0..2].
+__device__ __forceinline__ void add_counts(unsigned long long* prof, const TraceCount& tc) {
+  uint32_t v[3] = {tc.nodes, tc.tris, tc.insts};
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
+  if ((threadIdx.x & 63u) == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) atomicAdd(prof + k, (unsigned long long)v[k]);
+  }
+}
+
+template <int STACK, bool MOTION, bool QUAD, bool COUNT = false>
 __global__ __launch_bounds__(kTraceBlock) void extend_kernel(PtSceneView S, PathBuffers P, const uint32_t* __restrict__ q) {
   __shared__ uint32_t s_stack[STACK * kTraceBlock];
   uint32_t* stk = s_stack + threadIdx.x;
   const uint32_t n = P.counts[0];
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) extend_one<MOTION, QUAD>(S, P, q[i], stk);
+  TraceCount tc;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    extend_one<MOTION, QUAD, COUNT>(S, P, q[i], stk, &tc);
+  if (COUNT) add_counts(P.prof, tc);
 }
 
 // pt/mod.rs:1000-1023
@@ -252,16 +273,21 @@ __device__ __forceinline__ void eval_world(const PtMaterial& m, const Frame& fr,
 }
 
 // sample_light + area_light_contribution (pt/mod.rs:1032-1209), without the
-// visibility test, which becomes the returned shadow ray.
-// `eval(wi, f, pdf)` gives f * cos (or the phase value) and the pdf of
-// the BSDF (or phase) sampling strategy for MIS; n = 0 for medium points.
-template <class Eval>
-__device__ __forceinline__ bool sample_light(const PtSceneView& S, const PtLight& L, V3 p, V3 n, const Eval& eval,
-                                             float pick, Pcg32& rng, ShadowOut& out) {
+// visibility test, which becomes the returned shadow ray from `origin()`
+// (evaluated last, to keep it out of the BSDF evaluation's registers; with
+// error-bounded origins p itself already lies outside its surface's
+// rounding box). `eval(wi, f, pdf)` gives f * cos (or the phase value) and
+// the pdf of the BSDF (or phase) sampling strategy for MIS. render.rs stops
+// shadow rays 1e-3 short of the light; with BOUNDED the segment also stops
+// short by the sampled light point's own rounding bound (projected on the
+// ray through the light's cosine).
+template <bool BOUNDED, class Eval>
+__device__ __forceinline__ bool sample_light(const PtSceneView& S, const PtLight& L, V3 p, V3 org,
+                                             const Eval& eval, float pick, Pcg32& rng, ShadowOut& out) {
   V3 f;
   float bsdf_pdf;
   const V3 rad = v3(L.radiance);
-  const V3 org = p + n * kRayOffset;
+  constexpr bool bounded = BOUNDED;
   switch (L.kind) {
     case kLightPoint: {
       const V3 to = v3(L.p) - p;
@@ -297,15 +323,18 @@ __device__ __forceinline__ bool sample_light(const PtSceneView& S, const PtLight
     case kLightRect:
     case kLightDisk: {
       const float u = rnd(rng), v = rnd(rng);
-      V3 sp;
+      V3 a, b;
       if (L.kind == kLightRect) {
-        sp = v3(L.p) + v3(L.e1) * u + v3(L.e2) * v;
+        a = v3(L.e1) * u;
+        b = v3(L.e2) * v;
       } else {
         const float r = sqrtf(u);
         float sn, cs;
         sincosf(2.f * kPi * v, &sn, &cs);
-        sp = v3(L.p) + v3(L.e1) * (r * cs) + v3(L.e2) * (r * sn);
+        a = v3(L.e1) * (r * cs);
+        b = v3(L.e2) * (r * sn);
       }
+      const V3 sp = v3(L.p) + a + b;
       const V3 to = sp - p;
       const float dist2 = fmaxf(dot(to, to), 1e-12f);
       const V3 wi = normalize(to);
@@ -318,6 +347,7 @@ __device__ __forceinline__ bool sample_light(const PtSceneView& S, const PtLight
       out.o = org;
       out.d = wi;
       out.tmax = len(to) - 1e-3f;
+      if (bounded) out.tmax -= L.err / fmaxf(cos_light, 1e-2f);
       out.c = f * rad * (w / pdf_sa);
       return true;
     }
@@ -342,6 +372,7 @@ __device__ __forceinline__ bool sample_light(const PtSceneView& S, const PtLight
       out.o = org;
       out.d = wi;
       out.tmax = dist - 1e-3f;
+      if (bounded) out.tmax -= L.err + sqrtf(dist2) * err_gamma(8);
       out.c = f * rad * (w / pdf_sa);
       return true;
     }
@@ -386,7 +417,9 @@ __device__ __forceinline__ bool sample_light(const PtSceneView& S, const PtLight
 // TR: the scene is transparent (medium stack, hulls, cutouts, shadow walks).
 // FILM: also the diffuse split of the radiance and the first real hit's
 // AOVs (pt/mod.rs:457-470, 559-573, 690-700, 959-961).
-template <bool NRC, bool TR, bool FILM = false>
+// FIXED: render.rs's fixed push of rays off surfaces (S.ray_offset > 0)
+// instead of error-bounded origins.
+template <bool NRC, bool TR, bool FILM = false, bool FIXED = false>
 __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffers& P, const PtParams& prm,
                                           const NrcPathBuffers& N, uint32_t slot, bool& cont, bool& shadow,
                                           bool& shadow2, ShadowOut& se, bool& query, ShadowOut& so, NrcQuery& nq,
@@ -397,6 +430,21 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
     Pcg32 rng = load_rng(P.rng[slot]);
     V3 beta = v3(b4.x, b4.y, b4.z), Lr = v3(l4.x, l4.y, l4.z);
     const V3 ro = v3(o4.x, o4.y, o4.z), rd = v3(d4.x, d4.y, d4.z);
+    // Surface hit point. With error-bounded origins it is the hit rebuilt
+    // from the surface and pushed outside its rounding box on the ray's
+    // side, which is also where reflected and shadow rays start; rays that
+    // cross the surface start from its twin on the far side. render.rs's
+    // fixed push takes the point along the ray.
+    constexpr bool bounded = !FIXED;
+    const auto hit_point = [&]() {
+      if (!bounded) return ro + rd * hn.w;
+      const float4 p4 = P.hit_front[slot];
+      return v3(p4.x, p4.y, p4.z);
+    };
+    const auto far_side = [&]() {
+      const float4 b4 = P.hit_back[slot];
+      return v3(b4.x, b4.y, b4.z);
+    };
     const float prev_pdf = o4.w;
     const V3 prev_origin = v3(pv4.x, pv4.y, pv4.z);
     const bool from_camera = (flags & kFlagFromCamera) != 0;
@@ -499,7 +547,7 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
             f = v3(ph, ph, ph);
             pdf = ph;
           };
-          if (sample_light(S, S.lights[seg_li], p, v3(0.f, 0.f, 0.f), eval, seg_pick, rng, so)) {
+          if (sample_light<bounded>(S, S.lights[seg_li], p, p, eval, seg_pick, rng, so)) {
             so.c = beta * so.c;
             so.medium = cur;
             if (!walk) so.c = so.c * medium_transmittance(md, so.o, so.d, so.tmax, rng);
@@ -548,7 +596,7 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
       const uint32_t hm = P.hit_m[slot];
       const PtMaterial m = S.materials[hm & 0x7FFFFFFFu];
       const bool entering = (hm >> 31) != 0;
-      const V3 p = ro + rd * hn.w;
+      const V3 p = hit_point();
       if (FILM && !aux_set) {
         if (TR && m.hull && skips < kMaxSkips) {
           travel += hn.w;  // a hull crossing is looked through (pt/mod.rs:671)
@@ -578,7 +626,7 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
         }
         if (passed) {
           skips++;
-          const V3 org = p + rd * kRayOffset;
+          const V3 org = bounded ? far_side() : p + rd * S.ray_offset;
           P.o[slot] = make_float4(org.x, org.y, org.z, prev_pdf);
           cont = true;
         }
@@ -590,7 +638,7 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
       const uint32_t hm = P.hit_m[slot];
       const PtMaterial m = S.materials[hm & 0x7FFFFFFFu];
       const bool entering = (hm >> 31) != 0;
-      const V3 p = ro + rd * hn.w;
+      const V3 p = hit_point();
       const V3 ng = v3(hn.x, hn.y, hn.z);
       const V3 wo = -rd;
       const V3 nrm = dot(ng, wo) >= 0.f ? ng : -ng;
@@ -668,7 +716,7 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
         float pick;
         if (ls_sample(S, p, rnd(rng), li, pick)) {
           const auto eval = [&](V3 wi, V3& f, float& pdf) { eval_world(m, fr, wo_l, eta, wi, f, pdf); };
-          if (sample_light(S, S.lights[li], p, nrm, eval, pick, rng, so)) {
+          if (sample_light<bounded>(S, S.lights[li], p, bounded ? p : p + nrm * S.ray_offset, eval, pick, rng, so)) {
             so.c = beta * so.c;
             so.medium = cur;
             // shadow rays through a medium carry its transmittance (or
@@ -685,7 +733,8 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
         const V3 wi = to_world(fr, bs.wi);
         beta = beta * bs.f * (fabsf(bs.wi.z) / bs.pdf);
         if (FILM && !(flags & kFlagFirstSet)) flags |= kFlagFirstSet | (bs.specular ? kFlagFirstSpec : 0u);
-        const V3 org = p + nrm * (bs.transmitted ? -kRayOffset : kRayOffset);
+        const V3 org = bounded ? (bs.transmitted ? far_side() : p)
+                               : p + nrm * (bs.transmitted ? -S.ray_offset : S.ray_offset);
         const V3 nd = normalize(wi);
         if (TR && bs.transmitted && m.interior >= 0) {  // pt/mod.rs:969-975
           if (entering)
@@ -759,7 +808,7 @@ __device__ __forceinline__ void push_shadow(const PathBuffers& P, bool act, uint
 
 // Three waves per SIMD (<= 168 VGPRs): shading is latency-bound on
 // scattered path-state loads, so occupancy matters more than registers.
-template <bool NRC, bool TR, bool FILM = false>
+template <bool NRC, bool TR, bool FILM = false, bool FIXED = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void shade_kernel(PtSceneView S, PathBuffers P, PtParams prm,
                                                       const uint32_t* __restrict__ q, uint32_t* __restrict__ qnext,
                                                       NrcPathBuffers N) {
@@ -776,7 +825,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     uint32_t slot = 0;
     if (act) {
       slot = q[i];
-      shade_one<NRC, TR, FILM>(S, P, prm, N, slot, cont, shadow, shadow2, se, query, so, nq, qbeta);
+      shade_one<NRC, TR, FILM, FIXED>(S, P, prm, N, slot, cont, shadow, shadow2, se, query, so, nq, qbeta);
     }
     push_shadow<TR, FILM>(P, act, slot, shadow, so, shadow2, se);
     const uint32_t nidx = wave_append(&P.counts[1], act && cont);
@@ -797,16 +846,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 // transmittance of the medium on top of the stack (the origin's medium
 // over the atmosphere); hulls push / pop their media, cutouts let 1 -
 // presence through, anything else blocks; at most 16 blockers.
-template <bool MOTION, bool QUAD>
+template <bool MOTION, bool QUAD, bool COUNT = false>
 __device__ __forceinline__ V3 walk_visibility(const PtSceneView& S, V3 o, V3 d, float time, float remaining,
-                                              int medium, Pcg32& rng, uint32_t* stk) {
+                                              int medium, Pcg32& rng, uint32_t* stk, TraceCount* tc = nullptr) {
   uint2 st = make_uint2(0u, 0u);
   uint32_t depth = 0;
   if (medium >= 0 && medium != S.atmosphere) stack_push(st, depth, (uint32_t)medium);
   V3 vis = v3(1.f, 1.f, 1.f);
   for (uint32_t i = 0; i < kMaxSkips; i++) {
     Hit h;
-    const bool hit = scene_closest<kTraceBlock, MOTION, QUAD>(S, o, d, time, stk, h);
+    const bool hit = scene_closest<kTraceBlock, MOTION, QUAD, true, COUNT>(S, o, d, time, stk, h, tc);
     const float seg = fminf(hit ? h.t : remaining, remaining);
     const int cur = stack_top(st, depth, S.atmosphere);
     if (cur >= 0) {
@@ -825,8 +874,14 @@ __device__ __forceinline__ V3 walk_visibility(const PtSceneView& S, V3 o, V3 d, 
       vis = vis * (1.f - m.presence);
       if (maxc(vis) < 1e-4f) return v3(0.f, 0.f, 0.f);
     }
-    o = o + d * (h.t + kRayOffset);
-    remaining -= h.t + kRayOffset;
+    if (S.ray_offset > 0.f) {
+      o = o + d * (h.t + S.ray_offset);
+      remaining -= h.t + S.ray_offset;
+    } else {  // past the blocker: its point pushed to the far side
+      const V3 on = offset_origin(h.p, -h.e);
+      remaining -= dot(on - o, d);
+      o = on;
+    }
   }
   return vis;
 }
@@ -834,8 +889,9 @@ __device__ __forceinline__ V3 walk_visibility(const PtSceneView& S, V3 o, V3 d, 
 // Shadow record i: adds each unoccluded ray's contribution to its path
 // (the first ray, then the second when the record has one); with WALK the
 // rays walk their blockers (transparent scenes), else one any-hit test.
-template <bool WALK, bool MOTION, bool QUAD>
-__device__ __forceinline__ void shadow_one_t(const PtSceneView& S, const PathBuffers& P, uint32_t i, uint32_t* stk) {
+template <bool WALK, bool MOTION, bool QUAD, bool COUNT = false>
+__device__ __forceinline__ void shadow_one_t(const PtSceneView& S, const PathBuffers& P, uint32_t i, uint32_t* stk,
+                                             TraceCount* tc = nullptr) {
   const float4 o = P.so[i], d = P.sd[i];
   float4 c = P.sc[i];
   const uint32_t cw = (uint32_t)c.w;  // bit 0: second ray, bit 1: diffuse AOV
@@ -847,23 +903,25 @@ __device__ __forceinline__ void shadow_one_t(const PtSceneView& S, const PathBuf
     const uint2 sd = P.srng[i];
     const uint64_t seed = ((uint64_t)sd.y << 32) | sd.x;
     Pcg32 r1 = Pcg32::make(seed, 1);
-    const V3 t1 = walk_visibility<MOTION, QUAD>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), time, o.w, (int)d.w, r1, stk);
+    const V3 t1 = walk_visibility<MOTION, QUAD, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), time, o.w, (int)d.w,
+                                                      r1, stk, tc);
     c = make_float4(c.x * t1.x, c.y * t1.y, c.z * t1.z, c.w);
     v1 = maxc(t1) > 0.f;
     if (two) {
       const float4 o2 = P.so2[i], d2 = P.sd2[i];
       Pcg32 r2 = Pcg32::make(seed, 2);
-      const V3 t2 = walk_visibility<MOTION, QUAD>(S, v3(o2.x, o2.y, o2.z), v3(d2.x, d2.y, d2.z), time, o2.w, (int)d2.w, r2,
-                                    stk);
+      const V3 t2 = walk_visibility<MOTION, QUAD, COUNT>(S, v3(o2.x, o2.y, o2.z), v3(d2.x, d2.y, d2.z), time, o2.w,
+                                                        (int)d2.w, r2, stk, tc);
       c2 = P.sc2[i];
       c2 = make_float4(c2.x * t2.x, c2.y * t2.y, c2.z * t2.z, 0.f);
       v2 = maxc(t2) > 0.f;
     }
   } else {
-    v1 = !scene_occluded<kTraceBlock, MOTION, QUAD>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), time, o.w, stk);
+    v1 = !scene_occluded<kTraceBlock, MOTION, QUAD, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), time, o.w, stk, tc);
     if (two) {
       const float4 o2 = P.so2[i], d2 = P.sd2[i];
-      v2 = !scene_occluded<kTraceBlock, MOTION, QUAD>(S, v3(o2.x, o2.y, o2.z), v3(d2.x, d2.y, d2.z), time, o2.w, stk);
+      v2 = !scene_occluded<kTraceBlock, MOTION, QUAD, COUNT>(S, v3(o2.x, o2.y, o2.z), v3(d2.x, d2.y, d2.z), time, o2.w,
+                                                            stk, tc);
       c2 = P.sc2[i];
     }
   }
@@ -895,27 +953,38 @@ __device__ __forceinline__ void shadow_one(const PtSceneView& S, const PathBuffe
     S.motion ? shadow_one_t<false, true, QUAD>(S, P, i, stk) : shadow_one_t<false, false, QUAD>(S, P, i, stk);
 }
 
-template <int STACK, bool WALK, bool MOTION, bool QUAD>
+template <int STACK, bool WALK, bool MOTION, bool QUAD, bool COUNT = false>
 __global__ __launch_bounds__(kTraceBlock) void shadow_kernel(PtSceneView S, PathBuffers P) {
   __shared__ uint32_t s_stack[STACK * kTraceBlock];
   uint32_t* stk = s_stack + threadIdx.x;
   const uint32_t n = P.counts[2];
   const uint32_t stride = gridDim.x * blockDim.x;
+  TraceCount tc;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    shadow_one_t<WALK, MOTION, QUAD>(S, P, i, stk);
+    shadow_one_t<WALK, MOTION, QUAD, COUNT>(S, P, i, stk, &tc);
+  if (COUNT) add_counts(P.prof + 3, tc);
 }
 
 // Launch of extend / shadow for the scene's stack size, transparency,
-// motion and quadrics.
+// motion and quadrics; with P.prof set, the counting variants.
+template <int STACK, bool MOTION, bool QUAD, bool COUNT>
+void launch_trace_counted(const PtSceneView& S, const PathBuffers& P, const uint32_t* q, uint32_t gt, bool extend,
+                          hipStream_t st) {
+  if (extend)
+    hipLaunchKernelGGL((extend_kernel<STACK, MOTION, QUAD, COUNT>), dim3(gt), dim3(kTraceBlock), 0, st, S, P, q);
+  else if (S.transparent)
+    hipLaunchKernelGGL((shadow_kernel<STACK, true, MOTION, QUAD, COUNT>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
+  else
+    hipLaunchKernelGGL((shadow_kernel<STACK, false, MOTION, QUAD, COUNT>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
+}
+
 template <int STACK, bool MOTION, bool QUAD>
 void launch_trace_kernels(const PtSceneView& S, const PathBuffers& P, const uint32_t* q, uint32_t gt, bool extend,
                           hipStream_t st) {
-  if (extend)
-    hipLaunchKernelGGL((extend_kernel<STACK, MOTION, QUAD>), dim3(gt), dim3(kTraceBlock), 0, st, S, P, q);
-  else if (S.transparent)
-    hipLaunchKernelGGL((shadow_kernel<STACK, true, MOTION, QUAD>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
+  if (P.prof)
+    launch_trace_counted<STACK, MOTION, QUAD, true>(S, P, q, gt, extend, st);
   else
-    hipLaunchKernelGGL((shadow_kernel<STACK, false, MOTION, QUAD>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
+    launch_trace_counted<STACK, MOTION, QUAD, false>(S, P, q, gt, extend, st);
 }
 
 // Scenes with quadrics take the full-stack kernels with the quadric solvers
@@ -1125,8 +1194,13 @@ __global__ __launch_bounds__(kBlock) void shade_batch_kernel(const PtJob* __rest
     if (act) {
       slot = q[i];
       const PtJob& J = jobs[P.job[slot]];
-      if (J.S.transparent)
+      const bool fixed = J.S.ray_offset > 0.f;
+      if (J.S.transparent && fixed)
+        shade_one<false, true, false, true>(J.S, P, J.prm, N, slot, cont, shadow, shadow2, se, query, so, nq, qbeta);
+      else if (J.S.transparent)
         shade_one<false, true>(J.S, P, J.prm, N, slot, cont, shadow, shadow2, se, query, so, nq, qbeta);
+      else if (fixed)
+        shade_one<false, false, false, true>(J.S, P, J.prm, N, slot, cont, shadow, shadow2, se, query, so, nq, qbeta);
       else
         shade_one<false, false>(J.S, P, J.prm, N, slot, cont, shadow, shadow2, se, query, so, nq, qbeta);
     }
@@ -1245,11 +1319,31 @@ hipError_t pt_launch_set_u64(uint64_t* dst, uint64_t v, hipStream_t st) {
   return hipGetLastError();
 }
 
+// The shade kernel for the pass's cache, film and transparency, with
+// render.rs's fixed origin push (FIXED) or error-bounded origins.
+template <bool FIXED>
+void launch_shade(const PtSceneView& S, const PathBuffers& P, const PtParams& prm, const uint32_t* qshade,
+                  uint32_t* qnext, const NrcPathBuffers& N, uint32_t g, hipStream_t st) {
+  if (N.q)
+    hipLaunchKernelGGL((shade_kernel<true, false, false, FIXED>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
+  else if (P.ld && S.transparent)
+    hipLaunchKernelGGL((shade_kernel<false, true, true, FIXED>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
+  else if (P.ld)
+    hipLaunchKernelGGL((shade_kernel<false, false, true, FIXED>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
+  else if (S.transparent)
+    hipLaunchKernelGGL((shade_kernel<false, true, false, FIXED>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
+  else
+    hipLaunchKernelGGL((shade_kernel<false, false, false, FIXED>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
+}
+
 hipError_t pt_launch_bounce(const PtSceneView& S, const PathBuffers& P, const PtParams& prm, const uint32_t* q,
-                            uint32_t* qnext, uint32_t capacity, const NrcPathBuffers& N, hipStream_t st) {
+                            uint32_t* qnext, uint32_t capacity, const NrcPathBuffers& N, hipStream_t st,
+                            const hipEvent_t* ev) {
   const uint32_t g = grid_for(capacity);
   const uint32_t gt = grid_for(capacity, kTraceBlock);
+  if (ev) (void)hipEventRecord(ev[0], st);
   launch_trace(S, P, q, gt, true, st);
+  if (ev) (void)hipEventRecord(ev[1], st);
   const uint32_t* qshade = q;
   if (P.qs) {  // group the queue by shading class so shade waves run one BSDF kind
     hipLaunchKernelGGL(sort_count_kernel, dim3(kSortBlocks), dim3(kBlock), 0, st, S, P, q);
@@ -1257,17 +1351,13 @@ hipError_t pt_launch_bounce(const PtSceneView& S, const PathBuffers& P, const Pt
     hipLaunchKernelGGL(sort_scatter_kernel, dim3(kSortBlocks), dim3(kBlock), 0, st, S, P, q);
     qshade = P.qs;
   }
-  if (N.q)
-    hipLaunchKernelGGL((shade_kernel<true, false>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
-  else if (P.ld && S.transparent)
-    hipLaunchKernelGGL((shade_kernel<false, true, true>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
-  else if (P.ld)
-    hipLaunchKernelGGL((shade_kernel<false, false, true>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
-  else if (S.transparent)
-    hipLaunchKernelGGL((shade_kernel<false, true>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
+  if (S.ray_offset > 0.f)
+    launch_shade<true>(S, P, prm, qshade, qnext, N, g, st);
   else
-    hipLaunchKernelGGL((shade_kernel<false, false>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
+    launch_shade<false>(S, P, prm, qshade, qnext, N, g, st);
+  if (ev) (void)hipEventRecord(ev[2], st);
   launch_trace(S, P, nullptr, gt, false, st);
+  if (ev) (void)hipEventRecord(ev[3], st);
   hipLaunchKernelGGL(rotate_kernel, dim3(1), dim3(1), 0, st, P);
   return hipGetLastError();
 }

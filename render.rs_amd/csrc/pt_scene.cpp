@@ -236,6 +236,7 @@ bool build_instances(const rrhip_scene_desc& d, PtSceneHost* out, std::string* e
   // Instances and their world bounds.
   std::vector<float> boxes(6 * (size_t)d.num_instances);
   std::vector<PtInstance> raw(d.num_instances);
+  std::vector<float> raw_fwd(12 * (size_t)d.num_instances);
   for (uint32_t i = 0; i < d.num_instances; i++) {
     const rrhip_instance_desc& in = d.instances[i];
     if (in.mesh >= d.num_meshes) return *err = "instance mesh out of range", false;
@@ -244,6 +245,7 @@ bool build_instances(const rrhip_scene_desc& d, PtSceneHost* out, std::string* e
     if (!affine_inverse(in.object_to_world, inv)) return *err = "instance transform is singular", false;
     PtInstance& g = raw[i];
     for (int k = 0; k < 12; k++) g.inv[k] = (float)inv[k];
+    for (int k = 0; k < 12; k++) raw_fwd[12 * (size_t)i + k] = (float)in.object_to_world[k];
     g.mesh = in.mesh;
     g.material = in.material;
     g.motion = 0xFFFFFFFFu;
@@ -291,7 +293,10 @@ bool build_instances(const rrhip_scene_desc& d, PtSceneHost* out, std::string* e
       out->hi[a] = std::max(out->hi[a], tlas.root_hi[a]);
     }
   }
-  for (uint32_t k : order) out->insts.push_back(raw[k]);
+  for (uint32_t k : order) {
+    out->insts.push_back(raw[k]);
+    out->inst_fwd.insert(out->inst_fwd.end(), raw_fwd.begin() + 12 * (size_t)k, raw_fwd.begin() + 12 * (size_t)k + 12);
+  }
   const int tlas_levels = tlas.nodes.empty() ? 0 : tlas.depth + 1;
   // Concatenate the BLASes, rebasing child references.
   out->meshes.resize(d.num_meshes);
@@ -381,6 +386,21 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
       default:
         put(g.p, d3(l.p));
         break;
+    }
+    // Rounding bound of the light points the kernels sample in f32 (the
+    // point p + e1 u + e2 v with u, v in [0, 1], or a sphere's surface
+    // point): gamma(4) of the summed magnitudes along the normal, gamma(8)
+    // of the centre and radius for spheres.
+    const auto gamma = [](int n) { return n * 0x1p-24 / (1.0 - n * 0x1p-24); };
+    if (l.kind == RRHIP_LIGHT_RECT || l.kind == RRHIP_LIGHT_DISK) {
+      double e = 0.0;
+      for (int a = 0; a < 3; a++)
+        e += std::fabs((double)g.n[a]) *
+             (std::fabs((double)g.p[a]) + std::fabs((double)g.e1[a]) + std::fabs((double)g.e2[a]));
+      g.err = std::nextafter((float)(gamma(4) * e), INFINITY);
+    } else if (l.kind == RRHIP_LIGHT_SPHERE) {
+      const double c = std::max({std::fabs((double)g.p[0]), std::fabs((double)g.p[1]), std::fabs((double)g.p[2])});
+      g.err = std::nextafter((float)(gamma(8) * (c + g.radius)), INFINITY);
     }
     out->lights.push_back(g);
   }

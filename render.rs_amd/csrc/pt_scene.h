@@ -32,6 +32,7 @@ struct PtSceneHost {
   uint32_t transparent = 0; // a material has an interior medium or presence < 1
   uint32_t motion = 0;      // moving instances or camera motion
   std::vector<PtInstMotion> inst_motion;
+  std::vector<float> inst_fwd;  // object-to-world of each instance in insts' order (12 floats)
   std::vector<PtMedium> media;                // grid pointers filled at upload
   std::vector<std::vector<float>> media_grids;
   int32_t atmosphere = -1;

@@ -23,6 +23,7 @@ using namespace rrhip;
 
 struct rrhip_session {
   rrhip_scene* scene = nullptr;
+  PtSceneView S{};  // the scene's view with this session's ray-origin rule
   PtParams prm{};
   uint64_t sample_base = 0;
   uint32_t spp_pass = 1;
@@ -63,6 +64,16 @@ struct rrhip_session {
   // accumulator (kFilmStride doubles per pixel) here.
   bool film = false;
   double* d_film = nullptr;
+  // Profiling (rrhip_session_set_profiling): counting trace kernels whose
+  // fetch totals land in prof_counts, events around each bounce's kernels
+  // and the cache's inference and optimiser steps, device times summed per
+  // pass; ray and cache counters are reported relative to prof_base.
+  bool profiling = false;
+  unsigned long long* prof_counts = nullptr;
+  std::vector<hipEvent_t> prof_ev;  // 4 per bounce round, then inference and step begin / end
+  unsigned long long prof_base[4] = {0, 0, 0, 0};
+  double prof_ms[5] = {0, 0, 0, 0, 0};  // extend, shade, shadow, cache inference, cache steps
+  uint64_t prof_passes = 0, prof_rounds = 0;
 };
 
 namespace {
@@ -96,9 +107,16 @@ void drop_graphs(rrhip_session* s) {
   s->graphs.clear();
 }
 
+void free_profile_events(rrhip_session* s) {
+  for (hipEvent_t e : s->prof_ev) (void)hipEventDestroy(e);
+  s->prof_ev.clear();
+}
+
 void free_session(rrhip_session* s) {
   drop_graphs(s);
   free_paths(s);
+  free_profile_events(s);
+  if (s->prof_counts) (void)hipFree(s->prof_counts);
   if (s->d_film) (void)hipFree(s->d_film);
   for (void* p : s->mem) (void)hipFree(p);
   s->mem.clear();
@@ -119,6 +137,8 @@ hipError_t alloc_paths(rrhip_session* s, uint64_t n) {
   if (e == hipSuccess) e = alloc_in(m, &P.flags, n);
   if (e == hipSuccess) e = alloc_in(m, &P.hit_n, n);
   if (e == hipSuccess) e = alloc_in(m, &P.hit_m, n);
+  if (e == hipSuccess) e = alloc_in(m, &P.hit_front, n);
+  if (e == hipSuccess) e = alloc_in(m, &P.hit_back, n);
   if (e == hipSuccess) e = alloc_in(m, &P.q0, n);
   if (e == hipSuccess) e = alloc_in(m, &P.q1, n);
   if (e == hipSuccess) e = alloc_in(m, &P.sq, n);
@@ -163,6 +183,8 @@ int rrhip_session_create(rrhip_scene* scene, const rrhip_render_params* params, 
   if (!scene->has_pt) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_create: scene has no camera");
   if (params->max_depth == 0 || params->max_depth > 255)
     return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_create: max_depth must be in 1..255");
+  if (!(params->ray_offset >= 0.0 && params->ray_offset < 1e30))
+    return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_create: ray_offset must be finite and >= 0");
   const uint64_t npix = (uint64_t)scene->pt.cam.width * scene->pt.cam.height;
   uint32_t spp = params->samples_per_pass;
   if (spp == 0) spp = (uint32_t)std::min<uint64_t>(32, std::max<uint64_t>(1, (4ull << 20) / npix));
@@ -172,6 +194,8 @@ int rrhip_session_create(rrhip_scene* scene, const rrhip_render_params* params, 
   auto* s = new (std::nothrow) rrhip_session();
   if (!s) return fail(RRHIP_ERR_OUT_OF_MEMORY, "rrhip_session_create: host allocation failed");
   s->scene = scene;
+  s->S = scene->pt;
+  s->S.ray_offset = (float)params->ray_offset;
   s->spp_pass = spp;
   s->slots = (uint64_t)spp * npix;
   s->sample_base = params->sample_base;
@@ -225,13 +249,29 @@ hipError_t enqueue_pass(rrhip_session* s, uint32_t n, uint32_t ncol, const uint3
   const uint32_t paths = total + N.ntrain;
   PathBuffers P = s->P;
   if (!(s->launch_mode & 2)) P.qs = nullptr;
-  hipError_t e = pt_launch_raygen(sc->pt, P, prm, total, N, sample0_dev, st);
   const uint32_t rounds = prm.max_depth + (sc->pt.transparent ? kPtMaxSkips : 0);
+  // profiling (never under graph capture): the counting kernels and events
+  const bool prof = s->profiling && !sample0_dev;
+  hipEvent_t* ev = nullptr;
+  hipError_t e = hipSuccess;
+  if (prof) {
+    P.prof = s->prof_counts;
+    while (e == hipSuccess && s->prof_ev.size() < 4 * (size_t)rounds + 4) {
+      hipEvent_t x = nullptr;
+      e = hipEventCreate(&x);
+      if (e == hipSuccess) s->prof_ev.push_back(x);
+    }
+    ev = s->prof_ev.data();
+  }
+  if (e == hipSuccess) e = pt_launch_raygen(s->S, P, prm, total, N, sample0_dev, st);
   for (uint32_t b = 0; b < rounds && e == hipSuccess; b++) {
     const uint32_t* q = (b & 1) ? P.q1 : P.q0;
     uint32_t* qn = (b & 1) ? P.q0 : P.q1;
-    e = pt_launch_bounce(sc->pt, P, prm, q, qn, paths, N, st);
+    e = pt_launch_bounce(s->S, P, prm, q, qn, paths, N, st, prof ? ev + 4 * b : nullptr);
   }
+  hipEvent_t* ev_nrc = prof ? ev + 4 * rounds : nullptr;
+  bool nrc_timed = false, steps_timed = false;
+  hipStream_t ts_used = st;
   if (nrc && e == hipSuccess) {
     // cache radiance at the terminal vertices (and training tails) once the
     // previous pass's optimiser steps are done, then this pass's training
@@ -240,7 +280,12 @@ hipError_t enqueue_pass(rrhip_session* s, uint32_t n, uint32_t ncol, const uint3
     // launch mode bit 2 or graph capture keeps them on this stream
     const bool overlap = !sample0_dev && !(s->launch_mode & 4);
     e = nrc_join(nrc, st);
+    if (ev_nrc && e == hipSuccess) e = hipEventRecord(ev_nrc[0], st);
     if (e == hipSuccess) e = nrc_launch_infer_paths(nrc_infer_view(nrc), N.q, &N.counts[0], paths, N.q_slot, N.q_beta, s->P.L, st);
+    if (ev_nrc && e == hipSuccess) {
+      e = hipEventRecord(ev_nrc[1], st);
+      nrc_timed = e == hipSuccess;
+    }
     if (e == hipSuccess)
       e = pt_launch_nrc_samples(s->P, N, prm.firefly_clamp, s->nrc_scan, s->nrc_bcap, s->nrc_bq, s->nrc_bt, st);
     const uint32_t batch = nrc->cfg.batch;
@@ -251,11 +296,17 @@ hipError_t enqueue_pass(rrhip_session* s, uint32_t n, uint32_t ncol, const uint3
       if (e == hipSuccess) e = hipStreamWaitEvent(nrc->train_st, nrc->ev_samples, 0);
       ts = nrc->train_st;
     }
+    if (ev_nrc && steps && e == hipSuccess) e = hipEventRecord(ev_nrc[2], ts);
     for (uint32_t k = 0; k < nrc->cfg.steps_per_pass && steps && e == hipSuccess; k++) {
       if ((uint64_t)k * batch >= s->nrc_bcap) break;
       e = pt_launch_nrc_set_batch(N, k, batch, &nrc->d_ctl->n, ts);
       if (e == hipSuccess)
         e = nrc_step_device(nrc, s->nrc_bq + (size_t)k * batch, s->nrc_bt + 3 * (size_t)k * batch, batch, ts);
+    }
+    if (ev_nrc && steps && e == hipSuccess) {
+      e = hipEventRecord(ev_nrc[3], ts);
+      steps_timed = e == hipSuccess;
+      ts_used = ts;
     }
     if (steps && overlap && e == hipSuccess) {
       e = hipEventRecord(nrc->ev_trained, ts);
@@ -263,6 +314,27 @@ hipError_t enqueue_pass(rrhip_session* s, uint32_t n, uint32_t ncol, const uint3
     }
   }
   if (e == hipSuccess) e = pt_launch_resolve(s->P, prm, s->d_accum, s->film ? s->d_film : nullptr, st);
+  if (prof && e == hipSuccess) {
+    // wait for the pass (and its optimiser steps) and sum the kernel times
+    e = hipStreamSynchronize(st);
+    if (e == hipSuccess && ts_used != st) e = hipStreamSynchronize(ts_used);
+    float ms = 0.f;
+    for (uint32_t b = 0; b < rounds && e == hipSuccess; b++)
+      for (int k = 0; k < 3 && e == hipSuccess; k++) {
+        e = hipEventElapsedTime(&ms, ev[4 * b + k], ev[4 * b + k + 1]);
+        s->prof_ms[k] += ms;
+      }
+    if (e == hipSuccess && nrc_timed) {
+      e = hipEventElapsedTime(&ms, ev_nrc[0], ev_nrc[1]);
+      s->prof_ms[3] += ms;
+    }
+    if (e == hipSuccess && steps_timed) {
+      e = hipEventElapsedTime(&ms, ev_nrc[2], ev_nrc[3]);
+      s->prof_ms[4] += ms;
+    }
+    s->prof_passes++;
+    s->prof_rounds += rounds;
+  }
   return e;
 }
 
@@ -331,7 +403,7 @@ int rrhip_render_samples(rrhip_session* s, uint32_t start, uint32_t count) {
         RRHIP_HIP_TRY(enqueue_pass(s, n, s->nlive, s->ad_live[s->ad_cur], sample0, nullptr, st));
       else
         RRHIP_HIP_TRY(enqueue_pass(s, n, s->prm.npix, nullptr, sample0, nullptr, st));
-    } else if ((s->launch_mode & 1) && !s->nrc) {
+    } else if ((s->launch_mode & 1) && !s->nrc && !s->profiling) {
       hipGraphExec_t exec = nullptr;
       RRHIP_HIP_TRY(pass_graph(s, n, st, &exec));
       RRHIP_HIP_TRY(pt_launch_set_u64(s->d_sample0, sample0, st));
@@ -593,6 +665,60 @@ int rrhip_pcg32_draws(rrhip_ctx* ctx, const uint64_t* pixels, const uint64_t* sa
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   (void)hipFree(d);
   if (e != hipSuccess) return fail_hip(e, "rrhip_pcg32_draws");
+  return RRHIP_OK;
+}
+
+int rrhip_session_set_profiling(rrhip_session* s, int enable) {
+  if (int st = check_session(s)) return st;
+  RRHIP_HIP_TRY(hipSetDevice(s->scene->ctx->device));
+  hipStream_t st = s->scene->ctx->stream;
+  RRHIP_HIP_TRY(hipStreamSynchronize(st));
+  if (s->nrc) RRHIP_HIP_TRY(nrc_drain(s->nrc));
+  if (!enable) {
+    s->profiling = false;
+    free_profile_events(s);
+    if (s->prof_counts) (void)hipFree(s->prof_counts);
+    s->prof_counts = nullptr;
+    return RRHIP_OK;
+  }
+  if (!s->prof_counts) RRHIP_HIP_TRY(hipMalloc(&s->prof_counts, 8 * sizeof(unsigned long long)));
+  RRHIP_HIP_TRY(hipMemsetAsync(s->prof_counts, 0, 8 * sizeof(unsigned long long), st));
+  RRHIP_HIP_TRY(hipMemcpyAsync(s->prof_base, s->P.stats, sizeof(s->prof_base), hipMemcpyDeviceToHost, st));
+  RRHIP_HIP_TRY(hipStreamSynchronize(st));
+  for (double& m : s->prof_ms) m = 0.0;
+  s->prof_passes = s->prof_rounds = 0;
+  s->profiling = true;
+  return RRHIP_OK;
+}
+
+int rrhip_session_get_profile(rrhip_session* s, rrhip_pt_profile* out) {
+  if (int st = check_session(s)) return st;
+  if (!out) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_get_profile: null output");
+  if (!s->profiling) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_get_profile: profiling is off");
+  RRHIP_HIP_TRY(hipSetDevice(s->scene->ctx->device));
+  hipStream_t st = s->scene->ctx->stream;
+  unsigned long long c[6], r[4];
+  RRHIP_HIP_TRY(hipMemcpyAsync(c, s->prof_counts, sizeof(c), hipMemcpyDeviceToHost, st));
+  RRHIP_HIP_TRY(hipMemcpyAsync(r, s->P.stats, sizeof(r), hipMemcpyDeviceToHost, st));
+  RRHIP_HIP_TRY(hipStreamSynchronize(st));
+  *out = rrhip_pt_profile{};
+  out->extend_rays = r[0] - s->prof_base[0];
+  out->extend_nodes = c[0];
+  out->extend_triangles = c[1];
+  out->extend_instances = c[2];
+  out->shadow_rays = r[1] - s->prof_base[1];
+  out->shadow_nodes = c[3];
+  out->shadow_triangles = c[4];
+  out->shadow_instances = c[5];
+  out->cache_queries = r[2] - s->prof_base[2];
+  out->cache_samples = r[3] - s->prof_base[3];
+  out->passes = s->prof_passes;
+  out->bounce_rounds = s->prof_rounds;
+  out->extend_ms = s->prof_ms[0];
+  out->shade_ms = s->prof_ms[1];
+  out->shadow_ms = s->prof_ms[2];
+  out->cache_infer_ms = s->prof_ms[3];
+  out->cache_steps_ms = s->prof_ms[4];
   return RRHIP_OK;
 }
 

@@ -31,6 +31,13 @@ struct MeshHit {
   float u, v;
 };
 
+// Fetches of one traversal, for the profiling kernels: 64-byte nodes
+// (TLAS and BLAS), 48-byte BLAS triangle records, instances entered (a
+// 64-byte instance record and its 16-byte mesh record).
+struct TraceCount {
+  uint32_t nodes = 0, tris = 0, insts = 0;
+};
+
 __device__ __forceinline__ V3 xform_point(const float* m, V3 p) {
   return {m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
           m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]};
@@ -121,9 +128,9 @@ __device__ __forceinline__ void instance_inverse(const PtSceneView& S, const PtI
 // Closest (ANY = false) or any (ANY = true) instance hit with t < tmax, for
 // a ray at shutter time `time` (MOTION: the scene has moving instances;
 // without it the time is ignored and the code is the static traversal).
-template <int STRIDE, bool ANY, bool MOTION>
+template <int STRIDE, bool ANY, bool MOTION, bool COUNT = false>
 __device__ __forceinline__ bool trace_instances(const PtSceneView& S, V3 o, V3 d, float time, float tmax,
-                                                uint32_t* stk, MeshHit& hit) {
+                                                uint32_t* stk, MeshHit& hit, TraceCount* tc = nullptr) {
   if (S.num_insts == 0) return false;
   const Ray32 world = make_ray32(o, d);
   Ray32 r = world;
@@ -133,6 +140,7 @@ __device__ __forceinline__ bool trace_instances(const PtSceneView& S, V3 o, V3 d
   bool have = true;
   for (;;) {
     if (have) {
+      if (COUNT) tc->nodes++;
       const Node2 nd = S.nodes[node];
       bool h0, h1;
       float t0, t1;
@@ -149,6 +157,7 @@ __device__ __forceinline__ bool trace_instances(const PtSceneView& S, V3 o, V3 d
         } else {
           for (uint32_t k = ref; k < ref + cnt; k++) {
             float t, u, v;
+            if (COUNT) tc->tris++;
             if (mesh_tri(S.btris[k], r, tmax, t, u, v)) {
               hit.t = t;
               hit.inst = inst;
@@ -185,6 +194,7 @@ __device__ __forceinline__ bool trace_instances(const PtSceneView& S, V3 o, V3 d
       inst = e & kInstFirst;
       const uint32_t more = (e >> 28) & 7u;  // instances of the leaf after this one
       if (more) stk[(sp++) * STRIDE] = kInstEntry | ((more - 1) << 28) | (inst + 1);
+      if (COUNT) tc->insts++;
       const PtInstance I = S.insts[inst];
       const uint32_t root = S.meshes[I.mesh].root;
       if (root == 0xFFFFFFFFu) continue;
@@ -237,26 +247,67 @@ __device__ __forceinline__ void instance_normal(const PtSceneView& S, const Mesh
   material = I.material;
 }
 
+// Hit point and push of an instance hit (see Hit): the object-space point
+// v0 + e1 u + e2 v on the plane mesh_tri intersects (its edges e = v - v0
+// rounded as there; gamma(3) of the summed magnitudes), mapped by the
+// instance's object-to-world transform with the transform's own rounding
+// (Pharr et al., PBRT 3rd ed., eq. 3.10-3.11), plus gamma(4) of the world
+// magnitudes for the world-to-object mapping the next ray's traversal
+// applies to its origin. The geometric normal is the cross product of the
+// mapped edges.
+template <bool MOTION>
+__device__ __forceinline__ void instance_spawn(const PtSceneView& S, const MeshHit& h, V3 d, float time, V3& p,
+                                               V3& e) {
+  const TriRec tr = S.btris[h.slot];
+  float m[12];
+  const PtInstance& I = S.insts[h.inst];
+  if (!MOTION || I.motion == 0xFFFFFFFFu || !(time > 0.f)) {
+    const float* f = S.inst_fwd + 12 * (size_t)h.inst;
+#pragma unroll
+    for (int k = 0; k < 12; k++) m[k] = f[k];
+  } else {
+    const PtInstMotion& M = S.inst_motion[I.motion];
+#pragma unroll
+    for (int k = 0; k < 12; k++) m[k] = M.fwd0[k] + (M.fwd1[k] - M.fwd0[k]) * time;
+  }
+  const V3 v0 = v3(tr.v0x, tr.v0y, tr.v0z);
+  const V3 e1 = v3(tr.v1x, tr.v1y, tr.v1z) - v0, e2 = v3(tr.v2x, tr.v2y, tr.v2z) - v0;
+  const V3 b = e1 * h.u, c = e2 * h.v;
+  const V3 po = v0 + b + c;
+  const V3 eo = (vabs(v0) + vabs(b) + vabs(c)) * err_gamma(3);
+  p = xform_point(m, po);
+  const float am[12] = {fabsf(m[0]), fabsf(m[1]), fabsf(m[2]),  fabsf(m[3]), fabsf(m[4]),  fabsf(m[5]),
+                        fabsf(m[6]), fabsf(m[7]), fabsf(m[8]), fabsf(m[9]), fabsf(m[10]), fabsf(m[11])};
+  const V3 tabs = v3(am[3], am[7], am[11]);
+  const V3 err = xform_vec(am, eo) * (1.f + err_gamma(3)) + xform_point(am, vabs(po)) * err_gamma(3) +
+                 (vabs(p) + tabs) * err_gamma(4);
+  const V3 ng = normalize(cross(xform_vec(m, e1), xform_vec(m, e2)));
+  e = bound_push(ng, err, d);
+}
+
 // Closest hit over loose objects then instances (QUAD: with quadrics).
-template <int STRIDE, bool MOTION, bool QUAD>
+// SPAWN: also the winning hit's point and push (Hit::p, Hit::e).
+template <int STRIDE, bool MOTION, bool QUAD, bool SPAWN = false, bool COUNT = false>
 __device__ __forceinline__ bool scene_closest(const PtSceneView& S, V3 o, V3 d, float time, uint32_t* stk,
-                                              Hit& best) {
+                                              Hit& best, TraceCount* tc = nullptr) {
   bool found = closest_hit<QUAD>(S, o, d, best);
   MeshHit mh;
-  if (trace_instances<STRIDE, false, MOTION>(S, o, d, time, found ? best.t : INFINITY, stk, mh)) {
+  if (trace_instances<STRIDE, false, MOTION, COUNT>(S, o, d, time, found ? best.t : INFINITY, stk, mh, tc)) {
     best.t = mh.t;
     instance_normal<MOTION>(S, mh, d, time, best.n, best.front, best.material);
-    found = true;
+    if (SPAWN) instance_spawn<MOTION>(S, mh, d, time, best.p, best.e);
+    return true;
   }
+  if (SPAWN && found) loose_spawn(S, o, d, best);
   return found;
 }
 
-template <int STRIDE, bool MOTION, bool QUAD>
+template <int STRIDE, bool MOTION, bool QUAD, bool COUNT = false>
 __device__ __forceinline__ bool scene_occluded(const PtSceneView& S, V3 o, V3 d, float time, float tmax,
-                                               uint32_t* stk) {
+                                               uint32_t* stk, TraceCount* tc = nullptr) {
   if (occluded<QUAD>(S, o, d, tmax)) return true;
   MeshHit mh;
-  return trace_instances<STRIDE, true, MOTION>(S, o, d, time, tmax, stk, mh);
+  return trace_instances<STRIDE, true, MOTION, COUNT>(S, o, d, time, tmax, stk, mh, tc);
 }
 
 }  // namespace pt

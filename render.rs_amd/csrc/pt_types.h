@@ -55,7 +55,8 @@ struct alignas(16) PtLight {
   float e2[3];
   float cos_max;      // distant: cos(angular_radius)
   float n[3];         // rect / disk normal
-  float pad;
+  float err;          // rounding bound of a sampled light point: along n (rect / disk), or of the
+                      // sphere's surface points; shadow rays with error-bounded origins stop short by it
 };
 
 struct LightNode {  // light_sampler.rs:19-29
@@ -176,6 +177,11 @@ struct PtSceneView {
                                 // a medium stack, shadow rays walk their blockers
   uint32_t motion;              // moving instances or camera: rays carry a shutter time
   const PtInstMotion* inst_motion;
+  const float* inst_fwd;        // object-to-world of each instance (3x4 row-major, TLAS leaf order)
+  // Ray origins: 0 = error-bounded (pushed outside the hit point's rounding
+  // box); > 0 = render.rs's fixed push of this length along the shading
+  // normal (pt/mod.rs:30, 1e-4). Set per session.
+  float ray_offset;
   // Dome environment map (null: constant dome): rgb texels, row-marginal
   // and per-row conditional CDFs of sin-weighted luminance, their total,
   // and W * H / (2 pi^2 total), the factor of a texel's solid-angle pdf.
@@ -206,6 +212,8 @@ struct PathBuffers {
   uint32_t* flags;  // depth (bits 0-7), from_camera (bit 8), training path (9), suffix (10)
   float4* hit_n;    // hit normal.xyz, t (t < 0: miss)
   uint32_t* hit_m;  // material | front_face << 31
+  float4* hit_front; // error-bounded origins: the hit rebuilt from the surface and pushed outside its
+  float4* hit_back;  //   rounding box on the ray's side (shading point), and on the far side
   uint32_t* q0;
   uint32_t* q1;
   uint32_t* sq;     // shadow queue: slot ids
@@ -229,6 +237,9 @@ struct PathBuffers {
   float4* aux0;     // albedo rgb, depth (0 until set)
   float4* aux1;     // shading normal facing the viewer xyz, distance travelled before the first real hit
   uint32_t* aux_id; // object id of the first real hit
+  // Profiling (null otherwise): traversal fetches of extend [0..2] and
+  // shadow [3..5] rays: nodes, triangle records, instances entered.
+  unsigned long long* prof;
 };
 
 // One radiance-cache query / training sample (the 64-byte rrhip_nrc_query

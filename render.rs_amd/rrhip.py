@@ -47,7 +47,7 @@ EXPORTED = [
     "rrhip_batch_get_stats", "rrhip_session_set_launch_mode", "rrhip_session_autotune",
     "rrhip_session_set_adaptive", "rrhip_session_average_spp",
     "rrhip_session_set_film", "rrhip_read_film", "rrhip_pcg32_draws",
-    "rrhip_trace_closest_noalloc",
+    "rrhip_trace_closest_noalloc", "rrhip_session_set_profiling", "rrhip_session_get_profile",
 ]
 
 
@@ -158,12 +158,16 @@ class SceneDesc(ctypes.Structure):
 class RenderParams(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
                 ("rr_start", ctypes.c_uint32), ("samples_per_pass", ctypes.c_uint32),
-                ("firefly_clamp", ctypes.c_double), ("sample_base", ctypes.c_uint64)]
+                ("firefly_clamp", ctypes.c_double), ("sample_base", ctypes.c_uint64),
+                ("ray_offset", ctypes.c_double)]
 
     @staticmethod
-    def make(max_depth=10, rr_start=3, samples_per_pass=0, firefly_clamp=60.0, sample_base=0):
+    def make(max_depth=10, rr_start=3, samples_per_pass=0, firefly_clamp=60.0, sample_base=0, ray_offset=0.0):
+        """ray_offset 0: error-bounded ray origins (the default); > 0:
+        render.rs's fixed push along the shading normal (1e-4). The oracle
+        always follows render.rs (1e-4)."""
         return RenderParams(ctypes.sizeof(RenderParams), max_depth, rr_start, samples_per_pass,
-                            firefly_clamp, sample_base)
+                            firefly_clamp, sample_base, ray_offset)
 
 
 class BatchJob(ctypes.Structure):
@@ -177,6 +181,17 @@ class SessionStats(ctypes.Structure):
                 ("samples_per_pass", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
                 ("last_render_ms", ctypes.c_double), ("cache_queries", ctypes.c_uint64),
                 ("cache_samples", ctypes.c_uint64)]
+
+
+class PtProfile(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in (
+        "extend_rays", "extend_nodes", "extend_triangles", "extend_instances",
+        "shadow_rays", "shadow_nodes", "shadow_triangles", "shadow_instances",
+        "cache_queries", "cache_samples", "passes", "bounce_rounds")] + \
+        [(k, ctypes.c_double) for k in ("extend_ms", "shade_ms", "shadow_ms", "cache_infer_ms", "cache_steps_ms")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class NrcConfig(ctypes.Structure):
@@ -275,6 +290,8 @@ def load():
         "rrhip_read_accum": (i32, [vp, vp]),
         "rrhip_load_accum": (i32, [vp, vp]),
         "rrhip_session_get_stats": (i32, [vp, ctypes.POINTER(SessionStats)]),
+        "rrhip_session_set_profiling": (i32, [vp, i32]),
+        "rrhip_session_get_profile": (i32, [vp, ctypes.POINTER(PtProfile)]),
         "rrhip_relmse": (i32, [vp, vp, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]),
         "rrhip_pcg32_draws": (i32, [vp, vp, vp, u32, u32, vp]),
         "rrhip_nrc_create": (i32, [vp, ctypes.POINTER(NrcConfig), ctypes.POINTER(vp)]),
@@ -581,6 +598,16 @@ class Session:
         st = SessionStats()
         _check(load().rrhip_session_get_stats(self.h, ctypes.byref(st)))
         return {f: getattr(st, f) for f, _ in SessionStats._fields_ if f != "reserved"}
+
+    def set_profiling(self, enable=True):
+        """Counting trace kernels and per-kernel device times from now on
+        (the profile starts empty); False turns it off."""
+        _check(load().rrhip_session_set_profiling(self.h, int(bool(enable))))
+
+    def profile(self):
+        p = PtProfile()
+        _check(load().rrhip_session_get_profile(self.h, ctypes.byref(p)))
+        return p.as_dict()
 
 
 class Scene:

@@ -236,7 +236,7 @@ class Scene:
         return self.camera.width * self.camera.height
 
 
-def box_room(width=256, height=256, emission=15.0):
+def box_room(width=256, height=256, emission=15.0, offset=(0.0, 0.0, 0.0), instanced=False, light_y=0.99):
     """The box-room workload: a closed room spanning [-1,1]^3 of 10
     axis-aligned quads (floor, ceiling, back and front walls each split in
     two halves at x = 0, left and right walls whole: 20 triangles), walls
@@ -246,26 +246,52 @@ def box_room(width=256, height=256, emission=15.0):
     with emission 15, whose quad is also a white matte emitter surface (as
     render.rs's AreaLightSource + Polygon, scene_builder.rs:924-936). Camera
     inside the room at z = -0.9 looking at +z, 60 degree vertical fov, box
-    filter of width 1."""
-    s = Scene(width, height, 60.0, eye=(0, 0, -0.9), look_at=(0, 0, 1), up=(0, 1, 0))
+    filter of width 1.
+
+    offset moves the whole scene (geometry, light, camera); instanced puts
+    the walls and the light quad in one mesh placed by an instance whose
+    transform carries the offset (the spheres stay loose); light_y moves
+    the light (and its quad) below the ceiling."""
+    o = np.asarray(offset, np.float64)
+
+    def at(p):
+        return tuple(np.asarray(p, np.float64) + o)
+
+    s = Scene(width, height, 60.0, eye=at((0, 0, -0.9)), look_at=at((0, 0, 1)), up=(0, 1, 0))
     white = s.add_material(matte((0.75, 0.75, 0.75)))
     red = s.add_material(matte((0.75, 0.1, 0.1)))
     green = s.add_material(matte((0.1, 0.75, 0.1)))
     li = s.add_light(light(rrhip.LIGHT_RECT, intensity=emission, color=(1, 1, 1),
-                           p=(-0.25, 0.99, -0.25), e1=(0.5, 0, 0), e2=(0, 0, 0.5)))
+                           p=at((-0.25, light_y, -0.25)), e1=(0.5, 0, 0), e2=(0, 0, 0.5)))
     emitter = s.add_material(matte((1, 1, 1), emission=(emission,) * 3, area_light=li))
+    quads = []
     for x0, x1 in ((-1, 0), (0, 1)):
-        s.add_polygon([(x0, -1, -1), (x1, -1, -1), (x1, -1, 1), (x0, -1, 1)], white)   # floor
-        s.add_polygon([(x0, 1, -1), (x0, 1, 1), (x1, 1, 1), (x1, 1, -1)], white)       # ceiling
-        s.add_polygon([(x0, -1, 1), (x1, -1, 1), (x1, 1, 1), (x0, 1, 1)], white)       # back
-        s.add_polygon([(x0, -1, -1), (x0, 1, -1), (x1, 1, -1), (x1, -1, -1)], white)   # front
-    s.add_polygon([(-1, -1, -1), (-1, -1, 1), (-1, 1, 1), (-1, 1, -1)], red)    # left
-    s.add_polygon([(1, -1, -1), (1, 1, -1), (1, 1, 1), (1, -1, 1)], green)      # right
+        quads.append(([(x0, -1, -1), (x1, -1, -1), (x1, -1, 1), (x0, -1, 1)], white))   # floor
+        quads.append(([(x0, 1, -1), (x0, 1, 1), (x1, 1, 1), (x1, 1, -1)], white))       # ceiling
+        quads.append(([(x0, -1, 1), (x1, -1, 1), (x1, 1, 1), (x0, 1, 1)], white))       # back
+        quads.append(([(x0, -1, -1), (x0, 1, -1), (x1, 1, -1), (x1, -1, -1)], white))   # front
+    quads.append(([(-1, -1, -1), (-1, -1, 1), (-1, 1, 1), (-1, 1, -1)], red))    # left
+    quads.append(([(1, -1, -1), (1, 1, -1), (1, 1, 1), (1, -1, 1)], green))      # right
     # The light quad (corner, corner+e1, corner+e1+e2, corner+e2).
-    s.add_polygon([(-0.25, 0.99, -0.25), (0.25, 0.99, -0.25), (0.25, 0.99, 0.25), (-0.25, 0.99, 0.25)],
-                  emitter)
-    s.add_sphere((-0.4, -0.7, 0.2), 0.3, white)
-    s.add_sphere((0.4, -0.65, -0.3), 0.35, white)
+    y = light_y
+    quads.append(([(-0.25, y, -0.25), (0.25, y, -0.25), (0.25, y, 0.25), (-0.25, y, 0.25)], emitter))
+    if instanced:
+        # one mesh per material (an instance carries one material)
+        xf = np.hstack([np.eye(3), o[:, None]])
+        for mat in (white, red, green, emitter):
+            pos, idx = [], []
+            for verts, m in quads:
+                if m != mat:
+                    continue
+                b = len(pos)
+                pos += verts
+                idx += [(b, b + i, b + i + 1) for i in range(1, len(verts) - 1)]
+            s.add_instance(s.add_mesh(np.asarray(pos, np.float32), np.asarray(idx, np.uint32)), mat, xf)
+    else:
+        for verts, m in quads:
+            s.add_polygon([at(v) for v in verts], m)
+    s.add_sphere(at((-0.4, -0.7, 0.2)), 0.3, white)
+    s.add_sphere(at((0.4, -0.65, -0.3)), 0.35, white)
     return s
 
 

@@ -272,14 +272,16 @@ RAY_OFFSET = 1e-4    # render.rs pt/mod.rs:30
 SHADOW_MARGIN = 1e-3  # visibility_to's tmax = distance - 1e-3 (pt/mod.rs:1212-1224)
 
 
-def _clip_loss(c, R, p, integrand, nq=128, nphi=256):
+def _clip_loss(c, R, p, integrand, nq=128, nphi=256, offset=RAY_OFFSET):
     """The share of the cone from p that render.rs's shadow rays lose to
     the light's own sphere: the ray to the cone sample at distance s_t
-    leaves from p + RAY_OFFSET n (n = +y) and runs s_t - SHADOW_MARGIN; it
-    is blocked when it enters the sphere before that. A shift of the
-    origin moves the entry by RAY_OFFSET (n.N) / |wi.N| (N the sphere
-    normal there), so only |wi.N| < 0.1 can block: q = 1 - (b/R)^2 < 0.012
-    for the impact parameter b. Midpoint rule over q and phi."""
+    leaves from p + offset n (n = +y; render.rs: RAY_OFFSET) and runs
+    s_t - SHADOW_MARGIN; it is blocked when it enters the sphere before
+    that. A shift of the origin moves the entry by offset (n.N) / |wi.N|
+    (N the sphere normal there), so only |wi.N| < 0.1 can block: q = 1 -
+    (b/R)^2 < 0.012 for the impact parameter b. Midpoint rule over q and
+    phi. Origins on the surface itself (offset 0, as error-bounded origins
+    are up to a push of ~1e-7 here) lose nothing."""
     d = c - p
     dist = np.linalg.norm(d)
     axis = d / dist
@@ -292,7 +294,7 @@ def _clip_loss(c, R, p, integrand, nq=128, nphi=256):
     ct = np.sqrt(1 - st * st)
     wi = (st * np.cos(PH))[..., None] * t1 + (st * np.sin(PH))[..., None] * t2 + ct[..., None] * axis
     s_t = dist * ct - np.sqrt(np.maximum(R * R - dist * dist * st * st, 0.0))
-    o2 = p + np.array([0.0, RAY_OFFSET, 0.0])
+    o2 = p + np.array([0.0, offset, 0.0])
     oc = c - o2
     B = wi @ oc
     C = oc @ oc - R * R
@@ -304,10 +306,11 @@ def _clip_loss(c, R, p, integrand, nq=128, nphi=256):
     return (dw * blocked * integrand(wi)).sum()
 
 
-def sphere_nee_arm(light, pts, n=64, heuristic="power", clip=False):
+def sphere_nee_arm(light, pts, n=64, heuristic="power", clip=False, offset=RAY_OFFSET):
     """The NEE arm of a sphere light: uniform cone sampling, p_l = 1 / (2 pi
-    (1 - cos_max)), integrated over the cone; with `clip`, less what
-    render.rs's shadow rays lose to the light's own sphere (_clip_loss)."""
+    (1 - cos_max)), integrated over the cone; with `clip`, less what shadow
+    rays leaving from p + offset n lose to the light's own sphere
+    (_clip_loss; render.rs: offset RAY_OFFSET)."""
     out = np.zeros((pts.shape[0], 3))
     for k in range(pts.shape[0]):
         d = light.c - pts[k]
@@ -319,16 +322,17 @@ def sphere_nee_arm(light, pts, n=64, heuristic="power", clip=False):
             cos = np.maximum(wi[..., 1], 0.0)
             return cos * _weight(pl, cos / np.pi, heuristic)
         v = _cone_integral(light.c, light.R, pts[k], integrand, n)
-        if clip:
-            v -= _clip_loss(light.c, light.R, pts[k], integrand)
+        if clip and offset > 0:
+            v -= _clip_loss(light.c, light.R, pts[k], integrand, offset=offset)
         out[k] = v * light.L
     return RHO / np.pi * out
 
 
-def sphere_bsdf_arm(light, pts, n=64):
+def sphere_bsdf_arm(light, pts, n=64, offset=RAY_OFFSET):
     """The emitter-hit arm of a sphere light: BSDF rays leave from p +
-    RAY_OFFSET n, so they meet the sphere over the cone seen from there,
-    weighted by the power heuristic against the light pdf from p."""
+    offset n (render.rs: RAY_OFFSET), so they meet the sphere over the cone
+    seen from there, weighted by the power heuristic against the light pdf
+    from p."""
     out = np.zeros((pts.shape[0], 3))
     for k in range(pts.shape[0]):
         d = light.c - pts[k]
@@ -340,15 +344,16 @@ def sphere_bsdf_arm(light, pts, n=64):
             cos = np.maximum(wi[..., 1], 0.0)
             pb = cos / np.pi
             return cos * pb * pb / (pb * pb + pl * pl)
-        o2 = pts[k] + np.array([0.0, RAY_OFFSET, 0.0])
+        o2 = pts[k] + np.array([0.0, offset, 0.0])
         out[k] = _cone_integral(light.c, light.R, o2, integrand, n) * light.L
     return RHO / np.pi * out
 
 
-def sphere_total(light, pts):
+def sphere_total(light, pts, offset=RAY_OFFSET):
     """Both strategies of a sphere light with its emitter sphere, as
-    render.rs's estimator computes them (offset origins included)."""
-    return sphere_nee_arm(light, pts, clip=True) + sphere_bsdf_arm(light, pts)
+    render.rs's estimator computes them with origins pushed by `offset`
+    (render.rs: RAY_OFFSET; error-bounded origins: 0)."""
+    return sphere_nee_arm(light, pts, clip=True, offset=offset) + sphere_bsdf_arm(light, pts, offset=offset)
 
 
 # ---- statistics ----------------------------------------------------------------

@@ -10,30 +10,44 @@ and makes these z-scores too small, so the oracle side of every GPU
 comparison is rendered with `oracle_render`, whose sample indices are
 offset by ORACLE_SAMPLE_BASE: PCG32 seeds mix the full 64-bit sample index
 (sampler.rs:24-32), so the two renders draw independent streams.
+
+Ray origins: the GPU's default error-bounded origins start rays on the
+surface up to a push far below render.rs's fixed 1e-4 (pt/mod.rs:30). That
+fixed push has effects of its own — it darkens the box room by 0.33 %
+(the same seeds rendered by the oracle with pushes of 1e-4 and 1e-9) —
+so a GPU render with error-bounded origins is compared with the oracle
+pushing by EXACT_ORIGIN_PUSH (f64 hit points near unit scale are exact to
+~1e-16, so 1e-9 only keeps the reference's t > 1e-9 / 1e-6 tests from
+seeing the surface just left), and a GPU render with render.rs's push
+(params.ray_offset > 0) with the oracle pushing by the same amount.
 """
 import numpy as np
 
 ORACLE_SAMPLE_BASE = 1 << 48
+EXACT_ORIGIN_PUSH = 1e-9
+
+
+def _oracle_params(params):
+    import rrhip
+    p = rrhip.RenderParams.from_buffer_copy(params)
+    p.sample_base = params.sample_base + ORACLE_SAMPLE_BASE
+    p.ray_offset = params.ray_offset if params.ray_offset > 0 else EXACT_ORIGIN_PUSH
+    return p
 
 
 def oracle_render(desc, params, start, count, **kw):
     """orc.render over samples [start, start + count) seeded independently
-    of a GPU render of the same range."""
+    of a GPU render of the same range, with the GPU render's origin rule."""
     import orc
-    import rrhip
-    p = rrhip.RenderParams.from_buffer_copy(params)
-    p.sample_base = params.sample_base + ORACLE_SAMPLE_BASE
-    return orc.render(desc, p, start, count, **kw)
+    return orc.render(desc, _oracle_params(params), start, count, **kw)
 
 
 def oracle_render_pixels(desc, params, pixels, start, count, **kw):
     """orc.render_pixels (a tile at its whole-image seeds) seeded
-    independently of a GPU render of the same range."""
+    independently of a GPU render of the same range, with the GPU render's
+    origin rule."""
     import orc
-    import rrhip
-    p = rrhip.RenderParams.from_buffer_copy(params)
-    p.sample_base = params.sample_base + ORACLE_SAMPLE_BASE
-    return orc.render_pixels(desc, p, pixels, start, count, **kw)
+    return orc.render_pixels(desc, _oracle_params(params), pixels, start, count, **kw)
 
 
 def mean_var(acc):

@@ -5,7 +5,13 @@ meets in tests/test_oracle_direct.py: a rect, a disk and a sphere light
 1 against its power-heuristic share), point and distant lights (exact per
 pixel up to f32 rounding), and three lights of different kinds under the
 light BVH's pick pmf. Each render must also reject a 2 % error and the
-balance heuristic, so the bars can see those mistakes."""
+balance heuristic, so the bars can see those mistakes.
+
+Sphere lights are rendered twice: with the default error-bounded ray
+origins (expected: shadow and BSDF rays from the surface point itself) and
+with render.rs's fixed 1e-4 push (expected: the share its shadow rays lose
+to the light's own sphere near the silhouette, and BSDF rays from the
+pushed point)."""
 import sys
 
 import numpy as np
@@ -25,10 +31,10 @@ def say(msg):
     sys.stdout.flush()
 
 
-def render(ctx, lights, depth, spp=SPP):
+def render(ctx, lights, depth, spp=SPP, ray_offset=0.0):
     s = dl.scene(lights, W, H)
     ps = rrhip.PtScene(ctx, s.desc())
-    se = ps.session(rrhip.RenderParams.make(max_depth=depth, firefly_clamp=0.0))
+    se = ps.session(rrhip.RenderParams.make(max_depth=depth, firefly_clamp=0.0, ray_offset=ray_offset))
     se.render(0, spp)
     acc = se.accum()
     se.close()
@@ -36,9 +42,9 @@ def render(ctx, lights, depth, spp=SPP):
     return acc
 
 
-def expected(case, lights, pts):
+def expected(case, lights, pts, offset=0.0):
     if case == "sphere":
-        return dl.sphere_total(lights[0], pts)
+        return dl.sphere_total(lights[0], pts, offset=offset)
     return dl.expected_total(lights, pts)
 
 
@@ -46,6 +52,10 @@ def expected(case, lights, pts):
 def test_gpu_both_strategies_equal_the_closed_form(ctx, case):
     say(f"direct light {case}: depth 2, {W}x{H}, {SPP} spp")
     lights = dl.CASES[case]()
+    if case == "sphere":  # render.rs's fixed push first
+        acc = render(ctx, lights, 2, ray_offset=dl.RAY_OFFSET)
+        want = expected(case, lights, dl.pixel_points(W, H), offset=dl.RAY_OFFSET)
+        dl.check(acc, want, "gpu sphere depth 2, fixed 1e-4 push", say=say)
     acc = render(ctx, lights, 2)
     want = expected(case, lights, dl.pixel_points(W, H))
     if case in ("point", "distant"):
@@ -78,5 +88,8 @@ def test_gpu_sphere_light_arm(ctx):
     assert dl.rejects(acc, dl.sphere_nee_arm(bare, pts, heuristic="balance"))
     light = dl.sphere_light()[0]
     acc = render(ctx, [light], 1)
-    dl.check(acc, dl.sphere_nee_arm(light, pts, clip=True), "gpu sphere NEE arm, emitter sphere present",
-             say=say)
+    dl.check(acc, dl.sphere_nee_arm(light, pts, clip=True, offset=0.0),
+             "gpu sphere NEE arm, emitter sphere present", say=say)
+    acc = render(ctx, [light], 1, ray_offset=dl.RAY_OFFSET)
+    dl.check(acc, dl.sphere_nee_arm(light, pts, clip=True), "gpu sphere NEE arm, emitter sphere present, "
+             "fixed 1e-4 push", say=say)
