@@ -491,6 +491,36 @@ def copy_calibration(ctx, args):
     ctx.close()
 
 
+def nrc_steps(ctx, args):
+    """K optimiser steps of the radiance cache (default configuration,
+    bounds [0, 1]^3) on one fixed batch of --nrc-batch random samples
+    staged on the device once: the step chain the time-to-relMSE frames
+    run (training, grid-gradient scatter, MLP reduction + Adam, grid Adam,
+    bf16 pack), timed with HIP events on the cache's stream."""
+    import numpy as np
+    import rrhip
+    n = args.nrc_batch
+    rng = np.random.default_rng(args.seed)
+    q = np.zeros((n, 16), np.float32)
+    q[:, 0:3] = rng.random((n, 3))
+    q[:, 3] = rng.random(n)
+    for a in (4, 8):
+        d = rng.normal(size=(n, 3))
+        q[:, a:a + 3] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    q[:, 12:15] = rng.random((n, 3))
+    t = rng.random((n, 3)).astype(np.float32)
+    c = rrhip.Nrc(ctx, rrhip.NrcConfig.make(lo=(0, 0, 0), hi=(1, 1, 1)))
+    for _ in range(args.warmup):
+        c.train_step(q, t)
+    ms = c.time_steps(q, t, args.steps)
+    flop = n * 3 * 2 * (48 * 64 + 4 * 64 * 64 + 64 * 3)
+    print(json.dumps({"mode": "nrc-step", "samples_per_step": n, "steps": args.steps,
+                      "us_per_step": round(ms / args.steps * 1e3, 1),
+                      "mlp_tflops": round(flop / (ms / args.steps / 1e3) / 1e12, 2)}), flush=True)
+    c.close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -508,8 +538,10 @@ def main():
     ap.add_argument("--no-instanced", action="store_true", help="skip the 64-spp instanced-field frame")
     ap.add_argument("--ref-spp", type=int, default=4096, help="spp of the relMSE reference render")
     ap.add_argument("--max-frames", type=int, default=400, help="frame cap of the time-to-relMSE runs")
-    ap.add_argument("--mode", choices=("trace", "copy"), default="trace",
-                    help="copy: time the plain copy kernel that calibrates HBM counters")
+    ap.add_argument("--mode", choices=("trace", "copy", "nrc-step"), default="trace",
+                    help="copy: time the plain copy kernel that calibrates HBM counters; nrc-step: time the "
+                         "radiance cache's optimiser steps alone (profiling runs)")
+    ap.add_argument("--nrc-batch", type=int, default=4096, help="samples per step of --mode nrc-step")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -522,6 +554,9 @@ def main():
     ctx = rrhip.Context(0)
     if args.mode == "copy":
         copy_calibration(ctx, args)
+        return
+    if args.mode == "nrc-step":
+        nrc_steps(ctx, args)
         return
     pos, idx = rrhip.gen_heightfield(GRID[0], GRID[1], args.seed)
     rays = rrhip.gen_query_rays(pos.min(0), pos.max(0), 0.1, args.seed, N_RAYS)

@@ -614,6 +614,12 @@ int rrhip_nrc_infer(rrhip_nrc* nrc, const rrhip_nrc_query* queries, uint64_t n, 
  * batch's loss (before the step) into *loss when non-NULL. */
 int rrhip_nrc_train_step(rrhip_nrc* nrc, const rrhip_nrc_query* queries, const float* targets, uint64_t n,
                          float* loss);
+/* Timing: stages the n samples once, runs `steps` optimiser steps on them
+ * back to back between HIP events on the context stream and returns the
+ * milliseconds of the whole sequence (the parameters advance as with
+ * rrhip_nrc_train_step). */
+int rrhip_nrc_time_steps(rrhip_nrc* nrc, const rrhip_nrc_query* queries, const float* targets, uint64_t n,
+                         uint32_t steps, float* elapsed_ms);
 /* Loss and gradients of a batch without a step (mlp_grad / grid_grad in
  * the get_params layouts; either may be NULL). */
 int rrhip_nrc_gradients(rrhip_nrc* nrc, const rrhip_nrc_query* queries, const float* targets, uint64_t n,

@@ -72,9 +72,10 @@ hipError_t nrc_launch_infer_paths(const NrcGridView& G, const NrcQuery* q, const
                                   uint32_t capacity, const uint32_t* slots, const float4* beta, float4* L,
                                   hipStream_t st);
 // Forward + backward over the batch (ctl->n samples, <= capacity): block
-// partials into `part`, grid gradients into `grid_grad`.
+// partials into `part`, each sample's 32 encoding gradients into `genc`
+// (capacity x 32 floats), then the grid gradients into `grid_grad`.
 hipError_t nrc_launch_train(const NrcGridView& G, const NrcQuery* q, const float* targets,
-                            const NrcControl* ctl, uint32_t capacity, float* part,
+                            const NrcControl* ctl, uint32_t capacity, float* part, float* genc,
                             unsigned long long* grid_grad, hipStream_t st);
 // Sums the partials; with `apply`, Adam on the MLP master weights (step
 // ctl->step + 1), else the summed gradient into `grad_out`; loss into ctl.

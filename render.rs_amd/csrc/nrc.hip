@@ -331,7 +331,7 @@ __device__ __forceinline__ unsigned long long to_fixed(float v) {
 __global__ __launch_bounds__(128) void nrc_train_kernel(NrcGridView G, const NrcQuery* __restrict__ q,
                                                        const float* __restrict__ targets,
                                                        const NrcControl* __restrict__ ctl, float* __restrict__ part,
-                                                       unsigned long long* __restrict__ gg) {
+                                                       float* __restrict__ genc) {
   __shared__ __align__(16) uint16_t s_act[6 * kActRows];
   __shared__ __align__(16) uint16_t s_dlt[2 * kActRows];
   __shared__ float s_dy[3 * 64];
@@ -439,30 +439,47 @@ __global__ __launch_bounds__(128) void nrc_train_kernel(NrcGridView G, const Nrc
       __syncthreads();
     }
   }
-  // input gradient of the 32 encoding dims, then the grid entries
+  // input gradient of the 32 encoding dims (level l, feature f at 2l + f),
+  // scattered into the grid by nrc_grid_scatter_kernel
   f32x16 gx = zero16();
 #pragma unroll
   for (int c = 0; c < 4; c++) gx = mfma(load8(G.wb + kNrcBwdOff[0] + col * 64 + 16 * c + 8 * h), d[c], gx);
   if (!valid) return;
-  float pn[3];
-  normalise(G, qq, pn);
-  unsigned long long* gl = gg;
+  float2* gs = (float2*)(genc + (size_t)si * 32);
 #pragma unroll
   for (int qd = 0; qd < 4; qd++)
 #pragma unroll
-    for (int e = 0; e < 2; e++) {
-      const uint32_t l = 4 * qd + 2 * h + e;
-      if (l >= G.levels) continue;
-      const float g0 = gx[4 * qd + 2 * e], g1 = gx[4 * qd + 2 * e + 1];
-      const Cell c = cell_of(G, pn, l);
-      unsigned long long* tl = gl + ((size_t)l << G.log2_table) * 2;
-      for (int k = 0; k < 8; k++) {
-        float wk;
-        const uint32_t id = corner(G, l, c, k, wk);
-        atomicAdd(tl + 2 * (size_t)id, to_fixed(g0 * wk));
-        atomicAdd(tl + 2 * (size_t)id + 1, to_fixed(g1 * wk));
-      }
-    }
+    for (int e = 0; e < 2; e++) gs[4 * qd + 2 * h + e] = make_float2(gx[4 * qd + 2 * e], gx[4 * qd + 2 * e + 1]);
+}
+
+// Grid gradients: each (sample, level) adds the level's two encoding
+// gradients times each corner's trilinear weight to the eight corner
+// entries, as 2^-36 fixed-point 64-bit integers (the sum is exact and
+// independent of the order the atomics land in). Sixteen adjacent lanes
+// take one (sample, level): lane k the feature k & 1 of corner k >> 1, so a
+// wave's atomic touches the few cache lines of four cells (a corner's two
+// features share a line, and at densely indexed levels so do the +x
+// neighbours) instead of 64 unrelated lines, and the atomics of every
+// sample-level pair are in flight at once.
+__global__ __launch_bounds__(256) void nrc_grid_scatter_kernel(NrcGridView G, const NrcQuery* __restrict__ q,
+                                                              const NrcControl* __restrict__ ctl,
+                                                              const float* __restrict__ genc,
+                                                              unsigned long long* __restrict__ gg) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t pair = i >> 4, k = i & 15u;
+  const uint32_t s = pair / kNrcMaxLevels, l = pair % kNrcMaxLevels;
+  if (s >= ctl->n || l >= G.levels) return;
+  NrcQuery qq;
+  qq.pos[0] = q[s].pos[0];
+  qq.pos[1] = q[s].pos[1];
+  qq.pos[2] = q[s].pos[2];
+  float pn[3];
+  normalise(G, qq, pn);
+  const float gk = genc[(size_t)s * 32 + 2 * l + (k & 1u)];
+  const Cell c = cell_of(G, pn, l);
+  float wk;
+  const uint32_t id = corner(G, l, c, (int)(k >> 1), wk);
+  atomicAdd(gg + (((size_t)l << G.log2_table) + id) * 2 + (k & 1u), to_fixed(gk * wk));
 }
 
 __device__ __forceinline__ void adam_update(float& p, float g, float& m, float& v, const NrcHyper& hp, float c1,
@@ -478,17 +495,32 @@ __device__ __forceinline__ void bias_corrections(const NrcControl* ctl, const Nr
   c2 = (float)(1.0 - pow((double)hp.b2, t));
 }
 
+// Sums the training blocks' partials of each parameter: kReduceSeg threads
+// per parameter each add a contiguous run of blocks in block order, then the
+// runs are added in run order (a fixed order, so results are reproducible).
+constexpr uint32_t kReduceSeg = 8;
+constexpr uint32_t kReduceParams = 256 / kReduceSeg;  // parameters per block
+
 __global__ __launch_bounds__(256) void nrc_reduce_mlp_kernel(const float* __restrict__ part, NrcControl* ctl,
                                                             float* __restrict__ w, float* __restrict__ m,
                                                             float* __restrict__ v, float* __restrict__ grad_out,
                                                             NrcHyper hp, int apply) {
+  __shared__ float s_run[kReduceSeg][kReduceParams];
   const uint32_t n = ctl->n;
   if (n == 0) return;
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > kNrcMlpParams) return;
+  const uint32_t pl = threadIdx.x % kReduceParams, seg = threadIdx.x / kReduceParams;
+  const uint32_t i = blockIdx.x * kReduceParams + pl;
   const uint32_t nb = (n + kNrcTrainBlock - 1) / kNrcTrainBlock;
+  const uint32_t per = (nb + kReduceSeg - 1) / kReduceSeg;
+  float r = 0.f;
+  if (i <= kNrcMlpParams)
+    for (uint32_t b = seg * per; b < min(nb, (seg + 1) * per); b++) r += part[(size_t)b * kNrcPartStride + i];
+  s_run[seg][pl] = r;
+  __syncthreads();
+  if (seg != 0 || i > kNrcMlpParams) return;
   float s = 0.f;
-  for (uint32_t b = 0; b < nb; b++) s += part[(size_t)b * kNrcPartStride + i];
+#pragma unroll
+  for (uint32_t k = 0; k < kReduceSeg; k++) s += s_run[k][pl];
   if (i == kNrcMlpParams) {
     ctl->last_loss = s;
     return;
@@ -524,13 +556,12 @@ __global__ __launch_bounds__(256) void nrc_adam_grid_kernel(float* __restrict__ 
   if (apply) bias_corrections(ctl, hp, c1, c2);
   const float corr = ema ? ema_debias(ctl, alpha) : 1.f;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const long long fx = (long long)gg[i];
+  const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const auto one = [&](uint64_t i, long long fx) {
     const float g = (float)fx * (1.f / 68719476736.f);
-    if (fx != 0) gg[i] = 0ull;
     if (!apply) {
       grad_out[i] = g;
-      continue;
+      return;
     }
     float p = table[i], mm = m[i], vv = v[i];
     adam_update(p, g, mm, vv, hp, c1, c2);
@@ -542,6 +573,39 @@ __global__ __launch_bounds__(256) void nrc_adam_grid_kernel(float* __restrict__ 
       ema[i] = e;
       inf[i] = e * corr;
     }
+  };
+  if (apply && !ema) {
+    // four parameters per thread with 16-byte accesses (the tables are
+    // 256-byte aligned); the gradients are cleared where any was set
+    const uint64_t n4 = n / 4;
+    for (uint64_t j = t0; j < n4; j += stride) {
+      ulonglong2* g2 = (ulonglong2*)gg + 2 * j;
+      const ulonglong2 ga = g2[0], gb = g2[1];
+      const long long fx[4] = {(long long)ga.x, (long long)ga.y, (long long)gb.x, (long long)gb.y};
+      if (fx[0] | fx[1] | fx[2] | fx[3]) {
+        g2[0] = make_ulonglong2(0ull, 0ull);
+        g2[1] = make_ulonglong2(0ull, 0ull);
+      }
+      float4 p = ((float4*)table)[j], mm = ((float4*)m)[j], vv = ((float4*)v)[j];
+      adam_update(p.x, (float)fx[0] * (1.f / 68719476736.f), mm.x, vv.x, hp, c1, c2);
+      adam_update(p.y, (float)fx[1] * (1.f / 68719476736.f), mm.y, vv.y, hp, c1, c2);
+      adam_update(p.z, (float)fx[2] * (1.f / 68719476736.f), mm.z, vv.z, hp, c1, c2);
+      adam_update(p.w, (float)fx[3] * (1.f / 68719476736.f), mm.w, vv.w, hp, c1, c2);
+      ((float4*)table)[j] = p;
+      ((float4*)m)[j] = mm;
+      ((float4*)v)[j] = vv;
+    }
+    for (uint64_t i = 4 * n4 + t0; i < n; i += stride) {
+      const long long fx = (long long)gg[i];
+      if (fx != 0) gg[i] = 0ull;
+      one(i, fx);
+    }
+    return;
+  }
+  for (uint64_t i = t0; i < n; i += stride) {
+    const long long fx = (long long)gg[i];
+    if (fx != 0) gg[i] = 0ull;
+    one(i, fx);
   }
 }
 
@@ -615,18 +679,22 @@ hipError_t nrc_launch_infer_paths(const NrcGridView& G, const NrcQuery* q, const
 }
 
 hipError_t nrc_launch_train(const NrcGridView& G, const NrcQuery* q, const float* targets, const NrcControl* ctl,
-                            uint32_t capacity, float* part, unsigned long long* grid_grad, hipStream_t st) {
+                            uint32_t capacity, float* part, float* genc, unsigned long long* grid_grad,
+                            hipStream_t st) {
   const uint32_t blocks = (capacity + kNrcTrainBlock - 1) / kNrcTrainBlock;
   hipLaunchKernelGGL(nrc_train_kernel, dim3(blocks < 1 ? 1 : blocks), dim3(128), 0, st, G, q, targets, ctl, part,
-                     grid_grad);
+                     genc);
+  const uint64_t lanes = (uint64_t)(capacity < 1 ? 1 : capacity) * kNrcMaxLevels * 16;
+  hipLaunchKernelGGL(nrc_grid_scatter_kernel, dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0, st, G, q, ctl,
+                     genc, grid_grad);
   return hipGetLastError();
 }
 
 hipError_t nrc_launch_reduce_mlp(const float* part, NrcControl* ctl, uint32_t capacity, float* w, float* m, float* v,
                                  float* grad_out, NrcHyper hp, bool apply, hipStream_t st) {
   (void)capacity;
-  hipLaunchKernelGGL(nrc_reduce_mlp_kernel, dim3((kNrcMlpParams + 1 + 255) / 256), dim3(256), 0, st, part, ctl, w,
-                     m, v, grad_out, hp, apply ? 1 : 0);
+  hipLaunchKernelGGL(nrc_reduce_mlp_kernel, dim3((kNrcMlpParams + 1 + kReduceParams - 1) / kReduceParams), dim3(256),
+                     0, st, part, ctl, w, m, v, grad_out, hp, apply ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -137,7 +137,7 @@ void nrc_set_bounds(rrhip_nrc* c, const float lo[3], const float hi[3]) {
 
 hipError_t nrc_step_device(rrhip_nrc* c, const NrcQuery* q, const float* targets, uint32_t capacity,
                            hipStream_t st) {
-  hipError_t e = nrc_launch_train(c->G, q, targets, c->d_ctl, capacity, c->d_part, c->d_tg, st);
+  hipError_t e = nrc_launch_train(c->G, q, targets, c->d_ctl, capacity, c->d_part, c->d_genc, c->d_tg, st);
   if (e == hipSuccess)
     e = nrc_launch_reduce_mlp(c->d_part, c->d_ctl, capacity, c->d_w, c->d_wm, c->d_wv, nullptr, c->hp, true, st);
   if (e == hipSuccess)
@@ -197,6 +197,7 @@ int rrhip_nrc_create(rrhip_ctx* ctx, const rrhip_nrc_config* config, rrhip_nrc**
   if (e == hipSuccess) e = alloc_n(c, &c->d_wf, (uint64_t)kNrcFwdSize);
   if (e == hipSuccess) e = alloc_n(c, &c->d_wb, (uint64_t)kNrcBwdSize);
   if (e == hipSuccess) e = alloc_n(c, &c->d_part, (uint64_t)(kNrcMaxBatch / kNrcTrainBlock) * kNrcPartStride);
+  if (e == hipSuccess) e = alloc_n(c, &c->d_genc, (uint64_t)kNrcMaxBatch * 32);
   if (e == hipSuccess) e = alloc_n(c, &c->d_ctl, 1);
   if (cfg.ema > 0.f) {
     if (e == hipSuccess) e = alloc_n(c, &c->d_ema_table, c->grid_n);
@@ -345,13 +346,27 @@ int rrhip_nrc_train_step(rrhip_nrc* c, const rrhip_nrc_query* queries, const flo
   return RRHIP_OK;
 }
 
+int rrhip_nrc_time_steps(rrhip_nrc* c, const rrhip_nrc_query* queries, const float* targets, uint64_t n,
+                         uint32_t steps, float* elapsed_ms) {
+  if (int s = check_nrc(c)) return s;
+  if (!elapsed_ms || steps == 0) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_nrc_time_steps: null output or 0 steps");
+  if (int s = stage_batch(c, queries, targets, n, "rrhip_nrc_time_steps")) return s;
+  hipStream_t st = c->ctx->stream;
+  RRHIP_HIP_TRY(hipEventRecord(c->ctx->ev0, st));
+  for (uint32_t k = 0; k < steps; k++) RRHIP_HIP_TRY(nrc_step_device(c, c->d_q, c->d_t, (uint32_t)n, st));
+  RRHIP_HIP_TRY(hipEventRecord(c->ctx->ev1, st));
+  RRHIP_HIP_TRY(hipEventSynchronize(c->ctx->ev1));
+  RRHIP_HIP_TRY(hipEventElapsedTime(elapsed_ms, c->ctx->ev0, c->ctx->ev1));
+  return RRHIP_OK;
+}
+
 int rrhip_nrc_gradients(rrhip_nrc* c, const rrhip_nrc_query* queries, const float* targets, uint64_t n, float* loss,
                         float* mlp_grad, float* grid_grad) {
   if (int s = check_nrc(c)) return s;
   if (int s = stage_batch(c, queries, targets, n, "rrhip_nrc_gradients")) return s;
   hipStream_t st = c->ctx->stream;
   if (!c->d_gout) RRHIP_HIP_TRY(alloc_n(c, &c->d_gout, c->grid_n));
-  RRHIP_HIP_TRY(nrc_launch_train(c->G, c->d_q, c->d_t, c->d_ctl, (uint32_t)n, c->d_part, c->d_tg, st));
+  RRHIP_HIP_TRY(nrc_launch_train(c->G, c->d_q, c->d_t, c->d_ctl, (uint32_t)n, c->d_part, c->d_genc, c->d_tg, st));
   RRHIP_HIP_TRY(nrc_launch_reduce_mlp(c->d_part, c->d_ctl, (uint32_t)n, c->d_w, c->d_wm, c->d_wv, c->d_wgrad, c->hp,
                                       false, st));
   RRHIP_HIP_TRY(nrc_launch_adam_grid(c->d_table, c->d_tm, c->d_tv, c->d_tg, c->grid_n, c->d_ctl, c->d_gout, c->hp,

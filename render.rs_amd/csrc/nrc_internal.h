@@ -25,6 +25,7 @@ struct rrhip_nrc {
   uint16_t* d_wf = nullptr;
   uint16_t* d_wb = nullptr;
   float* d_part = nullptr;
+  float* d_genc = nullptr;  // per-sample encoding gradients of a step (kNrcMaxBatch x 32)
   rrhip::NrcControl* d_ctl = nullptr;
   // Inference parameters when cfg.ema > 0: moving averages of the grid and
   // MLP weights, their debiased values, and the bf16 forward copy.

@@ -48,6 +48,7 @@ EXPORTED = [
     "rrhip_session_set_adaptive", "rrhip_session_average_spp",
     "rrhip_session_set_film", "rrhip_read_film", "rrhip_pcg32_draws",
     "rrhip_trace_closest_noalloc", "rrhip_session_set_profiling", "rrhip_session_get_profile",
+    "rrhip_nrc_time_steps",
 ]
 
 
@@ -291,6 +292,7 @@ def load():
         "rrhip_load_accum": (i32, [vp, vp]),
         "rrhip_session_get_stats": (i32, [vp, ctypes.POINTER(SessionStats)]),
         "rrhip_session_set_profiling": (i32, [vp, i32]),
+        "rrhip_nrc_time_steps": (i32, [vp, vp, vp, u64, u32, ctypes.POINTER(ctypes.c_float)]),
         "rrhip_session_get_profile": (i32, [vp, ctypes.POINTER(PtProfile)]),
         "rrhip_relmse": (i32, [vp, vp, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]),
         "rrhip_pcg32_draws": (i32, [vp, vp, vp, u32, u32, vp]),
@@ -757,6 +759,14 @@ class Nrc:
         loss = ctypes.c_float()
         _check(load().rrhip_nrc_train_step(self.h, _ptr(q), _ptr(t), q.shape[0], ctypes.byref(loss)))
         return loss.value
+
+    def time_steps(self, queries, targets, steps):
+        """Milliseconds of `steps` optimiser steps on one staged batch."""
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 16)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        ms = ctypes.c_float()
+        _check(load().rrhip_nrc_time_steps(self.h, _ptr(q), _ptr(t), q.shape[0], steps, ctypes.byref(ms)))
+        return ms.value
 
     def gradients(self, queries, targets):
         q = np.ascontiguousarray(queries, np.float32).reshape(-1, 16)
