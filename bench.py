@@ -403,9 +403,10 @@ def time_to_relmse(ctx, args, with_cpu):
 def batched_jobs(ctx, args, with_cpu):
     """32 heterogeneous room jobs (1-40 spheres, 64^2..512^2, 4-64 spp,
     every other one filled with a 128^3 fBm cloud) rendered (a) one after
-    another, each by its own session, and (b) as one batch whose passes
-    advance every unfinished job through a single launch sequence.
-    Completion times are per job from the start of the whole workload."""
+    another, each by its own session, (b) as one batch whose passes
+    advance every unfinished job through a single launch sequence, and (c)
+    in one persistent launch. Completion times are per job from the start
+    of the whole workload."""
     import numpy as np
     import rrhip
     import scenes
@@ -433,6 +434,13 @@ def batched_jobs(ctx, args, with_cpu):
     wall_ms = (time.perf_counter() - t0) * 1e3
     bst = b.stats()
     brays = bst["extension_rays"] + bst["shadow_rays"]
+    # (c) every job in one persistent launch
+    b.render_persistent()  # warm-up
+    t0 = time.perf_counter()
+    pjob_ms, ptotal_ms = b.render_persistent()
+    pwall_ms = (time.perf_counter() - t0) * 1e3
+    pst = b.stats()
+    prays = pst["extension_rays"] + pst["shadow_rays"]
 
     def pct(v, q):
         return round(float(np.percentile(v, q)), 2)
@@ -446,6 +454,11 @@ def batched_jobs(ctx, args, with_cpu):
         "batched": {"total_ms": round(wall_ms, 2), "device_ms": round(total_ms, 2),
                     "mrays_per_s": round(brays / (wall_ms / 1e3) / 1e6, 1), "p50_ms": pct(job_ms, 50),
                     "p99_ms": pct(job_ms, 99), "passes": bst["passes"], "samples_per_pass": bst["samples_per_pass"]},
+        "persistent": {"total_ms": round(pwall_ms, 2), "device_ms": round(ptotal_ms, 2),
+                       "mrays_per_s": round(prays / (pwall_ms / 1e3) / 1e6, 1), "p50_ms": pct(pjob_ms, 50),
+                       "p99_ms": pct(pjob_ms, 99), "launches": 1,
+                       "note": "one persistent launch: lanes pull (job, pixel) items shortest job first and "
+                               "regenerate paths in place; job completion from the device clock"},
         "rays": int(brays),
     }
     b.close()

@@ -657,6 +657,18 @@ int rrhip_batch_destroy(rrhip_batch* batch);
  * completion time from the start, measured with device events at pass
  * granularity; total_ms the whole batch. */
 int rrhip_batch_render(rrhip_batch* batch, double* job_ms, double* total_ms);
+/* Every job in ONE persistent launch: each resident lane pulls (job,
+ * pixel) work items from a device counter — pixels of the shortest jobs
+ * (samples x pixels) first — and runs all of the pixel's samples through
+ * the same camera / closest-hit / shading / shadow device functions as the
+ * passes, adding them in sample order; the last pixel of a job stamps the
+ * device clock. job_ms (may be NULL) receives each job's completion time
+ * from the launch's first block start (device clock), total_ms the
+ * launch (events). Accumulators are cleared first and hold what a session
+ * with samples_per_pass = the job's spp renders. Error-bounded ray origins
+ * only (RRHIP_ERR_UNSUPPORTED if a job sets ray_offset). No counterpart in
+ * render.rs. */
+int rrhip_batch_render_persistent(rrhip_batch* batch, double* job_ms, double* total_ms);
 int rrhip_batch_read_accum(rrhip_batch* batch, uint32_t job, double* out);
 /* Rays traced by the last render (extension, shadow), passes, samples_per_pass. */
 int rrhip_batch_get_stats(rrhip_batch* batch, rrhip_session_stats* stats);

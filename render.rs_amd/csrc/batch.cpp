@@ -27,6 +27,15 @@ struct rrhip_batch {
   PtJob* d_jobs = nullptr;  // one table per pass, each `jobs.size()` entries
   uint32_t passes = 0;
   unsigned long long last_rays[2] = {0, 0};
+  // One persistent launch (rrhip_batch_render_persistent): a path slot per
+  // resident lane, the job table in item order, the work counter, per-job
+  // pixels left and completion clocks (+ the launch's first block start).
+  uint32_t p_flags = 0, p_lanes = 0;
+  PathBuffers PP{};
+  PtJob* d_pjobs = nullptr;
+  uint32_t* d_next = nullptr;
+  uint32_t* d_left = nullptr;
+  unsigned long long* d_clock = nullptr;
 };
 
 namespace {
@@ -272,6 +281,130 @@ int rrhip_batch_render(rrhip_batch* b, double* job_ms, double* total_ms) {
   if (e != hipSuccess) status = fail_hip(e, "rrhip_batch_render");
   for (auto& x : ev) (void)hipEventDestroy(x);
   return status;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Path slots and work tables of the persistent launch (allocated on first
+// use, kept with the batch).
+hipError_t persistent_setup(rrhip_batch* b) {
+  if (b->d_pjobs) return hipSuccess;
+  const uint32_t nj = (uint32_t)b->jobs.size();
+  bool eq = false;
+  for (const rrhip_batch_job& J : b->jobs) {
+    b->p_flags |= (J.scene->pt.transparent ? 1u : 0u) | (J.scene->pt.motion ? 2u : 0u) |
+                  (J.scene->pt.num_quadrics ? 4u : 0u);
+    eq |= J.scene->pt_eq;
+  }
+  b->p_lanes = pt_batch_persistent_lanes(b->p_flags);
+  if (b->p_lanes == 0) return hipErrorInvalidValue;
+  const uint64_t n = b->p_lanes;
+  PathBuffers& P = b->PP;
+  hipError_t e = alloc_n(b, &P.o, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.d, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.beta, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.L, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.prev, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.rng, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.flags, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.hit_n, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.hit_m, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.hit_front, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.hit_back, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.q0, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.sq, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.so, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.sd, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.sc, n);
+  if (e == hipSuccess && eq) {
+    e = alloc_n(b, &P.so2, n);
+    if (e == hipSuccess) e = alloc_n(b, &P.sd2, n);
+    if (e == hipSuccess) e = alloc_n(b, &P.sc2, n);
+  }
+  if (e == hipSuccess && (b->p_flags & 1u)) {
+    e = alloc_n(b, &P.mstk, n);
+    if (e == hipSuccess) e = alloc_n(b, &P.srng, n);
+  }
+  if (e == hipSuccess && (b->p_flags & 2u)) e = alloc_n(b, &P.stime, n);
+  if (e == hipSuccess) e = alloc_n(b, &P.stats, 4);
+  if (e == hipSuccess) e = alloc_n(b, &b->d_next, 1);
+  if (e == hipSuccess) e = alloc_n(b, &b->d_left, nj);
+  if (e == hipSuccess) e = alloc_n(b, &b->d_clock, (uint64_t)nj + 1);
+  if (e == hipSuccess) e = alloc_n(b, &b->d_pjobs, nj);
+  return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rrhip_batch_render_persistent(rrhip_batch* b, double* job_ms, double* total_ms) {
+  if (!b) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_batch_render_persistent: null batch");
+  for (const rrhip_batch_job& J : b->jobs)
+    if (J.params.ray_offset > 0.0)
+      return fail(RRHIP_ERR_UNSUPPORTED, "rrhip_batch_render_persistent: a job uses the fixed ray-origin push");
+  RRHIP_HIP_TRY(hipSetDevice(b->ctx->device));
+  hipStream_t st = b->ctx->stream;
+  RRHIP_HIP_TRY(persistent_setup(b));
+  const uint32_t nj = (uint32_t)b->jobs.size();
+  // items: every pixel of every job, jobs shortest first (samples x pixels)
+  std::vector<uint32_t> order(nj);
+  for (uint32_t j = 0; j < nj; j++) order[j] = j;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+    return (uint64_t)b->jobs[x].spp * b->npix[x] < (uint64_t)b->jobs[y].spp * b->npix[y];
+  });
+  std::vector<PtJob> table(nj);
+  std::vector<uint32_t> left(nj), pos(nj);
+  uint64_t items = 0;
+  for (uint32_t k = 0; k < nj; k++) {
+    const uint32_t j = order[k];
+    const rrhip_batch_job& J = b->jobs[j];
+    PtJob& t = table[k];
+    t.S = J.scene->pt;
+    t.S.ray_offset = 0.f;
+    t.prm.max_depth = J.params.max_depth;
+    t.prm.rr_start = J.params.rr_start;
+    t.prm.firefly_clamp = (float)J.params.firefly_clamp;
+    t.prm.npix = b->npix[j];
+    t.prm.spp_pass = J.spp;
+    t.prm.sample0 = J.params.sample_base;
+    t.pix_begin = (uint32_t)items;
+    t.accum = b->accum[j];
+    left[k] = b->npix[j];
+    pos[j] = k;
+    items += b->npix[j];
+  }
+  if (items >= (1ull << 32)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_batch_render_persistent: too many pixels");
+  for (uint32_t j = 0; j < nj; j++)
+    RRHIP_HIP_TRY(hipMemsetAsync(b->accum[j], 0, (size_t)b->npix[j] * RRHIP_ACCUM_STRIDE * sizeof(double), st));
+  const std::vector<unsigned long long> clock0(nj + 1, ~0ull);
+  const uint32_t zero = 0;
+  RRHIP_HIP_TRY(hipMemsetAsync(b->PP.stats, 0, 4 * sizeof(unsigned long long), st));
+  RRHIP_HIP_TRY(hipMemcpyAsync(b->d_pjobs, table.data(), nj * sizeof(PtJob), hipMemcpyHostToDevice, st));
+  RRHIP_HIP_TRY(hipMemcpyAsync(b->d_left, left.data(), nj * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  RRHIP_HIP_TRY(hipMemcpyAsync(b->d_clock, clock0.data(), (nj + 1) * sizeof(unsigned long long),
+                               hipMemcpyHostToDevice, st));
+  RRHIP_HIP_TRY(hipMemcpyAsync(b->d_next, &zero, sizeof(zero), hipMemcpyHostToDevice, st));
+  RRHIP_HIP_TRY(hipEventRecord(b->ctx->ev0, st));
+  RRHIP_HIP_TRY(pt_launch_batch_persistent(b->d_pjobs, nj, b->PP, (uint32_t)items, b->p_lanes, b->p_flags, b->d_next,
+                                           b->d_left, b->d_clock, st));
+  RRHIP_HIP_TRY(hipEventRecord(b->ctx->ev1, st));
+  RRHIP_HIP_TRY(hipEventSynchronize(b->ctx->ev1));
+  float ms = 0.f;
+  RRHIP_HIP_TRY(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1));
+  if (total_ms) *total_ms = ms;
+  std::vector<unsigned long long> clock(nj + 1);
+  RRHIP_HIP_TRY(hipMemcpy(clock.data(), b->d_clock, (nj + 1) * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  RRHIP_HIP_TRY(hipMemcpy(b->last_rays, b->PP.stats, sizeof(b->last_rays), hipMemcpyDeviceToHost));
+  if (job_ms) {
+    int khz = 0;
+    RRHIP_HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->ctx->device));
+    if (khz <= 0) return fail(RRHIP_ERR_DEVICE, "rrhip_batch_render_persistent: no device clock rate");
+    for (uint32_t j = 0; j < nj; j++) job_ms[j] = (double)(clock[pos[j]] - clock[nj]) / (double)khz;
+  }
+  return RRHIP_OK;
 }
 
 int rrhip_batch_read_accum(rrhip_batch* b, uint32_t job, double* out) {

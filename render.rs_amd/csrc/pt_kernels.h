@@ -43,6 +43,19 @@ hipError_t pt_launch_bounce(const PtSceneView& S, const PathBuffers& P, const Pt
 // job's scene holds quadrics (the trace kernels then carry their solvers).
 hipError_t pt_launch_batch_pass(const PtJob* jobs, uint32_t nj, const PathBuffers& P, uint32_t total,
                                  uint32_t total_pix, uint32_t bounces, bool quadrics, hipStream_t st);
+// Every job of a batch in one persistent launch: `lanes` path slots (a
+// multiple of the block), each pulling (job, pixel) items from *next_item
+// (zero on entry) until `total_items`; a job's prm.spp_pass is all of its
+// samples and pix_begin its first item. job_left[j] counts the job's
+// pixels down; the last one writes the device clock to job_clock[j], and
+// job_clock[nj] takes the earliest block start (set it to ~0 first).
+// flags: bit 0 some job is transparent, bit 1 some job moves, bit 2 some
+// job holds quadrics (error-bounded ray origins only).
+hipError_t pt_launch_batch_persistent(const PtJob* jobs, uint32_t nj, const PathBuffers& P, uint32_t total_items,
+                                      uint32_t lanes, uint32_t flags, uint32_t* next_item, uint32_t* job_left,
+                                      unsigned long long* job_clock, hipStream_t st);
+// Path slots of one round of resident persistent blocks (0 if unknown).
+uint32_t pt_batch_persistent_lanes(uint32_t flags);
 // Adds each pixel's pass samples (firefly-clamped) into the f64
 // accumulator in sample order (deterministic, no atomics).
 hipError_t pt_launch_resolve(const PathBuffers& P, const PtParams& prm, double* accum, double* film,

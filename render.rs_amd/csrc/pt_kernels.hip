@@ -778,17 +778,14 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
     P.flags[slot] = flags;
 }
 
-// Appends a lane's shadow record (all lanes of the wave call it): the
-// first ray is `a` (or `b` alone); with both, `b` goes to the second
-// arrays and sc.w = 1 marks the record.
-// sc.w: bit 0 = the record has a second ray, bit 1 (FILM) = its
-// contributions go to the diffuse AOV as well.
+// Shadow record idx of path `slot`: the first ray is `a` (or `b` alone);
+// with both, `b` goes to the second arrays. sc.w: bit 0 = the record has
+// a second ray, bit 1 (FILM) = its contributions go to the diffuse AOV as
+// well; transparent scenes: d.w holds the medium at the origin, srng the
+// walk's seed.
 template <bool TR, bool FILM = false>
-__device__ __forceinline__ void push_shadow(const PathBuffers& P, bool act, uint32_t slot, bool s1, const ShadowOut& a,
-                                            bool s2, const ShadowOut& b) {
-  const bool any = act && (s1 || s2);
-  const uint32_t idx = wave_append(&P.counts[2], any);
-  if (!any) return;
+__device__ __forceinline__ void put_shadow(const PathBuffers& P, uint32_t idx, uint32_t slot, bool s1,
+                                           const ShadowOut& a, bool s2, const ShadowOut& b) {
   const ShadowOut& f = s1 ? a : b;
   const bool both = s1 && s2;
   // transparent scenes: d.w holds the medium at the origin, srng the walk's seed
@@ -804,6 +801,16 @@ __device__ __forceinline__ void push_shadow(const PathBuffers& P, bool act, uint
     P.sd2[idx] = make_float4(b.d.x, b.d.y, b.d.z, walk ? (float)b.medium : 0.f);
     P.sc2[idx] = make_float4(b.c.x, b.c.y, b.c.z, 0.f);
   }
+}
+
+// Appends a lane's shadow record (all lanes of the wave call it).
+template <bool TR, bool FILM = false>
+__device__ __forceinline__ void push_shadow(const PathBuffers& P, bool act, uint32_t slot, bool s1, const ShadowOut& a,
+                                            bool s2, const ShadowOut& b) {
+  const bool any = act && (s1 || s2);
+  const uint32_t idx = wave_append(&P.counts[2], any);
+  if (!any) return;
+  put_shadow<TR, FILM>(P, idx, slot, s1, a, s2, b);
 }
 
 // Three waves per SIMD (<= 168 VGPRs): shading is latency-bound on
@@ -1232,6 +1239,128 @@ __global__ __launch_bounds__(kBlock) void resolve_batch_kernel(const PtJob* __re
   }
 }
 
+// ---- batched jobs in one persistent launch ---------------------------------------
+// Every lane owns one path slot and pulls work items — one pixel of one
+// job with all of its samples — from a device counter until the items run
+// out; items are numbered job by job (pix_begin) in the order the host
+// chose (shortest job first). A pixel's samples run one after another
+// through the same device functions the wavefront kernels call (camera
+// ray, closest hit, shading, shadow records evaluated in place), and are
+// added in sample order into the pixel's accumulator row. The last pixel
+// of a job records the device clock, so per-job completion is measured
+// inside the one launch. There are no barriers: a lane exits when the
+// counter passes the last item.
+
+// Samples of one (pixel, sample) path through the bounce loop in slot
+// `slot`; returns its radiance (P.L) and counts rays and shadow records.
+// One bounce of the path in slot `slot` (closest hit, shading, its shadow
+// record evaluated in place); returns whether the path continues and
+// counts rays and shadow records. TR: the job is transparent (its own
+// scene's flag, as its session shades it); MOTION / QUAD: some job of the
+// batch moves / holds quadrics (the kernel is instantiated for the
+// batch's union of them; static and quadric-free scenes take the same
+// paths through those templates).
+template <bool TR, bool MOTION, bool QUAD>
+__device__ __forceinline__ bool persist_bounce(const PtJob& J, const PathBuffers& P, uint32_t slot, uint32_t* stk,
+                                               unsigned long long& ext, unsigned long long& shd) {
+  const NrcPathBuffers N{};
+  extend_one<MOTION, QUAD>(J.S, P, slot, stk);
+  ext++;
+  bool cont = false, shadow = false, shadow2 = false, query = false;
+  ShadowOut so, se;
+  NrcQuery nq;
+  V3 qbeta;
+  shade_one<false, TR>(J.S, P, J.prm, N, slot, cont, shadow, shadow2, se, query, so, nq, qbeta);
+  if (shadow || shadow2) {
+    put_shadow<TR>(P, slot, slot, shadow, so, shadow2, se);
+    shadow_one_t<TR, MOTION, QUAD>(J.S, P, slot, stk);
+    shd++;
+  }
+  return cont;
+}
+
+// A lane runs one flat loop of bounces: when its path ends it adds the
+// sample to its pixel's sums and starts the pixel's next sample in the same
+// iteration, and when the pixel is done it writes the pixel's accumulator
+// row and pulls the next item, so no lane waits for the others' paths.
+template <bool TR, bool MOTION, bool QUAD>
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(2))) void batch_persistent_kernel(
+    const PtJob* __restrict__ jobs, uint32_t nj, PathBuffers P, uint32_t total_items, uint32_t* __restrict__ next_item,
+    uint32_t* __restrict__ job_left, unsigned long long* __restrict__ job_clock) {
+  __shared__ uint32_t s_stack[kTraceStack * kTraceBlock];
+  uint32_t* stk = s_stack + threadIdx.x;
+  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (threadIdx.x == 0) atomicMin(job_clock + nj, (unsigned long long)wall_clock64());
+  unsigned long long ext = 0, shd = 0;
+  uint32_t j = 0, pixel = 0, k = 0, b = 0, rounds = 0;
+  bool have_item = false, have_path = false;
+  double sum[3] = {0, 0, 0}, sq[3] = {0, 0, 0}, ql = 0.0;
+  for (;;) {
+    if (!have_path) {
+      if (have_item && k == jobs[j].prm.spp_pass) {  // the pixel is done
+        const PtJob& J = jobs[j];
+        double* a = J.accum + 8 * (size_t)pixel;
+        for (int c = 0; c < 3; c++) {
+          a[c] += sum[c];
+          a[3 + c] += sq[c];
+        }
+        a[6] += (double)J.prm.spp_pass;
+        a[7] += ql;
+        if (atomicSub(job_left + j, 1u) == 1u) job_clock[j] = wall_clock64();
+        have_item = false;
+      }
+      if (!have_item) {
+        const uint32_t item = atomicAdd(next_item, 1u);
+        if (item >= total_items) break;
+        j = find_job<true>(jobs, nj, item);
+        pixel = item - jobs[j].pix_begin;
+        k = 0;
+        have_item = true;
+        for (int c = 0; c < 3; c++) sum[c] = sq[c] = 0.0;
+        ql = 0.0;
+      }
+      const PtJob& J = jobs[j];
+      init_path(J.S, P, slot, pixel, J.prm.sample0 + k, kFlagFromCamera);
+      rounds = J.prm.max_depth + (TR && J.S.transparent ? kPtMaxSkips : 0);
+      b = 0;
+      have_path = true;
+    }
+    const PtJob& J = jobs[j];
+    const bool cont = TR && J.S.transparent ? persist_bounce<true, MOTION, QUAD>(J, P, slot, stk, ext, shd)
+                                            : persist_bounce<false, MOTION, QUAD>(J, P, slot, stk, ext, shd);
+    if (!cont || ++b >= rounds) {
+      // the sample as resolve_one adds it (firefly clamp on luminance)
+      const float4 l4 = P.L[slot];
+      double l[3] = {l4.x, l4.y, l4.z};
+      double lm = 0.2126 * l[0] + 0.7152 * l[1] + 0.0722 * l[2];
+      if (J.prm.firefly_clamp > 0.f && lm > (double)J.prm.firefly_clamp) {
+        const double sc = (double)J.prm.firefly_clamp / lm;
+        l[0] *= sc;
+        l[1] *= sc;
+        l[2] *= sc;
+        lm = 0.2126 * l[0] + 0.7152 * l[1] + 0.0722 * l[2];
+      }
+      for (int c = 0; c < 3; c++) {
+        sum[c] += l[c];
+        sq[c] += l[c] * l[c];
+      }
+      ql += lm * lm;
+      k++;
+      have_path = false;
+    }
+  }
+  // rays traced (per wave, one atomic); every lane of the wave has left the
+  // loop here
+  for (int off = 32; off > 0; off >>= 1) {
+    ext += __shfl_xor(ext, off);
+    shd += __shfl_xor(shd, off);
+  }
+  if ((threadIdx.x & 63u) == 0) {
+    atomicAdd(P.stats + 0, ext);
+    atomicAdd(P.stats + 1, shd);
+  }
+}
+
 // ---- shading-class sort of the live queue (before shade) ------------------------
 // Key: 0 = miss, else the hit material's sort_key (1 + active-lobe mask).
 // Counting sort over fixed contiguous block ranges: per-block histograms,
@@ -1383,6 +1512,42 @@ hipError_t pt_launch_batch_pass(const PtJob* jobs, uint32_t nj, const PathBuffer
   }
   hipLaunchKernelGGL(resolve_batch_kernel, dim3(grid_for(total_pix)), dim3(kBlock), 0, st, jobs, nj, P, total_pix);
   return hipGetLastError();
+}
+
+namespace {
+// The persistent kernel of a batch's flags (flags bit 0 transparent, 1
+// motion, 2 quadrics).
+using PersistFn = void (*)(const PtJob*, uint32_t, PathBuffers, uint32_t, uint32_t*, uint32_t*, unsigned long long*);
+PersistFn persist_fn(uint32_t flags) {
+  switch (flags & 7u) {
+    case 0: return batch_persistent_kernel<false, false, false>;
+    case 1: return batch_persistent_kernel<true, false, false>;
+    case 2: return batch_persistent_kernel<false, true, false>;
+    case 3: return batch_persistent_kernel<true, true, false>;
+    case 4: return batch_persistent_kernel<false, false, true>;
+    case 5: return batch_persistent_kernel<true, false, true>;
+    case 6: return batch_persistent_kernel<false, true, true>;
+    default: return batch_persistent_kernel<true, true, true>;
+  }
+}
+}  // namespace
+
+hipError_t pt_launch_batch_persistent(const PtJob* jobs, uint32_t nj, const PathBuffers& P, uint32_t total_items,
+                                      uint32_t lanes, uint32_t flags, uint32_t* next_item, uint32_t* job_left,
+                                      unsigned long long* job_clock, hipStream_t st) {
+  hipLaunchKernelGGL(persist_fn(flags), dim3(lanes / kTraceBlock), dim3(kTraceBlock), 0, st, jobs, nj, P,
+                     total_items, next_item, job_left, job_clock);
+  return hipGetLastError();
+}
+
+uint32_t pt_batch_persistent_lanes(uint32_t flags) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persist_fn(flags), kTraceBlock, 0) != hipSuccess)
+    return 0;
+  if (per_cu <= 0 || cus <= 0) return 0;
+  return (uint32_t)(per_cu * cus) * kTraceBlock;
 }
 
 hipError_t pt_launch_pcg32_draws(const uint64_t* pixel, const uint64_t* sample, uint32_t n, uint32_t count,

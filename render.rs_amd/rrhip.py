@@ -48,7 +48,7 @@ EXPORTED = [
     "rrhip_session_set_adaptive", "rrhip_session_average_spp",
     "rrhip_session_set_film", "rrhip_read_film", "rrhip_pcg32_draws",
     "rrhip_trace_closest_noalloc", "rrhip_session_set_profiling", "rrhip_session_get_profile",
-    "rrhip_nrc_time_steps",
+    "rrhip_nrc_time_steps", "rrhip_batch_render_persistent",
 ]
 
 
@@ -318,6 +318,7 @@ def load():
         "rrhip_batch_create": (i32, [vp, u32, vp, u32, ctypes.POINTER(vp)]),
         "rrhip_batch_destroy": (i32, [vp]),
         "rrhip_batch_render": (i32, [vp, vp, ctypes.POINTER(ctypes.c_double)]),
+        "rrhip_batch_render_persistent": (i32, [vp, vp, ctypes.POINTER(ctypes.c_double)]),
         "rrhip_batch_read_accum": (i32, [vp, u32, vp]),
         "rrhip_batch_get_stats": (i32, [vp, ctypes.POINTER(SessionStats)]),
         "rrhip_gen_density_grid": (i32, [u32, u64, ctypes.c_float, u32, ctypes.c_float, ctypes.c_float,
@@ -811,6 +812,14 @@ class Batch:
         job_ms = np.zeros(self.n)
         total = ctypes.c_double()
         _check(load().rrhip_batch_render(self.h, _ptr(job_ms), ctypes.byref(total)))
+        return job_ms, total.value
+
+    def render_persistent(self):
+        """Every job in one persistent launch: (per-job completion ms from
+        the launch start, total ms)."""
+        job_ms = np.zeros(self.n)
+        total = ctypes.c_double()
+        _check(load().rrhip_batch_render_persistent(self.h, _ptr(job_ms), ctypes.byref(total)))
         return job_ms, total.value
 
     def accum(self, job):

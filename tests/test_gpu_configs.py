@@ -9,7 +9,8 @@
 * batched jobs (32 rooms, 64^2-512^2, 4-64 spp, every other one with a
   128^3 fBm cloud): each job's accumulator equals its own session's bit
   for bit, and four jobs (two clouds, two surface rooms) match the
-  oracle;
+  oracle; the same in one persistent launch (those four bit for bit
+  against sessions rendering them in one pass, and against the oracle);
 * radiance cache on the instanced field at 1920x1080 with the default
   cache (65,536 training paths, 16 optimiser steps of 4,096 samples per
   frame): finite non-negative images, exactly 65,536 training samples per
@@ -98,11 +99,29 @@ def test_batched_rooms_full_workload(ctx):
     rooms = [j for j, (s, _, _) in enumerate(jobs) if not s.media]
     cost = {j: jobs[j][0].npix * jobs[j][1] for j in range(len(jobs))}
     pick = sorted(clouds, key=cost.get)[:2] + sorted(rooms, key=cost.get)[:2]
+    oracle = {}
     for j in pick:
         s, spp, d = jobs[j]
-        acc_c, _ = oracle_render(s.desc(), rrhip.RenderParams.make(max_depth=d), 0, spp)
-        check_parity(b.accum(j), acc_c, f"job {j} ({s.camera.width}^2, {spp} spp, "
-                                        f"{'cloud' if s.media else 'surfaces'})")
+        oracle[j], _ = oracle_render(s.desc(), rrhip.RenderParams.make(max_depth=d), 0, spp)
+        check_parity(b.accum(j), oracle[j], f"job {j} ({s.camera.width}^2, {spp} spp, "
+                                            f"{'cloud' if s.media else 'surfaces'})")
+    # the same workload in one persistent launch: every job's sample count,
+    # the ray totals, the picked jobs bit for bit against sessions rendering
+    # them in one pass, and against the oracle
+    say("the 32 rooms in one persistent launch")
+    job_ms, total = b.render_persistent()
+    pst = b.stats()
+    say(f"persistent: {total:.1f} ms, job ms p50 {np.percentile(job_ms, 50):.1f} p99 {np.percentile(job_ms, 99):.1f}")
+    assert pst["extension_rays"] + pst["shadow_rays"] > 0
+    for j, (s, spp, d) in enumerate(jobs):
+        assert (b.accum(j)[:, 6] == spp).all()
+    for j in pick:
+        s, spp, d = jobs[j]
+        se = pss[j].session(rrhip.RenderParams.make(max_depth=d, samples_per_pass=spp))
+        se.render(0, spp)
+        assert b.accum(j).tobytes() == se.accum().tobytes(), f"job {j}: persistent launch differs from its session"
+        se.close()
+        check_parity(b.accum(j), oracle[j], f"persistent job {j}")
     b.close()
     for ps in pss:
         ps.close()
