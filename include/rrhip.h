@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define RRHIP_ABI_VERSION 6u
+#define RRHIP_ABI_VERSION 7u
 #define RRHIP_NO_HIT 0xFFFFFFFFu
 
 /* Status codes. render.rs returns anyhow::Result with a message
@@ -88,16 +88,35 @@ typedef struct rrhip_mesh_desc {
   const float* normals;
 } rrhip_mesh_desc;
 
-/* A placed copy of a mesh: row-major 3x4 object-to-world affine transform
- * and a material (render.rs Instance, src/geometry/mesh.rs:223-305). */
+/* A set of curve segments (render.rs CurveSet, src/geometry/curves.rs:
+ * 15-26): each segment is a rounded cone — a sphere swept from p0 to p1
+ * with its radius lerped between them, spherical caps at both ends — given
+ * as xyz + radius at each end (4 floats per end), and the curve parameter v
+ * at both ends (0 root, 1 tip). A point (p0 == p1) is a sphere. Curves and
+ * points are diced into segments by the caller (curves.rs:191-258,
+ * scene_builder.rs:1050-1170). */
+typedef struct rrhip_curves_desc {
+  const float* p0;
+  const float* p1;
+  const float* v0;
+  const float* v1;
+  uint64_t num_segments;
+} rrhip_curves_desc;
+
+/* Instance kinds (render.rs GeomKind, src/geometry/mesh.rs:223-240). */
+enum { RRHIP_INSTANCE_MESH = 0, RRHIP_INSTANCE_CURVES = 1 };
+
+/* A placed copy of a mesh or curve set: row-major 3x4 object-to-world
+ * affine transform and a material (render.rs Instance,
+ * src/geometry/mesh.rs:223-305). */
 typedef struct rrhip_instance_desc {
-  uint32_t mesh;
+  uint32_t mesh;                /* index into meshes, or into curve_sets for RRHIP_INSTANCE_CURVES */
   uint32_t material;
   double object_to_world[12];
   double object_to_world1[12];  /* at shutter close (Instance::transform1) when `moving`; rays at
                                    time t see the element-wise lerp of the two (mesh.rs:294-305) */
   uint32_t moving;
-  uint32_t reserved;
+  uint32_t kind;                /* RRHIP_INSTANCE_MESH or RRHIP_INSTANCE_CURVES */
 } rrhip_instance_desc;
 
 /* Loose world-space triangle (render.rs builds these from Polygon,
@@ -168,6 +187,14 @@ typedef struct rrhip_material_desc {
   uint32_t id;                    /* object id for the id AOV (Material::id); 0 = none */
   double cutout;                  /* 1 - presence (PbrParams::presence): the probability that
                                      a ray passes through the surface; 0 = opaque */
+  uint32_t hair;                  /* 1: a hair fiber (Material::hair, Bxdf "PxrMarschnerHair"):
+                                     curve hits scatter with render.rs's R / TT / TRT fiber BSDF
+                                     (src/raytracer/pt/hair.rs); other hits use the lobes above */
+  uint32_t reserved_hair;
+  double hair_sigma_a[3];         /* absorption per unit fiber diameter (HairParams::sigma_a) */
+  double hair_beta_m;             /* longitudinal roughness, clamped to [0.01, 1] */
+  double hair_beta_n;             /* azimuthal roughness, clamped to [0.01, 1] */
+  double hair_eta;                /* fiber index of refraction */
 } rrhip_material_desc;
 
 /* Light kinds (render.rs LightType, src/scene/light.rs:7-33). */
@@ -276,6 +303,10 @@ typedef struct rrhip_scene_desc {
   uint32_t num_quadrics;
   uint32_t reserved;
   const rrhip_quadric_desc* quadrics;
+  /* Curve sets, placed by instances of kind RRHIP_INSTANCE_CURVES. */
+  uint32_t num_curve_sets;
+  uint32_t reserved_curves;
+  const rrhip_curves_desc* curve_sets;
 } rrhip_scene_desc;
 
 /* Path-tracing parameters (render.rs constants, src/raytracer/pt/mod.rs:
@@ -551,6 +582,18 @@ int rrhip_relmse(rrhip_session* session, rrhip_session* reference, double eps, d
  * lets a caller check the device stream against the CPU path's. */
 int rrhip_pcg32_draws(rrhip_ctx* ctx, const uint64_t* pixels, const uint64_t* samples, uint32_t n,
                       uint32_t count, uint32_t* out);
+
+/* Diagnostic: render.rs's hair fiber BSDF (src/raytracer/pt/hair.rs:
+ * f, pdf, sample) as the path tracer's shade kernel evaluates it on the
+ * device, for n queries in the fiber frame (x along the fiber): f (rgb) and
+ * pdf towards wi[3i..] from wo[3i..] at azimuthal offset h[i], and a sample
+ * drawn from the uniforms u[4i..4i+3] in hair.rs's order (lobe, two
+ * longitudinal, azimuthal): its direction ws, value fs and pdf pdfs (0 where
+ * sampling fails). `material` must have hair = 1. Host arrays; lets a
+ * caller check the device fiber BSDF against the CPU path's. */
+int rrhip_hair_bsdf(rrhip_ctx* ctx, const rrhip_material_desc* material, uint32_t n, const float* wo,
+                    const float* wi, const float* h, const float* u, float* f, float* pdf, float* ws, float* fs,
+                    float* pdfs);
 
 /* ---- neural radiance cache ------------------------------------------------ */
 /* render.rs has no radiance cache: this section is the learned part of the

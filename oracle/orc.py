@@ -174,6 +174,7 @@ def _pt_sigs():
         "orc_light_sampler_pmf": (f64, [vp, vp, u32]),
         "orc_scene_intersect": (i32, [vp, vp, vp, ctypes.POINTER(f64), vp, vp, ctypes.POINTER(i32),
                                       ctypes.POINTER(u32)]),
+        "orc_scene_intersect_n": (None, [vp, vp, u64, vp, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -340,6 +341,18 @@ def scene_intersect(desc, o, d):
                                    _p(normal), ctypes.byref(front), ctypes.byref(mat)):
         return None
     return t.value, point, normal, bool(front.value), mat.value
+
+
+def scene_intersect_n(desc, rays):
+    """Closest hits of rays (n x 6: origin, direction) in one scene:
+    (t (-1 on a miss), normal, tangent, front, material)."""
+    lib = _pt_sigs()
+    r = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
+    n = r.shape[0]
+    t, nrm, tan = np.zeros(n), np.zeros((n, 3)), np.zeros((n, 3))
+    front, mat = np.zeros(n, np.int32), np.zeros(n, np.uint32)
+    lib.orc_scene_intersect_n(ctypes.addressof(desc), _p(r), n, _p(t), _p(nrm), _p(tan), _p(front), _p(mat))
+    return t, nrm, tan, front.astype(bool), mat
 
 
 # ---- neural radiance cache ---------------------------------------------------
@@ -541,3 +554,133 @@ def hg_sample(g, wo, seed, stream, n):
     pdf = np.zeros(n)
     _med_sigs().orc_hg_sample_n(g, _p(wo), seed, stream, n, _p(wi), _p(pdf))
     return wi, pdf
+
+
+# ---- curves and hair (curves.rs, hair.rs) ---------------------------------------------
+
+def _hair_sigs():
+    lib = load()
+    if getattr(lib, "_hair_ready", False):
+        return lib
+    vp, u64, i32, f64 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_double
+    sig = {
+        "orc_curve_intersect": (None, [vp, vp, ctypes.c_float, ctypes.c_float, vp, u64, f64, vp, vp, vp, vp]),
+        "orc_dice_curve": (u64, [vp, u64, vp, u64, f64, f64, u64, vp, vp, vp, vp, u64]),
+        "orc_hair_params": (None, [vp, vp, ctypes.POINTER(f64)]),
+        "orc_hair_sigma_a_from_color": (None, [vp, f64, vp]),
+        "orc_hair_scalar": (None, [i32, vp, u64, f64, vp]),
+        "orc_hair_mp": (None, [vp, u64, vp]),
+        "orc_hair_ap": (None, [f64, f64, f64, vp, vp]),
+        "orc_hair_fresnel": (f64, [f64, f64, f64]),
+        "orc_hair_eval": (None, [vp, vp, vp, vp, u64, vp, vp]),
+        "orc_hair_sample": (None, [vp, vp, vp, vp, u64, vp, vp, vp, vp]),
+        "orc_fiber_frame": (None, [vp, vp, vp, vp, ctypes.POINTER(f64)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    lib._hair_ready = True
+    return lib
+
+
+def curve_intersect(p0, p1, rays, v0=0.0, v1=1.0, t_max=np.inf):
+    """One rounded-cone segment (p0, p1: xyz + radius) against rays
+    (n x 6: origin, direction). Returns (t (-1 on a miss), normal, tangent, v)."""
+    a = np.ascontiguousarray(p0, np.float32)
+    b = np.ascontiguousarray(p1, np.float32)
+    r = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
+    n = r.shape[0]
+    t, nrm, tan, v = np.zeros(n), np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+    _hair_sigs().orc_curve_intersect(_p(a), _p(b), v0, v1, _p(r), n, float(t_max), _p(t), _p(nrm), _p(tan), _p(v))
+    return t, nrm, tan, v
+
+
+def dice_curve(ctrl, basis=None, step=3, width_root=0.01, width_tip=0.01, segs=8):
+    """curves.rs:191-258. Returns (p0, p1 (n x 4 float32), v0, v1)."""
+    c = np.ascontiguousarray(ctrl, np.float64).reshape(-1, 3)
+    bas = None if basis is None else np.ascontiguousarray(basis, np.float64).reshape(16)
+    cap = max(1, c.shape[0] * (segs + 1))
+    p0, p1 = np.zeros((cap, 4), np.float32), np.zeros((cap, 4), np.float32)
+    v0, v1 = np.zeros(cap, np.float32), np.zeros(cap, np.float32)
+    m = _hair_sigs().orc_dice_curve(_p(c), c.shape[0], None if bas is None else _p(bas), step, width_root,
+                                    width_tip, segs, _p(p0), _p(p1), _p(v0), _p(v1), cap)
+    return p0[:m], p1[:m], v0[:m], v1[:m]
+
+
+def _hp(sigma_a, beta_m, beta_n, eta):
+    return np.array([*sigma_a, beta_m, beta_n, eta], np.float64)
+
+
+def hair_params(sigma_a=(0, 0, 0), beta_m=0.3, beta_n=0.3, eta=1.55):
+    """(v[4], s) of HairParams::new."""
+    v = np.zeros(4)
+    s = ctypes.c_double()
+    _hair_sigs().orc_hair_params(_p(_hp(sigma_a, beta_m, beta_n, eta)), _p(v), ctypes.byref(s))
+    return v, s.value
+
+
+def hair_sigma_a_from_color(c, beta_n):
+    out = np.zeros(3)
+    _hair_sigs().orc_hair_sigma_a_from_color(_p(np.ascontiguousarray(c, np.float64)), beta_n, _p(out))
+    return out
+
+
+def hair_scalar(which, x, a=0.0):
+    """which: 'i0', 'log_i0', 'logistic', 'logistic_cdf', 'trimmed_logistic',
+    'sample_trimmed_logistic' (a = the logistic scale)."""
+    k = ["i0", "log_i0", "logistic", "logistic_cdf", "trimmed_logistic", "sample_trimmed_logistic"].index(which)
+    x = np.ascontiguousarray(x, np.float64).ravel()
+    out = np.zeros_like(x)
+    _hair_sigs().orc_hair_scalar(k, _p(x), x.size, float(a), _p(out))
+    return out
+
+
+def hair_mp(cos_ti, cos_to, sin_ti, sin_to, v):
+    rows = np.ascontiguousarray(np.broadcast_arrays(cos_ti, cos_to, sin_ti, sin_to, v), np.float64)
+    rows = np.ascontiguousarray(rows.reshape(5, -1).T)
+    out = np.zeros(rows.shape[0])
+    _hair_sigs().orc_hair_mp(_p(rows), rows.shape[0], _p(out))
+    return out
+
+
+def hair_ap(cos_to, eta, h, t):
+    out = np.zeros((4, 3))
+    _hair_sigs().orc_hair_ap(cos_to, eta, h, _p(np.ascontiguousarray(t, np.float64)), _p(out))
+    return out
+
+
+def hair_fresnel(cos_i, eta_i, eta_t):
+    return _hair_sigs().orc_hair_fresnel(cos_i, eta_i, eta_t)
+
+
+def hair_eval(wo, wi, h, sigma_a=(0, 0, 0), beta_m=0.3, beta_n=0.3, eta=1.55):
+    """f (n x 3) and pdf (n) in the fiber frame."""
+    wo = np.ascontiguousarray(wo, np.float64).reshape(-1, 3)
+    wi = np.ascontiguousarray(wi, np.float64).reshape(-1, 3)
+    h = np.ascontiguousarray(np.broadcast_to(h, (wo.shape[0],)), np.float64)
+    f, pdf = np.zeros(wo.shape), np.zeros(wo.shape[0])
+    _hair_sigs().orc_hair_eval(_p(_hp(sigma_a, beta_m, beta_n, eta)), _p(wo), _p(wi), _p(h), wo.shape[0], _p(f),
+                               _p(pdf))
+    return f, pdf
+
+
+def hair_sample(wo, h, u, sigma_a=(0, 0, 0), beta_m=0.3, beta_n=0.3, eta=1.55):
+    """hair::sample from uniforms u (n x 4): (wi, f, pdf, ok)."""
+    u = np.ascontiguousarray(u, np.float64).reshape(-1, 4)
+    n = u.shape[0]
+    wo = np.ascontiguousarray(np.broadcast_to(wo, (n, 3)), np.float64)
+    h = np.ascontiguousarray(np.broadcast_to(h, (n,)), np.float64)
+    wi, f, pdf, ok = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n), np.zeros(n, np.uint8)
+    _hair_sigs().orc_hair_sample(_p(_hp(sigma_a, beta_m, beta_n, eta)), _p(wo), _p(h), _p(u), n, _p(wi), _p(f),
+                                 _p(pdf), _p(ok))
+    return wi, f, pdf, ok.astype(bool)
+
+
+def fiber_frame(tangent, n, wo):
+    """(x, y, z rows, h) of FiberFrame::new."""
+    xyz = np.zeros((3, 3))
+    h = ctypes.c_double()
+    _hair_sigs().orc_fiber_frame(_p(np.ascontiguousarray(tangent, np.float64)), _p(np.ascontiguousarray(n, np.float64)),
+                                 _p(np.ascontiguousarray(wo, np.float64)), _p(xyz), ctypes.byref(h))
+    return xyz, h.value

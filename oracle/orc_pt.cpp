@@ -747,9 +747,31 @@ Mat4 InstanceObj::inverse_at(double time) const {
 
 // mesh.rs:308-381 -> the parametric hit and world normal of one instance
 bool Scene::instance_hit(const InstanceObj& inst, const Ray& ray, double t_max, double* t_param,
-                         Vec3* world_normal) const {
+                         Vec3* world_normal, Vec3* world_tangent) const {
   const Mat4 inverse = inst.inverse_at(ray.time);
   Ray local = Ray::raw(inverse.transform_point(ray.origin), inverse.transform_vec(ray.direction));
+  if (world_tangent) *world_tangent = Vec3::zero();
+  if (inst.curves) {  // mesh.rs:324-343: the closest segment, its normal and tangent
+    const CurveSet& cs = curve_sets[inst.mesh];
+    Vec3 bn, bt;
+    auto isect = [&](uint32_t seg, double tcur, double* tout) {
+      double tt, vv;
+      Vec3 nn, tg;
+      if (!cs.intersect_segment(local, seg, tcur, &tt, &nn, &tg, &vv)) return false;
+      *tout = tt;
+      return true;
+    };
+    uint32_t seg;
+    double t;
+    if (!cs.blas.traverse(local, t_max, isect, &seg, &t)) return false;
+    double tt, vv;
+    cs.intersect_segment(local, seg, INF, &tt, &bn, &bt, &vv);
+    *t_param = t;
+    *world_normal = inverse.transform_normal(bn).normalize();
+    // the tangent goes through the shutter-open transform (mesh.rs:334)
+    if (world_tangent) *world_tangent = inst.transform.transform_vec(bt).normalize();
+    return true;
+  }
   const MeshObj& mo = meshes[inst.mesh];
   const Mesh& m = mo.mesh;
   double hu = 0, hv = 0;
@@ -1145,7 +1167,8 @@ bool Scene::intersect(const Ray& ray, Intersection* out) const {
     if (tlas.traverse(ray, budget, isect, &win, &wt)) {
       best_inst = win;
       best_t = wt;
-      instance_hit(instances[best_inst], ray, INF, &best_t, &best_n);
+      Vec3 best_tan;
+      instance_hit(instances[best_inst], ray, INF, &best_t, &best_n, &best_tan);
       // mesh.rs:427-439 (to_intersection)
       Vec3 p = ray.at(best_t);
       Vec3 n = best_n;
@@ -1158,6 +1181,7 @@ bool Scene::intersect(const Ray& ray, Intersection* out) const {
         out->normal = n.normalize();
         out->front_face = front;
         out->material = instances[best_inst].material;
+        out->tangent = best_tan;
         found = true;
       }
     }
@@ -1743,6 +1767,45 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays, A
       }
       emit(beta * emission * weight);
     }
+    // Hair fibers scatter over the whole sphere with the fiber BSDF, in the
+    // fiber frame of the curve hit (pt/mod.rs:738-800): NEE with f and its
+    // pdf (no cosine), then a sampled continuation that leaves from the
+    // side of the normal it points to.
+    if (mat.has_hair && hit.tangent.length_squared() > 0.5) {
+      const FiberFrame ff(hit.tangent, hit.normal, wo);
+      const Vec3 wo_f = ff.to_local(wo);
+      const HairParams& hp = mat.hair;
+      if (!lights.empty()) {
+        const EvalFn eval = [&](const Vec3& wi_world, Vec3* f, double* pdf) {
+          const Vec3 wi_f = ff.to_local(wi_world);
+          *f = hair::f(hp, wo_f, wi_f, ff.h);
+          *pdf = hair::pdf(hp, wo_f, wi_f, ff.h);
+        };
+        size_t li;
+        double pmf;
+        if (light_sampler.sample(hit.point, rng.next_f64(), &li, &pmf))
+          emit(beta * sample_light(lights[li], hit.point, n, eval, pmf, rng, rays, medium, time));
+      }
+      Vec3 wi_f, fv;
+      double pv;
+      if (!hair::sample(hp, wo_f, ff.h, rng, &wi_f, &fv, &pv)) break;
+      const Vec3 wi_world = ff.to_world(wi_f);
+      beta = beta * fv / pv;
+      if (first_specular < 0) first_specular = 0;
+      prev_pdf = pv;
+      prev_origin = hit.point;
+      from_camera = false;
+      const Vec3 side = wi_world.dot(hit.normal) >= 0.0 ? hit.normal : -hit.normal;
+      ray = Ray::make(hit.point + side * ray_offset, wi_world);
+      ray.time = time;
+      depth++;
+      if (depth >= rp.rr_start) {
+        const double q = std::fmin(max_component(beta), 0.95);
+        if (q <= 0.0 || rng.next_f64() > q) break;
+        beta = beta / q;
+      }
+      continue;
+    }
     Frame frame(n);
     Vec3 wo_l = frame.to_local(wo);
     double w5[5];
@@ -1897,6 +1960,10 @@ void Scene::load(const rrhip_scene_desc& d) {
     mt.presence = 1.0 - m.cutout;
     mt.interior = (int)m.interior - 1;
     mt.id = m.id;
+    if (m.hair) {  // hair.rs:40-53
+      mt.has_hair = true;
+      mt.hair = HairParams::make(v3(m.hair_sigma_a), m.hair_beta_m, m.hair_beta_n, m.hair_eta);
+    }
     materials.push_back(mt);
   }
 
@@ -1988,6 +2055,12 @@ void Scene::load(const rrhip_scene_desc& d) {
     meshes[i].mesh.build(md.positions, md.num_vertices, md.indices, md.num_triangles, bp);
     if (md.normals) meshes[i].normals.assign(md.normals, md.normals + 3 * md.num_vertices);
   }
+  curve_sets.clear();
+  curve_sets.resize(d.num_curve_sets);
+  for (uint32_t i = 0; i < d.num_curve_sets; i++) {
+    const rrhip_curves_desc& cd = d.curve_sets[i];
+    curve_sets[i].build(cd.p0, cd.p1, cd.v0, cd.v1, (size_t)cd.num_segments);
+  }
   instances.clear();
   std::vector<Aabb> ib;
   for (uint32_t i = 0; i < d.num_instances; i++) {
@@ -1995,6 +2068,7 @@ void Scene::load(const rrhip_scene_desc& d) {
     InstanceObj in;
     in.mesh = id.mesh;
     in.material = id.material;
+    in.curves = id.kind == RRHIP_INSTANCE_CURVES;
     in.transform = Mat4::identity();
     for (int r = 0; r < 3; r++)
       for (int cc = 0; cc < 4; cc++) in.transform.m[r][cc] = id.object_to_world[4 * r + cc];
@@ -2006,7 +2080,7 @@ void Scene::load(const rrhip_scene_desc& d) {
         for (int cc = 0; cc < 4; cc++) in.transform1.m[r][cc] = id.object_to_world1[4 * r + cc];
     // mesh.rs:266-292 world bounds from the 8 transformed corners, at both
     // shutter ends when moving
-    const Aabb& lb = meshes[in.mesh].mesh.local_bounds;
+    const Aabb& lb = in.curves ? curve_sets[in.mesh].local_bounds : meshes[in.mesh].mesh.local_bounds;
     Aabb wb;
     for (int e = 0; e < (in.moving ? 2 : 1); e++)
       for (int k = 0; k < 8; k++) {

@@ -1,11 +1,11 @@
 // ORACLE — test infrastructure only (see orc_math.hpp header).
 //
 // f64 restatement of render.rs's CPU path tracer for the subset the
-// benchmark scenes use: src/raytracer/pt/mod.rs (trace_full with a global
-// atmosphere medium; no hulls, hair, subsurface, presence or textures),
-// src/raytracer/pt/{bxdf,volume}.rs, src/scene/medium.rs,
+// backend renders: src/raytracer/pt/mod.rs (trace_full with media, volume
+// hulls, presence cutouts and hair fibers; no subsurface or textures),
+// src/raytracer/pt/{bxdf,volume,hair}.rs, src/scene/medium.rs,
 // src/scene/{pbr,light,light_sampler,camera}.rs, the Sphere / Triangle /
-// Instance intersectors and Scene::intersect.
+// quadric / curve / Instance intersectors and Scene::intersect.
 #pragma once
 #include <cstdint>
 #include <functional>
@@ -14,6 +14,7 @@
 
 #include "../include/rrhip.h"
 #include "orc_accel.hpp"
+#include "orc_hair.hpp"
 #include "orc_sampling.hpp"
 
 namespace orc {
@@ -49,6 +50,8 @@ struct Material {
   double presence = 1.0;  // PbrParams::presence (pbr.rs:39)
   int interior = -1;      // Material::interior (material.rs:37)
   uint32_t id = 0;        // Material::id, the id AOV's object id (material.rs:39)
+  bool has_hair = false;  // Material::hair (material.rs): curve hits scatter with the fiber BSDF
+  HairParams hair;
   // pt/mod.rs:662-667: an interior medium and no lobes or emission.
   bool hull() const;
 };
@@ -175,6 +178,7 @@ struct Intersection {
   Vec3 point, normal;
   bool front_face = true;
   uint32_t material = 0;
+  Vec3 tangent;  // curve hits: the world tangent (zero for surfaces)
 };
 
 struct TriangleObj {
@@ -206,7 +210,8 @@ struct MeshObj {
 };
 
 struct InstanceObj {
-  uint32_t mesh, material;
+  uint32_t mesh, material;  // mesh: index of the mesh, or of the curve set when `curves`
+  bool curves = false;      // GeomKind::Curves (mesh.rs:223-240)
   Mat4 transform, inverse;
   bool moving = false;
   Mat4 transform1;  // at shutter close
@@ -260,6 +265,7 @@ class Scene {
   std::vector<SphereObj> spheres;
   std::vector<QuadricObj> quadrics;
   std::vector<MeshObj> meshes;
+  std::vector<CurveSet> curve_sets;
   std::vector<InstanceObj> instances;
   Bvh tlas;
   std::vector<Light> lights;
@@ -284,7 +290,7 @@ class Scene {
 
  private:
   bool instance_hit(const InstanceObj& inst, const Ray& ray, double t_max, double* t_param,
-                    Vec3* world_normal) const;
+                    Vec3* world_normal, Vec3* world_tangent = nullptr) const;
   // eval(wi, f, pdf): f * cos (or the phase value) towards world wi and the
   // pdf of the scattering strategy, for MIS.
   using EvalFn = std::function<void(const Vec3&, Vec3*, double*)>;

@@ -378,6 +378,30 @@ int orc_scene_intersect(const rrhip_scene_desc* desc, const double* o, const dou
   return 1;
 }
 
+// Closest hits of n rays (origin, direction: 6 doubles each, normalised as
+// Ray::new) in one loaded scene: t (-1 on a miss), the ray-facing normal,
+// the curve tangent (zero for surfaces), front face and material.
+void orc_scene_intersect_n(const rrhip_scene_desc* desc, const double* rays, uint64_t n, double* t,
+                           double* normal, double* tangent, int32_t* front, uint32_t* material) {
+  Scene s;
+  s.load(*desc);
+  for (uint64_t i = 0; i < n; i++) {
+    const double* r = rays + 6 * i;
+    Intersection h;
+    if (!s.intersect(Ray::make(Vec3{r[0], r[1], r[2]}, Vec3{r[3], r[4], r[5]}), &h)) {
+      t[i] = -1.0;
+      continue;
+    }
+    t[i] = h.t;
+    for (int k = 0; k < 3; k++) {
+      normal[3 * i + k] = h.normal[k];
+      tangent[3 * i + k] = h.tangent[k];
+    }
+    front[i] = h.front_face ? 1 : 0;
+    material[i] = h.material;
+  }
+}
+
 }  // extern "C"
 
 // ---- participating media (tests) -----------------------------------------------------

@@ -360,7 +360,10 @@ hipError_t upload_pt(rrhip_scene* s, const PtSceneHost& h) {
   if (e == hipSuccess) e = upload_vec(h.env_cond, &s->pt_mem[18], &v.env_cond);
   if (e == hipSuccess) e = upload_vec(h.quadrics, &s->pt_mem[19], &v.quadrics);
   if (e == hipSuccess) e = upload_vec(h.inst_fwd, &s->pt_mem[20], &v.inst_fwd);
+  if (e == hipSuccess) e = upload_vec(h.csegs, &s->pt_mem[21], &v.csegs);
   v.num_quadrics = (uint32_t)h.quadrics.size();
+  v.curves = h.curves;
+  v.hair = h.hair;
   v.env_w = h.env_w;
   v.env_h = h.env_h;
   v.env_total = h.env_total;

@@ -136,6 +136,8 @@ int rrhip_batch_create(rrhip_ctx* ctx, uint32_t num_jobs, const rrhip_batch_job*
     if (J.spp == 0) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_batch_create: spp must be > 0");
     if (!(J.params.ray_offset >= 0.0 && J.params.ray_offset < 1e30))
       return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_batch_create: ray_offset must be finite and >= 0");
+    if (J.scene->pt.hair)
+      return fail(RRHIP_ERR_UNSUPPORTED, "rrhip_batch_create: hair fibers render in sessions, not in batches");
     total_pix += (uint64_t)J.scene->pt.cam.width * J.scene->pt.cam.height;
   }
   uint32_t spp = samples_per_pass;
@@ -255,7 +257,7 @@ int rrhip_batch_render(rrhip_batch* b, double* job_ms, double* total_ms) {
     uint32_t total = 0, pix = 0, depth = 0;
     bool quadrics = false;
     for (const PtJob& t : pass) {
-      quadrics |= t.S.num_quadrics > 0;
+      quadrics |= t.S.num_quadrics > 0 || t.S.curves;
       total += t.prm.spp_pass * t.prm.npix;
       pix += t.prm.npix;
       depth = std::max(depth, t.prm.max_depth + (t.S.transparent ? kPtMaxSkips : 0));
@@ -295,7 +297,7 @@ hipError_t persistent_setup(rrhip_batch* b) {
   bool eq = false;
   for (const rrhip_batch_job& J : b->jobs) {
     b->p_flags |= (J.scene->pt.transparent ? 1u : 0u) | (J.scene->pt.motion ? 2u : 0u) |
-                  (J.scene->pt.num_quadrics ? 4u : 0u);
+                  (J.scene->pt.num_quadrics || J.scene->pt.curves ? 4u : 0u);
     eq |= J.scene->pt_eq;
   }
   b->p_lanes = pt_batch_persistent_lanes(b->p_flags);

@@ -514,6 +514,7 @@ struct Hit {
   // the surface (barycentrics, sphere reprojection) and the push that puts
   // an origin outside its rounding-error box, towards the ray's side.
   V3 p, e;
+  V3 tan;  // curve hits: the world tangent (zero otherwise)
 };
 constexpr uint32_t kObjTri = 0, kObjSphere = 1, kObjQuadric = 2, kObjIndex = 0x3FFFFFFFu;
 

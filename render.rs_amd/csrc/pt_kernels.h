@@ -63,6 +63,10 @@ hipError_t pt_launch_resolve(const PathBuffers& P, const PtParams& prm, double* 
 
 // out[i * count + k] = draw k of Pcg32::for_pixel_sample(pixel[i],
 // sample[i]), the stream a camera path of that pixel and sample runs on.
+// The device fiber BSDF on n queries (f, pdf towards wi; a sample from u).
+hipError_t pt_launch_hair_bsdf(const PtMaterial& m, uint32_t n, const float* wo, const float* wi, const float* h,
+                               const float* u, float* f, float* pdf, float* ws, float* fs, float* pdfs,
+                               hipStream_t st);
 hipError_t pt_launch_pcg32_draws(const uint64_t* pixel, const uint64_t* sample, uint32_t n, uint32_t count,
                                  uint32_t* out, hipStream_t st);
 

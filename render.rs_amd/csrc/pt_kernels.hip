@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include "pt_device.h"
+#include "pt_hair.h"
 #include "pt_kernels.h"
 #include "pt_trace.h"
 
@@ -193,8 +194,10 @@ __device__ __forceinline__ void extend_one(const PtSceneView& S, const PathBuffe
     P.hit_n[slot] = make_float4(h.n.x, h.n.y, h.n.z, h.t);
     P.hit_m[slot] = h.material | (h.front ? 0x80000000u : 0u);
     const V3 front = offset_origin(h.p, h.e), back = offset_origin(h.p, -h.e);
-    P.hit_front[slot] = make_float4(front.x, front.y, front.z, 0.f);
-    P.hit_back[slot] = make_float4(back.x, back.y, back.z, 0.f);
+    float2 tw = make_float2(kNoTangent, kNoTangent);
+    if (QUAD && dot(h.tan, h.tan) > 0.5f) tw = oct_encode(h.tan);
+    P.hit_front[slot] = make_float4(front.x, front.y, front.z, tw.x);
+    P.hit_back[slot] = make_float4(back.x, back.y, back.z, tw.y);
   } else {
     P.hit_n[slot] = make_float4(0.f, 0.f, 0.f, -1.f);
     P.hit_m[slot] = 0;
@@ -419,7 +422,9 @@ __device__ __forceinline__ bool sample_light(const PtSceneView& S, const PtLight
 // AOVs (pt/mod.rs:457-470, 559-573, 690-700, 959-961).
 // FIXED: render.rs's fixed push of rays off surfaces (S.ray_offset > 0)
 // instead of error-bounded origins.
-template <bool NRC, bool TR, bool FILM = false, bool FIXED = false>
+// HAIR: the scene has hair fibers; their curve hits scatter with the fiber
+// BSDF (pt/mod.rs:738-800).
+template <bool NRC, bool TR, bool FILM = false, bool FIXED = false, bool HAIR = false>
 __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffers& P, const PtParams& prm,
                                           const NrcPathBuffers& N, uint32_t slot, bool& cont, bool& shadow,
                                           bool& shadow2, ShadowOut& se, bool& query, ShadowOut& so, NrcQuery& nq,
@@ -655,6 +660,59 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
         emit(beta * em * w);
       }
       bool scatter = true;
+      if constexpr (HAIR) {
+        // A hair fiber's curve hit (the w lanes hold its tangent): NEE with
+        // the fiber BSDF's value and pdf, whose shadow rays leave from the
+        // viewer's side, then a sampled continuation leaving from the side
+        // of the normal it points to (pt/mod.rs:738-800). The surface
+        // lobes below are skipped.
+        const float4 t4 = P.hit_front[slot];
+        if (m.hair && t4.w < 2.f) {
+          scatter = false;
+          const FiberFrame ff = fiber_frame(oct_decode(t4.w, P.hit_back[slot].w), ng, wo);
+          const HairGeom g = hair_geom(m, fiber_local(ff, wo), ff.h);
+          if (S.num_lights > 0) {
+            uint32_t li;
+            float pick;
+            if (ls_sample(S, p, rnd(rng), li, pick)) {
+              const auto eval = [&](V3 wi, V3& f, float& pdf) { hair_eval(m, g, fiber_local(ff, wi), f, pdf); };
+              if (sample_light<bounded>(S, S.lights[li], p, bounded ? p : p + ng * S.ray_offset, eval, pick, rng,
+                                        so)) {
+                so.c = beta * so.c;
+                so.medium = cur;
+                if (!NRC && cur >= 0 && !walk)
+                  so.c = so.c * medium_transmittance(S.media[cur], so.o, so.d, so.tmax, rng);
+                shadow = maxc(so.c) > 0.f;
+              }
+            }
+          }
+          const float u[4] = {rnd(rng), rnd(rng), rnd(rng), rnd(rng)};
+          V3 wf, fv;
+          float pv;
+          if (hair_sample(m, g, u, wf, fv, pv)) {
+            const V3 wi = normalize(fiber_world(ff, wf));
+            beta = beta * fv / pv;
+            if (FILM && !(flags & kFlagFirstSet)) flags |= kFlagFirstSet;
+            const bool out = dot(wi, ng) >= 0.f;
+            const V3 org = bounded ? (out ? p : far_side()) : p + ng * (out ? S.ray_offset : -S.ray_offset);
+            depth++;
+            bool alive = true;
+            if (depth >= prm.rr_start) {
+              const float qq = fminf(maxc(beta), 0.95f);
+              if (qq <= 0.f || rnd(rng) > qq) {
+                alive = false;
+              } else {
+                beta = beta / qq;
+              }
+            }
+            cont = alive && depth < prm.max_depth;
+            P.o[slot] = make_float4(org.x, org.y, org.z, pv);
+            P.d[slot] = make_float4(wi.x, wi.y, wi.z, a0);
+            P.prev[slot] = make_float4(p.x, p.y, p.z, asum);
+            flags = depth | (flags & kFlagKeep);
+          }
+        }
+      }
       if constexpr (NRC) {
         const float cosv = fmaxf(fabsf(dot(ng, rd)), 1e-4f);
         const float t2 = hn.w * hn.w;
@@ -813,12 +871,12 @@ __device__ __forceinline__ void push_shadow(const PathBuffers& P, bool act, uint
   put_shadow<TR, FILM>(P, idx, slot, s1, a, s2, b);
 }
 
-// Three waves per SIMD (<= 168 VGPRs): shading is latency-bound on
-// scattered path-state loads, so occupancy matters more than registers.
-template <bool NRC, bool TR, bool FILM = false, bool FIXED = false>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void shade_kernel(PtSceneView S, PathBuffers P, PtParams prm,
-                                                      const uint32_t* __restrict__ q, uint32_t* __restrict__ qnext,
-                                                      NrcPathBuffers N) {
+// The shade pass over the queue q: each path's step, then the wave-wide
+// appends of its shadow records, continuation and cache query.
+template <bool NRC, bool TR, bool FILM, bool FIXED, bool HAIR>
+__device__ __forceinline__ void shade_pass(const PtSceneView& S, const PathBuffers& P, const PtParams& prm,
+                                           const uint32_t* __restrict__ q, uint32_t* __restrict__ qnext,
+                                           const NrcPathBuffers& N) {
   const uint32_t n = P.counts[0];
   const uint32_t stride = gridDim.x * blockDim.x;
   // Block-uniform loop bound so every lane reaches the wave-wide appends.
@@ -832,7 +890,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     uint32_t slot = 0;
     if (act) {
       slot = q[i];
-      shade_one<NRC, TR, FILM, FIXED>(S, P, prm, N, slot, cont, shadow, shadow2, se, query, so, nq, qbeta);
+      shade_one<NRC, TR, FILM, FIXED, HAIR>(S, P, prm, N, slot, cont, shadow, shadow2, se, query, so, nq, qbeta);
     }
     push_shadow<TR, FILM>(P, act, slot, shadow, so, shadow2, se);
     const uint32_t nidx = wave_append(&P.counts[1], act && cont);
@@ -846,6 +904,24 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
       }
     }
   }
+}
+
+// Three waves per SIMD (<= 168 VGPRs): shading is latency-bound on
+// scattered path-state loads, so occupancy matters more than registers.
+template <bool NRC, bool TR, bool FILM = false, bool FIXED = false>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void shade_kernel(
+    PtSceneView S, PathBuffers P, PtParams prm, const uint32_t* __restrict__ q, uint32_t* __restrict__ qnext,
+    NrcPathBuffers N) {
+  shade_pass<NRC, TR, FILM, FIXED, false>(S, P, prm, q, qnext, N);
+}
+
+// Scenes with hair fibers: the fiber BSDF's state does not fit 168
+// registers next to the surface lobes', so two waves per SIMD.
+template <bool TR, bool FILM, bool FIXED>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void shade_hair_kernel(
+    PtSceneView S, PathBuffers P, PtParams prm, const uint32_t* __restrict__ q, uint32_t* __restrict__ qnext,
+    NrcPathBuffers N) {
+  shade_pass<false, TR, FILM, FIXED, true>(S, P, prm, q, qnext, N);
 }
 
 // Fractional visibility along a shadow ray in a transparent scene
@@ -994,15 +1070,16 @@ void launch_trace_kernels(const PtSceneView& S, const PathBuffers& P, const uint
     launch_trace_counted<STACK, MOTION, QUAD, false>(S, P, q, gt, extend, st);
 }
 
-// Scenes with quadrics take the full-stack kernels with the quadric solvers
-// (a rare, small case: one instantiation per motion flag); every other
-// scene keeps kernels without them.
+// Scenes with quadrics or curve sets take the full-stack kernels with the
+// quadric and curve solvers (one instantiation per motion flag); every
+// other scene keeps kernels without them.
 void launch_trace(const PtSceneView& S, const PathBuffers& P, const uint32_t* q, uint32_t gt, bool extend,
                   hipStream_t st) {
   const bool small = S.stack_need <= 32;
-  if (S.num_quadrics && S.motion)
+  const bool special = S.num_quadrics || S.curves;
+  if (special && S.motion)
     launch_trace_kernels<kTraceStack, true, true>(S, P, q, gt, extend, st);
-  else if (S.num_quadrics)
+  else if (special)
     launch_trace_kernels<kTraceStack, false, true>(S, P, q, gt, extend, st);
   else if (small && S.motion)
     launch_trace_kernels<32, true, false>(S, P, q, gt, extend, st);
@@ -1448,21 +1525,33 @@ hipError_t pt_launch_set_u64(uint64_t* dst, uint64_t v, hipStream_t st) {
   return hipGetLastError();
 }
 
-// The shade kernel for the pass's cache, film and transparency, with
-// render.rs's fixed origin push (FIXED) or error-bounded origins.
+// The shade kernel for the pass's cache, film, transparency and hair, with
+// render.rs's fixed origin push (FIXED) or error-bounded origins (caches
+// are never attached to scenes with hair).
 template <bool FIXED>
 void launch_shade(const PtSceneView& S, const PathBuffers& P, const PtParams& prm, const uint32_t* qshade,
                   uint32_t* qnext, const NrcPathBuffers& N, uint32_t g, hipStream_t st) {
-  if (N.q)
-    hipLaunchKernelGGL((shade_kernel<true, false, false, FIXED>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
-  else if (P.ld && S.transparent)
-    hipLaunchKernelGGL((shade_kernel<false, true, true, FIXED>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
-  else if (P.ld)
-    hipLaunchKernelGGL((shade_kernel<false, false, true, FIXED>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
-  else if (S.transparent)
-    hipLaunchKernelGGL((shade_kernel<false, true, false, FIXED>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
-  else
-    hipLaunchKernelGGL((shade_kernel<false, false, false, FIXED>), dim3(g), dim3(kBlock), 0, st, S, P, prm, qshade, qnext, N);
+  const dim3 G(g), B(kBlock);
+  if (S.hair) {
+    if (P.ld && S.transparent)
+      hipLaunchKernelGGL((shade_hair_kernel<true, true, FIXED>), G, B, 0, st, S, P, prm, qshade, qnext, N);
+    else if (P.ld)
+      hipLaunchKernelGGL((shade_hair_kernel<false, true, FIXED>), G, B, 0, st, S, P, prm, qshade, qnext, N);
+    else if (S.transparent)
+      hipLaunchKernelGGL((shade_hair_kernel<true, false, FIXED>), G, B, 0, st, S, P, prm, qshade, qnext, N);
+    else
+      hipLaunchKernelGGL((shade_hair_kernel<false, false, FIXED>), G, B, 0, st, S, P, prm, qshade, qnext, N);
+  } else if (N.q) {
+    hipLaunchKernelGGL((shade_kernel<true, false, false, FIXED>), G, B, 0, st, S, P, prm, qshade, qnext, N);
+  } else if (P.ld && S.transparent) {
+    hipLaunchKernelGGL((shade_kernel<false, true, true, FIXED>), G, B, 0, st, S, P, prm, qshade, qnext, N);
+  } else if (P.ld) {
+    hipLaunchKernelGGL((shade_kernel<false, false, true, FIXED>), G, B, 0, st, S, P, prm, qshade, qnext, N);
+  } else if (S.transparent) {
+    hipLaunchKernelGGL((shade_kernel<false, true, false, FIXED>), G, B, 0, st, S, P, prm, qshade, qnext, N);
+  } else {
+    hipLaunchKernelGGL((shade_kernel<false, false, false, FIXED>), G, B, 0, st, S, P, prm, qshade, qnext, N);
+  }
 }
 
 hipError_t pt_launch_bounce(const PtSceneView& S, const PathBuffers& P, const PtParams& prm, const uint32_t* q,
@@ -1548,6 +1637,46 @@ uint32_t pt_batch_persistent_lanes(uint32_t flags) {
     return 0;
   if (per_cu <= 0 || cus <= 0) return 0;
   return (uint32_t)(per_cu * cus) * kTraceBlock;
+}
+
+namespace {
+// One fiber-BSDF query per thread (rrhip_hair_bsdf).
+__global__ __launch_bounds__(kBlock) void hair_bsdf_kernel(PtMaterial m, uint32_t n, const float* __restrict__ wo,
+                                                          const float* __restrict__ wi, const float* __restrict__ h,
+                                                          const float* __restrict__ u, float* __restrict__ f,
+                                                          float* __restrict__ pdf, float* __restrict__ ws,
+                                                          float* __restrict__ fs, float* __restrict__ pdfs) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const HairGeom g = hair_geom(m, v3(wo + 3 * (size_t)i), h[i]);
+  V3 fv;
+  float pv;
+  hair_eval(m, g, v3(wi + 3 * (size_t)i), fv, pv);
+  f[3 * (size_t)i] = fv.x;
+  f[3 * (size_t)i + 1] = fv.y;
+  f[3 * (size_t)i + 2] = fv.z;
+  pdf[i] = pv;
+  const float uu[4] = {u[4 * (size_t)i], u[4 * (size_t)i + 1], u[4 * (size_t)i + 2], u[4 * (size_t)i + 3]};
+  V3 w, sf;
+  float sp;
+  const bool ok = hair_sample(m, g, uu, w, sf, sp);
+  ws[3 * (size_t)i] = w.x;
+  ws[3 * (size_t)i + 1] = w.y;
+  ws[3 * (size_t)i + 2] = w.z;
+  fs[3 * (size_t)i] = sf.x;
+  fs[3 * (size_t)i + 1] = sf.y;
+  fs[3 * (size_t)i + 2] = sf.z;
+  pdfs[i] = ok ? sp : 0.f;
+}
+}  // namespace
+
+hipError_t pt_launch_hair_bsdf(const PtMaterial& m, uint32_t n, const float* wo, const float* wi, const float* h,
+                               const float* u, float* f, float* pdf, float* ws, float* fs, float* pdfs,
+                               hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(hair_bsdf_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, m, n, wo, wi, h, u, f,
+                     pdf, ws, fs, pdfs);
+  return hipGetLastError();
 }
 
 hipError_t pt_launch_pcg32_draws(const uint64_t* pixel, const uint64_t* sample, uint32_t n, uint32_t count,

@@ -101,7 +101,25 @@ PtMaterial make_material(const rrhip_material_desc& m) {
   g.id = m.id;
   g.interior = (int32_t)m.interior - 1;
   g.presence = (float)std::min(std::max(1.0 - m.cutout, 0.0), 1.0);
-  g.hull = g.interior >= 0 && !g.has_lobes && std::max(g.emission[0], std::max(g.emission[1], g.emission[2])) <= 0.f;
+  g.hull = !m.hair && g.interior >= 0 && !g.has_lobes &&
+           std::max(g.emission[0], std::max(g.emission[1], g.emission[2])) <= 0.f;
+  // Hair fiber (HairParams::new, hair.rs:40-53): roughness clamped to
+  // [0.01, 1], longitudinal variances per lobe, azimuthal logistic scale
+  // (sqrt(pi / 8) times a fit in beta_n).
+  g.hair = m.hair ? 1u : 0u;
+  if (m.hair) {
+    const double bm = std::min(std::max(m.hair_beta_m, 0.01), 1.0);
+    const double bn = std::min(std::max(m.hair_beta_n, 0.01), 1.0);
+    const double root = 0.726 * bm + 0.812 * bm * bm + 3.7 * std::pow(bm, 20);
+    const double v0 = root * root;
+    g.hair_v[0] = (float)v0;
+    g.hair_v[1] = (float)(0.25 * v0);
+    g.hair_v[2] = (float)(4.0 * v0);
+    g.hair_v[3] = (float)(4.0 * v0);
+    g.hair_s = (float)(0.626657069 * (0.265 * bn + 1.194 * bn * bn + 5.372 * std::pow(bn, 22)));
+    g.hair_eta = (float)m.hair_eta;
+    put(g.hair_sigma_a, d3(m.hair_sigma_a));
+  }
   return g;
 }
 
@@ -211,22 +229,76 @@ bool affine_inverse(const double m[12], double inv[12]) {
 // (render.rs: Mesh::new -> Bvh::build, Instance::build's 8-corner world
 // bounds, Scene::build_tlas; src/geometry/mesh.rs:48-84, 266-292,
 // src/scene/mod.rs:82-86). Node references are rebased into one array.
+// BLAS over a curve set's segments (curves.rs:28-62): each segment's box
+// holds both end spheres, taken from the f32 values in f64 and rounded
+// outward to f32; records in leaf order.
+bool build_curve_blas(const rrhip_curves_desc& c, const BuildParams& bp, BuiltBvh* out,
+                      std::vector<CurveRec>* recs, std::string* err) {
+  const uint64_t n = c.num_segments;
+  if (n && (!c.p0 || !c.p1 || !c.v0 || !c.v1)) return *err = "null curve arrays", false;
+  std::vector<float> boxes(6 * n);
+  for (uint64_t i = 0; i < n; i++) {
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (const float* e : {c.p0 + 4 * i, c.p1 + 4 * i}) {
+      const double r = (double)e[3];
+      if (!(r >= 0.0) || !std::isfinite(r)) return *err = "curve radii must be finite and >= 0", false;
+      for (int a = 0; a < 3; a++) {
+        lo[a] = std::min(lo[a], (double)e[a] - r);
+        hi[a] = std::max(hi[a], (double)e[a] + r);
+      }
+    }
+    for (int a = 0; a < 3; a++) {
+      boxes[6 * i + a] = std::nextafter((float)lo[a], -INFINITY);
+      boxes[6 * i + 3 + a] = std::nextafter((float)hi[a], INFINITY);
+    }
+  }
+  std::vector<uint32_t> order;
+  if (!build_box_bvh(boxes.data(), n, bp, out, &order, err)) return false;
+  recs->resize(order.size());
+  for (size_t k = 0; k < order.size(); k++) {
+    const uint64_t i = order[k];
+    CurveRec& r = (*recs)[k];
+    for (int a = 0; a < 3; a++) {
+      r.p0[a] = c.p0[4 * i + a];
+      r.p1[a] = c.p1[4 * i + a];
+    }
+    r.r0 = c.p0[4 * i + 3];
+    r.r1 = c.p1[4 * i + 3];
+    r.v0 = c.v0[i];
+    r.v1 = c.v1[i];
+    r.prim = (uint32_t)i;
+    r.pad = 0;
+  }
+  return true;
+}
+
 bool build_instances(const rrhip_scene_desc& d, PtSceneHost* out, std::string* err) {
   if (d.num_instances == 0) return true;
-  if (!d.instances || !d.meshes) return *err = "null instances or meshes", false;
+  if (!d.instances || (d.num_meshes && !d.meshes)) return *err = "null instances or meshes", false;
+  if (d.num_curve_sets && !d.curve_sets) return *err = "null curve sets", false;
   if (d.num_instances >= (1u << 28)) return *err = "too many instances (limit 2^28)", false;
   BuildParams bp;
   bp.bins = d.build.bins ? (int)d.build.bins : 32;
   bp.max_leaf = d.build.max_leaf ? (int)d.build.max_leaf : 4;
-  // BLAS per mesh.
-  std::vector<BuiltBvh> blas(d.num_meshes);
-  std::vector<std::array<float, 6>> local(d.num_meshes);
+  // BLAS per mesh, then per curve set (geometry index num_meshes + set).
+  const uint32_t ngeo = d.num_meshes + d.num_curve_sets;
+  std::vector<BuiltBvh> blas(ngeo);
+  std::vector<std::vector<CurveRec>> crecs(d.num_curve_sets);
+  std::vector<std::array<float, 6>> local(ngeo);
+  std::vector<bool> geo_empty(ngeo);
   int blas_levels = 0;
-  for (uint32_t k = 0; k < d.num_meshes; k++) {
-    const rrhip_mesh_desc& m = d.meshes[k];
-    if (m.num_triangles && (!m.positions || !m.indices)) return *err = "null mesh arrays", false;
-    if (!build_triangle_bvh(m.positions, m.num_vertices, m.indices, m.num_triangles, bp, &blas[k], err))
-      return false;
+  for (uint32_t k = 0; k < ngeo; k++) {
+    if (k < d.num_meshes) {
+      const rrhip_mesh_desc& m = d.meshes[k];
+      if (m.num_triangles && (!m.positions || !m.indices)) return *err = "null mesh arrays", false;
+      if (!build_triangle_bvh(m.positions, m.num_vertices, m.indices, m.num_triangles, bp, &blas[k], err))
+        return false;
+      geo_empty[k] = m.num_triangles == 0;
+    } else {
+      const rrhip_curves_desc& c = d.curve_sets[k - d.num_meshes];
+      if (!build_curve_blas(c, bp, &blas[k], &crecs[k - d.num_meshes], err)) return false;
+      geo_empty[k] = c.num_segments == 0;
+    }
     for (int a = 0; a < 3; a++) {
       local[k][a] = blas[k].root_lo[a];
       local[k][3 + a] = blas[k].root_hi[a];
@@ -239,14 +311,20 @@ bool build_instances(const rrhip_scene_desc& d, PtSceneHost* out, std::string* e
   std::vector<float> raw_fwd(12 * (size_t)d.num_instances);
   for (uint32_t i = 0; i < d.num_instances; i++) {
     const rrhip_instance_desc& in = d.instances[i];
-    if (in.mesh >= d.num_meshes) return *err = "instance mesh out of range", false;
+    if (in.kind != RRHIP_INSTANCE_MESH && in.kind != RRHIP_INSTANCE_CURVES)
+      return *err = "instance kind must be RRHIP_INSTANCE_MESH or RRHIP_INSTANCE_CURVES", false;
+    const bool curves = in.kind == RRHIP_INSTANCE_CURVES;
+    if (in.mesh >= (curves ? d.num_curve_sets : d.num_meshes))
+      return *err = curves ? "instance curve set out of range" : "instance mesh out of range", false;
     if (in.material >= d.num_materials) return *err = "instance material out of range", false;
+    const uint32_t geo = curves ? d.num_meshes + in.mesh : in.mesh;
+    if (curves) out->curves = 1;
     double inv[12];
     if (!affine_inverse(in.object_to_world, inv)) return *err = "instance transform is singular", false;
     PtInstance& g = raw[i];
     for (int k = 0; k < 12; k++) g.inv[k] = (float)inv[k];
     for (int k = 0; k < 12; k++) raw_fwd[12 * (size_t)i + k] = (float)in.object_to_world[k];
-    g.mesh = in.mesh;
+    g.mesh = geo;
     g.material = in.material;
     g.motion = 0xFFFFFFFFu;
     if (in.moving) {
@@ -264,8 +342,8 @@ bool build_instances(const rrhip_scene_desc& d, PtSceneHost* out, std::string* e
     // mesh.rs:274-288: world bounds from the 8 transformed corners, at both
     // shutter ends when moving (corners move linearly in between)
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    const auto& lb = local[in.mesh];
-    const bool empty = d.meshes[in.mesh].num_triangles == 0;
+    const auto& lb = local[geo];
+    const bool empty = geo_empty[geo];
     for (int e = 0; e < (in.moving ? 2 : 1); e++)
       for (int c = 0; c < 8 && !empty; c++) {
         const double p[3] = {(c & 1) ? lb[3] : lb[0], (c & 2) ? lb[4] : lb[1], (c & 4) ? lb[5] : lb[2]};
@@ -298,20 +376,30 @@ bool build_instances(const rrhip_scene_desc& d, PtSceneHost* out, std::string* e
     out->inst_fwd.insert(out->inst_fwd.end(), raw_fwd.begin() + 12 * (size_t)k, raw_fwd.begin() + 12 * (size_t)k + 12);
   }
   const int tlas_levels = tlas.nodes.empty() ? 0 : tlas.depth + 1;
-  // Concatenate the BLASes, rebasing child references.
-  out->meshes.resize(d.num_meshes);
-  for (uint32_t k = 0; k < d.num_meshes; k++) {
+  // Concatenate the BLASes, rebasing child references (leaves of meshes
+  // into the triangle records, of curve sets into the segment records).
+  out->meshes.resize(ngeo);
+  for (uint32_t k = 0; k < ngeo; k++) {
+    const bool curves = k >= d.num_meshes;
     const uint32_t node_base = (uint32_t)out->nodes.size();
-    const uint32_t tri_base = (uint32_t)out->btris.size();
+    const uint32_t rec_base = (uint32_t)(curves ? out->csegs.size() : out->btris.size());
     for (Node2 n : blas[k].nodes) {
-      n.ref0 += n.cnt0 ? tri_base : node_base;
-      n.ref1 += n.cnt1 ? tri_base : node_base;
+      n.ref0 += n.cnt0 ? rec_base : node_base;
+      n.ref1 += n.cnt1 ? rec_base : node_base;
       out->nodes.push_back(n);
+    }
+    PtMeshInfo& mi = out->meshes[k];
+    mi.root = blas[k].nodes.empty() ? 0xFFFFFFFFu : node_base;
+    mi.curves = curves ? 1u : 0u;
+    if (curves) {
+      const auto& cr = crecs[k - d.num_meshes];
+      out->csegs.insert(out->csegs.end(), cr.begin(), cr.end());
+      mi.idx_base = 0;
+      mi.nrm_base = 0xFFFFFFFFu;
+      continue;
     }
     out->btris.insert(out->btris.end(), blas[k].tris.begin(), blas[k].tris.end());
     const rrhip_mesh_desc& m = d.meshes[k];
-    PtMeshInfo& mi = out->meshes[k];
-    mi.root = blas[k].nodes.empty() ? 0xFFFFFFFFu : node_base;
     mi.idx_base = (uint32_t)out->mesh_idx.size();
     out->mesh_idx.insert(out->mesh_idx.end(), m.indices, m.indices + 3 * m.num_triangles);
     if (m.normals) {
@@ -321,7 +409,7 @@ bool build_instances(const rrhip_scene_desc& d, PtSceneHost* out, std::string* e
       mi.nrm_base = 0xFFFFFFFFu;
     }
   }
-  if (out->nodes.size() >= (1ull << 31) || out->btris.size() >= (1ull << 28))
+  if (out->nodes.size() >= (1ull << 31) || out->btris.size() >= (1ull << 28) || out->csegs.size() >= (1ull << 28))
     return *err = "instanced scene too large", false;
   // One entry per TLAS level (two at the last), one per BLAS level, and
   // the remainder of a TLAS leaf re-pushed while its first instance runs.
@@ -330,6 +418,8 @@ bool build_instances(const rrhip_scene_desc& d, PtSceneHost* out, std::string* e
 }
 
 }  // namespace
+
+PtMaterial pt_material(const rrhip_material_desc& m) { return make_material(m); }
 
 bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* err) {
   *out = PtSceneHost{};
@@ -342,7 +432,14 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
     if (m.area_light >= (int32_t)d.num_lights) return *err = "material area_light out of range", false;
     if (m.interior > d.num_media) return *err = "material interior medium out of range", false;
     if (!(m.cutout >= 0.0 && m.cutout <= 1.0)) return *err = "material cutout must be in [0, 1]", false;
+    if (m.hair > 1) return *err = "material hair must be 0 or 1", false;
+    if (m.hair && (!finite3(m.hair_sigma_a) || !(m.hair_sigma_a[0] >= 0.0 && m.hair_sigma_a[1] >= 0.0 &&
+                                                 m.hair_sigma_a[2] >= 0.0) ||
+                   !std::isfinite(m.hair_beta_m) || !std::isfinite(m.hair_beta_n) || !(m.hair_eta > 1.0) ||
+                   !std::isfinite(m.hair_eta)))
+      return *err = "hair material: sigma_a must be finite and >= 0, roughness finite, eta > 1", false;
     out->materials.push_back(make_material(m));
+    out->hair |= m.hair ? 1u : 0u;
   }
   // Lights (light.rs:59-151)
   for (uint32_t i = 0; i < d.num_lights; i++) {

@@ -33,6 +33,9 @@ struct PtSceneHost {
   uint32_t motion = 0;      // moving instances or camera motion
   std::vector<PtInstMotion> inst_motion;
   std::vector<float> inst_fwd;  // object-to-world of each instance in insts' order (12 floats)
+  std::vector<CurveRec> csegs;  // curve segments of every curve set, BLAS leaf order
+  uint32_t curves = 0;          // some instance places a curve set
+  uint32_t hair = 0;            // some material is a hair fiber
   std::vector<PtMedium> media;                // grid pointers filled at upload
   std::vector<std::vector<float>> media_grids;
   int32_t atmosphere = -1;
@@ -53,5 +56,8 @@ struct PtSceneHost {
 // (src/scene/light_sampler.rs:97-142, 242-312: median split on the widest
 // axis, power sums in f64 stored as f32). Returns false with `err` set.
 bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* err);
+
+// One material's device record (lobe constants, hair parameters).
+PtMaterial pt_material(const rrhip_material_desc& m);
 
 }  // namespace rrhip

@@ -17,6 +17,7 @@
 #include "api_internal.h"
 #include "nrc_internal.h"
 #include "pt_kernels.h"
+#include "pt_scene.h"
 #include "scene_internal.h"
 
 using namespace rrhip;
@@ -668,6 +669,41 @@ int rrhip_pcg32_draws(rrhip_ctx* ctx, const uint64_t* pixels, const uint64_t* sa
   return RRHIP_OK;
 }
 
+int rrhip_hair_bsdf(rrhip_ctx* ctx, const rrhip_material_desc* material, uint32_t n, const float* wo,
+                    const float* wi, const float* h, const float* u, float* f, float* pdf, float* ws, float* fs,
+                    float* pdfs) {
+  if (!ctx || !material) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_hair_bsdf: null argument");
+  if (n && (!wo || !wi || !h || !u || !f || !pdf || !ws || !fs || !pdfs))
+    return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_hair_bsdf: null buffer");
+  if (material->hair != 1) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_hair_bsdf: not a hair material");
+  if (n > (1u << 24)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_hair_bsdf: at most 2^24 queries");
+  if (n == 0) return RRHIP_OK;
+  const PtMaterial m = pt_material(*material);
+  RRHIP_HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  // inputs: wo, wi (3 each), h, u (4); outputs: f (3), pdf, ws (3), fs (3), pdfs
+  const size_t nn = n;
+  const size_t in_f = nn * 11, out_f = nn * 11;
+  float* d = nullptr;
+  RRHIP_HIP_TRY(hipMalloc((void**)&d, (in_f + out_f) * sizeof(float)));
+  float *dwo = d, *dwi = d + 3 * nn, *dh = d + 6 * nn, *du = d + 7 * nn;
+  float *df = d + in_f, *dpdf = df + 3 * nn, *dws = dpdf + nn, *dfs = dws + 3 * nn, *dpdfs = dfs + 3 * nn;
+  hipError_t e = hipMemcpyAsync(dwo, wo, 3 * nn * sizeof(float), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dwi, wi, 3 * nn * sizeof(float), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dh, h, nn * sizeof(float), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(du, u, 4 * nn * sizeof(float), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = pt_launch_hair_bsdf(m, n, dwo, dwi, dh, du, df, dpdf, dws, dfs, dpdfs, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(f, df, 3 * nn * sizeof(float), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(pdf, dpdf, nn * sizeof(float), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(ws, dws, 3 * nn * sizeof(float), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(fs, dfs, 3 * nn * sizeof(float), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(pdfs, dpdfs, nn * sizeof(float), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail_hip(e, "rrhip_hair_bsdf");
+  return RRHIP_OK;
+}
+
 int rrhip_session_set_profiling(rrhip_session* s, int enable) {
   if (int st = check_session(s)) return st;
   RRHIP_HIP_TRY(hipSetDevice(s->scene->ctx->device));
@@ -758,6 +794,7 @@ extern "C" int rrhip_session_set_nrc(rrhip_session* s, rrhip_nrc* nrc) {
     return fail(RRHIP_ERR_UNSUPPORTED,
                 "rrhip_session_set_nrc: the radiance cache does not model participating media or cutouts");
   if (s->film) return fail(RRHIP_ERR_UNSUPPORTED, "rrhip_session_set_nrc: not in film mode");
+  if (sc->pt.hair) return fail(RRHIP_ERR_UNSUPPORTED, "rrhip_session_set_nrc: the radiance cache does not model hair fibers");
   const uint64_t ntrain = nrc->cfg.train_paths;
   const uint64_t cap = s->slots + ntrain;
   if (cap >= (1ull << 31)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_set_nrc: path slots exceed 2^31");

@@ -11,6 +11,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "pt_curves.h"
 #include "pt_device.h"
 #include "trace_math.h"
 
@@ -128,7 +129,9 @@ __device__ __forceinline__ void instance_inverse(const PtSceneView& S, const PtI
 // Closest (ANY = false) or any (ANY = true) instance hit with t < tmax, for
 // a ray at shutter time `time` (MOTION: the scene has moving instances;
 // without it the time is ignored and the code is the static traversal).
-template <int STRIDE, bool ANY, bool MOTION, bool COUNT = false>
+// CURVES: some instances place curve sets, whose BLAS leaves index curve
+// segment records (mesh.rs:324-343, 397-402).
+template <int STRIDE, bool ANY, bool MOTION, bool COUNT = false, bool CURVES = false>
 __device__ __forceinline__ bool trace_instances(const PtSceneView& S, V3 o, V3 d, float time, float tmax,
                                                 uint32_t* stk, MeshHit& hit, TraceCount* tc = nullptr) {
   if (S.num_insts == 0) return false;
@@ -138,6 +141,7 @@ __device__ __forceinline__ bool trace_instances(const PtSceneView& S, V3 o, V3 d
   int sp = 0, base = -1;
   uint32_t node = 0, inst = 0;
   bool have = true;
+  bool curve = false;  // the instance being traversed places a curve set
   for (;;) {
     if (have) {
       if (COUNT) tc->nodes++;
@@ -156,9 +160,10 @@ __device__ __forceinline__ bool trace_instances(const PtSceneView& S, V3 o, V3 d
           stk[(sp++) * STRIDE] = kInstEntry | ((cnt - 1) << 28) | ref;
         } else {
           for (uint32_t k = ref; k < ref + cnt; k++) {
-            float t, u, v;
+            float t, u = 0.f, v = 0.f;
             if (COUNT) tc->tris++;
-            if (mesh_tri(S.btris[k], r, tmax, t, u, v)) {
+            if (CURVES && curve ? curve_seg(S.csegs[k], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, tmax, t)
+                                : mesh_tri(S.btris[k], r, tmax, t, u, v)) {
               hit.t = t;
               hit.inst = inst;
               hit.slot = k;
@@ -198,6 +203,7 @@ __device__ __forceinline__ bool trace_instances(const PtSceneView& S, V3 o, V3 d
       const PtInstance I = S.insts[inst];
       const uint32_t root = S.meshes[I.mesh].root;
       if (root == 0xFFFFFFFFu) continue;
+      if (CURVES) curve = S.meshes[I.mesh].curves != 0;
       if (!MOTION || I.motion == 0xFFFFFFFFu) {
         r = make_ray32(xform_point(I.inv, o), xform_vec(I.inv, d));
       } else {
@@ -285,15 +291,81 @@ __device__ __forceinline__ void instance_spawn(const PtSceneView& S, const MeshH
   e = bound_push(ng, err, d);
 }
 
-// Closest hit over loose objects then instances (QUAD: with quadrics).
-// SPAWN: also the winning hit's point and push (Hit::p, Hit::e).
+// A curve-set instance hit (mesh.rs:324-343, 427-439): the segment test
+// repeated in f64 on the same object-space ray for its normal, mapped to
+// world space through the inverse transpose and flipped to face the ray;
+// the world tangent through the shutter-open transform (mesh.rs:334); with
+// SPAWN the hit point o + t d (f64, rounded) mapped to world space and its
+// push, bounded like instance_spawn with gamma(2) of the object-space
+// magnitudes for the rounded point and the root.
+template <bool MOTION, bool SPAWN>
+__device__ __forceinline__ void curve_instance_hit(const PtSceneView& S, const MeshHit& h, V3 o, V3 d, float time,
+                                                   Hit& best) {
+  const PtInstance I = S.insts[h.inst];
+  const CurveRec c = S.csegs[h.slot];
+  float inv[12];
+  if (!MOTION || I.motion == 0xFFFFFFFFu) {
+#pragma unroll
+    for (int k = 0; k < 12; k++) inv[k] = I.inv[k];
+  } else {
+    instance_inverse(S, I, time, inv);
+  }
+  const Ray32 r = make_ray32(xform_point(inv, o), xform_vec(inv, d));
+  const D3 lo = {r.ox, r.oy, r.oz}, ld = {r.dx, r.dy, r.dz};
+  double t;
+  int part;
+  if (!cone_hit(c, lo, ld, INFINITY, t, part)) {  // a grazing root lost on the repeat: the traced t
+    t = (double)h.t;
+    part = 0;
+  }
+  const D3 nl = cone_normal(c, lo, ld, t, part);
+  V3 n = normalize(xform_normal(inv, v3((float)nl.x, (float)nl.y, (float)nl.z)));
+  best.front = dot(n, d) < 0.f;
+  if (!best.front) n = -n;
+  best.n = n;
+  best.material = I.material;
+  const V3 axis = v3(c.p1[0] - c.p0[0], c.p1[1] - c.p0[1], c.p1[2] - c.p0[2]);
+  const V3 tl = dot(axis, axis) > 1e-24f ? axis : v3(0.f, 0.f, 1.f);
+  best.tan = normalize(xform_vec(S.inst_fwd + 12 * (size_t)h.inst, tl));
+  if (SPAWN) {
+    float m[12];
+    if (!MOTION || I.motion == 0xFFFFFFFFu || !(time > 0.f)) {
+      const float* f = S.inst_fwd + 12 * (size_t)h.inst;
+#pragma unroll
+      for (int k = 0; k < 12; k++) m[k] = f[k];
+    } else {
+      const PtInstMotion& M = S.inst_motion[I.motion];
+#pragma unroll
+      for (int k = 0; k < 12; k++) m[k] = M.fwd0[k] + (M.fwd1[k] - M.fwd0[k]) * time;
+    }
+    const D3 pd = lo + ld * t;
+    const V3 po = v3((float)pd.x, (float)pd.y, (float)pd.z);
+    const V3 eo = (vabs(po) + vabs(v3(r.dx, r.dy, r.dz)) * (float)t) * err_gamma(2);
+    best.p = xform_point(m, po);
+    const float am[12] = {fabsf(m[0]), fabsf(m[1]), fabsf(m[2]),  fabsf(m[3]), fabsf(m[4]),  fabsf(m[5]),
+                          fabsf(m[6]), fabsf(m[7]), fabsf(m[8]), fabsf(m[9]), fabsf(m[10]), fabsf(m[11])};
+    const V3 tabs = v3(am[3], am[7], am[11]);
+    const V3 err = xform_vec(am, eo) * (1.f + err_gamma(3)) + xform_point(am, vabs(po)) * err_gamma(3) +
+                   (vabs(best.p) + tabs) * err_gamma(4);
+    best.e = bound_push(n, err, d);
+  }
+}
+
+// Closest hit over loose objects then instances (QUAD: with quadrics or
+// curve sets). SPAWN: also the winning hit's point and push (Hit::p,
+// Hit::e). Hit::tan is the world tangent of a curve hit, zero otherwise.
 template <int STRIDE, bool MOTION, bool QUAD, bool SPAWN = false, bool COUNT = false>
 __device__ __forceinline__ bool scene_closest(const PtSceneView& S, V3 o, V3 d, float time, uint32_t* stk,
                                               Hit& best, TraceCount* tc = nullptr) {
   bool found = closest_hit<QUAD>(S, o, d, best);
+  best.tan = v3(0.f, 0.f, 0.f);
   MeshHit mh;
-  if (trace_instances<STRIDE, false, MOTION, COUNT>(S, o, d, time, found ? best.t : INFINITY, stk, mh, tc)) {
+  if (trace_instances<STRIDE, false, MOTION, COUNT, QUAD>(S, o, d, time, found ? best.t : INFINITY, stk, mh, tc)) {
     best.t = mh.t;
+    if (QUAD && S.meshes[S.insts[mh.inst].mesh].curves) {
+      curve_instance_hit<MOTION, SPAWN>(S, mh, o, d, time, best);
+      return true;
+    }
     instance_normal<MOTION>(S, mh, d, time, best.n, best.front, best.material);
     if (SPAWN) instance_spawn<MOTION>(S, mh, d, time, best.p, best.e);
     return true;
@@ -307,7 +379,7 @@ __device__ __forceinline__ bool scene_occluded(const PtSceneView& S, V3 o, V3 d,
                                                uint32_t* stk, TraceCount* tc = nullptr) {
   if (occluded<QUAD>(S, o, d, tmax)) return true;
   MeshHit mh;
-  return trace_instances<STRIDE, true, MOTION, COUNT>(S, o, d, time, tmax, stk, mh, tc);
+  return trace_instances<STRIDE, true, MOTION, COUNT, QUAD>(S, o, d, time, tmax, stk, mh, tc);
 }
 
 }  // namespace pt

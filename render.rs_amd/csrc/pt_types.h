@@ -41,6 +41,16 @@ struct alignas(16) PtMaterial {
   uint32_t pad;
   float film_albedo[3]; // albedo AOV: diffuse + primary specular F0 + refraction, each <= 1
   float pad2;
+  // Hair fibers (hair.rs HairParams): curve hits scatter with the fiber
+  // BSDF instead of the lobes above. Longitudinal variances per lobe,
+  // azimuthal logistic scale, absorption, index of refraction.
+  uint32_t hair;
+  float hair_s;
+  float hair_eta;
+  float pad3;
+  float hair_v[4];
+  float hair_sigma_a[3];
+  float pad4;
 };
 
 enum : uint32_t { kLightPoint = 0, kLightDistant = 1, kLightRect = 2, kLightSphere = 3, kLightDisk = 4, kLightDome = 5 };
@@ -135,8 +145,20 @@ struct PtMeshInfo {
   uint32_t root;      // BLAS root in the shared node array
   uint32_t idx_base;  // first index of the mesh in mesh_idx
   uint32_t nrm_base;  // first normal (vertex) of the mesh in mesh_nrm; ~0u = geometric normals
+  uint32_t curves;    // 1: a curve set, its BLAS leaves index curve segment records
+};
+
+// One curve segment (curves.rs CurveSet): a rounded cone from p0 (radius
+// r0) to p1 (radius r1), the curve parameter at both ends; 48 bytes in
+// BLAS leaf order like the triangle records.
+struct alignas(16) CurveRec {
+  float p0[3], r0;
+  float p1[3], r1;
+  float v0, v1;
+  uint32_t prim;  // segment index in its set
   uint32_t pad;
 };
+static_assert(sizeof(CurveRec) == 48, "CurveRec must be 48 bytes");
 
 struct PtCamera {
   uint32_t width, height;
@@ -194,6 +216,12 @@ struct PtSceneView {
   const PtQuadric* quadrics;
   uint32_t num_quadrics;
   uint32_t num_insts;
+  // Curve segments of every curve set in BLAS leaf order; `curves`: some
+  // instance places a curve set (the trace kernels then take the variants
+  // with the quadric and curve solvers).
+  const CurveRec* csegs;
+  uint32_t curves;
+  uint32_t hair;                // some material is a hair fiber
   uint32_t num_tris, num_spheres, num_lights, num_lnodes, num_inf;
   int32_t dome;
   double p_infinite, inf_total;
@@ -213,7 +241,8 @@ struct PathBuffers {
   float4* hit_n;    // hit normal.xyz, t (t < 0: miss)
   uint32_t* hit_m;  // material | front_face << 31
   float4* hit_front; // error-bounded origins: the hit rebuilt from the surface and pushed outside its
-  float4* hit_back;  //   rounding box on the ray's side (shading point), and on the far side
+  float4* hit_back;  //   rounding box on the ray's side (shading point), and on the far side; the two
+                     //   w lanes hold a curve hit's world tangent (octahedral), 4 on other hits
   uint32_t* q0;
   uint32_t* q1;
   uint32_t* sq;     // shadow queue: slot ids

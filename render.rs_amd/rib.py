@@ -17,9 +17,13 @@ Two stages, as in render.rs:
   Pxr lights, Interior / Atmosphere media, the camera options and the id
   AOV's object ids.
 
+Curves (linear, or cubic under the current v Basis, diced by ShadingRate)
+and Points become curve sets placed by instances; `Bxdf
+"PxrMarschnerHair"` makes hair fibers of them (scene_builder.rs:685-697,
+1050-1170).
 Requests the GPU path has no counterpart for (subdivision surfaces,
-curves, patches, displacement, pattern graphs, hair, procedural density,
-deformation motion) raise `RibUnsupported` by default; with
+patches, displacement, pattern graphs, procedural density, deformation
+motion) raise `RibUnsupported` by default; with
 `strict=False` they are skipped with one warning each, as render.rs skips
 requests it does not implement (a motion block over other requests then
 keeps its first sample, as render.rs does). Transform motion blocks move
@@ -285,14 +289,30 @@ _NO_EFFECT = {"version", "WorldEnd", "FrameBegin", "FrameEnd", "Display", "Decla
               "Clipping", "ClippingPlane", "CropWindow", "FrameAspectRatio", "PixelVariance",
               "Exposure", "Quantize", "Hider", "Integrator", "TextureCoordinates",
               "ShadingInterpolation", "RelativeDetail", "Opacity", "Orientation",
-              "ReverseOrientation", "Sides", "ShadingRate", "Shutter", "Exterior",
+              "ReverseOrientation", "Sides", "Shutter", "Exterior",
               "CoordinateSystem", "ScopedCoordinateSystem", "IfBegin", "ElseIf", "Else", "IfEnd",
-              "Basis", "Attribute"}
+              "Attribute"}
 
 # Requests render.rs implements that this backend does not.
-_UNSUPPORTED = {"SubdivisionMesh", "HierarchicalSubdivisionMesh", "Curves", "Points", "PatchMesh",
+_UNSUPPORTED = {"SubdivisionMesh", "HierarchicalSubdivisionMesh", "PatchMesh",
                 "Patch", "NuPatch", "GeneralPolygon", "PointsGeneralPolygons", "Displace", "Pattern",
                 "Procedural"}
+
+# RiSpec cubic bases and their default steps (patches.rs:10-53); the
+# matrix M of p(u) = [u^3 u^2 u 1] M G, row-major.
+_BASES = {
+    "bezier": ([[-1, 3, -3, 1], [3, -6, 3, 0], [-3, 3, 0, 0], [1, 0, 0, 0]], 3),
+    "b-spline": ([[-1 / 6, 3 / 6, -3 / 6, 1 / 6], [3 / 6, -6 / 6, 3 / 6, 0], [-3 / 6, 0, 3 / 6, 0],
+                  [1 / 6, 4 / 6, 1 / 6, 0]], 1),
+    "catmull-rom": ([[-0.5, 1.5, -1.5, 0.5], [1, -2.5, 2, -0.5], [-0.5, 0, 0.5, 0], [0, 1, 0, 0]], 1),
+    "hermite": ([[2, 1, -2, 1], [-3, -2, 3, -1], [0, 1, 0, 0], [1, 0, 0, 0]], 2),
+    "power": (np.eye(4).tolist(), 4),
+}
+_BASES["bspline"] = _BASES["b-spline"]
+_BASES["catmullrom"] = _BASES["catmull-rom"]
+
+
+dice_curve = scenes.dice_curve
 
 _QUADRIC_ARGS = {"Sphere": 4, "Cylinder": 4, "Cone": 3, "Torus": 5, "Disk": 3, "Paraboloid": 4,
                  "Hyperboloid": 7}
@@ -310,6 +330,9 @@ class _State:
         self.interior = None
         self.identifier = None
         self.motion = None      # (t0, t1) of the last transform motion block
+        self.hair = None        # PxrMarschnerHair parameters as keyword arguments of scenes.hair
+        self.basis_v = _BASES["bezier"]
+        self.shading_rate = 1.0
 
     def copy(self):
         s = _State()
@@ -369,6 +392,7 @@ class Builder:
         # scene data, collected until the camera is known
         self.materials, self.lights, self.tris, self.quadrics = [], [], [], []
         self.meshes, self.instances, self.media = [], [], []
+        self.curve_sets, self.curve_instances = [], []
         self.atmosphere = -1
         self.env = None
         self.archives = {}
@@ -405,7 +429,9 @@ class Builder:
     def _material(self):
         """make_material (scene_builder.rs:1850-1879) -> a new material index."""
         st = self.state
-        if st.bxdf is not None:
+        if st.hair is not None:
+            m = scenes.hair(**st.hair)
+        elif st.bxdf is not None:
             m = scenes.material(**st.bxdf)
         elif st.surface == "plastic":
             m = scenes.plastic(st.color, st.roughness)
@@ -596,9 +622,16 @@ class Builder:
                 st.roughness = _Params(vals, 1).number("roughness", 0.1)
         elif name == "Bxdf":
             kind = _str(vals, 0) or ""
-            if kind == "PxrMarschnerHair":
-                return self._skip("Bxdf PxrMarschnerHair is not supported by this backend")
             p = _Params(vals, _pstart(vals))
+            if kind == "PxrMarschnerHair":  # HairParams::from_bxdf_params (hair.rs:79-98)
+                hp = dict(beta_m=p.number("roughness", 0.3), beta_n=p.number("azimuthalRoughness", 0.3),
+                          eta=p.number("ior", 1.55), color=p.color("color", (0.35, 0.2, 0.06)))
+                a = p.numbers("absorption")
+                if a:
+                    hp["absorption"] = a
+                st.hair, st.bxdf = hp, None
+                return
+            st.hair = None
             if kind != "PxrSurface":
                 self._skip(f'Bxdf "{kind}" treated as PxrSurface')
             if any("reference" in tok.split() for tok, _ in p.items):
@@ -615,6 +648,15 @@ class Builder:
             self._pxr_light(vals)
         elif name in ("Interior", "Atmosphere"):
             self._medium(name, vals)
+        elif name == "Basis":
+            self._basis(vals)
+        elif name == "ShadingRate":
+            if _num(vals, 0) is not None:
+                st.shading_rate = _num(vals, 0)
+        elif name == "Curves":
+            self._curves(vals)
+        elif name == "Points":
+            self._points(vals)
         elif name in _QUADRIC_ARGS:
             self._quadric_request(name, vals)
         elif name == "Polygon":
@@ -763,6 +805,95 @@ class Builder:
         else:
             self._quadric(Q.QUADRIC_HYPERBOLOID, mat, self.xf, p1=tuple(v[0:3]), p2=tuple(v[3:6]), thetamax=v[6])
 
+    def _basis(self, vals):
+        """Basis <u> <ustep> <v> <vstep> (scene_builder.rs:576-601): values
+        0 and 2 a name or a 16-number matrix, 1 and 3 the steps (default:
+        the named basis's, 1 for a matrix); only the v basis dices curves
+        here (the u basis dices patches, which this backend does not take)."""
+        def one(v, step):
+            if v is None:
+                return None
+            name = _as_str(v)
+            if name is not None:
+                if name not in _BASES:
+                    return None
+                m, dflt = _BASES[name]
+                return m, int(step) if step is not None else dflt
+            nums = _as_numbers(v)
+            if nums is None or len(nums) != 16:
+                return None
+            return np.asarray(nums, float).reshape(4, 4).tolist(), int(step) if step is not None else 1
+        b = one(vals[2] if len(vals) > 2 else None, _num(vals, 3))
+        if b is not None:
+            self.state.basis_v = b
+
+    def _curve_set(self, p0, p1, v0, v1):
+        """add_curve_set (scene_builder.rs:1156-1170): a curve set placed by an
+        instance at the current transform (moving under a motion block)."""
+        mat = self._material()
+        m1 = self._endpoint(self.xf)
+        self.curve_sets.append((p0, p1, v0, v1))
+        self.curve_instances.append((len(self.curve_sets) - 1, mat, self.xf[:3, :4].copy(),
+                                     None if m1 is None else m1[:3, :4].copy()))
+
+    def _curves(self, vals):
+        """curves_request (scene_builder.rs:1050-1119): "linear" or cubic
+        (the v basis) curves, "width" root -> tip (or "constantwidth", 0.01),
+        8 / ShadingRate segments per cubic span (2-24); "periodic" is taken
+        as nonperiodic."""
+        if self.defining is not None:
+            return self._skip("Curves inside ObjectBegin is not instanced")
+        kind = _str(vals, 0) or "cubic"
+        nverts = _as_numbers(vals[1]) if len(vals) > 1 else None
+        if nverts is None:
+            return self._skip("Curves without [nvertices]")
+        if (_str(vals, 2) or "nonperiodic") == "periodic" and "periodic" not in self.warned:
+            self.warned.add("periodic")  # render.rs warns and dices the open curve
+            warnings.warn('RIB: Curves "periodic" taken as nonperiodic')
+        p = _Params(vals, 3)
+        P = p.numbers("P")
+        if P is None:
+            return self._skip('Curves without "P"')
+        w = p.numbers("width")
+        if w:
+            wr, wt = w[0], w[-1]
+        else:
+            wr = wt = p.number("constantwidth", 0.01)
+        cubic = None if kind == "linear" else self.state.basis_v
+        segs = int(min(max(8.0 / max(self.state.shading_rate, 0.1), 2), 24))
+        parts = []
+        cur = 0
+        for nv in nverts:
+            nv = int(nv)
+            if (cur + nv) * 3 > len(P):
+                break
+            parts.append(dice_curve(np.asarray(P[3 * cur:3 * (cur + nv)], float), cubic, wr, wt, segs))
+            cur += nv
+        parts = [q for q in parts if len(q[0])]
+        if not parts:
+            return
+        self._curve_set(*(np.concatenate([q[k] for q in parts]) for k in range(4)))
+
+    def _points(self, vals):
+        """points_request (scene_builder.rs:1121-1154): each point a
+        zero-length segment of radius width / 2 ("width" per point, else
+        "constantwidth", 0.05), v = 0.5."""
+        if self.defining is not None:
+            return self._skip("Points inside ObjectBegin is not instanced")
+        p = _Params(vals, 0)
+        P = p.numbers("P")
+        if P is None:
+            return self._skip('Points without "P"')
+        n = len(P) // 3
+        if n == 0:
+            return
+        widths = p.numbers("width")
+        cw = p.number("constantwidth", 0.05)
+        w = np.array([widths[i] if widths is not None and i < len(widths) else cw for i in range(n)], float)
+        ends = np.hstack([np.asarray(P[:3 * n], float).reshape(n, 3), (w * 0.5)[:, None]]).astype(np.float32)
+        half = np.full(n, 0.5, np.float32)
+        self._curve_set(ends, ends.copy(), half, half.copy())
+
     def _polygon(self, vals):
         """scene_builder.rs:911-949: fan triangles through the current
         transform; under an AreaLightSource the material emits and a quad
@@ -862,6 +993,10 @@ class Builder:
             s.add_mesh(pos, idx, nrm)
         for mesh, mat, m, m1 in self.instances:
             s.add_instance(mesh, mat, m, m1)
+        for p0, p1, v0, v1 in self.curve_sets:
+            s.add_curves(p0, p1, v0, v1)
+        for cs, mat, m, m1 in self.curve_instances:
+            s.add_instance(cs, mat, m, m1, curves=True)
         for md in self.media:
             s.add_medium(**md)
         s.atmosphere = self.atmosphere

@@ -48,7 +48,7 @@ EXPORTED = [
     "rrhip_session_set_adaptive", "rrhip_session_average_spp",
     "rrhip_session_set_film", "rrhip_read_film", "rrhip_pcg32_draws",
     "rrhip_trace_closest_noalloc", "rrhip_session_set_profiling", "rrhip_session_get_profile",
-    "rrhip_nrc_time_steps", "rrhip_batch_render_persistent",
+    "rrhip_nrc_time_steps", "rrhip_batch_render_persistent", "rrhip_hair_bsdf",
 ]
 
 
@@ -72,10 +72,19 @@ class MeshDesc(ctypes.Structure):
 D3 = ctypes.c_double * 3
 
 
+# rrhip_instance_desc kinds
+INSTANCE_MESH, INSTANCE_CURVES = 0, 1
+
+
 class InstanceDesc(ctypes.Structure):
     _fields_ = [("mesh", ctypes.c_uint32), ("material", ctypes.c_uint32),
                 ("object_to_world", ctypes.c_double * 12), ("object_to_world1", ctypes.c_double * 12),
-                ("moving", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("moving", ctypes.c_uint32), ("kind", ctypes.c_uint32)]
+
+
+class CurvesDesc(ctypes.Structure):
+    _fields_ = [("p0", ctypes.c_void_p), ("p1", ctypes.c_void_p), ("v0", ctypes.c_void_p),
+                ("v1", ctypes.c_void_p), ("num_segments", ctypes.c_uint64)]
 
 
 class TriangleDesc(ctypes.Structure):
@@ -98,7 +107,10 @@ class MaterialDesc(ctypes.Structure):
                 ("glass_gain", ctypes.c_double), ("glass_ior", ctypes.c_double),
                 ("glass_roughness", ctypes.c_double), ("refraction_color", D3),
                 ("glow", D3), ("emission", D3), ("area_light", ctypes.c_int32),
-                ("interior", ctypes.c_uint32), ("id", ctypes.c_uint32), ("cutout", ctypes.c_double)]
+                ("interior", ctypes.c_uint32), ("id", ctypes.c_uint32), ("cutout", ctypes.c_double),
+                ("hair", ctypes.c_uint32), ("reserved_hair", ctypes.c_uint32), ("hair_sigma_a", D3),
+                ("hair_beta_m", ctypes.c_double), ("hair_beta_n", ctypes.c_double),
+                ("hair_eta", ctypes.c_double)]
 
 
 class LightDesc(ctypes.Structure):
@@ -153,7 +165,9 @@ class SceneDesc(ctypes.Structure):
                 ("env_width", ctypes.c_uint32), ("env_height", ctypes.c_uint32),
                 ("env_rgb", ctypes.POINTER(ctypes.c_float)),
                 ("num_quadrics", ctypes.c_uint32), ("reserved_q", ctypes.c_uint32),
-                ("quadrics", ctypes.POINTER(QuadricDesc))]
+                ("quadrics", ctypes.POINTER(QuadricDesc)),
+                ("num_curve_sets", ctypes.c_uint32), ("reserved_c", ctypes.c_uint32),
+                ("curve_sets", ctypes.POINTER(CurvesDesc))]
 
 
 class RenderParams(ctypes.Structure):
@@ -296,6 +310,7 @@ def load():
         "rrhip_session_get_profile": (i32, [vp, ctypes.POINTER(PtProfile)]),
         "rrhip_relmse": (i32, [vp, vp, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]),
         "rrhip_pcg32_draws": (i32, [vp, vp, vp, u32, u32, vp]),
+        "rrhip_hair_bsdf": (i32, [vp, ctypes.POINTER(MaterialDesc), u32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
         "rrhip_nrc_create": (i32, [vp, ctypes.POINTER(NrcConfig), ctypes.POINTER(vp)]),
         "rrhip_nrc_destroy": (i32, [vp]),
         "rrhip_nrc_get_info": (i32, [vp, ctypes.POINTER(NrcInfo)]),
@@ -484,6 +499,21 @@ class Context:
         out = np.empty((px.size, count), np.uint32)
         _check(load().rrhip_pcg32_draws(self.h, _ptr(px), _ptr(sm), px.size, count, _ptr(out)))
         return out
+
+    def hair_bsdf(self, material, wo, wi, h, u):
+        """The device fiber BSDF of a hair MaterialDesc on n fiber-frame
+        queries: (f, pdf) towards wi, and a sample (ws, fs, pdfs) from the
+        uniforms u (n x 4); pdfs = 0 where sampling fails."""
+        wo = np.ascontiguousarray(wo, np.float32).reshape(-1, 3)
+        n = wo.shape[0]
+        wi = np.ascontiguousarray(wi, np.float32).reshape(n, 3)
+        h = np.ascontiguousarray(np.broadcast_to(h, (n,)), np.float32)
+        u = np.ascontiguousarray(u, np.float32).reshape(n, 4)
+        f, pdf = np.empty((n, 3), np.float32), np.empty(n, np.float32)
+        ws, fs, pdfs = np.empty((n, 3), np.float32), np.empty((n, 3), np.float32), np.empty(n, np.float32)
+        _check(load().rrhip_hair_bsdf(self.h, ctypes.byref(material), n, _ptr(wo), _ptr(wi), _ptr(h), _ptr(u),
+                                      _ptr(f), _ptr(pdf), _ptr(ws), _ptr(fs), _ptr(pdfs)))
+        return f, pdf, ws, fs, pdfs
 
 
 class PtScene:

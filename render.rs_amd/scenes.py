@@ -74,6 +74,67 @@ def glass(ior=1.5, roughness=0.05):
                     glass_roughness=roughness)
 
 
+def sigma_a_from_color(color, beta_n):
+    """Fiber absorption that gives a hair volume the colour `color` under
+    azimuthal roughness beta_n (hair.rs:64-77, after Chiang et al. 2016,
+    "A practical and controllable hair and fur model", eq. 9)."""
+    b = beta_n
+    denom = 5.969 - 0.215 * b + 2.532 * b ** 2 - 10.73 * b ** 3 + 5.574 * b ** 4 + 0.245 * b ** 5
+    return tuple((np.log(np.clip(c, 1e-4, 0.999)) / denom) ** 2 for c in color)
+
+
+def hair(color=(0.35, 0.2, 0.06), beta_m=0.3, beta_n=0.3, eta=1.55, absorption=None, **kw):
+    """A hair fiber (Bxdf "PxrMarschnerHair", HairParams::from_bxdf_params,
+    hair.rs:79-98): absorption given, or from `color`; curve hits scatter
+    with the fiber BSDF, other hits see the brown matte render.rs keeps as
+    the material's surface (scene_builder.rs:1851-1857)."""
+    m = matte((0.35, 0.22, 0.12), **kw)
+    m.hair = 1
+    bn = min(max(beta_n, 0.01), 1.0)
+    if absorption is None:
+        sa = sigma_a_from_color(color, bn)
+    else:
+        a = list(absorption)
+        sa = tuple(a[:3]) if len(a) >= 3 else (a[0],) * 3
+    m.hair_sigma_a = _d3(sa)
+    m.hair_beta_m, m.hair_beta_n, m.hair_eta = beta_m, beta_n, eta
+    return m
+
+
+def dice_curve(ctrl, cubic, width_root, width_tip, segs_per_span):
+    """Dices one curve into rounded-cone segments (curves.rs:191-258):
+    the control points of a linear curve (cubic None), or for a cubic one
+    (cubic = (basis 4x4, step)) segs_per_span points per span plus the last
+    span's end; widths lerp root -> tip by sample index and become radii
+    (width / 2); v = index / (samples - 1). Returns (p0, p1) as n x 4 f32
+    (xyz, radius) and (v0, v1) as f32."""
+    ctrl = np.asarray(ctrl, np.float64).reshape(-1, 3)
+    if cubic is None:
+        pts = ctrl
+    else:
+        basis, step = np.asarray(cubic[0], np.float64).reshape(4, 4), int(cubic[1])
+        n = ctrl.shape[0]
+        if n < 4:
+            return (np.zeros((0, 4), np.float32),) * 2 + (np.zeros(0, np.float32),) * 2
+        spans = (n - 4) // step + 1
+        rows = []
+        for span in range(spans):
+            g = ctrl[span * step:span * step + 4]
+            count = segs_per_span + 1 if span == spans - 1 else segs_per_span
+            u = np.arange(count, dtype=np.float64) / segs_per_span
+            uv = np.stack([u * u * u, u * u, u, np.ones_like(u)], 1)
+            w = uv[:, 0:1] * basis[0] + uv[:, 1:2] * basis[1] + uv[:, 2:3] * basis[2] + uv[:, 3:4] * basis[3]
+            rows.append(w[:, 0:1] * g[0] + w[:, 1:2] * g[1] + w[:, 2:3] * g[2] + w[:, 3:4] * g[3])
+        pts = np.concatenate(rows)
+    if pts.shape[0] < 2:
+        return (np.zeros((0, 4), np.float32),) * 2 + (np.zeros(0, np.float32),) * 2
+    k = np.arange(pts.shape[0], dtype=np.float64) / (pts.shape[0] - 1)
+    r = (width_root + (width_tip - width_root) * k) * 0.5
+    ends = np.hstack([pts, r[:, None]]).astype(np.float32)
+    v = k.astype(np.float32)
+    return ends[:-1], ends[1:], v[:-1], v[1:]
+
+
 def light(kind, intensity=1.0, color=(1, 1, 1), p=(0, 0, 0), e1=(0, 0, 0), e2=(0, 0, 0), radius=0.0):
     l = rrhip.LightDesc()
     l.kind = kind
@@ -101,6 +162,7 @@ class Scene:
         self.background = background
         self.materials, self.lights, self.triangles, self.spheres = [], [], [], []
         self.meshes, self.instances = [], []
+        self.curve_sets = []
         self.media = []
         self.atmosphere = -1
         self._keep = []
@@ -140,12 +202,27 @@ class Scene:
                                           None if nrm is None else nrm.ctypes.data))
         return len(self.meshes) - 1
 
-    def add_instance(self, mesh, mat, object_to_world_3x4, object_to_world1_3x4=None):
-        """An instance; with object_to_world1_3x4 it moves from the first
-        transform at shutter open to the second at close."""
+    def add_curves(self, p0, p1, v0=None, v1=None):
+        """A curve set (rrhip_curves_desc): per segment the start and end
+        as xyz + radius (n x 4 each) and the curve parameter at both ends
+        (default 0 and 1). Returns its index for add_instance(kind=curves)."""
+        a = np.ascontiguousarray(p0, np.float32).reshape(-1, 4)
+        b = np.ascontiguousarray(p1, np.float32).reshape(-1, 4)
+        n = a.shape[0]
+        va = np.zeros(n, np.float32) if v0 is None else np.ascontiguousarray(v0, np.float32).reshape(n)
+        vb = np.ones(n, np.float32) if v1 is None else np.ascontiguousarray(v1, np.float32).reshape(n)
+        self._keep += [a, b, va, vb]
+        self.curve_sets.append(rrhip.CurvesDesc(a.ctypes.data, b.ctypes.data, va.ctypes.data, vb.ctypes.data, n))
+        return len(self.curve_sets) - 1
+
+    def add_instance(self, mesh, mat, object_to_world_3x4, object_to_world1_3x4=None, curves=False):
+        """An instance of mesh `mesh` (of curve set `mesh` with curves=True);
+        with object_to_world1_3x4 it moves from the first transform at
+        shutter open to the second at close."""
         i = rrhip.InstanceDesc()
         i.mesh = mesh
         i.material = mat
+        i.kind = rrhip.INSTANCE_CURVES if curves else rrhip.INSTANCE_MESH
         i.object_to_world = (ctypes.c_double * 12)(*[float(x) for x in np.asarray(object_to_world_3x4).ravel()])
         if object_to_world1_3x4 is not None:
             i.object_to_world1 = (ctypes.c_double * 12)(*[float(x) for x in
@@ -201,6 +278,8 @@ class Scene:
         qs = getattr(self, "quadrics", [])
         d.num_quadrics = len(qs)
         d.quadrics = self._arr(rrhip.QuadricDesc, qs)
+        d.num_curve_sets = len(self.curve_sets)
+        d.curve_sets = self._arr(rrhip.CurvesDesc, self.curve_sets)
         env = getattr(self, "env", None)
         if env is not None:
             env = np.ascontiguousarray(env, np.float32)
@@ -509,4 +588,93 @@ def quadric_zoo(width=96, height=54):
         radius=0.8, zmin=-0.8, zmax=0.8, thetamax=270.0)
     ground = s.add_material(matte((0.75, 0.75, 0.75)))
     s.add_polygon([(-12, -2.2, 0.5), (12, -2.2, 0.5), (12, -2.2, 18), (-12, -2.2, 18)], ground)
+    return s
+
+
+def hair_strands(n=400, seed=12345):
+    """Control points (n x 4 x 3) of the hair-ball strands of render.rs's
+    hair parity scene (tests/metal_pt_tests.rs:246-300): roots on a unit
+    sphere by golden-angle spiral in longitude and a stepped latitude
+    (z = -0.2 + 1.15 frac(0.377 i), clamped to [-0.2, 0.98]), each strand
+    growing outward by 45 % with a 0.25 droop and a jitter of 0.08 t^2 from
+    a 64-bit LCG (draws (state >> 33) / 2^31 - 1), coordinates rounded to 4
+    decimals as the RIB text carries them."""
+    state = seed
+    mask = (1 << 64) - 1
+
+    def draw():
+        nonlocal state
+        state = (state * 6364136223846793005 + 1442695040888963407) & mask
+        return (state >> 33) / float(1 << 31) - 1.0
+
+    out = np.zeros((n, 4, 3))
+    for i in range(n):
+        phi = i * 0.61803 * 2.0 * np.pi
+        z = min(max(-0.2 + 1.15 * ((i * 0.377) % 1.0), -0.2), 0.98)
+        r = np.sqrt(max(1.0 - z * z, 0.0))
+        nx, ny, nz = r * np.cos(phi), z, r * np.sin(phi)
+        for k in range(4):
+            t = k / 3.0
+            g = 1.0 + 0.45 * t
+            x = nx * g + 0.08 * t * t * draw()
+            y = ny * g - 0.25 * t * t
+            zz = nz * g + 0.08 * t * t * draw()
+            out[i, k] = [round(x, 4), round(y, 4), round(zz, 4)]
+    return out
+
+
+def hair_ball(width=96, height=96, n=400, seed=12345):
+    """render.rs's hair parity scene (tests/metal_pt_tests.rs:246-300): 400
+    linear 4-point strands (1,200 rounded-cone segments, width 0.03 root to
+    0.008 tip) of PxrMarschnerHair fiber (colour 0.5 0.3 0.12, roughness
+    0.3, azimuthal roughness 0.35) around the origin, placed at (0, 0.4, 7);
+    a grey matte ground at y = -1.3; a distant light from (4, 8, -6) towards
+    (0, 0, 6) of intensity 1.4; background (0.4, 0.45, 0.55); 35-degree
+    RenderMan camera at the origin."""
+    s = Scene(width, height, 35.0, eye=(0, 0, 0), look_at=(0, 0, 1), up=(0, 1, 0), filter=0, filter_width=1.0,
+              background=(0.4, 0.45, 0.55))
+    d = np.array([0.0, 0.0, 6.0]) - np.array([4.0, 8.0, -6.0])
+    s.add_light(light(rrhip.LIGHT_DISTANT, intensity=1.4, p=tuple(d / np.linalg.norm(d))))
+    ground = s.add_material(matte((0.6, 0.6, 0.6)))
+    s.add_polygon([(-30, -1.3, -5), (30, -1.3, -5), (30, -1.3, 40), (-30, -1.3, 40)], ground)
+    parts = [dice_curve(c, None, 0.03, 0.008, 8) for c in hair_strands(n, seed)]
+    cs = s.add_curves(*(np.concatenate([q[k] for q in parts]) for k in range(4)))
+    fur = s.add_material(hair(color=(0.5, 0.3, 0.12), beta_m=0.3, beta_n=0.35))
+    s.add_instance(cs, fur, _place((0.0, 0.4, 7.0)), curves=True)
+    return s
+
+
+BEZIER_BASIS = [[-1, 3, -3, 1], [3, -6, 3, 0], [-3, 3, 0, 0], [1, 0, 0, 0]]
+
+
+def curve_zoo(width=48, height=48, hair_fiber=True, dome=0.5):
+    """Every kind of curve geometry: a capsule, a segment tapering 0.35 ->
+    0.1, a Bezier strand diced into 8 segments (width 0.16 -> 0.04), three
+    point spheres, and a second copy of the capsule set rotated 30 degrees
+    about x and scaled by 0.6; the capsule and strand of hair fiber (or
+    matte with hair_fiber False), the rest matte; a rect light above, a
+    constant dome, a ground quad. Object ids 1 (fiber), 2 (red), 3 (blue),
+    4 (ground)."""
+    s = Scene(width, height, 40.0, eye=(0, 0.5, -6), look_at=(0, 0, 0))
+    s.add_light(light(rrhip.LIGHT_RECT, intensity=6.0, p=(-1, 3, -1), e1=(2, 0, 0), e2=(0, 0, 2)))
+    if dome > 0:
+        s.add_light(light(rrhip.LIGHT_DOME, intensity=dome))
+    fur = s.add_material(hair(color=(0.6, 0.35, 0.15), beta_m=0.35, beta_n=0.4, object_id=1) if hair_fiber
+                         else matte((0.6, 0.35, 0.15), object_id=1))
+    red = s.add_material(matte((0.8, 0.2, 0.2), object_id=2))
+    blue = s.add_material(matte((0.2, 0.3, 0.8), object_id=3))
+    ground = s.add_material(matte((0.7, 0.7, 0.7), object_id=4))
+    s.add_polygon([(-10, -1.2, -10), (10, -1.2, -10), (10, -1.2, 10), (-10, -1.2, 10)], ground)
+    capsule = s.add_curves([[-1.5, 0.8, 0, 0.3]], [[0.2, 0.8, 0, 0.3]], [0.0], [1.0])
+    taper = s.add_curves([[0.6, 0.6, 0, 0.35]], [[2.0, 1.0, 0.3, 0.1]], [0.0], [1.0])
+    p0, p1, v0, v1 = dice_curve([[-2, -1, 0.5], [-1, 1, 0], [0.5, -1.5, -0.5], [1.8, 0.0, 0.5]],
+                                (BEZIER_BASIS, 3), 0.16, 0.04, 8)
+    strand = s.add_curves(p0, p1, v0, v1)
+    pts = np.array([[0.8, -0.6, -0.5, 0.2], [1.4, -0.8, 0.2, 0.15], [2.0, -0.5, -0.3, 0.1]], np.float32)
+    points = s.add_curves(pts, pts, np.full(3, 0.5), np.full(3, 0.5))
+    s.add_instance(capsule, fur, _place((0, 0, 0)), curves=True)
+    s.add_instance(taper, red, _place((0, 0, 0)), curves=True)
+    s.add_instance(strand, fur, _place((0, 0, 0)), curves=True)
+    s.add_instance(points, blue, _place((0, 0, 0)), curves=True)
+    s.add_instance(capsule, red, _place((0.3, -1.6, 1.0), -30, 0.6), curves=True)
     return s

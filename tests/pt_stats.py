@@ -59,10 +59,14 @@ def mean_var(acc):
 
 def sigma_test(acc_a, acc_b):
     """z-scores of the per-pixel, per-channel mean difference of two
-    independent estimates."""
+    independent estimates. The standard error has a floor at 4e-7 of the
+    pixel value, the f32 rounding of a radiance: a pixel every sample of
+    which sees the same constant (a background that f32 does not hold
+    exactly) has an f32 renderer's exact zero variance against an f64
+    renderer's ~1e-17 of rounding noise, and differs by that rounding."""
     ma, va, na = mean_var(acc_a)
     mb, vb, nb = mean_var(acc_b)
-    se = np.sqrt(va / na + vb / nb)
+    se = np.sqrt(va / na + vb / nb + (4e-7 * np.maximum(np.abs(ma), np.abs(mb))) ** 2)
     d = np.abs(ma - mb)
     z = np.where(se > 0, d / np.where(se > 0, se, 1), np.where(d > 1e-6, np.inf, 0.0))
     return z

@@ -335,3 +335,108 @@ def test_dome_light_map_from_file():
     s = b.scene()
     d = s.desc()
     assert (d.env_width, d.env_height) == (512, 256)
+
+
+# ---- curves, points and hair (scene_builder.rs:685-697, 1050-1170) ----------------------
+
+def hair_ball_rib(n=400):
+    """render.rs's hair parity scene as RIB text (tests/metal_pt_tests.rs:
+    246-300), its control points from scenes.hair_strands."""
+    strands = scenes.hair_strands(n)
+    pts = " ".join("%.4f %.4f %.4f" % tuple(p) for s in strands for p in s)
+    return f"""
+        Format 96 96 1.0
+        Projection "perspective" "fov" [35]
+        Option "background" "color" [0.4 0.45 0.55]
+        WorldBegin
+            LightSource "distantlight" "sun" "from" [4 8 -6] "to" [0 0 6] "intensity" [1.4]
+            AttributeBegin
+                Color 0.6 0.6 0.6
+                Polygon "P" [-30 -1.3 -5  30 -1.3 -5  30 -1.3 40  -30 -1.3 40]
+            AttributeEnd
+            AttributeBegin
+                Bxdf "PxrMarschnerHair" "fur" "color" [0.5 0.3 0.12]
+                    "roughness" [0.3] "azimuthalRoughness" [0.35]
+                Translate 0 0.4 7
+                Curves "linear" [{" ".join(["4"] * n)}] "nonperiodic"
+                    "P" [{pts}]
+                    "width" [0.03 0.008]
+            AttributeEnd
+        WorldEnd
+    """
+
+
+def test_hair_ball_through_rib_equals_scene():
+    b = build(hair_ball_rib())
+    s = b.scene()
+    want = scenes.hair_ball()
+    assert len(s.curve_sets) == 1 and s.curve_sets[0].num_segments == 1200  # metal_pt_tests.rs:303: > 1000
+    got_c, want_c = s.curve_sets[0], want.curve_sets[0]
+    for f in ("p0", "p1", "v0", "v1"):
+        n = 4 if f in ("p0", "p1") else 1
+        a = np.ctypeslib.as_array(ctypes_ptr(getattr(got_c, f)), (1200 * n,))
+        w = np.ctypeslib.as_array(ctypes_ptr(getattr(want_c, f)), (1200 * n,))
+        np.testing.assert_array_equal(a, w)
+    (inst,) = [i for i in s.instances if i.kind == rrhip.INSTANCE_CURVES]
+    np.testing.assert_allclose(np.array(inst.object_to_world).reshape(3, 4)[:, 3], [0, 0.4, 7])
+    m = s.materials[inst.material]
+    assert m.hair == 1 and m.hair_beta_m == 0.3 and m.hair_beta_n == 0.35 and m.hair_eta == 1.55
+    np.testing.assert_allclose(list(m.hair_sigma_a), scenes.sigma_a_from_color((0.5, 0.3, 0.12), 0.35))
+    np.testing.assert_allclose(list(m.diffuse_color), [0.35, 0.22, 0.12])  # the matte render.rs keeps
+    np.testing.assert_allclose(list(s.lights[0].p), list(want.lights[0].p), atol=1e-15)
+    assert tuple(s.background) == (0.4, 0.45, 0.55)
+
+
+def ctypes_ptr(addr):
+    import ctypes
+    return ctypes.cast(addr, ctypes.POINTER(ctypes.c_float))
+
+
+def test_cubic_curves_basis_shading_rate_and_points():
+    b = build("""
+        WorldBegin
+        Basis "bezier" 3 "b-spline" 1
+        ShadingRate 2
+        Curves "cubic" [5] "nonperiodic" "P" [0 0 0  1 1 0  2 0 0  3 1 0  4 0 0] "constantwidth" [0.2]
+        Bxdf "PxrMarschnerHair" "h" "absorption" [0.5] "ior" [1.6]
+        Translate 1 2 3
+        Points "P" [0 0 0  1 0 0  2 0 0] "width" [0.1 0.3]
+        WorldEnd""")
+    s = b.scene()
+    assert len(s.curve_sets) == 2
+    # b-spline step 1 over 5 points: 2 spans of 8 / ShadingRate = 4 segments
+    want = scenes.dice_curve([[0, 0, 0], [1, 1, 0], [2, 0, 0], [3, 1, 0], [4, 0, 0]],
+                             (rib._BASES["b-spline"][0], 1), 0.2, 0.2, 4)
+    c = s.curve_sets[0]
+    assert c.num_segments == 8 == len(want[0])
+    np.testing.assert_array_equal(np.ctypeslib.as_array(ctypes_ptr(c.p0), (32,)), want[0].ravel())
+    # points: zero-length segments, radius = width / 2 (constantwidth 0.05 past the widths), v = 0.5
+    pt = s.curve_sets[1]
+    p0 = np.ctypeslib.as_array(ctypes_ptr(pt.p0), (12,)).reshape(3, 4)
+    np.testing.assert_array_equal(p0, np.ctypeslib.as_array(ctypes_ptr(pt.p1), (12,)).reshape(3, 4))
+    np.testing.assert_allclose(p0[:, 3], [0.05, 0.15, 0.025])
+    np.testing.assert_array_equal(np.ctypeslib.as_array(ctypes_ptr(pt.v0), (3,)), 0.5)
+    insts = [i for i in s.instances if i.kind == rrhip.INSTANCE_CURVES]
+    np.testing.assert_allclose(np.array(insts[1].object_to_world).reshape(3, 4)[:, 3], [1, 2, 3])
+    m = s.materials[insts[1].material]
+    assert m.hair == 1 and m.hair_eta == 1.6 and list(m.hair_sigma_a) == [0.5, 0.5, 0.5]
+    assert s.materials[insts[0].material].hair == 0  # before the Bxdf: the default matte
+
+
+def test_curve_requests_edge_cases():
+    # periodic is diced open with a warning (scene_builder.rs:1063-1065)
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        b = build('WorldBegin Curves "linear" [3] "periodic" "P" [0 0 0 1 0 0 2 0 0] WorldEnd')
+    assert any("periodic" in str(x.message) for x in w)
+    assert b.scene().curve_sets[0].num_segments == 2
+    # inside an object definition curves are not instanced (reject_in_object_def)
+    with pytest.raises(rib.RibUnsupported):
+        build('ObjectBegin "a" Curves "linear" [2] "nonperiodic" "P" [0 0 0 1 0 0] ObjectEnd')
+    # a cubic curve with fewer than 4 points dices to nothing
+    assert len(build('WorldBegin Curves "cubic" [3] "nonperiodic" "P" [0 0 0 1 0 0 2 0 0] WorldEnd')
+               .scene().curve_sets) == 0
+    # a later PxrSurface clears the hair state
+    s = build('WorldBegin Bxdf "PxrMarschnerHair" "h" Bxdf "PxrSurface" "s" '
+              'Curves "linear" [2] "nonperiodic" "P" [0 0 0 1 0 0] WorldEnd').scene()
+    assert s.materials[s.instances[0].material].hair == 0
