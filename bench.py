@@ -10,10 +10,11 @@ the sphere, tmax = inf. Rays, BVH and hits stay resident in HBM; the
 timed region is K back-to-back closest-hit launches.
 
 Prints one JSON line (value = whole-job Mrays/s at N=1). The line also
-carries the box-room path-tracing throughput, the batched-jobs comparison
-(32 rooms, half with clouds: one launch sequence vs 32 sequential
-sessions) and the time to relMSE 0.01 at 1080p of the radiance cache vs
-plain wavefront path tracing.
+carries the box-room path-tracing throughput, the 64-spp 1080p instanced
+field, a 1080p fur ball of 300,000 hair segments, the batched-jobs
+comparison (32 rooms, half with clouds: one launch sequence, one
+persistent launch, 32 sequential sessions) and the time to relMSE 0.01 at
+1080p of the radiance cache vs plain wavefront path tracing.
 """
 import argparse
 import json
@@ -245,6 +246,66 @@ def instanced_pt(ctx, args, with_cpu):
             "cores": threads, "kind": "port",
             "sample": f"oracle plain PT, same scene at 480x270 x 2 spp ({cdt:.2f} s after {setup:.2f} s setup), "
                       f"scaled by pixels and samples to {frame_s:.0f} s per 64-spp 1080p frame",
+        }
+    ps.close()
+    return out
+
+
+def hair_curves(ctx, args, with_cpu):
+    """Curves and hair fibers at 1920x1080: a 100,000-strand fur ball
+    (300,000 rounded-cone segments in one curve BLAS, render.rs's fiber
+    BSDF, a distant light over a ground), 16 spp, depth 8; one frame =
+    rrhip_render_samples over 16 samples of every pixel in passes of 8.
+    The B-model roofline counts segment records (48 B) where the triangle
+    column says triangles."""
+    import rrhip
+    import scenes
+    W, H, SPP = 1920, 1080, 16
+    s = scenes.fur_ball(W, H)
+    d = s.desc()
+    ps = rrhip.PtScene(ctx, d)
+    p = rrhip.RenderParams.make(max_depth=8, samples_per_pass=8)
+    se = ps.session(p)
+    se.render(0, 8)  # warm-up pass
+    st0 = se.stats()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    se.render(8, SPP)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    st1 = se.stats()
+    se.close()
+    rays = st1["extension_rays"] + st1["shadow_rays"] - st0["extension_rays"] - st0["shadow_rays"]
+    segs = int(d.curve_sets[0].num_segments)
+    out = {
+        "workload": f"fur ball 1920x1080 ({segs // 3:,} strands, {segs:,} rounded-cone segments, hair fiber "
+                    "BSDF, distant light, ground), 16 spp, depth 8, NEE + MIS + RR (wavefront, passes of 8 spp)",
+        "ms_per_frame": round(dt * 1e3, 1),
+        "mrays_per_s": round(rays / dt / 1e6, 1),
+        "rays_per_frame": int(rays),
+        "samples_per_s": round(W * H * SPP / dt, 1),
+        "roofline": trace_roofline(profiled_frame(ps, p, 0, 8), d),
+    }
+    if with_cpu:
+        # the oracle on the same groom at 240x135 and 2 spp, scaled by
+        # pixels and samples to the frame; setup (scene load, the curve
+        # BLAS build) measured on an 8x8 render and subtracted
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import orc
+        threads = host_threads()
+        small, tiny = scenes.fur_ball(240, 135), scenes.fur_ball(8, 8)
+        t1 = time.perf_counter()
+        orc.render(tiny.desc(), p, 0, 1, threads=threads)
+        setup = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        _, crays = orc.render(small.desc(), p, 0, 2, threads=threads)
+        cdt = max(time.perf_counter() - t1 - setup, 1e-6)
+        frame_s = cdt * (W * H) / (240 * 135) * SPP / 2
+        out["cpu_baseline"] = {
+            "value": round(crays / cdt / 1e6, 3), "unit": "Mrays/s", "frame_s": round(frame_s, 1),
+            "cores": threads, "kind": "port",
+            "sample": f"oracle PT, same groom at 240x135 x 2 spp ({cdt:.2f} s after {setup:.2f} s setup), "
+                      f"scaled by pixels and samples to {frame_s:.1f} s per 16-spp 1080p frame",
         }
     ps.close()
     return out
@@ -549,6 +610,7 @@ def main():
     ap.add_argument("--no-relmse", action="store_true", help="skip the time-to-relMSE comparison")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs comparison")
     ap.add_argument("--no-instanced", action="store_true", help="skip the 64-spp instanced-field frame")
+    ap.add_argument("--no-hair", action="store_true", help="skip the fur-ball (curves and hair) frame")
     ap.add_argument("--ref-spp", type=int, default=4096, help="spp of the relMSE reference render")
     ap.add_argument("--max-frames", type=int, default=400, help="frame cap of the time-to-relMSE runs")
     ap.add_argument("--mode", choices=("trace", "copy", "nrc-step"), default="trace",
@@ -675,6 +737,8 @@ def main():
         out["path_tracing"] = box_room(ctx, args, not args.no_cpu_baseline)
     if not args.no_instanced:
         out["instanced_path_tracing"] = instanced_pt(ctx, args, not args.no_cpu_baseline)
+    if not args.no_hair:
+        out["hair_curves"] = hair_curves(ctx, args, not args.no_cpu_baseline)
     if not args.no_batch:
         out["batched_jobs"] = batched_jobs(ctx, args, not args.no_cpu_baseline)
     if not args.no_relmse:

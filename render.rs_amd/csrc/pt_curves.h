@@ -110,9 +110,30 @@ __device__ __forceinline__ D3 cone_normal(const CurveRec& c, D3 o, D3 d, double 
   return l > 1e-300 ? n * (1.0 / l) : D3{0.0, 0.0, 1.0};
 }
 
+// Conservative f32 rejection ahead of the f64 test: every surface point
+// of the segment lies within max(r0, r1) / cos(alpha) of its axis line
+// (sin(alpha) = |r0 - r1| / |p1 - p0|; the cone's widest cross-section
+// within the accepted body range, the caps within max(r0, r1)), so a ray
+// whose line passes farther from the axis line misses. The distance is
+// |w . n| / |n| with w = p0 - o, n = d x (p1 - p0); the slack gamma(32)
+// (|p0| + |o| + r) |d| |p1 - p0| covers its f32 rounding (w's subtraction,
+// the cross and dot products, |n|). Parallel rays (n ~ 0) and zero-length
+// or fully tapered segments always pass on to the f64 test.
+__device__ __forceinline__ bool curve_may_hit(const CurveRec& c, V3 o, V3 d) {
+  const V3 p0 = v3(c.p0);
+  const V3 a = v3(c.p1) - p0;
+  const float aa = dot(a, a), rr = c.r0 - c.r1, den = aa - rr * rr;
+  if (!(aa > 1e-12f) || !(den > 0.f)) return true;
+  const V3 n = cross(d, a);
+  const float rmax = fmaxf(c.r0, c.r1) * sqrtf(aa / den) * 1.0001f;
+  const float slack = err_gamma(32) * (len(p0) + len(o) + rmax) * len(d) * sqrtf(aa);
+  return fabsf(dot(p0 - o, n)) <= rmax * len(n) + slack;
+}
+
 // Closest-hit test inside the BLAS traversal: t as f32, strictly below tmax.
 __device__ __forceinline__ bool curve_seg(const CurveRec& c, float ox, float oy, float oz, float dx, float dy,
                                           float dz, float tmax, float& t) {
+  if (!curve_may_hit(c, v3(ox, oy, oz), v3(dx, dy, dz))) return false;
   double x;
   int part;
   if (!cone_hit(c, D3{ox, oy, oz}, D3{dx, dy, dz}, (double)tmax, x, part)) return false;

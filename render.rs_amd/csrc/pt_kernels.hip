@@ -182,20 +182,20 @@ __global__ __launch_bounds__(kBlock) void raygen_kernel(PtSceneView S, PathBuffe
 }
 
 // Closest hit of the path in `slot` into hit_n / hit_m (MOTION: at the
-// path's shutter time; QUAD: the scene holds quadrics).
-template <bool MOTION, bool QUAD, bool COUNT = false>
+// path's shutter time; GEO: the geometry classes the scene holds).
+template <bool MOTION, int GEO, bool COUNT = false>
 __device__ __forceinline__ void extend_one(const PtSceneView& S, const PathBuffers& P, uint32_t slot, uint32_t* stk,
                                            TraceCount* tc = nullptr) {
   const float4 o4 = P.o[slot], d4 = P.d[slot];
   const float time = MOTION ? P.beta[slot].w : 0.f;
   Hit h;
-  if (scene_closest<kTraceBlock, MOTION, QUAD, true, COUNT>(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), time,
+  if (scene_closest<kTraceBlock, MOTION, GEO, true, COUNT>(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), time,
                                                              stk, h, tc)) {
     P.hit_n[slot] = make_float4(h.n.x, h.n.y, h.n.z, h.t);
     P.hit_m[slot] = h.material | (h.front ? 0x80000000u : 0u);
     const V3 front = offset_origin(h.p, h.e), back = offset_origin(h.p, -h.e);
     float2 tw = make_float2(kNoTangent, kNoTangent);
-    if (QUAD && dot(h.tan, h.tan) > 0.5f) tw = oct_encode(h.tan);
+    if ((GEO & kGeoCurves) && dot(h.tan, h.tan) > 0.5f) tw = oct_encode(h.tan);
     P.hit_front[slot] = make_float4(front.x, front.y, front.z, tw.x);
     P.hit_back[slot] = make_float4(back.x, back.y, back.z, tw.y);
   } else {
@@ -219,7 +219,7 @@ __device__ __forceinline__ void add_counts(unsigned long long* prof, const Trace
   }
 }
 
-template <int STACK, bool MOTION, bool QUAD, bool COUNT = false>
+template <int STACK, bool MOTION, int GEO, bool COUNT = false>
 __global__ __launch_bounds__(kTraceBlock) void extend_kernel(PtSceneView S, PathBuffers P, const uint32_t* __restrict__ q) {
   __shared__ uint32_t s_stack[STACK * kTraceBlock];
   uint32_t* stk = s_stack + threadIdx.x;
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(kTraceBlock) void extend_kernel(PtSceneView S, Path
   const uint32_t stride = gridDim.x * blockDim.x;
   TraceCount tc;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    extend_one<MOTION, QUAD, COUNT>(S, P, q[i], stk, &tc);
+    extend_one<MOTION, GEO, COUNT>(S, P, q[i], stk, &tc);
   if (COUNT) add_counts(P.prof, tc);
 }
 
@@ -929,7 +929,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
 // transmittance of the medium on top of the stack (the origin's medium
 // over the atmosphere); hulls push / pop their media, cutouts let 1 -
 // presence through, anything else blocks; at most 16 blockers.
-template <bool MOTION, bool QUAD, bool COUNT = false>
+template <bool MOTION, int GEO, bool COUNT = false>
 __device__ __forceinline__ V3 walk_visibility(const PtSceneView& S, V3 o, V3 d, float time, float remaining,
                                               int medium, Pcg32& rng, uint32_t* stk, TraceCount* tc = nullptr) {
   uint2 st = make_uint2(0u, 0u);
@@ -938,7 +938,7 @@ __device__ __forceinline__ V3 walk_visibility(const PtSceneView& S, V3 o, V3 d, 
   V3 vis = v3(1.f, 1.f, 1.f);
   for (uint32_t i = 0; i < kMaxSkips; i++) {
     Hit h;
-    const bool hit = scene_closest<kTraceBlock, MOTION, QUAD, true, COUNT>(S, o, d, time, stk, h, tc);
+    const bool hit = scene_closest<kTraceBlock, MOTION, GEO, true, COUNT>(S, o, d, time, stk, h, tc);
     const float seg = fminf(hit ? h.t : remaining, remaining);
     const int cur = stack_top(st, depth, S.atmosphere);
     if (cur >= 0) {
@@ -972,7 +972,7 @@ __device__ __forceinline__ V3 walk_visibility(const PtSceneView& S, V3 o, V3 d, 
 // Shadow record i: adds each unoccluded ray's contribution to its path
 // (the first ray, then the second when the record has one); with WALK the
 // rays walk their blockers (transparent scenes), else one any-hit test.
-template <bool WALK, bool MOTION, bool QUAD, bool COUNT = false>
+template <bool WALK, bool MOTION, int GEO, bool COUNT = false>
 __device__ __forceinline__ void shadow_one_t(const PtSceneView& S, const PathBuffers& P, uint32_t i, uint32_t* stk,
                                              TraceCount* tc = nullptr) {
   const float4 o = P.so[i], d = P.sd[i];
@@ -986,24 +986,24 @@ __device__ __forceinline__ void shadow_one_t(const PtSceneView& S, const PathBuf
     const uint2 sd = P.srng[i];
     const uint64_t seed = ((uint64_t)sd.y << 32) | sd.x;
     Pcg32 r1 = Pcg32::make(seed, 1);
-    const V3 t1 = walk_visibility<MOTION, QUAD, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), time, o.w, (int)d.w,
+    const V3 t1 = walk_visibility<MOTION, GEO, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), time, o.w, (int)d.w,
                                                       r1, stk, tc);
     c = make_float4(c.x * t1.x, c.y * t1.y, c.z * t1.z, c.w);
     v1 = maxc(t1) > 0.f;
     if (two) {
       const float4 o2 = P.so2[i], d2 = P.sd2[i];
       Pcg32 r2 = Pcg32::make(seed, 2);
-      const V3 t2 = walk_visibility<MOTION, QUAD, COUNT>(S, v3(o2.x, o2.y, o2.z), v3(d2.x, d2.y, d2.z), time, o2.w,
+      const V3 t2 = walk_visibility<MOTION, GEO, COUNT>(S, v3(o2.x, o2.y, o2.z), v3(d2.x, d2.y, d2.z), time, o2.w,
                                                         (int)d2.w, r2, stk, tc);
       c2 = P.sc2[i];
       c2 = make_float4(c2.x * t2.x, c2.y * t2.y, c2.z * t2.z, 0.f);
       v2 = maxc(t2) > 0.f;
     }
   } else {
-    v1 = !scene_occluded<kTraceBlock, MOTION, QUAD, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), time, o.w, stk, tc);
+    v1 = !scene_occluded<kTraceBlock, MOTION, GEO, COUNT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), time, o.w, stk, tc);
     if (two) {
       const float4 o2 = P.so2[i], d2 = P.sd2[i];
-      v2 = !scene_occluded<kTraceBlock, MOTION, QUAD, COUNT>(S, v3(o2.x, o2.y, o2.z), v3(d2.x, d2.y, d2.z), time, o2.w,
+      v2 = !scene_occluded<kTraceBlock, MOTION, GEO, COUNT>(S, v3(o2.x, o2.y, o2.z), v3(d2.x, d2.y, d2.z), time, o2.w,
                                                             stk, tc);
       c2 = P.sc2[i];
     }
@@ -1031,12 +1031,12 @@ __device__ __forceinline__ void shadow_one_t(const PtSceneView& S, const PathBuf
 template <bool QUAD>
 __device__ __forceinline__ void shadow_one(const PtSceneView& S, const PathBuffers& P, uint32_t i, uint32_t* stk) {
   if (S.transparent)
-    S.motion ? shadow_one_t<true, true, QUAD>(S, P, i, stk) : shadow_one_t<true, false, QUAD>(S, P, i, stk);
+    S.motion ? shadow_one_t<true, true, QUAD ? 3 : 0>(S, P, i, stk) : shadow_one_t<true, false, QUAD ? 3 : 0>(S, P, i, stk);
   else
-    S.motion ? shadow_one_t<false, true, QUAD>(S, P, i, stk) : shadow_one_t<false, false, QUAD>(S, P, i, stk);
+    S.motion ? shadow_one_t<false, true, QUAD ? 3 : 0>(S, P, i, stk) : shadow_one_t<false, false, QUAD ? 3 : 0>(S, P, i, stk);
 }
 
-template <int STACK, bool WALK, bool MOTION, bool QUAD, bool COUNT = false>
+template <int STACK, bool WALK, bool MOTION, int GEO, bool COUNT = false>
 __global__ __launch_bounds__(kTraceBlock) void shadow_kernel(PtSceneView S, PathBuffers P) {
   __shared__ uint32_t s_stack[STACK * kTraceBlock];
   uint32_t* stk = s_stack + threadIdx.x;
@@ -1044,51 +1044,57 @@ __global__ __launch_bounds__(kTraceBlock) void shadow_kernel(PtSceneView S, Path
   const uint32_t stride = gridDim.x * blockDim.x;
   TraceCount tc;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    shadow_one_t<WALK, MOTION, QUAD, COUNT>(S, P, i, stk, &tc);
+    shadow_one_t<WALK, MOTION, GEO, COUNT>(S, P, i, stk, &tc);
   if (COUNT) add_counts(P.prof + 3, tc);
 }
 
 // Launch of extend / shadow for the scene's stack size, transparency,
 // motion and quadrics; with P.prof set, the counting variants.
-template <int STACK, bool MOTION, bool QUAD, bool COUNT>
+template <int STACK, bool MOTION, int GEO, bool COUNT>
 void launch_trace_counted(const PtSceneView& S, const PathBuffers& P, const uint32_t* q, uint32_t gt, bool extend,
                           hipStream_t st) {
   if (extend)
-    hipLaunchKernelGGL((extend_kernel<STACK, MOTION, QUAD, COUNT>), dim3(gt), dim3(kTraceBlock), 0, st, S, P, q);
+    hipLaunchKernelGGL((extend_kernel<STACK, MOTION, GEO, COUNT>), dim3(gt), dim3(kTraceBlock), 0, st, S, P, q);
   else if (S.transparent)
-    hipLaunchKernelGGL((shadow_kernel<STACK, true, MOTION, QUAD, COUNT>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
+    hipLaunchKernelGGL((shadow_kernel<STACK, true, MOTION, GEO, COUNT>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
   else
-    hipLaunchKernelGGL((shadow_kernel<STACK, false, MOTION, QUAD, COUNT>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
+    hipLaunchKernelGGL((shadow_kernel<STACK, false, MOTION, GEO, COUNT>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
 }
 
-template <int STACK, bool MOTION, bool QUAD>
+template <int STACK, bool MOTION, int GEO>
 void launch_trace_kernels(const PtSceneView& S, const PathBuffers& P, const uint32_t* q, uint32_t gt, bool extend,
                           hipStream_t st) {
   if (P.prof)
-    launch_trace_counted<STACK, MOTION, QUAD, true>(S, P, q, gt, extend, st);
+    launch_trace_counted<STACK, MOTION, GEO, true>(S, P, q, gt, extend, st);
   else
-    launch_trace_counted<STACK, MOTION, QUAD, false>(S, P, q, gt, extend, st);
+    launch_trace_counted<STACK, MOTION, GEO, false>(S, P, q, gt, extend, st);
 }
 
-// Scenes with quadrics or curve sets take the full-stack kernels with the
-// quadric and curve solvers (one instantiation per motion flag); every
-// other scene keeps kernels without them.
+// Scenes with loose quadrics take the full-stack kernels with the quadric
+// solvers (and the curve test, for scenes with both), scenes with curve
+// sets alone the full-stack kernels with the curve test only (the f64
+// quartic's registers stay out of them); one instantiation per motion
+// flag. Every other scene keeps the lean kernels.
 void launch_trace(const PtSceneView& S, const PathBuffers& P, const uint32_t* q, uint32_t gt, bool extend,
                   hipStream_t st) {
   const bool small = S.stack_need <= 32;
-  const bool special = S.num_quadrics || S.curves;
-  if (special && S.motion)
-    launch_trace_kernels<kTraceStack, true, true>(S, P, q, gt, extend, st);
-  else if (special)
-    launch_trace_kernels<kTraceStack, false, true>(S, P, q, gt, extend, st);
+  constexpr int kAll = kGeoQuadrics | kGeoCurves;
+  if (S.num_quadrics && S.motion)
+    launch_trace_kernels<kTraceStack, true, kAll>(S, P, q, gt, extend, st);
+  else if (S.num_quadrics)
+    launch_trace_kernels<kTraceStack, false, kAll>(S, P, q, gt, extend, st);
+  else if (S.curves && S.motion)
+    launch_trace_kernels<kTraceStack, true, kGeoCurves>(S, P, q, gt, extend, st);
+  else if (S.curves)
+    launch_trace_kernels<kTraceStack, false, kGeoCurves>(S, P, q, gt, extend, st);
   else if (small && S.motion)
-    launch_trace_kernels<32, true, false>(S, P, q, gt, extend, st);
+    launch_trace_kernels<32, true, 0>(S, P, q, gt, extend, st);
   else if (small)
-    launch_trace_kernels<32, false, false>(S, P, q, gt, extend, st);
+    launch_trace_kernels<32, false, 0>(S, P, q, gt, extend, st);
   else if (S.motion)
-    launch_trace_kernels<kTraceStack, true, false>(S, P, q, gt, extend, st);
+    launch_trace_kernels<kTraceStack, true, 0>(S, P, q, gt, extend, st);
   else
-    launch_trace_kernels<kTraceStack, false, false>(S, P, q, gt, extend, st);
+    launch_trace_kernels<kTraceStack, false, 0>(S, P, q, gt, extend, st);
 }
 
 __global__ void rotate_kernel(PathBuffers P) {
@@ -1256,9 +1262,9 @@ __global__ __launch_bounds__(kTraceBlock) void extend_batch_kernel(const PtJob* 
     const uint32_t slot = q[i];
     const PtSceneView& S = jobs[P.job[slot]].S;
     if (S.motion)
-      extend_one<true, QUAD>(S, P, slot, stk);
+      extend_one<true, QUAD ? 3 : 0>(S, P, slot, stk);
     else
-      extend_one<false, QUAD>(S, P, slot, stk);
+      extend_one<false, QUAD ? 3 : 0>(S, P, slot, stk);
   }
 }
 
@@ -1341,7 +1347,7 @@ template <bool TR, bool MOTION, bool QUAD>
 __device__ __forceinline__ bool persist_bounce(const PtJob& J, const PathBuffers& P, uint32_t slot, uint32_t* stk,
                                                unsigned long long& ext, unsigned long long& shd) {
   const NrcPathBuffers N{};
-  extend_one<MOTION, QUAD>(J.S, P, slot, stk);
+  extend_one<MOTION, QUAD ? 3 : 0>(J.S, P, slot, stk);
   ext++;
   bool cont = false, shadow = false, shadow2 = false, query = false;
   ShadowOut so, se;
@@ -1350,7 +1356,7 @@ __device__ __forceinline__ bool persist_bounce(const PtJob& J, const PathBuffers
   shade_one<false, TR>(J.S, P, J.prm, N, slot, cont, shadow, shadow2, se, query, so, nq, qbeta);
   if (shadow || shadow2) {
     put_shadow<TR>(P, slot, slot, shadow, so, shadow2, se);
-    shadow_one_t<TR, MOTION, QUAD>(J.S, P, slot, stk);
+    shadow_one_t<TR, MOTION, QUAD ? 3 : 0>(J.S, P, slot, stk);
     shd++;
   }
   return cont;

@@ -229,34 +229,68 @@ bool affine_inverse(const double m[12], double inv[12]) {
 // (render.rs: Mesh::new -> Bvh::build, Instance::build's 8-corner world
 // bounds, Scene::build_tlas; src/geometry/mesh.rs:48-84, 266-292,
 // src/scene/mod.rs:82-86). Node references are rebased into one array.
-// BLAS over a curve set's segments (curves.rs:28-62): each segment's box
-// holds both end spheres, taken from the f32 values in f64 and rounded
-// outward to f32; records in leaf order.
+// BLAS over a curve set's segments (curves.rs:28-62). A thin segment
+// running diagonally fills a small part of its box, so a segment longer
+// than kPieceLen of its radius is entered as up to kMaxPieces pieces,
+// each a box over a stretch of the axis, all referring to a copy of the
+// same record (a ray may test a segment once per piece it enters; the
+// test is the whole segment's, so hits are unchanged). A piece's box holds
+// the spheres at its ends, their radii divided by cos(alpha) (sin(alpha) =
+// |r0 - r1| / length: the cone's widest cross-section there) and the
+// stretch widened by r sin(alpha) / length on both sides (where the cone
+// touches a sphere), taken in f64 and rounded outward to f32.
+constexpr double kPieceLen = 4.0;
+constexpr int kMaxPieces = 8;
+
 bool build_curve_blas(const rrhip_curves_desc& c, const BuildParams& bp, BuiltBvh* out,
                       std::vector<CurveRec>* recs, std::string* err) {
   const uint64_t n = c.num_segments;
   if (n && (!c.p0 || !c.p1 || !c.v0 || !c.v1)) return *err = "null curve arrays", false;
-  std::vector<float> boxes(6 * n);
+  std::vector<float> boxes;
+  std::vector<uint32_t> seg_of;  // the segment of each piece
+  boxes.reserve(6 * n);
+  seg_of.reserve(n);
   for (uint64_t i = 0; i < n; i++) {
-    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (const float* e : {c.p0 + 4 * i, c.p1 + 4 * i}) {
-      const double r = (double)e[3];
-      if (!(r >= 0.0) || !std::isfinite(r)) return *err = "curve radii must be finite and >= 0", false;
-      for (int a = 0; a < 3; a++) {
-        lo[a] = std::min(lo[a], (double)e[a] - r);
-        hi[a] = std::max(hi[a], (double)e[a] + r);
-      }
-    }
+    const float* e0 = c.p0 + 4 * i;
+    const float* e1 = c.p1 + 4 * i;
+    const double r0 = e0[3], r1 = e1[3];
+    if (!(r0 >= 0.0) || !(r1 >= 0.0) || !std::isfinite(r0) || !std::isfinite(r1))
+      return *err = "curve radii must be finite and >= 0", false;
+    double ax[3], len2 = 0.0;
     for (int a = 0; a < 3; a++) {
-      boxes[6 * i + a] = std::nextafter((float)lo[a], -INFINITY);
-      boxes[6 * i + 3 + a] = std::nextafter((float)hi[a], INFINITY);
+      ax[a] = (double)e1[a] - (double)e0[a];
+      len2 += ax[a] * ax[a];
+    }
+    const double len = std::sqrt(len2), rmax = std::max(r0, r1);
+    const double sin_a = len > 0.0 ? std::min(std::fabs(r0 - r1) / len, 1.0) : 0.0;
+    const double cos_a = std::sqrt(std::max(1.0 - sin_a * sin_a, 0.0));
+    int pieces = 1;
+    if (rmax > 0.0 && cos_a > 0.05 && len > kPieceLen * rmax)
+      pieces = (int)std::min<double>(kMaxPieces, std::ceil(len / (kPieceLen * rmax)));
+    const double grow = len > 0.0 ? rmax * sin_a / len : 0.0;
+    for (int j = 0; j < pieces; j++) {
+      const double fa = pieces == 1 ? 0.0 : std::max(0.0, (double)j / pieces - grow);
+      const double fb = pieces == 1 ? 1.0 : std::min(1.0, (double)(j + 1) / pieces + grow);
+      double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      for (const double f : {fa, fb}) {
+        // one piece: the end spheres as they are (curves.rs:29-58)
+        const double r = pieces == 1 ? (f == 0.0 ? r0 : r1) : (r0 + (r1 - r0) * f) / cos_a;
+        for (int a = 0; a < 3; a++) {
+          const double p = f == 1.0 ? (double)e1[a] : (double)e0[a] + ax[a] * f;
+          lo[a] = std::min(lo[a], p - r);
+          hi[a] = std::max(hi[a], p + r);
+        }
+      }
+      for (int a = 0; a < 3; a++) boxes.push_back(std::nextafter((float)lo[a], -INFINITY));
+      for (int a = 0; a < 3; a++) boxes.push_back(std::nextafter((float)hi[a], INFINITY));
+      seg_of.push_back((uint32_t)i);
     }
   }
   std::vector<uint32_t> order;
-  if (!build_box_bvh(boxes.data(), n, bp, out, &order, err)) return false;
+  if (!build_box_bvh(boxes.data(), seg_of.size(), bp, out, &order, err)) return false;
   recs->resize(order.size());
   for (size_t k = 0; k < order.size(); k++) {
-    const uint64_t i = order[k];
+    const uint64_t i = seg_of[order[k]];
     CurveRec& r = (*recs)[k];
     for (int a = 0; a < 3; a++) {
       r.p0[a] = c.p0[4 * i + a];

@@ -351,18 +351,25 @@ __device__ __forceinline__ void curve_instance_hit(const PtSceneView& S, const M
   }
 }
 
-// Closest hit over loose objects then instances (QUAD: with quadrics or
-// curve sets). SPAWN: also the winning hit's point and push (Hit::p,
+// Geometry classes of a trace instantiation: loose quadrics (their
+// solvers) and curve sets (the segment test). GEO = 0 is the lean
+// triangle / sphere traversal.
+constexpr int kGeoQuadrics = 1, kGeoCurves = 2;
+
+// Closest hit over loose objects then instances (GEO: the geometry
+// classes above). SPAWN: also the winning hit's point and push (Hit::p,
 // Hit::e). Hit::tan is the world tangent of a curve hit, zero otherwise.
-template <int STRIDE, bool MOTION, bool QUAD, bool SPAWN = false, bool COUNT = false>
+template <int STRIDE, bool MOTION, int GEO, bool SPAWN = false, bool COUNT = false>
 __device__ __forceinline__ bool scene_closest(const PtSceneView& S, V3 o, V3 d, float time, uint32_t* stk,
                                               Hit& best, TraceCount* tc = nullptr) {
+  constexpr bool QUAD = (GEO & kGeoQuadrics) != 0, CURVES = (GEO & kGeoCurves) != 0;
   bool found = closest_hit<QUAD>(S, o, d, best);
   best.tan = v3(0.f, 0.f, 0.f);
   MeshHit mh;
-  if (trace_instances<STRIDE, false, MOTION, COUNT, QUAD>(S, o, d, time, found ? best.t : INFINITY, stk, mh, tc)) {
+  if (trace_instances<STRIDE, false, MOTION, COUNT, CURVES>(S, o, d, time, found ? best.t : INFINITY, stk, mh,
+                                                            tc)) {
     best.t = mh.t;
-    if (QUAD && S.meshes[S.insts[mh.inst].mesh].curves) {
+    if (CURVES && S.meshes[S.insts[mh.inst].mesh].curves) {
       curve_instance_hit<MOTION, SPAWN>(S, mh, o, d, time, best);
       return true;
     }
@@ -374,12 +381,12 @@ __device__ __forceinline__ bool scene_closest(const PtSceneView& S, V3 o, V3 d, 
   return found;
 }
 
-template <int STRIDE, bool MOTION, bool QUAD, bool COUNT = false>
+template <int STRIDE, bool MOTION, int GEO, bool COUNT = false>
 __device__ __forceinline__ bool scene_occluded(const PtSceneView& S, V3 o, V3 d, float time, float tmax,
                                                uint32_t* stk, TraceCount* tc = nullptr) {
-  if (occluded<QUAD>(S, o, d, tmax)) return true;
+  if (occluded<(GEO & kGeoQuadrics) != 0>(S, o, d, tmax)) return true;
   MeshHit mh;
-  return trace_instances<STRIDE, true, MOTION, COUNT, QUAD>(S, o, d, time, tmax, stk, mh, tc);
+  return trace_instances<STRIDE, true, MOTION, COUNT, (GEO & kGeoCurves) != 0>(S, o, d, time, tmax, stk, mh, tc);
 }
 
 }  // namespace pt

@@ -637,8 +637,49 @@ def hair_ball(width=96, height=96, n=400, seed=12345):
     s.add_light(light(rrhip.LIGHT_DISTANT, intensity=1.4, p=tuple(d / np.linalg.norm(d))))
     ground = s.add_material(matte((0.6, 0.6, 0.6)))
     s.add_polygon([(-30, -1.3, -5), (30, -1.3, -5), (30, -1.3, 40), (-30, -1.3, 40)], ground)
-    parts = [dice_curve(c, None, 0.03, 0.008, 8) for c in hair_strands(n, seed)]
-    cs = s.add_curves(*(np.concatenate([q[k] for q in parts]) for k in range(4)))
+    cs = s.add_curves(*dice_linear_strands(hair_strands(n, seed), 0.03, 0.008))
+    fur = s.add_material(hair(color=(0.5, 0.3, 0.12), beta_m=0.3, beta_n=0.35))
+    s.add_instance(cs, fur, _place((0.0, 0.4, 7.0)), curves=True)
+    return s
+
+
+def dice_linear_strands(strands, width_root, width_tip):
+    """dice_curve over many linear curves of equal point count at once
+    (strands: n x m x 3): the same segments, radii and v, bit for bit."""
+    strands = np.asarray(strands, np.float64)
+    n, m, _ = strands.shape
+    k = np.arange(m, dtype=np.float64) / (m - 1)
+    r = (width_root + (width_tip - width_root) * k) * 0.5
+    ends = np.concatenate([strands, np.broadcast_to(r[None, :, None], (n, m, 1))], 2).astype(np.float32)
+    v = np.broadcast_to(k.astype(np.float32), (n, m))
+    return (ends[:, :-1].reshape(-1, 4), ends[:, 1:].reshape(-1, 4), v[:, :-1].reshape(-1).copy(),
+            v[:, 1:].reshape(-1).copy())
+
+
+def fur_ball(width=1920, height=1080, n=100000, seed=7):
+    """A dense groom for throughput: the hair ball's layout (roots over
+    the upper part of a unit sphere, strands growing 45 % outward with a
+    droop, same fiber, light, ground and camera) with n strands of 4 points
+    whose roots and jitter come from numpy's PCG64 (seed), 3 n segments."""
+    rng = np.random.default_rng(seed)
+    phi = rng.uniform(0, 2 * np.pi, n)
+    z = rng.uniform(-0.2, 0.98, n)
+    r = np.sqrt(np.maximum(1 - z * z, 0))
+    root = np.stack([r * np.cos(phi), z, r * np.sin(phi)], 1)
+    t = np.arange(4) / 3.0
+    g = 1.0 + 0.45 * t
+    strands = root[:, None, :] * g[None, :, None]
+    strands[:, :, 1] -= 0.25 * t * t
+    jit = rng.uniform(-1, 1, (n, 4, 2)) * (0.08 * t * t)[None, :, None]
+    strands[:, :, 0] += jit[:, :, 0]
+    strands[:, :, 2] += jit[:, :, 1]
+    s = Scene(width, height, 35.0, eye=(0, 0, 0), look_at=(0, 0, 1), up=(0, 1, 0), filter=0, filter_width=1.0,
+              background=(0.4, 0.45, 0.55))
+    d = np.array([0.0, 0.0, 6.0]) - np.array([4.0, 8.0, -6.0])
+    s.add_light(light(rrhip.LIGHT_DISTANT, intensity=1.4, p=tuple(d / np.linalg.norm(d))))
+    ground = s.add_material(matte((0.6, 0.6, 0.6)))
+    s.add_polygon([(-30, -1.3, -5), (30, -1.3, -5), (30, -1.3, 40), (-30, -1.3, 40)], ground)
+    cs = s.add_curves(*dice_linear_strands(strands, 0.012, 0.003))
     fur = s.add_material(hair(color=(0.5, 0.3, 0.12), beta_m=0.3, beta_n=0.35))
     s.add_instance(cs, fur, _place((0.0, 0.4, 7.0)), curves=True)
     return s
