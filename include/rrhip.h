@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define RRHIP_ABI_VERSION 7u
+#define RRHIP_ABI_VERSION 8u
 #define RRHIP_NO_HIT 0xFFFFFFFFu
 
 /* Status codes. render.rs returns anyhow::Result with a message
@@ -78,14 +78,16 @@ typedef struct rrhip_build_params {
 } rrhip_build_params;
 
 /* One triangle mesh: f32 xyz positions, u32 vertex indices (3 per
- * triangle) and optional per-vertex f32 shading normals (NULL: geometric
- * normals) — render.rs's Mesh storage (src/geometry/mesh.rs:20-34). */
+ * triangle), optional per-vertex f32 shading normals (NULL: geometric
+ * normals) and texture coordinates (st pairs, NULL: none) — render.rs's
+ * Mesh storage (src/geometry/mesh.rs:20-34). */
 typedef struct rrhip_mesh_desc {
   const float* positions;
   uint64_t num_vertices;
   const uint32_t* indices;
   uint64_t num_triangles;
   const float* normals;
+  const float* st;
 } rrhip_mesh_desc;
 
 /* A set of curve segments (render.rs CurveSet, src/geometry/curves.rs:
@@ -268,6 +270,78 @@ typedef struct rrhip_medium_desc {
   double grid_hi[3];
 } rrhip_medium_desc;
 
+/* A texture: linear RGB f32 texels of level 0, rows from the top
+ * (render.rs's .tex contents, src/texture/tex.rs). The library builds the
+ * mip chain with tex.rs's 2x2 box filter down to 1x1. */
+typedef struct rrhip_texture_desc {
+  uint32_t width, height;
+  const float* texels;  /* width * height rgb triples */
+} rrhip_texture_desc;
+
+/* Pattern node kinds (render.rs PatternNode, src/texture/pattern.rs:42-68)
+ * and texture wrap modes (cache.rs:20-35). */
+enum {
+  RRHIP_PATTERN_TEXTURE = 0,       /* PxrTexture: st-mapped texture (UDIM tiles optional) */
+  RRHIP_PATTERN_CHECKER = 1,       /* PxrChecker, filtered towards the mean by the footprint */
+  RRHIP_PATTERN_FRACTAL = 2,       /* PxrFractal: fBm over the world position, to [0, 1] */
+  RRHIP_PATTERN_MIX = 3,           /* PxrMix: lerp(color1, color2, mix.r) */
+  RRHIP_PATTERN_COLOR_CORRECT = 4, /* PxrColorCorrect: gain, offset, gamma, saturation */
+  RRHIP_PATTERN_RAMP = 5,          /* PxrRamp: piecewise-linear over s or t */
+  RRHIP_PATTERN_TRIPLANAR = 6      /* triplanar texture blended by |N| */
+};
+enum { RRHIP_WRAP_PERIODIC = 0, RRHIP_WRAP_CLAMP = 1, RRHIP_WRAP_BLACK = 2 };
+
+/* One pattern node. Inputs name another node, or -1 for the constant
+ * beside them; an input more than 16 levels down, or naming no node,
+ * evaluates to magenta (pattern.rs:203-212). The GPU evaluates each
+ * binding's input tree in post-order: at most 256 nodes per tree with at
+ * most 8 inputs pending at once (RRHIP_ERR_UNSUPPORTED beyond). */
+typedef struct rrhip_pattern_desc {
+  uint32_t kind;
+  uint32_t wrap;            /* texture, triplanar */
+  int32_t texture;          /* texture, triplanar: texture index; -1 = missing (renders magenta) */
+  uint32_t use_t;           /* ramp: over t instead of s */
+  const int32_t* udim;      /* texture: NULL, or 100 texture indices of UDIM tiles 1001..1100 (-1: none) */
+  int32_t input[3];         /* mix: color1, color2, mix; color correct: input */
+  uint32_t octaves;         /* fractal layers */
+  double constant[3][3];    /* the inputs' constants */
+  double color_a[3], color_b[3];    /* checker */
+  double cc_gain[3], cc_offset[3];  /* color correct */
+  double scale[2];          /* texture scaleS / scaleT; checker sScale / tScale */
+  double frequency;         /* fractal, triplanar */
+  double gain, lacunarity;  /* fractal */
+  double gamma, saturation; /* color correct */
+  uint32_t num_stops;       /* ramp */
+  uint32_t reserved;
+  const double* positions;  /* ramp: num_stops positions */
+  const double* colors;     /* ramp: num_stops rgb */
+} rrhip_pattern_desc;
+
+/* PbrParams fields a pattern drives (BoundField, pattern.rs:227-272):
+ * colours take the pattern's rgb, scalars its red channel. */
+enum {
+  RRHIP_BIND_DIFFUSE_COLOR = 0,
+  RRHIP_BIND_DIFFUSE_GAIN = 1,
+  RRHIP_BIND_DIFFUSE_ROUGHNESS = 2,
+  RRHIP_BIND_SPECULAR_FACE_COLOR = 3,
+  RRHIP_BIND_SPECULAR_EDGE_COLOR = 4,
+  RRHIP_BIND_SPECULAR_ROUGHNESS = 5, /* clamped to [0.005, 1] */
+  RRHIP_BIND_CLEARCOAT_GAIN = 6,
+  RRHIP_BIND_FUZZ_GAIN = 7,
+  RRHIP_BIND_FUZZ_COLOR = 8,
+  RRHIP_BIND_GLASS_GAIN = 9,
+  RRHIP_BIND_GLASS_ROUGHNESS = 10,   /* clamped to [0.005, 1] */
+  RRHIP_BIND_REFRACTION_COLOR = 11,
+  RRHIP_BIND_GLOW = 12,
+  RRHIP_BIND_PRESENCE = 13           /* clamped to [0, 1] */
+};
+typedef struct rrhip_binding_desc {  /* Material::pattern_bindings entry */
+  uint32_t material;
+  uint32_t field;    /* RRHIP_BIND_* */
+  uint32_t pattern;  /* node index */
+  uint32_t reserved;
+} rrhip_binding_desc;
+
 typedef struct rrhip_scene_desc {
   uint32_t struct_size;
   uint32_t num_meshes;
@@ -307,6 +381,21 @@ typedef struct rrhip_scene_desc {
   uint32_t num_curve_sets;
   uint32_t reserved_curves;
   const rrhip_curves_desc* curve_sets;
+  /* Textures, pattern nodes and the materials' pattern bindings
+   * (Scene::patterns, Material::pattern_bindings): a bound field of a
+   * material follows its pattern at every shading point, filtered by the
+   * ray cone's footprint (pt/mod.rs:389-440). Texture coordinates come
+   * from meshes' st, spheres, quadrics (cones none) and curves ([0.5, v]);
+   * loose triangles have none. At most 15 bindings per material; a bound
+   * presence is refused (RRHIP_ERR_UNSUPPORTED), as are radiance caches
+   * and batches over scenes with bindings. */
+  uint32_t num_textures;
+  uint32_t num_patterns;
+  const rrhip_texture_desc* textures;
+  const rrhip_pattern_desc* patterns;
+  uint32_t num_bindings;
+  uint32_t reserved_patterns;
+  const rrhip_binding_desc* bindings;
 } rrhip_scene_desc;
 
 /* Path-tracing parameters (render.rs constants, src/raytracer/pt/mod.rs:

@@ -175,6 +175,10 @@ def _pt_sigs():
         "orc_scene_intersect": (i32, [vp, vp, vp, ctypes.POINTER(f64), vp, vp, ctypes.POINTER(i32),
                                       ctypes.POINTER(u32)]),
         "orc_scene_intersect_n": (None, [vp, vp, u64, vp, vp, vp, vp, vp]),
+        "orc_scene_st_n": (None, [vp, vp, u64, vp, vp, vp, vp]),
+        "orc_texture_level": (u64, [u32, u32, vp, u32, vp]),
+        "orc_texture_sample": (None, [u32, u32, vp, u32, vp, vp, vp, vp, vp, u64, vp]),
+        "orc_pattern_eval": (None, [u32, vp, u32, vp, i32, u64, vp, vp, vp, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -353,6 +357,72 @@ def scene_intersect_n(desc, rays):
     front, mat = np.zeros(n, np.int32), np.zeros(n, np.uint32)
     lib.orc_scene_intersect_n(ctypes.addressof(desc), _p(r), n, _p(t), _p(nrm), _p(tan), _p(front), _p(mat))
     return t, nrm, tan, front.astype(bool), mat
+
+
+def scene_st_n(desc, rays):
+    """Texture coordinates at the closest hits of rays (n x 6): (t (-1 on
+    a miss), st (n x 2), st density (n), gradients of s and t (n x 2 x 3))."""
+    lib = _pt_sigs()
+    r = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
+    n = r.shape[0]
+    t, st, dens, grad = np.zeros(n), np.zeros((n, 2)), np.zeros(n), np.zeros((n, 2, 3))
+    lib.orc_scene_st_n(ctypes.addressof(desc), _p(r), n, _p(t), _p(st), _p(dens), _p(grad))
+    return t, st, dens, grad
+
+
+# ---- textures and patterns ---------------------------------------------------
+
+def texture_chain(rgb):
+    """Every mip level of a texture (tex.rs's 2x2 box chain) as h x w x 3."""
+    lib = _pt_sigs()
+    a = np.ascontiguousarray(rgb, np.float32)
+    h, w = a.shape[:2]
+    out, lvl = [], 0
+    while True:
+        wh = lib.orc_texture_level(w, h, _p(a), lvl, None)
+        if wh == 0:
+            return out
+        lw, lh = wh >> 32, wh & 0xFFFFFFFF
+        b = np.zeros((lh, lw, 3), np.float32)
+        lib.orc_texture_level(w, h, _p(a), lvl, _p(b))
+        out.append(b)
+        lvl += 1
+
+
+def texture_sample(rgb, s, t, footprint=None, du=None, dv=None, wrap=0):
+    """cache.rs lookups: trilinear from the footprint, or EWA from the
+    texture-space axes du, dv (n x 2 each)."""
+    lib = _pt_sigs()
+    a = np.ascontiguousarray(rgb, np.float32)
+    h, w = a.shape[:2]
+    s = np.ascontiguousarray(np.atleast_1d(s), np.float64)
+    n = s.size
+    t = np.ascontiguousarray(np.broadcast_to(t, (n,)), np.float64)
+    fp = np.ascontiguousarray(np.broadcast_to(0.0 if footprint is None else footprint, (n,)), np.float64)
+    out = np.zeros((n, 3), np.float32)
+    if du is not None:
+        du = np.ascontiguousarray(np.broadcast_to(du, (n, 2)), np.float64)
+        dv = np.ascontiguousarray(np.broadcast_to(dv, (n, 2)), np.float64)
+    lib.orc_texture_sample(w, h, _p(a), wrap, _p(s), _p(t), _p(fp), None if du is None else _p(du),
+                           None if dv is None else _p(dv), n, _p(out))
+    return out
+
+
+def pattern_eval(scene, index, st, p=(0, 0, 0), n=(0, 0, 1), footprint=0.0, dst_major=(0, 0), dst_minor=(0, 0)):
+    """pattern::eval of node `index` of a scenes.Scene's graph at shading
+    points (st n x 2; the others broadcast): n x 3 rgb."""
+    lib = _pt_sigs()
+    st = np.ascontiguousarray(np.atleast_2d(st), np.float64)
+    m = st.shape[0]
+    b = lambda v, k: np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (m, k) if k > 1 else (m,)),
+                                          np.float64)
+    P, N, F, A, B = b(p, 3), b(n, 3), b(footprint, 1), b(dst_major, 2), b(dst_minor, 2)
+    d = scene.desc()
+    out = np.zeros((m, 3))
+    lib.orc_pattern_eval(d.num_textures, ctypes.cast(d.textures, ctypes.c_void_p), d.num_patterns,
+                         ctypes.cast(d.patterns, ctypes.c_void_p), index, m, _p(st), _p(P), _p(N), _p(F), _p(A),
+                         _p(B), _p(out))
+    return out
 
 
 # ---- neural radiance cache ---------------------------------------------------

@@ -98,6 +98,12 @@ struct Mat4 {
     return r;
   }
   bool inverse(Mat4& out) const;  // cofactor inverse (matrix.rs:86-197)
+  // matrix.rs:246-251: the cube root of the axes' stretch product
+  double approx_scale() const {
+    const double lx = transform_vec({1, 0, 0}).length(), ly = transform_vec({0, 1, 0}).length(),
+                 lz = transform_vec({0, 0, 1}).length();
+    return std::fmax(std::cbrt(lx * ly * lz), 1e-12);
+  }
 };
 
 }  // namespace orc

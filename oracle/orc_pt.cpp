@@ -733,6 +733,14 @@ bool SphereObj::intersect(const Ray& ray, Intersection* out) const {
   out->normal = lp.normalize().normalize();
   out->front_face = ray.direction.dot(out->normal) < 0.0;
   out->material = material;
+  // sphere.rs:86-100 with thetamax 360, z in [-r, r], unit scale
+  double phi = std::atan2(lp.y, lp.x) * 180.0 / PI;
+  if (phi < 0.0) phi += 360.0;
+  const double zr = std::fmax(2.0 * radius, 1e-12);
+  const double ring = std::fmax(std::sqrt(lp.x * lp.x + lp.y * lp.y), 1e-6 * std::fmax(radius, 1e-12));
+  out->st[0] = std::clamp(phi / 360.0, 0.0, 1.0);
+  out->st[1] = std::clamp((lp.z + radius) / zr, 0.0, 1.0);
+  out->st_density = 1.0 / std::sqrt(std::fmax(ring * 2.0 * PI * (zr * radius / ring), 1e-24));
   return true;
 }
 
@@ -747,7 +755,7 @@ Mat4 InstanceObj::inverse_at(double time) const {
 
 // mesh.rs:308-381 -> the parametric hit and world normal of one instance
 bool Scene::instance_hit(const InstanceObj& inst, const Ray& ray, double t_max, double* t_param,
-                         Vec3* world_normal, Vec3* world_tangent) const {
+                         Vec3* world_normal, Vec3* world_tangent, Intersection* st_out) const {
   const Mat4 inverse = inst.inverse_at(ray.time);
   Ray local = Ray::raw(inverse.transform_point(ray.origin), inverse.transform_vec(ray.direction));
   if (world_tangent) *world_tangent = Vec3::zero();
@@ -770,6 +778,12 @@ bool Scene::instance_hit(const InstanceObj& inst, const Ray& ray, double t_max, 
     *world_normal = inverse.transform_normal(bn).normalize();
     // the tangent goes through the shutter-open transform (mesh.rs:334)
     if (world_tangent) *world_tangent = inst.transform.transform_vec(bt).normalize();
+    if (st_out) {  // mesh.rs:339-342: st = [0.5, v], no density
+      st_out->st[0] = 0.5;
+      st_out->st[1] = vv;
+      st_out->st_density = 0.0;
+      st_out->st_grad[0] = st_out->st_grad[1] = Vec3::zero();
+    }
     return true;
   }
   const MeshObj& mo = meshes[inst.mesh];
@@ -803,6 +817,33 @@ bool Scene::instance_hit(const InstanceObj& inst, const Ray& ray, double t_max, 
   }
   *t_param = t;
   *world_normal = inverse.transform_normal(ln).normalize();
+  if (st_out) {
+    // mesh.rs:145-197, 357-370: st interpolated at the hit, density the
+    // square root of st area over geometric area over the instance scale,
+    // world gradients of s and t through the (static) inverse transform
+    st_out->st[0] = st_out->st[1] = st_out->st_density = 0.0;
+    st_out->st_grad[0] = st_out->st_grad[1] = Vec3::zero();
+    if (!mo.st.empty()) {
+      auto stv = [&](uint32_t i, int c) { return (double)mo.st[2 * i + c]; };
+      const double w = 1.0 - hu - hv;
+      st_out->st[0] = stv(i0, 0) * w + stv(i1, 0) * hu + stv(i2, 0) * hv;
+      st_out->st[1] = stv(i0, 1) * w + stv(i1, 1) * hu + stv(i2, 1) * hv;
+      const double st_area = 0.5 * std::fabs((double)(mo.st[2 * i1] - mo.st[2 * i0]) * (double)(mo.st[2 * i2 + 1] - mo.st[2 * i0 + 1]) -
+                                             (double)(mo.st[2 * i2] - mo.st[2 * i0]) * (double)(mo.st[2 * i1 + 1] - mo.st[2 * i0 + 1]));
+      const Vec3 p0 = m.vertex(i0), e1 = m.vertex(i1) - p0, e2 = m.vertex(i2) - p0;
+      const double geo_area = 0.5 * e1.cross(e2).length();
+      st_out->st_density = (geo_area > 1e-18 ? std::sqrt(st_area / geo_area) : 0.0) / inst.scale;
+      const Vec3 n = e1.cross(e2);
+      const double n2 = n.length_squared();
+      if (n2 >= 1e-24) {
+        const Vec3 dual1 = e2.cross(n) / n2, dual2 = e1.cross(n) / -n2;
+        const double ds1 = (double)(mo.st[2 * i1] - mo.st[2 * i0]), dt1 = (double)(mo.st[2 * i1 + 1] - mo.st[2 * i0 + 1]);
+        const double ds2 = (double)(mo.st[2 * i2] - mo.st[2 * i0]), dt2 = (double)(mo.st[2 * i2 + 1] - mo.st[2 * i0 + 1]);
+        st_out->st_grad[0] = inst.inverse.transform_normal(dual1 * ds1 + dual2 * ds2);
+        st_out->st_grad[1] = inst.inverse.transform_normal(dual1 * dt1 + dual2 * dt2);
+      }
+    }
+  }
   return true;
 }
 
@@ -1120,7 +1161,84 @@ bool QuadricObj::intersect(const Ray& ray, Intersection* out) const {
   out->normal = inverse.transform_normal(ln).normalize();
   out->front_face = kind == RRHIP_QUADRIC_SPHERE ? ray.direction.dot(out->normal) < 0.0 : true;
   out->material = material;
+  quadric_st(p, out);
   return true;
+}
+
+// The RiSpec parameterisation of each quadric and its st density: u along
+// theta, v along the profile, density 1 / sqrt(|dp/du| |dp/dv|) over the
+// transform's scale (sphere.rs:86-100, cylinder.rs:100-109, extra.rs Torus
+// 272-289, Disk 364-369, Paraboloid 461-470, Hyperboloid 603-618). Cones
+// carry none (cone.rs).
+static double phi_u(double x, double y, double thetamax) {  // extra.rs:35-41
+  double phi = std::atan2(y, x) * 180.0 / PI;
+  if (phi < 0.0) phi += 360.0;
+  return std::clamp(phi / std::fmax(thetamax, 1e-9), 0.0, 1.0);
+}
+
+void QuadricObj::quadric_st(const Vec3& p, Intersection* out) const {
+  const double rad_theta = thetamax * PI / 180.0;
+  double u = 0, v = 0, d = 0;
+  switch (kind) {
+    case RRHIP_QUADRIC_SPHERE: {
+      u = phi_u(p.x, p.y, thetamax);  // sphere.rs clamps phi / thetamax the same way
+      const double zr = std::fmax(zmax - zmin, 1e-12);
+      v = std::clamp((p.z - zmin) / zr, 0.0, 1.0);
+      const double ring = std::fmax(std::sqrt(p.x * p.x + p.y * p.y), 1e-6 * std::fmax(radius, 1e-12));
+      d = 1.0 / std::sqrt(std::fmax(ring * rad_theta * (zr * radius / ring), 1e-24)) / scale;
+      break;
+    }
+    case RRHIP_QUADRIC_CYLINDER: {
+      u = phi_u(p.x, p.y, thetamax);
+      const double zr = std::fmax(zmax - zmin, 1e-12);
+      v = std::clamp((p.z - zmin) / zr, 0.0, 1.0);
+      d = 1.0 / std::sqrt(std::fmax(std::fmax(radius, 1e-12) * rad_theta * zr, 1e-24)) / scale;
+      break;
+    }
+    case RRHIP_QUADRIC_DISK: {
+      const double rh = std::sqrt(p.x * p.x + p.y * p.y);
+      u = phi_u(p.x, p.y, thetamax);
+      v = std::clamp((radius - rh) / std::fmax(radius, 1e-12), 0.0, 1.0);
+      const double dpdu = std::fmax(rh, 1e-6 * std::fmax(radius, 1e-12)) * rad_theta;
+      d = 1.0 / std::sqrt(std::fmax(dpdu * radius, 1e-24)) / scale;
+      break;
+    }
+    case RRHIP_QUADRIC_PARABOLOID: {
+      u = phi_u(p.x, p.y, thetamax);
+      const double zr = std::fmax(zmax - zmin, 1e-12);
+      v = std::clamp((p.z - zmin) / zr, 0.0, 1.0);
+      const double ring = std::fmax(std::sqrt(p.x * p.x + p.y * p.y), 1e-6 * std::fmax(radius, 1e-12));
+      d = 1.0 / std::sqrt(std::fmax(ring * rad_theta * zr, 1e-24)) / scale;
+      break;
+    }
+    case RRHIP_QUADRIC_HYPERBOLOID: {
+      if (std::fabs(p2[2] - p1[2]) < EPSILON) break;  // the annulus case carries none here
+      u = phi_u(p.x, p.y, thetamax);
+      v = std::clamp((p.z - p1[2]) / (p2[2] - p1[2]), 0.0, 1.0);
+      const double ring = std::fmax(std::sqrt(p.x * p.x + p.y * p.y), 1e-12);
+      const double dx = p2[0] - p1[0], dy = p2[1] - p1[1], dz = p2[2] - p1[2];
+      const double sweep = std::fmax(std::sqrt(dx * dx + dy * dy + dz * dz), 1e-12);
+      d = 1.0 / std::sqrt(std::fmax(ring * rad_theta * sweep, 1e-24)) / scale;
+      break;
+    }
+    case RRHIP_QUADRIC_TORUS: {
+      const double R = radius, r = height;
+      const double rxy = std::fmax(std::sqrt(p.x * p.x + p.y * p.y), 1e-12);
+      u = phi_u(p.x, p.y, thetamax);
+      double vphi = std::atan2(p.z, rxy - R) * 180.0 / PI;
+      while (vphi < zmin) vphi += 360.0;
+      while (vphi >= zmin + 360.0) vphi -= 360.0;
+      const double vr = std::fmax(zmax - zmin, 1e-9);
+      v = std::clamp((vphi - zmin) / vr, 0.0, 1.0);
+      d = 1.0 / std::sqrt(std::fmax(rxy * rad_theta * (std::fmax(r, 1e-12) * vr * PI / 180.0), 1e-24)) / scale;
+      break;
+    }
+    default:
+      break;
+  }
+  out->st[0] = u;
+  out->st[1] = v;
+  out->st_density = d;
 }
 
 bool Scene::intersect(const Ray& ray, Intersection* out) const {
@@ -1168,7 +1286,8 @@ bool Scene::intersect(const Ray& ray, Intersection* out) const {
       best_inst = win;
       best_t = wt;
       Vec3 best_tan;
-      instance_hit(instances[best_inst], ray, INF, &best_t, &best_n, &best_tan);
+      Intersection best_st;
+      instance_hit(instances[best_inst], ray, INF, &best_t, &best_n, &best_tan, &best_st);
       // mesh.rs:427-439 (to_intersection)
       Vec3 p = ray.at(best_t);
       Vec3 n = best_n;
@@ -1182,6 +1301,11 @@ bool Scene::intersect(const Ray& ray, Intersection* out) const {
         out->front_face = front;
         out->material = instances[best_inst].material;
         out->tangent = best_tan;
+        out->st[0] = best_st.st[0];
+        out->st[1] = best_st.st[1];
+        out->st_density = best_st.st_density;
+        out->st_grad[0] = best_st.st_grad[0];
+        out->st_grad[1] = best_st.st_grad[1];
         found = true;
       }
     }
@@ -1425,7 +1549,8 @@ Vec3 Scene::visibility(const Vec3& origin0, const Vec3& dir, double remaining, P
       remaining -= h.t + ray_offset;
       continue;
     }
-    const double presence = std::fmin(std::fmax(m.presence, 0.0), 1.0);
+    double presence = 1.0;
+    shade_params(m, h, dir, 0.0, &presence);
     if (presence >= 1.0) return Vec3::zero();
     vis = vis * (1.0 - presence);
     if (max_component(vis) < 1e-4) return Vec3::zero();
@@ -1582,6 +1707,93 @@ static double light_pdf_solid_angle(const Light& light, const Vec3& origin, cons
   return 0.0;
 }
 
+PatternNode pattern_from_desc(const rrhip_pattern_desc& pd) {
+  PatternNode n;
+  n.kind = (int)pd.kind;
+  n.texture = pd.texture;
+  if (pd.udim) n.udim.assign(pd.udim, pd.udim + 100);
+  n.wrap = (Wrap)pd.wrap;
+  n.scale[0] = pd.scale[0];
+  n.scale[1] = pd.scale[1];
+  n.color_a = Vec3{pd.color_a[0], pd.color_a[1], pd.color_a[2]};
+  n.color_b = Vec3{pd.color_b[0], pd.color_b[1], pd.color_b[2]};
+  n.frequency = pd.frequency;
+  n.octaves = pd.octaves;
+  n.gain = pd.gain;
+  n.lacunarity = pd.lacunarity;
+  for (int k = 0; k < 3; k++) {
+    n.input[k] = pd.input[k];
+    n.constant[k] = Vec3{pd.constant[k][0], pd.constant[k][1], pd.constant[k][2]};
+  }
+  n.cc_gain = Vec3{pd.cc_gain[0], pd.cc_gain[1], pd.cc_gain[2]};
+  n.cc_offset = Vec3{pd.cc_offset[0], pd.cc_offset[1], pd.cc_offset[2]};
+  n.gamma = pd.gamma;
+  n.saturation = pd.saturation;
+  n.use_t = pd.use_t != 0;
+  for (uint32_t k = 0; k < pd.num_stops; k++) {
+    n.positions.push_back(pd.positions[k]);
+    n.colors.push_back(Vec3{pd.colors[3 * k], pd.colors[3 * k + 1], pd.colors[3 * k + 2]});
+  }
+  return n;
+}
+
+PbrParams Scene::shade_params(const Material& m, const Intersection& hit, const Vec3& ray_dir, double cone_width,
+                              double* presence) const {
+  *presence = std::fmin(std::fmax(m.presence, 0.0), 1.0);
+  if (m.bindings.empty()) return m.pbr;
+  // the ray cone's disk projected on the surface: minor axis the cone
+  // width, major axis stretched by 1 / |cos| along the view direction; the
+  // st Jacobian turns both into texture-space axes (EWA)
+  double dmaj[2] = {0, 0}, dmin[2] = {0, 0};
+  const Vec3 gs = hit.st_grad[0], gt = hit.st_grad[1];
+  if (gs.length_squared() + gt.length_squared() > 1e-18 && cone_width > 0.0) {
+    const Vec3 n = hit.normal, d = ray_dir.normalize();
+    const double cs = std::fmax(std::fabs(d.dot(n)), 1.0 / 16.0);
+    Vec3 t1 = d - n * d.dot(n);
+    if (t1.length_squared() < 1e-12) t1 = std::fabs(n.x) < 0.9 ? Vec3{1, 0, 0}.cross(n) : Vec3{0, 1, 0}.cross(n);
+    t1 = t1.normalize() * (cone_width / cs);
+    const Vec3 t2 = n.cross(t1).normalize() * cone_width;
+    dmaj[0] = gs.dot(t1);
+    dmaj[1] = gt.dot(t1);
+    dmin[0] = gs.dot(t2);
+    dmin[1] = gt.dot(t2);
+  }
+  ShadeCtx ctx;
+  ctx.st[0] = hit.st[0];
+  ctx.st[1] = hit.st[1];
+  ctx.p = hit.point;
+  ctx.n = hit.normal;
+  ctx.footprint = cone_width * hit.st_density;
+  for (int k = 0; k < 2; k++) {
+    ctx.dst_major[k] = dmaj[k];
+    ctx.dst_minor[k] = dmin[k];
+  }
+  PbrParams p = m.pbr;
+  double pres = m.presence;
+  for (const auto& b : m.bindings) {  // BoundField::apply (pattern.rs:256-272)
+    const Vec3 v = pattern_eval(patterns, textures, b.second, ctx);
+    switch (b.first) {
+      case BIND_DIFFUSE_COLOR: p.diffuse_color = v; break;
+      case BIND_DIFFUSE_GAIN: p.diffuse_gain = v.x; break;
+      case BIND_DIFFUSE_ROUGHNESS: p.diffuse_roughness = v.x; break;
+      case BIND_SPECULAR_FACE_COLOR: p.specular_face_color = v; break;
+      case BIND_SPECULAR_EDGE_COLOR: p.specular_edge_color = v; break;
+      case BIND_SPECULAR_ROUGHNESS: p.specular_roughness = std::clamp(v.x, 0.005, 1.0); break;
+      case BIND_CLEARCOAT_GAIN: p.clearcoat_gain = v.x; break;
+      case BIND_FUZZ_GAIN: p.fuzz_gain = v.x; break;
+      case BIND_FUZZ_COLOR: p.fuzz_color = v; break;
+      case BIND_GLASS_GAIN: p.glass_gain = v.x; break;
+      case BIND_GLASS_ROUGHNESS: p.glass_roughness = std::clamp(v.x, 0.005, 1.0); break;
+      case BIND_REFRACTION_COLOR: p.refraction_color = v; break;
+      case BIND_GLOW: p.glow = v; break;
+      case BIND_PRESENCE: pres = std::clamp(v.x, 0.0, 1.0); break;
+      default: break;
+    }
+  }
+  *presence = std::fmin(std::fmax(pres, 0.0), 1.0);
+  return p;
+}
+
 // pt/mod.rs:449-990, surface subset
 Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays, AuxSample* aux) const {
   const double time = ray.time;  // the path's shutter time
@@ -1603,6 +1815,11 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays, A
   MediumStack ms;  // pt/mod.rs:482
   ms.atmosphere = atmosphere;
   uint32_t skips = 0;  // hull crossings and cutout pass-throughs (pt/mod.rs:476)
+  // The ray cone for texture footprints (pt/mod.rs:477-480, 687): its width
+  // grows by t times the spread at every surface hit; the spread starts at
+  // one pixel's angle (pt/mod.rs:76-77) and widens after rough bounces.
+  double cone_width = 0.0;
+  double cone_spread = std::tan(camera.fov * PI / 180.0 / 2.0) * 2.0 / (double)camera.height;
   while (depth < rp.max_depth) {
     if (rays) ++*rays;
     Intersection hit;
@@ -1685,6 +1902,7 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays, A
         double pdf;
         const Vec3 wi = hg_sample(md.g, wo, u1, u2, &pdf);
         prev_pdf = pdf;
+        cone_spread += 0.4;
         prev_origin = p;
         from_camera = false;
         ray = Ray::make(p, wi);
@@ -1729,11 +1947,15 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays, A
       ray.time = time;
       continue;
     }
-    // The first real hit's AOVs, cutouts included (pt/mod.rs:688-700).
+    // The first real hit's AOVs, cutouts included (pt/mod.rs:688-700),
+    // from the parameters the material's patterns give here.
     travel += hit.t;
+    cone_width += hit.t * cone_spread;
+    double presence = 1.0;
+    const PbrParams pbr = shade_params(mat, hit, ray.direction, cone_width, &presence);
     if (!aux_set) {
       aux_set = true;
-      const PbrParams& pb = mat.pbr;
+      const PbrParams& pb = pbr;
       const Vec3 a = pb.diffuse_color * pb.diffuse_gain + pb.specular_f0() * (pb.has_specular() ? 1.0 : 0.0) +
                      pb.refraction_color * pb.glass_gain;
       aux->albedo = Vec3{std::fmin(a.x, 1.0), std::fmin(a.y, 1.0), std::fmin(a.z, 1.0)};
@@ -1744,14 +1966,12 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays, A
     }
     // Presence cutout: pass through with probability 1 - presence
     // (pt/mod.rs:702-710).
-    const double presence = std::fmin(std::fmax(mat.presence, 0.0), 1.0);
     if (presence < 1.0 && rng.next_f64() >= presence && skips < 16) {
       skips++;
       ray = Ray::make(hit.point + ray.direction.normalize() * ray_offset, ray.direction);
       ray.time = time;
       continue;
     }
-    const PbrParams& pbr = mat.pbr;
     Vec3 wo = -ray.direction.normalize();
     bool entering = hit.front_face;
     Vec3 n = hit.normal.dot(wo) >= 0.0 ? hit.normal : -hit.normal;
@@ -1795,6 +2015,7 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays, A
       prev_pdf = pv;
       prev_origin = hit.point;
       from_camera = false;
+      cone_spread += 0.4;
       const Vec3 side = wi_world.dot(hit.normal) >= 0.0 ? hit.normal : -hit.normal;
       ray = Ray::make(hit.point + side * ray_offset, wi_world);
       ray.time = time;
@@ -1835,6 +2056,7 @@ Vec3 Scene::trace(Ray ray, Pcg32& rng, const RenderParams& rp, uint64_t* rays, A
     prev_pdf = s.pdf;
     prev_origin = hit.point;
     from_camera = false;
+    cone_spread += std::fmin(1.0 / (1.0 + s.pdf), 0.4);
     if (s.transmitted && mat.interior >= 0) {  // pt/mod.rs:969-975
       if (entering)
         ms.push(mat.interior);
@@ -2045,6 +2267,7 @@ void Scene::load(const rrhip_scene_desc& d) {
     for (int r = 0; r < 3; r++)
       for (int c = 0; c < 4; c++) o.transform.m[r][c] = q.object_to_world[4 * r + c];
     if (!o.transform.inverse(o.inverse)) o.inverse = Mat4::identity();
+    o.scale = o.transform.approx_scale();
     quadrics.push_back(o);
   }
   meshes.clear();
@@ -2054,6 +2277,7 @@ void Scene::load(const rrhip_scene_desc& d) {
     const rrhip_mesh_desc& md = d.meshes[i];
     meshes[i].mesh.build(md.positions, md.num_vertices, md.indices, md.num_triangles, bp);
     if (md.normals) meshes[i].normals.assign(md.normals, md.normals + 3 * md.num_vertices);
+    if (md.st) meshes[i].st.assign(md.st, md.st + 2 * md.num_vertices);
   }
   curve_sets.clear();
   curve_sets.resize(d.num_curve_sets);
@@ -2074,6 +2298,7 @@ void Scene::load(const rrhip_scene_desc& d) {
       for (int cc = 0; cc < 4; cc++) in.transform.m[r][cc] = id.object_to_world[4 * r + cc];
     if (!in.transform.inverse(in.inverse)) in.inverse = Mat4::identity();
     in.moving = id.moving != 0;
+    in.scale = in.transform.approx_scale();
     in.transform1 = Mat4::identity();
     if (in.moving)
       for (int r = 0; r < 3; r++)
@@ -2093,6 +2318,15 @@ void Scene::load(const rrhip_scene_desc& d) {
     instances.push_back(in);
   }
   tlas.build(ib, bp);
+  // textures (the caller's level 0; the chain rebuilt as tex.rs writes it),
+  // pattern nodes and the materials' bindings
+  textures.assign(d.num_textures, Texture{});
+  for (uint32_t i = 0; i < d.num_textures; i++)
+    textures[i].build_chain(d.textures[i].width, d.textures[i].height, d.textures[i].texels);
+  patterns.assign(d.num_patterns, PatternNode{});
+  for (uint32_t i = 0; i < d.num_patterns; i++) patterns[i] = pattern_from_desc(d.patterns[i]);
+  for (uint32_t i = 0; i < d.num_bindings; i++)
+    materials[d.bindings[i].material].bindings.push_back({(int)d.bindings[i].field, (int)d.bindings[i].pattern});
   has_motion = camera.has_motion_inv;
   for (const InstanceObj& in : instances) has_motion = has_motion || in.moving;
 }

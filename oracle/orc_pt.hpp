@@ -15,6 +15,7 @@
 #include "../include/rrhip.h"
 #include "orc_accel.hpp"
 #include "orc_hair.hpp"
+#include "orc_pattern.hpp"
 #include "orc_sampling.hpp"
 
 namespace orc {
@@ -52,6 +53,8 @@ struct Material {
   uint32_t id = 0;        // Material::id, the id AOV's object id (material.rs:39)
   bool has_hair = false;  // Material::hair (material.rs): curve hits scatter with the fiber BSDF
   HairParams hair;
+  // Material::pattern_bindings (material.rs:96-102): (BoundField, node)
+  std::vector<std::pair<int, int>> bindings;
   // pt/mod.rs:662-667: an interior medium and no lobes or emission.
   bool hull() const;
 };
@@ -179,6 +182,11 @@ struct Intersection {
   bool front_face = true;
   uint32_t material = 0;
   Vec3 tangent;  // curve hits: the world tangent (zero for surfaces)
+  // intersection.rs:18-24: surface parameters, st units per world unit,
+  // world-space gradients of s and t (zero where the primitive has none)
+  double st[2] = {0, 0};
+  double st_density = 0;
+  Vec3 st_grad[2];
 };
 
 struct TriangleObj {
@@ -198,15 +206,18 @@ struct SphereObj {
 // Paraboloid) with their object-to-world transform.
 struct QuadricObj {
   uint32_t kind = 0, material = 0;
+  double scale = 1.0;  // transform.approx_scale(), for st density
   double radius = 0, zmin = 0, zmax = 0, height = 0, thetamax = 360;
   double p1[3] = {0, 0, 0}, p2[3] = {0, 0, 0};  // hyperboloid
   Mat4 transform, inverse;
   bool intersect(const Ray& ray, Intersection* out) const;
+  void quadric_st(const Vec3& p, Intersection* out) const;
 };
 
 struct MeshObj {
   Mesh mesh;
   std::vector<float> normals;  // empty: geometric
+  std::vector<float> st;       // per-vertex st pairs, empty: none (mesh.rs:20-34)
 };
 
 struct InstanceObj {
@@ -216,6 +227,7 @@ struct InstanceObj {
   bool moving = false;
   Mat4 transform1;  // at shutter close
   Aabb world_bounds;
+  double scale = 1.0;  // Instance::scale = transform.approx_scale() (mesh.rs:236-238, matrix.rs:246-251)
   // mesh.rs:294-305: the inverse transform at shutter time `time`.
   Mat4 inverse_at(double time) const;
 };
@@ -258,6 +270,9 @@ struct RenderParams {
   double ray_offset = 1e-4;  // pt/mod.rs:30
 };
 
+// One pattern node from its ABI description (pattern.rs:42-68 fields).
+PatternNode pattern_from_desc(const rrhip_pattern_desc& pd);
+
 class Scene {
  public:
   Camera camera;
@@ -275,6 +290,8 @@ class Scene {
   int dome = -1;
   std::vector<Medium> media;
   int atmosphere = -1;
+  std::vector<Texture> textures;
+  std::vector<PatternNode> patterns;
   bool has_motion = false;  // scene_builder.rs:239-241 (moving instances or camera)
   // Push of ray origins off surfaces (pt/mod.rs:30: 1e-4). Tests set a
   // negligible push to model error-bounded origins (f64 rounding of hit
@@ -290,7 +307,12 @@ class Scene {
 
  private:
   bool instance_hit(const InstanceObj& inst, const Ray& ray, double t_max, double* t_param,
-                    Vec3* world_normal, Vec3* world_tangent = nullptr) const;
+                    Vec3* world_normal, Vec3* world_tangent = nullptr, Intersection* st_out = nullptr) const;
+  // shade_params (pt/mod.rs:391-439): the material's PbrParams with its
+  // pattern bindings applied at the hit (cone_width: the ray cone's width
+  // there), and the presence they give.
+  PbrParams shade_params(const Material& m, const Intersection& hit, const Vec3& ray_dir, double cone_width,
+                         double* presence) const;
   // eval(wi, f, pdf): f * cos (or the phase value) towards world wi and the
   // pdf of the scattering strategy, for MIS.
   using EvalFn = std::function<void(const Vec3&, Vec3*, double*)>;

@@ -529,3 +529,91 @@ void orc_hg_sample_n(double g, const double* wo, uint64_t seed, uint64_t stream,
 }
 
 }  // extern "C"
+
+// ---- textures and patterns (tests) ---------------------------------------------------
+namespace {
+orc::Texture texture_from(uint32_t w, uint32_t h, const float* rgb) {
+  orc::Texture t;
+  t.build_chain(w, h, rgb);
+  return t;
+}
+}  // namespace
+
+extern "C" {
+
+// Mip level `level` of a texture's chain (tex.rs downsample) into out
+// (its w * h * 3 floats); returns the level's width << 32 | height, 0 past
+// the chain's end.
+uint64_t orc_texture_level(uint32_t w, uint32_t h, const float* rgb, uint32_t level, float* out) {
+  const orc::Texture t = texture_from(w, h, rgb);
+  if (level >= t.levels.size()) return 0;
+  const auto& L = t.levels[level];
+  if (out) std::copy(L.rgb.begin(), L.rgb.end(), out);
+  return ((uint64_t)L.w << 32) | L.h;
+}
+
+// Trilinear lookups (cache.rs sample) at n (s, t, footprint) triples, or
+// EWA lookups (sample_ewa) when du / dv (2 per query) are given.
+void orc_texture_sample(uint32_t w, uint32_t h, const float* rgb, uint32_t wrap, const double* s, const double* t,
+                        const double* footprint, const double* du, const double* dv, uint64_t n, float* out) {
+  const orc::Texture tex = texture_from(w, h, rgb);
+  for (uint64_t i = 0; i < n; i++) {
+    if (du)
+      tex.sample_ewa(s[i], t[i], du + 2 * i, dv + 2 * i, (orc::Wrap)wrap, out + 3 * i);
+    else
+      tex.sample(s[i], t[i], footprint[i], (orc::Wrap)wrap, out + 3 * i);
+  }
+}
+
+// pattern::eval of node `index` at n shading points: st (2), p (3), n (3),
+// footprint (1), EWA axes dst_major, dst_minor (2 each) per point.
+void orc_pattern_eval(uint32_t ntex, const rrhip_texture_desc* texs, uint32_t npat, const rrhip_pattern_desc* pats,
+                      int32_t index, uint64_t n, const double* st, const double* p, const double* nrm,
+                      const double* footprint, const double* dmaj, const double* dmin, double* out) {
+  std::vector<orc::Texture> textures(ntex);
+  for (uint32_t i = 0; i < ntex; i++) textures[i].build_chain(texs[i].width, texs[i].height, texs[i].texels);
+  std::vector<orc::PatternNode> nodes(npat);
+  for (uint32_t i = 0; i < npat; i++) nodes[i] = orc::pattern_from_desc(pats[i]);
+  for (uint64_t i = 0; i < n; i++) {
+    orc::ShadeCtx c;
+    c.st[0] = st[2 * i];
+    c.st[1] = st[2 * i + 1];
+    c.p = Vec3{p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+    c.n = Vec3{nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]};
+    c.footprint = footprint[i];
+    for (int k = 0; k < 2; k++) {
+      c.dst_major[k] = dmaj[2 * i + k];
+      c.dst_minor[k] = dmin[2 * i + k];
+    }
+    const Vec3 v = orc::pattern_eval(nodes, textures, index, c);
+    out[3 * i] = v.x;
+    out[3 * i + 1] = v.y;
+    out[3 * i + 2] = v.z;
+  }
+}
+
+// Closest hits' texture coordinates: st (2), st density (1), world
+// gradients of s and t (6) per ray; t = -1 on a miss.
+void orc_scene_st_n(const rrhip_scene_desc* desc, const double* rays, uint64_t n, double* t, double* st,
+                    double* density, double* grad) {
+  Scene s;
+  s.load(*desc);
+  for (uint64_t i = 0; i < n; i++) {
+    const double* r = rays + 6 * i;
+    Intersection h;
+    if (!s.intersect(Ray::make(Vec3{r[0], r[1], r[2]}, Vec3{r[3], r[4], r[5]}), &h)) {
+      t[i] = -1.0;
+      continue;
+    }
+    t[i] = h.t;
+    st[2 * i] = h.st[0];
+    st[2 * i + 1] = h.st[1];
+    density[i] = h.st_density;
+    for (int k = 0; k < 3; k++) {
+      grad[6 * i + k] = h.st_grad[0][k];
+      grad[6 * i + 3 + k] = h.st_grad[1][k];
+    }
+  }
+}
+
+}  // extern "C"

@@ -361,6 +361,23 @@ hipError_t upload_pt(rrhip_scene* s, const PtSceneHost& h) {
   if (e == hipSuccess) e = upload_vec(h.quadrics, &s->pt_mem[19], &v.quadrics);
   if (e == hipSuccess) e = upload_vec(h.inst_fwd, &s->pt_mem[20], &v.inst_fwd);
   if (e == hipSuccess) e = upload_vec(h.csegs, &s->pt_mem[21], &v.csegs);
+  if (h.patterns) {  // pattern-bound materials: their data and the hits' texture coordinates
+    if (e == hipSuccess) e = upload_vec(h.raw_pbr, &s->pt_mem[22], &v.raw_pbr);
+    if (e == hipSuccess) e = upload_vec(h.binds, &s->pt_mem[23], &v.binds);
+    if (e == hipSuccess) e = upload_vec(h.prog, &s->pt_mem[24], &v.prog);
+    if (e == hipSuccess) e = upload_vec(h.pats, &s->pt_mem[25], &v.pats);
+    if (e == hipSuccess) e = upload_vec(h.ramp, &s->pt_mem[26], &v.ramp);
+    if (e == hipSuccess) e = upload_vec(h.udim, &s->pt_mem[27], &v.udim);
+    if (e == hipSuccess) e = upload_vec(h.texs, &s->pt_mem[28], &v.texs);
+    if (e == hipSuccess) e = upload_vec(h.tlev, &s->pt_mem[29], &v.tlev);
+    if (e == hipSuccess) e = upload_vec(h.texels, &s->pt_mem[30], &v.texels);
+    if (e == hipSuccess) e = upload_vec(h.mesh_st, &s->pt_mem[31], &v.mesh_st);
+    if (e == hipSuccess) e = upload_vec(h.st, &s->pt_mem[32], &v.st);
+    if (e == hipSuccess) e = upload_vec(h.inst_scale, &s->pt_mem[33], &v.inst_scale);
+    if (e == hipSuccess) e = upload_vec(h.quad_st, &s->pt_mem[34], &v.quad_st);
+  }
+  v.patterns = h.patterns;
+  v.pixel_spread = h.pixel_spread;
   v.num_quadrics = (uint32_t)h.quadrics.size();
   v.curves = h.curves;
   v.hair = h.hair;

@@ -138,6 +138,8 @@ int rrhip_batch_create(rrhip_ctx* ctx, uint32_t num_jobs, const rrhip_batch_job*
       return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_batch_create: ray_offset must be finite and >= 0");
     if (J.scene->pt.hair)
       return fail(RRHIP_ERR_UNSUPPORTED, "rrhip_batch_create: hair fibers render in sessions, not in batches");
+    if (J.scene->pt.patterns)
+      return fail(RRHIP_ERR_UNSUPPORTED, "rrhip_batch_create: pattern-bound materials render in sessions, not in batches");
     total_pix += (uint64_t)J.scene->pt.cam.width * J.scene->pt.cam.height;
   }
   uint32_t spp = samples_per_pass;

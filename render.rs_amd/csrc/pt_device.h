@@ -515,6 +515,10 @@ struct Hit {
   // an origin outside its rounding-error box, towards the ray's side.
   V3 p, e;
   V3 tan;  // curve hits: the world tangent (zero otherwise)
+  // Instance hits: the instance (~0u on loose objects), BLAS record and
+  // barycentrics (texture coordinates are rebuilt from them).
+  uint32_t inst, slot;
+  float bu, bv;
 };
 constexpr uint32_t kObjTri = 0, kObjSphere = 1, kObjQuadric = 2, kObjIndex = 0x3FFFFFFFu;
 

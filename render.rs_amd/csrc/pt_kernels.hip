@@ -18,6 +18,7 @@
 #include "pt_device.h"
 #include "pt_hair.h"
 #include "pt_kernels.h"
+#include "pt_pattern.h"
 #include "pt_trace.h"
 
 namespace rrhip {
@@ -198,6 +199,12 @@ __device__ __forceinline__ void extend_one(const PtSceneView& S, const PathBuffe
     if ((GEO & kGeoCurves) && dot(h.tan, h.tan) > 0.5f) tw = oct_encode(h.tan);
     P.hit_front[slot] = make_float4(front.x, front.y, front.z, tw.x);
     P.hit_back[slot] = make_float4(back.x, back.y, back.z, tw.y);
+    if constexpr ((GEO & kGeoSt) != 0) {  // pattern scenes: texture coordinates of the hit
+      const HitSt hs = hit_st(S, h);
+      P.hit_st[slot] = make_float4(hs.s, hs.t, hs.density, 0.f);
+      P.hit_gs[slot] = make_float4(hs.gs.x, hs.gs.y, hs.gs.z, 0.f);
+      P.hit_gt[slot] = make_float4(hs.gt.x, hs.gt.y, hs.gt.z, 0.f);
+    }
   } else {
     P.hit_n[slot] = make_float4(0.f, 0.f, 0.f, -1.f);
     P.hit_m[slot] = 0;
@@ -424,7 +431,11 @@ __device__ __forceinline__ bool sample_light(const PtSceneView& S, const PtLight
 // instead of error-bounded origins.
 // HAIR: the scene has hair fibers; their curve hits scatter with the fiber
 // BSDF (pt/mod.rs:738-800).
-template <bool NRC, bool TR, bool FILM = false, bool FIXED = false, bool HAIR = false>
+// PAT: the scene has pattern-bound materials: paths carry a ray cone (its
+// spread beyond one pixel in d.w, its width in prev.w, pt/mod.rs:477-480),
+// and a hit on such a material shades with the fields its patterns give
+// there (pt/mod.rs:685-689, 389-440).
+template <bool NRC, bool TR, bool FILM = false, bool FIXED = false, bool HAIR = false, bool PAT = false>
 __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffers& P, const PtParams& prm,
                                           const NrcPathBuffers& N, uint32_t slot, bool& cont, bool& shadow,
                                           bool& shadow2, ShadowOut& se, bool& query, ShadowOut& so, NrcQuery& nq,
@@ -483,6 +494,7 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
     };
     constexpr bool walk = TR;  // shadow rays pick up media transmittance on their walk
     bool passed = false;                   // hull crossing or cutout pass-through
+    PtMaterial mres;                       // PAT: the hit's material with its patterns applied
     // The medium along this segment (pt/mod.rs:486-633): one light pick
     // per segment; for a point light an equiangular single-scatter
     // connection (pt/mod.rs:501-551); distance sampling, and at a real
@@ -567,6 +579,7 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
         float pdf;
         const float u1 = rnd(rng), u2 = rnd(rng);
         const V3 wi = hg_sample(md.g, wo, u1, u2, pdf);
+        if (PAT) a0 += 0.4f;  // the cone widens after a scatter (pt/mod.rs:622)
         depth++;
         bool alive = true;
         if (depth >= prm.rr_start) {
@@ -599,9 +612,22 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
       }
     } else {
       const uint32_t hm = P.hit_m[slot];
-      const PtMaterial m = S.materials[hm & 0x7FFFFFFFu];
+      PtMaterial m = S.materials[hm & 0x7FFFFFFFu];
       const bool entering = (hm >> 31) != 0;
       const V3 p = hit_point();
+      if constexpr (PAT) {
+        // The cone grows by this segment at every surface the path does not
+        // cross as a hull; bound fields follow their patterns here.
+        if (!(TR && m.hull && skips < kMaxSkips)) {
+          asum += hn.w * (S.pixel_spread + a0);
+          if (m.bind) {
+            const PatCtx pc = pat_ctx(P.hit_st[slot], P.hit_gs[slot], P.hit_gt[slot], p, v3(hn.x, hn.y, hn.z), rd,
+                                      asum);
+            pat_resolve_material(S, hm & 0x7FFFFFFFu, m.bind, pc, m);
+          }
+        }
+        mres = m;
+      }
       if (FILM && !aux_set) {
         if (TR && m.hull && skips < kMaxSkips) {
           travel += hn.w;  // a hull crossing is looked through (pt/mod.rs:671)
@@ -633,6 +659,7 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
           skips++;
           const V3 org = bounded ? far_side() : p + rd * S.ray_offset;
           P.o[slot] = make_float4(org.x, org.y, org.z, prev_pdf);
+          if (PAT) P.prev[slot] = make_float4(pv4.x, pv4.y, pv4.z, asum);
           cont = true;
         }
       }
@@ -641,7 +668,7 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
       // the ray continues unchanged past the surface
     } else if (hn.w >= 0.f && !in_medium && !absorbed) {
       const uint32_t hm = P.hit_m[slot];
-      const PtMaterial m = S.materials[hm & 0x7FFFFFFFu];
+      const PtMaterial m = PAT ? mres : S.materials[hm & 0x7FFFFFFFu];
       const bool entering = (hm >> 31) != 0;
       const V3 p = hit_point();
       const V3 ng = v3(hn.x, hn.y, hn.z);
@@ -692,6 +719,7 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
           if (hair_sample(m, g, u, wf, fv, pv)) {
             const V3 wi = normalize(fiber_world(ff, wf));
             beta = beta * fv / pv;
+            if (PAT) a0 += 0.4f;  // pt/mod.rs:780
             if (FILM && !(flags & kFlagFirstSet)) flags |= kFlagFirstSet;
             const bool out = dot(wi, ng) >= 0.f;
             const V3 org = bounded ? (out ? p : far_side()) : p + ng * (out ? S.ray_offset : -S.ray_offset);
@@ -790,6 +818,7 @@ __device__ __forceinline__ void shade_one(const PtSceneView& S, const PathBuffer
       if (scatter && bsdf_sample(m, wo_l, eta, rng, bs)) {
         const V3 wi = to_world(fr, bs.wi);
         beta = beta * bs.f * (fabsf(bs.wi.z) / bs.pdf);
+        if (PAT) a0 += fminf(1.f / (1.f + bs.pdf), 0.4f);  // pt/mod.rs:967
         if (FILM && !(flags & kFlagFirstSet)) flags |= kFlagFirstSet | (bs.specular ? kFlagFirstSpec : 0u);
         const V3 org = bounded ? (bs.transmitted ? far_side() : p)
                                : p + nrm * (bs.transmitted ? -S.ray_offset : S.ray_offset);
@@ -873,7 +902,7 @@ __device__ __forceinline__ void push_shadow(const PathBuffers& P, bool act, uint
 
 // The shade pass over the queue q: each path's step, then the wave-wide
 // appends of its shadow records, continuation and cache query.
-template <bool NRC, bool TR, bool FILM, bool FIXED, bool HAIR>
+template <bool NRC, bool TR, bool FILM, bool FIXED, bool HAIR, bool PAT = false>
 __device__ __forceinline__ void shade_pass(const PtSceneView& S, const PathBuffers& P, const PtParams& prm,
                                            const uint32_t* __restrict__ q, uint32_t* __restrict__ qnext,
                                            const NrcPathBuffers& N) {
@@ -890,7 +919,8 @@ __device__ __forceinline__ void shade_pass(const PtSceneView& S, const PathBuffe
     uint32_t slot = 0;
     if (act) {
       slot = q[i];
-      shade_one<NRC, TR, FILM, FIXED, HAIR>(S, P, prm, N, slot, cont, shadow, shadow2, se, query, so, nq, qbeta);
+      shade_one<NRC, TR, FILM, FIXED, HAIR, PAT>(S, P, prm, N, slot, cont, shadow, shadow2, se, query, so, nq,
+                                                 qbeta);
     }
     push_shadow<TR, FILM>(P, act, slot, shadow, so, shadow2, se);
     const uint32_t nidx = wave_append(&P.counts[1], act && cont);
@@ -922,6 +952,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     PtSceneView S, PathBuffers P, PtParams prm, const uint32_t* __restrict__ q, uint32_t* __restrict__ qnext,
     NrcPathBuffers N) {
   shade_pass<false, TR, FILM, FIXED, true>(S, P, prm, q, qnext, N);
+}
+
+// Scenes with pattern-bound materials (and hair fibers, if any): the
+// pattern evaluator's value stack and the texture filters next to both
+// BSDFs, at two waves per SIMD.
+template <bool TR, bool FILM, bool FIXED>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void shade_pattern_kernel(
+    PtSceneView S, PathBuffers P, PtParams prm, const uint32_t* __restrict__ q, uint32_t* __restrict__ qnext,
+    NrcPathBuffers N) {
+  shade_pass<false, TR, FILM, FIXED, true, true>(S, P, prm, q, qnext, N);
 }
 
 // Fractional visibility along a shadow ray in a transparent scene
@@ -1053,12 +1093,13 @@ __global__ __launch_bounds__(kTraceBlock) void shadow_kernel(PtSceneView S, Path
 template <int STACK, bool MOTION, int GEO, bool COUNT>
 void launch_trace_counted(const PtSceneView& S, const PathBuffers& P, const uint32_t* q, uint32_t gt, bool extend,
                           hipStream_t st) {
+  constexpr int SG = GEO & ~kGeoSt;  // shadow rays need no texture coordinates
   if (extend)
     hipLaunchKernelGGL((extend_kernel<STACK, MOTION, GEO, COUNT>), dim3(gt), dim3(kTraceBlock), 0, st, S, P, q);
   else if (S.transparent)
-    hipLaunchKernelGGL((shadow_kernel<STACK, true, MOTION, GEO, COUNT>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
+    hipLaunchKernelGGL((shadow_kernel<STACK, true, MOTION, SG, COUNT>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
   else
-    hipLaunchKernelGGL((shadow_kernel<STACK, false, MOTION, GEO, COUNT>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
+    hipLaunchKernelGGL((shadow_kernel<STACK, false, MOTION, SG, COUNT>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
 }
 
 template <int STACK, bool MOTION, int GEO>
@@ -1070,7 +1111,9 @@ void launch_trace_kernels(const PtSceneView& S, const PathBuffers& P, const uint
     launch_trace_counted<STACK, MOTION, GEO, false>(S, P, q, gt, extend, st);
 }
 
-// Scenes with loose quadrics take the full-stack kernels with the quadric
+// Scenes with pattern-bound materials take the full kernels whose
+// extension rays also record each hit's texture coordinates. Scenes with
+// loose quadrics take the full-stack kernels with the quadric
 // solvers (and the curve test, for scenes with both), scenes with curve
 // sets alone the full-stack kernels with the curve test only (the f64
 // quartic's registers stay out of them); one instantiation per motion
@@ -1079,7 +1122,11 @@ void launch_trace(const PtSceneView& S, const PathBuffers& P, const uint32_t* q,
                   hipStream_t st) {
   const bool small = S.stack_need <= 32;
   constexpr int kAll = kGeoQuadrics | kGeoCurves;
-  if (S.num_quadrics && S.motion)
+  if (S.patterns && S.motion)  // hits also record their texture coordinates
+    launch_trace_kernels<kTraceStack, true, kAll | kGeoSt>(S, P, q, gt, extend, st);
+  else if (S.patterns)
+    launch_trace_kernels<kTraceStack, false, kAll | kGeoSt>(S, P, q, gt, extend, st);
+  else if (S.num_quadrics && S.motion)
     launch_trace_kernels<kTraceStack, true, kAll>(S, P, q, gt, extend, st);
   else if (S.num_quadrics)
     launch_trace_kernels<kTraceStack, false, kAll>(S, P, q, gt, extend, st);
@@ -1538,7 +1585,16 @@ template <bool FIXED>
 void launch_shade(const PtSceneView& S, const PathBuffers& P, const PtParams& prm, const uint32_t* qshade,
                   uint32_t* qnext, const NrcPathBuffers& N, uint32_t g, hipStream_t st) {
   const dim3 G(g), B(kBlock);
-  if (S.hair) {
+  if (S.patterns) {
+    if (P.ld && S.transparent)
+      hipLaunchKernelGGL((shade_pattern_kernel<true, true, FIXED>), G, B, 0, st, S, P, prm, qshade, qnext, N);
+    else if (P.ld)
+      hipLaunchKernelGGL((shade_pattern_kernel<false, true, FIXED>), G, B, 0, st, S, P, prm, qshade, qnext, N);
+    else if (S.transparent)
+      hipLaunchKernelGGL((shade_pattern_kernel<true, false, FIXED>), G, B, 0, st, S, P, prm, qshade, qnext, N);
+    else
+      hipLaunchKernelGGL((shade_pattern_kernel<false, false, FIXED>), G, B, 0, st, S, P, prm, qshade, qnext, N);
+  } else if (S.hair) {
     if (P.ld && S.transparent)
       hipLaunchKernelGGL((shade_hair_kernel<true, true, FIXED>), G, B, 0, st, S, P, prm, qshade, qnext, N);
     else if (P.ld)

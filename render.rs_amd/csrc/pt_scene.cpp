@@ -1,11 +1,14 @@
 // Host build of path-tracing scenes (see pt_scene.h).
 #include "pt_scene.h"
 
+#include "pt_material.h"
+
 #include "bvh_build.h"
 
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <functional>
 #include <numeric>
 
 namespace rrhip {
@@ -35,69 +38,46 @@ void put(float* dst, D3 v) {
 }
 bool finite3(const double a[3]) { return std::isfinite(a[0]) && std::isfinite(a[1]) && std::isfinite(a[2]); }
 
-double alpha_of(double r) { return std::min(std::max(r * r, 2.5e-5), 1.0); }  // bxdf.rs:187-189
+// Matrix4::approx_scale: the cube root of the product of the linear part's
+// column lengths (the uniform scale an st density divides by).
+double approx_scale(const double* m) {
+  const auto col = [&](int c) { return std::sqrt(m[c] * m[c] + m[4 + c] * m[4 + c] + m[8 + c] * m[8 + c]); };
+  return std::max(std::cbrt(col(0) * col(1) * col(2)), 1e-12);
+}
 
-// pbr.rs:84-93, 191-202; bxdf.rs:137-143, 145-171 (lobe constants)
+// The PbrParams of a material description (pbr.rs:11-45).
+template <class T>
+RawPbr<T> raw_pbr(const rrhip_material_desc& m) {
+  RawPbr<T> r{};
+  const auto cp = [](T* dst, const double* src) {
+    for (int c = 0; c < 3; c++) dst[c] = (T)src[c];
+  };
+  r.diffuse_gain = (T)m.diffuse_gain;
+  cp(r.diffuse_color, m.diffuse_color);
+  r.diffuse_roughness = (T)m.diffuse_roughness;
+  cp(r.specular_face_color, m.specular_face_color);
+  cp(r.specular_edge_color, m.specular_edge_color);
+  r.specular_roughness = (T)m.specular_roughness;
+  r.specular_ior = (T)m.specular_ior;
+  r.clearcoat_gain = (T)m.clearcoat_gain;
+  r.clearcoat_roughness = (T)m.clearcoat_roughness;
+  r.fuzz_gain = (T)m.fuzz_gain;
+  cp(r.fuzz_color, m.fuzz_color);
+  r.glass_gain = (T)m.glass_gain;
+  r.glass_ior = (T)m.glass_ior;
+  r.glass_roughness = (T)m.glass_roughness;
+  cp(r.refraction_color, m.refraction_color);
+  cp(r.glow, m.glow);
+  cp(r.emission, m.emission);
+  return r;
+}
+
+// Lobe constants (pt_material.h) plus the material's identity, medium,
+// cutout and hair fiber parameters.
 PtMaterial make_material(const rrhip_material_desc& m) {
   PtMaterial g{};
-  D3 face = d3(m.specular_face_color);
-  D3 f0{0, 0, 0};
-  if (lum(face) > 1e-6) {
-    f0 = face;
-  } else if (m.specular_ior > 1.0) {
-    double r = (m.specular_ior - 1.0) / (m.specular_ior + 1.0);
-    f0 = {r * r, r * r, r * r};
-  }
-  double wd = m.diffuse_gain * lum(d3(m.diffuse_color));
-  double ws = std::sqrt(lum(f0));
-  double wc = m.clearcoat_gain * 0.2;
-  double wf = m.fuzz_gain * lum(d3(m.fuzz_color)) * 0.3;
-  double wg = m.glass_gain;
-  double total = wd + ws + wc + wf + wg;
-  g.has_lobes = total > 1e-9 ? 1u : 0u;
-  if (g.has_lobes) {
-    g.w[0] = (float)(wd / total);
-    g.w[1] = (float)(ws / total);
-    g.w[2] = (float)(wc / total);
-    g.w[3] = (float)(wf / total);
-    g.w[4] = (float)(wg / total);
-  }
-  double spec_take = lum(f0);
-  double coat_take = 0.04 * m.clearcoat_gain;
-  double under = std::min(std::max((1.0 - spec_take) * (1.0 - coat_take), 0.0), 1.0);
-  put(g.diffuse, mul(d3(m.diffuse_color), m.diffuse_gain * under / kPi));
-  double s2 = m.diffuse_roughness * m.diffuse_roughness;
-  g.on_a = (float)(1.0 - s2 / (2.0 * (s2 + 0.33)));
-  g.on_b = (float)(0.45 * s2 / (s2 + 0.09));
-  put(g.spec_f0, f0);
-  put(g.spec_f90, d3(m.specular_edge_color));
-  g.spec_alpha = (float)alpha_of(m.specular_roughness);
-  g.coat_f0 = (float)(0.04 * m.clearcoat_gain);
-  g.coat_f90 = (float)m.clearcoat_gain;
-  g.coat_alpha = (float)alpha_of(m.clearcoat_roughness);
-  put(g.fuzz, mul(d3(m.fuzz_color), m.fuzz_gain / kPi));
-  g.glass_scale = (float)(m.glass_gain * under);
-  put(g.refraction, d3(m.refraction_color));
-  g.glass_alpha = (float)alpha_of(m.glass_roughness);
-  g.glass_ior = (float)m.glass_ior;
-  put(g.emission, add(d3(m.emission), d3(m.glow)));
+  lobe_constants(raw_pbr<double>(m), g);
   g.area_light = m.area_light;
-  // Radiance-cache inputs: the lobes' summed albedo (clamped to [0, 1])
-  // and the roughness of the strongest non-diffuse lobe (1 if none).
-  D3 alb = add(add(mul(d3(m.diffuse_color), m.diffuse_gain * under), ws > 0 ? f0 : D3{0, 0, 0}),
-               mul(d3(m.refraction_color), m.glass_gain * under));
-  const double av[3] = {alb.x, alb.y, alb.z};
-  for (int c = 0; c < 3; c++) g.nrc_albedo[c] = (float)std::min(std::max(av[c], 0.0), 1.0);
-  g.sort_key = 1;
-  for (int i = 0; i < 5; i++)
-    if (g.w[i] > 0.f) g.sort_key += 1u << i;
-  g.nrc_rough = 1.f;
-  if (ws > 0 || wg > 0) g.nrc_rough = (float)(wg > ws ? m.glass_roughness : m.specular_roughness);
-  // Albedo AOV (pt/mod.rs:690-696): diffuse, the primary specular F0 when
-  // that lobe is active, refraction; each channel capped at 1.
-  const D3 fa = add(add(mul(d3(m.diffuse_color), m.diffuse_gain), lum(f0) > 1e-6 ? f0 : D3{0, 0, 0}),
-                    mul(d3(m.refraction_color), m.glass_gain));
-  put(g.film_albedo, D3{std::min(fa.x, 1.0), std::min(fa.y, 1.0), std::min(fa.z, 1.0)});
   g.id = m.id;
   g.interior = (int32_t)m.interior - 1;
   g.presence = (float)std::min(std::max(1.0 - m.cutout, 0.0), 1.0);
@@ -406,6 +386,7 @@ bool build_instances(const rrhip_scene_desc& d, PtSceneHost* out, std::string* e
     }
   }
   for (uint32_t k : order) {
+    out->inst_scale.push_back((float)approx_scale(d.instances[k].object_to_world));
     out->insts.push_back(raw[k]);
     out->inst_fwd.insert(out->inst_fwd.end(), raw_fwd.begin() + 12 * (size_t)k, raw_fwd.begin() + 12 * (size_t)k + 12);
   }
@@ -425,6 +406,11 @@ bool build_instances(const rrhip_scene_desc& d, PtSceneHost* out, std::string* e
     PtMeshInfo& mi = out->meshes[k];
     mi.root = blas[k].nodes.empty() ? 0xFFFFFFFFu : node_base;
     mi.curves = curves ? 1u : 0u;
+    out->mesh_st.push_back(0xFFFFFFFFu);
+    if (!curves && d.meshes[k].st) {
+      out->mesh_st.back() = (uint32_t)(out->st.size() / 2);
+      out->st.insert(out->st.end(), d.meshes[k].st, d.meshes[k].st + 2 * d.meshes[k].num_vertices);
+    }
     if (curves) {
       const auto& cr = crecs[k - d.num_meshes];
       out->csegs.insert(out->csegs.end(), cr.begin(), cr.end());
@@ -448,6 +434,149 @@ bool build_instances(const rrhip_scene_desc& d, PtSceneHost* out, std::string* e
   // One entry per TLAS level (two at the last), one per BLAS level, and
   // the remainder of a TLAS leaf re-pushed while its first instance runs.
   out->stack_need = (uint32_t)(tlas_levels + blas_levels + 2);
+  return true;
+}
+
+// Textures, pattern nodes and bindings (Scene::patterns,
+// Material::pattern_bindings): the mip chains (tex.rs:83-108, the 2x2 box
+// filter with odd sizes clamping the second tap, f32 sums in its order),
+// node records, and per binding the program the shade kernels run: the
+// node's input tree in post-order, as pattern::eval walks it (an input
+// deeper than 16 levels or naming no node evaluates to magenta).
+bool build_patterns(const rrhip_scene_desc& d, PtSceneHost* out, std::string* err) {
+  if (d.num_bindings == 0) return true;
+  if (!d.bindings) return *err = "num_bindings > 0 with null bindings", false;
+  if ((d.num_patterns && !d.patterns) || (d.num_textures && !d.textures))
+    return *err = "null pattern or texture arrays", false;
+  for (uint32_t i = 0; i < d.num_textures; i++) {
+    const rrhip_texture_desc& t = d.textures[i];
+    if (t.width == 0 || t.height == 0 || !t.texels || (uint64_t)t.width * t.height > (1ull << 28))
+      return *err = "texture needs 1..2^28 texels and data", false;
+    PtTexture g{};
+    g.first_level = (uint32_t)out->tlev.size();
+    g.w = t.width;
+    g.h = t.height;
+    std::vector<float> cur(t.texels, t.texels + 3 * (size_t)t.width * t.height), next;
+    uint32_t w = t.width, h = t.height;
+    while (true) {
+      out->tlev.push_back(PtTexLevel{w, h, (uint64_t)out->texels.size()});
+      for (size_t k = 0; k < (size_t)w * h; k++)
+        out->texels.push_back(make_float4(cur[3 * k], cur[3 * k + 1], cur[3 * k + 2], 0.f));
+      if (w == 1 && h == 1) break;
+      const uint32_t nw = std::max(w / 2, 1u), nh = std::max(h / 2, 1u);
+      next.assign(3 * (size_t)nw * nh, 0.f);
+      for (uint32_t y = 0; y < nh; y++) {
+        const uint32_t y0 = std::min(2 * y, h - 1), y1 = std::min(2 * y + 1, h - 1);
+        for (uint32_t x = 0; x < nw; x++) {
+          const uint32_t x0 = std::min(2 * x, w - 1), x1 = std::min(2 * x + 1, w - 1);
+          for (int c = 0; c < 3; c++) {
+            const float sum = cur[3 * ((size_t)y0 * w + x0) + c] + cur[3 * ((size_t)y0 * w + x1) + c] +
+                              cur[3 * ((size_t)y1 * w + x0) + c] + cur[3 * ((size_t)y1 * w + x1) + c];
+            next[3 * ((size_t)y * nw + x) + c] = sum * 0.25f;
+          }
+        }
+      }
+      cur.swap(next);
+      w = nw;
+      h = nh;
+    }
+    g.nlevels = (uint32_t)out->tlev.size() - g.first_level;
+    out->texs.push_back(g);
+  }
+  const int32_t ntex = (int32_t)d.num_textures;
+  for (uint32_t i = 0; i < d.num_patterns; i++) {
+    const rrhip_pattern_desc& p = d.patterns[i];
+    if (p.kind > RRHIP_PATTERN_TRIPLANAR) return *err = "unknown pattern kind", false;
+    if (p.wrap > RRHIP_WRAP_BLACK) return *err = "unknown texture wrap mode", false;
+    if (p.texture < -1 || p.texture >= ntex) return *err = "pattern texture out of range", false;
+    if (p.num_stops && (!p.positions || !p.colors)) return *err = "ramp stops need positions and colors", false;
+    PtPattern g{};
+    g.kind = p.kind;
+    g.wrap = p.wrap;
+    g.texture = p.texture;
+    g.udim = -1;
+    if (p.udim) {
+      g.udim = (int32_t)out->udim.size();
+      for (int k = 0; k < 100; k++) {
+        if (p.udim[k] < -1 || p.udim[k] >= ntex) return *err = "UDIM tile texture out of range", false;
+        out->udim.push_back(p.udim[k]);
+      }
+    }
+    for (int k = 0; k < 2; k++) g.scale[k] = (float)p.scale[k];
+    g.frequency = (float)p.frequency;
+    g.gain = (float)p.gain;
+    g.lacunarity = (float)p.lacunarity;
+    g.gamma = (float)p.gamma;
+    g.saturation = (float)p.saturation;
+    g.octaves = p.octaves;
+    put(g.color_a, d3(p.color_a));
+    put(g.color_b, d3(p.color_b));
+    put(g.cc_gain, d3(p.cc_gain));
+    put(g.cc_offset, d3(p.cc_offset));
+    g.use_t = p.use_t ? 1u : 0u;
+    g.ramp = (uint32_t)out->ramp.size();
+    g.nstops = p.num_stops;
+    for (uint32_t k = 0; k < p.num_stops; k++)
+      out->ramp.push_back(make_float4((float)p.positions[k], (float)p.colors[3 * k], (float)p.colors[3 * k + 1],
+                                      (float)p.colors[3 * k + 2]));
+    out->pats.push_back(g);
+  }
+  // Programs: post-order over the input tree with the value-stack depth.
+  const auto flatten = [&](uint32_t root, std::vector<PtPatOp>* prog, int* max_sp) {
+    int sp = 0;
+    const auto push_const = [&](const double c[3]) {
+      prog->push_back(PtPatOp{-1, {(float)c[0], (float)c[1], (float)c[2]}});
+      *max_sp = std::max(*max_sp, ++sp);
+    };
+    const double magenta[3] = {1.0, 0.0, 1.0};
+    std::function<void(int32_t, int)> emit = [&](int32_t idx, int depth) {
+      if (prog->size() > kPatProgMax) return;
+      if (depth > 16 || idx < 0 || idx >= (int32_t)d.num_patterns) return push_const(magenta);
+      const rrhip_pattern_desc& p = d.patterns[idx];
+      const int nin = p.kind == RRHIP_PATTERN_MIX ? 3 : p.kind == RRHIP_PATTERN_COLOR_CORRECT ? 1 : 0;
+      for (int k = 0; k < nin; k++) {
+        if (p.input[k] >= 0)
+          emit(p.input[k], depth + 1);
+        else
+          push_const(p.constant[k]);
+      }
+      prog->push_back(PtPatOp{idx, {0.f, 0.f, 0.f}});
+      sp += 1 - nin;
+      *max_sp = std::max(*max_sp, sp);
+    };
+    emit((int32_t)root, 0);
+  };
+  std::vector<std::vector<uint32_t>> per_mat(d.num_materials);
+  for (uint32_t i = 0; i < d.num_bindings; i++) {
+    const rrhip_binding_desc& b = d.bindings[i];
+    if (b.material >= d.num_materials) return *err = "binding material out of range", false;
+    if (b.field > RRHIP_BIND_PRESENCE) return *err = "unknown binding field", false;
+    if (b.pattern >= d.num_patterns) return *err = "binding pattern out of range", false;
+    if (b.field == RRHIP_BIND_PRESENCE)
+      return *err = "UNSUPPORTED: pattern-bound presence (cutouts follow the material's constant presence)", false;
+    per_mat[b.material].push_back(i);
+  }
+  for (uint32_t m = 0; m < d.num_materials; m++) {
+    if (per_mat[m].empty()) continue;
+    if (per_mat[m].size() > 15) return *err = "UNSUPPORTED: more than 15 pattern bindings on one material", false;
+    out->materials[m].bind = ((uint32_t)out->binds.size() << 4) | (uint32_t)per_mat[m].size();
+    for (uint32_t i : per_mat[m]) {
+      const rrhip_binding_desc& b = d.bindings[i];
+      std::vector<PtPatOp> prog;
+      int max_sp = 0;
+      flatten(b.pattern, &prog, &max_sp);
+      if (prog.size() > kPatProgMax || max_sp > kPatStack)
+        return *err = "UNSUPPORTED: pattern graph too large for the device evaluator (" +
+                      std::to_string(kPatProgMax) + " nodes, " + std::to_string(kPatStack) + " pending inputs)",
+               false;
+      out->binds.push_back(PtBinding{b.field, (uint32_t)out->prog.size(), (uint32_t)prog.size(), 0});
+      out->prog.insert(out->prog.end(), prog.begin(), prog.end());
+    }
+  }
+  if (out->binds.size() >= (1u << 27)) return *err = "too many pattern bindings", false;
+  for (uint32_t m = 0; m < d.num_materials; m++) out->raw_pbr.push_back(raw_pbr<float>(d.materials[m]));
+  out->pixel_spread = (float)(std::tan(d.camera.fov_deg * kPi / 180.0 / 2.0) * 2.0 / (double)d.camera.height);
+  out->patterns = 1;
   return true;
 }
 
@@ -714,6 +843,8 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
       g2.fwd[k] = (float)m[k];
     }
     out->quadrics.push_back(g2);
+    const double sx = q.p2[0] - q.p1[0], sy = q.p2[1] - q.p1[1], sz = q.p2[2] - q.p1[2];
+    out->quad_st.push_back(make_float2((float)approx_scale(m), (float)std::max(std::sqrt(sx * sx + sy * sy + sz * sz), 1e-12)));
   }
   put(out->background, d3(d.background));
   // Media (medium.rs:28-65)
@@ -860,7 +991,7 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
     cam.motion_inv[k] = (float)c.motion_inv[k];
   }
   if (cam.has_motion_inv) out->motion = 1;
-  return true;
+  return build_patterns(d, out, err);
 }
 
 }  // namespace rrhip

@@ -44,6 +44,22 @@ struct PtSceneHost {
   // (height + 1) and per-row conditional CDFs ((width + 1) x height) of
   // sin-weighted luminance, in f64, and their total.
   std::vector<PtQuadric> quadrics;
+  std::vector<float2> quad_st;    // per quadric: approximate scale, hyperboloid sweep length
+  std::vector<float> inst_scale;  // per instance (TLAS leaf order): approximate scale
+  // Pattern-bound materials (PtSceneView's pattern fields); empty without.
+  uint32_t patterns = 0;
+  std::vector<RawPbr<float>> raw_pbr;
+  std::vector<PtBinding> binds;
+  std::vector<PtPatOp> prog;
+  std::vector<PtPattern> pats;
+  std::vector<float4> ramp;
+  std::vector<int32_t> udim;
+  std::vector<PtTexture> texs;
+  std::vector<PtTexLevel> tlev;
+  std::vector<float4> texels;
+  std::vector<uint32_t> mesh_st;  // per geometry: first st pair, ~0u none
+  std::vector<float> st;
+  float pixel_spread = 0.f;
   uint32_t env_w = 0, env_h = 0;
   std::vector<float> env_rgb;
   std::vector<double> env_marg, env_cond;

@@ -156,6 +156,11 @@ hipError_t alloc_paths(rrhip_session* s, uint64_t n) {
     if (e == hipSuccess) e = alloc_in(m, &P.srng, n);
   }
   if (e == hipSuccess && s->scene->pt.motion) e = alloc_in(m, &P.stime, n);
+  if (e == hipSuccess && s->scene->pt.patterns) {
+    e = alloc_in(m, &P.hit_st, n);
+    if (e == hipSuccess) e = alloc_in(m, &P.hit_gs, n);
+    if (e == hipSuccess) e = alloc_in(m, &P.hit_gt, n);
+  }
   if (e == hipSuccess && s->film) {
     e = alloc_in(m, &P.ld, n);
     if (e == hipSuccess) e = alloc_in(m, &P.aux0, n);
@@ -795,6 +800,8 @@ extern "C" int rrhip_session_set_nrc(rrhip_session* s, rrhip_nrc* nrc) {
                 "rrhip_session_set_nrc: the radiance cache does not model participating media or cutouts");
   if (s->film) return fail(RRHIP_ERR_UNSUPPORTED, "rrhip_session_set_nrc: not in film mode");
   if (sc->pt.hair) return fail(RRHIP_ERR_UNSUPPORTED, "rrhip_session_set_nrc: the radiance cache does not model hair fibers");
+  if (sc->pt.patterns)
+    return fail(RRHIP_ERR_UNSUPPORTED, "rrhip_session_set_nrc: the radiance cache does not model pattern-bound materials");
   const uint64_t ntrain = nrc->cfg.train_paths;
   const uint64_t cap = s->slots + ntrain;
   if (cap >= (1ull << 31)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_session_set_nrc: path slots exceed 2^31");

@@ -354,7 +354,7 @@ __device__ __forceinline__ void curve_instance_hit(const PtSceneView& S, const M
 // Geometry classes of a trace instantiation: loose quadrics (their
 // solvers) and curve sets (the segment test). GEO = 0 is the lean
 // triangle / sphere traversal.
-constexpr int kGeoQuadrics = 1, kGeoCurves = 2;
+constexpr int kGeoQuadrics = 1, kGeoCurves = 2, kGeoSt = 4;
 
 // Closest hit over loose objects then instances (GEO: the geometry
 // classes above). SPAWN: also the winning hit's point and push (Hit::p,
@@ -365,10 +365,15 @@ __device__ __forceinline__ bool scene_closest(const PtSceneView& S, V3 o, V3 d, 
   constexpr bool QUAD = (GEO & kGeoQuadrics) != 0, CURVES = (GEO & kGeoCurves) != 0;
   bool found = closest_hit<QUAD>(S, o, d, best);
   best.tan = v3(0.f, 0.f, 0.f);
+  best.inst = 0xFFFFFFFFu;
   MeshHit mh;
   if (trace_instances<STRIDE, false, MOTION, COUNT, CURVES>(S, o, d, time, found ? best.t : INFINITY, stk, mh,
                                                             tc)) {
     best.t = mh.t;
+    best.inst = mh.inst;
+    best.slot = mh.slot;
+    best.bu = mh.u;
+    best.bv = mh.v;
     if (CURVES && S.meshes[S.insts[mh.inst].mesh].curves) {
       curve_instance_hit<MOTION, SPAWN>(S, mh, o, d, time, best);
       return true;

@@ -38,7 +38,7 @@ struct alignas(16) PtMaterial {
   float presence;       // probability that a ray stops at the surface (pbr.rs:39)
   uint32_t hull;        // interior medium, no lobes, no emission: an invisible volume boundary
   uint32_t id;          // object id of the id AOV (Material::id), 0 = none
-  uint32_t pad;
+  uint32_t bind;        // pattern-bound fields: first binding << 4 | count (0: none)
   float film_albedo[3]; // albedo AOV: diffuse + primary specular F0 + refraction, each <= 1
   float pad2;
   // Hair fibers (hair.rs HairParams): curve hits scatter with the fiber
@@ -52,6 +52,64 @@ struct alignas(16) PtMaterial {
   float hair_sigma_a[3];
   float pad4;
 };
+
+// The PbrParams fields the lobes read (render.rs src/scene/pbr.rs:11-45)
+// and the material's own emission: f64 in the host build, f32 per material
+// on the device for pattern-bound materials.
+template <class T>
+struct RawPbr {
+  T diffuse_gain, diffuse_color[3], diffuse_roughness;
+  T specular_face_color[3], specular_edge_color[3], specular_roughness, specular_ior;
+  T clearcoat_gain, clearcoat_roughness;
+  T fuzz_gain, fuzz_color[3];
+  T glass_gain, glass_ior, glass_roughness, refraction_color[3];
+  T glow[3], emission[3];
+  T pad;
+};
+static_assert(sizeof(RawPbr<float>) == 128, "RawPbr<float> is 128 bytes");
+
+// Pattern graph (render.rs src/texture/pattern.rs:42-68), one record per
+// node: kind (RRHIP_PATTERN_*) and the parameters of every kind.
+struct alignas(16) PtPattern {
+  uint32_t kind, wrap;
+  int32_t texture;    // texture / triplanar: texture index, -1 = missing
+  int32_t udim;       // first of the node's 100 UDIM entries in `udim`, -1 = none
+  float scale[2];     // texture scaleS / scaleT, checker sScale / tScale
+  float frequency, gain;
+  float lacunarity, gamma, saturation;
+  uint32_t octaves;
+  float color_a[3];
+  uint32_t use_t;     // ramp over t
+  float color_b[3];
+  uint32_t ramp;      // first ramp stop
+  float cc_gain[3];
+  uint32_t nstops;
+  float cc_offset[3];
+  float pad;
+};
+// One step of a binding's evaluation program (its node tree in post-order,
+// every input expanded in place): node >= 0 evaluates that node (mix and
+// colour correct take their inputs from the value stack), -1 pushes c.
+struct alignas(16) PtPatOp {
+  int32_t node;
+  float c[3];
+};
+struct PtBinding {
+  uint32_t field;       // RRHIP_BIND_*
+  uint32_t first, len;  // program in `prog`
+  uint32_t pad;
+};
+// A texture's mip chain: its levels in `tlev`; texels rgb (w unused) from
+// the top row, level after level.
+struct PtTexture {
+  uint32_t first_level, nlevels, w, h;
+};
+struct PtTexLevel {
+  uint32_t w, h;
+  uint64_t offset;  // first texel in `texels`
+};
+constexpr int kPatStack = 8;     // value-stack depth of a binding program
+constexpr uint32_t kPatProgMax = 256;  // ops of one binding program
 
 enum : uint32_t { kLightPoint = 0, kLightDistant = 1, kLightRect = 2, kLightSphere = 3, kLightDisk = 4, kLightDome = 5 };
 
@@ -222,6 +280,27 @@ struct PtSceneView {
   const CurveRec* csegs;
   uint32_t curves;
   uint32_t hair;                // some material is a hair fiber
+  // Pattern-bound materials (all null / 0 without): raw PbrParams per
+  // material, bindings, their programs, nodes, ramp stops (position, rgb),
+  // UDIM tables, textures; texture coordinates of hits: per-mesh first st
+  // pair (~0u: none) and the pairs, the scale of each instance (TLAS leaf
+  // order) and of each quadric with a hyperboloid's sweep length; the
+  // camera's pixel angle that starts every ray cone.
+  const RawPbr<float>* raw_pbr;
+  const PtBinding* binds;
+  const PtPatOp* prog;
+  const PtPattern* pats;
+  const float4* ramp;
+  const int32_t* udim;
+  const PtTexture* texs;
+  const PtTexLevel* tlev;
+  const float4* texels;
+  const uint32_t* mesh_st;
+  const float* st;
+  const float* inst_scale;
+  const float2* quad_st;
+  float pixel_spread;
+  uint32_t patterns;
   uint32_t num_tris, num_spheres, num_lights, num_lnodes, num_inf;
   int32_t dome;
   double p_infinite, inf_total;
@@ -232,10 +311,11 @@ struct PtSceneView {
 // Wavefront path state, structure of arrays over path slots.
 struct PathBuffers {
   float4* o;        // origin.xyz, prev_pdf
-  float4* d;        // direction.xyz, primary footprint a0 (radiance cache)
+  float4* d;        // direction.xyz, primary footprint a0 (radiance cache) or, in pattern scenes, the
+                    //   ray cone's spread beyond one pixel
   float4* beta;     // throughput.xyz, unused
   float4* L;        // radiance.xyz, unused
-  float4* prev;     // previous vertex xyz, area-spread sum (radiance cache)
+  float4* prev;     // previous vertex xyz, area-spread sum (radiance cache) or the ray cone's width
   uint4* rng;       // PCG32 state lo/hi, inc lo/hi
   uint32_t* flags;  // depth (bits 0-7), from_camera (bit 8), training path (9), suffix (10)
   float4* hit_n;    // hit normal.xyz, t (t < 0: miss)
@@ -243,6 +323,9 @@ struct PathBuffers {
   float4* hit_front; // error-bounded origins: the hit rebuilt from the surface and pushed outside its
   float4* hit_back;  //   rounding box on the ray's side (shading point), and on the far side; the two
                      //   w lanes hold a curve hit's world tangent (octahedral), 4 on other hits
+  float4* hit_st;    // pattern scenes: the hit's st, st density; world gradients of s and t
+  float4* hit_gs;    //   (null in other scenes)
+  float4* hit_gt;
   uint32_t* q0;
   uint32_t* q1;
   uint32_t* sq;     // shadow queue: slot ids

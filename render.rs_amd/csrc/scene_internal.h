@@ -36,7 +36,7 @@ struct rrhip_scene {
   // Path-tracing scene (camera given): device arrays behind `pt`.
   bool has_pt = false;
   rrhip::PtSceneView pt{};
-  void* pt_mem[22] = {};
+  void* pt_mem[36] = {};
   std::vector<void*> media_mem;  // density grids
   float pt_lo[3] = {0, 0, 0}, pt_hi[3] = {0, 0, 0};  // bounds of all geometry
   bool pt_eq = false;  // point lights in an atmosphere: shadow records carry a second ray

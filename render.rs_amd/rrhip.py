@@ -66,7 +66,7 @@ class BuildParams(ctypes.Structure):
 class MeshDesc(ctypes.Structure):
     _fields_ = [("positions", ctypes.c_void_p), ("num_vertices", ctypes.c_uint64),
                 ("indices", ctypes.c_void_p), ("num_triangles", ctypes.c_uint64),
-                ("normals", ctypes.c_void_p)]
+                ("normals", ctypes.c_void_p), ("st", ctypes.c_void_p)]
 
 
 D3 = ctypes.c_double * 3
@@ -148,6 +148,69 @@ class QuadricDesc(ctypes.Structure):
                 ("object_to_world", ctypes.c_double * 12)]
 
 
+class TextureDesc(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("texels", ctypes.POINTER(ctypes.c_float))]
+
+
+# rrhip_pattern_desc kinds, wrap modes, bound fields
+(PATTERN_TEXTURE, PATTERN_CHECKER, PATTERN_FRACTAL, PATTERN_MIX, PATTERN_COLOR_CORRECT, PATTERN_RAMP,
+ PATTERN_TRIPLANAR) = range(7)
+WRAP_PERIODIC, WRAP_CLAMP, WRAP_BLACK = range(3)
+BIND_FIELDS = ("diffuseColor", "diffuseGain", "diffuseRoughness", "specularFaceColor", "specularEdgeColor",
+               "specularRoughness", "clearcoatGain", "fuzzGain", "fuzzColor", "glassRefractionGain",
+               "glassRoughness", "refractionColor", "glowColor", "presence")
+BIND_PRESENCE = 13
+
+
+class PatternDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("wrap", ctypes.c_uint32), ("texture", ctypes.c_int32),
+                ("use_t", ctypes.c_uint32), ("udim", ctypes.POINTER(ctypes.c_int32)),
+                ("input", ctypes.c_int32 * 3), ("octaves", ctypes.c_uint32),
+                ("constant", D3 * 3), ("color_a", D3), ("color_b", D3), ("cc_gain", D3), ("cc_offset", D3),
+                ("scale", ctypes.c_double * 2), ("frequency", ctypes.c_double), ("gain", ctypes.c_double),
+                ("lacunarity", ctypes.c_double), ("gamma", ctypes.c_double), ("saturation", ctypes.c_double),
+                ("num_stops", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("positions", ctypes.POINTER(ctypes.c_double)), ("colors", ctypes.POINTER(ctypes.c_double))]
+
+    @classmethod
+    def make(cls, kind, **kw):
+        """A node with render.rs's defaults (pattern.rs, rib/mod.rs Pattern
+        parsing): unit scales, checker white/black, fBm of 4 octaves, gain
+        0.5, lacunarity 2, identity colour correction, inputs on their
+        constants."""
+        p = cls()
+        p.kind = kind
+        p.texture = -1
+        p.input[:] = [-1, -1, -1]
+        p.octaves = 4
+        p.scale[:] = [1.0, 1.0]
+        p.color_a[:] = [1.0, 1.0, 1.0]
+        p.color_b[:] = [0.0, 0.0, 0.0]
+        p.cc_gain[:] = [1.0, 1.0, 1.0]
+        p.frequency = 1.0
+        p.gain = 0.5
+        p.lacunarity = 2.0
+        p.gamma = 1.0
+        p.saturation = 1.0
+        for k, v in kw.items():
+            if k == "input":
+                p.input[:] = list(v)
+            elif k == "constant":
+                for i, c in enumerate(v):
+                    p.constant[i][:] = list(c)
+            elif k in ("color_a", "color_b", "cc_gain", "cc_offset", "scale"):
+                getattr(p, k)[:] = list(v)
+            else:
+                setattr(p, k, v)
+        return p
+
+
+class BindingDesc(ctypes.Structure):
+    _fields_ = [("material", ctypes.c_uint32), ("field", ctypes.c_uint32), ("pattern", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
 class SceneDesc(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("num_meshes", ctypes.c_uint32),
                 ("meshes", ctypes.POINTER(MeshDesc)), ("build", BuildParams),
@@ -167,7 +230,11 @@ class SceneDesc(ctypes.Structure):
                 ("num_quadrics", ctypes.c_uint32), ("reserved_q", ctypes.c_uint32),
                 ("quadrics", ctypes.POINTER(QuadricDesc)),
                 ("num_curve_sets", ctypes.c_uint32), ("reserved_c", ctypes.c_uint32),
-                ("curve_sets", ctypes.POINTER(CurvesDesc))]
+                ("curve_sets", ctypes.POINTER(CurvesDesc)),
+                ("num_textures", ctypes.c_uint32), ("num_patterns", ctypes.c_uint32),
+                ("textures", ctypes.POINTER(TextureDesc)), ("patterns", ctypes.POINTER(PatternDesc)),
+                ("num_bindings", ctypes.c_uint32), ("reserved_p", ctypes.c_uint32),
+                ("bindings", ctypes.POINTER(BindingDesc))]
 
 
 class RenderParams(ctypes.Structure):
@@ -415,7 +482,7 @@ def _info_dict(i):
 def _mesh_desc(positions, indices, bins, max_leaf):
     pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
     idx = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
-    mesh = MeshDesc(_ptr(pos).value, pos.shape[0], _ptr(idx).value, idx.shape[0], None)
+    mesh = MeshDesc(_ptr(pos).value, pos.shape[0], _ptr(idx).value, idx.shape[0], None, None)
     desc = SceneDesc()
     desc.struct_size = ctypes.sizeof(SceneDesc)
     desc.num_meshes = 1

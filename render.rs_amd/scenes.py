@@ -163,6 +163,9 @@ class Scene:
         self.materials, self.lights, self.triangles, self.spheres = [], [], [], []
         self.meshes, self.instances = [], []
         self.curve_sets = []
+        self.textures = []
+        self.patterns = []
+        self.bindings = []
         self.media = []
         self.atmosphere = -1
         self._keep = []
@@ -193,14 +196,54 @@ class Scene:
         s.material = mat
         self.spheres.append(s)
 
-    def add_mesh(self, positions, indices, normals=None):
+    def add_mesh(self, positions, indices, normals=None, st=None):
+        """A mesh for instances; st: optional per-vertex texture
+        coordinates (n x 2)."""
         pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
         idx = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
         nrm = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
-        self._keep += [pos, idx, nrm]
+        stv = None if st is None else np.ascontiguousarray(st, np.float32).reshape(-1, 2)
+        self._keep += [pos, idx, nrm, stv]
         self.meshes.append(rrhip.MeshDesc(pos.ctypes.data, pos.shape[0], idx.ctypes.data, idx.shape[0],
-                                          None if nrm is None else nrm.ctypes.data))
+                                          None if nrm is None else nrm.ctypes.data,
+                                          None if stv is None else stv.ctypes.data))
         return len(self.meshes) - 1
+
+    def add_texture(self, rgb):
+        """A texture from level 0 (height x width x 3 linear rgb, rows from
+        the top); returns its index."""
+        t = np.ascontiguousarray(rgb, np.float32)
+        assert t.ndim == 3 and t.shape[2] == 3
+        self._keep.append(t)
+        self.textures.append(rrhip.TextureDesc(t.shape[1], t.shape[0], t.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return len(self.textures) - 1
+
+    def add_pattern(self, kind, udim=None, stops=None, **kw):
+        """A pattern node (rrhip.PatternDesc.make's keywords); udim: a dict
+        tile -> texture index; stops: ramp [(position, (r, g, b)), ...].
+        Returns its index."""
+        p = rrhip.PatternDesc.make(kind, **kw)
+        if udim is not None:
+            u = np.full(100, -1, np.int32)
+            for tile, tex in udim.items():
+                u[tile - 1001] = tex
+            self._keep.append(u)
+            p.udim = u.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        if stops:
+            pos = np.ascontiguousarray([a for a, _ in stops], np.float64)
+            col = np.ascontiguousarray([c for _, c in stops], np.float64).reshape(-1, 3)
+            self._keep += [pos, col]
+            p.num_stops = len(stops)
+            p.positions = pos.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            p.colors = col.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        self.patterns.append(p)
+        return len(self.patterns) - 1
+
+    def bind(self, mat, field, pattern):
+        """Drive material `mat`'s field (a name of rrhip.BIND_FIELDS or its
+        index) by pattern node `pattern`."""
+        f = rrhip.BIND_FIELDS.index(field) if isinstance(field, str) else int(field)
+        self.bindings.append(rrhip.BindingDesc(mat, f, pattern, 0))
 
     def add_curves(self, p0, p1, v0=None, v1=None):
         """A curve set (rrhip_curves_desc): per segment the start and end
@@ -280,6 +323,12 @@ class Scene:
         d.quadrics = self._arr(rrhip.QuadricDesc, qs)
         d.num_curve_sets = len(self.curve_sets)
         d.curve_sets = self._arr(rrhip.CurvesDesc, self.curve_sets)
+        d.num_textures = len(self.textures)
+        d.textures = self._arr(rrhip.TextureDesc, self.textures)
+        d.num_patterns = len(self.patterns)
+        d.patterns = self._arr(rrhip.PatternDesc, self.patterns)
+        d.num_bindings = len(self.bindings)
+        d.bindings = self._arr(rrhip.BindingDesc, self.bindings)
         env = getattr(self, "env", None)
         if env is not None:
             env = np.ascontiguousarray(env, np.float32)
@@ -718,4 +767,96 @@ def curve_zoo(width=48, height=48, hair_fiber=True, dome=0.5):
     s.add_instance(strand, fur, _place((0, 0, 0)), curves=True)
     s.add_instance(points, blue, _place((0, 0, 0)), curves=True)
     s.add_instance(capsule, red, _place((0.3, -1.6, 1.0), -30, 0.6), curves=True)
+    return s
+
+
+def smooth_texture(w, h, seed=0, cells=6):
+    """A smooth random rgb texture (bilinear upsampling of a coarse random
+    grid, periodic), values in [0.05, 0.95]."""
+    rng = np.random.default_rng(seed)
+    g = rng.uniform(0.05, 0.95, (cells, cells, 3))
+    ys = (np.arange(h) + 0.5) / h * cells - 0.5
+    xs = (np.arange(w) + 0.5) / w * cells - 0.5
+    y0, x0 = np.floor(ys).astype(int), np.floor(xs).astype(int)
+    ay, ax = (ys - y0)[:, None, None], (xs - x0)[None, :, None]
+    a = lambda yy, xx: g[yy[:, None] % cells, xx[None, :] % cells]
+    img = (a(y0, x0) * (1 - ax) * (1 - ay) + a(y0, x0 + 1) * ax * (1 - ay) + a(y0 + 1, x0) * (1 - ax) * ay +
+           a(y0 + 1, x0 + 1) * ax * ay)
+    return img.astype(np.float32)
+
+
+def stripes_texture(w, h, period=8):
+    """Black and white columns `period` texels wide (cache.rs's EWA test
+    texture)."""
+    v = ((np.arange(w) // period) % 2 == 0).astype(np.float32)
+    return np.repeat(np.broadcast_to(v[None, :], (h, w))[..., None], 3, axis=2).copy()
+
+
+def pattern_zoo(width=64, height=64, lights=True):
+    """Every pattern kind on every kind of texture coordinates: a ground
+    quad (mesh st, tiled 4 x 4) with a smooth texture at grazing angles
+    (EWA), a RenderMan sphere with a checker mixed over a ramp, a loose
+    sphere with a triplanar stripe texture, a mesh quad spanning UDIM tiles
+    1001 and 1002 (colour-corrected), a metal quad whose specular roughness
+    follows a ramp over t, a box whose diffuse roughness and colour follow
+    a fractal, a curve set coloured by a ramp over v; a rect light and a
+    distant light. Object ids 1..7 in that order."""
+    s = Scene(width, height, 45.0, eye=(0, 1.2, -7), look_at=(0, 0.3, 0))
+    if lights:
+        s.add_light(light(rrhip.LIGHT_RECT, intensity=5.0, p=(-1.5, 4, -2), e1=(3, 0, 0), e2=(0, 0, 3)))
+        s.add_light(light(rrhip.LIGHT_DISTANT, intensity=1.5, p=(0.3, -1, 0.4)))
+    tex = s.add_texture(smooth_texture(64, 64, seed=1))
+    stripes = s.add_texture(stripes_texture(64, 64, 4))
+    tile_a = s.add_texture(smooth_texture(32, 32, seed=2, cells=4))
+    tile_b = s.add_texture(smooth_texture(16, 32, seed=3, cells=3))
+    # ground: mesh with st tiled 4x4, texture with scale 1
+    ground = s.add_material(matte((0.5, 0.5, 0.5), object_id=1))
+    gm = s.add_mesh([[-8, -1, -8], [8, -1, -8], [8, -1, 8], [-8, -1, 8]], [0, 1, 2, 0, 2, 3],
+                    st=[[0, 0], [4, 0], [4, 4], [0, 4]])
+    s.add_instance(gm, ground, np.eye(4)[:3])
+    s.bind(ground, "diffuseColor", s.add_pattern(rrhip.PATTERN_TEXTURE, texture=tex))
+    # RenderMan sphere: checker mixed with a ramp over s by a constant 0.3
+    ball = s.add_material(material(diffuse_color=(0.5, 0.5, 0.5), specular_roughness=0.3, object_id=2))
+    s.add_quadric(rrhip.QUADRIC_SPHERE, ball, _place((-1.8, 0.0, 0.5)), radius=1.0, zmin=-1.0, zmax=1.0)
+    chk = s.add_pattern(rrhip.PATTERN_CHECKER, color_a=(0.9, 0.8, 0.2), color_b=(0.1, 0.2, 0.7), scale=(12, 6))
+    ramp = s.add_pattern(rrhip.PATTERN_RAMP, stops=[(0.0, (0.9, 0.1, 0.1)), (0.5, (0.1, 0.9, 0.1)),
+                                                    (1.0, (0.1, 0.1, 0.9))])
+    s.bind(ball, "diffuseColor", s.add_pattern(rrhip.PATTERN_MIX, input=(chk, ramp, -1),
+                                               constant=[(0, 0, 0), (0, 0, 0), (0.3, 0, 0)]))
+    # loose sphere: triplanar stripes
+    tri = s.add_material(matte((0.5, 0.5, 0.5), object_id=3))
+    s.add_sphere((0.2, -0.3, 1.5), 0.7, tri)
+    s.bind(tri, "diffuseColor", s.add_pattern(rrhip.PATTERN_TRIPLANAR, texture=stripes, frequency=0.8))
+    # UDIM quad over tiles 1001 and 1002, colour-corrected
+    ud = s.add_material(matte((0.5, 0.5, 0.5), object_id=4))
+    um = s.add_mesh([[0, 0, 0], [2, 0, 0], [2, 1, 0], [0, 1, 0]], [0, 1, 2, 0, 2, 3],
+                    st=[[0, 0], [2, 0], [2, 1], [0, 1]])
+    s.add_instance(um, ud, _place((1.0, -0.2, 2.5), 0, 1.2))
+    un = s.add_pattern(rrhip.PATTERN_TEXTURE, udim={1001: tile_a, 1002: tile_b}, wrap=rrhip.WRAP_CLAMP)
+    s.bind(ud, "diffuseColor", s.add_pattern(rrhip.PATTERN_COLOR_CORRECT, input=(un, -1, -1), cc_gain=(1.2, 1.0, 0.8),
+                                             cc_offset=(0.02, 0.0, 0.05), gamma=1.4, saturation=1.3))
+    # metal quad: specular roughness from a ramp over t
+    met = s.add_material(material(diffuse_gain=0.0, specular_face_color=(0.9, 0.7, 0.5), specular_roughness=0.3,
+                                  object_id=5))
+    mm = s.add_mesh([[0, 0, 0], [1.6, 0, 0], [1.6, 1.6, 0], [0, 1.6, 0]], [0, 1, 2, 0, 2, 3],
+                    st=[[0, 0], [1, 0], [1, 1], [0, 1]])
+    s.add_instance(mm, met, _place((2.2, -0.9, 1.0), -25, 1.0))
+    s.bind(met, "specularRoughness", s.add_pattern(rrhip.PATTERN_RAMP, use_t=1, stops=[(0.0, (0.05, 0, 0)),
+                                                                                         (1.0, (0.6, 0, 0))]))
+    # box (loose-triangle-free mesh, no st): fractal roughness and colour
+    bx = s.add_material(matte((0.5, 0.5, 0.5), object_id=6))
+    frac = s.add_pattern(rrhip.PATTERN_FRACTAL, frequency=3.0, octaves=4)
+    s.bind(bx, "diffuseRoughness", frac)
+    s.bind(bx, "diffuseColor", s.add_pattern(rrhip.PATTERN_COLOR_CORRECT, input=(frac, -1, -1), cc_gain=(0.9, 0.6, 0.3)))
+    bpos = np.array([[x, y, z] for x in (0, 1) for y in (0, 1) for z in (0, 1)], np.float32) - 0.5
+    bidx = [0, 1, 3, 0, 3, 2, 4, 6, 7, 4, 7, 5, 0, 4, 5, 0, 5, 1, 2, 3, 7, 2, 7, 6, 0, 2, 6, 0, 6, 4, 1, 5, 7, 1, 7, 3]
+    bm = s.add_mesh(bpos, bidx)
+    s.add_instance(bm, bx, _place((-0.3, -0.55, -0.6), 30, 0.9))
+    # curves: ramp over v
+    cv = s.add_material(matte((0.5, 0.5, 0.5), object_id=7))
+    p0, p1, v0, v1 = dice_curve([[-3.2, -1, -1], [-2.5, 1.5, -1.2], [-1.0, 0.5, -1.5], [-0.4, 1.8, -1.0]],
+                                (BEZIER_BASIS, 3), 0.12, 0.06, 8)
+    s.add_instance(s.add_curves(p0, p1, v0, v1), cv, _place((0, 0, 0)), curves=True)
+    s.bind(cv, "diffuseColor", s.add_pattern(rrhip.PATTERN_RAMP, use_t=1, stops=[(0.0, (0.9, 0.9, 0.1)),
+                                                                                  (1.0, (0.1, 0.6, 0.9))]))
     return s
