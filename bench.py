@@ -11,7 +11,8 @@ timed region is K back-to-back closest-hit launches.
 
 Prints one JSON line (value = whole-job Mrays/s at N=1). The line also
 carries the box-room path-tracing throughput, the 64-spp 1080p instanced
-field, a 1080p fur ball of 300,000 hair segments, the batched-jobs
+field, a 1080p fur ball of 300,000 hair segments, a 1080p textured
+scene (the pattern zoo, with and without its bindings), the batched-jobs
 comparison (32 rooms, half with clouds: one launch sequence, one
 persistent launch, 32 sequential sessions) and the time to relMSE 0.01 at
 1080p of the radiance cache vs plain wavefront path tracing.
@@ -308,6 +309,66 @@ def hair_curves(ctx, args, with_cpu):
                       f"scaled by pixels and samples to {frame_s:.1f} s per 16-spp 1080p frame",
         }
     ps.close()
+    return out
+
+
+def textured(ctx, args, with_cpu):
+    """Textures and the pattern graph at 1920x1080: the pattern zoo (every
+    node kind; mesh, quadric, sphere and curve texture coordinates; EWA on
+    a grazing ground; UDIM tiles), 16 spp, depth 8, and the same scene with
+    its bindings removed (constant materials) to price the patterns: both
+    frames run the same trace kernels, so the difference is st at the hits
+    and the pattern evaluation in the shade kernel."""
+    import rrhip
+    import scenes
+    W, H, SPP = 1920, 1080, 16
+    p = rrhip.RenderParams.make(max_depth=8, samples_per_pass=8)
+
+    def frame(s):
+        d = s.desc()
+        ps = rrhip.PtScene(ctx, d)
+        se = ps.session(p)
+        se.render(0, 8)  # warm-up pass
+        st0 = se.stats()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        se.render(8, SPP)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        st1 = se.stats()
+        se.close()
+        rays = st1["extension_rays"] + st1["shadow_rays"] - st0["extension_rays"] - st0["shadow_rays"]
+        return ps, d, dt, rays
+
+    ps, d, dt, rays = frame(scenes.pattern_zoo(W, H))
+    out = {
+        "workload": "pattern zoo 1920x1080 (textures with trilinear and EWA lookups, checker, fractal, mix, "
+                    "colour correct, ramp, triplanar, UDIM; 7 bound materials), 16 spp, depth 8, NEE + MIS + RR "
+                    "(wavefront, passes of 8 spp)",
+        "ms_per_frame": round(dt * 1e3, 1),
+        "mrays_per_s": round(rays / dt / 1e6, 1),
+        "rays_per_frame": int(rays),
+        "roofline": trace_roofline(profiled_frame(ps, p, 0, 8), d),
+    }
+    ps.close()
+    plain = scenes.pattern_zoo(W, H)
+    plain.bindings = []
+    ps2, _, dt2, rays2 = frame(plain)
+    ps2.close()
+    out["unbound"] = {"ms_per_frame": round(dt2 * 1e3, 1), "mrays_per_s": round(rays2 / dt2 / 1e6, 1),
+                      "note": "the same scene with every binding removed (constant materials)"}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import orc
+        threads = host_threads()
+        t1 = time.perf_counter()
+        _, crays = orc.render(scenes.pattern_zoo(W, H).desc(), p, 8, SPP, threads=threads)
+        cdt = max(time.perf_counter() - t1, 1e-6)
+        out["cpu_baseline"] = {
+            "value": round(crays / cdt / 1e6, 3), "unit": "Mrays/s", "frame_s": round(cdt, 2),
+            "cores": threads, "kind": "port",
+            "sample": f"oracle PT, the whole 16-spp 1080p frame ({cdt:.2f} s including the scene load)",
+        }
     return out
 
 
@@ -611,6 +672,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs comparison")
     ap.add_argument("--no-instanced", action="store_true", help="skip the 64-spp instanced-field frame")
     ap.add_argument("--no-hair", action="store_true", help="skip the fur-ball (curves and hair) frame")
+    ap.add_argument("--no-textured", action="store_true", help="skip the textured (pattern zoo) frame")
     ap.add_argument("--ref-spp", type=int, default=4096, help="spp of the relMSE reference render")
     ap.add_argument("--max-frames", type=int, default=400, help="frame cap of the time-to-relMSE runs")
     ap.add_argument("--mode", choices=("trace", "copy", "nrc-step"), default="trace",
@@ -739,6 +801,8 @@ def main():
         out["instanced_path_tracing"] = instanced_pt(ctx, args, not args.no_cpu_baseline)
     if not args.no_hair:
         out["hair_curves"] = hair_curves(ctx, args, not args.no_cpu_baseline)
+    if not args.no_textured:
+        out["textured"] = textured(ctx, args, not args.no_cpu_baseline)
     if not args.no_batch:
         out["batched_jobs"] = batched_jobs(ctx, args, not args.no_cpu_baseline)
     if not args.no_relmse:

@@ -1111,9 +1111,10 @@ void launch_trace_kernels(const PtSceneView& S, const PathBuffers& P, const uint
     launch_trace_counted<STACK, MOTION, GEO, false>(S, P, q, gt, extend, st);
 }
 
-// Scenes with pattern-bound materials take the full kernels whose
-// extension rays also record each hit's texture coordinates. Scenes with
-// loose quadrics take the full-stack kernels with the quadric
+// Scenes with pattern-bound materials take full-stack extension kernels
+// that also record each hit's texture coordinates, with the geometry
+// classes chosen as below (their shadow rays take the kernels below).
+// Scenes with loose quadrics take the full-stack kernels with the quadric
 // solvers (and the curve test, for scenes with both), scenes with curve
 // sets alone the full-stack kernels with the curve test only (the f64
 // quartic's registers stay out of them); one instantiation per motion
@@ -1122,11 +1123,20 @@ void launch_trace(const PtSceneView& S, const PathBuffers& P, const uint32_t* q,
                   hipStream_t st) {
   const bool small = S.stack_need <= 32;
   constexpr int kAll = kGeoQuadrics | kGeoCurves;
-  if (S.patterns && S.motion)  // hits also record their texture coordinates
-    launch_trace_kernels<kTraceStack, true, kAll | kGeoSt>(S, P, q, gt, extend, st);
-  else if (S.patterns)
-    launch_trace_kernels<kTraceStack, false, kAll | kGeoSt>(S, P, q, gt, extend, st);
-  else if (S.num_quadrics && S.motion)
+  if (S.patterns) {  // hits also record their texture coordinates
+    if (S.num_quadrics && S.motion)
+      launch_trace_kernels<kTraceStack, true, kAll | kGeoSt>(S, P, q, gt, extend, st);
+    else if (S.num_quadrics)
+      launch_trace_kernels<kTraceStack, false, kAll | kGeoSt>(S, P, q, gt, extend, st);
+    else if (S.curves && S.motion)
+      launch_trace_kernels<kTraceStack, true, kGeoCurves | kGeoSt>(S, P, q, gt, extend, st);
+    else if (S.curves)
+      launch_trace_kernels<kTraceStack, false, kGeoCurves | kGeoSt>(S, P, q, gt, extend, st);
+    else if (S.motion)
+      launch_trace_kernels<kTraceStack, true, kGeoSt>(S, P, q, gt, extend, st);
+    else
+      launch_trace_kernels<kTraceStack, false, kGeoSt>(S, P, q, gt, extend, st);
+  } else if (S.num_quadrics && S.motion)
     launch_trace_kernels<kTraceStack, true, kAll>(S, P, q, gt, extend, st);
   else if (S.num_quadrics)
     launch_trace_kernels<kTraceStack, false, kAll>(S, P, q, gt, extend, st);

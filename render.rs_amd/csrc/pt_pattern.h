@@ -252,18 +252,65 @@ __device__ __forceinline__ V3 tex_sample_ewa(const PtSceneView& S, uint32_t tex,
   const int x0 = tex_floor(px - urad), x1 = (int)ceilf(clampf(px + urad, -1e9f, 1e9f));
   const int y0 = tex_floor(py - vrad), y1 = (int)ceilf(clampf(py + vrad, -1e9f, 1e9f));
   if ((long long)(x1 - x0) * (long long)(y1 - y0) > 4096) return tex_sample(S, tex, s, t, minor, wrap);
+  // Row by row over the box, in cache.rs's order; each row visits only
+  // the span where A dx^2 + B dx dy + C dy^2 < 1 can hold (the roots of
+  // the row's quadratic, widened by a texel each side; A > 0), with the
+  // texel column wrapped incrementally. The r2 < 1 test still decides, so
+  // the sum takes the same texels in the same order as the whole box.
   V3 sum = v3(0.f, 0.f, 0.f);
   float wsum = 0.f;
+  const int w = (int)L.w, h = (int)L.h;
+  const float inv2a = 0.5f / A;
   for (int y = y0; y <= y1; y++) {
     const float dy = (float)y - py;
-    for (int x = x0; x <= x1; x++) {
+    const float b = B * dy, c = C * dy * dy - 1.f;
+    const float disc = b * b - 4.f * A * c;
+    if (disc < -1e-6f) continue;
+    const float sq = sqrtf(fmaxf(disc, 0.f));
+    const int xs = max(x0, tex_floor(px + (-b - sq) * inv2a) - 1);
+    const int xe = min(x1, tex_floor(px + (-b + sq) * inv2a) + 2);
+    if (xs > xe) continue;
+    int yy = y;
+    if (wrap == RRHIP_WRAP_PERIODIC) {
+      yy %= h;
+      if (yy < 0) yy += h;
+    } else if (wrap == RRHIP_WRAP_CLAMP) {
+      yy = min(max(yy, 0), h - 1);
+    } else if (yy < 0 || yy >= h) {
+      // black rows: their texels weigh in as zero
+      for (int x = xs; x <= xe; x++) {
+        const float dx = (float)x - px;
+        const float r2 = A * dx * dx + B * dx * dy + C * dy * dy;
+        if (r2 < 1.f) wsum += __expf(-2.f * r2);
+      }
+      continue;
+    }
+    const float4* row = S.texels + L.offset + (size_t)yy * L.w;
+    int xw = xs;
+    if (wrap == RRHIP_WRAP_PERIODIC) {
+      xw %= w;
+      if (xw < 0) xw += w;
+    }
+    for (int x = xs; x <= xe; x++) {
       const float dx = (float)x - px;
       const float r2 = A * dx * dx + B * dx * dy + C * dy * dy;
       if (r2 < 1.f) {
-        const float w = __expf(-2.f * r2);
-        sum = sum + tex_texel(S, L, x, y, wrap) * w;
-        wsum += w;
+        const float wt = __expf(-2.f * r2);
+        V3 tc = v3(0.f, 0.f, 0.f);
+        if (wrap == RRHIP_WRAP_PERIODIC) {
+          const float4 t4 = row[xw];
+          tc = v3(t4.x, t4.y, t4.z);
+        } else if (wrap == RRHIP_WRAP_CLAMP) {
+          const float4 t4 = row[min(max(x, 0), w - 1)];
+          tc = v3(t4.x, t4.y, t4.z);
+        } else if (x >= 0 && x < w) {
+          const float4 t4 = row[x];
+          tc = v3(t4.x, t4.y, t4.z);
+        }
+        sum = sum + tc * wt;
+        wsum += wt;
       }
+      if (++xw == w) xw = 0;
     }
   }
   if (wsum <= 1e-9f) return tex_sample(S, tex, s, t, minor, wrap);

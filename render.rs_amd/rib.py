@@ -41,6 +41,7 @@ import numpy as np
 
 import rgbe
 import rrhip
+import rtex
 import scenes
 
 
@@ -295,7 +296,7 @@ _NO_EFFECT = {"version", "WorldEnd", "FrameBegin", "FrameEnd", "Display", "Decla
 
 # Requests render.rs implements that this backend does not.
 _UNSUPPORTED = {"SubdivisionMesh", "HierarchicalSubdivisionMesh", "PatchMesh",
-                "Patch", "NuPatch", "GeneralPolygon", "PointsGeneralPolygons", "Displace", "Pattern",
+                "Patch", "NuPatch", "GeneralPolygon", "PointsGeneralPolygons", "Displace",
                 "Procedural"}
 
 # RiSpec cubic bases and their default steps (patches.rs:10-53); the
@@ -331,6 +332,7 @@ class _State:
         self.identifier = None
         self.motion = None      # (t0, t1) of the last transform motion block
         self.hair = None        # PxrMarschnerHair parameters as keyword arguments of scenes.hair
+        self.bxdf_bindings = []  # PxrSurface "reference" connections: (field name, pattern node)
         self.basis_v = _BASES["bezier"]
         self.shading_rate = 1.0
 
@@ -400,6 +402,11 @@ class Builder:
         self.motion = None     # requests of an open MotionBegin block
         self.archive_depth = 0
         self.camera_motion_inv = None
+        # pattern graph (scene_builder.rs:1703-1848): textures (level 0 of each
+        # .tex file, by path), nodes as scenes.Scene.add_pattern arguments,
+        # handles, and the materials' bindings
+        self.textures, self.tex_index = [], {}
+        self.patterns, self.pattern_handles, self.bindings = [], {}, []
 
     # -- helpers
     def _skip(self, what):
@@ -408,6 +415,12 @@ class Builder:
         if what not in self.warned:
             self.warned.add(what)
             warnings.warn(f"RIB: {what}; skipping")
+
+    def _warn(self, what):
+        """warn_once: the request is still honoured without the named part."""
+        if what not in self.warned:
+            self.warned.add(what)
+            warnings.warn(f"RIB: {what}")
 
     @property
     def xf(self):
@@ -442,6 +455,9 @@ class Builder:
         m.interior = 0 if st.interior is None else st.interior + 1
         m.id = self._object_id()
         self.materials.append(m)
+        if st.hair is None and st.bxdf is not None:
+            for field, node in st.bxdf_bindings:
+                self.bindings.append((len(self.materials) - 1, field, node))
         return len(self.materials) - 1
 
     def _emitter(self, color, scale, light_index):
@@ -629,16 +645,17 @@ class Builder:
                 a = p.numbers("absorption")
                 if a:
                     hp["absorption"] = a
-                st.hair, st.bxdf = hp, None
+                st.hair, st.bxdf, st.bxdf_bindings = hp, None, []
                 return
             st.hair = None
             if kind != "PxrSurface":
                 self._skip(f'Bxdf "{kind}" treated as PxrSurface')
-            if any("reference" in tok.split() for tok, _ in p.items):
-                self._skip("Bxdf pattern references are not supported by this backend")
             if p.number("subsurfaceGain", 0.0) > 0.0:
                 self._skip("PxrSurface subsurface is not supported by this backend")
             st.bxdf = _pxr_surface(p)
+            st.bxdf_bindings = self._bxdf_bindings(p)
+        elif name == "Pattern":
+            self._pattern(vals)
         elif name == "AreaLightSource":
             p = _Params(vals, _pstart(vals))
             st.area_light = (p.number("intensity", 1.0), p.color("lightcolor", (1.0, 1.0, 1.0)))
@@ -934,6 +951,9 @@ class Builder:
         N = p.numbers("N")
         if N is not None and len(N) != len(P):
             N = None
+        ST = p.numbers("st")
+        if ST is not None and len(ST) * 3 != len(P) * 2:  # one (s, t) per point, else none
+            ST = None
         tri = []
         cur = 0
         for nv in nverts:
@@ -948,7 +968,8 @@ class Builder:
             return
         mesh = len(self.meshes)
         self.meshes.append((np.array(P, np.float32).reshape(-1, 3), np.array(tri, np.uint32).reshape(-1, 3),
-                            None if N is None else np.array(N, np.float32).reshape(-1, 3)))
+                            None if N is None else np.array(N, np.float32).reshape(-1, 3),
+                            None if ST is None else np.array(ST, np.float32).reshape(-1, 2)))
         mat = self._material()
         if self.state.area_light is not None:
             inten, col = self.state.area_light
@@ -958,6 +979,134 @@ class Builder:
         else:
             m1 = self._endpoint(self.xf)
             self.instances.append((mesh, mat, self.xf[:3, :4].copy(), None if m1 is None else m1[:3, :4].copy()))
+
+    # -- patterns
+    def _bxdf_bindings(self, p):
+        """PxrSurface "reference <type> <param>" ["handle:output"]
+        connections (scene_builder.rs:705-728) as (field, node) pairs."""
+        out = []
+        for tok, v in p.items:
+            words = tok.split()
+            if "reference" not in words:
+                continue
+            param, target = words[-1], _as_str(v)
+            if target is None:
+                continue
+            handle = target.split(":")[0]
+            if param not in rrhip.BIND_FIELDS:
+                self._warn(f'reference to unsupported Bxdf param "{param}"; ignoring')
+                continue
+            if handle not in self.pattern_handles:
+                self._warn(f'Bxdf references unknown Pattern "{handle}"; ignoring')
+                continue
+            if param == "presence":
+                self._skip("pattern-bound presence is not supported by this backend")
+                continue
+            out.append((param, self.pattern_handles[handle]))
+        return out
+
+    def _pattern_input(self, p, name, default):
+        """pattern_input (scene_builder.rs:1740-1758): a referenced node, else
+        the constant (a colour, or one number as grey), else the default."""
+        for tok, v in p.items:
+            words = tok.split()
+            if "reference" in words and words[-1] == name:
+                target = _as_str(v)
+                if target is not None and target.split(":")[0] in self.pattern_handles:
+                    return self.pattern_handles[target.split(":")[0]], (0.0, 0.0, 0.0)
+        v = p.numbers(name)
+        if v is not None and len(v) >= 3:
+            return -1, tuple(v[:3])
+        if v:
+            return -1, (v[0], v[0], v[0])
+        return -1, default
+
+    def _open_texture(self, filename):
+        """open_texture (scene_builder.rs:1703-1736): a .tex file, or the
+        UDIM tiles 1001..1100 that exist for a "<UDIM>" name; (texture,
+        udim dict), texture -1 when none opens (renders magenta)."""
+        if not filename:
+            self._warn('Pattern texture without "filename"; rendering magenta')
+            return -1, None
+
+        def open_one(name):
+            path = name if self.base_dir is None else os.path.join(self.base_dir, name)
+            if path in self.tex_index:
+                return self.tex_index[path]
+            if not os.path.exists(path):
+                return None
+            try:
+                level0 = rtex.read_tex(path)
+            except (OSError, rtex.TexError) as e:
+                self._warn(f"texture {path}: {e}")
+                return None
+            self.tex_index[path] = len(self.textures)
+            self.textures.append(level0)
+            return self.tex_index[path]
+
+        if "<UDIM>" in filename:
+            tiles = {}
+            for tile in range(1001, 1101):
+                t = open_one(filename.replace("<UDIM>", str(tile)))
+                if t is not None:
+                    tiles[tile] = t
+            if not tiles:
+                self._warn(f'no UDIM tiles found for "{filename}"')
+                return -1, None
+            return -1, tiles
+        t = open_one(filename)
+        if t is None:
+            self._warn(f"texture {filename}: cannot open; rendering magenta")
+            return -1, None
+        return t, None
+
+    def _pattern(self, vals):
+        """pattern_request (scene_builder.rs:1760-1848): one graph node under
+        its handle."""
+        ptype = _str(vals, 0) or ""
+        start = 2 if len(vals) % 2 == 0 else 1
+        handle = _str(vals, 1) if start == 2 else None
+        if handle is None:
+            return self._warn("Pattern without a handle; skipping")
+        p = _Params(vals, start)
+        wrap = {"clamp": rrhip.WRAP_CLAMP, "black": rrhip.WRAP_BLACK}.get(p.string("wrapMode") or "periodic",
+                                                                          rrhip.WRAP_PERIODIC)
+        if ptype in ("PxrTexture", "texture"):
+            tex, udim = self._open_texture(p.string("filename") or "")
+            kind, kw = rrhip.PATTERN_TEXTURE, dict(texture=tex, udim=udim, wrap=wrap,
+                                                    scale=(p.number("scaleS", 1.0), p.number("scaleT", 1.0)))
+        elif ptype in ("PxrChecker", "checker"):
+            kind, kw = rrhip.PATTERN_CHECKER, dict(color_a=p.color("colorA", (1.0, 1.0, 1.0)),
+                                                    color_b=p.color("colorB", (0.0, 0.0, 0.0)),
+                                                    scale=(p.number("sScale", 8.0), p.number("tScale", 8.0)))
+        elif ptype in ("PxrFractal", "fractal", "noise"):
+            kind, kw = rrhip.PATTERN_FRACTAL, dict(frequency=p.number("frequency", 1.0),
+                                                    octaves=int(p.number("layers", 4.0)), gain=p.number("gain", 0.5),
+                                                    lacunarity=p.number("lacunarity", 2.0))
+        elif ptype in ("PxrMix", "mix"):
+            ins = [self._pattern_input(p, n, d) for n, d in (("color1", (0.0, 0.0, 0.0)), ("color2", (1.0, 1.0, 1.0)),
+                                                             ("mix", (0.0, 0.0, 0.0)))]
+            kind, kw = rrhip.PATTERN_MIX, dict(input=[i for i, _ in ins], constant=[c for _, c in ins])
+        elif ptype in ("PxrColorCorrect", "colorCorrect"):
+            i, c = self._pattern_input(p, "inputRGB", (0.5, 0.5, 0.5))
+            kind, kw = rrhip.PATTERN_COLOR_CORRECT, dict(input=(i, -1, -1), constant=[c], cc_gain=p.color(
+                "gain", (1.0, 1.0, 1.0)), cc_offset=p.color("offset", (0.0, 0.0, 0.0)), gamma=p.number("gamma", 1.0),
+                saturation=p.number("saturation", 1.0))
+        elif ptype in ("PxrRamp", "ramp"):
+            pos = p.numbers("positions") or [0.0, 1.0]
+            col = p.numbers("colors") or [0.0, 0.0, 0.0, 1.0, 1.0, 1.0]
+            cols = [tuple(col[3 * k:3 * k + 3]) for k in range(len(col) // 3)]
+            n = min(len(pos), len(cols))
+            kind, kw = rrhip.PATTERN_RAMP, dict(stops=list(zip(pos[:n], cols[:n])) or None,
+                                                 use_t=1 if p.string("axis") == "t" else 0)
+        elif ptype == "triplanar":
+            tex, udim = self._open_texture(p.string("filename") or "")
+            kind, kw = rrhip.PATTERN_TRIPLANAR, dict(texture=tex, udim=udim, wrap=wrap,
+                                                      frequency=p.number("frequency", 1.0))
+        else:
+            return self._warn(f'Pattern "{ptype}" not implemented; skipping')
+        self.pattern_handles[handle] = len(self.patterns)
+        self.patterns.append((kind, kw))
 
     # -- result
     @property
@@ -989,8 +1138,8 @@ class Builder:
             s.add_triangle(v0, v1, v2, mat)
         for kind, mat, m, kw in self.quadrics:
             s.add_quadric(kind, mat, m, **kw)
-        for pos, idx, nrm in self.meshes:
-            s.add_mesh(pos, idx, nrm)
+        for pos, idx, nrm, st in self.meshes:
+            s.add_mesh(pos, idx, nrm, st)
         for mesh, mat, m, m1 in self.instances:
             s.add_instance(mesh, mat, m, m1)
         for p0, p1, v0, v1 in self.curve_sets:
@@ -1000,6 +1149,12 @@ class Builder:
         for md in self.media:
             s.add_medium(**md)
         s.atmosphere = self.atmosphere
+        for tex in self.textures:
+            s.add_texture(tex)
+        for kind, kw in self.patterns:
+            s.add_pattern(kind, **kw)
+        for mat, field, node in self.bindings:
+            s.bind(mat, field, node)
         if self.env is not None:
             s.set_env(self.env)
         return s

@@ -805,7 +805,7 @@ def pattern_zoo(width=64, height=64, lights=True):
     if lights:
         s.add_light(light(rrhip.LIGHT_RECT, intensity=5.0, p=(-1.5, 4, -2), e1=(3, 0, 0), e2=(0, 0, 3)))
         s.add_light(light(rrhip.LIGHT_DISTANT, intensity=1.5, p=(0.3, -1, 0.4)))
-    tex = s.add_texture(smooth_texture(64, 64, seed=1))
+    tex = s.add_texture(smooth_texture(512, 512, seed=1, cells=24))
     stripes = s.add_texture(stripes_texture(64, 64, 4))
     tile_a = s.add_texture(smooth_texture(32, 32, seed=2, cells=4))
     tile_b = s.add_texture(smooth_texture(16, 32, seed=3, cells=3))

@@ -13,6 +13,9 @@ src/texture/pattern.rs, cache.rs; pt/mod.rs:389-440, 477-480, 685-689):
   means within 3 sigma): secondary hits filter with the cone's spread
   after rough bounces, and pattern-driven roughness / colour feed the
   lobes, with error-bounded origins and with render.rs's fixed push;
+* a textured still life through the RIB front end (.tex files with UDIM
+  tiles, PointsPolygons st, Pattern requests, PxrSurface references): first
+  hits and 3 sigma as above;
 * a bound presence, radiance caches and batches are refused.
 """
 import sys
@@ -132,3 +135,22 @@ def test_pattern_refusals(ctx):
     with pytest.raises(rrhip.RrhipError) as e:
         rrhip.PtScene(ctx, s2.desc())
     assert e.value.status == 5
+
+
+def test_rib_still_life_matches_oracle(ctx, tmp_path):
+    # a textured still life through the RIB front end: .tex files (UDIM deck,
+    # label), PointsPolygons st, Pattern requests and PxrSurface references
+    import rib
+    import rib_scenes
+    s, _ = rib.load(rib_scenes.write_still_life(str(tmp_path), 96, 48))
+    say(f"RIB still life 96x48: {len(s.textures)} textures, {len(s.patterns)} nodes, {len(s.bindings)} bindings")
+    g = film_gpu(ctx, s, rrhip.RenderParams.make(max_depth=2), 8)
+    o = orc.render_film(s.desc(), rrhip.RenderParams.make(max_depth=2, ray_offset=1e-9), 0, 8)
+    same = (g[:, 16] == o[:, 16]) & (np.abs(g[:, 17] - o[:, 17]) < 0.5 / 8)
+    ok = same & (np.abs(g[:, 9:12] - o[:, 9:12]).max(1) < 2e-3)
+    say(f"still life first-hit albedo differs on {(~ok).sum()} of {len(ok)} pixels")
+    assert (~ok).mean() <= 0.03
+    p = rrhip.RenderParams.make(max_depth=6)
+    gr, _ = gpu_render(ctx, s, p, 0, 64)
+    c, _ = oracle_render(s.desc(), p, 0, 64)
+    parity(gr, c, "RIB still life")

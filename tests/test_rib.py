@@ -21,6 +21,7 @@ import pytest
 
 import orc
 import rib
+import rtex  # noqa: F401
 import rrhip
 import scenes
 
@@ -440,3 +441,120 @@ def test_curve_requests_edge_cases():
     s = build('WorldBegin Bxdf "PxrMarschnerHair" "h" Bxdf "PxrSurface" "s" '
               'Curves "linear" [2] "nonperiodic" "P" [0 0 0 1 0 0] WorldEnd').scene()
     assert s.materials[s.instances[0].material].hair == 0
+
+
+# ---- patterns and textures ---------------------------------------------------------
+
+def test_pattern_graph_binds_to_bxdf():
+    # scene_builder.rs:1993-2027: a checker mixed 25 % towards green drives
+    # diffuseColor; the checker's cell (0, 0) is red
+    s, _ = rib.loads("""
+        Format 64 64 1.0
+        WorldBegin
+            Pattern "PxrChecker" "check" "colorA" [1 0 0] "colorB" [0 0 1] "sScale" [4] "tScale" [4]
+            Pattern "PxrMix" "blend" "reference color color1" ["check:resultRGB"] "color2" [0 1 0] "mix" [0.25]
+            Bxdf "PxrSurface" "mat" "reference color diffuseColor" ["blend:resultRGB"] "specularIor" [1]
+            Sphere 1 -1 1 360
+        WorldEnd""")
+    assert len(s.patterns) == 2
+    assert [(b.material, b.field, b.pattern) for b in s.bindings] == [(len(s.materials) - 1, 0, 1)]
+    c = orc.pattern_eval(s, 1, [0.1, 0.1])[0]
+    assert abs(c[0] - 0.75) < 1e-9 and abs(c[1] - 0.25) < 1e-9
+
+
+def test_pattern_request_defaults_and_inputs():
+    # pattern_request (scene_builder.rs:1760-1848): checker scales default
+    # to 8, fractal layers -> octaves, colour-correct input defaults to
+    # grey 0.5, a single number is a grey constant, ramp axis "t"
+    b = build("""
+        WorldBegin
+            Pattern "PxrChecker" "c"
+            Pattern "PxrFractal" "f" "frequency" [2] "layers" [6] "gain" [0.4]
+            Pattern "PxrColorCorrect" "cc" "gamma" [2]
+            Pattern "PxrMix" "m" "color1" [0.2] "reference color mix" ["f:resultRGB"]
+            Pattern "PxrRamp" "r" "axis" ["t"]
+        WorldEnd""")
+    (kc, c), (kf, f), (kcc, cc), (km, m), (kr, r) = b.patterns
+    assert kc == rrhip.PATTERN_CHECKER and c["scale"] == (8.0, 8.0)
+    assert c["color_a"] == (1.0, 1.0, 1.0) and c["color_b"] == (0.0, 0.0, 0.0)
+    assert kf == rrhip.PATTERN_FRACTAL and f["octaves"] == 6 and f["gain"] == 0.4 and f["frequency"] == 2
+    assert kcc == rrhip.PATTERN_COLOR_CORRECT and cc["input"][0] == -1 and cc["constant"][0] == (0.5, 0.5, 0.5)
+    assert km == rrhip.PATTERN_MIX and list(m["input"]) == [-1, -1, 1]
+    assert m["constant"][0] == (0.2, 0.2, 0.2) and m["constant"][1] == (1.0, 1.0, 1.0)
+    assert kr == rrhip.PATTERN_RAMP and r["use_t"] == 1 and len(r["stops"]) == 2
+    s = b.scene()
+    # colour correct of the grey default: 0.5^(1/2)
+    np.testing.assert_allclose(orc.pattern_eval(s, 2, [0, 0])[0], np.sqrt(0.5), rtol=1e-12)
+
+
+def test_pattern_references_that_do_not_resolve():
+    text = """
+        WorldBegin
+            Pattern "PxrChecker" "c"
+            Pattern "PxrWorley" "w"
+            Pattern "PxrTexture" "t"
+            Bxdf "PxrSurface" "m" "reference color diffuseColor" ["nope:resultRGB"]
+                "reference float bumpAmount" ["c:resultR"] "reference color specularFaceColor" ["t:resultRGB"]
+            Sphere 1 -1 1 360
+        WorldEnd"""
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        b = build(text)
+    msgs = " ".join(str(x.message) for x in w)
+    assert "PxrWorley" in msgs and "nope" in msgs and "bumpAmount" in msgs and "filename" in msgs
+    assert len(b.patterns) == 2  # the texture node stays, magenta
+    assert b.patterns[1][1]["texture"] == -1
+    assert [(f, n) for _, f, n in b.bindings] == [("specularFaceColor", 1)]
+    s = b.scene()
+    np.testing.assert_array_equal(orc.pattern_eval(s, 1, [0.3, 0.3])[0], [1, 0, 1])
+    # a bound presence is a scene this backend refuses
+    with pytest.raises(rib.RibUnsupported):
+        build("""WorldBegin Pattern "PxrChecker" "c"
+                 Bxdf "PxrSurface" "m" "reference float presence" ["c:resultR"] WorldEnd""")
+
+
+def test_points_polygons_st():
+    b = build("""
+        WorldBegin
+            PointsPolygons [3] [0 1 2] "P" [0 0 0 1 0 0 0 1 0] "st" [0 0 1 0 0 1]
+            PointsPolygons [3] [0 1 2] "P" [0 0 0 1 0 0 0 1 0] "st" [0 0 1 0]
+        WorldEnd""")
+    np.testing.assert_array_equal(b.meshes[0][3], [[0, 0], [1, 0], [0, 1]])
+    assert b.meshes[1][3] is None  # not one pair per point: ignored
+
+
+def test_textured_still_life_builds(tmp_path):
+    import rib_scenes
+    path = rib_scenes.write_still_life(str(tmp_path))
+    s, _ = rib.load(path)
+    kinds = [p.kind for p in s.patterns]
+    assert kinds == [rrhip.PATTERN_TEXTURE, rrhip.PATTERN_TEXTURE, rrhip.PATTERN_COLOR_CORRECT,
+                     rrhip.PATTERN_TRIPLANAR, rrhip.PATTERN_FRACTAL, rrhip.PATTERN_CHECKER, rrhip.PATTERN_MIX,
+                     rrhip.PATTERN_RAMP]
+    # the four wood tiles and the label, each read once (the triplanar
+    # ball reuses tile 1001)
+    assert len(s.textures) == 5
+    udim = np.ctypeslib.as_array(s.patterns[0].udim, (100,))
+    assert [int(udim[t - 1001]) for t in (1001, 1002, 1011, 1012)] == [0, 1, 2, 3]
+    assert (np.delete(udim, [0, 1, 10, 11]) == -1).all()
+    assert s.patterns[3].texture == 0
+    fields = sorted((b.material, rrhip.BIND_FIELDS[b.field]) for b in s.bindings)
+    assert [f for _, f in fields] == ["diffuseColor"] * 3 + ["specularRoughness", "diffuseColor",
+                                                             "diffuseColor"]
+    a, _ = orc.render(s.desc(), rrhip.RenderParams.make(max_depth=4), 0, 2)
+    assert np.isfinite(a).all() and a[:, :3].mean() > 0
+
+
+REF_FIXTURES = "/root/reference/tests/fixtures"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_FIXTURES), reason="the reference tree is not present")
+@pytest.mark.parametrize("name", ["label.tex", "wood_1001.tex"])
+def test_tex_writer_reproduces_reference_files(tmp_path, name):
+    # level 0 of the reference's own .tex file, written again: every mip
+    # level, tile and header byte as render.rs's tex.rs wrote them
+    import rtex
+    src = os.path.join(REF_FIXTURES, name)
+    dst = str(tmp_path / name)
+    rtex.write_tex(dst, rtex.read_tex(src))
+    assert open(dst, "rb").read() == open(src, "rb").read()
