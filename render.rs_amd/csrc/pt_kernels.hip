@@ -200,7 +200,7 @@ __device__ __forceinline__ void extend_one(const PtSceneView& S, const PathBuffe
     P.hit_front[slot] = make_float4(front.x, front.y, front.z, tw.x);
     P.hit_back[slot] = make_float4(back.x, back.y, back.z, tw.y);
     if constexpr ((GEO & kGeoSt) != 0) {  // pattern scenes: texture coordinates of the hit
-      const HitSt hs = hit_st(S, h);
+      const HitSt hs = hit_st<MOTION>(S, h, time);
       P.hit_st[slot] = make_float4(hs.s, hs.t, hs.density, 0.f);
       P.hit_gs[slot] = make_float4(hs.gs.x, hs.gs.y, hs.gs.z, 0.f);
       P.hit_gt[slot] = make_float4(hs.gt.x, hs.gt.y, hs.gt.z, 0.f);
@@ -956,7 +956,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
 
 // Scenes with pattern-bound materials (and hair fibers, if any): the
 // pattern evaluator's value stack and the texture filters next to both
-// BSDFs, at two waves per SIMD.
+// BSDFs, at two waves per SIMD (218-243 VGPRs without the fiber BSDF, so
+// no smaller variant; a three-wave cap spills 140 VGPRs).
 template <bool TR, bool FILM, bool FIXED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void shade_pattern_kernel(
     PtSceneView S, PathBuffers P, PtParams prm, const uint32_t* __restrict__ q, uint32_t* __restrict__ qnext,

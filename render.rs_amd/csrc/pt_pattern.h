@@ -101,8 +101,10 @@ __device__ __forceinline__ void quadric_st(const PtQuadric& q, float2 qs, V3 p, 
 // interpolate them, with the density sqrt(st area / object area) over the
 // instance's scale and world gradients through the instance's inverse
 // (mesh.rs:145-197, 357-370); curves give [0.5, v] (mesh.rs:339-342);
-// spheres and quadrics their parametric st; loose triangles none.
-__device__ __forceinline__ HitSt hit_st(const PtSceneView& S, const Hit& h) {
+// spheres and quadrics their parametric st; loose triangles none. A
+// moving curve instance takes v at the ray's shutter time.
+template <bool MOTION>
+__device__ __forceinline__ HitSt hit_st(const PtSceneView& S, const Hit& h, float time) {
   HitSt o;
   o.s = o.t = o.density = 0.f;
   o.gs = o.gt = v3(0.f, 0.f, 0.f);
@@ -110,7 +112,14 @@ __device__ __forceinline__ HitSt hit_st(const PtSceneView& S, const Hit& h) {
     const PtInstance& I = S.insts[h.inst];
     if (S.meshes[I.mesh].curves) {
       const CurveRec c = S.csegs[h.slot];
-      const V3 lp = xform_point(I.inv, h.p);
+      V3 lp;
+      if (!MOTION || I.motion == 0xFFFFFFFFu) {
+        lp = xform_point(I.inv, h.p);
+      } else {
+        float inv[12];
+        instance_inverse(S, I, time, inv);
+        lp = xform_point(inv, h.p);
+      }
       const V3 p0 = v3(c.p0[0], c.p0[1], c.p0[2]), ax = v3(c.p1[0], c.p1[1], c.p1[2]) - p0;
       const float l2 = dot(ax, ax);
       const float along = l2 > 1e-24f ? clampf(dot(lp - p0, ax) / l2, 0.f, 1.f) : 0.5f;

@@ -154,3 +154,30 @@ def test_rib_still_life_matches_oracle(ctx, tmp_path):
     gr, _ = gpu_render(ctx, s, p, 0, 64)
     c, _ = oracle_render(s.desc(), p, 0, 64)
     parity(gr, c, "RIB still life")
+
+
+def test_moving_patterned_geometry_first_hits(ctx):
+    # texture coordinates at the shutter time: a moving curve set coloured
+    # by a ramp over v and a moving textured quad, first-hit albedo per pixel
+    s = scenes.Scene(48, 48, 40.0, eye=(0, 0.5, -6), look_at=(0, 0, 0))
+    s.add_light(scenes.light(rrhip.LIGHT_DISTANT, intensity=2.0, p=(0.2, -1, 0.5)))
+    cm = s.add_material(scenes.matte((0.5, 0.5, 0.5), object_id=1))
+    qm = s.add_material(scenes.matte((0.5, 0.5, 0.5), object_id=2))
+    cs = s.add_curves([[-1.5, 0.8, 0, 0.25]], [[1.5, 1.2, 0, 0.15]], [0.0], [1.0])
+    s.add_instance(cs, cm, scenes._place((-0.4, 0, 0)), scenes._place((0.6, 0.3, 0)), curves=True)
+    s.bind(cm, "diffuseColor", s.add_pattern(rrhip.PATTERN_RAMP, use_t=1, stops=[(0.0, (0.9, 0.1, 0.1)),
+                                                                                 (1.0, (0.1, 0.2, 0.9))]))
+    qmesh = s.add_mesh([[0, 0, 0], [2, 0, 0], [2, 2, 0], [0, 2, 0]], [0, 1, 2, 0, 2, 3],
+                       st=[[0, 0], [1, 0], [1, 1], [0, 1]])
+    s.add_instance(qmesh, qm, scenes._place((-1.5, -1.8, 1)), scenes._place((-0.9, -1.6, 1)))
+    s.bind(qm, "diffuseColor", s.add_pattern(rrhip.PATTERN_TEXTURE, texture=s.add_texture(
+        scenes.smooth_texture(128, 128, seed=5, cells=10))))
+    g = film_gpu(ctx, s, rrhip.RenderParams.make(max_depth=2), 8)
+    o = orc.render_film(s.desc(), rrhip.RenderParams.make(max_depth=2, ray_offset=1e-9), 0, 8)
+    same = (g[:, 16] == o[:, 16]) & (np.abs(g[:, 17] - o[:, 17]) < 0.5 / 8)
+    ok = same & (np.abs(g[:, 9:12] - o[:, 9:12]).max(1) < 2e-3)
+    ids = o[:, 16].astype(int)
+    say(f"moving patterned geometry: albedo differs on {(~ok).sum()} of {len(ok)} pixels "
+        f"({(ids == 1).sum()} curve, {(ids == 2).sum()} quad)")
+    assert (ids == 1).sum() > 50 and (ids == 2).sum() > 50
+    assert (~ok).mean() <= 0.03
