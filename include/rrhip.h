@@ -333,7 +333,7 @@ enum {
   RRHIP_BIND_GLASS_ROUGHNESS = 10,   /* clamped to [0.005, 1] */
   RRHIP_BIND_REFRACTION_COLOR = 11,
   RRHIP_BIND_GLOW = 12,
-  RRHIP_BIND_PRESENCE = 13           /* clamped to [0, 1] */
+  RRHIP_BIND_PRESENCE = 13           /* clamped to [0, 1]; cutouts for camera and shadow rays */
 };
 typedef struct rrhip_binding_desc {  /* Material::pattern_bindings entry */
   uint32_t material;
@@ -386,9 +386,10 @@ typedef struct rrhip_scene_desc {
    * material follows its pattern at every shading point, filtered by the
    * ray cone's footprint (pt/mod.rs:389-440). Texture coordinates come
    * from meshes' st, spheres, quadrics (cones none) and curves ([0.5, v]);
-   * loose triangles have none. At most 15 bindings per material; a bound
-   * presence is refused (RRHIP_ERR_UNSUPPORTED), as are radiance caches
-   * and batches over scenes with bindings. */
+   * loose triangles have none; a bound presence also decides shadow rays
+   * at every blocker (pt/mod.rs:1294). At most 15 bindings per material;
+   * radiance caches and batches refuse scenes with bindings
+   * (RRHIP_ERR_UNSUPPORTED). */
   uint32_t num_textures;
   uint32_t num_patterns;
   const rrhip_texture_desc* textures;

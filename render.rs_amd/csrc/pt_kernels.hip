@@ -994,8 +994,17 @@ __device__ __forceinline__ V3 walk_visibility(const PtSceneView& S, V3 o, V3 d, 
       else
         stack_pop(depth);
     } else {
-      if (m.presence >= 1.f) return v3(0.f, 0.f, 0.f);
-      vis = vis * (1.f - m.presence);
+      float presence = m.presence;
+      if constexpr ((GEO & kGeoSt) != 0) {  // bound presence at the blocker
+        if (m.bind) {
+          const HitSt hs = hit_st<MOTION>(S, h, time);
+          const PatCtx pc = pat_ctx(make_float4(hs.s, hs.t, hs.density, 0.f), make_float4(0.f, 0.f, 0.f, 0.f),
+                                    make_float4(0.f, 0.f, 0.f, 0.f), h.p, h.n, d, 0.f);
+          presence = pat_presence(S, m.bind, pc, presence);
+        }
+      }
+      if (presence >= 1.f) return v3(0.f, 0.f, 0.f);
+      vis = vis * (1.f - presence);
       if (maxc(vis) < 1e-4f) return v3(0.f, 0.f, 0.f);
     }
     if (S.ray_offset > 0.f) {
@@ -1094,9 +1103,11 @@ __global__ __launch_bounds__(kTraceBlock) void shadow_kernel(PtSceneView S, Path
 template <int STACK, bool MOTION, int GEO, bool COUNT>
 void launch_trace_counted(const PtSceneView& S, const PathBuffers& P, const uint32_t* q, uint32_t gt, bool extend,
                           hipStream_t st) {
-  constexpr int SG = GEO & ~kGeoSt;  // shadow rays need no texture coordinates
+  constexpr int SG = GEO & ~kGeoSt;  // shadow rays need texture coordinates only for a bound presence
   if (extend)
     hipLaunchKernelGGL((extend_kernel<STACK, MOTION, GEO, COUNT>), dim3(gt), dim3(kTraceBlock), 0, st, S, P, q);
+  else if (S.transparent && (GEO & kGeoSt) && (S.patterns & 2u))
+    hipLaunchKernelGGL((shadow_kernel<STACK, true, MOTION, GEO, COUNT>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
   else if (S.transparent)
     hipLaunchKernelGGL((shadow_kernel<STACK, true, MOTION, SG, COUNT>), dim3(gt), dim3(kTraceBlock), 0, st, S, P);
   else

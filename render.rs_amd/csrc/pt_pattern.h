@@ -467,6 +467,7 @@ __device__ __forceinline__ V3 pat_run(const PtSceneView& S, const PtBinding& b, 
 __device__ __forceinline__ void pat_resolve_material(const PtSceneView& S, uint32_t mat, uint32_t bind,
                                                      const PatCtx& c, PtMaterial& m) {
   RawPbr<float> r = S.raw_pbr[mat];
+  float presence = m.presence;
   const uint32_t first = bind >> 4, count = bind & 15u;
   for (uint32_t k = 0; k < count; k++) {
     const PtBinding b = S.binds[first + k];
@@ -488,10 +489,23 @@ __device__ __forceinline__ void pat_resolve_material(const PtSceneView& S, uint3
       case RRHIP_BIND_REFRACTION_COLOR:
         r.refraction_color[0] = v.x; r.refraction_color[1] = v.y; r.refraction_color[2] = v.z; break;
       case RRHIP_BIND_GLOW: r.glow[0] = v.x; r.glow[1] = v.y; r.glow[2] = v.z; break;
+      case RRHIP_BIND_PRESENCE: presence = clampf(v.x, 0.f, 1.f); break;
       default: break;
     }
   }
   lobe_constants(r, m);
+  m.presence = presence;
+}
+
+// A blocker's presence for shadow rays: the material's, or its bound
+// pattern's at the blocker with no footprint (pt/mod.rs:1294).
+__device__ __forceinline__ float pat_presence(const PtSceneView& S, uint32_t bind, const PatCtx& c, float presence) {
+  const uint32_t first = bind >> 4, count = bind & 15u;
+  for (uint32_t k = 0; k < count; k++) {
+    const PtBinding b = S.binds[first + k];
+    if (b.field == RRHIP_BIND_PRESENCE) presence = clampf(pat_run(S, b, c).x, 0.f, 1.f);
+  }
+  return presence;
 }
 
 // shade_params (pt/mod.rs:389-440): the shading context of a hit from its

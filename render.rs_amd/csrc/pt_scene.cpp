@@ -552,8 +552,7 @@ bool build_patterns(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
     if (b.material >= d.num_materials) return *err = "binding material out of range", false;
     if (b.field > RRHIP_BIND_PRESENCE) return *err = "unknown binding field", false;
     if (b.pattern >= d.num_patterns) return *err = "binding pattern out of range", false;
-    if (b.field == RRHIP_BIND_PRESENCE)
-      return *err = "UNSUPPORTED: pattern-bound presence (cutouts follow the material's constant presence)", false;
+    if (b.field == RRHIP_BIND_PRESENCE) out->patterns |= 2u;  // shadow rays walk and evaluate it at blockers
     per_mat[b.material].push_back(i);
   }
   for (uint32_t m = 0; m < d.num_materials; m++) {
@@ -576,7 +575,11 @@ bool build_patterns(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
   if (out->binds.size() >= (1u << 27)) return *err = "too many pattern bindings", false;
   for (uint32_t m = 0; m < d.num_materials; m++) out->raw_pbr.push_back(raw_pbr<float>(d.materials[m]));
   out->pixel_spread = (float)(std::tan(d.camera.fov_deg * kPi / 180.0 / 2.0) * 2.0 / (double)d.camera.height);
-  out->patterns = 1;
+  out->patterns |= 1u;
+  if (out->patterns & 2u) {  // a bound presence makes the scene transparent (pt/mod.rs:702-710, 1294)
+    out->transparent = 1;
+    if (d.num_media > 255) return *err = "at most 255 media with interior media or cutouts (8-bit medium stack)", false;
+  }
   return true;
 }
 

@@ -999,9 +999,6 @@ class Builder:
             if handle not in self.pattern_handles:
                 self._warn(f'Bxdf references unknown Pattern "{handle}"; ignoring')
                 continue
-            if param == "presence":
-                self._skip("pattern-bound presence is not supported by this backend")
-                continue
             out.append((param, self.pattern_handles[handle]))
         return out
 

@@ -16,7 +16,10 @@ src/texture/pattern.rs, cache.rs; pt/mod.rs:389-440, 477-480, 685-689):
 * a textured still life through the RIB front end (.tex files with UDIM
   tiles, PointsPolygons st, Pattern requests, PxrSurface references): first
   hits and 3 sigma as above;
-* a bound presence, radiance caches and batches are refused.
+* a checker-bound presence cuts holes for camera rays and for shadow rays
+  at every blocker (first hits per pixel, 3 sigma);
+* radiance caches, batches and more than 15 bindings on a material are
+  refused.
 """
 import sys
 
@@ -116,6 +119,37 @@ def test_patterns_with_hair_match_oracle(ctx):
     parity(g, c, "curve zoo with patterns")
 
 
+def cutout_scene(width=48, height=48):
+    """A leaf-like card whose presence follows a checker (cells 0 / 1,
+    partly filtered) over a ground, lit from above: holes for camera rays
+    and for the light's shadow rays."""
+    s = scenes.Scene(width, height, 40.0, eye=(0, 2.5, -5), look_at=(0, 0, 0.5))
+    s.add_light(scenes.light(rrhip.LIGHT_RECT, intensity=8.0, p=(-1, 3, -1), e1=(2, 0, 0), e2=(0, 0, 2)))
+    s.add_light(scenes.light(rrhip.LIGHT_DISTANT, intensity=1.0, p=(0.3, -1, 0.2)))
+    ground = s.add_material(scenes.matte((0.7, 0.7, 0.7), object_id=1))
+    s.add_polygon([(-6, -1, -6), (6, -1, -6), (6, -1, 6), (-6, -1, 6)], ground)
+    card = s.add_material(scenes.matte((0.3, 0.7, 0.2), object_id=2))
+    mesh = s.add_mesh([[0, 0, 0], [2, 0, 0], [2, 0, 2], [0, 0, 2]], [0, 1, 2, 0, 2, 3],
+                      st=[[0, 0], [1, 0], [1, 1], [0, 1]])
+    s.add_instance(mesh, card, scenes._place((-1, 0, -0.2), 10, 1.0))
+    s.bind(card, "presence", s.add_pattern(rrhip.PATTERN_CHECKER, color_a=(1, 1, 1), color_b=(0.15, 0, 0),
+                                           scale=(5, 5)))
+    return s
+
+
+def test_bound_presence_cuts_camera_and_shadow_rays(ctx):
+    s = cutout_scene()
+    g = film_gpu(ctx, s, rrhip.RenderParams.make(max_depth=2), 8)
+    o = orc.render_film(s.desc(), rrhip.RenderParams.make(max_depth=2, ray_offset=1e-9), 0, 8)
+    ok = (g[:, 16] == o[:, 16]) & (np.abs(g[:, 17] - o[:, 17]) < 0.5 / 8 + 1e-9)
+    say(f"cutout card: first-hit id / coverage differ on {(~ok).sum()} of {len(ok)} pixels")
+    assert (~ok).mean() <= 0.03
+    p = rrhip.RenderParams.make(max_depth=6)
+    gr, _ = gpu_render(ctx, s, p, 0, 128)
+    c, _ = oracle_render(s.desc(), p, 0, 128)
+    parity(gr, c, "cutout card")
+
+
 def test_pattern_refusals(ctx):
     s = scenes.pattern_zoo(16, 16)
     ps = rrhip.PtScene(ctx, s.desc())
@@ -131,7 +165,8 @@ def test_pattern_refusals(ctx):
     se.close()
     ps.close()
     s2 = scenes.pattern_zoo(16, 16)
-    s2.bind(1, "presence", 0)
+    for k in range(16):  # more than 15 bindings on one material
+        s2.bind(1, "diffuseGain", 0)
     with pytest.raises(rrhip.RrhipError) as e:
         rrhip.PtScene(ctx, s2.desc())
     assert e.value.status == 5

@@ -507,10 +507,10 @@ def test_pattern_references_that_do_not_resolve():
     assert [(f, n) for _, f, n in b.bindings] == [("specularFaceColor", 1)]
     s = b.scene()
     np.testing.assert_array_equal(orc.pattern_eval(s, 1, [0.3, 0.3])[0], [1, 0, 1])
-    # a bound presence is a scene this backend refuses
-    with pytest.raises(rib.RibUnsupported):
-        build("""WorldBegin Pattern "PxrChecker" "c"
-                 Bxdf "PxrSurface" "m" "reference float presence" ["c:resultR"] WorldEnd""")
+    # a bound presence is a binding like the others
+    b = build("""WorldBegin Pattern "PxrChecker" "c"
+                 Bxdf "PxrSurface" "m" "reference float presence" ["c:resultR"] Sphere 1 -1 1 360 WorldEnd""")
+    assert [(f, n) for _, f, n in b.bindings] == [("presence", 0)]
 
 
 def test_points_polygons_st():
