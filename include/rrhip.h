@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define RRHIP_ABI_VERSION 8u
+#define RRHIP_ABI_VERSION 9u
 #define RRHIP_NO_HIT 0xFFFFFFFFu
 
 /* Status codes. render.rs returns anyhow::Result with a message
@@ -88,6 +88,12 @@ typedef struct rrhip_mesh_desc {
   uint64_t num_triangles;
   const float* normals;
   const float* st;
+  /* Deformation motion (Mesh::positions1, mesh.rs:31): the vertices at
+   * shutter close, lerped with `positions` at each ray's shutter time;
+   * NULL: static. A deforming mesh shades with geometric normals at that
+   * time (its vertex normals belong to the rest pose) and takes its
+   * texture coordinates' density and gradients from the rest pose. */
+  const float* positions1;
 } rrhip_mesh_desc;
 
 /* A set of curve segments (render.rs CurveSet, src/geometry/curves.rs:
@@ -303,7 +309,7 @@ typedef struct rrhip_pattern_desc {
   uint32_t use_t;           /* ramp: over t instead of s */
   const int32_t* udim;      /* texture: NULL, or 100 texture indices of UDIM tiles 1001..1100 (-1: none) */
   int32_t input[3];         /* mix: color1, color2, mix; color correct: input */
-  uint32_t octaves;         /* fractal layers */
+  uint32_t octaves;         /* fractal layers (at most 64) */
   double constant[3][3];    /* the inputs' constants */
   double color_a[3], color_b[3];    /* checker */
   double cc_gain[3], cc_offset[3];  /* color correct */

@@ -104,8 +104,10 @@ void Bvh::subdivide(const std::vector<Aabb>& bounds, const std::vector<Vec3>& ce
 
 // mesh.rs:48-84 (no deformation endpoint)
 void Mesh::build(const float* pos, size_t nverts, const uint32_t* idx, size_t ntris,
-                 const BvhParams& params) {
+                 const BvhParams& params, const float* pos1) {
   positions.assign(pos, pos + 3 * nverts);
+  positions1.clear();
+  if (pos1) positions1.assign(pos1, pos1 + 3 * nverts);
   indices.assign(idx, idx + 3 * ntris);
   std::vector<Aabb> bounds(ntris);
   local_bounds = Aabb{};
@@ -114,6 +116,11 @@ void Mesh::build(const float* pos, size_t nverts, const uint32_t* idx, size_t nt
     b.grow_point(vertex(indices[3 * t]));
     b.grow_point(vertex(indices[3 * t + 1]));
     b.grow_point(vertex(indices[3 * t + 2]));
+    if (pos1)
+      for (int k = 0; k < 3; k++) {
+        const float* q = pos1 + 3 * (size_t)indices[3 * t + k];
+        b.grow_point(Vec3{(double)q[0], (double)q[1], (double)q[2]});
+      }
     bounds[t] = b;
     local_bounds.grow(b);
   }
@@ -125,9 +132,9 @@ void Mesh::build(const float* pos, size_t nverts, const uint32_t* idx, size_t nt
 bool Mesh::intersect_triangle(const Ray& ray, uint32_t tri, double t_min, double t_max,
                               double* t_out, double* u_out, double* v_out) const {
   uint32_t i0 = indices[3 * tri], i1 = indices[3 * tri + 1], i2 = indices[3 * tri + 2];
-  Vec3 v0 = vertex(i0);
-  Vec3 e1 = vertex(i1) - v0;
-  Vec3 e2 = vertex(i2) - v0;
+  Vec3 v0 = vertex_at(i0, ray.time);
+  Vec3 e1 = vertex_at(i1, ray.time) - v0;
+  Vec3 e2 = vertex_at(i2, ray.time) - v0;
   Vec3 p = ray.direction.cross(e2);
   double det = e1.dot(p);
   if (std::fabs(det) < 1e-14) return false;

@@ -173,19 +173,30 @@ bool Bvh::any_hit(const Ray& ray, double t_limit, F&& hit_prim) const {
 
 // Triangle mesh with f32 storage and f64 intersection (mesh.rs:20-142).
 struct Mesh {
-  std::vector<float> positions;  // xyz triples
+  std::vector<float> positions;   // xyz triples
+  std::vector<float> positions1;  // deformation endpoint at shutter close (mesh.rs:31), empty: none
   std::vector<uint32_t> indices;
   Bvh blas;
   Aabb local_bounds;
 
+  // mesh.rs:48-84: triangle boxes cover both endpoints when pos1 is given
   void build(const float* pos, size_t nverts, const uint32_t* idx, size_t ntris,
-             const BvhParams& params);
+             const BvhParams& params, const float* pos1 = nullptr);
   Vec3 vertex(uint32_t i) const {
     return {(double)positions[3 * i], (double)positions[3 * i + 1], (double)positions[3 * i + 2]};
   }
+  // mesh.rs:96-111: the vertex at shutter time, lerped in f64 when the
+  // mesh deforms and time > 0
+  Vec3 vertex_at(uint32_t i, double time) const {
+    if (positions1.empty() || !(time > 0.0)) return vertex(i);
+    const float* a = &positions[3 * i];
+    const float* b = &positions1[3 * i];
+    return {(double)a[0] + ((double)b[0] - (double)a[0]) * time, (double)a[1] + ((double)b[1] - (double)a[1]) * time,
+            (double)a[2] + ((double)b[2] - (double)a[2]) * time};
+  }
   size_t triangle_count() const { return indices.size() / 3; }
   // mesh.rs:117-142 with the lower t bound as a parameter (render.rs uses
-  // 1e-9 there).
+  // 1e-9 there), at the ray's shutter time.
   bool intersect_triangle(const Ray& ray, uint32_t tri, double t_min, double t_max, double* t,
                           double* u, double* v) const;
 };

@@ -9,6 +9,11 @@
 //   (d'Eon et al. 2011 longitudinal lobes, trimmed-logistic azimuthal
 //   lobes, Fresnel + absorption attenuation), its lobe-selection pdf and
 //   sampling routine, and the fiber frame at a curve hit.
+//
+// Parity unpinned: HairParams::sigma_a_from_color (hair.rs:63-77, the
+// published fit of Chiang et al. 2016) is checked only for its fixed
+// point (white -> no absorption) and its monotonicity; nothing the
+// reference holds fixes its coefficients.
 #pragma once
 #include <array>
 #include <cstdint>

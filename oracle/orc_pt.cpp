@@ -758,6 +758,7 @@ bool Scene::instance_hit(const InstanceObj& inst, const Ray& ray, double t_max, 
                          Vec3* world_normal, Vec3* world_tangent, Intersection* st_out) const {
   const Mat4 inverse = inst.inverse_at(ray.time);
   Ray local = Ray::raw(inverse.transform_point(ray.origin), inverse.transform_vec(ray.direction));
+  local.time = ray.time;  // mesh.rs:319-323: the local ray keeps the shutter time
   if (world_tangent) *world_tangent = Vec3::zero();
   if (inst.curves) {  // mesh.rs:324-343: the closest segment, its normal and tangent
     const CurveSet& cs = curve_sets[inst.mesh];
@@ -804,16 +805,18 @@ bool Scene::instance_hit(const InstanceObj& inst, const Ray& ray, double t_max, 
   double tt;
   m.intersect_triangle(local, tri, 1e-9, INF, &tt, &hu, &hv);
   uint32_t i0 = m.indices[3 * tri], i1 = m.indices[3 * tri + 1], i2 = m.indices[3 * tri + 2];
+  // mesh.rs:199-220: vertex normals for the rest pose; a deforming mesh
+  // takes the geometric normal at the shutter time
   Vec3 ln;
-  if (!mo.normals.empty()) {
+  if (!mo.normals.empty() && m.positions1.empty()) {
     auto nv = [&](uint32_t i) {
       return Vec3{(double)mo.normals[3 * i], (double)mo.normals[3 * i + 1], (double)mo.normals[3 * i + 2]};
     };
     double w = 1.0 - hu - hv;
     ln = (nv(i0) * w + nv(i1) * hu + nv(i2) * hv).normalize();
   } else {
-    Vec3 v0 = m.vertex(i0);
-    ln = (m.vertex(i1) - v0).cross(m.vertex(i2) - v0).normalize();
+    Vec3 v0 = m.vertex_at(i0, ray.time);
+    ln = (m.vertex_at(i1, ray.time) - v0).cross(m.vertex_at(i2, ray.time) - v0).normalize();
   }
   *t_param = t;
   *world_normal = inverse.transform_normal(ln).normalize();
@@ -2275,7 +2278,7 @@ void Scene::load(const rrhip_scene_desc& d) {
   BvhParams bp;
   for (uint32_t i = 0; i < d.num_meshes; i++) {
     const rrhip_mesh_desc& md = d.meshes[i];
-    meshes[i].mesh.build(md.positions, md.num_vertices, md.indices, md.num_triangles, bp);
+    meshes[i].mesh.build(md.positions, md.num_vertices, md.indices, md.num_triangles, bp, md.positions1);
     if (md.normals) meshes[i].normals.assign(md.normals, md.normals + 3 * md.num_vertices);
     if (md.st) meshes[i].st.assign(md.st, md.st + 2 * md.num_vertices);
   }
@@ -2327,8 +2330,9 @@ void Scene::load(const rrhip_scene_desc& d) {
   for (uint32_t i = 0; i < d.num_patterns; i++) patterns[i] = pattern_from_desc(d.patterns[i]);
   for (uint32_t i = 0; i < d.num_bindings; i++)
     materials[d.bindings[i].material].bindings.push_back({(int)d.bindings[i].field, (int)d.bindings[i].pattern});
-  has_motion = camera.has_motion_inv;
+  has_motion = camera.has_motion_inv;  // scene_builder.rs:239-241
   for (const InstanceObj& in : instances) has_motion = has_motion || in.moving;
+  for (const MeshObj& mo : meshes) has_motion = has_motion || !mo.mesh.positions1.empty();
 }
 
 }  // namespace orc

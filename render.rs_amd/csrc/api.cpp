@@ -361,6 +361,7 @@ hipError_t upload_pt(rrhip_scene* s, const PtSceneHost& h) {
   if (e == hipSuccess) e = upload_vec(h.quadrics, &s->pt_mem[19], &v.quadrics);
   if (e == hipSuccess) e = upload_vec(h.inst_fwd, &s->pt_mem[20], &v.inst_fwd);
   if (e == hipSuccess) e = upload_vec(h.csegs, &s->pt_mem[21], &v.csegs);
+  if (e == hipSuccess) e = upload_vec(h.btris1, &s->pt_mem[35], &v.btris1);
   if (h.patterns) {  // pattern-bound materials: their data and the hits' texture coordinates
     if (e == hipSuccess) e = upload_vec(h.raw_pbr, &s->pt_mem[22], &v.raw_pbr);
     if (e == hipSuccess) e = upload_vec(h.binds, &s->pt_mem[23], &v.binds);

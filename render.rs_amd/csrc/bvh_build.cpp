@@ -240,7 +240,7 @@ struct Flattener {
 
 bool build_triangle_bvh(const float* positions, uint64_t nverts, const uint32_t* indices,
                         uint64_t ntris, const BuildParams& params, BuiltBvh* out,
-                        std::string* err) {
+                        std::string* err, const float* positions1) {
   out->nodes.clear();
   out->tris.clear();
   out->depth = 0;
@@ -256,7 +256,7 @@ bool build_triangle_bvh(const float* positions, uint64_t nverts, const uint32_t*
     return false;
   }
   for (uint64_t i = 0; i < 3 * nverts; i++) {
-    if (!std::isfinite(positions[i])) {
+    if (!std::isfinite(positions[i]) || (positions1 && !std::isfinite(positions1[i]))) {
       *err = "mesh positions must be finite";
       return false;
     }
@@ -273,6 +273,7 @@ bool build_triangle_bvh(const float* positions, uint64_t nverts, const uint32_t*
         return false;
       }
       b.grow(positions + 3 * (size_t)vi);
+      if (positions1) b.grow(positions1 + 3 * (size_t)vi);  // boxes cover both shutter ends
     }
     pb[t] = b;
     for (int a = 0; a < 3; a++) cen[3 * t + a] = 0.5f * (b.lo[a] + b.hi[a]);

@@ -37,9 +37,11 @@ void collapse_bvh4(const std::vector<Node2>& nodes, std::vector<Node4>* out, int
 
 // Builds over `ntris` triangles given by `indices` into `positions`
 // (xyz f32 triples). Returns false with `err` set on invalid input.
+// positions1: a deformation endpoint (same vertex count) whose triangles
+// the boxes also cover, or null.
 bool build_triangle_bvh(const float* positions, uint64_t nverts, const uint32_t* indices,
                         uint64_t ntris, const BuildParams& params, BuiltBvh* out,
-                        std::string* err);
+                        std::string* err, const float* positions1 = nullptr);
 
 // Builds over `n` axis-aligned boxes (6 floats each: lo xyz, hi xyz), e.g.
 // instance world bounds for a TLAS. Leaves reference slots of `order`,

@@ -305,7 +305,8 @@ bool build_instances(const rrhip_scene_desc& d, PtSceneHost* out, std::string* e
     if (k < d.num_meshes) {
       const rrhip_mesh_desc& m = d.meshes[k];
       if (m.num_triangles && (!m.positions || !m.indices)) return *err = "null mesh arrays", false;
-      if (!build_triangle_bvh(m.positions, m.num_vertices, m.indices, m.num_triangles, bp, &blas[k], err))
+      if (!build_triangle_bvh(m.positions, m.num_vertices, m.indices, m.num_triangles, bp, &blas[k], err,
+                              m.positions1))
         return false;
       geo_empty[k] = m.num_triangles == 0;
     } else {
@@ -341,6 +342,7 @@ bool build_instances(const rrhip_scene_desc& d, PtSceneHost* out, std::string* e
     g.mesh = geo;
     g.material = in.material;
     g.motion = 0xFFFFFFFFu;
+    g.deform = (!curves && d.meshes[in.mesh].positions1) ? 1u : 0u;
     if (in.moving) {
       for (int k = 0; k < 12; k++)
         if (!std::isfinite(in.object_to_world1[k])) return *err = "instance transform1 must be finite", false;
@@ -420,6 +422,20 @@ bool build_instances(const rrhip_scene_desc& d, PtSceneHost* out, std::string* e
     }
     out->btris.insert(out->btris.end(), blas[k].tris.begin(), blas[k].tris.end());
     const rrhip_mesh_desc& m = d.meshes[k];
+    for (const TriRec& r : blas[k].tris) {  // shutter-close records (rest copies for static meshes)
+      TriRec r1 = r;
+      if (m.positions1) {
+        const uint32_t* ix = m.indices + 3 * (size_t)r.prim;
+        const float* a = m.positions1 + 3 * (size_t)ix[0];
+        const float* b = m.positions1 + 3 * (size_t)ix[1];
+        const float* c = m.positions1 + 3 * (size_t)ix[2];
+        r1.v0x = a[0], r1.v0y = a[1], r1.v0z = a[2];
+        r1.v1x = b[0], r1.v1y = b[1], r1.v1z = b[2];
+        r1.v2x = c[0], r1.v2y = c[1], r1.v2z = c[2];
+      }
+      out->btris1.push_back(r1);
+    }
+    out->deform |= m.positions1 ? 1u : 0u;
     mi.idx_base = (uint32_t)out->mesh_idx.size();
     out->mesh_idx.insert(out->mesh_idx.end(), m.indices, m.indices + 3 * m.num_triangles);
     if (m.normals) {
@@ -431,6 +447,10 @@ bool build_instances(const rrhip_scene_desc& d, PtSceneHost* out, std::string* e
   }
   if (out->nodes.size() >= (1ull << 31) || out->btris.size() >= (1ull << 28) || out->csegs.size() >= (1ull << 28))
     return *err = "instanced scene too large", false;
+  if (out->deform)
+    out->motion = 1;  // scene_builder.rs:239-241: deforming meshes make the scene a motion scene
+  else
+    out->btris1.clear();
   // One entry per TLAS level (two at the last), one per BLAS level, and
   // the remainder of a TLAS leaf re-pushed while its first instance runs.
   out->stack_need = (uint32_t)(tlas_levels + blas_levels + 2);
@@ -490,6 +510,8 @@ bool build_patterns(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
     if (p.wrap > RRHIP_WRAP_BLACK) return *err = "unknown texture wrap mode", false;
     if (p.texture < -1 || p.texture >= ntex) return *err = "pattern texture out of range", false;
     if (p.num_stops && (!p.positions || !p.colors)) return *err = "ramp stops need positions and colors", false;
+    if (p.kind == RRHIP_PATTERN_FRACTAL && p.octaves > 64)  // each octave is a Perlin lookup per shading point
+      return *err = "fractal octaves must be at most 64", false;
     PtPattern g{};
     g.kind = p.kind;
     g.wrap = p.wrap;

@@ -24,13 +24,15 @@ struct PtSceneHost {
   // Instanced meshes (see PtSceneView).
   std::vector<Node2> nodes;
   std::vector<TriRec> btris;
+  std::vector<TriRec> btris1;  // deforming meshes: the records at shutter close (rest copies elsewhere)
   std::vector<PtInstance> insts;
   std::vector<PtMeshInfo> meshes;
   std::vector<uint32_t> mesh_idx;
   std::vector<float> mesh_nrm;
   uint32_t stack_need = 0;  // traversal stack entries the instance path needs
   uint32_t transparent = 0; // a material has an interior medium or presence < 1
-  uint32_t motion = 0;      // moving instances or camera motion
+  uint32_t motion = 0;      // moving instances, deforming meshes or camera motion
+  uint32_t deform = 0;      // some mesh deforms (btris1 holds the shutter-close records)
   std::vector<PtInstMotion> inst_motion;
   std::vector<float> inst_fwd;  // object-to-world of each instance in insts' order (12 floats)
   std::vector<CurveRec> csegs;  // curve segments of every curve set, BLAS leaf order

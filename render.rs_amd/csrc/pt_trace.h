@@ -89,6 +89,31 @@ __device__ __forceinline__ bool mesh_tri(const TriRec& tr, const Ray32& r, float
   return t > 1e-9f && t < tmax;
 }
 
+// A deforming mesh's triangle at shutter time t > 0 (mesh.rs:96-111): each
+// vertex lerped from the rest record a towards the shutter-close record b.
+__device__ __forceinline__ TriRec lerp_tri(const TriRec& a, const TriRec& b, float t) {
+  TriRec r = a;
+  r.v0x = a.v0x + (b.v0x - a.v0x) * t;
+  r.v0y = a.v0y + (b.v0y - a.v0y) * t;
+  r.v0z = a.v0z + (b.v0z - a.v0z) * t;
+  r.v1x = a.v1x + (b.v1x - a.v1x) * t;
+  r.v1y = a.v1y + (b.v1y - a.v1y) * t;
+  r.v1z = a.v1z + (b.v1z - a.v1z) * t;
+  r.v2x = a.v2x + (b.v2x - a.v2x) * t;
+  r.v2y = a.v2y + (b.v2y - a.v2y) * t;
+  r.v2z = a.v2z + (b.v2z - a.v2z) * t;
+  return r;
+}
+
+// The record instance I's traversal tests at slot k: the rest record, or
+// for a deforming mesh at time > 0 the lerped one.
+template <bool MOTION>
+__device__ __forceinline__ TriRec tri_at(const PtSceneView& S, bool deform, uint32_t k, float time) {
+  const TriRec a = S.btris[k];
+  if (MOTION && deform && time > 0.f) return lerp_tri(a, S.btris1[k], time);
+  return a;
+}
+
 // World-to-object transform of instance I at shutter time `time`
 // (mesh.rs:294-305): its stored inverse, or for a moving instance the
 // inverse of the element-wise lerp of its two transforms (identity when
@@ -141,7 +166,8 @@ __device__ __forceinline__ bool trace_instances(const PtSceneView& S, V3 o, V3 d
   int sp = 0, base = -1;
   uint32_t node = 0, inst = 0;
   bool have = true;
-  bool curve = false;  // the instance being traversed places a curve set
+  bool curve = false;   // the instance being traversed places a curve set
+  bool deform = false;  // MOTION: the instance being traversed deforms its mesh
   for (;;) {
     if (have) {
       if (COUNT) tc->nodes++;
@@ -163,7 +189,7 @@ __device__ __forceinline__ bool trace_instances(const PtSceneView& S, V3 o, V3 d
             float t, u = 0.f, v = 0.f;
             if (COUNT) tc->tris++;
             if (CURVES && curve ? curve_seg(S.csegs[k], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, tmax, t)
-                                : mesh_tri(S.btris[k], r, tmax, t, u, v)) {
+                                : mesh_tri(tri_at<MOTION>(S, deform, k, time), r, tmax, t, u, v)) {
               hit.t = t;
               hit.inst = inst;
               hit.slot = k;
@@ -204,6 +230,7 @@ __device__ __forceinline__ bool trace_instances(const PtSceneView& S, V3 o, V3 d
       const uint32_t root = S.meshes[I.mesh].root;
       if (root == 0xFFFFFFFFu) continue;
       if (CURVES) curve = S.meshes[I.mesh].curves != 0;
+      if (MOTION) deform = I.deform != 0;
       if (!MOTION || I.motion == 0xFFFFFFFFu) {
         r = make_ray32(xform_point(I.inv, o), xform_vec(I.inv, d));
       } else {
@@ -229,10 +256,10 @@ template <bool MOTION>
 __device__ __forceinline__ void instance_normal(const PtSceneView& S, const MeshHit& h, V3 d, float time, V3& n,
                                                 bool& front, uint32_t& material) {
   const PtInstance I = S.insts[h.inst];
-  const TriRec tr = S.btris[h.slot];
+  const TriRec tr = tri_at<MOTION>(S, I.deform != 0, h.slot, time);
   const PtMeshInfo mi = S.meshes[I.mesh];
   V3 ln;
-  if (mi.nrm_base != 0xFFFFFFFFu) {
+  if (mi.nrm_base != 0xFFFFFFFFu && !(MOTION && I.deform)) {  // a deforming mesh: geometric normals
     const uint32_t* ix = S.mesh_idx + mi.idx_base + 3 * (size_t)tr.prim;
     const float* nb = S.mesh_nrm + 3 * (size_t)mi.nrm_base;
     const float w = 1.f - h.u - h.v;
@@ -264,9 +291,10 @@ __device__ __forceinline__ void instance_normal(const PtSceneView& S, const Mesh
 template <bool MOTION>
 __device__ __forceinline__ void instance_spawn(const PtSceneView& S, const MeshHit& h, V3 d, float time, V3& p,
                                                V3& e) {
-  const TriRec tr = S.btris[h.slot];
-  float m[12];
   const PtInstance& I = S.insts[h.inst];
+  const bool lerped = MOTION && I.deform && time > 0.f;
+  const TriRec tr = tri_at<MOTION>(S, I.deform != 0, h.slot, time);
+  float m[12];
   if (!MOTION || I.motion == 0xFFFFFFFFu || !(time > 0.f)) {
     const float* f = S.inst_fwd + 12 * (size_t)h.inst;
 #pragma unroll
@@ -280,7 +308,13 @@ __device__ __forceinline__ void instance_spawn(const PtSceneView& S, const MeshH
   const V3 e1 = v3(tr.v1x, tr.v1y, tr.v1z) - v0, e2 = v3(tr.v2x, tr.v2y, tr.v2z) - v0;
   const V3 b = e1 * h.u, c = e2 * h.v;
   const V3 po = v0 + b + c;
-  const V3 eo = (vabs(v0) + vabs(b) + vabs(c)) * err_gamma(3);
+  V3 eo = (vabs(v0) + vabs(b) + vabs(c)) * err_gamma(3);
+  if (lerped) {  // each lerped vertex a + (b - a) t is within gamma(3) (2|a| + |b|) of the exact one
+    const TriRec ra = S.btris[h.slot], rb = S.btris1[h.slot];
+    const V3 va = vabs(v3(ra.v0x, ra.v0y, ra.v0z)) + vabs(v3(ra.v1x, ra.v1y, ra.v1z)) + vabs(v3(ra.v2x, ra.v2y, ra.v2z));
+    const V3 vb = vabs(v3(rb.v0x, rb.v0y, rb.v0z)) + vabs(v3(rb.v1x, rb.v1y, rb.v1z)) + vabs(v3(rb.v2x, rb.v2y, rb.v2z));
+    eo = eo + (va * 2.f + vb) * err_gamma(3);
+  }
   p = xform_point(m, po);
   const float am[12] = {fabsf(m[0]), fabsf(m[1]), fabsf(m[2]),  fabsf(m[3]), fabsf(m[4]),  fabsf(m[5]),
                         fabsf(m[6]), fabsf(m[7]), fabsf(m[8]), fabsf(m[9]), fabsf(m[10]), fabsf(m[11])};

@@ -171,7 +171,7 @@ struct alignas(16) PtInstance {
   uint32_t mesh;
   uint32_t material;
   uint32_t motion;  // index of its PtInstMotion, ~0u: static
-  uint32_t pad;
+  uint32_t deform;  // 1: its mesh deforms (vertices lerped towards btris1 at the shutter time)
 };
 
 // A moving instance (Instance::transform, transform1): object-to-world at
@@ -246,6 +246,7 @@ struct PtSceneView {
   // BLAS triangle records; instances in TLAS leaf order.
   const Node2* nodes;
   const TriRec* btris;
+  const TriRec* btris1;         // scenes with deforming meshes: every record at shutter close (null without)
   const PtInstance* insts;
   const PtMeshInfo* meshes;
   const uint32_t* mesh_idx;

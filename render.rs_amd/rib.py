@@ -402,6 +402,7 @@ class Builder:
         self.motion = None     # requests of an open MotionBegin block
         self.archive_depth = 0
         self.camera_motion_inv = None
+        self.pending_deform_p = None  # "P" of a deformation block's last sample
         # pattern graph (scene_builder.rs:1703-1848): textures (level 0 of each
         # .tex file, by path), nodes as scenes.Scene.add_pattern arguments,
         # handles, and the materials' bindings
@@ -702,6 +703,14 @@ class Builder:
                 self.xf_stack[-1] = t0
                 self.state.motion = (t0, t1)
             return
+        if all(n == block[0][0] for n, _ in block) and block[0][0] == "PointsPolygons":
+            # deformation motion: the last sample's "P" is the mesh's
+            # shutter-close endpoint (pending_deform_p)
+            if len(block) > 1:
+                self.pending_deform_p = _Params(block[-1][1], 2).numbers("P")
+            self.process(*block[0])
+            self.pending_deform_p = None
+            return
         self._skip(f'MotionBegin over "{block[0][0]}" is not supported by this backend')
         self.process(*block[0])
 
@@ -954,6 +963,9 @@ class Builder:
         ST = p.numbers("st")
         if ST is not None and len(ST) * 3 != len(P) * 2:  # one (s, t) per point, else none
             ST = None
+        P1 = self.pending_deform_p
+        if P1 is not None and len(P1) != len(P):  # an endpoint of another vertex count is dropped
+            P1 = None
         tri = []
         cur = 0
         for nv in nverts:
@@ -969,7 +981,8 @@ class Builder:
         mesh = len(self.meshes)
         self.meshes.append((np.array(P, np.float32).reshape(-1, 3), np.array(tri, np.uint32).reshape(-1, 3),
                             None if N is None else np.array(N, np.float32).reshape(-1, 3),
-                            None if ST is None else np.array(ST, np.float32).reshape(-1, 2)))
+                            None if ST is None else np.array(ST, np.float32).reshape(-1, 2),
+                            None if P1 is None else np.array(P1, np.float32).reshape(-1, 3)))
         mat = self._material()
         if self.state.area_light is not None:
             inten, col = self.state.area_light
@@ -1135,8 +1148,8 @@ class Builder:
             s.add_triangle(v0, v1, v2, mat)
         for kind, mat, m, kw in self.quadrics:
             s.add_quadric(kind, mat, m, **kw)
-        for pos, idx, nrm, st in self.meshes:
-            s.add_mesh(pos, idx, nrm, st)
+        for pos, idx, nrm, st, pos1 in self.meshes:
+            s.add_mesh(pos, idx, nrm, st, pos1)
         for mesh, mat, m, m1 in self.instances:
             s.add_instance(mesh, mat, m, m1)
         for p0, p1, v0, v1 in self.curve_sets:

@@ -66,7 +66,7 @@ class BuildParams(ctypes.Structure):
 class MeshDesc(ctypes.Structure):
     _fields_ = [("positions", ctypes.c_void_p), ("num_vertices", ctypes.c_uint64),
                 ("indices", ctypes.c_void_p), ("num_triangles", ctypes.c_uint64),
-                ("normals", ctypes.c_void_p), ("st", ctypes.c_void_p)]
+                ("normals", ctypes.c_void_p), ("st", ctypes.c_void_p), ("positions1", ctypes.c_void_p)]
 
 
 D3 = ctypes.c_double * 3
@@ -482,7 +482,7 @@ def _info_dict(i):
 def _mesh_desc(positions, indices, bins, max_leaf):
     pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
     idx = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
-    mesh = MeshDesc(_ptr(pos).value, pos.shape[0], _ptr(idx).value, idx.shape[0], None, None)
+    mesh = MeshDesc(_ptr(pos).value, pos.shape[0], _ptr(idx).value, idx.shape[0], None, None, None)
     desc = SceneDesc()
     desc.struct_size = ctypes.sizeof(SceneDesc)
     desc.num_meshes = 1

@@ -196,17 +196,21 @@ class Scene:
         s.material = mat
         self.spheres.append(s)
 
-    def add_mesh(self, positions, indices, normals=None, st=None):
+    def add_mesh(self, positions, indices, normals=None, st=None, positions1=None):
         """A mesh for instances; st: optional per-vertex texture
-        coordinates (n x 2)."""
+        coordinates (n x 2); positions1: optional vertices at shutter close
+        (deformation motion, same count as positions)."""
         pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
         idx = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
         nrm = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
         stv = None if st is None else np.ascontiguousarray(st, np.float32).reshape(-1, 2)
-        self._keep += [pos, idx, nrm, stv]
+        p1 = None if positions1 is None else np.ascontiguousarray(positions1, np.float32).reshape(-1, 3)
+        assert p1 is None or p1.shape == pos.shape
+        self._keep += [pos, idx, nrm, stv, p1]
         self.meshes.append(rrhip.MeshDesc(pos.ctypes.data, pos.shape[0], idx.ctypes.data, idx.shape[0],
                                           None if nrm is None else nrm.ctypes.data,
-                                          None if stv is None else stv.ctypes.data))
+                                          None if stv is None else stv.ctypes.data,
+                                          None if p1 is None else p1.ctypes.data))
         return len(self.meshes) - 1
 
     def add_texture(self, rgb):

@@ -170,6 +170,19 @@ def test_pattern_refusals(ctx):
     with pytest.raises(rrhip.RrhipError) as e:
         rrhip.PtScene(ctx, s2.desc())
     assert e.value.status == 5
+    # invalid descriptions: a texture index past the textures, a fractal
+    # of more than 64 octaves, a binding to no node
+    for bad in ("texture", "octaves", "binding"):
+        s3 = scenes.pattern_zoo(16, 16)
+        if bad == "texture":
+            s3.add_pattern(rrhip.PATTERN_TEXTURE, texture=len(s3.textures))
+        elif bad == "octaves":
+            s3.bind(1, "diffuseGain", s3.add_pattern(rrhip.PATTERN_FRACTAL, octaves=65))
+        else:
+            s3.bind(1, "diffuseGain", len(s3.patterns))
+        with pytest.raises(rrhip.RrhipError) as e:
+            rrhip.PtScene(ctx, s3.desc())
+        assert e.value.status == 1, bad  # RRHIP_ERR_INVALID_ARGUMENT
 
 
 def test_rib_still_life_matches_oracle(ctx, tmp_path):

@@ -316,15 +316,16 @@ def test_thin_lens_and_orthographic_cameras_match_oracle(ctx):
         rrhip.PtScene(ctx, s4.desc())
 
 
-@pytest.mark.parametrize("camera_motion", [False, True])
-def test_motion_blur_matches_shutter_fractions(ctx, camera_motion):
-    """Moving instance (mesh.rs:294-305) or moving camera (camera.rs:
-    134-146): each pixel sees the dome for 1 - the fraction of the shutter
-    the sliding black quad covers it."""
+@pytest.mark.parametrize("mode", ["instance", "camera", "deform"])
+def test_motion_blur_matches_shutter_fractions(ctx, mode):
+    """Moving instance (mesh.rs:294-305), moving camera (camera.rs:
+    134-146) or deforming mesh (mesh.rs:96-111): each pixel sees the dome
+    for 1 - the fraction of the shutter the sliding black quad covers it."""
     from test_oracle_pt import covered_fraction, moving_quad_scene
-    say(f"motion blur ({'camera' if camera_motion else 'instance'}): 32x4, 4096 spp vs shutter fractions")
+    say(f"motion blur ({mode}): 32x4, 4096 spp vs shutter fractions")
     W = 32
-    s = moving_quad_scene(W, 4, camera_motion)
+    camera_motion = mode == "camera"
+    s = moving_quad_scene(W, 4, camera_motion, deform=mode == "deform")
     acc, _ = gpu_render(ctx, s, rrhip.RenderParams.make(max_depth=2, firefly_clamp=0.0), 0, 4096)
     m = (acc[:, 0] / acc[:, 6]).reshape(4, W)
     x = ((np.arange(W) + 0.5) / W * 2 - 1) * 2.0
@@ -334,6 +335,43 @@ def test_motion_blur_matches_shutter_fractions(ctx, camera_motion):
     say(f"max |got - want| {np.abs(got - want).max():.4f}, max z {(np.abs(got - want) / np.maximum(se, 1e-9)).max():.2f}")
     assert np.array_equal(m[0], np.ones(W)) and np.array_equal(m[3], np.ones(W))
     assert np.all(np.abs(got - want) < 4.5 * se + 1e-6)
+
+
+def deforming_scene(width=64, height=64, offset=(0.0, 0.0, 0.0)):
+    """A displaced icosphere whose vertices swell by 1.35 and shift by 0.4
+    in x over the shutter (Mesh::positions1), carrying rest-pose vertex
+    normals that render.rs ignores while it deforms, rotated and scaled by
+    its instance, over a ground under a rect and a distant light."""
+    s = scenes.Scene(width, height, 40.0, eye=(0.0 + offset[0], 1.0 + offset[1], -5.0 + offset[2]),
+                     look_at=offset)
+    s.add_light(scenes.light(rrhip.LIGHT_RECT, intensity=7.0, p=(-1 + offset[0], 3 + offset[1], -1 + offset[2]),
+                             e1=(2, 0, 0), e2=(0, 0, 2)))
+    s.add_light(scenes.light(rrhip.LIGHT_DISTANT, intensity=1.0, p=(0.3, -1, 0.2)))
+    g = s.add_material(scenes.matte((0.6, 0.6, 0.6)))
+    o = np.asarray(offset)
+    s.add_polygon([tuple(o + (-6, -1.2, -6)), tuple(o + (6, -1.2, -6)), tuple(o + (6, -1.2, 6)),
+                   tuple(o + (-6, -1.2, 6))], g)
+    pos, idx, nrm = rrhip.gen_icosphere(2, 3)
+    pos1 = pos * 1.35 + np.array([0.4, 0.0, 0.0], np.float32)
+    mesh = s.add_mesh(pos, idx, nrm, positions1=pos1)
+    m = s.add_material(scenes.material(diffuse_color=(0.7, 0.3, 0.2), specular_roughness=0.25))
+    xf = scenes._place((0, 0, 0), 25, 0.9)
+    xf[:, 3] += o
+    s.add_instance(mesh, m, xf)
+    return s
+
+
+@pytest.mark.parametrize("offset", [0.0, 1e-4])
+def test_deforming_mesh_matches_oracle(ctx, offset):
+    """mesh.rs:96-111, 199-220: vertices lerped at each ray's shutter time,
+    geometric normals at that time, BLAS boxes over both shutter ends; the
+    GPU's error-bounded origins carry the lerp's rounding."""
+    say(f"deforming icosphere 64x64, 128 spp, ray_offset {offset}")
+    s = deforming_scene()
+    p = rrhip.RenderParams.make(max_depth=6, ray_offset=offset)
+    g, _ = gpu_render(ctx, s, p, 0, 128)
+    c, _ = oracle_render(s.desc(), p, 0, 128)
+    check_parity(g, c, label=f"deforming icosphere offset {offset}")
 
 
 def test_moving_instances_match_oracle(ctx):

@@ -307,20 +307,26 @@ def test_orthographic_rays_are_parallel_on_the_screen_window():
     assert np.allclose(o, want, atol=1e-9)
 
 
-def moving_quad_scene(w, h, camera_motion=False):
+def moving_quad_scene(w, h, camera_motion=False, deform=False):
     """An orthographic camera (window 2 x 1) looking down +z at a black
     unit quad instance at z = 2 whose centre slides from x = -1 (shutter
     open) to x = +1 (close) — or, with camera_motion, a static quad at the
-    origin seen by rays that camera motion shifts by the same amount —
-    against a unit dome: pixel x = X is covered during a known fraction of
-    the shutter."""
+    origin seen by rays that camera motion shifts by the same amount; or,
+    with deform, a static instance of a mesh whose vertices move by the
+    same amount (mesh.rs:96-111) — against a unit dome: pixel x = X is
+    covered during a known fraction of the shutter."""
     s = scenes.Scene(w, h, 40.0, eye=(0, 0, 0), look_at=(0, 0, 1))
     s.camera.projection = 1
     s.camera.ortho_half = (ctypes.c_double * 2)(2.0, 1.0)
     s.camera.filter_width = 1e-9
     s.add_light(scenes.light(rrhip.LIGHT_DOME, intensity=1.0))
     black = s.add_material(scenes.material(diffuse_gain=0.0, specular_ior=1.0))
-    quad = s.add_mesh([(-0.5, -0.5, 0), (0.5, -0.5, 0), (0.5, 0.5, 0), (-0.5, 0.5, 0)], [(0, 1, 2), (0, 2, 3)])
+    corners = np.array([(-0.5, -0.5, 0), (0.5, -0.5, 0), (0.5, 0.5, 0), (-0.5, 0.5, 0)])
+    if deform:
+        quad = s.add_mesh(corners + (-1, 0, 0), [(0, 1, 2), (0, 2, 3)], positions1=corners + (1, 0, 0))
+        s.add_instance(quad, black, [[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 2]])
+        return s
+    quad = s.add_mesh(corners, [(0, 1, 2), (0, 2, 3)])
     if camera_motion:
         s.add_instance(quad, black, [[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 2]])
         # camera at x = -1: the quad sits at image x = +1 at t = 0; rays at
@@ -345,13 +351,15 @@ def covered_fraction(x):
     return hi - lo
 
 
-@pytest.mark.parametrize("camera_motion", [False, True])
-def test_motion_blur_covers_pixels_for_their_shutter_fraction(camera_motion):
+@pytest.mark.parametrize("mode", ["instance", "camera", "deform"])
+def test_motion_blur_covers_pixels_for_their_shutter_fraction(mode):
     """mesh.rs:294-305 (instance transform lerped at the ray's shutter
-    time) and camera.rs:134-146 (camera motion): each pixel of a row sees
-    the dome for 1 - (fraction of the shutter the quad covers it)."""
+    time), camera.rs:134-146 (camera motion) and mesh.rs:96-111 (vertices
+    lerped towards positions1): each pixel of a row sees the dome for 1 -
+    (fraction of the shutter the quad covers it)."""
     W = 32
-    s = moving_quad_scene(W, 4, camera_motion)
+    camera_motion = mode == "camera"
+    s = moving_quad_scene(W, 4, camera_motion, deform=mode == "deform")
     acc, _ = orc.render(s.desc(), rrhip.RenderParams.make(max_depth=2, firefly_clamp=0.0), 0, 4096)
     m = (acc[:, 0] / acc[:, 6]).reshape(4, W)
     x = ((np.arange(W) + 0.5) / W * 2 - 1) * 2.0

@@ -558,3 +558,30 @@ def test_tex_writer_reproduces_reference_files(tmp_path, name):
     dst = str(tmp_path / name)
     rtex.write_tex(dst, rtex.read_tex(src))
     assert open(dst, "rb").read() == open(src, "rb").read()
+
+
+def test_motion_block_deforms_points_polygons():
+    # scene_builder.rs:1661-1672 and its transform / deformation motion test
+    # (1945-1987): a block of PointsPolygons gives the mesh its last sample's
+    # "P" as the shutter-close endpoint; a block whose samples differ in
+    # vertex count keeps the first sample static
+    b = build('''WorldBegin
+        AttributeBegin Translate 0 0 8
+          MotionBegin [0 1]
+            PointsPolygons [4] [0 1 2 3] "P" [-1 -1 0  1 -1 0  1 1 0  -1 1 0]
+            PointsPolygons [4] [0 1 2 3] "P" [-1 -1 2  1 -1 2  1 1 2  -1 1 2]
+          MotionEnd
+        AttributeEnd
+        MotionBegin [0 1]
+          PointsPolygons [3] [0 1 2] "P" [0 0 0 1 0 0 0 1 0]
+          PointsPolygons [3] [0 1 2] "P" [0 0 0 1 0 0 0 1 0 5 5 5]
+        MotionEnd
+        WorldEnd''')
+    pos, _, _, _, pos1 = b.meshes[0]
+    np.testing.assert_allclose(pos1[:, 2] - pos[:, 2], 2.0)
+    np.testing.assert_allclose(pos1[:, :2], pos[:, :2])
+    assert b.meshes[1][4] is None
+    s = b.scene()
+    # the oracle sees the quad at z = 8 at shutter open and z = 10 at close
+    t, *_ = orc.scene_intersect_n(s.desc(), [[0, 0, 0, 0, 0, 1]])
+    assert abs(t[0] - 8.0) < 1e-9
