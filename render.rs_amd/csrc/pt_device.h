@@ -129,11 +129,16 @@ __device__ __forceinline__ float fresnel_dielectric(float cos_i, float eta) {
   return 0.5f * (r_par * r_par + r_perp * r_perp);
 }
 
-// bxdf.rs:77-101
+// bxdf.rs:77-101. The denominator cos²(a² - 1) + 1 is evaluated as
+// sin² + cos² a² with sin² = (1 - cos)(1 + cos) (at least 0 for a
+// normalised h that rounds past 1): in f32, a² - 1 loses a² to rounding
+// for small roughness and becomes exactly -1 below roughness ~0.013 (a² is
+// 6e-10 at render.rs's floor of 0.005), which made D infinite for h on the
+// normal.
 __device__ __forceinline__ float ggx_d(V3 h, float alpha) {
   if (h.z <= 0.f) return 0.f;
   const float a2 = alpha * alpha;
-  const float d = h.z * h.z * (a2 - 1.f) + 1.f;
+  const float d = fmaf(h.z * h.z, a2, fmaxf((1.f - h.z) * (1.f + h.z), 0.f));
   return a2 / (kPi * d * d);
 }
 __device__ __forceinline__ float ggx_lambda(V3 w, float alpha) {

@@ -43,6 +43,7 @@ import rgbe
 import rrhip
 import rtex
 import scenes
+import tessellate
 
 
 class RibError(ValueError):
@@ -295,9 +296,8 @@ _NO_EFFECT = {"version", "WorldEnd", "FrameBegin", "FrameEnd", "Display", "Decla
               "Attribute"}
 
 # Requests render.rs implements that this backend does not.
-_UNSUPPORTED = {"SubdivisionMesh", "HierarchicalSubdivisionMesh", "PatchMesh",
-                "Patch", "NuPatch", "GeneralPolygon", "PointsGeneralPolygons", "Displace",
-                "Procedural"}
+_UNSUPPORTED = {"SubdivisionMesh", "HierarchicalSubdivisionMesh", "GeneralPolygon", "PointsGeneralPolygons",
+                "Displace", "Procedural"}
 
 # RiSpec cubic bases and their default steps (patches.rs:10-53); the
 # matrix M of p(u) = [u^3 u^2 u 1] M G, row-major.
@@ -333,6 +333,7 @@ class _State:
         self.motion = None      # (t0, t1) of the last transform motion block
         self.hair = None        # PxrMarschnerHair parameters as keyword arguments of scenes.hair
         self.bxdf_bindings = []  # PxrSurface "reference" connections: (field name, pattern node)
+        self.basis_u = _BASES["bezier"]
         self.basis_v = _BASES["bezier"]
         self.shading_rate = 1.0
 
@@ -681,6 +682,10 @@ class Builder:
             self._polygon(vals)
         elif name == "PointsPolygons":
             self._points_polygons(vals)
+        elif name in ("PatchMesh", "Patch"):
+            self._patch_mesh(name, vals)
+        elif name == "NuPatch":
+            self._nu_patch(vals)
         else:
             self._skip(f"request {name} is not implemented")
 
@@ -834,8 +839,8 @@ class Builder:
     def _basis(self, vals):
         """Basis <u> <ustep> <v> <vstep> (scene_builder.rs:576-601): values
         0 and 2 a name or a 16-number matrix, 1 and 3 the steps (default:
-        the named basis's, 1 for a matrix); only the v basis dices curves
-        here (the u basis dices patches, which this backend does not take)."""
+        the named basis's, 1 for a matrix); the v basis dices curves, both
+        dice bicubic patches."""
         def one(v, step):
             if v is None:
                 return None
@@ -849,6 +854,9 @@ class Builder:
             if nums is None or len(nums) != 16:
                 return None
             return np.asarray(nums, float).reshape(4, 4).tolist(), int(step) if step is not None else 1
+        a = one(vals[0] if len(vals) > 0 else None, _num(vals, 1))
+        if a is not None:
+            self.state.basis_u = a
         b = one(vals[2] if len(vals) > 2 else None, _num(vals, 3))
         if b is not None:
             self.state.basis_v = b
@@ -978,11 +986,18 @@ class Builder:
             cur += nv
         if not tri:
             return
+        self._add_mesh(np.array(P, np.float32).reshape(-1, 3), np.array(tri, np.uint32).reshape(-1, 3),
+                       None if N is None else np.array(N, np.float32).reshape(-1, 3),
+                       None if ST is None else np.array(ST, np.float32).reshape(-1, 2),
+                       None if P1 is None else np.array(P1, np.float32).reshape(-1, 3))
+
+    def _add_mesh(self, pos, tri, nrm, st, pos1=None):
+        """add_tessellated_mesh (scene_builder.rs:1195-1233): the mesh, and an
+        instance of it at the current transform with the current material
+        (emitting under an AreaLightSource), or an entry of the object
+        being defined."""
         mesh = len(self.meshes)
-        self.meshes.append((np.array(P, np.float32).reshape(-1, 3), np.array(tri, np.uint32).reshape(-1, 3),
-                            None if N is None else np.array(N, np.float32).reshape(-1, 3),
-                            None if ST is None else np.array(ST, np.float32).reshape(-1, 2),
-                            None if P1 is None else np.array(P1, np.float32).reshape(-1, 3)))
+        self.meshes.append((pos, tri, nrm, st, pos1))
         mat = self._material()
         if self.state.area_light is not None:
             inten, col = self.state.area_light
@@ -992,6 +1007,71 @@ class Builder:
         else:
             m1 = self._endpoint(self.xf)
             self.instances.append((mesh, mat, self.xf[:3, :4].copy(), None if m1 is None else m1[:3, :4].copy()))
+
+    # -- patches
+    def _dice_segments(self):
+        """dice_segments (scene_builder.rs:1173-1175): 16 / ShadingRate
+        spans per patch, 2..64."""
+        return int(min(max(16.0 / max(self.state.shading_rate, 0.05), 2.0), 64.0))
+
+    def _patch_mesh(self, name, vals):
+        """patch_mesh (scene_builder.rs:1324-1361): PatchMesh "type" nu uwrap
+        nv vwrap params, or Patch "type" params (one 2x2 / 4x4 grid);
+        bilinear or bicubic under the current Basis, diced by ShadingRate."""
+        ptype = _str(vals, 0)
+        if ptype is None:
+            return
+        if name == "Patch":
+            nu = nv = 2 if ptype == "bilinear" else 4
+            uw = vw = False
+            start = 1
+        else:
+            nu, nv = _num(vals, 1), _num(vals, 3)
+            if nu is None or nv is None:
+                return
+            nu, nv = int(nu), int(nv)
+            uw, vw = _str(vals, 2) == "periodic", _str(vals, 4) == "periodic"
+            start = 5
+        P = _Params(vals, start).numbers("P")
+        if P is None:
+            return self._warn(f'{name} without "P"; skipping')
+        segs = self._dice_segments()
+        if ptype == "bilinear":
+            m = tessellate.tessellate_bilinear(P, nu, nv, uw, vw, segs)
+        elif ptype == "bicubic":
+            (bu, us), (bv, vs) = self.state.basis_u, self.state.basis_v
+            m = tessellate.tessellate_bicubic(P, nu, nv, uw, vw, bu, us, bv, vs, segs)
+        else:
+            return self._warn(f'{name} type "{ptype}" not supported')
+        if m is None:
+            return self._warn(f"{name}: invalid control grid; skipping")
+        self._add_mesh(*m)
+
+    def _nu_patch(self, vals):
+        """nu_patch (scene_builder.rs:1363-1410): NuPatch nu uorder [uknot]
+        umin umax nv vorder [vknot] vmin vmax with "P" or homogeneous "Pw";
+        segments 16 / ShadingRate per knot span, 4..128; trim curves
+        ignored."""
+        nu, uo, umin, umax = _num(vals, 0), _num(vals, 1), _num(vals, 3), _num(vals, 4)
+        nv, vo, vmin, vmax = _num(vals, 5), _num(vals, 6), _num(vals, 8), _num(vals, 9)
+        uk = _as_numbers(vals[2]) if len(vals) > 2 else None
+        vk = _as_numbers(vals[7]) if len(vals) > 7 else None
+        if None in (nu, uo, umin, umax, nv, vo, vmin, vmax) or uk is None or vk is None:
+            return
+        p = _Params(vals, 10)
+        pts, rational = (p.numbers("Pw"), True) if p.numbers("Pw") is not None else (p.numbers("P"), False)
+        if pts is None:
+            return self._warn('NuPatch without "P"/"Pw"; skipping')
+        if p.get("trimcurve") is not None:
+            self._warn("NuPatch trim curves not supported yet; rendering untrimmed")
+        base = self._dice_segments()
+        nu, uo, nv, vo = int(nu), int(uo), int(nv), int(vo)
+        su = min(max(base * (max(nu - uo, 0) + 1), 4), 128)
+        sv = min(max(base * (max(nv - vo, 0) + 1), 4), 128)
+        m = tessellate.tessellate_nurbs(nu, uo, uk, umin, umax, nv, vo, vk, vmin, vmax, pts, rational, su, sv)
+        if m is None:
+            return self._warn("NuPatch: invalid control data; skipping")
+        self._add_mesh(*m)
 
     # -- patterns
     def _bxdf_bindings(self, p):

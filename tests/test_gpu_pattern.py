@@ -14,8 +14,10 @@ src/texture/pattern.rs, cache.rs; pt/mod.rs:389-440, 477-480, 685-689):
   after rough bounces, and pattern-driven roughness / colour feed the
   lobes, with error-bounded origins and with render.rs's fixed push;
 * a textured still life through the RIB front end (.tex files with UDIM
-  tiles, PointsPolygons st, Pattern requests, PxrSurface references): first
-  hits and 3 sigma as above;
+  tiles, PointsPolygons st, Pattern requests, PxrSurface references), and
+  render.rs's own stilllife.rib fixture (with its PatchMesh cloth and
+  NuPatch band, synthetic textures under its names): first hits and 3
+  sigma as above;
 * a checker-bound presence cuts holes for camera rays and for shadow rays
   at every blocker (first hits per pixel, 3 sigma);
 * radiance caches, batches and more than 15 bindings on a material are
@@ -229,3 +231,38 @@ def test_moving_patterned_geometry_first_hits(ctx):
         f"({(ids == 1).sum()} curve, {(ids == 2).sum()} quad)")
     assert (ids == 1).sum() > 50 and (ids == 2).sum() > 50
     assert (~ok).mean() <= 0.03
+
+
+def test_stilllife_fixture_matches_oracle(ctx, tmp_path):
+    # render.rs's own UDIM still life (tests/golden/rib/stilllife.rib:
+    # UDIM deck, graded label crate, triplanar ball with fractal
+    # roughness, a b-spline PatchMesh cloth with a softened checker, a
+    # NuPatch band with a ramp, a glass ball, the sky.hdr dome) through the
+    # RIB front end, with synthetic textures under the fixture's names
+    import os
+    import shutil
+
+    import rib
+    import rtex
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    shutil.copy(os.path.join(gold, "rib", "stilllife.rib"), tmp_path / "stilllife.rib")
+    shutil.copy(os.path.join(gold, "sky.hdr"), tmp_path / "sky.hdr")
+    for k, tile in enumerate((1001, 1002, 1011, 1012)):
+        rtex.write_tex(str(tmp_path / f"wood_{tile}.tex"),
+                       scenes.smooth_texture(64, 64, seed=20 + k, cells=8) * np.float32(0.8))
+    rtex.write_tex(str(tmp_path / "label.tex"), scenes.stripes_texture(64, 64, 8) * np.float32(0.7) +
+                   np.float32(0.15))
+    s, _ = rib.load(str(tmp_path / "stilllife.rib"))
+    s.camera.width, s.camera.height = 128, 64
+    say(f"stilllife.rib at 128x64: {len(s.meshes)} meshes, {len(s.patterns)} pattern nodes, "
+        f"{len(s.bindings)} bindings")
+    g = film_gpu(ctx, s, rrhip.RenderParams.make(max_depth=2), 8)
+    o = orc.render_film(s.desc(), rrhip.RenderParams.make(max_depth=2, ray_offset=1e-9), 0, 8)
+    same = (g[:, 16] == o[:, 16]) & (np.abs(g[:, 17] - o[:, 17]) < 0.5 / 8)
+    ok = same & (np.abs(g[:, 9:12] - o[:, 9:12]).max(1) < 2e-3)
+    say(f"stilllife first-hit albedo differs on {(~ok).sum()} of {len(ok)} pixels")
+    assert (~ok).mean() <= 0.03
+    p = rrhip.RenderParams.make(max_depth=8)
+    gr, _ = gpu_render(ctx, s, p, 0, 64)
+    c, _ = oracle_render(s.desc(), p, 0, 64)
+    parity(gr, c, "stilllife.rib")

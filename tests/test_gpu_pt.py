@@ -395,3 +395,27 @@ def test_moving_instances_match_oracle(ctx):
     acc_g, _ = gpu_render(ctx, s, p, 0, 128)
     acc_c, _ = oracle_render(s.desc(), p, 0, 128)
     check_parity(acc_g, acc_c, label="moving instances")
+
+
+def test_near_mirror_specular_stays_finite(ctx):
+    """PxrSurface roughness at render.rs's floor of 0.005 (alpha 2.5e-5):
+    half vectors on the normal must not make the GGX density infinite in
+    f32 (a regression test: a fractal-driven roughness reaching the floor
+    once produced a NaN pixel); parity with the oracle."""
+    s = scenes.Scene(48, 48, 40.0, eye=(0, 0.5, -4), look_at=(0, 0, 0))
+    s.add_light(scenes.light(rrhip.LIGHT_DOME, intensity=0.5))
+    s.add_light(scenes.light(rrhip.LIGHT_SPHERE, intensity=20.0, p=(1.5, 2.5, -2.0), radius=0.3))
+    mirror = s.add_material(scenes.material(diffuse_gain=0.0, specular_face_color=(0.95, 0.9, 0.8),
+                                            specular_roughness=0.005))
+    coat = s.add_material(scenes.material(diffuse_color=(0.2, 0.3, 0.6), specular_roughness=0.005,
+                                          clearcoat_gain=1.0, clearcoat_roughness=0.005))
+    s.add_sphere((-0.6, 0, 0), 0.55, mirror)
+    s.add_sphere((0.6, 0, 0), 0.55, coat)
+    ground = s.add_material(scenes.matte((0.5, 0.5, 0.5)))
+    s.add_polygon([(-5, -0.55, -5), (5, -0.55, -5), (5, -0.55, 5), (-5, -0.55, 5)], ground)
+    p = rrhip.RenderParams.make(max_depth=6)
+    g, _ = gpu_render(ctx, s, p, 0, 1024)
+    say(f"near-mirror: finite {np.isfinite(g).all()}")
+    assert np.isfinite(g).all()
+    c, _ = oracle_render(s.desc(), p, 0, 256)
+    check_parity(g, c, label="near-mirror spheres")

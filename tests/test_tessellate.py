@@ -1,0 +1,170 @@
+"""Patch tessellation of the RIB front end (render.rs_amd/tessellate.py)
+against render.rs src/geometry/patches.rs:
+
+* patches.rs's own tests: a Bézier sheet interpolates its corner control
+  points, bulges below its raised cage and dices to 81 vertices; a 7 x 4
+  B-spline grid (4 x 1 patches) is one welded sheet of 17 x 5 vertices
+  and 128 triangles; a flat quadratic NURBS net evaluates to its plane
+  within its hull and interpolates its clamped corners;
+* closed forms: a bicubic Bézier patch against de Casteljau's algorithm
+  in both directions, bilinear patches reproducing their corner lerps;
+  render.rs's NURBS basis recursion (patches.rs:228-260) worked by hand
+  for order 2 and 3 on clamped knots — it updates its values in place
+  from the span down, so it is not the textbook basis (order 2 gives
+  [t (1 - t), t] instead of [1 - t, t]) and the tessellation keeps that
+  to produce render.rs's surfaces;
+* the RIB requests: Basis u / v steps, Patch, periodic PatchMesh seams,
+  NuPatch "Pw", and the reference's stilllife.rib fixture building its
+  cloth and band from PatchMesh / NuPatch.
+"""
+import math
+import os
+import shutil
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "render.rs_amd"))
+
+import rib  # noqa: E402
+import tessellate as T  # noqa: E402
+
+BEZIER = rib._BASES["bezier"][0]
+BSPLINE = rib._BASES["b-spline"][0]
+
+
+def test_bezier_patch_interpolates_corners():
+    # patches.rs:341-361
+    pts = []
+    for j in range(4):
+        for i in range(4):
+            pts += [i, j, 1.0 if (1 <= i <= 2 and 1 <= j <= 2) else 0.0]
+    pos, tris, nrm, st = T.tessellate_bicubic(pts, 4, 4, False, False, BEZIER, 3, BEZIER, 3, 8)
+    assert abs(pos[0][0]) < 1e-5 and abs(pos[0][1]) < 1e-5
+    assert abs(pos[-1][0] - 3) < 1e-5 and abs(pos[-1][1] - 3) < 1e-5
+    assert 0.3 < pos[:, 2].max() < 1.0
+    assert len(pos) == 81
+
+
+def test_bspline_mesh_multiple_patches_no_cracks():
+    # patches.rs:363-377
+    pts = []
+    for j in range(4):
+        for i in range(7):
+            pts += [i, j, ((i * 3 + j) % 2) * 0.3]
+    pos, tris, nrm, st = T.tessellate_bicubic(pts, 7, 4, False, False, BSPLINE, 1, BSPLINE, 1, 4)
+    assert len(pos) == (4 * 4 + 1) * (1 * 4 + 1)
+    assert len(tris) == (4 * 4) * 4 * 2
+    # welded: every interior grid vertex is shared by six triangles
+    use = np.bincount(tris.ravel(), minlength=len(pos)).reshape(5, 17)
+    assert (use[1:-1, 1:-1] == 6).all()
+
+
+def test_nurbs_flat_sheet():
+    # patches.rs:379-416
+    uknot = [0, 0, 0, 0.5, 1, 1, 1]
+    vknot = [0, 0, 0, 1, 1, 1]
+    pts = [c for j in range(3) for i in range(4) for c in (i, j, 0.0)]
+    pos, tris, nrm, st = T.tessellate_nurbs(4, 3, uknot, 0, 1, 3, 3, vknot, 0, 1, pts, False, 8, 8)
+    assert np.abs(pos[:, 2]).max() < 1e-5
+    assert pos[:, 0].min() >= -1e-5 and pos[:, 0].max() <= 3 + 1e-4
+    assert abs(pos[0][0]) < 1e-5 and abs(pos[0][1]) < 1e-5
+
+
+def de_casteljau(ctrl, t):
+    p = [np.asarray(c, float) for c in ctrl]
+    while len(p) > 1:
+        p = [p[i] * (1 - t) + p[i + 1] * t for i in range(len(p) - 1)]
+    return p[0]
+
+
+def test_bicubic_bezier_against_de_casteljau():
+    rng = np.random.default_rng(3)
+    ctrl = rng.normal(size=(4, 4, 3))
+    segs = 6
+    pos, _, _, st = T.tessellate_bicubic(ctrl.ravel(), 4, 4, False, False, BEZIER, 3, BEZIER, 3, segs)
+    for k, (u, v) in enumerate(st.astype(float)):
+        rows = [de_casteljau(ctrl[j], u) for j in range(4)]
+        want = de_casteljau(rows, v)
+        np.testing.assert_allclose(pos[k], want, atol=2e-6)
+
+
+def test_bilinear_reproduces_corner_lerps():
+    ctrl = np.array([[0, 0, 0], [2, 0, 1], [0, 3, 0], [2, 3, -1]], float)
+    pos, tris, _, st = T.tessellate_bilinear(ctrl.ravel(), 2, 2, False, False, 4)
+    u, v = st[:, 0:1].astype(float), st[:, 1:2].astype(float)
+    want = (ctrl[0] * (1 - u) + ctrl[1] * u) * (1 - v) + (ctrl[2] * (1 - u) + ctrl[3] * u) * v
+    np.testing.assert_allclose(pos, want, atol=1e-6)
+    assert len(tris) == 4 * 4 * 2
+
+
+def test_nurbs_basis_follows_render_rs_recursion():
+    # order 2, knots 0 0 1 1: span 1; level 1 sets b1 = t, then b0 reads the
+    # already-raised b1: b0 = b1 (1 - t) = t (1 - t)
+    for t in (0.0, 0.2, 0.5, 0.9):
+        np.testing.assert_allclose(T.bspline_basis([0, 0, 1, 1], 2, 2, t), [t * (1 - t), t], atol=1e-15)
+    # order 3, knots 0 0 0 1 1 1: span 2; level 1: b2 = t, b1 = t (1 - t);
+    # level 2: b2 = t * t, b1 = b1 t + b2 (1 - t) = t (1 - t) t + t^2 (1 - t),
+    # b0 = b1 (1 - t) with the new b1
+    for t in (0.0, 0.3, 0.7, 1.0):
+        b2 = t * t
+        b1 = t * (1 - t) * t + b2 * (1 - t)
+        b0 = b1 * (1 - t)
+        np.testing.assert_allclose(T.bspline_basis([0, 0, 0, 1, 1, 1], 3, 3, t), [b0, b1, b2], atol=1e-15)
+    # hence a rational quadratic arc is not the textbook circle: points come
+    # out as the weighted sums of this basis
+    w = math.sqrt(0.5)
+    arc = [(1, 0, 1), (w, w, w), (0, 1, 1)]  # homogeneous (x w, y w, w)
+    pts = [c for j in range(2) for (x, y, ww) in arc for c in (x, y, j * ww, ww)]
+    pos, _, _, _ = T.tessellate_nurbs(3, 3, [0, 0, 0, 1, 1, 1], 0, 1, 2, 2, [0, 0, 1, 1], 0, 1, pts, True, 4, 2)
+    t = 0.5
+    bu = T.bspline_basis([0, 0, 0, 1, 1, 1], 3, 3, t)
+    bv = T.bspline_basis([0, 0, 1, 1], 2, 2, 0.5)
+    hom = np.array([[x, y, j * ww, ww] for j in range(2) for (x, y, ww) in arc]).reshape(2, 3, 4)
+    acc = sum(bv[j] * bu[i] * hom[j, i] for j in range(2) for i in range(3))
+    np.testing.assert_allclose(pos[1 * 5 + 2], acc[:3] / acc[3], atol=1e-6)
+
+
+def test_rib_patch_requests():
+    b = rib.Builder().run(rib.parse('''WorldBegin
+        ShadingRate 4
+        Basis "b-spline" 1 "catmull-rom" 1
+        PatchMesh "bicubic" 6 "periodic" 4 "nonperiodic"
+          "P" [''' + " ".join(f"{math.cos(i * math.pi / 3):.6f} {math.sin(i * math.pi / 3):.6f} {j}"
+                               for j in range(4) for i in range(6)) + ''']
+        Patch "bilinear" "P" [0 0 0  1 0 0  0 1 0  1 1 0]
+        NuPatch 3 3 [0 0 0 1 1 1] 0 1 2 2 [0 0 1 1] 0 1
+          "Pw" [1 0 0 1  0.7071068 0.7071068 0 0.7071068  0 1 0 1  1 0 1 1  0.7071068 0.7071068 0.7071068 0.7071068  0 1 1 1]
+        WorldEnd'''))
+    assert b.state.basis_u[1] == 1 and b.state.basis_v[1] == 1
+    (pos, tris, nrm, st, _), (pb, tb, _, _, _), (pn, tn, _, _, _) = b.meshes
+    segs = 4  # 16 / ShadingRate 4
+    # periodic in u: 6 patches x 4 spans around, no seam column; 1 patch in v
+    assert len(pos) == 6 * segs * (1 * segs + 1)
+    assert len(tris) == 6 * segs * segs * 2
+    r = np.sqrt(pos[:, 0] ** 2 + pos[:, 1] ** 2)
+    assert r.std() < 0.05  # a closed tube around the axis
+    assert len(pb) == (segs + 1) ** 2
+    assert len(pn) == 5 * 5  # order 3 / 2, one span each: 4 segments (the minimum) per direction
+    assert len(b.instances) == 3
+
+
+REF = "/root/reference/tests/fixtures/stilllife.rib"
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="the reference tree is not present")
+def test_stilllife_fixture_builds(tmp_path):
+    # render.rs's UDIM still life: its cloth is a b-spline PatchMesh and its
+    # band a NuPatch; with the fixture's textures beside it
+    for f in ("stilllife.rib", "sky.hdr", "label.tex", "wood_1001.tex", "wood_1002.tex", "wood_1011.tex",
+              "wood_1012.tex"):
+        shutil.copy(os.path.join(os.path.dirname(REF), f), tmp_path / f)
+    s, spp = rib.load(str(tmp_path / "stilllife.rib"))
+    assert len(s.textures) == 5 and len(s.patterns) == 8
+    assert len(s.meshes) == 4  # deck, crate, cloth, band
+    cloth, band = s.meshes[2], s.meshes[3]
+    assert cloth.num_vertices == (4 * 32 + 1) ** 2  # 7x7 b-spline grid: 4x4 patches, 32 spans (ShadingRate 0.5)
+    # NuPatch 7 x 3 of order 3: 32 x (7 - 3 + 1) segments clamped to 128 in u, 32 x 1 in v
+    assert band.num_vertices == (128 + 1) * (32 + 1)
