@@ -856,6 +856,15 @@ int rrhip_gen_density_grid(uint32_t n, uint64_t seed, float frequency, uint32_t 
 int rrhip_gen_icosphere(uint32_t levels, uint64_t seed, float amplitude, float frequency,
                         uint32_t octaves, float* positions, uint32_t* indices, float* normals);
 
+/* render.rs's fBm (src/geometry/displace.rs:60-97: sum over `octaves` of
+ * amp * perlin(p * freq + offset), amp *= gain, freq *= lacunarity, from
+ * frequency and amplitude 1) at n points (xyz, f64), on the host: the
+ * value RIB `Displace "noise"` moves tessellated vertices by (times its
+ * amplitude, along the smooth normals; displace.rs:101-123). Replaces
+ * displace_mesh's noise evaluation for callers that dice on the host. */
+int rrhip_gen_fbm(uint64_t n, const double* points, double frequency, uint32_t octaves, double gain,
+                  double lacunarity, const double offset[3], double* out);
+
 #ifdef __cplusplus
 }
 #endif

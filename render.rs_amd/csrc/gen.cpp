@@ -211,3 +211,14 @@ extern "C" int rrhip_gen_density_grid(uint32_t n, uint64_t seed, float frequency
   for (auto& th : pool) th.join();
   return RRHIP_OK;
 }
+
+extern "C" int rrhip_gen_fbm(uint64_t n, const double* points, double frequency, uint32_t octaves, double gain,
+                             double lacunarity, const double offset[3], double* out) {
+  if (n && (!points || !out)) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_gen_fbm: null buffer");
+  if (octaves > 64) return fail(RRHIP_ERR_INVALID_ARGUMENT, "rrhip_gen_fbm: at most 64 octaves");
+  const double zero[3] = {0.0, 0.0, 0.0};
+  const double* off = offset ? offset : zero;
+  for (uint64_t i = 0; i < n; i++)
+    out[i] = fbm3(points[3 * i], points[3 * i + 1], points[3 * i + 2], frequency, octaves, gain, lacunarity, off);
+  return RRHIP_OK;
+}

@@ -296,8 +296,7 @@ _NO_EFFECT = {"version", "WorldEnd", "FrameBegin", "FrameEnd", "Display", "Decla
               "Attribute"}
 
 # Requests render.rs implements that this backend does not.
-_UNSUPPORTED = {"SubdivisionMesh", "HierarchicalSubdivisionMesh", "GeneralPolygon", "PointsGeneralPolygons",
-                "Displace", "Procedural"}
+_UNSUPPORTED = {"GeneralPolygon", "PointsGeneralPolygons", "Procedural"}
 
 # RiSpec cubic bases and their default steps (patches.rs:10-53); the
 # matrix M of p(u) = [u^3 u^2 u 1] M G, row-major.
@@ -333,6 +332,7 @@ class _State:
         self.motion = None      # (t0, t1) of the last transform motion block
         self.hair = None        # PxrMarschnerHair parameters as keyword arguments of scenes.hair
         self.bxdf_bindings = []  # PxrSurface "reference" connections: (field name, pattern node)
+        self.displace = None    # Displace "noise" parameters (displace.rs DisplaceParams) for diced meshes
         self.basis_u = _BASES["bezier"]
         self.basis_v = _BASES["bezier"]
         self.shading_rate = 1.0
@@ -658,6 +658,8 @@ class Builder:
             st.bxdf_bindings = self._bxdf_bindings(p)
         elif name == "Pattern":
             self._pattern(vals)
+        elif name == "Displace":
+            self._displace(vals)
         elif name == "AreaLightSource":
             p = _Params(vals, _pstart(vals))
             st.area_light = (p.number("intensity", 1.0), p.color("lightcolor", (1.0, 1.0, 1.0)))
@@ -686,6 +688,10 @@ class Builder:
             self._patch_mesh(name, vals)
         elif name == "NuPatch":
             self._nu_patch(vals)
+        elif name in ("SubdivisionMesh", "HierarchicalSubdivisionMesh"):
+            if name == "HierarchicalSubdivisionMesh":
+                self._warn("HierarchicalSubdivisionMesh treated as SubdivisionMesh (string tags ignored)")
+            self._subdivision_mesh(vals)
         else:
             self._skip(f"request {name} is not implemented")
 
@@ -996,6 +1002,14 @@ class Builder:
         instance of it at the current transform with the current material
         (emitting under an AreaLightSource), or an entry of the object
         being defined."""
+        if self.state.displace is not None:  # displace_mesh (displace.rs:101-123)
+            dp = self.state.displace
+            base = nrm if nrm is not None else tessellate.smooth_normals(pos, tri)
+            d = rrhip.gen_fbm(pos.astype(np.float64), dp["frequency"], dp["octaves"], dp["gain"], dp["lacunarity"],
+                              dp["offset"]) * dp["amplitude"]
+            pos = (pos.astype(np.float64) + base.astype(np.float64) * d[:, None]).astype(np.float32)
+            nrm = tessellate.smooth_normals(pos, tri)
+            pos1 = None  # the displaced mesh is rebuilt without a deformation endpoint
         mesh = len(self.meshes)
         self.meshes.append((pos, tri, nrm, st, pos1))
         mat = self._material()
@@ -1071,6 +1085,82 @@ class Builder:
         m = tessellate.tessellate_nurbs(nu, uo, uk, umin, umax, nv, vo, vk, vmin, vmax, pts, rational, su, sv)
         if m is None:
             return self._warn("NuPatch: invalid control data; skipping")
+        self._add_mesh(*m)
+
+    def _displace(self, vals):
+        """Displace (scene_builder.rs:776-795): the built-in "noise"
+        displacement with DisplaceParams' defaults (amplitude 0.1, frequency
+        1, 4 octaves, gain 0.5, lacunarity 2, offset 0) for the meshes diced
+        under it."""
+        name = _str(vals, 0) or ""
+        if name != "noise":
+            self._warn(f'Displace "{name}": only the built-in "noise" displacement exists; using it')
+        p = _Params(vals, _pstart(vals))
+        off = p.numbers("offset")
+        self.state.displace = dict(amplitude=p.number("amplitude", 0.1), frequency=p.number("frequency", 1.0),
+                                   octaves=int(p.number("octaves", 4)), gain=p.number("gain", 0.5),
+                                   lacunarity=p.number("lacunarity", 2.0),
+                                   offset=tuple(off[:3]) if off is not None and len(off) >= 3 else (0.0, 0.0, 0.0))
+
+    def _subdiv_levels(self):
+        """subdiv_levels (scene_builder.rs:1178-1192): uniform depth from
+        ShadingRate."""
+        sr = self.state.shading_rate
+        return 5 if sr <= 0.15 else 4 if sr <= 0.35 else 3 if sr <= 1.5 else 2 if sr <= 6.0 else 1
+
+    def _subdivision_mesh(self, vals):
+        """subdivision_mesh (scene_builder.rs:1236-1322): SubdivisionMesh
+        "scheme" [nverts] [verts] (["tags"] [nargs] [ints] [floats])? params,
+        uniform Catmull-Clark with crease / corner / hole /
+        interpolateboundary tags (sharpness 10 when a tag has none)."""
+        scheme = _str(vals, 0)
+        if scheme is None:
+            return
+        if scheme != "catmull-clark":
+            self._warn(f'SubdivisionMesh scheme "{scheme}" treated as catmull-clark')
+        nverts = _as_numbers(vals[1]) if len(vals) > 1 else None
+        verts = _as_numbers(vals[2]) if len(vals) > 2 else None
+        if nverts is None or verts is None:
+            return
+        creases, corners, holes, interp = [], [], [], False
+        start = 3
+        if len(vals) > 3 and isinstance(vals[3], list):  # the tag block
+            tags = vals[3]
+            nargs = (_as_numbers(vals[4]) if len(vals) > 4 else None) or ()
+            ints = (_as_numbers(vals[5]) if len(vals) > 5 else None) or ()
+            floats = (_as_numbers(vals[6]) if len(vals) > 6 else None) or ()
+            ic = fc = 0
+            for ti, tag in enumerate(tags):
+                ni = int(nargs[2 * ti]) if 2 * ti < len(nargs) else 0
+                nf = int(nargs[2 * ti + 1]) if 2 * ti + 1 < len(nargs) else 0
+                ia = [int(x) for x in ints[min(ic, len(ints)):min(ic + ni, len(ints))]]
+                fa = list(floats[min(fc, len(floats)):min(fc + nf, len(floats))])
+                if tag == "crease":
+                    sharp = fa[0] if fa else 10.0
+                    creases += [(a, b, sharp) for a, b in zip(ia, ia[1:])]
+                elif tag == "corner":
+                    corners += [(v, fa[i] if i < len(fa) else (fa[0] if fa else 10.0)) for i, v in enumerate(ia)]
+                elif tag == "hole":
+                    holes += ia
+                elif tag == "interpolateboundary":
+                    interp = True
+                else:
+                    self._warn(f'SubdivisionMesh tag "{tag}" not supported; ignoring')
+                ic += ni
+                fc += nf
+            start = 7
+        P = _Params(vals, start).numbers("P")
+        if P is None:
+            return self._warn('SubdivisionMesh without "P"; skipping')
+        faces, cur = [], 0
+        for nv in nverts:
+            nv = int(nv)
+            if cur + nv > len(verts):
+                break
+            faces.append([int(x) for x in verts[cur:cur + nv]])
+            cur += nv
+        m = tessellate.tessellate_subdiv(np.asarray(P, np.float64).reshape(-1, 3), faces, creases, corners, holes,
+                                         interp, self._subdiv_levels())
         self._add_mesh(*m)
 
     # -- patterns

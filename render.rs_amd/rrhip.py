@@ -39,7 +39,7 @@ EXPORTED = [
     "rrhip_scene_get_tuning", "rrhip_scene_set_tuning", "rrhip_scene_autotune",
     "rrhip_device_copy",
     "rrhip_session_create", "rrhip_session_destroy", "rrhip_render_samples", "rrhip_read_accum",
-    "rrhip_load_accum", "rrhip_session_get_stats", "rrhip_gen_icosphere", "rrhip_relmse",
+    "rrhip_load_accum", "rrhip_session_get_stats", "rrhip_gen_icosphere", "rrhip_gen_fbm", "rrhip_relmse",
     "rrhip_nrc_create", "rrhip_nrc_destroy", "rrhip_nrc_get_info", "rrhip_nrc_get_params",
     "rrhip_nrc_set_params", "rrhip_nrc_infer", "rrhip_nrc_train_step", "rrhip_nrc_gradients",
     "rrhip_session_set_nrc", "rrhip_gen_density_grid",
@@ -406,6 +406,7 @@ def load():
         "rrhip_gen_density_grid": (i32, [u32, u64, ctypes.c_float, u32, ctypes.c_float, ctypes.c_float,
                                          ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), vp]),
         "rrhip_gen_icosphere": (i32, [u32, u64, ctypes.c_float, ctypes.c_float, u32, vp, vp, vp]),
+        "rrhip_gen_fbm": (i32, [u64, vp, ctypes.c_double, u32, ctypes.c_double, ctypes.c_double, vp, vp]),
         "rrhip_gen_heightfield": (i32, [u32, u32, u64, ctypes.c_float, ctypes.c_float, u32, vp, vp]),
         "rrhip_gen_query_rays": (i32, [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                        ctypes.c_float, u64, u64, vp]),
@@ -437,6 +438,17 @@ def gen_heightfield(nx, ny, seed, amplitude=0.1, frequency=4.0, octaves=6):
     idx = np.empty((nx * ny * 2, 3), np.uint32)
     _check(lib.rrhip_gen_heightfield(nx, ny, seed, amplitude, frequency, octaves, _ptr(pos), _ptr(idx)))
     return pos, idx
+
+
+def gen_fbm(points, frequency=1.0, octaves=4, gain=0.5, lacunarity=2.0, offset=(0.0, 0.0, 0.0)):
+    """render.rs's fBm (displace.rs) at points (n x 3), on the host."""
+    lib = load()
+    p = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    out = np.empty(p.shape[0], np.float64)
+    off = np.ascontiguousarray(offset, np.float64)
+    _check(lib.rrhip_gen_fbm(p.shape[0], _ptr(p), float(frequency), int(octaves), float(gain), float(lacunarity),
+                             _ptr(off), _ptr(out)))
+    return out
 
 
 def gen_icosphere(levels, seed, amplitude=0.15, frequency=3.0, octaves=4):

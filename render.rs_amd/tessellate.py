@@ -158,3 +158,116 @@ def tessellate_nurbs(nu, uorder, uknot, umin, umax, nv, vorder, vknot, vmin, vma
     st = np.stack(np.meshgrid(su, sv), -1).reshape(-1, 2).astype(np.float32)
     tris = _grid_tris(segs_u + 1, segs_v + 1, False, False)
     return positions, tris, smooth_normals(positions, tris), st
+
+
+def _subdivide_once(pos, faces, creases, corners, holes, interp):
+    """One Catmull-Clark round as subdiv.rs subdivide_once: face points,
+    edge points (boundary or sharpness >= 1: the midpoint; semi-sharp: a lerp
+    of the smooth point towards it), vertex points (corner >= 1: pinned;
+    interpolateboundary with two boundary edges, or two creases: the (6P +
+    A + B) / 8 crease rule, semi-sharp creases lerped from the smooth rule;
+    three creases: pinned; else (F + 2M + (n - 3) P) / n, corners lerped);
+    one quad per face corner, holes dropped; sharpnesses decay by one."""
+    key = lambda a, b: (a, b) if a < b else (b, a)
+    edge_faces, vedges, vfaces = {}, {}, {}
+    for fi, f in enumerate(faces):
+        for i in range(len(f)):
+            a, b = f[i], f[(i + 1) % len(f)]
+            k = key(a, b)
+            edge_faces.setdefault(k, []).append(fi)
+            for v in (a, b):
+                lst = vedges.setdefault(v, [])
+                if k not in lst:
+                    lst.append(k)
+            lst = vfaces.setdefault(a, [])
+            if fi not in lst:
+                lst.append(fi)
+    crease = {key(a, b): max(s, 0.0) for a, b, s in creases}
+    corner = {v: max(s, 0.0) for v, s in corners}
+    hole = set(h for h in holes if h < len(faces))
+    new = []
+    fpi = []
+    for f in faces:
+        new.append(pos[f].mean(0) if len(f) else np.zeros(3))
+        fpi.append(len(new) - 1)
+    epi = {}
+    for k, fl in edge_faces.items():
+        a, b = pos[k[0]], pos[k[1]]
+        mid = (a + b) * 0.5
+        sharp = crease.get(k, 0.0)
+        if len(fl) < 2 or sharp >= 1.0:
+            pt = mid
+        else:
+            fp = sum(new[fpi[f]] for f in fl)
+            smooth = (a + b + fp) * (1.0 / (2.0 + len(fl)))
+            pt = smooth + (mid - smooth) * min(sharp, 1.0) if sharp > 0.0 else smooth
+        new.append(pt)
+        epi[k] = len(new) - 1
+    vpi = []
+    for v in range(len(pos)):
+        p = pos[v]
+        edges = vedges.get(v, [])
+        vf = vfaces.get(v, [])
+        bnd = [e for e in edges if len(edge_faces.get(e, ())) < 2]
+        sharp_e = [(e, crease[e]) for e in edges if crease.get(e, 0.0) > 0.0]
+        cs = corner.get(v, 0.0)
+        other = lambda e: e[1] if e[0] == v else e[0]
+
+        def smooth_point():
+            n = max(len(edges), 1)
+            favg = sum((new[fpi[f]] for f in vf), np.zeros(3)) / max(len(vf), 1)
+            mavg = sum(((pos[e[0]] + pos[e[1]]) * 0.5 for e in edges), np.zeros(3)) / n
+            return (favg + mavg * 2.0 + p * (n - 3.0)) / n
+
+        def crease_point(pair):
+            return (p * 6.0 + pos[other(pair[0])] + pos[other(pair[1])]) / 8.0
+
+        if cs >= 1.0:
+            pt = p
+        elif interp and len(bnd) >= 2:
+            pt = crease_point(bnd[:2])
+        elif len(sharp_e) >= 3:
+            pt = p
+        elif len(sharp_e) == 2:
+            avg = (sharp_e[0][1] + sharp_e[1][1]) * 0.5
+            cp = crease_point([sharp_e[0][0], sharp_e[1][0]])
+            pt = cp if avg >= 1.0 else smooth_point() + (cp - smooth_point()) * avg
+        else:
+            sp = smooth_point()
+            pt = sp + (p - sp) * min(cs, 1.0) if cs > 0.0 else sp
+        new.append(pt)
+        vpi.append(len(new) - 1)
+    nf = []
+    for fi, f in enumerate(faces):
+        if fi in hole:
+            continue
+        n = len(f)
+        for i in range(n):
+            prev, cur, nxt = f[(i + n - 1) % n], f[i], f[(i + 1) % n]
+            nf.append([vpi[cur], epi[key(cur, nxt)], fpi[fi], epi[key(prev, cur)]])
+    nc = []
+    for a, b, s in creases:
+        ns = s - 1.0
+        if ns > 0.0 and key(a, b) in epi:
+            ep = epi[key(a, b)]
+            nc += [(vpi[a], ep, ns), (vpi[b], ep, ns)]
+    ncor = [(vpi[v], s if s >= 1e9 else s - 1.0) for v, s in corners if s - 1.0 > 0.0 or s >= 1e9]
+    return np.asarray(new, np.float64), nf, nc, ncor
+
+
+def tessellate_subdiv(positions, faces, creases=(), corners=(), holes=(), interpolate_boundary=False, levels=3):
+    """SubdivCage::tessellate (subdiv.rs:275-326): sharpnesses >= 10 become
+    infinite, `levels` uniform Catmull-Clark rounds, faces fanned into
+    triangles with smooth normals; no st."""
+    pos = np.asarray(positions, np.float64).reshape(-1, 3)
+    fl = [list(f) for f in faces]
+    cr = [(a, b, 1e9 if s >= 10.0 else s) for a, b, s in creases]
+    co = [(v, 1e9 if s >= 10.0 else s) for v, s in corners]
+    ho = list(holes)
+    for _ in range(levels):
+        pos, fl, cr, co = _subdivide_once(pos, fl, cr, co, ho, interpolate_boundary)
+        ho = []
+    p32 = pos.astype(np.float32)
+    tris = [[f[0], f[i], f[i + 1]] for f in fl for i in range(1, len(f) - 1)]
+    tris = np.asarray(tris, np.uint32).reshape(-1, 3)
+    return p32, tris, smooth_normals(p32, tris), None

@@ -15,9 +15,10 @@ src/texture/pattern.rs, cache.rs; pt/mod.rs:389-440, 477-480, 685-689):
   lobes, with error-bounded origins and with render.rs's fixed push;
 * a textured still life through the RIB front end (.tex files with UDIM
   tiles, PointsPolygons st, Pattern requests, PxrSurface references), and
-  render.rs's own stilllife.rib fixture (with its PatchMesh cloth and
-  NuPatch band, synthetic textures under its names): first hits and 3
-  sigma as above;
+  render.rs's own stilllife.rib, pan.rib and surfaces.rib fixtures (patches,
+  NURBS, subdivision surfaces with creases and displacement, moving and
+  deforming geometry; synthetic textures under their names): first hits
+  and 3 sigma as above;
 * a checker-bound presence cuts holes for camera rays and for shadow rays
   at every blocker (first hits per pixel, 3 sigma);
 * radiance caches, batches and more than 15 bindings on a material are
@@ -233,36 +234,47 @@ def test_moving_patterned_geometry_first_hits(ctx):
     assert (~ok).mean() <= 0.03
 
 
-def test_stilllife_fixture_matches_oracle(ctx, tmp_path):
-    # render.rs's own UDIM still life (tests/golden/rib/stilllife.rib:
-    # UDIM deck, graded label crate, triplanar ball with fractal
-    # roughness, a b-spline PatchMesh cloth with a softened checker, a
-    # NuPatch band with a ramp, a glass ball, the sky.hdr dome) through the
-    # RIB front end, with synthetic textures under the fixture's names
+@pytest.mark.parametrize("name", ["stilllife", "pan", "surfaces"])
+def test_reference_fixture_matches_oracle(ctx, tmp_path, name):
+    # render.rs's own RIB fixtures (tests/golden/rib) through the RIB front
+    # end with synthetic textures under their names and the sky.hdr dome:
+    # stilllife (UDIM deck, graded label crate, triplanar ball with fractal
+    # roughness, a b-spline PatchMesh cloth, a NuPatch band, glass), pan
+    # (a subdivided crate, moving and deforming geometry, UDIM wood) and
+    # surfaces (subdivision surfaces with creases and noise displacement,
+    # patches)
     import os
     import shutil
 
     import rib
     import rtex
     gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-    shutil.copy(os.path.join(gold, "rib", "stilllife.rib"), tmp_path / "stilllife.rib")
+    shutil.copy(os.path.join(gold, "rib", name + ".rib"), tmp_path / (name + ".rib"))
     shutil.copy(os.path.join(gold, "sky.hdr"), tmp_path / "sky.hdr")
     for k, tile in enumerate((1001, 1002, 1011, 1012)):
         rtex.write_tex(str(tmp_path / f"wood_{tile}.tex"),
                        scenes.smooth_texture(64, 64, seed=20 + k, cells=8) * np.float32(0.8))
     rtex.write_tex(str(tmp_path / "label.tex"), scenes.stripes_texture(64, 64, 8) * np.float32(0.7) +
                    np.float32(0.15))
-    s, _ = rib.load(str(tmp_path / "stilllife.rib"))
+    s, _ = rib.load(str(tmp_path / (name + ".rib")))
     s.camera.width, s.camera.height = 128, 64
-    say(f"stilllife.rib at 128x64: {len(s.meshes)} meshes, {len(s.patterns)} pattern nodes, "
+    say(f"{name}.rib at 128x64: {len(s.meshes)} meshes, {len(s.patterns)} pattern nodes, "
         f"{len(s.bindings)} bindings")
+    # first hits per pixel compare the same camera samples, which a box
+    # filter gives both sides: the gaussian filter's second uniform hashes
+    # the first one's low bits (f32 on the GPU, f64 in the oracle), so its
+    # per-sample offsets differ; the render below keeps the fixture's filter
+    kept = s.camera.filter, s.camera.filter_width
+    s.camera.filter, s.camera.filter_width = 0, 1.0
     g = film_gpu(ctx, s, rrhip.RenderParams.make(max_depth=2), 8)
     o = orc.render_film(s.desc(), rrhip.RenderParams.make(max_depth=2, ray_offset=1e-9), 0, 8)
+    s.camera.filter, s.camera.filter_width = kept
     same = (g[:, 16] == o[:, 16]) & (np.abs(g[:, 17] - o[:, 17]) < 0.5 / 8)
     ok = same & (np.abs(g[:, 9:12] - o[:, 9:12]).max(1) < 2e-3)
-    say(f"stilllife first-hit albedo differs on {(~ok).sum()} of {len(ok)} pixels")
+    say(f"{name} first-hit albedo differs on {(~ok).sum()} of {len(ok)} pixels")
     assert (~ok).mean() <= 0.03
     p = rrhip.RenderParams.make(max_depth=8)
     gr, _ = gpu_render(ctx, s, p, 0, 64)
+    assert np.isfinite(gr).all()
     c, _ = oracle_render(s.desc(), p, 0, 64)
-    parity(gr, c, "stilllife.rib")
+    parity(gr, c, f"{name}.rib")

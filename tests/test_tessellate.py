@@ -168,3 +168,124 @@ def test_stilllife_fixture_builds(tmp_path):
     assert cloth.num_vertices == (4 * 32 + 1) ** 2  # 7x7 b-spline grid: 4x4 patches, 32 spans (ShadingRate 0.5)
     # NuPatch 7 x 3 of order 3: 32 x (7 - 3 + 1) segments clamped to 128 in u, 32 x 1 in v
     assert band.num_vertices == (128 + 1) * (32 + 1)
+
+
+# ---- Catmull-Clark (subdiv.rs) -------------------------------------------------------
+
+CUBE = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]],
+                float)
+CUBE_FACES = [[0, 1, 2, 3], [5, 4, 7, 6], [4, 0, 3, 7], [1, 5, 6, 2], [3, 2, 6, 7], [4, 5, 1, 0]]
+RING = [(0, 1, 1e9), (1, 2, 1e9), (2, 3, 1e9), (3, 0, 1e9)]
+
+
+def test_cube_rounds_toward_sphere():
+    # subdiv.rs:393-410
+    pos, tris, _, _ = T.tessellate_subdiv(CUBE, CUBE_FACES, interpolate_boundary=True, levels=3)
+    assert len(tris) > 500
+    r = np.linalg.norm(pos.astype(float), axis=1)
+    assert r.max() < 1.5 and r.max() / r.min() < 1.35
+
+
+def test_infinite_crease_keeps_edge_sharp():
+    # subdiv.rs:412-440
+    pos, _, _, _ = T.tessellate_subdiv(CUBE, CUBE_FACES, RING, interpolate_boundary=True, levels=3)
+    on = np.abs(pos[:, 2] + 1) < 1e-4
+    assert on.sum() > 20
+    assert np.maximum(np.abs(pos[on, 0]), np.abs(pos[on, 1])).max() > 0.9
+
+
+def test_corner_tags_pin_vertices():
+    # subdiv.rs:442-460
+    pos, _, _, _ = T.tessellate_subdiv(CUBE, CUBE_FACES, RING, [(v, 1e9) for v in range(4)], interpolate_boundary=True,
+                                       levels=3)
+    pinned = ((np.abs(np.abs(pos[:, 0]) - 1) < 1e-5) & (np.abs(np.abs(pos[:, 1]) - 1) < 1e-5) &
+              (np.abs(pos[:, 2] + 1) < 1e-5))
+    assert pinned.sum() >= 4
+
+
+def test_hole_faces_removed():
+    # subdiv.rs:462-469
+    with_hole = len(T.tessellate_subdiv(CUBE, CUBE_FACES, holes=[0], interpolate_boundary=True, levels=1)[1])
+    without = len(T.tessellate_subdiv(CUBE, CUBE_FACES, interpolate_boundary=True, levels=1)[1])
+    assert with_hole < without
+
+
+def test_one_round_against_the_catmull_clark_rules():
+    # a closed cube, one round: face points are face centroids, edge points
+    # (a + b + f1 + f2) / 4, vertex points (F + 2M + (n - 3) P) / n with
+    # n = 3; the result has 24 quads over 26 vertices
+    pos, tris, _, _ = T.tessellate_subdiv(CUBE, CUBE_FACES, levels=1)
+    assert len(pos) == 6 + 12 + 8 and len(tris) == 48
+    p = pos.astype(float)
+    for c in ([0, 0, -1], [1, 0, 0], [0, 1, 0]):  # face points
+        assert np.min(np.linalg.norm(p - c, axis=1)) < 1e-6
+    # edge (0, 1): (-1,-1,-1) and (1,-1,-1) with faces z=-1 and y=-1
+    want_e = (np.array([-1, -1, -1]) + [1, -1, -1] + [0, 0, -1] + [0, -1, 0]) / 4
+    assert np.min(np.linalg.norm(p - want_e, axis=1)) < 1e-6
+    # vertex 0: F = average of 3 face points, M = average of 3 edge midpoints
+    F = (np.array([0, 0, -1]) + [-1, 0, 0] + [0, -1, 0]) / 3
+    M = (np.array([0, -1, -1]) + [-1, 0, -1] + [-1, -1, 0]) / 3
+    want_v = (F + 2 * M + 0 * CUBE[0]) / 3
+    assert np.min(np.linalg.norm(p - want_v, axis=1)) < 1e-6
+
+
+def test_rib_subdivision_mesh_tags():
+    b = rib.Builder().run(rib.parse('''WorldBegin ShadingRate 8
+        SubdivisionMesh "catmull-clark" [4 4 4 4 4 4] [0 1 2 3  5 4 7 6  4 0 3 7  1 5 6 2  3 2 6 7  4 5 1 0]
+          ["crease" "corner" "hole" "interpolateboundary"] [5 1  1 1  1 0  0 0] [0 1 2 3 0  0  5] [20 20]
+          "P" [-1 -1 -1  1 -1 -1  1 1 -1  -1 1 -1  -1 -1 1  1 -1 1  1 1 1  -1 1 1]
+        WorldEnd'''))
+    pos, tris, nrm, st, _ = b.meshes[0]
+    assert st is None and nrm is not None
+    # ShadingRate 8 -> one level; the hole face (5) drops its 4 quads
+    assert len(tris) == (6 - 1) * 4 * 2
+    pinned = np.all(np.abs(pos - [-1, -1, -1]) < 1e-6, axis=1)
+    assert pinned.any()  # vertex 0 is a corner of sharpness 20 (infinite)
+
+
+@pytest.mark.skipif(not os.path.exists("/root/reference/tests/fixtures/pan.rib"),
+                    reason="the reference tree is not present")
+def test_pan_fixture_builds(tmp_path):
+    # render.rs's pan.rib: a subdivided crate among moving and deforming
+    # geometry, UDIM wood and a label
+    src = os.path.dirname(REF)
+    for f in ("pan.rib", "sky.hdr", "label.tex", "wood_1001.tex", "wood_1002.tex", "wood_1011.tex",
+              "wood_1012.tex"):
+        if os.path.exists(os.path.join(src, f)):
+            shutil.copy(os.path.join(src, f), tmp_path / f)
+    s, _ = rib.load(str(tmp_path / "pan.rib"))
+    assert any(m.positions1 for m in s.meshes)  # the flapping banner deforms
+    assert any(i.moving for i in s.instances)
+
+
+def test_gen_fbm_equals_the_oracle():
+    # the host fBm the RIB Displace request uses equals the oracle's
+    # restatement of displace.rs (itself pinned by displace.rs's tests)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    import rrhip
+    p = np.random.default_rng(4).uniform(-5, 5, (500, 3))
+    got = rrhip.gen_fbm(p, 1.6, 5, 0.55, 2.0, (0.3, -1.2, 7.0))
+    want = orc.fbm(p, 1.6, 5, 0.55, 2.0, (0.3, -1.2, 7.0))
+    np.testing.assert_array_equal(got, want)
+
+
+def test_rib_displace_moves_vertices_along_normals():
+    # displace_mesh (displace.rs:101-123): each vertex moves by amplitude *
+    # fbm(p) along its smooth normal; the attribute block scopes it
+    import rrhip
+    text = '''WorldBegin ShadingRate 8
+        AttributeBegin
+        Displace "noise" "amplitude" [0.3] "frequency" [1.5] "octaves" [3]
+        SubdivisionMesh "catmull-clark" [4 4 4 4 4 4] [0 1 2 3  5 4 7 6  4 0 3 7  1 5 6 2  3 2 6 7  4 5 1 0]
+          "P" [-1 -1 -1  1 -1 -1  1 1 -1  -1 1 -1  -1 -1 1  1 -1 1  1 1 1  -1 1 1]
+        AttributeEnd
+        SubdivisionMesh "catmull-clark" [4 4 4 4 4 4] [0 1 2 3  5 4 7 6  4 0 3 7  1 5 6 2  3 2 6 7  4 5 1 0]
+          "P" [-1 -1 -1  1 -1 -1  1 1 -1  -1 1 -1  -1 -1 1  1 -1 1  1 1 1  -1 1 1]
+        WorldEnd'''
+    b = rib.Builder().run(rib.parse(text))
+    (pd, td, nd, _, _), (p0, t0, n0, _, _) = b.meshes
+    d = rrhip.gen_fbm(p0.astype(float), 1.5, 3, 0.5, 2.0) * 0.3
+    np.testing.assert_allclose(pd, (p0.astype(float) + n0.astype(float) * d[:, None]).astype(np.float32), atol=1e-6)
+    np.testing.assert_array_equal(td, t0)
+    assert np.abs(pd - p0).max() > 1e-3
