@@ -21,15 +21,20 @@ Curves (linear, or cubic under the current v Basis, diced by ShadingRate)
 and Points become curve sets placed by instances; `Bxdf
 "PxrMarschnerHair"` makes hair fibers of them (scene_builder.rs:685-697,
 1050-1170).
-Requests the GPU path has no counterpart for (subdivision surfaces,
-patches, displacement, pattern graphs, procedural density, deformation
-motion) raise `RibUnsupported` by default; with
-`strict=False` they are skipped with one warning each, as render.rs skips
-requests it does not implement (a motion block over other requests then
-keeps its first sample, as render.rs does). Transform motion blocks move
-the mesh instances created under them, and before WorldBegin the camera
-(scene_builder.rs:272-287, 1637-1692). Inline archives (ArchiveBegin /
-ReadArchive) and file archives are replayed.
+Patterns (Pattern requests and PxrSurface "reference" connections) build
+the pattern graph and its bindings; .tex files are read by `rtex`.
+PatchMesh / Patch / NuPatch, SubdivisionMesh, GeneralPolygon /
+PointsGeneralPolygons and Displace "noise" are diced or triangulated on
+the host (`tessellate`) into meshes. Requests the GPU path has no
+counterpart for (Procedural, the procedural fBm density of media)
+raise `RibUnsupported` by default; with `strict=False` they are skipped
+with one warning each, as render.rs skips requests it does not implement
+(a motion block over other requests then keeps its first sample, as
+render.rs does). Transform motion blocks move the mesh instances created
+under them, and before WorldBegin the camera; PointsPolygons /
+PointsGeneralPolygons motion blocks give the mesh a shutter-close
+endpoint (scene_builder.rs:272-287, 1637-1692). Inline archives
+(ArchiveBegin / ReadArchive) and file archives are replayed.
 Requests that carry no render effect (Display, Declare, ShadingRate,
 Sides, ...) are accepted silently.
 """
@@ -296,7 +301,7 @@ _NO_EFFECT = {"version", "WorldEnd", "FrameBegin", "FrameEnd", "Display", "Decla
               "Attribute"}
 
 # Requests render.rs implements that this backend does not.
-_UNSUPPORTED = {"GeneralPolygon", "PointsGeneralPolygons", "Procedural"}
+_UNSUPPORTED = {"Procedural"}
 
 # RiSpec cubic bases and their default steps (patches.rs:10-53); the
 # matrix M of p(u) = [u^3 u^2 u 1] M G, row-major.
@@ -684,6 +689,10 @@ class Builder:
             self._polygon(vals)
         elif name == "PointsPolygons":
             self._points_polygons(vals)
+        elif name == "GeneralPolygon":
+            self._general_polygon(vals)
+        elif name == "PointsGeneralPolygons":
+            self._points_general_polygons(vals)
         elif name in ("PatchMesh", "Patch"):
             self._patch_mesh(name, vals)
         elif name == "NuPatch":
@@ -714,11 +723,12 @@ class Builder:
                 self.xf_stack[-1] = t0
                 self.state.motion = (t0, t1)
             return
-        if all(n == block[0][0] for n, _ in block) and block[0][0] == "PointsPolygons":
+        if all(n == block[0][0] for n, _ in block) and block[0][0] in ("PointsPolygons", "PointsGeneralPolygons"):
             # deformation motion: the last sample's "P" is the mesh's
             # shutter-close endpoint (pending_deform_p)
             if len(block) > 1:
-                self.pending_deform_p = _Params(block[-1][1], 2).numbers("P")
+                lead = 2 if block[0][0] == "PointsPolygons" else 3
+                self.pending_deform_p = _Params(block[-1][1], lead).numbers("P")
             self.process(*block[0])
             self.pending_deform_p = None
             return
@@ -993,6 +1003,94 @@ class Builder:
         if not tri:
             return
         self._add_mesh(np.array(P, np.float32).reshape(-1, 3), np.array(tri, np.uint32).reshape(-1, 3),
+                       None if N is None else np.array(N, np.float32).reshape(-1, 3),
+                       None if ST is None else np.array(ST, np.float32).reshape(-1, 2),
+                       None if P1 is None else np.array(P1, np.float32).reshape(-1, 3))
+
+    def _general_polygon(self, vals):
+        """general_polygon (scene_builder.rs:1413-1446): GeneralPolygon
+        [nverts per loop] "P", the first loop the outline and the rest
+        holes, triangulated with its holes, smooth normals, no st."""
+        nloops = _as_numbers(vals[0]) if len(vals) > 0 else None
+        if nloops is None:
+            return
+        P = _Params(vals, 1).numbers("P")
+        if P is None:
+            return self._warn('GeneralPolygon without "P"; skipping')
+        pos = np.asarray(P, np.float64)[:len(P) // 3 * 3].reshape(-1, 3)
+        loops, cur = [], 0
+        for nv in nloops:
+            nv = int(nv)
+            if cur + nv > len(pos):
+                break
+            loops.append(list(range(cur, cur + nv)))
+            cur += nv
+        if not loops or len(loops[0]) < 3:
+            return
+        tri = tessellate.triangulate_loops(pos, loops)
+        if tri is None or len(tri) == 0:
+            return self._warn("GeneralPolygon: triangulation failed; skipping")
+        p32 = pos.astype(np.float32)
+        self._add_mesh(p32, tri, tessellate.smooth_normals(p32, tri), None)
+
+    def _points_general_polygons(self, vals):
+        """PointsGeneralPolygons [nloops] [nverts] [vertices] params: a
+        polygon of one loop is fanned as PointsPolygons fans it (the
+        reference's handling, scene_builder.rs:796-810); a polygon with holes
+        is triangulated with them, where render.rs instead fills each loop
+        as a polygon of its own (with a warning). "P", "N", "st" and a
+        deformation endpoint are per point, as for PointsPolygons."""
+        nloops = _as_numbers(vals[0]) if len(vals) > 0 else None
+        nverts = _as_numbers(vals[1]) if len(vals) > 1 else None
+        idx = _as_numbers(vals[2]) if len(vals) > 2 else None
+        if nloops is None or nverts is None or idx is None:
+            return
+        P = _Params(vals, 3).numbers("P")
+        if P is None:
+            return self._skip('PointsGeneralPolygons without "P"')
+        pos = np.asarray(P, np.float64)[:len(P) // 3 * 3].reshape(-1, 3)
+        npts = len(pos)
+        polys, li, cur = [], 0, 0
+        for nl in nloops:
+            loops = []
+            for _ in range(int(nl)):
+                if li >= len(nverts):
+                    break
+                nv = int(nverts[li])
+                li += 1
+                if cur + nv > len(idx):
+                    break
+                loops.append([int(i) for i in idx[cur:cur + nv]])
+                cur += nv
+            if loops:
+                polys.append(loops)
+        if any(v < 0 or v >= npts for loops in polys for lp in loops for v in lp):
+            return self._warn("PointsGeneralPolygons: vertex index out of range; skipping")
+        tri = []
+        for loops in polys:
+            if len(loops[0]) < 3:
+                continue
+            if len(loops) == 1:
+                lp = loops[0]
+                tri += [(lp[0], lp[i], lp[i + 1]) for i in range(1, len(lp) - 1)]
+            else:
+                t = tessellate.triangulate_loops(pos, loops)
+                if t is None:
+                    self._warn("PointsGeneralPolygons: triangulation failed; polygon skipped")
+                    continue
+                tri += [tuple(int(x) for x in r) for r in t]
+        if not tri:
+            return
+        p = _Params(vals, 3)
+        N, ST = p.numbers("N"), p.numbers("st")
+        if N is not None and len(N) != len(P):
+            N = None
+        if ST is not None and len(ST) * 3 != len(P) * 2:
+            ST = None
+        P1 = self.pending_deform_p
+        if P1 is not None and len(P1) != len(P):
+            P1 = None
+        self._add_mesh(pos.astype(np.float32), np.asarray(tri, np.uint32).reshape(-1, 3),
                        None if N is None else np.array(N, np.float32).reshape(-1, 3),
                        None if ST is None else np.array(ST, np.float32).reshape(-1, 2),
                        None if P1 is None else np.array(P1, np.float32).reshape(-1, 3))

@@ -1,7 +1,9 @@
-"""Host tessellation of RenderMan patches into meshes (render.rs
-src/geometry/patches.rs): bilinear and bicubic PatchMesh / Patch under
-RiSpec basis matrices, and NuPatch NURBS surfaces by Cox-de Boor basis
-functions (trim curves ignored, as render.rs does). Each returns
+"""Host tessellation of RenderMan surfaces into meshes (render.rs
+src/geometry/patches.rs, subdiv.rs, earclip.rs): bilinear and bicubic
+PatchMesh / Patch under RiSpec basis matrices, NuPatch NURBS surfaces by
+render.rs's basis recursion (trim curves ignored, as render.rs does),
+Catmull-Clark subdivision surfaces, and planar polygons with holes
+(`triangulate_loops`). Each returns
 (positions (n, 3) f32, triangles (m, 3) u32, smooth normals (n, 3) f32, st
 (n, 2) f32); grid vertices are shared across patch boundaries, so sheets
 are welded. Patches are diced once at scene build; the GPU traces the
@@ -271,3 +273,104 @@ def tessellate_subdiv(positions, faces, creases=(), corners=(), holes=(), interp
     tris = [[f[0], f[i], f[i + 1]] for f in fl for i in range(1, len(f) - 1)]
     tris = np.asarray(tris, np.uint32).reshape(-1, 3)
     return p32, tris, smooth_normals(p32, tris), None
+
+
+def _cross2(o, a, b):
+    return (a[0] - o[0]) * (b[1] - o[1]) - (a[1] - o[1]) * (b[0] - o[0])
+
+
+def _proper_cross(a, b, c, d):
+    """Segments ab and cd cross at a point interior to both."""
+    d1, d2 = _cross2(c, d, a), _cross2(c, d, b)
+    d3, d4 = _cross2(a, b, c), _cross2(a, b, d)
+    return d1 * d2 < 0.0 and d3 * d4 < 0.0
+
+
+def triangulate_loops(points, loops):
+    """Triangles covering a planar polygon given as loops of vertex indices
+    into `points` (n, 3): the first loop is the outline, the others holes
+    (GeneralPolygon, scene_builder.rs:1413-1446, earclip.rs). The loops are
+    projected on the plane that drops the largest component of the
+    outline's Newell normal, cyclically ((y, z), (z, x) or (x, y)), the
+    outline turned counter-clockwise and the holes clockwise there; each
+    hole, in order of its rightmost x, is joined to the ring by a two-way
+    bridge from that rightmost vertex to the nearest ring vertex whose
+    segment properly crosses no edge of the ring or of any hole; the ring
+    is then cut into ears (convex corners whose triangle holds no other
+    ring vertex, copies of its own corners aside), resuming the scan next
+    to the last ear, and what is left when no ear is found is fanned.
+    Triangles are (m, 3) indices into `points` (bridges reuse vertices),
+    counter-clockwise in the projection as render.rs emits them."""
+    p = np.asarray(points, np.float64).reshape(-1, 3)
+    outline = [int(i) for i in loops[0]]
+    nrm = np.zeros(3)
+    for i in range(len(outline)):  # Newell's normal
+        a, b = p[outline[i]], p[outline[(i + 1) % len(outline)]]
+        nrm += ((a[1] - b[1]) * (a[2] + b[2]), (a[2] - b[2]) * (a[0] + b[0]), (a[0] - b[0]) * (a[1] + b[1]))
+    ax = int(np.argmax(np.abs(nrm)))  # ties go to the lower axis
+    q = p[:, [(ax + 1) % 3, (ax + 2) % 3]]
+
+    def area(loop):
+        xy = q[loop]
+        return 0.5 * float(np.sum(xy[:, 0] * np.roll(xy[:, 1], -1) - np.roll(xy[:, 0], -1) * xy[:, 1]))
+
+    ring = outline if area(outline) >= 0.0 else outline[::-1]
+    holes = []
+    for h in loops[1:]:
+        h = [int(i) for i in h]
+        if len(h) >= 3:
+            holes.append(h if area(h) <= 0.0 else h[::-1])
+    holes.sort(key=lambda h: -q[h, 0].max())
+    for k, h in enumerate(holes):
+        m = int(np.argmax(q[h, 0]))
+        mp = q[h[m]]
+        edges = [(ring[i], ring[(i + 1) % len(ring)]) for i in range(len(ring))]
+        for o in holes[k:]:
+            edges += [(o[i], o[(i + 1) % len(o)]) for i in range(len(o))]
+        order = np.argsort(((q[ring] - mp) ** 2).sum(1), kind="stable")
+        bridge = None
+        for ri in order:
+            rp = q[ring[ri]]
+            if not any(_proper_cross(rp, mp, q[e0], q[e1]) for e0, e1 in edges):
+                bridge = int(ri)
+                break
+        if bridge is None:
+            return None
+        ring = ring[:bridge + 1] + h[m:] + h[:m + 1] + ring[bridge:]
+    n = len(ring)
+    if n < 3:
+        return None
+    xy = q[ring]
+    alive = list(range(n))
+    tris = []
+    start = 0
+    while len(alive) > 3:
+        m = len(alive)
+        found = -1
+        for step in range(m):
+            i = (start + step) % m
+            ia, ib, ic = alive[i - 1], alive[i], alive[(i + 1) % m]
+            a, b, c = xy[ia], xy[ib], xy[ic]
+            if _cross2(a, b, c) <= 1e-12:
+                continue  # reflex or flat corner
+            rest = xy[[j for j in alive if j not in (ia, ib, ic)]]
+            if len(rest):
+                same = np.zeros(len(rest), bool)
+                for v in (a, b, c):
+                    same |= (np.abs(rest - v) < 1e-12).all(1)
+                d1 = (b[0] - a[0]) * (rest[:, 1] - a[1]) - (b[1] - a[1]) * (rest[:, 0] - a[0])
+                d2 = (c[0] - b[0]) * (rest[:, 1] - b[1]) - (c[1] - b[1]) * (rest[:, 0] - b[0])
+                d3 = (a[0] - c[0]) * (rest[:, 1] - c[1]) - (a[1] - c[1]) * (rest[:, 0] - c[0])
+                inside = (d1 >= 0.0) & (d2 >= 0.0) & (d3 >= 0.0)
+                if (inside & ~same).any():
+                    continue
+            found = i
+            break
+        if found < 0:
+            break
+        tris.append((ring[alive[found - 1]], ring[alive[found]], ring[alive[(found + 1) % m]]))
+        del alive[found]
+        start = max(found - 1, 0)
+    if len(alive) >= 3:
+        tris += [(ring[alive[0]], ring[alive[i]], ring[alive[i + 1]]) for i in range(1, len(alive) - 1)]
+    return np.asarray(tris, np.uint32).reshape(-1, 3)

@@ -131,3 +131,39 @@ def test_motion_blocks_match_oracle(ctx):
     assert s.camera.has_motion_inv == 1 and s.instances[0].moving == 1
     p = rrhip.RenderParams.make(max_depth=6)
     parity(gpu_render(ctx, s, p, 0, 64), oracle_render(s.desc(), p, 0, 64)[0], "motion")
+
+
+POLYGON_RIB = '''
+Format 48 36 1
+Projection "perspective" "fov" [45]
+WorldBegin
+  Light "PxrDomeLight" "sky" "intensity" [0.6]
+  AttributeBegin
+    Translate 0 3 3 Rotate 90 1 0 0 Scale 2 2 1
+    Light "PxrRectLight" "key" "intensity" [6]
+  AttributeEnd
+  Bxdf "PxrSurface" "floor" "diffuseColor" [0.6 0.6 0.6] "specularIor" [1]
+  Polygon "P" [-4 -1 1  4 -1 1  4 -1 9  -4 -1 9]
+  AttributeBegin
+    Bxdf "PxrSurface" "frame" "diffuseColor" [0.8 0.3 0.1] "specularIor" [1.5] "specularRoughness" [0.3]
+    GeneralPolygon [6 4] "P" [-1.6 -1 5  1.6 -1 5  1.6 1.4 5  0.3 1.4 5  0.3 0.6 5  -1.6 0.6 5
+                              -1 -0.6 5  0.8 -0.6 5  0.8 0.3 5  -1 0.3 5]
+  AttributeEnd
+  AttributeBegin
+    Bxdf "PxrSurface" "plate" "diffuseColor" [0.2 0.5 0.8]
+    Translate 0.9 -0.3 3.2 Rotate 35 0 1 0
+    PointsGeneralPolygons [2] [4 4] [0 1 2 3  4 5 6 7]
+      "P" [-0.5 -0.5 0  0.5 -0.5 0  0.5 0.5 0  -0.5 0.5 0  -0.2 -0.2 0  0.2 -0.2 0  0.2 0.2 0  -0.2 0.2 0]
+  AttributeEnd
+WorldEnd
+'''
+
+
+def test_polygons_with_holes_match_oracle(ctx):
+    # a concave frame (GeneralPolygon) and a plate with a square window
+    # (PointsGeneralPolygons): light reaches the floor through both holes
+    say("RIB GeneralPolygon / PointsGeneralPolygons with holes, 48x36, 64 spp")
+    s, _ = rib.loads(POLYGON_RIB)
+    assert len(s.meshes) == 2
+    p = rrhip.RenderParams.make(max_depth=6)
+    parity(gpu_render(ctx, s, p, 0, 64), oracle_render(s.desc(), p, 0, 64)[0], "polygons with holes")

@@ -15,7 +15,14 @@ against render.rs src/geometry/patches.rs:
   to produce render.rs's surfaces;
 * the RIB requests: Basis u / v steps, Patch, periodic PatchMesh seams,
   NuPatch "Pw", and the reference's stilllife.rib fixture building its
-  cloth and band from PatchMesh / NuPatch.
+  cloth and band from PatchMesh / NuPatch;
+* subdivision (subdiv.rs): one round against the Catmull-Clark rules,
+  infinite creases and corners, holes, the pan.rib crate;
+* polygons with holes (earclip.rs's own L and square-with-hole cases by
+  closed-form area; random concave outlines with holes on tilted planes
+  covered exactly once per point inside, brute-force even-odd, nothing
+  outside), GeneralPolygon / PointsGeneralPolygons requests and their
+  deformation blocks.
 """
 import math
 import os
@@ -289,3 +296,117 @@ def test_rib_displace_moves_vertices_along_normals():
     np.testing.assert_allclose(pd, (p0.astype(float) + n0.astype(float) * d[:, None]).astype(np.float32), atol=1e-6)
     np.testing.assert_array_equal(td, t0)
     assert np.abs(pd - p0).max() > 1e-3
+
+
+def _tri_areas(p, t):
+    p = np.asarray(p, np.float64)
+    a, b, c = p[t[:, 0]], p[t[:, 1]], p[t[:, 2]]
+    return 0.5 * np.linalg.norm(np.cross(b - a, c - a), axis=1)
+
+
+def _even_odd(xy, loops, pts):
+    """Brute-force point-in-polygon over all loops (even-odd rule)."""
+    inside = np.zeros(len(pts), bool)
+    for lp in loops:
+        v = xy[lp]
+        w = np.roll(v, -1, axis=0)
+        for (x0, y0), (x1, y1) in zip(v, w):
+            cross = (y0 > pts[:, 1]) != (y1 > pts[:, 1])
+            xint = x0 + (pts[:, 1] - y0) * (x1 - x0) / np.where(y1 != y0, y1 - y0, 1.0)
+            inside ^= cross & (pts[:, 0] < xint)
+    return inside
+
+
+def _covered(xy, tris, pts):
+    """Number of triangles (2D) holding each point."""
+    n = np.zeros(len(pts), int)
+    for a, b, c in xy[tris]:
+        d1 = (b[0] - a[0]) * (pts[:, 1] - a[1]) - (b[1] - a[1]) * (pts[:, 0] - a[0])
+        d2 = (c[0] - b[0]) * (pts[:, 1] - b[1]) - (c[1] - b[1]) * (pts[:, 0] - b[0])
+        d3 = (a[0] - c[0]) * (pts[:, 1] - c[1]) - (a[1] - c[1]) * (pts[:, 0] - c[0])
+        n += ((d1 > 0) & (d2 > 0) & (d3 > 0)) | ((d1 < 0) & (d2 < 0) & (d3 < 0))
+    return n
+
+
+def test_concave_outline_closed_form():
+    # earclip.rs's own case: an L of area 3 cut into 4 triangles, all
+    # counter-clockwise in the plane
+    p = np.array([[0, 0, 0], [2, 0, 0], [2, 1, 0], [1, 1, 0], [1, 2, 0], [0, 2, 0]], float)
+    t = T.triangulate_loops(p, [list(range(6))])
+    assert t.shape == (4, 3)
+    assert abs(_tri_areas(p, t).sum() - 3.0) < 1e-12
+    e = p[t]
+    assert (np.cross(e[:, 1] - e[:, 0], e[:, 2] - e[:, 0])[:, 2] > 0).all()
+
+
+def test_square_with_hole_closed_form():
+    # earclip.rs's own case: a 4x4 square less a centred 2x2 hole, area 12;
+    # no triangle lies in the hole and bridges reuse the loops' vertices
+    p = np.array([[0, 0, 0], [4, 0, 0], [4, 4, 0], [0, 4, 0], [1, 1, 0], [3, 1, 0], [3, 3, 0], [1, 3, 0]], float)
+    t = T.triangulate_loops(p, [[0, 1, 2, 3], [4, 5, 6, 7]])
+    assert abs(_tri_areas(p, t).sum() - 12.0) < 1e-9
+    assert len(t) == 8  # n + 2h - 2 triangles for 8 vertices, one hole
+    c = p[t].mean(1)
+    assert not ((c[:, 0] > 1) & (c[:, 0] < 3) & (c[:, 1] > 1) & (c[:, 1] < 3)).any()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_polygons_with_holes_cover_exactly(seed):
+    # star-shaped concave outlines with up to three disjoint square holes,
+    # on a tilted plane: the triangles cover each sample point inside the
+    # polygon (even-odd over all loops) exactly once, and nothing outside;
+    # areas add up to outline minus holes
+    rng = np.random.default_rng(seed)
+    k = 7 + seed * 3
+    ang = np.sort(rng.uniform(0, 2 * np.pi, k))
+    rad = rng.uniform(4.0, 8.0, k)
+    xy = np.stack([rad * np.cos(ang), rad * np.sin(ang)], 1)
+    loops = [list(range(k))]
+    holes = [(-1.5, -1.5), (1.2, 0.3), (-0.8, 1.6)][:seed % 4]
+    for cx, cy in holes:
+        s = 0.45
+        base = len(xy)
+        xy = np.vstack([xy, [[cx - s, cy - s], [cx + s, cy - s], [cx + s, cy + s], [cx - s, cy + s]]])
+        loops.append(list(range(base, base + 4)))
+    if seed % 2:
+        loops[0] = loops[0][::-1]  # clockwise outline
+    # embed in a plane whose normal has a negative dominant component
+    rot = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+    p3 = np.column_stack([xy, np.zeros(len(xy))]) @ rot.T + rng.normal(size=3)
+    t = T.triangulate_loops(p3, loops)
+    assert t is not None and len(t) == sum(len(lp) for lp in loops) + 2 * (len(loops) - 1) - 2
+    a = sum(abs(0.5 * np.sum(xy[lp, 0] * np.roll(xy[lp, 1], -1) - np.roll(xy[lp, 0], -1) * xy[lp, 1]))
+            for lp in loops[:1]) - len(holes) * 0.9 ** 2
+    assert abs(_tri_areas(p3, t).sum() - a) < 1e-9
+    pts = rng.uniform(-8.5, 8.5, (4000, 2))
+    inside = _even_odd(xy, loops, pts)
+    np.testing.assert_array_equal(_covered(xy, t, pts), inside.astype(int))
+
+
+def test_rib_general_polygon_requests():
+    b = rib.Builder().run(rib.parse('''WorldBegin
+        GeneralPolygon [4 4] "P" [0 0 0  4 0 0  4 4 0  0 4 0   1 1 0  3 1 0  3 3 0  1 3 0]
+        PointsGeneralPolygons [1 2] [4 4 4] [0 1 2 3  4 5 6 7  8 9 10 11]
+          "P" [0 0 1  1 0 1  1 1 1  0 1 1   0 0 2  4 0 2  4 4 2  0 4 2   1 1 2  3 1 2  3 3 2  1 3 2]
+          "st" [0 0  1 0  1 1  0 1   0 0  1 0  1 1  0 1   .25 .25  .75 .25  .75 .75  .25 .75]
+        PointsPolygons [4] [0 1 2 3] "P" [0 0 1  1 0 1  1 1 1  0 1 1]
+        WorldEnd'''))
+    (pg, tg, ng, sg, _), (pp, tp, npg, sp, _), (p1, t1, _, _, _) = b.meshes
+    assert sg is None and ng is not None
+    assert abs(_tri_areas(pg, tg).sum() - 12.0) < 1e-6
+    assert sp is not None and sp.shape == (12, 2)
+    # a one-loop polygon is fanned exactly as PointsPolygons fans it
+    np.testing.assert_array_equal(tp[:2], t1)
+    assert abs(_tri_areas(pp, tp[2:]).sum() - 12.0) < 1e-6
+    assert len(b.instances) == 3
+
+
+def test_rib_points_general_polygons_deformation_block():
+    b = rib.Builder().run(rib.parse('''WorldBegin
+        MotionBegin [0 1]
+          PointsGeneralPolygons [1] [3] [0 1 2] "P" [0 0 0  1 0 0  0 1 0]
+          PointsGeneralPolygons [1] [3] [0 1 2] "P" [0 0 1  1 0 1  0 1 1]
+        MotionEnd
+        WorldEnd'''))
+    pos, tri, _, _, pos1 = b.meshes[0]
+    np.testing.assert_array_equal(pos1, pos + np.float32([0, 0, 1]))
