@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define RRHIP_ABI_VERSION 9u
+#define RRHIP_ABI_VERSION 10u
 #define RRHIP_NO_HIT 0xFFFFFFFFu
 
 /* Status codes. render.rs returns anyhow::Result with a message
@@ -259,21 +259,30 @@ typedef struct rrhip_camera_desc {
  * spheres (render.rs Scene::objects, src/scene/mod.rs:27-56). */
 /* A participating medium (render.rs Medium, src/scene/medium.rs:28-65):
  * absorption and scattering coefficients, Henyey-Greenstein anisotropy,
- * emission, extent along any ray, and an optional density field. render.rs
- * evaluates a procedural fBm per lookup; here the density is a voxel grid
- * (nx x ny x nz samples, x fastest, values in [0, 1], trilinear between
- * samples placed on the corners of [grid_lo, grid_hi]; zero outside).
- * nx = 0: homogeneous. */
+ * emission, extent along any ray, and an optional density field, either
+ * - a voxel grid (nx x ny x nz samples, x fastest, values in [0, 1],
+ *   trilinear between samples placed on the corners of [grid_lo, grid_hi];
+ *   zero outside), or
+ * - procedural = RRHIP_MEDIUM_FBM: render.rs's cloud field
+ *   DensityField::Fbm (medium.rs:16-27, 55-63), evaluated per lookup at the
+ *   world point p: rho = clamp((fbm(p) * 0.5 + 0.5 - (1 - coverage)) *
+ *   sharpness, 0, 1), fbm with frequency, max(octaves, 1) (at most 64),
+ *   gain, lacunarity and offset 0 (render.rs's RIB defaults: 0.3, 5, 0.55,
+ *   2, coverage 0.5, sharpness 4); nx must be 0.
+ * nx = 0 and procedural = 0: homogeneous. */
+#define RRHIP_MEDIUM_FBM 1u
 typedef struct rrhip_medium_desc {
   double sigma_a[3];
   double sigma_s[3];
   double g;
   double emission[3];
   double max_distance;  /* <= 0: unbounded */
-  uint32_t nx, ny, nz, reserved;
+  uint32_t nx, ny, nz, procedural;
   const float* density;
   double grid_lo[3];
   double grid_hi[3];
+  uint32_t octaves, reserved;
+  double frequency, gain, lacunarity, coverage, sharpness;
 } rrhip_medium_desc;
 
 /* A texture: linear RGB f32 texels of level 0, rows from the top

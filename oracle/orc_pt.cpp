@@ -1344,8 +1344,18 @@ double Medium::majorant() const {
   return std::fmax(std::fmax(st.x, st.y), st.z);
 }
 
-// Trilinear over the vertex grid, zero outside; 1 when homogeneous.
+// medium.rs:55-63 for the fBm cloud field (offset 0, at the world point);
+// trilinear over the vertex grid, zero outside; 1 when homogeneous.
 double Medium::density_at(const Vec3& p) const {
+  if (fbm_octaves > 0) {
+    FbmParams fp;
+    fp.frequency = fbm_frequency;
+    fp.octaves = fbm_octaves;
+    fp.gain = fbm_gain;
+    fp.lacunarity = fbm_lacunarity;
+    const double n = fbm(p.x, p.y, p.z, fp) * 0.5 + 0.5;
+    return std::clamp((n - (1.0 - fbm_coverage)) * fbm_sharpness, 0.0, 1.0);
+  }
   if (nx == 0) return 1.0;
   const double pc[3] = {p.x, p.y, p.z};
   const uint32_t n[3] = {nx, ny, nz};
@@ -1399,7 +1409,7 @@ MediumEvent sample_distance(const Medium& m, const Vec3& o, const Vec3& d, doubl
                             Pcg32& rng) {
   const Vec3 st = m.sigma_t();
   MediumEvent ev;
-  if (m.nx == 0) {  // volume.rs:60-102
+  if (!m.heterogeneous()) {  // volume.rs:60-102
     const double tm = std::fmin(t_max, m.max_distance);
     if (avg3(st) <= 1e-9) return ev;
     const Vec3 b{std::fmax(beta.x, 0.0), std::fmax(beta.y, 0.0), std::fmax(beta.z, 0.0)};
@@ -1443,7 +1453,7 @@ MediumEvent sample_distance(const Medium& m, const Vec3& o, const Vec3& d, doubl
 // volume.rs:153-189
 Vec3 medium_transmittance(const Medium& m, const Vec3& o, const Vec3& d, double dist, Pcg32& rng) {
   const Vec3 st = m.sigma_t();
-  if (m.nx == 0) return exp3(st, std::fmin(dist, m.max_distance));
+  if (!m.heterogeneous()) return exp3(st, std::fmin(dist, m.max_distance));
   const double maj = m.majorant();
   double t, t_end;
   Vec3 tr{1, 1, 1};
@@ -2144,6 +2154,14 @@ void Scene::load(const rrhip_scene_desc& d) {
     m.emission = v3(md.emission);
     m.g = md.g;
     m.max_distance = md.max_distance > 0.0 ? md.max_distance : 1e300;
+    if (md.procedural == RRHIP_MEDIUM_FBM) {  // the device keeps the parameters in f32
+      m.fbm_octaves = std::max(md.octaves, 1u);
+      m.fbm_frequency = (float)md.frequency;
+      m.fbm_gain = (float)md.gain;
+      m.fbm_lacunarity = (float)md.lacunarity;
+      m.fbm_coverage = (float)md.coverage;
+      m.fbm_sharpness = (float)md.sharpness;
+    }
     if (md.nx > 0) {
       m.nx = md.nx;
       m.ny = md.ny;

@@ -232,7 +232,9 @@ struct InstanceObj {
   Mat4 inverse_at(double time) const;
 };
 
-// medium.rs:28-65 with the density as a vertex grid (empty: homogeneous).
+// medium.rs:28-65 with the density as a vertex grid, or the procedural
+// fBm cloud field DensityField::Fbm (fbm_octaves > 0), or neither
+// (homogeneous).
 struct Medium {
   Vec3 sigma_a, sigma_s, emission;
   double g = 0;
@@ -240,6 +242,9 @@ struct Medium {
   uint32_t nx = 0, ny = 0, nz = 0;
   std::vector<float> grid;
   double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  uint32_t fbm_octaves = 0;
+  double fbm_frequency = 0, fbm_gain = 0, fbm_lacunarity = 0, fbm_coverage = 0, fbm_sharpness = 0;
+  bool heterogeneous() const { return nx > 0 || fbm_octaves > 0; }
   Vec3 sigma_t() const { return sigma_a + sigma_s; }
   double majorant() const;
   double density_at(const Vec3& p) const;

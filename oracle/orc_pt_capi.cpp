@@ -413,6 +413,14 @@ orc::Medium medium_from(const rrhip_medium_desc* md) {
   m.emission = orc::Vec3{md->emission[0], md->emission[1], md->emission[2]};
   m.g = md->g;
   m.max_distance = md->max_distance > 0.0 ? md->max_distance : 1e300;
+  if (md->procedural == RRHIP_MEDIUM_FBM) {  // parameters in f32, as the device keeps them
+    m.fbm_octaves = std::max(md->octaves, 1u);
+    m.fbm_frequency = (float)md->frequency;
+    m.fbm_gain = (float)md->gain;
+    m.fbm_lacunarity = (float)md->lacunarity;
+    m.fbm_coverage = (float)md->coverage;
+    m.fbm_sharpness = (float)md->sharpness;
+  }
   if (md->nx > 0) {
     m.nx = md->nx;
     m.ny = md->ny;

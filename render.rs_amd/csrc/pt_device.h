@@ -1047,9 +1047,20 @@ __device__ __forceinline__ bool occluded(const PtSceneView& S, V3 o, V3 d, float
 
 // ---- participating media (medium.rs:28-101, pt/volume.rs:1-189) ---------------------
 
-// Density multiplier at p: trilinear over the vertex grid, 0 outside it;
-// 1 for homogeneous media.
+__device__ __forceinline__ bool heterogeneous(const PtMedium& m) { return m.grid || m.fbm_octaves; }
+
+// Density multiplier at p: render.rs's cloud field clamp((fbm(p) * 0.5 +
+// 0.5 - (1 - coverage)) * sharpness, 0, 1) at the world point, evaluated in
+// f64 (medium.rs:55-63); trilinear over the vertex grid, 0 outside it; 1
+// for homogeneous media.
 __device__ __forceinline__ float density_at(const PtMedium& m, V3 p) {
+  if (m.fbm_octaves) {
+    const double off[3] = {0.0, 0.0, 0.0};
+    const double n = fbm3((double)p.x, (double)p.y, (double)p.z, (double)m.fbm_frequency, m.fbm_octaves,
+                          (double)m.fbm_gain, (double)m.fbm_lacunarity, off) * 0.5 + 0.5;
+    const double r = (n - (1.0 - (double)m.fbm_coverage)) * (double)m.fbm_sharpness;
+    return (float)fmin(fmax(r, 0.0), 1.0);
+  }
   if (!m.grid) return 1.f;
   const float u[3] = {(p.x - m.lo[0]) * m.scale[0], (p.y - m.lo[1]) * m.scale[1], (p.z - m.lo[2]) * m.scale[2]};
   const uint32_t n[3] = {m.nx, m.ny, m.nz};
@@ -1078,7 +1089,7 @@ __device__ __forceinline__ float density_at(const PtMedium& m, V3 p) {
 __device__ __forceinline__ bool medium_range(const PtMedium& m, V3 o, V3 d, float t_max, float& t0, float& t1) {
   t0 = 0.f;
   t1 = fminf(t_max, m.max_distance);
-  if (!m.grid) return t1 > t0;
+  if (!m.grid) return t1 > t0;  // homogeneous, or a procedural field over the whole line
   const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
 #pragma unroll
   for (int a = 0; a < 3; a++) {
@@ -1113,7 +1124,7 @@ __device__ __forceinline__ float exp_step(Pcg32& rng, float sigma) {
 __device__ __forceinline__ MediumEvent sample_distance(const PtMedium& m, V3 o, V3 d, float t_max, V3 beta,
                                                       Pcg32& rng) {
   const V3 st = v3(m.sigma_t);
-  if (!m.grid) {
+  if (!heterogeneous(m)) {
     const float tm = fminf(t_max, m.max_distance);
     const float sa = avg3(st);
     if (sa <= 1e-9f) return {false, 0.f, v3(1.f, 1.f, 1.f)};
@@ -1157,7 +1168,7 @@ __device__ __forceinline__ MediumEvent sample_distance(const PtMedium& m, V3 o, 
 // volume.rs:153-189: closed form (homogeneous) or ratio tracking.
 __device__ __forceinline__ V3 medium_transmittance(const PtMedium& m, V3 o, V3 d, float dist, Pcg32& rng) {
   const V3 st = v3(m.sigma_t);
-  if (!m.grid) return exp3(st, fminf(dist, m.max_distance));
+  if (!heterogeneous(m)) return exp3(st, fminf(dist, m.max_distance));
   const float maj = m.majorant;
   float t, t_end;
   V3 tr = v3(1.f, 1.f, 1.f);

@@ -892,6 +892,20 @@ bool build_pt_scene(const rrhip_scene_desc& d, PtSceneHost* out, std::string* er
     put(g.emission, d3(md.emission));
     g.max_distance = md.max_distance > 0.0 ? (float)md.max_distance : INFINITY;
     std::vector<float> grid;
+    if (md.procedural > RRHIP_MEDIUM_FBM) return *err = "unknown medium procedural density", false;
+    if (md.procedural == RRHIP_MEDIUM_FBM) {
+      if (md.nx > 0) return *err = "a medium has either a density grid or a procedural field", false;
+      if (md.octaves > 64) return *err = "medium fBm: at most 64 octaves", false;
+      const double f[5] = {md.frequency, md.gain, md.lacunarity, md.coverage, md.sharpness};
+      for (double x : f)
+        if (!std::isfinite(x)) return *err = "medium fBm parameters must be finite", false;
+      g.fbm_octaves = std::max(md.octaves, 1u);  // fbm runs at least one octave (displace.rs:86)
+      g.fbm_frequency = (float)md.frequency;
+      g.fbm_gain = (float)md.gain;
+      g.fbm_lacunarity = (float)md.lacunarity;
+      g.fbm_coverage = (float)md.coverage;
+      g.fbm_sharpness = (float)md.sharpness;
+    }
     if (md.nx > 0) {
       if (md.nx < 2 || md.ny < 2 || md.nz < 2 || !md.density)
         return *err = "medium density grid needs >= 2 samples per axis and data", false;

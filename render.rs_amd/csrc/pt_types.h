@@ -182,7 +182,8 @@ struct alignas(16) PtInstMotion {
 };
 
 // A participating medium (medium.rs:28-65) with its density as a vertex
-// grid over [lo, hi] (grid == nullptr: homogeneous).
+// grid over [lo, hi], or render.rs's procedural fBm cloud field
+// (fbm_octaves > 0), or neither (homogeneous).
 struct alignas(16) PtMedium {
   float sigma_a[3], g;
   float sigma_s[3], majorant;  // channel-max extinction
@@ -194,10 +195,12 @@ struct alignas(16) PtMedium {
   float hi[3];
   uint32_t nz;
   float scale[3];              // (n - 1) / (hi - lo) per axis
-  uint32_t pad;
+  uint32_t fbm_octaves;        // 0: no procedural field
   const float* grid;
-  const void* pad2;
+  float fbm_frequency, fbm_gain;
+  float fbm_lacunarity, fbm_coverage, fbm_sharpness, pad;
 };
+static_assert(sizeof(PtMedium) == 144, "PtMedium layout");
 
 struct PtMeshInfo {
   uint32_t root;      // BLAS root in the shared node array

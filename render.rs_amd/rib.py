@@ -26,8 +26,7 @@ the pattern graph and its bindings; .tex files are read by `rtex`.
 PatchMesh / Patch / NuPatch, SubdivisionMesh, GeneralPolygon /
 PointsGeneralPolygons and Displace "noise" are diced or triangulated on
 the host (`tessellate`) into meshes. Requests the GPU path has no
-counterpart for (Procedural, the procedural fBm density of media)
-raise `RibUnsupported` by default; with `strict=False` they are skipped
+counterpart (Procedural "DynamicLoad") raise `RibUnsupported` by default; with `strict=False` they are skipped
 with one warning each, as render.rs skips requests it does not implement
 (a motion block over other requests then keeps its first sample, as
 render.rs does). Transform motion blocks move the mesh instances created
@@ -301,7 +300,7 @@ _NO_EFFECT = {"version", "WorldEnd", "FrameBegin", "FrameEnd", "Display", "Decla
               "Attribute"}
 
 # Requests render.rs implements that this backend does not.
-_UNSUPPORTED = {"Procedural"}
+_UNSUPPORTED = set()
 
 # RiSpec cubic bases and their default steps (patches.rs:10-53); the
 # matrix M of p(u) = [u^3 u^2 u 1] M G, row-major.
@@ -510,6 +509,51 @@ class Builder:
         finally:
             self.archive_depth -= 1
 
+    def _procedural(self, vals):
+        """Procedural (scene_builder.rs:381-438): "DelayedReadArchive" reads
+        its archive at once (bounds ignored); "RunProgram" runs the program
+        (relative to the RIB's directory, extra arguments split on
+        whitespace, in that directory) and replays the RIB it prints;
+        "DynamicLoad" has no counterpart."""
+        kind = _str(vals, 0) or ""
+        raw = vals[1] if len(vals) > 1 else None
+        if isinstance(raw, str):
+            args = [raw]
+        elif isinstance(raw, list):
+            args = [a for a in raw if isinstance(a, str)]
+        else:
+            args = []
+        if kind == "DelayedReadArchive":
+            self._warn('Procedural "DelayedReadArchive" loads eagerly (no lazy loading)')
+            if args:
+                self._read_archive(args[0])
+        elif kind == "RunProgram":
+            if not args:
+                return self._warn('Procedural "RunProgram" without a program; skipping')
+            import subprocess
+            prog = args[0] if self.base_dir is None or os.path.isabs(args[0]) else \
+                os.path.join(self.base_dir, args[0])
+            cmd = [prog] + (args[1].split() if len(args) > 1 else [])
+            try:
+                out = subprocess.run(cmd, cwd=self.base_dir, capture_output=True, timeout=600)
+            except (OSError, subprocess.SubprocessError) as e:
+                return self._warn(f"RunProgram {args[0]}: {e}")
+            if out.returncode != 0:
+                return self._warn(f"RunProgram {args[0]} exited with {out.returncode}")
+            try:
+                reqs = parse(out.stdout.decode("utf-8", errors="replace"))
+            except RibError as e:
+                return self._warn(f"RunProgram {args[0]}: output failed to parse: {e}")
+            if self.archive_depth >= 16:
+                raise RibError(f'RunProgram "{args[0]}": nesting deeper than 16')
+            self.archive_depth += 1
+            try:
+                self.run(reqs)
+            finally:
+                self.archive_depth -= 1
+        else:
+            self._skip(f'Procedural "{kind}" is not implemented (DynamicLoad)')
+
     def process(self, name, vals):
         st = self.state
         if name == "WorldBegin":
@@ -529,6 +573,8 @@ class Builder:
         elif name == "ReadArchive":
             if _str(vals, 0) is not None:
                 self._read_archive(_str(vals, 0))
+        elif name == "Procedural":
+            self._procedural(vals)
         elif name == "MotionBegin":
             self.motion = []
         elif name == "MotionEnd":
@@ -812,17 +858,25 @@ class Builder:
             self._skip(f'Light "{kind}" is not implemented')
 
     def _medium(self, name, vals):
-        """scene_builder.rs:328-373: homogeneous media (the procedural fBm
-        density field is not supported by this backend)."""
+        """scene_builder.rs:328-373: homogeneous media, or with density
+        "fbm" / "noise" render.rs's procedural cloud field (frequency 0.3,
+        5 octaves, gain 0.55, lacunarity 2, coverage 0.5, sharpness 4 by
+        default); another density name warns and stays homogeneous."""
         p = _Params(vals, 1)
-        if p.string("density") is not None:
-            return self._skip(f'{name} density "{p.string("density")}" is not supported by this backend')
+        dens = p.string("density")
+        fbm = None
+        if dens in ("fbm", "noise"):
+            fbm = dict(frequency=p.number("frequency", 0.3), octaves=int(p.number("octaves", 5.0)),
+                       gain=p.number("gain", 0.55), lacunarity=p.number("lacunarity", 2.0),
+                       coverage=p.number("coverage", 0.5), sharpness=p.number("sharpness", 4.0))
+        elif dens is not None:
+            self._warn(f'{name} density "{dens}" not implemented; homogeneous')
         far = 1200.0 if name == "Atmosphere" else 0.0  # 0: unbounded
         idx = len(self.media)
         self.media.append(dict(sigma_a=p.color("sigma_a", (0.1, 0.1, 0.1)),
                                sigma_s=p.color("sigma_s", (0.5, 0.5, 0.5)), g=p.number("g", 0.0),
                                emission=p.color("emission", (0.0, 0.0, 0.0)),
-                               max_distance=p.number("maxdistance", far)))
+                               max_distance=p.number("maxdistance", far), fbm=fbm))
         if name == "Atmosphere":
             self.atmosphere = idx
         else:

@@ -132,8 +132,13 @@ class CameraDesc(ctypes.Structure):
 class MediumDesc(ctypes.Structure):
     _fields_ = [("sigma_a", D3), ("sigma_s", D3), ("g", ctypes.c_double), ("emission", D3),
                 ("max_distance", ctypes.c_double), ("nx", ctypes.c_uint32), ("ny", ctypes.c_uint32),
-                ("nz", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("density", ctypes.c_void_p),
-                ("grid_lo", D3), ("grid_hi", D3)]
+                ("nz", ctypes.c_uint32), ("procedural", ctypes.c_uint32), ("density", ctypes.c_void_p),
+                ("grid_lo", D3), ("grid_hi", D3), ("octaves", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("frequency", ctypes.c_double), ("gain", ctypes.c_double), ("lacunarity", ctypes.c_double),
+                ("coverage", ctypes.c_double), ("sharpness", ctypes.c_double)]
+
+
+MEDIUM_FBM = 1  # rrhip_medium_desc.procedural: render.rs's fBm cloud density
 
 
 # rrhip_quadric_desc kinds

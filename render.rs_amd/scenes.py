@@ -278,13 +278,23 @@ class Scene:
         self.instances.append(i)
 
     def add_medium(self, sigma_a, sigma_s, g=0.0, emission=(0, 0, 0), max_distance=0.0, density=None,
-                   grid_lo=(-1, -1, -1), grid_hi=(1, 1, 1)):
+                   grid_lo=(-1, -1, -1), grid_hi=(1, 1, 1), fbm=None):
         """A medium (medium.rs Medium); density: None (homogeneous) or a
-        (nz, ny, nx) array of [0, 1] values on the vertices of the grid box."""
+        (nz, ny, nx) array of [0, 1] values on the vertices of the grid box;
+        fbm: None or a dict of render.rs's cloud field parameters
+        (frequency, octaves, gain, lacunarity, coverage, sharpness; the RIB
+        defaults where missing), evaluated per lookup."""
         m = rrhip.MediumDesc()
         m.sigma_a, m.sigma_s, m.emission = _d3(sigma_a), _d3(sigma_s), _d3(emission)
         m.g = g
         m.max_distance = max_distance
+        if fbm is not None:
+            f = dict(frequency=0.3, octaves=5, gain=0.55, lacunarity=2.0, coverage=0.5, sharpness=4.0)
+            f.update(fbm)
+            m.procedural = rrhip.MEDIUM_FBM
+            m.octaves = int(f["octaves"])
+            m.frequency, m.gain, m.lacunarity = f["frequency"], f["gain"], f["lacunarity"]
+            m.coverage, m.sharpness = f["coverage"], f["sharpness"]
         if density is not None:
             dens = np.ascontiguousarray(density, np.float32)
             self._keep.append(dens)

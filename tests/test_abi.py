@@ -31,7 +31,7 @@ def test_every_declared_symbol_is_exported():
 
 def test_abi_version_and_error_string():
     lib = rrhip.load()
-    assert lib.rrhip_abi_version() == 9
+    assert lib.rrhip_abi_version() == 10
     h = ctypes.c_void_p()
     st = lib.rrhip_init(-1, ctypes.byref(h))
     assert st != 0 and not h

@@ -263,3 +263,21 @@ def test_transparent_room_matches_oracle(ctx):
     say(f"outside 3 sigma {frac:.4%}, image-mean z {np.round(zi, 2)}, means gpu {mg.mean(0).round(5)} "
         f"cpu {mc.mean(0).round(5)}")
     assert frac <= 0.01 and (zi < 3).all()
+
+
+def test_procedural_density_rejections(ctx):
+    # a medium with both a grid and the procedural field, too many
+    # octaves, an unknown procedural kind, non-finite parameters
+    def scene(**kw):
+        s = scenes.box_room(16, 16)
+        mi = s.add_medium((0.1, 0.1, 0.1), (0.5, 0.5, 0.5), fbm={})
+        for k, v in kw.items():
+            setattr(s.media[mi], k, v)
+        s.add_sphere((0, 0, 0), 0.2, s.add_material(scenes.hull(mi)))
+        return s
+    rrhip.PtScene(ctx, scene().desc()).close()  # the valid one builds
+    ones = np.ones((2, 2, 2), np.float32)
+    for kw in (dict(nx=2, ny=2, nz=2, density=ones.ctypes.data), dict(octaves=65), dict(procedural=2),
+               dict(frequency=float("nan")), dict(sharpness=float("inf"))):
+        with pytest.raises(rrhip.RrhipError):
+            rrhip.PtScene(ctx, scene(**kw).desc())

@@ -316,3 +316,61 @@ def test_cutout_attenuates_shadow_rays_and_passes_camera_rays():
     acc2, _ = orc.render(s2.desc(), rrhip.RenderParams.make(max_depth=4, firefly_clamp=0.0), 0, 4096)
     mean = acc2[:, 0].sum() / acc2[:, 6].sum()
     assert abs(mean - 0.7) < 4 * np.sqrt(0.21 / acc2[:, 6].sum()), mean
+
+
+def fbm_medium(sigma_a, sigma_s, frequency=0.3, octaves=5, gain=0.55, lacunarity=2.0, coverage=0.5,
+               sharpness=4.0):
+    m = medium(sigma_a, sigma_s)
+    m.procedural = rrhip.MEDIUM_FBM
+    m.octaves, m.frequency, m.gain, m.lacunarity = octaves, frequency, gain, lacunarity
+    m.coverage, m.sharpness = coverage, sharpness
+    return m
+
+
+def test_procedural_cloud_density_follows_medium_rs():
+    """DensityField::Fbm (medium.rs:55-63): clamp((fbm(p) * 0.5 + 0.5 - (1 -
+    coverage)) * sharpness, 0, 1) at the world point, with the oracle's
+    fBm (pinned by displace.rs's tests) and the parameters as f32."""
+    f32 = lambda x: float(np.float32(x))  # noqa: E731
+    m = fbm_medium(0.2, 0.6, frequency=0.7, octaves=4, gain=0.6, lacunarity=2.1, coverage=0.55, sharpness=3.5)
+    pts = np.random.default_rng(2).uniform(-4, 4, (400, 3))
+    n = orc.fbm(pts, f32(0.7), 4, f32(0.6), f32(2.1), (0.0, 0.0, 0.0)) * 0.5 + 0.5
+    want = np.clip((n - (1.0 - f32(0.55))) * f32(3.5), 0.0, 1.0)
+    np.testing.assert_array_equal(orc.medium_density(m, pts), want)
+    assert 0.0 < want.mean() < 1.0 and (want == 0.0).any() and (want > 0.0).any()
+
+
+def test_procedural_cloud_limits_match_closed_forms():
+    """Constant limits of the cloud field through delta and ratio tracking:
+    sharpness 0 makes the medium vanish (transmittance 1, no scattering);
+    coverage 2 with sharpness 100 fills it (|fbm| < 1.3 keeps the ramp
+    above 1), so transmittance is exp(-sigma_t d) and the scattered
+    fraction sigma_s / sigma_t (1 - T), as for a homogeneous medium."""
+    o, d = np.array([0.1, -0.3, 0.2]), np.array([0.0, 0.0, 1.0])
+    empty = fbm_medium(0.3, 0.5, sharpness=0.0)
+    assert np.all(orc.medium_transmittance(empty, o, d, 3.0, 5, 1, 2000) == 1.0)
+    sc, _, _ = orc.medium_sample(empty, o, d, 3.0, (1, 1, 1), 5, 2, 2000)
+    assert not sc.any()
+    full = fbm_medium(0.3, 0.5, coverage=2.0, sharpness=100.0)
+    assert np.all(orc.medium_density(full, np.random.default_rng(3).uniform(-9, 9, (300, 3))) == 1.0)
+    tr = orc.medium_transmittance(full, o, d, 2.0, 7, 3, 40000)
+    assert abs(tr[:, 0].mean() - np.exp(-0.8 * 2.0)) < 0.01
+    sc, t, w = orc.medium_sample(full, o, d, 2.0, (1, 1, 1), 9, 4, 40000)
+    assert abs(sc.mean() - (1.0 - np.exp(-1.6))) < 0.01  # channel-equal sigma_t: majorant = sigma_t
+    assert np.allclose(w[sc], 0.5 / 0.8)
+
+
+def test_procedural_cloud_ratio_tracking_matches_quadrature():
+    """volume.rs:153-189 ratio tracking through the procedural cloud vs
+    exp(-Riemann optical depth) of the field along the segment."""
+    m = fbm_medium(0.4, 0.8, frequency=0.9, octaves=4, coverage=0.6, sharpness=3.0)
+    o = np.array([0.3, 0.2, -1.0])
+    d = np.array([0.2, 0.1, 1.0])
+    d /= np.linalg.norm(d)
+    dist, steps = 3.0, 20000
+    ts = (np.arange(steps) + 0.5) / steps * dist
+    rho = orc.medium_density(m, o[None, :] + ts[:, None] * d[None, :])
+    expected = np.exp(-1.2 * rho.sum() * dist / steps)
+    tr = orc.medium_transmittance(m, o, d, dist, 13, 5, 30000)
+    assert 0.05 < expected < 0.95
+    assert abs(tr[:, 0].mean() - expected) < 0.02, (tr[:, 0].mean(), expected)

@@ -585,3 +585,44 @@ def test_motion_block_deforms_points_polygons():
     # the oracle sees the quad at z = 8 at shutter open and z = 10 at close
     t, *_ = orc.scene_intersect_n(s.desc(), [[0, 0, 0, 0, 0, 1]])
     assert abs(t[0] - 8.0) < 1e-9
+
+
+def test_procedural_requests(tmp_path):
+    # Procedural "DelayedReadArchive" reads its archive at once; "RunProgram"
+    # replays what the program prints; "DynamicLoad" has no counterpart
+    import os
+    import stat
+    (tmp_path / "part.rib").write_text('Sphere 0.5 -0.5 0.5 360\n')
+    prog = tmp_path / "gen.sh"
+    prog.write_text('#!/bin/sh\necho "Translate $1 0 0 Sphere 0.25 -0.25 0.25 360"\n')
+    prog.chmod(prog.stat().st_mode | stat.S_IEXEC)
+    (tmp_path / "scene.rib").write_text('''WorldBegin
+        Procedural "DelayedReadArchive" ["part.rib"] [-1 1 -1 1 -1 1]
+        AttributeBegin Procedural "RunProgram" ["gen.sh" "3"] [-1 1 -1 1 -1 1] AttributeEnd
+        WorldEnd''')
+    s, _ = rib.load(str(tmp_path / "scene.rib"), strict=False)
+    assert len(s.quadrics) == 2
+    assert sorted(q.radius for q in s.quadrics) == [0.25, 0.5]
+    assert any(abs(q.object_to_world[3] - 3.0) < 1e-12 for q in s.quadrics)
+    with pytest.raises(rib.RibUnsupported):
+        rib.loads('WorldBegin Procedural "DynamicLoad" ["x.so" ""] [-1 1 -1 1 -1 1] WorldEnd')
+    assert os.path.exists(tmp_path / "part.rib")
+
+
+def test_media_density_parameters():
+    # Interior / Atmosphere "density" "fbm" | "noise" (scene_builder.rs:
+    # 328-373): render.rs's defaults, overrides, another name stays
+    # homogeneous with a warning
+    s, _ = rib.loads('''WorldBegin
+        Atmosphere "haze" "density" "noise"
+        AttributeBegin Interior "c" "density" "fbm" "frequency" [1.5] "octaves" [3] "gain" [0.6]
+          "lacunarity" [2.5] "coverage" [0.7] "sharpness" [2] Sphere 1 -1 1 360 AttributeEnd
+        WorldEnd''')
+    a, c = s.media
+    assert (a.procedural, a.octaves, a.frequency, a.gain, a.lacunarity, a.coverage, a.sharpness) == \
+        (rrhip.MEDIUM_FBM, 5, 0.3, 0.55, 2.0, 0.5, 4.0)
+    assert (c.procedural, c.octaves, c.frequency, c.gain, c.lacunarity, c.coverage, c.sharpness) == \
+        (rrhip.MEDIUM_FBM, 3, 1.5, 0.6, 2.5, 0.7, 2.0)
+    with pytest.warns(UserWarning):
+        s2, _ = rib.loads('WorldBegin Atmosphere "h" "density" "voxels" WorldEnd')
+    assert s2.media[0].procedural == 0 and s2.media[0].nx == 0
