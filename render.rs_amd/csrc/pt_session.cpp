@@ -214,8 +214,12 @@ int rrhip_session_create(rrhip_scene* scene, const rrhip_render_params* params, 
   if (e == hipSuccess) e = alloc_n(s, &stats, 4);
   s->P.stats = stats;
   if (e == hipSuccess) e = alloc_n(s, &s->d_accum, npix * RRHIP_ACCUM_STRIDE);
-  if (e == hipSuccess) e = hipMemset(s->d_accum, 0, npix * RRHIP_ACCUM_STRIDE * sizeof(double));
-  if (e == hipSuccess) e = hipMemset(stats, 0, 4 * sizeof(unsigned long long));
+  // cleared on the context's stream (non-blocking, so not ordered after a
+  // null-stream memset) and finished before the session is handed out
+  hipStream_t zst = scene->ctx->stream;
+  if (e == hipSuccess) e = hipMemsetAsync(s->d_accum, 0, npix * RRHIP_ACCUM_STRIDE * sizeof(double), zst);
+  if (e == hipSuccess) e = hipMemsetAsync(stats, 0, 4 * sizeof(unsigned long long), zst);
+  if (e == hipSuccess) e = hipStreamSynchronize(zst);
   if (e != hipSuccess) {
     free_session(s);
     return fail_hip(e, "rrhip_session_create");
@@ -538,7 +542,8 @@ int rrhip_session_set_film(rrhip_session* s, int enable) {
   hipError_t e = alloc_paths(s, s->path_cap);
   const uint64_t fbytes = (uint64_t)s->prm.npix * kFilmStride * sizeof(double);
   if (e == hipSuccess && s->film) e = hipMalloc((void**)&s->d_film, fbytes);
-  if (e == hipSuccess && s->film) e = hipMemset(s->d_film, 0, fbytes);
+  if (e == hipSuccess && s->film) e = hipMemsetAsync(s->d_film, 0, fbytes, s->scene->ctx->stream);
+  if (e == hipSuccess && s->film) e = hipStreamSynchronize(s->scene->ctx->stream);
   if (e != hipSuccess) {
     // leave a usable plain session behind
     (void)hipGetLastError();

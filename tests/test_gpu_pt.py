@@ -419,3 +419,29 @@ def test_near_mirror_specular_stays_finite(ctx):
     assert np.isfinite(g).all()
     c, _ = oracle_render(s.desc(), p, 0, 256)
     check_parity(g, c, label="near-mirror spheres")
+
+
+def test_fresh_session_starts_from_zero_after_a_freed_one(ctx):
+    # a session's accumulators and ray counters are cleared on the stream
+    # its work runs on: a session created where a busy one was freed reads
+    # zero before rendering, and its profile counts exactly its own rays
+    s = scenes.box_room(64, 64)
+    ps = rrhip.PtScene(ctx, s.desc())
+    p = rrhip.RenderParams.make(max_depth=8)
+    busy = ps.session(p)
+    busy.render(0, 64)
+    assert busy.stats()["extension_rays"] > 0
+    busy.close()
+    se = ps.session(p)
+    assert not se.accum().any()
+    st0 = se.stats()
+    assert st0["extension_rays"] == st0["shadow_rays"] == st0["samples_rendered"] == 0
+    se.set_profiling(True)
+    se.render(0, 4)
+    prof, st = se.profile(), se.stats()
+    say(f"fresh session: profile {prof['extend_rays']} / {prof['shadow_rays']} rays, "
+        f"stats {st['extension_rays']} / {st['shadow_rays']}")
+    assert 0 < prof["extend_rays"] == st["extension_rays"]
+    assert 0 < prof["shadow_rays"] == st["shadow_rays"]
+    se.close()
+    ps.close()
